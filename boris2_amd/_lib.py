@@ -1,0 +1,92 @@
+"""ctypes binding of libboris_mi355x.so (the C-ABI declared in include/boris_mi355x.h).
+
+The library is the only compute path: if it is missing or no gfx950 device is visible the
+calls raise instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libboris_mi355x.so"
+
+BM_EVAL_TEULER, BM_EVAL_RK4, BM_EVAL_AHEUN, BM_EVAL_RKDP54 = 1, 2, 7, 9
+BM_MOD_DEMAG, BM_MOD_EXCHANGE, BM_MOD_ANIS_UNI, BM_MOD_ZEEMAN = 1, 2, 4, 8
+
+# every entry point declared in include/boris_mi355x.h
+EXPORTS = [
+    "bm_last_error", "bm_device_ok",
+    "bm_demag_create", "bm_demag_destroy", "bm_demag_info", "bm_demag_convolute",
+    "bm_demag_convolute_planar", "bm_demag_copy_kernels",
+    "bm_ode_create", "bm_ode_destroy", "bm_ode_set_ms_map", "bm_ode_set_params", "bm_ode_set_dt",
+    "bm_ode_set_astep", "bm_ode_advance_time", "bm_ode_run_to_iteration", "bm_ode_iterate_stage",
+    "bm_ode_restore", "bm_ode_get_state", "bm_ode_keep_heff", "bm_ode_copy_heff", "bm_ode_average_m",
+]
+
+
+class FmParams(C.Structure):
+    _fields_ = [
+        ("Ms", C.c_double), ("A", C.c_double), ("alpha", C.c_double),
+        ("K1", C.c_double), ("K2", C.c_double), ("ea", C.c_double * 3), ("grel", C.c_double),
+        ("Ha", C.c_double * 3), ("wave", C.c_int), ("amp", C.c_double), ("w", C.c_double),
+        ("t0", C.c_double), ("dir", C.c_double * 3), ("slope", C.c_double * 3), ("modules", C.c_int),
+    ]
+
+
+class OdeState(C.Structure):
+    _fields_ = [
+        ("time", C.c_double), ("stagetime", C.c_double), ("dT", C.c_double), ("dT_last", C.c_double),
+        ("lte", C.c_double), ("iteration", C.c_longlong), ("stageiteration", C.c_longlong),
+        ("rejections", C.c_longlong), ("evaluations", C.c_longlong),
+        ("evalStep", C.c_int), ("primed", C.c_int), ("available", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m boris2_amd.build`")
+    L = C.CDLL(str(LIB_PATH))
+    vp, i, d, ll = C.c_void_p, C.c_int, C.c_double, C.c_longlong
+    sig = {
+        "bm_last_error": (C.c_char_p, []),
+        "bm_device_ok": (i, []),
+        "bm_demag_create": (i, [i, i, i, d, d, d, i, C.POINTER(vp)]),
+        "bm_demag_destroy": (i, [vp]),
+        "bm_demag_info": (i, [vp, C.POINTER(i), C.POINTER(ll)]),
+        "bm_demag_convolute": (i, [vp, vp, vp, i, vp, vp]),
+        "bm_demag_convolute_planar": (i, [vp, vp, vp, vp]),
+        "bm_demag_copy_kernels": (i, [vp, vp, ll]),
+        "bm_ode_create": (i, [i, i, i, i, d, d, d, C.POINTER(FmParams), vp, vp, C.POINTER(vp)]),
+        "bm_ode_destroy": (i, [vp]),
+        "bm_ode_set_ms_map": (i, [vp, vp]),
+        "bm_ode_set_params": (i, [vp, C.POINTER(FmParams)]),
+        "bm_ode_set_dt": (i, [vp, d]),
+        "bm_ode_set_astep": (i, [vp, d, d, d, d]),
+        "bm_ode_advance_time": (i, [vp, vp]),
+        "bm_ode_run_to_iteration": (i, [vp, ll, vp]),
+        "bm_ode_iterate_stage": (i, [vp, vp]),
+        "bm_ode_restore": (i, [vp, vp]),
+        "bm_ode_get_state": (i, [vp, C.POINTER(OdeState)]),
+        "bm_ode_keep_heff": (i, [vp, i]),
+        "bm_ode_copy_heff": (i, [vp, vp, vp]),
+        "bm_ode_average_m": (i, [vp, C.POINTER(d), vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().bm_last_error().decode(errors="replace")
+        raise RuntimeError(f"boris_mi355x error {rc}: {msg}")

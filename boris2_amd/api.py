@@ -1,0 +1,266 @@
+"""Host-side mirror of the reference's module / ODE interface for this hot path.
+
+Objects here own device memory through torch (the allocator and stream provider only) and
+call the C-ABI for every piece of compute:
+
+* :class:`Demag` — ``DemagCUDA`` / ``ConvolutionCUDA::Convolute``
+  (Boris/DemagCUDA.cpp:31-121, Boris/ConvolutionCUDA.h:217-223).
+* :class:`FMSolver` — a ferromagnetic mesh with its Zeeman / Demag / Exchange / uniaxial
+  anisotropy modules and ``DifferentialEquationFMCUDA`` + ``ODECommon_Base`` step control
+  (Boris/DiffEqFMCUDA.h:38-137, Boris/DiffEq_CommonBase_IterateCUDA.cpp:13,
+  Boris/SuperMeshSimulation.cpp:122-149).
+* :func:`random_magnetisation` — ``FMesh::SetRandomMag`` with the ``BorisRand`` LCG
+  (Boris/Mesh_Ferromagnetic_Control.cpp:230-255, BorisLib/BLib_prng.h:18-90).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import (BM_EVAL_AHEUN, BM_EVAL_RK4, BM_EVAL_RKDP54, BM_EVAL_TEULER,  # noqa: F401
+                   BM_MOD_ANIS_UNI, BM_MOD_DEMAG, BM_MOD_EXCHANGE, BM_MOD_ZEEMAN, check)
+
+MU0 = 1.256637061435916e-6      # BorisLib/Funcs_Math_base.h:9
+GAMMA = 2.212761569e5           # BorisLib/Funcs_Math_base.h:16
+
+
+def _stream() -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t: torch.Tensor) -> C.c_void_p:
+    if not t.is_cuda or t.dtype != torch.float32 or not t.is_contiguous():
+        raise ValueError("expected a contiguous float32 CUDA tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def require_device() -> None:
+    """Raise unless the HIP library is loadable and a gfx950 device is present."""
+    L = _lib.load()
+    if not torch.cuda.is_available() or not L.bm_device_ok():
+        raise RuntimeError("boris2_amd needs an MI355X (gfx950) device; no fallback path exists")
+
+
+class Demag:
+    """FFT demag convolution of one nx*ny*nz mesh of cells h (m)."""
+
+    def __init__(self, n: tuple[int, int, int], h: tuple[float, float, float]):
+        require_device()
+        self.L = _lib.load()
+        self.n = tuple(int(v) for v in n)
+        self.h = tuple(float(v) for v in h)
+        self.handle = C.c_void_p()
+        check(self.L.bm_demag_create(*self.n, *self.h, 0, C.byref(self.handle)))
+
+    @property
+    def ncells(self) -> int:
+        return self.n[0] * self.n[1] * self.n[2]
+
+    def info(self) -> tuple[tuple[int, int, int], int]:
+        N = (C.c_int * 3)()
+        b = C.c_longlong()
+        check(self.L.bm_demag_info(self.handle, N, C.byref(b)))
+        return (N[0], N[1], N[2]), b.value
+
+    def convolute(self, M: torch.Tensor, Heff: torch.Tensor | None = None, clear_out: bool = True,
+                  energy: torch.Tensor | None = None) -> torch.Tensor:
+        """Heff (=, +=) H_demag(M); M, Heff are (n, 3) float32 CUDA tensors."""
+        if Heff is None:
+            Heff = torch.empty_like(M)
+            clear_out = True
+        e = C.c_void_p(energy.data_ptr()) if energy is not None else None
+        check(self.L.bm_demag_convolute(self.handle, _ptr(M), _ptr(Heff), int(clear_out), e, _stream()))
+        return Heff
+
+    def convolute_planar(self, M: torch.Tensor, H3n: torch.Tensor | None = None) -> torch.Tensor:
+        if H3n is None:
+            H3n = torch.empty((3, self.ncells), dtype=torch.float32, device=M.device)
+        check(self.L.bm_demag_convolute_planar(self.handle, _ptr(M), _ptr(H3n), _stream()))
+        return H3n
+
+    def kernels(self) -> np.ndarray:
+        (Nx, Ny, Nz), _ = self.info()
+        nc, nyh, nzh = Nx // 2 + 1, Ny // 2 + 1, Nz // 2 + 1
+        cnt = 4 * nc * nyh if self.n[2] == 1 else 6 * nzh * nc * nyh
+        out = np.zeros(cnt, dtype=np.float32)
+        check(self.L.bm_demag_copy_kernels(self.handle, out.ctypes.data_as(C.c_void_p), cnt))
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            self.L.bm_demag_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class FMParams:
+    """Material parameters and applied field of one ferromagnetic mesh (SI units)."""
+    Ms: float = 8e5
+    A: float = 1.3e-11
+    alpha: float = 0.02
+    K1: float = 0.0
+    K2: float = 0.0
+    ea: tuple = (1.0, 0.0, 0.0)
+    grel: float = 1.0
+    Ha: tuple = (0.0, 0.0, 0.0)
+    wave: int = 0
+    amp: float = 0.0
+    w: float = 0.0
+    t0: float = 0.0
+    dir: tuple = (0.0, 0.0, 0.0)
+    slope: tuple = (0.0, 0.0, 0.0)
+    modules: int = BM_MOD_DEMAG | BM_MOD_EXCHANGE | BM_MOD_ZEEMAN
+
+    def to_c(self) -> _lib.FmParams:
+        p = _lib.FmParams()
+        p.Ms, p.A, p.alpha, p.K1, p.K2, p.grel = self.Ms, self.A, self.alpha, self.K1, self.K2, self.grel
+        p.ea[:] = list(self.ea)
+        p.Ha[:] = list(self.Ha)
+        p.wave, p.amp, p.w, p.t0 = int(self.wave), self.amp, self.w, self.t0
+        p.dir[:] = list(self.dir)
+        p.slope[:] = list(self.slope)
+        p.modules = int(self.modules)
+        return p
+
+    def oracle_vector(self) -> list[float]:
+        """Packed form used by the oracle's C-ABI (22 doubles)."""
+        return [self.Ms, self.A, self.alpha, self.K1, self.K2, *self.ea, *self.Ha, float(self.wave),
+                self.amp, self.w, self.t0, *self.dir, *self.slope]
+
+    def oracle_flags(self) -> int:
+        return ((1 if self.modules & BM_MOD_DEMAG else 0) | (2 if self.modules & BM_MOD_EXCHANGE else 0)
+                | (4 if self.modules & BM_MOD_ANIS_UNI else 0) | (8 if self.modules & BM_MOD_ZEEMAN else 0))
+
+
+class FMSolver:
+    """One ferromagnetic mesh integrated with the LLG equation."""
+
+    def __init__(self, method: int, n: tuple[int, int, int], h: tuple[float, float, float],
+                 params: FMParams, M0: np.ndarray | torch.Tensor | None = None):
+        require_device()
+        self.L = _lib.load()
+        self.n = tuple(int(v) for v in n)
+        self.h = tuple(float(v) for v in h)
+        self.params = params
+        ncell = self.n[0] * self.n[1] * self.n[2]
+        self.M = torch.zeros((ncell, 3), dtype=torch.float32, device="cuda")
+        if M0 is None:
+            self.M[:, 0] = -params.Ms      # a new mesh starts at (-Ms, 0, 0), Boris/Mesh_Ferromagnetic.cpp:194
+        else:
+            self.M.copy_(torch.as_tensor(np.asarray(M0, dtype=np.float32).reshape(ncell, 3)))
+        self.demag = Demag(self.n, self.h) if params.modules & BM_MOD_DEMAG else None
+        self._pc = params.to_c()
+        self.handle = C.c_void_p()
+        dh = self.demag.handle if self.demag else None
+        check(self.L.bm_ode_create(method, *self.n, *self.h, C.byref(self._pc), dh, _ptr(self.M), C.byref(self.handle)))
+        self.Ms_map = None
+
+    def set_ms_map(self, ms: np.ndarray | torch.Tensor | None) -> None:
+        if ms is None:
+            self.Ms_map = None
+            check(self.L.bm_ode_set_ms_map(self.handle, None))
+            return
+        self.Ms_map = torch.as_tensor(np.asarray(ms, dtype=np.float32).reshape(-1)).cuda().contiguous()
+        check(self.L.bm_ode_set_ms_map(self.handle, _ptr(self.Ms_map)))
+
+    def set_params(self, params: FMParams) -> None:
+        self.params = params
+        self._pc = params.to_c()
+        check(self.L.bm_ode_set_params(self.handle, C.byref(self._pc)))
+
+    def set_dt(self, dT: float) -> None:
+        check(self.L.bm_ode_set_dt(self.handle, float(dT)))
+
+    def set_astep(self, err_fail=-1.0, dT_increase=-1.0, dT_min=-1.0, dT_max=-1.0) -> None:
+        check(self.L.bm_ode_set_astep(self.handle, err_fail, dT_increase, dT_min, dT_max))
+
+    def advance_time(self) -> None:
+        check(self.L.bm_ode_advance_time(self.handle, _stream()))
+
+    def run_to_iteration(self, it: int) -> None:
+        check(self.L.bm_ode_run_to_iteration(self.handle, int(it), _stream()))
+
+    def iterate_stage(self) -> None:
+        check(self.L.bm_ode_iterate_stage(self.handle, _stream()))
+
+    def restore(self) -> None:
+        check(self.L.bm_ode_restore(self.handle, _stream()))
+
+    def state(self) -> dict:
+        s = _lib.OdeState()
+        check(self.L.bm_ode_get_state(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _lib.OdeState._fields_}
+
+    def keep_heff(self, on: bool = True) -> None:
+        check(self.L.bm_ode_keep_heff(self.handle, int(on)))
+
+    def heff(self) -> torch.Tensor:
+        H = torch.empty_like(self.M)
+        check(self.L.bm_ode_copy_heff(self.handle, _ptr(H), _stream()))
+        return H
+
+    def average_m(self) -> np.ndarray:
+        out = (C.c_double * 3)()
+        check(self.L.bm_ode_average_m(self.handle, out, _stream()))
+        return np.array(out[:])
+
+    def close(self) -> None:
+        if self.handle:
+            torch.cuda.synchronize()
+            self.L.bm_ode_destroy(self.handle)
+            self.handle = C.c_void_p()
+        if self.demag is not None:
+            self.demag.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def random_magnetisation(ncell: int, Ms: float, seed: int, M_norm: np.ndarray | None = None) -> np.ndarray:
+    """Reference ``random <seed>``: i.i.d. directions on the unit sphere from BorisRand.
+
+    Serial LCG x <- 1664525 x + 1013904223 (mod 2^32) seeded with ``seed``;
+    u = x / (2^32 - 1); theta = acos(1 - 2 u1), phi = 2 pi u2.  Returns (ncell, 3) float64.
+    """
+    a, c = np.uint64(1664525), np.uint64(1013904223)
+    mask = np.uint64(0xFFFFFFFF)
+    # draw 2*ncell numbers with a vectorised jump-ahead of the LCG
+    cnt = 2 * ncell
+    # affine composition: x_k = A_k x0 + C_k (mod 2^32), computed by doubling
+    out = np.empty(cnt, dtype=np.uint64)
+    A = np.uint64(1)
+    Cc = np.uint64(0)
+    # powers for a block of size B, then iterate blocks
+    B = min(cnt, 4096)
+    Ab = np.empty(B, dtype=np.uint64)
+    Cb = np.empty(B, dtype=np.uint64)
+    for k in range(B):
+        A = (A * a) & mask
+        Cc = (Cc * a + c) & mask
+        Ab[k] = A
+        Cb[k] = Cc
+    x = np.uint64(seed) & mask
+    for s in range(0, cnt, B):
+        m = min(B, cnt - s)
+        blk = (Ab[:m] * x + Cb[:m]) & mask
+        out[s:s + m] = blk
+        x = blk[-1]
+    u = out.astype(np.float64) / 4294967295.0
+    th = np.arccos(1.0 - 2.0 * u[0::2])
+    ph = u[1::2] * 2.0 * np.pi
+    norm = Ms if M_norm is None else M_norm
+    m = np.stack([np.cos(ph) * np.sin(th), np.sin(ph) * np.sin(th), np.cos(th)], axis=1)
+    return m * np.asarray(norm).reshape(-1, 1)

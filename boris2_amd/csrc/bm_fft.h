@@ -1,0 +1,297 @@
+// Device FFT engine for CDNA4 (gfx950), FP32 and FP64.
+//
+// One transform of length N is carried by T = N/E threads, each holding E complex values
+// in registers.  Stages are Stockham radix-R passes (R divides E); between stages values
+// go through LDS once (one write + one read per element), the first stage consumes the
+// registers directly and the last stage produces them directly, so a pass kernel can load
+// from and store to HBM in natural order with coalesced accesses:
+//     on entry and on exit, register v[e] of thread t holds element  t + T*e.
+// Zero padding: when only the first N/2 inputs are non-zero, callers set v[e] = 0 for
+// e >= E/2 and the compiler folds the zeros through the first radix butterfly; when only
+// the first N/2 outputs are needed, callers store only e < E/2 and the dead outputs of the
+// last butterfly are eliminated.
+//
+// Twiddles come from FP64 sincospi (rounded to the working precision), computed once per
+// thread per kernel (they depend only on the thread's butterfly indices) and kept in
+// registers across every line the thread transforms.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace bm {
+
+template <typename T> struct C2 { T x, y; };
+
+template <typename T> __device__ __forceinline__ C2<T> mk(T a, T b) { C2<T> r; r.x = a; r.y = b; return r; }
+template <typename T> __device__ __forceinline__ C2<T> operator+(C2<T> a, C2<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename T> __device__ __forceinline__ C2<T> operator-(C2<T> a, C2<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename T> __device__ __forceinline__ C2<T> operator*(T s, C2<T> a) { return mk(s * a.x, s * a.y); }
+template <typename T> __device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// a * conj(b)
+template <typename T> __device__ __forceinline__ C2<T> cmulc(C2<T> a, C2<T> b) { return mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }
+template <typename T> __device__ __forceinline__ C2<T> conj(C2<T> a) { return mk(a.x, -a.y); }
+// multiply by DIR*i  (DIR = -1 forward, +1 inverse)
+template <int DIR, typename T> __device__ __forceinline__ C2<T> mul_di(C2<T> a) { return DIR < 0 ? mk(a.y, -a.x) : mk(-a.y, a.x); }
+
+// ----------------------------------------------------------------------------------------
+// Radix plans.  E = values per thread; each radix divides E; the first radix is even so a
+// zero upper half of the input maps onto whole zero inputs of the first butterflies.
+template <int N> struct Plan;
+#define BM_PLAN(n, e, ...)                                                      \
+	template <> struct Plan<n> {                                                \
+		static constexpr int E = e;                                             \
+		static constexpr int R[] = { __VA_ARGS__ };                             \
+		static constexpr int S = sizeof(R) / sizeof(int);                       \
+	};
+BM_PLAN(2, 2, 2)
+BM_PLAN(6, 6, 2, 3)
+BM_PLAN(12, 12, 4, 3)
+BM_PLAN(24, 12, 4, 3, 2)
+BM_PLAN(48, 12, 4, 4, 3)
+BM_PLAN(4, 4, 4)
+BM_PLAN(8, 8, 8)
+BM_PLAN(16, 8, 8, 2)
+BM_PLAN(32, 8, 8, 4)
+BM_PLAN(64, 8, 8, 8)
+BM_PLAN(96, 12, 4, 4, 3, 2)
+BM_PLAN(128, 8, 8, 4, 4)
+BM_PLAN(192, 12, 4, 4, 4, 3)
+BM_PLAN(256, 8, 8, 8, 4)
+BM_PLAN(384, 12, 4, 4, 4, 3, 2)
+BM_PLAN(512, 8, 8, 8, 8)
+BM_PLAN(768, 12, 4, 4, 4, 4, 3)
+BM_PLAN(1024, 16, 16, 16, 4)
+BM_PLAN(1536, 12, 4, 4, 4, 4, 3, 2)
+BM_PLAN(2048, 16, 16, 16, 8)
+BM_PLAN(4096, 16, 16, 16, 16)
+#undef BM_PLAN
+
+template <int N> constexpr int plan_T() { return N / Plan<N>::E; }
+template <int N> constexpr int plan_Ns(int s) { int p = 1; for (int q = 0; q < s; q++) p *= Plan<N>::R[q]; return p; }
+template <int N> constexpr int plan_ntw()
+{
+	int n = 0;
+	for (int s = 1; s < Plan<N>::S; s++) n += (Plan<N>::E / Plan<N>::R[s]) * (Plan<N>::R[s] - 1);
+	return n > 0 ? n : 1;
+}
+template <int N> constexpr int plan_twoff(int s)
+{
+	int n = 0;
+	for (int q = 1; q < s; q++) n += (Plan<N>::E / Plan<N>::R[q]) * (Plan<N>::R[q] - 1);
+	return n;
+}
+
+// ----------------------------------------------------------------------------------------
+// In-register DFTs of length R, direction DIR (-1: exp(-2 pi i jk/R)).
+template <int R, int DIR, typename T> struct Dft;
+
+template <int DIR, typename T> struct Dft<2, DIR, T> {
+	static __device__ __forceinline__ void run(C2<T>* v) { C2<T> a = v[0], b = v[1]; v[0] = a + b; v[1] = a - b; }
+};
+
+template <int DIR, typename T> struct Dft<3, DIR, T> {
+	static __device__ __forceinline__ void run(C2<T>* v)
+	{
+		const T c = (T)-0.5, s = (T)(DIR * 0.86602540378443864676372317075294);
+		C2<T> t1 = v[1] + v[2], t2 = v[1] - v[2];
+		C2<T> m = v[0] + c * t1;
+		C2<T> r = mk(-s * t2.y, s * t2.x);   // i*s*t2
+		v[0] = v[0] + t1; v[1] = m + r; v[2] = m - r;
+	}
+};
+
+template <int DIR, typename T> struct Dft<4, DIR, T> {
+	static __device__ __forceinline__ void run(C2<T>* v)
+	{
+		C2<T> a0 = v[0] + v[2], a1 = v[0] - v[2], a2 = v[1] + v[3], a3 = mul_di<DIR>(v[1] - v[3]);
+		v[0] = a0 + a2; v[1] = a1 + a3; v[2] = a0 - a2; v[3] = a1 - a3;
+	}
+};
+
+template <int DIR, typename T> struct Dft<8, DIR, T> {
+	static __device__ __forceinline__ void run(C2<T>* v)
+	{
+		C2<T> e[4] = { v[0], v[2], v[4], v[6] }, o[4] = { v[1], v[3], v[5], v[7] };
+		Dft<4, DIR, T>::run(e); Dft<4, DIR, T>::run(o);
+		const T r = (T)0.70710678118654752440084436210485;
+		// o[k] *= w8^k, w8 = exp(DIR i pi/4)
+		o[1] = mk(r * (o[1].x - DIR * o[1].y), r * (o[1].y + DIR * o[1].x));
+		o[2] = mul_di<DIR>(o[2]);
+		o[3] = mk(r * (-o[3].x - DIR * o[3].y), r * (-o[3].y + DIR * o[3].x));
+		for (int k = 0; k < 4; k++) { v[k] = e[k] + o[k]; v[k + 4] = e[k] - o[k]; }
+	}
+};
+
+template <int DIR, typename T> struct Dft<16, DIR, T> {
+	static __device__ __forceinline__ void run(C2<T>* v)
+	{
+		// 4 x 4: columns q (stride 4), twiddle w16^(q k1), then rows.
+		C2<T> a[4][4];
+#pragma unroll
+		for (int q = 0; q < 4; q++) {
+			C2<T> c[4] = { v[q], v[q + 4], v[q + 8], v[q + 12] };
+			Dft<4, DIR, T>::run(c);
+#pragma unroll
+			for (int k = 0; k < 4; k++) a[q][k] = c[k];
+		}
+		const T c1 = (T)0.92387953251128675612818318939679, s1 = (T)0.38268343236508977172845998403040;
+		const T r2 = (T)0.70710678118654752440084436210485;
+		// w16^m = exp(DIR i 2 pi m / 16) for m = q*k1 in {1,2,3,4,6,9}
+		auto tw = [&](C2<T> x, T c, T s) { s = (T)DIR * s; return mk(x.x * c - x.y * s, x.x * s + x.y * c); };
+		a[1][1] = tw(a[1][1], c1, s1); a[1][2] = tw(a[1][2], r2, r2); a[1][3] = tw(a[1][3], s1, c1);
+		a[2][1] = tw(a[2][1], r2, r2); a[2][2] = mul_di<DIR>(a[2][2]); a[2][3] = tw(a[2][3], -r2, r2);
+		a[3][1] = tw(a[3][1], s1, c1); a[3][2] = tw(a[3][2], -r2, r2); a[3][3] = tw(a[3][3], -c1, -s1);
+#pragma unroll
+		for (int k = 0; k < 4; k++) {
+			C2<T> c[4] = { a[0][k], a[1][k], a[2][k], a[3][k] };
+			Dft<4, DIR, T>::run(c);
+#pragma unroll
+			for (int m = 0; m < 4; m++) v[k + 4 * m] = c[m];
+		}
+	}
+};
+
+// ----------------------------------------------------------------------------------------
+// Per-thread twiddle cache: forward twiddles exp(-2 pi i (j mod Ns) r / (Ns R)) for every
+// stage s >= 1, butterfly b and r >= 1.  The inverse direction uses their conjugates.
+template <int N, typename T> struct Twiddles {
+	C2<T> w[plan_ntw<N>()];
+
+	template <int s>
+	__device__ __forceinline__ void init_stage(int t)
+	{
+		if constexpr (s < Plan<N>::S) {
+			constexpr int E = Plan<N>::E, TT = plan_T<N>(), R = Plan<N>::R[s], B = E / R, Ns = plan_Ns<N>(s);
+			constexpr int off = plan_twoff<N>(s);
+#pragma unroll
+			for (int b = 0; b < B; b++) {
+				int j = t + b * TT, k = j % Ns;
+#pragma unroll
+				for (int r = 1; r < R; r++) {
+					double sn, cs;
+					sincospi(-2.0 * (double)(k * r) / (double)(Ns * R), &sn, &cs);
+					w[off + b * (R - 1) + r - 1] = mk((T)cs, (T)sn);
+				}
+			}
+			init_stage<s + 1>(t);
+		}
+	}
+
+	__device__ __forceinline__ void init(int t) { init_stage<1>(t); }
+};
+
+// One Stockham stage on registers.  LDS element i of this line lives at lds[i * stride].
+template <int N, int s, int DIR, typename T>
+__device__ __forceinline__ void fft_stage(C2<T>* v, C2<T>* lds, int stride, int t, const Twiddles<N, T>& tw)
+{
+	constexpr int E = Plan<N>::E, TT = plan_T<N>(), R = Plan<N>::R[s], B = E / R, Ns = plan_Ns<N>(s);
+	if constexpr (s > 0) {
+#pragma unroll
+		for (int b = 0; b < B; b++)
+#pragma unroll
+			for (int r = 0; r < R; r++) v[b + r * B] = lds[(t + b * TT + r * (N / R)) * stride];
+#pragma unroll
+		for (int b = 0; b < B; b++)
+#pragma unroll
+			for (int r = 1; r < R; r++) {
+				C2<T> w = tw.w[plan_twoff<N>(s) + b * (R - 1) + r - 1];
+				v[b + r * B] = (DIR < 0) ? cmul(v[b + r * B], w) : cmulc(v[b + r * B], w);
+			}
+	}
+#pragma unroll
+	for (int b = 0; b < B; b++) {
+		C2<T> u[R];
+#pragma unroll
+		for (int r = 0; r < R; r++) u[r] = v[b + r * B];
+		Dft<R, DIR, T>::run(u);
+#pragma unroll
+		for (int r = 0; r < R; r++) v[b + r * B] = u[r];
+	}
+	if constexpr (s < Plan<N>::S - 1) {
+		__syncthreads();
+#pragma unroll
+		for (int b = 0; b < B; b++) {
+			int j = t + b * TT;
+			int d = (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+			for (int r = 0; r < R; r++) lds[(d + r * Ns) * stride] = v[b + r * B];
+		}
+		__syncthreads();
+	}
+}
+
+template <int N, int s, int DIR, typename T>
+__device__ __forceinline__ void fft_stages(C2<T>* v, C2<T>* lds, int stride, int t, const Twiddles<N, T>& tw)
+{
+	fft_stage<N, s, DIR, T>(v, lds, stride, t, tw);
+	if constexpr (s + 1 < Plan<N>::S) fft_stages<N, s + 1, DIR, T>(v, lds, stride, t, tw);
+}
+
+// Full transform.  All threads of the block must call it together (it contains
+// __syncthreads between stages); lines handled by different thread groups of the block
+// must use disjoint LDS regions.
+template <int N, int DIR, typename T>
+__device__ __forceinline__ void fft(C2<T>* v, C2<T>* lds, int stride, int t, const Twiddles<N, T>& tw)
+{
+	fft_stages<N, 0, DIR, T>(v, lds, stride, t, tw);
+}
+
+// Three transforms of the same length in lock step (shared barriers).
+template <int N, int s, int DIR, typename T>
+__device__ __forceinline__ void fft3_stages(C2<T>* v0, C2<T>* v1, C2<T>* v2, C2<T>* l0, C2<T>* l1, C2<T>* l2,
+	int stride, int t, const Twiddles<N, T>& tw)
+{
+	constexpr int E = Plan<N>::E, TT = plan_T<N>(), R = Plan<N>::R[s], B = E / R, Ns = plan_Ns<N>(s);
+	C2<T>* vv[3] = { v0, v1, v2 };
+	C2<T>* ll[3] = { l0, l1, l2 };
+	if constexpr (s > 0) {
+#pragma unroll
+		for (int c = 0; c < 3; c++) {
+#pragma unroll
+			for (int b = 0; b < B; b++)
+#pragma unroll
+				for (int r = 0; r < R; r++) vv[c][b + r * B] = ll[c][(t + b * TT + r * (N / R)) * stride];
+#pragma unroll
+			for (int b = 0; b < B; b++)
+#pragma unroll
+				for (int r = 1; r < R; r++) {
+					C2<T> w = tw.w[plan_twoff<N>(s) + b * (R - 1) + r - 1];
+					vv[c][b + r * B] = (DIR < 0) ? cmul(vv[c][b + r * B], w) : cmulc(vv[c][b + r * B], w);
+				}
+		}
+	}
+#pragma unroll
+	for (int c = 0; c < 3; c++)
+#pragma unroll
+		for (int b = 0; b < B; b++) {
+			C2<T> u[R];
+#pragma unroll
+			for (int r = 0; r < R; r++) u[r] = vv[c][b + r * B];
+			Dft<R, DIR, T>::run(u);
+#pragma unroll
+			for (int r = 0; r < R; r++) vv[c][b + r * B] = u[r];
+		}
+	if constexpr (s < Plan<N>::S - 1) {
+		__syncthreads();
+#pragma unroll
+		for (int c = 0; c < 3; c++)
+#pragma unroll
+			for (int b = 0; b < B; b++) {
+				int j = t + b * TT;
+				int d = (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+				for (int r = 0; r < R; r++) ll[c][(d + r * Ns) * stride] = vv[c][b + r * B];
+			}
+		__syncthreads();
+		fft3_stages<N, s + 1, DIR, T>(v0, v1, v2, l0, l1, l2, stride, t, tw);
+	}
+}
+
+template <int N, int DIR, typename T>
+__device__ __forceinline__ void fft3(C2<T>* v0, C2<T>* v1, C2<T>* v2, C2<T>* l0, C2<T>* l1, C2<T>* l2,
+	int stride, int t, const Twiddles<N, T>& tw)
+{
+	fft3_stages<N, 0, DIR, T>(v0, v1, v2, l0, l1, l2, stride, t, tw);
+}
+
+} // namespace bm

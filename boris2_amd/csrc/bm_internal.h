@@ -1,0 +1,38 @@
+// Internal declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <string>
+
+#include "../../include/boris_mi355x.h"
+
+namespace bm {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define BM_HIP(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) return ::bm::hip_fail(e_, #call); } while (0)
+
+int device_cu_count();
+
+} // namespace bm
+
+struct bm_demag_s {
+	int nx, ny, nz;
+	int Nx, Ny, Nz, NC, NYH, NZH;
+	double h[3];
+	bool is2d;
+	float2* S = nullptr;      // spectra: 3 * nz * NC * Ny complex
+	float* K = nullptr;       // kernels (see bm_demag_copy_kernels)
+	long long kcount = 0;
+	double* ebuf = nullptr;   // energy partials (one per P5 work item)
+	int ecap = 0;
+	long long bytes = 0;
+	int ncu = 256;
+};
+
+namespace bm {
+// H_demag(M) into planar output H3n (3 planes of n floats)
+int demag_planar(bm_demag_s* d, const float* M, float* H3n, hipStream_t st);
+}

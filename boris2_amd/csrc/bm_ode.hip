@@ -1,0 +1,571 @@
+// LLG integration of one ferromagnetic mesh on MI355X: fused field + RK stage kernels and
+// the host-side step-control state machine.
+//
+// Each stage kernel evaluates, per cell, the effective field in the reference's module
+// order — Zeeman sets it (Boris/ZeemanCUDA.cu:31), demag adds (the planar H_demag written
+// by the convolution), exchange adds (6-neighbour Laplacian with free-boundary one-sided
+// differences, Boris/ExchangeCUDA.cu:15 + BorisLib/VEC_VC_Del.h:10-130), uniaxial
+// anisotropy adds (Boris/AnisotropyUniCUDA.cu:13) — then the LLG right-hand side
+// (Boris/DiffEqFM_Equations.cpp:12-24) and the stage update of the chosen method:
+//   RKDP54  Boris/DiffEqFM_EvalsCUDA_RKDP54.cu:15-330  (FSAL lte at stage 0)
+//   RK4     Boris/DiffEqFM_EvalsLLGCUDA.cu:416-641
+//   TEuler  Boris/DiffEqFM_EvalsLLGCUDA.cu:121-255 (Heun)
+//   AHeun   Boris/DiffEqFM_EvalsLLGCUDA.cu:299-363
+// The host drives stages exactly as ODECommon_Base::IterateCUDA does
+// (Boris/DiffEq_CommonBase_IterateCUDA.cpp:13-936): one lte read-back per adaptive step,
+// SetAdaptiveTimeStep in FP64 on the host (Boris/DiffEq_CommonBase_Iterate.cpp:15-59),
+// restore + re-prime on rejection.
+#include "bm_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+
+namespace {
+
+constexpr int kBS = 256;
+constexpr double kMU0 = 1.256637061435916e-6;    // BorisLib/Funcs_Math_base.h:9
+constexpr double kGAMMA = 2.212761569e5;         // BorisLib/Funcs_Math_base.h:16
+
+struct F3 { float x, y, z; };
+__device__ __forceinline__ F3 ld3(const float* p, size_t i) { return F3{ p[3 * i], p[3 * i + 1], p[3 * i + 2] }; }
+__device__ __forceinline__ void st3(float* p, size_t i, F3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return F3{ a.x + b.x, a.y + b.y, a.z + b.z }; }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return F3{ a.x - b.x, a.y - b.y, a.z - b.z }; }
+__device__ __forceinline__ F3 operator*(float s, F3 a) { return F3{ s * a.x, s * a.y, s * a.z }; }
+__device__ __forceinline__ F3 cross(F3 a, F3 b) { return F3{ a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x }; }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float norm(F3 a) { return sqrtf(dot(a, a)); }
+
+struct StageArgs {
+	float* M;
+	const float* Hd;        // planar demag field (3 planes of n) or null
+	float* sM1;
+	float* E[6];
+	const float* Ms_map;
+	float* Heff_out;        // optional AoS copy of the effective field
+	unsigned* lte;          // max lte as float bits
+	int nx, ny, nz;
+	float hx2, hy2, hz2;
+	float Ms, A, alpha, K1, K2, grel;
+	float ea[3], Ha[3];
+	float dT;
+	int modules;
+};
+
+enum { M_RKDP = 0, M_RK4 = 1, M_TEULER = 2, M_AHEUN = 3 };
+
+// Block-wide max of a non-negative float, folded into *dst with an integer atomicMax on
+// its bit pattern (order-independent, so deterministic).
+__device__ __forceinline__ void block_max_to(float v, unsigned* dst)
+{
+	__shared__ float red[kBS / 64];
+	for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+	if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		float m = red[0];
+		for (int q = 1; q < kBS / 64; q++) m = fmaxf(m, red[q]);
+		atomicMax(dst, __float_as_uint(m));
+	}
+}
+
+template <int METHOD, int STEP>
+__global__ void __launch_bounds__(kBS) k_stage(StageArgs a)
+{
+	const size_t n = (size_t)a.nx * a.ny * a.nz;
+	float lte_loc = 0.f;
+	for (size_t idx = blockIdx.x * (size_t)kBS + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kBS) {
+		const int i = (int)(idx % a.nx), j = (int)((idx / a.nx) % a.ny), k = (int)(idx / ((size_t)a.nx * a.ny));
+		const F3 m = ld3(a.M, idx);
+		const float Ms = a.Ms_map ? a.Ms_map[idx] : a.Ms;
+		// --- effective field, module order Zeeman, Demag, Exchange, Anisotropy
+		F3 H = (a.modules & BM_MOD_ZEEMAN) ? F3{ a.Ha[0], a.Ha[1], a.Ha[2] } : F3{ 0.f, 0.f, 0.f };
+		if (a.modules & BM_MOD_DEMAG) H = H + F3{ a.Hd[idx], a.Hd[n + idx], a.Hd[2 * n + idx] };
+		if (a.modules & BM_MOD_EXCHANGE) {
+			F3 lx{ 0, 0, 0 }, ly{ 0, 0, 0 }, lz{ 0, 0, 0 };
+			const size_t sy = a.nx, sz = (size_t)a.nx * a.ny;
+			if (a.nx > 1) {
+				if (i > 0 && i < a.nx - 1) lx = ld3(a.M, idx + 1) + ld3(a.M, idx - 1) - 2.f * m;
+				else if (i == 0) lx = ld3(a.M, idx + 1) - m;
+				else lx = ld3(a.M, idx - 1) - m;
+			}
+			if (a.ny > 1) {
+				if (j > 0 && j < a.ny - 1) ly = ld3(a.M, idx + sy) + ld3(a.M, idx - sy) - 2.f * m;
+				else if (j == 0) ly = ld3(a.M, idx + sy) - m;
+				else ly = ld3(a.M, idx - sy) - m;
+			}
+			if (a.nz > 1) {
+				if (k > 0 && k < a.nz - 1) lz = ld3(a.M, idx + sz) + ld3(a.M, idx - sz) - 2.f * m;
+				else if (k == 0) lz = ld3(a.M, idx + sz) - m;
+				else lz = ld3(a.M, idx - sz) - m;
+			}
+			F3 lap{ lx.x / a.hx2 + ly.x / a.hy2 + lz.x / a.hz2, lx.y / a.hx2 + ly.y / a.hy2 + lz.y / a.hz2, lx.z / a.hx2 + ly.z / a.hy2 + lz.z / a.hz2 };
+			const float pre = (2.f * a.A) / ((float)kMU0 * Ms * Ms);
+			H = H + pre * lap;
+		}
+		if (a.modules & BM_MOD_ANIS_UNI) {
+			const F3 e{ a.ea[0], a.ea[1], a.ea[2] };
+			const float d = dot(m, e) / Ms;
+			H = H + ((2.f / ((float)kMU0 * Ms)) * d * (a.K1 + 2.f * a.K2 * (1.f - d * d))) * e;
+		}
+		if (a.Heff_out) st3(a.Heff_out, idx, H);
+		// --- LLG
+		const F3 mxh = cross(m, H);
+		const F3 mu{ m.x / Ms, m.y / Ms, m.z / Ms };
+		const float pre = (float)(-kGAMMA) * a.grel / (1.f + a.alpha * a.alpha);
+		const F3 rhs = pre * (mxh + a.alpha * cross(mu, mxh));
+		const float dT = a.dT;
+		// --- stage update
+		if (METHOD == M_RKDP) {
+			if (STEP == 0) {
+				const F3 pred = ld3(a.sM1, idx) + dT * ((5179.f / 57600.f) * ld3(a.E[0], idx) + (7571.f / 16695.f) * ld3(a.E[2], idx)
+					+ (393.f / 640.f) * ld3(a.E[3], idx) - (92097.f / 339200.f) * ld3(a.E[4], idx) + (187.f / 2100.f) * ld3(a.E[5], idx)
+					+ (1.f / 40.f) * rhs);
+				lte_loc = fmaxf(lte_loc, norm(m - pred) / norm(m));
+				st3(a.E[0], idx, rhs);
+			}
+			else if (STEP == 1) {
+				st3(a.E[1], idx, rhs);
+				st3(a.M, idx, ld3(a.sM1, idx) + dT * ((3.f / 40.f) * ld3(a.E[0], idx) + (9.f / 40.f) * rhs));
+			}
+			else if (STEP == 2) {
+				st3(a.E[2], idx, rhs);
+				st3(a.M, idx, ld3(a.sM1, idx) + dT * ((44.f / 45.f) * ld3(a.E[0], idx) - (56.f / 15.f) * ld3(a.E[1], idx) + (32.f / 9.f) * rhs));
+			}
+			else if (STEP == 3) {
+				st3(a.E[3], idx, rhs);
+				st3(a.M, idx, ld3(a.sM1, idx) + dT * ((19372.f / 6561.f) * ld3(a.E[0], idx) - (25360.f / 2187.f) * ld3(a.E[1], idx)
+					+ (64448.f / 6561.f) * ld3(a.E[2], idx) - (212.f / 729.f) * rhs));
+			}
+			else if (STEP == 4) {
+				st3(a.E[4], idx, rhs);
+				st3(a.M, idx, ld3(a.sM1, idx) + dT * ((9017.f / 3168.f) * ld3(a.E[0], idx) - (355.f / 33.f) * ld3(a.E[1], idx)
+					+ (46732.f / 5247.f) * ld3(a.E[2], idx) + (49.f / 176.f) * ld3(a.E[3], idx) - (5103.f / 18656.f) * rhs));
+			}
+			else {
+				st3(a.E[5], idx, rhs);
+				F3 mn = ld3(a.sM1, idx) + dT * ((35.f / 384.f) * ld3(a.E[0], idx) + (500.f / 1113.f) * ld3(a.E[2], idx)
+					+ (125.f / 192.f) * ld3(a.E[3], idx) - (2187.f / 6784.f) * ld3(a.E[4], idx) + (11.f / 84.f) * rhs);
+				mn = (Ms / norm(mn)) * mn;
+				st3(a.M, idx, mn);
+			}
+		}
+		else if (METHOD == M_RK4) {
+			if (STEP == 0) { st3(a.sM1, idx, m); st3(a.E[0], idx, rhs); st3(a.M, idx, m + (dT / 2) * rhs); }
+			else if (STEP == 1) { st3(a.E[1], idx, rhs); st3(a.M, idx, ld3(a.sM1, idx) + (dT / 2) * rhs); }
+			else if (STEP == 2) { st3(a.E[2], idx, rhs); st3(a.M, idx, ld3(a.sM1, idx) + dT * rhs); }
+			else {
+				F3 mn = ld3(a.sM1, idx) + (dT / 6) * (ld3(a.E[0], idx) + 2.f * ld3(a.E[1], idx) + 2.f * ld3(a.E[2], idx) + rhs);
+				mn = (Ms / norm(mn)) * mn;
+				st3(a.M, idx, mn);
+			}
+		}
+		else {
+			if (STEP == 0) { st3(a.sM1, idx, m); st3(a.M, idx, m + dT * rhs); }
+			else {
+				F3 mn = 0.5f * (ld3(a.sM1, idx) + m + dT * rhs);
+				mn = (Ms / norm(mn)) * mn;
+				st3(a.M, idx, mn);
+				if (METHOD == M_AHEUN) lte_loc = fmaxf(lte_loc, norm(mn - m) / norm(mn));
+			}
+		}
+	}
+	if ((METHOD == M_RKDP && STEP == 0) || (METHOD == M_AHEUN && STEP == 1)) block_max_to(lte_loc, a.lte);
+}
+
+// RKDP54 advance after the step size is fixed: sM1 = M; M += E0 dT/5.
+__global__ void __launch_bounds__(kBS) k_rkdp_advance(float* __restrict__ M, float* __restrict__ sM1, const float* __restrict__ E0, size_t n3, float dT5)
+{
+	for (size_t q = blockIdx.x * (size_t)kBS + threadIdx.x; q < n3; q += (size_t)gridDim.x * kBS) {
+		float m = M[q];
+		sM1[q] = m;
+		M[q] = m + E0[q] * dT5;
+	}
+}
+
+__global__ void k_avg_partial(const float* __restrict__ M, const float* __restrict__ Ms_map, float Ms, size_t n, double* part)
+{
+	__shared__ double red[3][kBS];
+	double s[3] = { 0, 0, 0 };
+	for (size_t i = blockIdx.x * (size_t)kBS + threadIdx.x; i < n; i += (size_t)gridDim.x * kBS) {
+		const float ms = Ms_map ? Ms_map[i] : Ms;
+		s[0] += (double)(M[3 * i] / ms); s[1] += (double)(M[3 * i + 1] / ms); s[2] += (double)(M[3 * i + 2] / ms);
+	}
+	for (int c = 0; c < 3; c++) red[c][threadIdx.x] = s[c];
+	__syncthreads();
+	for (int o = kBS / 2; o > 0; o >>= 1) {
+		if (threadIdx.x < o) for (int c = 0; c < 3; c++) red[c][threadIdx.x] += red[c][threadIdx.x + o];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) for (int c = 0; c < 3; c++) part[3 * blockIdx.x + c] = red[c][0];
+}
+
+} // namespace
+
+struct bm_ode_s {
+	int method, mcode;
+	int nx, ny, nz;
+	size_t n;
+	double h[3];
+	bm_fm_params p;
+	bm_demag_t demag = nullptr;
+	float* M = nullptr;
+	const float* Ms_map = nullptr;
+	float* Hd = nullptr;
+	float* sM1 = nullptr;
+	float* E[6] = { nullptr };
+	float* Heff_keep = nullptr;   // internal AoS copy of the last effective field
+	bool keep_heff = false;
+	unsigned* d_lte = nullptr;
+	unsigned* h_lte = nullptr;    // pinned
+	double* d_part = nullptr;
+	double* h_part = nullptr;     // pinned
+	int npart = 0;
+	int ncu = 256;
+	// ODE state (ODECommon_Base)
+	double time = 0, stagetime = 0, dT = 0, dT_last = 0, lte = 0;
+	double err_high_fail = 1e-4, dT_increase = 2, dT_min = 1e-15, dT_max = 3e-12;
+	int order = 5, evalStep = 0;
+	long long iteration = 0, stageiteration = 0, rejections = 0, evaluations = 0;
+	bool primed = false, available = true;
+};
+
+namespace {
+
+int stage_grid(bm_ode_s* o) { return (int)std::max<size_t>(1, std::min<size_t>((o->n + kBS - 1) / kBS, (size_t)o->ncu * 16)); }
+
+void applied_field(const bm_fm_params& p, double t, double H[3])
+{
+	for (int c = 0; c < 3; c++) H[c] = p.Ha[c];
+	if (p.wave == 1) {
+		double x = p.w * (t - p.t0);
+		double s = (std::fabs(x) < 1e-300) ? 1.0 : std::sin(x) / x;
+		for (int c = 0; c < 3; c++) H[c] += p.amp * p.dir[c] * s;
+	}
+	else if (p.wave == 2) {
+		for (int c = 0; c < 3; c++) H[c] += p.slope[c] * t;
+	}
+}
+
+StageArgs make_args(bm_ode_s* o, float dT)
+{
+	StageArgs a;
+	std::memset(&a, 0, sizeof(a));
+	a.M = o->M; a.Hd = o->Hd; a.sM1 = o->sM1;
+	for (int q = 0; q < 6; q++) a.E[q] = o->E[q];
+	a.Ms_map = o->Ms_map; a.Heff_out = o->keep_heff ? o->Heff_keep : nullptr; a.lte = o->d_lte;
+	a.nx = o->nx; a.ny = o->ny; a.nz = o->nz;
+	a.hx2 = (float)o->h[0] * (float)o->h[0]; a.hy2 = (float)o->h[1] * (float)o->h[1]; a.hz2 = (float)o->h[2] * (float)o->h[2];
+	a.Ms = (float)o->p.Ms; a.A = (float)o->p.A; a.alpha = (float)o->p.alpha; a.K1 = (float)o->p.K1; a.K2 = (float)o->p.K2;
+	a.grel = (float)o->p.grel;
+	for (int c = 0; c < 3; c++) a.ea[c] = (float)o->p.ea[c];
+	double H[3]; applied_field(o->p, o->stagetime, H);
+	for (int c = 0; c < 3; c++) a.Ha[c] = (float)H[c];
+	a.dT = dT;
+	a.modules = o->p.modules;
+	return a;
+}
+
+int launch_stage(bm_ode_s* o, int step, float dT, hipStream_t st)
+{
+	StageArgs a = make_args(o, dT);
+	dim3 g(stage_grid(o)), b(kBS);
+	switch (o->mcode * 8 + step) {
+	case M_RKDP * 8 + 0: hipLaunchKernelGGL((k_stage<M_RKDP, 0>), g, b, 0, st, a); break;
+	case M_RKDP * 8 + 1: hipLaunchKernelGGL((k_stage<M_RKDP, 1>), g, b, 0, st, a); break;
+	case M_RKDP * 8 + 2: hipLaunchKernelGGL((k_stage<M_RKDP, 2>), g, b, 0, st, a); break;
+	case M_RKDP * 8 + 3: hipLaunchKernelGGL((k_stage<M_RKDP, 3>), g, b, 0, st, a); break;
+	case M_RKDP * 8 + 4: hipLaunchKernelGGL((k_stage<M_RKDP, 4>), g, b, 0, st, a); break;
+	case M_RKDP * 8 + 5: hipLaunchKernelGGL((k_stage<M_RKDP, 5>), g, b, 0, st, a); break;
+	case M_RK4 * 8 + 0: hipLaunchKernelGGL((k_stage<M_RK4, 0>), g, b, 0, st, a); break;
+	case M_RK4 * 8 + 1: hipLaunchKernelGGL((k_stage<M_RK4, 1>), g, b, 0, st, a); break;
+	case M_RK4 * 8 + 2: hipLaunchKernelGGL((k_stage<M_RK4, 2>), g, b, 0, st, a); break;
+	case M_RK4 * 8 + 3: hipLaunchKernelGGL((k_stage<M_RK4, 3>), g, b, 0, st, a); break;
+	case M_TEULER * 8 + 0: hipLaunchKernelGGL((k_stage<M_TEULER, 0>), g, b, 0, st, a); break;
+	case M_TEULER * 8 + 1: hipLaunchKernelGGL((k_stage<M_TEULER, 1>), g, b, 0, st, a); break;
+	case M_AHEUN * 8 + 0: hipLaunchKernelGGL((k_stage<M_AHEUN, 0>), g, b, 0, st, a); break;
+	case M_AHEUN * 8 + 1: hipLaunchKernelGGL((k_stage<M_AHEUN, 1>), g, b, 0, st, a); break;
+	default: return bm::fail(BM_ERR_ARG, "bad stage");
+	}
+	return BM_OK;
+}
+
+// Field update: the demag part (planar H_demag); the rest is fused into the stage kernel.
+int update_demag(bm_ode_s* o, hipStream_t st)
+{
+	o->evaluations++;
+	if (!(o->p.modules & BM_MOD_DEMAG)) return BM_OK;
+	return bm::demag_planar(o->demag, o->M, o->Hd, st);
+}
+
+// ODECommon_Base::SetAdaptiveTimeStep (Boris/DiffEq_CommonBase_Iterate.cpp:15-59).
+bool set_adaptive_time_step(bm_ode_s* o)
+{
+	if (o->dT_min == o->dT_max) return true;
+	if (o->lte > o->err_high_fail && o->dT > o->dT_min) {
+		o->time -= o->dT; o->stagetime -= o->dT; o->iteration--; o->stageiteration--;
+		double c = std::pow(o->err_high_fail * 0.8 / o->lte, 1.0 / (o->order + 1));
+		if (c < 0.01) c = 0.01;
+		o->dT *= c;
+		if (o->dT < o->dT_min) o->dT = o->dT_min;
+		o->available = false;
+		o->rejections++;
+		return false;
+	}
+	else if (o->lte > 0) {
+		double c = std::pow(o->err_high_fail * 0.8 / o->lte, 1.0 / (o->order + 1));
+		if (c > o->dT_increase) c = o->dT_increase;
+		if (c < 0.01) c = 0.01;
+		o->dT *= c;
+		if (o->dT < o->dT_min) o->dT = o->dT_min;
+		if (o->dT > o->dT_max) o->dT = o->dT_max;
+	}
+	return true;
+}
+
+int read_lte(bm_ode_s* o, hipStream_t st)
+{
+	BM_HIP(hipMemcpyAsync(o->h_lte, o->d_lte, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+	BM_HIP(hipStreamSynchronize(st));
+	float f;
+	std::memcpy(&f, o->h_lte, sizeof(float));
+	o->lte = (double)f;
+	return BM_OK;
+}
+
+int restore(bm_ode_s* o, hipStream_t st)
+{
+	BM_HIP(hipMemcpyAsync(o->M, o->sM1, o->n * 3 * sizeof(float), hipMemcpyDeviceToDevice, st));
+	return BM_OK;
+}
+
+void end_step(bm_ode_s* o)
+{
+	o->evalStep = 0; o->time += o->dT; o->stagetime += o->dT; o->iteration++; o->stageiteration++; o->available = true;
+}
+
+// One field update + one stage: ODECommon_Base::IterateCUDA.
+int iterate_stage(bm_ode_s* o, hipStream_t st)
+{
+	int rc = update_demag(o, st);
+	if (rc) return rc;
+	o->dT_last = o->dT;
+	switch (o->mcode) {
+	case M_RKDP: {
+		if (o->evalStep == 0) {
+			BM_HIP(hipMemsetAsync(o->d_lte, 0, sizeof(unsigned), st));
+			if ((rc = launch_stage(o, 0, (float)o->dT, st))) return rc;
+			o->available = false;
+			if (o->primed) {
+				o->dT_last = o->dT;
+				if ((rc = read_lte(o, st))) return rc;
+				if (!set_adaptive_time_step(o)) {
+					if ((rc = restore(o, st))) return rc;
+					o->primed = false;
+					break;
+				}
+			}
+			else o->primed = true;
+			const size_t n3 = o->n * 3;
+			hipLaunchKernelGGL(k_rkdp_advance, dim3(stage_grid(o)), dim3(kBS), 0, st, o->M, o->sM1, o->E[0], n3, (float)o->dT / 5);
+			o->evalStep++;
+		}
+		else if (o->evalStep < 5) {
+			if ((rc = launch_stage(o, o->evalStep, (float)o->dT, st))) return rc;
+			o->evalStep++;
+		}
+		else {
+			if ((rc = launch_stage(o, 5, (float)o->dT, st))) return rc;
+			end_step(o);
+		}
+	} break;
+	case M_RK4: {
+		if ((rc = launch_stage(o, o->evalStep, (float)o->dT, st))) return rc;
+		if (o->evalStep < 3) { o->evalStep++; o->available = false; }
+		else end_step(o);
+	} break;
+	case M_TEULER:
+	case M_AHEUN: {
+		if (o->evalStep == 0) {
+			if ((rc = launch_stage(o, 0, (float)o->dT, st))) return rc;
+			o->evalStep = 1; o->available = false;
+		}
+		else {
+			if (o->mcode == M_AHEUN) BM_HIP(hipMemsetAsync(o->d_lte, 0, sizeof(unsigned), st));
+			if ((rc = launch_stage(o, 1, (float)o->dT, st))) return rc;
+			end_step(o);
+			if (o->mcode == M_AHEUN) {
+				o->dT_last = o->dT;
+				if ((rc = read_lte(o, st))) return rc;
+				if (!set_adaptive_time_step(o)) { if ((rc = restore(o, st))) return rc; }
+			}
+		}
+	} break;
+	}
+	BM_HIP(hipGetLastError());
+	return BM_OK;
+}
+
+int advance_time(bm_ode_s* o, hipStream_t st)
+{
+	int rc;
+	do { if ((rc = iterate_stage(o, st))) return rc; } while (!o->available);
+	return BM_OK;
+}
+
+void free_all(bm_ode_s* o)
+{
+	if (o->Hd) (void)hipFree(o->Hd);
+	if (o->sM1) (void)hipFree(o->sM1);
+	for (int q = 0; q < 6; q++) if (o->E[q]) (void)hipFree(o->E[q]);
+	if (o->Heff_keep) (void)hipFree(o->Heff_keep);
+	if (o->d_lte) (void)hipFree(o->d_lte);
+	if (o->h_lte) (void)hipHostFree(o->h_lte);
+	if (o->d_part) (void)hipFree(o->d_part);
+	if (o->h_part) (void)hipHostFree(o->h_part);
+}
+
+} // namespace
+
+using bm::fail;
+
+extern "C" int bm_ode_create(int method, int nx, int ny, int nz, double hx, double hy, double hz,
+	const bm_fm_params* p, bm_demag_t demag, float* M, bm_ode_t* out)
+{
+	if (!out || !p || !M || nx < 1 || ny < 1 || nz < 1) return fail(BM_ERR_ARG, "bm_ode_create: bad arguments");
+	*out = nullptr;
+	if ((p->modules & BM_MOD_DEMAG) && (!demag || demag->nx != nx || demag->ny != ny || demag->nz != nz))
+		return fail(BM_ERR_ARG, "bm_ode_create: demag handle missing or of another mesh size");
+	bm_ode_s* o = new bm_ode_s();
+	o->method = method; o->nx = nx; o->ny = ny; o->nz = nz; o->n = (size_t)nx * ny * nz;
+	o->h[0] = hx; o->h[1] = hy; o->h[2] = hz; o->p = *p; o->demag = demag; o->M = M;
+	o->ncu = bm::device_cu_count();
+	int nE = 0;
+	// method defaults: Boris/DiffEq_CommonBase_Control.cpp:10-140 with Boris/DiffEq_Defs.h
+	switch (method) {
+	case BM_EVAL_RKDP54: o->mcode = M_RKDP; nE = 6; o->dT = 0.5e-12; o->err_high_fail = 1e-4; o->dT_increase = 2; o->dT_max = 3e-12; o->dT_min = 1e-15; o->order = 5; break;
+	case BM_EVAL_RK4: o->mcode = M_RK4; nE = 3; o->dT = 0.5e-12; o->order = 4; break;
+	case BM_EVAL_TEULER: o->mcode = M_TEULER; o->dT = 0.5e-13; o->order = 2; break;
+	case BM_EVAL_AHEUN: o->mcode = M_AHEUN; o->dT = 0.5e-13; o->err_high_fail = 1e-4; o->dT_increase = 2; o->dT_max = 1e-12; o->dT_min = 1e-15; o->order = 2; break;
+	default: delete o; return fail(BM_ERR_ARG, "bm_ode_create: unsupported evaluation method");
+	}
+	o->dT_last = o->dT;
+	const size_t b3 = o->n * 3 * sizeof(float);
+	bool ok = hipMalloc(&o->sM1, b3) == hipSuccess;
+	for (int q = 0; q < nE && ok; q++) ok = hipMalloc(&o->E[q], b3) == hipSuccess && hipMemset(o->E[q], 0, b3) == hipSuccess;
+	if (ok && (p->modules & BM_MOD_DEMAG)) ok = hipMalloc(&o->Hd, b3) == hipSuccess;
+	if (ok) ok = hipMalloc(&o->Heff_keep, b3) == hipSuccess;
+	if (ok) ok = hipMalloc(&o->d_lte, sizeof(unsigned)) == hipSuccess;
+	if (ok) ok = hipHostMalloc(&o->h_lte, sizeof(unsigned)) == hipSuccess;
+	o->npart = (int)std::max<size_t>(1, std::min<size_t>((o->n + kBS - 1) / kBS, 1024));
+	if (ok) ok = hipMalloc(&o->d_part, (size_t)o->npart * 3 * sizeof(double)) == hipSuccess;
+	if (ok) ok = hipHostMalloc(&o->h_part, (size_t)o->npart * 3 * sizeof(double)) == hipSuccess;
+	if (ok) ok = hipMemset(o->sM1, 0, b3) == hipSuccess;
+	if (!ok) { free_all(o); delete o; return fail(BM_ERR_NOMEM, "bm_ode_create: device allocation failed"); }
+	*out = o;
+	return BM_OK;
+}
+
+extern "C" int bm_ode_destroy(bm_ode_t o)
+{
+	if (!o) return BM_OK;
+	(void)hipDeviceSynchronize();
+	free_all(o);
+	delete o;
+	return BM_OK;
+}
+
+extern "C" int bm_ode_set_ms_map(bm_ode_t o, const float* Ms_map)
+{
+	if (!o) return fail(BM_ERR_ARG, "null handle");
+	o->Ms_map = Ms_map;
+	return BM_OK;
+}
+
+extern "C" int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p)
+{
+	if (!o || !p) return fail(BM_ERR_ARG, "null argument");
+	if ((p->modules & BM_MOD_DEMAG) && !o->Hd) return fail(BM_ERR_STATE, "demag was not enabled at creation");
+	o->p = *p;
+	return BM_OK;
+}
+
+extern "C" int bm_ode_set_dt(bm_ode_t o, double dT)
+{
+	if (!o || !(dT > 0)) return fail(BM_ERR_ARG, "bad dT");
+	o->dT = dT; o->dT_last = dT;
+	return BM_OK;
+}
+
+extern "C" int bm_ode_set_astep(bm_ode_t o, double err, double inc, double dtmin, double dtmax)
+{
+	if (!o) return fail(BM_ERR_ARG, "null handle");
+	if (err > 0) o->err_high_fail = err;
+	if (inc > 0) o->dT_increase = inc;
+	if (dtmin > 0) o->dT_min = dtmin;
+	if (dtmax > 0) o->dT_max = dtmax;
+	return BM_OK;
+}
+
+extern "C" int bm_ode_advance_time(bm_ode_t o, void* stream)
+{
+	if (!o) return fail(BM_ERR_ARG, "null handle");
+	return advance_time(o, (hipStream_t)stream);
+}
+
+extern "C" int bm_ode_run_to_iteration(bm_ode_t o, long long iteration, void* stream)
+{
+	if (!o) return fail(BM_ERR_ARG, "null handle");
+	while (o->iteration < iteration) { int rc = advance_time(o, (hipStream_t)stream); if (rc) return rc; }
+	return BM_OK;
+}
+
+extern "C" int bm_ode_iterate_stage(bm_ode_t o, void* stream)
+{
+	if (!o) return fail(BM_ERR_ARG, "null handle");
+	return iterate_stage(o, (hipStream_t)stream);
+}
+
+extern "C" int bm_ode_restore(bm_ode_t o, void* stream)
+{
+	if (!o) return fail(BM_ERR_ARG, "null handle");
+	return restore(o, (hipStream_t)stream);
+}
+
+extern "C" int bm_ode_get_state(bm_ode_t o, bm_ode_state* s)
+{
+	if (!o || !s) return fail(BM_ERR_ARG, "null argument");
+	s->time = o->time; s->stagetime = o->stagetime; s->dT = o->dT; s->dT_last = o->dT_last; s->lte = o->lte;
+	s->iteration = o->iteration; s->stageiteration = o->stageiteration; s->rejections = o->rejections; s->evaluations = o->evaluations;
+	s->evalStep = o->evalStep; s->primed = o->primed; s->available = o->available;
+	return BM_OK;
+}
+
+extern "C" int bm_ode_keep_heff(bm_ode_t o, int on)
+{
+	if (!o) return fail(BM_ERR_ARG, "null handle");
+	o->keep_heff = on != 0;
+	return BM_OK;
+}
+
+extern "C" int bm_ode_copy_heff(bm_ode_t o, float* H, void* stream)
+{
+	if (!o || !H) return fail(BM_ERR_ARG, "null argument");
+	if (!o->keep_heff) return fail(BM_ERR_STATE, "bm_ode_copy_heff: enable bm_ode_keep_heff first");
+	BM_HIP(hipMemcpyAsync(H, o->Heff_keep, o->n * 3 * sizeof(float), hipMemcpyDeviceToDevice, (hipStream_t)stream));
+	return BM_OK;
+}
+
+extern "C" int bm_ode_average_m(bm_ode_t o, double* out3, void* stream)
+{
+	if (!o || !out3) return fail(BM_ERR_ARG, "null argument");
+	hipStream_t st = (hipStream_t)stream;
+	hipLaunchKernelGGL(k_avg_partial, dim3(o->npart), dim3(kBS), 0, st, o->M, o->Ms_map, (float)o->p.Ms, o->n, o->d_part);
+	BM_HIP(hipMemcpyAsync(o->h_part, o->d_part, (size_t)o->npart * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+	BM_HIP(hipStreamSynchronize(st));
+	double s[3] = { 0, 0, 0 };
+	for (int b = 0; b < o->npart; b++) for (int c = 0; c < 3; c++) s[c] += o->h_part[3 * b + c];
+	for (int c = 0; c < 3; c++) out3[c] = s[c] / (double)o->n;
+	return BM_OK;
+}

@@ -1,0 +1,146 @@
+/*
+ * boris_mi355x.h — C-ABI of the MI355X demag-convolution + LLG-stage hot path.
+ *
+ * Plain pointers and sizes only.  All device pointers are HIP device memory owned by the
+ * caller unless stated otherwise; `stream` is a hipStream_t passed as void* (NULL = the
+ * default stream).  Every call returns an int status (BM_OK = 0); bm_last_error() gives a
+ * message for the calling thread's last failure.  No exceptions cross this boundary.
+ *
+ * Data layout (the reference's cuVEC<cuReal3> layout, BorisCUDALib/cuVEC.h:58-85):
+ * a vector field over an nx*ny*nz mesh is float[3*n], AoS (x,y,z per cell), cell index
+ * i + j*nx + k*nx*ny.  M is in A/m with |M| = Ms; fields are in A/m.
+ */
+#ifndef BORIS_MI355X_H
+#define BORIS_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+	BM_OK = 0,
+	BM_ERR_ARG = 1,      /* invalid argument */
+	BM_ERR_SIZE = 2,     /* mesh size has no FFT plan in this build */
+	BM_ERR_NOMEM = 3,    /* device allocation failed (BERROR_OUTOFGPUMEMORY_CRIT) */
+	BM_ERR_HIP = 4,      /* HIP runtime error */
+	BM_ERR_STATE = 5     /* call not valid in the object's current state */
+};
+
+const char* bm_last_error(void);
+/* 1 if a device is visible and the kernels in this library can run on it. */
+int bm_device_ok(void);
+
+/* ------------------------------------------------------------------------------------
+ * Demag convolution of one ferromagnetic mesh (no PBC).
+ *
+ * Replaces: DemagCUDA::Initialize + ConvolutionDataCUDA::SetConvolutionDimensions +
+ * DemagKernelCUDA::Calculate_Demag_Kernels (Boris/DemagCUDA.cpp:31,
+ * Boris/ConvolutionDataCUDA.cpp:24, Boris/DemagKernelCUDA.cpp:58/248/358/603):
+ * computes the FP64 Newell tensor and its FFT on the GPU once, stores the FP32 kernels
+ * and allocates the spectral scratch.  Sizes: N.x = 2nx, N.y = 2ny, N.z = 1 (nz = 1) or
+ * the next power of two >= 2nz, as the reference's GPU path does.
+ */
+typedef struct bm_demag_s* bm_demag_t;
+
+int bm_demag_create(int nx, int ny, int nz, double hx, double hy, double hz, int flags, bm_demag_t* out);
+int bm_demag_destroy(bm_demag_t d);
+/* N.x, N.y, N.z of the zero-padded convolution and the device bytes held. */
+int bm_demag_info(bm_demag_t d, int* N3, long long* device_bytes);
+
+/*
+ * Heff (=, +=) H_demag(M).
+ * Replaces: ConvolutionCUDA<DemagCUDA,DemagKernelCUDA>::Convolute(M, Heff, energy,
+ * get_energy, clearOut) (Boris/ConvolutionCUDA.h:217-223), called from
+ * DemagCUDA::UpdateField (Boris/DemagCUDA.cpp:106-121).  clear_out = 1 sets Heff,
+ * 0 adds to it.  If energy is non-NULL it receives (device float, overwritten)
+ * -mu0/(2 n) * sum_cells M.H_demag, the reference's demag energy density.
+ */
+int bm_demag_convolute(bm_demag_t d, const float* M, float* Heff, int clear_out, float* energy, void* stream);
+
+/* H_demag(M) written as three planes Hx[n], Hy[n], Hz[n] (the layout the fused LLG
+ * stage kernels read).  No reference counterpart: the split the reference makes between
+ * Convolute and the field modules, with the output planar instead of interleaved. */
+int bm_demag_convolute_planar(bm_demag_t d, const float* M, float* H3n, void* stream);
+
+/* Copies the FP32 demag kernels to host memory (test hook).  3D: 6 planes Kxx, Kyy, Kzz,
+ * Kxy, Kxz, Kyz each (N.z/2+1)*(N.x/2+1)*(N.y/2+1), index (kz*(N.x/2+1)+kx)*(N.y/2+1)+ky.
+ * 2D: 4 planes Kxx, Kyy, Kzz, Kxy each (N.x/2+1)*(N.y/2+1), index kx*(N.y/2+1)+ky.
+ * Values equal the reference's Kdiag/Kodiag/K2D_odiag (Boris/DemagKernel.cpp:171-516). */
+int bm_demag_copy_kernels(bm_demag_t d, float* host_out, long long count);
+
+/* ------------------------------------------------------------------------------------
+ * LLG integrator for one ferromagnetic mesh: Zeeman (sets), demag, exchange and uniaxial
+ * anisotropy (add) fused into each RK stage.
+ *
+ * Replaces: DifferentialEquationFMCUDA (Boris/DiffEqFMCUDA.h:38-137), its stage kernels
+ * (Boris/DiffEqFM_EvalsCUDA_RKDP54.cu:15-330, Boris/DiffEqFM_EvalsLLGCUDA.cu:121-641),
+ * the field modules' UpdateField (Boris/ZeemanCUDA.cu:215, Boris/ExchangeCUDA.cu:122,
+ * Boris/AnisotropyUniCUDA.cu:112) and ODECommon_Base::IterateCUDA
+ * (Boris/DiffEq_CommonBase_IterateCUDA.cpp:13) with SetAdaptiveTimeStep
+ * (Boris/DiffEq_CommonBase_Iterate.cpp:15-59).
+ */
+typedef struct bm_ode_s* bm_ode_t;
+
+/* evaluation methods: the reference's EVAL_ enum values (Boris/DiffEq_Defs.h) */
+enum { BM_EVAL_TEULER = 1, BM_EVAL_RK4 = 2, BM_EVAL_AHEUN = 7, BM_EVAL_RKDP54 = 9 };
+
+/* module switches */
+enum { BM_MOD_DEMAG = 1, BM_MOD_EXCHANGE = 2, BM_MOD_ANIS_UNI = 4, BM_MOD_ZEEMAN = 8 };
+
+typedef struct {
+	double Ms;              /* A/m (used when Ms_map is NULL) */
+	double A;               /* exchange stiffness J/m */
+	double alpha;           /* Gilbert damping */
+	double K1, K2;          /* uniaxial anisotropy J/m^3 */
+	double ea[3];           /* easy axis (unit) */
+	double grel;            /* gyromagnetic ratio scaling (1) */
+	/* applied field H(t) = Ha + amp*dir*sinc(w*(t - t0)) [wave 1] or Ha + slope*t [wave 2],
+	 * evaluated at the start of each step, A/m */
+	double Ha[3];
+	int wave;
+	double amp, w, t0, dir[3], slope[3];
+	int modules;            /* BM_MOD_* bits */
+} bm_fm_params;
+
+/* M: caller-owned device float[3n], the mesh magnetisation, read and updated in place.
+ * demag may be NULL when BM_MOD_DEMAG is not set. */
+int bm_ode_create(int method, int nx, int ny, int nz, double hx, double hy, double hz,
+	const bm_fm_params* p, bm_demag_t demag, float* M, bm_ode_t* out);
+int bm_ode_destroy(bm_ode_t o);
+/* per-cell Ms map (device float[n], caller-owned, NULL to use p->Ms) */
+int bm_ode_set_ms_map(bm_ode_t o, const float* Ms_map);
+int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p);
+/* dT, and the adaptive controls err_fail, dT_increase, dT_min, dT_max (values <= 0 keep
+ * the current ones); the defaults are the reference's per-method defaults. */
+int bm_ode_set_dt(bm_ode_t o, double dT);
+int bm_ode_set_astep(bm_ode_t o, double err_fail, double dT_increase, double dT_min, double dT_max);
+
+/* One SuperMesh::AdvanceTimeCUDA call (Boris/SuperMeshSimulation.cpp:122-149): field
+ * updates and stages until the step is solved (adaptive methods may reject and redo). */
+int bm_ode_advance_time(bm_ode_t o, void* stream);
+/* Advance until the accepted-iteration counter reaches `iteration`. */
+int bm_ode_run_to_iteration(bm_ode_t o, long long iteration, void* stream);
+/* One field update + one stage (ODECommon_Base::IterateCUDA granularity). */
+int bm_ode_iterate_stage(bm_ode_t o, void* stream);
+/* M <- sM1 (DifferentialEquationFMCUDA::RestoreMagnetization). */
+int bm_ode_restore(bm_ode_t o, void* stream);
+
+typedef struct {
+	double time, stagetime, dT, dT_last, lte;
+	long long iteration, stageiteration, rejections, evaluations;
+	int evalStep, primed, available;
+} bm_ode_state;
+int bm_ode_get_state(bm_ode_t o, bm_ode_state* st);
+/* Keep a copy of the effective field of each stage (off by default: it costs one more
+ * array write per stage).  Test hook; no reference counterpart. */
+int bm_ode_keep_heff(bm_ode_t o, int on);
+/* Effective field of the last stage (device float[3n], caller-owned); needs keep_heff. */
+int bm_ode_copy_heff(bm_ode_t o, float* Heff_out, void* stream);
+/* <M>/Ms averaged over cells (host double[3]); synchronises the stream. */
+int bm_ode_average_m(bm_ode_t o, double* out3, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

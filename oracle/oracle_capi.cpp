@@ -1,0 +1,226 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY.  C-ABI over the oracle for the Python tests
+// (ctypes) and bench.py's cpu_baseline leg.  Nothing in the product links it.
+#include "oracle_core.hpp"
+
+#include <cstring>
+#include <exception>
+
+using namespace oracle;
+
+extern "C" {
+
+// --- tensors and kernels ---------------------------------------------------------------
+
+// Full-N tensors, -N convention.  Ddiag, Dodiag: Nx*Ny*Nz*3 doubles.
+int or_tensor3d(int nx, int ny, int nz, int Nx, int Ny, int Nz, double hx, double hy, double hz,
+	double* Ddiag, double* Dodiag)
+{
+	try {
+		double hm = std::max(hx, std::max(hy, hz));
+		std::vector<double> Dd, Do;
+		tensor3d(nx, ny, nz, Nx, Ny, Nz, hx / hm, hy / hm, hz / hm, Dd, Do);
+		std::memcpy(Ddiag, Dd.data(), Dd.size() * sizeof(double));
+		std::memcpy(Dodiag, Do.data(), Do.size() * sizeof(double));
+		return 0;
+	}
+	catch (...) { return 1; }
+}
+
+// Ddiag: Nx*Ny*3, Dxy: Nx*Ny
+int or_tensor2d(int nx, int ny, int Nx, int Ny, double hx, double hy, double hz, double* Ddiag, double* Dxy)
+{
+	try {
+		double hm = std::max(hx, std::max(hy, hz));
+		std::vector<double> Dd, Do;
+		tensor2d(nx, ny, Nx, Ny, hx / hm, hy / hm, hz / hm, Dd, Do);
+		std::memcpy(Ddiag, Dd.data(), Dd.size() * sizeof(double));
+		std::memcpy(Dxy, Do.data(), Do.size() * sizeof(double));
+		return 0;
+	}
+	catch (...) { return 1; }
+}
+
+void or_self_demag(double hx, double hy, double hz, double* out3) { self_demag(hx, hy, hz, out3); }
+double or_newell_f(double x, double y, double z) { return nw_f(x, y, z); }
+double or_newell_g(double x, double y, double z) { return nw_g(x, y, z); }
+double or_asympt_diag(double hx, double hy, double hz, double x, double y, double z) { return AsymptDiag(hx, hy, hz).eval(x, y, z); }
+double or_asympt_offdiag(double hx, double hy, double hz, double x, double y, double z) { return AsymptOffDiag(hx, hy, hz).eval(x, y, z); }
+
+// Kernels: 3D Kdiag, Kodiag each (Nx/2+1)(Ny/2+1)(Nz/2+1)*3; 2D Kdiag (..)*3, Kodiag (..)*1
+int or_kernels(int nx, int ny, int nz, int Nx, int Ny, int Nz, double hx, double hy, double hz, double* Kdiag, double* Kodiag)
+{
+	try {
+		Kernels K = make_kernels(nx, ny, nz, Nx, Ny, Nz, hx, hy, hz);
+		std::memcpy(Kdiag, K.Kdiag.data(), K.Kdiag.size() * sizeof(double));
+		std::memcpy(Kodiag, K.Kodiag.data(), K.Kodiag.size() * sizeof(double));
+		return 0;
+	}
+	catch (...) { return 1; }
+}
+
+// Complex FFT of length n (interleaved re,im doubles), sign -1 / +1, unnormalised.
+int or_fft(int n, double* data, int sign)
+{
+	Fft<double> f(n);
+	std::vector<std::complex<double>> a(n), s(n);
+	for (int i = 0; i < n; i++) a[i] = { data[2 * i], data[2 * i + 1] };
+	f.run(a.data(), s.data(), sign);
+	for (int i = 0; i < n; i++) { data[2 * i] = a[i].real(); data[2 * i + 1] = a[i].imag(); }
+	return 0;
+}
+
+// --- demag convolution -----------------------------------------------------------------
+
+void* or_demag_create(int nx, int ny, int nz, double hx, double hy, double hz, int Nz_override, int fp32)
+{
+	try {
+		if (fp32) return new Demag<float>(nx, ny, nz, hx, hy, hz, Nz_override);
+		return new Demag<double>(nx, ny, nz, hx, hy, hz, Nz_override);
+	}
+	catch (...) { return nullptr; }
+}
+
+void or_demag_destroy(void* h, int fp32)
+{
+	if (fp32) delete (Demag<float>*)h; else delete (Demag<double>*)h;
+}
+
+// M, H: n*3 doubles; in FP32 mode values are rounded to float on entry.
+int or_demag_convolute(void* h, int fp32, const double* M, double* H, int clear_out)
+{
+	if (!fp32) { ((Demag<double>*)h)->convolute(M, H, clear_out != 0); return 0; }
+	Demag<float>* d = (Demag<float>*)h;
+	size_t n = (size_t)d->nx * d->ny * d->nz * 3;
+	std::vector<float> m(M, M + n), hh(H, H + n);
+	d->convolute(m.data(), hh.data(), clear_out != 0);
+	for (size_t i = 0; i < n; i++) H[i] = hh[i];
+	return 0;
+}
+
+// --- full simulation (fields + ODE) ----------------------------------------------------
+
+struct OrSim { int fp32; void* s; };
+
+// params: Ms, A, alpha, K1, K2, ea[3], Ha[3], wave, amp, w, t0, dir[3], slope[3] (22 doubles)
+// flags: bit0 demag, bit1 exchange, bit2 anisotropy, bit3 zeeman
+void* or_sim_create(int method, int nx, int ny, int nz, double hx, double hy, double hz,
+	const double* prm, int flags, int Nz_override, int fp32)
+{
+	try {
+		Params p;
+		p.Ms = prm[0]; p.A = prm[1]; p.alpha = prm[2]; p.K1 = prm[3]; p.K2 = prm[4];
+		for (int c = 0; c < 3; c++) { p.ea[c] = prm[5 + c]; p.Ha[c] = prm[8 + c]; }
+		p.wave = (int)prm[11]; p.amp = prm[12]; p.w = prm[13]; p.t0 = prm[14];
+		for (int c = 0; c < 3; c++) { p.dir[c] = prm[15 + c]; p.slope[c] = prm[18 + c]; }
+		p.demag = flags & 1; p.exchange = (flags >> 1) & 1; p.anisotropy = (flags >> 2) & 1; p.zeeman = (flags >> 3) & 1;
+		OrSim* o = new OrSim;
+		o->fp32 = fp32;
+		if (fp32) o->s = new Sim<float>(method, nx, ny, nz, hx, hy, hz, p, Nz_override);
+		else o->s = new Sim<double>(method, nx, ny, nz, hx, hy, hz, p, Nz_override);
+		return o;
+	}
+	catch (...) { return nullptr; }
+}
+
+void or_sim_destroy(void* h)
+{
+	OrSim* o = (OrSim*)h;
+	if (o->fp32) delete (Sim<float>*)o->s; else delete (Sim<double>*)o->s;
+	delete o;
+}
+
+#define SIM_DISPATCH(o, body) do { if ((o)->fp32) { Sim<float>* s = (Sim<float>*)(o)->s; body; } else { Sim<double>* s = (Sim<double>*)(o)->s; body; } } while (0)
+
+void or_sim_set_M(void* h, const double* M)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, { for (size_t i = 0; i < s->n * 3; i++) s->M[i] = M[i]; });
+}
+
+void or_sim_get_M(void* h, double* M)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, { for (size_t i = 0; i < s->n * 3; i++) M[i] = s->M[i]; });
+}
+
+void or_sim_get_Heff(void* h, double* H)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, { for (size_t i = 0; i < s->n * 3; i++) H[i] = s->Heff[i]; });
+}
+
+void or_sim_set_Msmap(void* h, const double* Ms)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, { if (!Ms) s->Msmap.clear(); else { s->Msmap.resize(s->n); for (size_t i = 0; i < s->n; i++) s->Msmap[i] = Ms[i]; } });
+}
+
+// dT0 (<=0 keeps the method default)
+void or_sim_set_dt(void* h, double dT)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, { s->dT = dT; s->dT_last = dT; });
+}
+
+// Step-control overrides: err_fail, dT_increase, dT_min, dT_max (<=0 keeps defaults).
+void or_sim_set_astep(void* h, double err, double inc, double dtmin, double dtmax)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, { if (err > 0) s->err_high_fail = err; if (inc > 0) s->dT_increase = inc; if (dtmin > 0) s->dT_min = dtmin; if (dtmax > 0) s->dT_max = dtmax; });
+}
+
+void or_sim_update_fields(void* h)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, { s->update_fields(); });
+}
+
+void or_sim_iterate_stage(void* h)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, { s->iterate_stage(); });
+}
+
+// Advance until the reference's iteration counter reaches `iterations`.
+void or_sim_run_to_iteration(void* h, int iterations)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, { while (s->iteration < iterations) s->advance_time(); });
+}
+
+void or_sim_advance_time(void* h)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, { s->advance_time(); });
+}
+
+// state: time, dT, lte, iteration, evalStep, primed, rejections, evaluations, stagetime
+void or_sim_state(void* h, double* st)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, {
+		st[0] = s->time; st[1] = s->dT; st[2] = s->lte; st[3] = s->iteration; st[4] = s->evalStep;
+		st[5] = s->primed; st[6] = (double)s->rejections; st[7] = (double)s->evaluations; st[8] = s->stagetime; });
+}
+
+void or_sim_average_m(void* h, double* out3)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, { s->average_m(out3); });
+}
+
+// Reference SetRandomMag restated (single-thread BorisRand sequence).
+void or_set_random_mag(double* M, long long n, unsigned seed)
+{
+	std::vector<double> m(M, M + n * 3);
+	set_random_mag(m, (size_t)n, seed);
+	std::memcpy(M, m.data(), sizeof(double) * n * 3);
+}
+
+void or_borisrand(unsigned seed, int count, double* out)
+{
+	BorisRand g(seed);
+	for (int i = 0; i < count; i++) out[i] = g.rand();
+}
+
+} // extern "C"

@@ -1,0 +1,89 @@
+// Test infrastructure only: a tiny driver that links the reference's own Newell-tensor
+// sources (Boris/DemagTFunc*.cpp, compiled from where they lie by oracle/Makefile) and
+// writes the FP64 demag tensors it produces, so the oracle's restatement can be pinned
+// bit-for-bit against them.  Nothing in the product links or loads this file.
+//
+// Usage:
+//   ref_dump tensor3d nx ny nz Nx Ny Nz hx hy hz out.bin
+//        -> Ddiag (Nx*Ny*Nz DBL3) then Dodiag (Nx*Ny*Nz DBL3), x fastest
+//   ref_dump tensor2d nx ny Nx Ny hx hy hz out.bin
+//        -> Ddiag (Nx*Ny DBL3) then Dodiag (Nx*Ny doubles)
+//   ref_dump self hx hy hz out.bin      -> SelfDemag (3 doubles)
+//   ref_dump single x y z hx hy hz out.bin -> Ldia_single (3) then Lodia_single (3)
+//
+// Reference entry points used: DemagTFunc::CalcDiagTens3D/CalcOffDiagTens3D/
+// CalcDiagTens2D/CalcOffDiagTens2D (Boris/DemagTFunc.cpp:21,95,172,237),
+// DemagTFunc::SelfDemag (Boris/DemagTFunc_Self.cpp:46),
+// DemagTFunc::Ldia_single/Lodia_single (Boris/DemagTFunc_Test.cpp).
+
+#include "stdafx.h"
+#include "DemagTFunc.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+static void write_doubles(const char* path, const std::vector<double>& v)
+{
+	FILE* f = fopen(path, "wb");
+	if (!f) { perror(path); exit(2); }
+	if (fwrite(v.data(), sizeof(double), v.size(), f) != v.size()) { perror("fwrite"); exit(2); }
+	fclose(f);
+}
+
+int main(int argc, char** argv)
+{
+	if (argc < 2) { fprintf(stderr, "usage: ref_dump tensor3d|tensor2d|self|single ...\n"); return 1; }
+	DemagTFunc dtf;
+	std::string mode = argv[1];
+
+	if (mode == "tensor3d" && argc == 12) {
+		INT3 n(atoi(argv[2]), atoi(argv[3]), atoi(argv[4]));
+		INT3 N(atoi(argv[5]), atoi(argv[6]), atoi(argv[7]));
+		DBL3 h(atof(argv[8]), atof(argv[9]), atof(argv[10]));
+		DBL3 hr = h / maximum(h.x, h.y, h.z);
+		VEC<DBL3> Dd, Do;
+		if (!Dd.resize(SZ3(N.x, N.y, N.z)) || !Do.resize(SZ3(N.x, N.y, N.z))) return 3;
+		if (!dtf.CalcDiagTens3D(Dd, n, N, hr, true)) return 4;
+		if (!dtf.CalcOffDiagTens3D(Do, n, N, hr)) return 4;
+		std::vector<double> out;
+		out.reserve((size_t)6 * Dd.linear_size());
+		for (size_t i = 0; i < Dd.linear_size(); i++) { out.push_back(Dd[i].x); out.push_back(Dd[i].y); out.push_back(Dd[i].z); }
+		for (size_t i = 0; i < Do.linear_size(); i++) { out.push_back(Do[i].x); out.push_back(Do[i].y); out.push_back(Do[i].z); }
+		write_doubles(argv[11], out);
+		return 0;
+	}
+	if (mode == "tensor2d" && argc == 10) {
+		INT3 n(atoi(argv[2]), atoi(argv[3]), 1);
+		INT3 N(atoi(argv[4]), atoi(argv[5]), 1);
+		DBL3 h(atof(argv[6]), atof(argv[7]), atof(argv[8]));
+		DBL3 hr = h / maximum(h.x, h.y, h.z);
+		VEC<DBL3> Dd;
+		if (!Dd.resize(SZ3(N.x, N.y, 1))) return 3;
+		std::vector<double> Do((size_t)N.x * N.y, 0.0);
+		if (!dtf.CalcDiagTens2D(Dd, n, N, hr, true)) return 4;
+		if (!dtf.CalcOffDiagTens2D(Do, n, N, hr)) return 4;
+		std::vector<double> out;
+		for (size_t i = 0; i < Dd.linear_size(); i++) { out.push_back(Dd[i].x); out.push_back(Dd[i].y); out.push_back(Dd[i].z); }
+		for (double d : Do) out.push_back(d);
+		write_doubles(argv[9], out);
+		return 0;
+	}
+	if (mode == "self" && argc == 6) {
+		DBL3 h(atof(argv[2]), atof(argv[3]), atof(argv[4]));
+		DBL3 s = dtf.SelfDemag(h, true);
+		write_doubles(argv[5], { s.x, s.y, s.z });
+		return 0;
+	}
+	if (mode == "single" && argc == 9) {
+		DBL3 d(atof(argv[2]), atof(argv[3]), atof(argv[4]));
+		DBL3 h(atof(argv[5]), atof(argv[6]), atof(argv[7]));
+		DBL3 a = dtf.Ldia_single(d, h, true);
+		DBL3 b = dtf.Lodia_single(d, h, true);
+		write_doubles(argv[8], { a.x, a.y, a.z, b.x, b.y, b.z });
+		return 0;
+	}
+	fprintf(stderr, "bad arguments\n");
+	return 1;
+}

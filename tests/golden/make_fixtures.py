@@ -1,0 +1,100 @@
+"""Writes the committed fixtures under tests/golden/ (run in the build container, where the
+reference checkout and oracle/_ref exist; the GPU box only reads the results).
+
+* vortex_hh_clk_up.npz — the reference-held OVF2 file Boris/vortex_hh_clk_up.ovf
+  (512 x 128 x 1 cells of 5 x 5 x 40 nm, binary-8 data after the 123456789012345.0 check
+  value, x fastest) converted to float32, used as a realistic demag input.
+* tensor_golden.npz — FP64 Newell demag tensors written by oracle/_ref/ref_dump, i.e. by the
+  reference's own Boris/DemagTFunc*.cpp compiled here: first-octant values (the rest are
+  mirror images) for small meshes, and a seeded sample of entries for meshes large enough to
+  use the far-field series (distance >= 40 cells); plus SelfDemag and single-value
+  Ldia/Lodia.
+"""
+from __future__ import annotations
+
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+REF = Path("/root/reference")
+REF_DUMP = ROOT / "oracle" / "_ref" / "ref_dump"
+
+
+def read_ovf2(path: Path):
+    raw = path.read_bytes()
+    head_end = raw.index(b"# Begin: data binary 8\n") + len(b"# Begin: data binary 8\n")
+    hdr = raw[:head_end].decode()
+    kv = {}
+    for line in hdr.splitlines():
+        if ":" in line:
+            k, v = line[1:].split(":", 1)
+            kv[k.strip()] = v.strip()
+    n = (int(kv["xnodes"]), int(kv["ynodes"]), int(kv["znodes"]))
+    h = (float(kv["xstepsize"]), float(kv["ystepsize"]), float(kv["zstepsize"]))
+    check = np.frombuffer(raw, dtype="<f8", count=1, offset=head_end)[0]
+    assert check == 123456789012345.0, check
+    data = np.frombuffer(raw, dtype="<f8", count=3 * n[0] * n[1] * n[2], offset=head_end + 8)
+    return n, h, data.reshape(-1, 3)
+
+
+def dump(*args) -> np.ndarray:
+    out = Path("/tmp/_golden_dump.bin")
+    subprocess.run([str(REF_DUMP), *[a if isinstance(a, str) else repr(a) if isinstance(a, float) else str(a) for a in args], str(out)], check=True)
+    v = np.fromfile(out)
+    out.unlink()
+    return v
+
+
+def main() -> None:
+    n, h, M = read_ovf2(REF / "Boris" / "vortex_hh_clk_up.ovf")
+    np.savez_compressed(HERE / "vortex_hh_clk_up.npz", M=M.astype(np.float32), n=np.array(n), h=np.array(h))
+
+    if not REF_DUMP.exists():
+        sys.exit("oracle/_ref/ref_dump is not built: make -f oracle/Makefile.ref")
+    rng = np.random.default_rng(20261202)
+    out = {}
+    cases3 = [("c3a", (6, 5, 3), (12, 10, 8), (4e-9, 4e-9, 4e-9)),
+              ("c3b", (7, 4, 5), (14, 8, 10), (3.9e-9, 3.9e-9, 3e-9)),
+              ("c3c", (50, 44, 3), (100, 88, 6), (4e-9, 4e-9, 4e-9)),
+              ("c3d", (45, 3, 42), (90, 6, 128), (2e-9, 5e-9, 3e-9))]
+    for name, nn, NN, hh in cases3:
+        v = dump("tensor3d", *nn, *NN, *hh)
+        m = NN[0] * NN[1] * NN[2]
+        Dd = v[:3 * m].reshape(NN[2], NN[1], NN[0], 3)
+        Do = v[3 * m:].reshape(NN[2], NN[1], NN[0], 3)
+        if nn[0] * nn[1] * nn[2] <= 200:
+            idx = np.array([(k, j, i) for k in range(NN[2]) for j in range(NN[1]) for i in range(NN[0])])
+        else:
+            idx = np.stack([rng.integers(0, NN[2], 1500), rng.integers(0, NN[1], 1500), rng.integers(0, NN[0], 1500)], axis=1)
+        out[name + "_n"], out[name + "_N"], out[name + "_h"] = np.array(nn), np.array(NN), np.array(hh)
+        out[name + "_idx"] = idx
+        out[name + "_diag"] = Dd[idx[:, 0], idx[:, 1], idx[:, 2]]
+        out[name + "_off"] = Do[idx[:, 0], idx[:, 1], idx[:, 2]]
+    cases2 = [("c2a", (128, 32), (256, 64), (3.90625e-9, 3.90625e-9, 3e-9)),
+              ("c2b", (41, 45), (82, 90), (4e-9, 4e-9, 8e-9))]
+    for name, nn, NN, hh in cases2:
+        v = dump("tensor2d", *nn, *NN, *hh)
+        m = NN[0] * NN[1]
+        Dd = v[:3 * m].reshape(NN[1], NN[0], 3)
+        Do = v[3 * m:].reshape(NN[1], NN[0])
+        idx = np.stack([rng.integers(0, NN[1], 1500), rng.integers(0, NN[0], 1500)], axis=1)
+        out[name + "_n"], out[name + "_N"], out[name + "_h"] = np.array(nn), np.array(NN), np.array(hh)
+        out[name + "_idx"] = idx
+        out[name + "_diag"] = Dd[idx[:, 0], idx[:, 1]]
+        out[name + "_off"] = Do[idx[:, 0], idx[:, 1]]
+    selfs = [(1.0, 1.0, 1.0), (1.0, 1.0, 0.768), (0.5, 1.0, 0.25), (5e-9, 5e-9, 4e-8)]
+    out["self_h"] = np.array(selfs)
+    out["self_val"] = np.array([dump("self", *s) for s in selfs])
+    singles = [((3.0, 2.0, 1.0), (1.0, 1.0, 1.0)), ((10.5, -4.0, 2.0), (1.0, 0.8, 0.6)), ((0.0, 7.0, -3.0), (2.0, 1.0, 1.0))]
+    out["single_in"] = np.array([list(d) + list(hh) for d, hh in singles])
+    out["single_val"] = np.array([dump("single", *d, *hh) for d, hh in singles])
+    np.savez_compressed(HERE / "tensor_golden.npz", **out)
+    print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
+
+
+if __name__ == "__main__":
+    main()

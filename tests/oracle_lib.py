@@ -1,0 +1,201 @@
+"""ctypes access to the oracle (oracle/liboracle.so) for tests and bench.py's CPU baseline.
+
+The oracle is the checker: it is never part of the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_DIR = ROOT / "oracle"
+LIB = ORACLE_DIR / "liboracle.so"
+REF_DUMP = ORACLE_DIR / "_ref" / "ref_dump"
+
+_L = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+
+
+def lib() -> C.CDLL:
+    global _L
+    if _L is not None:
+        return _L
+    if not LIB.exists():
+        build()
+    L = C.CDLL(str(LIB))
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    i, d, vp = C.c_int, C.c_double, C.c_void_p
+    L.or_tensor3d.argtypes = [i] * 6 + [d] * 3 + [dp, dp]
+    L.or_tensor2d.argtypes = [i] * 4 + [d] * 3 + [dp, dp]
+    L.or_self_demag.argtypes = [d, d, d, dp]
+    L.or_newell_f.argtypes = [d, d, d]; L.or_newell_f.restype = d
+    L.or_newell_g.argtypes = [d, d, d]; L.or_newell_g.restype = d
+    L.or_asympt_diag.argtypes = [d] * 6; L.or_asympt_diag.restype = d
+    L.or_asympt_offdiag.argtypes = [d] * 6; L.or_asympt_offdiag.restype = d
+    L.or_kernels.argtypes = [i] * 6 + [d] * 3 + [dp, dp]
+    L.or_fft.argtypes = [i, dp, i]
+    L.or_demag_create.argtypes = [i, i, i, d, d, d, i, i]; L.or_demag_create.restype = vp
+    L.or_demag_destroy.argtypes = [vp, i]
+    L.or_demag_convolute.argtypes = [vp, i, dp, dp, i]
+    L.or_sim_create.argtypes = [i, i, i, i, d, d, d, dp, i, i, i]; L.or_sim_create.restype = vp
+    L.or_sim_destroy.argtypes = [vp]
+    for nm in ("or_sim_set_M", "or_sim_get_M", "or_sim_get_Heff", "or_sim_state", "or_sim_average_m"):
+        getattr(L, nm).argtypes = [vp, dp]
+    L.or_sim_set_Msmap.argtypes = [vp, vp]
+    L.or_sim_set_dt.argtypes = [vp, d]
+    L.or_sim_set_astep.argtypes = [vp, d, d, d, d]
+    L.or_sim_update_fields.argtypes = [vp]
+    L.or_sim_iterate_stage.argtypes = [vp]
+    L.or_sim_advance_time.argtypes = [vp]
+    L.or_sim_run_to_iteration.argtypes = [vp, i]
+    L.or_set_random_mag.argtypes = [dp, C.c_longlong, C.c_uint]
+    L.or_borisrand.argtypes = [C.c_uint, i, dp]
+    _L = L
+    return L
+
+
+def tensor3d(n, N, h):
+    L = lib()
+    m = N[0] * N[1] * N[2] * 3
+    a, b = np.zeros(m), np.zeros(m)
+    assert L.or_tensor3d(*n, *N, *h, a, b) == 0
+    return a.reshape(N[2], N[1], N[0], 3), b.reshape(N[2], N[1], N[0], 3)
+
+
+def tensor2d(n, N, h):
+    L = lib()
+    a, b = np.zeros(N[0] * N[1] * 3), np.zeros(N[0] * N[1])
+    assert L.or_tensor2d(*n, *N, *h, a, b) == 0
+    return a.reshape(N[1], N[0], 3), b.reshape(N[1], N[0])
+
+
+def kernels(n, N, h):
+    """(Kdiag, Kodiag): 3D (Nz/2+1, Ny/2+1, Nx/2+1, 3) each; 2D Kdiag (Ny/2+1, Nx/2+1, 3), Kxy (Ny/2+1, Nx/2+1)."""
+    L = lib()
+    nc, nyh, nzh = N[0] // 2 + 1, N[1] // 2 + 1, N[2] // 2 + 1
+    if n[2] == 1:
+        a, b = np.zeros(nc * nyh * 3), np.zeros(nc * nyh)
+        assert L.or_kernels(*n, *N, *h, a, b) == 0
+        return a.reshape(nyh, nc, 3), b.reshape(nyh, nc)
+    a, b = np.zeros(nc * nyh * nzh * 3), np.zeros(nc * nyh * nzh * 3)
+    assert L.or_kernels(*n, *N, *h, a, b) == 0
+    return a.reshape(nzh, nyh, nc, 3), b.reshape(nzh, nyh, nc, 3)
+
+
+def fft(x: np.ndarray, sign: int = -1) -> np.ndarray:
+    L = lib()
+    a = np.ascontiguousarray(np.stack([x.real, x.imag], axis=-1).reshape(-1), dtype=np.float64)
+    L.or_fft(len(x), a, sign)
+    return a[0::2] + 1j * a[1::2]
+
+
+class OracleDemag:
+    """The oracle's demag convolution (FP64, or FP32 with fp32=True)."""
+
+    def __init__(self, n, h, fp32=False, Nz_override=0):
+        self.L = lib()
+        self.n = tuple(n)
+        self.fp32 = int(fp32)
+        self.h = self.L.or_demag_create(*n, *h, Nz_override, self.fp32)
+        assert self.h
+
+    def __call__(self, M: np.ndarray, H: np.ndarray | None = None, clear_out=True) -> np.ndarray:
+        M = np.ascontiguousarray(M, dtype=np.float64).reshape(-1)
+        out = np.zeros_like(M) if H is None else np.ascontiguousarray(H, dtype=np.float64).reshape(-1).copy()
+        self.L.or_demag_convolute(self.h, self.fp32, M, out, int(clear_out))
+        return out.reshape(-1, 3)
+
+    def __del__(self):
+        try:
+            self.L.or_demag_destroy(self.h, self.fp32)
+        except Exception:
+            pass
+
+
+class OracleSim:
+    """The oracle's mesh + fields + ODE (mirror of the reference's AdvanceTime loop)."""
+
+    def __init__(self, method, n, h, params, fp32=False, Nz_override=0):
+        self.L = lib()
+        self.n = tuple(n)
+        self.ncell = n[0] * n[1] * n[2]
+        prm = np.array(params.oracle_vector(), dtype=np.float64)
+        self.h = self.L.or_sim_create(method, *n, *h, prm, params.oracle_flags(), Nz_override, int(fp32))
+        assert self.h
+
+    def set_M(self, M):
+        self.L.or_sim_set_M(self.h, np.ascontiguousarray(M, dtype=np.float64).reshape(-1))
+
+    def get_M(self):
+        out = np.zeros(self.ncell * 3)
+        self.L.or_sim_get_M(self.h, out)
+        return out.reshape(-1, 3)
+
+    def heff(self):
+        out = np.zeros(self.ncell * 3)
+        self.L.or_sim_get_Heff(self.h, out)
+        return out.reshape(-1, 3)
+
+    def set_ms_map(self, ms):
+        self._ms = None if ms is None else np.ascontiguousarray(ms, dtype=np.float64)
+        self.L.or_sim_set_Msmap(self.h, None if ms is None else self._ms.ctypes.data_as(C.c_void_p))
+
+    def set_dt(self, dt):
+        self.L.or_sim_set_dt(self.h, dt)
+
+    def set_astep(self, err=-1.0, inc=-1.0, dtmin=-1.0, dtmax=-1.0):
+        self.L.or_sim_set_astep(self.h, err, inc, dtmin, dtmax)
+
+    def update_fields(self):
+        self.L.or_sim_update_fields(self.h)
+
+    def iterate_stage(self):
+        self.L.or_sim_iterate_stage(self.h)
+
+    def advance_time(self):
+        self.L.or_sim_advance_time(self.h)
+
+    def run_to_iteration(self, it):
+        self.L.or_sim_run_to_iteration(self.h, int(it))
+
+    def state(self):
+        s = np.zeros(9)
+        self.L.or_sim_state(self.h, s)
+        keys = ["time", "dT", "lte", "iteration", "evalStep", "primed", "rejections", "evaluations", "stagetime"]
+        return dict(zip(keys, s.tolist()))
+
+    def average_m(self):
+        out = np.zeros(3)
+        self.L.or_sim_average_m(self.h, out)
+        return out
+
+    def __del__(self):
+        try:
+            self.L.or_sim_destroy(self.h)
+        except Exception:
+            pass
+
+
+def borisrand(seed: int, count: int) -> np.ndarray:
+    out = np.zeros(count)
+    lib().or_borisrand(seed, count, out)
+    return out
+
+
+def ref_available() -> bool:
+    return REF_DUMP.exists()
+
+
+def ref_dump(*args) -> np.ndarray:
+    """Run oracle/_ref/ref_dump (the reference's own tensor sources) and read its output."""
+    out = Path("/tmp") / f"ref_dump_{abs(hash(args))}.bin"
+    subprocess.run([str(REF_DUMP), *[str(a) if not isinstance(a, float) else repr(a) for a in args], str(out)], check=True)
+    v = np.fromfile(out)
+    out.unlink()
+    return v
