@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark: plain RKDP54 LLG integration with demag on one MI355X.
+
+Workload (the large 3D plain-method case): 512 x 512 x 32 cells of 4 nm cubes, Ms = 1e6 A/m,
+A = 1.5e-11 J/m, uniaxial K1 = 5e5 J/m^3 along z, alpha = 0.1, Hz = 50 mT; demag + exchange
++ anisotropy + Zeeman; initial M i.i.d. uniform on the sphere from the reference's BorisRand
+sequence (seed 1); FP32 per-step arithmetic, FP64 demag-kernel setup once (not timed).
+
+A step is one accepted RKDP54 step (6 field evaluations, each a full FFT demag convolution,
+plus the fused field/stage kernels); rejected attempts cost time but are not counted, like
+the reference's iteration counter.  value = cells * accepted steps / wall seconds.
+
+The JSON line also carries:
+  roofline     — the kernel with the largest total time in the timed region: algorithmic
+                 HBM bytes per launch / its average launch duration (HIP events recorded on
+                 its own stream around every launch in the timed region), against 8 TB/s;
+  cpu_baseline — the oracle (this repository's CPU restatement of the reference path, FP64,
+                 OpenMP over all host threads) timed on a bounded sample of the same
+                 workload on this machine's host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "LLG cell-updates/s (RKDP54+demag, FP32) batched vs plain; adjoint/forward cost"
+HBM_PEAK_GBS = 8000.0
+
+
+def workload():
+    from boris2_amd import api
+    n, h = (512, 512, 32), (4e-9, 4e-9, 4e-9)
+    p = api.FMParams(Ms=1e6, A=1.5e-11, alpha=0.1, K1=5e5, ea=(0.0, 0.0, 1.0), Ha=(0.0, 0.0, 50e-3 / api.MU0),
+                     modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ANIS_UNI | api.BM_MOD_ZEEMAN)
+    return n, h, p
+
+
+def algorithmic_bytes(kind: str, n, N) -> float:
+    """HBM bytes one launch of `kind` must move at minimum (FP32, complex 8 B, 3 components)."""
+    nx, ny, nz = n
+    Nx, Ny, Nz = N
+    NC, NYH, NZH = Nx // 2 + 1, Ny // 2 + 1, Nz // 2 + 1
+    cells = nx * ny * nz
+    half = 24.0 * NC * ny * nz          # spectra with only the ny data rows, 3 components
+    full = 24.0 * NC * Ny * nz          # spectra after the y transform
+    kern = 24.0 * NC * NYH * NZH        # 6-component FP32 kernel octant
+    return {
+        "fwd_x_r2c": 12.0 * cells + half,
+        "y_fft": half + full,
+        "z_fft_kernel_ifft": full + kern + full,
+        "y_ifft": full + half,
+        "inv_x_c2r": half + 12.0 * cells,
+        "y2d_fft_kernel_ifft": half + 16.0 * NC * NYH + half,
+        # RKDP54 stages 0..5 move 9,6,7,8,9,9 float3 arrays of the mesh: 8 on average
+        "llg_stage": 8 * 12.0 * cells,
+        "rkdp_advance": 4 * 12.0 * cells,
+    }[kind]
+
+
+# rocprof kernel-name prefix of each launch kind (for the PMC traffic table in profiles/)
+KIND_TO_KERNEL = {"fwd_x_r2c": "k_fwd_x<", "y_fft": "k_y<1024, -1>", "z_fft_kernel_ifft": "k_z<",
+                  "y_ifft": "k_y<1024, 1>", "inv_x_c2r": "k_inv_x<", "llg_stage": "k_stage<", "rkdp_advance": "k_rkdp_advance"}
+TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_rkdp54_512x512x32.json"
+
+
+def measured_traffic(kind: str):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs (copy-calibrated)."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    t = json.loads(TRAFFIC_FILE.read_text())["kernels"]
+    pre = KIND_TO_KERNEL.get(kind)
+    vals = [v["traffic_bytes"] for k, v in t.items() if pre and pre in k]
+    return round(sum(vals) / len(vals)) if vals else None
+
+
+def gpu_run(steps: int, warmup: int):
+    import torch
+    from boris2_amd import api, _lib
+    n, h, p = workload()
+    ncell = n[0] * n[1] * n[2]
+    M0 = api.random_magnetisation(ncell, p.Ms, seed=1).astype(np.float32)
+    t0 = time.perf_counter()
+    s = api.FMSolver(api.BM_EVAL_RKDP54, n, h, p, M0)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    N, dev_bytes = s.demag.info()
+    s.run_to_iteration(warmup)
+    torch.cuda.synchronize()
+    st0 = s.state()
+    L = _lib.load()
+    _lib.check(L.bm_prof_begin(64 * (steps + 8) * 2))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.run_to_iteration(warmup + steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    L.bm_prof_end()
+    st1 = s.state()
+    kinds = {}
+    for k, name in enumerate(_lib.PROF_KINDS):
+        cnt, tot, mn, mx = C.c_longlong(), C.c_double(), C.c_double(), C.c_double()
+        _lib.check(L.bm_prof_query(k, C.byref(cnt), C.byref(tot), C.byref(mn), C.byref(mx)))
+        if cnt.value:
+            avg = tot.value / cnt.value
+            b = algorithmic_bytes(name, n, N)
+            kinds[name] = {"launches": cnt.value, "total_ms": round(tot.value, 3), "avg_ms": round(avg, 5),
+                           "alg_GB_per_launch": round(b / 1e9, 4), "GB_per_s": round(b / (avg * 1e-3) / 1e9, 1)}
+    accepted = st1["iteration"] - st0["iteration"]
+    m_avg = s.average_m()
+    s.close()
+    return {"n": n, "N": N, "ncell": ncell, "wall": wall, "accepted": accepted,
+            "rejected": st1["rejections"] - st0["rejections"], "evals": st1["evaluations"] - st0["evaluations"],
+            "kinds": kinds, "setup_s": setup_s, "dev_bytes": dev_bytes, "sim_time": st1["time"], "m_avg": m_avg.tolist()}
+
+
+def cpu_baseline(cpu_steps: int):
+    """Oracle FP64 on the host: kernel setup (reported separately) then `cpu_steps` steps."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    from boris2_amd import api
+    n, h, p = workload()
+    ncell = n[0] * n[1] * n[2]
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    o = O.OracleSim(api.BM_EVAL_RKDP54, n, h, p, Nz_override=64)
+    setup = time.perf_counter() - t0
+    o.set_M(api.random_magnetisation(ncell, p.Ms, seed=1))
+    t0 = time.perf_counter()
+    o.run_to_iteration(cpu_steps)
+    dt = time.perf_counter() - t0
+    return {"value": ncell * cpu_steps / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{cpu_steps} RKDP54 steps ({6 * cpu_steps} field evaluations) of the same 512x512x32 workload, "
+                      f"oracle FP64 with its own mixed-radix FFT (no FFTW on this host), OpenMP {cores} threads; "
+                      f"{dt:.1f} s timed, FP64 kernel setup {setup:.1f} s not included"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 512 MiB device copy 5 times (scales the HBM byte counters)")
+    a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        x = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each": 2 * x.numel() * 4, "copies": 5}))
+        return 0
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: measures one GPU only; a mesh or batch sharded over several GPUs is not built.",
+              file=sys.stderr)
+        return 2
+
+    r = gpu_run(a.steps, a.warmup)
+    kinds = r["kinds"]
+    dom = max(kinds, key=lambda k: kinds[k]["total_ms"])
+    d = kinds[dom]
+    achieved = d["GB_per_s"]
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
+            "traffic_source": str(TRAFFIC_FILE.relative_to(ROOT)), "kernel": dom,
+            "alg_bytes_per_launch": round(d["alg_GB_per_launch"] * 1e9), "avg_launch_ms": d["avg_ms"]}
+    cpu = None if a.no_cpu else cpu_baseline(a.cpu_steps)
+    value = r["ncell"] * r["accepted"] / r["wall"]
+    line = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": r["accepted"],
+        "warmup": a.warmup, "ms_per_step": 1e3 * r["wall"] / max(1, r["accepted"]), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain RKDP54 + demag/exchange/uniaxial anisotropy/Zeeman, 512x512x32 cells of 4 nm, "
+                               "FP32 steps, FP64 kernel setup", "cells": r["ncell"], "fft_N": list(r["N"]),
+                   "accepted_steps": r["accepted"], "rejected_attempts": r["rejected"],
+                   "field_evaluations": r["evals"], "sim_time_s": r["sim_time"], "setup_s": round(r["setup_s"], 2),
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": kinds,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

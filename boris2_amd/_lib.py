@@ -21,7 +21,11 @@ EXPORTS = [
     "bm_ode_create", "bm_ode_destroy", "bm_ode_set_ms_map", "bm_ode_set_params", "bm_ode_set_dt",
     "bm_ode_set_astep", "bm_ode_advance_time", "bm_ode_run_to_iteration", "bm_ode_iterate_stage",
     "bm_ode_restore", "bm_ode_get_state", "bm_ode_keep_heff", "bm_ode_copy_heff", "bm_ode_average_m",
+    "bm_prof_begin", "bm_prof_end", "bm_prof_query",
 ]
+
+PROF_KINDS = ["fwd_x_r2c", "y_fft", "z_fft_kernel_ifft", "y_ifft", "inv_x_c2r", "y2d_fft_kernel_ifft",
+              "llg_stage", "rkdp_advance"]
 
 
 class FmParams(C.Structure):
@@ -77,6 +81,9 @@ def load() -> C.CDLL:
         "bm_ode_keep_heff": (i, [vp, i]),
         "bm_ode_copy_heff": (i, [vp, vp, vp]),
         "bm_ode_average_m": (i, [vp, C.POINTER(d), vp]),
+        "bm_prof_begin": (i, [i]),
+        "bm_prof_end": (i, []),
+        "bm_prof_query": (i, [i, C.POINTER(ll), C.POINTER(d), C.POINTER(d), C.POINTER(d)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

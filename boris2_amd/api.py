@@ -132,15 +132,6 @@ class FMParams:
         p.modules = int(self.modules)
         return p
 
-    def oracle_vector(self) -> list[float]:
-        """Packed form used by the oracle's C-ABI (22 doubles)."""
-        return [self.Ms, self.A, self.alpha, self.K1, self.K2, *self.ea, *self.Ha, float(self.wave),
-                self.amp, self.w, self.t0, *self.dir, *self.slope]
-
-    def oracle_flags(self) -> int:
-        return ((1 if self.modules & BM_MOD_DEMAG else 0) | (2 if self.modules & BM_MOD_EXCHANGE else 0)
-                | (4 if self.modules & BM_MOD_ANIS_UNI else 0) | (8 if self.modules & BM_MOD_ZEEMAN else 0))
-
 
 class FMSolver:
     """One ferromagnetic mesh integrated with the LLG equation."""

@@ -619,20 +619,43 @@ int compute_kernels(bm_demag_s* d, hipStream_t st)
 // C-ABI
 // =======================================================================================
 
-int bm::demag_planar(bm_demag_s* d, const float* M, float* H3n, hipStream_t st)
+// The convolution passes up to (not including) the final x C2R pass.
+static int forward_and_multiply(bm_demag_s* d, const float* M, hipStream_t st)
 {
 	const int HX = d->nx, nlines = 3 * d->nz * d->NC;
+	prof_mark_begin(PK_FWD_X, st);
 	int rc = launch_fwd_x(HX, M, d->S, d->ny, d->nz, d->Ny, d->ncu, st);
+	prof_mark_end(PK_FWD_X, st);
 	if (rc) return rc;
-	if (d->is2d) rc = launch_y2d(d->Ny, d->S, d->K, d->NC, d->NYH, d->ncu, st);
-	else {
-		rc = launch_y(d->Ny, -1, d->S, nlines, d->ncu, st);
-		if (!rc) rc = launch_z(d->Nz, d->S, d->K, d->nz, d->NC, d->Ny, d->NYH, d->ncu, st);
-		if (!rc) rc = launch_y(d->Ny, 1, d->S, nlines, d->ncu, st);
+	if (d->is2d) {
+		prof_mark_begin(PK_Y2D, st);
+		rc = launch_y2d(d->Ny, d->S, d->K, d->NC, d->NYH, d->ncu, st);
+		prof_mark_end(PK_Y2D, st);
+		return rc;
 	}
+	prof_mark_begin(PK_Y_FWD, st);
+	rc = launch_y(d->Ny, -1, d->S, nlines, d->ncu, st);
+	prof_mark_end(PK_Y_FWD, st);
+	if (rc) return rc;
+	prof_mark_begin(PK_Z, st);
+	rc = launch_z(d->Nz, d->S, d->K, d->nz, d->NC, d->Ny, d->NYH, d->ncu, st);
+	prof_mark_end(PK_Z, st);
+	if (rc) return rc;
+	prof_mark_begin(PK_Y_INV, st);
+	rc = launch_y(d->Ny, 1, d->S, nlines, d->ncu, st);
+	prof_mark_end(PK_Y_INV, st);
+	return rc;
+}
+
+int bm::demag_planar(bm_demag_s* d, const float* M, float* H3n, hipStream_t st)
+{
+	int rc = forward_and_multiply(d, M, st);
 	if (rc) return rc;
 	const float scale = (float)(1.0 / ((double)d->Nx * d->Ny * d->Nz));
-	return launch_inv_x(HX, 0, d->S, H3n, d->ny, d->nz, d->Ny, scale, d->ncu, st);
+	prof_mark_begin(PK_INV_X, st);
+	rc = launch_inv_x(d->nx, 0, d->S, H3n, d->ny, d->nz, d->Ny, scale, d->ncu, st);
+	prof_mark_end(PK_INV_X, st);
+	return rc;
 }
 
 extern "C" int bm_demag_create(int nx, int ny, int nz, double hx, double hy, double hz, int flags, bm_demag_t* out)
@@ -690,27 +713,24 @@ extern "C" int bm_demag_convolute(bm_demag_t d, const float* M, float* Heff, int
 {
 	if (!d || !M || !Heff) return fail(BM_ERR_ARG, "bm_demag_convolute: null argument");
 	hipStream_t st = (hipStream_t)stream;
-	const int HX = d->nx, nlines = 3 * d->nz * d->NC;
-	int rc = launch_fwd_x(HX, M, d->S, d->ny, d->nz, d->Ny, d->ncu, st);
-	if (rc) return rc;
-	if (d->is2d) rc = launch_y2d(d->Ny, d->S, d->K, d->NC, d->NYH, d->ncu, st);
-	else {
-		rc = launch_y(d->Ny, -1, d->S, nlines, d->ncu, st);
-		if (!rc) rc = launch_z(d->Nz, d->S, d->K, d->nz, d->NC, d->Ny, d->NYH, d->ncu, st);
-		if (!rc) rc = launch_y(d->Ny, 1, d->S, nlines, d->ncu, st);
-	}
+	const int HX = d->nx;
+	int rc = forward_and_multiply(d, M, st);
 	if (rc) return rc;
 	const float scale = (float)(1.0 / ((double)d->Nx * d->Ny * d->Nz));
 	if (energy) {
 		const int np = inv_x_nwork(HX, d->ny, d->nz);
 		if (np > d->ecap) return fail(BM_ERR_STATE, "energy partial buffer too small");
+		prof_mark_begin(PK_INV_X, st);
 		rc = launch_inv_x(HX, clear_out ? 1 : 2, d->S, Heff, d->ny, d->nz, d->Ny, scale, d->ncu, st, M, d->ebuf);
+		prof_mark_end(PK_INV_X, st);
 		if (rc) return rc;
 		size_t n = (size_t)d->nx * d->ny * d->nz;
 		hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, st, d->ebuf, np, n, energy);
 	}
 	else {
+		prof_mark_begin(PK_INV_X, st);
 		rc = launch_inv_x(HX, clear_out ? 1 : 2, d->S, Heff, d->ny, d->nz, d->Ny, scale, d->ncu, st);
+		prof_mark_end(PK_INV_X, st);
 		if (rc) return rc;
 	}
 	BM_HIP(hipGetLastError());

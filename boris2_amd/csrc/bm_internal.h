@@ -16,6 +16,12 @@ int hip_fail(hipError_t e, const char* what);
 
 int device_cu_count();
 
+// Launch timing: when enabled (bm_prof_begin), every kernel launch of the hot path is
+// bracketed by HIP events on its own stream and tagged with its kind.
+enum ProfKind { PK_FWD_X = 0, PK_Y_FWD, PK_Z, PK_Y_INV, PK_INV_X, PK_Y2D, PK_STAGE, PK_ADVANCE, PK_COUNT };
+void prof_mark_begin(int kind, hipStream_t st);
+void prof_mark_end(int kind, hipStream_t st);
+
 } // namespace bm
 
 struct bm_demag_s {

@@ -38,7 +38,8 @@ __device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y 
 __device__ __forceinline__ float norm(F3 a) { return sqrtf(dot(a, a)); }
 
 struct StageArgs {
-	float* M;
+	const float* Min;       // magnetisation this stage evaluates (neighbours read)
+	float* Mout;            // magnetisation of the next stage (never the same array)
 	const float* Hd;        // planar demag field (3 planes of n) or null
 	float* sM1;
 	float* E[6];
@@ -77,7 +78,7 @@ __global__ void __launch_bounds__(kBS) k_stage(StageArgs a)
 	float lte_loc = 0.f;
 	for (size_t idx = blockIdx.x * (size_t)kBS + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kBS) {
 		const int i = (int)(idx % a.nx), j = (int)((idx / a.nx) % a.ny), k = (int)(idx / ((size_t)a.nx * a.ny));
-		const F3 m = ld3(a.M, idx);
+		const F3 m = ld3(a.Min, idx);
 		const float Ms = a.Ms_map ? a.Ms_map[idx] : a.Ms;
 		// --- effective field, module order Zeeman, Demag, Exchange, Anisotropy
 		F3 H = (a.modules & BM_MOD_ZEEMAN) ? F3{ a.Ha[0], a.Ha[1], a.Ha[2] } : F3{ 0.f, 0.f, 0.f };
@@ -86,19 +87,19 @@ __global__ void __launch_bounds__(kBS) k_stage(StageArgs a)
 			F3 lx{ 0, 0, 0 }, ly{ 0, 0, 0 }, lz{ 0, 0, 0 };
 			const size_t sy = a.nx, sz = (size_t)a.nx * a.ny;
 			if (a.nx > 1) {
-				if (i > 0 && i < a.nx - 1) lx = ld3(a.M, idx + 1) + ld3(a.M, idx - 1) - 2.f * m;
-				else if (i == 0) lx = ld3(a.M, idx + 1) - m;
-				else lx = ld3(a.M, idx - 1) - m;
+				if (i > 0 && i < a.nx - 1) lx = ld3(a.Min, idx + 1) + ld3(a.Min, idx - 1) - 2.f * m;
+				else if (i == 0) lx = ld3(a.Min, idx + 1) - m;
+				else lx = ld3(a.Min, idx - 1) - m;
 			}
 			if (a.ny > 1) {
-				if (j > 0 && j < a.ny - 1) ly = ld3(a.M, idx + sy) + ld3(a.M, idx - sy) - 2.f * m;
-				else if (j == 0) ly = ld3(a.M, idx + sy) - m;
-				else ly = ld3(a.M, idx - sy) - m;
+				if (j > 0 && j < a.ny - 1) ly = ld3(a.Min, idx + sy) + ld3(a.Min, idx - sy) - 2.f * m;
+				else if (j == 0) ly = ld3(a.Min, idx + sy) - m;
+				else ly = ld3(a.Min, idx - sy) - m;
 			}
 			if (a.nz > 1) {
-				if (k > 0 && k < a.nz - 1) lz = ld3(a.M, idx + sz) + ld3(a.M, idx - sz) - 2.f * m;
-				else if (k == 0) lz = ld3(a.M, idx + sz) - m;
-				else lz = ld3(a.M, idx - sz) - m;
+				if (k > 0 && k < a.nz - 1) lz = ld3(a.Min, idx + sz) + ld3(a.Min, idx - sz) - 2.f * m;
+				else if (k == 0) lz = ld3(a.Min, idx + sz) - m;
+				else lz = ld3(a.Min, idx - sz) - m;
 			}
 			F3 lap{ lx.x / a.hx2 + ly.x / a.hy2 + lz.x / a.hz2, lx.y / a.hx2 + ly.y / a.hy2 + lz.y / a.hz2, lx.z / a.hx2 + ly.z / a.hy2 + lz.z / a.hz2 };
 			const float pre = (2.f * a.A) / ((float)kMU0 * Ms * Ms);
@@ -127,20 +128,20 @@ __global__ void __launch_bounds__(kBS) k_stage(StageArgs a)
 			}
 			else if (STEP == 1) {
 				st3(a.E[1], idx, rhs);
-				st3(a.M, idx, ld3(a.sM1, idx) + dT * ((3.f / 40.f) * ld3(a.E[0], idx) + (9.f / 40.f) * rhs));
+				st3(a.Mout, idx, ld3(a.sM1, idx) + dT * ((3.f / 40.f) * ld3(a.E[0], idx) + (9.f / 40.f) * rhs));
 			}
 			else if (STEP == 2) {
 				st3(a.E[2], idx, rhs);
-				st3(a.M, idx, ld3(a.sM1, idx) + dT * ((44.f / 45.f) * ld3(a.E[0], idx) - (56.f / 15.f) * ld3(a.E[1], idx) + (32.f / 9.f) * rhs));
+				st3(a.Mout, idx, ld3(a.sM1, idx) + dT * ((44.f / 45.f) * ld3(a.E[0], idx) - (56.f / 15.f) * ld3(a.E[1], idx) + (32.f / 9.f) * rhs));
 			}
 			else if (STEP == 3) {
 				st3(a.E[3], idx, rhs);
-				st3(a.M, idx, ld3(a.sM1, idx) + dT * ((19372.f / 6561.f) * ld3(a.E[0], idx) - (25360.f / 2187.f) * ld3(a.E[1], idx)
+				st3(a.Mout, idx, ld3(a.sM1, idx) + dT * ((19372.f / 6561.f) * ld3(a.E[0], idx) - (25360.f / 2187.f) * ld3(a.E[1], idx)
 					+ (64448.f / 6561.f) * ld3(a.E[2], idx) - (212.f / 729.f) * rhs));
 			}
 			else if (STEP == 4) {
 				st3(a.E[4], idx, rhs);
-				st3(a.M, idx, ld3(a.sM1, idx) + dT * ((9017.f / 3168.f) * ld3(a.E[0], idx) - (355.f / 33.f) * ld3(a.E[1], idx)
+				st3(a.Mout, idx, ld3(a.sM1, idx) + dT * ((9017.f / 3168.f) * ld3(a.E[0], idx) - (355.f / 33.f) * ld3(a.E[1], idx)
 					+ (46732.f / 5247.f) * ld3(a.E[2], idx) + (49.f / 176.f) * ld3(a.E[3], idx) - (5103.f / 18656.f) * rhs));
 			}
 			else {
@@ -148,25 +149,25 @@ __global__ void __launch_bounds__(kBS) k_stage(StageArgs a)
 				F3 mn = ld3(a.sM1, idx) + dT * ((35.f / 384.f) * ld3(a.E[0], idx) + (500.f / 1113.f) * ld3(a.E[2], idx)
 					+ (125.f / 192.f) * ld3(a.E[3], idx) - (2187.f / 6784.f) * ld3(a.E[4], idx) + (11.f / 84.f) * rhs);
 				mn = (Ms / norm(mn)) * mn;
-				st3(a.M, idx, mn);
+				st3(a.Mout, idx, mn);
 			}
 		}
 		else if (METHOD == M_RK4) {
-			if (STEP == 0) { st3(a.sM1, idx, m); st3(a.E[0], idx, rhs); st3(a.M, idx, m + (dT / 2) * rhs); }
-			else if (STEP == 1) { st3(a.E[1], idx, rhs); st3(a.M, idx, ld3(a.sM1, idx) + (dT / 2) * rhs); }
-			else if (STEP == 2) { st3(a.E[2], idx, rhs); st3(a.M, idx, ld3(a.sM1, idx) + dT * rhs); }
+			if (STEP == 0) { st3(a.sM1, idx, m); st3(a.E[0], idx, rhs); st3(a.Mout, idx, m + (dT / 2) * rhs); }
+			else if (STEP == 1) { st3(a.E[1], idx, rhs); st3(a.Mout, idx, ld3(a.sM1, idx) + (dT / 2) * rhs); }
+			else if (STEP == 2) { st3(a.E[2], idx, rhs); st3(a.Mout, idx, ld3(a.sM1, idx) + dT * rhs); }
 			else {
 				F3 mn = ld3(a.sM1, idx) + (dT / 6) * (ld3(a.E[0], idx) + 2.f * ld3(a.E[1], idx) + 2.f * ld3(a.E[2], idx) + rhs);
 				mn = (Ms / norm(mn)) * mn;
-				st3(a.M, idx, mn);
+				st3(a.Mout, idx, mn);
 			}
 		}
 		else {
-			if (STEP == 0) { st3(a.sM1, idx, m); st3(a.M, idx, m + dT * rhs); }
+			if (STEP == 0) { st3(a.sM1, idx, m); st3(a.Mout, idx, m + dT * rhs); }
 			else {
 				F3 mn = 0.5f * (ld3(a.sM1, idx) + m + dT * rhs);
 				mn = (Ms / norm(mn)) * mn;
-				st3(a.M, idx, mn);
+				st3(a.Mout, idx, mn);
 				if (METHOD == M_AHEUN) lte_loc = fmaxf(lte_loc, norm(mn - m) / norm(mn));
 			}
 		}
@@ -174,13 +175,15 @@ __global__ void __launch_bounds__(kBS) k_stage(StageArgs a)
 	if ((METHOD == M_RKDP && STEP == 0) || (METHOD == M_AHEUN && STEP == 1)) block_max_to(lte_loc, a.lte);
 }
 
-// RKDP54 advance after the step size is fixed: sM1 = M; M += E0 dT/5.
-__global__ void __launch_bounds__(kBS) k_rkdp_advance(float* __restrict__ M, float* __restrict__ sM1, const float* __restrict__ E0, size_t n3, float dT5)
+// RKDP54 advance after the step size is fixed: sM1 = M; M1 = M + E0 dT/5 (into the
+// second magnetisation buffer, which stage 1 then reads).
+__global__ void __launch_bounds__(kBS) k_rkdp_advance(const float* __restrict__ M, float* __restrict__ M1, float* __restrict__ sM1,
+	const float* __restrict__ E0, size_t n3, float dT5)
 {
 	for (size_t q = blockIdx.x * (size_t)kBS + threadIdx.x; q < n3; q += (size_t)gridDim.x * kBS) {
 		float m = M[q];
 		sM1[q] = m;
-		M[q] = m + E0[q] * dT5;
+		M1[q] = m + E0[q] * dT5;
 	}
 }
 
@@ -210,7 +213,9 @@ struct bm_ode_s {
 	double h[3];
 	bm_fm_params p;
 	bm_demag_t demag = nullptr;
-	float* M = nullptr;
+	float* M = nullptr;            // the mesh's magnetisation (caller-owned); holds M between steps
+	float* Mb = nullptr;           // second magnetisation buffer for intermediate stages
+	float* Mcur = nullptr;         // the array holding the magnetisation of the current stage
 	const float* Ms_map = nullptr;
 	float* Hd = nullptr;
 	float* sM1 = nullptr;
@@ -252,7 +257,7 @@ StageArgs make_args(bm_ode_s* o, float dT)
 {
 	StageArgs a;
 	std::memset(&a, 0, sizeof(a));
-	a.M = o->M; a.Hd = o->Hd; a.sM1 = o->sM1;
+	a.Min = o->Mcur; a.Mout = (o->Mcur == o->M) ? o->Mb : o->M; a.Hd = o->Hd; a.sM1 = o->sM1;
 	for (int q = 0; q < 6; q++) a.E[q] = o->E[q];
 	a.Ms_map = o->Ms_map; a.Heff_out = o->keep_heff ? o->Heff_keep : nullptr; a.lte = o->d_lte;
 	a.nx = o->nx; a.ny = o->ny; a.nz = o->nz;
@@ -271,6 +276,7 @@ int launch_stage(bm_ode_s* o, int step, float dT, hipStream_t st)
 {
 	StageArgs a = make_args(o, dT);
 	dim3 g(stage_grid(o)), b(kBS);
+	bm::prof_mark_begin(bm::PK_STAGE, st);
 	switch (o->mcode * 8 + step) {
 	case M_RKDP * 8 + 0: hipLaunchKernelGGL((k_stage<M_RKDP, 0>), g, b, 0, st, a); break;
 	case M_RKDP * 8 + 1: hipLaunchKernelGGL((k_stage<M_RKDP, 1>), g, b, 0, st, a); break;
@@ -288,6 +294,9 @@ int launch_stage(bm_ode_s* o, int step, float dT, hipStream_t st)
 	case M_AHEUN * 8 + 1: hipLaunchKernelGGL((k_stage<M_AHEUN, 1>), g, b, 0, st, a); break;
 	default: return bm::fail(BM_ERR_ARG, "bad stage");
 	}
+	bm::prof_mark_end(bm::PK_STAGE, st);
+	// every stage except RKDP54 stage 0 writes the next magnetisation into the other array
+	if (!(o->mcode == M_RKDP && step == 0)) o->Mcur = (o->Mcur == o->M) ? o->Mb : o->M;
 	return BM_OK;
 }
 
@@ -296,7 +305,7 @@ int update_demag(bm_ode_s* o, hipStream_t st)
 {
 	o->evaluations++;
 	if (!(o->p.modules & BM_MOD_DEMAG)) return BM_OK;
-	return bm::demag_planar(o->demag, o->M, o->Hd, st);
+	return bm::demag_planar(o->demag, o->Mcur, o->Hd, st);
 }
 
 // ODECommon_Base::SetAdaptiveTimeStep (Boris/DiffEq_CommonBase_Iterate.cpp:15-59).
@@ -337,6 +346,7 @@ int read_lte(bm_ode_s* o, hipStream_t st)
 int restore(bm_ode_s* o, hipStream_t st)
 {
 	BM_HIP(hipMemcpyAsync(o->M, o->sM1, o->n * 3 * sizeof(float), hipMemcpyDeviceToDevice, st));
+	o->Mcur = o->M;
 	return BM_OK;
 }
 
@@ -368,7 +378,10 @@ int iterate_stage(bm_ode_s* o, hipStream_t st)
 			}
 			else o->primed = true;
 			const size_t n3 = o->n * 3;
-			hipLaunchKernelGGL(k_rkdp_advance, dim3(stage_grid(o)), dim3(kBS), 0, st, o->M, o->sM1, o->E[0], n3, (float)o->dT / 5);
+			bm::prof_mark_begin(bm::PK_ADVANCE, st);
+			hipLaunchKernelGGL(k_rkdp_advance, dim3(stage_grid(o)), dim3(kBS), 0, st, o->Mcur, o->Mb, o->sM1, o->E[0], n3, (float)o->dT / 5);
+			o->Mcur = o->Mb;
+			bm::prof_mark_end(bm::PK_ADVANCE, st);
 			o->evalStep++;
 		}
 		else if (o->evalStep < 5) {
@@ -418,6 +431,7 @@ void free_all(bm_ode_s* o)
 {
 	if (o->Hd) (void)hipFree(o->Hd);
 	if (o->sM1) (void)hipFree(o->sM1);
+	if (o->Mb) (void)hipFree(o->Mb);
 	for (int q = 0; q < 6; q++) if (o->E[q]) (void)hipFree(o->E[q]);
 	if (o->Heff_keep) (void)hipFree(o->Heff_keep);
 	if (o->d_lte) (void)hipFree(o->d_lte);
@@ -452,7 +466,8 @@ extern "C" int bm_ode_create(int method, int nx, int ny, int nz, double hx, doub
 	}
 	o->dT_last = o->dT;
 	const size_t b3 = o->n * 3 * sizeof(float);
-	bool ok = hipMalloc(&o->sM1, b3) == hipSuccess;
+	bool ok = hipMalloc(&o->sM1, b3) == hipSuccess && hipMalloc(&o->Mb, b3) == hipSuccess;
+	o->Mcur = o->M;
 	for (int q = 0; q < nE && ok; q++) ok = hipMalloc(&o->E[q], b3) == hipSuccess && hipMemset(o->E[q], 0, b3) == hipSuccess;
 	if (ok && (p->modules & BM_MOD_DEMAG)) ok = hipMalloc(&o->Hd, b3) == hipSuccess;
 	if (ok) ok = hipMalloc(&o->Heff_keep, b3) == hipSuccess;
@@ -561,7 +576,7 @@ extern "C" int bm_ode_average_m(bm_ode_t o, double* out3, void* stream)
 {
 	if (!o || !out3) return fail(BM_ERR_ARG, "null argument");
 	hipStream_t st = (hipStream_t)stream;
-	hipLaunchKernelGGL(k_avg_partial, dim3(o->npart), dim3(kBS), 0, st, o->M, o->Ms_map, (float)o->p.Ms, o->n, o->d_part);
+	hipLaunchKernelGGL(k_avg_partial, dim3(o->npart), dim3(kBS), 0, st, o->Mcur, o->Ms_map, (float)o->p.Ms, o->n, o->d_part);
 	BM_HIP(hipMemcpyAsync(o->h_part, o->d_part, (size_t)o->npart * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
 	BM_HIP(hipStreamSynchronize(st));
 	double s[3] = { 0, 0, 0 };

@@ -102,7 +102,10 @@ typedef struct {
 	int modules;            /* BM_MOD_* bits */
 } bm_fm_params;
 
-/* M: caller-owned device float[3n], the mesh magnetisation, read and updated in place.
+/* M: caller-owned device float[3n], the mesh magnetisation.  It holds the magnetisation
+ * whenever a step is complete (after bm_ode_advance_time / bm_ode_run_to_iteration);
+ * intermediate RK stages alternate between M and an internal array so that no stage
+ * overwrites the magnetisation its exchange stencil is still reading.
  * demag may be NULL when BM_MOD_DEMAG is not set. */
 int bm_ode_create(int method, int nx, int ny, int nz, double hx, double hy, double hz,
 	const bm_fm_params* p, bm_demag_t demag, float* M, bm_ode_t* out);
@@ -138,6 +141,18 @@ int bm_ode_keep_heff(bm_ode_t o, int on);
 int bm_ode_copy_heff(bm_ode_t o, float* Heff_out, void* stream);
 /* <M>/Ms averaged over cells (host double[3]); synchronises the stream. */
 int bm_ode_average_m(bm_ode_t o, double* out3, void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * Launch timing (no reference counterpart: the reference only times whole runs,
+ * Boris/Commands.cpp:4759).  Between bm_prof_begin and bm_prof_end every hot-path kernel
+ * launch is bracketed by HIP events recorded on the launch's own stream; bm_prof_query
+ * returns the count and the total / min / max duration (ms) of one kind:
+ * 0 x-R2C pass, 1 y-FFT pass, 2 z-pass with kernel multiply, 3 y-IFFT pass,
+ * 4 x-C2R pass, 5 2D y pass with kernel multiply, 6 fused field + RK stage, 7 RKDP54
+ * advance.  Not thread safe; meant for benchmarks. */
+int bm_prof_begin(int max_launches);
+int bm_prof_end(void);
+int bm_prof_query(int kind, long long* count, double* total_ms, double* min_ms, double* max_ms);
 
 #ifdef __cplusplus
 }

@@ -118,6 +118,14 @@ class OracleDemag:
             pass
 
 
+def pack_params(p) -> tuple[np.ndarray, int]:
+    """boris2_amd.api.FMParams -> the oracle C-ABI's 22-double vector and module flags."""
+    v = [p.Ms, p.A, p.alpha, p.K1, p.K2, *p.ea, *p.Ha, float(p.wave), p.amp, p.w, p.t0, *p.dir, *p.slope]
+    m = p.modules
+    flags = (1 if m & 1 else 0) | (2 if m & 2 else 0) | (4 if m & 4 else 0) | (8 if m & 8 else 0)
+    return np.array(v, dtype=np.float64), flags
+
+
 class OracleSim:
     """The oracle's mesh + fields + ODE (mirror of the reference's AdvanceTime loop)."""
 
@@ -125,8 +133,8 @@ class OracleSim:
         self.L = lib()
         self.n = tuple(n)
         self.ncell = n[0] * n[1] * n[2]
-        prm = np.array(params.oracle_vector(), dtype=np.float64)
-        self.h = self.L.or_sim_create(method, *n, *h, prm, params.oracle_flags(), Nz_override, int(fp32))
+        prm, flags = pack_params(params)
+        self.h = self.L.or_sim_create(method, *n, *h, prm, flags, Nz_override, int(fp32))
         assert self.h
 
     def set_M(self, M):

@@ -1,0 +1,36 @@
+"""The C-ABI library loads and exports every entry point declared in include/boris_mi355x.h
+(no compute calls: this runs without a GPU); the product library does not link the oracle."""
+import re
+import subprocess
+from pathlib import Path
+
+from boris2_amd import _lib
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared():
+    txt = (ROOT / "include" / "boris_mi355x.h").read_text()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(bm_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_header_declares_what_the_binding_lists():
+    assert declared() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    for name in declared():
+        assert hasattr(L, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True, text=True, check=True).stdout
+    syms = {l.split()[-1] for l in out.splitlines() if l.strip()}
+    for name in declared():
+        assert name in syms, name
+    assert not any(s.startswith("or_") for s in syms), "product library must not contain oracle code"
+
+
+def test_product_sources_do_not_reference_the_oracle():
+    for p in (ROOT / "boris2_amd").rglob("*"):
+        if p.suffix in (".py", ".hip", ".cpp", ".h"):
+            txt = p.read_text()
+            assert "oracle" not in txt.lower(), p
