@@ -19,7 +19,7 @@
 //
 // Kernel setup runs once on the GPU in FP64 (Newell f/g grids, 27-point stencil or
 // far-field series, mirror placement, three real-input FFT passes) and stores FP32
-// kernels on the reduced octant, ky fastest.
+// kernels on the reduced octant (3D: kz fastest; 2D: ky fastest).
 #include "bm_internal.h"
 #include "bm_fft.h"
 #include "bm_newell.h"
@@ -28,6 +28,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <cstdlib>
 
 using namespace bm;
 
@@ -35,85 +36,111 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// x-pass LDS tile row: HX+1 spectrum values, and room for the padded FFT exchange
+template <int HX> constexpr int row_pitch() { return lds_len<HX>() > HX + 1 ? lds_len<HX>() : HX + 1; }
+
 template <int HX> struct TileY {
 	static constexpr int RP = kBlock / plan_T<HX>();
 	static constexpr int pw(int v) { int p = 1; while (p * 2 <= v) p *= 2; return p; }
-	static constexpr int cap = pw(8400 / (HX + 1)) < 32 ? pw(8400 / (HX + 1)) : 32;
+	static constexpr int cap = pw(8400 / row_pitch<HX>()) < 32 ? pw(8400 / row_pitch<HX>()) : 32;
 	static constexpr int v = RP > cap ? RP : cap;
 };
 
 // ---------------------------------------------------------------------------------------
-// P1: x-direction real-to-complex FFT of TY rows of one z plane for one component, via a
-// half-length complex FFT (z[k] = x[2k] + i x[2k+1]), then the usual split into the Nc
-// half spectrum, transposed into S through LDS.
-template <int HX>
+// P1: x-direction real-to-complex FFT of TY rows of one z plane, via a half-length complex
+// FFT (z[k] = x[2k] + i x[2k+1]) and the usual split into the Nc half spectrum, transposed
+// into S through LDS.  One workgroup does all three components of its rows, so M is read
+// from HBM once (each thread keeps the cells it transforms in registers).
+template <int HX, int TY>
 __global__ void __launch_bounds__(kBlock) k_fwd_x(const float* __restrict__ M, float2* __restrict__ S,
-	int ny, int nz, int Ny, int ntiles)
+	int ny, int nz, int Sp, int ntiles)
 {
-	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, TY = TileY<HX>::v, NC = HX + 1;
-	__shared__ C2<float> tile[TY * NC];
+	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, NC = HX + 1, RPITCH = row_pitch<HX>();
+	constexpr int NG = TY / RP;
+	__shared__ C2<float> tile[TY * RPITCH];
 	const int t = threadIdx.x % T, rl = threadIdx.x / T;
 	Twiddles<HX, float> tw; tw.init(t);
 	C2<float> wk[E];   // exp(-i pi k / HX), k = t + T e
 #pragma unroll
 	for (int e = 0; e < E; e++) { double s, c; sincospi(-(double)(t + T * e) / HX, &s, &c); wk[e] = mk((float)c, (float)s); }
 
-	const int nwork = 3 * nz * ntiles;
+	const int nwork = nz * ntiles;
 	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-		const int c = w % 3, z = (w / 3) % nz, y0 = (w / (3 * nz)) * TY;
-#pragma unroll 1
-		for (int g = 0; g < TY; g += RP) {
-			const int yl = g + rl, y = y0 + yl;
+		const int z = w % nz, y0 = (w / nz) * TY;
+		// cells 2k and 2k+1 of each of this thread's rows: 6 floats = 3 aligned float2
+		float2 m[NG][E / 2][3];
+#pragma unroll
+		for (int g = 0; g < NG; g++) {
+			const int y = y0 + g * RP + rl;
 			const bool ok = y < ny;
-			const float* row = M + ((size_t)z * ny + (ok ? y : 0)) * (size_t)HX * 3 + c;
-			C2<float> v[E];
+			const float2* row = reinterpret_cast<const float2*>(M + ((size_t)z * ny + (ok ? y : 0)) * (size_t)HX * 3);
 #pragma unroll
-			for (int e = 0; e < E; e++) {
+			for (int e = 0; e < E / 2; e++) {
 				const int k = t + T * e;
-				v[e] = (e < E / 2 && ok) ? mk(row[(size_t)(2 * k) * 3], row[(size_t)(2 * k + 1) * 3]) : mk(0.f, 0.f);
-			}
-			C2<float>* lrow = tile + yl * NC;
-			fft<HX, -1, float>(v, lrow, 1, t, tw);
-			__syncthreads();
 #pragma unroll
-			for (int e = 0; e < E; e++) lrow[t + T * e] = v[e];
-			__syncthreads();
-			C2<float> X[E];
-#pragma unroll
-			for (int e = 0; e < E; e++) {
-				const int k = t + T * e;
-				C2<float> zk = v[e], zm = conj(lrow[(HX - k) % HX]);
-				C2<float> a = zk + zm, b = zk - zm;
-				C2<float> wb = cmul(wk[e], b);              // W^k (Z[k] - conj Z[HX-k])
-				X[e] = mk(0.5f * (a.x + wb.y), 0.5f * (a.y - wb.x));   // a/2 - i wb/2
-			}
-			__syncthreads();
-#pragma unroll
-			for (int e = 0; e < E; e++) lrow[t + T * e] = X[e];
-			if (t == 0) lrow[HX] = mk(v[0].x - v[0].y, 0.f);
-		}
-		__syncthreads();
-		for (int idx = threadIdx.x; idx < TY * NC; idx += kBlock) {
-			const int yl = idx % TY, kx = idx / TY, y = y0 + yl;
-			if (y < ny) {
-				C2<float> a = tile[yl * NC + kx];
-				S[(((size_t)c * nz + z) * NC + kx) * (size_t)Ny + y] = make_float2(a.x, a.y);
+				for (int q = 0; q < 3; q++) m[g][e][q] = ok ? row[3 * k + q] : make_float2(0.f, 0.f);
 			}
 		}
-		__syncthreads();
+#pragma unroll 1
+		for (int c = 0; c < 3; c++) {
+#pragma unroll
+			for (int g = 0; g < NG; g++) {
+				const int yl = g * RP + rl;
+				C2<float> v[E];
+#pragma unroll
+				for (int e = 0; e < E; e++) {
+					if (e < E / 2) {
+						// component c of cell 2k is float c of the 6; of cell 2k+1 float 3 + c
+						const float2 a = m[g][e][0], b = m[g][e][1], d = m[g][e][2];
+						const float x0 = (c == 0) ? a.x : (c == 1) ? a.y : b.x;
+						const float x1 = (c == 0) ? b.y : (c == 1) ? d.x : d.y;
+						v[e] = mk(x0, x1);
+					}
+					else v[e] = mk(0.f, 0.f);
+				}
+				C2<float>* lrow = tile + yl * RPITCH;
+				fft<HX, -1, float>(v, lrow, 1, t, tw);
+				__syncthreads();
+#pragma unroll
+				for (int e = 0; e < E; e++) lrow[t + T * e] = v[e];
+				__syncthreads();
+				C2<float> X[E];
+#pragma unroll
+				for (int e = 0; e < E; e++) {
+					const int k = t + T * e;
+					C2<float> zk = v[e], zm = conj(lrow[(HX - k) % HX]);
+					C2<float> a = zk + zm, b = zk - zm;
+					C2<float> wb = cmul(wk[e], b);              // W^k (Z[k] - conj Z[HX-k])
+					X[e] = mk(0.5f * (a.x + wb.y), 0.5f * (a.y - wb.x));   // a/2 - i wb/2
+				}
+				__syncthreads();
+#pragma unroll
+				for (int e = 0; e < E; e++) lrow[t + T * e] = X[e];
+				if (t == 0) lrow[HX] = mk(v[0].x - v[0].y, 0.f);
+			}
+			__syncthreads();
+			for (int idx = threadIdx.x; idx < TY * NC; idx += kBlock) {
+				const int yl = idx % TY, kx = idx / TY, y = y0 + yl;
+				if (y < ny) {
+					C2<float> a = tile[yl * RPITCH + kx];
+					S[(((size_t)c * nz + z) * NC + kx) * (size_t)Sp + y] = make_float2(a.x, a.y);
+				}
+			}
+			__syncthreads();
+		}
 	}
 }
 
 // ---------------------------------------------------------------------------------------
 // P5: x-direction complex-to-real inverse FFT.  Output modes: 0 planar set, 1 AoS set,
 // 2 AoS add.
-template <int HX, int MODE, bool EN>
+template <int HX, int MODE, bool EN, int TY>
 __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ S, float* __restrict__ out,
-	int ny, int nz, int Ny, int ntiles, float scale, const float* __restrict__ M, double* __restrict__ epart)
+	int ny, int nz, int Sp, int ntiles, float scale, const float* __restrict__ M, double* __restrict__ epart)
 {
 	__shared__ double ered[EN ? kBlock : 1];
-	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, TY = TileY<HX>::v, NC = HX + 1;
-	__shared__ C2<float> tile[TY * NC];
+	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, NC = HX + 1, RPITCH = row_pitch<HX>();
+	__shared__ C2<float> tile[TY * RPITCH];
 	const int t = threadIdx.x % T, rl = threadIdx.x / T;
 	Twiddles<HX, float> tw; tw.init(t);
 	C2<float> wk[E];   // exp(-i pi k / HX)
@@ -122,21 +149,36 @@ __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ S, 
 	const int nx = HX;
 	const size_t ncell = (size_t)nx * ny * nz;
 
+	// the next work item's tile is loaded into registers while this one is transformed
+	constexpr int NLD = (TY * NC + kBlock - 1) / kBlock;
+	float2 pre[NLD];
+	auto fetch = [&](int wf) {
+		const int c = wf % 3, z = (wf / 3) % nz, y0 = (wf / (3 * nz)) * TY;
+#pragma unroll
+		for (int q = 0; q < NLD; q++) {
+			const int idx = threadIdx.x + kBlock * q;
+			const int yl = idx % TY, kx = idx / TY, y = y0 + yl;
+			pre[q] = (idx < TY * NC && y < ny) ? S[(((size_t)c * nz + z) * NC + kx) * (size_t)Sp + y] : make_float2(0.f, 0.f);
+		}
+	};
 	const int nwork = 3 * nz * ntiles;
+	if (blockIdx.x < nwork) fetch(blockIdx.x);
 	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
 		const int c = w % 3, z = (w / 3) % nz, y0 = (w / (3 * nz)) * TY;
 		double esum = 0.0;
-		for (int idx = threadIdx.x; idx < TY * NC; idx += kBlock) {
-			const int yl = idx % TY, kx = idx / TY, y = y0 + yl;
-			float2 a = (y < ny) ? S[(((size_t)c * nz + z) * NC + kx) * (size_t)Ny + y] : make_float2(0.f, 0.f);
-			tile[yl * NC + kx] = mk(a.x, a.y);
+		__syncthreads();
+#pragma unroll
+		for (int q = 0; q < NLD; q++) {
+			const int idx = threadIdx.x + kBlock * q;
+			if (idx < TY * NC) tile[(idx % TY) * RPITCH + idx / TY] = mk(pre[q].x, pre[q].y);
 		}
 		__syncthreads();
+		if (w + (int)gridDim.x < nwork) fetch(w + gridDim.x);
 #pragma unroll 1
 		for (int g = 0; g < TY; g += RP) {
 			const int yl = g + rl, y = y0 + yl;
 			const bool ok = y < ny;
-			C2<float>* lrow = tile + yl * NC;
+			C2<float>* lrow = tile + yl * RPITCH;
 			C2<float> v[E];
 #pragma unroll
 			for (int e = 0; e < E; e++) {
@@ -180,25 +222,32 @@ __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ S, 
 // ---------------------------------------------------------------------------------------
 // P2 / P4 (3D): y-direction FFT of lines S[c][z][kx][0..Ny).
 template <int NY, int DIR>
-__global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, int nlines)
+__global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, int nlines, int Sp)
 {
-	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = kBlock / T;
-	__shared__ C2<float> lds[LP * NY];
+	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = kBlock / T, LL = lds_len<NY>();
+	__shared__ C2<float> lds[LP * LL];
 	const int t = threadIdx.x % T, l = threadIdx.x / T;
 	Twiddles<NY, float> tw; tw.init(t);
 	const int nwork = (nlines + LP - 1) / LP;
+	// the next line group is loaded into registers while this one is transformed
+	float2 pre[E];
+	auto fetch = [&](int wf) {
+		const int line = wf * LP + l;
+		const float2* q = S + (size_t)(line < nlines ? line : 0) * Sp;
+#pragma unroll
+		for (int e = 0; e < E; e++)
+			pre[e] = (line < nlines && (DIR > 0 || e < E / 2)) ? q[t + T * e] : make_float2(0.f, 0.f);
+	};
+	if (blockIdx.x < nwork) fetch(blockIdx.x);
 	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
 		const int line = w * LP + l;
 		const bool ok = line < nlines;
-		float2* p = S + (size_t)(ok ? line : 0) * NY;
+		float2* p = S + (size_t)(ok ? line : 0) * Sp;
 		C2<float> v[E];
 #pragma unroll
-		for (int e = 0; e < E; e++) {
-			float2 a = make_float2(0.f, 0.f);
-			if (ok && (DIR > 0 || e < E / 2)) a = p[t + T * e];
-			v[e] = mk(a.x, a.y);
-		}
-		fft<NY, DIR, float>(v, lds + l * NY, 1, t, tw);
+		for (int e = 0; e < E; e++) v[e] = mk(pre[e].x, pre[e].y);
+		if (w + (int)gridDim.x < nwork) fetch(w + gridDim.x);
+		fft<NY, DIR, float>(v, lds + l * LL, 1, t, tw);
 		if (ok) {
 #pragma unroll
 			for (int e = 0; e < E; e++)
@@ -212,42 +261,78 @@ __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, int nlines
 // symmetric kernel multiply, z-IFFT, first nz planes written back.  Kernel symmetries
 // (Boris/DemagKernel.cpp:85-168): diagonal even in y and z; Kxy odd in y; Kxz odd in z;
 // Kyz odd in y and z.
+// A workgroup takes CH columns ky = CH*b + c below N.y/2 and the CH mirror columns
+// ky = N.y - CH*(b+1) + c at or above it, so both halves use kernel rows
+// kyr in [CH*b, CH*b + CH]: that slab (all kz, 6 components, contiguous because the 3D
+// kernel is stored kz fastest) is read from HBM once into LDS and serves the kz and N.z-kz
+// mirrors of every column.
 template <int NZ>
 __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const float* __restrict__ K,
-	int nz, int NC, int Ny, int NYH)
+	int nz, int NC, int Ny, int NYH, int Sp)
 {
-	constexpr int E = Plan<NZ>::E, TZ = plan_T<NZ>(), CY = kBlock / TZ, NZH = NZ / 2 + 1;
+	constexpr int E = Plan<NZ>::E, TZ = plan_T<NZ>(), CY = kBlock / TZ, CH = CY / 2, NZH = NZ / 2 + 1, KR = CH + 1;
+	constexpr int KS = 6 * KR * NZH, NKL = (KS + kBlock - 1) / kBlock;
 	__shared__ C2<float> lds[3 * NZ * CY];
+	__shared__ float ks[KS];
 	const int col = threadIdx.x % CY, t = threadIdx.x / CY;
+	const bool hi = col >= CH;
+	const int c = hi ? col - CH : col;
 	Twiddles<NZ, float> tw; tw.init(t);
-	const int nyb = (Ny + CY - 1) / CY;
-	const size_t plane = (size_t)NC * Ny, comp = (size_t)nz * plane;
-	const size_t kq = (size_t)NZH * NC * NYH;
-	for (int w = blockIdx.x; w < NC * nyb; w += gridDim.x) {
-		const int kx = w / nyb, ky = (w % nyb) * CY + col;
-		const bool ok = ky < Ny;
-		float2* p = S + (size_t)kx * Ny + (ok ? ky : 0);
+	const int half = Ny / 2, nwb = (half + CH - 1) / CH, nwork = NC * nwb;
+	const size_t plane = (size_t)NC * Sp, comp = (size_t)nz * plane;
+	const size_t kq = (size_t)NC * NYH * NZH;
+
+	// Work item w: its columns' data (registers) and its kernel slab (registers, then LDS)
+	// are fetched one item ahead, so loads overlap the transforms of the current item.
+	float2 pd[3][E / 2];
+	float pk[NKL];
+	auto fetch = [&](int wf) {
+		const int kx = wf / nwb, k0 = CH * (wf % nwb);
+		const int ky = hi ? Ny - (k0 + CH) + c : k0 + c;
+		const bool ok = hi ? (ky >= half && ky < Ny) : (ky < half);
+		const float2* p = S + (size_t)kx * Sp + (ok ? ky : 0);
+#pragma unroll
+		for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+			for (int e = 0; e < E / 2; e++) {
+				const int z = t + TZ * e;
+				pd[cc][e] = (ok && z < nz) ? p[cc * comp + z * plane] : make_float2(0.f, 0.f);
+			}
+		const int nrows = min(KR, NYH - k0);
+		const float* kb = K + ((size_t)kx * NYH + k0) * NZH;
+#pragma unroll
+		for (int q = 0; q < NKL; q++) {
+			const int i = threadIdx.x + kBlock * q;
+			const int qq = i / (KR * NZH), rem = i - qq * (KR * NZH);
+			pk[q] = (i < KS && rem / NZH < nrows) ? kb[qq * kq + rem] : 0.f;
+		}
+	};
+	if ((int)blockIdx.x < nwork) fetch(blockIdx.x);
+	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+		const int kx = w / nwb, k0 = CH * (w % nwb);
+		const int ky = hi ? Ny - (k0 + CH) + c : k0 + c;
+		const bool ok = hi ? (ky >= half && ky < Ny) : (ky < half);
+		const int lr = hi ? CH - c : c;                    // kernel row of this column, relative to k0
 		C2<float> v[3][E];
 #pragma unroll
-		for (int c = 0; c < 3; c++)
+		for (int cc = 0; cc < 3; cc++)
 #pragma unroll
-			for (int e = 0; e < E; e++) {
-				const int z = t + TZ * e;
-				float2 a = make_float2(0.f, 0.f);
-				if (e < E / 2 && ok && z < nz) a = p[c * comp + z * plane];
-				v[c][e] = mk(a.x, a.y);
-			}
+			for (int e = 0; e < E; e++) v[cc][e] = (e < E / 2) ? mk(pd[cc][e].x, pd[cc][e].y) : mk(0.f, 0.f);
+		__syncthreads();                                   // the previous slab is no longer read
+#pragma unroll
+		for (int q = 0; q < NKL; q++) { const int i = threadIdx.x + kBlock * q; if (i < KS) ks[i] = pk[q]; }
+		if (w + (int)gridDim.x < nwork) fetch(w + gridDim.x);
 		fft3<NZ, -1, float>(v[0], v[1], v[2], lds + col, lds + NZ * CY + col, lds + 2 * NZ * CY + col, CY, t, tw);
-		const int kyr = ok ? (ky <= Ny / 2 ? ky : Ny - ky) : 0;
-		const float sy = (ky <= Ny / 2) ? 1.f : -1.f;
+		if constexpr (Plan<NZ>::S == 1) __syncthreads();  // single-stage plans have no barrier inside
+		const float sy = (ky <= half) ? 1.f : -1.f;
+		const float* kr = ks + (ok ? lr : 0) * NZH;
 #pragma unroll
 		for (int e = 0; e < E; e++) {
 			const int kz = t + TZ * e;
 			const int kzr = kz <= NZ / 2 ? kz : NZ - kz;
 			const float sz = kz <= NZ / 2 ? 1.f : -1.f;
-			const size_t ki = ((size_t)kzr * NC + kx) * NYH + kyr;
-			const float kxx = K[ki], kyy = K[kq + ki], kzz = K[2 * kq + ki];
-			const float kxy = sy * K[3 * kq + ki], kxz = sz * K[4 * kq + ki], kyz = sy * sz * K[5 * kq + ki];
+			const float kxx = kr[kzr], kyy = kr[KR * NZH + kzr], kzz = kr[2 * KR * NZH + kzr];
+			const float kxy = sy * kr[3 * KR * NZH + kzr], kxz = sz * kr[4 * KR * NZH + kzr], kyz = sy * sz * kr[5 * KR * NZH + kzr];
 			C2<float> mx = v[0][e], my = v[1][e], mz = v[2][e];
 			v[0][e] = mk(kxx * mx.x + kxy * my.x + kxz * mz.x, kxx * mx.y + kxy * my.y + kxz * mz.y);
 			v[1][e] = mk(kxy * mx.x + kyy * my.x + kyz * mz.x, kxy * mx.y + kyy * my.y + kyz * mz.y);
@@ -255,12 +340,13 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 		}
 		fft3<NZ, 1, float>(v[0], v[1], v[2], lds + col, lds + NZ * CY + col, lds + 2 * NZ * CY + col, CY, t, tw);
 		if (ok) {
+			float2* p = S + (size_t)kx * Sp + ky;
 #pragma unroll
-			for (int c = 0; c < 3; c++)
+			for (int cc = 0; cc < 3; cc++)
 #pragma unroll
 				for (int e = 0; e < E / 2; e++) {
 					const int z = t + TZ * e;
-					if (z < nz) p[c * comp + z * plane] = make_float2(v[c][e].x, v[c][e].y);
+					if (z < nz) p[cc * comp + z * plane] = make_float2(v[cc][e].x, v[cc][e].y);
 				}
 		}
 	}
@@ -269,18 +355,18 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 // ---------------------------------------------------------------------------------------
 // P2' (2D): per kx line, y-FFT of 3 components, 2D kernel multiply (Kxy odd in y), y-IFFT.
 template <int NY, int BS>
-__global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ S, const float* __restrict__ K, int NC, int NYH)
+__global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ S, const float* __restrict__ K, int NC, int NYH, int Sp)
 {
-	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = BS / T;
-	__shared__ C2<float> lds[3 * LP * NY];
+	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = BS / T, LL = lds_len<NY>();
+	__shared__ C2<float> lds[3 * LP * LL];
 	const int t = threadIdx.x % T, l = threadIdx.x / T;
 	Twiddles<NY, float> tw; tw.init(t);
-	const size_t comp = (size_t)NC * NY, kq = (size_t)NC * NYH;
+	const size_t comp = (size_t)NC * Sp, kq = (size_t)NC * NYH;
 	const int nwork = (NC + LP - 1) / LP;
 	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
 		const int kx = w * LP + l;
 		const bool ok = kx < NC;
-		float2* p = S + (size_t)(ok ? kx : 0) * NY;
+		float2* p = S + (size_t)(ok ? kx : 0) * Sp;
 		C2<float> v[3][E];
 #pragma unroll
 		for (int c = 0; c < 3; c++)
@@ -289,8 +375,8 @@ __global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ S, const float*
 				float2 a = (ok && e < E / 2) ? p[c * comp + t + T * e] : make_float2(0.f, 0.f);
 				v[c][e] = mk(a.x, a.y);
 			}
-		C2<float>* b0 = lds + l * NY;
-		fft3<NY, -1, float>(v[0], v[1], v[2], b0, b0 + LP * NY, b0 + 2 * LP * NY, 1, t, tw);
+		C2<float>* b0 = lds + l * LL;
+		fft3<NY, -1, float>(v[0], v[1], v[2], b0, b0 + LP * LL, b0 + 2 * LP * LL, 1, t, tw);
 		const int kxs = ok ? kx : 0;
 #pragma unroll
 		for (int e = 0; e < E; e++) {
@@ -304,7 +390,7 @@ __global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ S, const float*
 			v[1][e] = mk(kxy * mx.x + kyy * my.x, kxy * mx.y + kyy * my.y);
 			v[2][e] = mk(kzz * mz.x, kzz * mz.y);
 		}
-		fft3<NY, 1, float>(v[0], v[1], v[2], b0, b0 + LP * NY, b0 + 2 * LP * NY, 1, t, tw);
+		fft3<NY, 1, float>(v[0], v[1], v[2], b0, b0 + LP * LL, b0 + 2 * LP * LL, 1, t, tw);
 		if (ok) {
 #pragma unroll
 			for (int c = 0; c < 3; c++)
@@ -409,8 +495,8 @@ template <int N, typename TO>
 __global__ void __launch_bounds__(kBlock) k_line_fft(const double* __restrict__ src, TO* __restrict__ dst, int nlines, int na,
 	long long sa, long long sb, long long se, long long da, long long db, long long de, int keep, int part, double sign)
 {
-	constexpr int E = Plan<N>::E, T = plan_T<N>(), LP = kBlock / T;
-	__shared__ C2<double> lds[LP * N];
+	constexpr int E = Plan<N>::E, T = plan_T<N>(), LP = kBlock / T, LL = lds_len<N>();
+	__shared__ C2<double> lds[LP * LL];
 	const int t = threadIdx.x % T, l = threadIdx.x / T;
 	Twiddles<N, double> tw; tw.init(t);
 	const int nwork = (nlines + LP - 1) / LP;
@@ -422,7 +508,7 @@ __global__ void __launch_bounds__(kBlock) k_line_fft(const double* __restrict__ 
 		C2<double> v[E];
 #pragma unroll
 		for (int e = 0; e < E; e++) v[e] = mk(ok ? p[(t + T * e) * se] : 0.0, 0.0);
-		fft<N, -1, double>(v, lds + l * N, 1, t, tw);
+		fft<N, -1, double>(v, lds + l * LL, 1, t, tw);
 		if (ok) {
 			TO* q = dst + a * da + b * db;
 #pragma unroll
@@ -453,27 +539,46 @@ bool z_supported(int n) { switch (n) {
 
 int grid_for(int nwork, int ncu, int per_cu) { return std::max(1, std::min(nwork, ncu * per_cu)); }
 
-int launch_fwd_x(int HX, const float* M, float2* S, int ny, int nz, int Ny, int ncu, hipStream_t st)
+// Row-tile height factor of the x passes (1 or 2 times TileY).  Measured at 512x512x32:
+// the forward pass is fastest with the taller tile (its transposed stores become whole
+// 128-byte lines), the inverse pass with the shorter one (its register prefetch of the next
+// tile keeps occupancy up).  BM_XTILE_FWD / BM_XTILE_INV override for tuning runs.
+int xtile_factor(bool fwd)
 {
+	static int f[2] = { -1, -1 };
+	int& v = f[fwd ? 1 : 0];
+	if (v < 0) {
+		const char* e = getenv(fwd ? "BM_XTILE_FWD" : "BM_XTILE_INV");
+		v = e ? (atoi(e) == 2 ? 2 : 1) : (fwd ? 2 : 1);
+	}
+	return v;
+}
+
+int launch_fwd_x(int HX, const float* M, float2* S, int ny, int nz, int Sp, int ncu, hipStream_t st)
+{
+	const int f = xtile_factor(true);
 	switch (HX) {
-#define C_(V_) case V_: { int TY = TileY<V_>::v, nt = (ny + TY - 1) / TY; \
-		hipLaunchKernelGGL(k_fwd_x<V_>, dim3(grid_for(3 * nz * nt, ncu, 4)), dim3(kBlock), 0, st, M, S, ny, nz, Ny, nt); return BM_OK; }
+#define C_(V_) case V_: { int TY = TileY<V_>::v * f, nt = (ny + TY - 1) / TY; dim3 g(grid_for(nz * nt, ncu, 2)); \
+		if (f == 1) hipLaunchKernelGGL((k_fwd_x<V_, TileY<V_>::v>), g, dim3(kBlock), 0, st, M, S, ny, nz, Sp, nt); \
+		else hipLaunchKernelGGL((k_fwd_x<V_, 2 * TileY<V_>::v>), g, dim3(kBlock), 0, st, M, S, ny, nz, Sp, nt); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no x plan");
 }
 
-int launch_inv_x(int HX, int mode, const float2* S, float* out, int ny, int nz, int Ny, float scale, int ncu, hipStream_t st,
+int launch_inv_x(int HX, int mode, const float2* S, float* out, int ny, int nz, int Sp, float scale, int ncu, hipStream_t st,
 	const float* M = nullptr, double* epart = nullptr)
 {
+	const int f = (mode == 0) ? xtile_factor(false) : 1;
 	switch (HX) {
-#define C_(V_) case V_: { int TY = TileY<V_>::v, nt = (ny + TY - 1) / TY; dim3 g(grid_for(3 * nz * nt, ncu, 4)); \
-		if (mode == 0) hipLaunchKernelGGL((k_inv_x<V_, 0, false>), g, dim3(kBlock), 0, st, S, out, ny, nz, Ny, nt, scale, M, epart); \
-		else if (mode == 1 && !epart) hipLaunchKernelGGL((k_inv_x<V_, 1, false>), g, dim3(kBlock), 0, st, S, out, ny, nz, Ny, nt, scale, M, epart); \
-		else if (mode == 1) hipLaunchKernelGGL((k_inv_x<V_, 1, true>), g, dim3(kBlock), 0, st, S, out, ny, nz, Ny, nt, scale, M, epart); \
-		else if (!epart) hipLaunchKernelGGL((k_inv_x<V_, 2, false>), g, dim3(kBlock), 0, st, S, out, ny, nz, Ny, nt, scale, M, epart); \
-		else hipLaunchKernelGGL((k_inv_x<V_, 2, true>), g, dim3(kBlock), 0, st, S, out, ny, nz, Ny, nt, scale, M, epart); return BM_OK; }
+#define C_(V_) case V_: { constexpr int TY1 = TileY<V_>::v; int TY = TY1 * f, nt = (ny + TY - 1) / TY; dim3 g(grid_for(3 * nz * nt, ncu, 2)); \
+		if (mode == 0 && f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); \
+		else if (mode == 0) hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); \
+		else if (mode == 1 && !epart) hipLaunchKernelGGL((k_inv_x<V_, 1, false, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); \
+		else if (mode == 1) hipLaunchKernelGGL((k_inv_x<V_, 1, true, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); \
+		else if (!epart) hipLaunchKernelGGL((k_inv_x<V_, 2, false, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); \
+		else hipLaunchKernelGGL((k_inv_x<V_, 2, true, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -490,34 +595,34 @@ int inv_x_nwork(int HX, int ny, int nz)
 	return 0;
 }
 
-int launch_y(int NY, int dir, float2* S, int nlines, int ncu, hipStream_t st)
+int launch_y(int NY, int dir, float2* S, int nlines, int Sp, int ncu, hipStream_t st)
 {
 	switch (NY) {
 #define C_(V_) case V_: { int LP = kBlock / plan_T<V_>(); dim3 g(grid_for((nlines + LP - 1) / LP, ncu, 8)); \
-		if (dir < 0) hipLaunchKernelGGL((k_y<V_, -1>), g, dim3(kBlock), 0, st, S, nlines); \
-		else hipLaunchKernelGGL((k_y<V_, 1>), g, dim3(kBlock), 0, st, S, nlines); return BM_OK; }
+		if (dir < 0) hipLaunchKernelGGL((k_y<V_, -1>), g, dim3(kBlock), 0, st, S, nlines, Sp); \
+		else hipLaunchKernelGGL((k_y<V_, 1>), g, dim3(kBlock), 0, st, S, nlines, Sp); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no y plan");
 }
 
-int launch_z(int NZ, float2* S, const float* K, int nz, int NC, int Ny, int NYH, int ncu, hipStream_t st)
+int launch_z(int NZ, float2* S, const float* K, int nz, int NC, int Ny, int NYH, int Sp, int ncu, hipStream_t st)
 {
 	switch (NZ) {
-#define C_(V_) case V_: { int CY = kBlock / plan_T<V_>(); int nw = NC * ((Ny + CY - 1) / CY); \
-		hipLaunchKernelGGL(k_z<V_>, dim3(grid_for(nw, ncu, 8)), dim3(kBlock), 0, st, S, K, nz, NC, Ny, NYH); return BM_OK; }
+#define C_(V_) case V_: { int CH = kBlock / plan_T<V_>() / 2; int nw = NC * ((Ny / 2 + CH - 1) / CH); \
+		hipLaunchKernelGGL(k_z<V_>, dim3(grid_for(nw, ncu, 2)), dim3(kBlock), 0, st, S, K, nz, NC, Ny, NYH, Sp); return BM_OK; }
 	BM_FOR_Z_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no z plan");
 }
 
-int launch_y2d(int NY, float2* S, const float* K, int NC, int NYH, int ncu, hipStream_t st)
+int launch_y2d(int NY, float2* S, const float* K, int NC, int NYH, int Sp, int ncu, hipStream_t st)
 {
 	switch (NY) {
 #define C_(V_) case V_: { constexpr int BS = Plan<V_>::E > 8 ? 128 : 256; int LP = BS / plan_T<V_>(); \
-		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid_for((NC + LP - 1) / LP, ncu, 8)), dim3(BS), 0, st, S, K, NC, NYH); return BM_OK; }
+		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid_for((NC + LP - 1) / LP, ncu, 8)), dim3(BS), 0, st, S, K, NC, NYH, Sp); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -604,8 +709,8 @@ int compute_kernels(bm_demag_s* d, hipStream_t st)
 			rc = launch_line_fft<double>(Nx, Dq, T1, Ny * Nz, Ny * Nz, Nx, 0, 1, NC, 0, 1, NC, part[q][0], 1.0, st);
 			// y: line (kx, z) -> T2[z][ky][kx]
 			if (rc == BM_OK) rc = launch_line_fft<double>(Ny, T1, T2, NC * Nz, NC, 1, (long long)NC * Ny, NC, 1, (long long)NC * NYH, NC, NYH, part[q][1], 1.0, st);
-			// z: line (kx, ky) -> K[kz][kx][ky]
-			if (rc == BM_OK) rc = launch_line_fft<float>(Nz, T2, Kq, NC * NYH, NC, 1, NC, (long long)NC * NYH, NYH, 1, (long long)NC * NYH, NZH, part[q][2], sgn, st);
+			// z: line (kx, ky) -> K[kx][ky][kz] (kz fastest: the z pass stages kernel slabs contiguously)
+			if (rc == BM_OK) rc = launch_line_fft<float>(Nz, T2, Kq, NC * NYH, NC, 1, NC, (long long)NC * NYH, (long long)NYH * NZH, NZH, 1, NZH, part[q][2], sgn, st);
 		}
 	}
 	if (rc == BM_OK) { hipError_t e = hipStreamSynchronize(st); if (e != hipSuccess) rc = hip_fail(e, "kernel setup"); }
@@ -624,25 +729,25 @@ static int forward_and_multiply(bm_demag_s* d, const float* M, hipStream_t st)
 {
 	const int HX = d->nx, nlines = 3 * d->nz * d->NC;
 	prof_mark_begin(PK_FWD_X, st);
-	int rc = launch_fwd_x(HX, M, d->S, d->ny, d->nz, d->Ny, d->ncu, st);
+	int rc = launch_fwd_x(HX, M, d->S, d->ny, d->nz, d->Sp, d->ncu, st);
 	prof_mark_end(PK_FWD_X, st);
 	if (rc) return rc;
 	if (d->is2d) {
 		prof_mark_begin(PK_Y2D, st);
-		rc = launch_y2d(d->Ny, d->S, d->K, d->NC, d->NYH, d->ncu, st);
+		rc = launch_y2d(d->Ny, d->S, d->K, d->NC, d->NYH, d->Sp, d->ncu, st);
 		prof_mark_end(PK_Y2D, st);
 		return rc;
 	}
 	prof_mark_begin(PK_Y_FWD, st);
-	rc = launch_y(d->Ny, -1, d->S, nlines, d->ncu, st);
+	rc = launch_y(d->Ny, -1, d->S, nlines, d->Sp, d->ncu, st);
 	prof_mark_end(PK_Y_FWD, st);
 	if (rc) return rc;
 	prof_mark_begin(PK_Z, st);
-	rc = launch_z(d->Nz, d->S, d->K, d->nz, d->NC, d->Ny, d->NYH, d->ncu, st);
+	rc = launch_z(d->Nz, d->S, d->K, d->nz, d->NC, d->Ny, d->NYH, d->Sp, d->ncu, st);
 	prof_mark_end(PK_Z, st);
 	if (rc) return rc;
 	prof_mark_begin(PK_Y_INV, st);
-	rc = launch_y(d->Ny, 1, d->S, nlines, d->ncu, st);
+	rc = launch_y(d->Ny, 1, d->S, nlines, d->Sp, d->ncu, st);
 	prof_mark_end(PK_Y_INV, st);
 	return rc;
 }
@@ -653,7 +758,7 @@ int bm::demag_planar(bm_demag_s* d, const float* M, float* H3n, hipStream_t st)
 	if (rc) return rc;
 	const float scale = (float)(1.0 / ((double)d->Nx * d->Ny * d->Nz));
 	prof_mark_begin(PK_INV_X, st);
-	rc = launch_inv_x(d->nx, 0, d->S, H3n, d->ny, d->nz, d->Ny, scale, d->ncu, st);
+	rc = launch_inv_x(d->nx, 0, d->S, H3n, d->ny, d->nz, d->Sp, scale, d->ncu, st);
 	prof_mark_end(PK_INV_X, st);
 	return rc;
 }
@@ -677,7 +782,10 @@ extern "C" int bm_demag_create(int nx, int ny, int nz, double hx, double hy, dou
 		return fail(BM_ERR_SIZE, "bm_demag_create: no FFT plan for this mesh size (supported: nx in {2..2048 of 2^a 3^b plans}, Ny = 2ny likewise, Nz <= 128)");
 	}
 	d->ncu = device_cu_count();
-	size_t sbytes = (size_t)3 * nz * d->NC * d->Ny * sizeof(float2);
+	// line pitch of the spectra (complex elements); BM_S_PAD adds padding for tuning experiments
+	d->Sp = d->Ny;
+	if (const char* e = getenv("BM_S_PAD")) d->Sp += std::max(0, atoi(e));
+	size_t sbytes = (size_t)3 * nz * d->NC * d->Sp * sizeof(float2);
 	d->kcount = d->is2d ? 4LL * d->NC * d->NYH : 6LL * d->NZH * d->NC * d->NYH;
 	if (hipMalloc(&d->S, sbytes) != hipSuccess) { delete d; return fail(BM_ERR_NOMEM, "bm_demag_create: spectra"); }
 	if (hipMalloc(&d->K, d->kcount * sizeof(float)) != hipSuccess) { (void)hipFree(d->S); delete d; return fail(BM_ERR_NOMEM, "bm_demag_create: kernels"); }
@@ -721,7 +829,7 @@ extern "C" int bm_demag_convolute(bm_demag_t d, const float* M, float* Heff, int
 		const int np = inv_x_nwork(HX, d->ny, d->nz);
 		if (np > d->ecap) return fail(BM_ERR_STATE, "energy partial buffer too small");
 		prof_mark_begin(PK_INV_X, st);
-		rc = launch_inv_x(HX, clear_out ? 1 : 2, d->S, Heff, d->ny, d->nz, d->Ny, scale, d->ncu, st, M, d->ebuf);
+		rc = launch_inv_x(HX, clear_out ? 1 : 2, d->S, Heff, d->ny, d->nz, d->Sp, scale, d->ncu, st, M, d->ebuf);
 		prof_mark_end(PK_INV_X, st);
 		if (rc) return rc;
 		size_t n = (size_t)d->nx * d->ny * d->nz;
@@ -729,7 +837,7 @@ extern "C" int bm_demag_convolute(bm_demag_t d, const float* M, float* Heff, int
 	}
 	else {
 		prof_mark_begin(PK_INV_X, st);
-		rc = launch_inv_x(HX, clear_out ? 1 : 2, d->S, Heff, d->ny, d->nz, d->Ny, scale, d->ncu, st);
+		rc = launch_inv_x(HX, clear_out ? 1 : 2, d->S, Heff, d->ny, d->nz, d->Sp, scale, d->ncu, st);
 		prof_mark_end(PK_INV_X, st);
 		if (rc) return rc;
 	}

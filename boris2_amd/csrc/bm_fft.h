@@ -180,7 +180,14 @@ template <int N, typename T> struct Twiddles {
 	__device__ __forceinline__ void init(int t) { init_stage<1>(t); }
 };
 
-// One Stockham stage on registers.  LDS element i of this line lives at lds[i * stride].
+// LDS element i of a line: lines with unit stride are padded by one element every 16 so
+// the radix-R scatter of a Stockham stage (lane stride R elements) hits distinct banks;
+// strided lines (several lines interleaved element by element) are conflict-free as is.
+__device__ __forceinline__ int lidx(int i, int stride) { return stride == 1 ? i + (i >> 4) : i * stride; }
+// LDS elements one padded unit-stride line of length N occupies
+template <int N> constexpr int lds_len() { return N + N / 16 + 1; }
+
+// One Stockham stage on registers.  LDS element i of this line lives at lds[lidx(i, stride)].
 template <int N, int s, int DIR, typename T>
 __device__ __forceinline__ void fft_stage(C2<T>* v, C2<T>* lds, int stride, int t, const Twiddles<N, T>& tw)
 {
@@ -189,7 +196,7 @@ __device__ __forceinline__ void fft_stage(C2<T>* v, C2<T>* lds, int stride, int 
 #pragma unroll
 		for (int b = 0; b < B; b++)
 #pragma unroll
-			for (int r = 0; r < R; r++) v[b + r * B] = lds[(t + b * TT + r * (N / R)) * stride];
+			for (int r = 0; r < R; r++) v[b + r * B] = lds[lidx(t + b * TT + r * (N / R), stride)];
 #pragma unroll
 		for (int b = 0; b < B; b++)
 #pragma unroll
@@ -214,7 +221,7 @@ __device__ __forceinline__ void fft_stage(C2<T>* v, C2<T>* lds, int stride, int 
 			int j = t + b * TT;
 			int d = (j / Ns) * Ns * R + (j % Ns);
 #pragma unroll
-			for (int r = 0; r < R; r++) lds[(d + r * Ns) * stride] = v[b + r * B];
+			for (int r = 0; r < R; r++) lds[lidx(d + r * Ns, stride)] = v[b + r * B];
 		}
 		__syncthreads();
 	}
@@ -250,7 +257,7 @@ __device__ __forceinline__ void fft3_stages(C2<T>* v0, C2<T>* v1, C2<T>* v2, C2<
 #pragma unroll
 			for (int b = 0; b < B; b++)
 #pragma unroll
-				for (int r = 0; r < R; r++) vv[c][b + r * B] = ll[c][(t + b * TT + r * (N / R)) * stride];
+				for (int r = 0; r < R; r++) vv[c][b + r * B] = ll[c][lidx(t + b * TT + r * (N / R), stride)];
 #pragma unroll
 			for (int b = 0; b < B; b++)
 #pragma unroll
@@ -280,7 +287,7 @@ __device__ __forceinline__ void fft3_stages(C2<T>* v0, C2<T>* v1, C2<T>* v2, C2<
 				int j = t + b * TT;
 				int d = (j / Ns) * Ns * R + (j % Ns);
 #pragma unroll
-				for (int r = 0; r < R; r++) ll[c][(d + r * Ns) * stride] = vv[c][b + r * B];
+				for (int r = 0; r < R; r++) ll[c][lidx(d + r * Ns, stride)] = vv[c][b + r * B];
 			}
 		__syncthreads();
 		fft3_stages<N, s + 1, DIR, T>(v0, v1, v2, l0, l1, l2, stride, t, tw);

@@ -27,6 +27,7 @@ void prof_mark_end(int kind, hipStream_t st);
 struct bm_demag_s {
 	int nx, ny, nz;
 	int Nx, Ny, Nz, NC, NYH, NZH;
+	int Sp;                   // line pitch of the spectra (complex elements, >= Ny)
 	double h[3];
 	bool is2d;
 	float2* S = nullptr;      // spectra: 3 * nz * NC * Ny complex

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstring>
 #include <algorithm>
+#include <cstdlib>
 
 namespace {
 
@@ -52,6 +53,7 @@ struct StageArgs {
 	float ea[3], Ha[3];
 	float dT;
 	int modules;
+	int map;
 };
 
 enum { M_RKDP = 0, M_RK4 = 1, M_TEULER = 2, M_AHEUN = 3 };
@@ -71,12 +73,32 @@ __device__ __forceinline__ void block_max_to(float v, unsigned* dst)
 	}
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).  Chunk c of
+// kBS consecutive cells is given to block b with c = (b mod 8) * ceil(nb/8) + b/8, so every XCD
+// streams one contiguous eighth of the mesh and the exchange stencil's +-y rows (a few chunks
+// away) are read from the same L2.
+__device__ __forceinline__ long long xcd_chunk(int b, int nb)
+{
+	const int per = (nb + 7) / 8;
+	return (long long)(b % 8) * per + b / 8;
+}
+
 template <int METHOD, int STEP>
 __global__ void __launch_bounds__(kBS) k_stage(StageArgs a)
 {
 	const size_t n = (size_t)a.nx * a.ny * a.nz;
+	const long long nchunk = (long long)((n + kBS - 1) / kBS);
 	float lte_loc = 0.f;
-	for (size_t idx = blockIdx.x * (size_t)kBS + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kBS) {
+	// a.map 0: grid-stride over chunks; 1: one chunk per block, each XCD a contiguous eighth;
+	// 2: grid-stride sweeps of gridDim chunks, each XCD a contiguous eighth of every sweep
+	for (long long r = 0;; r++) {
+		long long chunk;
+		if (a.map == 1) { if (r) break; chunk = xcd_chunk(blockIdx.x, gridDim.x); }
+		else if (a.map == 2) chunk = r * gridDim.x + xcd_chunk(blockIdx.x, gridDim.x);
+		else chunk = r * gridDim.x + blockIdx.x;
+		if (chunk >= nchunk) { if (a.map == 1) break; if (r * gridDim.x >= nchunk) break; continue; }
+		const size_t idx = (size_t)chunk * kBS + threadIdx.x;
+		if (idx >= n) continue;
 		const int i = (int)(idx % a.nx), j = (int)((idx / a.nx) % a.ny), k = (int)(idx / ((size_t)a.nx * a.ny));
 		const F3 m = ld3(a.Min, idx);
 		const float Ms = a.Ms_map ? a.Ms_map[idx] : a.Ms;
@@ -239,6 +261,20 @@ struct bm_ode_s {
 namespace {
 
 int stage_grid(bm_ode_s* o) { return (int)std::max<size_t>(1, std::min<size_t>((o->n + kBS - 1) / kBS, (size_t)o->ncu * 16)); }
+// stage kernels: one kBS-cell chunk per block, rounded up to a multiple of 8 blocks (XCD map)
+// Block-to-cell mapping of the stage kernels (BM_STAGE_MAP overrides; 2 measured fastest).
+int stage_map()
+{
+	static int m = -1;
+	if (m < 0) { const char* e = getenv("BM_STAGE_MAP"); m = e ? atoi(e) : 2; }
+	return m;
+}
+int stage_chunks(bm_ode_s* o)
+{
+	size_t nb = (o->n + kBS - 1) / kBS;
+	if (stage_map() == 1) return (int)(((nb + 7) / 8) * 8);
+	return (int)std::max<size_t>(8, std::min<size_t>(((nb + 7) / 8) * 8, (size_t)o->ncu * 16));
+}
 
 void applied_field(const bm_fm_params& p, double t, double H[3])
 {
@@ -269,13 +305,14 @@ StageArgs make_args(bm_ode_s* o, float dT)
 	for (int c = 0; c < 3; c++) a.Ha[c] = (float)H[c];
 	a.dT = dT;
 	a.modules = o->p.modules;
+	a.map = stage_map();
 	return a;
 }
 
 int launch_stage(bm_ode_s* o, int step, float dT, hipStream_t st)
 {
 	StageArgs a = make_args(o, dT);
-	dim3 g(stage_grid(o)), b(kBS);
+	dim3 g(stage_chunks(o)), b(kBS);
 	bm::prof_mark_begin(bm::PK_STAGE, st);
 	switch (o->mcode * 8 + step) {
 	case M_RKDP * 8 + 0: hipLaunchKernelGGL((k_stage<M_RKDP, 0>), g, b, 0, st, a); break;

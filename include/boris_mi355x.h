@@ -63,7 +63,7 @@ int bm_demag_convolute(bm_demag_t d, const float* M, float* Heff, int clear_out,
 int bm_demag_convolute_planar(bm_demag_t d, const float* M, float* H3n, void* stream);
 
 /* Copies the FP32 demag kernels to host memory (test hook).  3D: 6 planes Kxx, Kyy, Kzz,
- * Kxy, Kxz, Kyz each (N.z/2+1)*(N.x/2+1)*(N.y/2+1), index (kz*(N.x/2+1)+kx)*(N.y/2+1)+ky.
+ * Kxy, Kxz, Kyz each (N.x/2+1)*(N.y/2+1)*(N.z/2+1), index (kx*(N.y/2+1)+ky)*(N.z/2+1)+kz.
  * 2D: 4 planes Kxx, Kyy, Kzz, Kxy each (N.x/2+1)*(N.y/2+1), index kx*(N.y/2+1)+ky.
  * Values equal the reference's Kdiag/Kodiag/K2D_odiag (Boris/DemagKernel.cpp:171-516). */
 int bm_demag_copy_kernels(bm_demag_t d, float* host_out, long long count);

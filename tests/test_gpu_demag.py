@@ -48,8 +48,8 @@ def test_kernels_match_oracle(gpu, n, h):
         K = K.reshape(4, nc, nyh)
         ref = [Kd[..., 0].T, Kd[..., 1].T, Kd[..., 2].T, Ko.T]
     else:
-        K = K.reshape(6, nzh, nc, nyh)
-        ref = [np.transpose(Kd[..., q], (0, 2, 1)) for q in range(3)] + [np.transpose(Ko[..., q], (0, 2, 1)) for q in range(3)]
+        K = K.reshape(6, nc, nyh, nzh)
+        ref = [np.transpose(Kd[..., q], (2, 1, 0)) for q in range(3)] + [np.transpose(Ko[..., q], (2, 1, 0)) for q in range(3)]
     for q, r in enumerate(ref):
         scale = np.abs(r).max() + 1e-300
         err = np.abs(K[q] - r).max() / scale
