@@ -25,7 +25,7 @@ EXPORTS = [
 ]
 
 PROF_KINDS = ["fwd_x_r2c", "y_fft", "z_fft_kernel_ifft", "y_ifft", "inv_x_c2r", "y2d_fft_kernel_ifft",
-              "llg_stage", "rkdp_advance"]
+              "llg_stage", "rkdp_advance", "row_c2r_stage_r2c", "fwd_x_advance"]
 
 
 class FmParams(C.Structure):

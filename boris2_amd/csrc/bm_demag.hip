@@ -22,6 +22,7 @@
 // kernels on the reduced octant (3D: kz fastest; 2D: ky fastest).
 #include "bm_internal.h"
 #include "bm_fft.h"
+#include "bm_xpass.h"
 #include "bm_newell.h"
 
 #include <cstdio>
@@ -36,96 +37,72 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// x-pass LDS tile row: HX+1 spectrum values, and room for the padded FFT exchange
-template <int HX> constexpr int row_pitch() { return lds_len<HX>() > HX + 1 ? lds_len<HX>() : HX + 1; }
-
-template <int HX> struct TileY {
-	static constexpr int RP = kBlock / plan_T<HX>();
-	static constexpr int pw(int v) { int p = 1; while (p * 2 <= v) p *= 2; return p; }
-	static constexpr int cap = pw(8400 / row_pitch<HX>()) < 32 ? pw(8400 / row_pitch<HX>()) : 32;
-	static constexpr int v = RP > cap ? RP : cap;
+// ---------------------------------------------------------------------------------------
+// P1: x-direction real-to-complex FFT of TY rows of one z plane (all three components, so M
+// is read from HBM once), transposed into S through LDS.  With ADV it also performs the
+// RKDP54 advance on the way: sM1 = M, Mout = M + E0 dT/5, and transforms Mout.
+struct FwdArgs {
+	const float* M; float2* S;
+	int ny, nz, Sp, ntiles;
+	const float* E0; float* sM1; float* Mout; float dT5;
 };
 
-// ---------------------------------------------------------------------------------------
-// P1: x-direction real-to-complex FFT of TY rows of one z plane, via a half-length complex
-// FFT (z[k] = x[2k] + i x[2k+1]) and the usual split into the Nc half spectrum, transposed
-// into S through LDS.  One workgroup does all three components of its rows, so M is read
-// from HBM once (each thread keeps the cells it transforms in registers).
-template <int HX, int TY>
-__global__ void __launch_bounds__(kBlock) k_fwd_x(const float* __restrict__ M, float2* __restrict__ S,
-	int ny, int nz, int Sp, int ntiles)
+template <int HX, int TY, bool ADV>
+__global__ void __launch_bounds__(kBlock) k_fwd_x(FwdArgs a)
 {
-	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, NC = HX + 1, RPITCH = row_pitch<HX>();
-	constexpr int NG = TY / RP;
+	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, RPITCH = xrow_pitch<HX>(), NG = TY / RP;
 	__shared__ C2<float> tile[TY * RPITCH];
-	const int t = threadIdx.x % T, rl = threadIdx.x / T;
-	Twiddles<HX, float> tw; tw.init(t);
-	C2<float> wk[E];   // exp(-i pi k / HX), k = t + T e
-#pragma unroll
-	for (int e = 0; e < E; e++) { double s, c; sincospi(-(double)(t + T * e) / HX, &s, &c); wk[e] = mk((float)c, (float)s); }
-
-	const int nwork = nz * ntiles;
+	XThread<HX> th; th.init();
+	const int nwork = a.nz * a.ntiles;
 	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-		const int z = w % nz, y0 = (w / nz) * TY;
+		const int z = w % a.nz, y0 = (w / a.nz) * TY;
 		// cells 2k and 2k+1 of each of this thread's rows: 6 floats = 3 aligned float2
 		float2 m[NG][E / 2][3];
 #pragma unroll
 		for (int g = 0; g < NG; g++) {
-			const int y = y0 + g * RP + rl;
-			const bool ok = y < ny;
-			const float2* row = reinterpret_cast<const float2*>(M + ((size_t)z * ny + (ok ? y : 0)) * (size_t)HX * 3);
+			const int y = y0 + g * RP + th.rl;
+			const bool ok = y < a.ny;
+			const size_t rowoff = ((size_t)z * a.ny + (ok ? y : 0)) * (size_t)HX * 3;
+			const float2* row = reinterpret_cast<const float2*>(a.M + rowoff);
 #pragma unroll
 			for (int e = 0; e < E / 2; e++) {
-				const int k = t + T * e;
+				const int k = th.t + T * e;
 #pragma unroll
 				for (int q = 0; q < 3; q++) m[g][e][q] = ok ? row[3 * k + q] : make_float2(0.f, 0.f);
+				if (ADV && ok) {
+					const float2* er = reinterpret_cast<const float2*>(a.E0 + rowoff);
+					float2* sr = reinterpret_cast<float2*>(a.sM1 + rowoff);
+					float2* mr = reinterpret_cast<float2*>(a.Mout + rowoff);
+#pragma unroll
+					for (int q = 0; q < 3; q++) {
+						const float2 e0 = er[3 * k + q];
+						sr[3 * k + q] = m[g][e][q];
+						m[g][e][q] = make_float2(m[g][e][q].x + e0.x * a.dT5, m[g][e][q].y + e0.y * a.dT5);
+						mr[3 * k + q] = m[g][e][q];
+					}
+				}
 			}
 		}
 #pragma unroll 1
 		for (int c = 0; c < 3; c++) {
 #pragma unroll
 			for (int g = 0; g < NG; g++) {
-				const int yl = g * RP + rl;
 				C2<float> v[E];
 #pragma unroll
 				for (int e = 0; e < E; e++) {
 					if (e < E / 2) {
 						// component c of cell 2k is float c of the 6; of cell 2k+1 float 3 + c
-						const float2 a = m[g][e][0], b = m[g][e][1], d = m[g][e][2];
-						const float x0 = (c == 0) ? a.x : (c == 1) ? a.y : b.x;
-						const float x1 = (c == 0) ? b.y : (c == 1) ? d.x : d.y;
+						const float2 p = m[g][e][0], q = m[g][e][1], r = m[g][e][2];
+						const float x0 = (c == 0) ? p.x : (c == 1) ? p.y : q.x;
+						const float x1 = (c == 0) ? q.y : (c == 1) ? r.x : r.y;
 						v[e] = mk(x0, x1);
 					}
 					else v[e] = mk(0.f, 0.f);
 				}
-				C2<float>* lrow = tile + yl * RPITCH;
-				fft<HX, -1, float>(v, lrow, 1, t, tw);
-				__syncthreads();
-#pragma unroll
-				for (int e = 0; e < E; e++) lrow[t + T * e] = v[e];
-				__syncthreads();
-				C2<float> X[E];
-#pragma unroll
-				for (int e = 0; e < E; e++) {
-					const int k = t + T * e;
-					C2<float> zk = v[e], zm = conj(lrow[(HX - k) % HX]);
-					C2<float> a = zk + zm, b = zk - zm;
-					C2<float> wb = cmul(wk[e], b);              // W^k (Z[k] - conj Z[HX-k])
-					X[e] = mk(0.5f * (a.x + wb.y), 0.5f * (a.y - wb.x));   // a/2 - i wb/2
-				}
-				__syncthreads();
-#pragma unroll
-				for (int e = 0; e < E; e++) lrow[t + T * e] = X[e];
-				if (t == 0) lrow[HX] = mk(v[0].x - v[0].y, 0.f);
+				r2c_row<HX>(th, tile + (g * RP + th.rl) * RPITCH, v);
 			}
 			__syncthreads();
-			for (int idx = threadIdx.x; idx < TY * NC; idx += kBlock) {
-				const int yl = idx % TY, kx = idx / TY, y = y0 + yl;
-				if (y < ny) {
-					C2<float> a = tile[yl * RPITCH + kx];
-					S[(((size_t)c * nz + z) * NC + kx) * (size_t)Sp + y] = make_float2(a.x, a.y);
-				}
-			}
+			tile_to_S<HX, TY>(tile, a.S, c, z, y0, a.ny, a.nz, a.Sp);
 			__syncthreads();
 		}
 	}
@@ -133,67 +110,40 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x(const float* __restrict__ M, f
 
 // ---------------------------------------------------------------------------------------
 // P5: x-direction complex-to-real inverse FFT.  Output modes: 0 planar set, 1 AoS set,
-// 2 AoS add.
+// 2 AoS add; EN also forms per-work-item partial sums of M.H for the energy.
 template <int HX, int MODE, bool EN, int TY>
 __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ S, float* __restrict__ out,
 	int ny, int nz, int Sp, int ntiles, float scale, const float* __restrict__ M, double* __restrict__ epart)
 {
 	__shared__ double ered[EN ? kBlock : 1];
-	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, NC = HX + 1, RPITCH = row_pitch<HX>();
+	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, RPITCH = xrow_pitch<HX>();
 	__shared__ C2<float> tile[TY * RPITCH];
-	const int t = threadIdx.x % T, rl = threadIdx.x / T;
-	Twiddles<HX, float> tw; tw.init(t);
-	C2<float> wk[E];   // exp(-i pi k / HX)
-#pragma unroll
-	for (int e = 0; e < E; e++) { double s, c; sincospi(-(double)(t + T * e) / HX, &s, &c); wk[e] = mk((float)c, (float)s); }
+	XThread<HX> th; th.init();
 	const int nx = HX;
 	const size_t ncell = (size_t)nx * ny * nz;
-
 	// the next work item's tile is loaded into registers while this one is transformed
-	constexpr int NLD = (TY * NC + kBlock - 1) / kBlock;
-	float2 pre[NLD];
-	auto fetch = [&](int wf) {
-		const int c = wf % 3, z = (wf / 3) % nz, y0 = (wf / (3 * nz)) * TY;
-#pragma unroll
-		for (int q = 0; q < NLD; q++) {
-			const int idx = threadIdx.x + kBlock * q;
-			const int yl = idx % TY, kx = idx / TY, y = y0 + yl;
-			pre[q] = (idx < TY * NC && y < ny) ? S[(((size_t)c * nz + z) * NC + kx) * (size_t)Sp + y] : make_float2(0.f, 0.f);
-		}
-	};
+	XTileRegs<HX, TY> pre;
 	const int nwork = 3 * nz * ntiles;
-	if (blockIdx.x < nwork) fetch(blockIdx.x);
+	auto fetch = [&](int wf) { pre.load(S, wf % 3, (wf / 3) % nz, (wf / (3 * nz)) * TY, ny, nz, Sp); };
+	if ((int)blockIdx.x < nwork) fetch(blockIdx.x);
 	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
 		const int c = w % 3, z = (w / 3) % nz, y0 = (w / (3 * nz)) * TY;
 		double esum = 0.0;
 		__syncthreads();
-#pragma unroll
-		for (int q = 0; q < NLD; q++) {
-			const int idx = threadIdx.x + kBlock * q;
-			if (idx < TY * NC) tile[(idx % TY) * RPITCH + idx / TY] = mk(pre[q].x, pre[q].y);
-		}
+		pre.to_tile(tile);
 		__syncthreads();
 		if (w + (int)gridDim.x < nwork) fetch(w + gridDim.x);
 #pragma unroll 1
 		for (int g = 0; g < TY; g += RP) {
-			const int yl = g + rl, y = y0 + yl;
+			const int y = y0 + g + th.rl;
 			const bool ok = y < ny;
-			C2<float>* lrow = tile + yl * RPITCH;
 			C2<float> v[E];
-#pragma unroll
-			for (int e = 0; e < E; e++) {
-				const int k = t + T * e;
-				C2<float> xk = lrow[k], xm = conj(lrow[HX - k]);
-				C2<float> a = xk + xm, b = xk - xm;
-				C2<float> wb = cmulc(b, wk[e]);            // W^-k (X[k] - conj X[HX-k])
-				v[e] = mk(a.x - wb.y, a.y + wb.x);          // a + i wb
-			}
-			fft<HX, 1, float>(v, lrow, 1, t, tw);
+			c2r_row<HX>(th, tile + (g + th.rl) * RPITCH, v);
 			if (ok) {
 				const size_t rowc = ((size_t)z * ny + y) * nx;
 #pragma unroll
 				for (int e = 0; e < E / 2; e++) {
-					const int n2 = 2 * (t + T * e);
+					const int n2 = 2 * (th.t + T * e);
 					float a = v[e].x * scale, b = v[e].y * scale;
 					if (MODE == 0) {
 						*reinterpret_cast<float2*>(out + (size_t)c * ncell + rowc + n2) = make_float2(a, b);
@@ -215,7 +165,6 @@ __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ S, 
 			for (int o = kBlock / 2; o > 0; o >>= 1) { if (threadIdx.x < o) ered[threadIdx.x] += ered[threadIdx.x + o]; __syncthreads(); }
 			if (threadIdx.x == 0) epart[w] = ered[0];
 		}
-		__syncthreads();
 	}
 }
 
@@ -522,7 +471,6 @@ __global__ void __launch_bounds__(kBlock) k_line_fft(const double* __restrict__ 
 
 // ---------------------------------------------------------------------------------------
 // Size dispatch.
-#define BM_FOR_X_SIZES(M) M(2) M(4) M(6) M(8) M(12) M(16) M(24) M(32) M(48) M(64) M(96) M(128) M(192) M(256) M(384) M(512) M(768) M(1024) M(2048)
 #define BM_FOR_Z_SIZES(M) M(4) M(8) M(16) M(32) M(64) M(128)
 
 bool x_supported(int n) { switch (n) {
@@ -554,13 +502,18 @@ int xtile_factor(bool fwd)
 	return v;
 }
 
-int launch_fwd_x(int HX, const float* M, float2* S, int ny, int nz, int Sp, int ncu, hipStream_t st)
+int launch_fwd_x(int HX, const float* M, float2* S, int ny, int nz, int Sp, int ncu, hipStream_t st,
+	const float* E0 = nullptr, float* sM1 = nullptr, float* Mout = nullptr, float dT5 = 0.f)
 {
 	const int f = xtile_factor(true);
+	FwdArgs a{ M, S, ny, nz, Sp, 0, E0, sM1, Mout, dT5 };
+	const bool adv = E0 != nullptr;
 	switch (HX) {
-#define C_(V_) case V_: { int TY = TileY<V_>::v * f, nt = (ny + TY - 1) / TY; dim3 g(grid_for(nz * nt, ncu, 2)); \
-		if (f == 1) hipLaunchKernelGGL((k_fwd_x<V_, TileY<V_>::v>), g, dim3(kBlock), 0, st, M, S, ny, nz, Sp, nt); \
-		else hipLaunchKernelGGL((k_fwd_x<V_, 2 * TileY<V_>::v>), g, dim3(kBlock), 0, st, M, S, ny, nz, Sp, nt); return BM_OK; }
+#define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; int TY = TY1 * f; a.ntiles = (ny + TY - 1) / TY; dim3 g(grid_for(nz * a.ntiles, ncu, 2)); \
+		if (f == 1 && !adv) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false>), g, dim3(kBlock), 0, st, a); \
+		else if (f == 1) hipLaunchKernelGGL((k_fwd_x<V_, TY1, true>), g, dim3(kBlock), 0, st, a); \
+		else if (!adv) hipLaunchKernelGGL((k_fwd_x<V_, 2 * TY1, false>), g, dim3(kBlock), 0, st, a); \
+		else hipLaunchKernelGGL((k_fwd_x<V_, 2 * TY1, true>), g, dim3(kBlock), 0, st, a); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -572,7 +525,7 @@ int launch_inv_x(int HX, int mode, const float2* S, float* out, int ny, int nz, 
 {
 	const int f = (mode == 0) ? xtile_factor(false) : 1;
 	switch (HX) {
-#define C_(V_) case V_: { constexpr int TY1 = TileY<V_>::v; int TY = TY1 * f, nt = (ny + TY - 1) / TY; dim3 g(grid_for(3 * nz * nt, ncu, 2)); \
+#define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; int TY = TY1 * f, nt = (ny + TY - 1) / TY; dim3 g(grid_for(3 * nz * nt, ncu, 2)); \
 		if (mode == 0 && f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); \
 		else if (mode == 0) hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); \
 		else if (mode == 1 && !epart) hipLaunchKernelGGL((k_inv_x<V_, 1, false, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); \
@@ -588,7 +541,7 @@ int launch_inv_x(int HX, int mode, const float2* S, float* out, int ny, int nz, 
 int inv_x_nwork(int HX, int ny, int nz)
 {
 	switch (HX) {
-#define C_(V_) case V_: { int TY = TileY<V_>::v; return 3 * nz * ((ny + TY - 1) / TY); }
+#define C_(V_) case V_: { int TY = XTile<V_>::v; return 3 * nz * ((ny + TY - 1) / TY); }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -724,14 +677,30 @@ int compute_kernels(bm_demag_s* d, hipStream_t st)
 // C-ABI
 // =======================================================================================
 
-// The convolution passes up to (not including) the final x C2R pass.
-static int forward_and_multiply(bm_demag_s* d, const float* M, hipStream_t st)
+// P1: the x-direction forward pass of M into the spectra.
+int bm::demag_fwd_x(bm_demag_s* d, const float* M, hipStream_t st)
 {
-	const int HX = d->nx, nlines = 3 * d->nz * d->NC;
 	prof_mark_begin(PK_FWD_X, st);
-	int rc = launch_fwd_x(HX, M, d->S, d->ny, d->nz, d->Sp, d->ncu, st);
+	int rc = launch_fwd_x(d->nx, M, d->S, d->ny, d->nz, d->Sp, d->ncu, st);
 	prof_mark_end(PK_FWD_X, st);
-	if (rc) return rc;
+	return rc;
+}
+
+// P1 fused with the RKDP54 advance: sM1 = M, Mout = M + E0 dT/5, spectra of Mout.
+int bm::demag_fwd_x_advance(bm_demag_s* d, const float* M, const float* E0, float* sM1, float* Mout, float dT5, hipStream_t st)
+{
+	prof_mark_begin(PK_FWD_ADV, st);
+	int rc = launch_fwd_x(d->nx, M, d->S, d->ny, d->nz, d->Sp, d->ncu, st, E0, sM1, Mout, dT5);
+	prof_mark_end(PK_FWD_ADV, st);
+	return rc;
+}
+
+// The passes between the two x passes: y FFT, z FFT . K . z IFFT, y IFFT (3D) or the fused
+// y FFT . K . y IFFT (2D).
+int bm::demag_mid(bm_demag_s* d, hipStream_t st)
+{
+	const int nlines = 3 * d->nz * d->NC;
+	int rc;
 	if (d->is2d) {
 		prof_mark_begin(PK_Y2D, st);
 		rc = launch_y2d(d->Ny, d->S, d->K, d->NC, d->NYH, d->Sp, d->ncu, st);
@@ -750,6 +719,13 @@ static int forward_and_multiply(bm_demag_s* d, const float* M, hipStream_t st)
 	rc = launch_y(d->Ny, 1, d->S, nlines, d->Sp, d->ncu, st);
 	prof_mark_end(PK_Y_INV, st);
 	return rc;
+}
+
+// The convolution passes up to (not including) the final x C2R pass.
+static int forward_and_multiply(bm_demag_s* d, const float* M, hipStream_t st)
+{
+	int rc = bm::demag_fwd_x(d, M, st);
+	return rc ? rc : bm::demag_mid(d, st);
 }
 
 int bm::demag_planar(bm_demag_s* d, const float* M, float* H3n, hipStream_t st)

@@ -4,7 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <string>
 
-#include "../../include/boris_mi355x.h"
+#include "boris_mi355x.h"
 
 namespace bm {
 
@@ -18,7 +18,7 @@ int device_cu_count();
 
 // Launch timing: when enabled (bm_prof_begin), every kernel launch of the hot path is
 // bracketed by HIP events on its own stream and tagged with its kind.
-enum ProfKind { PK_FWD_X = 0, PK_Y_FWD, PK_Z, PK_Y_INV, PK_INV_X, PK_Y2D, PK_STAGE, PK_ADVANCE, PK_COUNT };
+enum ProfKind { PK_FWD_X = 0, PK_Y_FWD, PK_Z, PK_Y_INV, PK_INV_X, PK_Y2D, PK_STAGE, PK_ADVANCE, PK_ROW, PK_FWD_ADV, PK_COUNT };
 void prof_mark_begin(int kind, hipStream_t st);
 void prof_mark_end(int kind, hipStream_t st);
 
@@ -42,4 +42,8 @@ struct bm_demag_s {
 namespace bm {
 // H_demag(M) into planar output H3n (3 planes of n floats)
 int demag_planar(bm_demag_s* d, const float* M, float* H3n, hipStream_t st);
+// the convolution's passes, for the integrator's fused evaluation
+int demag_fwd_x(bm_demag_s* d, const float* M, hipStream_t st);
+int demag_fwd_x_advance(bm_demag_s* d, const float* M, const float* E0, float* sM1, float* Mout, float dT5, hipStream_t st);
+int demag_mid(bm_demag_s* d, hipStream_t st);
 }

@@ -1,21 +1,18 @@
-// LLG integration of one ferromagnetic mesh on MI355X: fused field + RK stage kernels and
-// the host-side step-control state machine.
+// LLG integration of one ferromagnetic mesh on MI355X: stage kernels and the host-side
+// step-control state machine.
 //
-// Each stage kernel evaluates, per cell, the effective field in the reference's module
-// order — Zeeman sets it (Boris/ZeemanCUDA.cu:31), demag adds (the planar H_demag written
-// by the convolution), exchange adds (6-neighbour Laplacian with free-boundary one-sided
-// differences, Boris/ExchangeCUDA.cu:15 + BorisLib/VEC_VC_Del.h:10-130), uniaxial
-// anisotropy adds (Boris/AnisotropyUniCUDA.cu:13) — then the LLG right-hand side
-// (Boris/DiffEqFM_Equations.cpp:12-24) and the stage update of the chosen method:
-//   RKDP54  Boris/DiffEqFM_EvalsCUDA_RKDP54.cu:15-330  (FSAL lte at stage 0)
-//   RK4     Boris/DiffEqFM_EvalsLLGCUDA.cu:416-641
-//   TEuler  Boris/DiffEqFM_EvalsLLGCUDA.cu:121-255 (Heun)
-//   AHeun   Boris/DiffEqFM_EvalsLLGCUDA.cu:299-363
+// With demag enabled, a field evaluation is the convolution's middle passes (y / z FFTs and
+// the kernel product) followed by one fused row kernel — x-direction inverse FFT of the
+// demag field, the per-cell field + LLG + stage update (bm_stage.h), and the x-direction
+// forward FFT of the stage's output magnetisation, which is the next evaluation's input.
+// Without demag a plain per-cell stage kernel runs.
 // The host drives stages exactly as ODECommon_Base::IterateCUDA does
 // (Boris/DiffEq_CommonBase_IterateCUDA.cpp:13-936): one lte read-back per adaptive step,
 // SetAdaptiveTimeStep in FP64 on the host (Boris/DiffEq_CommonBase_Iterate.cpp:15-59),
 // restore + re-prime on rejection.
 #include "bm_internal.h"
+#include "bm_stage.h"
+#include "bm_row.h"
 
 #include <cmath>
 #include <cstring>
@@ -24,64 +21,7 @@
 
 namespace {
 
-constexpr int kBS = 256;
-constexpr double kMU0 = 1.256637061435916e-6;    // BorisLib/Funcs_Math_base.h:9
-constexpr double kGAMMA = 2.212761569e5;         // BorisLib/Funcs_Math_base.h:16
-
-struct F3 { float x, y, z; };
-__device__ __forceinline__ F3 ld3(const float* p, size_t i) { return F3{ p[3 * i], p[3 * i + 1], p[3 * i + 2] }; }
-__device__ __forceinline__ void st3(float* p, size_t i, F3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
-__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return F3{ a.x + b.x, a.y + b.y, a.z + b.z }; }
-__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return F3{ a.x - b.x, a.y - b.y, a.z - b.z }; }
-__device__ __forceinline__ F3 operator*(float s, F3 a) { return F3{ s * a.x, s * a.y, s * a.z }; }
-__device__ __forceinline__ F3 cross(F3 a, F3 b) { return F3{ a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x }; }
-__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float norm(F3 a) { return sqrtf(dot(a, a)); }
-
-struct StageArgs {
-	const float* Min;       // magnetisation this stage evaluates (neighbours read)
-	float* Mout;            // magnetisation of the next stage (never the same array)
-	const float* Hd;        // planar demag field (3 planes of n) or null
-	float* sM1;
-	float* E[6];
-	const float* Ms_map;
-	float* Heff_out;        // optional AoS copy of the effective field
-	unsigned* lte;          // max lte as float bits
-	int nx, ny, nz;
-	float hx2, hy2, hz2;
-	float Ms, A, alpha, K1, K2, grel;
-	float ea[3], Ha[3];
-	float dT;
-	int modules;
-	int map;
-};
-
-enum { M_RKDP = 0, M_RK4 = 1, M_TEULER = 2, M_AHEUN = 3 };
-
-// Block-wide max of a non-negative float, folded into *dst with an integer atomicMax on
-// its bit pattern (order-independent, so deterministic).
-__device__ __forceinline__ void block_max_to(float v, unsigned* dst)
-{
-	__shared__ float red[kBS / 64];
-	for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-	if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		float m = red[0];
-		for (int q = 1; q < kBS / 64; q++) m = fmaxf(m, red[q]);
-		atomicMax(dst, __float_as_uint(m));
-	}
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).  Chunk c of
-// kBS consecutive cells is given to block b with c = (b mod 8) * ceil(nb/8) + b/8, so every XCD
-// streams one contiguous eighth of the mesh and the exchange stencil's +-y rows (a few chunks
-// away) are read from the same L2.
-__device__ __forceinline__ long long xcd_chunk(int b, int nb)
-{
-	const int per = (nb + 7) / 8;
-	return (long long)(b % 8) * per + b / 8;
-}
+using namespace bm;
 
 template <int METHOD, int STEP>
 __global__ void __launch_bounds__(kBS) k_stage(StageArgs a)
@@ -100,101 +40,10 @@ __global__ void __launch_bounds__(kBS) k_stage(StageArgs a)
 		const size_t idx = (size_t)chunk * kBS + threadIdx.x;
 		if (idx >= n) continue;
 		const int i = (int)(idx % a.nx), j = (int)((idx / a.nx) % a.ny), k = (int)(idx / ((size_t)a.nx * a.ny));
-		const F3 m = ld3(a.Min, idx);
-		const float Ms = a.Ms_map ? a.Ms_map[idx] : a.Ms;
-		// --- effective field, module order Zeeman, Demag, Exchange, Anisotropy
-		F3 H = (a.modules & BM_MOD_ZEEMAN) ? F3{ a.Ha[0], a.Ha[1], a.Ha[2] } : F3{ 0.f, 0.f, 0.f };
-		if (a.modules & BM_MOD_DEMAG) H = H + F3{ a.Hd[idx], a.Hd[n + idx], a.Hd[2 * n + idx] };
-		if (a.modules & BM_MOD_EXCHANGE) {
-			F3 lx{ 0, 0, 0 }, ly{ 0, 0, 0 }, lz{ 0, 0, 0 };
-			const size_t sy = a.nx, sz = (size_t)a.nx * a.ny;
-			if (a.nx > 1) {
-				if (i > 0 && i < a.nx - 1) lx = ld3(a.Min, idx + 1) + ld3(a.Min, idx - 1) - 2.f * m;
-				else if (i == 0) lx = ld3(a.Min, idx + 1) - m;
-				else lx = ld3(a.Min, idx - 1) - m;
-			}
-			if (a.ny > 1) {
-				if (j > 0 && j < a.ny - 1) ly = ld3(a.Min, idx + sy) + ld3(a.Min, idx - sy) - 2.f * m;
-				else if (j == 0) ly = ld3(a.Min, idx + sy) - m;
-				else ly = ld3(a.Min, idx - sy) - m;
-			}
-			if (a.nz > 1) {
-				if (k > 0 && k < a.nz - 1) lz = ld3(a.Min, idx + sz) + ld3(a.Min, idx - sz) - 2.f * m;
-				else if (k == 0) lz = ld3(a.Min, idx + sz) - m;
-				else lz = ld3(a.Min, idx - sz) - m;
-			}
-			F3 lap{ lx.x / a.hx2 + ly.x / a.hy2 + lz.x / a.hz2, lx.y / a.hx2 + ly.y / a.hy2 + lz.y / a.hz2, lx.z / a.hx2 + ly.z / a.hy2 + lz.z / a.hz2 };
-			const float pre = (2.f * a.A) / ((float)kMU0 * Ms * Ms);
-			H = H + pre * lap;
-		}
-		if (a.modules & BM_MOD_ANIS_UNI) {
-			const F3 e{ a.ea[0], a.ea[1], a.ea[2] };
-			const float d = dot(m, e) / Ms;
-			H = H + ((2.f / ((float)kMU0 * Ms)) * d * (a.K1 + 2.f * a.K2 * (1.f - d * d))) * e;
-		}
-		if (a.Heff_out) st3(a.Heff_out, idx, H);
-		// --- LLG
-		const F3 mxh = cross(m, H);
-		const F3 mu{ m.x / Ms, m.y / Ms, m.z / Ms };
-		const float pre = (float)(-kGAMMA) * a.grel / (1.f + a.alpha * a.alpha);
-		const F3 rhs = pre * (mxh + a.alpha * cross(mu, mxh));
-		const float dT = a.dT;
-		// --- stage update
-		if (METHOD == M_RKDP) {
-			if (STEP == 0) {
-				const F3 pred = ld3(a.sM1, idx) + dT * ((5179.f / 57600.f) * ld3(a.E[0], idx) + (7571.f / 16695.f) * ld3(a.E[2], idx)
-					+ (393.f / 640.f) * ld3(a.E[3], idx) - (92097.f / 339200.f) * ld3(a.E[4], idx) + (187.f / 2100.f) * ld3(a.E[5], idx)
-					+ (1.f / 40.f) * rhs);
-				lte_loc = fmaxf(lte_loc, norm(m - pred) / norm(m));
-				st3(a.E[0], idx, rhs);
-			}
-			else if (STEP == 1) {
-				st3(a.E[1], idx, rhs);
-				st3(a.Mout, idx, ld3(a.sM1, idx) + dT * ((3.f / 40.f) * ld3(a.E[0], idx) + (9.f / 40.f) * rhs));
-			}
-			else if (STEP == 2) {
-				st3(a.E[2], idx, rhs);
-				st3(a.Mout, idx, ld3(a.sM1, idx) + dT * ((44.f / 45.f) * ld3(a.E[0], idx) - (56.f / 15.f) * ld3(a.E[1], idx) + (32.f / 9.f) * rhs));
-			}
-			else if (STEP == 3) {
-				st3(a.E[3], idx, rhs);
-				st3(a.Mout, idx, ld3(a.sM1, idx) + dT * ((19372.f / 6561.f) * ld3(a.E[0], idx) - (25360.f / 2187.f) * ld3(a.E[1], idx)
-					+ (64448.f / 6561.f) * ld3(a.E[2], idx) - (212.f / 729.f) * rhs));
-			}
-			else if (STEP == 4) {
-				st3(a.E[4], idx, rhs);
-				st3(a.Mout, idx, ld3(a.sM1, idx) + dT * ((9017.f / 3168.f) * ld3(a.E[0], idx) - (355.f / 33.f) * ld3(a.E[1], idx)
-					+ (46732.f / 5247.f) * ld3(a.E[2], idx) + (49.f / 176.f) * ld3(a.E[3], idx) - (5103.f / 18656.f) * rhs));
-			}
-			else {
-				st3(a.E[5], idx, rhs);
-				F3 mn = ld3(a.sM1, idx) + dT * ((35.f / 384.f) * ld3(a.E[0], idx) + (500.f / 1113.f) * ld3(a.E[2], idx)
-					+ (125.f / 192.f) * ld3(a.E[3], idx) - (2187.f / 6784.f) * ld3(a.E[4], idx) + (11.f / 84.f) * rhs);
-				mn = (Ms / norm(mn)) * mn;
-				st3(a.Mout, idx, mn);
-			}
-		}
-		else if (METHOD == M_RK4) {
-			if (STEP == 0) { st3(a.sM1, idx, m); st3(a.E[0], idx, rhs); st3(a.Mout, idx, m + (dT / 2) * rhs); }
-			else if (STEP == 1) { st3(a.E[1], idx, rhs); st3(a.Mout, idx, ld3(a.sM1, idx) + (dT / 2) * rhs); }
-			else if (STEP == 2) { st3(a.E[2], idx, rhs); st3(a.Mout, idx, ld3(a.sM1, idx) + dT * rhs); }
-			else {
-				F3 mn = ld3(a.sM1, idx) + (dT / 6) * (ld3(a.E[0], idx) + 2.f * ld3(a.E[1], idx) + 2.f * ld3(a.E[2], idx) + rhs);
-				mn = (Ms / norm(mn)) * mn;
-				st3(a.Mout, idx, mn);
-			}
-		}
-		else {
-			if (STEP == 0) { st3(a.sM1, idx, m); st3(a.Mout, idx, m + dT * rhs); }
-			else {
-				F3 mn = 0.5f * (ld3(a.sM1, idx) + m + dT * rhs);
-				mn = (Ms / norm(mn)) * mn;
-				st3(a.Mout, idx, mn);
-				if (METHOD == M_AHEUN) lte_loc = fmaxf(lte_loc, norm(mn - m) / norm(mn));
-			}
-		}
+		const F3 Hd = (a.modules & BM_MOD_DEMAG) ? F3{ a.Hd[idx], a.Hd[n + idx], a.Hd[2 * n + idx] } : F3{ 0.f, 0.f, 0.f };
+		stage_cell<METHOD, STEP>(a, idx, i, j, k, Hd, lte_loc);
 	}
-	if ((METHOD == M_RKDP && STEP == 0) || (METHOD == M_AHEUN && STEP == 1)) block_max_to(lte_loc, a.lte);
+	if (stage_has_lte<METHOD, STEP>()) block_max_to(lte_loc, a.lte);
 }
 
 // RKDP54 advance after the step size is fixed: sM1 = M; M1 = M + E0 dT/5 (into the
@@ -243,6 +92,8 @@ struct bm_ode_s {
 	float* sM1 = nullptr;
 	float* E[6] = { nullptr };
 	float* Heff_keep = nullptr;   // internal AoS copy of the last effective field
+	bool use_row = true;          // demag evaluations through the fused row kernel
+	bool S_valid = false;         // the demag spectra hold the x-transform of Mcur
 	bool keep_heff = false;
 	unsigned* d_lte = nullptr;
 	unsigned* h_lte = nullptr;    // pinned
@@ -297,7 +148,7 @@ StageArgs make_args(bm_ode_s* o, float dT)
 	for (int q = 0; q < 6; q++) a.E[q] = o->E[q];
 	a.Ms_map = o->Ms_map; a.Heff_out = o->keep_heff ? o->Heff_keep : nullptr; a.lte = o->d_lte;
 	a.nx = o->nx; a.ny = o->ny; a.nz = o->nz;
-	a.hx2 = (float)o->h[0] * (float)o->h[0]; a.hy2 = (float)o->h[1] * (float)o->h[1]; a.hz2 = (float)o->h[2] * (float)o->h[2];
+	a.ihx2 = (float)(1.0 / (o->h[0] * o->h[0])); a.ihy2 = (float)(1.0 / (o->h[1] * o->h[1])); a.ihz2 = (float)(1.0 / (o->h[2] * o->h[2]));
 	a.Ms = (float)o->p.Ms; a.A = (float)o->p.A; a.alpha = (float)o->p.alpha; a.K1 = (float)o->p.K1; a.K2 = (float)o->p.K2;
 	a.grel = (float)o->p.grel;
 	for (int c = 0; c < 3; c++) a.ea[c] = (float)o->p.ea[c];
@@ -337,12 +188,37 @@ int launch_stage(bm_ode_s* o, int step, float dT, hipStream_t st)
 	return BM_OK;
 }
 
-// Field update: the demag part (planar H_demag); the rest is fused into the stage kernel.
-int update_demag(bm_ode_s* o, hipStream_t st)
+// Fused evaluation: [x-R2C of Mcur unless the spectra already hold it] + y / z passes with
+// the kernel product + the row kernel (x-C2R of H_demag, field + stage, x-R2C of the stage's
+// output magnetisation, which the next evaluation starts from).
+int eval_row(bm_ode_s* o, int step, float dT, hipStream_t st)
+{
+	bm_demag_s* d = o->demag;
+	int rc;
+	if (!o->S_valid && (rc = bm::demag_fwd_x(d, o->Mcur, st))) return rc;
+	if ((rc = bm::demag_mid(d, st))) return rc;
+	StageArgs a = make_args(o, dT);
+	const float scale = (float)(1.0 / ((double)d->Nx * d->Ny * d->Nz));
+	bm::prof_mark_begin(bm::PK_ROW, st);
+	if (o->mcode == M_RKDP) rc = bm::launch_row_rkdp(d, a, step, scale, st);
+	else if (o->mcode == M_RK4) rc = bm::launch_row_rk4(d, a, step, scale, st);
+	else rc = bm::launch_row_heun(d, a, o->mcode, step, scale, st);
+	bm::prof_mark_end(bm::PK_ROW, st);
+	if (rc) return rc;
+	const bool writes_m = !(o->mcode == M_RKDP && step == 0);
+	o->S_valid = writes_m;
+	if (writes_m) o->Mcur = (o->Mcur == o->M) ? o->Mb : o->M;
+	return BM_OK;
+}
+
+// One field evaluation + stage `step` (the effective field is that of Mcur at stagetime).
+int evaluate(bm_ode_s* o, int step, float dT, hipStream_t st)
 {
 	o->evaluations++;
-	if (!(o->p.modules & BM_MOD_DEMAG)) return BM_OK;
-	return bm::demag_planar(o->demag, o->Mcur, o->Hd, st);
+	const bool demag = (o->p.modules & BM_MOD_DEMAG) != 0;
+	if (demag && o->use_row) return eval_row(o, step, dT, st);
+	if (demag) { int rc = bm::demag_planar(o->demag, o->Mcur, o->Hd, st); if (rc) return rc; }
+	return launch_stage(o, step, dT, st);
 }
 
 // ODECommon_Base::SetAdaptiveTimeStep (Boris/DiffEq_CommonBase_Iterate.cpp:15-59).
@@ -384,6 +260,7 @@ int restore(bm_ode_s* o, hipStream_t st)
 {
 	BM_HIP(hipMemcpyAsync(o->M, o->sM1, o->n * 3 * sizeof(float), hipMemcpyDeviceToDevice, st));
 	o->Mcur = o->M;
+	o->S_valid = false;
 	return BM_OK;
 }
 
@@ -395,14 +272,13 @@ void end_step(bm_ode_s* o)
 // One field update + one stage: ODECommon_Base::IterateCUDA.
 int iterate_stage(bm_ode_s* o, hipStream_t st)
 {
-	int rc = update_demag(o, st);
-	if (rc) return rc;
+	int rc;
 	o->dT_last = o->dT;
 	switch (o->mcode) {
 	case M_RKDP: {
 		if (o->evalStep == 0) {
 			BM_HIP(hipMemsetAsync(o->d_lte, 0, sizeof(unsigned), st));
-			if ((rc = launch_stage(o, 0, (float)o->dT, st))) return rc;
+			if ((rc = evaluate(o, 0, (float)o->dT, st))) return rc;
 			o->available = false;
 			if (o->primed) {
 				o->dT_last = o->dT;
@@ -414,36 +290,42 @@ int iterate_stage(bm_ode_s* o, hipStream_t st)
 				}
 			}
 			else o->primed = true;
-			const size_t n3 = o->n * 3;
-			bm::prof_mark_begin(bm::PK_ADVANCE, st);
-			hipLaunchKernelGGL(k_rkdp_advance, dim3(stage_grid(o)), dim3(kBS), 0, st, o->Mcur, o->Mb, o->sM1, o->E[0], n3, (float)o->dT / 5);
+			if ((o->p.modules & BM_MOD_DEMAG) && o->use_row) {
+				// the advance rides on the x-R2C pass of the next evaluation's input
+				if ((rc = bm::demag_fwd_x_advance(o->demag, o->Mcur, o->E[0], o->sM1, o->Mb, (float)o->dT / 5, st))) return rc;
+				o->S_valid = true;
+			}
+			else {
+				bm::prof_mark_begin(bm::PK_ADVANCE, st);
+				hipLaunchKernelGGL(k_rkdp_advance, dim3(stage_grid(o)), dim3(kBS), 0, st, o->Mcur, o->Mb, o->sM1, o->E[0], o->n * 3, (float)o->dT / 5);
+				bm::prof_mark_end(bm::PK_ADVANCE, st);
+			}
 			o->Mcur = o->Mb;
-			bm::prof_mark_end(bm::PK_ADVANCE, st);
 			o->evalStep++;
 		}
 		else if (o->evalStep < 5) {
-			if ((rc = launch_stage(o, o->evalStep, (float)o->dT, st))) return rc;
+			if ((rc = evaluate(o, o->evalStep, (float)o->dT, st))) return rc;
 			o->evalStep++;
 		}
 		else {
-			if ((rc = launch_stage(o, 5, (float)o->dT, st))) return rc;
+			if ((rc = evaluate(o, 5, (float)o->dT, st))) return rc;
 			end_step(o);
 		}
 	} break;
 	case M_RK4: {
-		if ((rc = launch_stage(o, o->evalStep, (float)o->dT, st))) return rc;
+		if ((rc = evaluate(o, o->evalStep, (float)o->dT, st))) return rc;
 		if (o->evalStep < 3) { o->evalStep++; o->available = false; }
 		else end_step(o);
 	} break;
 	case M_TEULER:
 	case M_AHEUN: {
 		if (o->evalStep == 0) {
-			if ((rc = launch_stage(o, 0, (float)o->dT, st))) return rc;
+			if ((rc = evaluate(o, 0, (float)o->dT, st))) return rc;
 			o->evalStep = 1; o->available = false;
 		}
 		else {
 			if (o->mcode == M_AHEUN) BM_HIP(hipMemsetAsync(o->d_lte, 0, sizeof(unsigned), st));
-			if ((rc = launch_stage(o, 1, (float)o->dT, st))) return rc;
+			if ((rc = evaluate(o, 1, (float)o->dT, st))) return rc;
 			end_step(o);
 			if (o->mcode == M_AHEUN) {
 				o->dT_last = o->dT;
@@ -492,6 +374,9 @@ extern "C" int bm_ode_create(int method, int nx, int ny, int nz, double hx, doub
 	o->method = method; o->nx = nx; o->ny = ny; o->nz = nz; o->n = (size_t)nx * ny * nz;
 	o->h[0] = hx; o->h[1] = hy; o->h[2] = hz; o->p = *p; o->demag = demag; o->M = M;
 	o->ncu = bm::device_cu_count();
+	// BM_ROW_KERNEL=0 selects the unfused evaluation (planar H_demag + stage kernel), kept
+	// as a cross-check of the fused row kernel
+	if (const char* e = getenv("BM_ROW_KERNEL")) o->use_row = atoi(e) != 0;
 	int nE = 0;
 	// method defaults: Boris/DiffEq_CommonBase_Control.cpp:10-140 with Boris/DiffEq_Defs.h
 	switch (method) {
@@ -506,7 +391,7 @@ extern "C" int bm_ode_create(int method, int nx, int ny, int nz, double hx, doub
 	bool ok = hipMalloc(&o->sM1, b3) == hipSuccess && hipMalloc(&o->Mb, b3) == hipSuccess;
 	o->Mcur = o->M;
 	for (int q = 0; q < nE && ok; q++) ok = hipMalloc(&o->E[q], b3) == hipSuccess && hipMemset(o->E[q], 0, b3) == hipSuccess;
-	if (ok && (p->modules & BM_MOD_DEMAG)) ok = hipMalloc(&o->Hd, b3) == hipSuccess;
+	if (ok && (p->modules & BM_MOD_DEMAG) && !o->use_row) ok = hipMalloc(&o->Hd, b3) == hipSuccess;
 	if (ok) ok = hipMalloc(&o->Heff_keep, b3) == hipSuccess;
 	if (ok) ok = hipMalloc(&o->d_lte, sizeof(unsigned)) == hipSuccess;
 	if (ok) ok = hipHostMalloc(&o->h_lte, sizeof(unsigned)) == hipSuccess;
@@ -538,8 +423,14 @@ extern "C" int bm_ode_set_ms_map(bm_ode_t o, const float* Ms_map)
 extern "C" int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p)
 {
 	if (!o || !p) return fail(BM_ERR_ARG, "null argument");
-	if ((p->modules & BM_MOD_DEMAG) && !o->Hd) return fail(BM_ERR_STATE, "demag was not enabled at creation");
+	if ((p->modules & BM_MOD_DEMAG) && (!o->demag || o->demag->nx != o->nx || o->demag->ny != o->ny || o->demag->nz != o->nz))
+		return fail(BM_ERR_STATE, "bm_ode_set_params: demag needs a demag handle of this mesh size at creation");
+	if ((p->modules & BM_MOD_DEMAG) && !o->use_row && !o->Hd) {
+		const size_t b3 = o->n * 3 * sizeof(float);
+		if (hipMalloc(&o->Hd, b3) != hipSuccess) return fail(BM_ERR_NOMEM, "bm_ode_set_params: demag field buffer");
+	}
 	o->p = *p;
+	o->S_valid = false;
 	return BM_OK;
 }
 
@@ -563,12 +454,14 @@ extern "C" int bm_ode_set_astep(bm_ode_t o, double err, double inc, double dtmin
 extern "C" int bm_ode_advance_time(bm_ode_t o, void* stream)
 {
 	if (!o) return fail(BM_ERR_ARG, "null handle");
+	o->S_valid = false;    // the caller may have written M since the last call
 	return advance_time(o, (hipStream_t)stream);
 }
 
 extern "C" int bm_ode_run_to_iteration(bm_ode_t o, long long iteration, void* stream)
 {
 	if (!o) return fail(BM_ERR_ARG, "null handle");
+	o->S_valid = false;
 	while (o->iteration < iteration) { int rc = advance_time(o, (hipStream_t)stream); if (rc) return rc; }
 	return BM_OK;
 }
@@ -576,6 +469,7 @@ extern "C" int bm_ode_run_to_iteration(bm_ode_t o, long long iteration, void* st
 extern "C" int bm_ode_iterate_stage(bm_ode_t o, void* stream)
 {
 	if (!o) return fail(BM_ERR_ARG, "null handle");
+	o->S_valid = false;
 	return iterate_stage(o, (hipStream_t)stream);
 }
 

@@ -1,0 +1,266 @@
+// Per-cell effective field + LLG + explicit-stage update, shared by the stand-alone stage
+// kernel (demag field read from a planar buffer, one cell per thread) and the fused row
+// kernel (demag field produced in registers by the x-direction inverse FFT, two adjacent
+// cells of a row per call).
+//
+// Effective field in the reference's module order — Zeeman sets it (Boris/ZeemanCUDA.cu:31),
+// demag adds, exchange adds (6-neighbour Laplacian with free-boundary one-sided differences,
+// Boris/ExchangeCUDA.cu:15 + BorisLib/VEC_VC_Del.h:10-130), uniaxial anisotropy adds
+// (Boris/AnisotropyUniCUDA.cu:13) — then the LLG right-hand side
+// (Boris/DiffEqFM_Equations.cpp:12-24) and the stage update of the chosen method:
+//   RKDP54  Boris/DiffEqFM_EvalsCUDA_RKDP54.cu:15-330  (FSAL lte at stage 0)
+//   RK4     Boris/DiffEqFM_EvalsLLGCUDA.cu:416-641
+//   TEuler  Boris/DiffEqFM_EvalsLLGCUDA.cu:121-255 (Heun)
+//   AHeun   Boris/DiffEqFM_EvalsLLGCUDA.cu:299-363
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "boris_mi355x.h"
+
+namespace bm {
+
+constexpr int kBS = 256;
+constexpr double kMU0 = 1.256637061435916e-6;    // BorisLib/Funcs_Math_base.h:9
+constexpr double kGAMMA = 2.212761569e5;         // BorisLib/Funcs_Math_base.h:16
+
+struct F3 { float x, y, z; };
+__device__ __forceinline__ F3 ld3(const float* p, size_t i) { return F3{ p[3 * i], p[3 * i + 1], p[3 * i + 2] }; }
+__device__ __forceinline__ void st3(float* p, size_t i, F3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return F3{ a.x + b.x, a.y + b.y, a.z + b.z }; }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return F3{ a.x - b.x, a.y - b.y, a.z - b.z }; }
+__device__ __forceinline__ F3 operator*(float s, F3 a) { return F3{ s * a.x, s * a.y, s * a.z }; }
+__device__ __forceinline__ F3 cross(F3 a, F3 b) { return F3{ a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x }; }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float norm(F3 a) { return sqrtf(dot(a, a)); }
+
+// Cells 2k and 2k+1 of a row: their 6 floats are contiguous in every AoS array, so they
+// move as three 8-byte accesses (idx even, arrays 8-byte aligned).
+struct P6 { F3 a, b; };
+__device__ __forceinline__ P6 operator+(P6 u, P6 v) { return P6{ u.a + v.a, u.b + v.b }; }
+__device__ __forceinline__ P6 operator-(P6 u, P6 v) { return P6{ u.a - v.a, u.b - v.b }; }
+__device__ __forceinline__ P6 operator*(float s, P6 u) { return P6{ s * u.a, s * u.b }; }
+__device__ __forceinline__ P6 ld6(const float* p, size_t i)
+{
+	const float2* q = reinterpret_cast<const float2*>(p + 3 * i);
+	const float2 u = q[0], v = q[1], w = q[2];
+	return P6{ F3{ u.x, u.y, v.x }, F3{ v.y, w.x, w.y } };
+}
+__device__ __forceinline__ void st6(float* p, size_t i, P6 x)
+{
+	float2* q = reinterpret_cast<float2*>(p + 3 * i);
+	q[0] = make_float2(x.a.x, x.a.y); q[1] = make_float2(x.a.z, x.b.x); q[2] = make_float2(x.b.y, x.b.z);
+}
+
+template <class V> __device__ __forceinline__ V ldv(const float* p, size_t i);
+template <> __device__ __forceinline__ F3 ldv<F3>(const float* p, size_t i) { return ld3(p, i); }
+template <> __device__ __forceinline__ P6 ldv<P6>(const float* p, size_t i) { return ld6(p, i); }
+__device__ __forceinline__ void stv(float* p, size_t i, F3 v) { st3(p, i, v); }
+__device__ __forceinline__ void stv(float* p, size_t i, P6 v) { st6(p, i, v); }
+// renormalisation to the cell's Ms and the relative difference used as lte
+__device__ __forceinline__ F3 renorm(F3 v, float Ms) { return (Ms / norm(v)) * v; }
+__device__ __forceinline__ P6 renorm(P6 v, float2 Ms) { return P6{ renorm(v.a, Ms.x), renorm(v.b, Ms.y) }; }
+__device__ __forceinline__ float rel_err(F3 m, F3 p) { return norm(m - p) / norm(m); }
+__device__ __forceinline__ float rel_err(P6 m, P6 p) { return fmaxf(rel_err(m.a, p.a), rel_err(m.b, p.b)); }
+
+struct StageArgs {
+	const float* Min;       // magnetisation this stage evaluates (neighbours read)
+	float* Mout;            // magnetisation of the next stage (never the same array)
+	const float* Hd;        // planar demag field (3 planes of n) or null (stand-alone kernel)
+	float* sM1;
+	float* E[6];
+	const float* Ms_map;
+	float* Heff_out;        // optional AoS copy of the effective field
+	unsigned* lte;          // max lte as float bits
+	int nx, ny, nz;
+	float ihx2, ihy2, ihz2; // 1 / h^2 per axis
+	float Ms, A, alpha, K1, K2, grel;
+	float ea[3], Ha[3];
+	float dT;
+	int modules;
+	int map;
+};
+
+enum { M_RKDP = 0, M_RK4 = 1, M_TEULER = 2, M_AHEUN = 3 };
+
+// Block-wide max of a non-negative float, folded into *dst with an integer atomicMax on
+// its bit pattern (order-independent, so deterministic).  All kBS threads must call it.
+__device__ __forceinline__ void block_max_to(float v, unsigned* dst)
+{
+	__shared__ float red[kBS / 64];
+	for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+	if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		float m = red[0];
+		for (int q = 1; q < kBS / 64; q++) m = fmaxf(m, red[q]);
+		atomicMax(dst, __float_as_uint(m));
+	}
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).  Item
+// c = (b mod 8) * ceil(nb/8) + b/8 of a sweep of nb items goes to block b, so each XCD works
+// on one contiguous eighth of the sweep and neighbouring items share an L2.
+__device__ __forceinline__ long long xcd_chunk(int b, int nb)
+{
+	const int per = (nb + 7) / 8;
+	return (long long)(b % 8) * per + b / 8;
+}
+
+// Effective field of one cell; lap = sum over axes of the second differences / h^2.
+__device__ __forceinline__ F3 eff_field(const StageArgs& a, F3 m, F3 Hd, F3 lap, float Ms)
+{
+	F3 H = (a.modules & BM_MOD_ZEEMAN) ? F3{ a.Ha[0], a.Ha[1], a.Ha[2] } : F3{ 0.f, 0.f, 0.f };
+	if (a.modules & BM_MOD_DEMAG) H = H + Hd;
+	if (a.modules & BM_MOD_EXCHANGE) H = H + ((2.f * a.A) / ((float)kMU0 * Ms * Ms)) * lap;
+	if (a.modules & BM_MOD_ANIS_UNI) {
+		const F3 e{ a.ea[0], a.ea[1], a.ea[2] };
+		const float d = dot(m, e) / Ms;
+		H = H + ((2.f / ((float)kMU0 * Ms)) * d * (a.K1 + 2.f * a.K2 * (1.f - d * d))) * e;
+	}
+	return H;
+}
+
+// LLG right-hand side (Landau-Lifshitz form with the Gilbert damping folded in)
+__device__ __forceinline__ F3 llg_rhs(const StageArgs& a, F3 m, F3 H, float Ms)
+{
+	const F3 mxh = cross(m, H);
+	const float iMs = 1.f / Ms;
+	const F3 mu{ m.x * iMs, m.y * iMs, m.z * iMs };
+	const float pre = (float)(-kGAMMA) * a.grel / (1.f + a.alpha * a.alpha);
+	return pre * (mxh + a.alpha * cross(mu, mxh));
+}
+
+// Stage update of one cell (V = F3, MS = float) or of a cell pair (V = P6, MS = float2):
+// stores E_s, sM1 and the output magnetisation as the method requires and returns the
+// output magnetisation (m itself for RKDP54 stage 0, which writes none).
+template <int METHOD, int STEP, class V, class MS>
+__device__ __forceinline__ V rk_update(const StageArgs& a, size_t idx, V m, V rhs, MS Ms, float& lte_loc)
+{
+	const float dT = a.dT;
+	V mn = m;
+	if (METHOD == M_RKDP) {
+		if (STEP == 0) {
+			const V pred = ldv<V>(a.sM1, idx) + dT * ((5179.f / 57600.f) * ldv<V>(a.E[0], idx) + (7571.f / 16695.f) * ldv<V>(a.E[2], idx)
+				+ (393.f / 640.f) * ldv<V>(a.E[3], idx) - (92097.f / 339200.f) * ldv<V>(a.E[4], idx) + (187.f / 2100.f) * ldv<V>(a.E[5], idx)
+				+ (1.f / 40.f) * rhs);
+			lte_loc = fmaxf(lte_loc, rel_err(m, pred));
+			stv(a.E[0], idx, rhs);
+			return m;
+		}
+		else if (STEP == 1) {
+			stv(a.E[1], idx, rhs);
+			mn = ldv<V>(a.sM1, idx) + dT * ((3.f / 40.f) * ldv<V>(a.E[0], idx) + (9.f / 40.f) * rhs);
+		}
+		else if (STEP == 2) {
+			stv(a.E[2], idx, rhs);
+			mn = ldv<V>(a.sM1, idx) + dT * ((44.f / 45.f) * ldv<V>(a.E[0], idx) - (56.f / 15.f) * ldv<V>(a.E[1], idx) + (32.f / 9.f) * rhs);
+		}
+		else if (STEP == 3) {
+			stv(a.E[3], idx, rhs);
+			mn = ldv<V>(a.sM1, idx) + dT * ((19372.f / 6561.f) * ldv<V>(a.E[0], idx) - (25360.f / 2187.f) * ldv<V>(a.E[1], idx)
+				+ (64448.f / 6561.f) * ldv<V>(a.E[2], idx) - (212.f / 729.f) * rhs);
+		}
+		else if (STEP == 4) {
+			stv(a.E[4], idx, rhs);
+			mn = ldv<V>(a.sM1, idx) + dT * ((9017.f / 3168.f) * ldv<V>(a.E[0], idx) - (355.f / 33.f) * ldv<V>(a.E[1], idx)
+				+ (46732.f / 5247.f) * ldv<V>(a.E[2], idx) + (49.f / 176.f) * ldv<V>(a.E[3], idx) - (5103.f / 18656.f) * rhs);
+		}
+		else {
+			stv(a.E[5], idx, rhs);
+			mn = ldv<V>(a.sM1, idx) + dT * ((35.f / 384.f) * ldv<V>(a.E[0], idx) + (500.f / 1113.f) * ldv<V>(a.E[2], idx)
+				+ (125.f / 192.f) * ldv<V>(a.E[3], idx) - (2187.f / 6784.f) * ldv<V>(a.E[4], idx) + (11.f / 84.f) * rhs);
+			mn = renorm(mn, Ms);
+		}
+	}
+	else if (METHOD == M_RK4) {
+		if (STEP == 0) { stv(a.sM1, idx, m); stv(a.E[0], idx, rhs); mn = m + (dT / 2) * rhs; }
+		else if (STEP == 1) { stv(a.E[1], idx, rhs); mn = ldv<V>(a.sM1, idx) + (dT / 2) * rhs; }
+		else if (STEP == 2) { stv(a.E[2], idx, rhs); mn = ldv<V>(a.sM1, idx) + dT * rhs; }
+		else {
+			mn = ldv<V>(a.sM1, idx) + (dT / 6) * (ldv<V>(a.E[0], idx) + 2.f * ldv<V>(a.E[1], idx) + 2.f * ldv<V>(a.E[2], idx) + rhs);
+			mn = renorm(mn, Ms);
+		}
+	}
+	else {
+		if (STEP == 0) { stv(a.sM1, idx, m); mn = m + dT * rhs; }
+		else {
+			mn = 0.5f * (ldv<V>(a.sM1, idx) + m + dT * rhs);
+			mn = renorm(mn, Ms);
+			if (METHOD == M_AHEUN) lte_loc = fmaxf(lte_loc, rel_err(mn, m));
+		}
+	}
+	stv(a.Mout, idx, mn);
+	return mn;
+}
+
+// One cell of the stand-alone stage kernel (any mesh size).
+template <int METHOD, int STEP>
+__device__ __forceinline__ void stage_cell(const StageArgs& a, size_t idx, int i, int j, int k, F3 Hd, float& lte_loc)
+{
+	const F3 m = ld3(a.Min, idx);
+	const float Ms = a.Ms_map ? a.Ms_map[idx] : a.Ms;
+	F3 lap{ 0.f, 0.f, 0.f };
+	if (a.modules & BM_MOD_EXCHANGE) {
+		F3 lx{ 0, 0, 0 }, ly{ 0, 0, 0 }, lz{ 0, 0, 0 };
+		const size_t sy = a.nx, sz = (size_t)a.nx * a.ny;
+		if (a.nx > 1) {
+			if (i > 0 && i < a.nx - 1) lx = ld3(a.Min, idx + 1) + ld3(a.Min, idx - 1) - 2.f * m;
+			else if (i == 0) lx = ld3(a.Min, idx + 1) - m;
+			else lx = ld3(a.Min, idx - 1) - m;
+		}
+		if (a.ny > 1) {
+			if (j > 0 && j < a.ny - 1) ly = ld3(a.Min, idx + sy) + ld3(a.Min, idx - sy) - 2.f * m;
+			else if (j == 0) ly = ld3(a.Min, idx + sy) - m;
+			else ly = ld3(a.Min, idx - sy) - m;
+		}
+		if (a.nz > 1) {
+			if (k > 0 && k < a.nz - 1) lz = ld3(a.Min, idx + sz) + ld3(a.Min, idx - sz) - 2.f * m;
+			else if (k == 0) lz = ld3(a.Min, idx + sz) - m;
+			else lz = ld3(a.Min, idx - sz) - m;
+		}
+		lap = a.ihx2 * lx + a.ihy2 * ly + a.ihz2 * lz;
+	}
+	const F3 H = eff_field(a, m, Hd, lap, Ms);
+	if (a.Heff_out) st3(a.Heff_out, idx, H);
+	(void)rk_update<METHOD, STEP>(a, idx, m, llg_rhs(a, m, H, Ms), Ms, lte_loc);
+}
+
+// Cells i0 = 2k and i0 + 1 of row (j, k) for the row kernel (nx even, so both exist).  Row
+// and plane boundaries are uniform over a row, so only the x ends take a per-lane select.
+// Returns the output magnetisation of both cells.
+template <int METHOD, int STEP>
+__device__ __forceinline__ P6 stage_pair(const StageArgs& a, size_t idx, int i0, int j, int k, P6 m, P6 Hd, float& lte_loc)
+{
+	const float2 Ms = a.Ms_map ? *reinterpret_cast<const float2*>(a.Ms_map + idx) : make_float2(a.Ms, a.Ms);
+	F3 lap0{ 0.f, 0.f, 0.f }, lap1{ 0.f, 0.f, 0.f };
+	if (a.modules & BM_MOD_EXCHANGE) {
+		// x: the two cells are each other's neighbours; clamped loads keep the ends branch-free
+		const bool hasL = i0 > 0, hasR = i0 + 2 < a.nx;
+		const F3 L = ld3(a.Min, hasL ? idx - 1 : idx), R = ld3(a.Min, hasR ? idx + 2 : idx + 1);
+		F3 lx0 = hasL ? (m.b + L) - 2.f * m.a : m.b - m.a;
+		F3 lx1 = hasR ? (R + m.a) - 2.f * m.b : m.a - m.b;
+		P6 ly{ F3{ 0, 0, 0 }, F3{ 0, 0, 0 } }, lz = ly;
+		const size_t sy = a.nx, sz = (size_t)a.nx * a.ny;
+		if (a.ny > 1) {
+			if (j > 0 && j < a.ny - 1) ly = ld6(a.Min, idx + sy) + ld6(a.Min, idx - sy) - 2.f * m;
+			else if (j == 0) ly = ld6(a.Min, idx + sy) - m;
+			else ly = ld6(a.Min, idx - sy) - m;
+		}
+		if (a.nz > 1) {
+			if (k > 0 && k < a.nz - 1) lz = ld6(a.Min, idx + sz) + ld6(a.Min, idx - sz) - 2.f * m;
+			else if (k == 0) lz = ld6(a.Min, idx + sz) - m;
+			else lz = ld6(a.Min, idx - sz) - m;
+		}
+		lap0 = a.ihx2 * lx0 + a.ihy2 * ly.a + a.ihz2 * lz.a;
+		lap1 = a.ihx2 * lx1 + a.ihy2 * ly.b + a.ihz2 * lz.b;
+	}
+	const F3 H0 = eff_field(a, m.a, Hd.a, lap0, Ms.x), H1 = eff_field(a, m.b, Hd.b, lap1, Ms.y);
+	if (a.Heff_out) st6(a.Heff_out, idx, P6{ H0, H1 });
+	const P6 rhs{ llg_rhs(a, m.a, H0, Ms.x), llg_rhs(a, m.b, H1, Ms.y) };
+	return rk_update<METHOD, STEP>(a, idx, m, rhs, Ms, lte_loc);
+}
+
+// Stages that produce an lte (reduced over the mesh to a max)
+template <int METHOD, int STEP> constexpr bool stage_has_lte() { return (METHOD == M_RKDP && STEP == 0) || (METHOD == M_AHEUN && STEP == 1); }
+
+} // namespace bm

@@ -148,8 +148,10 @@ int bm_ode_average_m(bm_ode_t o, double* out3, void* stream);
  * launch is bracketed by HIP events recorded on the launch's own stream; bm_prof_query
  * returns the count and the total / min / max duration (ms) of one kind:
  * 0 x-R2C pass, 1 y-FFT pass, 2 z-pass with kernel multiply, 3 y-IFFT pass,
- * 4 x-C2R pass, 5 2D y pass with kernel multiply, 6 fused field + RK stage, 7 RKDP54
- * advance.  Not thread safe; meant for benchmarks. */
+ * 4 x-C2R pass, 5 2D y pass with kernel multiply, 6 field + RK stage (no demag),
+ * 7 RKDP54 advance (no demag), 8 fused row kernel (x-C2R of H_demag + field + RK stage +
+ * x-R2C of the stage's output M), 9 x-R2C pass fused with the RKDP54 advance.
+ * Not thread safe; meant for benchmarks. */
 int bm_prof_begin(int max_launches);
 int bm_prof_end(void);
 int bm_prof_query(int kind, long long* count, double* total_ms, double* min_ms, double* max_ms);

@@ -1,4 +1,9 @@
-"""GPU parity of the fused field + LLG stage path against the oracle."""
+"""GPU parity of the field + LLG stage paths against the oracle.
+
+Both evaluation paths are covered: the fused row kernel (x-C2R of H_demag + field + stage +
+x-R2C of the stage's output M; the default) and the unfused one (planar H_demag written by
+the convolution, then the stage kernel; BM_ROW_KERNEL=0 at solver creation).
+"""
 import numpy as np
 import pytest
 import torch
@@ -25,6 +30,12 @@ def sp4_params():
                         modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ZEEMAN)
 
 
+@pytest.fixture(params=["row", "unfused"])
+def path(request, monkeypatch):
+    monkeypatch.setenv("BM_ROW_KERNEL", "1" if request.param == "row" else "0")
+    return request.param
+
+
 def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
 
@@ -32,7 +43,7 @@ def rel_l2(a, b):
 @pytest.mark.parametrize("modules", [
     api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ZEEMAN,
     api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ZEEMAN | api.BM_MOD_ANIS_UNI])
-def test_effective_field_matches_oracle(gpu, modules):
+def test_effective_field_matches_oracle(gpu, path, modules):
     n, h = (32, 24, 4), (4e-9, 4e-9, 4e-9)
     p = api.FMParams(Ms=1e6, A=1.5e-11, alpha=0.1, K1=5e5, ea=(0.0, 0.0, 1.0), Ha=(1e3, -2e3, 39788.736), modules=modules)
     ncell = n[0] * n[1] * n[2]
@@ -48,7 +59,7 @@ def test_effective_field_matches_oracle(gpu, modules):
 
 
 @pytest.mark.parametrize("method", [api.BM_EVAL_RKDP54, api.BM_EVAL_RK4, api.BM_EVAL_TEULER, api.BM_EVAL_AHEUN])
-def test_short_run_matches_oracle(gpu, method):
+def test_short_run_matches_oracle(gpu, path, method):
     n, h, p = SP4_N, SP4_H, sp4_params()
     ncell = n[0] * n[1] * n[2]
     M0 = tilted_x(ncell, p.Ms, 11)
@@ -65,3 +76,65 @@ def test_short_run_matches_oracle(gpu, method):
     assert np.abs(mg - mo).max() <= 1e-4, (mg, mo)
     Mg = s.M.cpu().numpy()
     assert rel_l2(Mg, o.get_M()) <= 1e-4
+
+
+def bench_like_params():
+    return api.FMParams(Ms=1e6, A=1.5e-11, alpha=0.1, K1=5e5, ea=(0.0, 0.0, 1.0), Ha=(0.0, 0.0, 50e-3 / api.MU0),
+                        modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ANIS_UNI | api.BM_MOD_ZEEMAN)
+
+
+@pytest.mark.parametrize("method", [api.BM_EVAL_RKDP54, api.BM_EVAL_RK4, api.BM_EVAL_TEULER, api.BM_EVAL_AHEUN])
+def test_3d_run_matches_oracle(gpu, path, method):
+    """3D mesh (z pass + kernel multiply), random start: all four methods, 12 steps."""
+    n, h, p = (64, 48, 6), (4e-9, 4e-9, 4e-9), bench_like_params()
+    ncell = n[0] * n[1] * n[2]
+    M0 = api.random_magnetisation(ncell, p.Ms, 5).astype(np.float32)
+    s = api.FMSolver(method, n, h, p, M0)
+    # the oracle with the same N.z (next power of two of 2 nz) as the GPU path
+    o = O.OracleSim(method, n, h, p, Nz_override=16)
+    o.set_M(M0.astype(np.float64))
+    steps = 12
+    s.run_to_iteration(steps)
+    o.run_to_iteration(steps)
+    st, so = s.state(), o.state()
+    assert st["iteration"] == so["iteration"] == steps
+    assert st["rejections"] == so["rejections"]
+    assert abs(st["time"] - so["time"]) <= 1e-3 * so["time"]
+    assert rel_l2(s.M.cpu().numpy(), o.get_M()) <= 1e-4
+
+
+def test_rkdp54_rejections_match_oracle(gpu, path):
+    """A 0.2 rad random cone about x under a 5e5 J/m^3 z anisotropy makes the default first
+    steps too long: rejected steps (restore of M, re-primed FSAL) in both."""
+    n, h, p = (64, 32, 4), (4e-9, 4e-9, 4e-9), bench_like_params()
+    ncell = n[0] * n[1] * n[2]
+    M0 = tilted_x(ncell, p.Ms, 9, tilt=0.2)
+    s = api.FMSolver(api.BM_EVAL_RKDP54, n, h, p, M0)
+    o = O.OracleSim(api.BM_EVAL_RKDP54, n, h, p, Nz_override=8)
+    o.set_M(M0.astype(np.float64))
+    s.run_to_iteration(8)
+    o.run_to_iteration(8)
+    st, so = s.state(), o.state()
+    assert so["rejections"] >= 2
+    assert st["rejections"] == so["rejections"]
+    assert abs(st["time"] - so["time"]) <= 1e-4 * so["time"]
+    assert rel_l2(s.M.cpu().numpy(), o.get_M()) <= 2e-5
+
+
+def test_caller_writes_m_between_steps(gpu, path):
+    """The solver must pick up a magnetisation the caller wrote between two calls."""
+    n, h, p = (64, 32, 4), (4e-9, 4e-9, 4e-9), bench_like_params()
+    ncell = n[0] * n[1] * n[2]
+    M0 = api.random_magnetisation(ncell, p.Ms, 3).astype(np.float32)
+    M1 = api.random_magnetisation(ncell, p.Ms, 4).astype(np.float32)
+    s = api.FMSolver(api.BM_EVAL_RK4, n, h, p, M0)
+    o = O.OracleSim(api.BM_EVAL_RK4, n, h, p, Nz_override=8)
+    o.set_M(M0.astype(np.float64))
+    s.advance_time()
+    o.advance_time()
+    s.M.copy_(torch.from_numpy(M1).to(s.M.device))
+    o.set_M(M1.astype(np.float64))
+    for _ in range(3):
+        s.advance_time()
+        o.advance_time()
+    assert rel_l2(s.M.cpu().numpy(), o.get_M()) <= 1e-5
