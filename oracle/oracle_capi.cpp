@@ -1,6 +1,7 @@
 // ORACLE — TEST INFRASTRUCTURE ONLY.  C-ABI over the oracle for the Python tests
 // (ctypes) and bench.py's cpu_baseline leg.  Nothing in the product links it.
 #include "oracle_core.hpp"
+#include "oracle_mlayer.hpp"
 
 #include <cstring>
 #include <exception>
@@ -39,6 +40,44 @@ int or_tensor2d(int nx, int ny, int Nx, int Ny, double hx, double hy, double hz,
 	}
 	catch (...) { return 1; }
 }
+
+// 2D z-shifted tensor between source cell s and destination cell d (already normalised),
+// shift shz: Ddiag and Dodiag Nx*Ny*3 each (-N convention).
+int or_tensor2d_zshift(int nx, int ny, int Nx, int Ny, const double* s3, const double* d3, double shz, double* Ddiag, double* Dodiag)
+{
+	try {
+		std::vector<double> Dd, Do;
+		tensor2d_zshift(nx, ny, Nx, Ny, s3, d3, shz, Dd, Do);
+		std::memcpy(Ddiag, Dd.data(), Dd.size() * sizeof(double));
+		std::memcpy(Dodiag, Do.data(), Do.size() * sizeof(double));
+		return 0;
+	}
+	catch (...) { return 1; }
+}
+
+// Real kernels of a z-shifted 2D tensor: Kd, Ko (Ny/2+1)*(Nx/2+1)*3 each.
+int or_kernels2d_zshift(int Nx, int Ny, const double* Ddiag, const double* Dodiag, double* Kd, double* Ko)
+{
+	try {
+		const size_t NN = (size_t)Nx * Ny * 3;
+		std::vector<double> Dd(Ddiag, Ddiag + NN), Do(Dodiag, Dodiag + NN), kd, ko;
+		kernels2d_zshift(Nx, Ny, Dd, Do, kd, ko);
+		std::memcpy(Kd, kd.data(), kd.size() * sizeof(double));
+		std::memcpy(Ko, ko.data(), ko.size() * sizeof(double));
+		return 0;
+	}
+	catch (...) { return 1; }
+}
+
+// Multilayered 2D demag of L stacked layers of nx*ny cells.
+void* or_mlayer_create(int L, int nx, int ny, double hx, double hy, const double* hz, const double* z0)
+{
+	try { return new MLayer<double>(L, nx, ny, hx, hy, hz, z0); }
+	catch (...) { return nullptr; }
+}
+void or_mlayer_destroy(void* h) { delete (MLayer<double>*)h; }
+// M, H: L*nx*ny*3 doubles, layer-major
+void or_mlayer_convolute(void* h, const double* M, double* H) { ((MLayer<double>*)h)->convolute(M, H); }
 
 void or_self_demag(double hx, double hy, double hz, double* out3) { self_demag(hx, hy, hz, out3); }
 double or_newell_f(double x, double y, double z) { return nw_f(x, y, z); }

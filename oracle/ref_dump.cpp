@@ -10,11 +10,18 @@
 //        -> Ddiag (Nx*Ny DBL3) then Dodiag (Nx*Ny doubles)
 //   ref_dump self hx hy hz out.bin      -> SelfDemag (3 doubles)
 //   ref_dump single x y z hx hy hz out.bin -> Ldia_single (3) then Lodia_single (3)
+//   ref_dump zshift2d nx ny Nx Ny sx sy sz dx dy dz shz out.bin
+//        -> Ddiag then Dodiag (Nx*Ny DBL3 each, x fastest) of the 2D z-shifted tensor
+//           between a source cell s and a destination cell d (both already normalised,
+//           differing at most in z), shift shz (normalised): the irregular entry point,
+//           which falls back to the regular shifted one when s == d
 //
 // Reference entry points used: DemagTFunc::CalcDiagTens3D/CalcOffDiagTens3D/
 // CalcDiagTens2D/CalcOffDiagTens2D (Boris/DemagTFunc.cpp:21,95,172,237),
 // DemagTFunc::SelfDemag (Boris/DemagTFunc_Self.cpp:46),
-// DemagTFunc::Ldia_single/Lodia_single (Boris/DemagTFunc_Test.cpp).
+// DemagTFunc::Ldia_single/Lodia_single (Boris/DemagTFunc_Test.cpp),
+// DemagTFunc::CalcDiagTens2D_Shifted_Irregular/CalcOffDiagTens2D_Shifted_Irregular
+// (Boris/DemagTFunc_Irregular.cpp:5,121).
 
 #include "stdafx.h"
 #include "DemagTFunc.h"
@@ -82,6 +89,22 @@ int main(int argc, char** argv)
 		DBL3 a = dtf.Ldia_single(d, h, true);
 		DBL3 b = dtf.Lodia_single(d, h, true);
 		write_doubles(argv[8], { a.x, a.y, a.z, b.x, b.y, b.z });
+		return 0;
+	}
+	if (mode == "zshift2d" && argc == 14) {
+		INT3 n(atoi(argv[2]), atoi(argv[3]), 1);
+		INT3 N(atoi(argv[4]), atoi(argv[5]), 1);
+		DBL3 s(atof(argv[6]), atof(argv[7]), atof(argv[8]));
+		DBL3 d(atof(argv[9]), atof(argv[10]), atof(argv[11]));
+		DBL3 shift(0.0, 0.0, atof(argv[12]));
+		VEC<DBL3> Dd, Do;
+		if (!Dd.resize(SZ3(N.x, N.y, 1)) || !Do.resize(SZ3(N.x, N.y, 1))) return 3;
+		if (!dtf.CalcDiagTens2D_Shifted_Irregular(Dd, n, N, s, d, shift)) return 4;
+		if (!dtf.CalcOffDiagTens2D_Shifted_Irregular(Do, n, N, s, d, shift)) return 4;
+		std::vector<double> out;
+		for (size_t i = 0; i < Dd.linear_size(); i++) { out.push_back(Dd[i].x); out.push_back(Dd[i].y); out.push_back(Dd[i].z); }
+		for (size_t i = 0; i < Do.linear_size(); i++) { out.push_back(Do[i].x); out.push_back(Do[i].y); out.push_back(Do[i].z); }
+		write_doubles(argv[13], out);
 		return 0;
 	}
 	fprintf(stderr, "bad arguments\n");

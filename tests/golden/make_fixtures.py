@@ -9,6 +9,11 @@ reference checkout and oracle/_ref exist; the GPU box only reads the results).
   mirror images) for small meshes, and a seeded sample of entries for meshes large enough to
   use the far-field series (distance >= 40 cells); plus SelfDemag and single-value
   Ldia/Lodia.
+* zshift_golden.npz — FP64 2D z-shifted source->destination tensors (multilayered demag)
+  written by oracle/_ref/ref_dump from the reference's Boris/DemagTFunc_Shifted.cpp and
+  DemagTFunc_Irregular.cpp: regular (equal cells) and irregular (thicknesses differ), with
+  near-field and far-field (>= 40 cells) entries; all entries for small meshes, a seeded
+  sample for large ones.
 """
 from __future__ import annotations
 
@@ -93,7 +98,42 @@ def main() -> None:
     out["single_in"] = np.array([list(d) + list(hh) for d, hh in singles])
     out["single_val"] = np.array([dump("single", *d, *hh) for d, hh in singles])
     np.savez_compressed(HERE / "tensor_golden.npz", **out)
+    zshift_fixtures()
     print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
+
+
+# (name, n, N, source cell, destination cell, shift), cells and shift normalised
+ZSHIFT_CASES = [
+    ("z_irr_overlap", (8, 6), (16, 12), (0.5, 0.5, 1.0), (0.5, 0.5, 0.5), 0.75),
+    ("z_reg_touch", (8, 6), (16, 12), (0.5, 0.5, 1.0), (0.5, 0.5, 1.0), 1.0),
+    ("z_irr_below", (9, 7), (18, 14), (0.5, 0.5, 0.25), (0.5, 0.5, 1.0), -0.625),
+    ("z_irr_far", (50, 45), (100, 90), (0.25, 0.25, 1.0), (0.25, 0.25, 0.5), 0.75),
+    ("z_reg_far", (50, 45), (100, 90), (0.25, 0.25, 0.5), (0.25, 0.25, 0.5), -0.5),
+    ("z_irr_distant", (12, 10), (24, 20), (0.25, 0.25, 0.5), (0.25, 0.25, 1.0), 45.0),
+    ("z_cfg5_a", (64, 64), (128, 128), (0.25, 0.25, 0.5), (0.25, 0.25, 0.25), 0.5),
+    ("z_cfg5_b", (64, 64), (128, 128), (0.25, 0.25, 1.0), (0.25, 0.25, 0.5), -1.0),
+]
+
+
+def zshift_fixtures() -> None:
+    rng = np.random.default_rng(20261203)
+    out = {}
+    for name, nn, NN, s, d, sh in ZSHIFT_CASES:
+        v = dump("zshift2d", *nn, *NN, *map(float, s), *map(float, d), float(sh))
+        m = NN[0] * NN[1]
+        Dd = v[:3 * m].reshape(NN[1], NN[0], 3)
+        Do = v[3 * m:].reshape(NN[1], NN[0], 3)
+        if m <= 400:
+            idx = np.array([(j, i) for j in range(NN[1]) for i in range(NN[0])])
+        else:
+            idx = np.stack([rng.integers(0, NN[1], 1500), rng.integers(0, NN[0], 1500)], axis=1)
+        out[name + "_n"], out[name + "_N"] = np.array(nn), np.array(NN)
+        out[name + "_s"], out[name + "_d"], out[name + "_shift"] = np.array(s), np.array(d), np.array(sh)
+        out[name + "_idx"] = idx
+        out[name + "_diag"] = Dd[idx[:, 0], idx[:, 1]]
+        out[name + "_off"] = Do[idx[:, 0], idx[:, 1]]
+    out["names"] = np.array([c[0] for c in ZSHIFT_CASES])
+    np.savez_compressed(HERE / "zshift_golden.npz", **out)
 
 
 if __name__ == "__main__":

@@ -54,6 +54,11 @@ def lib() -> C.CDLL:
     L.or_sim_iterate_stage.argtypes = [vp]
     L.or_sim_advance_time.argtypes = [vp]
     L.or_sim_run_to_iteration.argtypes = [vp, i]
+    L.or_tensor2d_zshift.argtypes = [i] * 4 + [dp, dp, d, dp, dp]
+    L.or_kernels2d_zshift.argtypes = [i, i, dp, dp, dp, dp]
+    L.or_mlayer_create.argtypes = [i, i, i, d, d, dp, dp]; L.or_mlayer_create.restype = vp
+    L.or_mlayer_destroy.argtypes = [vp]
+    L.or_mlayer_convolute.argtypes = [vp, dp, dp]
     L.or_set_random_mag.argtypes = [dp, C.c_longlong, C.c_uint]
     L.or_borisrand.argtypes = [C.c_uint, i, dp]
     _L = L
@@ -86,6 +91,48 @@ def kernels(n, N, h):
     a, b = np.zeros(nc * nyh * nzh * 3), np.zeros(nc * nyh * nzh * 3)
     assert L.or_kernels(*n, *N, *h, a, b) == 0
     return a.reshape(nzh, nyh, nc, 3), b.reshape(nzh, nyh, nc, 3)
+
+
+def tensor2d_zshift(n, N, s, d, shift):
+    """2D z-shifted tensor (normalised source cell s, destination cell d, shift): (Dd, Do),
+    each (Ny, Nx, 3); Do = (xy, xz, yz)."""
+    L = lib()
+    a, b = np.zeros(N[0] * N[1] * 3), np.zeros(N[0] * N[1] * 3)
+    assert L.or_tensor2d_zshift(n[0], n[1], N[0], N[1], np.array(s, float), np.array(d, float), float(shift), a, b) == 0
+    return a.reshape(N[1], N[0], 3), b.reshape(N[1], N[0], 3)
+
+
+def kernels2d_zshift(N, Dd, Do):
+    """Real kernels of a z-shifted 2D tensor: (Kd, Ko), each (Ny/2+1, Nx/2+1, 3)."""
+    L = lib()
+    nc, nyh = N[0] // 2 + 1, N[1] // 2 + 1
+    a, b = np.zeros(nc * nyh * 3), np.zeros(nc * nyh * 3)
+    assert L.or_kernels2d_zshift(N[0], N[1], np.ascontiguousarray(Dd, float).reshape(-1),
+                                 np.ascontiguousarray(Do, float).reshape(-1), a, b) == 0
+    return a.reshape(nyh, nc, 3), b.reshape(nyh, nc, 3)
+
+
+class OracleMLayer:
+    """The oracle's multilayered 2D demag: L layers of nx x ny cells (hx, hy, hz[l]) with
+    bottom faces at z0[l]; M and H are (L*nx*ny, 3), layer-major."""
+
+    def __init__(self, nx, ny, hx, hy, hz, z0):
+        self.L = lib()
+        self.nl = len(hz)
+        self.h = self.L.or_mlayer_create(self.nl, nx, ny, hx, hy, np.array(hz, float), np.array(z0, float))
+        assert self.h
+
+    def __call__(self, M):
+        M = np.ascontiguousarray(M, dtype=np.float64).reshape(-1)
+        H = np.zeros_like(M)
+        self.L.or_mlayer_convolute(self.h, M, H)
+        return H.reshape(-1, 3)
+
+    def __del__(self):
+        try:
+            self.L.or_mlayer_destroy(self.h)
+        except Exception:
+            pass
 
 
 def fft(x: np.ndarray, sign: int = -1) -> np.ndarray:

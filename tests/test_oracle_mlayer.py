@@ -1,0 +1,157 @@
+"""Pins the oracle's multilayered (stacked 2D layers) demag.
+
+* z-shifted source->destination tensors, regular and irregular, near and far field:
+  bit-identical to the reference's own Boris/DemagTFunc_Shifted.cpp / DemagTFunc_Irregular.cpp
+  (golden vectors written by oracle/_ref/ref_dump, tests/golden/zshift_golden.npz);
+* geometry, independent of the reference: a thick source layer equals the sum of thin
+  slices and a thick destination averages thin slices (xx, yy at the slices' positions; zz
+  and off-diagonal at the offset the reference's construction implies, see below); the
+  regular z-shifted tensor of touching layers equals the 3D tensor's kz = +-1 planes;
+* kernels: equal to the full complex 2D FFT of the tensor (the parts the product keeps,
+  and zero where it assumes zero);
+* multilayer convolution: equal to the direct real-space sum with the pinned tensors, and a
+  stack of equal touching layers equal to the 3D oracle convolution of the same cells.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+GOLD = Path(__file__).resolve().parent / "golden" / "zshift_golden.npz"
+
+
+def test_zshift_tensor_bit_identical_to_reference_golden():
+    g = np.load(GOLD)
+    for name in g["names"]:
+        n, N = tuple(g[name + "_n"]), tuple(g[name + "_N"])
+        Dd, Do = O.tensor2d_zshift(n, N, g[name + "_s"], g[name + "_d"], float(g[name + "_shift"]))
+        idx = g[name + "_idx"]
+        assert np.array_equal(Dd[idx[:, 0], idx[:, 1]], g[name + "_diag"]), name
+        assert np.array_equal(Do[idx[:, 0], idx[:, 1]], g[name + "_off"]), name
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+# The reference's irregular tensor (Boris/DemagTFunc_Irregular.cpp with the grids of
+# Boris/DemagTFunc_fg.cpp:507-620) places the xx and yy basis-function planes at
+# {-s.z, 0, +d.z} + shift but the zz and off-diagonal planes at {-d.z, 0, +s.z} + shift
+# (del = s.z - d.z enters with opposite signs).  With shift = destination bottom - source
+# bottom, xx and yy are exactly the sum (average) over thin slices of a thick source
+# (destination) at their true positions; zz, xy, xz and yz equal the same sums taken at
+# shift + del.  Both are exact identities, so they pin the 36-term stencils of the
+# restatement independently of the golden vectors.
+@pytest.mark.parametrize("S,sh", [(1.0, 0.5), (1.0, 1.0), (1.5, -1.5), (1.5, 2.25)])
+def test_thick_source_equals_sum_of_thin_slices(S, sh):
+    n, N = (6, 5), (12, 10)
+    c = 0.5
+    thin, thick = (0.8, 0.6, c), (0.8, 0.6, S)
+    m, dl = int(round(S / c)), S - c
+    Dd, Do = O.tensor2d_zshift(n, N, thick, thin, sh)
+    A = sum(O.tensor2d_zshift(n, N, thin, thin, sh - q * c)[0] for q in range(m))
+    B = [sum(O.tensor2d_zshift(n, N, thin, thin, sh + dl - q * c)[p] for q in range(m)) for p in (0, 1)]
+    for c_ in (0, 1):
+        assert _rel(Dd[..., c_], A[..., c_]) <= 1e-9
+    assert _rel(Dd[..., 2], B[0][..., 2]) <= 1e-9
+    assert _rel(Do, B[1]) <= 1e-9
+
+
+@pytest.mark.parametrize("S,sh", [(1.0, -0.5), (1.0, 1.0), (1.5, -2.0), (1.5, 3.5)])
+def test_thick_destination_averages_thin_slices(S, sh):
+    n, N = (6, 5), (12, 10)
+    c = 0.5
+    thin, thick = (0.8, 0.6, c), (0.8, 0.6, S)
+    m, dl = int(round(S / c)), c - S
+    Dd, Do = O.tensor2d_zshift(n, N, thin, thick, sh)
+    A = sum(O.tensor2d_zshift(n, N, thin, thin, sh + q * c)[0] for q in range(m)) / m
+    B = [sum(O.tensor2d_zshift(n, N, thin, thin, sh + dl + q * c)[p] for q in range(m)) / m for p in (0, 1)]
+    for c_ in (0, 1):
+        assert _rel(Dd[..., c_], A[..., c_]) <= 1e-9
+    assert _rel(Dd[..., 2], B[0][..., 2]) <= 1e-9
+    assert _rel(Do, B[1]) <= 1e-9
+
+
+@pytest.mark.parametrize("n", [(7, 5), (45, 6)])
+def test_regular_zshift_equals_3d_tensor_planes(n):
+    h = (4e-9, 3e-9, 5e-9)
+    hm = max(h)
+    hr = tuple(v / hm for v in h)
+    N3 = (2 * n[0], 2 * n[1], 4)
+    Dd3, Do3 = O.tensor3d((n[0], n[1], 2), N3, h)
+    # the +1 plane evaluates the same basis-function values in the same order: identical;
+    # the -1 plane is the 3D tensor's mirror image, here evaluated at negative z: equal up
+    # to the cancellation of the 27-term stencil far from the source (a few 1e-12 of max)
+    Dd, Do = O.tensor2d_zshift(n, N3[:2], hr, hr, hr[2])
+    assert np.array_equal(Dd, Dd3[1]) and np.array_equal(Do, Do3[1])
+    Dd, Do = O.tensor2d_zshift(n, N3[:2], hr, hr, -hr[2])
+    assert _rel(Dd, Dd3[3]) <= 1e-11 and _rel(Do, Do3[3]) <= 1e-11
+
+
+def test_zshift_kernels_equal_full_fft_of_tensor():
+    n, N = (9, 6), (18, 12)
+    Dd, Do = O.tensor2d_zshift(n, N, (0.5, 0.5, 1.0), (0.5, 0.5, 0.5), 0.75)
+    Kd, Ko = O.kernels2d_zshift(N, Dd, Do)
+    nc, nyh = N[0] // 2 + 1, N[1] // 2 + 1
+    F = [np.fft.fft2(Dd[..., c])[:nyh, :nc] for c in range(3)]
+    G = [np.fft.fft2(Do[..., c])[:nyh, :nc] for c in range(3)]
+    scale = max(np.abs(x).max() for x in F + G)
+    for c in range(3):
+        assert np.abs(Kd[..., c] - F[c].real).max() <= 1e-13 * scale
+        assert np.abs(F[c].imag).max() <= 1e-13 * scale
+    # xy is real; xz and yz are i times the stored values
+    assert np.abs(Ko[..., 0] - G[0].real).max() <= 1e-13 * scale
+    assert np.abs(Ko[..., 1] - G[1].imag).max() <= 1e-13 * scale
+    assert np.abs(Ko[..., 2] - G[2].imag).max() <= 1e-13 * scale
+    for c in range(3):
+        assert np.abs(G[c].imag if c == 0 else G[c].real).max() <= 1e-13 * scale
+
+
+def direct_sum_mlayer(nx, ny, hx, hy, hz, z0, M):
+    L = len(hz)
+    hmax = max(max(hx, hy), max(hz))
+    N = (2 * nx, 2 * ny)
+    Mr = M.reshape(L, ny, nx, 3)
+    H = np.zeros_like(Mr)
+    for d in range(L):
+        for s in range(L):
+            if d == s:
+                Dd, Dxy = O.tensor2d((nx, ny), N, (hx, hy, hz[d]))
+                Do = np.zeros(Dd.shape)
+                Do[..., 0] = Dxy
+            else:
+                sc = (hx / hmax, hy / hmax, hz[s] / hmax)
+                dc = (hx / hmax, hy / hmax, hz[d] / hmax)
+                Dd, Do = O.tensor2d_zshift((nx, ny), N, sc, dc, (z0[d] - z0[s]) / hmax)
+            mx, my, mz = Mr[s, ..., 0], Mr[s, ..., 1], Mr[s, ..., 2]
+            for j in range(ny):
+                for i in range(nx):
+                    di = (i - np.arange(nx)) % N[0]
+                    dj = (j - np.arange(ny)) % N[1]
+                    dd = Dd[np.ix_(dj, di)]
+                    oo = Do[np.ix_(dj, di)]
+                    H[d, j, i, 0] += (dd[..., 0] * mx + oo[..., 0] * my + oo[..., 1] * mz).sum()
+                    H[d, j, i, 1] += (oo[..., 0] * mx + dd[..., 1] * my + oo[..., 2] * mz).sum()
+                    H[d, j, i, 2] += (oo[..., 1] * mx + oo[..., 2] * my + dd[..., 2] * mz).sum()
+    return H.reshape(-1, 3)
+
+
+def test_mlayer_convolution_equals_direct_sum():
+    nx, ny, hx, hy = 6, 5, 4e-9, 4e-9
+    hz, z0 = [8e-9, 4e-9, 16e-9], [0.0, 9e-9, 15e-9]
+    rng = np.random.default_rng(5)
+    M = rng.normal(size=(3 * nx * ny, 3)) * 8e5
+    ref = direct_sum_mlayer(nx, ny, hx, hy, hz, z0, M)
+    H = O.OracleMLayer(nx, ny, hx, hy, hz, z0)(M)
+    assert np.linalg.norm(H - ref) / np.linalg.norm(ref) <= 1e-13
+
+
+def test_stack_of_touching_layers_equals_3d_convolution():
+    nx, ny, L, h = 8, 6, 4, 4e-9
+    rng = np.random.default_rng(6)
+    M = rng.normal(size=(L * nx * ny, 3)) * 8e5
+    H = O.OracleMLayer(nx, ny, h, h, [h] * L, [l * h for l in range(L)])(M)
+    ref = O.OracleDemag((nx, ny, L), (h, h, h), Nz_override=8)(M)
+    assert np.linalg.norm(H - ref) / np.linalg.norm(ref) <= 1e-12
