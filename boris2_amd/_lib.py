@@ -22,10 +22,13 @@ EXPORTS = [
     "bm_ode_set_astep", "bm_ode_advance_time", "bm_ode_run_to_iteration", "bm_ode_iterate_stage",
     "bm_ode_restore", "bm_ode_get_state", "bm_ode_keep_heff", "bm_ode_copy_heff", "bm_ode_average_m",
     "bm_prof_begin", "bm_prof_end", "bm_prof_query",
+    "bm_mlayer_create", "bm_mlayer_destroy", "bm_mlayer_convolute", "bm_mlayer_copy_kernels",
+    "bm_batch_create", "bm_batch_destroy", "bm_batch_set_dt", "bm_batch_run", "bm_batch_get_state",
+    "bm_batch_mesh_offset",
 ]
 
 PROF_KINDS = ["fwd_x_r2c", "y_fft", "z_fft_kernel_ifft", "y_ifft", "inv_x_c2r", "y2d_fft_kernel_ifft",
-              "llg_stage", "rkdp_advance", "row_c2r_stage_r2c", "fwd_x_advance"]
+              "llg_stage", "rkdp_advance", "row_c2r_stage_r2c", "fwd_x_advance", "ml_kernel_product"]
 
 
 class FmParams(C.Structure):
@@ -67,6 +70,16 @@ def load() -> C.CDLL:
         "bm_demag_convolute": (i, [vp, vp, vp, i, vp, vp]),
         "bm_demag_convolute_planar": (i, [vp, vp, vp, vp]),
         "bm_demag_copy_kernels": (i, [vp, vp, ll]),
+        "bm_mlayer_create": (i, [i, i, i, d, d, vp, vp, i, C.POINTER(vp)]),
+        "bm_mlayer_destroy": (i, [vp]),
+        "bm_mlayer_convolute": (i, [vp, vp, vp, i, vp]),
+        "bm_mlayer_copy_kernels": (i, [vp, vp, ll]),
+        "bm_batch_create": (i, [i, i, vp, vp, vp, vp, i, C.POINTER(vp)]),
+        "bm_batch_destroy": (i, [vp]),
+        "bm_batch_set_dt": (i, [vp, d]),
+        "bm_batch_run": (i, [vp, ll, vp]),
+        "bm_batch_get_state": (i, [vp, C.POINTER(d), C.POINTER(ll)]),
+        "bm_batch_mesh_offset": (i, [vp, i, C.POINTER(ll), C.POINTER(ll)]),
         "bm_ode_create": (i, [i, i, i, i, d, d, d, C.POINTER(FmParams), vp, vp, C.POINTER(vp)]),
         "bm_ode_destroy": (i, [vp]),
         "bm_ode_set_ms_map": (i, [vp, vp]),

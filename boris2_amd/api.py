@@ -102,6 +102,56 @@ class Demag:
             pass
 
 
+class MultiLayerDemag:
+    """Multilayered demag of stacked 2D layers: nlayers layers of nx*ny cells (hx, hy, hz[l]),
+    layer l's bottom face at height z0[l] (m).  M and H are (nlayers*nx*ny, 3) float32 CUDA
+    tensors, layer-major."""
+
+    def __init__(self, nx: int, ny: int, hx: float, hy: float, hz, z0):
+        require_device()
+        self.L = _lib.load()
+        self.nx, self.ny = int(nx), int(ny)
+        self.hz = np.ascontiguousarray(hz, dtype=np.float64)
+        self.z0 = np.ascontiguousarray(z0, dtype=np.float64)
+        if self.hz.shape != self.z0.shape or self.hz.ndim != 1:
+            raise ValueError("hz and z0 must be 1-D of the same length")
+        self.nlayers = len(self.hz)
+        self.handle = C.c_void_p()
+        check(self.L.bm_mlayer_create(self.nlayers, self.nx, self.ny, float(hx), float(hy),
+                                      self.hz.ctypes.data_as(C.c_void_p), self.z0.ctypes.data_as(C.c_void_p), 0,
+                                      C.byref(self.handle)))
+
+    @property
+    def ncells(self) -> int:
+        return self.nlayers * self.nx * self.ny
+
+    def convolute(self, M: torch.Tensor, Heff: torch.Tensor | None = None, clear_out: bool = True) -> torch.Tensor:
+        """Heff (=, +=) the demag field of every layer from all layers' magnetisation."""
+        if Heff is None:
+            Heff = torch.empty_like(M)
+            clear_out = True
+        check(self.L.bm_mlayer_convolute(self.handle, _ptr(M), _ptr(Heff), int(clear_out), _stream()))
+        return Heff
+
+    def kernels(self) -> np.ndarray:
+        """(nlayers, nlayers, 6, nx+1, ny+1) float32: destination, source, component, kx, ky."""
+        L, nc, nyh = self.nlayers, self.nx + 1, self.ny + 1
+        out = np.zeros(L * L * 6 * nc * nyh, dtype=np.float32)
+        check(self.L.bm_mlayer_copy_kernels(self.handle, out.ctypes.data_as(C.c_void_p), out.size))
+        return out.reshape(L, L, 6, nc, nyh)
+
+    def close(self) -> None:
+        if self.handle:
+            self.L.bm_mlayer_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 @dataclass
 class FMParams:
     """Material parameters and applied field of one ferromagnetic mesh (SI units)."""
@@ -212,6 +262,61 @@ class FMSolver:
             self.handle = C.c_void_p()
         if self.demag is not None:
             self.demag.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchSolver:
+    """Many independent meshes of uneven sizes integrated together with a fixed-step method
+    (RK4 or TEuler): one launch per pass size per field evaluation, a step replayed from a
+    captured hipGraph.  meshes: list of (n, h, FMParams); M0: list of per-mesh (ncell, 3)
+    arrays or None.  self.M holds every mesh's magnetisation concatenated; mesh(b) is a view."""
+
+    def __init__(self, method: int, meshes, M0=None, use_graph: bool = True):
+        require_device()
+        self.L = _lib.load()
+        self.meshes = [(tuple(int(v) for v in n), tuple(float(v) for v in h), p) for n, h, p in meshes]
+        nb = len(self.meshes)
+        counts = [n[0] * n[1] * n[2] for n, _, _ in self.meshes]
+        self.offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        self.M = torch.zeros((int(self.offsets[-1]), 3), dtype=torch.float32, device="cuda")
+        for b, (n, h, p) in enumerate(self.meshes):
+            v = self.mesh(b)
+            if M0 is None or M0[b] is None:
+                v[:, 0] = -p.Ms
+            else:
+                v.copy_(torch.as_tensor(np.asarray(M0[b], dtype=np.float32).reshape(-1, 3)))
+        n3 = np.array([v for n, _, _ in self.meshes for v in n], dtype=np.int32)
+        h3 = np.array([v for _, h, _ in self.meshes for v in h], dtype=np.float64)
+        self._pc = (_lib.FmParams * nb)(*[p.to_c() for _, _, p in self.meshes])
+        self.handle = C.c_void_p()
+        check(self.L.bm_batch_create(method, nb, n3.ctypes.data_as(C.c_void_p), h3.ctypes.data_as(C.c_void_p),
+                                     C.cast(self._pc, C.c_void_p), _ptr(self.M), 0 if use_graph else 1,
+                                     C.byref(self.handle)))
+
+    def mesh(self, b: int) -> torch.Tensor:
+        return self.M[int(self.offsets[b]):int(self.offsets[b + 1])]
+
+    def set_dt(self, dT: float) -> None:
+        check(self.L.bm_batch_set_dt(self.handle, float(dT)))
+
+    def run(self, nsteps: int) -> None:
+        check(self.L.bm_batch_run(self.handle, int(nsteps), _stream()))
+
+    def state(self) -> dict:
+        t, it = C.c_double(), C.c_longlong()
+        check(self.L.bm_batch_get_state(self.handle, C.byref(t), C.byref(it)))
+        return {"time": t.value, "iteration": it.value}
+
+    def close(self) -> None:
+        if self.handle:
+            torch.cuda.synchronize()
+            self.L.bm_batch_destroy(self.handle)
+            self.handle = C.c_void_p()
 
     def __del__(self):
         try:

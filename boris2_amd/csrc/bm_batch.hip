@@ -1,0 +1,351 @@
+// Batched fixed-step LLG integration of many independent meshes of uneven sizes.
+//
+// No reference counterpart: the reference integrates one mesh set per simulation.  Each
+// mesh keeps its own demag convolution (kernels and spectra of a bm_demag handle), fields
+// and parameters; what is shared is the launch: every pass of every field evaluation is
+// one launch per distinct pass size (x passes per nx, y passes per N.y, z passes per N.z)
+// with the meshes of that size as grid rows, and one stage launch covers all meshes.  A
+// whole step (RK4: 4 evaluations, Heun: 2) is captured once into a hipGraph and replayed,
+// so the host issues one graph launch per step however many meshes there are.
+//
+// Per-mesh arrays live concatenated (mesh b's cells at offset off[b]); the caller owns M.
+// Fixed-step methods only (RK4, TEuler); the per-mesh arguments (pointers, sizes,
+// parameters, dT) sit in device tables read by the kernels, so changing dT rewrites a
+// table and leaves the captured graph valid.
+#include "bm_internal.h"
+#include "bm_stage.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+using namespace bm;
+
+namespace {
+
+template <typename T> struct Tab {
+	std::vector<T> h;
+	T* d = nullptr;
+};
+
+struct Group { int size; std::vector<int> meshes; };
+
+} // namespace
+
+struct bm_batch_s {
+	int nb = 0, mcode = 0, nstages = 0, use_graph = 1;
+	std::vector<int> nx, ny, nz;
+	std::vector<size_t> off, ncell;
+	size_t ntot = 0, max_cells = 0;
+	std::vector<bm_fm_params> p;
+	std::vector<double> h;
+	std::vector<bm_demag_t> dm;
+	float* M = nullptr;            // caller-owned, concatenated
+	float* Mb = nullptr;
+	float* sM1 = nullptr;
+	float* E[3] = { nullptr, nullptr, nullptr };
+	float* Hd = nullptr;           // planar per mesh: 3 * ncell[b] at 3 * off[b]
+	// pass groups and their tables
+	std::vector<Group> gx, gy, gz, gy2;
+	std::vector<Tab<FwdArgs>> fwd[2];      // [input parity][nx group]
+	std::vector<Tab<PassBatch>> inv, ty, tz, ty2;
+	Tab<StageArgs> stage[2];               // [input parity]
+	void* dev_tables = nullptr;
+	size_t dev_bytes = 0;
+	int ncu = 256;
+	double dT = 0, time = 0;
+	long long iteration = 0;
+	hipStream_t gs = nullptr;
+	hipEvent_t ev_in = nullptr, ev_out = nullptr;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void free_all(bm_batch_s* o)
+{
+	if (o->exec) (void)hipGraphExecDestroy(o->exec);
+	if (o->graph) (void)hipGraphDestroy(o->graph);
+	if (o->ev_in) (void)hipEventDestroy(o->ev_in);
+	if (o->ev_out) (void)hipEventDestroy(o->ev_out);
+	if (o->gs) (void)hipStreamDestroy(o->gs);
+	for (bm_demag_t d : o->dm) if (d) bm_demag_destroy(d);
+	if (o->Mb) (void)hipFree(o->Mb);
+	if (o->sM1) (void)hipFree(o->sM1);
+	for (int q = 0; q < 3; q++) if (o->E[q]) (void)hipFree(o->E[q]);
+	if (o->Hd) (void)hipFree(o->Hd);
+	if (o->dev_tables) (void)hipFree(o->dev_tables);
+}
+
+void add_to(std::vector<Group>& gs, int size, int b)
+{
+	for (Group& g : gs) if (g.size == size) { g.meshes.push_back(b); return; }
+	gs.push_back(Group{ size, { b } });
+}
+
+StageArgs stage_args(bm_batch_s* o, int b, int parity)
+{
+	StageArgs a;
+	std::memset(&a, 0, sizeof(a));
+	const size_t off3 = 3 * o->off[b];
+	float* Ma = o->M + off3;
+	float* Mbb = o->Mb + off3;
+	a.Min = parity ? Mbb : Ma;
+	a.Mout = parity ? Ma : Mbb;
+	a.Hd = o->Hd + off3;
+	a.sM1 = o->sM1 + off3;
+	for (int q = 0; q < 3; q++) a.E[q] = o->E[q] ? o->E[q] + off3 : nullptr;
+	a.nx = o->nx[b]; a.ny = o->ny[b]; a.nz = o->nz[b];
+	const double* h = &o->h[3 * b];
+	a.ihx2 = (float)(1.0 / (h[0] * h[0])); a.ihy2 = (float)(1.0 / (h[1] * h[1])); a.ihz2 = (float)(1.0 / (h[2] * h[2]));
+	const bm_fm_params& p = o->p[b];
+	a.Ms = (float)p.Ms; a.A = (float)p.A; a.alpha = (float)p.alpha; a.K1 = (float)p.K1; a.K2 = (float)p.K2; a.grel = (float)p.grel;
+	for (int c = 0; c < 3; c++) { a.ea[c] = (float)p.ea[c]; a.Ha[c] = (float)p.Ha[c]; }
+	a.dT = (float)o->dT;
+	a.modules = p.modules;
+	a.map = 2;
+	return a;
+}
+
+// Group the meshes by pass size, build every table on the host and upload them in one
+// allocation.
+int build_tables(bm_batch_s* o)
+{
+	const int nb = o->nb;
+	for (int b = 0; b < nb; b++) {
+		if (!o->dm[b]) continue;
+		add_to(o->gx, o->nx[b], b);
+		if (o->nz[b] > 1) { add_to(o->gy, o->dm[b]->Ny, b); add_to(o->gz, o->dm[b]->Nz, b); }
+		else add_to(o->gy2, o->dm[b]->Ny, b);
+	}
+	for (int par = 0; par < 2; par++) o->fwd[par].assign(o->gx.size(), Tab<FwdArgs>());
+	o->inv.assign(o->gx.size(), Tab<PassBatch>());
+	o->ty.assign(o->gy.size(), Tab<PassBatch>());
+	o->tz.assign(o->gz.size(), Tab<PassBatch>());
+	o->ty2.assign(o->gy2.size(), Tab<PassBatch>());
+	for (size_t g = 0; g < o->gx.size(); g++) {
+		const int nx = o->gx[g].size;
+		const int tf = x_tile_rows(nx, true), ti = x_tile_rows(nx, false);
+		for (int b : o->gx[g].meshes) {
+			bm_demag_s* d = o->dm[b];
+			for (int par = 0; par < 2; par++) {
+				FwdArgs f{};
+				f.M = (par ? o->Mb : o->M) + 3 * o->off[b]; f.S = d->S;
+				f.ny = d->ny; f.nz = d->nz; f.Sp = d->Sp; f.ntiles = (d->ny + tf - 1) / tf;
+				o->fwd[par][g].h.push_back(f);
+			}
+			PassBatch q{};
+			q.S = d->S; q.out = o->Hd + 3 * o->off[b]; q.ny = d->ny; q.nz = d->nz; q.Sp = d->Sp; q.ntiles = (d->ny + ti - 1) / ti;
+			q.scale = (float)(1.0 / ((double)d->Nx * d->Ny * d->Nz));
+			o->inv[g].h.push_back(q);
+		}
+	}
+	for (size_t g = 0; g < o->gy.size(); g++)
+		for (int b : o->gy[g].meshes) {
+			bm_demag_s* d = o->dm[b];
+			PassBatch q{}; q.S = d->S; q.Sp = d->Sp; q.nlines = 3 * d->nz * d->NC;
+			o->ty[g].h.push_back(q);
+		}
+	for (size_t g = 0; g < o->gz.size(); g++)
+		for (int b : o->gz[g].meshes) {
+			bm_demag_s* d = o->dm[b];
+			PassBatch q{}; q.S = d->S; q.K = d->K; q.nz = d->nz; q.NC = d->NC; q.Ny = d->Ny; q.NYH = d->NYH; q.Sp = d->Sp;
+			o->tz[g].h.push_back(q);
+		}
+	for (size_t g = 0; g < o->gy2.size(); g++)
+		for (int b : o->gy2[g].meshes) {
+			bm_demag_s* d = o->dm[b];
+			PassBatch q{}; q.S = d->S; q.K = d->K; q.NC = d->NC; q.NYH = d->NYH; q.Sp = d->Sp;
+			o->ty2[g].h.push_back(q);
+		}
+	for (int par = 0; par < 2; par++) {
+		o->stage[par].h.clear();
+		for (int b = 0; b < nb; b++) o->stage[par].h.push_back(stage_args(o, b, par));
+	}
+	// one device allocation, 256-byte aligned pieces
+	size_t bytes = 0;
+	auto reserve = [&](size_t n) { size_t at = bytes; bytes += (n + 255) / 256 * 256; return at; };
+	std::vector<std::pair<size_t, std::pair<const void*, size_t>>> pieces;
+	auto place = [&](auto& t) {
+		const size_t n = t.h.size() * sizeof(t.h[0]);
+		const size_t at = reserve(n);
+		pieces.push_back({ at, { (const void*)t.h.data(), n } });
+		return at;
+	};
+	std::vector<size_t> at_fwd[2], at_inv, at_y, at_z, at_y2;
+	for (int par = 0; par < 2; par++) for (auto& t : o->fwd[par]) at_fwd[par].push_back(place(t));
+	for (auto& t : o->inv) at_inv.push_back(place(t));
+	for (auto& t : o->ty) at_y.push_back(place(t));
+	for (auto& t : o->tz) at_z.push_back(place(t));
+	for (auto& t : o->ty2) at_y2.push_back(place(t));
+	const size_t at_s0 = place(o->stage[0]), at_s1 = place(o->stage[1]);
+	if (hipMalloc(&o->dev_tables, std::max<size_t>(bytes, 256)) != hipSuccess) return fail(BM_ERR_NOMEM, "batch tables");
+	o->dev_bytes = bytes;
+	char* base = (char*)o->dev_tables;
+	for (auto& pc : pieces) BM_HIP(hipMemcpy(base + pc.first, pc.second.first, pc.second.second, hipMemcpyHostToDevice));
+	for (int par = 0; par < 2; par++) for (size_t g = 0; g < o->fwd[par].size(); g++) o->fwd[par][g].d = (FwdArgs*)(base + at_fwd[par][g]);
+	for (size_t g = 0; g < o->inv.size(); g++) o->inv[g].d = (PassBatch*)(base + at_inv[g]);
+	for (size_t g = 0; g < o->ty.size(); g++) o->ty[g].d = (PassBatch*)(base + at_y[g]);
+	for (size_t g = 0; g < o->tz.size(); g++) o->tz[g].d = (PassBatch*)(base + at_z[g]);
+	for (size_t g = 0; g < o->ty2.size(); g++) o->ty2[g].d = (PassBatch*)(base + at_y2[g]);
+	o->stage[0].d = (StageArgs*)(base + at_s0);
+	o->stage[1].d = (StageArgs*)(base + at_s1);
+	return BM_OK;
+}
+
+// rewrite dT in the stage tables (the only per-step-size data the kernels read)
+int upload_dt(bm_batch_s* o)
+{
+	for (int par = 0; par < 2; par++) {
+		for (StageArgs& a : o->stage[par].h) a.dT = (float)o->dT;
+		BM_HIP(hipMemcpy(o->stage[par].d, o->stage[par].h.data(), o->stage[par].h.size() * sizeof(StageArgs), hipMemcpyHostToDevice));
+	}
+	return BM_OK;
+}
+
+// One field evaluation + stage `step` for every mesh; input magnetisation M (parity 0) or Mb.
+int eval_all(bm_batch_s* o, int step, int par, hipStream_t st)
+{
+	int rc = BM_OK;
+	for (size_t g = 0; g < o->gx.size() && !rc; g++)
+		rc = batch_fwd_x(o->gx[g].size, o->fwd[par][g].h.data(), o->fwd[par][g].d, (int)o->fwd[par][g].h.size(), o->ncu, st);
+	for (size_t g = 0; g < o->gy.size() && !rc; g++)
+		rc = batch_y(o->gy[g].size, -1, o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st);
+	for (size_t g = 0; g < o->gz.size() && !rc; g++)
+		rc = batch_z(o->gz[g].size, o->tz[g].h.data(), o->tz[g].d, (int)o->tz[g].h.size(), o->ncu, st);
+	for (size_t g = 0; g < o->gy.size() && !rc; g++)
+		rc = batch_y(o->gy[g].size, 1, o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st);
+	for (size_t g = 0; g < o->gy2.size() && !rc; g++)
+		rc = batch_y2d(o->gy2[g].size, o->ty2[g].h.data(), o->ty2[g].d, (int)o->ty2[g].h.size(), o->ncu, st);
+	for (size_t g = 0; g < o->gx.size() && !rc; g++)
+		rc = batch_inv_x_planar(o->gx[g].size, o->inv[g].h.data(), o->inv[g].d, (int)o->inv[g].h.size(), o->ncu, st);
+	if (!rc) rc = batch_stage(o->mcode, step, o->stage[par].d, o->nb, o->max_cells, o->ncu, st);
+	return rc;
+}
+
+int one_step(bm_batch_s* o, hipStream_t st)
+{
+	for (int s = 0; s < o->nstages; s++) {
+		int rc = eval_all(o, s, s & 1, st);
+		if (rc) return rc;
+	}
+	return BM_OK;
+}
+
+} // namespace
+
+extern "C" int bm_batch_create(int method, int nmesh, const int* n3, const double* h3, const bm_fm_params* params,
+	float* M, int flags, bm_batch_t* out)
+{
+	if (!out || nmesh < 1 || !n3 || !h3 || !params || !M) return fail(BM_ERR_ARG, "bm_batch_create: bad arguments");
+	*out = nullptr;
+	bm_batch_s* o = new bm_batch_s();
+	o->nb = nmesh;
+	o->use_graph = (flags & BM_BATCH_NO_GRAPH) ? 0 : 1;
+	switch (method) {
+	case BM_EVAL_RK4: o->mcode = M_RK4; o->nstages = 4; o->dT = 0.5e-12; break;
+	case BM_EVAL_TEULER: o->mcode = M_TEULER; o->nstages = 2; o->dT = 0.5e-13; break;
+	default: delete o; return fail(BM_ERR_ARG, "bm_batch_create: fixed-step methods only (RK4, TEuler)");
+	}
+	o->M = M;
+	o->ncu = device_cu_count();
+	for (int b = 0; b < nmesh; b++) {
+		const int nx = n3[3 * b], ny = n3[3 * b + 1], nz = n3[3 * b + 2];
+		if (nx < 1 || ny < 1 || nz < 1) { delete o; return fail(BM_ERR_ARG, "bm_batch_create: bad mesh size"); }
+		o->nx.push_back(nx); o->ny.push_back(ny); o->nz.push_back(nz);
+		o->off.push_back(o->ntot);
+		o->ncell.push_back((size_t)nx * ny * nz);
+		o->ntot += (size_t)nx * ny * nz;
+		o->max_cells = std::max(o->max_cells, (size_t)nx * ny * nz);
+		for (int c = 0; c < 3; c++) o->h.push_back(h3[3 * b + c]);
+		o->p.push_back(params[b]);
+	}
+	o->dm.assign(nmesh, nullptr);
+	int rc = BM_OK;
+	for (int b = 0; b < nmesh && rc == BM_OK; b++)
+		if (o->p[b].modules & BM_MOD_DEMAG) rc = bm_demag_create(o->nx[b], o->ny[b], o->nz[b], o->h[3 * b], o->h[3 * b + 1], o->h[3 * b + 2], 0, &o->dm[b]);
+	const size_t b3 = o->ntot * 3 * sizeof(float);
+	if (rc == BM_OK) {
+		bool ok = hipMalloc(&o->Mb, b3) == hipSuccess && hipMalloc(&o->sM1, b3) == hipSuccess && hipMalloc(&o->Hd, b3) == hipSuccess;
+		const int nE = o->mcode == M_RK4 ? 3 : 0;
+		for (int q = 0; q < nE && ok; q++) ok = hipMalloc(&o->E[q], b3) == hipSuccess;
+		if (ok) ok = hipMemset(o->Hd, 0, b3) == hipSuccess;
+		if (!ok) rc = fail(BM_ERR_NOMEM, "bm_batch_create: device arrays");
+	}
+	if (rc == BM_OK) rc = build_tables(o);
+	if (rc == BM_OK && o->use_graph) {
+		if (hipStreamCreate(&o->gs) != hipSuccess || hipEventCreateWithFlags(&o->ev_in, hipEventDisableTiming) != hipSuccess
+			|| hipEventCreateWithFlags(&o->ev_out, hipEventDisableTiming) != hipSuccess) rc = fail(BM_ERR_HIP, "bm_batch_create: stream");
+	}
+	if (rc) { free_all(o); delete o; return rc; }
+	*out = o;
+	return BM_OK;
+}
+
+extern "C" int bm_batch_destroy(bm_batch_t o)
+{
+	if (!o) return BM_OK;
+	(void)hipDeviceSynchronize();
+	free_all(o);
+	delete o;
+	return BM_OK;
+}
+
+extern "C" int bm_batch_set_dt(bm_batch_t o, double dT)
+{
+	if (!o || !(dT > 0)) return fail(BM_ERR_ARG, "bm_batch_set_dt: bad arguments");
+	o->dT = dT;
+	return upload_dt(o);
+}
+
+extern "C" int bm_batch_run(bm_batch_t o, long long nsteps, void* stream)
+{
+	if (!o || nsteps < 0) return fail(BM_ERR_ARG, "bm_batch_run: bad arguments");
+	hipStream_t st = (hipStream_t)stream;
+	int rc;
+	if (!o->use_graph) {
+		for (long long k = 0; k < nsteps; k++) {
+			if ((rc = one_step(o, st))) return rc;
+			o->time += o->dT; o->iteration++;
+		}
+		BM_HIP(hipGetLastError());
+		return BM_OK;
+	}
+	if (!o->exec) {
+		// capture one step on the internal stream
+		BM_HIP(hipStreamBeginCapture(o->gs, hipStreamCaptureModeThreadLocal));
+		rc = one_step(o, o->gs);
+		hipGraph_t g = nullptr;
+		hipError_t e = hipStreamEndCapture(o->gs, &g);
+		if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+		if (e != hipSuccess) return hip_fail(e, "bm_batch_run: capture");
+		o->graph = g;
+		BM_HIP(hipGraphInstantiate(&o->exec, o->graph, nullptr, nullptr, 0));
+	}
+	BM_HIP(hipEventRecord(o->ev_in, st));
+	BM_HIP(hipStreamWaitEvent(o->gs, o->ev_in, 0));
+	for (long long k = 0; k < nsteps; k++) {
+		BM_HIP(hipGraphLaunch(o->exec, o->gs));
+		o->time += o->dT; o->iteration++;
+	}
+	BM_HIP(hipEventRecord(o->ev_out, o->gs));
+	BM_HIP(hipStreamWaitEvent(st, o->ev_out, 0));
+	return BM_OK;
+}
+
+extern "C" int bm_batch_get_state(bm_batch_t o, double* time, long long* iteration)
+{
+	if (!o) return fail(BM_ERR_ARG, "null handle");
+	if (time) *time = o->time;
+	if (iteration) *iteration = o->iteration;
+	return BM_OK;
+}
+
+extern "C" int bm_batch_mesh_offset(bm_batch_t o, int mesh, long long* cell_offset, long long* ncells)
+{
+	if (!o || mesh < 0 || mesh >= o->nb) return fail(BM_ERR_ARG, "bm_batch_mesh_offset: bad mesh");
+	if (cell_offset) *cell_offset = (long long)o->off[mesh];
+	if (ncells) *ncells = (long long)o->ncell[mesh];
+	return BM_OK;
+}

@@ -38,18 +38,19 @@ namespace {
 constexpr int kBlock = 256;
 
 // ---------------------------------------------------------------------------------------
+// Batched launches (many meshes of one pass size in one launch): grid row blockIdx.y takes
+// its mesh's pointers and sizes from a device table (FwdArgs / PassBatch, bm_internal.h);
+// the single-mesh launches pass null.
+
+// ---------------------------------------------------------------------------------------
 // P1: x-direction real-to-complex FFT of TY rows of one z plane (all three components, so M
 // is read from HBM once), transposed into S through LDS.  With ADV it also performs the
 // RKDP54 advance on the way: sM1 = M, Mout = M + E0 dT/5, and transforms Mout.
-struct FwdArgs {
-	const float* M; float2* S;
-	int ny, nz, Sp, ntiles;
-	const float* E0; float* sM1; float* Mout; float dT5;
-};
 
 template <int HX, int TY, bool ADV>
-__global__ void __launch_bounds__(kBlock) k_fwd_x(FwdArgs a)
+__global__ void __launch_bounds__(kBlock) k_fwd_x(FwdArgs a, const FwdArgs* __restrict__ bt)
 {
+	if (bt) a = bt[blockIdx.y];     // batched: one mesh per grid row
 	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, RPITCH = xrow_pitch<HX>(), NG = TY / RP;
 	__shared__ C2<float> tile[TY * RPITCH];
 	XThread<HX> th; th.init();
@@ -113,8 +114,10 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x(FwdArgs a)
 // 2 AoS add; EN also forms per-work-item partial sums of M.H for the energy.
 template <int HX, int MODE, bool EN, int TY>
 __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ S, float* __restrict__ out,
-	int ny, int nz, int Sp, int ntiles, float scale, const float* __restrict__ M, double* __restrict__ epart)
+	int ny, int nz, int Sp, int ntiles, float scale, const float* __restrict__ M, double* __restrict__ epart,
+	const PassBatch* __restrict__ bt)
 {
+	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; out = b.out; ny = b.ny; nz = b.nz; Sp = b.Sp; ntiles = b.ntiles; scale = b.scale; }
 	__shared__ double ered[EN ? kBlock : 1];
 	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, RPITCH = xrow_pitch<HX>();
 	__shared__ C2<float> tile[TY * RPITCH];
@@ -171,8 +174,9 @@ __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ S, 
 // ---------------------------------------------------------------------------------------
 // P2 / P4 (3D): y-direction FFT of lines S[c][z][kx][0..Ny).
 template <int NY, int DIR>
-__global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, int nlines, int Sp)
+__global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, int nlines, int Sp, const PassBatch* __restrict__ bt)
 {
+	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; nlines = b.nlines; Sp = b.Sp; }
 	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = kBlock / T, LL = lds_len<NY>();
 	__shared__ C2<float> lds[LP * LL];
 	const int t = threadIdx.x % T, l = threadIdx.x / T;
@@ -217,8 +221,9 @@ __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, int nlines
 // mirrors of every column.
 template <int NZ>
 __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const float* __restrict__ K,
-	int nz, int NC, int Ny, int NYH, int Sp)
+	int nz, int NC, int Ny, int NYH, int Sp, const PassBatch* __restrict__ bt)
 {
+	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; K = b.K; nz = b.nz; NC = b.NC; Ny = b.Ny; NYH = b.NYH; Sp = b.Sp; }
 	constexpr int E = Plan<NZ>::E, TZ = plan_T<NZ>(), CY = kBlock / TZ, CH = CY / 2, NZH = NZ / 2 + 1, KR = CH + 1;
 	constexpr int KS = 6 * KR * NZH, NKL = (KS + kBlock - 1) / kBlock;
 	__shared__ C2<float> lds[3 * NZ * CY];
@@ -304,8 +309,10 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 // ---------------------------------------------------------------------------------------
 // P2' (2D): per kx line, y-FFT of 3 components, 2D kernel multiply (Kxy odd in y), y-IFFT.
 template <int NY, int BS>
-__global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ S, const float* __restrict__ K, int NC, int NYH, int Sp)
+__global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ S, const float* __restrict__ K, int NC, int NYH, int Sp,
+	const PassBatch* __restrict__ bt)
 {
+	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; K = b.K; NC = b.NC; NYH = b.NYH; Sp = b.Sp; }
 	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = BS / T, LL = lds_len<NY>();
 	__shared__ C2<float> lds[3 * LP * LL];
 	const int t = threadIdx.x % T, l = threadIdx.x / T;
@@ -510,10 +517,11 @@ int launch_fwd_x(int HX, const float* M, float2* S, int ny, int nz, int Sp, int 
 	const bool adv = E0 != nullptr;
 	switch (HX) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; int TY = TY1 * f; a.ntiles = (ny + TY - 1) / TY; dim3 g(grid_for(nz * a.ntiles, ncu, 2)); \
-		if (f == 1 && !adv) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false>), g, dim3(kBlock), 0, st, a); \
-		else if (f == 1) hipLaunchKernelGGL((k_fwd_x<V_, TY1, true>), g, dim3(kBlock), 0, st, a); \
-		else if (!adv) hipLaunchKernelGGL((k_fwd_x<V_, 2 * TY1, false>), g, dim3(kBlock), 0, st, a); \
-		else hipLaunchKernelGGL((k_fwd_x<V_, 2 * TY1, true>), g, dim3(kBlock), 0, st, a); return BM_OK; }
+		const FwdArgs* nb = nullptr; \
+		if (f == 1 && !adv) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false>), g, dim3(kBlock), 0, st, a, nb); \
+		else if (f == 1) hipLaunchKernelGGL((k_fwd_x<V_, TY1, true>), g, dim3(kBlock), 0, st, a, nb); \
+		else if (!adv) hipLaunchKernelGGL((k_fwd_x<V_, 2 * TY1, false>), g, dim3(kBlock), 0, st, a, nb); \
+		else hipLaunchKernelGGL((k_fwd_x<V_, 2 * TY1, true>), g, dim3(kBlock), 0, st, a, nb); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -526,12 +534,12 @@ int launch_inv_x(int HX, int mode, const float2* S, float* out, int ny, int nz, 
 	const int f = (mode == 0) ? xtile_factor(false) : 1;
 	switch (HX) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; int TY = TY1 * f, nt = (ny + TY - 1) / TY; dim3 g(grid_for(3 * nz * nt, ncu, 2)); \
-		if (mode == 0 && f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); \
-		else if (mode == 0) hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); \
-		else if (mode == 1 && !epart) hipLaunchKernelGGL((k_inv_x<V_, 1, false, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); \
-		else if (mode == 1) hipLaunchKernelGGL((k_inv_x<V_, 1, true, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); \
-		else if (!epart) hipLaunchKernelGGL((k_inv_x<V_, 2, false, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); \
-		else hipLaunchKernelGGL((k_inv_x<V_, 2, true, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart); return BM_OK; }
+		if (mode == 0 && f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart, (const PassBatch*)nullptr); \
+		else if (mode == 0) hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart, (const PassBatch*)nullptr); \
+		else if (mode == 1 && !epart) hipLaunchKernelGGL((k_inv_x<V_, 1, false, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart, (const PassBatch*)nullptr); \
+		else if (mode == 1) hipLaunchKernelGGL((k_inv_x<V_, 1, true, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart, (const PassBatch*)nullptr); \
+		else if (!epart) hipLaunchKernelGGL((k_inv_x<V_, 2, false, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart, (const PassBatch*)nullptr); \
+		else hipLaunchKernelGGL((k_inv_x<V_, 2, true, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart, (const PassBatch*)nullptr); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -552,8 +560,8 @@ int launch_y(int NY, int dir, float2* S, int nlines, int Sp, int ncu, hipStream_
 {
 	switch (NY) {
 #define C_(V_) case V_: { int LP = kBlock / plan_T<V_>(); dim3 g(grid_for((nlines + LP - 1) / LP, ncu, 8)); \
-		if (dir < 0) hipLaunchKernelGGL((k_y<V_, -1>), g, dim3(kBlock), 0, st, S, nlines, Sp); \
-		else hipLaunchKernelGGL((k_y<V_, 1>), g, dim3(kBlock), 0, st, S, nlines, Sp); return BM_OK; }
+		if (dir < 0) hipLaunchKernelGGL((k_y<V_, -1>), g, dim3(kBlock), 0, st, S, nlines, Sp, (const PassBatch*)nullptr); \
+		else hipLaunchKernelGGL((k_y<V_, 1>), g, dim3(kBlock), 0, st, S, nlines, Sp, (const PassBatch*)nullptr); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -564,7 +572,7 @@ int launch_z(int NZ, float2* S, const float* K, int nz, int NC, int Ny, int NYH,
 {
 	switch (NZ) {
 #define C_(V_) case V_: { int CH = kBlock / plan_T<V_>() / 2; int nw = NC * ((Ny / 2 + CH - 1) / CH); \
-		hipLaunchKernelGGL(k_z<V_>, dim3(grid_for(nw, ncu, 2)), dim3(kBlock), 0, st, S, K, nz, NC, Ny, NYH, Sp); return BM_OK; }
+		hipLaunchKernelGGL(k_z<V_>, dim3(grid_for(nw, ncu, 2)), dim3(kBlock), 0, st, S, K, nz, NC, Ny, NYH, Sp, (const PassBatch*)nullptr); return BM_OK; }
 	BM_FOR_Z_SIZES(C_)
 #undef C_
 	}
@@ -575,7 +583,7 @@ int launch_y2d(int NY, float2* S, const float* K, int NC, int NYH, int Sp, int n
 {
 	switch (NY) {
 #define C_(V_) case V_: { constexpr int BS = Plan<V_>::E > 8 ? 128 : 256; int LP = BS / plan_T<V_>(); \
-		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid_for((NC + LP - 1) / LP, ncu, 8)), dim3(BS), 0, st, S, K, NC, NYH, Sp); return BM_OK; }
+		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid_for((NC + LP - 1) / LP, ncu, 8)), dim3(BS), 0, st, S, K, NC, NYH, Sp, (const PassBatch*)nullptr); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -835,4 +843,129 @@ extern "C" int bm_demag_copy_kernels(bm_demag_t d, float* host_out, long long co
 	if (!d || !host_out || count < d->kcount) return fail(BM_ERR_ARG, "bm_demag_copy_kernels: bad arguments");
 	BM_HIP(hipMemcpy(host_out, d->K, d->kcount * sizeof(float), hipMemcpyDeviceToHost));
 	return BM_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Pass launchers shared with the multilayer convolution (bm_mlayer.hip).
+int bm::pass_fwd_x(int nx, const float* M, float2* S, int ny, int nplanes, int Sp, int ncu, hipStream_t st)
+{
+	prof_mark_begin(PK_FWD_X, st);
+	int rc = launch_fwd_x(nx, M, S, ny, nplanes, Sp, ncu, st);
+	prof_mark_end(PK_FWD_X, st);
+	return rc;
+}
+
+int bm::pass_y(int Ny, int dir, float2* S, int nlines, int Sp, int ncu, hipStream_t st)
+{
+	const int kind = dir < 0 ? PK_Y_FWD : PK_Y_INV;
+	prof_mark_begin(kind, st);
+	int rc = launch_y(Ny, dir, S, nlines, Sp, ncu, st);
+	prof_mark_end(kind, st);
+	return rc;
+}
+
+int bm::pass_inv_x(int nx, int mode, const float2* S, float* out, int ny, int nplanes, int Sp, float scale, int ncu, hipStream_t st)
+{
+	prof_mark_begin(PK_INV_X, st);
+	int rc = launch_inv_x(nx, mode, S, out, ny, nplanes, Sp, scale, ncu, st);
+	prof_mark_end(PK_INV_X, st);
+	return rc;
+}
+
+int bm::setup_line_fft(int N, const double* src, double* dst, int nlines, int na, long long sa, long long sb, long long se,
+	long long da, long long db, long long de, int keep, int part, double sign, hipStream_t st)
+{
+	return launch_line_fft<double>(N, src, dst, nlines, na, sa, sb, se, da, db, de, keep, part, sign, st);
+}
+
+int bm::setup_line_fft(int N, const double* src, float* dst, int nlines, int na, long long sa, long long sb, long long se,
+	long long da, long long db, long long de, int keep, int part, double sign, hipStream_t st)
+{
+	return launch_line_fft<float>(N, src, dst, nlines, na, sa, sb, se, da, db, de, keep, part, sign, st);
+}
+
+bool bm::plan_2d_ok(int nx, int Ny) { return x_supported(nx) && x_supported(Ny) && line_supported(2 * nx); }
+
+// ---------------------------------------------------------------------------------------
+// Batched pass launchers: nb meshes of one pass size; h_tab (host) sizes the grid, d_tab
+// (device, same contents) is read by the kernels.
+int bm::x_tile_rows(int nx, bool fwd)
+{
+	switch (nx) {
+#define C_(V_) case V_: return XTile<V_>::v * xtile_factor(fwd);
+	BM_FOR_X_SIZES(C_)
+#undef C_
+	}
+	return 0;
+}
+
+int bm::batch_fwd_x(int nx, const FwdArgs* h_tab, const FwdArgs* d_tab, int nb, int ncu, hipStream_t st)
+{
+	int mx = 1;
+	for (int b = 0; b < nb; b++) mx = std::max(mx, h_tab[b].nz * h_tab[b].ntiles);
+	const int f = xtile_factor(true);
+	const dim3 g(grid_for(mx, ncu, 2), nb);
+	FwdArgs a{};
+	switch (nx) {
+#define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; \
+		if (f == 1) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false>), g, dim3(kBlock), 0, st, a, d_tab); \
+		else hipLaunchKernelGGL((k_fwd_x<V_, 2 * TY1, false>), g, dim3(kBlock), 0, st, a, d_tab); return BM_OK; }
+	BM_FOR_X_SIZES(C_)
+#undef C_
+	}
+	return fail(BM_ERR_SIZE, "no x plan");
+}
+
+int bm::batch_inv_x_planar(int nx, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+{
+	int mx = 1;
+	for (int b = 0; b < nb; b++) mx = std::max(mx, 3 * h_tab[b].nz * h_tab[b].ntiles);
+	const int f = xtile_factor(false);
+	const dim3 g(grid_for(mx, ncu, 2), nb);
+	switch (nx) {
+#define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; \
+		if (f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, d_tab); \
+		else hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, d_tab); return BM_OK; }
+	BM_FOR_X_SIZES(C_)
+#undef C_
+	}
+	return fail(BM_ERR_SIZE, "no x plan");
+}
+
+int bm::batch_y(int Ny, int dir, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+{
+	switch (Ny) {
+#define C_(V_) case V_: { const int LP = kBlock / plan_T<V_>(); int mx = 1; \
+		for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].nlines + LP - 1) / LP); \
+		const dim3 g(grid_for(mx, ncu, 8), nb); \
+		if (dir < 0) hipLaunchKernelGGL((k_y<V_, -1>), g, dim3(kBlock), 0, st, nullptr, 0, 0, d_tab); \
+		else hipLaunchKernelGGL((k_y<V_, 1>), g, dim3(kBlock), 0, st, nullptr, 0, 0, d_tab); return BM_OK; }
+	BM_FOR_X_SIZES(C_)
+#undef C_
+	}
+	return fail(BM_ERR_SIZE, "no y plan");
+}
+
+int bm::batch_z(int Nz, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+{
+	switch (Nz) {
+#define C_(V_) case V_: { const int CH = kBlock / plan_T<V_>() / 2; int mx = 1; \
+		for (int b = 0; b < nb; b++) mx = std::max(mx, h_tab[b].NC * ((h_tab[b].Ny / 2 + CH - 1) / CH)); \
+		hipLaunchKernelGGL(k_z<V_>, dim3(grid_for(mx, ncu, 2), nb), dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, d_tab); return BM_OK; }
+	BM_FOR_Z_SIZES(C_)
+#undef C_
+	}
+	return fail(BM_ERR_SIZE, "no z plan");
+}
+
+int bm::batch_y2d(int Ny, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+{
+	switch (Ny) {
+#define C_(V_) case V_: { constexpr int BS = Plan<V_>::E > 8 ? 128 : 256; const int LP = BS / plan_T<V_>(); int mx = 1; \
+		for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].NC + LP - 1) / LP); \
+		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid_for(mx, ncu, 8), nb), dim3(BS), 0, st, nullptr, nullptr, 0, 0, 0, d_tab); return BM_OK; }
+	BM_FOR_X_SIZES(C_)
+#undef C_
+	}
+	return fail(BM_ERR_SIZE, "no y plan");
 }

@@ -18,7 +18,7 @@ int device_cu_count();
 
 // Launch timing: when enabled (bm_prof_begin), every kernel launch of the hot path is
 // bracketed by HIP events on its own stream and tagged with its kind.
-enum ProfKind { PK_FWD_X = 0, PK_Y_FWD, PK_Z, PK_Y_INV, PK_INV_X, PK_Y2D, PK_STAGE, PK_ADVANCE, PK_ROW, PK_FWD_ADV, PK_COUNT };
+enum ProfKind { PK_FWD_X = 0, PK_Y_FWD, PK_Z, PK_Y_INV, PK_INV_X, PK_Y2D, PK_STAGE, PK_ADVANCE, PK_ROW, PK_FWD_ADV, PK_ML_MULT, PK_COUNT };
 void prof_mark_begin(int kind, hipStream_t st);
 void prof_mark_end(int kind, hipStream_t st);
 
@@ -46,4 +46,38 @@ int demag_planar(bm_demag_s* d, const float* M, float* H3n, hipStream_t st);
 int demag_fwd_x(bm_demag_s* d, const float* M, hipStream_t st);
 int demag_fwd_x_advance(bm_demag_s* d, const float* M, const float* E0, float* sM1, float* Mout, float dT5, hipStream_t st);
 int demag_mid(bm_demag_s* d, hipStream_t st);
+// single passes over "planes" of ny rows (a z plane or a layer) for other pipelines
+int pass_fwd_x(int nx, const float* M, float2* S, int ny, int nplanes, int Sp, int ncu, hipStream_t st);
+int pass_y(int Ny, int dir, float2* S, int nlines, int Sp, int ncu, hipStream_t st);
+int pass_inv_x(int nx, int mode, const float2* S, float* out, int ny, int nplanes, int Sp, float scale, int ncu, hipStream_t st);
+// FP64 real-input line FFTs of the kernel setup (see k_line_fft)
+int setup_line_fft(int N, const double* src, double* dst, int nlines, int na, long long sa, long long sb, long long se,
+	long long da, long long db, long long de, int keep, int part, double sign, hipStream_t st);
+int setup_line_fft(int N, const double* src, float* dst, int nlines, int na, long long sa, long long sb, long long se,
+	long long da, long long db, long long de, int keep, int part, double sign, hipStream_t st);
+// are the x pass (nx), y pass (Ny) and setup x FFT (2 nx) plans available?
+bool plan_2d_ok(int nx, int Ny);
+
+// Per-mesh arguments of the x R2C pass (also its single-mesh launch arguments); with
+// E0 set it performs the RKDP54 advance on the way (sM1 = M, Mout = M + E0 dT/5).
+struct FwdArgs {
+	const float* M; float2* S;
+	int ny, nz, Sp, ntiles;
+	const float* E0; float* sM1; float* Mout; float dT5;
+};
+// Per-mesh arguments of the other passes in a batched launch (the fields each pass uses).
+struct PassBatch {
+	float2* S; const float* K; float* out;
+	int ny, nz, Sp, ntiles, nlines, NC, Ny, NYH;
+	float scale;            // x C2R: 1 / (Nx Ny Nz) of the mesh
+};
+// rows per work item of the x passes for nx (0 if there is no plan)
+int x_tile_rows(int nx, bool fwd);
+// Batched passes over nb meshes sharing the pass size (tables: host copy for the grid size,
+// device copy for the kernels).
+int batch_fwd_x(int nx, const FwdArgs* h_tab, const FwdArgs* d_tab, int nb, int ncu, hipStream_t st);
+int batch_inv_x_planar(int nx, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
+int batch_y(int Ny, int dir, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
+int batch_z(int Nz, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
+int batch_y2d(int Ny, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
 }

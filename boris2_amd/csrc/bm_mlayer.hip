@@ -1,0 +1,418 @@
+// Multilayered demag of stacked 2D layers on MI355X: L layers of the same nx x ny cells
+// (cell sizes hx, hy, hz[l]), each layer's bottom face at z0[l];
+//   H_d = IFFT( sum_s K_ds . FFT(M_s) ) / (Nx Ny),  N = (2 nx, 2 ny).
+//
+// Reference behaviour: SDemagCUDA::UpdateField with multilayered convolution
+// (Boris/SDemagCUDA.cpp:270) over DemagKernelCollectionCUDA (kernels
+// Boris/DemagKernelCollectionCUDA_Calc*.cpp, product Boris/DemagKernelCollectionCUDA_Mult.cu:610)
+// for 2D layers related by z shifts, the configuration SDemag builds for stacked layers of
+// equal extent (Boris/SDemag.cpp:173-300).  Kernels: the layer's own 2D kernel for d == s;
+// otherwise the z-shifted tensor of Boris/DemagTFunc_Shifted.cpp (equal cells) or
+// Boris/DemagTFunc_Irregular.cpp (thicknesses differ) turned into six real kernels
+// (xx, yy, zz, xy real; xz, yz the coefficients of i), Boris/DemagKernelCollection_Calc.cpp:288-420.
+//
+// Pipeline per evaluation (all layers at once, layers in place of z planes of the spectra
+// S[c][l][kx][ky]): x R2C pass, y FFT pass, one pass that forms every destination's
+// kernel sum from all layers' spectra at a (kx, ky) point in registers, y IFFT pass, x C2R
+// pass.  The x and y passes are the single-mesh convolution's own.
+#include "bm_internal.h"
+#include "bm_newell.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+using namespace bm;
+
+constexpr int kMaxLayers = 8;
+
+// Position of grid index i (-1, 0, +1, ...) along one axis: i*h + o, or a table for -1, 0, +1.
+struct Axis { double h, o, p[3]; int table; };
+__host__ __device__ inline double axis_pos(const Axis& a, int i) { return a.table ? a.p[i + 1] : i * a.h + a.o; }
+Axis lin(double h, double o) { Axis a; a.h = h; a.o = o; a.p[0] = a.p[1] = a.p[2] = 0; a.table = 0; return a; }
+Axis tab(double m, double z, double p) { Axis a; a.h = a.o = 0; a.p[0] = m; a.p[1] = z; a.p[2] = p; a.table = 1; return a; }
+
+// Basis-function grid over i in [-1, dx], j in [-1, dy], k in [-1, dz] (stored with +1, x
+// fastest); dz = -1 gives the single plane k = -1.
+__global__ void k_fill_grid_at(double* g, int which, int dx, int dy, int dz, Axis X, Axis Y, Axis Z)
+{
+	const int gx = dx + 2, gy = dy + 2, gz = dz + 2;
+	const size_t total = (size_t)gx * gy * gz;
+	for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+		const int i = (int)(q % gx) - 1, j = (int)((q / gx) % gy) - 1, k = (int)(q / ((size_t)gx * gy)) - 1;
+		const double x = axis_pos(X, i), y = axis_pos(Y, j), z = axis_pos(Z, k);
+		g[q] = which ? newell_g(x, y, z) : newell_f(x, y, z);
+	}
+}
+
+// 36-term second differences for cells whose source and destination thicknesses differ
+// (Boris/DemagTFunc_L.cpp:596-830): g holds three planes, del one extra plane; tau is the
+// destination cell volume.  Term order as the reference's (compensated sum).
+struct G3 {
+	const double* v; int gx, gy;
+	__device__ double operator()(int a, int b, int c) const { return v[(size_t)a + (size_t)b * gx + (size_t)c * gx * gy]; }
+};
+
+// xx, yy, xy: the plane axis is the third; evaluated at (i, j, 0)
+__device__ double L36_plane3(const G3& G, const G3& Dl, int i, int j, double tau)
+{
+	const int k = 1; i++; j++;
+	auto D = [&](int a, int b) { return Dl(a, b, 0); };
+	double t[36] = {
+		+4 * G(i, j, k), +4 * D(i, j),
+		-2 * G(i + 1, j, k), -2 * D(i + 1, j), -2 * G(i - 1, j, k), -2 * D(i - 1, j),
+		-2 * G(i, j + 1, k), -2 * D(i, j + 1), -2 * G(i, j - 1, k), -2 * D(i, j - 1),
+		-4 * G(i, j, k + 1), -4 * G(i, j, k - 1),
+		+G(i + 1, j + 1, k), +D(i + 1, j + 1), +G(i + 1, j - 1, k), +D(i + 1, j - 1),
+		+G(i - 1, j + 1, k), +D(i - 1, j + 1), +G(i - 1, j - 1, k), +D(i - 1, j - 1),
+		+2 * G(i + 1, j, k + 1), +2 * G(i + 1, j, k - 1), +2 * G(i - 1, j, k + 1), +2 * G(i - 1, j, k - 1),
+		+2 * G(i, j + 1, k + 1), +2 * G(i, j + 1, k - 1), +2 * G(i, j - 1, k + 1), +2 * G(i, j - 1, k - 1),
+		-G(i - 1, j - 1, k - 1), -G(i - 1, j - 1, k + 1), -G(i - 1, j + 1, k - 1), -G(i + 1, j - 1, k - 1),
+		-G(i - 1, j + 1, k + 1), -G(i + 1, j - 1, k + 1), -G(i + 1, j + 1, k - 1), -G(i + 1, j + 1, k + 1) };
+	return comp_sum(t, 36) / (4 * kPi * tau);
+}
+
+// zz: the plane axis is the first; evaluated at (0, j, i)
+__device__ double L36_zz(const G3& G, const G3& Dl, int j, int k, double tau)
+{
+	const int i = 1; j++; k++;
+	auto D = [&](int b, int c) { return Dl(0, b, c); };
+	double t[36] = {
+		+4 * G(i, j, k), +4 * D(j, k),
+		-4 * G(i + 1, j, k), -4 * G(i - 1, j, k),
+		-2 * G(i, j, k + 1), -2 * D(j, k + 1), -2 * G(i, j, k - 1), -2 * D(j, k - 1),
+		-2 * G(i, j + 1, k), -2 * D(j + 1, k), -2 * G(i, j - 1, k), -2 * D(j - 1, k),
+		+2 * G(i + 1, j + 1, k), +2 * G(i + 1, j - 1, k), +2 * G(i - 1, j + 1, k), +2 * G(i - 1, j - 1, k),
+		+2 * G(i + 1, j, k + 1), +2 * G(i + 1, j, k - 1), +2 * G(i - 1, j, k + 1), +2 * G(i - 1, j, k - 1),
+		+G(i, j + 1, k + 1), +D(j + 1, k + 1), +G(i, j + 1, k - 1), +D(j + 1, k - 1),
+		+G(i, j - 1, k + 1), +D(j - 1, k + 1), +G(i, j - 1, k - 1), +D(j - 1, k - 1),
+		-G(i - 1, j - 1, k - 1), -G(i - 1, j - 1, k + 1), -G(i - 1, j + 1, k - 1), -G(i + 1, j - 1, k - 1),
+		-G(i - 1, j + 1, k + 1), -G(i + 1, j - 1, k + 1), -G(i + 1, j + 1, k - 1), -G(i + 1, j + 1, k + 1) };
+	return comp_sum(t, 36) / (4 * kPi * tau);
+}
+
+// xz, yz: the plane axis is the second; evaluated at (i, 0, k)
+__device__ double L36_plane2(const G3& G, const G3& Dl, int i, int k, double tau)
+{
+	const int j = 1; i++; k++;
+	auto D = [&](int a, int c) { return Dl(a, 0, c); };
+	double t[36] = {
+		+4 * G(i, j, k), +4 * D(i, k),
+		-2 * G(i + 1, j, k), -2 * D(i + 1, k), -2 * G(i - 1, j, k), -2 * D(i - 1, k),
+		-4 * G(i, j + 1, k), -4 * G(i, j - 1, k),
+		-2 * G(i, j, k + 1), -2 * D(i, k + 1), -2 * G(i, j, k - 1), -2 * D(i, k - 1),
+		+2 * G(i + 1, j + 1, k), +2 * G(i + 1, j - 1, k), +2 * G(i - 1, j + 1, k), +2 * G(i - 1, j - 1, k),
+		+G(i + 1, j, k + 1), +D(i + 1, k + 1), +G(i + 1, j, k - 1), +D(i + 1, k - 1),
+		+G(i - 1, j, k + 1), +D(i - 1, k + 1), +G(i - 1, j, k - 1), +D(i - 1, k - 1),
+		+2 * G(i, j + 1, k + 1), +2 * G(i, j + 1, k - 1), +2 * G(i, j - 1, k + 1), +2 * G(i, j - 1, k - 1),
+		-G(i - 1, j - 1, k - 1), -G(i - 1, j - 1, k + 1), -G(i - 1, j + 1, k - 1), -G(i + 1, j - 1, k - 1),
+		-G(i - 1, j + 1, k + 1), -G(i + 1, j - 1, k + 1), -G(i + 1, j + 1, k - 1), -G(i + 1, j + 1, k + 1) };
+	return comp_sum(t, 36) / (4 * kPi * tau);
+}
+
+struct ZArgs {
+	int nx, ny, Nx, Ny, regular;
+	double hx, hy, hz, shz, ks, tau, far;
+	G3 g[6], del[6];          // f_xx, f_yy, f_zz, g_xy, g_xz, g_yz (+ their extra planes)
+	AsymDiag ad[3];
+	AsymOff ao[3];
+};
+
+// z-shifted 2D tensor (-N convention) into the 6 planes D[q][Ny][Nx] (xx, yy, zz, xy, xz,
+// yz): near field by the 27-term (equal cells) or 36-term (irregular) stencils, far field
+// (distance >= 40 cells) by the equal-cell series of the destination cell scaled by the
+// source thickness for irregular pairs, as the reference does; xy plane mirror images
+// with the reference's write order (Boris/DemagTFunc_Shifted.cpp:235-449,
+// Boris/DemagTFunc_Irregular.cpp:5-235).
+__global__ void k_tensor_zshift(ZArgs a, double* D)
+{
+	const size_t NN = (size_t)a.Nx * a.Ny;
+	const int A = kAsymDist;
+	const int mir[4][2] = { {1,1},{-1,1},{1,-1},{-1,-1} };
+	for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < (size_t)a.nx * a.ny; q += (size_t)gridDim.x * blockDim.x) {
+		const int i = (int)(q % a.nx), j = (int)(q / a.nx);
+		const double hx = a.hx, hy = a.hy, hz = a.hz, sz = a.shz;
+		double vd[3], vo[3];
+		if (i >= A || j >= A || (int)floor(a.ks + 1e-4) >= A || (int)floor(i * i + j * j + a.ks * a.ks + 1e-4) >= A * A) {
+			vd[0] = (-asym_diag(a.ad[0], i * hx, j * hy, sz)) * a.far;
+			vd[1] = (-asym_diag(a.ad[1], j * hy, i * hx, sz)) * a.far;
+			vd[2] = (-asym_diag(a.ad[2], sz, j * hy, i * hx)) * a.far;
+			vo[0] = (-asym_off(a.ao[0], i * hx, j * hy, sz)) * a.far;
+			vo[1] = (-asym_off(a.ao[1], i * hx, sz, j * hy)) * a.far;
+			vo[2] = (-asym_off(a.ao[2], j * hy, sz, i * hx)) * a.far;
+		}
+		else if (a.regular) {
+			vd[0] = -stencil27(a.g[0].v, a.g[0].gx, a.g[0].gy, i, j, 0, hx, hy, hz);
+			vd[1] = -stencil27(a.g[1].v, a.g[1].gx, a.g[1].gy, j, i, 0, hy, hx, hz);
+			vd[2] = -stencil27(a.g[2].v, a.g[2].gx, a.g[2].gy, 0, j, i, hz, hy, hx);
+			vo[0] = -stencil27(a.g[3].v, a.g[3].gx, a.g[3].gy, i, j, 0, hx, hy, hz);
+			vo[1] = -stencil27(a.g[4].v, a.g[4].gx, a.g[4].gy, i, 0, j, hx, hz, hy);
+			vo[2] = -stencil27(a.g[5].v, a.g[5].gx, a.g[5].gy, j, 0, i, hy, hz, hx);
+		}
+		else {
+			vd[0] = -L36_plane3(a.g[0], a.del[0], i, j, a.tau);
+			vd[1] = -L36_plane3(a.g[1], a.del[1], j, i, a.tau);
+			vd[2] = -L36_zz(a.g[2], a.del[2], j, i, a.tau);
+			vo[0] = -L36_plane3(a.g[3], a.del[3], i, j, a.tau);
+			vo[1] = -L36_plane2(a.g[4], a.del[4], i, j, a.tau);
+			vo[2] = -L36_plane2(a.g[5], a.del[5], j, i, a.tau);
+		}
+		for (int m = 0; m < 4; m++) {
+			bool later = false;
+			for (int p = m + 1; p < 4; p++)
+				if ((mir[p][0] == mir[m][0] || i == 0) && (mir[p][1] == mir[m][1] || j == 0)) later = true;
+			if (later) continue;
+			const int si = mir[m][0], sj = mir[m][1];
+			const size_t idx = (size_t)((si * i + a.Nx) % a.Nx) + (size_t)((sj * j + a.Ny) % a.Ny) * a.Nx;
+			D[idx] = vd[0]; D[NN + idx] = vd[1]; D[2 * NN + idx] = vd[2];
+			D[3 * NN + idx] = vo[0] * (si * sj);
+			D[4 * NN + idx] = vo[1] * si;
+			D[5 * NN + idx] = vo[2] * sj;
+		}
+	}
+}
+
+// Kernel sums of all destinations at one (kx, ky) point:
+//   out_d = sum_s [Kxx Fx + Kxy Fy + i Kxz Fz,  Kxy Fx + Kyy Fy + i Kyz Fz,  i Kxz Fx + i Kyz Fy + Kzz Fz]_ds
+// with the kernels of (kx, |ky| folded to <= Ny/2) and Kxy, Kyz negated above Ny/2 (odd in
+// y); the own layer first, then the others in index order (Boris/DemagKernelCollection_Mult.cpp:457-500).
+// Self kernels have Kxz = Kyz = 0.  In place: all layers' inputs are read before any output.
+template <int L>
+__global__ void __launch_bounds__(256) k_ml_mult(float2* __restrict__ S, const float* __restrict__ K, int NC, int Ny, int Sp)
+{
+	const int NYH = Ny / 2 + 1;
+	const size_t kq = (size_t)NC * NYH;
+	const int npts = NC * Ny;
+	for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < npts; q += gridDim.x * blockDim.x) {
+		const int ky = q % Ny, kx = q / Ny;
+		const bool up = ky > Ny / 2;
+		const int jj = up ? Ny - ky : ky;
+		const float sy = up ? -1.f : 1.f;
+		float2 F[L][3];
+#pragma unroll
+		for (int s = 0; s < L; s++)
+#pragma unroll
+			for (int c = 0; c < 3; c++) F[s][c] = S[((size_t)(c * L + s) * NC + kx) * Sp + ky];
+#pragma unroll
+		for (int d = 0; d < L; d++) {
+			float2 o0 = make_float2(0.f, 0.f), o1 = o0, o2 = o0;
+#pragma unroll
+			for (int r = 0; r < L; r++) {
+				const int s = r == 0 ? d : (r <= d ? r - 1 : r);
+				const float* Kp = K + (size_t)(d * L + s) * 6 * kq + (size_t)kx * NYH + jj;
+				const float kxx = Kp[0], kyy = Kp[kq], kzz = Kp[2 * kq];
+				const float kxy = sy * Kp[3 * kq], kxz = Kp[4 * kq], kyz = sy * Kp[5 * kq];
+				const float2 fx = F[s][0], fy = F[s][1], fz = F[s][2];
+				// i k z = k * (-z.y, z.x)
+				o0.x += kxx * fx.x + kxy * fy.x - kxz * fz.y;  o0.y += kxx * fx.y + kxy * fy.y + kxz * fz.x;
+				o1.x += kxy * fx.x + kyy * fy.x - kyz * fz.y;  o1.y += kxy * fx.y + kyy * fy.y + kyz * fz.x;
+				o2.x += -kxz * fx.y - kyz * fy.y + kzz * fz.x; o2.y += kxz * fx.x + kyz * fy.x + kzz * fz.y;
+			}
+			S[((size_t)(0 * L + d) * NC + kx) * Sp + ky] = o0;
+			S[((size_t)(1 * L + d) * NC + kx) * Sp + ky] = o1;
+			S[((size_t)(2 * L + d) * NC + kx) * Sp + ky] = o2;
+		}
+	}
+}
+
+int launch_ml_mult(int L, float2* S, const float* K, int NC, int Ny, int Sp, int ncu, hipStream_t st)
+{
+	const int npts = NC * Ny;
+	dim3 g((unsigned)std::max(1, std::min((npts + 255) / 256, ncu * 8))), b(256);
+	switch (L) {
+#define C_(V_) case V_: hipLaunchKernelGGL((k_ml_mult<V_>), g, b, 0, st, S, K, NC, Ny, Sp); return BM_OK;
+	C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8)
+#undef C_
+	}
+	return fail(BM_ERR_SIZE, "multilayer demag: at most 8 layers");
+}
+
+} // namespace
+
+struct bm_mlayer_s {
+	int L, nx, ny, Nx, Ny, NC, NYH, Sp;
+	double hx, hy;
+	std::vector<double> hz, z0;
+	float2* S = nullptr;     // spectra: 3 * L * NC * Ny complex
+	float* K = nullptr;      // kernels: L*L pairs x 6 planes x NC x (Ny/2+1), ky fastest
+	long long kcount = 0, bytes = 0;
+	int ncu = 256;
+};
+
+namespace {
+
+// z-shifted kernels of pair (d, s) into Kp (6 planes of NC*NYH floats).
+int zshift_kernels(bm_mlayer_s* m, int d, int s, double hmax, double* D, double* T1, double* grids[12], float* Kp, hipStream_t st)
+{
+	const int nx = m->nx, ny = m->ny, Nx = m->Nx, Ny = m->Ny, NC = m->NC, NYH = m->NYH;
+	const double sc[3] = { m->hx / hmax, m->hy / hmax, m->hz[s] / hmax }, dc[3] = { m->hx / hmax, m->hy / hmax, m->hz[d] / hmax };
+	const double shz = (m->z0[d] - m->z0[s]) / hmax;
+	bool regular = true;
+	for (int c = 0; c < 3; c++) if (nzero(sc[c] * 1e-9 - dc[c] * 1e-9)) regular = false;
+	// the regular path runs on the source cell (Boris/DemagTFunc_Irregular.cpp:11)
+	const double hx = regular ? sc[0] : dc[0], hy = regular ? sc[1] : dc[1], hz = regular ? sc[2] : dc[2];
+	const int dx = std::min(nx, kAsymDist), dy = std::min(ny, kAsymDist);
+	struct Spec { int which, gx, gy, gz; Axis X, Y, Z; };
+	Spec sp[12];
+	if (regular) {
+		sp[0] = { 0, dx, dy, 1, lin(hx, 0), lin(hy, 0), lin(hz, shz) };
+		sp[1] = { 0, dy, dx, 1, lin(hy, 0), lin(hx, 0), lin(hz, shz) };
+		sp[2] = { 0, 1, dy, dx, lin(hz, shz), lin(hy, 0), lin(hx, 0) };
+		sp[3] = { 1, dx, dy, 1, lin(hx, 0), lin(hy, 0), lin(hz, shz) };
+		sp[4] = { 1, dx, 1, dy, lin(hx, 0), lin(hz, shz), lin(hy, 0) };
+		sp[5] = { 1, dy, 1, dx, lin(hy, 0), lin(hz, shz), lin(hx, 0) };
+	}
+	else {
+		// planes -1, 0, +1 at -s.z, 0, +d.z (xx, yy) and -d.z, 0, +s.z (zz and g), plus the
+		// extra plane at -del (xx, yy) or +del (zz and g), all offset by the shift
+		// (Boris/DemagTFunc_fg.cpp:507-620)
+		const double del = sc[2] - dc[2];
+		const Axis p_sd = tab(-sc[2] + shz, shz, dc[2] + shz), p_ds = tab(-dc[2] + shz, shz, sc[2] + shz);
+		const Axis c_m = tab(-del + shz, 0, 0), c_p = tab(del + shz, 0, 0);
+		sp[0] = { 0, dx, dy, 1, lin(hx, 0), lin(hy, 0), p_sd };
+		sp[1] = { 0, dy, dx, 1, lin(hy, 0), lin(hx, 0), p_sd };
+		sp[2] = { 0, 1, dy, dx, p_ds, lin(hy, 0), lin(hx, 0) };
+		sp[3] = { 1, dx, dy, 1, lin(hx, 0), lin(hy, 0), p_ds };
+		sp[4] = { 1, dx, 1, dy, lin(hx, 0), p_ds, lin(hy, 0) };
+		sp[5] = { 1, dy, 1, dx, lin(hy, 0), p_ds, lin(hx, 0) };
+		sp[6] = { 0, dx, dy, -1, lin(hx, 0), lin(hy, 0), c_m };
+		sp[7] = { 0, dy, dx, -1, lin(hy, 0), lin(hx, 0), c_m };
+		sp[8] = { 0, -1, dy, dx, c_p, lin(hy, 0), lin(hx, 0) };
+		sp[9] = { 1, dx, dy, -1, lin(hx, 0), lin(hy, 0), c_p };
+		sp[10] = { 1, dx, -1, dy, lin(hx, 0), c_p, lin(hy, 0) };
+		sp[11] = { 1, dy, -1, dx, lin(hy, 0), c_p, lin(hx, 0) };
+	}
+	ZArgs a;
+	std::memset(&a, 0, sizeof(a));
+	a.nx = nx; a.ny = ny; a.Nx = Nx; a.Ny = Ny; a.regular = regular ? 1 : 0;
+	a.hx = hx; a.hy = hy; a.hz = hz; a.shz = shz; a.ks = std::fabs(shz / hz); a.tau = hx * hy * hz;
+	a.far = regular ? 1.0 : sc[2];
+	const int ng = regular ? 6 : 12;
+	for (int q = 0; q < ng; q++) {
+		const size_t cnt = (size_t)(sp[q].gx + 2) * (sp[q].gy + 2) * (sp[q].gz + 2);
+		hipLaunchKernelGGL(k_fill_grid_at, dim3((unsigned)std::min<size_t>((cnt + 255) / 256, 4096)), dim3(256), 0, st,
+			grids[q], sp[q].which, sp[q].gx, sp[q].gy, sp[q].gz, sp[q].X, sp[q].Y, sp[q].Z);
+		G3& G = q < 6 ? a.g[q] : a.del[q - 6];
+		G.v = grids[q]; G.gx = sp[q].gx + 2; G.gy = sp[q].gy + 2;
+	}
+	asym_diag_init(a.ad[0], hx, hy, hz); asym_diag_init(a.ad[1], hy, hx, hz); asym_diag_init(a.ad[2], hz, hy, hx);
+	asym_off_init(a.ao[0], hx, hy, hz); asym_off_init(a.ao[1], hx, hz, hy); asym_off_init(a.ao[2], hy, hz, hx);
+	const size_t NN = (size_t)Nx * Ny;
+	BM_HIP(hipMemsetAsync(D, 0, NN * 6 * sizeof(double), st));
+	hipLaunchKernelGGL(k_tensor_zshift, dim3((unsigned)std::min<size_t>(((size_t)nx * ny + 127) / 128, 65535)), dim3(128), 0, st, a, D);
+	// x then y real-input FFTs with the part each component keeps
+	// (Boris/DemagKernelCollection_Calc.cpp:288-420): diagonal Re, Re; xy Im, -Im; xz Im, Re; yz Re, Im
+	static const int px[6] = { 0, 0, 0, 1, 1, 0 }, py[6] = { 0, 0, 0, 1, 0, 1 };
+	static const double sg[6] = { 1, 1, 1, -1, 1, 1 };
+	for (int q = 0; q < 6; q++) {
+		int rc = setup_line_fft(Nx, D + q * NN, T1, Ny, Ny, Nx, 0, 1, NC, 0, 1, NC, px[q], 1.0, st);
+		if (rc == BM_OK) rc = setup_line_fft(Ny, T1, Kp + (size_t)q * NC * NYH, NC, NC, 1, 0, NC, NYH, 0, 1, NYH, py[q], sg[q], st);
+		if (rc) return rc;
+	}
+	return BM_OK;
+}
+
+int compute_ml_kernels(bm_mlayer_s* m, hipStream_t st)
+{
+	const int L = m->L, Nx = m->Nx, Ny = m->Ny, NC = m->NC, NYH = m->NYH;
+	const size_t kq = (size_t)NC * NYH;
+	double hmax = 0;
+	for (int l = 0; l < L; l++) hmax = std::max(hmax, std::max(m->hx, std::max(m->hy, m->hz[l])));
+	BM_HIP(hipMemsetAsync(m->K, 0, (size_t)m->kcount * sizeof(float), st));
+	// own-layer kernels: the single-layer 2D kernels (xx, yy, zz, xy)
+	for (int l = 0; l < L; l++) {
+		bm_demag_t dl = nullptr;
+		int rc = bm_demag_create(m->nx, m->ny, 1, m->hx, m->hy, m->hz[l], 0, &dl);
+		if (rc) return rc;
+		float* dst = m->K + (size_t)(l * L + l) * 6 * kq;
+		hipError_t e = hipMemcpyAsync(dst, dl->K, 4 * kq * sizeof(float), hipMemcpyDeviceToDevice, st);
+		if (e == hipSuccess) e = hipStreamSynchronize(st);
+		bm_demag_destroy(dl);
+		if (e != hipSuccess) return hip_fail(e, "multilayer self kernels");
+	}
+	if (L == 1) return BM_OK;
+	const int dx = std::min(m->nx, kAsymDist), dy = std::min(m->ny, kAsymDist);
+	const size_t gmax = (size_t)(std::max(dx, dy) + 2) * (std::max(dx, dy) + 2) * 3;
+	double* D = nullptr; double* T1 = nullptr; double* grids[12] = { nullptr };
+	int rc = BM_OK;
+	bool ok = hipMalloc(&D, (size_t)Nx * Ny * 6 * sizeof(double)) == hipSuccess && hipMalloc(&T1, (size_t)NC * Ny * sizeof(double)) == hipSuccess;
+	for (int q = 0; q < 12 && ok; q++) ok = hipMalloc(&grids[q], gmax * sizeof(double)) == hipSuccess;
+	if (!ok) rc = fail(BM_ERR_NOMEM, "multilayer kernel setup");
+	for (int d = 0; d < L && rc == BM_OK; d++)
+		for (int s = 0; s < L && rc == BM_OK; s++)
+			if (d != s) rc = zshift_kernels(m, d, s, hmax, D, T1, grids, m->K + (size_t)(d * L + s) * 6 * kq, st);
+	if (rc == BM_OK) { hipError_t e = hipStreamSynchronize(st); if (e != hipSuccess) rc = hip_fail(e, "multilayer kernel setup"); }
+	(void)hipStreamSynchronize(st);
+	if (D) (void)hipFree(D);
+	if (T1) (void)hipFree(T1);
+	for (int q = 0; q < 12; q++) if (grids[q]) (void)hipFree(grids[q]);
+	return rc;
+}
+
+} // namespace
+
+using bm::fail;
+
+extern "C" int bm_mlayer_create(int nlayers, int nx, int ny, double hx, double hy, const double* hz, const double* z0, int flags, bm_mlayer_t* out)
+{
+	(void)flags;
+	if (!out || !hz || !z0 || nlayers < 1 || nx < 2 || ny < 1 || !(hx > 0) || !(hy > 0)) return fail(BM_ERR_ARG, "bm_mlayer_create: bad arguments");
+	*out = nullptr;
+	if (nlayers > kMaxLayers) return fail(BM_ERR_SIZE, "bm_mlayer_create: at most 8 layers");
+	for (int l = 0; l < nlayers; l++) if (!(hz[l] > 0)) return fail(BM_ERR_ARG, "bm_mlayer_create: layer thickness must be positive");
+	if (nx % 2 || !bm::plan_2d_ok(nx, 2 * ny)) return fail(BM_ERR_SIZE, "bm_mlayer_create: no FFT plan for this layer size");
+	bm_mlayer_s* m = new bm_mlayer_s();
+	m->L = nlayers; m->nx = nx; m->ny = ny; m->Nx = 2 * nx; m->Ny = 2 * ny; m->NC = nx + 1; m->NYH = ny + 1; m->Sp = m->Ny;
+	m->hx = hx; m->hy = hy; m->hz.assign(hz, hz + nlayers); m->z0.assign(z0, z0 + nlayers);
+	m->ncu = bm::device_cu_count();
+	const size_t sbytes = (size_t)3 * nlayers * m->NC * m->Sp * sizeof(float2);
+	m->kcount = (long long)nlayers * nlayers * 6 * m->NC * m->NYH;
+	if (hipMalloc(&m->S, sbytes) != hipSuccess || hipMalloc(&m->K, (size_t)m->kcount * sizeof(float)) != hipSuccess) {
+		bm_mlayer_destroy(m);
+		return fail(BM_ERR_NOMEM, "bm_mlayer_create: device memory");
+	}
+	m->bytes = (long long)sbytes + m->kcount * 4;
+	int rc = compute_ml_kernels(m, nullptr);
+	if (rc) { bm_mlayer_destroy(m); return rc; }
+	*out = m;
+	return BM_OK;
+}
+
+extern "C" int bm_mlayer_destroy(bm_mlayer_t m)
+{
+	if (!m) return BM_OK;
+	if (m->S) (void)hipFree(m->S);
+	if (m->K) (void)hipFree(m->K);
+	delete m;
+	return BM_OK;
+}
+
+extern "C" int bm_mlayer_convolute(bm_mlayer_t m, const float* M, float* Heff, int clear_out, void* stream)
+{
+	if (!m || !M || !Heff) return fail(BM_ERR_ARG, "bm_mlayer_convolute: null argument");
+	hipStream_t st = (hipStream_t)stream;
+	const int L = m->L, nlines = 3 * L * m->NC;
+	int rc = bm::pass_fwd_x(m->nx, M, m->S, m->ny, L, m->Sp, m->ncu, st);
+	if (rc == BM_OK) rc = bm::pass_y(m->Ny, -1, m->S, nlines, m->Sp, m->ncu, st);
+	if (rc == BM_OK) {
+		bm::prof_mark_begin(bm::PK_ML_MULT, st);
+		rc = launch_ml_mult(L, m->S, m->K, m->NC, m->Ny, m->Sp, m->ncu, st);
+		bm::prof_mark_end(bm::PK_ML_MULT, st);
+	}
+	if (rc == BM_OK) rc = bm::pass_y(m->Ny, 1, m->S, nlines, m->Sp, m->ncu, st);
+	const float scale = (float)(1.0 / ((double)m->Nx * m->Ny));
+	if (rc == BM_OK) rc = bm::pass_inv_x(m->nx, clear_out ? 1 : 2, m->S, Heff, m->ny, L, m->Sp, scale, m->ncu, st);
+	if (rc) return rc;
+	BM_HIP(hipGetLastError());
+	return BM_OK;
+}
+
+extern "C" int bm_mlayer_copy_kernels(bm_mlayer_t m, float* host_out, long long count)
+{
+	if (!m || !host_out || count < m->kcount) return fail(BM_ERR_ARG, "bm_mlayer_copy_kernels: bad arguments");
+	BM_HIP(hipMemcpy(host_out, m->K, (size_t)m->kcount * sizeof(float), hipMemcpyDeviceToHost));
+	return BM_OK;
+}

@@ -24,8 +24,9 @@ namespace {
 using namespace bm;
 
 template <int METHOD, int STEP>
-__global__ void __launch_bounds__(kBS) k_stage(StageArgs a)
+__global__ void __launch_bounds__(kBS) k_stage(StageArgs a, const StageArgs* __restrict__ bt)
 {
+	if (bt) a = bt[blockIdx.y];     // batched: one mesh per grid row
 	const size_t n = (size_t)a.nx * a.ny * a.nz;
 	const long long nchunk = (long long)((n + kBS - 1) / kBS);
 	float lte_loc = 0.f;
@@ -166,20 +167,20 @@ int launch_stage(bm_ode_s* o, int step, float dT, hipStream_t st)
 	dim3 g(stage_chunks(o)), b(kBS);
 	bm::prof_mark_begin(bm::PK_STAGE, st);
 	switch (o->mcode * 8 + step) {
-	case M_RKDP * 8 + 0: hipLaunchKernelGGL((k_stage<M_RKDP, 0>), g, b, 0, st, a); break;
-	case M_RKDP * 8 + 1: hipLaunchKernelGGL((k_stage<M_RKDP, 1>), g, b, 0, st, a); break;
-	case M_RKDP * 8 + 2: hipLaunchKernelGGL((k_stage<M_RKDP, 2>), g, b, 0, st, a); break;
-	case M_RKDP * 8 + 3: hipLaunchKernelGGL((k_stage<M_RKDP, 3>), g, b, 0, st, a); break;
-	case M_RKDP * 8 + 4: hipLaunchKernelGGL((k_stage<M_RKDP, 4>), g, b, 0, st, a); break;
-	case M_RKDP * 8 + 5: hipLaunchKernelGGL((k_stage<M_RKDP, 5>), g, b, 0, st, a); break;
-	case M_RK4 * 8 + 0: hipLaunchKernelGGL((k_stage<M_RK4, 0>), g, b, 0, st, a); break;
-	case M_RK4 * 8 + 1: hipLaunchKernelGGL((k_stage<M_RK4, 1>), g, b, 0, st, a); break;
-	case M_RK4 * 8 + 2: hipLaunchKernelGGL((k_stage<M_RK4, 2>), g, b, 0, st, a); break;
-	case M_RK4 * 8 + 3: hipLaunchKernelGGL((k_stage<M_RK4, 3>), g, b, 0, st, a); break;
-	case M_TEULER * 8 + 0: hipLaunchKernelGGL((k_stage<M_TEULER, 0>), g, b, 0, st, a); break;
-	case M_TEULER * 8 + 1: hipLaunchKernelGGL((k_stage<M_TEULER, 1>), g, b, 0, st, a); break;
-	case M_AHEUN * 8 + 0: hipLaunchKernelGGL((k_stage<M_AHEUN, 0>), g, b, 0, st, a); break;
-	case M_AHEUN * 8 + 1: hipLaunchKernelGGL((k_stage<M_AHEUN, 1>), g, b, 0, st, a); break;
+	case M_RKDP * 8 + 0: hipLaunchKernelGGL((k_stage<M_RKDP, 0>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
+	case M_RKDP * 8 + 1: hipLaunchKernelGGL((k_stage<M_RKDP, 1>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
+	case M_RKDP * 8 + 2: hipLaunchKernelGGL((k_stage<M_RKDP, 2>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
+	case M_RKDP * 8 + 3: hipLaunchKernelGGL((k_stage<M_RKDP, 3>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
+	case M_RKDP * 8 + 4: hipLaunchKernelGGL((k_stage<M_RKDP, 4>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
+	case M_RKDP * 8 + 5: hipLaunchKernelGGL((k_stage<M_RKDP, 5>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
+	case M_RK4 * 8 + 0: hipLaunchKernelGGL((k_stage<M_RK4, 0>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
+	case M_RK4 * 8 + 1: hipLaunchKernelGGL((k_stage<M_RK4, 1>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
+	case M_RK4 * 8 + 2: hipLaunchKernelGGL((k_stage<M_RK4, 2>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
+	case M_RK4 * 8 + 3: hipLaunchKernelGGL((k_stage<M_RK4, 3>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
+	case M_TEULER * 8 + 0: hipLaunchKernelGGL((k_stage<M_TEULER, 0>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
+	case M_TEULER * 8 + 1: hipLaunchKernelGGL((k_stage<M_TEULER, 1>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
+	case M_AHEUN * 8 + 0: hipLaunchKernelGGL((k_stage<M_AHEUN, 0>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
+	case M_AHEUN * 8 + 1: hipLaunchKernelGGL((k_stage<M_AHEUN, 1>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
 	default: return bm::fail(BM_ERR_ARG, "bad stage");
 	}
 	bm::prof_mark_end(bm::PK_STAGE, st);
@@ -187,6 +188,29 @@ int launch_stage(bm_ode_s* o, int step, float dT, hipStream_t st)
 	if (!(o->mcode == M_RKDP && step == 0)) o->Mcur = (o->Mcur == o->M) ? o->Mb : o->M;
 	return BM_OK;
 }
+
+} // namespace
+
+// Batched stage launch: nb meshes, per-mesh StageArgs in d_tab; grid rows are meshes.
+int bm::batch_stage(int mcode, int step, const StageArgs* d_tab, int nb, size_t max_cells, int ncu, hipStream_t st)
+{
+	size_t nch = (max_cells + kBS - 1) / kBS;
+	const int gx = (int)std::max<size_t>(8, std::min<size_t>(((nch + 7) / 8) * 8, (size_t)ncu * 16));
+	const dim3 g(gx, nb), b(kBS);
+	StageArgs a{};
+	switch (mcode * 8 + step) {
+	case M_RK4 * 8 + 0: hipLaunchKernelGGL((k_stage<M_RK4, 0>), g, b, 0, st, a, d_tab); break;
+	case M_RK4 * 8 + 1: hipLaunchKernelGGL((k_stage<M_RK4, 1>), g, b, 0, st, a, d_tab); break;
+	case M_RK4 * 8 + 2: hipLaunchKernelGGL((k_stage<M_RK4, 2>), g, b, 0, st, a, d_tab); break;
+	case M_RK4 * 8 + 3: hipLaunchKernelGGL((k_stage<M_RK4, 3>), g, b, 0, st, a, d_tab); break;
+	case M_TEULER * 8 + 0: hipLaunchKernelGGL((k_stage<M_TEULER, 0>), g, b, 0, st, a, d_tab); break;
+	case M_TEULER * 8 + 1: hipLaunchKernelGGL((k_stage<M_TEULER, 1>), g, b, 0, st, a, d_tab); break;
+	default: return bm::fail(BM_ERR_ARG, "batched stage: fixed-step methods only (RK4, TEuler)");
+	}
+	return BM_OK;
+}
+
+namespace {
 
 // Fused evaluation: [x-R2C of Mcur unless the spectra already hold it] + y / z passes with
 // the kernel product + the row kernel (x-C2R of H_demag, field + stage, x-R2C of the stage's

@@ -260,6 +260,9 @@ __device__ __forceinline__ P6 stage_pair(const StageArgs& a, size_t idx, int i0,
 	return rk_update<METHOD, STEP>(a, idx, m, rhs, Ms, lte_loc);
 }
 
+// Batched stage kernel over nb meshes (fixed-step methods), bm_ode.hip.
+int batch_stage(int mcode, int step, const StageArgs* d_tab, int nb, size_t max_cells, int ncu, hipStream_t st);
+
 // Stages that produce an lte (reduced over the mesh to a max)
 template <int METHOD, int STEP> constexpr bool stage_has_lte() { return (METHOD == M_RKDP && STEP == 0) || (METHOD == M_AHEUN && STEP == 1); }
 

@@ -69,6 +69,31 @@ int bm_demag_convolute_planar(bm_demag_t d, const float* M, float* H3n, void* st
 int bm_demag_copy_kernels(bm_demag_t d, float* host_out, long long count);
 
 /* ------------------------------------------------------------------------------------
+ * Multilayered demag of stacked 2D layers: L (<= 8) layers of the same nx x ny cells
+ * (hx, hy, hz[l]), layer l's bottom face at height z0[l] (m); layer thicknesses may differ.
+ * M and H are L layers of nx*ny cells stacked (layer-major, AoS per cell).
+ *
+ * Replaces: SDemagCUDA's multilayered convolution for 2D layers related by z shifts —
+ * SDemag::Initialize / set_Rect_collection (Boris/SDemag.cpp:173-300),
+ * DemagKernelCollectionCUDA::Calculate_Demag_Kernels (Boris/DemagKernelCollectionCUDA_Calc.cpp:13;
+ * self kernels :182, z-shifted :373; tensors Boris/DemagTFunc_Shifted.cpp:235-449 and
+ * Boris/DemagTFunc_Irregular.cpp:5-235) computed once in FP64 on the GPU. */
+typedef struct bm_mlayer_s* bm_mlayer_t;
+int bm_mlayer_create(int nlayers, int nx, int ny, double hx, double hy, const double* hz, const double* z0,
+	int flags, bm_mlayer_t* out);
+int bm_mlayer_destroy(bm_mlayer_t m);
+/* H (=, += if !clear_out) the demag field of every layer from the magnetisation of all
+ * layers.  Replaces SDemagCUDA::UpdateField's multilayered branch (Boris/SDemagCUDA.cpp:270):
+ * forward FFTs of all layers, KernelMultiplication_2D over all sources per destination
+ * (Boris/DemagKernelCollectionCUDA_Mult.cu:610), inverse FFTs. */
+int bm_mlayer_convolute(bm_mlayer_t m, const float* M, float* Heff, int clear_out, void* stream);
+/* Copies the FP32 kernels to host memory (test hook): L*L pairs (destination-major), each
+ * 6 planes Kxx, Kyy, Kzz, Kxy, Kxz, Kyz of (nx+1)*(ny+1), index kx*(ny+1)+ky.  Own-layer
+ * pairs hold the 2D kernel (Kxz = Kyz = 0); Kxz and Kyz of z-shifted pairs are the
+ * coefficients of i. */
+int bm_mlayer_copy_kernels(bm_mlayer_t m, float* host_out, long long count);
+
+/* ------------------------------------------------------------------------------------
  * LLG integrator for one ferromagnetic mesh: Zeeman (sets), demag, exchange and uniaxial
  * anisotropy (add) fused into each RK stage.
  *
@@ -143,6 +168,26 @@ int bm_ode_copy_heff(bm_ode_t o, float* Heff_out, void* stream);
 int bm_ode_average_m(bm_ode_t o, double* out3, void* stream);
 
 /* ------------------------------------------------------------------------------------
+ * Batched fixed-step integration of many independent meshes of uneven sizes (no
+ * reference counterpart: the reference integrates one mesh set per simulation).  Mesh b
+ * has n3[3b..3b+2] cells of h3[3b..3b+2] (m) and its own parameters params[b]; M is the
+ * caller's device array holding every mesh's magnetisation concatenated in mesh order
+ * (bm_batch_mesh_offset gives each mesh's cell offset).  Methods: BM_EVAL_RK4,
+ * BM_EVAL_TEULER (fixed step; constant applied fields).  Every pass of a field evaluation
+ * is one launch per distinct pass size over all meshes of that size; a whole step is
+ * captured once into a hipGraph and replayed (BM_BATCH_NO_GRAPH: launch directly). */
+typedef struct bm_batch_s* bm_batch_t;
+enum { BM_BATCH_NO_GRAPH = 1 };
+int bm_batch_create(int method, int nmesh, const int* n3, const double* h3, const bm_fm_params* params,
+	float* M, int flags, bm_batch_t* out);
+int bm_batch_destroy(bm_batch_t b);
+int bm_batch_set_dt(bm_batch_t b, double dT);
+/* nsteps fixed steps of every mesh, ordered after earlier work on `stream`. */
+int bm_batch_run(bm_batch_t b, long long nsteps, void* stream);
+int bm_batch_get_state(bm_batch_t b, double* time, long long* iteration);
+int bm_batch_mesh_offset(bm_batch_t b, int mesh, long long* cell_offset, long long* ncells);
+
+/* ------------------------------------------------------------------------------------
  * Launch timing (no reference counterpart: the reference only times whole runs,
  * Boris/Commands.cpp:4759).  Between bm_prof_begin and bm_prof_end every hot-path kernel
  * launch is bracketed by HIP events recorded on the launch's own stream; bm_prof_query
@@ -150,7 +195,8 @@ int bm_ode_average_m(bm_ode_t o, double* out3, void* stream);
  * 0 x-R2C pass, 1 y-FFT pass, 2 z-pass with kernel multiply, 3 y-IFFT pass,
  * 4 x-C2R pass, 5 2D y pass with kernel multiply, 6 field + RK stage (no demag),
  * 7 RKDP54 advance (no demag), 8 fused row kernel (x-C2R of H_demag + field + RK stage +
- * x-R2C of the stage's output M), 9 x-R2C pass fused with the RKDP54 advance.
+ * x-R2C of the stage's output M), 9 x-R2C pass fused with the RKDP54 advance,
+ * 10 multilayer kernel product.
  * Not thread safe; meant for benchmarks. */
 int bm_prof_begin(int max_launches);
 int bm_prof_end(void);

@@ -1,0 +1,223 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY.
+// CPU FP64 discrete adjoint of the fixed-step RK4 LLG rollout with demag, exchange and a
+// Zeeman field H(t) = Ha + amp * dir * sinc(w (t - t0)), per-cell Ms map.  Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load or call anything under
+// oracle/; the product (boris2_amd/) never does.
+//
+// The reference has no differentiation (a capability added beside its plain path); the
+// forward scheme it differentiates is the oracle's own RK4 step (Boris/DiffEqFM_Evals_RK4.cpp:13-296
+// with the field order of Boris/MeshModules.cpp:131-135, Zeeman at the step's start time,
+// renormalisation to the cell's Ms after the step).
+//
+// Loss: L = 1 - (1/n) sum_i (M_i / Ms_i) . mt_i of the final state; initial state
+// M0_i = Ms_i m0_i.  Gradients: dL/dMs_i (per cell) and dL/d amp.
+// Parity: pinned by central finite differences of the same FP64 forward
+// (tests/test_oracle_adjoint.py).
+#pragma once
+
+#include "oracle_core.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace oracle {
+
+struct AdjRK4 {
+	int nx, ny, nz; size_t n;
+	double h[3];
+	Params p;
+	std::vector<double> Ms;
+	Demag<double>* dem = nullptr;
+	double dt = 0; int nsteps = 0;
+	// tape: per step the four stage inputs X1..X4 and the pre-renormalisation sum Y
+	std::vector<std::vector<double>> X[4], Y;
+
+	AdjRK4(int nx_, int ny_, int nz_, const double* hh, const Params& p_, const double* Ms_, double dt_, int nsteps_, int Nz_override)
+		: nx(nx_), ny(ny_), nz(nz_), p(p_), dt(dt_), nsteps(nsteps_)
+	{
+		n = (size_t)nx * ny * nz;
+		for (int a = 0; a < 3; a++) h[a] = hh[a];
+		Ms.assign(Ms_, Ms_ + n);
+		if (p.demag) dem = new Demag<double>(nx, ny, nz, h[0], h[1], h[2], Nz_override);
+	}
+	~AdjRK4() { delete dem; }
+
+	double zeeman_scale(double t) const
+	{
+		if (p.wave != 1) return 0.0;
+		const double x = p.w * (t - p.t0);
+		return (std::fabs(x) < 1e-300) ? 1.0 : std::sin(x) / x;
+	}
+
+	// second differences / h^2 with free boundaries (the exchange Laplacian, unscaled)
+	void laplacian(const double* V, double* out) const
+	{
+		const int d[3] = { nx, ny, nz };
+		const size_t st[3] = { 1, (size_t)nx, (size_t)nx * ny };
+#pragma omp parallel for collapse(2)
+		for (int k = 0; k < nz; k++) for (int j = 0; j < ny; j++) for (int i = 0; i < nx; i++) {
+			const size_t idx = (size_t)i + (size_t)j * nx + (size_t)k * nx * ny;
+			const int ijk[3] = { i, j, k };
+			double l[3] = { 0, 0, 0 };
+			for (int a = 0; a < 3; a++) {
+				const bool lo = ijk[a] > 0, hi = ijk[a] < d[a] - 1;
+				for (int c = 0; c < 3; c++) {
+					const double m0 = V[idx * 3 + c];
+					double df = 0;
+					if (lo && hi) df = V[(idx + st[a]) * 3 + c] + V[(idx - st[a]) * 3 + c] - 2 * m0;
+					else if (hi) df = V[(idx + st[a]) * 3 + c] - m0;
+					else if (lo) df = V[(idx - st[a]) * 3 + c] - m0;
+					l[c] += df / (h[a] * h[a]);
+				}
+			}
+			for (int c = 0; c < 3; c++) out[idx * 3 + c] = l[c];
+		}
+	}
+
+	double cex(size_t i) const { return 2 * p.A / (kMU0 * Ms[i] * Ms[i]); }
+
+	void field(const double* Mv, double t, std::vector<double>& H, std::vector<double>& lap) const
+	{
+		double Hz[3]; applied_field(p, t, Hz);
+		for (size_t i = 0; i < n; i++) for (int c = 0; c < 3; c++) H[i * 3 + c] = p.zeeman ? Hz[c] : 0.0;
+		if (dem) dem->convolute(Mv, H.data(), false);
+		if (p.exchange) {
+			laplacian(Mv, lap.data());
+			for (size_t i = 0; i < n; i++) { const double ce = cex(i); for (int c = 0; c < 3; c++) H[i * 3 + c] += ce * lap[i * 3 + c]; }
+		}
+	}
+
+	static void cross(const double* a, const double* b, double* o) { o[0] = a[1] * b[2] - a[2] * b[1]; o[1] = a[2] * b[0] - a[0] * b[2]; o[2] = a[0] * b[1] - a[1] * b[0]; }
+	static double dot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+	double pre() const { return -kGAMMA * p.grel / (1 + p.alpha * p.alpha); }
+
+	void rhs(const double* Mv, const double* H, double* k) const
+	{
+		const double pr = pre(), al = p.alpha;
+		for (size_t i = 0; i < n; i++) {
+			const double* m = Mv + i * 3; const double* hh = H + i * 3;
+			double mxh[3], mmh[3], mu[3] = { m[0] / Ms[i], m[1] / Ms[i], m[2] / Ms[i] };
+			cross(m, hh, mxh); cross(mu, mxh, mmh);
+			for (int c = 0; c < 3; c++) k[i * 3 + c] = pr * (mxh[c] + al * mmh[c]);
+		}
+	}
+
+	// forward rollout from M0 = Ms * m0; returns the final M, keeps the tape
+	void forward(const double* m0, double* Mf)
+	{
+		std::vector<double> M(3 * n), H(3 * n), lap(3 * n), k(3 * n), acc(3 * n);
+		for (size_t i = 0; i < n; i++) for (int c = 0; c < 3; c++) M[i * 3 + c] = Ms[i] * m0[i * 3 + c];
+		for (int s = 0; s < 4; s++) X[s].assign(nsteps, {});
+		Y.assign(nsteps, {});
+		static const double cst[4] = { 0.5, 0.5, 1.0, 0.0 }, wgt[4] = { 1, 2, 2, 1 };
+		for (int st = 0; st < nsteps; st++) {
+			const double t = st * dt;
+			std::vector<double> Xs = M;
+			std::fill(acc.begin(), acc.end(), 0.0);
+			for (int s = 0; s < 4; s++) {
+				X[s][st] = Xs;
+				field(Xs.data(), t, H, lap);
+				rhs(Xs.data(), H.data(), k.data());
+				for (size_t q = 0; q < 3 * n; q++) acc[q] += wgt[s] * k[q];
+				if (s < 3) for (size_t q = 0; q < 3 * n; q++) Xs[q] = M[q] + cst[s] * dt * k[q];
+			}
+			std::vector<double> Yv(3 * n);
+			for (size_t q = 0; q < 3 * n; q++) Yv[q] = M[q] + (dt / 6) * acc[q];
+			Y[st] = Yv;
+			for (size_t i = 0; i < n; i++) {
+				const double nr = std::sqrt(dot(&Yv[i * 3], &Yv[i * 3]));
+				for (int c = 0; c < 3; c++) M[i * 3 + c] = Ms[i] * Yv[i * 3 + c] / nr;
+			}
+		}
+		for (size_t q = 0; q < 3 * n; q++) Mf[q] = M[q];
+	}
+
+	double loss(const double* Mf, const double* mt) const
+	{
+		double s = 0;
+		for (size_t i = 0; i < n; i++) s += dot(Mf + i * 3, mt + i * 3) / Ms[i];
+		return 1.0 - s / (double)n;
+	}
+
+	// lamX = J_f(X)^T w (adjoint of k = f(X) at time t); accumulates dL/dMs and dL/d amp.
+	void vjp(const double* Xv, double t, const double* w, double* lamX, double* gMs, double& gamp) const
+	{
+		std::vector<double> H(3 * n), lap(3 * n), u(3 * n), cu(3 * n), Nu(3 * n, 0.0), Lcu(3 * n, 0.0);
+		field(Xv, t, H, lap);
+		const double pr = pre(), al = p.alpha, sc = zeeman_scale(t);
+		double ga = 0;
+		for (size_t i = 0; i < n; i++) {
+			const double* m = Xv + i * 3;
+			const double* Hi = &H[i * 3]; const double* wi = w + i * 3;
+			const double ms = Ms[i], mh = dot(m, Hi), mw = dot(m, wi), hw = dot(Hi, wi), mm = dot(m, m);
+			double hxw[3], wxm[3], mxh[3], mmh[3];
+			cross(Hi, wi, hxw); cross(wi, m, wxm); cross(m, Hi, mxh); cross(m, mxh, mmh);
+			for (int c = 0; c < 3; c++) {
+				lamX[i * 3 + c] = pr * (hxw[c] + (al / ms) * (wi[c] * mh + Hi[c] * mw - 2 * m[c] * hw));
+				u[i * 3 + c] = pr * (wxm[c] + (al / ms) * (m[c] * mw - mm * wi[c]));
+			}
+			// Ms enters f through alpha/Ms and H through the exchange prefactor 2A/(mu0 Ms^2)
+			gMs[i] += -pr * al / (ms * ms) * dot(mmh, wi);
+			if (p.exchange) gMs[i] += dot(&u[i * 3], &lap[i * 3]) * (-2 * cex(i) / ms);
+			if (p.zeeman && p.wave == 1) ga += dot(&u[i * 3], p.dir) * sc;
+			for (int c = 0; c < 3; c++) cu[i * 3 + c] = cex(i) * u[i * 3 + c];
+		}
+		gamp += ga;
+		// through the fields: the demag operator is symmetric (N_ij = N_ji, N(r) = N(-r)), the
+		// free-boundary Laplacian is symmetric; the exchange prefactor sits on the output side
+		if (dem) dem->convolute(u.data(), Nu.data(), true);
+		if (p.exchange) laplacian(cu.data(), Lcu.data());
+		for (size_t q = 0; q < 3 * n; q++) lamX[q] += Nu[q] + Lcu[q];
+	}
+
+	// gradients of L(forward(m0)) with the tape of the last forward()
+	void backward(const double* m0, const double* mt, double* gMs, double& gamp)
+	{
+		std::vector<double> lam(3 * n), lamY(3 * n), w(3 * n), lx(3 * n), lacc(3 * n);
+		for (size_t i = 0; i < n; i++) gMs[i] = 0;
+		gamp = 0;
+		// final state M_N = Ms * Y/|Y| of the last step; dL/dM_N and the explicit Ms term
+		{
+			const std::vector<double>& Yv = Y[nsteps - 1];
+			for (size_t i = 0; i < n; i++) {
+				const double nr = std::sqrt(dot(&Yv[i * 3], &Yv[i * 3]));
+				double Mi[3];
+				for (int c = 0; c < 3; c++) Mi[c] = Ms[i] * Yv[i * 3 + c] / nr;
+				for (int c = 0; c < 3; c++) lam[i * 3 + c] = -mt[i * 3 + c] / ((double)n * Ms[i]);
+				gMs[i] += dot(Mi, mt + i * 3) / ((double)n * Ms[i] * Ms[i]);
+			}
+		}
+		static const double cst[4] = { 0.5, 0.5, 1.0, 0.0 }, wgt[4] = { 1, 2, 2, 1 };
+		for (int st = nsteps - 1; st >= 0; st--) {
+			const double t = st * dt;
+			const std::vector<double>& Yv = Y[st];
+			// renormalisation M' = Ms Y/|Y|
+			for (size_t i = 0; i < n; i++) {
+				const double nr = std::sqrt(dot(&Yv[i * 3], &Yv[i * 3]));
+				double yh[3] = { Yv[i * 3] / nr, Yv[i * 3 + 1] / nr, Yv[i * 3 + 2] / nr };
+				const double yl = dot(yh, &lam[i * 3]);
+				gMs[i] += yl;
+				for (int c = 0; c < 3; c++) lamY[i * 3 + c] = (Ms[i] / nr) * (lam[i * 3 + c] - yh[c] * yl);
+			}
+			// Y = M + dt/6 (k1 + 2k2 + 2k3 + k4), X_{s+1} = M + c_s dt k_s
+			for (size_t q = 0; q < 3 * n; q++) lacc[q] = lamY[q];      // adjoint of M_st
+			std::vector<double> lamXnext(3 * n, 0.0);                  // adjoint of X_{s+1}
+			for (int s = 3; s >= 0; s--) {
+				for (size_t q = 0; q < 3 * n; q++) w[q] = (dt / 6) * wgt[s] * lamY[q] + (s < 3 ? cst[s] * dt * lamXnext[q] : 0.0);
+				std::fill(lx.begin(), lx.end(), 0.0);
+				vjp(X[s][st].data(), t, w.data(), lx.data(), gMs, gamp);
+				// X_s = M + c_{s-1} dt k_{s-1} (s > 0) or X_0 = M
+				for (size_t q = 0; q < 3 * n; q++) lacc[q] += lx[q];
+				lamXnext = lx;
+				if (s == 0) break;
+				// lx (adjoint of X_s) also feeds k_{s-1} through w in the next iteration
+			}
+			lam = lacc;
+		}
+		// M0 = Ms m0
+		for (size_t i = 0; i < n; i++) gMs[i] += dot(m0 + i * 3, &lam[i * 3]);
+	}
+};
+
+} // namespace oracle

@@ -2,6 +2,7 @@
 // (ctypes) and bench.py's cpu_baseline leg.  Nothing in the product links it.
 #include "oracle_core.hpp"
 #include "oracle_mlayer.hpp"
+#include "oracle_adjoint.hpp"
 
 #include <cstring>
 #include <exception>
@@ -142,6 +143,40 @@ struct OrSim { int fp32; void* s; };
 
 // params: Ms, A, alpha, K1, K2, ea[3], Ha[3], wave, amp, w, t0, dir[3], slope[3] (22 doubles)
 // flags: bit0 demag, bit1 exchange, bit2 anisotropy, bit3 zeeman
+static Params unpack_params(const double* prm, int flags)
+{
+	Params p;
+	p.Ms = prm[0]; p.A = prm[1]; p.alpha = prm[2]; p.K1 = prm[3]; p.K2 = prm[4];
+	for (int c = 0; c < 3; c++) { p.ea[c] = prm[5 + c]; p.Ha[c] = prm[8 + c]; }
+	p.wave = (int)prm[11]; p.amp = prm[12]; p.w = prm[13]; p.t0 = prm[14];
+	for (int c = 0; c < 3; c++) { p.dir[c] = prm[15 + c]; p.slope[c] = prm[18 + c]; }
+	p.demag = flags & 1; p.exchange = (flags >> 1) & 1; p.anisotropy = (flags >> 2) & 1; p.zeeman = (flags >> 3) & 1;
+	return p;
+}
+
+// Discrete adjoint of the fixed-step RK4 rollout (FP64): per-cell Ms map Ms[n], h[3].
+void* or_adj_create(int nx, int ny, int nz, const double* h, const double* prm, int flags, const double* Ms,
+	double dt, int nsteps, int Nz_override)
+{
+	try { return new AdjRK4(nx, ny, nz, h, unpack_params(prm, flags), Ms, dt, nsteps, Nz_override); }
+	catch (...) { return nullptr; }
+}
+void or_adj_destroy(void* h) { delete (AdjRK4*)h; }
+// forward from M0 = Ms m0 (m0: n*3); final M into Mf; returns the loss against mt
+double or_adj_forward(void* h, const double* m0, const double* mt, double* Mf)
+{
+	AdjRK4* a = (AdjRK4*)h;
+	a->forward(m0, Mf);
+	return a->loss(Mf, mt);
+}
+// gradients after or_adj_forward: dL/dMs (n) and dL/d amp
+void or_adj_backward(void* h, const double* m0, const double* mt, double* gMs, double* gamp)
+{
+	double g = 0;
+	((AdjRK4*)h)->backward(m0, mt, gMs, g);
+	*gamp = g;
+}
+
 void* or_sim_create(int method, int nx, int ny, int nz, double hx, double hy, double hz,
 	const double* prm, int flags, int Nz_override, int fp32)
 {

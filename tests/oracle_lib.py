@@ -59,6 +59,10 @@ def lib() -> C.CDLL:
     L.or_mlayer_create.argtypes = [i, i, i, d, d, dp, dp]; L.or_mlayer_create.restype = vp
     L.or_mlayer_destroy.argtypes = [vp]
     L.or_mlayer_convolute.argtypes = [vp, dp, dp]
+    L.or_adj_create.argtypes = [i, i, i, dp, dp, i, dp, d, i, i]; L.or_adj_create.restype = vp
+    L.or_adj_destroy.argtypes = [vp]
+    L.or_adj_forward.argtypes = [vp, dp, dp, dp]; L.or_adj_forward.restype = d
+    L.or_adj_backward.argtypes = [vp, dp, dp, dp, dp]
     L.or_set_random_mag.argtypes = [dp, C.c_longlong, C.c_uint]
     L.or_borisrand.argtypes = [C.c_uint, i, dp]
     _L = L
@@ -233,6 +237,38 @@ class OracleSim:
     def __del__(self):
         try:
             self.L.or_sim_destroy(self.h)
+        except Exception:
+            pass
+
+
+class OracleAdjoint:
+    """FP64 discrete adjoint of the fixed-step RK4 rollout: loss 1 - <(M/Ms) . m_target> of
+    the final state from M0 = Ms m0; gradients dL/dMs (per cell) and dL/d amp."""
+
+    def __init__(self, n, h, params, Ms_map, dt, nsteps, Nz_override=0):
+        self.L = lib()
+        self.n = tuple(n)
+        self.ncell = n[0] * n[1] * n[2]
+        prm, flags = pack_params(params)
+        self.h = self.L.or_adj_create(*n, np.array(h, float), prm, flags, np.ascontiguousarray(Ms_map, float).reshape(-1),
+                                      float(dt), int(nsteps), int(Nz_override))
+        assert self.h
+
+    def forward(self, m0, mt):
+        Mf = np.zeros(self.ncell * 3)
+        loss = self.L.or_adj_forward(self.h, np.ascontiguousarray(m0, float).reshape(-1),
+                                     np.ascontiguousarray(mt, float).reshape(-1), Mf)
+        return loss, Mf.reshape(-1, 3)
+
+    def backward(self, m0, mt):
+        g = np.zeros(self.ncell)
+        ga = np.zeros(1)
+        self.L.or_adj_backward(self.h, np.ascontiguousarray(m0, float).reshape(-1), np.ascontiguousarray(mt, float).reshape(-1), g, ga)
+        return g, float(ga[0])
+
+    def __del__(self):
+        try:
+            self.L.or_adj_destroy(self.h)
         except Exception:
             pass
 
