@@ -130,6 +130,97 @@ def gpu_run(steps: int, warmup: int):
             "kinds": kinds, "setup_s": setup_s, "dev_bytes": dev_bytes, "sim_time": st1["time"], "m_avg": m_avg.tolist()}
 
 
+def batched_meshes(seed: int = 64, count: int = 64):
+    """64 independent films: nx, ny from {64, 96, 128, 192, 256}, nz from {1, 2, 4, 8}, 4 nm cells,
+    Ms uniform in [4e5, 1.2e6] A/m, A 1.3e-11 J/m, alpha uniform in [0.01, 0.5], demag + exchange,
+    M0 random on the sphere (seeded)."""
+    from boris2_amd import api
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(count):
+        n = (int(rng.choice([64, 96, 128, 192, 256])), int(rng.choice([64, 96, 128, 192, 256])), int(rng.choice([1, 2, 4, 8])))
+        p = api.FMParams(Ms=float(rng.uniform(4e5, 1.2e6)), A=1.3e-11, alpha=float(rng.uniform(0.01, 0.5)),
+                         modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE)
+        M0 = api.random_magnetisation(n[0] * n[1] * n[2], p.Ms, 1000 + b).astype(np.float32)
+        out.append((n, (4e-9, 4e-9, 4e-9), p, M0))
+    return out
+
+
+def batch_run(steps: int):
+    """Batched RK4 (dt 50 fs) over the 64 films vs the same films run one by one on the plain
+    single-mesh path; both timed over `steps` steps after one warm-up step."""
+    import torch
+    from boris2_amd import api
+    meshes = batched_meshes()
+    dt = 50e-15
+    cells = sum(n[0] * n[1] * n[2] for n, _, _, _ in meshes)
+    bs = api.BatchSolver(api.BM_EVAL_RK4, [(n, h, p) for n, h, p, _ in meshes], [M0 for *_, M0 in meshes])
+    bs.set_dt(dt)
+    bs.run(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bs.run(steps)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    bs.close()
+    solvers = []
+    for n, h, p, M0 in meshes:
+        s = api.FMSolver(api.BM_EVAL_RK4, n, h, p, M0)
+        s.set_dt(dt)
+        s.run_to_iteration(1)
+        solvers.append(s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.run_to_iteration(1 + steps)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    for s in solvers:
+        s.close()
+    return {"meshes": len(meshes), "cells": cells, "steps": steps, "method": "RK4 fixed dt 50 fs",
+            "batched_cell_updates_per_s": cells * steps / tb, "plain_cell_updates_per_s": cells * steps / tp,
+            "batched_over_plain": tp / tb, "batched_ms_per_step": 1e3 * tb / steps, "plain_ms_per_step": 1e3 * tp / steps}
+
+
+def adjoint_run(steps: int, ckpt: int):
+    """128x128x4 film of 4 nm cells, Ms map = sum of 8 Gaussian blobs in [5e5, 1.1e6] A/m,
+    A 1.3e-11, alpha 0.05, 1 ns sinc pulse of 20 mT (seeded direction); RK4 dt 250 fs,
+    checkpoints every `ckpt` steps; loss 1 - <m_final . m_target>.  Forward and backward
+    wall times (the backward includes the segment recomputation)."""
+    import torch
+    from boris2_amd import api
+    rng = np.random.default_rng(4)
+    n = (128, 128, 4)
+    ncell = n[0] * n[1] * n[2]
+    x, y = np.meshgrid(np.arange(n[0]), np.arange(n[1]))
+    blob = np.zeros((n[1], n[0]))
+    for _ in range(8):
+        cx, cy, r = rng.uniform(0, n[0]), rng.uniform(0, n[1]), rng.uniform(6, 20)
+        blob += np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * r * r))
+    blob = (blob - blob.min()) / (blob.max() - blob.min())
+    Ms = np.tile((5e5 + 6e5 * blob).reshape(-1), n[2]).astype(np.float32)
+    d = rng.normal(size=3); d /= np.linalg.norm(d)
+    p = api.FMParams(Ms=8e5, A=1.3e-11, alpha=0.05, modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ZEEMAN,
+                     wave=1, amp=20e-3 / api.MU0, w=2 * np.pi / 1e-9, t0=0.5e-9, dir=tuple(d))
+    m0 = rng.normal(size=(ncell, 3)); m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
+    mt = rng.normal(size=(ncell, 3)); mt /= np.linalg.norm(mt, axis=1, keepdims=True)
+    r = api.RK4Adjoint(n, (4e-9,) * 3, p, torch.tensor(Ms), 250e-15, steps, ckpt)
+    m0t, mtt = torch.tensor(m0, dtype=torch.float32), torch.tensor(mt, dtype=torch.float32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss, _ = r.forward(m0t, mtt)
+    torch.cuda.synchronize()
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g, ga = r.backward()
+    torch.cuda.synchronize()
+    tbk = time.perf_counter() - t0
+    r.close()
+    return {"mesh": list(n), "steps": steps, "checkpoint_every": ckpt, "forward_s": round(tf, 3), "backward_s": round(tbk, 3),
+            "adjoint_over_forward": round(tbk / tf, 3), "loss": loss, "dL_damp": ga,
+            "forward_cell_updates_per_s": ncell * steps / tf}
+
+
 def cpu_baseline(cpu_steps: int):
     """Oracle FP64 on the host: kernel setup (reported separately) then `cpu_steps` steps."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -158,6 +249,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--batch-steps", type=int, default=50, help="steps of the batched vs one-by-one comparison (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=4000, help="RK4 steps of the adjoint rollout (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy 5 times (scales the HBM byte counters)")
     a = ap.parse_args()
@@ -184,6 +277,8 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
             "traffic_source": str(TRAFFIC_FILE.relative_to(ROOT)), "kernel": dom,
             "alg_bytes_per_launch": round(d["alg_GB_per_launch"] * 1e9), "avg_launch_ms": d["avg_ms"]}
+    batched = batch_run(a.batch_steps) if a.batch_steps > 0 else None
+    adjoint = adjoint_run(a.adjoint_steps, 100) if a.adjoint_steps > 0 else None
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_steps)
     value = r["ncell"] * r["accepted"] / r["wall"]
     line = {
@@ -197,6 +292,8 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "adjoint": adjoint,
         "kernels": kinds,
     }
     print(json.dumps(line), flush=True)

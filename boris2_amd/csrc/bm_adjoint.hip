@@ -1,0 +1,431 @@
+// Discrete adjoint of the fixed-step RK4 LLG rollout with demag, exchange and a Zeeman
+// field H(t) = Ha + amp * dir * sinc(w (t - t0)), per-cell Ms map.
+//
+// No reference counterpart (the reference has no differentiation); the forward scheme is
+// the plain path's RK4 step (Boris/DiffEqFM_EvalsLLGCUDA.cu:416-641: Zeeman at the step's
+// start time, renormalisation to the cell's Ms after the step).
+//
+// Loss L = 1 - (1/n) sum_i (M_i / Ms_i) . mt_i of the final state, M0_i = Ms_i m0_i.
+// Gradients dL/dMs_i and dL/d amp.  The adjoint of one RK4 step runs the stages backwards;
+// each stage's vector-Jacobian product needs the field at the stage input (taped during the
+// forward recomputation) and one demag convolution of the field adjoint u — the demag
+// operator is symmetric (N_ij = N_ji, N(r) = N(-r)), so its adjoint is the same
+// convolution; the free-boundary exchange Laplacian is symmetric too, with the per-cell
+// prefactor 2A/(mu0 Ms^2) moved to the input side.
+//
+// Memory: the state is checkpointed every K steps; the backward pass recomputes one segment
+// of K steps at a time with a tape of the four stage inputs, their demag fields and the
+// pre-renormalisation sum (27 floats per cell per step), then sweeps it backwards.  Cost:
+// forward 4 convolutions per step; backward 4 (recompute) + 4 (adjoint) per step.
+#include "bm_internal.h"
+#include "bm_stage.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace bm;
+
+namespace {
+
+constexpr int kAB = 256;
+
+struct AdjArgs {
+	int nx, ny, nz;
+	float ihx2, ihy2, ihz2;
+	float A, alpha, grel;
+	float Hz[3];            // applied field at the step's start time
+	float dir[3], sc;       // pulse direction and sinc value at that time (d H / d amp = sc dir)
+	int modules;
+	const float* Ms;
+};
+
+__device__ __forceinline__ F3 lap_at(const float* X, const AdjArgs& a, size_t idx, int i, int j, int k)
+{
+	const F3 m = ld3(X, idx);
+	F3 lx{ 0, 0, 0 }, ly{ 0, 0, 0 }, lz{ 0, 0, 0 };
+	const size_t sy = a.nx, sz = (size_t)a.nx * a.ny;
+	if (a.nx > 1) {
+		if (i > 0 && i < a.nx - 1) lx = ld3(X, idx + 1) + ld3(X, idx - 1) - 2.f * m;
+		else if (i == 0) lx = ld3(X, idx + 1) - m;
+		else lx = ld3(X, idx - 1) - m;
+	}
+	if (a.ny > 1) {
+		if (j > 0 && j < a.ny - 1) ly = ld3(X, idx + sy) + ld3(X, idx - sy) - 2.f * m;
+		else if (j == 0) ly = ld3(X, idx + sy) - m;
+		else ly = ld3(X, idx - sy) - m;
+	}
+	if (a.nz > 1) {
+		if (k > 0 && k < a.nz - 1) lz = ld3(X, idx + sz) + ld3(X, idx - sz) - 2.f * m;
+		else if (k == 0) lz = ld3(X, idx + sz) - m;
+		else lz = ld3(X, idx - sz) - m;
+	}
+	return a.ihx2 * lx + a.ihy2 * ly + a.ihz2 * lz;
+}
+
+__device__ __forceinline__ float cex(const AdjArgs& a, float Ms) { return (2.f * a.A) / ((float)kMU0 * Ms * Ms); }
+__device__ __forceinline__ float pre_llg(const AdjArgs& a) { return (float)(-kGAMMA) * a.grel / (1.f + a.alpha * a.alpha); }
+
+// field at X: Zeeman + demag (planar Hd) + exchange; also returns the unscaled Laplacian
+__device__ __forceinline__ F3 field_at(const AdjArgs& a, const float* X, const float* Hd, size_t n, size_t idx, int i, int j, int k, float Ms, F3& lap)
+{
+	F3 H = (a.modules & BM_MOD_ZEEMAN) ? F3{ a.Hz[0], a.Hz[1], a.Hz[2] } : F3{ 0.f, 0.f, 0.f };
+	if (a.modules & BM_MOD_DEMAG) H = H + F3{ Hd[idx], Hd[n + idx], Hd[2 * n + idx] };
+	lap = F3{ 0.f, 0.f, 0.f };
+	if (a.modules & BM_MOD_EXCHANGE) { lap = lap_at(X, a, idx, i, j, k); H = H + cex(a, Ms) * lap; }
+	return H;
+}
+
+__device__ __forceinline__ void cell_ijk(const AdjArgs& a, size_t idx, int& i, int& j, int& k)
+{
+	i = (int)(idx % a.nx); j = (int)((idx / a.nx) % a.ny); k = (int)(idx / ((size_t)a.nx * a.ny));
+}
+
+__global__ void k_init(float* M, const float* Ms, const float* m0, size_t n)
+{
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		st3(M, i, Ms[i] * ld3(m0, i));
+}
+
+// Forward stage S of the step from Mn: k = f(X); S < 3: acc (=, +=) w_S k, Xn = Mn + c_S dt k;
+// S == 3: Y = Mn + dt/6 (acc + k) (stored when Yout), Mnext = Ms Y / |Y|.
+template <int S>
+__global__ void __launch_bounds__(kAB) k_fwd(AdjArgs a, const float* __restrict__ X, const float* __restrict__ Hd, const float* __restrict__ Mn,
+	float* __restrict__ acc, float* __restrict__ Xn, float* __restrict__ Yout, float* __restrict__ Mnext, float dt)
+{
+	const size_t n = (size_t)a.nx * a.ny * a.nz;
+	for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+		int i, j, k; cell_ijk(a, idx, i, j, k);
+		const float Ms = a.Ms[idx];
+		F3 lap;
+		const F3 m = ld3(X, idx);
+		const F3 H = field_at(a, X, Hd, n, idx, i, j, k, Ms, lap);
+		const F3 mxh = cross(m, H);
+		const float iMs = 1.f / Ms;
+		const F3 mu{ m.x * iMs, m.y * iMs, m.z * iMs };
+		const F3 f = pre_llg(a) * (mxh + a.alpha * cross(mu, mxh));
+		if (S == 0) { st3(acc, idx, f); st3(Xn, idx, ld3(Mn, idx) + (dt / 2) * f); }
+		else if (S == 1) { st3(acc, idx, ld3(acc, idx) + 2.f * f); st3(Xn, idx, ld3(Mn, idx) + (dt / 2) * f); }
+		else if (S == 2) { st3(acc, idx, ld3(acc, idx) + 2.f * f); st3(Xn, idx, ld3(Mn, idx) + dt * f); }
+		else {
+			const F3 Y = ld3(Mn, idx) + (dt / 6) * (ld3(acc, idx) + f);
+			if (Yout) st3(Yout, idx, Y);
+			st3(Mnext, idx, (Ms / norm(Y)) * Y);
+		}
+	}
+}
+
+// deterministic per-block partial sums of sum_i (M_i . mt_i) / Ms_i
+__global__ void __launch_bounds__(kAB) k_loss_part(const float* M, const float* mt, const float* Ms, size_t n, double* part)
+{
+	__shared__ double red[kAB];
+	double s = 0;
+	for (size_t i = blockIdx.x * (size_t)kAB + threadIdx.x; i < n; i += (size_t)gridDim.x * kAB)
+		s += (double)dot(ld3(M, i), ld3(mt, i)) / (double)Ms[i];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for (int o = kAB / 2; o > 0; o >>= 1) { if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o]; __syncthreads(); }
+	if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// lam = dL/dM of the final state, gMs = explicit Ms term of the loss
+__global__ void k_final(const float* M, const float* mt, const float* Ms, size_t n, float* lam, double* gMs)
+{
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		const float ms = Ms[i];
+		const F3 t = ld3(mt, i);
+		st3(lam, i, (-1.f / ((float)n * ms)) * t);
+		gMs[i] = (double)dot(ld3(M, i), t) / ((double)n * (double)ms * (double)ms);
+	}
+}
+
+// renormalisation adjoint: lamY = (Ms/|Y|)(lam - y (y.lam)), gMs += y.lam; lacc = lamY
+__global__ void k_adj_renorm(const float* Y, const float* Ms, const float* lam, float* lamY, float* lacc, double* gMs, size_t n)
+{
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		const F3 y = ld3(Y, i);
+		const float nr = norm(y);
+		const F3 yh = (1.f / nr) * y;
+		const F3 l = ld3(lam, i);
+		const float yl = dot(yh, l);
+		gMs[i] += (double)yl;
+		const F3 ly = (Ms[i] / nr) * (l - yl * yh);
+		st3(lamY, i, ly);
+		st3(lacc, i, ly);
+	}
+}
+
+// Stage VJP, part 1: w = (dt/6) wS lamY + cS dt lxn; lx = direct term; u (field adjoint) and
+// c u (AoS); gMs += Ms terms; per-block partial of sum_i u_i . dir sc.
+__global__ void __launch_bounds__(kAB) k_vjp_a(AdjArgs a, const float* __restrict__ X, const float* __restrict__ Hd,
+	const float* __restrict__ lamY, const float* __restrict__ lxn, float wS, float cS, float dt,
+	float* __restrict__ lx, float* __restrict__ u, float* __restrict__ cu, double* __restrict__ gMs, double* __restrict__ apart)
+{
+	__shared__ double red[kAB];
+	const size_t n = (size_t)a.nx * a.ny * a.nz;
+	const float pr = pre_llg(a), al = a.alpha;
+	double ga = 0;
+	for (size_t idx = blockIdx.x * (size_t)kAB + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kAB) {
+		int i, j, k; cell_ijk(a, idx, i, j, k);
+		const float Ms = a.Ms[idx];
+		F3 lap;
+		const F3 m = ld3(X, idx);
+		const F3 H = field_at(a, X, Hd, n, idx, i, j, k, Ms, lap);
+		F3 w = ((dt / 6) * wS) * ld3(lamY, idx);
+		if (lxn) w = w + (cS * dt) * ld3(lxn, idx);
+		const float mh = dot(m, H), mw = dot(m, w), hw = dot(H, w), mm = dot(m, m);
+		const F3 d = pr * (cross(H, w) + (al / Ms) * (mh * w + mw * H - (2.f * hw) * m));
+		const F3 uu = pr * (cross(w, m) + (al / Ms) * (mw * m - mm * w));
+		st3(lx, idx, d);
+		st3(u, idx, uu);
+		const float ce = cex(a, Ms);
+		st3(cu, idx, ce * uu);
+		double g = -(double)pr * (double)al / ((double)Ms * Ms) * (double)dot(cross(m, cross(m, H)), w);
+		if (a.modules & BM_MOD_EXCHANGE) g += (double)dot(uu, lap) * (-2.0 * (double)ce / (double)Ms);
+		gMs[idx] += g;
+		ga += (double)(dot(uu, F3{ a.dir[0], a.dir[1], a.dir[2] }) * a.sc);
+	}
+	red[threadIdx.x] = ga;
+	__syncthreads();
+	for (int o = kAB / 2; o > 0; o >>= 1) { if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o]; __syncthreads(); }
+	if (threadIdx.x == 0) apart[blockIdx.x] += red[0];
+}
+
+// Stage VJP, part 2: lx += N u (planar) + Lap(c u); lacc += lx
+__global__ void __launch_bounds__(kAB) k_vjp_b(AdjArgs a, const float* __restrict__ Nu, const float* __restrict__ cu,
+	float* __restrict__ lx, float* __restrict__ lacc)
+{
+	const size_t n = (size_t)a.nx * a.ny * a.nz;
+	for (size_t idx = blockIdx.x * (size_t)kAB + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kAB) {
+		int i, j, k; cell_ijk(a, idx, i, j, k);
+		F3 v = ld3(lx, idx);
+		if (a.modules & BM_MOD_DEMAG) v = v + F3{ Nu[idx], Nu[n + idx], Nu[2 * n + idx] };
+		if (a.modules & BM_MOD_EXCHANGE) v = v + lap_at(cu, a, idx, i, j, k);
+		st3(lx, idx, v);
+		st3(lacc, idx, ld3(lacc, idx) + v);
+	}
+}
+
+__global__ void k_m0_term(const float* m0, const float* lam, double* gMs, size_t n)
+{
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		gMs[i] += (double)dot(ld3(m0, i), ld3(lam, i));
+}
+
+__global__ void k_to_float(const double* g, float* out, size_t n)
+{
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = (float)g[i];
+}
+
+} // namespace
+
+struct bm_adj_s {
+	int nx, ny, nz; size_t n;
+	double h[3];
+	bm_fm_params p;
+	bm_demag_t demag;
+	const float* Ms;
+	double dt;
+	int nsteps, K, nck;
+	int grid = 1024;
+	// device buffers
+	float* ck = nullptr;          // checkpoints: nck x 3n
+	float* tape = nullptr;        // K steps x (X0..X3, Y) x 3n, then K x 4 demag fields x 3n
+	float* Mw = nullptr;          // work states: 2 x 3n
+	float* acc = nullptr;
+	float* Hd = nullptr;          // planar 3n (forward without tape)
+	float* Xs = nullptr;          // stage inputs without tape: 3 x 3n
+	float* lam = nullptr; float* lamY = nullptr; float* lacc = nullptr; float* lx = nullptr; float* lxn = nullptr;
+	float* u = nullptr; float* cu = nullptr; float* Nu = nullptr;
+	double* gMs = nullptr;
+	double* part = nullptr;       // per-block partials (loss, amp)
+	double* h_part = nullptr;     // pinned
+	const float* m0 = nullptr; const float* mt = nullptr;
+	bool forward_done = false;
+};
+
+namespace {
+
+AdjArgs adj_args(bm_adj_s* o, int step)
+{
+	AdjArgs a;
+	std::memset(&a, 0, sizeof(a));
+	a.nx = o->nx; a.ny = o->ny; a.nz = o->nz;
+	a.ihx2 = (float)(1.0 / (o->h[0] * o->h[0])); a.ihy2 = (float)(1.0 / (o->h[1] * o->h[1])); a.ihz2 = (float)(1.0 / (o->h[2] * o->h[2]));
+	a.A = (float)o->p.A; a.alpha = (float)o->p.alpha; a.grel = (float)o->p.grel;
+	const double t = step * o->dt;
+	double sc = 0.0;
+	if (o->p.wave == 1) { const double x = o->p.w * (t - o->p.t0); sc = (std::fabs(x) < 1e-300) ? 1.0 : std::sin(x) / x; }
+	for (int c = 0; c < 3; c++) {
+		a.Hz[c] = (float)(o->p.Ha[c] + (o->p.wave == 1 ? o->p.amp * o->p.dir[c] * sc : 0.0) + (o->p.wave == 2 ? o->p.slope[c] * t : 0.0));
+		a.dir[c] = (float)o->p.dir[c];
+	}
+	a.sc = (o->p.wave == 1 && (o->p.modules & BM_MOD_ZEEMAN)) ? (float)sc : 0.f;
+	a.modules = o->p.modules;
+	a.Ms = o->Ms;
+	return a;
+}
+
+size_t n3(bm_adj_s* o) { return o->n * 3; }
+float* tX(bm_adj_s* o, int q, int s) { return o->tape + ((size_t)q * 5 + s) * n3(o); }                 // s = 0..3 stage inputs, 4 = Y
+float* tH(bm_adj_s* o, int q, int s) { return o->tape + (size_t)o->K * 5 * n3(o) + ((size_t)q * 4 + s) * n3(o); }
+
+// one forward step from Mn into Mnext; with taping (q >= 0) the stage inputs, their demag
+// fields and Y go to tape slot q
+int fwd_step(bm_adj_s* o, int step, const float* Mn, float* Mnext, int q, hipStream_t st)
+{
+	const AdjArgs a = adj_args(o, step);
+	const float dt = (float)o->dt;
+	const float* X = Mn;
+	const dim3 g(o->grid), b(kAB);
+	for (int s = 0; s < 4; s++) {
+		float* Hd = q >= 0 ? tH(o, q, s) : o->Hd;
+		if (q >= 0 && s == 0) BM_HIP(hipMemcpyAsync(tX(o, q, 0), Mn, n3(o) * sizeof(float), hipMemcpyDeviceToDevice, st));
+		if (o->p.modules & BM_MOD_DEMAG) { int rc = demag_planar(o->demag, X, Hd, st); if (rc) return rc; }
+		float* Xn = s < 3 ? (q >= 0 ? tX(o, q, s + 1) : o->Xs + (size_t)s * n3(o)) : nullptr;
+		float* Y = (s == 3 && q >= 0) ? tX(o, q, 4) : nullptr;
+		switch (s) {
+		case 0: hipLaunchKernelGGL(k_fwd<0>, g, b, 0, st, a, X, Hd, Mn, o->acc, Xn, Y, Mnext, dt); break;
+		case 1: hipLaunchKernelGGL(k_fwd<1>, g, b, 0, st, a, X, Hd, Mn, o->acc, Xn, Y, Mnext, dt); break;
+		case 2: hipLaunchKernelGGL(k_fwd<2>, g, b, 0, st, a, X, Hd, Mn, o->acc, Xn, Y, Mnext, dt); break;
+		default: hipLaunchKernelGGL(k_fwd<3>, g, b, 0, st, a, X, Hd, Mn, o->acc, Xn, Y, Mnext, dt); break;
+		}
+		X = Xn;
+	}
+	return BM_OK;
+}
+
+// adjoint of one step with tape slot q: lam (dL/dM_{step+1}) -> lam (dL/dM_step)
+int adj_step(bm_adj_s* o, int step, int q, hipStream_t st)
+{
+	const AdjArgs a = adj_args(o, step);
+	const float dt = (float)o->dt;
+	const dim3 g(o->grid), b(kAB);
+	static const float wgt[4] = { 1, 2, 2, 1 }, cst[3] = { 0.5f, 0.5f, 1.0f };
+	hipLaunchKernelGGL(k_adj_renorm, g, b, 0, st, tX(o, q, 4), o->Ms, o->lam, o->lamY, o->lacc, o->gMs, o->n);
+	for (int s = 3; s >= 0; s--) {
+		const float* lxn = s < 3 ? o->lxn : nullptr;
+		hipLaunchKernelGGL(k_vjp_a, g, b, 0, st, a, tX(o, q, s), tH(o, q, s), o->lamY, lxn, wgt[s], s < 3 ? cst[s] : 0.f, dt,
+			o->lx, o->u, o->cu, o->gMs, o->part + o->grid);
+		if (o->p.modules & BM_MOD_DEMAG) { int rc = demag_planar(o->demag, o->u, o->Nu, st); if (rc) return rc; }
+		hipLaunchKernelGGL(k_vjp_b, g, b, 0, st, a, o->Nu, o->cu, o->lx, o->lacc);
+		std::swap(o->lx, o->lxn);
+	}
+	std::swap(o->lam, o->lacc);
+	return BM_OK;
+}
+
+void free_all(bm_adj_s* o)
+{
+	float* fl[] = { o->ck, o->tape, o->Mw, o->acc, o->Hd, o->Xs, o->lam, o->lamY, o->lacc, o->lx, o->lxn, o->u, o->cu, o->Nu };
+	for (float* p : fl) if (p) (void)hipFree(p);
+	if (o->gMs) (void)hipFree(o->gMs);
+	if (o->part) (void)hipFree(o->part);
+	if (o->h_part) (void)hipHostFree(o->h_part);
+}
+
+} // namespace
+
+extern "C" int bm_adj_create(int nx, int ny, int nz, double hx, double hy, double hz, const bm_fm_params* p,
+	bm_demag_t demag, const float* Ms_map, double dt, int nsteps, int ckpt_every, bm_adj_t* out)
+{
+	if (!out || !p || !Ms_map || nx < 1 || ny < 1 || nz < 1 || !(dt > 0) || nsteps < 1 || ckpt_every < 1)
+		return fail(BM_ERR_ARG, "bm_adj_create: bad arguments");
+	*out = nullptr;
+	if ((p->modules & BM_MOD_DEMAG) && (!demag || demag->nx != nx || demag->ny != ny || demag->nz != nz))
+		return fail(BM_ERR_ARG, "bm_adj_create: demag handle missing or of another mesh size");
+	if (p->modules & BM_MOD_ANIS_UNI) return fail(BM_ERR_ARG, "bm_adj_create: anisotropy is not differentiated");
+	bm_adj_s* o = new bm_adj_s();
+	o->nx = nx; o->ny = ny; o->nz = nz; o->n = (size_t)nx * ny * nz;
+	o->h[0] = hx; o->h[1] = hy; o->h[2] = hz; o->p = *p; o->demag = demag; o->Ms = Ms_map;
+	o->dt = dt; o->nsteps = nsteps; o->K = std::min(ckpt_every, nsteps);
+	o->nck = (nsteps + o->K - 1) / o->K + 1;
+	o->grid = (int)std::max<size_t>(1, std::min<size_t>((o->n + kAB - 1) / kAB, (size_t)device_cu_count() * 8));
+	const size_t b3 = n3(o) * sizeof(float);
+	bool ok = hipMalloc(&o->ck, (size_t)o->nck * b3) == hipSuccess
+		&& hipMalloc(&o->tape, (size_t)o->K * 9 * b3) == hipSuccess
+		&& hipMalloc(&o->Mw, 2 * b3) == hipSuccess && hipMalloc(&o->acc, b3) == hipSuccess
+		&& hipMalloc(&o->Hd, b3) == hipSuccess && hipMalloc(&o->Xs, 3 * b3) == hipSuccess
+		&& hipMalloc(&o->lam, b3) == hipSuccess && hipMalloc(&o->lamY, b3) == hipSuccess && hipMalloc(&o->lacc, b3) == hipSuccess
+		&& hipMalloc(&o->lx, b3) == hipSuccess && hipMalloc(&o->lxn, b3) == hipSuccess
+		&& hipMalloc(&o->u, b3) == hipSuccess && hipMalloc(&o->cu, b3) == hipSuccess && hipMalloc(&o->Nu, b3) == hipSuccess
+		&& hipMalloc(&o->gMs, o->n * sizeof(double)) == hipSuccess
+		&& hipMalloc(&o->part, 2 * (size_t)o->grid * sizeof(double)) == hipSuccess
+		&& hipHostMalloc(&o->h_part, 2 * (size_t)o->grid * sizeof(double)) == hipSuccess;
+	if (!ok) { free_all(o); delete o; return fail(BM_ERR_NOMEM, "bm_adj_create: device memory"); }
+	*out = o;
+	return BM_OK;
+}
+
+extern "C" int bm_adj_destroy(bm_adj_t o)
+{
+	if (!o) return BM_OK;
+	(void)hipDeviceSynchronize();
+	free_all(o);
+	delete o;
+	return BM_OK;
+}
+
+extern "C" int bm_adj_forward(bm_adj_t o, const float* m0, const float* mt, float* M_final, double* loss, void* stream)
+{
+	if (!o || !m0 || !mt) return fail(BM_ERR_ARG, "bm_adj_forward: null argument");
+	hipStream_t st = (hipStream_t)stream;
+	o->m0 = m0; o->mt = mt;
+	const dim3 g(o->grid), b(kAB);
+	hipLaunchKernelGGL(k_init, g, b, 0, st, o->ck, o->Ms, m0, o->n);
+	float* cur = o->ck;
+	for (int s = 0; s < o->nsteps; s++) {
+		const bool at_ck = (s + 1) % o->K == 0 || s + 1 == o->nsteps;
+		float* next = at_ck ? o->ck + (size_t)((s + 1 + o->K - 1) / o->K) * n3(o) : (cur == o->Mw ? o->Mw + n3(o) : o->Mw);
+		int rc = fwd_step(o, s, cur, next, -1, st);
+		if (rc) return rc;
+		cur = next;
+	}
+	if (M_final) BM_HIP(hipMemcpyAsync(M_final, cur, n3(o) * sizeof(float), hipMemcpyDeviceToDevice, st));
+	hipLaunchKernelGGL(k_loss_part, g, b, 0, st, cur, mt, o->Ms, o->n, o->part);
+	BM_HIP(hipMemcpyAsync(o->h_part, o->part, (size_t)o->grid * sizeof(double), hipMemcpyDeviceToHost, st));
+	BM_HIP(hipStreamSynchronize(st));
+	double sum = 0;
+	for (int q = 0; q < o->grid; q++) sum += o->h_part[q];
+	if (loss) *loss = 1.0 - sum / (double)o->n;
+	o->forward_done = true;
+	BM_HIP(hipGetLastError());
+	return BM_OK;
+}
+
+extern "C" int bm_adj_backward(bm_adj_t o, float* gMs_out, double* gamp, void* stream)
+{
+	if (!o || !gMs_out) return fail(BM_ERR_ARG, "bm_adj_backward: null argument");
+	if (!o->forward_done) return fail(BM_ERR_STATE, "bm_adj_backward: run bm_adj_forward first");
+	hipStream_t st = (hipStream_t)stream;
+	const dim3 g(o->grid), b(kAB);
+	const int nseg = o->nck - 1;
+	float* Mfin = o->ck + (size_t)nseg * n3(o);
+	hipLaunchKernelGGL(k_final, g, b, 0, st, Mfin, o->mt, o->Ms, o->n, o->lam, o->gMs);
+	BM_HIP(hipMemsetAsync(o->part + o->grid, 0, (size_t)o->grid * sizeof(double), st));
+	for (int sg = nseg - 1; sg >= 0; sg--) {
+		const int s0 = sg * o->K, s1 = std::min(o->nsteps, s0 + o->K);
+		// recompute the segment from its checkpoint, taping every step
+		const float* cur = o->ck + (size_t)sg * n3(o);
+		for (int s = s0; s < s1; s++) {
+			float* next = (cur == o->Mw) ? o->Mw + n3(o) : o->Mw;
+			int rc = fwd_step(o, s, cur, next, s - s0, st);
+			if (rc) return rc;
+			cur = next;
+		}
+		for (int s = s1 - 1; s >= s0; s--) {
+			int rc = adj_step(o, s, s - s0, st);
+			if (rc) return rc;
+		}
+	}
+	hipLaunchKernelGGL(k_m0_term, g, b, 0, st, o->m0, o->lam, o->gMs, o->n);
+	hipLaunchKernelGGL(k_to_float, g, b, 0, st, o->gMs, gMs_out, o->n);
+	BM_HIP(hipMemcpyAsync(o->h_part, o->part + o->grid, (size_t)o->grid * sizeof(double), hipMemcpyDeviceToHost, st));
+	BM_HIP(hipStreamSynchronize(st));
+	double ga = 0;
+	for (int q = 0; q < o->grid; q++) ga += o->h_part[q];
+	if (gamp) *gamp = ga;
+	BM_HIP(hipGetLastError());
+	return BM_OK;
+}

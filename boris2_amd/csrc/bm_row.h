@@ -22,10 +22,10 @@
 
 namespace bm {
 
-template <int HX, int TY, int METHOD, int STEP, bool R2C>
-__global__ void __launch_bounds__(kXBlock) k_row(StageArgs a, float2* S, int Sp, int ntiles, float scale)
+template <int HX, int TY, int METHOD, int STEP, bool R2C, int BS = kXBlock>
+__global__ void __launch_bounds__(BS) k_row(StageArgs a, float2* S, int Sp, int ntiles, float scale)
 {
-	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kXBlock / T, RPITCH = xrow_pitch<HX>(), NG = TY / RP, EH = E / 2;
+	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = BS / T, RPITCH = xrow_pitch<HX>(), NG = TY / RP, EH = E / 2;
 	static_assert(NG * RP == TY, "tile height must be a multiple of the row group");
 	__shared__ C2<float> tile[TY * RPITCH];
 	XThread<HX> th; th.init();
@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(kXBlock) k_row(StageArgs a, float2* S, int Sp,
 		const int z = w % nz, y0 = (w / nz) * TY;
 		// cells 2k and 2k+1 of this thread's rows: component c in h[g][e][c] (.x, .y)
 		float2 h[NG][EH][3];
-		XTileRegs<HX, TY> pre;
+		XTileRegs<HX, TY, BS> pre;
 		pre.load(S, 0, z, y0, ny, nz, Sp);
 #pragma unroll 1
 		for (int c = 0; c < 3; c++) {
@@ -96,12 +96,12 @@ __global__ void __launch_bounds__(kXBlock) k_row(StageArgs a, float2* S, int Sp,
 					r2c_row<HX>(th, tile + (g * RP + th.rl) * RPITCH, v);
 				}
 				__syncthreads();
-				tile_to_S<HX, TY>(tile, S, c, z, y0, ny, nz, Sp);
+				tile_to_S<HX, TY, BS>(tile, S, c, z, y0, ny, nz, Sp);
 				__syncthreads();
 			}
 		}
 	}
-	if (stage_has_lte<METHOD, STEP>()) block_max_to(lte_loc, a.lte);
+	if (stage_has_lte<METHOD, STEP>()) block_max_to<BS>(lte_loc, a.lte);
 }
 
 // Grid of the row kernel: a multiple of 8 blocks (the XCD split needs it), about two per CU.

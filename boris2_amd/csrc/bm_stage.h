@@ -83,16 +83,18 @@ struct StageArgs {
 enum { M_RKDP = 0, M_RK4 = 1, M_TEULER = 2, M_AHEUN = 3 };
 
 // Block-wide max of a non-negative float, folded into *dst with an integer atomicMax on
-// its bit pattern (order-independent, so deterministic).  All kBS threads must call it.
+// its bit pattern (order-independent, so deterministic).  All NT threads of the block must
+// call it.
+template <int NT = kBS>
 __device__ __forceinline__ void block_max_to(float v, unsigned* dst)
 {
-	__shared__ float red[kBS / 64];
+	__shared__ float red[NT / 64];
 	for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
 	if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
 	__syncthreads();
 	if (threadIdx.x == 0) {
 		float m = red[0];
-		for (int q = 1; q < kBS / 64; q++) m = fmaxf(m, red[q]);
+		for (int q = 1; q < NT / 64; q++) m = fmaxf(m, red[q]);
 		atomicMax(dst, __float_as_uint(m));
 	}
 }

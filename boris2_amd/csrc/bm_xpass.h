@@ -92,12 +92,13 @@ __device__ __forceinline__ void c2r_row(const XThread<HX>& th, C2<float>* lrow, 
 	fft<HX, 1, float>(v, lrow, 1, th.t, th.tw);               // first LDS write waits on a barrier
 }
 
-// Transposed copy of a tile of TY spectrum rows (component c, plane z, rows y0..) to S.
-template <int HX, int TY>
+// Transposed copy of a tile of TY spectrum rows (component c, plane z, rows y0..) to S by a
+// block of BS threads.
+template <int HX, int TY, int BS = kXBlock>
 __device__ __forceinline__ void tile_to_S(const C2<float>* tile, float2* S, int c, int z, int y0, int ny, int nz, int Sp)
 {
 	constexpr int NC = HX + 1, RPITCH = xrow_pitch<HX>();
-	for (int idx = threadIdx.x; idx < TY * NC; idx += kXBlock) {
+	for (int idx = threadIdx.x; idx < TY * NC; idx += BS) {
 		const int yl = idx % TY, kx = idx / TY, y = y0 + yl;
 		if (y < ny) {
 			C2<float> a = tile[yl * RPITCH + kx];
@@ -108,15 +109,15 @@ __device__ __forceinline__ void tile_to_S(const C2<float>* tile, float2* S, int 
 
 // Register staging of a tile of TY spectrum rows read from S (transposed), so the next
 // tile's loads can be in flight while the current one is transformed.
-template <int HX, int TY> struct XTileRegs {
-	static constexpr int NC = HX + 1, N = (TY * NC + kXBlock - 1) / kXBlock;
+template <int HX, int TY, int BS = kXBlock> struct XTileRegs {
+	static constexpr int NC = HX + 1, N = (TY * NC + BS - 1) / BS;
 	float2 r[N];
 
 	__device__ __forceinline__ void load(const float2* S, int c, int z, int y0, int ny, int nz, int Sp)
 	{
 #pragma unroll
 		for (int q = 0; q < N; q++) {
-			const int idx = threadIdx.x + kXBlock * q;
+			const int idx = threadIdx.x + BS * q;
 			const int yl = idx % TY, kx = idx / TY, y = y0 + yl;
 			r[q] = (idx < TY * NC && y < ny) ? S[(((size_t)c * nz + z) * NC + kx) * (size_t)Sp + y] : make_float2(0.f, 0.f);
 		}
@@ -126,7 +127,7 @@ template <int HX, int TY> struct XTileRegs {
 		constexpr int RPITCH = xrow_pitch<HX>();
 #pragma unroll
 		for (int q = 0; q < N; q++) {
-			const int idx = threadIdx.x + kXBlock * q;
+			const int idx = threadIdx.x + BS * q;
 			if (idx < TY * NC) tile[(idx % TY) * RPITCH + idx / TY] = mk(r[q].x, r[q].y);
 		}
 	}
