@@ -74,9 +74,9 @@ def algorithmic_bytes(kind: str, n, N) -> float:
 
 
 # rocprof kernel-name prefix of each launch kind (for the PMC traffic table in profiles/)
-KIND_TO_KERNEL = {"fwd_x_r2c": "k_fwd_x<256, 32, false>", "y_fft": "k_y<1024, -1>", "z_fft_kernel_ifft": "k_z<",
+KIND_TO_KERNEL = {"fwd_x_r2c": "k_fwd_x<512, 16, false>", "y_fft": "k_y<1024, -1>", "z_fft_kernel_ifft": "k_z<",
                   "y_ifft": "k_y<1024, 1>", "inv_x_c2r": "k_inv_x<", "llg_stage": "k_stage<", "rkdp_advance": "k_rkdp_advance",
-                  "row_c2r_stage_r2c": "k_row<", "fwd_x_advance": "k_fwd_x<256, 32, true>"}
+                  "row_c2r_stage_r2c": "k_row<", "fwd_x_advance": "k_fwd_x<512, 16, true>"}
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_rkdp54_512x512x32.json"
 
 
