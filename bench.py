@@ -61,22 +61,22 @@ def algorithmic_bytes(kind: str, n, N) -> float:
         "y_ifft": full + half,
         "inv_x_c2r": half + 12.0 * cells,
         "y2d_fft_kernel_ifft": half + 16.0 * NC * NYH + half,
-        # RKDP54 stages 0..5 move 9,6,7,8,9,9 float3 arrays of the mesh: 8 on average
-        "llg_stage": 8 * 12.0 * cells,
+        # RKDP54 stages 0..5 move 4,6,7,8,9,10 float3 arrays of the mesh (planar Hd included)
+        "llg_stage": 44 / 6 * 12.0 * cells,
         "rkdp_advance": 4 * 12.0 * cells,
         # fused row kernel, average over the 6 launches of an accepted RKDP54 step: stage 0
-        # reads the spectra rows + 8 float3 arrays (M, sM1, E0, E2..E5 in, E0 out); stages
-        # 1..5 read and write the spectra rows and move 5,6,7,8,8 arrays
-        "row_c2r_stage_r2c": (11 * half + 42 * 12.0 * cells) / 6,
+        # reads the spectra rows + 3 float3 arrays (M and the partial embedded solution in,
+        # E0 out); stages 1..5 read and write the spectra rows and move 5,6,7,8,9 arrays
+        "row_c2r_stage_r2c": (11 * half + 38 * 12.0 * cells) / 6,
         # x-R2C fused with the RKDP54 advance: M, E0 in; sM1, M1 out; spectra rows out
         "fwd_x_advance": 4 * 12.0 * cells + half,
     }[kind]
 
 
 # rocprof kernel-name prefix of each launch kind (for the PMC traffic table in profiles/)
-KIND_TO_KERNEL = {"fwd_x_r2c": "k_fwd_x<512, 16, false>", "y_fft": "k_y<1024, -1>", "z_fft_kernel_ifft": "k_z<",
+KIND_TO_KERNEL = {"fwd_x_r2c": "k_fwd_x<512, 8, false>", "y_fft": "k_y<1024, -1>", "z_fft_kernel_ifft": "k_z<",
                   "y_ifft": "k_y<1024, 1>", "inv_x_c2r": "k_inv_x<", "llg_stage": "k_stage<", "rkdp_advance": "k_rkdp_advance",
-                  "row_c2r_stage_r2c": "k_row<", "fwd_x_advance": "k_fwd_x<512, 16, true>"}
+                  "row_c2r_stage_r2c": "k_row<", "fwd_x_advance": "k_fwd_x<512, 8, true>"}
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_rkdp54_512x512x32.json"
 
 

@@ -131,12 +131,12 @@ int build_tables(bm_batch_s* o)
 			bm_demag_s* d = o->dm[b];
 			for (int par = 0; par < 2; par++) {
 				FwdArgs f{};
-				f.M = (par ? o->Mb : o->M) + 3 * o->off[b]; f.S = d->S;
-				f.ny = d->ny; f.nz = d->nz; f.Sp = d->Sp; f.ntiles = (d->ny + tf - 1) / tf;
+				f.M = (par ? o->Mb : o->M) + 3 * o->off[b]; f.Sx = d->Sx;
+				f.ny = d->ny; f.nz = d->nz; f.ntl = d->ntl; f.ntiles = (d->ny + tf - 1) / tf;
 				o->fwd[par][g].h.push_back(f);
 			}
 			PassBatch q{};
-			q.S = d->S; q.out = o->Hd + 3 * o->off[b]; q.ny = d->ny; q.nz = d->nz; q.Sp = d->Sp; q.ntiles = (d->ny + ti - 1) / ti;
+			q.Sx = d->Sx; q.out = o->Hd + 3 * o->off[b]; q.ny = d->ny; q.nz = d->nz; q.ntl = d->ntl; q.ntiles = (d->ny + ti - 1) / ti;
 			q.scale = (float)(1.0 / ((double)d->Nx * d->Ny * d->Nz));
 			o->inv[g].h.push_back(q);
 		}
@@ -144,7 +144,7 @@ int build_tables(bm_batch_s* o)
 	for (size_t g = 0; g < o->gy.size(); g++)
 		for (int b : o->gy[g].meshes) {
 			bm_demag_s* d = o->dm[b];
-			PassBatch q{}; q.S = d->S; q.Sp = d->Sp; q.nlines = 3 * d->nz * d->NC;
+			PassBatch q{}; q.S = d->S; q.Sx = d->Sx; q.Sp = d->Sp; q.NC = d->NC; q.ntl = d->ntl; q.nlines = 3 * d->nz * d->NC;
 			o->ty[g].h.push_back(q);
 		}
 	for (size_t g = 0; g < o->gz.size(); g++)
@@ -156,7 +156,7 @@ int build_tables(bm_batch_s* o)
 	for (size_t g = 0; g < o->gy2.size(); g++)
 		for (int b : o->gy2[g].meshes) {
 			bm_demag_s* d = o->dm[b];
-			PassBatch q{}; q.S = d->S; q.K = d->K; q.NC = d->NC; q.NYH = d->NYH; q.Sp = d->Sp;
+			PassBatch q{}; q.Sx = d->Sx; q.K = d->K; q.NC = d->NC; q.NYH = d->NYH; q.ntl = d->ntl;
 			o->ty2[g].h.push_back(q);
 		}
 	for (int par = 0; par < 2; par++) {

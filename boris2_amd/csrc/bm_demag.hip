@@ -6,16 +6,17 @@
 // hand-written LDS FFT passes with the zero-padded planes never read or written and the
 // kernel multiply fused into the z pass (3D) or the y pass (2D).
 //
-// Spectral layout (one complex FP32 buffer S, y fastest — the reference's xy-transposed
-// order):  S[c][z][kx][ky], c = component, z < nz (only the planes that hold data),
-// kx < NC = Nx/2+1, ky < Ny.
+// Spectral layouts (complex FP32), c = component, z < nz (only the planes that hold data),
+// kx < NC = Nx/2+1:
+//   Sx  x-side spectra of the ny data rows, row-tiled: Sx[c][z][y/8][kx][y%8] (bm_xpass.h);
+//   S   full spectra (3D), y fastest — the reference's xy-transposed order: S[c][z][kx][ky<Ny].
 //
-//   3D:  P1 x-R2C      M (AoS) -> S[.][z][kx][y<ny]           (rows of a z plane, LDS transpose)
-//        P2 y-FFT      S lines (c,z,kx): ny in -> Ny out
-//        P3 z-FFT * K * z-IFFT  per (kx,ky) column: nz in -> nz out
-//        P4 y-IFFT     Ny in -> ny out
-//        P5 x-C2R      -> H (planar or AoS, set or add), scaled by 1/(Nx Ny Nz)
-//   2D:  P1, P2' (y-FFT * K * y-IFFT per kx line, 3 components), P5.
+//   3D:  P1 x-R2C      M (AoS) -> Sx                      (rows of a z plane through LDS)
+//        P2 y-FFT      Sx lines (c,z,kx): ny in -> S lines: Ny out
+//        P3 z-FFT * K * z-IFFT  per (kx,ky) column of S: nz in -> nz out
+//        P4 y-IFFT     S: Ny in -> Sx: ny out
+//        P5 x-C2R      Sx -> H (planar or AoS, set or add), scaled by 1/(Nx Ny Nz)
+//   2D:  P1, P2' (y-FFT * K * y-IFFT per kx line of Sx, 3 components, in place), P5.
 //
 // Kernel setup runs once on the GPU in FP64 (Newell f/g grids, 27-point stencil or
 // far-field series, mirror placement, three real-input FFT passes) and stores FP32
@@ -44,7 +45,7 @@ constexpr int kBlock = 256;
 
 // ---------------------------------------------------------------------------------------
 // P1: x-direction real-to-complex FFT of TY rows of one z plane (all three components, so M
-// is read from HBM once), transposed into S through LDS.  With ADV it also performs the
+// is read from HBM once), through LDS into Sx.  With ADV it also performs the
 // RKDP54 advance on the way: sM1 = M, Mout = M + E0 dT/5, and transforms Mout.
 
 template <int HX, int TY, bool ADV>
@@ -103,7 +104,7 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x(FwdArgs a, const FwdArgs* __re
 				r2c_row<HX>(th, tile + (g * RP + th.rl) * RPITCH, v);
 			}
 			__syncthreads();
-			tile_to_S<HX, TY>(tile, a.S, c, z, y0, a.ny, a.nz, a.Sp);
+			tile_to_S<HX, TY>(tile, a.Sx, c, z, y0, a.ny, a.nz, a.ntl);
 			__syncthreads();
 		}
 	}
@@ -113,11 +114,11 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x(FwdArgs a, const FwdArgs* __re
 // P5: x-direction complex-to-real inverse FFT.  Output modes: 0 planar set, 1 AoS set,
 // 2 AoS add; EN also forms per-work-item partial sums of M.H for the energy.
 template <int HX, int MODE, bool EN, int TY>
-__global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ S, float* __restrict__ out,
-	int ny, int nz, int Sp, int ntiles, float scale, const float* __restrict__ M, double* __restrict__ epart,
+__global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ Sx, float* __restrict__ out,
+	int ny, int nz, int ntl, int ntiles, float scale, const float* __restrict__ M, double* __restrict__ epart,
 	const PassBatch* __restrict__ bt)
 {
-	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; out = b.out; ny = b.ny; nz = b.nz; Sp = b.Sp; ntiles = b.ntiles; scale = b.scale; }
+	if (bt) { const PassBatch& b = bt[blockIdx.y]; Sx = b.Sx; out = b.out; ny = b.ny; nz = b.nz; ntl = b.ntl; ntiles = b.ntiles; scale = b.scale; }
 	__shared__ double ered[EN ? kBlock : 1];
 	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, RPITCH = xrow_pitch<HX>();
 	__shared__ C2<float> tile[TY * RPITCH];
@@ -127,7 +128,7 @@ __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ S, 
 	// the next work item's tile is loaded into registers while this one is transformed
 	XTileRegs<HX, TY> pre;
 	const int nwork = 3 * nz * ntiles;
-	auto fetch = [&](int wf) { pre.load(S, wf % 3, (wf / 3) % nz, (wf / (3 * nz)) * TY, ny, nz, Sp); };
+	auto fetch = [&](int wf) { pre.load(Sx, wf % 3, (wf / 3) % nz, (wf / (3 * nz)) * TY, ny, nz, ntl); };
 	if ((int)blockIdx.x < nwork) fetch(blockIdx.x);
 	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
 		const int c = w % 3, z = (w / 3) % nz, y0 = (w / (3 * nz)) * TY;
@@ -172,11 +173,18 @@ __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ S, 
 }
 
 // ---------------------------------------------------------------------------------------
-// P2 / P4 (3D): y-direction FFT of lines S[c][z][kx][0..Ny).
+// P2 / P4 (3D): y-direction FFT of lines (c, z, kx), line = (c nz + z) NC + kx.  DIR -1:
+// the ny = Ny/2 data rows of a line come from the row-tiled Sx and its Ny transformed values
+// go to line `line` of S; DIR +1: S line in, first ny outputs to Sx.  A block takes LP
+// consecutive lines, one per group of T lanes: a wave moves its line's S values as whole
+// runs and its Sx values as kLT-value pieces, the block's waves together covering runs of
+// LP * kLT consecutive Sx values.  (Measured at 512x512x32: staging the Sx side through LDS
+// so that each wave moves whole runs, or interleaving the lanes over the lines, is slower.)
 template <int NY, int DIR>
-__global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, int nlines, int Sp, const PassBatch* __restrict__ bt)
+__global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, float2* __restrict__ Sx, int nlines, int Sp, int NC, int ntl,
+	const PassBatch* __restrict__ bt)
 {
-	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; nlines = b.nlines; Sp = b.Sp; }
+	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; Sx = b.Sx; nlines = b.nlines; Sp = b.Sp; NC = b.NC; ntl = b.ntl; }
 	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = kBlock / T, LL = lds_len<NY>();
 	__shared__ C2<float> lds[LP * LL];
 	const int t = threadIdx.x % T, l = threadIdx.x / T;
@@ -186,25 +194,37 @@ __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, int nlines
 	float2 pre[E];
 	auto fetch = [&](int wf) {
 		const int line = wf * LP + l;
-		const float2* q = S + (size_t)(line < nlines ? line : 0) * Sp;
+		const bool ok = line < nlines;
+		if constexpr (DIR < 0) {
+			const int cz = ok ? line / NC : 0, kx = ok ? line % NC : 0;
 #pragma unroll
-		for (int e = 0; e < E; e++)
-			pre[e] = (line < nlines && (DIR > 0 || e < E / 2)) ? q[t + T * e] : make_float2(0.f, 0.f);
+			for (int e = 0; e < E; e++) pre[e] = (ok && e < E / 2) ? Sx[sx_line(cz, kx, t + T * e, NC, ntl)] : make_float2(0.f, 0.f);
+		}
+		else {
+			const float2* q = S + (size_t)(ok ? line : 0) * Sp;
+#pragma unroll
+			for (int e = 0; e < E; e++) pre[e] = ok ? q[t + T * e] : make_float2(0.f, 0.f);
+		}
 	};
-	if (blockIdx.x < nwork) fetch(blockIdx.x);
+	if ((int)blockIdx.x < nwork) fetch(blockIdx.x);
 	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
 		const int line = w * LP + l;
-		const bool ok = line < nlines;
-		float2* p = S + (size_t)(ok ? line : 0) * Sp;
 		C2<float> v[E];
 #pragma unroll
 		for (int e = 0; e < E; e++) v[e] = mk(pre[e].x, pre[e].y);
 		if (w + (int)gridDim.x < nwork) fetch(w + gridDim.x);
 		fft<NY, DIR, float>(v, lds + l * LL, 1, t, tw);
-		if (ok) {
+		if (line < nlines) {
+			if constexpr (DIR < 0) {
+				float2* p = S + (size_t)line * Sp;
 #pragma unroll
-			for (int e = 0; e < E; e++)
-				if (DIR < 0 || e < E / 2) p[t + T * e] = make_float2(v[e].x, v[e].y);
+				for (int e = 0; e < E; e++) p[t + T * e] = make_float2(v[e].x, v[e].y);
+			}
+			else {
+				const int cz = line / NC, kx = line % NC;
+#pragma unroll
+				for (int e = 0; e < E / 2; e++) Sx[sx_line(cz, kx, t + T * e, NC, ntl)] = make_float2(v[e].x, v[e].y);
+			}
 		}
 	}
 }
@@ -221,7 +241,7 @@ __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, int nlines
 // mirrors of every column.
 template <int NZ>
 __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const float* __restrict__ K,
-	int nz, int NC, int Ny, int NYH, int Sp, const PassBatch* __restrict__ bt)
+	int nz, int NC, int Ny, int NYH, int Sp, int xmap, const PassBatch* __restrict__ bt)
 {
 	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; K = b.K; nz = b.nz; NC = b.NC; Ny = b.Ny; NYH = b.NYH; Sp = b.Sp; }
 	constexpr int E = Plan<NZ>::E, TZ = plan_T<NZ>(), CY = kBlock / TZ, CH = CY / 2, NZH = NZ / 2 + 1, KR = CH + 1;
@@ -261,8 +281,10 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 			pk[q] = (i < KS && rem / NZH < nrows) ? kb[qq * kq + rem] : 0.f;
 		}
 	};
-	if ((int)blockIdx.x < nwork) fetch(blockIdx.x);
-	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+	// xmap: XCD-contiguous order (each XCD sweeps consecutive ky chunks)
+	const int G = gridDim.x, w0 = xmap ? xcd_chunk(blockIdx.x, G) : (int)blockIdx.x;
+	if (w0 < nwork) fetch(w0);
+	for (int w = w0; w < nwork; w += G) {
 		const int kx = w / nwb, k0 = CH * (w % nwb);
 		const int ky = hi ? Ny - (k0 + CH) + c : k0 + c;
 		const bool ok = hi ? (ky >= half && ky < Ny) : (ky < half);
@@ -275,7 +297,7 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 		__syncthreads();                                   // the previous slab is no longer read
 #pragma unroll
 		for (int q = 0; q < NKL; q++) { const int i = threadIdx.x + kBlock * q; if (i < KS) ks[i] = pk[q]; }
-		if (w + (int)gridDim.x < nwork) fetch(w + gridDim.x);
+		if (w + G < nwork) fetch(w + G);
 		fft3<NZ, -1, float>(v[0], v[1], v[2], lds + col, lds + NZ * CY + col, lds + 2 * NZ * CY + col, CY, t, tw);
 		if constexpr (Plan<NZ>::S == 1) __syncthreads();  // single-stage plans have no barrier inside
 		const float sy = (ky <= half) ? 1.f : -1.f;
@@ -307,33 +329,33 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 }
 
 // ---------------------------------------------------------------------------------------
-// P2' (2D): per kx line, y-FFT of 3 components, 2D kernel multiply (Kxy odd in y), y-IFFT.
+// P2' (2D): per kx line of Sx (in place), y-FFT of 3 components, 2D kernel multiply (Kxy
+// odd in y), y-IFFT.  A block takes LP consecutive lines, one per group of T lanes.
 template <int NY, int BS>
-__global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ S, const float* __restrict__ K, int NC, int NYH, int Sp,
+__global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ Sx, const float* __restrict__ K, int NC, int NYH, int ntl,
 	const PassBatch* __restrict__ bt)
 {
-	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; K = b.K; NC = b.NC; NYH = b.NYH; Sp = b.Sp; }
+	if (bt) { const PassBatch& b = bt[blockIdx.y]; Sx = b.Sx; K = b.K; NC = b.NC; NYH = b.NYH; ntl = b.ntl; }
 	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = BS / T, LL = lds_len<NY>();
 	__shared__ C2<float> lds[3 * LP * LL];
 	const int t = threadIdx.x % T, l = threadIdx.x / T;
 	Twiddles<NY, float> tw; tw.init(t);
-	const size_t comp = (size_t)NC * Sp, kq = (size_t)NC * NYH;
+	const size_t kq = (size_t)NC * NYH;
 	const int nwork = (NC + LP - 1) / LP;
 	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
 		const int kx = w * LP + l;
 		const bool ok = kx < NC;
-		float2* p = S + (size_t)(ok ? kx : 0) * Sp;
+		const int kxs = ok ? kx : 0;
 		C2<float> v[3][E];
 #pragma unroll
 		for (int c = 0; c < 3; c++)
 #pragma unroll
 			for (int e = 0; e < E; e++) {
-				float2 a = (ok && e < E / 2) ? p[c * comp + t + T * e] : make_float2(0.f, 0.f);
+				const float2 a = (ok && e < E / 2) ? Sx[sx_line(c, kxs, t + T * e, NC, ntl)] : make_float2(0.f, 0.f);
 				v[c][e] = mk(a.x, a.y);
 			}
 		C2<float>* b0 = lds + l * LL;
 		fft3<NY, -1, float>(v[0], v[1], v[2], b0, b0 + LP * LL, b0 + 2 * LP * LL, 1, t, tw);
-		const int kxs = ok ? kx : 0;
 #pragma unroll
 		for (int e = 0; e < E; e++) {
 			const int ky = t + T * e;
@@ -351,7 +373,7 @@ __global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ S, const float*
 #pragma unroll
 			for (int c = 0; c < 3; c++)
 #pragma unroll
-				for (int e = 0; e < E / 2; e++) p[c * comp + t + T * e] = make_float2(v[c][e].x, v[c][e].y);
+				for (int e = 0; e < E / 2; e++) Sx[sx_line(c, kx, t + T * e, NC, ntl)] = make_float2(v[c][e].x, v[c][e].y);
 		}
 	}
 }
@@ -493,27 +515,29 @@ bool z_supported(int n) { switch (n) {
 	return true; default: return false; } }
 
 int grid_for(int nwork, int ncu, int per_cu) { return std::max(1, std::min(nwork, ncu * per_cu)); }
+// grid of a pass whose blocks take XCD-contiguous work (xcd_chunk): a multiple of 8
+int grid8(int nwork, int ncu, int per_cu) { return (grid_for(nwork, ncu, per_cu) + 7) / 8 * 8; }
 
-// Row-tile height factor of the x passes (1 or 2 times TileY).  Measured at 512x512x32:
-// the forward pass is fastest with the taller tile (its transposed stores become whole
-// 128-byte lines), the inverse pass with the shorter one (its register prefetch of the next
-// tile keeps occupancy up).  BM_XTILE_FWD / BM_XTILE_INV override for tuning runs.
+// Row-tile height factor of the x passes (1 or 2 times TileY).  With the row-tiled Sx every
+// tile is moved as whole runs either way, and the shorter tile (half the LDS, twice the
+// resident blocks) measured faster for both passes at 512x512x32.  BM_XTILE_FWD /
+// BM_XTILE_INV = 2 select the taller tile for tuning runs.
 int xtile_factor(bool fwd)
 {
 	static int f[2] = { -1, -1 };
 	int& v = f[fwd ? 1 : 0];
 	if (v < 0) {
 		const char* e = getenv(fwd ? "BM_XTILE_FWD" : "BM_XTILE_INV");
-		v = e ? (atoi(e) == 2 ? 2 : 1) : (fwd ? 2 : 1);
+		v = (e && atoi(e) == 2) ? 2 : 1;
 	}
 	return v;
 }
 
-int launch_fwd_x(int HX, const float* M, float2* S, int ny, int nz, int Sp, int ncu, hipStream_t st,
+int launch_fwd_x(int HX, const float* M, float2* Sx, int ny, int nz, int ntl, int ncu, hipStream_t st,
 	const float* E0 = nullptr, float* sM1 = nullptr, float* Mout = nullptr, float dT5 = 0.f)
 {
 	const int f = xtile_factor(true);
-	FwdArgs a{ M, S, ny, nz, Sp, 0, E0, sM1, Mout, dT5 };
+	FwdArgs a{ M, Sx, ny, nz, ntl, 0, E0, sM1, Mout, dT5 };
 	const bool adv = E0 != nullptr;
 	switch (HX) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; int TY = TY1 * f; a.ntiles = (ny + TY - 1) / TY; dim3 g(grid_for(nz * a.ntiles, ncu, 2)); \
@@ -528,18 +552,18 @@ int launch_fwd_x(int HX, const float* M, float2* S, int ny, int nz, int Sp, int 
 	return fail(BM_ERR_SIZE, "no x plan");
 }
 
-int launch_inv_x(int HX, int mode, const float2* S, float* out, int ny, int nz, int Sp, float scale, int ncu, hipStream_t st,
+int launch_inv_x(int HX, int mode, const float2* Sx, float* out, int ny, int nz, int ntl, float scale, int ncu, hipStream_t st,
 	const float* M = nullptr, double* epart = nullptr)
 {
 	const int f = (mode == 0) ? xtile_factor(false) : 1;
 	switch (HX) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; int TY = TY1 * f, nt = (ny + TY - 1) / TY; dim3 g(grid_for(3 * nz * nt, ncu, 2)); \
-		if (mode == 0 && f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart, (const PassBatch*)nullptr); \
-		else if (mode == 0) hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart, (const PassBatch*)nullptr); \
-		else if (mode == 1 && !epart) hipLaunchKernelGGL((k_inv_x<V_, 1, false, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart, (const PassBatch*)nullptr); \
-		else if (mode == 1) hipLaunchKernelGGL((k_inv_x<V_, 1, true, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart, (const PassBatch*)nullptr); \
-		else if (!epart) hipLaunchKernelGGL((k_inv_x<V_, 2, false, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart, (const PassBatch*)nullptr); \
-		else hipLaunchKernelGGL((k_inv_x<V_, 2, true, TY1>), g, dim3(kBlock), 0, st, S, out, ny, nz, Sp, nt, scale, M, epart, (const PassBatch*)nullptr); return BM_OK; }
+		if (mode == 0 && f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
+		else if (mode == 0) hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
+		else if (mode == 1 && !epart) hipLaunchKernelGGL((k_inv_x<V_, 1, false, TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
+		else if (mode == 1) hipLaunchKernelGGL((k_inv_x<V_, 1, true, TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
+		else if (!epart) hipLaunchKernelGGL((k_inv_x<V_, 2, false, TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
+		else hipLaunchKernelGGL((k_inv_x<V_, 2, true, TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -556,12 +580,21 @@ int inv_x_nwork(int HX, int ny, int nz)
 	return 0;
 }
 
-int launch_y(int NY, int dir, float2* S, int nlines, int Sp, int ncu, hipStream_t st)
+// z-pass work order switch for tuning runs: BM_Z_XCD=1 selects the XCD-contiguous order.
+int z_xcd() { static int v = getenv("BM_Z_XCD") ? atoi(getenv("BM_Z_XCD")) : 0; return v; }
+
+template <int NY>
+void launch_y_t(int dir, dim3 g, hipStream_t st, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, const PassBatch* bt)
+{
+	if (dir < 0) hipLaunchKernelGGL((k_y<NY, -1>), g, dim3(kBlock), 0, st, S, Sx, nlines, Sp, NC, ntl, bt);
+	else hipLaunchKernelGGL((k_y<NY, 1>), g, dim3(kBlock), 0, st, S, Sx, nlines, Sp, NC, ntl, bt);
+}
+
+int launch_y(int NY, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ncu, hipStream_t st)
 {
 	switch (NY) {
-#define C_(V_) case V_: { int LP = kBlock / plan_T<V_>(); dim3 g(grid_for((nlines + LP - 1) / LP, ncu, 8)); \
-		if (dir < 0) hipLaunchKernelGGL((k_y<V_, -1>), g, dim3(kBlock), 0, st, S, nlines, Sp, (const PassBatch*)nullptr); \
-		else hipLaunchKernelGGL((k_y<V_, 1>), g, dim3(kBlock), 0, st, S, nlines, Sp, (const PassBatch*)nullptr); return BM_OK; }
+#define C_(V_) case V_: { int LP = kBlock / plan_T<V_>(); \
+		launch_y_t<V_>(dir, dim3(grid8((nlines + LP - 1) / LP, ncu, 8)), st, S, Sx, nlines, Sp, NC, ntl, nullptr); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -572,18 +605,18 @@ int launch_z(int NZ, float2* S, const float* K, int nz, int NC, int Ny, int NYH,
 {
 	switch (NZ) {
 #define C_(V_) case V_: { int CH = kBlock / plan_T<V_>() / 2; int nw = NC * ((Ny / 2 + CH - 1) / CH); \
-		hipLaunchKernelGGL(k_z<V_>, dim3(grid_for(nw, ncu, 2)), dim3(kBlock), 0, st, S, K, nz, NC, Ny, NYH, Sp, (const PassBatch*)nullptr); return BM_OK; }
+		hipLaunchKernelGGL(k_z<V_>, dim3(grid8(nw, ncu, 2)), dim3(kBlock), 0, st, S, K, nz, NC, Ny, NYH, Sp, z_xcd(), (const PassBatch*)nullptr); return BM_OK; }
 	BM_FOR_Z_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no z plan");
 }
 
-int launch_y2d(int NY, float2* S, const float* K, int NC, int NYH, int Sp, int ncu, hipStream_t st)
+int launch_y2d(int NY, float2* Sx, const float* K, int NC, int NYH, int ntl, int ncu, hipStream_t st)
 {
 	switch (NY) {
 #define C_(V_) case V_: { constexpr int BS = Plan<V_>::E > 8 ? 128 : 256; int LP = BS / plan_T<V_>(); \
-		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid_for((NC + LP - 1) / LP, ncu, 8)), dim3(BS), 0, st, S, K, NC, NYH, Sp, (const PassBatch*)nullptr); return BM_OK; }
+		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid8((NC + LP - 1) / LP, ncu, 8)), dim3(BS), 0, st, Sx, K, NC, NYH, ntl, (const PassBatch*)nullptr); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -689,7 +722,7 @@ int compute_kernels(bm_demag_s* d, hipStream_t st)
 int bm::demag_fwd_x(bm_demag_s* d, const float* M, hipStream_t st)
 {
 	prof_mark_begin(PK_FWD_X, st);
-	int rc = launch_fwd_x(d->nx, M, d->S, d->ny, d->nz, d->Sp, d->ncu, st);
+	int rc = launch_fwd_x(d->nx, M, d->Sx, d->ny, d->nz, d->ntl, d->ncu, st);
 	prof_mark_end(PK_FWD_X, st);
 	return rc;
 }
@@ -698,7 +731,7 @@ int bm::demag_fwd_x(bm_demag_s* d, const float* M, hipStream_t st)
 int bm::demag_fwd_x_advance(bm_demag_s* d, const float* M, const float* E0, float* sM1, float* Mout, float dT5, hipStream_t st)
 {
 	prof_mark_begin(PK_FWD_ADV, st);
-	int rc = launch_fwd_x(d->nx, M, d->S, d->ny, d->nz, d->Sp, d->ncu, st, E0, sM1, Mout, dT5);
+	int rc = launch_fwd_x(d->nx, M, d->Sx, d->ny, d->nz, d->ntl, d->ncu, st, E0, sM1, Mout, dT5);
 	prof_mark_end(PK_FWD_ADV, st);
 	return rc;
 }
@@ -711,12 +744,12 @@ int bm::demag_mid(bm_demag_s* d, hipStream_t st)
 	int rc;
 	if (d->is2d) {
 		prof_mark_begin(PK_Y2D, st);
-		rc = launch_y2d(d->Ny, d->S, d->K, d->NC, d->NYH, d->Sp, d->ncu, st);
+		rc = launch_y2d(d->Ny, d->Sx, d->K, d->NC, d->NYH, d->ntl, d->ncu, st);
 		prof_mark_end(PK_Y2D, st);
 		return rc;
 	}
 	prof_mark_begin(PK_Y_FWD, st);
-	rc = launch_y(d->Ny, -1, d->S, nlines, d->Sp, d->ncu, st);
+	rc = launch_y(d->Ny, -1, d->S, d->Sx, nlines, d->Sp, d->NC, d->ntl, d->ncu, st);
 	prof_mark_end(PK_Y_FWD, st);
 	if (rc) return rc;
 	prof_mark_begin(PK_Z, st);
@@ -724,7 +757,7 @@ int bm::demag_mid(bm_demag_s* d, hipStream_t st)
 	prof_mark_end(PK_Z, st);
 	if (rc) return rc;
 	prof_mark_begin(PK_Y_INV, st);
-	rc = launch_y(d->Ny, 1, d->S, nlines, d->Sp, d->ncu, st);
+	rc = launch_y(d->Ny, 1, d->S, d->Sx, nlines, d->Sp, d->NC, d->ntl, d->ncu, st);
 	prof_mark_end(PK_Y_INV, st);
 	return rc;
 }
@@ -742,7 +775,7 @@ int bm::demag_planar(bm_demag_s* d, const float* M, float* H3n, hipStream_t st)
 	if (rc) return rc;
 	const float scale = (float)(1.0 / ((double)d->Nx * d->Ny * d->Nz));
 	prof_mark_begin(PK_INV_X, st);
-	rc = launch_inv_x(d->nx, 0, d->S, H3n, d->ny, d->nz, d->Sp, scale, d->ncu, st);
+	rc = launch_inv_x(d->nx, 0, d->Sx, H3n, d->ny, d->nz, d->ntl, scale, d->ncu, st);
 	prof_mark_end(PK_INV_X, st);
 	return rc;
 }
@@ -766,16 +799,20 @@ extern "C" int bm_demag_create(int nx, int ny, int nz, double hx, double hy, dou
 		return fail(BM_ERR_SIZE, "bm_demag_create: no FFT plan for this mesh size (supported: nx in {2..2048 of 2^a 3^b plans}, Ny = 2ny likewise, Nz <= 128)");
 	}
 	d->ncu = device_cu_count();
-	// line pitch of the spectra (complex elements); BM_S_PAD adds padding for tuning experiments
+	// line pitch of the full spectra (complex elements); BM_S_PAD adds padding for tuning experiments
 	d->Sp = d->Ny;
 	if (const char* e = getenv("BM_S_PAD")) d->Sp += std::max(0, atoi(e));
-	size_t sbytes = (size_t)3 * nz * d->NC * d->Sp * sizeof(float2);
+	d->ntl = sx_tiles(ny);
+	const size_t sxbytes = (size_t)3 * nz * d->ntl * d->NC * kLT * sizeof(float2);
+	const size_t sbytes = d->is2d ? 0 : (size_t)3 * nz * d->NC * d->Sp * sizeof(float2);
 	d->kcount = d->is2d ? 4LL * d->NC * d->NYH : 6LL * d->NZH * d->NC * d->NYH;
-	if (hipMalloc(&d->S, sbytes) != hipSuccess) { delete d; return fail(BM_ERR_NOMEM, "bm_demag_create: spectra"); }
-	if (hipMalloc(&d->K, d->kcount * sizeof(float)) != hipSuccess) { (void)hipFree(d->S); delete d; return fail(BM_ERR_NOMEM, "bm_demag_create: kernels"); }
 	d->ecap = std::max(1, inv_x_nwork(nx, ny, nz));
-	if (hipMalloc(&d->ebuf, (size_t)d->ecap * sizeof(double)) != hipSuccess) { (void)hipFree(d->S); (void)hipFree(d->K); delete d; return fail(BM_ERR_NOMEM, "bm_demag_create: energy"); }
-	d->bytes = (long long)sbytes + d->kcount * 4 + (long long)d->ecap * 8;
+	if (hipMalloc(&d->Sx, sxbytes) != hipSuccess || (sbytes && hipMalloc(&d->S, sbytes) != hipSuccess)
+		|| hipMalloc(&d->K, d->kcount * sizeof(float)) != hipSuccess || hipMalloc(&d->ebuf, (size_t)d->ecap * sizeof(double)) != hipSuccess) {
+		bm_demag_destroy(d);
+		return fail(BM_ERR_NOMEM, "bm_demag_create: device memory");
+	}
+	d->bytes = (long long)(sxbytes + sbytes) + d->kcount * 4 + (long long)d->ecap * 8;
 	hipStream_t st = nullptr;
 	int rc = compute_kernels(d, st);
 	if (rc) { bm_demag_destroy(d); return rc; }
@@ -786,6 +823,7 @@ extern "C" int bm_demag_create(int nx, int ny, int nz, double hx, double hy, dou
 extern "C" int bm_demag_destroy(bm_demag_t d)
 {
 	if (!d) return BM_OK;
+	if (d->Sx) (void)hipFree(d->Sx);
 	if (d->S) (void)hipFree(d->S);
 	if (d->K) (void)hipFree(d->K);
 	if (d->ebuf) (void)hipFree(d->ebuf);
@@ -813,7 +851,7 @@ extern "C" int bm_demag_convolute(bm_demag_t d, const float* M, float* Heff, int
 		const int np = inv_x_nwork(HX, d->ny, d->nz);
 		if (np > d->ecap) return fail(BM_ERR_STATE, "energy partial buffer too small");
 		prof_mark_begin(PK_INV_X, st);
-		rc = launch_inv_x(HX, clear_out ? 1 : 2, d->S, Heff, d->ny, d->nz, d->Sp, scale, d->ncu, st, M, d->ebuf);
+		rc = launch_inv_x(HX, clear_out ? 1 : 2, d->Sx, Heff, d->ny, d->nz, d->ntl, scale, d->ncu, st, M, d->ebuf);
 		prof_mark_end(PK_INV_X, st);
 		if (rc) return rc;
 		size_t n = (size_t)d->nx * d->ny * d->nz;
@@ -821,7 +859,7 @@ extern "C" int bm_demag_convolute(bm_demag_t d, const float* M, float* Heff, int
 	}
 	else {
 		prof_mark_begin(PK_INV_X, st);
-		rc = launch_inv_x(HX, clear_out ? 1 : 2, d->S, Heff, d->ny, d->nz, d->Sp, scale, d->ncu, st);
+		rc = launch_inv_x(HX, clear_out ? 1 : 2, d->Sx, Heff, d->ny, d->nz, d->ntl, scale, d->ncu, st);
 		prof_mark_end(PK_INV_X, st);
 		if (rc) return rc;
 	}
@@ -847,27 +885,27 @@ extern "C" int bm_demag_copy_kernels(bm_demag_t d, float* host_out, long long co
 
 // ---------------------------------------------------------------------------------------
 // Pass launchers shared with the multilayer convolution (bm_mlayer.hip).
-int bm::pass_fwd_x(int nx, const float* M, float2* S, int ny, int nplanes, int Sp, int ncu, hipStream_t st)
+int bm::pass_fwd_x(int nx, const float* M, float2* Sx, int ny, int nplanes, int ntl, int ncu, hipStream_t st)
 {
 	prof_mark_begin(PK_FWD_X, st);
-	int rc = launch_fwd_x(nx, M, S, ny, nplanes, Sp, ncu, st);
+	int rc = launch_fwd_x(nx, M, Sx, ny, nplanes, ntl, ncu, st);
 	prof_mark_end(PK_FWD_X, st);
 	return rc;
 }
 
-int bm::pass_y(int Ny, int dir, float2* S, int nlines, int Sp, int ncu, hipStream_t st)
+int bm::pass_y(int Ny, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ncu, hipStream_t st)
 {
 	const int kind = dir < 0 ? PK_Y_FWD : PK_Y_INV;
 	prof_mark_begin(kind, st);
-	int rc = launch_y(Ny, dir, S, nlines, Sp, ncu, st);
+	int rc = launch_y(Ny, dir, S, Sx, nlines, Sp, NC, ntl, ncu, st);
 	prof_mark_end(kind, st);
 	return rc;
 }
 
-int bm::pass_inv_x(int nx, int mode, const float2* S, float* out, int ny, int nplanes, int Sp, float scale, int ncu, hipStream_t st)
+int bm::pass_inv_x(int nx, int mode, const float2* Sx, float* out, int ny, int nplanes, int ntl, float scale, int ncu, hipStream_t st)
 {
 	prof_mark_begin(PK_INV_X, st);
-	int rc = launch_inv_x(nx, mode, S, out, ny, nplanes, Sp, scale, ncu, st);
+	int rc = launch_inv_x(nx, mode, Sx, out, ny, nplanes, ntl, scale, ncu, st);
 	prof_mark_end(PK_INV_X, st);
 	return rc;
 }
@@ -937,9 +975,7 @@ int bm::batch_y(int Ny, int dir, const PassBatch* h_tab, const PassBatch* d_tab,
 	switch (Ny) {
 #define C_(V_) case V_: { const int LP = kBlock / plan_T<V_>(); int mx = 1; \
 		for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].nlines + LP - 1) / LP); \
-		const dim3 g(grid_for(mx, ncu, 8), nb); \
-		if (dir < 0) hipLaunchKernelGGL((k_y<V_, -1>), g, dim3(kBlock), 0, st, nullptr, 0, 0, d_tab); \
-		else hipLaunchKernelGGL((k_y<V_, 1>), g, dim3(kBlock), 0, st, nullptr, 0, 0, d_tab); return BM_OK; }
+		launch_y_t<V_>(dir, dim3(grid8(mx, ncu, 8), nb), st, nullptr, nullptr, 0, 0, 0, 0, d_tab); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -951,7 +987,7 @@ int bm::batch_z(int Nz, const PassBatch* h_tab, const PassBatch* d_tab, int nb, 
 	switch (Nz) {
 #define C_(V_) case V_: { const int CH = kBlock / plan_T<V_>() / 2; int mx = 1; \
 		for (int b = 0; b < nb; b++) mx = std::max(mx, h_tab[b].NC * ((h_tab[b].Ny / 2 + CH - 1) / CH)); \
-		hipLaunchKernelGGL(k_z<V_>, dim3(grid_for(mx, ncu, 2), nb), dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, d_tab); return BM_OK; }
+		hipLaunchKernelGGL(k_z<V_>, dim3(grid8(mx, ncu, 2), nb), dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, z_xcd(), d_tab); return BM_OK; }
 	BM_FOR_Z_SIZES(C_)
 #undef C_
 	}
@@ -963,7 +999,7 @@ int bm::batch_y2d(int Ny, const PassBatch* h_tab, const PassBatch* d_tab, int nb
 	switch (Ny) {
 #define C_(V_) case V_: { constexpr int BS = Plan<V_>::E > 8 ? 128 : 256; const int LP = BS / plan_T<V_>(); int mx = 1; \
 		for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].NC + LP - 1) / LP); \
-		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid_for(mx, ncu, 8), nb), dim3(BS), 0, st, nullptr, nullptr, 0, 0, 0, d_tab); return BM_OK; }
+		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid8(mx, ncu, 8), nb), dim3(BS), 0, st, nullptr, nullptr, 0, 0, 0, d_tab); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}

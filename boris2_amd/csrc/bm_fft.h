@@ -36,6 +36,16 @@ template <int DIR, typename T> __device__ __forceinline__ C2<T> mul_di(C2<T> a) 
 // ----------------------------------------------------------------------------------------
 // Radix plans.  E = values per thread; each radix divides E; the first radix is even so a
 // zero upper half of the input maps onto whole zero inputs of the first butterflies.
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).  Item
+// c = (b mod 8) * ceil(nb/8) + b/8 of a sweep of nb items goes to block b, so each XCD works
+// on one contiguous eighth of the sweep and neighbouring items share an L2.  nb must be a
+// multiple of 8 for the map to be a permutation of 0..nb-1.
+__device__ __forceinline__ int xcd_chunk(int b, int nb)
+{
+	const int per = (nb + 7) / 8;
+	return (b % 8) * per + b / 8;
+}
+
 template <int N> struct Plan;
 #define BM_PLAN(n, e, ...)                                                      \
 	template <> struct Plan<n> {                                                \

@@ -27,10 +27,12 @@ void prof_mark_end(int kind, hipStream_t st);
 struct bm_demag_s {
 	int nx, ny, nz;
 	int Nx, Ny, Nz, NC, NYH, NZH;
-	int Sp;                   // line pitch of the spectra (complex elements, >= Ny)
+	int Sp;                   // line pitch of the full spectra S (complex elements, >= Ny)
+	int ntl;                  // row tiles per plane of the x-side spectra Sx (bm_xpass.h)
 	double h[3];
 	bool is2d;
-	float2* S = nullptr;      // spectra: 3 * nz * NC * Ny complex
+	float2* Sx = nullptr;     // x-side spectra, row-tiled: 3 * nz * ntl * NC * kLT complex
+	float2* S = nullptr;      // full spectra S[c][z][kx][ky < Ny] (3D only): 3 * nz * NC * Sp complex
 	float* K = nullptr;       // kernels (see bm_demag_copy_kernels)
 	long long kcount = 0;
 	double* ebuf = nullptr;   // energy partials (one per P5 work item)
@@ -46,10 +48,12 @@ int demag_planar(bm_demag_s* d, const float* M, float* H3n, hipStream_t st);
 int demag_fwd_x(bm_demag_s* d, const float* M, hipStream_t st);
 int demag_fwd_x_advance(bm_demag_s* d, const float* M, const float* E0, float* sM1, float* Mout, float dT5, hipStream_t st);
 int demag_mid(bm_demag_s* d, hipStream_t st);
-// single passes over "planes" of ny rows (a z plane or a layer) for other pipelines
-int pass_fwd_x(int nx, const float* M, float2* S, int ny, int nplanes, int Sp, int ncu, hipStream_t st);
-int pass_y(int Ny, int dir, float2* S, int nlines, int Sp, int ncu, hipStream_t st);
-int pass_inv_x(int nx, int mode, const float2* S, float* out, int ny, int nplanes, int Sp, float scale, int ncu, hipStream_t st);
+// single passes over "planes" of ny rows (a z plane or a layer) for other pipelines; the y
+// passes go between the row-tiled Sx (ny rows, ntl tiles per plane) and the full spectra S
+// (Ny rows, line pitch Sp), nlines = planes * NC lines
+int pass_fwd_x(int nx, const float* M, float2* Sx, int ny, int nplanes, int ntl, int ncu, hipStream_t st);
+int pass_y(int Ny, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ncu, hipStream_t st);
+int pass_inv_x(int nx, int mode, const float2* Sx, float* out, int ny, int nplanes, int ntl, float scale, int ncu, hipStream_t st);
 // FP64 real-input line FFTs of the kernel setup (see k_line_fft)
 int setup_line_fft(int N, const double* src, double* dst, int nlines, int na, long long sa, long long sb, long long se,
 	long long da, long long db, long long de, int keep, int part, double sign, hipStream_t st);
@@ -61,14 +65,14 @@ bool plan_2d_ok(int nx, int Ny);
 // Per-mesh arguments of the x R2C pass (also its single-mesh launch arguments); with
 // E0 set it performs the RKDP54 advance on the way (sM1 = M, Mout = M + E0 dT/5).
 struct FwdArgs {
-	const float* M; float2* S;
-	int ny, nz, Sp, ntiles;
+	const float* M; float2* Sx;
+	int ny, nz, ntl, ntiles;
 	const float* E0; float* sM1; float* Mout; float dT5;
 };
 // Per-mesh arguments of the other passes in a batched launch (the fields each pass uses).
 struct PassBatch {
-	float2* S; const float* K; float* out;
-	int ny, nz, Sp, ntiles, nlines, NC, Ny, NYH;
+	float2* S; float2* Sx; const float* K; float* out;
+	int ny, nz, Sp, ntl, ntiles, nlines, NC, Ny, NYH;
 	float scale;            // x C2R: 1 / (Nx Ny Nz) of the mesh
 };
 // rows per work item of the x passes for nx (0 if there is no plan)

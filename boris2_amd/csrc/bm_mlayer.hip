@@ -17,6 +17,7 @@
 // pass.  The x and y passes are the single-mesh convolution's own.
 #include "bm_internal.h"
 #include "bm_newell.h"
+#include "bm_xpass.h"
 
 #include <algorithm>
 #include <cmath>
@@ -233,10 +234,11 @@ int launch_ml_mult(int L, float2* S, const float* K, int NC, int Ny, int Sp, int
 } // namespace
 
 struct bm_mlayer_s {
-	int L, nx, ny, Nx, Ny, NC, NYH, Sp;
+	int L, nx, ny, Nx, Ny, NC, NYH, Sp, ntl;
 	double hx, hy;
 	std::vector<double> hz, z0;
-	float2* S = nullptr;     // spectra: 3 * L * NC * Ny complex
+	float2* Sx = nullptr;    // x-side spectra, row-tiled (bm_xpass.h): 3 * L * ntl * NC * kLT complex
+	float2* S = nullptr;     // full spectra S[c][layer][kx][ky]: 3 * L * NC * Ny complex
 	float* K = nullptr;      // kernels: L*L pairs x 6 planes x NC x (Ny/2+1), ky fastest
 	long long kcount = 0, bytes = 0;
 	int ncu = 256;
@@ -368,13 +370,16 @@ extern "C" int bm_mlayer_create(int nlayers, int nx, int ny, double hx, double h
 	m->L = nlayers; m->nx = nx; m->ny = ny; m->Nx = 2 * nx; m->Ny = 2 * ny; m->NC = nx + 1; m->NYH = ny + 1; m->Sp = m->Ny;
 	m->hx = hx; m->hy = hy; m->hz.assign(hz, hz + nlayers); m->z0.assign(z0, z0 + nlayers);
 	m->ncu = bm::device_cu_count();
+	m->ntl = bm::sx_tiles(ny);
 	const size_t sbytes = (size_t)3 * nlayers * m->NC * m->Sp * sizeof(float2);
+	const size_t sxbytes = (size_t)3 * nlayers * m->ntl * m->NC * bm::kLT * sizeof(float2);
 	m->kcount = (long long)nlayers * nlayers * 6 * m->NC * m->NYH;
-	if (hipMalloc(&m->S, sbytes) != hipSuccess || hipMalloc(&m->K, (size_t)m->kcount * sizeof(float)) != hipSuccess) {
+	if (hipMalloc(&m->Sx, sxbytes) != hipSuccess || hipMalloc(&m->S, sbytes) != hipSuccess
+		|| hipMalloc(&m->K, (size_t)m->kcount * sizeof(float)) != hipSuccess) {
 		bm_mlayer_destroy(m);
 		return fail(BM_ERR_NOMEM, "bm_mlayer_create: device memory");
 	}
-	m->bytes = (long long)sbytes + m->kcount * 4;
+	m->bytes = (long long)(sbytes + sxbytes) + m->kcount * 4;
 	int rc = compute_ml_kernels(m, nullptr);
 	if (rc) { bm_mlayer_destroy(m); return rc; }
 	*out = m;
@@ -384,6 +389,7 @@ extern "C" int bm_mlayer_create(int nlayers, int nx, int ny, double hx, double h
 extern "C" int bm_mlayer_destroy(bm_mlayer_t m)
 {
 	if (!m) return BM_OK;
+	if (m->Sx) (void)hipFree(m->Sx);
 	if (m->S) (void)hipFree(m->S);
 	if (m->K) (void)hipFree(m->K);
 	delete m;
@@ -395,16 +401,16 @@ extern "C" int bm_mlayer_convolute(bm_mlayer_t m, const float* M, float* Heff, i
 	if (!m || !M || !Heff) return fail(BM_ERR_ARG, "bm_mlayer_convolute: null argument");
 	hipStream_t st = (hipStream_t)stream;
 	const int L = m->L, nlines = 3 * L * m->NC;
-	int rc = bm::pass_fwd_x(m->nx, M, m->S, m->ny, L, m->Sp, m->ncu, st);
-	if (rc == BM_OK) rc = bm::pass_y(m->Ny, -1, m->S, nlines, m->Sp, m->ncu, st);
+	int rc = bm::pass_fwd_x(m->nx, M, m->Sx, m->ny, L, m->ntl, m->ncu, st);
+	if (rc == BM_OK) rc = bm::pass_y(m->Ny, -1, m->S, m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st);
 	if (rc == BM_OK) {
 		bm::prof_mark_begin(bm::PK_ML_MULT, st);
 		rc = launch_ml_mult(L, m->S, m->K, m->NC, m->Ny, m->Sp, m->ncu, st);
 		bm::prof_mark_end(bm::PK_ML_MULT, st);
 	}
-	if (rc == BM_OK) rc = bm::pass_y(m->Ny, 1, m->S, nlines, m->Sp, m->ncu, st);
+	if (rc == BM_OK) rc = bm::pass_y(m->Ny, 1, m->S, m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st);
 	const float scale = (float)(1.0 / ((double)m->Nx * m->Ny));
-	if (rc == BM_OK) rc = bm::pass_inv_x(m->nx, clear_out ? 1 : 2, m->S, Heff, m->ny, L, m->Sp, scale, m->ncu, st);
+	if (rc == BM_OK) rc = bm::pass_inv_x(m->nx, clear_out ? 1 : 2, m->Sx, Heff, m->ny, L, m->ntl, scale, m->ncu, st);
 	if (rc) return rc;
 	BM_HIP(hipGetLastError());
 	return BM_OK;

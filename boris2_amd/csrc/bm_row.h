@@ -5,8 +5,8 @@
 //      writing the stage's output magnetisation;
 //   3. forward-transforms that output magnetisation along x into the same spectra rows (the
 //      first pass of the next field evaluation's convolution).
-// Each work item reads and then overwrites only its own rows S[c][z][.][y0, y0+TY), so the
-// spectra are updated in place; neighbours for the exchange stencil are read from the
+// Each work item reads and then overwrites only its own rows y0..y0+TY-1 of the x-side
+// spectra Sx (all kx, three components), so they are updated in place; neighbours for the exchange stencil are read from the
 // stage's input magnetisation, which the kernel never writes.
 //
 // Compared with the separate passes (x-C2R writing H, stage reading H, x-R2C re-reading
@@ -23,7 +23,7 @@
 namespace bm {
 
 template <int HX, int TY, int METHOD, int STEP, bool R2C, int BS = kXBlock>
-__global__ void __launch_bounds__(BS) k_row(StageArgs a, float2* S, int Sp, int ntiles, float scale)
+__global__ void __launch_bounds__(BS) k_row(StageArgs a, float2* Sx, int ntl, int ntiles, float scale)
 {
 	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = BS / T, RPITCH = xrow_pitch<HX>(), NG = TY / RP, EH = E / 2;
 	static_assert(NG * RP == TY, "tile height must be a multiple of the row group");
@@ -41,13 +41,13 @@ __global__ void __launch_bounds__(BS) k_row(StageArgs a, float2* S, int Sp, int 
 		// cells 2k and 2k+1 of this thread's rows: component c in h[g][e][c] (.x, .y)
 		float2 h[NG][EH][3];
 		XTileRegs<HX, TY, BS> pre;
-		pre.load(S, 0, z, y0, ny, nz, Sp);
+		pre.load(Sx, 0, z, y0, ny, nz, ntl);
 #pragma unroll 1
 		for (int c = 0; c < 3; c++) {
 			__syncthreads();
 			pre.to_tile(tile);
 			__syncthreads();
-			if (c < 2) pre.load(S, c + 1, z, y0, ny, nz, Sp);
+			if (c < 2) pre.load(Sx, c + 1, z, y0, ny, nz, ntl);
 #pragma unroll
 			for (int g = 0; g < NG; g++) {
 				C2<float> v[E];
@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(BS) k_row(StageArgs a, float2* S, int Sp, int 
 					r2c_row<HX>(th, tile + (g * RP + th.rl) * RPITCH, v);
 				}
 				__syncthreads();
-				tile_to_S<HX, TY, BS>(tile, S, c, z, y0, ny, nz, Sp);
+				tile_to_S<HX, TY, BS>(tile, Sx, c, z, y0, ny, nz, ntl);
 				__syncthreads();
 			}
 		}
@@ -116,7 +116,7 @@ int launch_row_t(const bm_demag_s* d, const StageArgs& a, float scale, hipStream
 {
 	switch (d->nx) {
 #define C_(V_) case V_: { constexpr int TY = XTile<V_>::v; const int nt = (d->ny + TY - 1) / TY; \
-		hipLaunchKernelGGL((k_row<V_, TY, METHOD, STEP, R2C>), dim3(row_grid(d->nz * nt, d->ncu)), dim3(kXBlock), 0, st, a, d->S, d->Sp, nt, scale); \
+		hipLaunchKernelGGL((k_row<V_, TY, METHOD, STEP, R2C>), dim3(row_grid(d->nz * nt, d->ncu)), dim3(kXBlock), 0, st, a, d->Sx, d->ntl, nt, scale); \
 		return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_

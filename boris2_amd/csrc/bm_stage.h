@@ -16,6 +16,7 @@
 
 #include <hip/hip_runtime.h>
 #include "boris_mi355x.h"
+#include "bm_fft.h"
 
 namespace bm {
 
@@ -99,15 +100,6 @@ __device__ __forceinline__ void block_max_to(float v, unsigned* dst)
 	}
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).  Item
-// c = (b mod 8) * ceil(nb/8) + b/8 of a sweep of nb items goes to block b, so each XCD works
-// on one contiguous eighth of the sweep and neighbouring items share an L2.
-__device__ __forceinline__ long long xcd_chunk(int b, int nb)
-{
-	const int per = (nb + 7) / 8;
-	return (long long)(b % 8) * per + b / 8;
-}
-
 // Effective field of one cell; lap = sum over axes of the second differences / h^2.
 __device__ __forceinline__ F3 eff_field(const StageArgs& a, F3 m, F3 Hd, F3 lap, float Ms)
 {
@@ -142,9 +134,8 @@ __device__ __forceinline__ V rk_update(const StageArgs& a, size_t idx, V m, V rh
 	V mn = m;
 	if (METHOD == M_RKDP) {
 		if (STEP == 0) {
-			const V pred = ldv<V>(a.sM1, idx) + dT * ((5179.f / 57600.f) * ldv<V>(a.E[0], idx) + (7571.f / 16695.f) * ldv<V>(a.E[2], idx)
-				+ (393.f / 640.f) * ldv<V>(a.E[3], idx) - (92097.f / 339200.f) * ldv<V>(a.E[4], idx) + (187.f / 2100.f) * ldv<V>(a.E[5], idx)
-				+ (1.f / 40.f) * rhs);
+			// embedded 4th-order solution: stage 5 left all of it but the b*_7 k_7 term in E1
+			const V pred = ldv<V>(a.E[1], idx) + (dT * (1.f / 40.f)) * rhs;
 			lte_loc = fmaxf(lte_loc, rel_err(m, pred));
 			stv(a.E[0], idx, rhs);
 			return m;
@@ -169,8 +160,13 @@ __device__ __forceinline__ V rk_update(const StageArgs& a, size_t idx, V m, V rh
 		}
 		else {
 			stv(a.E[5], idx, rhs);
-			mn = ldv<V>(a.sM1, idx) + dT * ((35.f / 384.f) * ldv<V>(a.E[0], idx) + (500.f / 1113.f) * ldv<V>(a.E[2], idx)
-				+ (125.f / 192.f) * ldv<V>(a.E[3], idx) - (2187.f / 6784.f) * ldv<V>(a.E[4], idx) + (11.f / 84.f) * rhs);
+			const V s1 = ldv<V>(a.sM1, idx), e0 = ldv<V>(a.E[0], idx), e2 = ldv<V>(a.E[2], idx), e3 = ldv<V>(a.E[3], idx), e4 = ldv<V>(a.E[4], idx);
+			mn = s1 + dT * ((35.f / 384.f) * e0 + (500.f / 1113.f) * e2 + (125.f / 192.f) * e3 - (2187.f / 6784.f) * e4 + (11.f / 84.f) * rhs);
+			// the embedded 4th-order solution except its k_7 term (added by the next stage 0,
+			// which evaluates k_7 = f(new M)) goes to E1, free after stage 4: stage 0 then
+			// reads one array instead of six
+			stv(a.E[1], idx, s1 + dT * ((5179.f / 57600.f) * e0 + (7571.f / 16695.f) * e2 + (393.f / 640.f) * e3
+				- (92097.f / 339200.f) * e4 + (187.f / 2100.f) * rhs));
 			mn = renorm(mn, Ms);
 		}
 	}

@@ -6,7 +6,7 @@
 // N.x/2 + 1 = HX + 1 spectrum values; the inverse undoes the split and runs the HX-point
 // inverse FFT, whose first HX/2 outputs are the nx real values.  Rows of one z plane are
 // handled TY at a time; the spectrum rows sit in an LDS tile and move to and from the
-// y-fastest spectra array S[c][z][kx][y] with y-contiguous (transposed) accesses.
+// row-tiled x-side spectra Sx (layout below) in contiguous runs.
 //
 // Ownership: in a row group, thread t of row rl handles k = t + T*e, e < E/2, i.e. the cell
 // pairs (2k, 2k+1) — the same cells in the inverse transform, in any per-cell work done in
@@ -92,34 +92,71 @@ __device__ __forceinline__ void c2r_row(const XThread<HX>& th, C2<float>* lrow, 
 	fft<HX, 1, float>(v, lrow, 1, th.t, th.tw);               // first LDS write waits on a barrier
 }
 
-// Transposed copy of a tile of TY spectrum rows (component c, plane z, rows y0..) to S by a
+// x-side spectra layout.  The x passes and the y passes exchange the spectra of the ny data
+// rows through Sx[c][z][t][kx][kLT]: row y of line (c, z, kx) sits in row tile t = y / kLT,
+// slot y % kLT.  One row tile of a plane is a contiguous run of NC * kLT complex values, so
+// an x pass over kLT rows (or a multiple) moves whole runs, and the y passes read kLT-row
+// pieces of several consecutive kx lines at once (also contiguous).  ntl = ceil(ny / kLT)
+// tiles per plane; slots of rows >= ny are padding and never read.
+constexpr int kLT = 8;
+
+__host__ __device__ __forceinline__ int sx_tiles(int ny) { return (ny + kLT - 1) / kLT; }
+// index of (c, z, kx = 0, y0); for TY < kLT the tile's rows y0..y0+TY-1 share one row tile
+__host__ __device__ __forceinline__ size_t sx_base(int c, int z, int y0, int nz, int NC, int ntl)
+{
+	return (((size_t)c * nz + z) * ntl + y0 / kLT) * (size_t)NC * kLT + y0 % kLT;
+}
+// index of row y of line (plane cz = c * nz + z, kx)
+__host__ __device__ __forceinline__ size_t sx_line(int cz, int kx, int y, int NC, int ntl)
+{
+	return ((size_t)cz * ntl + y / kLT) * (size_t)NC * kLT + (size_t)kx * kLT + y % kLT;
+}
+
+// Element idx of a TY-row tile (TY a power of two) with NC values per row: its kx, row yl
+// (relative to y0) and offset from sx_base.  Consecutive idx are consecutive addresses
+// (TY >= kLT) or runs of TY (TY < kLT).
+template <int TY, int NC>
+__device__ __forceinline__ void sx_tile_elem(int idx, int& kx, int& yl, int& off)
+{
+	if constexpr (TY >= kLT) {
+		const int q = idx / kLT;
+		kx = q % NC; yl = (q / NC) * kLT + idx % kLT; off = idx;
+	}
+	else { kx = idx / TY; yl = idx % TY; off = kx * kLT + yl; }
+}
+
+// Copy of a tile of TY spectrum rows (component c, plane z, rows y0..) from LDS to Sx by a
 // block of BS threads.
 template <int HX, int TY, int BS = kXBlock>
-__device__ __forceinline__ void tile_to_S(const C2<float>* tile, float2* S, int c, int z, int y0, int ny, int nz, int Sp)
+__device__ __forceinline__ void tile_to_S(const C2<float>* tile, float2* Sx, int c, int z, int y0, int ny, int nz, int ntl)
 {
 	constexpr int NC = HX + 1, RPITCH = xrow_pitch<HX>();
+	float2* base = Sx + sx_base(c, z, y0, nz, NC, ntl);
 	for (int idx = threadIdx.x; idx < TY * NC; idx += BS) {
-		const int yl = idx % TY, kx = idx / TY, y = y0 + yl;
-		if (y < ny) {
+		int kx, yl, off;
+		sx_tile_elem<TY, NC>(idx, kx, yl, off);
+		if (y0 + yl < ny) {
 			C2<float> a = tile[yl * RPITCH + kx];
-			S[(((size_t)c * nz + z) * NC + kx) * (size_t)Sp + y] = make_float2(a.x, a.y);
+			base[off] = make_float2(a.x, a.y);
 		}
 	}
 }
 
-// Register staging of a tile of TY spectrum rows read from S (transposed), so the next
-// tile's loads can be in flight while the current one is transformed.
+// Register staging of a tile of TY spectrum rows read from Sx, so the next tile's loads can
+// be in flight while the current one is transformed.
 template <int HX, int TY, int BS = kXBlock> struct XTileRegs {
 	static constexpr int NC = HX + 1, N = (TY * NC + BS - 1) / BS;
 	float2 r[N];
 
-	__device__ __forceinline__ void load(const float2* S, int c, int z, int y0, int ny, int nz, int Sp)
+	__device__ __forceinline__ void load(const float2* Sx, int c, int z, int y0, int ny, int nz, int ntl)
 	{
+		const float2* base = Sx + sx_base(c, z, y0, nz, NC, ntl);
 #pragma unroll
 		for (int q = 0; q < N; q++) {
 			const int idx = threadIdx.x + BS * q;
-			const int yl = idx % TY, kx = idx / TY, y = y0 + yl;
-			r[q] = (idx < TY * NC && y < ny) ? S[(((size_t)c * nz + z) * NC + kx) * (size_t)Sp + y] : make_float2(0.f, 0.f);
+			int kx, yl, off;
+			sx_tile_elem<TY, NC>(idx, kx, yl, off);
+			r[q] = (idx < TY * NC && y0 + yl < ny) ? base[off] : make_float2(0.f, 0.f);
 		}
 	}
 	__device__ __forceinline__ void to_tile(C2<float>* tile) const
@@ -128,7 +165,9 @@ template <int HX, int TY, int BS = kXBlock> struct XTileRegs {
 #pragma unroll
 		for (int q = 0; q < N; q++) {
 			const int idx = threadIdx.x + BS * q;
-			if (idx < TY * NC) tile[(idx % TY) * RPITCH + idx / TY] = mk(r[q].x, r[q].y);
+			int kx, yl, off;
+			sx_tile_elem<TY, NC>(idx, kx, yl, off);
+			if (idx < TY * NC) tile[yl * RPITCH + kx] = mk(r[q].x, r[q].y);
 		}
 	}
 };
