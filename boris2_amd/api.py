@@ -270,6 +270,41 @@ class FMSolver:
             pass
 
 
+class LayerStackSolver(FMSolver):
+    """A stack of 2D layers integrated together: layer l has nx*ny cells (hx, hy, hz[l]) with
+    its bottom face at z0[l], the layers coupled by the multilayered demag and each with its
+    own Ms, A, alpha, K1 (per-layer sequences; None keeps params' value).  The reference
+    holds such a stack as one ferromagnetic mesh per layer under SDemag (Boris/SDemag.cpp).
+    M is (L*nx*ny, 3), layer-major."""
+
+    def __init__(self, method: int, nx: int, ny: int, hx: float, hy: float, hz, z0, params: FMParams,
+                 Ms=None, A=None, alpha=None, K1=None, M0=None):
+        hz = np.ascontiguousarray(hz, dtype=np.float64)
+        nl = len(hz)
+        self.mlayer = MultiLayerDemag(nx, ny, hx, hy, hz, z0) if params.modules & BM_MOD_DEMAG else None
+        base = FMParams(**{**params.__dict__, "modules": params.modules & ~BM_MOD_DEMAG})
+        if M0 is None and Ms is not None:
+            M0 = np.zeros((nl * nx * ny, 3), dtype=np.float32)
+            M0[:, 0] = -np.repeat(np.asarray(Ms, dtype=np.float32), nx * ny)
+        super().__init__(method, (nx, ny, nl), (hx, hy, float(hz.max())), base, M0)
+        self._lay = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (Ms, A, alpha, K1)]
+        for v in self._lay:
+            if v is not None and v.shape != (nl,):
+                raise ValueError("per-layer values need one entry per layer")
+        ptrs = [None if v is None else v.ctypes.data_as(C.c_void_p) for v in self._lay]
+        if self.mlayer is not None or any(v is not None for v in self._lay):
+            # without demag the stack still needs its per-layer values: attach a handle anyway
+            if self.mlayer is None:
+                self.mlayer = MultiLayerDemag(nx, ny, hx, hy, hz, z0)
+            check(self.L.bm_ode_set_layers(self.handle, self.mlayer.handle, *ptrs))
+        self.set_params(params)
+
+    def close(self) -> None:
+        super().close()
+        if getattr(self, "mlayer", None) is not None:
+            self.mlayer.close()
+
+
 class BatchSolver:
     """Many independent meshes of uneven sizes integrated together with a fixed-step method
     (RK4 or TEuler): one launch per pass size per field evaluation, a step replayed from a

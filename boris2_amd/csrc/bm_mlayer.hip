@@ -396,10 +396,9 @@ extern "C" int bm_mlayer_destroy(bm_mlayer_t m)
 	return BM_OK;
 }
 
-extern "C" int bm_mlayer_convolute(bm_mlayer_t m, const float* M, float* Heff, int clear_out, void* stream)
+// The convolution into out with the final x C2R in `mode` (0 planar set, 1 AoS set, 2 AoS add).
+static int ml_convolute(bm_mlayer_s* m, const float* M, float* out, int mode, hipStream_t st)
 {
-	if (!m || !M || !Heff) return fail(BM_ERR_ARG, "bm_mlayer_convolute: null argument");
-	hipStream_t st = (hipStream_t)stream;
 	const int L = m->L, nlines = 3 * L * m->NC;
 	int rc = bm::pass_fwd_x(m->nx, M, m->Sx, m->ny, L, m->ntl, m->ncu, st);
 	if (rc == BM_OK) rc = bm::pass_y(m->Ny, -1, m->S, m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st);
@@ -410,7 +409,22 @@ extern "C" int bm_mlayer_convolute(bm_mlayer_t m, const float* M, float* Heff, i
 	}
 	if (rc == BM_OK) rc = bm::pass_y(m->Ny, 1, m->S, m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st);
 	const float scale = (float)(1.0 / ((double)m->Nx * m->Ny));
-	if (rc == BM_OK) rc = bm::pass_inv_x(m->nx, clear_out ? 1 : 2, m->Sx, Heff, m->ny, L, m->ntl, scale, m->ncu, st);
+	if (rc == BM_OK) rc = bm::pass_inv_x(m->nx, mode, m->Sx, out, m->ny, L, m->ntl, scale, m->ncu, st);
+	return rc;
+}
+
+int bm::mlayer_planar(bm_mlayer_s* m, const float* M, float* H3n, hipStream_t st) { return ml_convolute(m, M, H3n, 0, st); }
+
+void bm::mlayer_dims(const bm_mlayer_s* m, int* L, int* nx, int* ny, double* hz)
+{
+	*L = m->L; *nx = m->nx; *ny = m->ny;
+	if (hz) for (int l = 0; l < m->L; l++) hz[l] = m->hz[l];
+}
+
+extern "C" int bm_mlayer_convolute(bm_mlayer_t m, const float* M, float* Heff, int clear_out, void* stream)
+{
+	if (!m || !M || !Heff) return fail(BM_ERR_ARG, "bm_mlayer_convolute: null argument");
+	int rc = ml_convolute(m, M, Heff, clear_out ? 1 : 2, (hipStream_t)stream);
 	if (rc) return rc;
 	BM_HIP(hipGetLastError());
 	return BM_OK;

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <algorithm>
 #include <cstdlib>
+#include <vector>
 
 namespace {
 
@@ -85,6 +86,11 @@ struct bm_ode_s {
 	double h[3];
 	bm_fm_params p;
 	bm_demag_t demag = nullptr;
+	// layer stack (bm_ode_set_layers): multilayered demag and per-layer material values
+	bm_mlayer_t ml = nullptr;
+	bool lay_set[3] = { false, false, false };   // A, alpha, K1 given per layer
+	float lay[3][bm::kMaxStackLayers] = {};
+	float* Ms_layers = nullptr;    // per-cell Ms built from per-layer values (owned)
 	float* M = nullptr;            // the mesh's magnetisation (caller-owned); holds M between steps
 	float* Mb = nullptr;           // second magnetisation buffer for intermediate stages
 	float* Mcur = nullptr;         // the array holding the magnetisation of the current stage
@@ -158,6 +164,15 @@ StageArgs make_args(bm_ode_s* o, float dT)
 	a.dT = dT;
 	a.modules = o->p.modules;
 	a.map = stage_map();
+	if (o->ml) {
+		a.layered = 1;
+		a.ihz2 = 0.f;
+		for (int l = 0; l < kMaxStackLayers; l++) {
+			a.lay_A[l] = o->lay_set[0] ? o->lay[0][l] : a.A;
+			a.lay_alpha[l] = o->lay_set[1] ? o->lay[1][l] : a.alpha;
+			a.lay_K1[l] = o->lay_set[2] ? o->lay[2][l] : a.K1;
+		}
+	}
 	return a;
 }
 
@@ -240,6 +255,7 @@ int evaluate(bm_ode_s* o, int step, float dT, hipStream_t st)
 {
 	o->evaluations++;
 	const bool demag = (o->p.modules & BM_MOD_DEMAG) != 0;
+	if (demag && o->ml) { int rc = bm::mlayer_planar(o->ml, o->Mcur, o->Hd, st); if (rc) return rc; return launch_stage(o, step, dT, st); }
 	if (demag && o->use_row) return eval_row(o, step, dT, st);
 	if (demag) { int rc = bm::demag_planar(o->demag, o->Mcur, o->Hd, st); if (rc) return rc; }
 	return launch_stage(o, step, dT, st);
@@ -381,6 +397,7 @@ void free_all(bm_ode_s* o)
 	if (o->h_lte) (void)hipHostFree(o->h_lte);
 	if (o->d_part) (void)hipFree(o->d_part);
 	if (o->h_part) (void)hipHostFree(o->h_part);
+	if (o->Ms_layers) (void)hipFree(o->Ms_layers);
 }
 
 } // namespace
@@ -444,11 +461,52 @@ extern "C" int bm_ode_set_ms_map(bm_ode_t o, const float* Ms_map)
 	return BM_OK;
 }
 
+// Layer stack: the ODE's nz planes are the layers of the multilayered demag ml.  Boris holds
+// such a stack as one ferromagnetic mesh per layer coupled by SDemag (Boris/SDemag.cpp:173-300,
+// SDemagCUDA::UpdateField); each mesh's exchange, anisotropy and LLG use that mesh's own
+// parameters (Boris/Exchange.cpp:108, Boris/Anisotropy.cpp:82, Boris/DiffEqFM_Equations.cpp:12),
+// so here exchange acts within each layer only and A, alpha, K1 (and Ms, through a per-cell
+// map built here) may differ per layer.
+extern "C" int bm_ode_set_layers(bm_ode_t o, bm_mlayer_t ml, const double* Ms, const double* A, const double* alpha, const double* K1)
+{
+	if (!o) return fail(BM_ERR_ARG, "null handle");
+	if (!ml) {
+		if ((o->p.modules & BM_MOD_DEMAG) && !o->demag) return fail(BM_ERR_STATE, "bm_ode_set_layers: demag is on and there is no other demag handle");
+		if (o->Ms_map == o->Ms_layers) o->Ms_map = nullptr;
+		o->ml = nullptr;
+		return BM_OK;
+	}
+	int L, nx, ny;
+	bm::mlayer_dims(ml, &L, &nx, &ny, nullptr);
+	if (L != o->nz || nx != o->nx || ny != o->ny) return fail(BM_ERR_ARG, "bm_ode_set_layers: the stack's layers must be the mesh's nz planes (same nx, ny)");
+	if (L > bm::kMaxStackLayers) return fail(BM_ERR_SIZE, "bm_ode_set_layers: at most 8 layers");
+	const size_t b3 = o->n * 3 * sizeof(float);
+	if (!o->Hd && hipMalloc(&o->Hd, b3) != hipSuccess) return fail(BM_ERR_NOMEM, "bm_ode_set_layers: demag field buffer");
+	const double* src[3] = { A, alpha, K1 };
+	for (int q = 0; q < 3; q++) {
+		o->lay_set[q] = src[q] != nullptr;
+		for (int l = 0; l < bm::kMaxStackLayers; l++) o->lay[q][l] = src[q] && l < L ? (float)src[q][l] : 0.f;
+	}
+	if (Ms) {
+		for (int l = 0; l < L; l++) if (!(Ms[l] > 0)) return fail(BM_ERR_ARG, "bm_ode_set_layers: Ms must be positive");
+		if (!o->Ms_layers && hipMalloc(&o->Ms_layers, o->n * sizeof(float)) != hipSuccess) return fail(BM_ERR_NOMEM, "bm_ode_set_layers: Ms map");
+		std::vector<float> h(o->n);
+		const size_t nl = (size_t)nx * ny;
+		for (size_t i = 0; i < o->n; i++) h[i] = (float)Ms[i / nl];
+		BM_HIP(hipMemcpy(o->Ms_layers, h.data(), o->n * sizeof(float), hipMemcpyHostToDevice));
+		o->Ms_map = o->Ms_layers;
+	}
+	o->ml = ml;
+	o->use_row = false;
+	o->S_valid = false;
+	return BM_OK;
+}
+
 extern "C" int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p)
 {
 	if (!o || !p) return fail(BM_ERR_ARG, "null argument");
-	if ((p->modules & BM_MOD_DEMAG) && (!o->demag || o->demag->nx != o->nx || o->demag->ny != o->ny || o->demag->nz != o->nz))
-		return fail(BM_ERR_STATE, "bm_ode_set_params: demag needs a demag handle of this mesh size at creation");
+	if ((p->modules & BM_MOD_DEMAG) && !o->ml && (!o->demag || o->demag->nx != o->nx || o->demag->ny != o->ny || o->demag->nz != o->nz))
+		return fail(BM_ERR_STATE, "bm_ode_set_params: demag needs a demag handle of this mesh size at creation, or a layer stack");
 	if ((p->modules & BM_MOD_DEMAG) && !o->use_row && !o->Hd) {
 		const size_t b3 = o->n * 3 * sizeof(float);
 		if (hipMalloc(&o->Hd, b3) != hipSuccess) return fail(BM_ERR_NOMEM, "bm_ode_set_params: demag field buffer");

@@ -63,6 +63,8 @@ __device__ __forceinline__ P6 renorm(P6 v, float2 Ms) { return P6{ renorm(v.a, M
 __device__ __forceinline__ float rel_err(F3 m, F3 p) { return norm(m - p) / norm(m); }
 __device__ __forceinline__ float rel_err(P6 m, P6 p) { return fmaxf(rel_err(m.a, p.a), rel_err(m.b, p.b)); }
 
+constexpr int kMaxStackLayers = 8;
+
 struct StageArgs {
 	const float* Min;       // magnetisation this stage evaluates (neighbours read)
 	float* Mout;            // magnetisation of the next stage (never the same array)
@@ -79,7 +81,24 @@ struct StageArgs {
 	float dT;
 	int modules;
 	int map;
+	// layer stack: the nz planes are separate 2D layers (one ferromagnetic mesh each, no
+	// exchange between them) with their own A, alpha and K1 (when layered != 0)
+	int layered;
+	float lay_A[kMaxStackLayers], lay_alpha[kMaxStackLayers], lay_K1[kMaxStackLayers];
 };
+
+// Material constants of a cell: the mesh's, or those of its layer in a layer stack (the
+// layer value is picked with compile-time indices so StageArgs stays in registers).
+struct CellMat { float A, alpha, K1; };
+__device__ __forceinline__ CellMat cell_mat(const StageArgs& a, int k)
+{
+	if (!a.layered) return CellMat{ a.A, a.alpha, a.K1 };
+	CellMat c{ a.lay_A[0], a.lay_alpha[0], a.lay_K1[0] };
+#pragma unroll
+	for (int l = 1; l < kMaxStackLayers; l++)
+		if (k == l) c = CellMat{ a.lay_A[l], a.lay_alpha[l], a.lay_K1[l] };
+	return c;
+}
 
 enum { M_RKDP = 0, M_RK4 = 1, M_TEULER = 2, M_AHEUN = 3 };
 
@@ -101,27 +120,27 @@ __device__ __forceinline__ void block_max_to(float v, unsigned* dst)
 }
 
 // Effective field of one cell; lap = sum over axes of the second differences / h^2.
-__device__ __forceinline__ F3 eff_field(const StageArgs& a, F3 m, F3 Hd, F3 lap, float Ms)
+__device__ __forceinline__ F3 eff_field(const StageArgs& a, const CellMat& cm, F3 m, F3 Hd, F3 lap, float Ms)
 {
 	F3 H = (a.modules & BM_MOD_ZEEMAN) ? F3{ a.Ha[0], a.Ha[1], a.Ha[2] } : F3{ 0.f, 0.f, 0.f };
 	if (a.modules & BM_MOD_DEMAG) H = H + Hd;
-	if (a.modules & BM_MOD_EXCHANGE) H = H + ((2.f * a.A) / ((float)kMU0 * Ms * Ms)) * lap;
+	if (a.modules & BM_MOD_EXCHANGE) H = H + ((2.f * cm.A) / ((float)kMU0 * Ms * Ms)) * lap;
 	if (a.modules & BM_MOD_ANIS_UNI) {
 		const F3 e{ a.ea[0], a.ea[1], a.ea[2] };
 		const float d = dot(m, e) / Ms;
-		H = H + ((2.f / ((float)kMU0 * Ms)) * d * (a.K1 + 2.f * a.K2 * (1.f - d * d))) * e;
+		H = H + ((2.f / ((float)kMU0 * Ms)) * d * (cm.K1 + 2.f * a.K2 * (1.f - d * d))) * e;
 	}
 	return H;
 }
 
 // LLG right-hand side (Landau-Lifshitz form with the Gilbert damping folded in)
-__device__ __forceinline__ F3 llg_rhs(const StageArgs& a, F3 m, F3 H, float Ms)
+__device__ __forceinline__ F3 llg_rhs(const StageArgs& a, float alpha, F3 m, F3 H, float Ms)
 {
 	const F3 mxh = cross(m, H);
 	const float iMs = 1.f / Ms;
 	const F3 mu{ m.x * iMs, m.y * iMs, m.z * iMs };
-	const float pre = (float)(-kGAMMA) * a.grel / (1.f + a.alpha * a.alpha);
-	return pre * (mxh + a.alpha * cross(mu, mxh));
+	const float pre = (float)(-kGAMMA) * a.grel / (1.f + alpha * alpha);
+	return pre * (mxh + alpha * cross(mu, mxh));
 }
 
 // Stage update of one cell (V = F3, MS = float) or of a cell pair (V = P6, MS = float2):
@@ -211,16 +230,17 @@ __device__ __forceinline__ void stage_cell(const StageArgs& a, size_t idx, int i
 			else if (j == 0) ly = ld3(a.Min, idx + sy) - m;
 			else ly = ld3(a.Min, idx - sy) - m;
 		}
-		if (a.nz > 1) {
+		if (a.nz > 1 && !a.layered) {
 			if (k > 0 && k < a.nz - 1) lz = ld3(a.Min, idx + sz) + ld3(a.Min, idx - sz) - 2.f * m;
 			else if (k == 0) lz = ld3(a.Min, idx + sz) - m;
 			else lz = ld3(a.Min, idx - sz) - m;
 		}
 		lap = a.ihx2 * lx + a.ihy2 * ly + a.ihz2 * lz;
 	}
-	const F3 H = eff_field(a, m, Hd, lap, Ms);
+	const CellMat cm = cell_mat(a, k);
+	const F3 H = eff_field(a, cm, m, Hd, lap, Ms);
 	if (a.Heff_out) st3(a.Heff_out, idx, H);
-	(void)rk_update<METHOD, STEP>(a, idx, m, llg_rhs(a, m, H, Ms), Ms, lte_loc);
+	(void)rk_update<METHOD, STEP>(a, idx, m, llg_rhs(a, cm.alpha, m, H, Ms), Ms, lte_loc);
 }
 
 // Cells i0 = 2k and i0 + 1 of row (j, k) for the row kernel (nx even, so both exist).  Row
@@ -252,9 +272,11 @@ __device__ __forceinline__ P6 stage_pair(const StageArgs& a, size_t idx, int i0,
 		lap0 = a.ihx2 * lx0 + a.ihy2 * ly.a + a.ihz2 * lz.a;
 		lap1 = a.ihx2 * lx1 + a.ihy2 * ly.b + a.ihz2 * lz.b;
 	}
-	const F3 H0 = eff_field(a, m.a, Hd.a, lap0, Ms.x), H1 = eff_field(a, m.b, Hd.b, lap1, Ms.y);
+	// the row kernel serves single meshes only (never a layer stack)
+	const CellMat cm{ a.A, a.alpha, a.K1 };
+	const F3 H0 = eff_field(a, cm, m.a, Hd.a, lap0, Ms.x), H1 = eff_field(a, cm, m.b, Hd.b, lap1, Ms.y);
 	if (a.Heff_out) st6(a.Heff_out, idx, P6{ H0, H1 });
-	const P6 rhs{ llg_rhs(a, m.a, H0, Ms.x), llg_rhs(a, m.b, H1, Ms.y) };
+	const P6 rhs{ llg_rhs(a, cm.alpha, m.a, H0, Ms.x), llg_rhs(a, cm.alpha, m.b, H1, Ms.y) };
 	return rk_update<METHOD, STEP>(a, idx, m, rhs, Ms, lte_loc);
 }
 

@@ -138,6 +138,15 @@ int bm_ode_destroy(bm_ode_t o);
 /* per-cell Ms map (device float[n], caller-owned, NULL to use p->Ms) */
 int bm_ode_set_ms_map(bm_ode_t o, const float* Ms_map);
 int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p);
+/* Layer stack: the mesh's nz planes are the layers of the multilayered demag ml (same nx,
+ * ny, nz == layer count; at most 8), as the reference's one-ferromagnetic-mesh-per-layer
+ * stack coupled by SDemag (Boris/SDemag.cpp:173-300, SDemagCUDA::UpdateField;
+ * Boris/Exchange.cpp:108, Boris/Anisotropy.cpp:82 and Boris/DiffEqFM_Equations.cpp:12 per
+ * layer mesh).  Exchange acts within each layer; Ms, A, alpha, K1 may be given per layer
+ * (double[L] each, NULL keeps the mesh value; Ms becomes an internal per-cell map, replaced
+ * by a later bm_ode_set_ms_map).  With BM_MOD_DEMAG set (bm_ode_set_params) the demag
+ * field is the multilayered convolution.  ml must outlive o; ml NULL detaches the stack. */
+int bm_ode_set_layers(bm_ode_t o, bm_mlayer_t ml, const double* Ms, const double* A, const double* alpha, const double* K1);
 /* dT, and the adaptive controls err_fail, dT_increase, dT_min, dT_max (values <= 0 keep
  * the current ones); the defaults are the reference's per-method defaults. */
 int bm_ode_set_dt(bm_ode_t o, double dT);

@@ -1,6 +1,8 @@
 // ORACLE — TEST INFRASTRUCTURE ONLY.  C-ABI over the oracle for the Python tests
 // (ctypes) and bench.py's cpu_baseline leg.  Nothing in the product links it.
 #include "oracle_core.hpp"
+#include <memory>
+#include <type_traits>
 #include "oracle_mlayer.hpp"
 #include "oracle_adjoint.hpp"
 
@@ -221,6 +223,32 @@ void or_sim_get_Heff(void* h, double* H)
 {
 	OrSim* o = (OrSim*)h;
 	SIM_DISPATCH(o, { for (size_t i = 0; i < s->n * 3; i++) H[i] = s->Heff[i]; });
+}
+
+// Layer stack (Sim::ext_demag): the sim's nz planes become L 2D layers (hx, hy, hz[l],
+// bottom faces z0[l]) coupled by the multilayered demag; per-layer Ms, A, alpha, K1
+// (NULL keeps the sim's value).  Create the sim with demag off; with_demag adds the
+// multilayered demag.
+void or_sim_set_layers(void* h, int L, double hx, double hy, const double* hz, const double* z0, int with_demag,
+	const double* Ms, const double* A, const double* alpha, const double* K1)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, {
+		s->layered = true;
+		if (with_demag) {
+			typedef std::remove_reference<decltype(s->M[0])>::type RR;
+			std::shared_ptr<MLayer<RR>> ml(new MLayer<RR>(L, s->nx, s->ny, hx, hy, hz, z0));
+			s->ext_demag = [ml](const RR* M, RR* H) { ml->convolute(M, H); };
+		}
+		if (A) s->layA.assign(A, A + L);
+		if (alpha) s->layAlpha.assign(alpha, alpha + L);
+		if (K1) s->layK1.assign(K1, K1 + L);
+		if (Ms) {
+			s->Msmap.resize(s->n);
+			const size_t nl = (size_t)s->nx * s->ny;
+			for (size_t i = 0; i < s->n; i++) s->Msmap[i] = Ms[i / nl];
+		}
+	});
 }
 
 void or_sim_set_Msmap(void* h, const double* Ms)

@@ -27,6 +27,7 @@
 #include <cstdint>
 #include <cstring>
 #include <vector>
+#include <functional>
 #include <algorithm>
 #include <stdexcept>
 
@@ -415,6 +416,12 @@ struct Sim {
 	std::vector<R> Msmap;              // optional per-cell Ms (empty = uniform p.Ms)
 	std::vector<R> M, Heff, sM1, E[6];
 	Demag<R>* demag = nullptr;
+	// Layer stack: the nz planes are separate 2D layers (one ferromagnetic mesh each, as the
+	// reference's SDemag stack, Boris/SDemag.cpp:173-300): demag from ext_demag (H = op(M)),
+	// exchange within each layer, per-layer A, alpha, K1 (empty = p's value).
+	std::function<void(const R*, R*)> ext_demag;
+	bool layered = false;
+	std::vector<double> layA, layAlpha, layK1;
 
 	int method;
 	double time = 0, stagetime = 0, dT, dT_last, dT_min = 1e-15, dT_max = 3e-12, dT_increase = 2, err_high_fail = 1e-4, lte = 0;
@@ -443,6 +450,10 @@ struct Sim {
 	~Sim() { delete demag; }
 
 	R Ms_at(size_t i) const { return Msmap.empty() ? (R)p.Ms : Msmap[i]; }
+	size_t layer_of(size_t i) const { return i / ((size_t)nx * ny); }
+	R A_at(size_t i) const { return layA.empty() ? (R)p.A : (R)layA[layer_of(i)]; }
+	R alpha_at(size_t i) const { return layAlpha.empty() ? (R)p.alpha : (R)layAlpha[layer_of(i)]; }
+	R K1_at(size_t i) const { return layK1.empty() ? (R)p.K1 : (R)layK1[layer_of(i)]; }
 
 	// Zeeman (sets), Demag, Exchange, Anisotropy (add) — Boris/MeshModules.cpp:131-135.
 	void update_fields()
@@ -450,7 +461,12 @@ struct Sim {
 		double Hst[3]; applied_field(p, stagetime, Hst);
 		R Ha[3] = { (R)Hst[0], (R)Hst[1], (R)Hst[2] };
 		for (size_t i = 0; i < n; i++) for (int c = 0; c < 3; c++) Heff[i * 3 + c] = p.zeeman ? Ha[c] : (R)0;
-		if (demag) demag->convolute(M.data(), Heff.data(), false);
+		if (ext_demag) {
+			std::vector<R> Hd(n * 3);
+			ext_demag(M.data(), Hd.data());
+			for (size_t q = 0; q < n * 3; q++) Heff[q] += Hd[q];
+		}
+		else if (demag) demag->convolute(M.data(), Heff.data(), false);
 		if (p.exchange) add_exchange();
 		if (p.anisotropy) add_anisotropy();
 		evaluations++;
@@ -467,7 +483,7 @@ struct Sim {
 			size_t idx = (size_t)i + (size_t)j * nx + (size_t)k * nx * ny;
 			int ijk[3] = { i, j, k };
 			R lap[3] = { 0, 0, 0 };
-			for (int a = 0; a < 3; a++) {
+			for (int a = 0; a < (layered ? 2 : 3); a++) {
 				R diff[3] = { 0, 0, 0 };
 				bool lo = ijk[a] > 0, hi = ijk[a] < d[a] - 1;
 				for (int c = 0; c < 3; c++) {
@@ -480,7 +496,7 @@ struct Sim {
 				for (int c = 0; c < 3; c++) lap[c] += diff[c];
 			}
 			R Ms = Ms_at(idx);
-			R pre = (R)(2 * p.A) / ((R)kMU0 * Ms * Ms);
+			R pre = (R)2 * A_at(idx) / ((R)kMU0 * Ms * Ms);
 			for (int c = 0; c < 3; c++) Heff[idx * 3 + c] += pre * lap[c];
 		}
 	}
@@ -494,7 +510,7 @@ struct Sim {
 			size_t i = (size_t)ii;
 			R Ms = Ms_at(i);
 			R dot = (M[i * 3] * e[0] + M[i * 3 + 1] * e[1] + M[i * 3 + 2] * e[2]) / Ms;
-			R pre = ((R)2 / ((R)kMU0 * Ms)) * dot * ((R)p.K1 + 2 * (R)p.K2 * (1 - dot * dot));
+			R pre = ((R)2 / ((R)kMU0 * Ms)) * dot * (K1_at(i) + 2 * (R)p.K2 * (1 - dot * dot));
 			for (int c = 0; c < 3; c++) Heff[i * 3 + c] += pre * e[c];
 		}
 	}
@@ -503,7 +519,7 @@ struct Sim {
 	void llg(size_t i, R out[3]) const
 	{
 		R Ms = Ms_at(i);
-		R a = (R)p.alpha;
+		R a = alpha_at(i);
 		const R* m = &M[i * 3]; const R* H = &Heff[i * 3];
 		R mxh[3] = { m[1] * H[2] - m[2] * H[1], m[2] * H[0] - m[0] * H[2], m[0] * H[1] - m[1] * H[0] };
 		R mu[3] = { m[0] / Ms, m[1] / Ms, m[2] / Ms };

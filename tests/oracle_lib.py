@@ -48,6 +48,7 @@ def lib() -> C.CDLL:
     for nm in ("or_sim_set_M", "or_sim_get_M", "or_sim_get_Heff", "or_sim_state", "or_sim_average_m"):
         getattr(L, nm).argtypes = [vp, dp]
     L.or_sim_set_Msmap.argtypes = [vp, vp]
+    L.or_sim_set_layers.argtypes = [vp, i, d, d, dp, dp, i, vp, vp, vp, vp]
     L.or_sim_set_dt.argtypes = [vp, d]
     L.or_sim_set_astep.argtypes = [vp, d, d, d, d]
     L.or_sim_update_fields.argtypes = [vp]
@@ -204,6 +205,14 @@ class OracleSim:
     def set_ms_map(self, ms):
         self._ms = None if ms is None else np.ascontiguousarray(ms, dtype=np.float64)
         self.L.or_sim_set_Msmap(self.h, None if ms is None else self._ms.ctypes.data_as(C.c_void_p))
+
+    def set_layers(self, hx, hy, hz, z0, with_demag=True, Ms=None, A=None, alpha=None, K1=None):
+        """The sim's nz planes become a stack of 2D layers (create the sim with demag off)."""
+        hz = np.ascontiguousarray(hz, dtype=np.float64)
+        z0 = np.ascontiguousarray(z0, dtype=np.float64)
+        self._lay = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (Ms, A, alpha, K1)]
+        ptrs = [None if v is None else v.ctypes.data_as(C.c_void_p) for v in self._lay]
+        self.L.or_sim_set_layers(self.h, len(hz), float(hx), float(hy), hz, z0, int(with_demag), *ptrs)
 
     def set_dt(self, dt):
         self.L.or_sim_set_dt(self.h, dt)
