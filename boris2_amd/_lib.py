@@ -25,7 +25,7 @@ EXPORTS = [
     "bm_mlayer_create", "bm_mlayer_destroy", "bm_mlayer_convolute", "bm_mlayer_copy_kernels",
     "bm_batch_create", "bm_batch_destroy", "bm_batch_set_dt", "bm_batch_run", "bm_batch_get_state",
     "bm_batch_mesh_offset",
-    "bm_adj_create", "bm_adj_destroy", "bm_adj_forward", "bm_adj_backward",
+    "bm_adj_create", "bm_adj_destroy", "bm_adj_forward", "bm_adj_backward", "bm_adj_zeeman_grads", "bm_adj_set_layers",
 ]
 
 PROF_KINDS = ["fwd_x_r2c", "y_fft", "z_fft_kernel_ifft", "y_ifft", "inv_x_c2r", "y2d_fft_kernel_ifft",
@@ -85,6 +85,8 @@ def load() -> C.CDLL:
         "bm_adj_destroy": (i, [vp]),
         "bm_adj_forward": (i, [vp, vp, vp, vp, C.POINTER(d), vp]),
         "bm_adj_backward": (i, [vp, vp, C.POINTER(d), vp]),
+        "bm_adj_zeeman_grads": (i, [vp, C.POINTER(d)]),
+        "bm_adj_set_layers": (i, [vp, vp, vp, vp, vp]),
         "bm_ode_create": (i, [i, i, i, i, d, d, d, C.POINTER(FmParams), vp, vp, C.POINTER(vp)]),
         "bm_ode_destroy": (i, [vp]),
         "bm_ode_set_ms_map": (i, [vp, vp]),

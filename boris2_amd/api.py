@@ -361,22 +361,39 @@ class BatchSolver:
 
 
 class RK4Adjoint:
-    """Differentiable fixed-step RK4 rollout of one mesh: demag, exchange and a sinc-pulse
-    Zeeman field (params.wave = 1), per-cell Ms map.  loss = 1 - <(M_final/Ms) . m_target>;
-    backward() gives dL/dMs (per cell) and dL/d amp by the discrete adjoint."""
+    """Differentiable fixed-step RK4 rollout of one mesh: demag, exchange, uniaxial anisotropy
+    and a Zeeman field (constant, sinc pulse params.wave = 1 or ramp wave = 2), per-cell Ms
+    map.  loss = 1 - <(M_final/Ms) . m_target>; backward() gives dL/dMs (per cell) and
+    dL/d amp by the discrete adjoint, zeeman_grads() (dL/d amp, dL/d slope xyz).
+    layers = dict(hz=..., z0=..., A=None, alpha=None, K1=None) makes the n[2] planes a stack
+    of 2D layers (hx, hy from h) coupled by the multilayered demag (params must enable demag)."""
 
-    def __init__(self, n, h, params: "FMParams", Ms_map: torch.Tensor, dt: float, nsteps: int, ckpt_every: int = 100):
+    def __init__(self, n, h, params: "FMParams", Ms_map: torch.Tensor, dt: float, nsteps: int, ckpt_every: int = 100,
+                 layers: dict | None = None):
         require_device()
         self.L = _lib.load()
         self.n = tuple(int(v) for v in n)
         self.h = tuple(float(v) for v in h)
         self.params = params
         self.Ms_map = Ms_map.detach().to(device="cuda", dtype=torch.float32).contiguous().reshape(-1)
-        self.demag = Demag(self.n, self.h) if params.modules & BM_MOD_DEMAG else None
-        self._pc = params.to_c()
+        self.demag = self.mlayer = None
+        pc = params
+        if layers is not None:
+            if not params.modules & BM_MOD_DEMAG:
+                raise ValueError("a layer stack needs the demag module")
+            self.mlayer = MultiLayerDemag(self.n[0], self.n[1], self.h[0], self.h[1], layers["hz"], layers["z0"])
+            pc = FMParams(**{**params.__dict__, "modules": params.modules & ~BM_MOD_DEMAG})
+        elif params.modules & BM_MOD_DEMAG:
+            self.demag = Demag(self.n, self.h)
+        self._pc = pc.to_c()
         self.handle = C.c_void_p()
         check(self.L.bm_adj_create(*self.n, *self.h, C.byref(self._pc), self.demag.handle if self.demag else None,
                                    _ptr(self.Ms_map), float(dt), int(nsteps), int(ckpt_every), C.byref(self.handle)))
+        if layers is not None:
+            self._lay = [None if layers.get(k) is None else np.ascontiguousarray(layers[k], dtype=np.float64)
+                         for k in ("A", "alpha", "K1")]
+            ptrs = [None if v is None else v.ctypes.data_as(C.c_void_p) for v in self._lay]
+            check(self.L.bm_adj_set_layers(self.handle, self.mlayer.handle, *ptrs))
 
     def forward(self, m0: torch.Tensor, mt: torch.Tensor) -> tuple[float, torch.Tensor]:
         self._m0 = m0.detach().to(device="cuda", dtype=torch.float32).contiguous()
@@ -392,10 +409,19 @@ class RK4Adjoint:
         check(self.L.bm_adj_backward(self.handle, _ptr(g), C.byref(ga), _stream()))
         return g, ga.value
 
+    def zeeman_grads(self) -> np.ndarray:
+        """(dL/d amp, dL/d slope_x, dL/d slope_y, dL/d slope_z) of the last backward()."""
+        out = (C.c_double * 4)()
+        check(self.L.bm_adj_zeeman_grads(self.handle, out))
+        return np.array(out[:])
+
     def close(self) -> None:
         if self.handle:
             self.L.bm_adj_destroy(self.handle)
             self.handle = C.c_void_p()
+        for obj in (self.demag, self.mlayer):
+            if obj is not None:
+                obj.close()
 
     def __del__(self):
         try:
@@ -423,6 +449,36 @@ class RK4RolloutLoss(torch.autograd.Function):
         ctx.roll.close()
         s = gout.to(torch.float32)
         return (g * s).reshape(ctx.roll.Ms_map.shape), gout * ga, None, None, None, None, None, None, None, None
+
+
+class LayerStackRolloutLoss(torch.autograd.Function):
+    """loss = LayerStackRolloutLoss.apply(Ms_layers, slope, nx, ny, hx, hy, layers, params, dt,
+    nsteps, m0, mt, ckpt_every): fixed-step RK4 rollout of a layer stack (layers: dict with
+    hz, z0 and optional per-layer A, alpha, K1), differentiable in the per-layer Ms
+    (Ms_layers, L values) and the Zeeman ramp slope (3 values, A/m/s; params.wave = 2)."""
+
+    @staticmethod
+    def forward(ctx, Ms_layers, slope, nx, ny, hx, hy, layers, params, dt, nsteps, m0, mt, ckpt_every=100):
+        import dataclasses
+        L = len(layers["hz"])
+        p = dataclasses.replace(params, wave=2, slope=tuple(float(v) for v in slope.detach().cpu().reshape(3)))
+        Ms_map = Ms_layers.detach().to(device="cuda", dtype=torch.float32).reshape(L, 1).expand(L, nx * ny).reshape(-1)
+        roll = RK4Adjoint((nx, ny, L), (hx, hy, float(max(layers["hz"]))), p, Ms_map, dt, nsteps, ckpt_every, layers=layers)
+        loss, _ = roll.forward(m0, mt)
+        ctx.roll, ctx.L, ctx.nl = roll, L, nx * ny
+        ctx.shapes = (Ms_layers.shape, Ms_layers.dtype, Ms_layers.device, slope.shape, slope.dtype, slope.device)
+        return torch.tensor(loss, dtype=torch.float64, device=Ms_layers.device)
+
+    @staticmethod
+    def backward(ctx, gout):
+        g, _ = ctx.roll.backward()
+        gz = ctx.roll.zeeman_grads()
+        ctx.roll.close()
+        ms_shape, ms_dt, ms_dev, sl_shape, sl_dt, sl_dev = ctx.shapes
+        g_layers = g.double().reshape(ctx.L, ctx.nl).sum(1).to(device=ms_dev, dtype=ms_dt).reshape(ms_shape)
+        g_slope = torch.tensor(gz[1:4], dtype=sl_dt, device=sl_dev).reshape(sl_shape)
+        s = gout.to(ms_dt)
+        return g_layers * s, g_slope * gout.to(sl_dt), None, None, None, None, None, None, None, None, None, None, None
 
 
 def random_magnetisation(ncell: int, Ms: float, seed: int, M_norm: np.ndarray | None = None) -> np.ndarray:

@@ -1,17 +1,20 @@
-// Discrete adjoint of the fixed-step RK4 LLG rollout with demag, exchange and a Zeeman
-// field H(t) = Ha + amp * dir * sinc(w (t - t0)), per-cell Ms map.
+// Discrete adjoint of the fixed-step RK4 LLG rollout with demag, exchange, uniaxial
+// anisotropy and a Zeeman field H(t) = Ha + amp * dir * sinc(w (t - t0)) or Ha + slope * t,
+// per-cell Ms map; optionally a layer stack (the nz planes are 2D layers coupled by the
+// multilayered demag, per-layer A, alpha, K1, exchange within layers).
 //
 // No reference counterpart (the reference has no differentiation); the forward scheme is
 // the plain path's RK4 step (Boris/DiffEqFM_EvalsLLGCUDA.cu:416-641: Zeeman at the step's
 // start time, renormalisation to the cell's Ms after the step).
 //
 // Loss L = 1 - (1/n) sum_i (M_i / Ms_i) . mt_i of the final state, M0_i = Ms_i m0_i.
-// Gradients dL/dMs_i and dL/d amp.  The adjoint of one RK4 step runs the stages backwards;
-// each stage's vector-Jacobian product needs the field at the stage input (taped during the
-// forward recomputation) and one demag convolution of the field adjoint u — the demag
-// operator is symmetric (N_ij = N_ji, N(r) = N(-r)), so its adjoint is the same
-// convolution; the free-boundary exchange Laplacian is symmetric too, with the per-cell
-// prefactor 2A/(mu0 Ms^2) moved to the input side.
+// Gradients dL/dMs_i, dL/d amp and dL/d slope.  The adjoint of one RK4 step runs the stages
+// backwards; each stage's vector-Jacobian product needs the field at the stage input (taped
+// during the forward recomputation) and one demag convolution of the field adjoint u — the
+// single-mesh demag operator is symmetric (N_ij = N_ji, N(r) = N(-r)), so its adjoint is the
+// same convolution; the multilayered one runs its exact transpose (pair kernels swapped,
+// imaginary xz/yz parts negated); the free-boundary exchange Laplacian is symmetric, with
+// the per-cell prefactor 2A/(mu0 Ms^2) moved to the input side; anisotropy is local.
 //
 // Memory: the state is checkpointed every K steps; the backward pass recomputes one segment
 // of K steps at a time with a tape of the four stage inputs, their demag fields and the
@@ -35,11 +38,25 @@ struct AdjArgs {
 	int nx, ny, nz;
 	float ihx2, ihy2, ihz2;
 	float A, alpha, grel;
+	float K1, K2, ea[3];    // uniaxial anisotropy
 	float Hz[3];            // applied field at the step's start time
 	float dir[3], sc;       // pulse direction and sinc value at that time (d H / d amp = sc dir)
+	float tz;               // ramp: d H / d slope_c = tz e_c (the step's start time), else 0
 	int modules;
 	const float* Ms;
+	int layered;            // layer stack: per-layer A, alpha, K1, no exchange across layers
+	float lay_A[kMaxStackLayers], lay_alpha[kMaxStackLayers], lay_K1[kMaxStackLayers];
 };
+
+__device__ __forceinline__ CellMat adj_mat(const AdjArgs& a, int k)
+{
+	if (!a.layered) return CellMat{ a.A, a.alpha, a.K1 };
+	CellMat c{ a.lay_A[0], a.lay_alpha[0], a.lay_K1[0] };
+#pragma unroll
+	for (int l = 1; l < kMaxStackLayers; l++)
+		if (k == l) c = CellMat{ a.lay_A[l], a.lay_alpha[l], a.lay_K1[l] };
+	return c;
+}
 
 __device__ __forceinline__ F3 lap_at(const float* X, const AdjArgs& a, size_t idx, int i, int j, int k)
 {
@@ -56,7 +73,7 @@ __device__ __forceinline__ F3 lap_at(const float* X, const AdjArgs& a, size_t id
 		else if (j == 0) ly = ld3(X, idx + sy) - m;
 		else ly = ld3(X, idx - sy) - m;
 	}
-	if (a.nz > 1) {
+	if (a.nz > 1 && !a.layered) {
 		if (k > 0 && k < a.nz - 1) lz = ld3(X, idx + sz) + ld3(X, idx - sz) - 2.f * m;
 		else if (k == 0) lz = ld3(X, idx + sz) - m;
 		else lz = ld3(X, idx - sz) - m;
@@ -64,16 +81,23 @@ __device__ __forceinline__ F3 lap_at(const float* X, const AdjArgs& a, size_t id
 	return a.ihx2 * lx + a.ihy2 * ly + a.ihz2 * lz;
 }
 
-__device__ __forceinline__ float cex(const AdjArgs& a, float Ms) { return (2.f * a.A) / ((float)kMU0 * Ms * Ms); }
-__device__ __forceinline__ float pre_llg(const AdjArgs& a) { return (float)(-kGAMMA) * a.grel / (1.f + a.alpha * a.alpha); }
+__device__ __forceinline__ float cex(float A, float Ms) { return (2.f * A) / ((float)kMU0 * Ms * Ms); }
+__device__ __forceinline__ float pre_llg(const AdjArgs& a, float alpha) { return (float)(-kGAMMA) * a.grel / (1.f + alpha * alpha); }
 
-// field at X: Zeeman + demag (planar Hd) + exchange; also returns the unscaled Laplacian
-__device__ __forceinline__ F3 field_at(const AdjArgs& a, const float* X, const float* Hd, size_t n, size_t idx, int i, int j, int k, float Ms, F3& lap)
+// field at X: Zeeman + demag (planar Hd) + exchange + anisotropy; also returns the unscaled
+// Laplacian
+__device__ __forceinline__ F3 field_at(const AdjArgs& a, const CellMat& cm, const float* X, const float* Hd, size_t n, size_t idx,
+	int i, int j, int k, float Ms, F3& lap)
 {
 	F3 H = (a.modules & BM_MOD_ZEEMAN) ? F3{ a.Hz[0], a.Hz[1], a.Hz[2] } : F3{ 0.f, 0.f, 0.f };
 	if (a.modules & BM_MOD_DEMAG) H = H + F3{ Hd[idx], Hd[n + idx], Hd[2 * n + idx] };
 	lap = F3{ 0.f, 0.f, 0.f };
-	if (a.modules & BM_MOD_EXCHANGE) { lap = lap_at(X, a, idx, i, j, k); H = H + cex(a, Ms) * lap; }
+	if (a.modules & BM_MOD_EXCHANGE) { lap = lap_at(X, a, idx, i, j, k); H = H + cex(cm.A, Ms) * lap; }
+	if (a.modules & BM_MOD_ANIS_UNI) {
+		const F3 e{ a.ea[0], a.ea[1], a.ea[2] };
+		const float d = dot(ld3(X, idx), e) / Ms;
+		H = H + ((2.f / ((float)kMU0 * Ms)) * d * (cm.K1 + 2.f * a.K2 * (1.f - d * d))) * e;
+	}
 	return H;
 }
 
@@ -98,13 +122,14 @@ __global__ void __launch_bounds__(kAB) k_fwd(AdjArgs a, const float* __restrict_
 	for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
 		int i, j, k; cell_ijk(a, idx, i, j, k);
 		const float Ms = a.Ms[idx];
+		const CellMat cm = adj_mat(a, k);
 		F3 lap;
 		const F3 m = ld3(X, idx);
-		const F3 H = field_at(a, X, Hd, n, idx, i, j, k, Ms, lap);
+		const F3 H = field_at(a, cm, X, Hd, n, idx, i, j, k, Ms, lap);
 		const F3 mxh = cross(m, H);
 		const float iMs = 1.f / Ms;
 		const F3 mu{ m.x * iMs, m.y * iMs, m.z * iMs };
-		const F3 f = pre_llg(a) * (mxh + a.alpha * cross(mu, mxh));
+		const F3 f = pre_llg(a, cm.alpha) * (mxh + cm.alpha * cross(mu, mxh));
 		if (S == 0) { st3(acc, idx, f); st3(Xn, idx, ld3(Mn, idx) + (dt / 2) * f); }
 		else if (S == 1) { st3(acc, idx, ld3(acc, idx) + 2.f * f); st3(Xn, idx, ld3(Mn, idx) + (dt / 2) * f); }
 		else if (S == 2) { st3(acc, idx, ld3(acc, idx) + 2.f * f); st3(Xn, idx, ld3(Mn, idx) + dt * f); }
@@ -156,40 +181,55 @@ __global__ void k_adj_renorm(const float* Y, const float* Ms, const float* lam, 
 	}
 }
 
-// Stage VJP, part 1: w = (dt/6) wS lamY + cS dt lxn; lx = direct term; u (field adjoint) and
-// c u (AoS); gMs += Ms terms; per-block partial of sum_i u_i . dir sc.
+// Stage VJP, part 1: w = (dt/6) wS lamY + cS dt lxn; lx = direct term (anisotropy's local
+// field Jacobian included); u (field adjoint) and c u (AoS); gMs += Ms terms; per-block
+// partials of sum_i u_i . dir sc (amplitude) and sum_i u_i tz (slope), 4 per block.
 __global__ void __launch_bounds__(kAB) k_vjp_a(AdjArgs a, const float* __restrict__ X, const float* __restrict__ Hd,
 	const float* __restrict__ lamY, const float* __restrict__ lxn, float wS, float cS, float dt,
 	float* __restrict__ lx, float* __restrict__ u, float* __restrict__ cu, double* __restrict__ gMs, double* __restrict__ apart)
 {
-	__shared__ double red[kAB];
+	__shared__ double red[4][kAB];
 	const size_t n = (size_t)a.nx * a.ny * a.nz;
-	const float pr = pre_llg(a), al = a.alpha;
-	double ga = 0;
+	double gz[4] = { 0, 0, 0, 0 };
 	for (size_t idx = blockIdx.x * (size_t)kAB + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kAB) {
 		int i, j, k; cell_ijk(a, idx, i, j, k);
 		const float Ms = a.Ms[idx];
+		const CellMat cm = adj_mat(a, k);
+		const float pr = pre_llg(a, cm.alpha), al = cm.alpha;
 		F3 lap;
 		const F3 m = ld3(X, idx);
-		const F3 H = field_at(a, X, Hd, n, idx, i, j, k, Ms, lap);
+		const F3 H = field_at(a, cm, X, Hd, n, idx, i, j, k, Ms, lap);
 		F3 w = ((dt / 6) * wS) * ld3(lamY, idx);
 		if (lxn) w = w + (cS * dt) * ld3(lxn, idx);
 		const float mh = dot(m, H), mw = dot(m, w), hw = dot(H, w), mm = dot(m, m);
-		const F3 d = pr * (cross(H, w) + (al / Ms) * (mh * w + mw * H - (2.f * hw) * m));
+		F3 d = pr * (cross(H, w) + (al / Ms) * (mh * w + mw * H - (2.f * hw) * m));
 		const F3 uu = pr * (cross(w, m) + (al / Ms) * (mw * m - mm * w));
-		st3(lx, idx, d);
-		st3(u, idx, uu);
-		const float ce = cex(a, Ms);
-		st3(cu, idx, ce * uu);
+		const float ce = cex(cm.A, Ms);
 		double g = -(double)pr * (double)al / ((double)Ms * Ms) * (double)dot(cross(m, cross(m, H)), w);
 		if (a.modules & BM_MOD_EXCHANGE) g += (double)dot(uu, lap) * (-2.0 * (double)ce / (double)Ms);
+		if (a.modules & BM_MOD_ANIS_UNI) {
+			// H_an = 2/mu0 s (K1 + 2K2 - 2K2 s^2/Ms^2)/Ms^2 e with s = m.e: its Jacobian is
+			// 2/(mu0 Ms^2) (K1 + 2K2 - 6K2 d^2) e e^T (d = s/Ms), its Ms derivative
+			// 2/mu0 s e (-2K1 - 4K2 + 8K2 d^2)/Ms^3
+			const F3 e{ a.ea[0], a.ea[1], a.ea[2] };
+			const float sm = dot(m, e), dd = sm / Ms, ue = dot(uu, e);
+			d = d + ((2.f / ((float)kMU0 * Ms * Ms)) * (cm.K1 + 2.f * a.K2 - 6.f * a.K2 * dd * dd) * ue) * e;
+			g += 2.0 / kMU0 * (double)sm * (double)ue * (-2.0 * cm.K1 - 4.0 * a.K2 + 8.0 * a.K2 * (double)dd * dd) / ((double)Ms * Ms * Ms);
+		}
+		st3(lx, idx, d);
+		st3(u, idx, uu);
+		st3(cu, idx, ce * uu);
 		gMs[idx] += g;
-		ga += (double)(dot(uu, F3{ a.dir[0], a.dir[1], a.dir[2] }) * a.sc);
+		gz[0] += (double)(dot(uu, F3{ a.dir[0], a.dir[1], a.dir[2] }) * a.sc);
+		gz[1] += (double)(uu.x * a.tz); gz[2] += (double)(uu.y * a.tz); gz[3] += (double)(uu.z * a.tz);
 	}
-	red[threadIdx.x] = ga;
+	for (int c = 0; c < 4; c++) red[c][threadIdx.x] = gz[c];
 	__syncthreads();
-	for (int o = kAB / 2; o > 0; o >>= 1) { if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o]; __syncthreads(); }
-	if (threadIdx.x == 0) apart[blockIdx.x] += red[0];
+	for (int o = kAB / 2; o > 0; o >>= 1) {
+		if (threadIdx.x < o) for (int c = 0; c < 4; c++) red[c][threadIdx.x] += red[c][threadIdx.x + o];
+		__syncthreads();
+	}
+	if (threadIdx.x < 4) apart[4 * blockIdx.x + threadIdx.x] += red[threadIdx.x][0];
 }
 
 // Stage VJP, part 2: lx += N u (planar) + Lap(c u); lacc += lx
@@ -225,6 +265,10 @@ struct bm_adj_s {
 	double h[3];
 	bm_fm_params p;
 	bm_demag_t demag;
+	bm_mlayer_t ml = nullptr;     // layer stack (bm_adj_set_layers)
+	bool lay_set[3] = { false, false, false };
+	float lay[3][bm::kMaxStackLayers] = {};
+	double gz[4] = { 0, 0, 0, 0 };  // dL/d amp, dL/d slope of the last backward pass
 	const float* Ms;
 	double dt;
 	int nsteps, K, nck;
@@ -239,7 +283,7 @@ struct bm_adj_s {
 	float* lam = nullptr; float* lamY = nullptr; float* lacc = nullptr; float* lx = nullptr; float* lxn = nullptr;
 	float* u = nullptr; float* cu = nullptr; float* Nu = nullptr;
 	double* gMs = nullptr;
-	double* part = nullptr;       // per-block partials (loss, amp)
+	double* part = nullptr;       // per-block partials: loss (grid), then 4 Zeeman sums per block
 	double* h_part = nullptr;     // pinned
 	const float* m0 = nullptr; const float* mt = nullptr;
 	bool forward_done = false;
@@ -254,6 +298,17 @@ AdjArgs adj_args(bm_adj_s* o, int step)
 	a.nx = o->nx; a.ny = o->ny; a.nz = o->nz;
 	a.ihx2 = (float)(1.0 / (o->h[0] * o->h[0])); a.ihy2 = (float)(1.0 / (o->h[1] * o->h[1])); a.ihz2 = (float)(1.0 / (o->h[2] * o->h[2]));
 	a.A = (float)o->p.A; a.alpha = (float)o->p.alpha; a.grel = (float)o->p.grel;
+	a.K1 = (float)o->p.K1; a.K2 = (float)o->p.K2;
+	for (int c = 0; c < 3; c++) a.ea[c] = (float)o->p.ea[c];
+	if (o->ml) {
+		a.layered = 1;
+		a.ihz2 = 0.f;
+		for (int l = 0; l < kMaxStackLayers; l++) {
+			a.lay_A[l] = o->lay_set[0] ? o->lay[0][l] : a.A;
+			a.lay_alpha[l] = o->lay_set[1] ? o->lay[1][l] : a.alpha;
+			a.lay_K1[l] = o->lay_set[2] ? o->lay[2][l] : a.K1;
+		}
+	}
 	const double t = step * o->dt;
 	double sc = 0.0;
 	if (o->p.wave == 1) { const double x = o->p.w * (t - o->p.t0); sc = (std::fabs(x) < 1e-300) ? 1.0 : std::sin(x) / x; }
@@ -262,6 +317,7 @@ AdjArgs adj_args(bm_adj_s* o, int step)
 		a.dir[c] = (float)o->p.dir[c];
 	}
 	a.sc = (o->p.wave == 1 && (o->p.modules & BM_MOD_ZEEMAN)) ? (float)sc : 0.f;
+	a.tz = (o->p.wave == 2 && (o->p.modules & BM_MOD_ZEEMAN)) ? (float)t : 0.f;
 	a.modules = o->p.modules;
 	a.Ms = o->Ms;
 	return a;
@@ -282,7 +338,10 @@ int fwd_step(bm_adj_s* o, int step, const float* Mn, float* Mnext, int q, hipStr
 	for (int s = 0; s < 4; s++) {
 		float* Hd = q >= 0 ? tH(o, q, s) : o->Hd;
 		if (q >= 0 && s == 0) BM_HIP(hipMemcpyAsync(tX(o, q, 0), Mn, n3(o) * sizeof(float), hipMemcpyDeviceToDevice, st));
-		if (o->p.modules & BM_MOD_DEMAG) { int rc = demag_planar(o->demag, X, Hd, st); if (rc) return rc; }
+		if (o->p.modules & BM_MOD_DEMAG) {
+			int rc = o->ml ? mlayer_planar(o->ml, X, Hd, st) : demag_planar(o->demag, X, Hd, st);
+			if (rc) return rc;
+		}
 		float* Xn = s < 3 ? (q >= 0 ? tX(o, q, s + 1) : o->Xs + (size_t)s * n3(o)) : nullptr;
 		float* Y = (s == 3 && q >= 0) ? tX(o, q, 4) : nullptr;
 		switch (s) {
@@ -308,7 +367,10 @@ int adj_step(bm_adj_s* o, int step, int q, hipStream_t st)
 		const float* lxn = s < 3 ? o->lxn : nullptr;
 		hipLaunchKernelGGL(k_vjp_a, g, b, 0, st, a, tX(o, q, s), tH(o, q, s), o->lamY, lxn, wgt[s], s < 3 ? cst[s] : 0.f, dt,
 			o->lx, o->u, o->cu, o->gMs, o->part + o->grid);
-		if (o->p.modules & BM_MOD_DEMAG) { int rc = demag_planar(o->demag, o->u, o->Nu, st); if (rc) return rc; }
+		if (o->p.modules & BM_MOD_DEMAG) {
+			int rc = o->ml ? mlayer_planar_transposed(o->ml, o->u, o->Nu, st) : demag_planar(o->demag, o->u, o->Nu, st);
+			if (rc) return rc;
+		}
 		hipLaunchKernelGGL(k_vjp_b, g, b, 0, st, a, o->Nu, o->cu, o->lx, o->lacc);
 		std::swap(o->lx, o->lxn);
 	}
@@ -335,7 +397,6 @@ extern "C" int bm_adj_create(int nx, int ny, int nz, double hx, double hy, doubl
 	*out = nullptr;
 	if ((p->modules & BM_MOD_DEMAG) && (!demag || demag->nx != nx || demag->ny != ny || demag->nz != nz))
 		return fail(BM_ERR_ARG, "bm_adj_create: demag handle missing or of another mesh size");
-	if (p->modules & BM_MOD_ANIS_UNI) return fail(BM_ERR_ARG, "bm_adj_create: anisotropy is not differentiated");
 	bm_adj_s* o = new bm_adj_s();
 	o->nx = nx; o->ny = ny; o->nz = nz; o->n = (size_t)nx * ny * nz;
 	o->h[0] = hx; o->h[1] = hy; o->h[2] = hz; o->p = *p; o->demag = demag; o->Ms = Ms_map;
@@ -351,8 +412,8 @@ extern "C" int bm_adj_create(int nx, int ny, int nz, double hx, double hy, doubl
 		&& hipMalloc(&o->lx, b3) == hipSuccess && hipMalloc(&o->lxn, b3) == hipSuccess
 		&& hipMalloc(&o->u, b3) == hipSuccess && hipMalloc(&o->cu, b3) == hipSuccess && hipMalloc(&o->Nu, b3) == hipSuccess
 		&& hipMalloc(&o->gMs, o->n * sizeof(double)) == hipSuccess
-		&& hipMalloc(&o->part, 2 * (size_t)o->grid * sizeof(double)) == hipSuccess
-		&& hipHostMalloc(&o->h_part, 2 * (size_t)o->grid * sizeof(double)) == hipSuccess;
+		&& hipMalloc(&o->part, 5 * (size_t)o->grid * sizeof(double)) == hipSuccess
+		&& hipHostMalloc(&o->h_part, 5 * (size_t)o->grid * sizeof(double)) == hipSuccess;
 	if (!ok) { free_all(o); delete o; return fail(BM_ERR_NOMEM, "bm_adj_create: device memory"); }
 	*out = o;
 	return BM_OK;
@@ -403,7 +464,7 @@ extern "C" int bm_adj_backward(bm_adj_t o, float* gMs_out, double* gamp, void* s
 	const int nseg = o->nck - 1;
 	float* Mfin = o->ck + (size_t)nseg * n3(o);
 	hipLaunchKernelGGL(k_final, g, b, 0, st, Mfin, o->mt, o->Ms, o->n, o->lam, o->gMs);
-	BM_HIP(hipMemsetAsync(o->part + o->grid, 0, (size_t)o->grid * sizeof(double), st));
+	BM_HIP(hipMemsetAsync(o->part + o->grid, 0, 4 * (size_t)o->grid * sizeof(double), st));
 	for (int sg = nseg - 1; sg >= 0; sg--) {
 		const int s0 = sg * o->K, s1 = std::min(o->nsteps, s0 + o->K);
 		// recompute the segment from its checkpoint, taping every step
@@ -421,11 +482,39 @@ extern "C" int bm_adj_backward(bm_adj_t o, float* gMs_out, double* gamp, void* s
 	}
 	hipLaunchKernelGGL(k_m0_term, g, b, 0, st, o->m0, o->lam, o->gMs, o->n);
 	hipLaunchKernelGGL(k_to_float, g, b, 0, st, o->gMs, gMs_out, o->n);
-	BM_HIP(hipMemcpyAsync(o->h_part, o->part + o->grid, (size_t)o->grid * sizeof(double), hipMemcpyDeviceToHost, st));
+	BM_HIP(hipMemcpyAsync(o->h_part, o->part + o->grid, 4 * (size_t)o->grid * sizeof(double), hipMemcpyDeviceToHost, st));
 	BM_HIP(hipStreamSynchronize(st));
-	double ga = 0;
-	for (int q = 0; q < o->grid; q++) ga += o->h_part[q];
-	if (gamp) *gamp = ga;
+	for (int c = 0; c < 4; c++) o->gz[c] = 0;
+	for (int q = 0; q < o->grid; q++) for (int c = 0; c < 4; c++) o->gz[c] += o->h_part[4 * q + c];
+	if (gamp) *gamp = o->gz[0];
 	BM_HIP(hipGetLastError());
+	return BM_OK;
+}
+
+extern "C" int bm_adj_zeeman_grads(bm_adj_t o, double* out4)
+{
+	if (!o || !out4) return fail(BM_ERR_ARG, "bm_adj_zeeman_grads: null argument");
+	for (int c = 0; c < 4; c++) out4[c] = o->gz[c];
+	return BM_OK;
+}
+
+// Layer stack for the adjoint (the same contract as bm_ode_set_layers; per-layer Ms comes in
+// through the Ms map): the multilayered demag replaces the single-mesh one and BM_MOD_DEMAG
+// is switched on; per-layer A, alpha, K1 (NULL keeps the mesh value).
+extern "C" int bm_adj_set_layers(bm_adj_t o, bm_mlayer_t ml, const double* A, const double* alpha, const double* K1)
+{
+	if (!o || !ml) return fail(BM_ERR_ARG, "bm_adj_set_layers: null argument");
+	int L, nx, ny;
+	mlayer_dims(ml, &L, &nx, &ny, nullptr);
+	if (L != o->nz || nx != o->nx || ny != o->ny) return fail(BM_ERR_ARG, "bm_adj_set_layers: the stack's layers must be the mesh's nz planes (same nx, ny)");
+	if (L > kMaxStackLayers) return fail(BM_ERR_SIZE, "bm_adj_set_layers: at most 8 layers");
+	const double* src[3] = { A, alpha, K1 };
+	for (int q = 0; q < 3; q++) {
+		o->lay_set[q] = src[q] != nullptr;
+		for (int l = 0; l < kMaxStackLayers; l++) o->lay[q][l] = src[q] && l < L ? (float)src[q][l] : 0.f;
+	}
+	o->ml = ml;
+	o->p.modules |= BM_MOD_DEMAG;
+	o->forward_done = false;
 	return BM_OK;
 }

@@ -78,6 +78,8 @@ struct PassBatch {
 // multilayered demag (bm_mlayer.hip): H_demag of all layers into planar H3n (3 planes of
 // L*nx*ny); the handle's layer count, in-plane size and layer thicknesses (hz: L values)
 int mlayer_planar(bm_mlayer_s* m, const float* M, float* H3n, hipStream_t st);
+// the transposed operator N^T (the adjoint of the multilayered convolution), planar output
+int mlayer_planar_transposed(bm_mlayer_s* m, const float* M, float* H3n, hipStream_t st);
 void mlayer_dims(const bm_mlayer_s* m, int* L, int* nx, int* ny, double* hz);
 // rows per work item of the x passes for nx (0 if there is no plan)
 int x_tile_rows(int nx, bool fwd);

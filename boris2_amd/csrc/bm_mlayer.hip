@@ -181,8 +181,11 @@ __global__ void k_tensor_zshift(ZArgs a, double* D)
 // with the kernels of (kx, |ky| folded to <= Ny/2) and Kxy, Kyz negated above Ny/2 (odd in
 // y); the own layer first, then the others in index order (Boris/DemagKernelCollection_Mult.cpp:457-500).
 // Self kernels have Kxz = Kyz = 0.  In place: all layers' inputs are read before any output.
+// tr: the transposed operator (N^T)_sd = N_ds^T (the adjoint of the convolution): output s
+// sums the pair kernels K_ds over inputs d with the imaginary xz, yz parts negated (the
+// real-space kernel N_ds(-r)^T).
 template <int L>
-__global__ void __launch_bounds__(256) k_ml_mult(float2* __restrict__ S, const float* __restrict__ K, int NC, int Ny, int Sp)
+__global__ void __launch_bounds__(256) k_ml_mult(float2* __restrict__ S, const float* __restrict__ K, int NC, int Ny, int Sp, int tr)
 {
 	const int NYH = Ny / 2 + 1;
 	const size_t kq = (size_t)NC * NYH;
@@ -203,9 +206,10 @@ __global__ void __launch_bounds__(256) k_ml_mult(float2* __restrict__ S, const f
 #pragma unroll
 			for (int r = 0; r < L; r++) {
 				const int s = r == 0 ? d : (r <= d ? r - 1 : r);
-				const float* Kp = K + (size_t)(d * L + s) * 6 * kq + (size_t)kx * NYH + jj;
+				const float* Kp = K + (size_t)(tr ? s * L + d : d * L + s) * 6 * kq + (size_t)kx * NYH + jj;
+				const float tz = tr ? -1.f : 1.f;
 				const float kxx = Kp[0], kyy = Kp[kq], kzz = Kp[2 * kq];
-				const float kxy = sy * Kp[3 * kq], kxz = Kp[4 * kq], kyz = sy * Kp[5 * kq];
+				const float kxy = sy * Kp[3 * kq], kxz = tz * Kp[4 * kq], kyz = tz * sy * Kp[5 * kq];
 				const float2 fx = F[s][0], fy = F[s][1], fz = F[s][2];
 				// i k z = k * (-z.y, z.x)
 				o0.x += kxx * fx.x + kxy * fy.x - kxz * fz.y;  o0.y += kxx * fx.y + kxy * fy.y + kxz * fz.x;
@@ -219,12 +223,12 @@ __global__ void __launch_bounds__(256) k_ml_mult(float2* __restrict__ S, const f
 	}
 }
 
-int launch_ml_mult(int L, float2* S, const float* K, int NC, int Ny, int Sp, int ncu, hipStream_t st)
+int launch_ml_mult(int L, float2* S, const float* K, int NC, int Ny, int Sp, int ncu, hipStream_t st, int tr = 0)
 {
 	const int npts = NC * Ny;
 	dim3 g((unsigned)std::max(1, std::min((npts + 255) / 256, ncu * 8))), b(256);
 	switch (L) {
-#define C_(V_) case V_: hipLaunchKernelGGL((k_ml_mult<V_>), g, b, 0, st, S, K, NC, Ny, Sp); return BM_OK;
+#define C_(V_) case V_: hipLaunchKernelGGL((k_ml_mult<V_>), g, b, 0, st, S, K, NC, Ny, Sp, tr); return BM_OK;
 	C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8)
 #undef C_
 	}
@@ -396,15 +400,16 @@ extern "C" int bm_mlayer_destroy(bm_mlayer_t m)
 	return BM_OK;
 }
 
-// The convolution into out with the final x C2R in `mode` (0 planar set, 1 AoS set, 2 AoS add).
-static int ml_convolute(bm_mlayer_s* m, const float* M, float* out, int mode, hipStream_t st)
+// The convolution (tr: its transpose) into out with the final x C2R in `mode` (0 planar
+// set, 1 AoS set, 2 AoS add).
+static int ml_convolute(bm_mlayer_s* m, const float* M, float* out, int mode, hipStream_t st, int tr = 0)
 {
 	const int L = m->L, nlines = 3 * L * m->NC;
 	int rc = bm::pass_fwd_x(m->nx, M, m->Sx, m->ny, L, m->ntl, m->ncu, st);
 	if (rc == BM_OK) rc = bm::pass_y(m->Ny, -1, m->S, m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st);
 	if (rc == BM_OK) {
 		bm::prof_mark_begin(bm::PK_ML_MULT, st);
-		rc = launch_ml_mult(L, m->S, m->K, m->NC, m->Ny, m->Sp, m->ncu, st);
+		rc = launch_ml_mult(L, m->S, m->K, m->NC, m->Ny, m->Sp, m->ncu, st, tr);
 		bm::prof_mark_end(bm::PK_ML_MULT, st);
 	}
 	if (rc == BM_OK) rc = bm::pass_y(m->Ny, 1, m->S, m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st);
@@ -414,6 +419,7 @@ static int ml_convolute(bm_mlayer_s* m, const float* M, float* out, int mode, hi
 }
 
 int bm::mlayer_planar(bm_mlayer_s* m, const float* M, float* H3n, hipStream_t st) { return ml_convolute(m, M, H3n, 0, st); }
+int bm::mlayer_planar_transposed(bm_mlayer_s* m, const float* M, float* H3n, hipStream_t st) { return ml_convolute(m, M, H3n, 0, st, 1); }
 
 void bm::mlayer_dims(const bm_mlayer_s* m, int* L, int* nx, int* ny, double* hz)
 {

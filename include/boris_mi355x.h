@@ -199,11 +199,15 @@ int bm_batch_mesh_offset(bm_batch_t b, int mesh, long long* cell_offset, long lo
 /* ------------------------------------------------------------------------------------
  * Differentiation of the fixed-step RK4 rollout (no reference counterpart: the reference
  * has no differentiation).  Forward: nsteps RK4 steps of dt from M0_i = Ms_i m0_i with
- * demag (bm_demag handle of this mesh), exchange and Zeeman H(t) = Ha + amp dir
- * sinc(w (t - t0)) (params->wave = 1), the per-cell Ms map Ms_map (device, caller-owned,
- * n floats); loss L = 1 - (1/n) sum_i (M_i/Ms_i) . mt_i.  Backward: the discrete adjoint,
- * dL/dMs (device, n floats) and dL/d amp.  The state is checkpointed every ckpt_every steps
- * and each segment recomputed once in the backward pass. */
+ * demag (bm_demag handle of this mesh), exchange, uniaxial anisotropy and Zeeman
+ * H(t) = Ha + amp dir sinc(w (t - t0)) (params->wave = 1) or Ha + slope t (wave = 2), the
+ * per-cell Ms map Ms_map (device, caller-owned, n floats); loss
+ * L = 1 - (1/n) sum_i (M_i/Ms_i) . mt_i.  Backward: the discrete adjoint, dL/dMs (device,
+ * n floats) and dL/d amp; bm_adj_zeeman_grads then gives (dL/d amp, dL/d slope[3]).  The
+ * state is checkpointed every ckpt_every steps and each segment recomputed once in the
+ * backward pass.  bm_adj_set_layers (before bm_adj_forward) makes the nz planes a layer
+ * stack as bm_ode_set_layers does (multilayered demag with its exact transpose in the
+ * backward pass, per-layer A, alpha, K1; per-layer Ms through Ms_map). */
 typedef struct bm_adj_s* bm_adj_t;
 int bm_adj_create(int nx, int ny, int nz, double hx, double hy, double hz, const bm_fm_params* params,
 	bm_demag_t demag, const float* Ms_map, double dt, int nsteps, int ckpt_every, bm_adj_t* out);
@@ -211,6 +215,8 @@ int bm_adj_destroy(bm_adj_t a);
 /* m0 and mt: device float[3n] unit vectors; M_final (optional, device float[3n]). */
 int bm_adj_forward(bm_adj_t a, const float* m0, const float* mt, float* M_final, double* loss, void* stream);
 int bm_adj_backward(bm_adj_t a, float* dL_dMs, double* dL_damp, void* stream);
+int bm_adj_zeeman_grads(bm_adj_t a, double* out4);
+int bm_adj_set_layers(bm_adj_t a, bm_mlayer_t ml, const double* A, const double* alpha, const double* K1);
 
 /* ------------------------------------------------------------------------------------
  * Launch timing (no reference counterpart: the reference only times whole runs,

@@ -1,6 +1,8 @@
 // ORACLE — TEST INFRASTRUCTURE ONLY.
-// CPU FP64 discrete adjoint of the fixed-step RK4 LLG rollout with demag, exchange and a
-// Zeeman field H(t) = Ha + amp * dir * sinc(w (t - t0)), per-cell Ms map.  Only tests/,
+// CPU FP64 discrete adjoint of the fixed-step RK4 LLG rollout with demag, exchange, uniaxial
+// anisotropy and a Zeeman field H(t) = Ha + amp * dir * sinc(w (t - t0)) or Ha + slope * t,
+// per-cell Ms map; optionally a layer stack (the nz planes are 2D layers coupled by the
+// multilayered demag, per-layer A, alpha, K1, exchange within layers).  Only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg may load or call anything under
 // oracle/; the product (boris2_amd/) never does.
 //
@@ -10,12 +12,15 @@
 // renormalisation to the cell's Ms after the step).
 //
 // Loss: L = 1 - (1/n) sum_i (M_i / Ms_i) . mt_i of the final state; initial state
-// M0_i = Ms_i m0_i.  Gradients: dL/dMs_i (per cell) and dL/d amp.
+// M0_i = Ms_i m0_i.  Gradients: dL/dMs_i (per cell), dL/d amp and dL/d slope.
 // Parity: pinned by central finite differences of the same FP64 forward
 // (tests/test_oracle_adjoint.py).
 #pragma once
 
 #include "oracle_core.hpp"
+#include "oracle_mlayer.hpp"
+
+#include <memory>
 
 #include <cmath>
 #include <vector>
@@ -28,6 +33,10 @@ struct AdjRK4 {
 	Params p;
 	std::vector<double> Ms;
 	Demag<double>* dem = nullptr;
+	// layer stack (set_layers): multilayered demag, per-layer A, alpha, K1 (empty: p's)
+	std::shared_ptr<MLayer<double>> ml;
+	bool layered = false;
+	std::vector<double> layA, layAlpha, layK1;
 	double dt = 0; int nsteps = 0;
 	// tape: per step the four stage inputs X1..X4 and the pre-renormalisation sum Y
 	std::vector<std::vector<double>> X[4], Y;
@@ -41,6 +50,20 @@ struct AdjRK4 {
 		if (p.demag) dem = new Demag<double>(nx, ny, nz, h[0], h[1], h[2], Nz_override);
 	}
 	~AdjRK4() { delete dem; }
+
+	void set_layers(int L, double hx, double hy, const double* hz, const double* z0, bool with_demag,
+		const double* A, const double* alpha, const double* K1)
+	{
+		layered = true;
+		if (with_demag) ml.reset(new MLayer<double>(L, nx, ny, hx, hy, hz, z0));
+		if (A) layA.assign(A, A + L);
+		if (alpha) layAlpha.assign(alpha, alpha + L);
+		if (K1) layK1.assign(K1, K1 + L);
+	}
+	size_t layer_of(size_t i) const { return i / ((size_t)nx * ny); }
+	double A_at(size_t i) const { return layA.empty() ? p.A : layA[layer_of(i)]; }
+	double alpha_at(size_t i) const { return layAlpha.empty() ? p.alpha : layAlpha[layer_of(i)]; }
+	double K1_at(size_t i) const { return layK1.empty() ? p.K1 : layK1[layer_of(i)]; }
 
 	double zeeman_scale(double t) const
 	{
@@ -59,7 +82,7 @@ struct AdjRK4 {
 			const size_t idx = (size_t)i + (size_t)j * nx + (size_t)k * nx * ny;
 			const int ijk[3] = { i, j, k };
 			double l[3] = { 0, 0, 0 };
-			for (int a = 0; a < 3; a++) {
+			for (int a = 0; a < (layered ? 2 : 3); a++) {
 				const bool lo = ijk[a] > 0, hi = ijk[a] < d[a] - 1;
 				for (int c = 0; c < 3; c++) {
 					const double m0 = V[idx * 3 + c];
@@ -74,28 +97,41 @@ struct AdjRK4 {
 		}
 	}
 
-	double cex(size_t i) const { return 2 * p.A / (kMU0 * Ms[i] * Ms[i]); }
+	double cex(size_t i) const { return 2 * A_at(i) / (kMU0 * Ms[i] * Ms[i]); }
 
+	// Zeeman (Boris/Zeeman.cpp:164-300), demag, exchange (Boris/Exchange.cpp:108-156),
+	// uniaxial anisotropy (Boris/Anisotropy.cpp:82-115); lap = the unscaled Laplacian
 	void field(const double* Mv, double t, std::vector<double>& H, std::vector<double>& lap) const
 	{
 		double Hz[3]; applied_field(p, t, Hz);
 		for (size_t i = 0; i < n; i++) for (int c = 0; c < 3; c++) H[i * 3 + c] = p.zeeman ? Hz[c] : 0.0;
-		if (dem) dem->convolute(Mv, H.data(), false);
+		if (ml) {
+			std::vector<double> Hd(3 * n);
+			ml->convolute(Mv, Hd.data());
+			for (size_t q = 0; q < 3 * n; q++) H[q] += Hd[q];
+		}
+		else if (dem) dem->convolute(Mv, H.data(), false);
 		if (p.exchange) {
 			laplacian(Mv, lap.data());
 			for (size_t i = 0; i < n; i++) { const double ce = cex(i); for (int c = 0; c < 3; c++) H[i * 3 + c] += ce * lap[i * 3 + c]; }
 		}
+		if (p.anisotropy)
+			for (size_t i = 0; i < n; i++) {
+				const double d = dot(Mv + i * 3, p.ea) / Ms[i];
+				const double pre = 2 / (kMU0 * Ms[i]) * d * (K1_at(i) + 2 * p.K2 * (1 - d * d));
+				for (int c = 0; c < 3; c++) H[i * 3 + c] += pre * p.ea[c];
+			}
 	}
 
 	static void cross(const double* a, const double* b, double* o) { o[0] = a[1] * b[2] - a[2] * b[1]; o[1] = a[2] * b[0] - a[0] * b[2]; o[2] = a[0] * b[1] - a[1] * b[0]; }
 	static double dot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
-	double pre() const { return -kGAMMA * p.grel / (1 + p.alpha * p.alpha); }
+	double pre(double al) const { return -kGAMMA * p.grel / (1 + al * al); }
 
 	void rhs(const double* Mv, const double* H, double* k) const
 	{
-		const double pr = pre(), al = p.alpha;
 		for (size_t i = 0; i < n; i++) {
+			const double al = alpha_at(i), pr = pre(al);
 			const double* m = Mv + i * 3; const double* hh = H + i * 3;
 			double mxh[3], mmh[3], mu[3] = { m[0] / Ms[i], m[1] / Ms[i], m[2] / Ms[i] };
 			cross(m, hh, mxh); cross(mu, mxh, mmh);
@@ -140,14 +176,16 @@ struct AdjRK4 {
 		return 1.0 - s / (double)n;
 	}
 
-	// lamX = J_f(X)^T w (adjoint of k = f(X) at time t); accumulates dL/dMs and dL/d amp.
-	void vjp(const double* Xv, double t, const double* w, double* lamX, double* gMs, double& gamp) const
+	// lamX = J_f(X)^T w (adjoint of k = f(X) at time t); accumulates dL/dMs and the Zeeman
+	// gradients gz = (dL/d amp, dL/d slope).
+	void vjp(const double* Xv, double t, const double* w, double* lamX, double* gMs, double* gz) const
 	{
 		std::vector<double> H(3 * n), lap(3 * n), u(3 * n), cu(3 * n), Nu(3 * n, 0.0), Lcu(3 * n, 0.0);
 		field(Xv, t, H, lap);
-		const double pr = pre(), al = p.alpha, sc = zeeman_scale(t);
-		double ga = 0;
+		const double sc = zeeman_scale(t);
+		double ga = 0, gs[3] = { 0, 0, 0 };
 		for (size_t i = 0; i < n; i++) {
+			const double al = alpha_at(i), pr = pre(al);
 			const double* m = Xv + i * 3;
 			const double* Hi = &H[i * 3]; const double* wi = w + i * 3;
 			const double ms = Ms[i], mh = dot(m, Hi), mw = dot(m, wi), hw = dot(Hi, wi), mm = dot(m, m);
@@ -158,25 +196,37 @@ struct AdjRK4 {
 				u[i * 3 + c] = pr * (wxm[c] + (al / ms) * (m[c] * mw - mm * wi[c]));
 			}
 			// Ms enters f through alpha/Ms and H through the exchange prefactor 2A/(mu0 Ms^2)
+			// and the anisotropy field
 			gMs[i] += -pr * al / (ms * ms) * dot(mmh, wi);
 			if (p.exchange) gMs[i] += dot(&u[i * 3], &lap[i * 3]) * (-2 * cex(i) / ms);
+			if (p.anisotropy) {
+				// H_an = 2/mu0 s (K1 + 2K2 - 2K2 s^2/Ms^2)/Ms^2 e, s = m.e
+				const double sm = dot(m, p.ea), d = sm / ms, ue = dot(&u[i * 3], p.ea), K1 = K1_at(i);
+				const double dd = 2 / (kMU0 * ms * ms) * (K1 + 2 * p.K2 - 6 * p.K2 * d * d) * ue;
+				for (int c = 0; c < 3; c++) lamX[i * 3 + c] += dd * p.ea[c];
+				gMs[i] += 2 / kMU0 * sm * ue * (-2 * K1 - 4 * p.K2 + 8 * p.K2 * d * d) / (ms * ms * ms);
+			}
 			if (p.zeeman && p.wave == 1) ga += dot(&u[i * 3], p.dir) * sc;
+			if (p.zeeman && p.wave == 2) for (int c = 0; c < 3; c++) gs[c] += u[i * 3 + c] * t;
 			for (int c = 0; c < 3; c++) cu[i * 3 + c] = cex(i) * u[i * 3 + c];
 		}
-		gamp += ga;
+		gz[0] += ga;
+		for (int c = 0; c < 3; c++) gz[1 + c] += gs[c];
 		// through the fields: the demag operator is symmetric (N_ij = N_ji, N(r) = N(-r)), the
-		// free-boundary Laplacian is symmetric; the exchange prefactor sits on the output side
-		if (dem) dem->convolute(u.data(), Nu.data(), true);
+		// multilayered one has the exact transpose (N^T)_sd = N_ds^T, the free-boundary
+		// Laplacian is symmetric; the exchange prefactor sits on the output side
+		if (ml) ml->convolute(u.data(), Nu.data(), true);
+		else if (dem) dem->convolute(u.data(), Nu.data(), true);
 		if (p.exchange) laplacian(cu.data(), Lcu.data());
 		for (size_t q = 0; q < 3 * n; q++) lamX[q] += Nu[q] + Lcu[q];
 	}
 
-	// gradients of L(forward(m0)) with the tape of the last forward()
-	void backward(const double* m0, const double* mt, double* gMs, double& gamp)
+	// gradients of L(forward(m0)) with the tape of the last forward(); gz = (amp, slope xyz)
+	void backward(const double* m0, const double* mt, double* gMs, double* gz)
 	{
 		std::vector<double> lam(3 * n), lamY(3 * n), w(3 * n), lx(3 * n), lacc(3 * n);
 		for (size_t i = 0; i < n; i++) gMs[i] = 0;
-		gamp = 0;
+		for (int c = 0; c < 4; c++) gz[c] = 0;
 		// final state M_N = Ms * Y/|Y| of the last step; dL/dM_N and the explicit Ms term
 		{
 			const std::vector<double>& Yv = Y[nsteps - 1];
@@ -206,7 +256,7 @@ struct AdjRK4 {
 			for (int s = 3; s >= 0; s--) {
 				for (size_t q = 0; q < 3 * n; q++) w[q] = (dt / 6) * wgt[s] * lamY[q] + (s < 3 ? cst[s] * dt * lamXnext[q] : 0.0);
 				std::fill(lx.begin(), lx.end(), 0.0);
-				vjp(X[s][st].data(), t, w.data(), lx.data(), gMs, gamp);
+				vjp(X[s][st].data(), t, w.data(), lx.data(), gMs, gz);
 				// X_s = M + c_{s-1} dt k_{s-1} (s > 0) or X_0 = M
 				for (size_t q = 0; q < 3 * n; q++) lacc[q] += lx[q];
 				lamXnext = lx;

@@ -81,6 +81,7 @@ void* or_mlayer_create(int L, int nx, int ny, double hx, double hy, const double
 void or_mlayer_destroy(void* h) { delete (MLayer<double>*)h; }
 // M, H: L*nx*ny*3 doubles, layer-major
 void or_mlayer_convolute(void* h, const double* M, double* H) { ((MLayer<double>*)h)->convolute(M, H); }
+void or_mlayer_convolute_transposed(void* h, const double* M, double* H) { ((MLayer<double>*)h)->convolute(M, H, true); }
 
 void or_self_demag(double hx, double hy, double hz, double* out3) { self_demag(hx, hy, hz, out3); }
 double or_newell_f(double x, double y, double z) { return nw_f(x, y, z); }
@@ -171,12 +172,16 @@ double or_adj_forward(void* h, const double* m0, const double* mt, double* Mf)
 	a->forward(m0, Mf);
 	return a->loss(Mf, mt);
 }
-// gradients after or_adj_forward: dL/dMs (n) and dL/d amp
-void or_adj_backward(void* h, const double* m0, const double* mt, double* gMs, double* gamp)
+// gradients after or_adj_forward: dL/dMs (n) and gz = (dL/d amp, dL/d slope xyz)
+void or_adj_backward(void* h, const double* m0, const double* mt, double* gMs, double* gz)
 {
-	double g = 0;
-	((AdjRK4*)h)->backward(m0, mt, gMs, g);
-	*gamp = g;
+	((AdjRK4*)h)->backward(m0, mt, gMs, gz);
+}
+// layer stack for the adjoint (create it with demag off; with_demag adds the multilayered demag)
+void or_adj_set_layers(void* h, int L, double hx, double hy, const double* hz, const double* z0, int with_demag,
+	const double* A, const double* alpha, const double* K1)
+{
+	((AdjRK4*)h)->set_layers(L, hx, hy, hz, z0, with_demag != 0, A, alpha, K1);
 }
 
 void* or_sim_create(int method, int nx, int ny, int nz, double hx, double hy, double hz,

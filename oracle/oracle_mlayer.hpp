@@ -329,8 +329,11 @@ struct MLayer {
 			}
 	}
 
-	// M, H: L layers of nx*ny cells of 3 reals (layer-major, x fastest).  H = H_demag.
-	void convolute(const R* M, R* H)
+	// M, H: L layers of nx*ny cells of 3 reals (layer-major, x fastest).  H = H_demag, or with
+	// transpose the transposed operator (N^T)_sd = N_ds^T: in Fourier space the pair (d, s)
+	// kernel K_ds serves output s from input d with its imaginary xz, yz parts negated
+	// (the real-space kernel N_ds(-r)^T).
+	void convolute(const R* M, R* H, bool transpose = false)
 	{
 		const size_t nl = (size_t)nx * ny, fl = (size_t)Nc * Ny;
 		std::vector<C> F((size_t)L * 3 * fl), O((size_t)3 * fl);
@@ -356,7 +359,8 @@ struct MLayer {
 			// self contribution first, then the others in layer order
 			for (int q = 0; q < L; q++) {
 				const int s = q == 0 ? d : (q <= d ? q - 1 : q);
-				const size_t p = (size_t)d * L + s;
+				const size_t p = transpose ? (size_t)s * L + d : (size_t)d * L + s;
+				const R tz = transpose ? (R)-1 : (R)1;
 				for (int j = 0; j < Ny; j++) {
 					const int jj = j <= Ny / 2 ? j : Ny - j;
 					const R sy = j <= Ny / 2 ? (R)1 : (R)-1;
@@ -371,7 +375,7 @@ struct MLayer {
 							O[2 * fl + o] += kzz * fzv;
 						}
 						else {
-							const R kxy = sy * Ko[p][k * 3], kxz = Ko[p][k * 3 + 1], kyz = sy * Ko[p][k * 3 + 2];
+							const R kxy = sy * Ko[p][k * 3], kxz = tz * Ko[p][k * 3 + 1], kyz = tz * sy * Ko[p][k * 3 + 2];
 							O[o] += kxx * fxv + kxy * fyv + I1 * (kxz * fzv);
 							O[fl + o] += kxy * fxv + kyy * fyv + I1 * (kyz * fzv);
 							O[2 * fl + o] += I1 * (kxz * fxv) + I1 * (kyz * fyv) + kzz * fzv;

@@ -60,10 +60,12 @@ def lib() -> C.CDLL:
     L.or_mlayer_create.argtypes = [i, i, i, d, d, dp, dp]; L.or_mlayer_create.restype = vp
     L.or_mlayer_destroy.argtypes = [vp]
     L.or_mlayer_convolute.argtypes = [vp, dp, dp]
+    L.or_mlayer_convolute_transposed.argtypes = [vp, dp, dp]
     L.or_adj_create.argtypes = [i, i, i, dp, dp, i, dp, d, i, i]; L.or_adj_create.restype = vp
     L.or_adj_destroy.argtypes = [vp]
     L.or_adj_forward.argtypes = [vp, dp, dp, dp]; L.or_adj_forward.restype = d
     L.or_adj_backward.argtypes = [vp, dp, dp, dp, dp]
+    L.or_adj_set_layers.argtypes = [vp, i, d, d, dp, dp, i, vp, vp, vp]
     L.or_set_random_mag.argtypes = [dp, C.c_longlong, C.c_uint]
     L.or_borisrand.argtypes = [C.c_uint, i, dp]
     _L = L
@@ -131,6 +133,13 @@ class OracleMLayer:
         M = np.ascontiguousarray(M, dtype=np.float64).reshape(-1)
         H = np.zeros_like(M)
         self.L.or_mlayer_convolute(self.h, M, H)
+        return H.reshape(-1, 3)
+
+    def transposed(self, M):
+        """The transposed operator N^T applied to M."""
+        M = np.ascontiguousarray(M, dtype=np.float64).reshape(-1)
+        H = np.zeros_like(M)
+        self.L.or_mlayer_convolute_transposed(self.h, M, H)
         return H.reshape(-1, 3)
 
     def __del__(self):
@@ -269,11 +278,20 @@ class OracleAdjoint:
                                      np.ascontiguousarray(mt, float).reshape(-1), Mf)
         return loss, Mf.reshape(-1, 3)
 
+    def set_layers(self, hx, hy, hz, z0, with_demag=True, A=None, alpha=None, K1=None):
+        """The nz planes become a stack of 2D layers (create with demag off)."""
+        self._lay = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (A, alpha, K1)]
+        ptrs = [None if v is None else v.ctypes.data_as(C.c_void_p) for v in self._lay]
+        self.L.or_adj_set_layers(self.h, len(hz), float(hx), float(hy), np.array(hz, float), np.array(z0, float),
+                                 int(with_demag), *ptrs)
+
     def backward(self, m0, mt):
+        """dL/dMs per cell and dL/d amp; self.gz holds (dL/d amp, dL/d slope xyz)."""
         g = np.zeros(self.ncell)
-        ga = np.zeros(1)
-        self.L.or_adj_backward(self.h, np.ascontiguousarray(m0, float).reshape(-1), np.ascontiguousarray(mt, float).reshape(-1), g, ga)
-        return g, float(ga[0])
+        gz = np.zeros(4)
+        self.L.or_adj_backward(self.h, np.ascontiguousarray(m0, float).reshape(-1), np.ascontiguousarray(mt, float).reshape(-1), g, gz)
+        self.gz = gz
+        return g, float(gz[0])
 
     def __del__(self):
         try:

@@ -58,3 +58,59 @@ def test_autograd_function_matches_direct(gpu):
     assert abs(loss.item() - L) <= 1e-12
     assert torch.allclose(Mst.grad, 3.0 * g, rtol=1e-6, atol=0)
     assert abs(amp.grad.item() - 3.0 * ga) <= 1e-9 * abs(ga)
+
+
+HZ, Z0 = [8e-9, 4e-9, 16e-9], [0.0, 12e-9, 20e-9]
+LAY = dict(hz=HZ, z0=Z0, A=[1.3e-11, 1.0e-11, 1.6e-11], alpha=[0.02, 0.08, 0.03], K1=[0.0, 3e5, 1e5])
+
+
+def stack_problem(nx=16, ny=12, seed=6):
+    rng = np.random.default_rng(seed)
+    ncell = 3 * nx * ny
+    Ms = np.repeat(rng.uniform(6e5, 1.2e6, 3), nx * ny)
+    p = api.FMParams(Ms=8e5, A=1.3e-11, alpha=0.02, K2=2e4, ea=(0.3, 0.0, 0.95),
+                     modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ANIS_UNI | api.BM_MOD_ZEEMAN,
+                     Ha=(-1e4, 5e3, 2e3), wave=2, slope=(4e14, -2e14, 1e14))
+    m0 = rng.normal(size=(ncell, 3)); m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
+    mt = rng.normal(size=(ncell, 3)); mt /= np.linalg.norm(mt, axis=1, keepdims=True)
+    return (nx, ny, 3), p, Ms, m0, mt
+
+
+def test_gpu_layer_stack_adjoint_matches_oracle(gpu):
+    """Layer stack with per-layer A, alpha, K1, anisotropy with K2 on a tilted axis and a field
+    ramp: per-cell Ms gradients and the ramp-slope gradients against the FP64 oracle adjoint."""
+    n, p, Ms, m0, mt = stack_problem()
+    dt, steps = 200e-15, 40
+    q = api.FMParams(**{**p.__dict__, "modules": p.modules & ~api.BM_MOD_DEMAG})
+    o = O.OracleAdjoint(n, (4e-9,) * 3, q, Ms, dt, steps)
+    o.set_layers(4e-9, 4e-9, HZ, Z0, with_demag=True, A=LAY["A"], alpha=LAY["alpha"], K1=LAY["K1"])
+    L0, _ = o.forward(m0, mt)
+    g0, _ = o.backward(m0, mt)
+    gz0 = o.gz.copy()
+    r = api.RK4Adjoint(n, (4e-9, 4e-9, 16e-9), p, torch.tensor(Ms, dtype=torch.float32), dt, steps, 9, layers=LAY)
+    L, _ = r.forward(torch.tensor(m0, dtype=torch.float32), torch.tensor(mt, dtype=torch.float32))
+    g, _ = r.backward()
+    gz = r.zeeman_grads()
+    g = g.cpu().numpy().astype(np.float64)
+    assert abs(L - L0) <= 1e-5
+    assert np.linalg.norm(g - g0) / np.linalg.norm(g0) <= 2e-3
+    np.testing.assert_allclose(gz[1:], gz0[1:], rtol=2e-3, atol=2e-3 * np.abs(gz0[1:]).max())
+    r.close()
+
+
+def test_layer_stack_autograd_matches_direct(gpu):
+    n, p, Ms, m0, mt = stack_problem(nx=8, ny=8, seed=7)
+    dt, steps = 200e-15, 20
+    Ms_l = torch.tensor(Ms.reshape(3, -1)[:, 0], dtype=torch.float64, requires_grad=True)
+    slope = torch.tensor(p.slope, dtype=torch.float64, requires_grad=True)
+    m0t, mtt = torch.tensor(m0, dtype=torch.float32), torch.tensor(mt, dtype=torch.float32)
+    loss = api.LayerStackRolloutLoss.apply(Ms_l, slope, n[0], n[1], 4e-9, 4e-9, LAY, p, dt, steps, m0t, mtt, 5)
+    (2.0 * loss).backward()
+    r = api.RK4Adjoint(n, (4e-9, 4e-9, 16e-9), p, torch.tensor(Ms, dtype=torch.float32), dt, steps, 5, layers=LAY)
+    L, _ = r.forward(m0t, mtt)
+    g, _ = r.backward()
+    gz = r.zeeman_grads()
+    assert abs(loss.item() - L) <= 1e-12
+    gl = g.double().cpu().reshape(3, -1).sum(1)
+    assert torch.allclose(Ms_l.grad, 2.0 * gl, rtol=1e-6, atol=0)
+    assert torch.allclose(slope.grad, 2.0 * torch.tensor(gz[1:]), rtol=1e-9, atol=0)

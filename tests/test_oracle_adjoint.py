@@ -49,3 +49,63 @@ def test_adjoint_matches_central_fd():
     pm = api.FMParams(**{**p.__dict__, "amp": p.amp - ha})
     fd = (loss_at(n, pp, Ms, m0, mt, dt, steps) - loss_at(n, pm, Ms, m0, mt, dt, steps)) / (2 * ha)
     assert abs(ga - fd) <= 1e-6 * abs(fd), (ga, fd)
+
+
+def test_multilayer_transpose_is_exact():
+    """<v, N u> = <N^T v, u> for the multilayered demag, irregular layers included (the
+    far-field series is not volume-symmetric, so the transpose uses the pair kernels)."""
+    for nx, ny in ((16, 12), (48, 48)):
+        ml = O.OracleMLayer(nx, ny, 4e-9, 4e-9, [8e-9, 4e-9, 16e-9], [0.0, 12e-9, 20e-9])
+        rng = np.random.default_rng(4)
+        u = rng.standard_normal((3 * nx * ny, 3)); v = rng.standard_normal((3 * nx * ny, 3))
+        a, b = (v * ml(u)).sum(), (ml.transposed(v) * u).sum()
+        assert abs(a - b) <= 1e-12 * abs(a), (nx, a, b)
+        # with equal layers the operator is symmetric: N^T = N
+        eq = O.OracleMLayer(nx, ny, 4e-9, 4e-9, [4e-9, 4e-9], [0.0, 8e-9])
+        w = rng.standard_normal((2 * nx * ny, 3))
+        np.testing.assert_allclose(eq.transposed(w), eq(w), rtol=0, atol=1e-10 * np.abs(eq(w)).max())
+
+
+HZ, Z0 = [8e-9, 4e-9, 16e-9], [0.0, 12e-9, 20e-9]
+LA, LAL, LK1 = [1.3e-11, 1.0e-11, 1.6e-11], [0.02, 0.08, 0.03], [0.0, 3e5, 1e5]
+
+
+def layered_setup(nx=6, ny=5, steps=25, dt=200e-15, seed=3):
+    rng = np.random.default_rng(seed)
+    ncell = 3 * nx * ny
+    Ms = np.repeat(rng.uniform(6e5, 1.2e6, 3), nx * ny) * rng.uniform(0.9, 1.1, ncell)
+    p = api.FMParams(Ms=8e5, A=1.3e-11, alpha=0.02, K1=0.0, K2=2e4, ea=(0.3, 0.0, 0.95),
+                     modules=api.BM_MOD_EXCHANGE | api.BM_MOD_ANIS_UNI | api.BM_MOD_ZEEMAN,
+                     Ha=(-1e4, 5e3, 2e3), wave=2, slope=(4e14, -2e14, 1e14))
+    return (nx, ny, 3), p, Ms, unit_field(rng, ncell), unit_field(rng, ncell), dt, steps
+
+
+def layered_adj(n, p, Ms, dt, steps):
+    a = O.OracleAdjoint(n, (4e-9,) * 3, p, Ms, dt, steps)
+    a.set_layers(4e-9, 4e-9, HZ, Z0, with_demag=True, A=LA, alpha=LAL, K1=LK1)
+    return a
+
+
+def test_layered_adjoint_with_anisotropy_and_ramp_matches_central_fd():
+    n, p, Ms, m0, mt, dt, steps = layered_setup()
+    a = layered_adj(n, p, Ms, dt, steps)
+    a.forward(m0, mt)
+    g, _ = a.backward(m0, mt)
+    gz = a.gz.copy()
+
+    def L(pp, MM):
+        return layered_adj(n, pp, MM, dt, steps).forward(m0, mt)[0]
+
+    rng = np.random.default_rng(2)
+    for i in rng.choice(len(Ms), 6, replace=False):
+        hs = 1e-5 * Ms[i]
+        Mp, Mm = Ms.copy(), Ms.copy()
+        Mp[i] += hs; Mm[i] -= hs
+        fd = (L(p, Mp) - L(p, Mm)) / (2 * hs)
+        assert abs(g[i] - fd) <= 1e-6 * abs(fd) + 1e-18, (i, g[i], fd)
+    for c in range(3):
+        hsl = 1e-5 * abs(p.slope[c])
+        sp, sm = list(p.slope), list(p.slope)
+        sp[c] += hsl; sm[c] -= hsl
+        fd = (L(api.FMParams(**{**p.__dict__, "slope": tuple(sp)}), Ms) - L(api.FMParams(**{**p.__dict__, "slope": tuple(sm)}), Ms)) / (2 * hsl)
+        assert abs(gz[1 + c] - fd) <= 1e-6 * abs(fd), (c, gz[1 + c], fd)
