@@ -23,7 +23,7 @@ EXPORTS = [
     "bm_ode_restore", "bm_ode_get_state", "bm_ode_keep_heff", "bm_ode_copy_heff", "bm_ode_average_m",
     "bm_prof_begin", "bm_prof_end", "bm_prof_query",
     "bm_mlayer_create", "bm_mlayer_destroy", "bm_mlayer_convolute", "bm_mlayer_copy_kernels",
-    "bm_batch_create", "bm_batch_destroy", "bm_batch_set_dt", "bm_batch_run", "bm_batch_get_state",
+    "bm_batch_create", "bm_batch_create_layers", "bm_batch_destroy", "bm_batch_set_dt", "bm_batch_run", "bm_batch_get_state",
     "bm_batch_mesh_offset",
     "bm_adj_create", "bm_adj_destroy", "bm_adj_forward", "bm_adj_backward", "bm_adj_zeeman_grads", "bm_adj_set_layers",
 ]
@@ -39,6 +39,11 @@ class FmParams(C.Structure):
         ("Ha", C.c_double * 3), ("wave", C.c_int), ("amp", C.c_double), ("w", C.c_double),
         ("t0", C.c_double), ("dir", C.c_double * 3), ("slope", C.c_double * 3), ("modules", C.c_int),
     ]
+
+
+class LayerDesc(C.Structure):
+    _fields_ = [("nlayers", C.c_int), ("hz", C.c_void_p), ("z0", C.c_void_p), ("Ms", C.c_void_p),
+                ("A", C.c_void_p), ("alpha", C.c_void_p), ("K1", C.c_void_p)]
 
 
 class OdeState(C.Structure):
@@ -76,6 +81,7 @@ def load() -> C.CDLL:
         "bm_mlayer_convolute": (i, [vp, vp, vp, i, vp]),
         "bm_mlayer_copy_kernels": (i, [vp, vp, ll]),
         "bm_batch_create": (i, [i, i, vp, vp, vp, vp, i, C.POINTER(vp)]),
+        "bm_batch_create_layers": (i, [i, i, vp, vp, vp, vp, vp, i, C.POINTER(vp)]),
         "bm_batch_destroy": (i, [vp]),
         "bm_batch_set_dt": (i, [vp, d]),
         "bm_batch_run": (i, [vp, ll, vp]),

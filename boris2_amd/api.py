@@ -309,9 +309,11 @@ class BatchSolver:
     """Many independent meshes of uneven sizes integrated together with a fixed-step method
     (RK4 or TEuler): one launch per pass size per field evaluation, a step replayed from a
     captured hipGraph.  meshes: list of (n, h, FMParams); M0: list of per-mesh (ncell, 3)
-    arrays or None.  self.M holds every mesh's magnetisation concatenated; mesh(b) is a view."""
+    arrays or None.  self.M holds every mesh's magnetisation concatenated; mesh(b) is a view.
+    layers: None, or per mesh None or dict(hz=..., z0=..., Ms=None, A=None, alpha=None,
+    K1=None) making that mesh's n[2] planes a layer stack (as LayerStackSolver)."""
 
-    def __init__(self, method: int, meshes, M0=None, use_graph: bool = True):
+    def __init__(self, method: int, meshes, M0=None, use_graph: bool = True, layers=None):
         require_device()
         self.L = _lib.load()
         self.meshes = [(tuple(int(v) for v in n), tuple(float(v) for v in h), p) for n, h, p in meshes]
@@ -322,16 +324,32 @@ class BatchSolver:
         for b, (n, h, p) in enumerate(self.meshes):
             v = self.mesh(b)
             if M0 is None or M0[b] is None:
-                v[:, 0] = -p.Ms
+                ms = p.Ms
+                if layers is not None and layers[b] is not None and layers[b].get("Ms") is not None:
+                    ms = torch.as_tensor(np.repeat(np.asarray(layers[b]["Ms"], np.float32), n[0] * n[1]), device="cuda")
+                v[:, 0] = -ms
             else:
                 v.copy_(torch.as_tensor(np.asarray(M0[b], dtype=np.float32).reshape(-1, 3)))
         n3 = np.array([v for n, _, _ in self.meshes for v in n], dtype=np.int32)
         h3 = np.array([v for _, h, _ in self.meshes for v in h], dtype=np.float64)
         self._pc = (_lib.FmParams * nb)(*[p.to_c() for _, _, p in self.meshes])
         self.handle = C.c_void_p()
-        check(self.L.bm_batch_create(method, nb, n3.ctypes.data_as(C.c_void_p), h3.ctypes.data_as(C.c_void_p),
-                                     C.cast(self._pc, C.c_void_p), _ptr(self.M), 0 if use_graph else 1,
-                                     C.byref(self.handle)))
+        ld = None
+        if layers is not None:
+            self._keep = []
+            ld = (_lib.LayerDesc * nb)()
+            for b, lay in enumerate(layers):
+                if lay is None:
+                    continue
+                arrs = {k: np.ascontiguousarray(lay[k], dtype=np.float64) for k in ("hz", "z0", "Ms", "A", "alpha", "K1")
+                        if lay.get(k) is not None}
+                self._keep.append(arrs)
+                ld[b].nlayers = len(arrs["hz"])
+                for k, a in arrs.items():
+                    setattr(ld[b], k, a.ctypes.data_as(C.c_void_p))
+        check(self.L.bm_batch_create_layers(method, nb, n3.ctypes.data_as(C.c_void_p), h3.ctypes.data_as(C.c_void_p),
+                                            C.cast(self._pc, C.c_void_p), C.cast(ld, C.c_void_p) if ld is not None else None,
+                                            _ptr(self.M), 0 if use_graph else 1, C.byref(self.handle)))
 
     def mesh(self, b: int) -> torch.Tensor:
         return self.M[int(self.offsets[b]):int(self.offsets[b + 1])]

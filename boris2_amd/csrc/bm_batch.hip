@@ -11,7 +11,13 @@
 // Per-mesh arrays live concatenated (mesh b's cells at offset off[b]); the caller owns M.
 // Fixed-step methods only (RK4, TEuler); the per-mesh arguments (pointers, sizes,
 // parameters, dT) sit in device tables read by the kernels, so changing dT rewrites a
-// table and leaves the captured graph valid.
+// table and leaves the captured graph valid.  Time-dependent applied fields (sinc pulse,
+// ramp) read the step time from a device scalar that the graph itself advances.
+//
+// A mesh may be a layer stack (bm_layer_desc: its nz planes are 2D layers coupled by a
+// multilayered demag, per-layer Ms, A, alpha, K1, as bm_ode_set_layers): its x and y passes
+// join the launches of its sizes, its kernel product is one launch per layer count, and its
+// stage arguments carry the per-layer values.
 #include "bm_internal.h"
 #include "bm_stage.h"
 
@@ -40,15 +46,22 @@ struct bm_batch_s {
 	std::vector<bm_fm_params> p;
 	std::vector<double> h;
 	std::vector<bm_demag_t> dm;
+	std::vector<bm_mlayer_t> ml;   // layer stacks: their multilayered demag (else null)
+	std::vector<std::vector<float>> lay;  // layer stacks: A, alpha, K1 per layer (3 x kMaxStackLayers)
+	std::vector<int> lay_set;      // bits 0..2: A, alpha, K1 given per layer
+	float* Msmap = nullptr;        // per-cell Ms of the stacks with per-layer Ms (ntot floats)
+	double* d_time = nullptr;      // step start time on the device (time-dependent fields)
+	double* d_dT = nullptr;
 	float* M = nullptr;            // caller-owned, concatenated
 	float* Mb = nullptr;
 	float* sM1 = nullptr;
 	float* E[3] = { nullptr, nullptr, nullptr };
 	float* Hd = nullptr;           // planar per mesh: 3 * ncell[b] at 3 * off[b]
 	// pass groups and their tables
-	std::vector<Group> gx, gy, gz, gy2;
+	std::vector<Group> gx, gy, gz, gy2, gml;
 	std::vector<Tab<FwdArgs>> fwd[2];      // [input parity][nx group]
 	std::vector<Tab<PassBatch>> inv, ty, tz, ty2;
+	std::vector<Tab<MlBatch>> tml;
 	Tab<StageArgs> stage[2];               // [input parity]
 	void* dev_tables = nullptr;
 	size_t dev_bytes = 0;
@@ -71,6 +84,10 @@ void free_all(bm_batch_s* o)
 	if (o->ev_out) (void)hipEventDestroy(o->ev_out);
 	if (o->gs) (void)hipStreamDestroy(o->gs);
 	for (bm_demag_t d : o->dm) if (d) bm_demag_destroy(d);
+	for (bm_mlayer_t m : o->ml) if (m) bm_mlayer_destroy(m);
+	if (o->Msmap) (void)hipFree(o->Msmap);
+	if (o->d_time) (void)hipFree(o->d_time);
+	if (o->d_dT) (void)hipFree(o->d_dT);
 	if (o->Mb) (void)hipFree(o->Mb);
 	if (o->sM1) (void)hipFree(o->sM1);
 	for (int q = 0; q < 3; q++) if (o->E[q]) (void)hipFree(o->E[q]);
@@ -105,8 +122,25 @@ StageArgs stage_args(bm_batch_s* o, int b, int parity)
 	a.dT = (float)o->dT;
 	a.modules = p.modules;
 	a.map = 2;
+	if ((p.modules & BM_MOD_ZEEMAN) && (p.wave == 1 || p.wave == 2)) {
+		a.tnow = o->d_time; a.wave = p.wave;
+		a.amp = (float)p.amp; a.w = (float)p.w; a.t0 = (float)p.t0;
+		for (int c = 0; c < 3; c++) { a.dir[c] = (float)p.dir[c]; a.slope[c] = (float)p.slope[c]; }
+	}
+	if (o->ml[b]) {
+		a.layered = 1;
+		a.ihz2 = 0.f;
+		for (int l = 0; l < kMaxStackLayers; l++) {
+			a.lay_A[l] = (o->lay_set[b] & 1) ? o->lay[b][l] : a.A;
+			a.lay_alpha[l] = (o->lay_set[b] & 2) ? o->lay[b][kMaxStackLayers + l] : a.alpha;
+			a.lay_K1[l] = (o->lay_set[b] & 4) ? o->lay[b][2 * kMaxStackLayers + l] : a.K1;
+		}
+		if (o->Msmap && (o->lay_set[b] & 8)) a.Ms_map = o->Msmap + o->off[b];
+	}
 	return a;
 }
+
+__global__ void k_time_advance(double* t, const double* dT) { *t += *dT; }
 
 // Group the meshes by pass size, build every table on the host and upload them in one
 // allocation.
@@ -114,6 +148,13 @@ int build_tables(bm_batch_s* o)
 {
 	const int nb = o->nb;
 	for (int b = 0; b < nb; b++) {
+		if (o->ml[b]) {
+			const MlView v = mlayer_view(o->ml[b]);
+			add_to(o->gx, o->nx[b], b);
+			add_to(o->gy, v.Ny, b);
+			add_to(o->gml, v.L, b);
+			continue;
+		}
 		if (!o->dm[b]) continue;
 		add_to(o->gx, o->nx[b], b);
 		if (o->nz[b] > 1) { add_to(o->gy, o->dm[b]->Ny, b); add_to(o->gz, o->dm[b]->Nz, b); }
@@ -124,28 +165,38 @@ int build_tables(bm_batch_s* o)
 	o->ty.assign(o->gy.size(), Tab<PassBatch>());
 	o->tz.assign(o->gz.size(), Tab<PassBatch>());
 	o->ty2.assign(o->gy2.size(), Tab<PassBatch>());
+	o->tml.assign(o->gml.size(), Tab<MlBatch>());
 	for (size_t g = 0; g < o->gx.size(); g++) {
 		const int nx = o->gx[g].size;
 		const int tf = x_tile_rows(nx, true), ti = x_tile_rows(nx, false);
 		for (int b : o->gx[g].meshes) {
-			bm_demag_s* d = o->dm[b];
+			// a layer stack runs the x passes over its L layer planes
+			float2* Sx; int ny, nz, ntl; double scale;
+			if (o->ml[b]) { const MlView v = mlayer_view(o->ml[b]); Sx = v.Sx; ny = v.ny; nz = v.L; ntl = v.ntl; scale = 1.0 / ((double)v.Nx * v.Ny); }
+			else { bm_demag_s* d = o->dm[b]; Sx = d->Sx; ny = d->ny; nz = d->nz; ntl = d->ntl; scale = 1.0 / ((double)d->Nx * d->Ny * d->Nz); }
 			for (int par = 0; par < 2; par++) {
 				FwdArgs f{};
-				f.M = (par ? o->Mb : o->M) + 3 * o->off[b]; f.Sx = d->Sx;
-				f.ny = d->ny; f.nz = d->nz; f.ntl = d->ntl; f.ntiles = (d->ny + tf - 1) / tf;
+				f.M = (par ? o->Mb : o->M) + 3 * o->off[b]; f.Sx = Sx;
+				f.ny = ny; f.nz = nz; f.ntl = ntl; f.ntiles = (ny + tf - 1) / tf;
 				o->fwd[par][g].h.push_back(f);
 			}
 			PassBatch q{};
-			q.Sx = d->Sx; q.out = o->Hd + 3 * o->off[b]; q.ny = d->ny; q.nz = d->nz; q.ntl = d->ntl; q.ntiles = (d->ny + ti - 1) / ti;
-			q.scale = (float)(1.0 / ((double)d->Nx * d->Ny * d->Nz));
+			q.Sx = Sx; q.out = o->Hd + 3 * o->off[b]; q.ny = ny; q.nz = nz; q.ntl = ntl; q.ntiles = (ny + ti - 1) / ti;
+			q.scale = (float)scale;
 			o->inv[g].h.push_back(q);
 		}
 	}
 	for (size_t g = 0; g < o->gy.size(); g++)
 		for (int b : o->gy[g].meshes) {
-			bm_demag_s* d = o->dm[b];
-			PassBatch q{}; q.S = d->S; q.Sx = d->Sx; q.Sp = d->Sp; q.NC = d->NC; q.ntl = d->ntl; q.nlines = 3 * d->nz * d->NC;
+			PassBatch q{};
+			if (o->ml[b]) { const MlView v = mlayer_view(o->ml[b]); q.S = v.S; q.Sx = v.Sx; q.Sp = v.Sp; q.NC = v.NC; q.ntl = v.ntl; q.nlines = 3 * v.L * v.NC; }
+			else { bm_demag_s* d = o->dm[b]; q.S = d->S; q.Sx = d->Sx; q.Sp = d->Sp; q.NC = d->NC; q.ntl = d->ntl; q.nlines = 3 * d->nz * d->NC; }
 			o->ty[g].h.push_back(q);
+		}
+	for (size_t g = 0; g < o->gml.size(); g++)
+		for (int b : o->gml[g].meshes) {
+			const MlView v = mlayer_view(o->ml[b]);
+			o->tml[g].h.push_back(MlBatch{ v.S, v.K, v.NC, v.Ny, v.Sp });
 		}
 	for (size_t g = 0; g < o->gz.size(); g++)
 		for (int b : o->gz[g].meshes) {
@@ -173,12 +224,13 @@ int build_tables(bm_batch_s* o)
 		pieces.push_back({ at, { (const void*)t.h.data(), n } });
 		return at;
 	};
-	std::vector<size_t> at_fwd[2], at_inv, at_y, at_z, at_y2;
+	std::vector<size_t> at_fwd[2], at_inv, at_y, at_z, at_y2, at_ml;
 	for (int par = 0; par < 2; par++) for (auto& t : o->fwd[par]) at_fwd[par].push_back(place(t));
 	for (auto& t : o->inv) at_inv.push_back(place(t));
 	for (auto& t : o->ty) at_y.push_back(place(t));
 	for (auto& t : o->tz) at_z.push_back(place(t));
 	for (auto& t : o->ty2) at_y2.push_back(place(t));
+	for (auto& t : o->tml) at_ml.push_back(place(t));
 	const size_t at_s0 = place(o->stage[0]), at_s1 = place(o->stage[1]);
 	if (hipMalloc(&o->dev_tables, std::max<size_t>(bytes, 256)) != hipSuccess) return fail(BM_ERR_NOMEM, "batch tables");
 	o->dev_bytes = bytes;
@@ -189,14 +241,17 @@ int build_tables(bm_batch_s* o)
 	for (size_t g = 0; g < o->ty.size(); g++) o->ty[g].d = (PassBatch*)(base + at_y[g]);
 	for (size_t g = 0; g < o->tz.size(); g++) o->tz[g].d = (PassBatch*)(base + at_z[g]);
 	for (size_t g = 0; g < o->ty2.size(); g++) o->ty2[g].d = (PassBatch*)(base + at_y2[g]);
+	for (size_t g = 0; g < o->tml.size(); g++) o->tml[g].d = (MlBatch*)(base + at_ml[g]);
 	o->stage[0].d = (StageArgs*)(base + at_s0);
 	o->stage[1].d = (StageArgs*)(base + at_s1);
 	return BM_OK;
 }
 
-// rewrite dT in the stage tables (the only per-step-size data the kernels read)
+// rewrite dT in the stage tables (the only per-step-size data the kernels read) and the
+// device copy the step-time advance reads
 int upload_dt(bm_batch_s* o)
 {
+	BM_HIP(hipMemcpy(o->d_dT, &o->dT, sizeof(double), hipMemcpyHostToDevice));
 	for (int par = 0; par < 2; par++) {
 		for (StageArgs& a : o->stage[par].h) a.dT = (float)o->dT;
 		BM_HIP(hipMemcpy(o->stage[par].d, o->stage[par].h.data(), o->stage[par].h.size() * sizeof(StageArgs), hipMemcpyHostToDevice));
@@ -214,6 +269,8 @@ int eval_all(bm_batch_s* o, int step, int par, hipStream_t st)
 		rc = batch_y(o->gy[g].size, -1, o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gz.size() && !rc; g++)
 		rc = batch_z(o->gz[g].size, o->tz[g].h.data(), o->tz[g].d, (int)o->tz[g].h.size(), o->ncu, st);
+	for (size_t g = 0; g < o->gml.size() && !rc; g++)
+		rc = batch_ml_mult(o->gml[g].size, o->tml[g].h.data(), o->tml[g].d, (int)o->tml[g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gy.size() && !rc; g++)
 		rc = batch_y(o->gy[g].size, 1, o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gy2.size() && !rc; g++)
@@ -230,6 +287,7 @@ int one_step(bm_batch_s* o, hipStream_t st)
 		int rc = eval_all(o, s, s & 1, st);
 		if (rc) return rc;
 	}
+	hipLaunchKernelGGL(k_time_advance, dim3(1), dim3(1), 0, st, o->d_time, o->d_dT);
 	return BM_OK;
 }
 
@@ -237,6 +295,12 @@ int one_step(bm_batch_s* o, hipStream_t st)
 
 extern "C" int bm_batch_create(int method, int nmesh, const int* n3, const double* h3, const bm_fm_params* params,
 	float* M, int flags, bm_batch_t* out)
+{
+	return bm_batch_create_layers(method, nmesh, n3, h3, params, nullptr, M, flags, out);
+}
+
+extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, const double* h3, const bm_fm_params* params,
+	const bm_layer_desc* layers, float* M, int flags, bm_batch_t* out)
 {
 	if (!out || nmesh < 1 || !n3 || !h3 || !params || !M) return fail(BM_ERR_ARG, "bm_batch_create: bad arguments");
 	*out = nullptr;
@@ -262,9 +326,40 @@ extern "C" int bm_batch_create(int method, int nmesh, const int* n3, const doubl
 		o->p.push_back(params[b]);
 	}
 	o->dm.assign(nmesh, nullptr);
+	o->ml.assign(nmesh, nullptr);
+	o->lay.assign(nmesh, std::vector<float>(3 * kMaxStackLayers, 0.f));
+	o->lay_set.assign(nmesh, 0);
 	int rc = BM_OK;
-	for (int b = 0; b < nmesh && rc == BM_OK; b++)
-		if (o->p[b].modules & BM_MOD_DEMAG) rc = bm_demag_create(o->nx[b], o->ny[b], o->nz[b], o->h[3 * b], o->h[3 * b + 1], o->h[3 * b + 2], 0, &o->dm[b]);
+	bool any_lay_ms = false;
+	for (int b = 0; b < nmesh && rc == BM_OK; b++) {
+		const bm_layer_desc* ld = layers ? &layers[b] : nullptr;
+		if (ld && ld->nlayers > 0) {
+			if (ld->nlayers != o->nz[b] || ld->nlayers > kMaxStackLayers || !ld->hz || !ld->z0)
+				rc = fail(BM_ERR_ARG, "bm_batch_create_layers: a stack needs nz == nlayers <= 8 and its hz, z0");
+			else if (!(o->p[b].modules & BM_MOD_DEMAG))
+				rc = fail(BM_ERR_ARG, "bm_batch_create_layers: a layer stack needs the demag module");
+			else rc = bm_mlayer_create(ld->nlayers, o->nx[b], o->ny[b], o->h[3 * b], o->h[3 * b + 1], ld->hz, ld->z0, 0, &o->ml[b]);
+			const double* src[3] = { ld->A, ld->alpha, ld->K1 };
+			for (int q = 0; q < 3; q++)
+				if (src[q]) { o->lay_set[b] |= 1 << q; for (int l = 0; l < ld->nlayers; l++) o->lay[b][q * kMaxStackLayers + l] = (float)src[q][l]; }
+			if (ld->Ms) { o->lay_set[b] |= 8; any_lay_ms = true; }
+		}
+		else if (o->p[b].modules & BM_MOD_DEMAG) rc = bm_demag_create(o->nx[b], o->ny[b], o->nz[b], o->h[3 * b], o->h[3 * b + 1], o->h[3 * b + 2], 0, &o->dm[b]);
+	}
+	if (rc == BM_OK && (hipMalloc(&o->d_time, sizeof(double)) != hipSuccess || hipMalloc(&o->d_dT, sizeof(double)) != hipSuccess
+		|| hipMemset(o->d_time, 0, sizeof(double)) != hipSuccess || hipMemcpy(o->d_dT, &o->dT, sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
+		rc = fail(BM_ERR_NOMEM, "bm_batch_create: time scalars");
+	if (rc == BM_OK && any_lay_ms) {
+		std::vector<float> hm(o->ntot, 0.f);
+		for (int b = 0; b < nmesh; b++) {
+			if (!(o->lay_set[b] & 8)) continue;
+			const size_t nl = (size_t)o->nx[b] * o->ny[b];
+			for (size_t i = 0; i < o->ncell[b]; i++) hm[o->off[b] + i] = (float)layers[b].Ms[i / nl];
+		}
+		if (hipMalloc(&o->Msmap, o->ntot * sizeof(float)) != hipSuccess
+			|| hipMemcpy(o->Msmap, hm.data(), o->ntot * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+			rc = fail(BM_ERR_NOMEM, "bm_batch_create: Ms map");
+	}
 	const size_t b3 = o->ntot * 3 * sizeof(float);
 	if (rc == BM_OK) {
 		bool ok = hipMalloc(&o->Mb, b3) == hipSuccess && hipMalloc(&o->sM1, b3) == hipSuccess && hipMalloc(&o->Hd, b3) == hipSuccess;

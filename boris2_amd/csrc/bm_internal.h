@@ -81,6 +81,15 @@ int mlayer_planar(bm_mlayer_s* m, const float* M, float* H3n, hipStream_t st);
 // the transposed operator N^T (the adjoint of the multilayered convolution), planar output
 int mlayer_planar_transposed(bm_mlayer_s* m, const float* M, float* H3n, hipStream_t st);
 void mlayer_dims(const bm_mlayer_s* m, int* L, int* nx, int* ny, double* hz);
+// Pointers and sizes of a multilayered demag handle (batched launches).
+struct MlView {
+	float2* Sx; float2* S; const float* K;
+	int L, nx, ny, Nx, Ny, NC, NYH, Sp, ntl;
+};
+MlView mlayer_view(bm_mlayer_s* m);
+// Per-stack arguments of the batched multilayer kernel product.
+struct MlBatch { float2* S; const float* K; int NC, Ny, Sp; };
+int batch_ml_mult(int L, const MlBatch* h_tab, const MlBatch* d_tab, int nb, int ncu, hipStream_t st);
 // rows per work item of the x passes for nx (0 if there is no plan)
 int x_tile_rows(int nx, bool fwd);
 // Batched passes over nb meshes sharing the pass size (tables: host copy for the grid size,

@@ -185,8 +185,10 @@ __global__ void k_tensor_zshift(ZArgs a, double* D)
 // sums the pair kernels K_ds over inputs d with the imaginary xz, yz parts negated (the
 // real-space kernel N_ds(-r)^T).
 template <int L>
-__global__ void __launch_bounds__(256) k_ml_mult(float2* __restrict__ S, const float* __restrict__ K, int NC, int Ny, int Sp, int tr)
+__global__ void __launch_bounds__(256) k_ml_mult(float2* __restrict__ S, const float* __restrict__ K, int NC, int Ny, int Sp, int tr,
+	const MlBatch* __restrict__ bt)
 {
+	if (bt) { const MlBatch& b = bt[blockIdx.y]; S = b.S; K = b.K; NC = b.NC; Ny = b.Ny; Sp = b.Sp; }   // batched: one stack per grid row
 	const int NYH = Ny / 2 + 1;
 	const size_t kq = (size_t)NC * NYH;
 	const int npts = NC * Ny;
@@ -228,7 +230,7 @@ int launch_ml_mult(int L, float2* S, const float* K, int NC, int Ny, int Sp, int
 	const int npts = NC * Ny;
 	dim3 g((unsigned)std::max(1, std::min((npts + 255) / 256, ncu * 8))), b(256);
 	switch (L) {
-#define C_(V_) case V_: hipLaunchKernelGGL((k_ml_mult<V_>), g, b, 0, st, S, K, NC, Ny, Sp, tr); return BM_OK;
+#define C_(V_) case V_: hipLaunchKernelGGL((k_ml_mult<V_>), g, b, 0, st, S, K, NC, Ny, Sp, tr, (const MlBatch*)nullptr); return BM_OK;
 	C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8)
 #undef C_
 	}
@@ -420,6 +422,24 @@ static int ml_convolute(bm_mlayer_s* m, const float* M, float* out, int mode, hi
 
 int bm::mlayer_planar(bm_mlayer_s* m, const float* M, float* H3n, hipStream_t st) { return ml_convolute(m, M, H3n, 0, st); }
 int bm::mlayer_planar_transposed(bm_mlayer_s* m, const float* M, float* H3n, hipStream_t st) { return ml_convolute(m, M, H3n, 0, st, 1); }
+
+bm::MlView bm::mlayer_view(bm_mlayer_s* m)
+{
+	return MlView{ m->Sx, m->S, m->K, m->L, m->nx, m->ny, m->Nx, m->Ny, m->NC, m->NYH, m->Sp, m->ntl };
+}
+
+int bm::batch_ml_mult(int L, const MlBatch* h_tab, const MlBatch* d_tab, int nb, int ncu, hipStream_t st)
+{
+	int mx = 1;
+	for (int b = 0; b < nb; b++) mx = std::max(mx, h_tab[b].NC * h_tab[b].Ny);
+	dim3 g((unsigned)std::max(1, std::min((mx + 255) / 256, ncu * 8)), nb), bl(256);
+	switch (L) {
+#define C_(V_) case V_: hipLaunchKernelGGL((k_ml_mult<V_>), g, bl, 0, st, nullptr, nullptr, 0, 0, 0, 0, d_tab); return BM_OK;
+	C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8)
+#undef C_
+	}
+	return fail(BM_ERR_SIZE, "multilayer demag: at most 8 layers");
+}
 
 void bm::mlayer_dims(const bm_mlayer_s* m, int* L, int* nx, int* ny, double* hz)
 {

@@ -28,6 +28,7 @@ template <int METHOD, int STEP>
 __global__ void __launch_bounds__(kBS) k_stage(StageArgs a, const StageArgs* __restrict__ bt)
 {
 	if (bt) a = bt[blockIdx.y];     // batched: one mesh per grid row
+	apply_device_time(a);
 	const size_t n = (size_t)a.nx * a.ny * a.nz;
 	const long long nchunk = (long long)((n + kBS - 1) / kBS);
 	float lte_loc = 0.f;

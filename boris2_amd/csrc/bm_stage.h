@@ -85,7 +85,27 @@ struct StageArgs {
 	// exchange between them) with their own A, alpha and K1 (when layered != 0)
 	int layered;
 	float lay_A[kMaxStackLayers], lay_alpha[kMaxStackLayers], lay_K1[kMaxStackLayers];
+	// time-dependent applied field read on the device (batched graphs): with tnow set, the
+	// stage kernel adds amp dir sinc(w (t - t0)) (wave 1) or slope t (wave 2) to Ha, t = *tnow
+	const double* tnow;
+	int wave;
+	float amp, w, t0, dir[3], slope[3];
 };
+
+// Ha of a stage argument set at the device-held step time (see StageArgs::tnow).
+__device__ __forceinline__ void apply_device_time(StageArgs& a)
+{
+	if (!a.tnow) return;
+	const double t = *a.tnow;
+	if (a.wave == 1) {
+		const double x = (double)a.w * (t - (double)a.t0);
+		const float sc = (fabs(x) < 1e-300) ? 1.f : (float)(sin(x) / x);
+		for (int c = 0; c < 3; c++) a.Ha[c] += a.amp * a.dir[c] * sc;
+	}
+	else if (a.wave == 2) {
+		for (int c = 0; c < 3; c++) a.Ha[c] += a.slope[c] * (float)t;
+	}
+}
 
 // Material constants of a cell: the mesh's, or those of its layer in a layer stack (the
 // layer value is picked with compile-time indices so StageArgs stays in registers).

@@ -182,13 +182,24 @@ int bm_ode_average_m(bm_ode_t o, double* out3, void* stream);
  * has n3[3b..3b+2] cells of h3[3b..3b+2] (m) and its own parameters params[b]; M is the
  * caller's device array holding every mesh's magnetisation concatenated in mesh order
  * (bm_batch_mesh_offset gives each mesh's cell offset).  Methods: BM_EVAL_RK4,
- * BM_EVAL_TEULER (fixed step; constant applied fields).  Every pass of a field evaluation
- * is one launch per distinct pass size over all meshes of that size; a whole step is
- * captured once into a hipGraph and replayed (BM_BATCH_NO_GRAPH: launch directly). */
+ * BM_EVAL_TEULER (fixed step; applied fields constant, sinc pulse or ramp, evaluated at each
+ * step's start time on the device).  Every pass of a field evaluation is one launch per
+ * distinct pass size over all meshes of that size; a whole step is captured once into a
+ * hipGraph and replayed (BM_BATCH_NO_GRAPH: launch directly).
+ * bm_batch_create_layers: layers[b].nlayers > 0 makes mesh b a layer stack as
+ * bm_ode_set_layers does (nz == nlayers <= 8, demag on; hz, z0 its layers' thicknesses and
+ * bottom heights, Ms/A/alpha/K1 per layer or NULL); layers NULL = bm_batch_create. */
 typedef struct bm_batch_s* bm_batch_t;
 enum { BM_BATCH_NO_GRAPH = 1 };
 int bm_batch_create(int method, int nmesh, const int* n3, const double* h3, const bm_fm_params* params,
 	float* M, int flags, bm_batch_t* out);
+typedef struct {
+	int nlayers;                                    /* 0: a plain mesh */
+	const double* hz; const double* z0;             /* nlayers values each (m) */
+	const double* Ms; const double* A; const double* alpha; const double* K1;   /* per layer or NULL */
+} bm_layer_desc;
+int bm_batch_create_layers(int method, int nmesh, const int* n3, const double* h3, const bm_fm_params* params,
+	const bm_layer_desc* layers, float* M, int flags, bm_batch_t* out);
 int bm_batch_destroy(bm_batch_t b);
 int bm_batch_set_dt(bm_batch_t b, double dT);
 /* nsteps fixed steps of every mesh, ordered after earlier work on `stream`. */
