@@ -16,7 +16,11 @@ The JSON line also carries:
                  its own stream around every launch in the timed region), against 8 TB/s;
   cpu_baseline — the oracle (this repository's CPU restatement of the reference path, FP64,
                  OpenMP over all host threads) timed on a bounded sample of the same
-                 workload on this machine's host cores.
+                 workload on this machine's host cores;
+  batched      — 64 films of uneven sizes, batched RK4 vs the same films one by one;
+  adjoint      — 128x128x4 RK4 rollout with its discrete adjoint: backward / forward cost;
+  multilayer   — 16 stacks of 3 layers of 256x256 cells, batched RK4 vs one by one, and one
+                 stack's adjoint (per-layer Ms and field-ramp slope gradients).
 """
 from __future__ import annotations
 
@@ -221,6 +225,90 @@ def adjoint_run(steps: int, ckpt: int):
             "forward_cell_updates_per_s": ncell * steps / tf}
 
 
+def multilayer_instances(seed: int = 5, count: int = 16):
+    """16 stacks of 3 layers of 256x256 cells of 4 nm: 2/1/4-cell layers (8/4/16 nm) with
+    1-cell spacers, per-layer Ms uniform in [6e5, 1.2e6] A/m, A 1.3e-11, K1 3e5 J/m^3 along z
+    in the thin layer, alpha 0.02, a seeded field ramp per stack, M0 random on the sphere."""
+    from boris2_amd import api
+    rng = np.random.default_rng(seed)
+    hz, z0 = [8e-9, 4e-9, 16e-9], [0.0, 12e-9, 20e-9]
+    out = []
+    for b in range(count):
+        lay = dict(hz=hz, z0=z0, Ms=[float(v) for v in rng.uniform(6e5, 1.2e6, 3)], A=[1.3e-11] * 3,
+                   alpha=[0.02] * 3, K1=[0.0, 3e5, 0.0])
+        p = api.FMParams(Ms=8e5, A=1.3e-11, alpha=0.02, ea=(0.0, 0.0, 1.0),
+                         Ha=tuple(float(v) for v in rng.normal(0.0, 8e3, 3)), wave=2,
+                         slope=tuple(float(v) for v in rng.normal(0.0, 2e13, 3)),
+                         modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ANIS_UNI | api.BM_MOD_ZEEMAN)
+        ms = np.repeat(lay["Ms"], 256 * 256)
+        M0 = (api.random_magnetisation(ms.size, 1.0, 2000 + b) * ms[:, None]).astype(np.float32)
+        out.append(((256, 256, 3), (4e-9, 4e-9, 16e-9), p, lay, M0))
+    return out
+
+
+def multilayer_run(steps: int, adj_steps: int, ckpt: int = 100):
+    """The 16 layer stacks: batched RK4 (dt 100 fs) vs the same stacks one by one on the
+    single-stack path over `steps` steps; then the adjoint of one stack's `adj_steps`-step
+    rollout (per-layer Ms and ramp-slope gradients), forward and backward wall times."""
+    import torch
+    from boris2_amd import api
+    items = multilayer_instances()
+    dt = 100e-15
+    cells = sum(n[0] * n[1] * n[2] for n, *_ in items)
+    bs = api.BatchSolver(api.BM_EVAL_RK4, [(n, h, p) for n, h, p, _, _ in items], [M0 for *_, M0 in items],
+                         layers=[lay for *_, lay, _ in items])
+    bs.set_dt(dt)
+    bs.run(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bs.run(steps)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    bs.close()
+    solvers = []
+    for n, h, p, lay, M0 in items:
+        s = api.LayerStackSolver(api.BM_EVAL_RK4, n[0], n[1], h[0], h[1], lay["hz"], lay["z0"], p, Ms=lay["Ms"],
+                                 A=lay["A"], alpha=lay["alpha"], K1=lay["K1"], M0=M0)
+        s.set_dt(dt)
+        s.run_to_iteration(1)
+        solvers.append(s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.run_to_iteration(1 + steps)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    for s in solvers:
+        s.close()
+    out = {"stacks": len(items), "layers": 3, "cells": cells, "steps": steps, "method": "RK4 fixed dt 100 fs",
+           "batched_cell_updates_per_s": cells * steps / tb, "plain_cell_updates_per_s": cells * steps / tp,
+           "batched_over_plain": tp / tb, "batched_ms_per_step": 1e3 * tb / steps, "plain_ms_per_step": 1e3 * tp / steps}
+    if adj_steps > 0:
+        n, h, p, lay, M0 = items[0]
+        rng = np.random.default_rng(8)
+        mt = rng.normal(size=M0.shape); mt /= np.linalg.norm(mt, axis=1, keepdims=True)
+        m0 = M0 / np.linalg.norm(M0, axis=1, keepdims=True)
+        Ms = torch.tensor(np.repeat(lay["Ms"], n[0] * n[1]), dtype=torch.float32)
+        r = api.RK4Adjoint(n, h, p, Ms, dt, adj_steps, ckpt, layers=lay)
+        m0t, mtt = torch.tensor(m0, dtype=torch.float32), torch.tensor(mt, dtype=torch.float32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss, _ = r.forward(m0t, mtt)
+        torch.cuda.synchronize()
+        tf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        g, _ = r.backward()
+        torch.cuda.synchronize()
+        tbk = time.perf_counter() - t0
+        gz = r.zeeman_grads()
+        gl = g.double().cpu().reshape(3, -1).sum(1).tolist()
+        r.close()
+        out["adjoint"] = {"stack": 0, "steps": adj_steps, "checkpoint_every": ckpt, "forward_s": round(tf, 3),
+                          "backward_s": round(tbk, 3), "adjoint_over_forward": round(tbk / tf, 3), "loss": loss,
+                          "dL_dMs_layer": gl, "dL_dslope": gz[1:].tolist()}
+    return out
+
+
 def cpu_baseline(cpu_steps: int):
     """Oracle FP64 on the host: kernel setup (reported separately) then `cpu_steps` steps."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -251,6 +339,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--batch-steps", type=int, default=50, help="steps of the batched vs one-by-one comparison (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=4000, help="RK4 steps of the adjoint rollout (0: skip)")
+    ap.add_argument("--ml-steps", type=int, default=50, help="steps of the batched layer-stack comparison (0: skip)")
+    ap.add_argument("--ml-adjoint-steps", type=int, default=1000, help="RK4 steps of the layer-stack adjoint (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy 5 times (scales the HBM byte counters)")
     a = ap.parse_args()
@@ -279,6 +369,7 @@ def main() -> int:
             "alg_bytes_per_launch": round(d["alg_GB_per_launch"] * 1e9), "avg_launch_ms": d["avg_ms"]}
     batched = batch_run(a.batch_steps) if a.batch_steps > 0 else None
     adjoint = adjoint_run(a.adjoint_steps, 100) if a.adjoint_steps > 0 else None
+    multilayer = multilayer_run(a.ml_steps, a.ml_adjoint_steps) if a.ml_steps > 0 else None
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_steps)
     value = r["ncell"] * r["accepted"] / r["wall"]
     line = {
@@ -294,6 +385,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
+        "multilayer": multilayer,
         "kernels": kinds,
     }
     print(json.dumps(line), flush=True)

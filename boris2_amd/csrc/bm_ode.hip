@@ -24,11 +24,13 @@ namespace {
 
 using namespace bm;
 
+// One stage over a mesh (all the block's cells); `a` is the kernel argument or, batched, the
+// mesh's table entry read in place (a local copy of the large argument set would go to
+// scratch).
 template <int METHOD, int STEP>
-__global__ void __launch_bounds__(kBS) k_stage(StageArgs a, const StageArgs* __restrict__ bt)
+__device__ __forceinline__ void stage_sweep(const StageArgs& a)
 {
-	if (bt) a = bt[blockIdx.y];     // batched: one mesh per grid row
-	apply_device_time(a);
+	const F3 Ha = stage_Ha(a);
 	const size_t n = (size_t)a.nx * a.ny * a.nz;
 	const long long nchunk = (long long)((n + kBS - 1) / kBS);
 	float lte_loc = 0.f;
@@ -44,9 +46,16 @@ __global__ void __launch_bounds__(kBS) k_stage(StageArgs a, const StageArgs* __r
 		if (idx >= n) continue;
 		const int i = (int)(idx % a.nx), j = (int)((idx / a.nx) % a.ny), k = (int)(idx / ((size_t)a.nx * a.ny));
 		const F3 Hd = (a.modules & BM_MOD_DEMAG) ? F3{ a.Hd[idx], a.Hd[n + idx], a.Hd[2 * n + idx] } : F3{ 0.f, 0.f, 0.f };
-		stage_cell<METHOD, STEP>(a, idx, i, j, k, Hd, lte_loc);
+		stage_cell<METHOD, STEP>(a, Ha, idx, i, j, k, Hd, lte_loc);
 	}
 	if (stage_has_lte<METHOD, STEP>()) block_max_to(lte_loc, a.lte);
+}
+
+template <int METHOD, int STEP>
+__global__ void __launch_bounds__(kBS) k_stage(StageArgs a, const StageArgs* __restrict__ bt)
+{
+	if (bt) stage_sweep<METHOD, STEP>(bt[blockIdx.y]);     // batched: one mesh per grid row
+	else stage_sweep<METHOD, STEP>(a);
 }
 
 // RKDP54 advance after the step size is fixed: sM1 = M; M1 = M + E0 dT/5 (into the

@@ -92,19 +92,20 @@ struct StageArgs {
 	float amp, w, t0, dir[3], slope[3];
 };
 
-// Ha of a stage argument set at the device-held step time (see StageArgs::tnow).
-__device__ __forceinline__ void apply_device_time(StageArgs& a)
+// Applied field of a stage: Ha, plus the time-dependent term at the device-held step time
+// when tnow is set (see StageArgs::tnow).
+__device__ __forceinline__ F3 stage_Ha(const StageArgs& a)
 {
-	if (!a.tnow) return;
+	F3 H{ a.Ha[0], a.Ha[1], a.Ha[2] };
+	if (!a.tnow) return H;
 	const double t = *a.tnow;
 	if (a.wave == 1) {
 		const double x = (double)a.w * (t - (double)a.t0);
 		const float sc = (fabs(x) < 1e-300) ? 1.f : (float)(sin(x) / x);
-		for (int c = 0; c < 3; c++) a.Ha[c] += a.amp * a.dir[c] * sc;
+		H = H + (a.amp * sc) * F3{ a.dir[0], a.dir[1], a.dir[2] };
 	}
-	else if (a.wave == 2) {
-		for (int c = 0; c < 3; c++) a.Ha[c] += a.slope[c] * (float)t;
-	}
+	else if (a.wave == 2) H = H + (float)t * F3{ a.slope[0], a.slope[1], a.slope[2] };
+	return H;
 }
 
 // Material constants of a cell: the mesh's, or those of its layer in a layer stack (the
@@ -140,9 +141,9 @@ __device__ __forceinline__ void block_max_to(float v, unsigned* dst)
 }
 
 // Effective field of one cell; lap = sum over axes of the second differences / h^2.
-__device__ __forceinline__ F3 eff_field(const StageArgs& a, const CellMat& cm, F3 m, F3 Hd, F3 lap, float Ms)
+__device__ __forceinline__ F3 eff_field(const StageArgs& a, const CellMat& cm, F3 Ha, F3 m, F3 Hd, F3 lap, float Ms)
 {
-	F3 H = (a.modules & BM_MOD_ZEEMAN) ? F3{ a.Ha[0], a.Ha[1], a.Ha[2] } : F3{ 0.f, 0.f, 0.f };
+	F3 H = (a.modules & BM_MOD_ZEEMAN) ? Ha : F3{ 0.f, 0.f, 0.f };
 	if (a.modules & BM_MOD_DEMAG) H = H + Hd;
 	if (a.modules & BM_MOD_EXCHANGE) H = H + ((2.f * cm.A) / ((float)kMU0 * Ms * Ms)) * lap;
 	if (a.modules & BM_MOD_ANIS_UNI) {
@@ -232,7 +233,7 @@ __device__ __forceinline__ V rk_update(const StageArgs& a, size_t idx, V m, V rh
 
 // One cell of the stand-alone stage kernel (any mesh size).
 template <int METHOD, int STEP>
-__device__ __forceinline__ void stage_cell(const StageArgs& a, size_t idx, int i, int j, int k, F3 Hd, float& lte_loc)
+__device__ __forceinline__ void stage_cell(const StageArgs& a, F3 Ha, size_t idx, int i, int j, int k, F3 Hd, float& lte_loc)
 {
 	const F3 m = ld3(a.Min, idx);
 	const float Ms = a.Ms_map ? a.Ms_map[idx] : a.Ms;
@@ -258,7 +259,7 @@ __device__ __forceinline__ void stage_cell(const StageArgs& a, size_t idx, int i
 		lap = a.ihx2 * lx + a.ihy2 * ly + a.ihz2 * lz;
 	}
 	const CellMat cm = cell_mat(a, k);
-	const F3 H = eff_field(a, cm, m, Hd, lap, Ms);
+	const F3 H = eff_field(a, cm, Ha, m, Hd, lap, Ms);
 	if (a.Heff_out) st3(a.Heff_out, idx, H);
 	(void)rk_update<METHOD, STEP>(a, idx, m, llg_rhs(a, cm.alpha, m, H, Ms), Ms, lte_loc);
 }
@@ -294,7 +295,8 @@ __device__ __forceinline__ P6 stage_pair(const StageArgs& a, size_t idx, int i0,
 	}
 	// the row kernel serves single meshes only (never a layer stack)
 	const CellMat cm{ a.A, a.alpha, a.K1 };
-	const F3 H0 = eff_field(a, cm, m.a, Hd.a, lap0, Ms.x), H1 = eff_field(a, cm, m.b, Hd.b, lap1, Ms.y);
+	const F3 Ha{ a.Ha[0], a.Ha[1], a.Ha[2] };
+	const F3 H0 = eff_field(a, cm, Ha, m.a, Hd.a, lap0, Ms.x), H1 = eff_field(a, cm, Ha, m.b, Hd.b, lap1, Ms.y);
 	if (a.Heff_out) st6(a.Heff_out, idx, P6{ H0, H1 });
 	const P6 rhs{ llg_rhs(a, cm.alpha, m.a, H0, Ms.x), llg_rhs(a, cm.alpha, m.b, H1, Ms.y) };
 	return rk_update<METHOD, STEP>(a, idx, m, rhs, Ms, lte_loc);
