@@ -313,7 +313,7 @@ class BatchSolver:
     layers: None, or per mesh None or dict(hz=..., z0=..., Ms=None, A=None, alpha=None,
     K1=None) making that mesh's n[2] planes a layer stack (as LayerStackSolver)."""
 
-    def __init__(self, method: int, meshes, M0=None, use_graph: bool = True, layers=None):
+    def __init__(self, method: int, meshes, M0=None, use_graph: bool = True, layers=None, autotune: bool = False):
         require_device()
         self.L = _lib.load()
         self.meshes = [(tuple(int(v) for v in n), tuple(float(v) for v in h), p) for n, h, p in meshes]
@@ -349,10 +349,17 @@ class BatchSolver:
                     setattr(ld[b], k, a.ctypes.data_as(C.c_void_p))
         check(self.L.bm_batch_create_layers(method, nb, n3.ctypes.data_as(C.c_void_p), h3.ctypes.data_as(C.c_void_p),
                                             C.cast(self._pc, C.c_void_p), C.cast(ld, C.c_void_p) if ld is not None else None,
-                                            _ptr(self.M), 0 if use_graph else 1, C.byref(self.handle)))
+                                            _ptr(self.M), (0 if use_graph else 1) | (2 if autotune else 0),
+                                            C.byref(self.handle)))
 
     def mesh(self, b: int) -> torch.Tensor:
         return self.M[int(self.offsets[b]):int(self.offsets[b + 1])]
+
+    def autotune_report(self) -> list[str]:
+        """The autotuner's choices (after the first run with autotune=True), one line per pass."""
+        buf = C.create_string_buffer(16384)
+        check(self.L.bm_batch_autotune_report(self.handle, buf, len(buf)))
+        return [l for l in buf.value.decode().splitlines() if l]
 
     def set_dt(self, dT: float) -> None:
         check(self.L.bm_batch_set_dt(self.handle, float(dT)))

@@ -24,6 +24,8 @@
 #include <algorithm>
 #include <cstring>
 #include <vector>
+#include <string>
+#include <cstdio>
 
 using namespace bm;
 
@@ -34,7 +36,14 @@ template <typename T> struct Tab {
 	T* d = nullptr;
 };
 
-struct Group { int size; std::vector<int> meshes; };
+// A pass-size group: its meshes and the launch shape of its pass(es) (x groups: [0] forward,
+// [1] inverse; y groups: [0] forward, [1] inverse; others [0]), with the autotuned time.
+struct Group {
+	int size;
+	std::vector<int> meshes;
+	LaunchShape sh[2];
+	double us[2] = { 0, 0 };
+};
 
 } // namespace
 
@@ -72,6 +81,7 @@ struct bm_batch_s {
 	hipEvent_t ev_in = nullptr, ev_out = nullptr;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
+	bool autotune = false, tuned = false;
 };
 
 namespace {
@@ -142,9 +152,8 @@ StageArgs stage_args(bm_batch_s* o, int b, int parity)
 
 __global__ void k_time_advance(double* t, const double* dT) { *t += *dT; }
 
-// Group the meshes by pass size, build every table on the host and upload them in one
-// allocation.
-int build_tables(bm_batch_s* o)
+// Group the meshes by pass size.
+void build_groups(bm_batch_s* o)
 {
 	const int nb = o->nb;
 	for (int b = 0; b < nb; b++) {
@@ -160,6 +169,14 @@ int build_tables(bm_batch_s* o)
 		if (o->nz[b] > 1) { add_to(o->gy, o->dm[b]->Ny, b); add_to(o->gz, o->dm[b]->Nz, b); }
 		else add_to(o->gy2, o->dm[b]->Ny, b);
 	}
+}
+
+// Build every table on the host (x tables depend on the groups' tile factors) and upload
+// them in one allocation, replacing earlier tables.
+int build_tables(bm_batch_s* o)
+{
+	const int nb = o->nb;
+	if (o->dev_tables) { (void)hipFree(o->dev_tables); o->dev_tables = nullptr; }
 	for (int par = 0; par < 2; par++) o->fwd[par].assign(o->gx.size(), Tab<FwdArgs>());
 	o->inv.assign(o->gx.size(), Tab<PassBatch>());
 	o->ty.assign(o->gy.size(), Tab<PassBatch>());
@@ -168,7 +185,7 @@ int build_tables(bm_batch_s* o)
 	o->tml.assign(o->gml.size(), Tab<MlBatch>());
 	for (size_t g = 0; g < o->gx.size(); g++) {
 		const int nx = o->gx[g].size;
-		const int tf = x_tile_rows(nx, true), ti = x_tile_rows(nx, false);
+		const int tf = x_tile_rows(nx, true, o->gx[g].sh[0].f), ti = x_tile_rows(nx, false, o->gx[g].sh[1].f);
 		for (int b : o->gx[g].meshes) {
 			// a layer stack runs the x passes over its L layer planes
 			float2* Sx; int ny, nz, ntl; double scale;
@@ -264,20 +281,85 @@ int eval_all(bm_batch_s* o, int step, int par, hipStream_t st)
 {
 	int rc = BM_OK;
 	for (size_t g = 0; g < o->gx.size() && !rc; g++)
-		rc = batch_fwd_x(o->gx[g].size, o->fwd[par][g].h.data(), o->fwd[par][g].d, (int)o->fwd[par][g].h.size(), o->ncu, st);
+		rc = batch_fwd_x(o->gx[g].size, o->gx[g].sh[0], o->fwd[par][g].h.data(), o->fwd[par][g].d, (int)o->fwd[par][g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gy.size() && !rc; g++)
-		rc = batch_y(o->gy[g].size, -1, o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st);
+		rc = batch_y(o->gy[g].size, -1, o->gy[g].sh[0], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gz.size() && !rc; g++)
-		rc = batch_z(o->gz[g].size, o->tz[g].h.data(), o->tz[g].d, (int)o->tz[g].h.size(), o->ncu, st);
+		rc = batch_z(o->gz[g].size, o->gz[g].sh[0], o->tz[g].h.data(), o->tz[g].d, (int)o->tz[g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gml.size() && !rc; g++)
-		rc = batch_ml_mult(o->gml[g].size, o->tml[g].h.data(), o->tml[g].d, (int)o->tml[g].h.size(), o->ncu, st);
+		rc = batch_ml_mult(o->gml[g].size, o->gml[g].sh[0], o->tml[g].h.data(), o->tml[g].d, (int)o->tml[g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gy.size() && !rc; g++)
-		rc = batch_y(o->gy[g].size, 1, o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st);
+		rc = batch_y(o->gy[g].size, 1, o->gy[g].sh[1], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gy2.size() && !rc; g++)
-		rc = batch_y2d(o->gy2[g].size, o->ty2[g].h.data(), o->ty2[g].d, (int)o->ty2[g].h.size(), o->ncu, st);
+		rc = batch_y2d(o->gy2[g].size, o->gy2[g].sh[0], o->ty2[g].h.data(), o->ty2[g].d, (int)o->ty2[g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gx.size() && !rc; g++)
-		rc = batch_inv_x_planar(o->gx[g].size, o->inv[g].h.data(), o->inv[g].d, (int)o->inv[g].h.size(), o->ncu, st);
+		rc = batch_inv_x_planar(o->gx[g].size, o->gx[g].sh[1], o->inv[g].h.data(), o->inv[g].d, (int)o->inv[g].h.size(), o->ncu, st);
 	if (!rc) rc = batch_stage(o->mcode, step, o->stage[par].d, o->nb, o->max_cells, o->ncu, st);
+	return rc;
+}
+
+// Autotuner: for every pass-size group and pass, time each candidate launch shape (five
+// launches after a warm-up, HIP events) on the group's real tables and keep the fastest.
+// Candidates: x passes tile factor {1, 2} x workgroups per CU {1, 2, 4}; y passes {4, 8, 16};
+// z pass {1, 2, 4}; 2D y pass {4, 8, 16}; multilayer product {4, 8, 16}.  Only the FFT passes
+// run (their inputs are recomputed from M by every evaluation), so the state is untouched.
+int autotune(bm_batch_s* o, hipStream_t st)
+{
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	BM_HIP(hipEventCreate(&e0));
+	BM_HIP(hipEventCreate(&e1));
+	int rc = BM_OK;
+	auto timed = [&](auto launch) -> double {
+		if (launch() != BM_OK) return -1;
+		(void)hipEventRecord(e0, st);
+		for (int r = 0; r < 5; r++) if (launch() != BM_OK) return -1;
+		(void)hipEventRecord(e1, st);
+		if (hipEventSynchronize(e1) != hipSuccess) return -1;
+		float ms = 0;
+		(void)hipEventElapsedTime(&ms, e0, e1);
+		return 1e3 * ms / 5;
+	};
+	auto pick = [&](Group& G, int slot, const std::vector<LaunchShape>& cands, bool retable, auto launch) {
+		double bt = 1e30;
+		LaunchShape best = cands[0];
+		for (const LaunchShape& c : cands) {
+			G.sh[slot] = c;
+			if (retable && (rc = build_tables(o))) return;
+			const double t = timed(launch);
+			if (t >= 0 && t < bt) { bt = t; best = c; }
+		}
+		G.sh[slot] = best; G.us[slot] = bt;
+	};
+	std::vector<LaunchShape> cx, cy, cz;
+	for (int f : { 1, 2 }) for (int pc : { 1, 2, 4 }) cx.push_back(LaunchShape{ f, pc });
+	for (int pc : { 4, 8, 16 }) cy.push_back(LaunchShape{ 0, pc });
+	for (int pc : { 1, 2, 4 }) cz.push_back(LaunchShape{ 0, pc });
+	for (size_t g = 0; g < o->gx.size() && !rc; g++) {
+		Group& G = o->gx[g];
+		pick(G, 0, cx, true, [&] { return batch_fwd_x(G.size, G.sh[0], o->fwd[0][g].h.data(), o->fwd[0][g].d, (int)o->fwd[0][g].h.size(), o->ncu, st); });
+		if (!rc) pick(G, 1, cx, true, [&] { return batch_inv_x_planar(G.size, G.sh[1], o->inv[g].h.data(), o->inv[g].d, (int)o->inv[g].h.size(), o->ncu, st); });
+	}
+	for (size_t g = 0; g < o->gy.size() && !rc; g++) {
+		Group& G = o->gy[g];
+		for (int slot = 0; slot < 2; slot++)
+			pick(G, slot, cy, false, [&] { return batch_y(G.size, slot ? 1 : -1, G.sh[slot], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st); });
+	}
+	for (size_t g = 0; g < o->gz.size() && !rc; g++) {
+		Group& G = o->gz[g];
+		pick(G, 0, cz, false, [&] { return batch_z(G.size, G.sh[0], o->tz[g].h.data(), o->tz[g].d, (int)o->tz[g].h.size(), o->ncu, st); });
+	}
+	for (size_t g = 0; g < o->gy2.size() && !rc; g++) {
+		Group& G = o->gy2[g];
+		pick(G, 0, cy, false, [&] { return batch_y2d(G.size, G.sh[0], o->ty2[g].h.data(), o->ty2[g].d, (int)o->ty2[g].h.size(), o->ncu, st); });
+	}
+	for (size_t g = 0; g < o->gml.size() && !rc; g++) {
+		Group& G = o->gml[g];
+		pick(G, 0, cy, false, [&] { return batch_ml_mult(G.size, G.sh[0], o->tml[g].h.data(), o->tml[g].d, (int)o->tml[g].h.size(), o->ncu, st); });
+	}
+	if (!rc) rc = build_tables(o);
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (!rc) o->tuned = true;
 	return rc;
 }
 
@@ -307,6 +389,7 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 	bm_batch_s* o = new bm_batch_s();
 	o->nb = nmesh;
 	o->use_graph = (flags & BM_BATCH_NO_GRAPH) ? 0 : 1;
+	o->autotune = (flags & BM_BATCH_AUTOTUNE) != 0;
 	switch (method) {
 	case BM_EVAL_RK4: o->mcode = M_RK4; o->nstages = 4; o->dT = 0.5e-12; break;
 	case BM_EVAL_TEULER: o->mcode = M_TEULER; o->nstages = 2; o->dT = 0.5e-13; break;
@@ -368,7 +451,7 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 		if (ok) ok = hipMemset(o->Hd, 0, b3) == hipSuccess;
 		if (!ok) rc = fail(BM_ERR_NOMEM, "bm_batch_create: device arrays");
 	}
-	if (rc == BM_OK) rc = build_tables(o);
+	if (rc == BM_OK) { build_groups(o); rc = build_tables(o); }
 	if (rc == BM_OK && o->use_graph) {
 		if (hipStreamCreate(&o->gs) != hipSuccess || hipEventCreateWithFlags(&o->ev_in, hipEventDisableTiming) != hipSuccess
 			|| hipEventCreateWithFlags(&o->ev_out, hipEventDisableTiming) != hipSuccess) rc = fail(BM_ERR_HIP, "bm_batch_create: stream");
@@ -399,6 +482,11 @@ extern "C" int bm_batch_run(bm_batch_t o, long long nsteps, void* stream)
 	if (!o || nsteps < 0) return fail(BM_ERR_ARG, "bm_batch_run: bad arguments");
 	hipStream_t st = (hipStream_t)stream;
 	int rc;
+	if (o->autotune && !o->tuned && nsteps > 0) {
+		if (o->exec) { (void)hipGraphExecDestroy(o->exec); o->exec = nullptr; }
+		if (o->graph) { (void)hipGraphDestroy(o->graph); o->graph = nullptr; }
+		if ((rc = autotune(o, st))) return rc;
+	}
 	if (!o->use_graph) {
 		for (long long k = 0; k < nsteps; k++) {
 			if ((rc = one_step(o, st))) return rc;
@@ -442,5 +530,26 @@ extern "C" int bm_batch_mesh_offset(bm_batch_t o, int mesh, long long* cell_offs
 	if (!o || mesh < 0 || mesh >= o->nb) return fail(BM_ERR_ARG, "bm_batch_mesh_offset: bad mesh");
 	if (cell_offset) *cell_offset = (long long)o->off[mesh];
 	if (ncells) *ncells = (long long)o->ncell[mesh];
+	return BM_OK;
+}
+
+// One line per tuned pass: "<pass> <size> f=<factor> per_cu=<n> <microseconds> us".
+extern "C" int bm_batch_autotune_report(bm_batch_t o, char* out, int cap)
+{
+	if (!o || !out || cap < 1) return fail(BM_ERR_ARG, "bm_batch_autotune_report: bad arguments");
+	std::string r;
+	char line[160];
+	auto add = [&](const char* pass, const Group& G, int slot) {
+		snprintf(line, sizeof(line), "%s %d f=%d per_cu=%d %.1f us\n", pass, G.size, G.sh[slot].f, G.sh[slot].per_cu, G.us[slot]);
+		r += line;
+	};
+	if (o->tuned) {
+		for (const Group& G : o->gx) { add("x_fwd", G, 0); add("x_inv", G, 1); }
+		for (const Group& G : o->gy) { add("y_fwd", G, 0); add("y_inv", G, 1); }
+		for (const Group& G : o->gz) add("z", G, 0);
+		for (const Group& G : o->gy2) add("y2d", G, 0);
+		for (const Group& G : o->gml) add("ml_mult", G, 0);
+	}
+	snprintf(out, (size_t)cap, "%s", r.c_str());
 	return BM_OK;
 }

@@ -927,22 +927,22 @@ bool bm::plan_2d_ok(int nx, int Ny) { return x_supported(nx) && x_supported(Ny) 
 // ---------------------------------------------------------------------------------------
 // Batched pass launchers: nb meshes of one pass size; h_tab (host) sizes the grid, d_tab
 // (device, same contents) is read by the kernels.
-int bm::x_tile_rows(int nx, bool fwd)
+int bm::x_tile_rows(int nx, bool fwd, int f)
 {
 	switch (nx) {
-#define C_(V_) case V_: return XTile<V_>::v * xtile_factor(fwd);
+#define C_(V_) case V_: return XTile<V_>::v * (f ? f : xtile_factor(fwd));
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
 	return 0;
 }
 
-int bm::batch_fwd_x(int nx, const FwdArgs* h_tab, const FwdArgs* d_tab, int nb, int ncu, hipStream_t st)
+int bm::batch_fwd_x(int nx, LaunchShape sh, const FwdArgs* h_tab, const FwdArgs* d_tab, int nb, int ncu, hipStream_t st)
 {
 	int mx = 1;
 	for (int b = 0; b < nb; b++) mx = std::max(mx, h_tab[b].nz * h_tab[b].ntiles);
-	const int f = xtile_factor(true);
-	const dim3 g(grid_for(mx, ncu, 2), nb);
+	const int f = sh.f ? sh.f : xtile_factor(true);
+	const dim3 g(grid_for(mx, ncu, sh.per_cu ? sh.per_cu : 2), nb);
 	FwdArgs a{};
 	switch (nx) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; \
@@ -954,12 +954,12 @@ int bm::batch_fwd_x(int nx, const FwdArgs* h_tab, const FwdArgs* d_tab, int nb, 
 	return fail(BM_ERR_SIZE, "no x plan");
 }
 
-int bm::batch_inv_x_planar(int nx, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+int bm::batch_inv_x_planar(int nx, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
 	int mx = 1;
 	for (int b = 0; b < nb; b++) mx = std::max(mx, 3 * h_tab[b].nz * h_tab[b].ntiles);
-	const int f = xtile_factor(false);
-	const dim3 g(grid_for(mx, ncu, 2), nb);
+	const int f = sh.f ? sh.f : xtile_factor(false);
+	const dim3 g(grid_for(mx, ncu, sh.per_cu ? sh.per_cu : 2), nb);
 	switch (nx) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; \
 		if (f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, d_tab); \
@@ -970,36 +970,39 @@ int bm::batch_inv_x_planar(int nx, const PassBatch* h_tab, const PassBatch* d_ta
 	return fail(BM_ERR_SIZE, "no x plan");
 }
 
-int bm::batch_y(int Ny, int dir, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+int bm::batch_y(int Ny, int dir, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
+	const int pc = sh.per_cu ? sh.per_cu : 8;
 	switch (Ny) {
 #define C_(V_) case V_: { const int LP = kBlock / plan_T<V_>(); int mx = 1; \
 		for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].nlines + LP - 1) / LP); \
-		launch_y_t<V_>(dir, dim3(grid8(mx, ncu, 8), nb), st, nullptr, nullptr, 0, 0, 0, 0, d_tab); return BM_OK; }
+		launch_y_t<V_>(dir, dim3(grid8(mx, ncu, pc), nb), st, nullptr, nullptr, 0, 0, 0, 0, d_tab); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no y plan");
 }
 
-int bm::batch_z(int Nz, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+int bm::batch_z(int Nz, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
+	const int pc = sh.per_cu ? sh.per_cu : 2;
 	switch (Nz) {
 #define C_(V_) case V_: { const int CH = kBlock / plan_T<V_>() / 2; int mx = 1; \
 		for (int b = 0; b < nb; b++) mx = std::max(mx, h_tab[b].NC * ((h_tab[b].Ny / 2 + CH - 1) / CH)); \
-		hipLaunchKernelGGL(k_z<V_>, dim3(grid8(mx, ncu, 2), nb), dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, z_xcd(), d_tab); return BM_OK; }
+		hipLaunchKernelGGL(k_z<V_>, dim3(grid8(mx, ncu, pc), nb), dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, z_xcd(), d_tab); return BM_OK; }
 	BM_FOR_Z_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no z plan");
 }
 
-int bm::batch_y2d(int Ny, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+int bm::batch_y2d(int Ny, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
+	const int pc = sh.per_cu ? sh.per_cu : 8;
 	switch (Ny) {
 #define C_(V_) case V_: { constexpr int BS = Plan<V_>::E > 8 ? 128 : 256; const int LP = BS / plan_T<V_>(); int mx = 1; \
 		for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].NC + LP - 1) / LP); \
-		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid8(mx, ncu, 8), nb), dim3(BS), 0, st, nullptr, nullptr, 0, 0, 0, d_tab); return BM_OK; }
+		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid8(mx, ncu, pc), nb), dim3(BS), 0, st, nullptr, nullptr, 0, 0, 0, d_tab); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}

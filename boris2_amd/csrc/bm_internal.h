@@ -89,14 +89,18 @@ struct MlView {
 MlView mlayer_view(bm_mlayer_s* m);
 // Per-stack arguments of the batched multilayer kernel product.
 struct MlBatch { float2* S; const float* K; int NC, Ny, Sp; };
-int batch_ml_mult(int L, const MlBatch* h_tab, const MlBatch* d_tab, int nb, int ncu, hipStream_t st);
+// Launch shape of a batched pass (0 = the pass's default): f the x-pass row-tile factor
+// (1 or 2 times the base tile), per_cu the grid's workgroups per CU.  Picked per pass-size
+// group by the batched integrator's autotuner.
+struct LaunchShape { int f = 0, per_cu = 0; };
+int batch_ml_mult(int L, LaunchShape sh, const MlBatch* h_tab, const MlBatch* d_tab, int nb, int ncu, hipStream_t st);
 // rows per work item of the x passes for nx (0 if there is no plan)
-int x_tile_rows(int nx, bool fwd);
+int x_tile_rows(int nx, bool fwd, int f = 0);
 // Batched passes over nb meshes sharing the pass size (tables: host copy for the grid size,
 // device copy for the kernels).
-int batch_fwd_x(int nx, const FwdArgs* h_tab, const FwdArgs* d_tab, int nb, int ncu, hipStream_t st);
-int batch_inv_x_planar(int nx, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
-int batch_y(int Ny, int dir, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
-int batch_z(int Nz, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
-int batch_y2d(int Ny, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
+int batch_fwd_x(int nx, LaunchShape sh, const FwdArgs* h_tab, const FwdArgs* d_tab, int nb, int ncu, hipStream_t st);
+int batch_inv_x_planar(int nx, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
+int batch_y(int Ny, int dir, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
+int batch_z(int Nz, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
+int batch_y2d(int Ny, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
 }

@@ -428,11 +428,11 @@ bm::MlView bm::mlayer_view(bm_mlayer_s* m)
 	return MlView{ m->Sx, m->S, m->K, m->L, m->nx, m->ny, m->Nx, m->Ny, m->NC, m->NYH, m->Sp, m->ntl };
 }
 
-int bm::batch_ml_mult(int L, const MlBatch* h_tab, const MlBatch* d_tab, int nb, int ncu, hipStream_t st)
+int bm::batch_ml_mult(int L, LaunchShape sh, const MlBatch* h_tab, const MlBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
 	int mx = 1;
 	for (int b = 0; b < nb; b++) mx = std::max(mx, h_tab[b].NC * h_tab[b].Ny);
-	dim3 g((unsigned)std::max(1, std::min((mx + 255) / 256, ncu * 8)), nb), bl(256);
+	dim3 g((unsigned)std::max(1, std::min((mx + 255) / 256, ncu * (sh.per_cu ? sh.per_cu : 8))), nb), bl(256);
 	switch (L) {
 #define C_(V_) case V_: hipLaunchKernelGGL((k_ml_mult<V_>), g, bl, 0, st, nullptr, nullptr, 0, 0, 0, 0, d_tab); return BM_OK;
 	C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8)

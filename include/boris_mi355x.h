@@ -185,12 +185,15 @@ int bm_ode_average_m(bm_ode_t o, double* out3, void* stream);
  * BM_EVAL_TEULER (fixed step; applied fields constant, sinc pulse or ramp, evaluated at each
  * step's start time on the device).  Every pass of a field evaluation is one launch per
  * distinct pass size over all meshes of that size; a whole step is captured once into a
- * hipGraph and replayed (BM_BATCH_NO_GRAPH: launch directly).
+ * hipGraph and replayed (BM_BATCH_NO_GRAPH: launch directly).  BM_BATCH_AUTOTUNE: before
+ * the first step, time candidate launch shapes of every pass-size group (x-pass row-tile
+ * factor, workgroups per CU) on the real tables and keep the fastest;
+ * bm_batch_autotune_report writes the choices and times as text lines.
  * bm_batch_create_layers: layers[b].nlayers > 0 makes mesh b a layer stack as
  * bm_ode_set_layers does (nz == nlayers <= 8, demag on; hz, z0 its layers' thicknesses and
  * bottom heights, Ms/A/alpha/K1 per layer or NULL); layers NULL = bm_batch_create. */
 typedef struct bm_batch_s* bm_batch_t;
-enum { BM_BATCH_NO_GRAPH = 1 };
+enum { BM_BATCH_NO_GRAPH = 1, BM_BATCH_AUTOTUNE = 2 };
 int bm_batch_create(int method, int nmesh, const int* n3, const double* h3, const bm_fm_params* params,
 	float* M, int flags, bm_batch_t* out);
 typedef struct {
@@ -206,6 +209,7 @@ int bm_batch_set_dt(bm_batch_t b, double dT);
 int bm_batch_run(bm_batch_t b, long long nsteps, void* stream);
 int bm_batch_get_state(bm_batch_t b, double* time, long long* iteration);
 int bm_batch_mesh_offset(bm_batch_t b, int mesh, long long* cell_offset, long long* ncells);
+int bm_batch_autotune_report(bm_batch_t b, char* out, int cap);
 
 /* ------------------------------------------------------------------------------------
  * Differentiation of the fixed-step RK4 rollout (no reference counterpart: the reference

@@ -147,3 +147,25 @@ def test_batched_layer_stack_graph_equals_direct(gpu):
         x.run(6)
     torch.cuda.synchronize()
     assert torch.equal(a.M, b.M)
+
+
+def test_autotuned_batch_equals_default(gpu):
+    """Launch shapes change only how work items are spread over workgroups, not their
+    arithmetic: the autotuned batch reproduces the default batch bit for bit."""
+    meshes, M0 = batch_meshes(9)
+    items, S0 = stack_batch(9)
+    meshes = meshes + [(n, h, p) for n, h, p, _ in items]
+    M0 = M0 + S0
+    layers = [None] * (len(meshes) - len(items)) + [lay for *_, lay in items]
+    a = api.BatchSolver(api.BM_EVAL_RK4, meshes, M0, layers=layers, autotune=True)
+    b = api.BatchSolver(api.BM_EVAL_RK4, meshes, M0, layers=layers)
+    for x in (a, b):
+        x.set_dt(50e-15)
+        x.run(4)
+    torch.cuda.synchronize()
+    assert torch.equal(a.M, b.M)
+    rep = a.autotune_report()
+    kinds = {l.split()[0] for l in rep}
+    assert {"x_fwd", "x_inv", "y_fwd", "y_inv", "z", "y2d", "ml_mult"} <= kinds, rep
+    assert all(float(l.split()[-2]) > 0 for l in rep), rep
+    assert b.autotune_report() == []
