@@ -17,6 +17,8 @@ The JSON line also carries:
   cpu_baseline — the oracle (this repository's CPU restatement of the reference path, FP64,
                  OpenMP over all host threads) timed on a bounded sample of the same
                  workload on this machine's host cores;
+  eval_speedup — the same workload with the demag evaluation speedup (saved demag fields
+                 extrapolated in time, one convolution per step);
   batched      — 64 films of uneven sizes, batched RK4 vs the same films one by one;
   adjoint      — 128x128x4 RK4 rollout with its discrete adjoint: backward / forward cost;
   multilayer   — 16 stacks of 3 layers of 256x256 cells, batched RK4 vs one by one, and one
@@ -129,9 +131,45 @@ def gpu_run(steps: int, warmup: int):
     accepted = st1["iteration"] - st0["iteration"]
     m_avg = s.average_m()
     s.close()
-    return {"n": n, "N": N, "ncell": ncell, "wall": wall, "accepted": accepted,
+    return {"n": n, "N": N, "ncell": ncell, "wall": wall, "accepted": accepted, "warmup": warmup,
             "rejected": st1["rejections"] - st0["rejections"], "evals": st1["evaluations"] - st0["evaluations"],
             "kinds": kinds, "setup_s": setup_s, "dev_bytes": dev_bytes, "sim_time": st1["time"], "m_avg": m_avg.tolist()}
+
+
+def speedup_run(steps: int, warmup: int, mode: int, plain):
+    """The same workload with the demag evaluation speedup (mode = number of saved demag fields,
+    extrapolated in time between convolutions): one convolution per accepted step instead of
+    six.  Reports the rate, the speedup over the plain run and <m> of both after the same
+    iteration count (the trajectories differ by the extrapolation error)."""
+    import torch
+    from boris2_amd import api
+    n, h, p = workload()
+    ncell = n[0] * n[1] * n[2]
+    M0 = api.random_magnetisation(ncell, p.Ms, seed=1).astype(np.float32)
+    s = api.FMSolver(api.BM_EVAL_RKDP54, n, h, p, M0)
+    s.set_eval_speedup(mode)
+    s.run_to_iteration(warmup)
+    torch.cuda.synchronize()
+    st0, c0 = s.state(), s.speedup_state()["convolutions"]
+    t0 = time.perf_counter()
+    s.run_to_iteration(warmup + steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st1, c1 = s.state(), s.speedup_state()["convolutions"]
+    m_avg = s.average_m()
+    s.close()
+    acc = st1["iteration"] - st0["iteration"]
+    rate = ncell * acc / wall
+    plain_rate = plain["ncell"] * plain["accepted"] / plain["wall"]
+    out = {"mode": mode, "method": "RKDP54 adaptive", "steps": acc, "rejected_attempts": st1["rejections"] - st0["rejections"],
+           "convolutions": c1 - c0, "field_evaluations": st1["evaluations"] - st0["evaluations"],
+           "cell_updates_per_s": rate, "ms_per_step": 1e3 * wall / max(1, acc), "over_plain": rate / plain_rate,
+           "sim_time_s": st1["time"], "m_avg": m_avg.tolist()}
+    if plain["accepted"] == acc and plain.get("warmup") == warmup:
+        out["plain_m_avg"] = plain["m_avg"]
+        out["plain_sim_time_s"] = plain["sim_time"]
+        out["max_abs_dm_avg"] = float(np.abs(np.array(plain["m_avg"]) - m_avg).max())
+    return out
 
 
 def batched_meshes(seed: int = 64, count: int = 64):
@@ -345,6 +383,8 @@ def main() -> int:
     ap.add_argument("--adjoint-steps", type=int, default=4000, help="RK4 steps of the adjoint rollout (0: skip)")
     ap.add_argument("--ml-steps", type=int, default=50, help="steps of the batched layer-stack comparison (0: skip)")
     ap.add_argument("--ml-adjoint-steps", type=int, default=1000, help="RK4 steps of the layer-stack adjoint (0: skip)")
+    ap.add_argument("--speedup-steps", type=int, default=500, help="RKDP54 steps with the demag evaluation speedup (0: skip)")
+    ap.add_argument("--speedup-mode", type=int, default=3, help="evaluation speedup mode (saved demag fields, 1..6)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy 5 times (scales the HBM byte counters)")
     a = ap.parse_args()
@@ -371,6 +411,7 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
             "traffic_source": str(TRAFFIC_FILE.relative_to(ROOT)), "kernel": dom,
             "alg_bytes_per_launch": round(d["alg_GB_per_launch"] * 1e9), "avg_launch_ms": d["avg_ms"]}
+    speedup = speedup_run(a.speedup_steps, a.warmup, a.speedup_mode, r) if a.speedup_steps > 0 else None
     batched = batch_run(a.batch_steps) if a.batch_steps > 0 else None
     adjoint = adjoint_run(a.adjoint_steps, 100) if a.adjoint_steps > 0 else None
     multilayer = multilayer_run(a.ml_steps, a.ml_adjoint_steps) if a.ml_steps > 0 else None
@@ -387,6 +428,7 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "eval_speedup": speedup,
         "batched": batched,
         "adjoint": adjoint,
         "multilayer": multilayer,

@@ -225,6 +225,19 @@ class FMSolver:
     def set_astep(self, err_fail=-1.0, dT_increase=-1.0, dT_min=-1.0, dT_max=-1.0) -> None:
         check(self.L.bm_ode_set_astep(self.handle, err_fail, dT_increase, dT_min, dT_max))
 
+    def set_eval_speedup(self, mode: int, dTspeedup: float = 0.0) -> None:
+        """Demag evaluation speedup (the reference's ``evalspeedup`` setting): mode 0 none,
+        1 step, 2 linear, 3 quadratic, 4 cubic, 5 quartic, 6 quintic extrapolation of saved
+        demag fields; dTspeedup > 0 convolves every dTspeedup of simulated time instead of
+        at the start of every step."""
+        check(self.L.bm_ode_set_eval_speedup(self.handle, int(mode), float(dTspeedup)))
+
+    def speedup_state(self) -> dict:
+        n, conv = C.c_int(), C.c_longlong()
+        tdm = (C.c_double * 6)()
+        check(self.L.bm_ode_get_speedup_state(self.handle, C.byref(n), tdm, C.byref(conv)))
+        return {"num_saved": n.value, "tdm": np.array(tdm[:]), "convolutions": conv.value}
+
     def advance_time(self) -> None:
         check(self.L.bm_ode_advance_time(self.handle, _stream()))
 

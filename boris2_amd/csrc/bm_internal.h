@@ -21,6 +21,8 @@ int device_cu_count();
 enum ProfKind { PK_FWD_X = 0, PK_Y_FWD, PK_Z, PK_Y_INV, PK_INV_X, PK_Y2D, PK_STAGE, PK_ADVANCE, PK_ROW, PK_FWD_ADV, PK_ML_MULT, PK_COUNT };
 void prof_mark_begin(int kind, hipStream_t st);
 void prof_mark_end(int kind, hipStream_t st);
+// self-demag coefficient (-N diagonal) of one cell of size hx x hy x hz (bm_tensor.cpp)
+void self_demag_coeff(double hx, double hy, double hz, double out[3]);
 
 } // namespace bm
 

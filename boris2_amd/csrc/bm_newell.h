@@ -89,6 +89,8 @@ struct AsymOff { int cubic; double lead, a[3], b[6], c[10]; };
 // host-side construction (bm_tensor.cpp)
 void asym_diag_init(AsymDiag& A, double hx, double hy, double hz);
 void asym_off_init(AsymOff& A, double hx, double hy, double hz);
+// self-demag coefficient (-N diagonal) of one cell of size hx x hy x hz (bm_tensor.cpp)
+void self_demag_coeff(double hx, double hy, double hz, double out[3]);
 
 BM_HD double asym_diag(const AsymDiag& A, double x, double y, double z)
 {

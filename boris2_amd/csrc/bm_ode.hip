@@ -45,7 +45,26 @@ __device__ __forceinline__ void stage_sweep(const StageArgs& a)
 		const size_t idx = (size_t)chunk * kBS + threadIdx.x;
 		if (idx >= n) continue;
 		const int i = (int)(idx % a.nx), j = (int)((idx / a.nx) % a.ny), k = (int)(idx / ((size_t)a.nx * a.ny));
-		const F3 Hd = (a.modules & BM_MOD_DEMAG) ? F3{ a.Hd[idx], a.Hd[n + idx], a.Hd[2 * n + idx] } : F3{ 0.f, 0.f, 0.f };
+		F3 Hd{ 0.f, 0.f, 0.f };
+		if (a.nsp) {
+			// Demag_EvalSpeedup_AddExtrapField_AddSelf: saved fields in slot order, then self term
+			const F3 m = ld3(a.Min, idx);
+			Hd = a.ca[0] * F3{ a.Hs[0][idx], a.Hs[0][n + idx], a.Hs[0][2 * n + idx] };
+#pragma unroll
+			for (int q = 1; q < 6; q++)
+				if (q < a.nsp) Hd = Hd + a.ca[q] * F3{ a.Hs[q][idx], a.Hs[q][n + idx], a.Hs[q][2 * n + idx] };
+			Hd = Hd + F3{ a.selfc[0] * m.x, a.selfc[1] * m.y, a.selfc[2] * m.z };
+		}
+		else if (a.modules & BM_MOD_DEMAG) {
+			Hd = F3{ a.Hd[idx], a.Hd[n + idx], a.Hd[2 * n + idx] };
+			if (a.Hsave) {
+				// Demag_EvalSpeedup_AddField_SubSelf: keep the convolution minus the self term
+				const F3 m = ld3(a.Min, idx);
+				a.Hsave[idx] = Hd.x - a.selfc[0] * m.x;
+				a.Hsave[n + idx] = Hd.y - a.selfc[1] * m.y;
+				a.Hsave[2 * n + idx] = Hd.z - a.selfc[2] * m.z;
+			}
+		}
 		stage_cell<METHOD, STEP>(a, Ha, idx, i, j, k, Hd, lte_loc);
 	}
 	if (stage_has_lte<METHOD, STEP>()) block_max_to(lte_loc, a.lte);
@@ -124,6 +143,18 @@ struct bm_ode_s {
 	int order = 5, evalStep = 0;
 	long long iteration = 0, stageiteration = 0, rejections = 0, evaluations = 0;
 	bool primed = false, available = true;
+	// demag evaluation speedup (DemagCUDA::UpdateField, Boris/DemagCUDA.cpp:106-468)
+	int speedup = 0, num_saved = 0;       // EVALSPEEDUP_ mode = number of saved fields
+	double tdm[6] = {};                   // save times
+	float* Hs[6] = {};                    // saved fields minus self term, planar
+	float selfc[3] = {};
+	bool link_dTspeedup = true;
+	double dTspeedup = 0, time_speedup = 0;
+	long long convolutions = 0;
+	// per-launch speedup use, read by make_args: extrapolate (sp_n saved fields) or save
+	int sp_n = 0;
+	float sp_ca[6] = {};
+	float* sp_save = nullptr;
 };
 
 namespace {
@@ -174,6 +205,12 @@ StageArgs make_args(bm_ode_s* o, float dT)
 	a.dT = dT;
 	a.modules = o->p.modules;
 	a.map = stage_map();
+	if (o->sp_n || o->sp_save) {
+		a.nsp = o->sp_n;
+		for (int q = 0; q < 6; q++) { a.Hs[q] = o->Hs[q]; a.ca[q] = o->sp_ca[q]; }
+		for (int c = 0; c < 3; c++) a.selfc[c] = o->selfc[c];
+		a.Hsave = o->sp_save;
+	}
 	if (o->ml) {
 		a.layered = 1;
 		a.ihz2 = 0.f;
@@ -260,14 +297,100 @@ int eval_row(bm_ode_s* o, int step, float dT, hipStream_t st)
 	return BM_OK;
 }
 
+// ODECommon_Base::Get_EvalStep_Time (Boris/DiffEq_CommonBase_Iterate.cpp:1536-1635): step
+// start time plus dT times the tableau's c value of the stage about to run.
+double eval_step_time(const bm_ode_s* o)
+{
+	static const double c_teuler[2] = { 0.0, 1.0 };
+	static const double c_rk4[4] = { 0.0, 0.5, 0.5, 1.0 };
+	static const double c_rkdp54[6] = { 0.0, 0.2, 0.3, 0.8, 8.0 / 9, 1.0 };
+	switch (o->mcode) {
+	case M_TEULER: case M_AHEUN: return o->time + o->dT * c_teuler[o->evalStep];
+	case M_RK4: return o->time + o->dT * c_rk4[o->evalStep];
+	case M_RKDP: return o->time + o->dT * c_rkdp54[o->evalStep];
+	}
+	return o->time;
+}
+
+// ODECommon_Base::Check_Step_Update (Boris/DiffEq_CommonBase_Iterate.cpp:1639-1657)
+bool check_step_update(bm_ode_s* o)
+{
+	if (!o->speedup) return true;
+	if (!o->link_dTspeedup) {
+		const double te = eval_step_time(o);
+		if (te < o->time_speedup + o->dTspeedup) return false;
+		o->time_speedup = te;
+		return true;
+	}
+	return o->available;
+}
+
+// Slot a new save replaces once all are filled: the oldest, from the order of the save
+// times (Boris/DemagCUDA.cpp:176-324).
+int next_slot(const bm_ode_s* o)
+{
+	const int p = o->speedup;
+	if (p == 1) return 0;
+	bool rising = true;
+	for (int q = 1; q < p; q++) rising = rising && o->tdm[q] > o->tdm[q - 1];
+	if (rising) return 0;
+	for (int q = 0; q + 2 < p; q++) if (o->tdm[q] > o->tdm[q + 1]) return q + 1;
+	return p - 1;
+}
+
+// Demag evaluation with the speedup (DemagCUDA::UpdateField, Boris/DemagCUDA.cpp:106-468):
+// a full convolution (saved, minus the self term, while the ODE recommends an update or
+// slots are free), or the Lagrange extrapolation of the saved fields plus the current self
+// term (coefficients in FP64 on the host, rounded to FP32 as the reference's cuBReal).
+int eval_speedup(bm_ode_s* o, int step, float dT, hipStream_t st)
+{
+	const int p = o->speedup;
+	int rc;
+	if (o->num_saved < p && !check_step_update(o)) {
+		if ((rc = bm::demag_planar(o->demag, o->Mcur, o->Hd, st))) return rc;
+		o->convolutions++;
+		return launch_stage(o, step, dT, st);
+	}
+	if (check_step_update(o) || o->num_saved < p) {
+		int slot;
+		if (o->num_saved < p) { slot = o->num_saved; o->tdm[slot] = eval_step_time(o); o->num_saved++; }
+		else { slot = next_slot(o); if (p > 1) o->tdm[slot] = eval_step_time(o); }
+		if ((rc = bm::demag_planar(o->demag, o->Mcur, o->Hd, st))) return rc;
+		o->convolutions++;
+		o->sp_save = o->Hs[slot];
+		rc = launch_stage(o, step, dT, st);
+		o->sp_save = nullptr;
+		return rc;
+	}
+	const double t = eval_step_time(o);
+	for (int q = 0; q < 6; q++) o->sp_ca[q] = 0.f;
+	if (p == 1) o->sp_ca[0] = 1.f;
+	else for (int q = 0; q < p; q++) {
+		double num = 1, den = 1;
+		for (int r = 0; r < p; r++) if (r != q) { num *= (t - o->tdm[r]); den *= (o->tdm[q] - o->tdm[r]); }
+		o->sp_ca[q] = (float)(num / den);
+	}
+	o->sp_n = p;
+	rc = launch_stage(o, step, dT, st);
+	o->sp_n = 0;
+	return rc;
+}
+
+// Demag evaluations go through the fused row kernel (single mesh, no speedup).
+bool row_path(const bm_ode_s* o)
+{
+	return (o->p.modules & BM_MOD_DEMAG) && o->use_row && !o->ml && !o->speedup;
+}
+
 // One field evaluation + stage `step` (the effective field is that of Mcur at stagetime).
 int evaluate(bm_ode_s* o, int step, float dT, hipStream_t st)
 {
 	o->evaluations++;
 	const bool demag = (o->p.modules & BM_MOD_DEMAG) != 0;
-	if (demag && o->ml) { int rc = bm::mlayer_planar(o->ml, o->Mcur, o->Hd, st); if (rc) return rc; return launch_stage(o, step, dT, st); }
-	if (demag && o->use_row) return eval_row(o, step, dT, st);
-	if (demag) { int rc = bm::demag_planar(o->demag, o->Mcur, o->Hd, st); if (rc) return rc; }
+	if (demag && o->ml) { int rc = bm::mlayer_planar(o->ml, o->Mcur, o->Hd, st); if (rc) return rc; o->convolutions++; return launch_stage(o, step, dT, st); }
+	if (demag && o->speedup) return eval_speedup(o, step, dT, st);
+	if (row_path(o)) { o->convolutions++; return eval_row(o, step, dT, st); }
+	if (demag) { int rc = bm::demag_planar(o->demag, o->Mcur, o->Hd, st); if (rc) return rc; o->convolutions++; }
 	return launch_stage(o, step, dT, st);
 }
 
@@ -340,7 +463,7 @@ int iterate_stage(bm_ode_s* o, hipStream_t st)
 				}
 			}
 			else o->primed = true;
-			if ((o->p.modules & BM_MOD_DEMAG) && o->use_row) {
+			if (row_path(o)) {
 				// the advance rides on the x-R2C pass of the next evaluation's input
 				if ((rc = bm::demag_fwd_x_advance(o->demag, o->Mcur, o->E[0], o->sM1, o->Mb, (float)o->dT / 5, st))) return rc;
 				o->S_valid = true;
@@ -403,6 +526,7 @@ void free_all(bm_ode_s* o)
 	if (o->Mb) (void)hipFree(o->Mb);
 	for (int q = 0; q < 6; q++) if (o->E[q]) (void)hipFree(o->E[q]);
 	if (o->Heff_keep) (void)hipFree(o->Heff_keep);
+	for (int q = 0; q < 6; q++) if (o->Hs[q]) (void)hipFree(o->Hs[q]);
 	if (o->d_lte) (void)hipFree(o->d_lte);
 	if (o->h_lte) (void)hipHostFree(o->h_lte);
 	if (o->d_part) (void)hipFree(o->d_part);
@@ -517,12 +641,48 @@ extern "C" int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p)
 	if (!o || !p) return fail(BM_ERR_ARG, "null argument");
 	if ((p->modules & BM_MOD_DEMAG) && !o->ml && (!o->demag || o->demag->nx != o->nx || o->demag->ny != o->ny || o->demag->nz != o->nz))
 		return fail(BM_ERR_STATE, "bm_ode_set_params: demag needs a demag handle of this mesh size at creation, or a layer stack");
-	if ((p->modules & BM_MOD_DEMAG) && !o->use_row && !o->Hd) {
+	if ((p->modules & BM_MOD_DEMAG) && (!o->use_row || o->speedup) && !o->Hd) {
 		const size_t b3 = o->n * 3 * sizeof(float);
 		if (hipMalloc(&o->Hd, b3) != hipSuccess) return fail(BM_ERR_NOMEM, "bm_ode_set_params: demag field buffer");
 	}
 	o->p = *p;
 	o->S_valid = false;
+	o->num_saved = 0;      // DemagCUDA::UpdateConfiguration drops the saved fields
+	return BM_OK;
+}
+
+extern "C" int bm_ode_set_eval_speedup(bm_ode_t o, int mode, double dTspeedup)
+{
+	if (!o || mode < 0 || mode > 6) return fail(BM_ERR_ARG, "bm_ode_set_eval_speedup: mode must be 0..6");
+	if (mode && o->ml) return fail(BM_ERR_STATE, "bm_ode_set_eval_speedup: not available for layer stacks");
+	const size_t b3 = o->n * 3 * sizeof(float);
+	for (int q = 0; q < 6; q++) {
+		if (q >= mode && o->Hs[q]) { (void)hipFree(o->Hs[q]); o->Hs[q] = nullptr; }
+		if (q < mode && !o->Hs[q] && hipMalloc(&o->Hs[q], b3) != hipSuccess) {
+			o->Hs[q] = nullptr;
+			return fail(BM_ERR_NOMEM, "bm_ode_set_eval_speedup: saved field buffers");
+		}
+	}
+	if (mode && !o->Hd && hipMalloc(&o->Hd, b3) != hipSuccess) { o->Hd = nullptr; return fail(BM_ERR_NOMEM, "bm_ode_set_eval_speedup: demag field buffer"); }
+	o->speedup = mode;
+	o->num_saved = 0;
+	for (int q = 0; q < 6; q++) o->tdm[q] = 0;
+	// DemagTFunc::SelfDemag_PBC without images, cell size in metres (Boris/DemagCUDA.cpp:37)
+	double sd[3];
+	bm::self_demag_coeff(o->h[0], o->h[1], o->h[2], sd);
+	for (int c = 0; c < 3; c++) o->selfc[c] = (float)sd[c];
+	if (dTspeedup > 0) { o->link_dTspeedup = false; o->dTspeedup = dTspeedup; o->time_speedup = o->time; }
+	else { o->link_dTspeedup = true; o->dTspeedup = o->dT; }
+	o->S_valid = false;
+	return BM_OK;
+}
+
+extern "C" int bm_ode_get_speedup_state(bm_ode_t o, int* num_saved, double* tdm6, long long* convolutions)
+{
+	if (!o) return fail(BM_ERR_ARG, "null handle");
+	if (num_saved) *num_saved = o->num_saved;
+	if (tdm6) for (int q = 0; q < 6; q++) tdm6[q] = o->tdm[q];
+	if (convolutions) *convolutions = o->convolutions;
 	return BM_OK;
 }
 

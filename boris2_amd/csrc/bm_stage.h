@@ -90,6 +90,14 @@ struct StageArgs {
 	const double* tnow;
 	int wave;
 	float amp, w, t0, dir[3], slope[3];
+	// demag evaluation speedup (Boris/DemagCUDA.cpp:106-468, kernels Boris/DemagCUDA.cu:44-413):
+	// with nsp > 0 the demag field of a cell is sum_{s<nsp} ca[s] Hs[s] + selfc * M (planar
+	// saved fields, AoS M of this stage); with Hsave set, the stage also stores Hd - selfc * M
+	// (planar) after a full convolution.
+	const float* Hs[6];
+	float* Hsave;
+	float ca[6], selfc[3];
+	int nsp;
 };
 
 // Applied field of a stage: Ha, plus the time-dependent term at the device-held step time

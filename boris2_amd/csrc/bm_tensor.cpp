@@ -100,6 +100,31 @@ void asym_diag_init(AsymDiag& A, double hx, double hy, double hz)
 		A.c[q] = A.cubic ? (A.lead / 192.0) * (kDiag9Cubic[q] * p.x[3]) : (A.lead / 192.0) * p.eval<10>(kDiag9[q], kDeg3);
 }
 
+// Self-demag coefficient of one cell, stored as -N: the 27-term second difference of f at
+// zero offset (Boris/DemagTFunc_Self.cpp:6-56).  The reference evaluates it with the cell
+// size in metres for the demag evaluation speedup (Boris/DemagCUDA.cpp:37); callers pass
+// the same values so the zero guards of f act identically.
+static double self_demag_1(double hx, double hy, double hz)
+{
+	const double t[27] = {
+		+8 * newell_f(0, 0, 0),
+		-4 * newell_f(hx, 0, 0), -4 * newell_f(-hx, 0, 0), -4 * newell_f(0, hy, 0), -4 * newell_f(0, -hy, 0),
+		-4 * newell_f(0, 0, hz), -4 * newell_f(0, 0, -hz),
+		+2 * newell_f(-hx, -hy, 0), +2 * newell_f(-hx, hy, 0), +2 * newell_f(hx, -hy, 0), +2 * newell_f(hx, hy, 0),
+		+2 * newell_f(-hx, 0, -hz), +2 * newell_f(-hx, 0, hz), +2 * newell_f(hx, 0, -hz), +2 * newell_f(hx, 0, hz),
+		+2 * newell_f(0, -hy, -hz), +2 * newell_f(0, -hy, hz), +2 * newell_f(0, hy, -hz), +2 * newell_f(0, hy, hz),
+		-1 * newell_f(-hx, -hy, -hz), -1 * newell_f(-hx, -hy, hz), -1 * newell_f(-hx, hy, -hz), -1 * newell_f(hx, -hy, -hz),
+		-1 * newell_f(-hx, hy, hz), -1 * newell_f(hx, -hy, hz), -1 * newell_f(hx, hy, -hz), -1 * newell_f(hx, hy, hz) };
+	return comp_sum(t, 27) / (4 * kPi * hx * hy * hz);
+}
+
+void self_demag_coeff(double hx, double hy, double hz, double out[3])
+{
+	out[0] = -self_demag_1(hx, hy, hz);
+	out[1] = -self_demag_1(hy, hx, hz);
+	out[2] = -self_demag_1(hz, hy, hx);
+}
+
 void asym_off_init(AsymOff& A, double hx, double hy, double hz)
 {
 	Pw p(hx, hy, hz);

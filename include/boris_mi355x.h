@@ -151,6 +151,20 @@ int bm_ode_set_layers(bm_ode_t o, bm_mlayer_t ml, const double* Ms, const double
  * the current ones); the defaults are the reference's per-method defaults. */
 int bm_ode_set_dt(bm_ode_t o, double dT);
 int bm_ode_set_astep(bm_ode_t o, double err_fail, double dT_increase, double dT_min, double dT_max);
+/* Demag evaluation speedup: mode is the reference's EVALSPEEDUP_ value (0 none, 1 step,
+ * 2 linear, 3 quadratic, 4 cubic, 5 quartic, 6 quintic; Boris/DiffEq_Defs.h:121-125) =
+ * the number of saved demag fields.  The demag field is convolved only when the ODE
+ * recommends an update (at the first stage of each step, or every dTspeedup of simulated
+ * time when dTspeedup > 0) and extrapolated in time from the saved fields otherwise.
+ * Replaces ODECommon_Base::SetEvaluationSpeedup / SetSpeedupTimeStep / SetLink_dTspeedup
+ * (Boris/DiffEq_CommonBase.h:252, Boris/DiffEq_CommonBase_Control.cpp:243-256) with
+ * DemagCUDA::UpdateField's speedup branch (Boris/DemagCUDA.cpp:106-468).  Single meshes
+ * only (not layer stacks); allocates one field array per saved field; drops saved fields. */
+int bm_ode_set_eval_speedup(bm_ode_t o, int mode, double dTspeedup);
+/* Saved-field count, their save times (double[6]) and the number of convolutions run so
+ * far (any pointer may be NULL).  DemagCUDA::num_Hdemag_saved / time_demag1..6
+ * (Boris/DemagCUDA.h:35-37). */
+int bm_ode_get_speedup_state(bm_ode_t o, int* num_saved, double* tdm6, long long* convolutions);
 
 /* One SuperMesh::AdvanceTimeCUDA call (Boris/SuperMeshSimulation.cpp:122-149): field
  * updates and stages until the step is solved (adaptive methods may reject and redo). */

@@ -310,6 +310,27 @@ void or_sim_state(void* h, double* st)
 		st[5] = s->primed; st[6] = (double)s->rejections; st[7] = (double)s->evaluations; st[8] = s->stagetime; });
 }
 
+// Demag evaluation speedup: mode 0..6 (EVALSPEEDUP_), dTs > 0 unlinks the speedup step.
+int or_sim_set_speedup(void* h, int mode, double dTs)
+{
+	OrSim* o = (OrSim*)h;
+	try { SIM_DISPATCH(o, { s->set_speedup(mode, dTs); }); return 0; }
+	catch (...) { return 1; }
+}
+
+// num_saved, tdm[6], convolutions (8 doubles)
+void or_sim_speedup_info(void* h, double* out)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, {
+		out[0] = s->num_saved;
+		for (int q = 0; q < 6; q++) out[1 + q] = s->tdm[q];
+		out[7] = (double)s->convolutions; });
+}
+
+int or_speedup_next_slot(int p, const double* tdm) { return speedup_next_slot(p, tdm); }
+void or_speedup_lagrange(int p, const double* tdm, double t, double* a6) { speedup_lagrange(p, tdm, t, a6); }
+
 void or_sim_average_m(void* h, double* out3)
 {
 	OrSim* o = (OrSim*)h;

@@ -20,6 +20,11 @@
 //                     of each tableau on scalar problems.
 //   Step control      closed formula (tests replay SetAdaptiveTimeStep by hand).
 //   BorisRand         textbook LCG sequence (Numerical Recipes constants).
+//   Eval speedup      Lagrange coefficients reproduce polynomials of degree speedup-1
+//                     exactly; slot choice keeps the last `speedup` computed fields; a
+//                     one-cell mesh (demag = self term only) integrates identically with
+//                     and without the speedup; trajectories converge onto the plain ones
+//                     as dT shrinks.
 #pragma once
 
 #include <cmath>
@@ -401,6 +406,32 @@ inline void applied_field(const Params& p, double t, double H[3])
 	}
 }
 
+// Demag evaluation speedup, slot that the next computed field replaces once all p slots
+// are filled: the oldest, found from the order of the save times tdm
+// (Boris/DemagCUDA.cpp:176-324).
+inline int speedup_next_slot(int p, const double* tdm)
+{
+	if (p == 1) return 0;
+	bool rising = true;
+	for (int s = 1; s < p; s++) rising = rising && tdm[s] > tdm[s - 1];
+	if (rising) return 0;
+	for (int s = 0; s + 2 < p; s++) if (tdm[s] > tdm[s + 1]) return s + 1;
+	return p - 1;
+}
+
+// Lagrange coefficients of the p saved fields at time t (Boris/DemagCUDA.cpp:365-467):
+// numerator and denominator products in ascending slot order, in FP64; slots >= p get 0.
+inline void speedup_lagrange(int p, const double* tdm, double t, double a[6])
+{
+	for (int s = 0; s < 6; s++) a[s] = 0;
+	if (p == 1) { a[0] = 1; return; }
+	for (int s = 0; s < p; s++) {
+		double num = 1, den = 1;
+		for (int q = 0; q < p; q++) if (q != s) { num *= (t - tdm[q]); den *= (tdm[s] - tdm[q]); }
+		a[s] = num / den;
+	}
+}
+
 // Evaluation methods (values of the reference's EVAL_ enum, Boris/DiffEq_Defs.h).
 enum { EV_TEULER = 1, EV_RK4 = 2, EV_AHEUN = 7, EV_RKDP54 = 9 };
 
@@ -427,7 +458,20 @@ struct Sim {
 	double time = 0, stagetime = 0, dT, dT_last, dT_min = 1e-15, dT_max = 3e-12, dT_increase = 2, err_high_fail = 1e-4, lte = 0;
 	int order, evalStep = 0, iteration = 0, stageiteration = 0;
 	bool primed = false, available = true, renormalize = true;
-	long long rejections = 0, evaluations = 0;
+	long long rejections = 0, evaluations = 0, convolutions = 0;
+
+	// Demag evaluation speedup (DemagCUDA::UpdateField, Boris/DemagCUDA.cpp:106-468):
+	// speedup = EVALSPEEDUP_ value (0 none, 1 step, 2 linear, 3 quadratic, 4 cubic,
+	// 5 quartic, 6 quintic, Boris/DiffEq_Defs.h:121-125) = number of saved demag fields.
+	// Saved fields hold the convolution minus the self-demag term; extrapolated
+	// evaluations add the Lagrange combination of the saved fields and the self term
+	// of the current M.
+	int speedup = 0, num_saved = 0;
+	double tdm[6] = { 0, 0, 0, 0, 0, 0 };
+	std::vector<R> Hs[6];
+	R selfc[3] = { 0, 0, 0 };
+	bool link_dTspeedup = true;
+	double dTspeedup = 0, time_speedup = 0;
 
 	Sim(int method_, int nx_, int ny_, int nz_, double hx, double hy, double hz, const Params& p_, int Nz_override = 0)
 		: nx(nx_), ny(ny_), nz(nz_), p(p_), method(method_)
@@ -466,10 +510,86 @@ struct Sim {
 			ext_demag(M.data(), Hd.data());
 			for (size_t q = 0; q < n * 3; q++) Heff[q] += Hd[q];
 		}
-		else if (demag) demag->convolute(M.data(), Heff.data(), false);
+		else if (demag) demag_update();
 		if (p.exchange) add_exchange();
 		if (p.anisotropy) add_anisotropy();
 		evaluations++;
+	}
+
+	// EVALSPEEDUP_ mode and, for dTs > 0, an independent speedup time step
+	// (ODECommon_Base::SetEvaluationSpeedup / SetSpeedupTimeStep,
+	// Boris/DiffEq_CommonBase.h:252, Boris/DiffEq_CommonBase_Control.cpp:243-256).
+	// Saved fields are dropped, as UpdateConfiguration does (Boris/DemagCUDA.cpp:76-103).
+	void set_speedup(int mode, double dTs)
+	{
+		if (mode < 0 || mode > 6) throw std::runtime_error("bad evaluation speedup mode");
+		speedup = mode; num_saved = 0;
+		for (int s = 0; s < 6; s++) { Hs[s].clear(); tdm[s] = 0; }
+		double sd[3]; self_demag(h[0], h[1], h[2], sd);   // SelfDemag_PBC without images
+		for (int c = 0; c < 3; c++) selfc[c] = (R)sd[c];
+		if (dTs > 0) { link_dTspeedup = false; dTspeedup = dTs; time_speedup = time; }
+		else { link_dTspeedup = true; dTspeedup = dT; }
+	}
+
+	// ODECommon_Base::Get_EvalStep_Time (Boris/DiffEq_CommonBase_Iterate.cpp:1536-1635):
+	// the step start time plus dT times the tableau's c value of the stage about to run.
+	double eval_step_time() const
+	{
+		static const double c_teuler[2] = { 0.0, 1.0 };
+		static const double c_rk4[4] = { 0.0, 0.5, 0.5, 1.0 };
+		static const double c_rkdp54[6] = { 0.0, 0.2, 0.3, 0.8, 8.0 / 9, 1.0 };
+		switch (method) {
+		case EV_TEULER: case EV_AHEUN: return time + dT * c_teuler[evalStep];
+		case EV_RK4: return time + dT * c_rk4[evalStep];
+		case EV_RKDP54: return time + dT * c_rkdp54[evalStep];
+		}
+		return time;
+	}
+
+	// ODECommon_Base::Check_Step_Update (Boris/DiffEq_CommonBase_Iterate.cpp:1639-1657)
+	bool check_step_update()
+	{
+		if (!speedup) return true;
+		if (!link_dTspeedup) {
+			double te = eval_step_time();
+			if (te < time_speedup + dTspeedup) return false;
+			time_speedup = te;
+			return true;
+		}
+		return available;
+	}
+
+	// DemagCUDA::UpdateField with the evaluation speedup (Boris/DemagCUDA.cpp:106-468).
+	void demag_update()
+	{
+		const size_t n3 = n * 3;
+		if (!speedup || (num_saved < speedup && !check_step_update())) {
+			demag->convolute(M.data(), Heff.data(), false);
+			convolutions++;
+			return;
+		}
+		if (check_step_update() || num_saved < speedup) {
+			int slot;
+			if (num_saved < speedup) { slot = num_saved; tdm[slot] = eval_step_time(); num_saved++; }
+			else { slot = speedup_next_slot(speedup, tdm); if (speedup > 1) tdm[slot] = eval_step_time(); }
+			std::vector<R>& S = Hs[slot];
+			S.assign(n3, (R)0);
+			demag->convolute(M.data(), S.data(), true);
+			convolutions++;
+			// Demag_EvalSpeedup_AddField_SubSelf (Boris/DemagCUDA.cu:44-57)
+			for (size_t q = 0; q < n3; q++) { Heff[q] += S[q]; S[q] -= selfc[q % 3] * M[q]; }
+		}
+		else {
+			// Demag_EvalSpeedup_AddExtrapField_AddSelf (Boris/DemagCUDA.cu:59-413)
+			double ad[6]; speedup_lagrange(speedup, tdm, eval_step_time(), ad);
+			R a[6]; for (int s = 0; s < 6; s++) a[s] = (R)ad[s];
+			for (size_t q = 0; q < n3; q++) {
+				R v = Hs[0][q] * a[0];
+				for (int s = 1; s < speedup; s++) v += Hs[s][q] * a[s];
+				v += selfc[q % 3] * M[q];
+				Heff[q] += v;
+			}
+		}
 	}
 
 	// Heff += 2A/(mu0 Ms^2) * delsq_neu(M)  (Boris/Exchange.cpp:108-156,

@@ -55,6 +55,10 @@ def lib() -> C.CDLL:
     L.or_sim_iterate_stage.argtypes = [vp]
     L.or_sim_advance_time.argtypes = [vp]
     L.or_sim_run_to_iteration.argtypes = [vp, i]
+    L.or_sim_set_speedup.argtypes = [vp, i, d]; L.or_sim_set_speedup.restype = i
+    L.or_sim_speedup_info.argtypes = [vp, dp]
+    L.or_speedup_next_slot.argtypes = [i, dp]; L.or_speedup_next_slot.restype = i
+    L.or_speedup_lagrange.argtypes = [i, dp, d, dp]
     L.or_tensor2d_zshift.argtypes = [i] * 4 + [dp, dp, d, dp, dp]
     L.or_kernels2d_zshift.argtypes = [i, i, dp, dp, dp, dp]
     L.or_mlayer_create.argtypes = [i, i, i, d, d, dp, dp]; L.or_mlayer_create.restype = vp
@@ -147,6 +151,20 @@ class OracleMLayer:
             self.L.or_mlayer_destroy(self.h)
         except Exception:
             pass
+
+
+def speedup_next_slot(p, tdm):
+    """Slot the next computed demag field replaces once p slots hold saves at times tdm."""
+    t = np.zeros(6); t[:len(tdm)] = tdm
+    return lib().or_speedup_next_slot(int(p), t)
+
+
+def speedup_lagrange(p, tdm, t):
+    """Extrapolation coefficients of the p saved fields at time t (6 values, zeros past p)."""
+    td = np.zeros(6); td[:len(tdm)] = tdm
+    a = np.zeros(6)
+    lib().or_speedup_lagrange(int(p), td, float(t), a)
+    return a
 
 
 def fft(x: np.ndarray, sign: int = -1) -> np.ndarray:
@@ -251,6 +269,16 @@ class OracleSim:
         out = np.zeros(3)
         self.L.or_sim_average_m(self.h, out)
         return out
+
+    def set_speedup(self, mode, dTspeedup=0.0):
+        """Demag evaluation speedup: mode 0..6 (none, step, linear ... quintic); dTspeedup > 0
+        unlinks the speedup time step from dT."""
+        assert self.L.or_sim_set_speedup(self.h, int(mode), float(dTspeedup)) == 0
+
+    def speedup_info(self):
+        out = np.zeros(8)
+        self.L.or_sim_speedup_info(self.h, out)
+        return {"num_saved": int(out[0]), "tdm": out[1:7].copy(), "convolutions": int(out[7])}
 
     def __del__(self):
         try:
