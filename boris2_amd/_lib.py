@@ -26,6 +26,7 @@ EXPORTS = [
     "bm_batch_create", "bm_batch_create_layers", "bm_batch_destroy", "bm_batch_set_dt", "bm_batch_run", "bm_batch_get_state",
     "bm_batch_mesh_offset", "bm_batch_autotune_report",
     "bm_adj_create", "bm_adj_destroy", "bm_adj_forward", "bm_adj_backward", "bm_adj_zeeman_grads", "bm_adj_set_layers",
+    "bm_ovf2_info", "bm_ovf2_read", "bm_ovf2_write",
 ]
 
 PROF_KINDS = ["fwd_x_r2c", "y_fft", "z_fft_kernel_ifft", "y_ifft", "inv_x_c2r", "y2d_fft_kernel_ifft",
@@ -111,6 +112,9 @@ def load() -> C.CDLL:
         "bm_ode_keep_heff": (i, [vp, i]),
         "bm_ode_copy_heff": (i, [vp, vp, vp]),
         "bm_ode_average_m": (i, [vp, C.POINTER(d), vp]),
+        "bm_ovf2_info": (i, [C.c_char_p, vp, vp, vp, C.POINTER(i), C.POINTER(i)]),
+        "bm_ovf2_read": (i, [C.c_char_p, vp, ll]),
+        "bm_ovf2_write": (i, [C.c_char_p, vp, i, vp, vp, vp, i, d]),
         "bm_prof_begin": (i, [i]),
         "bm_prof_end": (i, []),
         "bm_prof_query": (i, [i, C.POINTER(ll), C.POINTER(d), C.POINTER(d), C.POINTER(d)]),

@@ -11,6 +11,7 @@ call the C-ABI for every piece of compute:
   Boris/SuperMeshSimulation.cpp:122-149).
 * :func:`random_magnetisation` — ``FMesh::SetRandomMag`` with the ``BorisRand`` LCG
   (Boris/Mesh_Ferromagnetic_Control.cpp:230-255, BorisLib/BLib_prng.h:18-90).
+* :func:`read_ovf2` / :func:`write_ovf2` — ``OVF2`` file handlers (Boris/OVF2_Handlers.cpp).
 """
 from __future__ import annotations
 
@@ -554,3 +555,51 @@ def random_magnetisation(ncell: int, Ms: float, seed: int, M_norm: np.ndarray | 
     norm = Ms if M_norm is None else M_norm
     m = np.stack([np.cos(ph) * np.sin(th), np.sin(ph) * np.sin(th), np.cos(th)], axis=1)
     return m * np.asarray(norm).reshape(-1, 1)
+
+
+@dataclass
+class OVF2Data:
+    """Contents of an OVF 2.0 file: values (cells, valuedim) float64, node counts n, cell size h
+    and rectangle rect = (xmin, ymin, zmin, xmax, ymax, zmax), all in metres."""
+    values: np.ndarray
+    n: tuple
+    h: tuple
+    rect: tuple
+    data_bytes: int
+
+
+def read_ovf2(path) -> OVF2Data:
+    """``OVF2::Read_OVF2`` (Boris/OVF2_Handlers.cpp:41-690): vector (valuedim 3) or scalar
+    (valuedim 1) data, binary 4 / binary 8 / text, meshunit m or nm."""
+    L = _lib.load()
+    n3 = (C.c_int * 3)()
+    h3 = (C.c_double * 3)()
+    r6 = (C.c_double * 6)()
+    vd, nb = C.c_int(), C.c_int()
+    p = str(path).encode()
+    check(L.bm_ovf2_info(p, n3, h3, r6, C.byref(vd), C.byref(nb)))
+    ncell = n3[0] * n3[1] * n3[2]
+    out = np.zeros(ncell * vd.value, dtype=np.float64)
+    check(L.bm_ovf2_read(p, out.ctypes.data_as(C.c_void_p), out.size))
+    return OVF2Data(out.reshape(ncell, vd.value), tuple(n3[:]), tuple(h3[:]), tuple(r6[:]), nb.value)
+
+
+def write_ovf2(path, values, n, h, rect=None, data_type: str = "bin8", norm: float = 1.0) -> None:
+    """``OVF2::Write_OVF2_VEC`` / ``Write_OVF2_SCA`` (Boris/OVF2_Handlers.cpp:692-902): values
+    (cells, 3) or (cells,) / (cells, 1), divided by norm; data_type "bin4", "bin8" or "text";
+    rect defaults to the mesh at the origin."""
+    v = np.ascontiguousarray(np.asarray(values, dtype=np.float64))
+    ncell = int(n[0]) * int(n[1]) * int(n[2])
+    vd = 1 if v.size == ncell else 3
+    if v.size != vd * ncell:
+        raise ValueError("values must hold 1 or 3 components per cell")
+    if rect is None:
+        rect = (0.0, 0.0, 0.0, n[0] * h[0], n[1] * h[1], n[2] * h[2])
+    codes = {"bin4": 4, "bin8": 8, "text": 0}
+    if data_type not in codes:
+        raise ValueError("data_type must be bin4, bin8 or text")
+    L = _lib.load()
+    n3 = (C.c_int * 3)(*[int(x) for x in n])
+    h3 = (C.c_double * 3)(*[float(x) for x in h])
+    r6 = (C.c_double * 6)(*[float(x) for x in rect])
+    check(L.bm_ovf2_write(str(path).encode(), v.ctypes.data_as(C.c_void_p), vd, n3, h3, r6, codes[data_type], float(norm)))

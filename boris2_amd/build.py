@@ -25,7 +25,7 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-I", str(PKG.parent / "include"),
 ]
-SOURCES = ["bm_demag.hip", "bm_mlayer.hip", "bm_batch.hip", "bm_adjoint.hip", "bm_row_rkdp.hip", "bm_row_rk4.hip", "bm_row_heun.hip", "bm_ode.hip", "bm_common.cpp", "bm_tensor.cpp"]
+SOURCES = ["bm_demag.hip", "bm_mlayer.hip", "bm_batch.hip", "bm_adjoint.hip", "bm_row_rkdp.hip", "bm_row_rk4.hip", "bm_row_heun.hip", "bm_ode.hip", "bm_common.cpp", "bm_tensor.cpp", "bm_ovf2.cpp"]
 
 
 def _compile(src: Path, obj: Path) -> tuple[Path, str]:

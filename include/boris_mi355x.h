@@ -262,6 +262,23 @@ int bm_prof_begin(int max_launches);
 int bm_prof_end(void);
 int bm_prof_query(int kind, long long* count, double* total_ms, double* min_ms, double* max_ms);
 
+/* ------------------------------------------------------------------------------------
+ * OVF 2.0 files (host memory, no device work).  Replaces OVF2::Read_OVF2_VEC /
+ * Read_OVF2_SCA / Write_OVF2_VEC / Write_OVF2_SCA (Boris/OVF2_Handlers.cpp:41-902), used by
+ * the reference's loadovf2mag / saveovf2mag commands for magnetisation states.  Rectangular
+ * meshes, meshunit m or nm, valuedim 1 or 3, data "binary 4", "binary 8" or "text", x
+ * fastest; sizes in metres.
+ */
+/* Header: node counts n3, cell size h3, rectangle rect6 = (xmin, ymin, zmin, xmax, ymax,
+ * zmax), valuedim (1 or 3) and data_bytes (4, 8, or 1 for text); any pointer may be NULL. */
+int bm_ovf2_info(const char* path, int* n3, double* h3, double* rect6, int* valuedim, int* data_bytes);
+/* The data as doubles: count = valuedim * cells values, cell-major, x fastest. */
+int bm_ovf2_read(const char* path, double* out, long long count);
+/* Writes valuedim * cells values (divided by norm) with the reference's header layout;
+ * data_type 4 = "binary 4", 8 = "binary 8", 0 = "text".  The title is the file name. */
+int bm_ovf2_write(const char* path, const double* data, int valuedim, const int* n3, const double* h3,
+	const double* rect6, int data_type, double norm);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,6 +4,8 @@ reference checkout and oracle/_ref exist; the GPU box only reads the results).
 * vortex_hh_clk_up.npz — the reference-held OVF2 file Boris/vortex_hh_clk_up.ovf
   (512 x 128 x 1 cells of 5 x 5 x 40 nm, binary-8 data after the 123456789012345.0 check
   value, x fastest) converted to float32, used as a realistic demag input.
+* vortex_hh_clk_up.ovf — the same reference data file copied verbatim (1.5 MB): the OVF2
+  reader's input and the byte-level target of the OVF2 writer.
 * tensor_golden.npz — FP64 Newell demag tensors written by oracle/_ref/ref_dump, i.e. by the
   reference's own Boris/DemagTFunc*.cpp compiled here: first-octant values (the rest are
   mirror images) for small meshes, and a seeded sample of entries for meshes large enough to
@@ -17,6 +19,7 @@ reference checkout and oracle/_ref exist; the GPU box only reads the results).
 """
 from __future__ import annotations
 
+import shutil
 import subprocess
 import sys
 from pathlib import Path
@@ -57,6 +60,7 @@ def dump(*args) -> np.ndarray:
 def main() -> None:
     n, h, M = read_ovf2(REF / "Boris" / "vortex_hh_clk_up.ovf")
     np.savez_compressed(HERE / "vortex_hh_clk_up.npz", M=M.astype(np.float32), n=np.array(n), h=np.array(h))
+    shutil.copyfile(REF / "Boris" / "vortex_hh_clk_up.ovf", HERE / "vortex_hh_clk_up.ovf")
 
     if not REF_DUMP.exists():
         sys.exit("oracle/_ref/ref_dump is not built: make -f oracle/Makefile.ref")
