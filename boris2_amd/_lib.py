@@ -24,7 +24,7 @@ EXPORTS = [
     "bm_prof_begin", "bm_prof_end", "bm_prof_query",
     "bm_mlayer_create", "bm_mlayer_destroy", "bm_mlayer_convolute", "bm_mlayer_copy_kernels",
     "bm_batch_create", "bm_batch_create_layers", "bm_batch_destroy", "bm_batch_set_dt", "bm_batch_run", "bm_batch_get_state",
-    "bm_batch_mesh_offset", "bm_batch_autotune_report",
+    "bm_batch_mesh_offset", "bm_batch_autotune_report", "bm_batch_set_astep", "bm_batch_get_mesh_state",
     "bm_adj_create", "bm_adj_destroy", "bm_adj_forward", "bm_adj_backward", "bm_adj_zeeman_grads", "bm_adj_set_layers",
     "bm_ovf2_info", "bm_ovf2_read", "bm_ovf2_write",
 ]
@@ -89,6 +89,8 @@ def load() -> C.CDLL:
         "bm_batch_get_state": (i, [vp, C.POINTER(d), C.POINTER(ll)]),
         "bm_batch_mesh_offset": (i, [vp, i, C.POINTER(ll), C.POINTER(ll)]),
         "bm_batch_autotune_report": (i, [vp, C.c_char_p, i]),
+        "bm_batch_set_astep": (i, [vp, d, d, d, d]),
+        "bm_batch_get_mesh_state": (i, [vp, i, C.POINTER(d), C.POINTER(d), C.POINTER(ll), C.POINTER(ll)]),
         "bm_adj_create": (i, [i, i, i, d, d, d, C.POINTER(FmParams), vp, vp, d, i, i, C.POINTER(vp)]),
         "bm_adj_destroy": (i, [vp]),
         "bm_adj_forward": (i, [vp, vp, vp, vp, C.POINTER(d), vp]),

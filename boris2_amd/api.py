@@ -320,9 +320,9 @@ class LayerStackSolver(FMSolver):
 
 
 class BatchSolver:
-    """Many independent meshes of uneven sizes integrated together with a fixed-step method
-    (RK4 or TEuler): one launch per pass size per field evaluation, a step replayed from a
-    captured hipGraph.  meshes: list of (n, h, FMParams); M0: list of per-mesh (ncell, 3)
+    """Many independent meshes of uneven sizes integrated together, with a fixed-step method
+    (RK4 or TEuler) or RKDP54 (each mesh with its own adaptive step, controlled on the device):
+    one launch per pass size per field evaluation, a step replayed from a captured hipGraph.  meshes: list of (n, h, FMParams); M0: list of per-mesh (ncell, 3)
     arrays or None.  self.M holds every mesh's magnetisation concatenated; mesh(b) is a view.
     layers: None, or per mesh None or dict(hz=..., z0=..., Ms=None, A=None, alpha=None,
     K1=None) making that mesh's n[2] planes a layer stack (as LayerStackSolver)."""
@@ -381,10 +381,20 @@ class BatchSolver:
     def run(self, nsteps: int) -> None:
         check(self.L.bm_batch_run(self.handle, int(nsteps), _stream()))
 
+    def set_astep(self, err_fail=-1.0, dT_increase=-1.0, dT_min=-1.0, dT_max=-1.0) -> None:
+        """RKDP54 batches: adaptive step controls of every mesh (<= 0 keeps the current value)."""
+        check(self.L.bm_batch_set_astep(self.handle, err_fail, dT_increase, dT_min, dT_max))
+
     def state(self) -> dict:
         t, it = C.c_double(), C.c_longlong()
         check(self.L.bm_batch_get_state(self.handle, C.byref(t), C.byref(it)))
         return {"time": t.value, "iteration": it.value}
+
+    def mesh_state(self, b: int) -> dict:
+        """Mesh b's time, dT, accepted iterations and rejected attempts after the last run."""
+        t, dt, it, rj = C.c_double(), C.c_double(), C.c_longlong(), C.c_longlong()
+        check(self.L.bm_batch_get_mesh_state(self.handle, int(b), C.byref(t), C.byref(dt), C.byref(it), C.byref(rj)))
+        return {"time": t.value, "dT": dt.value, "iteration": it.value, "rejections": rj.value}
 
     def close(self) -> None:
         if self.handle:

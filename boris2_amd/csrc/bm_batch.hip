@@ -9,10 +9,17 @@
 // so the host issues one graph launch per step however many meshes there are.
 //
 // Per-mesh arrays live concatenated (mesh b's cells at offset off[b]); the caller owns M.
-// Fixed-step methods only (RK4, TEuler); the per-mesh arguments (pointers, sizes,
-// parameters, dT) sit in device tables read by the kernels, so changing dT rewrites a
-// table and leaves the captured graph valid.  Time-dependent applied fields (sinc pulse,
-// ramp) read the step time from a device scalar that the graph itself advances.
+// The per-mesh arguments (pointers, sizes, parameters, dT) sit in device tables read by the
+// kernels, so changing dT rewrites a table and leaves the captured graph valid.
+// Fixed-step methods (RK4, TEuler): time-dependent applied fields (sinc pulse, ramp) read the
+// step time from a device scalar that the graph itself advances.
+// RKDP54: every mesh keeps its own adaptive step (ODECommon_Base::Iterate's RKDP54 branch,
+// Boris/DiffEq_CommonBase_Iterate.cpp:1214-1395, with SetAdaptiveTimeStep :15-59) in a
+// device control block (BatchCtl).  One graph replay is one attempt of every active mesh:
+// stage 0 (FSAL error estimate), a one-thread-per-mesh control kernel (accept with a new dT
+// written into the stage tables, or reject: restore and re-prime), the advance, stages 1..5
+// (skipped by rejected meshes) and the step end.  The host replays until every mesh has
+// made its accepted steps, reading the control blocks back once per batch of replays.
 //
 // A mesh may be a layer stack (bm_layer_desc: its nz planes are 2D layers coupled by a
 // multilayered demag, per-layer Ms, A, alpha, K1, as bm_ode_set_layers): its x and y passes
@@ -64,7 +71,11 @@ struct bm_batch_s {
 	float* M = nullptr;            // caller-owned, concatenated
 	float* Mb = nullptr;
 	float* sM1 = nullptr;
-	float* E[3] = { nullptr, nullptr, nullptr };
+	float* E[6] = { nullptr, nullptr, nullptr, nullptr, nullptr, nullptr };
+	// RKDP54: per-mesh step control (device) and its host copy (pinned, refreshed by runs)
+	BatchCtl* d_ctl = nullptr;
+	BatchCtl* h_ctl = nullptr;
+	unsigned* d_lte = nullptr;
 	float* Hd = nullptr;           // planar per mesh: 3 * ncell[b] at 3 * off[b]
 	// pass groups and their tables
 	std::vector<Group> gx, gy, gz, gy2, gml;
@@ -100,7 +111,10 @@ void free_all(bm_batch_s* o)
 	if (o->d_dT) (void)hipFree(o->d_dT);
 	if (o->Mb) (void)hipFree(o->Mb);
 	if (o->sM1) (void)hipFree(o->sM1);
-	for (int q = 0; q < 3; q++) if (o->E[q]) (void)hipFree(o->E[q]);
+	for (int q = 0; q < 6; q++) if (o->E[q]) (void)hipFree(o->E[q]);
+	if (o->d_ctl) (void)hipFree(o->d_ctl);
+	if (o->h_ctl) (void)hipHostFree(o->h_ctl);
+	if (o->d_lte) (void)hipFree(o->d_lte);
 	if (o->Hd) (void)hipFree(o->Hd);
 	if (o->dev_tables) (void)hipFree(o->dev_tables);
 }
@@ -122,7 +136,7 @@ StageArgs stage_args(bm_batch_s* o, int b, int parity)
 	a.Mout = parity ? Ma : Mbb;
 	a.Hd = o->Hd + off3;
 	a.sM1 = o->sM1 + off3;
-	for (int q = 0; q < 3; q++) a.E[q] = o->E[q] ? o->E[q] + off3 : nullptr;
+	for (int q = 0; q < 6; q++) a.E[q] = o->E[q] ? o->E[q] + off3 : nullptr;
 	a.nx = o->nx[b]; a.ny = o->ny[b]; a.nz = o->nz[b];
 	const double* h = &o->h[3 * b];
 	a.ihx2 = (float)(1.0 / (h[0] * h[0])); a.ihy2 = (float)(1.0 / (h[1] * h[1])); a.ihz2 = (float)(1.0 / (h[2] * h[2]));
@@ -132,10 +146,15 @@ StageArgs stage_args(bm_batch_s* o, int b, int parity)
 	a.dT = (float)o->dT;
 	a.modules = p.modules;
 	a.map = 2;
+	if (o->d_ctl) {
+		a.ctl = o->d_ctl + b;
+		a.lte = o->d_lte + b;
+		a.dT = (float)o->h_ctl[b].dT;
+	}
 	if ((p.modules & BM_MOD_ZEEMAN) && (p.wave == 1 || p.wave == 2)) {
-		a.tnow = o->d_time; a.wave = p.wave;
-		a.amp = (float)p.amp; a.w = (float)p.w; a.t0 = (float)p.t0;
-		for (int c = 0; c < 3; c++) { a.dir[c] = (float)p.dir[c]; a.slope[c] = (float)p.slope[c]; }
+		a.tnow = o->d_ctl ? &o->d_ctl[b].time : o->d_time; a.wave = p.wave;
+		a.amp = p.amp; a.w = p.w; a.t0 = p.t0;
+		for (int c = 0; c < 3; c++) { a.Ha0[c] = p.Ha[c]; a.dir[c] = p.dir[c]; a.slope[c] = p.slope[c]; }
 	}
 	if (o->ml[b]) {
 		a.layered = 1;
@@ -151,6 +170,78 @@ StageArgs stage_args(bm_batch_s* o, int b, int parity)
 }
 
 __global__ void k_time_advance(double* t, const double* dT) { *t += *dT; }
+
+// RKDP54 step control after stage 0, one thread per mesh: the plain integrator's stage-0
+// branch (primed check, SetAdaptiveTimeStep, restore on rejection) on the device.  The new
+// dT goes into both stage tables (read in place by the stage kernels of this replay).
+__global__ void k_rkdp_control(BatchCtl* ctl, unsigned* lte, StageArgs* t0, StageArgs* t1, int nb)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= nb) return;
+	BatchCtl& c = ctl[b];
+	if (!c.active) { c.skip = 1; return; }
+	c.skip = 0;
+	if (c.primed) {
+		c.lte = (double)__uint_as_float(lte[b]);
+		if (c.dT_min != c.dT_max) {
+			if (c.lte > c.err_high_fail && c.dT > c.dT_min) {
+				c.time -= c.dT; c.iteration--;
+				double f = pow(c.err_high_fail * 0.8 / c.lte, 1.0 / (c.order + 1));
+				if (f < 0.01) f = 0.01;
+				c.dT *= f;
+				if (c.dT < c.dT_min) c.dT = c.dT_min;
+				c.rejections++;
+				c.primed = 0;
+				c.skip = 1;
+			}
+			else if (c.lte > 0) {
+				double f = pow(c.err_high_fail * 0.8 / c.lte, 1.0 / (c.order + 1));
+				if (f > c.dT_increase) f = c.dT_increase;
+				if (f < 0.01) f = 0.01;
+				c.dT *= f;
+				if (c.dT < c.dT_min) c.dT = c.dT_min;
+				if (c.dT > c.dT_max) c.dT = c.dT_max;
+			}
+		}
+	}
+	else c.primed = 1;
+	lte[b] = 0u;
+	t0[b].dT = (float)c.dT;
+	t1[b].dT = (float)c.dT;
+}
+
+// After the control: an accepted mesh advances (sM1 = M, M1 = M + E0 dT/5 into the second
+// array, which stage 1 reads); a rejected one restores M = sM1.  Grid rows are meshes.
+__global__ void __launch_bounds__(kBS) k_batch_rkdp_advance(const StageArgs* __restrict__ tab, const BatchCtl* __restrict__ ctl)
+{
+	const StageArgs& a = tab[blockIdx.y];
+	const BatchCtl& c = ctl[blockIdx.y];
+	if (!c.active) return;
+	const size_t n3 = (size_t)a.nx * a.ny * a.nz * 3;
+	float* M = const_cast<float*>(a.Min);
+	if (c.skip) {
+		for (size_t q = blockIdx.x * (size_t)kBS + threadIdx.x; q < n3; q += (size_t)gridDim.x * kBS) M[q] = a.sM1[q];
+		return;
+	}
+	const float dT5 = (float)c.dT / 5;
+	for (size_t q = blockIdx.x * (size_t)kBS + threadIdx.x; q < n3; q += (size_t)gridDim.x * kBS) {
+		const float m = M[q];
+		a.sM1[q] = m;
+		a.Mout[q] = m + a.E[0][q] * dT5;
+	}
+}
+
+// End of an attempt: an accepted step advances the mesh's time and iteration.
+__global__ void k_rkdp_end(BatchCtl* ctl, int nb)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= nb) return;
+	BatchCtl& c = ctl[b];
+	if (!c.active || c.skip) return;
+	c.time += c.dT;
+	c.iteration++;
+	if (c.iteration >= c.target) c.active = 0;
+}
 
 // Group the meshes by pass size.
 void build_groups(bm_batch_s* o)
@@ -365,6 +456,18 @@ int autotune(bm_batch_s* o, hipStream_t st)
 
 int one_step(bm_batch_s* o, hipStream_t st)
 {
+	if (o->mcode == M_RKDP) {
+		const int nbk = (o->nb + 63) / 64;
+		int rc = eval_all(o, 0, 0, st);
+		if (rc) return rc;
+		hipLaunchKernelGGL(k_rkdp_control, dim3(nbk), dim3(64), 0, st, o->d_ctl, o->d_lte, o->stage[0].d, o->stage[1].d, o->nb);
+		const size_t nch = (o->max_cells * 3 + kBS - 1) / kBS;
+		const int gx = (int)std::max<size_t>(1, std::min<size_t>(nch, (size_t)o->ncu * 4));
+		hipLaunchKernelGGL(k_batch_rkdp_advance, dim3(gx, o->nb), dim3(kBS), 0, st, (const StageArgs*)o->stage[0].d, (const BatchCtl*)o->d_ctl);
+		for (int s = 1; s < 6; s++) if ((rc = eval_all(o, s, s & 1, st))) return rc;
+		hipLaunchKernelGGL(k_rkdp_end, dim3(nbk), dim3(64), 0, st, o->d_ctl, o->nb);
+		return BM_OK;
+	}
 	for (int s = 0; s < o->nstages; s++) {
 		int rc = eval_all(o, s, s & 1, st);
 		if (rc) return rc;
@@ -393,7 +496,8 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 	switch (method) {
 	case BM_EVAL_RK4: o->mcode = M_RK4; o->nstages = 4; o->dT = 0.5e-12; break;
 	case BM_EVAL_TEULER: o->mcode = M_TEULER; o->nstages = 2; o->dT = 0.5e-13; break;
-	default: delete o; return fail(BM_ERR_ARG, "bm_batch_create: fixed-step methods only (RK4, TEuler)");
+	case BM_EVAL_RKDP54: o->mcode = M_RKDP; o->nstages = 6; o->dT = 0.5e-12; break;
+	default: delete o; return fail(BM_ERR_ARG, "bm_batch_create: RKDP54, RK4 or TEuler");
 	}
 	o->M = M;
 	o->ncu = device_cu_count();
@@ -446,9 +550,24 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 	const size_t b3 = o->ntot * 3 * sizeof(float);
 	if (rc == BM_OK) {
 		bool ok = hipMalloc(&o->Mb, b3) == hipSuccess && hipMalloc(&o->sM1, b3) == hipSuccess && hipMalloc(&o->Hd, b3) == hipSuccess;
-		const int nE = o->mcode == M_RK4 ? 3 : 0;
-		for (int q = 0; q < nE && ok; q++) ok = hipMalloc(&o->E[q], b3) == hipSuccess;
-		if (ok) ok = hipMemset(o->Hd, 0, b3) == hipSuccess;
+		const int nE = o->mcode == M_RK4 ? 3 : o->mcode == M_RKDP ? 6 : 0;
+		for (int q = 0; q < nE && ok; q++) ok = hipMalloc(&o->E[q], b3) == hipSuccess && hipMemset(o->E[q], 0, b3) == hipSuccess;
+		if (ok) ok = hipMemset(o->Hd, 0, b3) == hipSuccess && hipMemset(o->sM1, 0, b3) == hipSuccess;
+		if (ok && o->mcode == M_RKDP) {
+			// the reference's RKDP54 defaults (Boris/DiffEq_CommonBase_Control.cpp:10-140)
+			ok = hipHostMalloc(&o->h_ctl, nmesh * sizeof(BatchCtl)) == hipSuccess
+				&& hipMalloc(&o->d_ctl, nmesh * sizeof(BatchCtl)) == hipSuccess
+				&& hipMalloc(&o->d_lte, nmesh * sizeof(unsigned)) == hipSuccess
+				&& hipMemset(o->d_lte, 0, nmesh * sizeof(unsigned)) == hipSuccess;
+			if (ok) {
+				for (int b = 0; b < nmesh; b++) {
+					BatchCtl& c = o->h_ctl[b];
+					std::memset(&c, 0, sizeof(c));
+					c.dT = o->dT; c.err_high_fail = 1e-4; c.dT_increase = 2; c.dT_min = 1e-15; c.dT_max = 3e-12; c.order = 5;
+				}
+				ok = hipMemcpy(o->d_ctl, o->h_ctl, nmesh * sizeof(BatchCtl), hipMemcpyHostToDevice) == hipSuccess;
+			}
+		}
 		if (!ok) rc = fail(BM_ERR_NOMEM, "bm_batch_create: device arrays");
 	}
 	if (rc == BM_OK) { build_groups(o); rc = build_tables(o); }
@@ -470,28 +589,75 @@ extern "C" int bm_batch_destroy(bm_batch_t o)
 	return BM_OK;
 }
 
+// host copy of the control blocks -> device, and their dT into the stage tables
+static int upload_ctl(bm_batch_s* o)
+{
+	BM_HIP(hipMemcpy(o->d_ctl, o->h_ctl, o->nb * sizeof(BatchCtl), hipMemcpyHostToDevice));
+	for (int par = 0; par < 2; par++) {
+		for (int b = 0; b < o->nb; b++) o->stage[par].h[b].dT = (float)o->h_ctl[b].dT;
+		BM_HIP(hipMemcpy(o->stage[par].d, o->stage[par].h.data(), o->stage[par].h.size() * sizeof(StageArgs), hipMemcpyHostToDevice));
+	}
+	return BM_OK;
+}
+
+static int fetch_ctl(bm_batch_s* o, hipStream_t st)
+{
+	BM_HIP(hipMemcpyAsync(o->h_ctl, o->d_ctl, o->nb * sizeof(BatchCtl), hipMemcpyDeviceToHost, st));
+	BM_HIP(hipStreamSynchronize(st));
+	return BM_OK;
+}
+
 extern "C" int bm_batch_set_dt(bm_batch_t o, double dT)
 {
 	if (!o || !(dT > 0)) return fail(BM_ERR_ARG, "bm_batch_set_dt: bad arguments");
 	o->dT = dT;
+	if (o->mcode == M_RKDP) {
+		BM_HIP(hipDeviceSynchronize());
+		for (int b = 0; b < o->nb; b++) o->h_ctl[b].dT = dT;
+		return upload_ctl(o);
+	}
 	return upload_dt(o);
 }
 
-extern "C" int bm_batch_run(bm_batch_t o, long long nsteps, void* stream)
+extern "C" int bm_batch_set_astep(bm_batch_t o, double err_fail, double dT_increase, double dT_min, double dT_max)
 {
-	if (!o || nsteps < 0) return fail(BM_ERR_ARG, "bm_batch_run: bad arguments");
-	hipStream_t st = (hipStream_t)stream;
-	int rc;
-	if (o->autotune && !o->tuned && nsteps > 0) {
-		if (o->exec) { (void)hipGraphExecDestroy(o->exec); o->exec = nullptr; }
-		if (o->graph) { (void)hipGraphDestroy(o->graph); o->graph = nullptr; }
-		if ((rc = autotune(o, st))) return rc;
+	if (!o) return fail(BM_ERR_ARG, "null handle");
+	if (o->mcode != M_RKDP) return fail(BM_ERR_STATE, "bm_batch_set_astep: adaptive control is for RKDP54 batches");
+	BM_HIP(hipDeviceSynchronize());
+	for (int b = 0; b < o->nb; b++) {
+		BatchCtl& c = o->h_ctl[b];
+		if (err_fail > 0) c.err_high_fail = err_fail;
+		if (dT_increase > 0) c.dT_increase = dT_increase;
+		if (dT_min > 0) c.dT_min = dT_min;
+		if (dT_max > 0) c.dT_max = dT_max;
 	}
+	return upload_ctl(o);
+}
+
+extern "C" int bm_batch_get_mesh_state(bm_batch_t o, int mesh, double* time, double* dT, long long* iteration, long long* rejections)
+{
+	if (!o || mesh < 0 || mesh >= o->nb) return fail(BM_ERR_ARG, "bm_batch_get_mesh_state: bad mesh");
+	if (o->mcode != M_RKDP) {
+		if (time) *time = o->time;
+		if (dT) *dT = o->dT;
+		if (iteration) *iteration = o->iteration;
+		if (rejections) *rejections = 0;
+		return BM_OK;
+	}
+	const BatchCtl& c = o->h_ctl[mesh];
+	if (time) *time = c.time;
+	if (dT) *dT = c.dT;
+	if (iteration) *iteration = c.iteration;
+	if (rejections) *rejections = c.rejections;
+	return BM_OK;
+}
+
+// k replays of one step (one attempt per mesh for RKDP54): captured graph or direct launches
+static int replay(bm_batch_s* o, long long k, hipStream_t st)
+{
+	int rc;
 	if (!o->use_graph) {
-		for (long long k = 0; k < nsteps; k++) {
-			if ((rc = one_step(o, st))) return rc;
-			o->time += o->dT; o->iteration++;
-		}
+		for (long long q = 0; q < k; q++) if ((rc = one_step(o, st))) return rc;
 		BM_HIP(hipGetLastError());
 		return BM_OK;
 	}
@@ -508,12 +674,50 @@ extern "C" int bm_batch_run(bm_batch_t o, long long nsteps, void* stream)
 	}
 	BM_HIP(hipEventRecord(o->ev_in, st));
 	BM_HIP(hipStreamWaitEvent(o->gs, o->ev_in, 0));
-	for (long long k = 0; k < nsteps; k++) {
-		BM_HIP(hipGraphLaunch(o->exec, o->gs));
-		o->time += o->dT; o->iteration++;
-	}
+	for (long long q = 0; q < k; q++) BM_HIP(hipGraphLaunch(o->exec, o->gs));
 	BM_HIP(hipEventRecord(o->ev_out, o->gs));
 	BM_HIP(hipStreamWaitEvent(st, o->ev_out, 0));
+	return BM_OK;
+}
+
+// RKDP54: every mesh makes nsteps more accepted steps.  Attempts that a mesh rejects cost
+// it a replay, so after the first nsteps replays the control blocks are read back and the
+// replays repeat for the largest remaining count until every mesh is done.
+static int run_adaptive(bm_batch_s* o, long long nsteps, hipStream_t st)
+{
+	if (nsteps == 0) return BM_OK;
+	for (int b = 0; b < o->nb; b++) { o->h_ctl[b].target = o->h_ctl[b].iteration + nsteps; o->h_ctl[b].active = 1; }
+	BM_HIP(hipMemcpyAsync(o->d_ctl, o->h_ctl, o->nb * sizeof(BatchCtl), hipMemcpyHostToDevice, st));
+	long long remaining = nsteps, replays = 0;
+	const long long cap = 64 * nsteps + 4096;     // each rejection shrinks dT by >= 100x towards dT_min
+	int rc;
+	while (remaining > 0) {
+		if (replays > cap) return fail(BM_ERR_STATE, "bm_batch_run: step control did not converge");
+		if ((rc = replay(o, remaining, st))) return rc;
+		replays += remaining;
+		if ((rc = fetch_ctl(o, st))) return rc;
+		remaining = 0;
+		for (int b = 0; b < o->nb; b++) remaining = std::max(remaining, o->h_ctl[b].target - o->h_ctl[b].iteration);
+	}
+	o->iteration += nsteps;
+	o->time = o->h_ctl[0].time;
+	o->dT = o->h_ctl[0].dT;
+	return BM_OK;
+}
+
+extern "C" int bm_batch_run(bm_batch_t o, long long nsteps, void* stream)
+{
+	if (!o || nsteps < 0) return fail(BM_ERR_ARG, "bm_batch_run: bad arguments");
+	hipStream_t st = (hipStream_t)stream;
+	int rc;
+	if (o->autotune && !o->tuned && nsteps > 0) {
+		if (o->exec) { (void)hipGraphExecDestroy(o->exec); o->exec = nullptr; }
+		if (o->graph) { (void)hipGraphDestroy(o->graph); o->graph = nullptr; }
+		if ((rc = autotune(o, st))) return rc;
+	}
+	if (o->mcode == M_RKDP) return run_adaptive(o, nsteps, st);
+	if ((rc = replay(o, nsteps, st))) return rc;
+	for (long long k = 0; k < nsteps; k++) { o->time += o->dT; o->iteration++; }
 	return BM_OK;
 }
 

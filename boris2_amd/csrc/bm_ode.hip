@@ -30,6 +30,7 @@ using namespace bm;
 template <int METHOD, int STEP>
 __device__ __forceinline__ void stage_sweep(const StageArgs& a)
 {
+	if (a.ctl && (STEP == 0 ? !a.ctl->active : a.ctl->skip)) return;   // uniform over the block
 	const F3 Ha = stage_Ha(a);
 	const size_t n = (size_t)a.nx * a.ny * a.nz;
 	const long long nchunk = (long long)((n + kBS - 1) / kBS);
@@ -261,13 +262,19 @@ int bm::batch_stage(int mcode, int step, const StageArgs* d_tab, int nb, size_t 
 	const dim3 g(gx, nb), b(kBS);
 	StageArgs a{};
 	switch (mcode * 8 + step) {
+	case M_RKDP * 8 + 0: hipLaunchKernelGGL((k_stage<M_RKDP, 0>), g, b, 0, st, a, d_tab); break;
+	case M_RKDP * 8 + 1: hipLaunchKernelGGL((k_stage<M_RKDP, 1>), g, b, 0, st, a, d_tab); break;
+	case M_RKDP * 8 + 2: hipLaunchKernelGGL((k_stage<M_RKDP, 2>), g, b, 0, st, a, d_tab); break;
+	case M_RKDP * 8 + 3: hipLaunchKernelGGL((k_stage<M_RKDP, 3>), g, b, 0, st, a, d_tab); break;
+	case M_RKDP * 8 + 4: hipLaunchKernelGGL((k_stage<M_RKDP, 4>), g, b, 0, st, a, d_tab); break;
+	case M_RKDP * 8 + 5: hipLaunchKernelGGL((k_stage<M_RKDP, 5>), g, b, 0, st, a, d_tab); break;
 	case M_RK4 * 8 + 0: hipLaunchKernelGGL((k_stage<M_RK4, 0>), g, b, 0, st, a, d_tab); break;
 	case M_RK4 * 8 + 1: hipLaunchKernelGGL((k_stage<M_RK4, 1>), g, b, 0, st, a, d_tab); break;
 	case M_RK4 * 8 + 2: hipLaunchKernelGGL((k_stage<M_RK4, 2>), g, b, 0, st, a, d_tab); break;
 	case M_RK4 * 8 + 3: hipLaunchKernelGGL((k_stage<M_RK4, 3>), g, b, 0, st, a, d_tab); break;
 	case M_TEULER * 8 + 0: hipLaunchKernelGGL((k_stage<M_TEULER, 0>), g, b, 0, st, a, d_tab); break;
 	case M_TEULER * 8 + 1: hipLaunchKernelGGL((k_stage<M_TEULER, 1>), g, b, 0, st, a, d_tab); break;
-	default: return bm::fail(BM_ERR_ARG, "batched stage: fixed-step methods only (RK4, TEuler)");
+	default: return bm::fail(BM_ERR_ARG, "batched stage: RKDP54, RK4 or TEuler");
 	}
 	return BM_OK;
 }

@@ -65,6 +65,19 @@ __device__ __forceinline__ float rel_err(P6 m, P6 p) { return fmaxf(rel_err(m.a,
 
 constexpr int kMaxStackLayers = 8;
 
+// Per-mesh adaptive step control of a batched RKDP54 integration, kept on the device so a
+// captured step graph runs it (ODECommon_Base's time, dT, lte, iteration, primed state and
+// SetAdaptiveTimeStep controls, Boris/DiffEq_CommonBase.h, Boris/DiffEq_CommonBase_Iterate.cpp:15-59).
+// `time` comes first: the stage kernels read it as the step start time of time-dependent fields.
+struct BatchCtl {
+	double time, dT, lte;
+	double err_high_fail, dT_increase, dT_min, dT_max;
+	long long iteration, target, rejections;
+	int order, primed;
+	int active;   // the mesh still has accepted steps to make in this run
+	int skip;     // this attempt was rejected (or the mesh is inactive): no stages 1..5
+};
+
 struct StageArgs {
 	const float* Min;       // magnetisation this stage evaluates (neighbours read)
 	float* Mout;            // magnetisation of the next stage (never the same array)
@@ -86,10 +99,11 @@ struct StageArgs {
 	int layered;
 	float lay_A[kMaxStackLayers], lay_alpha[kMaxStackLayers], lay_K1[kMaxStackLayers];
 	// time-dependent applied field read on the device (batched graphs): with tnow set, the
-	// stage kernel adds amp dir sinc(w (t - t0)) (wave 1) or slope t (wave 2) to Ha, t = *tnow
+	// stage kernel evaluates Ha0 + amp dir sinc(w (t - t0)) (wave 1) or Ha0 + slope t (wave 2)
+	// at t = *tnow in FP64 and rounds it to FP32, as the host does for a single mesh
 	const double* tnow;
 	int wave;
-	float amp, w, t0, dir[3], slope[3];
+	double Ha0[3], amp, w, t0, dir[3], slope[3];
 	// demag evaluation speedup (Boris/DemagCUDA.cpp:106-468, kernels Boris/DemagCUDA.cu:44-413):
 	// with nsp > 0 the demag field of a cell is sum_{s<nsp} ca[s] Hs[s] + selfc * M (planar
 	// saved fields, AoS M of this stage); with Hsave set, the stage also stores Hd - selfc * M
@@ -98,22 +112,25 @@ struct StageArgs {
 	float* Hsave;
 	float ca[6], selfc[3];
 	int nsp;
+	// batched adaptive integration: the mesh's step control; stage 0 runs while the mesh is
+	// active, stages 1..5 unless the attempt was rejected
+	const BatchCtl* ctl;
 };
 
 // Applied field of a stage: Ha, plus the time-dependent term at the device-held step time
 // when tnow is set (see StageArgs::tnow).
 __device__ __forceinline__ F3 stage_Ha(const StageArgs& a)
 {
-	F3 H{ a.Ha[0], a.Ha[1], a.Ha[2] };
-	if (!a.tnow) return H;
+	if (!a.tnow) return F3{ a.Ha[0], a.Ha[1], a.Ha[2] };
 	const double t = *a.tnow;
+	double H[3] = { a.Ha0[0], a.Ha0[1], a.Ha0[2] };
 	if (a.wave == 1) {
-		const double x = (double)a.w * (t - (double)a.t0);
-		const float sc = (fabs(x) < 1e-300) ? 1.f : (float)(sin(x) / x);
-		H = H + (a.amp * sc) * F3{ a.dir[0], a.dir[1], a.dir[2] };
+		const double x = a.w * (t - a.t0);
+		const double sc = (fabs(x) < 1e-300) ? 1.0 : sin(x) / x;
+		for (int c = 0; c < 3; c++) H[c] += a.amp * a.dir[c] * sc;
 	}
-	else if (a.wave == 2) H = H + (float)t * F3{ a.slope[0], a.slope[1], a.slope[2] };
-	return H;
+	else if (a.wave == 2) for (int c = 0; c < 3; c++) H[c] += a.slope[c] * t;
+	return F3{ (float)H[0], (float)H[1], (float)H[2] };
 }
 
 // Material constants of a cell: the mesh's, or those of its layer in a layer stack (the

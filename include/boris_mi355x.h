@@ -191,13 +191,15 @@ int bm_ode_copy_heff(bm_ode_t o, float* Heff_out, void* stream);
 int bm_ode_average_m(bm_ode_t o, double* out3, void* stream);
 
 /* ------------------------------------------------------------------------------------
- * Batched fixed-step integration of many independent meshes of uneven sizes (no
- * reference counterpart: the reference integrates one mesh set per simulation).  Mesh b
- * has n3[3b..3b+2] cells of h3[3b..3b+2] (m) and its own parameters params[b]; M is the
+ * Batched integration of many independent meshes of uneven sizes (no reference
+ * counterpart: the reference integrates one mesh set per simulation).  Mesh b has
+ * n3[3b..3b+2] cells of h3[3b..3b+2] (m) and its own parameters params[b]; M is the
  * caller's device array holding every mesh's magnetisation concatenated in mesh order
  * (bm_batch_mesh_offset gives each mesh's cell offset).  Methods: BM_EVAL_RK4,
- * BM_EVAL_TEULER (fixed step; applied fields constant, sinc pulse or ramp, evaluated at each
- * step's start time on the device).  Every pass of a field evaluation is one launch per
+ * BM_EVAL_TEULER (fixed step) and BM_EVAL_RKDP54 (every mesh with its own adaptive step,
+ * controlled on the device as ODECommon_Base::Iterate does for one mesh,
+ * Boris/DiffEq_CommonBase_Iterate.cpp:15-59,1214-1395); applied fields constant, sinc
+ * pulse or ramp, evaluated at each step's start time on the device.  Every pass of a field evaluation is one launch per
  * distinct pass size over all meshes of that size; a whole step is captured once into a
  * hipGraph and replayed (BM_BATCH_NO_GRAPH: launch directly).  BM_BATCH_AUTOTUNE: before
  * the first step, time candidate launch shapes of every pass-size group (x-pass row-tile
@@ -219,9 +221,17 @@ int bm_batch_create_layers(int method, int nmesh, const int* n3, const double* h
 	const bm_layer_desc* layers, float* M, int flags, bm_batch_t* out);
 int bm_batch_destroy(bm_batch_t b);
 int bm_batch_set_dt(bm_batch_t b, double dT);
-/* nsteps fixed steps of every mesh, ordered after earlier work on `stream`. */
+/* nsteps steps of every mesh (RKDP54: nsteps accepted steps of each mesh; rejected attempts
+ * are redone), ordered after earlier work on `stream`. */
 int bm_batch_run(bm_batch_t b, long long nsteps, void* stream);
 int bm_batch_get_state(bm_batch_t b, double* time, long long* iteration);
+/* RKDP54 batches: the adaptive controls err_fail, dT_increase, dT_min, dT_max of every mesh
+ * (values <= 0 keep the current ones; defaults are the reference's RKDP54 defaults).
+ * Replaces ODECommon_Base::SetAdaptiveTimeStepCtrl (Boris/DiffEq_CommonBase_Control.cpp:258). */
+int bm_batch_set_astep(bm_batch_t b, double err_fail, double dT_increase, double dT_min, double dT_max);
+/* One mesh's time, dT, accepted iterations and rejected attempts after the last run
+ * (fixed-step batches: the common values, no rejections). */
+int bm_batch_get_mesh_state(bm_batch_t b, int mesh, double* time, double* dT, long long* iteration, long long* rejections);
 int bm_batch_mesh_offset(bm_batch_t b, int mesh, long long* cell_offset, long long* ncells);
 int bm_batch_autotune_report(bm_batch_t b, char* out, int cap);
 

@@ -110,6 +110,30 @@ def test_3d_run_matches_oracle(gpu, path, method):
     assert rel_l2(s.M.cpu().numpy(), o.get_M()) <= 1e-4
 
 
+@pytest.mark.parametrize("wave", ["sinc", "ramp"])
+@pytest.mark.parametrize("method", [api.BM_EVAL_RKDP54, api.BM_EVAL_RK4])
+def test_time_dependent_field_matches_oracle(gpu, path, method, wave):
+    """Zeeman waveforms evaluated at each step's start time (Boris/Zeeman.cpp:164-300): a
+    20 mT sinc pulse of 50 ps centred at 0.1 ns, or a field ramp, on the small film."""
+    n, h = SP4_N, SP4_H
+    p = sp4_params()
+    if wave == "sinc":
+        p.wave, p.amp, p.w, p.t0, p.dir = 1, 20e-3 / api.MU0, np.pi / 50e-12, 0.1e-9, (0.6, 0.0, 0.8)
+    else:
+        p.wave, p.slope = 2, (2e14, -1e14, 5e13)
+    ncell = n[0] * n[1] * n[2]
+    M0 = tilted_x(ncell, p.Ms, 13)
+    s = api.FMSolver(method, n, h, p, M0)
+    o = O.OracleSim(method, n, h, p)
+    o.set_M(M0.astype(np.float64))
+    steps = 60
+    s.run_to_iteration(steps)
+    o.run_to_iteration(steps)
+    assert abs(s.state()["time"] - o.state()["time"]) <= 1e-3 * o.state()["time"]
+    assert np.abs(s.average_m() - o.average_m()).max() <= 1e-4
+    assert rel_l2(s.M.cpu().numpy(), o.get_M()) <= 1e-4
+
+
 def test_rkdp54_rejections_match_oracle(gpu, path):
     """A 0.2 rad random cone about x under a 5e5 J/m^3 z anisotropy makes the default first
     steps too long: rejected steps (restore of M, re-primed FSAL) in both."""
@@ -159,12 +183,16 @@ def test_sp4_2000_step_trajectory(gpu):
     s = api.FMSolver(api.BM_EVAL_RKDP54, n, h, p, M0)
     o = O.OracleSim(api.BM_EVAL_RKDP54, n, h, p)
     o.set_M(M0.astype(np.float64))
-    for x in (s, o):
+    o32 = O.OracleSim(api.BM_EVAL_RKDP54, n, h, p, fp32=True)     # the same algorithm in FP32 on the CPU
+    o32.set_M(M0.astype(np.float64))
+    for x in (s, o, o32):
         x.set_astep(1e-5)
-    worst = 0.0
+    worst = worst32 = 0.0
     for it in range(100, 2001, 100):
-        s.run_to_iteration(it)
-        o.run_to_iteration(it)
+        for x in (s, o, o32):
+            x.run_to_iteration(it)
         worst = max(worst, float(np.abs(s.average_m() - o.average_m()).max()))
+        worst32 = max(worst32, float(np.abs(s.average_m() - o32.average_m()).max()))
     assert worst <= 1e-4, worst
+    assert worst32 <= 1e-4, worst32
     assert abs(s.state()["time"] - o.state()["time"]) <= 1e-3 * o.state()["time"]
