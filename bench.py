@@ -19,7 +19,8 @@ The JSON line also carries:
                  workload on this machine's host cores;
   eval_speedup — the same workload with the demag evaluation speedup (saved demag fields
                  extrapolated in time, one convolution per step);
-  batched      — 64 films of uneven sizes, batched RK4 vs the same films one by one;
+  batched      — 64 films of uneven sizes, batched RK4 vs the same films one by one, and
+                 the same with RKDP54 (per-film adaptive steps controlled on the device);
   adjoint      — 128x128x4 RK4 rollout with its discrete adjoint: backward / forward cost;
   multilayer   — 16 stacks of 3 layers of 256x256 cells, batched RK4 vs one by one, and one
                  stack's adjoint (per-layer Ms and field-ramp slope gradients).
@@ -228,6 +229,45 @@ def batch_run(steps: int):
             "autotune_s": round(t_tune, 2), "autotuned_passes": tuned}
 
 
+def batch_rkdp_run(steps: int, warm: int = 5):
+    """The 64 films with RKDP54 (each its own adaptive step, the reference's default control):
+    batched with the step control on the device vs the same films one by one on the plain
+    path (host step control, one lte read-back per step); `steps` accepted steps of every
+    film after `warm` warm-up steps (the first steps from random states are rejected and
+    redone)."""
+    import torch
+    from boris2_amd import api
+    meshes = batched_meshes()
+    cells = sum(n[0] * n[1] * n[2] for n, _, _, _ in meshes)
+    bs = api.BatchSolver(api.BM_EVAL_RKDP54, [(n, h, p) for n, h, p, _ in meshes], [M0 for *_, M0 in meshes], autotune=True)
+    bs.run(warm)
+    torch.cuda.synchronize()
+    rej0 = sum(bs.mesh_state(b)["rejections"] for b in range(len(meshes)))
+    t0 = time.perf_counter()
+    bs.run(steps)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    rej = sum(bs.mesh_state(b)["rejections"] for b in range(len(meshes))) - rej0
+    bs.close()
+    solvers = []
+    for n, h, p, M0 in meshes:
+        s = api.FMSolver(api.BM_EVAL_RKDP54, n, h, p, M0)
+        s.run_to_iteration(warm)
+        solvers.append(s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.run_to_iteration(warm + steps)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    for s in solvers:
+        s.close()
+    return {"meshes": len(meshes), "cells": cells, "steps": steps, "method": "RKDP54 adaptive, per-mesh dT",
+            "rejected_attempts": rej, "batched_cell_updates_per_s": cells * steps / tb,
+            "plain_cell_updates_per_s": cells * steps / tp, "batched_over_plain": tp / tb,
+            "batched_ms_per_step": 1e3 * tb / steps, "plain_ms_per_step": 1e3 * tp / steps}
+
+
 def adjoint_run(steps: int, ckpt: int):
     """128x128x4 film of 4 nm cells, Ms map = sum of 8 Gaussian blobs in [5e5, 1.1e6] A/m,
     A 1.3e-11, alpha 0.05, 1 ns sinc pulse of 20 mT (seeded direction); RK4 dt 250 fs,
@@ -413,6 +453,8 @@ def main() -> int:
             "alg_bytes_per_launch": round(d["alg_GB_per_launch"] * 1e9), "avg_launch_ms": d["avg_ms"]}
     speedup = speedup_run(a.speedup_steps, a.warmup, a.speedup_mode, r) if a.speedup_steps > 0 else None
     batched = batch_run(a.batch_steps) if a.batch_steps > 0 else None
+    if batched is not None:
+        batched["rkdp54"] = batch_rkdp_run(a.batch_steps)
     adjoint = adjoint_run(a.adjoint_steps, 100) if a.adjoint_steps > 0 else None
     multilayer = multilayer_run(a.ml_steps, a.ml_adjoint_steps) if a.ml_steps > 0 else None
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_steps)
