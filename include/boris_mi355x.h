@@ -20,7 +20,7 @@ extern "C" {
 enum {
 	BM_OK = 0,
 	BM_ERR_ARG = 1,      /* invalid argument */
-	BM_ERR_SIZE = 2,     /* mesh size has no FFT plan in this build */
+	BM_ERR_SIZE = 2,     /* mesh size has no FFT plan in this library */
 	BM_ERR_NOMEM = 3,    /* device allocation failed (BERROR_OUTOFGPUMEMORY_CRIT) */
 	BM_ERR_HIP = 4,      /* HIP runtime error */
 	BM_ERR_STATE = 5     /* call not valid in the object's current state */

@@ -1,5 +1,5 @@
 """Writes the committed fixtures under tests/golden/ (run in the build container, where the
-reference checkout and oracle/_ref exist; the GPU box only reads the results).
+reference checkout and oracle/_ref exist; the GPU test runs only read the results).
 
 * vortex_hh_clk_up.npz — the reference-held OVF2 file Boris/vortex_hh_clk_up.ovf
   (512 x 128 x 1 cells of 5 x 5 x 40 nm, binary-8 data after the 123456789012345.0 check
