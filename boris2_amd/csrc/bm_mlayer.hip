@@ -423,6 +423,11 @@ static int ml_convolute(bm_mlayer_s* m, const float* M, float* out, int mode, hi
 int bm::mlayer_planar(bm_mlayer_s* m, const float* M, float* H3n, hipStream_t st) { return ml_convolute(m, M, H3n, 0, st); }
 int bm::mlayer_planar_transposed(bm_mlayer_s* m, const float* M, float* H3n, hipStream_t st) { return ml_convolute(m, M, H3n, 0, st, 1); }
 
+void bm::mlayer_cell(const bm_mlayer_s* m, int l, double h[3])
+{
+	h[0] = m->hx; h[1] = m->hy; h[2] = m->hz[(size_t)l];
+}
+
 bm::MlView bm::mlayer_view(bm_mlayer_s* m)
 {
 	return MlView{ m->Sx, m->S, m->K, m->L, m->nx, m->ny, m->Nx, m->Ny, m->NC, m->NYH, m->Sp, m->ntl };

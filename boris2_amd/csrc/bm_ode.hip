@@ -47,25 +47,33 @@ __device__ __forceinline__ void stage_sweep(const StageArgs& a)
 		if (idx >= n) continue;
 		const int i = (int)(idx % a.nx), j = (int)((idx / a.nx) % a.ny), k = (int)(idx / ((size_t)a.nx * a.ny));
 		F3 Hd{ 0.f, 0.f, 0.f };
-		if (a.nsp) {
-			// Demag_EvalSpeedup_AddExtrapField_AddSelf: saved fields in slot order, then self term
-			const F3 m = ld3(a.Min, idx);
-			Hd = a.ca[0] * F3{ a.Hs[0][idx], a.Hs[0][n + idx], a.Hs[0][2 * n + idx] };
+		if (a.nsp || a.Hsave) {
+			// self coefficient of the cell: the mesh's, or its layer's in a layer stack
+			// (compile-time indices keep StageArgs in registers)
+			F3 sc{ a.selfc[0], a.selfc[1], a.selfc[2] };
+			if (a.layered) {
 #pragma unroll
-			for (int q = 1; q < 6; q++)
-				if (q < a.nsp) Hd = Hd + a.ca[q] * F3{ a.Hs[q][idx], a.Hs[q][n + idx], a.Hs[q][2 * n + idx] };
-			Hd = Hd + F3{ a.selfc[0] * m.x, a.selfc[1] * m.y, a.selfc[2] * m.z };
-		}
-		else if (a.modules & BM_MOD_DEMAG) {
-			Hd = F3{ a.Hd[idx], a.Hd[n + idx], a.Hd[2 * n + idx] };
-			if (a.Hsave) {
+				for (int l = 0; l < kMaxStackLayers; l++)
+					if (k == l) sc = F3{ a.lay_selfc[l][0], a.lay_selfc[l][1], a.lay_selfc[l][2] };
+			}
+			const F3 m = ld3(a.Min, idx);
+			if (a.nsp) {
+				// Demag_EvalSpeedup_AddExtrapField_AddSelf: saved fields in slot order, then self term
+				Hd = a.ca[0] * F3{ a.Hs[0][idx], a.Hs[0][n + idx], a.Hs[0][2 * n + idx] };
+#pragma unroll
+				for (int q = 1; q < 6; q++)
+					if (q < a.nsp) Hd = Hd + a.ca[q] * F3{ a.Hs[q][idx], a.Hs[q][n + idx], a.Hs[q][2 * n + idx] };
+				Hd = Hd + F3{ sc.x * m.x, sc.y * m.y, sc.z * m.z };
+			}
+			else {
 				// Demag_EvalSpeedup_AddField_SubSelf: keep the convolution minus the self term
-				const F3 m = ld3(a.Min, idx);
-				a.Hsave[idx] = Hd.x - a.selfc[0] * m.x;
-				a.Hsave[n + idx] = Hd.y - a.selfc[1] * m.y;
-				a.Hsave[2 * n + idx] = Hd.z - a.selfc[2] * m.z;
+				Hd = F3{ a.Hd[idx], a.Hd[n + idx], a.Hd[2 * n + idx] };
+				a.Hsave[idx] = Hd.x - sc.x * m.x;
+				a.Hsave[n + idx] = Hd.y - sc.y * m.y;
+				a.Hsave[2 * n + idx] = Hd.z - sc.z * m.z;
 			}
 		}
+		else if (a.modules & BM_MOD_DEMAG) Hd = F3{ a.Hd[idx], a.Hd[n + idx], a.Hd[2 * n + idx] };
 		stage_cell<METHOD, STEP>(a, Ha, idx, i, j, k, Hd, lte_loc);
 	}
 	if (stage_has_lte<METHOD, STEP>()) block_max_to(lte_loc, a.lte);
@@ -149,6 +157,7 @@ struct bm_ode_s {
 	double tdm[6] = {};                   // save times
 	float* Hs[6] = {};                    // saved fields minus self term, planar
 	float selfc[3] = {};
+	float lay_selfc[bm::kMaxStackLayers][3] = {};
 	bool link_dTspeedup = true;
 	double dTspeedup = 0, time_speedup = 0;
 	long long convolutions = 0;
@@ -210,6 +219,7 @@ StageArgs make_args(bm_ode_s* o, float dT)
 		a.nsp = o->sp_n;
 		for (int q = 0; q < 6; q++) { a.Hs[q] = o->Hs[q]; a.ca[q] = o->sp_ca[q]; }
 		for (int c = 0; c < 3; c++) a.selfc[c] = o->selfc[c];
+		for (int l = 0; l < kMaxStackLayers; l++) for (int c = 0; c < 3; c++) a.lay_selfc[l][c] = o->lay_selfc[l][c];
 		a.Hsave = o->sp_save;
 	}
 	if (o->ml) {
@@ -353,8 +363,9 @@ int eval_speedup(bm_ode_s* o, int step, float dT, hipStream_t st)
 {
 	const int p = o->speedup;
 	int rc;
+	auto convolve = [&]() { return o->ml ? bm::mlayer_planar(o->ml, o->Mcur, o->Hd, st) : bm::demag_planar(o->demag, o->Mcur, o->Hd, st); };
 	if (o->num_saved < p && !check_step_update(o)) {
-		if ((rc = bm::demag_planar(o->demag, o->Mcur, o->Hd, st))) return rc;
+		if ((rc = convolve())) return rc;
 		o->convolutions++;
 		return launch_stage(o, step, dT, st);
 	}
@@ -362,7 +373,7 @@ int eval_speedup(bm_ode_s* o, int step, float dT, hipStream_t st)
 		int slot;
 		if (o->num_saved < p) { slot = o->num_saved; o->tdm[slot] = eval_step_time(o); o->num_saved++; }
 		else { slot = next_slot(o); if (p > 1) o->tdm[slot] = eval_step_time(o); }
-		if ((rc = bm::demag_planar(o->demag, o->Mcur, o->Hd, st))) return rc;
+		if ((rc = convolve())) return rc;
 		o->convolutions++;
 		o->sp_save = o->Hs[slot];
 		rc = launch_stage(o, step, dT, st);
@@ -383,6 +394,23 @@ int eval_speedup(bm_ode_s* o, int step, float dT, hipStream_t st)
 	return rc;
 }
 
+// Self-demag coefficients of the evaluation speedup: the mesh's (DemagTFunc::SelfDemag_PBC
+// without images, cell size in metres, Boris/DemagCUDA.cpp:37) and, for a layer stack, each
+// layer mesh's own (SDemagCUDA_Demag::selfDemagCoeff).
+void set_self_coeffs(bm_ode_s* o)
+{
+	double sd[3];
+	bm::self_demag_coeff(o->h[0], o->h[1], o->h[2], sd);
+	for (int c = 0; c < 3; c++) o->selfc[c] = (float)sd[c];
+	if (o->ml)
+		for (int l = 0; l < o->nz && l < kMaxStackLayers; l++) {
+			double hl[3];
+			bm::mlayer_cell(o->ml, l, hl);
+			bm::self_demag_coeff(hl[0], hl[1], hl[2], sd);
+			for (int c = 0; c < 3; c++) o->lay_selfc[l][c] = (float)sd[c];
+		}
+}
+
 // Demag evaluations go through the fused row kernel (single mesh, no speedup).
 bool row_path(const bm_ode_s* o)
 {
@@ -394,8 +422,8 @@ int evaluate(bm_ode_s* o, int step, float dT, hipStream_t st)
 {
 	o->evaluations++;
 	const bool demag = (o->p.modules & BM_MOD_DEMAG) != 0;
-	if (demag && o->ml) { int rc = bm::mlayer_planar(o->ml, o->Mcur, o->Hd, st); if (rc) return rc; o->convolutions++; return launch_stage(o, step, dT, st); }
 	if (demag && o->speedup) return eval_speedup(o, step, dT, st);
+	if (demag && o->ml) { int rc = bm::mlayer_planar(o->ml, o->Mcur, o->Hd, st); if (rc) return rc; o->convolutions++; return launch_stage(o, step, dT, st); }
 	if (row_path(o)) { o->convolutions++; return eval_row(o, step, dT, st); }
 	if (demag) { int rc = bm::demag_planar(o->demag, o->Mcur, o->Hd, st); if (rc) return rc; o->convolutions++; }
 	return launch_stage(o, step, dT, st);
@@ -615,6 +643,7 @@ extern "C" int bm_ode_set_layers(bm_ode_t o, bm_mlayer_t ml, const double* Ms, c
 		if ((o->p.modules & BM_MOD_DEMAG) && !o->demag) return fail(BM_ERR_STATE, "bm_ode_set_layers: demag is on and there is no other demag handle");
 		if (o->Ms_map == o->Ms_layers) o->Ms_map = nullptr;
 		o->ml = nullptr;
+		o->num_saved = 0;
 		return BM_OK;
 	}
 	int L, nx, ny;
@@ -640,6 +669,8 @@ extern "C" int bm_ode_set_layers(bm_ode_t o, bm_mlayer_t ml, const double* Ms, c
 	o->ml = ml;
 	o->use_row = false;
 	o->S_valid = false;
+	o->num_saved = 0;              // saved demag fields belong to the previous configuration
+	set_self_coeffs(o);
 	return BM_OK;
 }
 
@@ -661,7 +692,6 @@ extern "C" int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p)
 extern "C" int bm_ode_set_eval_speedup(bm_ode_t o, int mode, double dTspeedup)
 {
 	if (!o || mode < 0 || mode > 6) return fail(BM_ERR_ARG, "bm_ode_set_eval_speedup: mode must be 0..6");
-	if (mode && o->ml) return fail(BM_ERR_STATE, "bm_ode_set_eval_speedup: not available for layer stacks");
 	const size_t b3 = o->n * 3 * sizeof(float);
 	for (int q = 0; q < 6; q++) {
 		if (q >= mode && o->Hs[q]) { (void)hipFree(o->Hs[q]); o->Hs[q] = nullptr; }
@@ -675,9 +705,7 @@ extern "C" int bm_ode_set_eval_speedup(bm_ode_t o, int mode, double dTspeedup)
 	o->num_saved = 0;
 	for (int q = 0; q < 6; q++) o->tdm[q] = 0;
 	// DemagTFunc::SelfDemag_PBC without images, cell size in metres (Boris/DemagCUDA.cpp:37)
-	double sd[3];
-	bm::self_demag_coeff(o->h[0], o->h[1], o->h[2], sd);
-	for (int c = 0; c < 3; c++) o->selfc[c] = (float)sd[c];
+	set_self_coeffs(o);
 	if (dTspeedup > 0) { o->link_dTspeedup = false; o->dTspeedup = dTspeedup; o->time_speedup = o->time; }
 	else { o->link_dTspeedup = true; o->dTspeedup = o->dT; }
 	o->S_valid = false;

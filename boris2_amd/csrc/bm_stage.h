@@ -112,6 +112,7 @@ struct StageArgs {
 	float* Hsave;
 	float ca[6], selfc[3];
 	int nsp;
+	float lay_selfc[kMaxStackLayers][3];   // layer stacks: each layer's own self coefficient
 	// batched adaptive integration: the mesh's step control; stage 0 runs while the mesh is
 	// active, stages 1..5 unless the attempt was rejected
 	const BatchCtl* ctl;

@@ -158,8 +158,10 @@ int bm_ode_set_astep(bm_ode_t o, double err_fail, double dT_increase, double dT_
  * time when dTspeedup > 0) and extrapolated in time from the saved fields otherwise.
  * Replaces ODECommon_Base::SetEvaluationSpeedup / SetSpeedupTimeStep / SetLink_dTspeedup
  * (Boris/DiffEq_CommonBase.h:252, Boris/DiffEq_CommonBase_Control.cpp:243-256) with
- * DemagCUDA::UpdateField's speedup branch (Boris/DemagCUDA.cpp:106-468).  Single meshes
- * only (not layer stacks); allocates one field array per saved field; drops saved fields. */
+ * DemagCUDA::UpdateField's speedup branch (Boris/DemagCUDA.cpp:106-468); for a layer stack
+ * (bm_ode_set_layers) SDemagCUDA::UpdateField's multilayered speedup branch
+ * (Boris/SDemagCUDA.cpp:583-1000), each layer with its own self-demag coefficient.
+ * Allocates one field array per saved field; drops saved fields. */
 int bm_ode_set_eval_speedup(bm_ode_t o, int mode, double dTspeedup);
 /* Saved-field count, their save times (double[6]) and the number of convolutions run so
  * far (any pointer may be NULL).  DemagCUDA::num_Hdemag_saved / time_demag1..6

@@ -240,6 +240,7 @@ void or_sim_set_layers(void* h, int L, double hx, double hy, const double* hz, c
 	OrSim* o = (OrSim*)h;
 	SIM_DISPATCH(o, {
 		s->layered = true;
+		s->layHz.assign(hz, hz + L);
 		if (with_demag) {
 			typedef std::remove_reference<decltype(s->M[0])>::type RR;
 			std::shared_ptr<MLayer<RR>> ml(new MLayer<RR>(L, s->nx, s->ny, hx, hy, hz, z0));

@@ -470,6 +470,8 @@ struct Sim {
 	double tdm[6] = { 0, 0, 0, 0, 0, 0 };
 	std::vector<R> Hs[6];
 	R selfc[3] = { 0, 0, 0 };
+	std::vector<R> layselfc;           // layer stacks: 3 per layer
+	std::vector<double> layHz;         // layer stacks: each layer's cell thickness
 	bool link_dTspeedup = true;
 	double dTspeedup = 0, time_speedup = 0;
 
@@ -505,12 +507,7 @@ struct Sim {
 		double Hst[3]; applied_field(p, stagetime, Hst);
 		R Ha[3] = { (R)Hst[0], (R)Hst[1], (R)Hst[2] };
 		for (size_t i = 0; i < n; i++) for (int c = 0; c < 3; c++) Heff[i * 3 + c] = p.zeeman ? Ha[c] : (R)0;
-		if (ext_demag) {
-			std::vector<R> Hd(n * 3);
-			ext_demag(M.data(), Hd.data());
-			for (size_t q = 0; q < n * 3; q++) Heff[q] += Hd[q];
-		}
-		else if (demag) demag_update();
+		if (ext_demag || demag) demag_update();
 		if (p.exchange) add_exchange();
 		if (p.anisotropy) add_anisotropy();
 		evaluations++;
@@ -527,6 +524,12 @@ struct Sim {
 		for (int s = 0; s < 6; s++) { Hs[s].clear(); tdm[s] = 0; }
 		double sd[3]; self_demag(h[0], h[1], h[2], sd);   // SelfDemag_PBC without images
 		for (int c = 0; c < 3; c++) selfc[c] = (R)sd[c];
+		// a layer stack: each layer mesh's own coefficient (Boris/SDemagCUDA.cpp:583-1000)
+		layselfc.clear();
+		for (size_t l = 0; l < layHz.size(); l++) {
+			self_demag(h[0], h[1], layHz[l], sd);
+			for (int c = 0; c < 3; c++) layselfc.push_back((R)sd[c]);
+		}
 		if (dTs > 0) { link_dTspeedup = false; dTspeedup = dTs; time_speedup = time; }
 		else { link_dTspeedup = true; dTspeedup = dT; }
 	}
@@ -559,13 +562,27 @@ struct Sim {
 		return available;
 	}
 
-	// DemagCUDA::UpdateField with the evaluation speedup (Boris/DemagCUDA.cpp:106-468).
+	// self coefficient of component c of cell i
+	R self_at(size_t i, int c) const { return layselfc.empty() ? selfc[c] : layselfc[layer_of(i) * 3 + c]; }
+
+	// H = demag field of M (set): the mesh's convolution or the stack's multilayered one
+	void convolve(const R* Min, R* H)
+	{
+		if (ext_demag) ext_demag(Min, H);
+		else demag->convolute(Min, H, true);
+		convolutions++;
+	}
+
+	// DemagCUDA::UpdateField with the evaluation speedup (Boris/DemagCUDA.cpp:106-468), and
+	// SDemagCUDA::UpdateField's multilayered branch for a layer stack (Boris/SDemagCUDA.cpp:
+	// 378-1000: the same saves and extrapolation per layer mesh, slot times shared).
 	void demag_update()
 	{
 		const size_t n3 = n * 3;
 		if (!speedup || (num_saved < speedup && !check_step_update())) {
-			demag->convolute(M.data(), Heff.data(), false);
-			convolutions++;
+			std::vector<R> Hd(n3);
+			convolve(M.data(), Hd.data());
+			for (size_t q = 0; q < n3; q++) Heff[q] += Hd[q];
 			return;
 		}
 		if (check_step_update() || num_saved < speedup) {
@@ -574,10 +591,9 @@ struct Sim {
 			else { slot = speedup_next_slot(speedup, tdm); if (speedup > 1) tdm[slot] = eval_step_time(); }
 			std::vector<R>& S = Hs[slot];
 			S.assign(n3, (R)0);
-			demag->convolute(M.data(), S.data(), true);
-			convolutions++;
+			convolve(M.data(), S.data());
 			// Demag_EvalSpeedup_AddField_SubSelf (Boris/DemagCUDA.cu:44-57)
-			for (size_t q = 0; q < n3; q++) { Heff[q] += S[q]; S[q] -= selfc[q % 3] * M[q]; }
+			for (size_t q = 0; q < n3; q++) { Heff[q] += S[q]; S[q] -= self_at(q / 3, (int)(q % 3)) * M[q]; }
 		}
 		else {
 			// Demag_EvalSpeedup_AddExtrapField_AddSelf (Boris/DemagCUDA.cu:59-413)
@@ -586,7 +602,7 @@ struct Sim {
 			for (size_t q = 0; q < n3; q++) {
 				R v = Hs[0][q] * a[0];
 				for (int s = 1; s < speedup; s++) v += Hs[s][q] * a[s];
-				v += selfc[q % 3] * M[q];
+				v += self_at(q / 3, (int)(q % 3)) * M[q];
 				Heff[q] += v;
 			}
 		}

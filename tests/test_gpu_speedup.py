@@ -105,3 +105,30 @@ def test_speedup_off_again_equals_plain(gpu):
     a.run_to_iteration(10)
     b.run_to_iteration(10)
     assert np.array_equal(a.M.cpu().numpy(), b.M.cpu().numpy())
+
+
+@pytest.mark.parametrize("mode", [1, 3])
+@pytest.mark.parametrize("method", [api.BM_EVAL_RK4, api.BM_EVAL_RKDP54])
+def test_speedup_layer_stack_matches_oracle(gpu, method, mode):
+    """A 3-layer stack (2/1/4 cells of 4 nm with 1-cell spacers, per-layer Ms, alpha, K1) with the
+    speedup: SDemag's multilayered branch, each layer with its own self coefficient."""
+    from test_gpu_layers import A, ALPHA, HZ, K1, MODS, MS, Z0, layer_avg, start
+    nx, ny = 32, 24
+    nl = nx * ny
+    p = api.FMParams(Ms=8e5, A=1.3e-11, alpha=0.02, ea=(0.0, 0.0, 1.0), Ha=(-2.0e4, 4.0e3, 0.0), modules=MODS)
+    M0 = start(nl, seed=11)
+    s = api.LayerStackSolver(method, nx, ny, 4e-9, 4e-9, HZ, Z0, p, Ms=MS, A=A, alpha=ALPHA, K1=K1, M0=M0.astype(np.float32))
+    q = api.FMParams(**{**p.__dict__, "modules": MODS & ~api.BM_MOD_DEMAG})
+    o = O.OracleSim(method, (nx, ny, 3), (4e-9, 4e-9, 4e-9), q)
+    o.set_layers(4e-9, 4e-9, HZ, Z0, with_demag=True, Ms=MS, A=A, alpha=ALPHA, K1=K1)
+    o.set_M(M0)
+    s.set_eval_speedup(mode)
+    o.set_speedup(mode)
+    steps = 40
+    s.run_to_iteration(steps)
+    o.run_to_iteration(steps)
+    assert s.speedup_state()["convolutions"] == o.speedup_info()["convolutions"]
+    assert s.state()["rejections"] == o.state()["rejections"]
+    Mg = s.M.cpu().numpy().astype(np.float64)
+    assert np.abs(layer_avg(Mg, nl) - layer_avg(o.get_M(), nl)).max() <= 1e-4
+    assert rel_l2(Mg, o.get_M()) <= 1e-4

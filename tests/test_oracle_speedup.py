@@ -119,3 +119,60 @@ def test_speedup_convolution_count_and_saved_times():
                 # the saves are the first evaluations of the last p steps, at their start times
                 np.testing.assert_allclose(np.sort(info["tdm"][:mode]), [(steps - 1 - q) * dt for q in range(mode - 1, -1, -1)],
                                            rtol=1e-9)
+
+
+def _stack(speedup, method=BM_EVAL_RK4, dt=1e-13, steps=0, modules=None, seed=3):
+    """Three 2D layers of 12x10 cells, thicknesses 2/1/4 nm with 1 nm gaps, as a layer stack."""
+    n, hx, hy = (12, 10, 3), 4e-9, 4e-9
+    hz, z0 = [2e-9, 1e-9, 4e-9], [0.0, 3e-9, 5e-9]
+    p = FMParams(Ms=8e5, A=1.3e-11, alpha=0.1, Ha=(2e4, 1e4, 0.0),
+                 modules=(BM_MOD_EXCHANGE | BM_MOD_ZEEMAN) if modules is None else modules)
+    s = O.OracleSim(method, n, (hx, hy, max(hz)), p)
+    s.set_layers(hx, hy, hz, z0, with_demag=True, Ms=[8e5, 6e5, 1.0e6])
+    ms = np.repeat([8e5, 6e5, 1.0e6], n[0] * n[1])
+    M0 = random_magnetisation(n[0] * n[1] * n[2], 1.0, seed) * ms[:, None]
+    s.set_M(M0)
+    s.set_dt(dt)
+    if method == BM_EVAL_RKDP54:
+        s.set_astep(dtmin=dt, dtmax=dt)
+    if speedup:
+        s.set_speedup(speedup)
+    if steps:
+        s.run_to_iteration(steps)
+    return s, M0, (n, hx, hy, hz, z0)
+
+
+def test_stack_step_mode_uses_each_layers_self_term():
+    """STEP mode on a layer stack: the second evaluation's demag field must be the first
+    convolution minus each layer's own self term of the first M, plus that layer's self term
+    of the current M (SDemagCUDA: per layer mesh selfDemagCoeff from its own cell size)."""
+    s, M0, (n, hx, hy, hz, z0) = _stack(1, modules=0)
+    s.update_fields()
+    s.iterate_stage()
+    M1 = s.get_M()
+    s.update_fields()
+    H = s.heff()
+    conv = O.OracleMLayer(n[0], n[1], hx, hy, hz, z0)(M0)
+    nl = n[0] * n[1]
+    c = np.zeros((n[0] * n[1] * n[2], 3))
+    for l in range(3):
+        sd = np.zeros(3)
+        O.lib().or_self_demag(hx, hy, hz[l], sd)
+        c[l * nl:(l + 1) * nl] = sd
+    want = conv - c * M0 + c * M1
+    np.testing.assert_allclose(H, want, rtol=0, atol=1e-9 * np.abs(want).max())
+    # the layers' coefficients differ (1, 2 and 4 nm cells), so one shared value would fail
+    assert np.ptp(c[:, 2]) > 0.1
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+def test_stack_speedup_deviation_order(mode):
+    t_end = 6.4e-12
+    d = []
+    for dt in (1e-13, 0.5e-13):
+        steps = int(round(t_end / dt))
+        a = _stack(0, dt=dt, steps=steps)[0].get_M()
+        b = _stack(mode, dt=dt, steps=steps)[0].get_M()
+        d.append(np.linalg.norm(a - b) / np.linalg.norm(a))
+    rate = np.log2(d[0] / d[1])
+    assert abs(rate - mode) < 0.45, (d, rate)
