@@ -331,7 +331,8 @@ def multilayer_instances(seed: int = 5, count: int = 16):
 def multilayer_run(steps: int, adj_steps: int, ckpt: int = 100):
     """The 16 layer stacks: batched RK4 (dt 100 fs) vs the same stacks one by one on the
     single-stack path over `steps` steps; then the adjoint of one stack's `adj_steps`-step
-    rollout (per-layer Ms and ramp-slope gradients), forward and backward wall times."""
+    rollout (per-layer Ms and ramp-slope gradients), forward and backward wall times, and the
+    adjoints of all 16 stacks run in lockstep (one stream each) against 16 times one stack."""
     import torch
     from boris2_amd import api
     items = multilayer_instances()
@@ -388,6 +389,33 @@ def multilayer_run(steps: int, adj_steps: int, ckpt: int = 100):
         out["adjoint"] = {"stack": 0, "steps": adj_steps, "checkpoint_every": ckpt, "forward_s": round(tf, 3),
                           "backward_s": round(tbk, 3), "adjoint_over_forward": round(tbk / tf, 3), "loss": loss,
                           "dL_dMs_layer": gl, "dL_dslope": gz[1:].tolist()}
+        # all 16 stacks' adjoints in lockstep, one stream each
+        rs, m0s, mts = [], [], []
+        for k, (n, h, p, lay, M0) in enumerate(items):
+            rng = np.random.default_rng(8 + k)
+            mt = rng.normal(size=M0.shape); mt /= np.linalg.norm(mt, axis=1, keepdims=True)
+            Ms = torch.tensor(np.repeat(lay["Ms"], n[0] * n[1]), dtype=torch.float32)
+            rs.append(api.RK4Adjoint(n, h, p, Ms, dt, adj_steps, ckpt, layers=lay))
+            m0s.append(torch.tensor(M0 / np.linalg.norm(M0, axis=1, keepdims=True), dtype=torch.float32))
+            mts.append(torch.tensor(mt, dtype=torch.float32))
+        bt = api.RK4AdjointBatch(rs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        losses, _ = bt.forward(m0s, mts)
+        torch.cuda.synchronize()
+        tbf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        grads = bt.backward()
+        torch.cuda.synchronize()
+        tbb = time.perf_counter() - t0
+        for r in rs:
+            r.close()
+        out["batched_adjoint"] = {"stacks": len(items), "steps": adj_steps, "checkpoint_every": ckpt,
+                                  "forward_s": round(tbf, 3), "backward_s": round(tbb, 3),
+                                  "adjoint_over_forward": round(tbb / tbf, 3),
+                                  "one_stack_alone_x16_s": round(16 * (tf + tbk), 3),
+                                  "batched_over_one_by_one": round(16 * (tf + tbk) / (tbf + tbb), 3),
+                                  "mean_loss": float(np.mean(losses))}
     return out
 
 

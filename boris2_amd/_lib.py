@@ -26,7 +26,7 @@ EXPORTS = [
     "bm_batch_create", "bm_batch_create_layers", "bm_batch_destroy", "bm_batch_set_dt", "bm_batch_run", "bm_batch_get_state",
     "bm_batch_mesh_offset", "bm_batch_autotune_report", "bm_batch_set_astep", "bm_batch_get_mesh_state",
     "bm_adj_create", "bm_adj_destroy", "bm_adj_forward", "bm_adj_backward", "bm_adj_zeeman_grads", "bm_adj_set_layers",
-    "bm_ovf2_info", "bm_ovf2_read", "bm_ovf2_write",
+    "bm_ovf2_info", "bm_ovf2_read", "bm_ovf2_write", "bm_adj_batch_forward", "bm_adj_batch_backward",
 ]
 
 PROF_KINDS = ["fwd_x_r2c", "y_fft", "z_fft_kernel_ifft", "y_ifft", "inv_x_c2r", "y2d_fft_kernel_ifft",
@@ -97,6 +97,8 @@ def load() -> C.CDLL:
         "bm_adj_backward": (i, [vp, vp, C.POINTER(d), vp]),
         "bm_adj_zeeman_grads": (i, [vp, C.POINTER(d)]),
         "bm_adj_set_layers": (i, [vp, vp, vp, vp, vp]),
+        "bm_adj_batch_forward": (i, [i, vp, vp, vp, vp, vp, vp]),
+        "bm_adj_batch_backward": (i, [i, vp, vp, vp, vp]),
         "bm_ode_create": (i, [i, i, i, i, d, d, d, C.POINTER(FmParams), vp, vp, C.POINTER(vp)]),
         "bm_ode_destroy": (i, [vp]),
         "bm_ode_set_ms_map": (i, [vp, vp]),

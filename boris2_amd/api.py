@@ -479,6 +479,34 @@ class RK4Adjoint:
             pass
 
 
+class RK4AdjointBatch:
+    """Several RK4Adjoint instances (same nsteps and checkpoint interval) run in lockstep, each
+    on its own stream, so their kernels overlap on the device: losses and gradients of all
+    instances from one forward() and one backward()."""
+
+    def __init__(self, adjoints: list[RK4Adjoint]):
+        self.L = _lib.load()
+        self.adjoints = list(adjoints)
+        self._h = (C.c_void_p * len(self.adjoints))(*[a.handle.value for a in self.adjoints])
+
+    def forward(self, m0s, mts) -> tuple[list[float], list[torch.Tensor]]:
+        nb = len(self.adjoints)
+        self._m0 = [m.detach().to(device="cuda", dtype=torch.float32).contiguous() for m in m0s]
+        self._mt = [m.detach().to(device="cuda", dtype=torch.float32).contiguous() for m in mts]
+        Mf = [torch.empty_like(m) for m in self._m0]
+        ptrs = lambda ts: (C.c_void_p * nb)(*[t.data_ptr() for t in ts])
+        loss = (C.c_double * nb)()
+        check(self.L.bm_adj_batch_forward(nb, self._h, ptrs(self._m0), ptrs(self._mt), ptrs(Mf), loss, _stream()))
+        return list(loss[:]), Mf
+
+    def backward(self) -> list[tuple[torch.Tensor, float]]:
+        nb = len(self.adjoints)
+        gs = [torch.empty_like(a.Ms_map) for a in self.adjoints]
+        ga = (C.c_double * nb)()
+        check(self.L.bm_adj_batch_backward(nb, self._h, (C.c_void_p * nb)(*[g.data_ptr() for g in gs]), ga, _stream()))
+        return [(g, ga[b]) for b, g in enumerate(gs)]
+
+
 class RK4RolloutLoss(torch.autograd.Function):
     """loss = RK4RolloutLoss.apply(Ms_map, amp, n, h, params, dt, nsteps, m0, mt, ckpt_every):
     differentiable in Ms_map (n floats, CUDA) and the pulse amplitude amp (0-d tensor, A/m)."""

@@ -287,6 +287,10 @@ struct bm_adj_s {
 	double* h_part = nullptr;     // pinned
 	const float* m0 = nullptr; const float* mt = nullptr;
 	bool forward_done = false;
+	float* cur = nullptr;         // forward: the state the next step starts from
+	// batched runs (bm_adj_batch_*): this handle's own stream and its completion event
+	hipStream_t own = nullptr;
+	hipEvent_t done = nullptr;
 };
 
 namespace {
@@ -380,6 +384,8 @@ int adj_step(bm_adj_s* o, int step, int q, hipStream_t st)
 
 void free_all(bm_adj_s* o)
 {
+	if (o->done) (void)hipEventDestroy(o->done);
+	if (o->own) (void)hipStreamDestroy(o->own);
 	float* fl[] = { o->ck, o->tape, o->Mw, o->acc, o->Hd, o->Xs, o->lam, o->lamY, o->lacc, o->lx, o->lxn, o->u, o->cu, o->Nu };
 	for (float* p : fl) if (p) (void)hipFree(p);
 	if (o->gMs) (void)hipFree(o->gMs);
@@ -428,29 +434,135 @@ extern "C" int bm_adj_destroy(bm_adj_t o)
 	return BM_OK;
 }
 
-extern "C" int bm_adj_forward(bm_adj_t o, const float* m0, const float* mt, float* M_final, double* loss, void* stream)
+namespace {
+
+// Forward rollout in phases, so that several handles can be driven in lockstep.
+int fwd_begin(bm_adj_s* o, const float* m0, const float* mt, hipStream_t st)
 {
-	if (!o || !m0 || !mt) return fail(BM_ERR_ARG, "bm_adj_forward: null argument");
-	hipStream_t st = (hipStream_t)stream;
 	o->m0 = m0; o->mt = mt;
-	const dim3 g(o->grid), b(kAB);
-	hipLaunchKernelGGL(k_init, g, b, 0, st, o->ck, o->Ms, m0, o->n);
-	float* cur = o->ck;
-	for (int s = 0; s < o->nsteps; s++) {
-		const bool at_ck = (s + 1) % o->K == 0 || s + 1 == o->nsteps;
-		float* next = at_ck ? o->ck + (size_t)((s + 1 + o->K - 1) / o->K) * n3(o) : (cur == o->Mw ? o->Mw + n3(o) : o->Mw);
-		int rc = fwd_step(o, s, cur, next, -1, st);
-		if (rc) return rc;
-		cur = next;
-	}
-	if (M_final) BM_HIP(hipMemcpyAsync(M_final, cur, n3(o) * sizeof(float), hipMemcpyDeviceToDevice, st));
-	hipLaunchKernelGGL(k_loss_part, g, b, 0, st, cur, mt, o->Ms, o->n, o->part);
+	hipLaunchKernelGGL(k_init, dim3(o->grid), dim3(kAB), 0, st, o->ck, o->Ms, m0, o->n);
+	o->cur = o->ck;
+	return BM_OK;
+}
+
+// step s, checkpointing every K steps and at the end
+int fwd_advance(bm_adj_s* o, int s, hipStream_t st)
+{
+	const bool at_ck = (s + 1) % o->K == 0 || s + 1 == o->nsteps;
+	float* next = at_ck ? o->ck + (size_t)((s + 1 + o->K - 1) / o->K) * n3(o) : (o->cur == o->Mw ? o->Mw + n3(o) : o->Mw);
+	int rc = fwd_step(o, s, o->cur, next, -1, st);
+	o->cur = next;
+	return rc;
+}
+
+int fwd_end(bm_adj_s* o, float* M_final, hipStream_t st)
+{
+	if (M_final) BM_HIP(hipMemcpyAsync(M_final, o->cur, n3(o) * sizeof(float), hipMemcpyDeviceToDevice, st));
+	hipLaunchKernelGGL(k_loss_part, dim3(o->grid), dim3(kAB), 0, st, o->cur, o->mt, o->Ms, o->n, o->part);
 	BM_HIP(hipMemcpyAsync(o->h_part, o->part, (size_t)o->grid * sizeof(double), hipMemcpyDeviceToHost, st));
-	BM_HIP(hipStreamSynchronize(st));
+	return BM_OK;
+}
+
+// after the stream has finished: the loss from the per-block partials (fixed order)
+void fwd_finish(bm_adj_s* o, double* loss)
+{
 	double sum = 0;
 	for (int q = 0; q < o->grid; q++) sum += o->h_part[q];
 	if (loss) *loss = 1.0 - sum / (double)o->n;
 	o->forward_done = true;
+}
+
+int bwd_begin(bm_adj_s* o, hipStream_t st)
+{
+	float* Mfin = o->ck + (size_t)(o->nck - 1) * n3(o);
+	hipLaunchKernelGGL(k_final, dim3(o->grid), dim3(kAB), 0, st, Mfin, o->mt, o->Ms, o->n, o->lam, o->gMs);
+	BM_HIP(hipMemsetAsync(o->part + o->grid, 0, 4 * (size_t)o->grid * sizeof(double), st));
+	return BM_OK;
+}
+
+// segment sg: recompute its steps from the checkpoint with the tape, then run them backwards
+int bwd_recompute(bm_adj_s* o, int sg, int s, hipStream_t st)
+{
+	const int s0 = sg * o->K;
+	if (s == s0) o->cur = o->ck + (size_t)sg * n3(o);
+	float* next = (o->cur == o->Mw) ? o->Mw + n3(o) : o->Mw;
+	int rc = fwd_step(o, s, o->cur, next, s - s0, st);
+	o->cur = next;
+	return rc;
+}
+
+int bwd_end(bm_adj_s* o, float* gMs_out, hipStream_t st)
+{
+	const dim3 g(o->grid), b(kAB);
+	hipLaunchKernelGGL(k_m0_term, g, b, 0, st, o->m0, o->lam, o->gMs, o->n);
+	hipLaunchKernelGGL(k_to_float, g, b, 0, st, o->gMs, gMs_out, o->n);
+	BM_HIP(hipMemcpyAsync(o->h_part, o->part + o->grid, 4 * (size_t)o->grid * sizeof(double), hipMemcpyDeviceToHost, st));
+	return BM_OK;
+}
+
+void bwd_finish(bm_adj_s* o, double* gamp)
+{
+	for (int c = 0; c < 4; c++) o->gz[c] = 0;
+	for (int q = 0; q < o->grid; q++) for (int c = 0; c < 4; c++) o->gz[c] += o->h_part[4 * q + c];
+	if (gamp) *gamp = o->gz[0];
+}
+
+// The streams of a batched run: every handle's own stream starts after earlier work on the
+// caller's stream; `join` makes the caller's stream wait for all of them.
+int batch_fork(bm_adj_s* const* hs, int count, hipStream_t st)
+{
+	hipEvent_t ev;
+	BM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+	int rc = BM_OK;
+	if (hipEventRecord(ev, st) != hipSuccess) rc = fail(BM_ERR_HIP, "bm_adj_batch: event");
+	for (int b = 0; b < count && !rc; b++) {
+		bm_adj_s* o = hs[b];
+		if (!o->own && (hipStreamCreate(&o->own) != hipSuccess || hipEventCreateWithFlags(&o->done, hipEventDisableTiming) != hipSuccess))
+			rc = fail(BM_ERR_HIP, "bm_adj_batch: stream");
+		else if (hipStreamWaitEvent(o->own, ev, 0) != hipSuccess) rc = fail(BM_ERR_HIP, "bm_adj_batch: wait");
+	}
+	(void)hipEventDestroy(ev);
+	return rc;
+}
+
+int batch_join(bm_adj_s* const* hs, int count, hipStream_t st)
+{
+	for (int b = 0; b < count; b++) {
+		BM_HIP(hipEventRecord(hs[b]->done, hs[b]->own));
+		BM_HIP(hipStreamWaitEvent(st, hs[b]->done, 0));
+	}
+	BM_HIP(hipStreamSynchronize(st));
+	return BM_OK;
+}
+
+int batch_check(int count, bm_adj_s* const* hs)
+{
+	if (count < 1 || !hs) return fail(BM_ERR_ARG, "bm_adj_batch: no handles");
+	for (int b = 0; b < count; b++) {
+		if (!hs[b]) return fail(BM_ERR_ARG, "bm_adj_batch: null handle");
+		if (hs[b]->nsteps != hs[0]->nsteps || hs[b]->K != hs[0]->K) return fail(BM_ERR_ARG, "bm_adj_batch: handles need the same nsteps and checkpoint interval");
+		for (int c = 0; c < b; c++) {
+			if (hs[c] == hs[b]) return fail(BM_ERR_ARG, "bm_adj_batch: a handle appears twice");
+			// the handles run concurrently: each needs its own convolution scratch
+			if ((hs[b]->demag && hs[c]->demag == hs[b]->demag) || (hs[b]->ml && hs[c]->ml == hs[b]->ml))
+				return fail(BM_ERR_ARG, "bm_adj_batch: handles share a demag or multilayer handle");
+		}
+	}
+	return BM_OK;
+}
+
+} // namespace
+
+extern "C" int bm_adj_forward(bm_adj_t o, const float* m0, const float* mt, float* M_final, double* loss, void* stream)
+{
+	if (!o || !m0 || !mt) return fail(BM_ERR_ARG, "bm_adj_forward: null argument");
+	hipStream_t st = (hipStream_t)stream;
+	int rc = fwd_begin(o, m0, mt, st);
+	for (int s = 0; s < o->nsteps && !rc; s++) rc = fwd_advance(o, s, st);
+	if (!rc) rc = fwd_end(o, M_final, st);
+	if (rc) return rc;
+	BM_HIP(hipStreamSynchronize(st));
+	fwd_finish(o, loss);
 	BM_HIP(hipGetLastError());
 	return BM_OK;
 }
@@ -460,33 +572,64 @@ extern "C" int bm_adj_backward(bm_adj_t o, float* gMs_out, double* gamp, void* s
 	if (!o || !gMs_out) return fail(BM_ERR_ARG, "bm_adj_backward: null argument");
 	if (!o->forward_done) return fail(BM_ERR_STATE, "bm_adj_backward: run bm_adj_forward first");
 	hipStream_t st = (hipStream_t)stream;
-	const dim3 g(o->grid), b(kAB);
-	const int nseg = o->nck - 1;
-	float* Mfin = o->ck + (size_t)nseg * n3(o);
-	hipLaunchKernelGGL(k_final, g, b, 0, st, Mfin, o->mt, o->Ms, o->n, o->lam, o->gMs);
-	BM_HIP(hipMemsetAsync(o->part + o->grid, 0, 4 * (size_t)o->grid * sizeof(double), st));
-	for (int sg = nseg - 1; sg >= 0; sg--) {
+	int rc = bwd_begin(o, st);
+	for (int sg = o->nck - 2; sg >= 0 && !rc; sg--) {
 		const int s0 = sg * o->K, s1 = std::min(o->nsteps, s0 + o->K);
-		// recompute the segment from its checkpoint, taping every step
-		const float* cur = o->ck + (size_t)sg * n3(o);
-		for (int s = s0; s < s1; s++) {
-			float* next = (cur == o->Mw) ? o->Mw + n3(o) : o->Mw;
-			int rc = fwd_step(o, s, cur, next, s - s0, st);
-			if (rc) return rc;
-			cur = next;
-		}
-		for (int s = s1 - 1; s >= s0; s--) {
-			int rc = adj_step(o, s, s - s0, st);
-			if (rc) return rc;
-		}
+		for (int s = s0; s < s1 && !rc; s++) rc = bwd_recompute(o, sg, s, st);
+		for (int s = s1 - 1; s >= s0 && !rc; s--) rc = adj_step(o, s, s - s0, st);
 	}
-	hipLaunchKernelGGL(k_m0_term, g, b, 0, st, o->m0, o->lam, o->gMs, o->n);
-	hipLaunchKernelGGL(k_to_float, g, b, 0, st, o->gMs, gMs_out, o->n);
-	BM_HIP(hipMemcpyAsync(o->h_part, o->part + o->grid, 4 * (size_t)o->grid * sizeof(double), hipMemcpyDeviceToHost, st));
+	if (!rc) rc = bwd_end(o, gMs_out, st);
+	if (rc) return rc;
 	BM_HIP(hipStreamSynchronize(st));
-	for (int c = 0; c < 4; c++) o->gz[c] = 0;
-	for (int q = 0; q < o->grid; q++) for (int c = 0; c < 4; c++) o->gz[c] += o->h_part[4 * q + c];
-	if (gamp) *gamp = o->gz[0];
+	bwd_finish(o, gamp);
+	BM_HIP(hipGetLastError());
+	return BM_OK;
+}
+
+extern "C" int bm_adj_batch_forward(int count, const bm_adj_t* hs, const float* const* m0, const float* const* mt,
+	float* const* M_final, double* loss, void* stream)
+{
+	int rc = batch_check(count, hs);
+	if (rc) return rc;
+	if (!m0 || !mt) return fail(BM_ERR_ARG, "bm_adj_batch_forward: null argument");
+	for (int b = 0; b < count; b++) if (!m0[b] || !mt[b]) return fail(BM_ERR_ARG, "bm_adj_batch_forward: null state");
+	hipStream_t st = (hipStream_t)stream;
+	if ((rc = batch_fork(hs, count, st))) return rc;
+	for (int b = 0; b < count && !rc; b++) rc = fwd_begin(hs[b], m0[b], mt[b], hs[b]->own);
+	for (int s = 0; s < hs[0]->nsteps && !rc; s++)
+		for (int b = 0; b < count && !rc; b++) rc = fwd_advance(hs[b], s, hs[b]->own);
+	for (int b = 0; b < count && !rc; b++) rc = fwd_end(hs[b], M_final ? M_final[b] : nullptr, hs[b]->own);
+	if (!rc) rc = batch_join(hs, count, st);
+	if (rc) return rc;
+	for (int b = 0; b < count; b++) fwd_finish(hs[b], loss ? &loss[b] : nullptr);
+	BM_HIP(hipGetLastError());
+	return BM_OK;
+}
+
+extern "C" int bm_adj_batch_backward(int count, const bm_adj_t* hs, float* const* gMs_out, double* gamp, void* stream)
+{
+	int rc = batch_check(count, hs);
+	if (rc) return rc;
+	if (!gMs_out) return fail(BM_ERR_ARG, "bm_adj_batch_backward: null argument");
+	for (int b = 0; b < count; b++) {
+		if (!gMs_out[b]) return fail(BM_ERR_ARG, "bm_adj_batch_backward: null gradient array");
+		if (!hs[b]->forward_done) return fail(BM_ERR_STATE, "bm_adj_batch_backward: run the forward pass first");
+	}
+	hipStream_t st = (hipStream_t)stream;
+	if ((rc = batch_fork(hs, count, st))) return rc;
+	for (int b = 0; b < count && !rc; b++) rc = bwd_begin(hs[b], hs[b]->own);
+	const int K = hs[0]->K, nsteps = hs[0]->nsteps;
+	for (int sg = hs[0]->nck - 2; sg >= 0 && !rc; sg--) {
+		const int s0 = sg * K, s1 = std::min(nsteps, s0 + K);
+		for (int s = s0; s < s1 && !rc; s++)
+			for (int b = 0; b < count && !rc; b++) rc = bwd_recompute(hs[b], sg, s, hs[b]->own);
+		for (int s = s1 - 1; s >= s0 && !rc; s--)
+			for (int b = 0; b < count && !rc; b++) rc = adj_step(hs[b], s, s - s0, hs[b]->own);
+	}
+	for (int b = 0; b < count && !rc; b++) rc = bwd_end(hs[b], gMs_out[b], hs[b]->own);
+	if (!rc) rc = batch_join(hs, count, st);
+	if (rc) return rc;
+	for (int b = 0; b < count; b++) bwd_finish(hs[b], gamp ? &gamp[b] : nullptr);
 	BM_HIP(hipGetLastError());
 	return BM_OK;
 }

@@ -258,6 +258,16 @@ int bm_adj_forward(bm_adj_t a, const float* m0, const float* mt, float* M_final,
 int bm_adj_backward(bm_adj_t a, float* dL_dMs, double* dL_damp, void* stream);
 int bm_adj_zeeman_grads(bm_adj_t a, double* out4);
 int bm_adj_set_layers(bm_adj_t a, bm_mlayer_t ml, const double* A, const double* alpha, const double* K1);
+/* Batched adjoint (no reference counterpart): `count` adjoint handles driven in lockstep, each
+ * on its own internal stream so that the instances' kernels overlap on the device (the
+ * differentiable multilayer workload runs 16 stacks this way).  Work is ordered after earlier
+ * work on `stream`, which waits for all of it on return.  All handles need the same nsteps and
+ * checkpoint interval and their own demag / multilayer handles.  m0[b], mt[b], M_final[b] (or
+ * NULL), gMs_out[b] per handle; loss and gamp are double[count]; bm_adj_zeeman_grads of each
+ * handle gives its field-waveform gradients afterwards. */
+int bm_adj_batch_forward(int count, const bm_adj_t* h, const float* const* m0, const float* const* mt,
+	float* const* M_final, double* loss, void* stream);
+int bm_adj_batch_backward(int count, const bm_adj_t* h, float* const* gMs_out, double* gamp, void* stream);
 
 /* ------------------------------------------------------------------------------------
  * Launch timing (no reference counterpart: the reference only times whole runs,
