@@ -77,6 +77,19 @@ class Demag:
         check(self.L.bm_demag_convolute(self.handle, _ptr(M), _ptr(Heff), int(clear_out), e, _stream()))
         return Heff
 
+    def convolute_averaged(self, M1: torch.Tensor, M2: torch.Tensor, H1: torch.Tensor | None = None,
+                           H2: torch.Tensor | None = None, clear_out: bool = True,
+                           energy: torch.Tensor | None = None) -> torch.Tensor:
+        """H1 (and H2 if given) (=, +=) H_demag((M1 + M2) / 2): the two-sublattice convolution
+        (ConvolutionCUDA::Convolute_AveragedInputs[_DuplicatedOutputs])."""
+        if H1 is None:
+            H1 = torch.empty_like(M1)
+            clear_out = True
+        e = C.c_void_p(energy.data_ptr()) if energy is not None else None
+        check(self.L.bm_demag_convolute_averaged(self.handle, _ptr(M1), _ptr(M2), _ptr(H1),
+                                                 _ptr(H2) if H2 is not None else None, int(clear_out), e, _stream()))
+        return H1
+
     def convolute_planar(self, M: torch.Tensor, H3n: torch.Tensor | None = None) -> torch.Tensor:
         if H3n is None:
             H3n = torch.empty((3, self.ncells), dtype=torch.float32, device=M.device)

@@ -380,6 +380,24 @@ __global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ Sx, const float
 
 // ---------------------------------------------------------------------------------------
 // Energy density  -mu0/(2n) sum M.H from the per-work-item partials of P5.
+// (In1 + In2) / 2 per component, as the reference forms the averaged input
+// (Boris/ConvolutionDataCUDA.cu:60-73)
+__global__ void k_avg_in(const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ out, size_t n3)
+{
+	for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n3; q += (size_t)gridDim.x * blockDim.x)
+		out[q] = (a[q] + b[q]) / 2;
+}
+
+// the field copied (clear) or added to both outputs (Boris/ConvolutionDataCUDA.cu:227-295)
+__global__ void k_dup_out(const float* __restrict__ h, float* __restrict__ o1, float* __restrict__ o2, int clear, size_t n3)
+{
+	for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n3; q += (size_t)gridDim.x * blockDim.x) {
+		const float v = h[q];
+		if (clear) { o1[q] = v; o2[q] = v; }
+		else { o1[q] += v; o2[q] += v; }
+	}
+}
+
 __global__ void k_energy_final(const double* part, int np, size_t n, float* energy)
 {
 	__shared__ double red[kBlock];
@@ -827,6 +845,8 @@ extern "C" int bm_demag_destroy(bm_demag_t d)
 	if (d->S) (void)hipFree(d->S);
 	if (d->K) (void)hipFree(d->K);
 	if (d->ebuf) (void)hipFree(d->ebuf);
+	if (d->avg) (void)hipFree(d->avg);
+	if (d->hbuf) (void)hipFree(d->hbuf);
 	delete d;
 	return BM_OK;
 }
@@ -872,6 +892,27 @@ extern "C" int bm_demag_convolute_planar(bm_demag_t d, const float* M, float* H3
 	if (!d || !M || !H3n) return fail(BM_ERR_ARG, "bm_demag_convolute_planar: null argument");
 	int rc = bm::demag_planar(d, M, H3n, (hipStream_t)stream);
 	if (rc) return rc;
+	BM_HIP(hipGetLastError());
+	return BM_OK;
+}
+
+// ConvolutionCUDA::Convolute_AveragedInputs / _DuplicatedOutputs (Boris/ConvolutionCUDA.h:
+// 225-247): the convolution of (M1 + M2) / 2, set or added to H1 and, when given, to H2; the
+// energy is -mu0/2 <(M1 + M2)/2 . H> / n, the reference's -mu0 (In1 + In2) . H / (4 n).
+extern "C" int bm_demag_convolute_averaged(bm_demag_t d, const float* M1, const float* M2, float* H1, float* H2,
+	int clear_out, float* energy, void* stream)
+{
+	if (!d || !M1 || !M2 || !H1) return fail(BM_ERR_ARG, "bm_demag_convolute_averaged: null argument");
+	hipStream_t st = (hipStream_t)stream;
+	const size_t n3 = (size_t)d->nx * d->ny * d->nz * 3;
+	if (!d->avg && hipMalloc(&d->avg, n3 * sizeof(float)) != hipSuccess) { d->avg = nullptr; return fail(BM_ERR_NOMEM, "averaged input buffer"); }
+	if (H2 && !d->hbuf && hipMalloc(&d->hbuf, n3 * sizeof(float)) != hipSuccess) { d->hbuf = nullptr; return fail(BM_ERR_NOMEM, "field buffer"); }
+	const int g = (int)std::min<size_t>((n3 + kBlock - 1) / kBlock, (size_t)d->ncu * 8);
+	hipLaunchKernelGGL(k_avg_in, dim3(g), dim3(kBlock), 0, st, M1, M2, d->avg, n3);
+	if (!H2) return bm_demag_convolute(d, d->avg, H1, clear_out, energy, stream);
+	int rc = bm_demag_convolute(d, d->avg, d->hbuf, 1, energy, stream);
+	if (rc) return rc;
+	hipLaunchKernelGGL(k_dup_out, dim3(g), dim3(kBlock), 0, st, d->hbuf, H1, H2, clear_out ? 1 : 0, n3);
 	BM_HIP(hipGetLastError());
 	return BM_OK;
 }

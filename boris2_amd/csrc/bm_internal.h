@@ -38,6 +38,8 @@ struct bm_demag_s {
 	float* K = nullptr;       // kernels (see bm_demag_copy_kernels)
 	long long kcount = 0;
 	double* ebuf = nullptr;   // energy partials (one per P5 work item)
+	float* avg = nullptr;     // averaged-input convolution: (M1 + M2) / 2 and the field (AoS,
+	float* hbuf = nullptr;    // allocated on first use)
 	int ecap = 0;
 	long long bytes = 0;
 	int ncu = 256;

@@ -56,6 +56,13 @@ int bm_demag_info(bm_demag_t d, int* N3, long long* device_bytes);
  * -mu0/(2 n) * sum_cells M.H_demag, the reference's demag energy density.
  */
 int bm_demag_convolute(bm_demag_t d, const float* M, float* Heff, int clear_out, float* energy, void* stream);
+/* Averaged inputs, optionally duplicated outputs (two-sublattice meshes): H1 (=, +=) and, if
+ * H2 is non-NULL, H2 (=, +=) the demag field of (M1 + M2) / 2; energy (device float or NULL)
+ * = -mu0/2 <(M1 + M2)/2 . H> / n.  Replaces ConvolutionCUDA::Convolute_AveragedInputs and
+ * Convolute_AveragedInputs_DuplicatedOutputs (Boris/ConvolutionCUDA.h:225-247), called by
+ * DemagCUDA::UpdateField for antiferromagnetic meshes (Boris/DemagCUDA.cpp:125-129). */
+int bm_demag_convolute_averaged(bm_demag_t d, const float* M1, const float* M2, float* H1, float* H2,
+	int clear_out, float* energy, void* stream);
 
 /* H_demag(M) written as three planes Hx[n], Hy[n], Hz[n] (the layout the fused LLG
  * stage kernels read).  No reference counterpart: the split the reference makes between

@@ -90,6 +90,33 @@ def test_output_modes_and_energy(gpu):
     assert abs(float(e.item()) - ref) <= 1e-5 * abs(ref)
 
 
+@pytest.mark.parametrize("n", [(64, 32, 4), (48, 24, 1)])
+def test_averaged_inputs_duplicated_outputs(gpu, n):
+    """Two-sublattice convolution: both outputs (set, then add) hold the field of (M1 + M2)/2
+    against the FP64 oracle; the energy is -mu0/2 <(M1 + M2)/2 . H> / n."""
+    h = H4
+    ncell = n[0] * n[1] * n[2]
+    M1 = random_m(ncell, 1e6, 21)
+    M2 = -0.8 * M1 + 0.2 * random_m(ncell, 1e6, 22)
+    ref = O.OracleDemag(n, h)((M1.astype(np.float64) + M2.astype(np.float64)) / 2)
+    d = api.Demag(n, h)
+    t1, t2 = torch.from_numpy(M1).cuda(), torch.from_numpy(M2).cuda()
+    e = torch.zeros(1, dtype=torch.float32, device="cuda")
+    H1, H2 = torch.empty_like(t1), torch.empty_like(t1)
+    d.convolute_averaged(t1, t2, H1, H2, clear_out=True, energy=e)
+    assert rel_l2(H1.cpu().numpy(), ref) <= 1e-6
+    assert torch.equal(H1, H2)
+    Mavg = (M1.astype(np.float64) + M2.astype(np.float64)) / 2
+    eref = -api.MU0 * float((Mavg * ref).sum()) / (2 * ncell)
+    assert abs(float(e.item()) - eref) <= 1e-5 * abs(eref)
+    A, B = torch.full_like(t1, 3.0), torch.full_like(t1, -5.0)
+    d.convolute_averaged(t1, t2, A, B, clear_out=False)
+    assert rel_l2(A.cpu().numpy() - 3.0, ref) <= 2e-6 and rel_l2(B.cpu().numpy() + 5.0, ref) <= 2e-6
+    # single output (Convolute_AveragedInputs)
+    H = d.convolute_averaged(t1, t2)
+    assert torch.equal(H, H1)
+
+
 def test_vortex_state_input(gpu):
     """Reference-held OVF2 vortex state (Boris/vortex_hh_clk_up.ovf) as the input M."""
     from pathlib import Path
