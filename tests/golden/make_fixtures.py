@@ -6,6 +6,9 @@ reference checkout and oracle/_ref exist; the GPU test runs only read the result
   value, x fastest) converted to float32, used as a realistic demag input.
 * vortex_hh_clk_up.ovf — the same reference data file copied verbatim (1.5 MB): the OVF2
   reader's input and the byte-level target of the OVF2 writer.
+* trilayer_switch_mconv.txt — the reference-held data file "MLConv Paper/3D/trilayer_switch_mconv.txt"
+  (a `savedata` table: header and <M> of three meshes vs time) copied verbatim: the target of
+  the data-table writer's format.
 * tensor_golden.npz — FP64 Newell demag tensors written by oracle/_ref/ref_dump, i.e. by the
   reference's own Boris/DemagTFunc*.cpp compiled here: first-octant values (the rest are
   mirror images) for small meshes, and a seeded sample of entries for meshes large enough to
@@ -61,6 +64,7 @@ def main() -> None:
     n, h, M = read_ovf2(REF / "Boris" / "vortex_hh_clk_up.ovf")
     np.savez_compressed(HERE / "vortex_hh_clk_up.npz", M=M.astype(np.float32), n=np.array(n), h=np.array(h))
     shutil.copyfile(REF / "Boris" / "vortex_hh_clk_up.ovf", HERE / "vortex_hh_clk_up.ovf")
+    shutil.copyfile(REF / "MLConv Paper" / "3D" / "trilayer_switch_mconv.txt", HERE / "trilayer_switch_mconv.txt")
 
     if not REF_DUMP.exists():
         sys.exit("oracle/_ref/ref_dump is not built: make -f oracle/Makefile.ref")
