@@ -50,7 +50,7 @@ def test_rows_reproduce_reference_values():
         row = [v[0], v[1:4], v[4:7], v[7:10]]
         assert DT.DataTable.format_row(cols, row) == line
         count += 1
-    assert count >= 290
+    assert count == 299
 
 
 def test_engineering_units():
