@@ -34,3 +34,21 @@ def test_product_sources_do_not_reference_the_oracle():
         if p.suffix in (".py", ".hip", ".cpp", ".h"):
             txt = p.read_text()
             assert "oracle" not in txt.lower(), p
+
+
+def test_unsupported_sizes_are_rejected_before_any_device_work():
+    """Sizes without an FFT plan or outside the supported range return BM_ERR_SIZE / BM_ERR_ARG
+    (checked before any allocation, so this runs without a device)."""
+    import ctypes as C
+    L = _lib.load()
+    h = C.c_void_p()
+    cases = [((3, 8, 1), 2),        # nx odd
+             ((10, 8, 1), 2),       # nx without a plan (2 * 5)
+             ((8, 5, 1), 2),        # N.y = 10 without a plan
+             ((8, 8, 65), 2),       # N.z = 256 > 128
+             ((4096, 8, 1), 2),     # nx above 2048
+             ((0, 8, 1), 1), ((8, 0, 1), 1)]
+    for n, code in cases:
+        assert L.bm_demag_create(*n, 4e-9, 4e-9, 4e-9, 0, C.byref(h)) == code, n
+        assert h.value is None
+    assert L.bm_demag_create(8, 8, 1, -4e-9, 4e-9, 4e-9, 0, C.byref(h)) == 1
