@@ -16,7 +16,7 @@ BM_MOD_DEMAG, BM_MOD_EXCHANGE, BM_MOD_ANIS_UNI, BM_MOD_ZEEMAN = 1, 2, 4, 8
 # every entry point declared in include/boris_mi355x.h
 EXPORTS = [
     "bm_last_error", "bm_device_ok",
-    "bm_demag_create", "bm_demag_destroy", "bm_demag_info", "bm_demag_convolute",
+    "bm_demag_create", "bm_demag_create_pbc", "bm_demag_destroy", "bm_demag_info", "bm_demag_convolute",
     "bm_demag_convolute_planar", "bm_demag_copy_kernels", "bm_demag_convolute_averaged",
     "bm_ode_create", "bm_ode_destroy", "bm_ode_set_ms_map", "bm_ode_set_params", "bm_ode_set_layers", "bm_ode_set_dt",
     "bm_ode_set_astep", "bm_ode_set_eval_speedup", "bm_ode_get_speedup_state", "bm_ode_advance_time", "bm_ode_run_to_iteration", "bm_ode_iterate_stage",
@@ -72,6 +72,7 @@ def load() -> C.CDLL:
         "bm_last_error": (C.c_char_p, []),
         "bm_device_ok": (i, []),
         "bm_demag_create": (i, [i, i, i, d, d, d, i, C.POINTER(vp)]),
+        "bm_demag_create_pbc": (i, [i, i, i, d, d, d, i, i, i, i, C.POINTER(vp)]),
         "bm_demag_destroy": (i, [vp]),
         "bm_demag_info": (i, [vp, C.POINTER(i), C.POINTER(ll)]),
         "bm_demag_convolute": (i, [vp, vp, vp, i, vp, vp]),

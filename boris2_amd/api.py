@@ -47,15 +47,20 @@ def require_device() -> None:
 
 
 class Demag:
-    """FFT demag convolution of one nx*ny*nz mesh of cells h (m)."""
+    """FFT demag convolution of one nx*ny*nz mesh of cells h (m).  pbc: periodic images
+    along x, y, z (0 = open boundary), as the reference's `pbc` command sets them."""
 
-    def __init__(self, n: tuple[int, int, int], h: tuple[float, float, float]):
+    def __init__(self, n: tuple[int, int, int], h: tuple[float, float, float], pbc: tuple[int, int, int] = (0, 0, 0)):
         require_device()
         self.L = _lib.load()
         self.n = tuple(int(v) for v in n)
         self.h = tuple(float(v) for v in h)
+        self.pbc = tuple(int(v) for v in pbc)
         self.handle = C.c_void_p()
-        check(self.L.bm_demag_create(*self.n, *self.h, 0, C.byref(self.handle)))
+        if any(self.pbc):
+            check(self.L.bm_demag_create_pbc(*self.n, *self.h, *self.pbc, 0, C.byref(self.handle)))
+        else:
+            check(self.L.bm_demag_create(*self.n, *self.h, 0, C.byref(self.handle)))
 
     @property
     def ncells(self) -> int:

@@ -47,27 +47,30 @@ constexpr int kBlock = 256;
 // P1: x-direction real-to-complex FFT of TY rows of one z plane (all three components, so M
 // is read from HBM once), through LDS into Sx.  With ADV it also performs the
 // RKDP54 advance on the way: sM1 = M, Mout = M + E0 dT/5, and transforms Mout.
+// FULL (periodic x, N.x = nx): a row holds 2 HX cells and every packed pair carries data;
+// otherwise a row holds HX cells and the upper half of the packed pairs is the zero padding.
 
-template <int HX, int TY, bool ADV>
+template <int HX, int TY, bool ADV, bool FULL = false>
 __global__ void __launch_bounds__(kBlock) k_fwd_x(FwdArgs a, const FwdArgs* __restrict__ bt)
 {
 	if (bt) a = bt[blockIdx.y];     // batched: one mesh per grid row
 	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, RPITCH = xrow_pitch<HX>(), NG = TY / RP;
+	constexpr int EL = FULL ? E : E / 2, NXR = FULL ? 2 * HX : HX;   // pairs per thread, cells per row
 	__shared__ C2<float> tile[TY * RPITCH];
 	XThread<HX> th; th.init();
 	const int nwork = a.nz * a.ntiles;
 	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
 		const int z = w % a.nz, y0 = (w / a.nz) * TY;
 		// cells 2k and 2k+1 of each of this thread's rows: 6 floats = 3 aligned float2
-		float2 m[NG][E / 2][3];
+		float2 m[NG][EL][3];
 #pragma unroll
 		for (int g = 0; g < NG; g++) {
 			const int y = y0 + g * RP + th.rl;
 			const bool ok = y < a.ny;
-			const size_t rowoff = ((size_t)z * a.ny + (ok ? y : 0)) * (size_t)HX * 3;
+			const size_t rowoff = ((size_t)z * a.ny + (ok ? y : 0)) * (size_t)NXR * 3;
 			const float2* row = reinterpret_cast<const float2*>(a.M + rowoff);
 #pragma unroll
-			for (int e = 0; e < E / 2; e++) {
+			for (int e = 0; e < EL; e++) {
 				const int k = th.t + T * e;
 #pragma unroll
 				for (int q = 0; q < 3; q++) m[g][e][q] = ok ? row[3 * k + q] : make_float2(0.f, 0.f);
@@ -92,7 +95,7 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x(FwdArgs a, const FwdArgs* __re
 				C2<float> v[E];
 #pragma unroll
 				for (int e = 0; e < E; e++) {
-					if (e < E / 2) {
+					if (e < EL) {
 						// component c of cell 2k is float c of the 6; of cell 2k+1 float 3 + c
 						const float2 p = m[g][e][0], q = m[g][e][1], r = m[g][e][2];
 						const float x0 = (c == 0) ? p.x : (c == 1) ? p.y : q.x;
@@ -112,8 +115,9 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x(FwdArgs a, const FwdArgs* __re
 
 // ---------------------------------------------------------------------------------------
 // P5: x-direction complex-to-real inverse FFT.  Output modes: 0 planar set, 1 AoS set,
-// 2 AoS add; EN also forms per-work-item partial sums of M.H for the energy.
-template <int HX, int MODE, bool EN, int TY>
+// 2 AoS add; EN also forms per-work-item partial sums of M.H for the energy.  FULL as in P1:
+// all 2 HX outputs of a row are cells (periodic x), otherwise the first HX.
+template <int HX, int MODE, bool EN, int TY, bool FULL = false>
 __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ Sx, float* __restrict__ out,
 	int ny, int nz, int ntl, int ntiles, float scale, const float* __restrict__ M, double* __restrict__ epart,
 	const PassBatch* __restrict__ bt)
@@ -123,7 +127,8 @@ __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ Sx,
 	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, RPITCH = xrow_pitch<HX>();
 	__shared__ C2<float> tile[TY * RPITCH];
 	XThread<HX> th; th.init();
-	const int nx = HX;
+	constexpr int EL = FULL ? E : E / 2;
+	const int nx = FULL ? 2 * HX : HX;
 	const size_t ncell = (size_t)nx * ny * nz;
 	// the next work item's tile is loaded into registers while this one is transformed
 	XTileRegs<HX, TY> pre;
@@ -146,7 +151,7 @@ __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ Sx,
 			if (ok) {
 				const size_t rowc = ((size_t)z * ny + y) * nx;
 #pragma unroll
-				for (int e = 0; e < E / 2; e++) {
+				for (int e = 0; e < EL; e++) {
 					const int n2 = 2 * (th.t + T * e);
 					float a = v[e].x * scale, b = v[e].y * scale;
 					if (MODE == 0) {
@@ -180,12 +185,13 @@ __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ Sx,
 // runs and its Sx values as kLT-value pieces, the block's waves together covering runs of
 // LP * kLT consecutive Sx values.  (Measured at 512x512x32: staging the Sx side through LDS
 // so that each wave moves whole runs, or interleaving the lanes over the lines, is slower.)
-template <int NY, int DIR>
+// FULL (periodic y, N.y = ny): all NY values of a line are data rows.
+template <int NY, int DIR, bool FULL = false>
 __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, float2* __restrict__ Sx, int nlines, int Sp, int NC, int ntl,
 	const PassBatch* __restrict__ bt)
 {
 	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; Sx = b.Sx; nlines = b.nlines; Sp = b.Sp; NC = b.NC; ntl = b.ntl; }
-	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = kBlock / T, LL = lds_len<NY>();
+	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = kBlock / T, LL = lds_len<NY>(), EL = FULL ? E : E / 2;
 	__shared__ C2<float> lds[LP * LL];
 	const int t = threadIdx.x % T, l = threadIdx.x / T;
 	Twiddles<NY, float> tw; tw.init(t);
@@ -198,7 +204,7 @@ __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, float2* __
 		if constexpr (DIR < 0) {
 			const int cz = ok ? line / NC : 0, kx = ok ? line % NC : 0;
 #pragma unroll
-			for (int e = 0; e < E; e++) pre[e] = (ok && e < E / 2) ? Sx[sx_line(cz, kx, t + T * e, NC, ntl)] : make_float2(0.f, 0.f);
+			for (int e = 0; e < E; e++) pre[e] = (ok && e < EL) ? Sx[sx_line(cz, kx, t + T * e, NC, ntl)] : make_float2(0.f, 0.f);
 		}
 		else {
 			const float2* q = S + (size_t)(ok ? line : 0) * Sp;
@@ -223,7 +229,7 @@ __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, float2* __
 			else {
 				const int cz = line / NC, kx = line % NC;
 #pragma unroll
-				for (int e = 0; e < E / 2; e++) Sx[sx_line(cz, kx, t + T * e, NC, ntl)] = make_float2(v[e].x, v[e].y);
+				for (int e = 0; e < EL; e++) Sx[sx_line(cz, kx, t + T * e, NC, ntl)] = make_float2(v[e].x, v[e].y);
 			}
 		}
 	}
@@ -239,12 +245,14 @@ __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, float2* __
 // kyr in [CH*b, CH*b + CH]: that slab (all kz, 6 components, contiguous because the 3D
 // kernel is stored kz fastest) is read from HBM once into LDS and serves the kz and N.z-kz
 // mirrors of every column.
-template <int NZ>
+// FULL (periodic z, N.z = nz): all NZ planes hold data.
+template <int NZ, bool FULL = false>
 __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const float* __restrict__ K,
 	int nz, int NC, int Ny, int NYH, int Sp, int xmap, const PassBatch* __restrict__ bt)
 {
 	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; K = b.K; nz = b.nz; NC = b.NC; Ny = b.Ny; NYH = b.NYH; Sp = b.Sp; }
 	constexpr int E = Plan<NZ>::E, TZ = plan_T<NZ>(), CY = kBlock / TZ, CH = CY / 2, NZH = NZ / 2 + 1, KR = CH + 1;
+	constexpr int EL = FULL ? E : E / 2;
 	constexpr int KS = 6 * KR * NZH, NKL = (KS + kBlock - 1) / kBlock;
 	__shared__ C2<float> lds[3 * NZ * CY];
 	__shared__ float ks[KS];
@@ -258,7 +266,7 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 
 	// Work item w: its columns' data (registers) and its kernel slab (registers, then LDS)
 	// are fetched one item ahead, so loads overlap the transforms of the current item.
-	float2 pd[3][E / 2];
+	float2 pd[3][EL];
 	float pk[NKL];
 	auto fetch = [&](int wf) {
 		const int kx = wf / nwb, k0 = CH * (wf % nwb);
@@ -268,7 +276,7 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 #pragma unroll
 		for (int cc = 0; cc < 3; cc++)
 #pragma unroll
-			for (int e = 0; e < E / 2; e++) {
+			for (int e = 0; e < EL; e++) {
 				const int z = t + TZ * e;
 				pd[cc][e] = (ok && z < nz) ? p[cc * comp + z * plane] : make_float2(0.f, 0.f);
 			}
@@ -293,7 +301,7 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 #pragma unroll
 		for (int cc = 0; cc < 3; cc++)
 #pragma unroll
-			for (int e = 0; e < E; e++) v[cc][e] = (e < E / 2) ? mk(pd[cc][e].x, pd[cc][e].y) : mk(0.f, 0.f);
+			for (int e = 0; e < E; e++) v[cc][e] = (e < EL) ? mk(pd[cc][e].x, pd[cc][e].y) : mk(0.f, 0.f);
 		__syncthreads();                                   // the previous slab is no longer read
 #pragma unroll
 		for (int q = 0; q < NKL; q++) { const int i = threadIdx.x + kBlock * q; if (i < KS) ks[i] = pk[q]; }
@@ -320,7 +328,7 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 #pragma unroll
 			for (int cc = 0; cc < 3; cc++)
 #pragma unroll
-				for (int e = 0; e < E / 2; e++) {
+				for (int e = 0; e < EL; e++) {
 					const int z = t + TZ * e;
 					if (z < nz) p[cc * comp + z * plane] = make_float2(v[cc][e].x, v[cc][e].y);
 				}
@@ -331,12 +339,12 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 // ---------------------------------------------------------------------------------------
 // P2' (2D): per kx line of Sx (in place), y-FFT of 3 components, 2D kernel multiply (Kxy
 // odd in y), y-IFFT.  A block takes LP consecutive lines, one per group of T lanes.
-template <int NY, int BS>
+template <int NY, int BS, bool FULL = false>
 __global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ Sx, const float* __restrict__ K, int NC, int NYH, int ntl,
 	const PassBatch* __restrict__ bt)
 {
 	if (bt) { const PassBatch& b = bt[blockIdx.y]; Sx = b.Sx; K = b.K; NC = b.NC; NYH = b.NYH; ntl = b.ntl; }
-	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = BS / T, LL = lds_len<NY>();
+	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = BS / T, LL = lds_len<NY>(), EL = FULL ? E : E / 2;
 	__shared__ C2<float> lds[3 * LP * LL];
 	const int t = threadIdx.x % T, l = threadIdx.x / T;
 	Twiddles<NY, float> tw; tw.init(t);
@@ -351,7 +359,7 @@ __global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ Sx, const float
 		for (int c = 0; c < 3; c++)
 #pragma unroll
 			for (int e = 0; e < E; e++) {
-				const float2 a = (ok && e < E / 2) ? Sx[sx_line(c, kxs, t + T * e, NC, ntl)] : make_float2(0.f, 0.f);
+				const float2 a = (ok && e < EL) ? Sx[sx_line(c, kxs, t + T * e, NC, ntl)] : make_float2(0.f, 0.f);
 				v[c][e] = mk(a.x, a.y);
 			}
 		C2<float>* b0 = lds + l * LL;
@@ -373,7 +381,7 @@ __global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ Sx, const float
 #pragma unroll
 			for (int c = 0; c < 3; c++)
 #pragma unroll
-				for (int e = 0; e < E / 2; e++) Sx[sx_line(c, kx, t + T * e, NC, ntl)] = make_float2(v[c][e].x, v[c][e].y);
+				for (int e = 0; e < EL; e++) Sx[sx_line(c, kx, t + T * e, NC, ntl)] = make_float2(v[c][e].x, v[c][e].y);
 		}
 	}
 }
@@ -432,6 +440,7 @@ struct TensorArgs {
 	int gdim[6][2];
 	AsymDiag ad[3];
 	AsymOff ao[3];
+	int img[3];             // periodic images per axis (k_tensor_pbc)
 };
 
 // Writes the 6 (3D) or 4 (2D) tensor planes D[q][Nz][Ny][Nx] (-N convention).  The
@@ -481,6 +490,65 @@ __global__ void k_tensor(TensorArgs a, double* D, int is2d)
 			D[idx] = vd[0]; D[NN + idx] = vd[1]; D[2 * NN + idx] = vd[2];
 			D[3 * NN + idx] = vo[0] * (si * sj);
 			if (!is2d) { D[4 * NN + idx] = vo[1] * (si * sk); D[5 * NN + idx] = vo[2] * (sj * sk); }
+		}
+	}
+}
+
+// Periodic tensor (Boris/DemagTFunc_PBC.cpp:5-422), -N convention.  Along a periodic axis
+// N equals the cell count and each offset o in 0..N-1 sums the open-boundary element at the
+// image offsets o + k N, k = -p..p, in the reference's loop order (x images outermost); in
+// 2D the z images sit at whole multiples of the cell thickness.  Off-diagonal terms take
+// sign(offset) per axis with sign(0) = 0 (BorisLib/Funcs_Math.h:103).  Along an open axis
+// offsets 0..n-1 are computed and mirrored (the mirror of a nonzero offset never coincides
+// with another write).
+__device__ __forceinline__ int isign0(int v) { return (v > 0) - (v < 0); }
+
+__global__ void k_tensor_pbc(TensorArgs a, double* D, int is2d)
+{
+	const size_t NN = (size_t)a.Nx * a.Ny * a.Nz;
+	const size_t total = (size_t)a.nx * a.ny * a.nz;
+	const int d = kAsymDist, px = a.img[0], py = a.img[1], pz = a.img[2];
+	const double hx = a.hx, hy = a.hy, hz = a.hz;
+	for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+		const int i0 = (int)(q % a.nx), j0 = (int)((q / a.nx) % a.ny), k0 = (int)(q / ((size_t)a.nx * a.ny));
+		double vd[3] = { 0.0, 0.0, 0.0 }, vo[3] = { 0.0, 0.0, 0.0 };
+		for (int ii = -px; ii <= px; ii++)
+			for (int jj = -py; jj <= py; jj++)
+				for (int kk = -pz; kk <= pz; kk++) {
+					const int is = ii * a.Nx + i0, js = jj * a.Ny + j0, ks = is2d ? kk : kk * a.Nz + k0;
+					const double si = isign0(is), sj = isign0(js), sk = isign0(ks);
+					const int i = abs(is), j = abs(js), k = abs(ks);
+					if (i >= d || j >= d || k >= d || i * i + j * j + k * k >= d * d) {
+						vd[0] += -asym_diag(a.ad[0], i * hx, j * hy, k * hz);
+						vd[1] += -asym_diag(a.ad[1], j * hy, i * hx, k * hz);
+						vd[2] += -asym_diag(a.ad[2], k * hz, j * hy, i * hx);
+						vo[0] += -asym_off(a.ao[0], i * hx, j * hy, k * hz) * (si * sj);
+						if (!is2d) {
+							vo[1] += -asym_off(a.ao[1], i * hx, k * hz, j * hy) * (si * sk);
+							vo[2] += -asym_off(a.ao[2], j * hy, k * hz, i * hx) * (sj * sk);
+						}
+					}
+					else {
+						vd[0] += -stencil27(a.grid[0], a.gdim[0][0], a.gdim[0][1], i, j, k, hx, hy, hz);
+						vd[1] += -stencil27(a.grid[1], a.gdim[1][0], a.gdim[1][1], j, i, k, hy, hx, hz);
+						vd[2] += -stencil27(a.grid[2], a.gdim[2][0], a.gdim[2][1], k, j, i, hz, hy, hx);
+						vo[0] += -stencil27(a.grid[3], a.gdim[3][0], a.gdim[3][1], i, j, k, hx, hy, hz) * (si * sj);
+						if (!is2d) {
+							vo[1] += -stencil27(a.grid[4], a.gdim[4][0], a.gdim[4][1], i, k, j, hx, hz, hy) * (si * sk);
+							vo[2] += -stencil27(a.grid[5], a.gdim[5][0], a.gdim[5][1], j, k, i, hy, hz, hx) * (sj * sk);
+						}
+					}
+				}
+		const bool fx = !px && i0, fy = !py && j0, fz = !is2d && !pz && k0;
+		for (int m = 0; m < 8; m++) {
+			const int mx = m & 1, my = (m >> 1) & 1, mz = (m >> 2) & 1;
+			if ((mx && !fx) || (my && !fy) || (mz && !fz)) continue;
+			const int sx = mx ? -1 : 1, sy = my ? -1 : 1, sz = mz ? -1 : 1;
+			const size_t idx = (size_t)((sx * i0 + a.Nx) % a.Nx) + (size_t)((sy * j0 + a.Ny) % a.Ny) * a.Nx
+				+ (size_t)((sz * k0 + a.Nz) % a.Nz) * a.Nx * a.Ny;
+			D[idx] = vd[0]; D[NN + idx] = vd[1]; D[2 * NN + idx] = vd[2];
+			D[3 * NN + idx] = vo[0] * (sx * sy);
+			if (!is2d) { D[4 * NN + idx] = vo[1] * (sx * sz); D[5 * NN + idx] = vo[2] * (sy * sz); }
 		}
 	}
 }
@@ -551,16 +619,20 @@ int xtile_factor(bool fwd)
 	return v;
 }
 
-int launch_fwd_x(int HX, const float* M, float2* Sx, int ny, int nz, int ntl, int ncu, hipStream_t st,
+// full: periodic x (rows of 2 HX cells, no zero padding); only the plain transform exists
+// for it (no fused RKDP54 advance, no taller tile)
+int launch_fwd_x(int HX, bool full, const float* M, float2* Sx, int ny, int nz, int ntl, int ncu, hipStream_t st,
 	const float* E0 = nullptr, float* sM1 = nullptr, float* Mout = nullptr, float dT5 = 0.f)
 {
-	const int f = xtile_factor(true);
-	FwdArgs a{ M, Sx, ny, nz, ntl, 0, E0, sM1, Mout, dT5 };
 	const bool adv = E0 != nullptr;
+	if (full && adv) return fail(BM_ERR_STATE, "fused advance with a periodic x axis");
+	const int f = full ? 1 : xtile_factor(true);
+	FwdArgs a{ M, Sx, ny, nz, ntl, 0, E0, sM1, Mout, dT5 };
 	switch (HX) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; int TY = TY1 * f; a.ntiles = (ny + TY - 1) / TY; dim3 g(grid_for(nz * a.ntiles, ncu, 2)); \
 		const FwdArgs* nb = nullptr; \
-		if (f == 1 && !adv) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false>), g, dim3(kBlock), 0, st, a, nb); \
+		if (full) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false, true>), g, dim3(kBlock), 0, st, a, nb); \
+		else if (f == 1 && !adv) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false>), g, dim3(kBlock), 0, st, a, nb); \
 		else if (f == 1) hipLaunchKernelGGL((k_fwd_x<V_, TY1, true>), g, dim3(kBlock), 0, st, a, nb); \
 		else if (!adv) hipLaunchKernelGGL((k_fwd_x<V_, 2 * TY1, false>), g, dim3(kBlock), 0, st, a, nb); \
 		else hipLaunchKernelGGL((k_fwd_x<V_, 2 * TY1, true>), g, dim3(kBlock), 0, st, a, nb); return BM_OK; }
@@ -570,13 +642,27 @@ int launch_fwd_x(int HX, const float* M, float2* Sx, int ny, int nz, int ntl, in
 	return fail(BM_ERR_SIZE, "no x plan");
 }
 
-int launch_inv_x(int HX, int mode, const float2* Sx, float* out, int ny, int nz, int ntl, float scale, int ncu, hipStream_t st,
+// full: periodic x, as in launch_fwd_x
+template <int HX, int TY1>
+void launch_inv_x_full(int mode, dim3 g, hipStream_t st, const float2* Sx, float* out, int ny, int nz, int ntl, int nt, float scale,
+	const float* M, double* epart)
+{
+	const PassBatch* nb = nullptr;
+	if (mode == 0) hipLaunchKernelGGL((k_inv_x<HX, 0, false, TY1, true>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nb);
+	else if (mode == 1 && !epart) hipLaunchKernelGGL((k_inv_x<HX, 1, false, TY1, true>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nb);
+	else if (mode == 1) hipLaunchKernelGGL((k_inv_x<HX, 1, true, TY1, true>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nb);
+	else if (!epart) hipLaunchKernelGGL((k_inv_x<HX, 2, false, TY1, true>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nb);
+	else hipLaunchKernelGGL((k_inv_x<HX, 2, true, TY1, true>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nb);
+}
+
+int launch_inv_x(int HX, bool full, int mode, const float2* Sx, float* out, int ny, int nz, int ntl, float scale, int ncu, hipStream_t st,
 	const float* M = nullptr, double* epart = nullptr)
 {
-	const int f = (mode == 0) ? xtile_factor(false) : 1;
+	const int f = (mode == 0 && !full) ? xtile_factor(false) : 1;
 	switch (HX) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; int TY = TY1 * f, nt = (ny + TY - 1) / TY; dim3 g(grid_for(3 * nz * nt, ncu, 2)); \
-		if (mode == 0 && f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
+		if (full) launch_inv_x_full<V_, TY1>(mode, g, st, Sx, out, ny, nz, ntl, nt, scale, M, epart); \
+		else if (mode == 0 && f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
 		else if (mode == 0) hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
 		else if (mode == 1 && !epart) hipLaunchKernelGGL((k_inv_x<V_, 1, false, TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
 		else if (mode == 1) hipLaunchKernelGGL((k_inv_x<V_, 1, true, TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
@@ -601,40 +687,46 @@ int inv_x_nwork(int HX, int ny, int nz)
 // z-pass work order switch for tuning runs: BM_Z_XCD=1 selects the XCD-contiguous order.
 int z_xcd() { static int v = getenv("BM_Z_XCD") ? atoi(getenv("BM_Z_XCD")) : 0; return v; }
 
-template <int NY>
+template <int NY, bool FULL = false>
 void launch_y_t(int dir, dim3 g, hipStream_t st, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, const PassBatch* bt)
 {
-	if (dir < 0) hipLaunchKernelGGL((k_y<NY, -1>), g, dim3(kBlock), 0, st, S, Sx, nlines, Sp, NC, ntl, bt);
-	else hipLaunchKernelGGL((k_y<NY, 1>), g, dim3(kBlock), 0, st, S, Sx, nlines, Sp, NC, ntl, bt);
+	if (dir < 0) hipLaunchKernelGGL((k_y<NY, -1, FULL>), g, dim3(kBlock), 0, st, S, Sx, nlines, Sp, NC, ntl, bt);
+	else hipLaunchKernelGGL((k_y<NY, 1, FULL>), g, dim3(kBlock), 0, st, S, Sx, nlines, Sp, NC, ntl, bt);
 }
 
-int launch_y(int NY, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ncu, hipStream_t st)
+// full: periodic y (N.y = ny)
+int launch_y(int NY, bool full, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ncu, hipStream_t st)
 {
 	switch (NY) {
-#define C_(V_) case V_: { int LP = kBlock / plan_T<V_>(); \
-		launch_y_t<V_>(dir, dim3(grid8((nlines + LP - 1) / LP, ncu, 8)), st, S, Sx, nlines, Sp, NC, ntl, nullptr); return BM_OK; }
+#define C_(V_) case V_: { int LP = kBlock / plan_T<V_>(); dim3 g(grid8((nlines + LP - 1) / LP, ncu, 8)); \
+		if (full) launch_y_t<V_, true>(dir, g, st, S, Sx, nlines, Sp, NC, ntl, nullptr); \
+		else launch_y_t<V_>(dir, g, st, S, Sx, nlines, Sp, NC, ntl, nullptr); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no y plan");
 }
 
-int launch_z(int NZ, float2* S, const float* K, int nz, int NC, int Ny, int NYH, int Sp, int ncu, hipStream_t st)
+// full: periodic z (N.z = nz)
+int launch_z(int NZ, bool full, float2* S, const float* K, int nz, int NC, int Ny, int NYH, int Sp, int ncu, hipStream_t st)
 {
 	switch (NZ) {
-#define C_(V_) case V_: { int CH = kBlock / plan_T<V_>() / 2; int nw = NC * ((Ny / 2 + CH - 1) / CH); \
-		hipLaunchKernelGGL(k_z<V_>, dim3(grid8(nw, ncu, 2)), dim3(kBlock), 0, st, S, K, nz, NC, Ny, NYH, Sp, z_xcd(), (const PassBatch*)nullptr); return BM_OK; }
+#define C_(V_) case V_: { int CH = kBlock / plan_T<V_>() / 2; int nw = NC * ((Ny / 2 + CH - 1) / CH); dim3 g(grid8(nw, ncu, 2)); \
+		if (full) hipLaunchKernelGGL((k_z<V_, true>), g, dim3(kBlock), 0, st, S, K, nz, NC, Ny, NYH, Sp, z_xcd(), (const PassBatch*)nullptr); \
+		else hipLaunchKernelGGL((k_z<V_>), g, dim3(kBlock), 0, st, S, K, nz, NC, Ny, NYH, Sp, z_xcd(), (const PassBatch*)nullptr); return BM_OK; }
 	BM_FOR_Z_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no z plan");
 }
 
-int launch_y2d(int NY, float2* Sx, const float* K, int NC, int NYH, int ntl, int ncu, hipStream_t st)
+// full: periodic y (N.y = ny)
+int launch_y2d(int NY, bool full, float2* Sx, const float* K, int NC, int NYH, int ntl, int ncu, hipStream_t st)
 {
 	switch (NY) {
-#define C_(V_) case V_: { constexpr int BS = Plan<V_>::E > 8 ? 128 : 256; int LP = BS / plan_T<V_>(); \
-		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid8((NC + LP - 1) / LP, ncu, 8)), dim3(BS), 0, st, Sx, K, NC, NYH, ntl, (const PassBatch*)nullptr); return BM_OK; }
+#define C_(V_) case V_: { constexpr int BS = Plan<V_>::E > 8 ? 128 : 256; int LP = BS / plan_T<V_>(); dim3 g(grid8((NC + LP - 1) / LP, ncu, 8)); \
+		if (full) hipLaunchKernelGGL((k_y2d<V_, BS, true>), g, dim3(BS), 0, st, Sx, K, NC, NYH, ntl, (const PassBatch*)nullptr); \
+		else hipLaunchKernelGGL((k_y2d<V_, BS>), g, dim3(BS), 0, st, Sx, K, NC, NYH, ntl, (const PassBatch*)nullptr); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -665,11 +757,16 @@ int compute_kernels(bm_demag_s* d, hipStream_t st)
 	double hm = std::max(d->h[0], std::max(d->h[1], d->h[2]));
 	double hx = d->h[0] / hm, hy = d->h[1] / hm, hz = d->h[2] / hm;
 	const int dd = kAsymDist;
-	int dx = std::min(nx, dd), dy = std::min(ny, dd), dz = std::min(nz, dd);
+	// basis-function grids: the asymptotic distance along periodic axes (images reach it),
+	// the cell count (at most that distance) along open ones
+	const int dx = d->pbc[0] ? dd : std::min(nx, dd), dy = d->pbc[1] ? dd : std::min(ny, dd);
+	const int dz = d->pbc[2] ? dd : std::min(nz, dd);
+	const bool periodic = d->pbc[0] || d->pbc[1] || d->pbc[2];
 
 	TensorArgs a;
 	std::memset(&a, 0, sizeof(a));
 	a.nx = nx; a.ny = ny; a.nz = nz; a.Nx = Nx; a.Ny = Ny; a.Nz = Nz; a.hx = hx; a.hy = hy; a.hz = hz;
+	for (int q = 0; q < 3; q++) a.img[q] = d->pbc[q];
 	// grid extents (offsets -1..d) and the argument scalings of each basis-function grid
 	const int gd[6][3] = { {dx,dy,dz},{dy,dx,dz},{dz,dy,dx},{dx,dy,dz},{dx,dz,dy},{dy,dz,dx} };
 	const double gh[6][3] = { {hx,hy,hz},{hy,hx,hz},{hz,hy,hx},{hx,hy,hz},{hx,hz,hy},{hy,hz,hx} };
@@ -698,7 +795,9 @@ int compute_kernels(bm_demag_s* d, hipStream_t st)
 	if (hipMemsetAsync(D, 0, NN * ncomp * sizeof(double), st) != hipSuccess) { cleanup(); return fail(BM_ERR_HIP, "memset"); }
 	{
 		size_t total = (size_t)nx * ny * nz;
-		hipLaunchKernelGGL(k_tensor, dim3((unsigned)std::min<size_t>((total + 127) / 128, 65535)), dim3(128), 0, st, a, D, d->is2d ? 1 : 0);
+		const dim3 g((unsigned)std::min<size_t>((total + 127) / 128, 65535));
+		if (periodic) hipLaunchKernelGGL(k_tensor_pbc, g, dim3(128), 0, st, a, D, d->is2d ? 1 : 0);
+		else hipLaunchKernelGGL(k_tensor, g, dim3(128), 0, st, a, D, d->is2d ? 1 : 0);
 	}
 	if (hipMalloc(&T1, (size_t)NC * Ny * Nz * sizeof(double)) != hipSuccess) { cleanup(); return fail(BM_ERR_NOMEM, "tensor scratch"); }
 	if (!d->is2d && hipMalloc(&T2, (size_t)NC * NYH * Nz * sizeof(double)) != hipSuccess) { cleanup(); return fail(BM_ERR_NOMEM, "tensor scratch"); }
@@ -740,7 +839,7 @@ int compute_kernels(bm_demag_s* d, hipStream_t st)
 int bm::demag_fwd_x(bm_demag_s* d, const float* M, hipStream_t st)
 {
 	prof_mark_begin(PK_FWD_X, st);
-	int rc = launch_fwd_x(d->nx, M, d->Sx, d->ny, d->nz, d->ntl, d->ncu, st);
+	int rc = launch_fwd_x(d->Nx / 2, d->full[0], M, d->Sx, d->ny, d->nz, d->ntl, d->ncu, st);
 	prof_mark_end(PK_FWD_X, st);
 	return rc;
 }
@@ -749,7 +848,7 @@ int bm::demag_fwd_x(bm_demag_s* d, const float* M, hipStream_t st)
 int bm::demag_fwd_x_advance(bm_demag_s* d, const float* M, const float* E0, float* sM1, float* Mout, float dT5, hipStream_t st)
 {
 	prof_mark_begin(PK_FWD_ADV, st);
-	int rc = launch_fwd_x(d->nx, M, d->Sx, d->ny, d->nz, d->ntl, d->ncu, st, E0, sM1, Mout, dT5);
+	int rc = launch_fwd_x(d->Nx / 2, d->full[0], M, d->Sx, d->ny, d->nz, d->ntl, d->ncu, st, E0, sM1, Mout, dT5);
 	prof_mark_end(PK_FWD_ADV, st);
 	return rc;
 }
@@ -762,20 +861,20 @@ int bm::demag_mid(bm_demag_s* d, hipStream_t st)
 	int rc;
 	if (d->is2d) {
 		prof_mark_begin(PK_Y2D, st);
-		rc = launch_y2d(d->Ny, d->Sx, d->K, d->NC, d->NYH, d->ntl, d->ncu, st);
+		rc = launch_y2d(d->Ny, d->full[1], d->Sx, d->K, d->NC, d->NYH, d->ntl, d->ncu, st);
 		prof_mark_end(PK_Y2D, st);
 		return rc;
 	}
 	prof_mark_begin(PK_Y_FWD, st);
-	rc = launch_y(d->Ny, -1, d->S, d->Sx, nlines, d->Sp, d->NC, d->ntl, d->ncu, st);
+	rc = launch_y(d->Ny, d->full[1], -1, d->S, d->Sx, nlines, d->Sp, d->NC, d->ntl, d->ncu, st);
 	prof_mark_end(PK_Y_FWD, st);
 	if (rc) return rc;
 	prof_mark_begin(PK_Z, st);
-	rc = launch_z(d->Nz, d->S, d->K, d->nz, d->NC, d->Ny, d->NYH, d->Sp, d->ncu, st);
+	rc = launch_z(d->Nz, d->full[2], d->S, d->K, d->nz, d->NC, d->Ny, d->NYH, d->Sp, d->ncu, st);
 	prof_mark_end(PK_Z, st);
 	if (rc) return rc;
 	prof_mark_begin(PK_Y_INV, st);
-	rc = launch_y(d->Ny, 1, d->S, d->Sx, nlines, d->Sp, d->NC, d->ntl, d->ncu, st);
+	rc = launch_y(d->Ny, d->full[1], 1, d->S, d->Sx, nlines, d->Sp, d->NC, d->ntl, d->ncu, st);
 	prof_mark_end(PK_Y_INV, st);
 	return rc;
 }
@@ -793,28 +892,35 @@ int bm::demag_planar(bm_demag_s* d, const float* M, float* H3n, hipStream_t st)
 	if (rc) return rc;
 	const float scale = (float)(1.0 / ((double)d->Nx * d->Ny * d->Nz));
 	prof_mark_begin(PK_INV_X, st);
-	rc = launch_inv_x(d->nx, 0, d->Sx, H3n, d->ny, d->nz, d->ntl, scale, d->ncu, st);
+	rc = launch_inv_x(d->Nx / 2, d->full[0], 0, d->Sx, H3n, d->ny, d->nz, d->ntl, scale, d->ncu, st);
 	prof_mark_end(PK_INV_X, st);
 	return rc;
 }
 
-extern "C" int bm_demag_create(int nx, int ny, int nz, double hx, double hy, double hz, int flags, bm_demag_t* out)
+// Sizes (Boris/ConvolutionDataCUDA.cpp:24-100): N = n along a periodic axis, otherwise
+// N.x = 2 nx, N.y = 2 ny and N.z the next power of two >= 2 nz (N.z = 1 in 2D).
+static int demag_create(int nx, int ny, int nz, double hx, double hy, double hz, const int* pbc, bm_demag_t* out)
 {
-	(void)flags;
 	if (!out || nx < 2 || ny < 1 || nz < 1 || !(hx > 0) || !(hy > 0) || !(hz > 0)) return fail(BM_ERR_ARG, "bm_demag_create: bad arguments");
 	*out = nullptr;
+	if (pbc && (pbc[0] < 0 || pbc[1] < 0 || pbc[2] < 0)) return fail(BM_ERR_ARG, "bm_demag_create_pbc: negative image count");
 	if (nx % 2) return fail(BM_ERR_SIZE, "bm_demag_create: nx must be even");
 	bm_demag_s* d = new bm_demag_s();
 	d->nx = nx; d->ny = ny; d->nz = nz; d->h[0] = hx; d->h[1] = hy; d->h[2] = hz;
-	d->Nx = 2 * nx; d->Ny = 2 * ny;
-	d->Nz = 1;
-	if (nz > 1) { while (d->Nz < 2 * nz) d->Nz *= 2; }
+	for (int q = 0; q < 3; q++) d->pbc[q] = pbc ? pbc[q] : 0;
 	d->is2d = (nz == 1);
+	d->full[0] = d->pbc[0] > 0; d->full[1] = d->pbc[1] > 0; d->full[2] = !d->is2d && d->pbc[2] > 0;
+	d->Nx = d->full[0] ? nx : 2 * nx;
+	d->Ny = d->full[1] ? ny : 2 * ny;
+	d->Nz = 1;
+	if (d->full[2]) d->Nz = nz;
+	else if (nz > 1) { while (d->Nz < 2 * nz) d->Nz *= 2; }
 	d->NC = d->Nx / 2 + 1; d->NYH = d->Ny / 2 + 1; d->NZH = d->Nz / 2 + 1;
-	// nx is the half-length complex FFT of the x pass; Nx, Ny, Nz also need FP64 setup plans
-	if (!x_supported(nx) || !x_supported(d->Ny) || !line_supported(d->Nx) || (!d->is2d && !z_supported(d->Nz))) {
+	// N.x / 2 is the half-length complex FFT of the x passes; Nx, Ny, Nz also need FP64 setup plans
+	if (!x_supported(d->Nx / 2) || !x_supported(d->Ny) || !line_supported(d->Nx) || (!d->is2d && !z_supported(d->Nz))) {
 		delete d;
-		return fail(BM_ERR_SIZE, "bm_demag_create: no FFT plan for this mesh size (supported: nx in {2..2048 of 2^a 3^b plans}, Ny = 2ny likewise, Nz <= 128)");
+		return fail(BM_ERR_SIZE, "bm_demag_create: no FFT plan for this mesh size (supported: N.x / 2 and N.y in the "
+			"2^a 3^b plans 2..2048, N.z a power of two 4..128; N.x = 2 nx, N.y = 2 ny, N.z >= 2 nz, or N = n along a periodic axis)");
 	}
 	d->ncu = device_cu_count();
 	// line pitch of the full spectra (complex elements); BM_S_PAD adds padding for tuning experiments
@@ -824,7 +930,7 @@ extern "C" int bm_demag_create(int nx, int ny, int nz, double hx, double hy, dou
 	const size_t sxbytes = (size_t)3 * nz * d->ntl * d->NC * kLT * sizeof(float2);
 	const size_t sbytes = d->is2d ? 0 : (size_t)3 * nz * d->NC * d->Sp * sizeof(float2);
 	d->kcount = d->is2d ? 4LL * d->NC * d->NYH : 6LL * d->NZH * d->NC * d->NYH;
-	d->ecap = std::max(1, inv_x_nwork(nx, ny, nz));
+	d->ecap = std::max(1, inv_x_nwork(d->Nx / 2, ny, nz));
 	if (hipMalloc(&d->Sx, sxbytes) != hipSuccess || (sbytes && hipMalloc(&d->S, sbytes) != hipSuccess)
 		|| hipMalloc(&d->K, d->kcount * sizeof(float)) != hipSuccess || hipMalloc(&d->ebuf, (size_t)d->ecap * sizeof(double)) != hipSuccess) {
 		bm_demag_destroy(d);
@@ -836,6 +942,20 @@ extern "C" int bm_demag_create(int nx, int ny, int nz, double hx, double hy, dou
 	if (rc) { bm_demag_destroy(d); return rc; }
 	*out = d;
 	return BM_OK;
+}
+
+extern "C" int bm_demag_create(int nx, int ny, int nz, double hx, double hy, double hz, int flags, bm_demag_t* out)
+{
+	(void)flags;
+	return demag_create(nx, ny, nz, hx, hy, hz, nullptr, out);
+}
+
+extern "C" int bm_demag_create_pbc(int nx, int ny, int nz, double hx, double hy, double hz, int pbc_x, int pbc_y, int pbc_z,
+	int flags, bm_demag_t* out)
+{
+	(void)flags;
+	const int pbc[3] = { pbc_x, pbc_y, pbc_z };
+	return demag_create(nx, ny, nz, hx, hy, hz, pbc, out);
 }
 
 extern "C" int bm_demag_destroy(bm_demag_t d)
@@ -863,7 +983,7 @@ extern "C" int bm_demag_convolute(bm_demag_t d, const float* M, float* Heff, int
 {
 	if (!d || !M || !Heff) return fail(BM_ERR_ARG, "bm_demag_convolute: null argument");
 	hipStream_t st = (hipStream_t)stream;
-	const int HX = d->nx;
+	const int HX = d->Nx / 2;
 	int rc = forward_and_multiply(d, M, st);
 	if (rc) return rc;
 	const float scale = (float)(1.0 / ((double)d->Nx * d->Ny * d->Nz));
@@ -871,7 +991,7 @@ extern "C" int bm_demag_convolute(bm_demag_t d, const float* M, float* Heff, int
 		const int np = inv_x_nwork(HX, d->ny, d->nz);
 		if (np > d->ecap) return fail(BM_ERR_STATE, "energy partial buffer too small");
 		prof_mark_begin(PK_INV_X, st);
-		rc = launch_inv_x(HX, clear_out ? 1 : 2, d->Sx, Heff, d->ny, d->nz, d->ntl, scale, d->ncu, st, M, d->ebuf);
+		rc = launch_inv_x(HX, d->full[0], clear_out ? 1 : 2, d->Sx, Heff, d->ny, d->nz, d->ntl, scale, d->ncu, st, M, d->ebuf);
 		prof_mark_end(PK_INV_X, st);
 		if (rc) return rc;
 		size_t n = (size_t)d->nx * d->ny * d->nz;
@@ -879,7 +999,7 @@ extern "C" int bm_demag_convolute(bm_demag_t d, const float* M, float* Heff, int
 	}
 	else {
 		prof_mark_begin(PK_INV_X, st);
-		rc = launch_inv_x(HX, clear_out ? 1 : 2, d->Sx, Heff, d->ny, d->nz, d->ntl, scale, d->ncu, st);
+		rc = launch_inv_x(HX, d->full[0], clear_out ? 1 : 2, d->Sx, Heff, d->ny, d->nz, d->ntl, scale, d->ncu, st);
 		prof_mark_end(PK_INV_X, st);
 		if (rc) return rc;
 	}
@@ -929,7 +1049,7 @@ extern "C" int bm_demag_copy_kernels(bm_demag_t d, float* host_out, long long co
 int bm::pass_fwd_x(int nx, const float* M, float2* Sx, int ny, int nplanes, int ntl, int ncu, hipStream_t st)
 {
 	prof_mark_begin(PK_FWD_X, st);
-	int rc = launch_fwd_x(nx, M, Sx, ny, nplanes, ntl, ncu, st);
+	int rc = launch_fwd_x(nx, false, M, Sx, ny, nplanes, ntl, ncu, st);
 	prof_mark_end(PK_FWD_X, st);
 	return rc;
 }
@@ -938,7 +1058,7 @@ int bm::pass_y(int Ny, int dir, float2* S, float2* Sx, int nlines, int Sp, int N
 {
 	const int kind = dir < 0 ? PK_Y_FWD : PK_Y_INV;
 	prof_mark_begin(kind, st);
-	int rc = launch_y(Ny, dir, S, Sx, nlines, Sp, NC, ntl, ncu, st);
+	int rc = launch_y(Ny, false, dir, S, Sx, nlines, Sp, NC, ntl, ncu, st);
 	prof_mark_end(kind, st);
 	return rc;
 }
@@ -946,7 +1066,7 @@ int bm::pass_y(int Ny, int dir, float2* S, float2* Sx, int nlines, int Sp, int N
 int bm::pass_inv_x(int nx, int mode, const float2* Sx, float* out, int ny, int nplanes, int ntl, float scale, int ncu, hipStream_t st)
 {
 	prof_mark_begin(PK_INV_X, st);
-	int rc = launch_inv_x(nx, mode, Sx, out, ny, nplanes, ntl, scale, ncu, st);
+	int rc = launch_inv_x(nx, false, mode, Sx, out, ny, nplanes, ntl, scale, ncu, st);
 	prof_mark_end(PK_INV_X, st);
 	return rc;
 }

@@ -29,6 +29,8 @@ void self_demag_coeff(double hx, double hy, double hz, double out[3]);
 struct bm_demag_s {
 	int nx, ny, nz;
 	int Nx, Ny, Nz, NC, NYH, NZH;
+	int pbc[3] = { 0, 0, 0 }; // periodic images per axis (0: open boundary, N = 2n or more)
+	bool full[3] = { false, false, false };  // periodic axis: N = n, no zero-padded half
 	int Sp;                   // line pitch of the full spectra S (complex elements, >= Ny)
 	int ntl;                  // row tiles per plane of the x-side spectra Sx (bm_xpass.h)
 	double h[3];

@@ -31,7 +31,7 @@ const char* bm_last_error(void);
 int bm_device_ok(void);
 
 /* ------------------------------------------------------------------------------------
- * Demag convolution of one ferromagnetic mesh (no PBC).
+ * Demag convolution of one ferromagnetic mesh.
  *
  * Replaces: DemagCUDA::Initialize + ConvolutionDataCUDA::SetConvolutionDimensions +
  * DemagKernelCUDA::Calculate_Demag_Kernels (Boris/DemagCUDA.cpp:31,
@@ -43,6 +43,21 @@ int bm_device_ok(void);
 typedef struct bm_demag_s* bm_demag_t;
 
 int bm_demag_create(int nx, int ny, int nz, double hx, double hy, double hz, int flags, bm_demag_t* out);
+/*
+ * Periodic boundary conditions along the axes with a nonzero image count.
+ * Replaces: the same set-up with pbc_images (ConvolutionDataCUDA::SetConvolutionDimensions,
+ * Boris/ConvolutionDataCUDA.cpp:24-100; DemagKernelCUDA::Calculate_Demag_Kernels_2D/3D with
+ * DemagTFunc::CalcDiagTens2D/3D_PBC and CalcOffDiagTens2D/3D_PBC,
+ * Boris/DemagKernelCUDA.cpp:91-108/556-582, Boris/DemagTFunc_PBC.cpp:5-422), reached from
+ * the `pbc` console command (Demag::Set_PBC).  Along a periodic axis N = n (no zero padding)
+ * and the tensor sums the images -p..p of the source; in 2D, z images sit at whole multiples
+ * of the cell thickness.  Sizes: N.x / 2 and N.y in the 2^a 3^b plans 2..2048 (so a periodic
+ * nx is a multiple of 4), a periodic nz (3D) a power of two 4..128; anything else returns
+ * BM_ERR_SIZE.  All three counts zero is bm_demag_create.  The convolution calls take the
+ * handle unchanged; the integrator and multilayer handles have no periodic variant.
+ */
+int bm_demag_create_pbc(int nx, int ny, int nz, double hx, double hy, double hz, int pbc_x, int pbc_y, int pbc_z,
+	int flags, bm_demag_t* out);
 int bm_demag_destroy(bm_demag_t d);
 /* N.x, N.y, N.z of the zero-padded convolution and the device bytes held. */
 int bm_demag_info(bm_demag_t d, int* N3, long long* device_bytes);

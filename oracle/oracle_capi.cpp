@@ -101,6 +101,21 @@ int or_kernels(int nx, int ny, int nz, int Nx, int Ny, int Nz, double hx, double
 	catch (...) { return 1; }
 }
 
+// Periodic tensors (stored as -N, N = n along periodic axes): 3D Dd, Do each Nx*Ny*Nz*3;
+// 2D (Nz = 1) Dd Nx*Ny*3 and Do Nx*Ny (Dxy only).  h normalised by the caller.
+int or_tensor_pbc(int Nx, int Ny, int Nz, double hx, double hy, double hz, int px, int py, int pz, double* Dd, double* Do)
+{
+	try {
+		std::vector<double> a, b;
+		if (Nz == 1) tensor2d_pbc(Nx, Ny, hx, hy, hz, px, py, pz, a, b);
+		else tensor3d_pbc(Nx, Ny, Nz, hx, hy, hz, px, py, pz, a, b);
+		std::memcpy(Dd, a.data(), a.size() * sizeof(double));
+		std::memcpy(Do, b.data(), b.size() * sizeof(double));
+		return 0;
+	}
+	catch (...) { return 1; }
+}
+
 // Complex FFT of length n (interleaved re,im doubles), sign -1 / +1, unnormalised.
 int or_fft(int n, double* data, int sign)
 {
@@ -119,6 +134,15 @@ void* or_demag_create(int nx, int ny, int nz, double hx, double hy, double hz, i
 	try {
 		if (fp32) return new Demag<float>(nx, ny, nz, hx, hy, hz, Nz_override);
 		return new Demag<double>(nx, ny, nz, hx, hy, hz, Nz_override);
+	}
+	catch (...) { return nullptr; }
+}
+// periodic boundary conditions: pbc[3] images per axis
+void* or_demag_create_pbc(int nx, int ny, int nz, double hx, double hy, double hz, const int* pbc, int fp32)
+{
+	try {
+		if (fp32) return new Demag<float>(nx, ny, nz, hx, hy, hz, 0, pbc);
+		return new Demag<double>(nx, ny, nz, hx, hy, hz, 0, pbc);
 	}
 	catch (...) { return nullptr; }
 }

@@ -195,8 +195,12 @@ inline void kernels_from_tensor_3d(Kernels& K, std::vector<double>& D, bool offd
 	}
 }
 
-inline Kernels make_kernels(int nx, int ny, int nz, int Nx, int Ny, int Nz, double hx, double hy, double hz)
+// pbc[3]: periodic images per axis (null or all zero: open boundaries); along a periodic
+// axis N equals the cell count (Boris/ConvolutionDataCUDA.cpp:56-90).
+inline Kernels make_kernels(int nx, int ny, int nz, int Nx, int Ny, int Nz, double hx, double hy, double hz,
+	const int* pbc = nullptr)
 {
+	const bool periodic = pbc && (pbc[0] || pbc[1] || pbc[2]);
 	Kernels K{ nx, ny, nz, Nx, Ny, Nz, hx, hy, hz, {}, {} };
 	double hm = std::max(hx, std::max(hy, hz));
 	double rx = hx / hm, ry = hy / hm, rz = hz / hm;
@@ -204,7 +208,8 @@ inline Kernels make_kernels(int nx, int ny, int nz, int Nx, int Ny, int Nz, doub
 	typedef std::complex<double> C;
 	if (nz == 1) {
 		std::vector<double> Dd, Dxy;
-		tensor2d(nx, ny, Nx, Ny, rx, ry, rz, Dd, Dxy);
+		if (periodic) tensor2d_pbc(Nx, Ny, rx, ry, rz, pbc[0], pbc[1], pbc[2], Dd, Dxy);
+		else tensor2d(nx, ny, Nx, Ny, rx, ry, rz, Dd, Dxy);
 		Fft<double> fx(Nx), fy(Ny);
 		std::vector<double> t3((size_t)Nc * Ny * 3), t1((size_t)Nc * Ny);
 		std::vector<C> a(std::max(Nx, Ny)), s(std::max(Nx, Ny));
@@ -232,7 +237,8 @@ inline Kernels make_kernels(int nx, int ny, int nz, int Nx, int Ny, int Nz, doub
 	}
 	else {
 		std::vector<double> Dd, Do;
-		tensor3d(nx, ny, nz, Nx, Ny, Nz, rx, ry, rz, Dd, Do);
+		if (periodic) tensor3d_pbc(Nx, Ny, Nz, rx, ry, rz, pbc[0], pbc[1], pbc[2], Dd, Do);
+		else tensor3d(nx, ny, nz, Nx, Ny, Nz, rx, ry, rz, Dd, Do);
 		kernels_from_tensor_3d(K, Dd, false, K.Kdiag);
 		std::vector<double>().swap(Dd);
 		kernels_from_tensor_3d(K, Do, true, K.Kodiag);
@@ -252,13 +258,15 @@ struct Demag {
 	Fft<R> fx, fy, fz;
 	std::vector<C> F[3];           // Nc x Ny x Nz spectra
 
-	Demag(int nx_, int ny_, int nz_, double hx, double hy, double hz, int Nz_override = 0)
+	// pbc: periodic images per axis (null: open boundaries); a periodic axis has N = n
+	Demag(int nx_, int ny_, int nz_, double hx, double hy, double hz, int Nz_override = 0, const int* pbc = nullptr)
 		: nx(nx_), ny(ny_), nz(nz_)
 	{
-		Nx = 2 * nx; Ny = 2 * ny;
-		Nz = (nz == 1) ? 1 : (Nz_override ? Nz_override : 2 * nz);
+		const int p0 = pbc ? pbc[0] : 0, p1 = pbc ? pbc[1] : 0, p2 = pbc ? pbc[2] : 0;
+		Nx = p0 ? nx : 2 * nx; Ny = p1 ? ny : 2 * ny;
+		Nz = (nz == 1) ? 1 : p2 ? nz : (Nz_override ? Nz_override : 2 * nz);
 		Nc = Nx / 2 + 1; Nyh = Ny / 2 + 1; Nzh = Nz / 2 + 1;
-		Kernels K = make_kernels(nx, ny, nz, Nx, Ny, Nz, hx, hy, hz);
+		Kernels K = make_kernels(nx, ny, nz, Nx, Ny, Nz, hx, hy, hz, pbc);
 		Kd.assign(K.Kdiag.begin(), K.Kdiag.end());
 		Ko.assign(K.Kodiag.begin(), K.Kodiag.end());
 		fx = Fft<R>(Nx); fy = Fft<R>(Ny); if (Nz > 1) fz = Fft<R>(Nz);

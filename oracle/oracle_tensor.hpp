@@ -417,4 +417,125 @@ inline void tensor2d(int nx, int ny, int Nx, int Ny, double hx, double hy, doubl
 	}
 }
 
+// ---------------------------------------------------------------------------------------
+// Periodic boundary conditions (Boris/DemagTFunc_PBC.cpp).  Along a periodic axis N equals
+// the cell count and every offset 0..N-1 sums the tensor over the images -p..p of the
+// source (offset + img*N); along an open axis N = 2n, offsets 0..n-1 are computed and their
+// mirrors written.  Off-diagonal terms take sign(offset) per axis with sign(0) = 0 (the
+// reference's get_sign, BorisLib/Funcs_Math.h:103), so a zero offset contributes nothing.
+inline int isign0(int v) { return (v > 0) - (v < 0); }
+
+// 3D: Boris/DemagTFunc_PBC.cpp:186-298 (diagonal) and :300-422 (off-diagonal).
+inline void tensor3d_pbc(int Nx, int Ny, int Nz, double hx, double hy, double hz, int px, int py, int pz,
+	std::vector<double>& Dd, std::vector<double>& Do)
+{
+	const int d = kAsymptoticDistance;
+	px = std::abs(px); py = std::abs(py); pz = std::abs(pz);
+	// f/g grids: asymptotic distance along periodic axes, n (capped at d) along open ones
+	const int dx = std::min(px ? d : Nx / 2, d), dy = std::min(py ? d : Ny / 2, d), dz = std::min(pz ? d : Nz / 2, d);
+	const size_t NN = (size_t)Nx * Ny * Nz;
+	Dd.assign(NN * 3, 0.0); Do.assign(NN * 3, 0.0);
+	Grid3 fxx = fill_grid(nw_f, dx, dy, dz, hx, hy, hz);
+	Grid3 fyy = fill_grid(nw_f, dy, dx, dz, hy, hx, hz);
+	Grid3 fzz = fill_grid(nw_f, dz, dy, dx, hz, hy, hx);
+	Grid3 gxy = fill_grid(nw_g, dx, dy, dz, hx, hy, hz);
+	Grid3 gxz = fill_grid(nw_g, dx, dz, dy, hx, hz, hy);
+	Grid3 gyz = fill_grid(nw_g, dy, dz, dx, hy, hz, hx);
+	AsymptDiag adx(hx, hy, hz), ady(hy, hx, hz), adz(hz, hy, hx);
+	AsymptOffDiag aoxy(hx, hy, hz), aoxz(hx, hz, hy), aoyz(hy, hz, hx);
+	const int ex = px ? Nx : Nx / 2, ey = py ? Ny : Ny / 2, ez = pz ? Nz : Nz / 2;
+
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
+	for (int j0 = 0; j0 < ey; j0++) {
+		for (int k0 = 0; k0 < ez; k0++) {
+			for (int i0 = 0; i0 < ex; i0++) {
+				double vd[3] = { 0, 0, 0 }, vo[3] = { 0, 0, 0 };
+				for (int ii = -px; ii <= px; ii++) for (int jj = -py; jj <= py; jj++) for (int kk = -pz; kk <= pz; kk++) {
+					const int is = ii * Nx + i0, js = jj * Ny + j0, ks = kk * Nz + k0;
+					const int si = isign0(is), sj = isign0(js), sk = isign0(ks);
+					const int i = std::abs(is), j = std::abs(js), k = std::abs(ks);
+					if (use_asympt(i, j, k, d)) {
+						vd[0] += -adx.eval(i * hx, j * hy, k * hz);
+						vd[1] += -ady.eval(j * hy, i * hx, k * hz);
+						vd[2] += -adz.eval(k * hz, j * hy, i * hx);
+						vo[0] += -aoxy.eval(i * hx, j * hy, k * hz) * (double)(si * sj);
+						vo[1] += -aoxz.eval(i * hx, k * hz, j * hy) * (double)(si * sk);
+						vo[2] += -aoyz.eval(j * hy, k * hz, i * hx) * (double)(sj * sk);
+					}
+					else {
+						vd[0] += -L27(fxx, i, j, k, hx, hy, hz);
+						vd[1] += -L27(fyy, j, i, k, hy, hx, hz);
+						vd[2] += -L27(fzz, k, j, i, hz, hy, hx);
+						vo[0] += -L27(gxy, i, j, k, hx, hy, hz) * (double)(si * sj);
+						vo[1] += -L27(gxz, i, k, j, hx, hz, hy) * (double)(si * sk);
+						vo[2] += -L27(gyz, j, k, i, hy, hz, hx) * (double)(sj * sk);
+					}
+				}
+				// mirrors along open axes with a nonzero offset (no two of them coincide)
+				const bool fx = !px && i0, fy = !py && j0, fz = !pz && k0;
+				for (int m = 0; m < 8; m++) {
+					const int mx = m & 1, my = (m >> 1) & 1, mz = (m >> 2) & 1;
+					if ((mx && !fx) || (my && !fy) || (mz && !fz)) continue;
+					const int sx = mx ? -1 : 1, sy = my ? -1 : 1, sz = mz ? -1 : 1;
+					const size_t idx = wrap(sx * i0, Nx) + wrap(sy * j0, Ny) * Nx + wrap(sz * k0, Nz) * (size_t)Nx * Ny;
+					Dd[idx * 3 + 0] = vd[0]; Dd[idx * 3 + 1] = vd[1]; Dd[idx * 3 + 2] = vd[2];
+					Do[idx * 3 + 0] = vo[0] * (sx * sy); Do[idx * 3 + 1] = vo[1] * (sx * sz); Do[idx * 3 + 2] = vo[2] * (sy * sz);
+				}
+			}
+		}
+	}
+}
+
+// 2D (n.z = 1): Boris/DemagTFunc_PBC.cpp:5-94 (diagonal) and :96-184 (Dxy).  z images sit
+// at whole multiples of the cell thickness.
+inline void tensor2d_pbc(int Nx, int Ny, double hx, double hy, double hz, int px, int py, int pz,
+	std::vector<double>& Dd, std::vector<double>& Dxy)
+{
+	const int d = kAsymptoticDistance;
+	px = std::abs(px); py = std::abs(py); pz = std::abs(pz);
+	const int dx = std::min(px ? d : Nx / 2, d), dy = std::min(py ? d : Ny / 2, d), dz = std::min(pz ? d : 1, d);
+	const size_t NN = (size_t)Nx * Ny;
+	Dd.assign(NN * 3, 0.0); Dxy.assign(NN, 0.0);
+	Grid3 fxx = fill_grid(nw_f, dx, dy, dz, hx, hy, hz);
+	Grid3 fyy = fill_grid(nw_f, dy, dx, dz, hy, hx, hz);
+	Grid3 fzz = fill_grid(nw_f, dz, dy, dx, hz, hy, hx);
+	Grid3 gxy = fill_grid(nw_g, dx, dy, dz, hx, hy, hz);
+	AsymptDiag adx(hx, hy, hz), ady(hy, hx, hz), adz(hz, hy, hx);
+	AsymptOffDiag aoxy(hx, hy, hz);
+	const int ex = px ? Nx : Nx / 2, ey = py ? Ny : Ny / 2;
+
+#pragma omp parallel for schedule(dynamic, 1)
+	for (int j0 = 0; j0 < ey; j0++) {
+		for (int i0 = 0; i0 < ex; i0++) {
+			double vd[3] = { 0, 0, 0 }, vo = 0;
+			for (int ii = -px; ii <= px; ii++) for (int jj = -py; jj <= py; jj++) for (int kk = -pz; kk <= pz; kk++) {
+				const int is = ii * Nx + i0, js = jj * Ny + j0;
+				const int si = isign0(is), sj = isign0(js);
+				const int i = std::abs(is), j = std::abs(js), k = std::abs(kk);
+				if (use_asympt(i, j, k, d)) {
+					vd[0] += -adx.eval(i * hx, j * hy, k * hz);
+					vd[1] += -ady.eval(j * hy, i * hx, k * hz);
+					vd[2] += -adz.eval(k * hz, j * hy, i * hx);
+					vo += -aoxy.eval(i * hx, j * hy, k * hz) * (double)(si * sj);
+				}
+				else {
+					vd[0] += -L27(fxx, i, j, k, hx, hy, hz);
+					vd[1] += -L27(fyy, j, i, k, hy, hx, hz);
+					vd[2] += -L27(fzz, k, j, i, hz, hy, hx);
+					vo += -L27(gxy, i, j, k, hx, hy, hz) * (double)(si * sj);
+				}
+			}
+			const bool fx = !px && i0, fy = !py && j0;
+			for (int m = 0; m < 4; m++) {
+				const int mx = m & 1, my = (m >> 1) & 1;
+				if ((mx && !fx) || (my && !fy)) continue;
+				const int sx = mx ? -1 : 1, sy = my ? -1 : 1;
+				const size_t idx = wrap(sx * i0, Nx) + wrap(sy * j0, Ny) * Nx;
+				Dd[idx * 3 + 0] = vd[0]; Dd[idx * 3 + 1] = vd[1]; Dd[idx * 3 + 2] = vd[2];
+				Dxy[idx] = vo * (sx * sy);
+			}
+		}
+	}
+}
+
 } // namespace oracle

@@ -15,13 +15,20 @@
 //           between a source cell s and a destination cell d (both already normalised,
 //           differing at most in z), shift shz (normalised): the irregular entry point,
 //           which falls back to the regular shifted one when s == d
+//   ref_dump pbc3d Nx Ny Nz hx hy hz px py pz out.bin
+//        -> Ddiag then Dodiag (Nx*Ny*Nz DBL3 each) of the periodic tensor with px, py, pz
+//           images (N = n along a periodic axis, 2n otherwise)
+//   ref_dump pbc2d Nx Ny hx hy hz px py pz out.bin
+//        -> Ddiag (Nx*Ny DBL3) then Dxy (Nx*Ny doubles) of the periodic 2D tensor
 //
 // Reference entry points used: DemagTFunc::CalcDiagTens3D/CalcOffDiagTens3D/
 // CalcDiagTens2D/CalcOffDiagTens2D (Boris/DemagTFunc.cpp:21,95,172,237),
 // DemagTFunc::SelfDemag (Boris/DemagTFunc_Self.cpp:46),
 // DemagTFunc::Ldia_single/Lodia_single (Boris/DemagTFunc_Test.cpp),
 // DemagTFunc::CalcDiagTens2D_Shifted_Irregular/CalcOffDiagTens2D_Shifted_Irregular
-// (Boris/DemagTFunc_Irregular.cpp:5,121).
+// (Boris/DemagTFunc_Irregular.cpp:5,121),
+// DemagTFunc::CalcDiagTens3D_PBC/CalcOffDiagTens3D_PBC/CalcDiagTens2D_PBC/
+// CalcOffDiagTens2D_PBC (Boris/DemagTFunc_PBC.cpp:186,301,5,96).
 
 #include "stdafx.h"
 #include "DemagTFunc.h"
@@ -105,6 +112,37 @@ int main(int argc, char** argv)
 		for (size_t i = 0; i < Dd.linear_size(); i++) { out.push_back(Dd[i].x); out.push_back(Dd[i].y); out.push_back(Dd[i].z); }
 		for (size_t i = 0; i < Do.linear_size(); i++) { out.push_back(Do[i].x); out.push_back(Do[i].y); out.push_back(Do[i].z); }
 		write_doubles(argv[13], out);
+		return 0;
+	}
+	if (mode == "pbc3d" && argc == 12) {
+		INT3 N(atoi(argv[2]), atoi(argv[3]), atoi(argv[4]));
+		DBL3 h(atof(argv[5]), atof(argv[6]), atof(argv[7]));
+		INT3 img(atoi(argv[8]), atoi(argv[9]), atoi(argv[10]));
+		DBL3 hr = h / maximum(h.x, h.y, h.z);
+		VEC<DBL3> Dd, Do;
+		if (!Dd.resize(SZ3(N.x, N.y, N.z)) || !Do.resize(SZ3(N.x, N.y, N.z))) return 3;
+		if (!dtf.CalcDiagTens3D_PBC(Dd, N, hr, true, ASYMPTOTIC_DISTANCE, img.x, img.y, img.z)) return 4;
+		if (!dtf.CalcOffDiagTens3D_PBC(Do, N, hr, true, ASYMPTOTIC_DISTANCE, img.x, img.y, img.z)) return 4;
+		std::vector<double> out;
+		for (size_t i = 0; i < Dd.linear_size(); i++) { out.push_back(Dd[i].x); out.push_back(Dd[i].y); out.push_back(Dd[i].z); }
+		for (size_t i = 0; i < Do.linear_size(); i++) { out.push_back(Do[i].x); out.push_back(Do[i].y); out.push_back(Do[i].z); }
+		write_doubles(argv[11], out);
+		return 0;
+	}
+	if (mode == "pbc2d" && argc == 11) {
+		INT3 N(atoi(argv[2]), atoi(argv[3]), 1);
+		DBL3 h(atof(argv[4]), atof(argv[5]), atof(argv[6]));
+		INT3 img(atoi(argv[7]), atoi(argv[8]), atoi(argv[9]));
+		DBL3 hr = h / maximum(h.x, h.y, h.z);
+		VEC<DBL3> Dd;
+		if (!Dd.resize(SZ3(N.x, N.y, 1))) return 3;
+		std::vector<double> Do((size_t)N.x * N.y, 0.0);
+		if (!dtf.CalcDiagTens2D_PBC(Dd, N, hr, true, ASYMPTOTIC_DISTANCE, img.x, img.y, img.z)) return 4;
+		if (!dtf.CalcOffDiagTens2D_PBC(Do, N, hr, true, ASYMPTOTIC_DISTANCE, img.x, img.y, img.z)) return 4;
+		std::vector<double> out;
+		for (size_t i = 0; i < Dd.linear_size(); i++) { out.push_back(Dd[i].x); out.push_back(Dd[i].y); out.push_back(Dd[i].z); }
+		for (double d : Do) out.push_back(d);
+		write_doubles(argv[10], out);
 		return 0;
 	}
 	fprintf(stderr, "bad arguments\n");

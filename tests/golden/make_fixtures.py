@@ -19,6 +19,9 @@ reference checkout and oracle/_ref exist; the GPU test runs only read the result
   DemagTFunc_Irregular.cpp: regular (equal cells) and irregular (thicknesses differ), with
   near-field and far-field (>= 40 cells) entries; all entries for small meshes, a seeded
   sample for large ones.
+* pbc_golden.npz — FP64 periodic demag tensors (N = n along periodic axes, images summed)
+  written by oracle/_ref/ref_dump from the reference's Boris/DemagTFunc_PBC.cpp: 2D and 3D,
+  images along x, y and z, near-field only and with far-field images.
 """
 from __future__ import annotations
 
@@ -107,7 +110,43 @@ def main() -> None:
     out["single_val"] = np.array([dump("single", *d, *hh) for d, hh in singles])
     np.savez_compressed(HERE / "tensor_golden.npz", **out)
     zshift_fixtures()
+    pbc_fixtures()
     print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
+
+
+# (name, N, cell size, images per axis): periodic tensors, N = n along a periodic axis
+PBC_CASES = [
+    ("p2_x", (8, 12, 1), (4e-9, 4e-9, 4e-9), (3, 0, 0)),
+    ("p2_yz", (16, 6, 1), (4e-9, 3e-9, 5e-9), (0, 4, 2)),
+    ("p2_xy_far", (64, 48, 1), (4e-9, 4e-9, 8e-9), (10, 10, 0)),
+    ("p3_x", (8, 10, 4), (4e-9, 4e-9, 4e-9), (2, 0, 0)),
+    ("p3_yz", (12, 6, 8), (3e-9, 4e-9, 2e-9), (0, 3, 4)),
+    ("p3_xyz_far", (16, 16, 16), (4e-9, 4e-9, 4e-9), (4, 4, 4)),
+]
+
+
+def pbc_fixtures() -> None:
+    """pbc_golden.npz: the reference's periodic tensors (Boris/DemagTFunc_PBC.cpp compiled
+    into oracle/_ref/ref_dump), whole for small N and at 1500 seeded points otherwise."""
+    rng = np.random.default_rng(20261204)
+    out = {}
+    for name, NN, hh, img in PBC_CASES:
+        m = NN[0] * NN[1] * NN[2]
+        if NN[2] == 1:
+            v = dump("pbc2d", NN[0], NN[1], *map(float, hh), *img)
+            Dd, Do = v[:3 * m].reshape(1, NN[1], NN[0], 3), v[3 * m:].reshape(1, NN[1], NN[0], 1)
+        else:
+            v = dump("pbc3d", *NN, *map(float, hh), *img)
+            Dd, Do = v[:3 * m].reshape(NN[2], NN[1], NN[0], 3), v[3 * m:].reshape(NN[2], NN[1], NN[0], 3)
+        if m <= 1000:
+            idx = np.array([(k, j, i) for k in range(NN[2]) for j in range(NN[1]) for i in range(NN[0])])
+        else:
+            idx = np.stack([rng.integers(0, NN[2], 1500), rng.integers(0, NN[1], 1500), rng.integers(0, NN[0], 1500)], axis=1)
+        out[name + "_N"], out[name + "_h"], out[name + "_img"] = np.array(NN), np.array(hh), np.array(img)
+        out[name + "_idx"] = idx
+        out[name + "_diag"] = Dd[idx[:, 0], idx[:, 1], idx[:, 2]]
+        out[name + "_off"] = Do[idx[:, 0], idx[:, 1], idx[:, 2]]
+    np.savez_compressed(HERE / "pbc_golden.npz", **out)
 
 
 # (name, n, N, source cell, destination cell, shift), cells and shift normalised

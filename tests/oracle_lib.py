@@ -31,6 +31,7 @@ def lib() -> C.CDLL:
     L = C.CDLL(str(LIB))
     dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
     i, d, vp = C.c_int, C.c_double, C.c_void_p
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
     L.or_tensor3d.argtypes = [i] * 6 + [d] * 3 + [dp, dp]
     L.or_tensor2d.argtypes = [i] * 4 + [d] * 3 + [dp, dp]
     L.or_self_demag.argtypes = [d, d, d, dp]
@@ -41,6 +42,8 @@ def lib() -> C.CDLL:
     L.or_kernels.argtypes = [i] * 6 + [d] * 3 + [dp, dp]
     L.or_fft.argtypes = [i, dp, i]
     L.or_demag_create.argtypes = [i, i, i, d, d, d, i, i]; L.or_demag_create.restype = vp
+    L.or_demag_create_pbc.argtypes = [i, i, i, d, d, d, ip, i]; L.or_demag_create_pbc.restype = vp
+    L.or_tensor_pbc.argtypes = [i, i, i, d, d, d, i, i, i, dp, dp]; L.or_tensor_pbc.restype = i
     L.or_demag_destroy.argtypes = [vp, i]
     L.or_demag_convolute.argtypes = [vp, i, dp, dp, i]
     L.or_sim_create.argtypes = [i, i, i, i, d, d, d, dp, i, i, i]; L.or_sim_create.restype = vp
@@ -81,6 +84,18 @@ def tensor3d(n, N, h):
     m = N[0] * N[1] * N[2] * 3
     a, b = np.zeros(m), np.zeros(m)
     assert L.or_tensor3d(*n, *N, *h, a, b) == 0
+    return a.reshape(N[2], N[1], N[0], 3), b.reshape(N[2], N[1], N[0], 3)
+
+
+def tensor_pbc(N, h, pbc):
+    """Periodic tensor (h normalised): 3D (Dd, Do) each (Nz, Ny, Nx, 3); 2D (Nz = 1) Dd
+    (Ny, Nx, 3) and Dxy (Ny, Nx)."""
+    L = lib()
+    m = N[0] * N[1] * N[2]
+    a, b = np.zeros(3 * m), np.zeros(m if N[2] == 1 else 3 * m)
+    assert L.or_tensor_pbc(*N, *h, *pbc, a, b) == 0
+    if N[2] == 1:
+        return a.reshape(N[1], N[0], 3), b.reshape(N[1], N[0])
     return a.reshape(N[2], N[1], N[0], 3), b.reshape(N[2], N[1], N[0], 3)
 
 
@@ -177,11 +192,14 @@ def fft(x: np.ndarray, sign: int = -1) -> np.ndarray:
 class OracleDemag:
     """The oracle's demag convolution (FP64, or FP32 with fp32=True)."""
 
-    def __init__(self, n, h, fp32=False, Nz_override=0):
+    def __init__(self, n, h, fp32=False, Nz_override=0, pbc=None):
         self.L = lib()
         self.n = tuple(n)
         self.fp32 = int(fp32)
-        self.h = self.L.or_demag_create(*n, *h, Nz_override, self.fp32)
+        if pbc is not None and any(pbc):
+            self.h = self.L.or_demag_create_pbc(*n, *h, np.ascontiguousarray(pbc, dtype=np.int32), self.fp32)
+        else:
+            self.h = self.L.or_demag_create(*n, *h, Nz_override, self.fp32)
         assert self.h
 
     def __call__(self, M: np.ndarray, H: np.ndarray | None = None, clear_out=True) -> np.ndarray:

@@ -52,3 +52,17 @@ def test_unsupported_sizes_are_rejected_before_any_device_work():
         assert L.bm_demag_create(*n, 4e-9, 4e-9, 4e-9, 0, C.byref(h)) == code, n
         assert h.value is None
     assert L.bm_demag_create(8, 8, 1, -4e-9, 4e-9, 4e-9, 0, C.byref(h)) == 1
+
+
+def test_pbc_sizes_are_checked_before_any_device_work():
+    import ctypes as C
+    L = _lib.load()
+    h = C.c_void_p()
+    cases = [((6, 8, 1, 1, 0, 0), 2),       # periodic nx = 6: N.x / 2 = 3 has no plan
+             ((8, 10, 1, 0, 1, 0), 2),      # periodic ny = 10 has no plan
+             ((8, 8, 3, 0, 0, 1), 2),       # periodic nz = 3: N.z = 3 is not a power of two
+             ((8, 8, 2, 0, 0, 1), 2),       # N.z = 2 below the smallest z plan
+             ((8, 8, 1, -1, 0, 0), 1)]      # negative image count
+    for (nx, ny, nz, px, py, pz), code in cases:
+        assert L.bm_demag_create_pbc(nx, ny, nz, 4e-9, 4e-9, 4e-9, px, py, pz, 0, C.byref(h)) == code
+        assert h.value is None
