@@ -8,6 +8,9 @@
 //       pinned bit-for-bit (and to 1e-15 rel) against oracle/_ref (the reference's own
 //       DemagTFunc*.cpp compiled here) through committed golden vectors; also against
 //       closed forms (cube self-demag -1/3, trace -1, far-field dipole).
+//   tensor3d_pbc/tensor2d_pbc (periodic images): pinned bit-for-bit against oracle/_ref
+//       (the reference's DemagTFunc_PBC.cpp) through tests/golden/pbc_golden.npz, and by
+//       the trace identity (tests/test_oracle_pbc.py).
 #pragma once
 
 #include <cmath>
