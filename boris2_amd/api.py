@@ -206,7 +206,8 @@ class FMSolver:
     """One ferromagnetic mesh integrated with the LLG equation."""
 
     def __init__(self, method: int, n: tuple[int, int, int], h: tuple[float, float, float],
-                 params: FMParams, M0: np.ndarray | torch.Tensor | None = None):
+                 params: FMParams, M0: np.ndarray | torch.Tensor | None = None, pbc: tuple[int, int, int] = (0, 0, 0)):
+        """pbc: periodic images per axis (demag and exchange), as the reference's `pbc` command."""
         require_device()
         self.L = _lib.load()
         self.n = tuple(int(v) for v in n)
@@ -218,11 +219,14 @@ class FMSolver:
             self.M[:, 0] = -params.Ms      # a new mesh starts at (-Ms, 0, 0), Boris/Mesh_Ferromagnetic.cpp:194
         else:
             self.M.copy_(torch.as_tensor(np.asarray(M0, dtype=np.float32).reshape(ncell, 3)))
-        self.demag = Demag(self.n, self.h) if params.modules & BM_MOD_DEMAG else None
+        self.pbc = tuple(int(v) for v in pbc)
+        self.demag = Demag(self.n, self.h, self.pbc) if params.modules & BM_MOD_DEMAG else None
         self._pc = params.to_c()
         self.handle = C.c_void_p()
         dh = self.demag.handle if self.demag else None
         check(self.L.bm_ode_create(method, *self.n, *self.h, C.byref(self._pc), dh, _ptr(self.M), C.byref(self.handle)))
+        if any(self.pbc):
+            check(self.L.bm_ode_set_pbc(self.handle, *self.pbc))
         self.Ms_map = None
 
     def set_ms_map(self, ms: np.ndarray | torch.Tensor | None) -> None:

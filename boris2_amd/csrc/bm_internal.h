@@ -23,6 +23,7 @@ void prof_mark_begin(int kind, hipStream_t st);
 void prof_mark_end(int kind, hipStream_t st);
 // self-demag coefficient (-N diagonal) of one cell of size hx x hy x hz (bm_tensor.cpp)
 void self_demag_coeff(double hx, double hy, double hz, double out[3]);
+void self_demag_coeff_pbc(double hx, double hy, double hz, int nx, int ny, int nz, int px, int py, int pz, double out[3]);
 
 } // namespace bm
 

@@ -91,6 +91,7 @@ void asym_diag_init(AsymDiag& A, double hx, double hy, double hz);
 void asym_off_init(AsymOff& A, double hx, double hy, double hz);
 // self-demag coefficient (-N diagonal) of one cell of size hx x hy x hz (bm_tensor.cpp)
 void self_demag_coeff(double hx, double hy, double hz, double out[3]);
+void self_demag_coeff_pbc(double hx, double hy, double hz, int nx, int ny, int nz, int px, int py, int pz, double out[3]);
 
 BM_HD double asym_diag(const AsymDiag& A, double x, double y, double z)
 {

@@ -161,6 +161,9 @@ struct bm_ode_s {
 	bool link_dTspeedup = true;
 	double dTspeedup = 0, time_speedup = 0;
 	long long convolutions = 0;
+	// periodic boundary conditions (bm_ode_set_pbc): images per axis; exchange wraps around
+	// along every axis with images, and the demag handle must carry the same images
+	int pbc[3] = { 0, 0, 0 };
 	// per-launch speedup use, read by make_args: extrapolate (sp_n saved fields) or save
 	int sp_n = 0;
 	float sp_ca[6] = {};
@@ -206,6 +209,7 @@ StageArgs make_args(bm_ode_s* o, float dT)
 	for (int q = 0; q < 6; q++) a.E[q] = o->E[q];
 	a.Ms_map = o->Ms_map; a.Heff_out = o->keep_heff ? o->Heff_keep : nullptr; a.lte = o->d_lte;
 	a.nx = o->nx; a.ny = o->ny; a.nz = o->nz;
+	a.per = (o->pbc[0] ? 1 : 0) | (o->pbc[1] ? 2 : 0) | (o->pbc[2] ? 4 : 0);
 	a.ihx2 = (float)(1.0 / (o->h[0] * o->h[0])); a.ihy2 = (float)(1.0 / (o->h[1] * o->h[1])); a.ihz2 = (float)(1.0 / (o->h[2] * o->h[2]));
 	a.Ms = (float)o->p.Ms; a.A = (float)o->p.A; a.alpha = (float)o->p.alpha; a.K1 = (float)o->p.K1; a.K2 = (float)o->p.K2;
 	a.grel = (float)o->p.grel;
@@ -395,12 +399,14 @@ int eval_speedup(bm_ode_s* o, int step, float dT, hipStream_t st)
 }
 
 // Self-demag coefficients of the evaluation speedup: the mesh's (DemagTFunc::SelfDemag_PBC
-// without images, cell size in metres, Boris/DemagCUDA.cpp:37) and, for a layer stack, each
-// layer mesh's own (SDemagCUDA_Demag::selfDemagCoeff).
+// with the mesh's periodic images, cell size in metres, Boris/DemagCUDA.cpp:37-39) and, for a
+// layer stack, each layer mesh's own (SDemagCUDA_Demag::selfDemagCoeff).
 void set_self_coeffs(bm_ode_s* o)
 {
 	double sd[3];
-	bm::self_demag_coeff(o->h[0], o->h[1], o->h[2], sd);
+	if (o->pbc[0] || o->pbc[1] || o->pbc[2])
+		bm::self_demag_coeff_pbc(o->h[0], o->h[1], o->h[2], o->nx, o->ny, o->nz, o->pbc[0], o->pbc[1], o->pbc[2], sd);
+	else bm::self_demag_coeff(o->h[0], o->h[1], o->h[2], sd);
 	for (int c = 0; c < 3; c++) o->selfc[c] = (float)sd[c];
 	if (o->ml)
 		for (int l = 0; l < o->nz && l < kMaxStackLayers; l++) {
@@ -411,10 +417,10 @@ void set_self_coeffs(bm_ode_s* o)
 		}
 }
 
-// Demag evaluations go through the fused row kernel (single mesh, no speedup).
+// Demag evaluations go through the fused row kernel (single mesh open along x, no speedup).
 bool row_path(const bm_ode_s* o)
 {
-	return (o->p.modules & BM_MOD_DEMAG) && o->use_row && !o->ml && !o->speedup;
+	return (o->p.modules & BM_MOD_DEMAG) && o->use_row && !o->ml && !o->speedup && !o->pbc[0] && o->demag && !o->demag->full[0];
 }
 
 // One field evaluation + stage `step` (the effective field is that of Mcur at stagetime).
@@ -587,6 +593,8 @@ extern "C" int bm_ode_create(int method, int nx, int ny, int nz, double hx, doub
 	// BM_ROW_KERNEL=0 selects the unfused evaluation (planar H_demag + stage kernel), kept
 	// as a cross-check of the fused row kernel
 	if (const char* e = getenv("BM_ROW_KERNEL")) o->use_row = atoi(e) != 0;
+	// a demag handle periodic along x has no zero-padded half for the row kernel's x passes
+	if (demag && demag->full[0]) o->use_row = false;
 	int nE = 0;
 	// method defaults: Boris/DiffEq_CommonBase_Control.cpp:10-140 with Boris/DiffEq_Defs.h
 	switch (method) {
@@ -686,6 +694,30 @@ extern "C" int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p)
 	o->p = *p;
 	o->S_valid = false;
 	o->num_saved = 0;      // DemagCUDA::UpdateConfiguration drops the saved fields
+	return BM_OK;
+}
+
+// Periodic boundary conditions (the `pbc` command: Demag::Set_PBC and Mesh::Set_Magnetic_PBC,
+// Boris/Demag.cpp:137-148, Boris/MeshControl.cpp:49-65, Boris/SuperMeshMeshes_Settings.cpp:
+// 659-700): exchange takes the cell on the other side as the neighbour at the mesh ends of
+// every axis with images, and with demag on the demag handle must have been created with the
+// same images.  Saved speedup fields are dropped and the speedup's self coefficient gains the
+// images' far-field terms.
+extern "C" int bm_ode_set_pbc(bm_ode_t o, int pbc_x, int pbc_y, int pbc_z)
+{
+	if (!o || pbc_x < 0 || pbc_y < 0 || pbc_z < 0) return fail(BM_ERR_ARG, "bm_ode_set_pbc: bad arguments");
+	if (o->ml) return fail(BM_ERR_STATE, "bm_ode_set_pbc: layer stacks have no periodic variant");
+	if ((o->p.modules & BM_MOD_DEMAG) && o->demag
+		&& (o->demag->pbc[0] != pbc_x || o->demag->pbc[1] != pbc_y || o->demag->pbc[2] != pbc_z))
+		return fail(BM_ERR_STATE, "bm_ode_set_pbc: the demag handle must have the same periodic images (bm_demag_create_pbc)");
+	o->pbc[0] = pbc_x; o->pbc[1] = pbc_y; o->pbc[2] = pbc_z;
+	if (pbc_x) o->use_row = false;
+	if ((o->p.modules & BM_MOD_DEMAG) && !row_path(o) && !o->Hd) {
+		if (hipMalloc(&o->Hd, o->n * 3 * sizeof(float)) != hipSuccess) { o->Hd = nullptr; return fail(BM_ERR_NOMEM, "bm_ode_set_pbc: demag field buffer"); }
+	}
+	o->S_valid = false;
+	o->num_saved = 0;
+	set_self_coeffs(o);
 	return BM_OK;
 }
 

@@ -88,6 +88,7 @@ struct StageArgs {
 	float* Heff_out;        // optional AoS copy of the effective field
 	unsigned* lte;          // max lte as float bits
 	int nx, ny, nz;
+	int per;                // periodic exchange along x, y, z: bits 0, 1, 2
 	float ihx2, ihy2, ihz2; // 1 / h^2 per axis
 	float Ms, A, alpha, K1, K2, grel;
 	float ea[3], Ha[3];
@@ -267,18 +268,26 @@ __device__ __forceinline__ void stage_cell(const StageArgs& a, F3 Ha, size_t idx
 	if (a.modules & BM_MOD_EXCHANGE) {
 		F3 lx{ 0, 0, 0 }, ly{ 0, 0, 0 }, lz{ 0, 0, 0 };
 		const size_t sy = a.nx, sz = (size_t)a.nx * a.ny;
+		// periodic axes at the mesh ends: (M[+] + M[-]) - 2M with the wrapped neighbour
+		// (BorisLib/VEC_VC_Del.h:24-45)
 		if (a.nx > 1) {
 			if (i > 0 && i < a.nx - 1) lx = ld3(a.Min, idx + 1) + ld3(a.Min, idx - 1) - 2.f * m;
+			else if (a.per & 1) lx = (i == 0) ? ld3(a.Min, idx + 1) + ld3(a.Min, idx + (a.nx - 1)) - 2.f * m
+			                                  : ld3(a.Min, idx - (a.nx - 1)) + ld3(a.Min, idx - 1) - 2.f * m;
 			else if (i == 0) lx = ld3(a.Min, idx + 1) - m;
 			else lx = ld3(a.Min, idx - 1) - m;
 		}
 		if (a.ny > 1) {
 			if (j > 0 && j < a.ny - 1) ly = ld3(a.Min, idx + sy) + ld3(a.Min, idx - sy) - 2.f * m;
+			else if (a.per & 2) ly = (j == 0) ? ld3(a.Min, idx + sy) + ld3(a.Min, idx + (a.ny - 1) * sy) - 2.f * m
+			                                  : ld3(a.Min, idx - (a.ny - 1) * sy) + ld3(a.Min, idx - sy) - 2.f * m;
 			else if (j == 0) ly = ld3(a.Min, idx + sy) - m;
 			else ly = ld3(a.Min, idx - sy) - m;
 		}
 		if (a.nz > 1 && !a.layered) {
 			if (k > 0 && k < a.nz - 1) lz = ld3(a.Min, idx + sz) + ld3(a.Min, idx - sz) - 2.f * m;
+			else if (a.per & 4) lz = (k == 0) ? ld3(a.Min, idx + sz) + ld3(a.Min, idx + (a.nz - 1) * sz) - 2.f * m
+			                                  : ld3(a.Min, idx - (a.nz - 1) * sz) + ld3(a.Min, idx - sz) - 2.f * m;
 			else if (k == 0) lz = ld3(a.Min, idx + sz) - m;
 			else lz = ld3(a.Min, idx - sz) - m;
 		}
@@ -300,6 +309,7 @@ __device__ __forceinline__ P6 stage_pair(const StageArgs& a, size_t idx, int i0,
 	F3 lap0{ 0.f, 0.f, 0.f }, lap1{ 0.f, 0.f, 0.f };
 	if (a.modules & BM_MOD_EXCHANGE) {
 		// x: the two cells are each other's neighbours; clamped loads keep the ends branch-free
+		// (the row kernel serves meshes open along x only)
 		const bool hasL = i0 > 0, hasR = i0 + 2 < a.nx;
 		const F3 L = ld3(a.Min, hasL ? idx - 1 : idx), R = ld3(a.Min, hasR ? idx + 2 : idx + 1);
 		F3 lx0 = hasL ? (m.b + L) - 2.f * m.a : m.b - m.a;
@@ -308,11 +318,15 @@ __device__ __forceinline__ P6 stage_pair(const StageArgs& a, size_t idx, int i0,
 		const size_t sy = a.nx, sz = (size_t)a.nx * a.ny;
 		if (a.ny > 1) {
 			if (j > 0 && j < a.ny - 1) ly = ld6(a.Min, idx + sy) + ld6(a.Min, idx - sy) - 2.f * m;
+			else if (a.per & 2) ly = (j == 0) ? ld6(a.Min, idx + sy) + ld6(a.Min, idx + (a.ny - 1) * sy) - 2.f * m
+			                                  : ld6(a.Min, idx - (a.ny - 1) * sy) + ld6(a.Min, idx - sy) - 2.f * m;
 			else if (j == 0) ly = ld6(a.Min, idx + sy) - m;
 			else ly = ld6(a.Min, idx - sy) - m;
 		}
 		if (a.nz > 1) {
 			if (k > 0 && k < a.nz - 1) lz = ld6(a.Min, idx + sz) + ld6(a.Min, idx - sz) - 2.f * m;
+			else if (a.per & 4) lz = (k == 0) ? ld6(a.Min, idx + sz) + ld6(a.Min, idx + (a.nz - 1) * sz) - 2.f * m
+			                                  : ld6(a.Min, idx - (a.nz - 1) * sz) + ld6(a.Min, idx - sz) - 2.f * m;
 			else if (k == 0) lz = ld6(a.Min, idx + sz) - m;
 			else lz = ld6(a.Min, idx - sz) - m;
 		}

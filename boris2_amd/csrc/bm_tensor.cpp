@@ -125,6 +125,34 @@ void self_demag_coeff(double hx, double hy, double hz, double out[3])
 	out[2] = -self_demag_1(hz, hy, hx);
 }
 
+// Self coefficient with periodic images (DemagTFunc::SelfDemag_PBC,
+// Boris/DemagTFunc_Self.cpp:58-99), stored as -N: the zero image gives the cell's own term,
+// every other image (offset |img| n cells) the far-field series, summed with x images
+// outermost.  The reference sets up its three series objects all with (hx, hy, hz); the one
+// object here reproduces that.
+void self_demag_coeff_pbc(double hx, double hy, double hz, int nx, int ny, int nz, int px, int py, int pz, double out[3])
+{
+	AsymDiag A;
+	asym_diag_init(A, hx, hy, hz);
+	double v[3] = { 0.0, 0.0, 0.0 };
+	for (int ii = -px; ii <= px; ii++)
+		for (int jj = -py; jj <= py; jj++)
+			for (int kk = -pz; kk <= pz; kk++) {
+				const int i = (int)std::fabs(ii * (double)nx), j = (int)std::fabs(jj * (double)ny), k = (int)std::fabs(kk * (double)nz);
+				if (i == 0 && j == 0 && k == 0) {
+					double s[3];
+					self_demag_coeff(hx, hy, hz, s);
+					for (int c = 0; c < 3; c++) v[c] += s[c];
+				}
+				else {
+					v[0] += -asym_diag(A, i * hx, j * hy, k * hz);
+					v[1] += -asym_diag(A, j * hy, i * hx, k * hz);
+					v[2] += -asym_diag(A, k * hz, j * hy, i * hx);
+				}
+			}
+	for (int c = 0; c < 3; c++) out[c] = v[c];
+}
+
 void asym_off_init(AsymOff& A, double hx, double hy, double hz)
 {
 	Pw p(hx, hy, hz);

@@ -173,6 +173,18 @@ int bm_ode_set_layers(bm_ode_t o, bm_mlayer_t ml, const double* Ms, const double
  * the current ones); the defaults are the reference's per-method defaults. */
 int bm_ode_set_dt(bm_ode_t o, double dT);
 int bm_ode_set_astep(bm_ode_t o, double err_fail, double dT_increase, double dT_min, double dT_max);
+/*
+ * Periodic boundary conditions of the mesh (images per axis, 0 = open).
+ * Replaces: the `pbc` command's mesh settings, Demag::Set_PBC + Mesh::Set_Magnetic_PBC
+ * (Boris/Demag.cpp:137-148, Boris/MeshControl.cpp:49-65,
+ * Boris/SuperMeshMeshes_Settings.cpp:659-700): exchange uses the cell on the other side as the
+ * neighbour at the mesh ends of every axis with images (BorisLib/VEC_VC_Del.h:10-130); with
+ * demag on, the handle given at creation must have been made by bm_demag_create_pbc with the
+ * same images (BM_ERR_STATE otherwise), as the reference sets both together.  The evaluation
+ * speedup's self coefficient becomes DemagTFunc::SelfDemag_PBC with these images
+ * (Boris/DemagTFunc_Self.cpp:58-99).  Not for layer stacks.
+ */
+int bm_ode_set_pbc(bm_ode_t o, int pbc_x, int pbc_y, int pbc_z);
 /* Demag evaluation speedup: mode is the reference's EVALSPEEDUP_ value (0 none, 1 step,
  * 2 linear, 3 quadratic, 4 cubic, 5 quartic, 6 quintic; Boris/DiffEq_Defs.h:121-125) =
  * the number of saved demag fields.  The demag field is convolved only when the ODE

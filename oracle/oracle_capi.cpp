@@ -84,6 +84,10 @@ void or_mlayer_convolute(void* h, const double* M, double* H) { ((MLayer<double>
 void or_mlayer_convolute_transposed(void* h, const double* M, double* H) { ((MLayer<double>*)h)->convolute(M, H, true); }
 
 void or_self_demag(double hx, double hy, double hz, double* out3) { self_demag(hx, hy, hz, out3); }
+void or_self_demag_pbc(double hx, double hy, double hz, int nx, int ny, int nz, int px, int py, int pz, double* out3)
+{
+	self_demag_pbc(hx, hy, hz, nx, ny, nz, px, py, pz, out3);
+}
 double or_newell_f(double x, double y, double z) { return nw_f(x, y, z); }
 double or_newell_g(double x, double y, double z) { return nw_g(x, y, z); }
 double or_asympt_diag(double hx, double hy, double hz, double x, double y, double z) { return AsymptDiag(hx, hy, hz).eval(x, y, z); }
@@ -336,6 +340,13 @@ void or_sim_state(void* h, double* st)
 }
 
 // Demag evaluation speedup: mode 0..6 (EVALSPEEDUP_), dTs > 0 unlinks the speedup step.
+int or_sim_set_pbc(void* h, int px, int py, int pz)
+{
+	OrSim* o = (OrSim*)h;
+	try { SIM_DISPATCH(o, { s->set_pbc(px, py, pz); }); return 0; }
+	catch (...) { return 1; }
+}
+
 int or_sim_set_speedup(void* h, int mode, double dTs)
 {
 	OrSim* o = (OrSim*)h;

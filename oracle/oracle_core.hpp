@@ -482,6 +482,11 @@ struct Sim {
 	std::vector<double> layHz;         // layer stacks: each layer's cell thickness
 	bool link_dTspeedup = true;
 	double dTspeedup = 0, time_speedup = 0;
+	// periodic boundary conditions (Demag::Set_PBC + Mesh::Set_Magnetic_PBC,
+	// Boris/Demag.cpp:137-148, Boris/MeshControl.cpp:49): images per axis for the demag
+	// tensor, and wrap-around neighbours for exchange along every axis with images
+	int pbc[3] = { 0, 0, 0 };
+	int Nz_open = 0;
 
 	Sim(int method_, int nx_, int ny_, int nz_, double hx, double hy, double hz, const Params& p_, int Nz_override = 0)
 		: nx(nx_), ny(ny_), nz(nz_), p(p_), method(method_)
@@ -491,6 +496,7 @@ struct Sim {
 		for (int s = 0; s < 6; s++) E[s].assign(n * 3, (R)0);
 		for (size_t i = 0; i < n; i++) M[i * 3] = (R)-p.Ms;   // default M = (-Ms, 0, 0)
 		if (p.demag) demag = new Demag<R>(nx, ny, nz, hx, hy, hz, Nz_override);
+		Nz_open = Nz_override;
 		// method defaults, Boris/DiffEq_CommonBase_Control.cpp:10-140, Boris/DiffEq_Defs.h
 		switch (method) {
 		case EV_TEULER: dT = 0.5e-13; order = 2; break;
@@ -530,7 +536,8 @@ struct Sim {
 		if (mode < 0 || mode > 6) throw std::runtime_error("bad evaluation speedup mode");
 		speedup = mode; num_saved = 0;
 		for (int s = 0; s < 6; s++) { Hs[s].clear(); tdm[s] = 0; }
-		double sd[3]; self_demag(h[0], h[1], h[2], sd);   // SelfDemag_PBC without images
+		double sd[3];
+		self_demag_pbc(h[0], h[1], h[2], nx, ny, nz, pbc[0], pbc[1], pbc[2], sd);   // SelfDemag_PBC
 		for (int c = 0; c < 3; c++) selfc[c] = (R)sd[c];
 		// a layer stack: each layer mesh's own coefficient (Boris/SDemagCUDA.cpp:583-1000)
 		layselfc.clear();
@@ -540,6 +547,20 @@ struct Sim {
 		}
 		if (dTs > 0) { link_dTspeedup = false; dTspeedup = dTs; time_speedup = time; }
 		else { link_dTspeedup = true; dTspeedup = dT; }
+	}
+
+	// Periodic images per axis (the `pbc` command): the demag tensor is rebuilt with them and
+	// exchange wraps around; saved speedup fields are dropped (UpdateConfiguration).
+	void set_pbc(int px, int py, int pz)
+	{
+		if (layered) throw std::runtime_error("periodic boundaries on a layer stack");
+		pbc[0] = px; pbc[1] = py; pbc[2] = pz;
+		if (p.demag) {
+			delete demag;
+			demag = (px || py || pz) ? new Demag<R>(nx, ny, nz, h[0], h[1], h[2], 0, pbc)
+			                         : new Demag<R>(nx, ny, nz, h[0], h[1], h[2], Nz_open);
+		}
+		if (speedup) set_speedup(speedup, link_dTspeedup ? 0.0 : dTspeedup);
 	}
 
 	// ODECommon_Base::Get_EvalStep_Time (Boris/DiffEq_CommonBase_Iterate.cpp:1536-1635):
@@ -617,7 +638,8 @@ struct Sim {
 	}
 
 	// Heff += 2A/(mu0 Ms^2) * delsq_neu(M)  (Boris/Exchange.cpp:108-156,
-	// BorisLib/VEC_VC_Del.h:10-130; free boundaries use the one-sided difference).
+	// BorisLib/VEC_VC_Del.h:10-130; free boundaries use the one-sided difference, periodic
+	// ones the cell on the other side: (M[+] + M[-]) - 2M with the wrapped neighbour).
 	void add_exchange()
 	{
 		int d[3] = { nx, ny, nz };
@@ -633,6 +655,11 @@ struct Sim {
 				for (int c = 0; c < 3; c++) {
 					R m0 = M[idx * 3 + c];
 					if (lo && hi) diff[c] = M[(idx + st[a]) * 3 + c] + M[(idx - st[a]) * 3 + c] - 2 * m0;
+					else if (pbc[a] && d[a] > 1) {
+						const size_t up = hi ? idx + st[a] : idx - (size_t)(d[a] - 1) * st[a];
+						const size_t dn = lo ? idx - st[a] : idx + (size_t)(d[a] - 1) * st[a];
+						diff[c] = M[up * 3 + c] + M[dn * 3 + c] - 2 * m0;
+					}
 					else if (hi) diff[c] = M[(idx + st[a]) * 3 + c] - m0;
 					else if (lo) diff[c] = M[(idx - st[a]) * 3 + c] - m0;
 					diff[c] /= ((R)h[a] * (R)h[a]);

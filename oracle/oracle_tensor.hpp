@@ -421,6 +421,31 @@ inline void tensor2d(int nx, int ny, int Nx, int Ny, double hx, double hy, doubl
 }
 
 // ---------------------------------------------------------------------------------------
+// Self coefficient with periodic images (DemagTFunc::SelfDemag_PBC,
+// Boris/DemagTFunc_Self.cpp:58-99), stored as -N: the zero image contributes the cell's own
+// self term, every other image (offset |img| n cells) the far-field series, summed in the
+// reference's loop order (x images outermost).  The reference sets up all three series
+// objects with (hx, hy, hz) (its yy and zz objects are not permuted); kept.
+inline void self_demag_pbc(double hx, double hy, double hz, int nx, int ny, int nz, int px, int py, int pz, double out[3])
+{
+	AsymptDiag a(hx, hy, hz);
+	double v[3] = { 0, 0, 0 };
+	for (int ii = -px; ii <= px; ii++) for (int jj = -py; jj <= py; jj++) for (int kk = -pz; kk <= pz; kk++) {
+		const int i = (int)std::fabs(ii * (double)nx), j = (int)std::fabs(jj * (double)ny), k = (int)std::fabs(kk * (double)nz);
+		if (i == 0 && j == 0 && k == 0) {
+			double s[3];
+			self_demag(hx, hy, hz, s);
+			for (int c = 0; c < 3; c++) v[c] += s[c];
+		}
+		else {
+			v[0] += -a.eval(i * hx, j * hy, k * hz);
+			v[1] += -a.eval(j * hy, i * hx, k * hz);
+			v[2] += -a.eval(k * hz, j * hy, i * hx);
+		}
+	}
+	for (int c = 0; c < 3; c++) out[c] = v[c];
+}
+
 // Periodic boundary conditions (Boris/DemagTFunc_PBC.cpp).  Along a periodic axis N equals
 // the cell count and every offset 0..N-1 sums the tensor over the images -p..p of the
 // source (offset + img*N); along an open axis N = 2n, offsets 0..n-1 are computed and their

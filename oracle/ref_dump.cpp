@@ -18,6 +18,7 @@
 //   ref_dump pbc3d Nx Ny Nz hx hy hz px py pz out.bin
 //        -> Ddiag then Dodiag (Nx*Ny*Nz DBL3 each) of the periodic tensor with px, py, pz
 //           images (N = n along a periodic axis, 2n otherwise)
+//   ref_dump selfpbc hx hy hz nx ny nz px py pz out.bin -> SelfDemag_PBC (3 doubles)
 //   ref_dump pbc2d Nx Ny hx hy hz px py pz out.bin
 //        -> Ddiag (Nx*Ny DBL3) then Dxy (Nx*Ny doubles) of the periodic 2D tensor
 //
@@ -28,7 +29,8 @@
 // DemagTFunc::CalcDiagTens2D_Shifted_Irregular/CalcOffDiagTens2D_Shifted_Irregular
 // (Boris/DemagTFunc_Irregular.cpp:5,121),
 // DemagTFunc::CalcDiagTens3D_PBC/CalcOffDiagTens3D_PBC/CalcDiagTens2D_PBC/
-// CalcOffDiagTens2D_PBC (Boris/DemagTFunc_PBC.cpp:186,301,5,96).
+// CalcOffDiagTens2D_PBC (Boris/DemagTFunc_PBC.cpp:186,301,5,96),
+// DemagTFunc::SelfDemag_PBC (Boris/DemagTFunc_Self.cpp:58).
 
 #include "stdafx.h"
 #include "DemagTFunc.h"
@@ -112,6 +114,14 @@ int main(int argc, char** argv)
 		for (size_t i = 0; i < Dd.linear_size(); i++) { out.push_back(Dd[i].x); out.push_back(Dd[i].y); out.push_back(Dd[i].z); }
 		for (size_t i = 0; i < Do.linear_size(); i++) { out.push_back(Do[i].x); out.push_back(Do[i].y); out.push_back(Do[i].z); }
 		write_doubles(argv[13], out);
+		return 0;
+	}
+	if (mode == "selfpbc" && argc == 12) {
+		DBL3 h(atof(argv[2]), atof(argv[3]), atof(argv[4]));
+		DBL3 n(atof(argv[5]), atof(argv[6]), atof(argv[7]));
+		INT3 img(atoi(argv[8]), atoi(argv[9]), atoi(argv[10]));
+		DBL3 s = dtf.SelfDemag_PBC(h, n, img);
+		write_doubles(argv[11], { s.x, s.y, s.z });
 		return 0;
 	}
 	if (mode == "pbc3d" && argc == 12) {

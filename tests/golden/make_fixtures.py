@@ -146,6 +146,11 @@ def pbc_fixtures() -> None:
         out[name + "_idx"] = idx
         out[name + "_diag"] = Dd[idx[:, 0], idx[:, 1], idx[:, 2]]
         out[name + "_off"] = Do[idx[:, 0], idx[:, 1], idx[:, 2]]
+    # SelfDemag_PBC: cell size in metres (as the evaluation speedup calls it), mesh cells, images
+    selfs = [((4e-9, 4e-9, 4e-9), (32, 24, 1), (3, 0, 0)), ((5e-9, 4e-9, 3e-9), (16, 16, 8), (2, 2, 1)),
+             ((4e-9, 4e-9, 8e-9), (64, 48, 1), (10, 10, 0)), ((3e-9, 3e-9, 3e-9), (8, 8, 4), (0, 0, 5))]
+    out["selfpbc_in"] = np.array([list(hh) + list(nn) + list(img) for hh, nn, img in selfs], dtype=np.float64)
+    out["selfpbc_val"] = np.array([dump("selfpbc", *map(float, hh), *nn, *img) for hh, nn, img in selfs])
     np.savez_compressed(HERE / "pbc_golden.npz", **out)
 
 

@@ -44,6 +44,8 @@ def lib() -> C.CDLL:
     L.or_demag_create.argtypes = [i, i, i, d, d, d, i, i]; L.or_demag_create.restype = vp
     L.or_demag_create_pbc.argtypes = [i, i, i, d, d, d, ip, i]; L.or_demag_create_pbc.restype = vp
     L.or_tensor_pbc.argtypes = [i, i, i, d, d, d, i, i, i, dp, dp]; L.or_tensor_pbc.restype = i
+    L.or_self_demag_pbc.argtypes = [d, d, d, i, i, i, i, i, i, dp]
+    L.or_sim_set_pbc.argtypes = [vp, i, i, i]; L.or_sim_set_pbc.restype = i
     L.or_demag_destroy.argtypes = [vp, i]
     L.or_demag_convolute.argtypes = [vp, i, dp, dp, i]
     L.or_sim_create.argtypes = [i, i, i, i, d, d, d, dp, i, i, i]; L.or_sim_create.restype = vp
@@ -97,6 +99,12 @@ def tensor_pbc(N, h, pbc):
     if N[2] == 1:
         return a.reshape(N[1], N[0], 3), b.reshape(N[1], N[0])
     return a.reshape(N[2], N[1], N[0], 3), b.reshape(N[2], N[1], N[0], 3)
+
+
+def self_demag_pbc(h, n, pbc):
+    out = np.zeros(3)
+    lib().or_self_demag_pbc(*h, *n, *pbc, out)
+    return out
 
 
 def tensor2d(n, N, h):
@@ -287,6 +295,10 @@ class OracleSim:
         out = np.zeros(3)
         self.L.or_sim_average_m(self.h, out)
         return out
+
+    def set_pbc(self, pbc):
+        """Periodic images per axis: demag tensor and wrap-around exchange."""
+        assert self.L.or_sim_set_pbc(self.h, *[int(v) for v in pbc]) == 0
 
     def set_speedup(self, mode, dTspeedup=0.0):
         """Demag evaluation speedup: mode 0..6 (none, step, linear ... quintic); dTspeedup > 0

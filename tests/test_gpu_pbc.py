@@ -80,3 +80,75 @@ def test_pbc_thin_film_limit(gpu):
     Hx = d.convolute(Mx)
     assert float((Hx[:, 0] - Hx[0, 0]).abs().max()) <= 1e-5
     assert -0.01 < float(Hx[0, 0]) < 0.0
+
+
+# ---------------------------------------------------------------------------------------
+# The integrator with periodic boundaries (bm_ode_set_pbc): demag with images and wrap-around
+# exchange, against the oracle's Sim with the same images.
+
+from test_gpu_ode import bench_like_params, sp4_params, tilted_x    # noqa: E402
+
+
+def _ode_pair(method, n, h, p, M0, pbc, speedup=0):
+    s = api.FMSolver(method, n, h, p, M0, pbc=pbc)
+    o = O.OracleSim(method, n, h, p, Nz_override=nz_open(n[2]) if n[2] > 1 else 0)
+    o.set_pbc(pbc)
+    o.set_M(M0.astype(np.float64))
+    if speedup:
+        s.set_eval_speedup(speedup)
+        o.set_speedup(speedup)
+    return s, o
+
+
+@pytest.mark.parametrize("path", ["row", "unfused"])
+@pytest.mark.parametrize("n,pbc", [((64, 32, 1), (0, 4, 0)), ((32, 16, 4), (0, 2, 3)), ((32, 16, 1), (3, 3, 0)),
+                                   ((32, 24, 4), (2, 0, 0))])
+def test_pbc_rkdp54_matches_oracle(gpu, monkeypatch, path, n, pbc):
+    """y/z-periodic meshes run the fused row kernel (its y and z neighbours wrap); x-periodic
+    ones always take the unfused path."""
+    monkeypatch.setenv("BM_ROW_KERNEL", "1" if path == "row" else "0")
+    h, p = (4e-9, 4e-9, 4e-9), bench_like_params()
+    M0 = tilted_x(n[0] * n[1] * n[2], p.Ms, 31, tilt=0.3)
+    s, o = _ode_pair(api.BM_EVAL_RKDP54, n, h, p, M0, pbc)
+    s.run_to_iteration(20)
+    o.run_to_iteration(20)
+    st, so = s.state(), o.state()
+    assert st["rejections"] == so["rejections"]
+    assert abs(st["time"] - so["time"]) <= 1e-3 * so["time"]
+    assert rel_l2(s.M.cpu().numpy(), o.get_M()) <= 1e-4
+
+
+def test_pbc_rk4_film_and_exchange_only(gpu):
+    n, h = (64, 32, 1), (3.90625e-9, 3.90625e-9, 3e-9)
+    M0 = tilted_x(n[0] * n[1], 8e5, 5, tilt=0.5)
+    for p in (sp4_params(), api.FMParams(Ms=8e5, A=1.3e-11, alpha=0.02, Ha=(0.0, 2e4, 0.0),
+                                         modules=api.BM_MOD_EXCHANGE | api.BM_MOD_ZEEMAN)):
+        s, o = _ode_pair(api.BM_EVAL_RK4, n, h, p, M0, (5, 5, 0))
+        s.set_dt(1e-13); o.set_dt(1e-13)
+        s.run_to_iteration(50)
+        o.run_to_iteration(50)
+        assert rel_l2(s.M.cpu().numpy(), o.get_M()) <= 1e-5
+
+
+@pytest.mark.parametrize("pbc", [(4, 4, 0), (2, 0, 0)])
+def test_pbc_speedup_matches_oracle(gpu, pbc):
+    """Evaluation speedup with periodic images: the saved fields exclude SelfDemag_PBC's self
+    term (the images' far-field terms included)."""
+    n, h, p = (64, 32, 1), (3.90625e-9, 3.90625e-9, 3e-9), sp4_params()
+    M0 = tilted_x(n[0] * n[1], p.Ms, 12)
+    s, o = _ode_pair(api.BM_EVAL_RKDP54, n, h, p, M0, pbc, speedup=3)
+    s.run_to_iteration(40)
+    o.run_to_iteration(40)
+    st, so = s.state(), o.state()
+    assert st["rejections"] == so["rejections"]
+    assert abs(st["time"] - so["time"]) <= 1e-3 * so["time"]
+    assert s.speedup_state()["convolutions"] == o.speedup_info()["convolutions"]
+    assert rel_l2(s.M.cpu().numpy(), o.get_M()) <= 1e-4
+
+
+def test_pbc_images_must_match_the_demag_handle(gpu):
+    import ctypes as C
+    n, h, p = (32, 16, 1), (4e-9, 4e-9, 4e-9), sp4_params()
+    s = api.FMSolver(api.BM_EVAL_RK4, n, h, p, tilted_x(512, p.Ms, 1), pbc=(2, 0, 0))
+    assert s.L.bm_ode_set_pbc(s.handle, 3, 0, 0) == 5          # BM_ERR_STATE
+    assert s.L.bm_ode_set_pbc(s.handle, 2, 0, 0) == 0

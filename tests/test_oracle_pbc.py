@@ -12,7 +12,13 @@ Boris/ConvolutionDataCUDA.cpp:56-90):
   that axis and -1 for an odd one.  This pins the wrap-around packing, the kernel extraction
   and the scaling;
 * trace identity: the images only add mutual terms, whose trace is zero, so the trace of the
-  periodic tensor is -1 at offset 0 and 0 elsewhere.
+  periodic tensor is -1 at offset 0 and 0 elsewhere;
+* SelfDemag_PBC (the evaluation speedup's self term with images) equals the reference's bit
+  for bit, and reduces to the plain self term without images;
+* the integrator with periodic boundaries: exchange of a uniform state is zero and a
+  periodic cosine is an eigenvector with eigenvalue (2 cos(2 pi / n) - 2) / h^2 at every
+  cell (boundary cells included); a periodic mesh advanced from a state shifted by whole
+  cells equals the shifted result (translation invariance, demag and exchange on).
 """
 from pathlib import Path
 
@@ -22,7 +28,7 @@ import pytest
 import oracle_lib as O
 
 GOLDEN = np.load(Path(__file__).parent / "golden" / "pbc_golden.npz")
-CASES = sorted({k.rsplit("_", 1)[0] for k in GOLDEN.files})
+CASES = sorted(k[:-2] for k in GOLDEN.files if k.endswith("_N"))
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -123,3 +129,57 @@ def test_pbc_tensor_trace(N, h, pbc):
     tr[0, 0, 0] = 0.0
     # offsets that no image reaches (the zero-padded half of an open axis) are zero anyway
     assert np.abs(tr).max() <= 1e-10
+
+
+def test_self_demag_pbc_matches_reference():
+    for row, val in zip(GOLDEN["selfpbc_in"], GOLDEN["selfpbc_val"]):
+        h, n, img = tuple(row[:3]), tuple(int(v) for v in row[3:6]), tuple(int(v) for v in row[6:])
+        assert np.array_equal(O.self_demag_pbc(h, n, img), val), (h, n, img)
+    h = (4e-9, 3e-9, 2e-9)
+    plain = np.zeros(3)
+    O.lib().or_self_demag(*h, plain)
+    assert np.array_equal(O.self_demag_pbc(h, (8, 8, 1), (0, 0, 0)), plain)
+
+
+def sim_params(**kw):
+    from boris2_amd import api
+    base = dict(Ms=8e5, A=1.3e-11, alpha=0.02, modules=api.BM_MOD_EXCHANGE)
+    base.update(kw)
+    return api.FMParams(**base)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_periodic_exchange_eigenvector(axis):
+    from boris2_amd import api
+    n, h = (8, 6, 4), (4e-9, 3e-9, 5e-9)
+    p = sim_params(Ha=(0.0, 0.0, 0.0))
+    o = O.OracleSim(api.BM_EVAL_RK4, n, h, p)
+    pbc = [0, 0, 0]; pbc[axis] = 1
+    o.set_pbc(pbc)
+    idx = np.indices((n[2], n[1], n[0]))[::-1].reshape(3, -1)     # (i, j, k) per cell
+    ph = 2 * np.pi * idx[axis] / n[axis]
+    M = np.zeros((idx.shape[1], 3)); M[:, 0] = p.Ms * np.cos(ph); M[:, 1] = p.Ms * np.sin(ph)
+    o.set_M(M)
+    o.update_fields()
+    lam = (2 * np.cos(2 * np.pi / n[axis]) - 2) / h[axis] ** 2
+    expect = 2 * p.A / (api.MU0 * p.Ms ** 2) * lam * M
+    assert np.abs(o.heff() - expect).max() <= 1e-9 * np.abs(expect).max()
+    M[:] = (p.Ms, 0.0, 0.0)
+    o.set_M(M); o.update_fields()
+    assert np.abs(o.heff()).max() == 0.0
+
+
+def test_periodic_translation_invariance():
+    from boris2_amd import api
+    n, h, pbc = (8, 8, 1), (4e-9, 4e-9, 4e-9), (2, 3, 0)
+    p = sim_params(Ha=(2e4, 0.0, 1e4), modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ZEEMAN)
+    M0 = api.random_magnetisation(64, p.Ms, 9).reshape(n[1], n[0], 3)
+    out = []
+    for sx, sy in ((0, 0), (3, 5)):
+        o = O.OracleSim(api.BM_EVAL_RK4, n, h, p)
+        o.set_pbc(pbc)
+        o.set_M(np.roll(M0, (sy, sx), axis=(0, 1)).reshape(-1, 3))
+        o.set_dt(1e-13)
+        o.run_to_iteration(20)
+        out.append(o.get_M().reshape(n[1], n[0], 3))
+    assert np.abs(np.roll(out[0], (5, 3), axis=(0, 1)) - out[1]).max() <= 1e-9 * p.Ms
