@@ -19,7 +19,7 @@ EXPORTS = [
     "bm_demag_create", "bm_demag_create_pbc", "bm_demag_destroy", "bm_demag_info", "bm_demag_convolute",
     "bm_demag_convolute_planar", "bm_demag_copy_kernels", "bm_demag_convolute_averaged",
     "bm_ode_create", "bm_ode_destroy", "bm_ode_set_ms_map", "bm_ode_set_params", "bm_ode_set_layers", "bm_ode_set_dt",
-    "bm_ode_set_astep", "bm_ode_set_pbc", "bm_ode_set_eval_speedup", "bm_ode_get_speedup_state", "bm_ode_advance_time", "bm_ode_run_to_iteration", "bm_ode_iterate_stage",
+    "bm_ode_set_astep", "bm_ode_set_pbc", "bm_ode_set_layer_meshes", "bm_ode_set_eval_speedup", "bm_ode_get_speedup_state", "bm_ode_advance_time", "bm_ode_run_to_iteration", "bm_ode_iterate_stage",
     "bm_ode_restore", "bm_ode_get_state", "bm_ode_keep_heff", "bm_ode_copy_heff", "bm_ode_average_m",
     "bm_prof_begin", "bm_prof_end", "bm_prof_query",
     "bm_mlayer_create", "bm_mlayer_destroy", "bm_mlayer_convolute", "bm_mlayer_copy_kernels",
@@ -109,6 +109,7 @@ def load() -> C.CDLL:
         "bm_ode_set_dt": (i, [vp, d]),
         "bm_ode_set_astep": (i, [vp, d, d, d, d]),
         "bm_ode_set_pbc": (i, [vp, i, i, i]),
+        "bm_ode_set_layer_meshes": (i, [vp, vp, vp]),
         "bm_ode_set_eval_speedup": (i, [vp, i, d]),
         "bm_ode_get_speedup_state": (i, [vp, C.POINTER(i), C.POINTER(d), C.POINTER(ll)]),
         "bm_ode_advance_time": (i, [vp, vp]),

@@ -335,10 +335,30 @@ class LayerStackSolver(FMSolver):
             check(self.L.bm_ode_set_layers(self.handle, self.mlayer.handle, *ptrs))
         self.set_params(params)
 
+    def set_layer_meshes(self, mesh_of_layer=None, Ha=None) -> None:
+        """Stacks of 3D meshes: consecutive layers with equal mesh_of_layer[l] are the z planes of
+        one mesh (exchange couples them; they must touch and have one thickness); Ha (L, 3) A/m
+        gives every layer its own constant applied field."""
+        g = None if mesh_of_layer is None else np.ascontiguousarray(mesh_of_layer, dtype=np.int32)
+        f = None if Ha is None else np.ascontiguousarray(Ha, dtype=np.float64).reshape(-1)
+        self._groups = (g, f)
+        check(self.L.bm_ode_set_layer_meshes(self.handle, None if g is None else g.ctypes.data_as(C.c_void_p),
+                                             None if f is None else f.ctypes.data_as(C.c_void_p)))
+
     def close(self) -> None:
         super().close()
         if getattr(self, "mlayer", None) is not None:
             self.mlayer.close()
+
+
+def stack_of_meshes(meshes):
+    """Planes of a multilayered set-up of 3D meshes, as LayerStackSolver arguments: meshes is a
+    list of (nz, hz, z_bottom); returns (hz per plane, z0 per plane, mesh index per plane)."""
+    hz, z0, grp = [], [], []
+    for m, (nz, h, zb) in enumerate(meshes):
+        for k in range(int(nz)):
+            hz.append(float(h)); z0.append(float(zb) + k * float(h)); grp.append(m)
+    return np.array(hz), np.array(z0), np.array(grp, dtype=np.int32)
 
 
 class BatchSolver:

@@ -650,7 +650,7 @@ extern "C" int bm_adj_set_layers(bm_adj_t o, bm_mlayer_t ml, const double* A, co
 	int L, nx, ny;
 	mlayer_dims(ml, &L, &nx, &ny, nullptr);
 	if (L != o->nz || nx != o->nx || ny != o->ny) return fail(BM_ERR_ARG, "bm_adj_set_layers: the stack's layers must be the mesh's nz planes (same nx, ny)");
-	if (L > kMaxStackLayers) return fail(BM_ERR_SIZE, "bm_adj_set_layers: at most 8 layers");
+	if (L > kMaxStackLayers) return fail(BM_ERR_SIZE, "bm_adj_set_layers: at most 16 layers");
 	const double* src[3] = { A, alpha, K1 };
 	for (int q = 0; q < 3; q++) {
 		o->lay_set[q] = src[q] != nullptr;

@@ -96,6 +96,7 @@ struct MlView {
 MlView mlayer_view(bm_mlayer_s* m);
 // cell size (m) of layer l of a multilayer handle
 void mlayer_cell(const bm_mlayer_s* m, int l, double h[3]);
+double mlayer_z0(const bm_mlayer_s* m, int l);
 // Per-stack arguments of the batched multilayer kernel product.
 struct MlBatch { float2* S; const float* K; int NC, Ny, Sp; };
 // Launch shape of a batched pass (0 = the pass's default): f the x-pass row-tile factor

@@ -28,7 +28,7 @@ namespace {
 
 using namespace bm;
 
-constexpr int kMaxLayers = 8;
+constexpr int kMaxLayers = 16;
 
 // Position of grid index i (-1, 0, +1, ...) along one axis: i*h + o, or a table for -1, 0, +1.
 struct Axis { double h, o, p[3]; int table; };
@@ -231,10 +231,10 @@ int launch_ml_mult(int L, float2* S, const float* K, int NC, int Ny, int Sp, int
 	dim3 g((unsigned)std::max(1, std::min((npts + 255) / 256, ncu * 8))), b(256);
 	switch (L) {
 #define C_(V_) case V_: hipLaunchKernelGGL((k_ml_mult<V_>), g, b, 0, st, S, K, NC, Ny, Sp, tr, (const MlBatch*)nullptr); return BM_OK;
-	C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8)
+	C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8) C_(9) C_(10) C_(11) C_(12) C_(13) C_(14) C_(15) C_(16)
 #undef C_
 	}
-	return fail(BM_ERR_SIZE, "multilayer demag: at most 8 layers");
+	return fail(BM_ERR_SIZE, "multilayer demag: at most 16 layers");
 }
 
 } // namespace
@@ -369,7 +369,7 @@ extern "C" int bm_mlayer_create(int nlayers, int nx, int ny, double hx, double h
 	(void)flags;
 	if (!out || !hz || !z0 || nlayers < 1 || nx < 2 || ny < 1 || !(hx > 0) || !(hy > 0)) return fail(BM_ERR_ARG, "bm_mlayer_create: bad arguments");
 	*out = nullptr;
-	if (nlayers > kMaxLayers) return fail(BM_ERR_SIZE, "bm_mlayer_create: at most 8 layers");
+	if (nlayers > kMaxLayers) return fail(BM_ERR_SIZE, "bm_mlayer_create: at most 16 layers");
 	for (int l = 0; l < nlayers; l++) if (!(hz[l] > 0)) return fail(BM_ERR_ARG, "bm_mlayer_create: layer thickness must be positive");
 	if (nx % 2 || !bm::plan_2d_ok(nx, 2 * ny)) return fail(BM_ERR_SIZE, "bm_mlayer_create: no FFT plan for this layer size");
 	bm_mlayer_s* m = new bm_mlayer_s();
@@ -428,6 +428,8 @@ void bm::mlayer_cell(const bm_mlayer_s* m, int l, double h[3])
 	h[0] = m->hx; h[1] = m->hy; h[2] = m->hz[(size_t)l];
 }
 
+double bm::mlayer_z0(const bm_mlayer_s* m, int l) { return m->z0[(size_t)l]; }
+
 bm::MlView bm::mlayer_view(bm_mlayer_s* m)
 {
 	return MlView{ m->Sx, m->S, m->K, m->L, m->nx, m->ny, m->Nx, m->Ny, m->NC, m->NYH, m->Sp, m->ntl };
@@ -440,10 +442,10 @@ int bm::batch_ml_mult(int L, LaunchShape sh, const MlBatch* h_tab, const MlBatch
 	dim3 g((unsigned)std::max(1, std::min((mx + 255) / 256, ncu * (sh.per_cu ? sh.per_cu : 8))), nb), bl(256);
 	switch (L) {
 #define C_(V_) case V_: hipLaunchKernelGGL((k_ml_mult<V_>), g, bl, 0, st, nullptr, nullptr, 0, 0, 0, 0, d_tab); return BM_OK;
-	C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8)
+	C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8) C_(9) C_(10) C_(11) C_(12) C_(13) C_(14) C_(15) C_(16)
 #undef C_
 	}
-	return fail(BM_ERR_SIZE, "multilayer demag: at most 8 layers");
+	return fail(BM_ERR_SIZE, "multilayer demag: at most 16 layers");
 }
 
 void bm::mlayer_dims(const bm_mlayer_s* m, int* L, int* nx, int* ny, double* hz)

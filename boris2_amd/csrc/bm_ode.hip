@@ -128,6 +128,12 @@ struct bm_ode_s {
 	bm_mlayer_t ml = nullptr;
 	bool lay_set[3] = { false, false, false };   // A, alpha, K1 given per layer
 	float lay[3][bm::kMaxStackLayers] = {};
+	// stacks of 3D meshes (bm_ode_set_layer_meshes): mesh index of each layer, exchange
+	// spacing across planes of a mesh, per-layer constant applied fields
+	bool lay_grouped = false, lay_field = false;
+	int lay_grp[bm::kMaxStackLayers] = {};
+	float lay_ihz2[bm::kMaxStackLayers] = {};
+	float lay_Ha[bm::kMaxStackLayers][3] = {};
 	float* Ms_layers = nullptr;    // per-cell Ms built from per-layer values (owned)
 	float* M = nullptr;            // the mesh's magnetisation (caller-owned); holds M between steps
 	float* Mb = nullptr;           // second magnetisation buffer for intermediate stages
@@ -229,6 +235,13 @@ StageArgs make_args(bm_ode_s* o, float dT)
 	if (o->ml) {
 		a.layered = 1;
 		a.ihz2 = 0.f;
+		a.lay_grouped = o->lay_grouped ? 1 : 0;
+		a.lay_field = o->lay_field ? 1 : 0;
+		for (int l = 0; l < kMaxStackLayers; l++) {
+			a.lay_grp[l] = o->lay_grp[l];
+			a.lay_ihz2[l] = o->lay_ihz2[l];
+			for (int c = 0; c < 3; c++) a.lay_Ha[l][c] = o->lay_Ha[l][c];
+		}
 		for (int l = 0; l < kMaxStackLayers; l++) {
 			a.lay_A[l] = o->lay_set[0] ? o->lay[0][l] : a.A;
 			a.lay_alpha[l] = o->lay_set[1] ? o->lay[1][l] : a.alpha;
@@ -657,7 +670,7 @@ extern "C" int bm_ode_set_layers(bm_ode_t o, bm_mlayer_t ml, const double* Ms, c
 	int L, nx, ny;
 	bm::mlayer_dims(ml, &L, &nx, &ny, nullptr);
 	if (L != o->nz || nx != o->nx || ny != o->ny) return fail(BM_ERR_ARG, "bm_ode_set_layers: the stack's layers must be the mesh's nz planes (same nx, ny)");
-	if (L > bm::kMaxStackLayers) return fail(BM_ERR_SIZE, "bm_ode_set_layers: at most 8 layers");
+	if (L > bm::kMaxStackLayers) return fail(BM_ERR_SIZE, "bm_ode_set_layers: at most 16 layers");
 	const size_t b3 = o->n * 3 * sizeof(float);
 	if (!o->Hd && hipMalloc(&o->Hd, b3) != hipSuccess) return fail(BM_ERR_NOMEM, "bm_ode_set_layers: demag field buffer");
 	const double* src[3] = { A, alpha, K1 };
@@ -678,7 +691,45 @@ extern "C" int bm_ode_set_layers(bm_ode_t o, bm_mlayer_t ml, const double* Ms, c
 	o->use_row = false;
 	o->S_valid = false;
 	o->num_saved = 0;              // saved demag fields belong to the previous configuration
+	o->lay_grouped = o->lay_field = false;
 	set_self_coeffs(o);
+	return BM_OK;
+}
+
+// Stacks of 3D meshes: consecutive layers of the stack with the same mesh index are the z
+// planes of one ferromagnetic mesh of Boris's multilayered set-up (each mesh's own exchange
+// couples its planes, Boris/Exchange.cpp:108 with BorisLib/VEC_VC_Del.h:10-130: free ends at
+// the mesh's top and bottom, spacing = the planes' thickness; no exchange between meshes).
+// Ha (3 doubles per layer, or null) gives each layer a constant applied field, as each mesh's
+// own Zeeman module does (Boris/Zeeman.cpp:164).  The planes of a mesh must be touching and
+// of one thickness.
+extern "C" int bm_ode_set_layer_meshes(bm_ode_t o, const int* mesh_of_layer, const double* Ha)
+{
+	if (!o) return fail(BM_ERR_ARG, "null handle");
+	if (!o->ml) return fail(BM_ERR_STATE, "bm_ode_set_layer_meshes: set a layer stack first (bm_ode_set_layers)");
+	const int L = o->nz;
+	if (Ha && o->p.wave != 0) return fail(BM_ERR_STATE, "bm_ode_set_layer_meshes: per-layer fields are constant (no applied-field waveform)");
+	if (mesh_of_layer) {
+		for (int l = 1; l < L; l++) {
+			if (mesh_of_layer[l] < mesh_of_layer[l - 1]) return fail(BM_ERR_ARG, "bm_ode_set_layer_meshes: the planes of a mesh must be consecutive layers");
+			if (mesh_of_layer[l] != mesh_of_layer[l - 1]) continue;
+			double h0[3], h1[3];
+			bm::mlayer_cell(o->ml, l - 1, h0); bm::mlayer_cell(o->ml, l, h1);
+			const double gap = bm::mlayer_z0(o->ml, l) - bm::mlayer_z0(o->ml, l - 1);
+			if (std::fabs(h0[2] - h1[2]) > 1e-6 * h0[2] || std::fabs(gap - h0[2]) > 1e-6 * h0[2])
+				return fail(BM_ERR_ARG, "bm_ode_set_layer_meshes: planes of one mesh must be touching and of equal thickness");
+		}
+	}
+	for (int l = 0; l < L; l++) {
+		double hl[3];
+		bm::mlayer_cell(o->ml, l, hl);
+		o->lay_grp[l] = mesh_of_layer ? mesh_of_layer[l] : l;
+		o->lay_ihz2[l] = (float)(1.0 / (hl[2] * hl[2]));
+		for (int c = 0; c < 3; c++) o->lay_Ha[l][c] = Ha ? (float)Ha[3 * l + c] : 0.f;
+	}
+	o->lay_grouped = mesh_of_layer != nullptr;
+	o->lay_field = Ha != nullptr;
+	o->num_saved = 0;
 	return BM_OK;
 }
 

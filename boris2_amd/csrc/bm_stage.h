@@ -63,7 +63,7 @@ __device__ __forceinline__ P6 renorm(P6 v, float2 Ms) { return P6{ renorm(v.a, M
 __device__ __forceinline__ float rel_err(F3 m, F3 p) { return norm(m - p) / norm(m); }
 __device__ __forceinline__ float rel_err(P6 m, P6 p) { return fmaxf(rel_err(m.a, p.a), rel_err(m.b, p.b)); }
 
-constexpr int kMaxStackLayers = 8;
+constexpr int kMaxStackLayers = 16;
 
 // Per-mesh adaptive step control of a batched RKDP54 integration, kept on the device so a
 // captured step graph runs it (ODECommon_Base's time, dT, lte, iteration, primed state and
@@ -99,6 +99,13 @@ struct StageArgs {
 	// exchange between them) with their own A, alpha and K1 (when layered != 0)
 	int layered;
 	float lay_A[kMaxStackLayers], lay_alpha[kMaxStackLayers], lay_K1[kMaxStackLayers];
+	// stacks of 3D meshes: layers with equal lay_grp are consecutive z planes of one mesh,
+	// coupled by exchange across planes with 1 / hz^2 = lay_ihz2 (when lay_grouped != 0);
+	// with lay_field != 0 each layer has its own constant applied field lay_Ha
+	int lay_grouped, lay_field;
+	int lay_grp[kMaxStackLayers];
+	float lay_ihz2[kMaxStackLayers];
+	float lay_Ha[kMaxStackLayers][3];
 	// time-dependent applied field read on the device (batched graphs): with tnow set, the
 	// stage kernel evaluates Ha0 + amp dir sinc(w (t - t0)) (wave 1) or Ha0 + slope t (wave 2)
 	// at t = *tnow in FP64 and rounds it to FP32, as the host does for a single mesh
@@ -146,6 +153,17 @@ __device__ __forceinline__ CellMat cell_mat(const StageArgs& a, int k)
 	for (int l = 1; l < kMaxStackLayers; l++)
 		if (k == l) c = CellMat{ a.lay_A[l], a.lay_alpha[l], a.lay_K1[l] };
 	return c;
+}
+
+// Value of layer k of a per-layer table, picked with compile-time indices.
+template <typename T>
+__device__ __forceinline__ T lay_pick(const T* v, int k)
+{
+	T r = v[0];
+#pragma unroll
+	for (int l = 1; l < kMaxStackLayers; l++)
+		if (k == l) r = v[l];
+	return r;
 }
 
 enum { M_RKDP = 0, M_RK4 = 1, M_TEULER = 2, M_AHEUN = 3 };
@@ -284,6 +302,7 @@ __device__ __forceinline__ void stage_cell(const StageArgs& a, F3 Ha, size_t idx
 			else if (j == 0) ly = ld3(a.Min, idx + sy) - m;
 			else ly = ld3(a.Min, idx - sy) - m;
 		}
+		float ihz2 = a.ihz2;
 		if (a.nz > 1 && !a.layered) {
 			if (k > 0 && k < a.nz - 1) lz = ld3(a.Min, idx + sz) + ld3(a.Min, idx - sz) - 2.f * m;
 			else if (a.per & 4) lz = (k == 0) ? ld3(a.Min, idx + sz) + ld3(a.Min, idx + (a.nz - 1) * sz) - 2.f * m
@@ -291,9 +310,25 @@ __device__ __forceinline__ void stage_cell(const StageArgs& a, F3 Ha, size_t idx
 			else if (k == 0) lz = ld3(a.Min, idx + sz) - m;
 			else lz = ld3(a.Min, idx - sz) - m;
 		}
-		lap = a.ihx2 * lx + a.ihy2 * ly + a.ihz2 * lz;
+		else if (a.lay_grouped && a.nz > 1) {
+			// planes of one 3D mesh in a stack: neighbours only within the mesh (free ends)
+			const int g = lay_pick(a.lay_grp, k);
+			const bool lo = k > 0 && lay_pick(a.lay_grp, k - 1) == g, hi = k < a.nz - 1 && lay_pick(a.lay_grp, k + 1) == g;
+			if (lo && hi) lz = ld3(a.Min, idx + sz) + ld3(a.Min, idx - sz) - 2.f * m;
+			else if (hi) lz = ld3(a.Min, idx + sz) - m;
+			else if (lo) lz = ld3(a.Min, idx - sz) - m;
+			ihz2 = lay_pick(a.lay_ihz2, k);
+		}
+		lap = a.ihx2 * lx + a.ihy2 * ly + ihz2 * lz;
 	}
 	const CellMat cm = cell_mat(a, k);
+	if (a.lay_field) {
+		F3 hl{ a.lay_Ha[0][0], a.lay_Ha[0][1], a.lay_Ha[0][2] };
+#pragma unroll
+		for (int l = 1; l < kMaxStackLayers; l++)
+			if (k == l) hl = F3{ a.lay_Ha[l][0], a.lay_Ha[l][1], a.lay_Ha[l][2] };
+		Ha = hl;
+	}
 	const F3 H = eff_field(a, cm, Ha, m, Hd, lap, Ms);
 	if (a.Heff_out) st3(a.Heff_out, idx, H);
 	(void)rk_update<METHOD, STEP>(a, idx, m, llg_rhs(a, cm.alpha, m, H, Ms), Ms, lte_loc);

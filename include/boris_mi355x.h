@@ -174,6 +174,18 @@ int bm_ode_set_layers(bm_ode_t o, bm_mlayer_t ml, const double* Ms, const double
 int bm_ode_set_dt(bm_ode_t o, double dT);
 int bm_ode_set_astep(bm_ode_t o, double err_fail, double dT_increase, double dT_min, double dT_max);
 /*
+ * Stacks of 3D meshes (after bm_ode_set_layers): mesh_of_layer (nz ints, non-decreasing, or
+ * NULL) marks consecutive layers that are the z planes of one ferromagnetic mesh, so that
+ * mesh's exchange couples them (free ends at its top and bottom, spacing = the planes'
+ * thickness; the planes must touch and be of equal thickness); Ha (3 doubles per layer, A/m,
+ * or NULL) gives every layer its own constant applied field.
+ * Replaces: the per-mesh modules of Boris's multilayered set-up of 3D meshes coupled by
+ * SDemag (Boris/SDemag.cpp:173-300; each mesh's Exchange, Boris/Exchange.cpp:108, and
+ * Zeeman, Boris/Zeeman.cpp:164).  The multilayered convolution of the planes, all pairs
+ * z-shifted, gives the same interactions as the reference's 3D multilayered kernels.
+ */
+int bm_ode_set_layer_meshes(bm_ode_t o, const int* mesh_of_layer, const double* Ha);
+/*
  * Periodic boundary conditions of the mesh (images per axis, 0 = open).
  * Replaces: the `pbc` command's mesh settings, Demag::Set_PBC + Mesh::Set_Magnetic_PBC
  * (Boris/Demag.cpp:137-148, Boris/MeshControl.cpp:49-65,

@@ -340,6 +340,17 @@ void or_sim_state(void* h, double* st)
 }
 
 // Demag evaluation speedup: mode 0..6 (EVALSPEEDUP_), dTs > 0 unlinks the speedup step.
+// stacks of 3D meshes: grp (L ints) mesh index per layer or null; Ha (3L) or null
+void or_sim_set_layer_meshes(void* h, int L, const int* grp, const double* Ha)
+{
+	OrSim* o = (OrSim*)h;
+	SIM_DISPATCH(o, {
+		s->layGrp.clear(); s->layHa.clear();
+		if (grp) s->layGrp.assign(grp, grp + L);
+		if (Ha) s->layHa.assign(Ha, Ha + 3 * L);
+	});
+}
+
 int or_sim_set_pbc(void* h, int px, int py, int pz)
 {
 	OrSim* o = (OrSim*)h;

@@ -461,6 +461,11 @@ struct Sim {
 	std::function<void(const R*, R*)> ext_demag;
 	bool layered = false;
 	std::vector<double> layA, layAlpha, layK1;
+	// stacks of 3D meshes: mesh index of each layer (empty: every layer its own mesh);
+	// exchange couples consecutive layers of one mesh across z (spacing = layHz), and layHa
+	// (3 per layer, empty = p's field) gives each layer a constant applied field
+	std::vector<int> layGrp;
+	std::vector<double> layHa;
 
 	int method;
 	double time = 0, stagetime = 0, dT, dT_last, dT_min = 1e-15, dT_max = 3e-12, dT_increase = 2, err_high_fail = 1e-4, lte = 0;
@@ -520,7 +525,8 @@ struct Sim {
 	{
 		double Hst[3]; applied_field(p, stagetime, Hst);
 		R Ha[3] = { (R)Hst[0], (R)Hst[1], (R)Hst[2] };
-		for (size_t i = 0; i < n; i++) for (int c = 0; c < 3; c++) Heff[i * 3 + c] = p.zeeman ? Ha[c] : (R)0;
+		for (size_t i = 0; i < n; i++) for (int c = 0; c < 3; c++)
+			Heff[i * 3 + c] = !p.zeeman ? (R)0 : layHa.empty() ? Ha[c] : (R)layHa[layer_of(i) * 3 + c];
 		if (ext_demag || demag) demag_update();
 		if (p.exchange) add_exchange();
 		if (p.anisotropy) add_anisotropy();
@@ -649,9 +655,14 @@ struct Sim {
 			size_t idx = (size_t)i + (size_t)j * nx + (size_t)k * nx * ny;
 			int ijk[3] = { i, j, k };
 			R lap[3] = { 0, 0, 0 };
-			for (int a = 0; a < (layered ? 2 : 3); a++) {
+			const bool grouped = layered && !layGrp.empty();
+			for (int a = 0; a < (layered && !grouped ? 2 : 3); a++) {
 				R diff[3] = { 0, 0, 0 };
 				bool lo = ijk[a] > 0, hi = ijk[a] < d[a] - 1;
+				if (grouped && a == 2) {
+					lo = lo && layGrp[k - 1] == layGrp[k];
+					hi = hi && layGrp[k + 1] == layGrp[k];
+				}
 				for (int c = 0; c < 3; c++) {
 					R m0 = M[idx * 3 + c];
 					if (lo && hi) diff[c] = M[(idx + st[a]) * 3 + c] + M[(idx - st[a]) * 3 + c] - 2 * m0;
@@ -662,7 +673,8 @@ struct Sim {
 					}
 					else if (hi) diff[c] = M[(idx + st[a]) * 3 + c] - m0;
 					else if (lo) diff[c] = M[(idx - st[a]) * 3 + c] - m0;
-					diff[c] /= ((R)h[a] * (R)h[a]);
+					const R ha = (grouped && a == 2) ? (R)layHz[k] : (R)h[a];
+					diff[c] /= (ha * ha);
 				}
 				for (int c = 0; c < 3; c++) lap[c] += diff[c];
 			}

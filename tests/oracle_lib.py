@@ -46,6 +46,7 @@ def lib() -> C.CDLL:
     L.or_tensor_pbc.argtypes = [i, i, i, d, d, d, i, i, i, dp, dp]; L.or_tensor_pbc.restype = i
     L.or_self_demag_pbc.argtypes = [d, d, d, i, i, i, i, i, i, dp]
     L.or_sim_set_pbc.argtypes = [vp, i, i, i]; L.or_sim_set_pbc.restype = i
+    L.or_sim_set_layer_meshes.argtypes = [vp, i, vp, vp]
     L.or_demag_destroy.argtypes = [vp, i]
     L.or_demag_convolute.argtypes = [vp, i, dp, dp, i]
     L.or_sim_create.argtypes = [i, i, i, i, d, d, d, dp, i, i, i]; L.or_sim_create.restype = vp
@@ -295,6 +296,15 @@ class OracleSim:
         out = np.zeros(3)
         self.L.or_sim_average_m(self.h, out)
         return out
+
+    def set_layer_meshes(self, groups=None, Ha=None):
+        """Stacks of 3D meshes: mesh index per layer (exchange across planes of a mesh) and a
+        constant applied field (L, 3) per layer."""
+        self._grp = None if groups is None else np.ascontiguousarray(groups, dtype=np.int32)
+        self._lha = None if Ha is None else np.ascontiguousarray(Ha, dtype=np.float64).reshape(-1)
+        L = self.n[2]
+        self.L.or_sim_set_layer_meshes(self.h, L, None if self._grp is None else self._grp.ctypes.data_as(C.c_void_p),
+                                       None if self._lha is None else self._lha.ctypes.data_as(C.c_void_p))
 
     def set_pbc(self, pbc):
         """Periodic images per axis: demag tensor and wrap-around exchange."""

@@ -84,3 +84,77 @@ def test_one_layer_stack_equals_plain_mesh(gpu):
     torch.cuda.synchronize()
     np.testing.assert_allclose(st.average_m(), plain.average_m(), rtol=0, atol=1e-5)
     assert np.abs(st.M.cpu().numpy() - plain.M.cpu().numpy()).max() <= 1e-3 * p.Ms
+
+
+# ---------------------------------------------------------------------------------------
+# Stacks of 3D meshes (bm_ode_set_layer_meshes): the planes of each mesh are layers of the
+# multilayered convolution, exchange couples the planes of a mesh, each mesh has its field.
+
+def trilayer_like(nx, ny, seed):
+    """Three meshes of 2, 1, 2 planes of 4 nm separated by 1 nm gaps, outer ones along -x and
+    the middle one along +x, fields of their own (the reference's trilayer set-up, small)."""
+    hz, z0, grp = api.stack_of_meshes([(2, 4e-9, -9e-9), (1, 4e-9, 0.0), (2, 4e-9, 5e-9)])
+    f_out, f_mid = np.array([19924.0, 1743.1, 0.0]), np.array([4981.0, 435.8, 0.0])
+    Ha = np.array([f_out if g != 1 else f_mid for g in grp])
+    nl = nx * ny
+    sign = np.where(grp == 1, 1.0, -1.0)
+    rng = np.random.default_rng(seed)
+    th, ph = 0.05 * rng.random(len(grp) * nl), 2 * np.pi * rng.random(len(grp) * nl)
+    m = np.stack([np.cos(th), np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph)], 1) * 8e5
+    m[:, 0] *= np.repeat(sign, nl)
+    return hz, z0, grp, Ha, m
+
+
+@pytest.mark.parametrize("method", [BM_EVAL_RK4, BM_EVAL_RKDP54])
+def test_stack_of_3d_meshes_matches_oracle(gpu, method):
+    nx, ny = 32, 16
+    hz, z0, grp, Ha, M0 = trilayer_like(nx, ny, 3)
+    p = FMParams(Ms=8e5, A=1.3e-11, alpha=0.02, modules=BM_MOD_DEMAG | BM_MOD_EXCHANGE | BM_MOD_ZEEMAN)
+    s = api.LayerStackSolver(method, nx, ny, 4e-9, 4e-9, hz, z0, p, M0=M0.astype(np.float32))
+    s.set_layer_meshes(grp, Ha)
+    q = FMParams(**{**p.__dict__, "modules": p.modules & ~BM_MOD_DEMAG})
+    o = O.OracleSim(method, (nx, ny, len(hz)), (4e-9, 4e-9, 4e-9), q)
+    o.set_layers(4e-9, 4e-9, hz, z0, with_demag=True)
+    o.set_layer_meshes(grp, Ha)
+    o.set_M(M0)
+    if method == BM_EVAL_RK4:
+        s.set_dt(2e-13); o.set_dt(2e-13)
+    s.run_to_iteration(40)
+    o.run_to_iteration(40)
+    assert s.state()["rejections"] == o.state()["rejections"]
+    assert abs(s.state()["time"] - o.state()["time"]) <= 1e-5 * o.state()["time"]
+    e = float(np.linalg.norm(s.M.cpu().numpy().astype(np.float64) - o.get_M()) / np.linalg.norm(o.get_M()))
+    assert e <= 1e-5, e
+
+
+def test_one_mesh_stack_equals_the_3d_mesh(gpu):
+    """Four touching 5 nm planes grouped as one mesh follow the plain 3D mesh of four planes."""
+    nx, ny, L, hz = 32, 16, 4, 5e-9
+    p = FMParams(Ms=8e5, A=1.3e-11, alpha=0.05, Ha=(2e4, 3e3, 0.0), modules=BM_MOD_DEMAG | BM_MOD_EXCHANGE | BM_MOD_ZEEMAN)
+    rng = np.random.default_rng(8)
+    M0 = api.random_magnetisation(L * nx * ny, p.Ms, 8).astype(np.float32)
+    plain = api.FMSolver(BM_EVAL_RK4, (nx, ny, L), (4e-9, 4e-9, hz), p, M0)
+    st = api.LayerStackSolver(BM_EVAL_RK4, nx, ny, 4e-9, 4e-9, [hz] * L, [k * hz for k in range(L)], p, M0=M0)
+    st.set_layer_meshes([0] * L)
+    for s in (plain, st):
+        s.set_dt(1e-13); s.run_to_iteration(30)
+    a, b = st.M.cpu().numpy().astype(np.float64), plain.M.cpu().numpy().astype(np.float64)
+    assert float(np.linalg.norm(a - b) / np.linalg.norm(b)) <= 2e-5
+
+
+def test_sixteen_layer_stack_matches_oracle(gpu):
+    """The largest stack (16 layers of uneven thickness and spacing), all pairs in registers."""
+    nx, ny, L = 16, 16, 16
+    hz = np.array([2e-9, 4e-9, 3e-9, 5e-9] * 4)
+    z0 = np.cumsum(np.concatenate([[0.0], hz[:-1] + 1e-9]))
+    p = FMParams(Ms=8e5, A=1.3e-11, alpha=0.1, Ha=(1e4, 0.0, 5e3), modules=BM_MOD_DEMAG | BM_MOD_EXCHANGE | BM_MOD_ZEEMAN)
+    M0 = api.random_magnetisation(L * nx * ny, p.Ms, 16).astype(np.float32)
+    s = api.LayerStackSolver(BM_EVAL_RK4, nx, ny, 4e-9, 4e-9, hz, z0, p, M0=M0)
+    q = FMParams(**{**p.__dict__, "modules": p.modules & ~BM_MOD_DEMAG})
+    o = O.OracleSim(BM_EVAL_RK4, (nx, ny, L), (4e-9, 4e-9, 4e-9), q)
+    o.set_layers(4e-9, 4e-9, hz, z0, with_demag=True)
+    o.set_M(M0.astype(np.float64))
+    for x in (s, o):
+        x.set_dt(1e-13); x.run_to_iteration(20)
+    e = float(np.linalg.norm(s.M.cpu().numpy().astype(np.float64) - o.get_M()) / np.linalg.norm(o.get_M()))
+    assert e <= 1e-5, e

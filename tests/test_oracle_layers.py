@@ -68,3 +68,40 @@ def test_two_adjacent_layers_are_the_two_plane_mesh():
     for s in (plain, st):
         s.set_dt(2e-13); s.run_to_iteration(30)
     np.testing.assert_allclose(st.get_M(), plain.get_M(), rtol=0, atol=1e-8 * p.Ms)
+
+
+def test_one_mesh_of_touching_planes_is_the_3d_mesh():
+    """Stacks of 3D meshes: four touching planes grouped as one mesh (exchange across the
+    planes, 5 nm spacing) integrate exactly as the plain four-plane 3D mesh with demag,
+    exchange and a field; per-layer fields all equal to the field change nothing."""
+    p = FMParams(Ms=8e5, A=1.3e-11, alpha=0.05, Ha=(2e4, 3e3, 0.0), modules=BM_MOD_DEMAG | BM_MOD_EXCHANGE | BM_MOD_ZEEMAN)
+    hz, L = 5e-9, 4
+    M0 = tilted(L * NX * NY, np.full(L * NX * NY, p.Ms), seed=21, tilt=0.8)
+    plain = O.OracleSim(BM_EVAL_RK4, (NX, NY, L), (HX, HY, hz), p)
+    q = FMParams(**{**p.__dict__, "modules": p.modules & ~BM_MOD_DEMAG})
+    st = O.OracleSim(BM_EVAL_RK4, (NX, NY, L), (HX, HY, hz), q)
+    st.set_layers(HX, HY, [hz] * L, [k * hz for k in range(L)], with_demag=True)
+    st.set_layer_meshes([0] * L, np.tile(p.Ha, (L, 1)))
+    for s in (plain, st):
+        s.set_M(M0); s.set_dt(1e-13); s.run_to_iteration(40)
+    np.testing.assert_allclose(st.get_M(), plain.get_M(), rtol=0, atol=1e-8 * p.Ms)
+
+
+def test_meshes_of_a_stack_exchange_only_within_themselves():
+    """Demag off, three meshes of 2, 1, 2 planes with their own fields: each evolves as the
+    separate 3D mesh (or 2D mesh) with its field."""
+    p = FMParams(Ms=8e5, A=1.3e-11, alpha=0.05, modules=BM_MOD_EXCHANGE | BM_MOD_ZEEMAN)
+    hz = 4e-9
+    groups, fields = [0, 0, 1, 2, 2], np.array([[2e4, 1e3, 0], [2e4, 1e3, 0], [-5e3, 4e3, 0], [1e4, -2e4, 3e3], [1e4, -2e4, 3e3]], float)
+    z0 = [0.0, 4e-9, 9e-9, 14e-9, 18e-9]
+    nl = NX * NY
+    M0 = tilted(5 * nl, np.full(5 * nl, p.Ms), seed=4, tilt=0.9)
+    st = O.OracleSim(BM_EVAL_RK4, (NX, NY, 5), (HX, HY, hz), p)
+    st.set_layers(HX, HY, [hz] * 5, z0, with_demag=False)
+    st.set_layer_meshes(groups, fields)
+    st.set_M(M0); st.set_dt(1e-13); st.run_to_iteration(50)
+    for first, count in ((0, 2), (2, 1), (3, 2)):
+        q = FMParams(**{**p.__dict__, "Ha": tuple(fields[first])})
+        s = O.OracleSim(BM_EVAL_RK4, (NX, NY, count), (HX, HY, hz), q)
+        s.set_M(M0[first * nl:(first + count) * nl]); s.set_dt(1e-13); s.run_to_iteration(50)
+        np.testing.assert_allclose(st.get_M()[first * nl:(first + count) * nl], s.get_M(), rtol=0, atol=1e-9 * p.Ms)
