@@ -8,7 +8,7 @@ reference checkout and oracle/_ref exist; the GPU test runs only read the result
   reader's input and the byte-level target of the OVF2 writer.
 * trilayer_switch_mconv.txt — the reference-held data file "MLConv Paper/3D/trilayer_switch_mconv.txt"
   (a `savedata` table: header and <M> of three meshes vs time) copied verbatim: the target of
-  the data-table writer's format.
+  the data-table writer's format, and the trajectory the GPU trilayer test must reproduce.
 * tensor_golden.npz — FP64 Newell demag tensors written by oracle/_ref/ref_dump, i.e. by the
   reference's own Boris/DemagTFunc*.cpp compiled here: first-octant values (the rest are
   mirror images) for small meshes, and a seeded sample of entries for meshes large enough to
