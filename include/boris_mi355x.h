@@ -54,7 +54,8 @@ int bm_demag_create(int nx, int ny, int nz, double hx, double hy, double hz, int
  * of the cell thickness.  Sizes: N.x / 2 and N.y in the 2^a 3^b plans 2..2048 (so a periodic
  * nx is a multiple of 4), a periodic nz (3D) a power of two 4..128; anything else returns
  * BM_ERR_SIZE.  All three counts zero is bm_demag_create.  The convolution calls take the
- * handle unchanged; the integrator and multilayer handles have no periodic variant.
+ * handle unchanged; the integrator takes such a handle (with bm_ode_set_pbc), the batch,
+ * adjoint and multilayer handles have no periodic variant.
  */
 int bm_demag_create_pbc(int nx, int ny, int nz, double hx, double hy, double hz, int pbc_x, int pbc_y, int pbc_z,
 	int flags, bm_demag_t* out);
@@ -91,7 +92,7 @@ int bm_demag_convolute_planar(bm_demag_t d, const float* M, float* H3n, void* st
 int bm_demag_copy_kernels(bm_demag_t d, float* host_out, long long count);
 
 /* ------------------------------------------------------------------------------------
- * Multilayered demag of stacked 2D layers: L (<= 8) layers of the same nx x ny cells
+ * Multilayered demag of stacked 2D layers: L (<= 16) layers of the same nx x ny cells
  * (hx, hy, hz[l]), layer l's bottom face at height z0[l] (m); layer thicknesses may differ.
  * M and H are L layers of nx*ny cells stacked (layer-major, AoS per cell).
  *
@@ -161,7 +162,7 @@ int bm_ode_destroy(bm_ode_t o);
 int bm_ode_set_ms_map(bm_ode_t o, const float* Ms_map);
 int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p);
 /* Layer stack: the mesh's nz planes are the layers of the multilayered demag ml (same nx,
- * ny, nz == layer count; at most 8), as the reference's one-ferromagnetic-mesh-per-layer
+ * ny, nz == layer count; at most 16), as the reference's one-ferromagnetic-mesh-per-layer
  * stack coupled by SDemag (Boris/SDemag.cpp:173-300, SDemagCUDA::UpdateField;
  * Boris/Exchange.cpp:108, Boris/Anisotropy.cpp:82 and Boris/DiffEqFM_Equations.cpp:12 per
  * layer mesh).  Exchange acts within each layer; Ms, A, alpha, K1 may be given per layer
