@@ -30,6 +30,9 @@
 
 using namespace bm;
 
+// layer stacks in the adjoint: per-layer values passed in the kernel arguments (8 layers)
+constexpr int kMaxAdjLayers = 8;
+
 namespace {
 
 constexpr int kAB = 256;
@@ -45,7 +48,7 @@ struct AdjArgs {
 	int modules;
 	const float* Ms;
 	int layered;            // layer stack: per-layer A, alpha, K1, no exchange across layers
-	float lay_A[kMaxStackLayers], lay_alpha[kMaxStackLayers], lay_K1[kMaxStackLayers];
+	float lay_A[kMaxAdjLayers], lay_alpha[kMaxAdjLayers], lay_K1[kMaxAdjLayers];
 };
 
 __device__ __forceinline__ CellMat adj_mat(const AdjArgs& a, int k)
@@ -53,7 +56,7 @@ __device__ __forceinline__ CellMat adj_mat(const AdjArgs& a, int k)
 	if (!a.layered) return CellMat{ a.A, a.alpha, a.K1 };
 	CellMat c{ a.lay_A[0], a.lay_alpha[0], a.lay_K1[0] };
 #pragma unroll
-	for (int l = 1; l < kMaxStackLayers; l++)
+	for (int l = 1; l < kMaxAdjLayers; l++)
 		if (k == l) c = CellMat{ a.lay_A[l], a.lay_alpha[l], a.lay_K1[l] };
 	return c;
 }
@@ -267,7 +270,7 @@ struct bm_adj_s {
 	bm_demag_t demag;
 	bm_mlayer_t ml = nullptr;     // layer stack (bm_adj_set_layers)
 	bool lay_set[3] = { false, false, false };
-	float lay[3][bm::kMaxStackLayers] = {};
+	float lay[3][kMaxAdjLayers] = {};
 	double gz[4] = { 0, 0, 0, 0 };  // dL/d amp, dL/d slope of the last backward pass
 	const float* Ms;
 	double dt;
@@ -307,7 +310,7 @@ AdjArgs adj_args(bm_adj_s* o, int step)
 	if (o->ml) {
 		a.layered = 1;
 		a.ihz2 = 0.f;
-		for (int l = 0; l < kMaxStackLayers; l++) {
+		for (int l = 0; l < kMaxAdjLayers; l++) {
 			a.lay_A[l] = o->lay_set[0] ? o->lay[0][l] : a.A;
 			a.lay_alpha[l] = o->lay_set[1] ? o->lay[1][l] : a.alpha;
 			a.lay_K1[l] = o->lay_set[2] ? o->lay[2][l] : a.K1;
@@ -650,11 +653,11 @@ extern "C" int bm_adj_set_layers(bm_adj_t o, bm_mlayer_t ml, const double* A, co
 	int L, nx, ny;
 	mlayer_dims(ml, &L, &nx, &ny, nullptr);
 	if (L != o->nz || nx != o->nx || ny != o->ny) return fail(BM_ERR_ARG, "bm_adj_set_layers: the stack's layers must be the mesh's nz planes (same nx, ny)");
-	if (L > kMaxStackLayers) return fail(BM_ERR_SIZE, "bm_adj_set_layers: at most 16 layers");
+	if (L > kMaxAdjLayers) return fail(BM_ERR_SIZE, "bm_adj_set_layers: at most 8 layers");
 	const double* src[3] = { A, alpha, K1 };
 	for (int q = 0; q < 3; q++) {
 		o->lay_set[q] = src[q] != nullptr;
-		for (int l = 0; l < kMaxStackLayers; l++) o->lay[q][l] = src[q] && l < L ? (float)src[q][l] : 0.f;
+		for (int l = 0; l < kMaxAdjLayers; l++) o->lay[q][l] = src[q] && l < L ? (float)src[q][l] : 0.f;
 	}
 	o->ml = ml;
 	o->p.modules |= BM_MOD_DEMAG;

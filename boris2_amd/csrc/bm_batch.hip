@@ -65,6 +65,7 @@ struct bm_batch_s {
 	std::vector<bm_mlayer_t> ml;   // layer stacks: their multilayered demag (else null)
 	std::vector<std::vector<float>> lay;  // layer stacks: A, alpha, K1 per layer (3 x kMaxStackLayers)
 	std::vector<int> lay_set;      // bits 0..2: A, alpha, K1 given per layer
+	LayerTab* d_laytab = nullptr;  // per-mesh layer tables of the stacks (device)
 	float* Msmap = nullptr;        // per-cell Ms of the stacks with per-layer Ms (ntot floats)
 	double* d_time = nullptr;      // step start time on the device (time-dependent fields)
 	double* d_dT = nullptr;
@@ -108,6 +109,7 @@ void free_all(bm_batch_s* o)
 	for (bm_mlayer_t m : o->ml) if (m) bm_mlayer_destroy(m);
 	if (o->Msmap) (void)hipFree(o->Msmap);
 	if (o->d_time) (void)hipFree(o->d_time);
+	if (o->d_laytab) (void)hipFree(o->d_laytab);
 	if (o->d_dT) (void)hipFree(o->d_dT);
 	if (o->Mb) (void)hipFree(o->Mb);
 	if (o->sM1) (void)hipFree(o->sM1);
@@ -159,11 +161,7 @@ StageArgs stage_args(bm_batch_s* o, int b, int parity)
 	if (o->ml[b]) {
 		a.layered = 1;
 		a.ihz2 = 0.f;
-		for (int l = 0; l < kMaxStackLayers; l++) {
-			a.lay_A[l] = (o->lay_set[b] & 1) ? o->lay[b][l] : a.A;
-			a.lay_alpha[l] = (o->lay_set[b] & 2) ? o->lay[b][kMaxStackLayers + l] : a.alpha;
-			a.lay_K1[l] = (o->lay_set[b] & 4) ? o->lay[b][2 * kMaxStackLayers + l] : a.K1;
-		}
+		a.lay = o->d_laytab + b;
 		if (o->Msmap && (o->lay_set[b] & 8)) a.Ms_map = o->Msmap + o->off[b];
 	}
 	return a;
@@ -522,7 +520,7 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 		const bm_layer_desc* ld = layers ? &layers[b] : nullptr;
 		if (ld && ld->nlayers > 0) {
 			if (ld->nlayers != o->nz[b] || ld->nlayers > kMaxStackLayers || !ld->hz || !ld->z0)
-				rc = fail(BM_ERR_ARG, "bm_batch_create_layers: a stack needs nz == nlayers <= 8 and its hz, z0");
+				rc = fail(BM_ERR_ARG, "bm_batch_create_layers: a stack needs nz == nlayers <= 16 and its hz, z0");
 			else if (!(o->p[b].modules & BM_MOD_DEMAG))
 				rc = fail(BM_ERR_ARG, "bm_batch_create_layers: a layer stack needs the demag module");
 			else rc = bm_mlayer_create(ld->nlayers, o->nx[b], o->ny[b], o->h[3 * b], o->h[3 * b + 1], ld->hz, ld->z0, 0, &o->ml[b]);
@@ -532,6 +530,20 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 			if (ld->Ms) { o->lay_set[b] |= 8; any_lay_ms = true; }
 		}
 		else if (o->p[b].modules & BM_MOD_DEMAG) rc = bm_demag_create(o->nx[b], o->ny[b], o->nz[b], o->h[3 * b], o->h[3 * b + 1], o->h[3 * b + 2], 0, &o->dm[b]);
+	}
+	// per-layer tables of the stacks (A, alpha, K1 given per layer or the mesh's)
+	if (rc == BM_OK && layers) {
+		std::vector<LayerTab> tab(nmesh);
+		std::memset(tab.data(), 0, tab.size() * sizeof(LayerTab));
+		for (int b = 0; b < nmesh; b++)
+			for (int l = 0; l < kMaxStackLayers; l++) {
+				tab[b].A[l] = (o->lay_set[b] & 1) ? o->lay[b][l] : (float)o->p[b].A;
+				tab[b].alpha[l] = (o->lay_set[b] & 2) ? o->lay[b][kMaxStackLayers + l] : (float)o->p[b].alpha;
+				tab[b].K1[l] = (o->lay_set[b] & 4) ? o->lay[b][2 * kMaxStackLayers + l] : (float)o->p[b].K1;
+			}
+		if (hipMalloc(&o->d_laytab, tab.size() * sizeof(LayerTab)) != hipSuccess
+			|| hipMemcpy(o->d_laytab, tab.data(), tab.size() * sizeof(LayerTab), hipMemcpyHostToDevice) != hipSuccess)
+			rc = fail(BM_ERR_NOMEM, "bm_batch_create_layers: layer tables");
 	}
 	if (rc == BM_OK && (hipMalloc(&o->d_time, sizeof(double)) != hipSuccess || hipMalloc(&o->d_dT, sizeof(double)) != hipSuccess
 		|| hipMemset(o->d_time, 0, sizeof(double)) != hipSuccess || hipMemcpy(o->d_dT, &o->dT, sizeof(double), hipMemcpyHostToDevice) != hipSuccess))

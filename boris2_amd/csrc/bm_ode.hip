@@ -49,13 +49,8 @@ __device__ __forceinline__ void stage_sweep(const StageArgs& a)
 		F3 Hd{ 0.f, 0.f, 0.f };
 		if (a.nsp || a.Hsave) {
 			// self coefficient of the cell: the mesh's, or its layer's in a layer stack
-			// (compile-time indices keep StageArgs in registers)
 			F3 sc{ a.selfc[0], a.selfc[1], a.selfc[2] };
-			if (a.layered) {
-#pragma unroll
-				for (int l = 0; l < kMaxStackLayers; l++)
-					if (k == l) sc = F3{ a.lay_selfc[l][0], a.lay_selfc[l][1], a.lay_selfc[l][2] };
-			}
+			if (a.layered) sc = F3{ a.lay->selfc[k][0], a.lay->selfc[k][1], a.lay->selfc[k][2] };
 			const F3 m = ld3(a.Min, idx);
 			if (a.nsp) {
 				// Demag_EvalSpeedup_AddExtrapField_AddSelf: saved fields in slot order, then self term
@@ -134,6 +129,7 @@ struct bm_ode_s {
 	int lay_grp[bm::kMaxStackLayers] = {};
 	float lay_ihz2[bm::kMaxStackLayers] = {};
 	float lay_Ha[bm::kMaxStackLayers][3] = {};
+	bm::LayerTab* d_lay = nullptr;   // device copy of the per-layer values (upload_layers)
 	float* Ms_layers = nullptr;    // per-cell Ms built from per-layer values (owned)
 	float* M = nullptr;            // the mesh's magnetisation (caller-owned); holds M between steps
 	float* Mb = nullptr;           // second magnetisation buffer for intermediate stages
@@ -229,7 +225,6 @@ StageArgs make_args(bm_ode_s* o, float dT)
 		a.nsp = o->sp_n;
 		for (int q = 0; q < 6; q++) { a.Hs[q] = o->Hs[q]; a.ca[q] = o->sp_ca[q]; }
 		for (int c = 0; c < 3; c++) a.selfc[c] = o->selfc[c];
-		for (int l = 0; l < kMaxStackLayers; l++) for (int c = 0; c < 3; c++) a.lay_selfc[l][c] = o->lay_selfc[l][c];
 		a.Hsave = o->sp_save;
 	}
 	if (o->ml) {
@@ -237,16 +232,7 @@ StageArgs make_args(bm_ode_s* o, float dT)
 		a.ihz2 = 0.f;
 		a.lay_grouped = o->lay_grouped ? 1 : 0;
 		a.lay_field = o->lay_field ? 1 : 0;
-		for (int l = 0; l < kMaxStackLayers; l++) {
-			a.lay_grp[l] = o->lay_grp[l];
-			a.lay_ihz2[l] = o->lay_ihz2[l];
-			for (int c = 0; c < 3; c++) a.lay_Ha[l][c] = o->lay_Ha[l][c];
-		}
-		for (int l = 0; l < kMaxStackLayers; l++) {
-			a.lay_A[l] = o->lay_set[0] ? o->lay[0][l] : a.A;
-			a.lay_alpha[l] = o->lay_set[1] ? o->lay[1][l] : a.alpha;
-			a.lay_K1[l] = o->lay_set[2] ? o->lay[2][l] : a.K1;
-		}
+		a.lay = o->d_lay;
 	}
 	return a;
 }
@@ -411,10 +397,30 @@ int eval_speedup(bm_ode_s* o, int step, float dT, hipStream_t st)
 	return rc;
 }
 
+// Per-layer values of a stack into its device table (A, alpha, K1 given per layer or the
+// mesh's, self coefficients, mesh grouping, fields); called whenever any of them changes.
+int upload_layers(bm_ode_s* o)
+{
+	if (!o->ml) return BM_OK;
+	if (!o->d_lay && hipMalloc(&o->d_lay, sizeof(bm::LayerTab)) != hipSuccess) { o->d_lay = nullptr; return bm::fail(BM_ERR_NOMEM, "layer table"); }
+	bm::LayerTab t;
+	std::memset(&t, 0, sizeof(t));
+	for (int l = 0; l < kMaxStackLayers; l++) {
+		t.A[l] = o->lay_set[0] ? o->lay[0][l] : (float)o->p.A;
+		t.alpha[l] = o->lay_set[1] ? o->lay[1][l] : (float)o->p.alpha;
+		t.K1[l] = o->lay_set[2] ? o->lay[2][l] : (float)o->p.K1;
+		t.grp[l] = o->lay_grp[l];
+		t.ihz2[l] = o->lay_ihz2[l];
+		for (int c = 0; c < 3; c++) { t.selfc[l][c] = o->lay_selfc[l][c]; t.Ha[l][c] = o->lay_Ha[l][c]; }
+	}
+	BM_HIP(hipMemcpy(o->d_lay, &t, sizeof(t), hipMemcpyHostToDevice));
+	return BM_OK;
+}
+
 // Self-demag coefficients of the evaluation speedup: the mesh's (DemagTFunc::SelfDemag_PBC
 // with the mesh's periodic images, cell size in metres, Boris/DemagCUDA.cpp:37-39) and, for a
 // layer stack, each layer mesh's own (SDemagCUDA_Demag::selfDemagCoeff).
-void set_self_coeffs(bm_ode_s* o)
+int set_self_coeffs(bm_ode_s* o)
 {
 	double sd[3];
 	if (o->pbc[0] || o->pbc[1] || o->pbc[2])
@@ -428,6 +434,7 @@ void set_self_coeffs(bm_ode_s* o)
 			bm::self_demag_coeff(hl[0], hl[1], hl[2], sd);
 			for (int c = 0; c < 3; c++) o->lay_selfc[l][c] = (float)sd[c];
 		}
+	return upload_layers(o);
 }
 
 // Demag evaluations go through the fused row kernel (single mesh open along x, no speedup).
@@ -586,6 +593,7 @@ void free_all(bm_ode_s* o)
 	if (o->d_part) (void)hipFree(o->d_part);
 	if (o->h_part) (void)hipHostFree(o->h_part);
 	if (o->Ms_layers) (void)hipFree(o->Ms_layers);
+	if (o->d_lay) (void)hipFree(o->d_lay);
 }
 
 } // namespace
@@ -692,8 +700,7 @@ extern "C" int bm_ode_set_layers(bm_ode_t o, bm_mlayer_t ml, const double* Ms, c
 	o->S_valid = false;
 	o->num_saved = 0;              // saved demag fields belong to the previous configuration
 	o->lay_grouped = o->lay_field = false;
-	set_self_coeffs(o);
-	return BM_OK;
+	return set_self_coeffs(o);
 }
 
 // Stacks of 3D meshes: consecutive layers of the stack with the same mesh index are the z
@@ -730,7 +737,7 @@ extern "C" int bm_ode_set_layer_meshes(bm_ode_t o, const int* mesh_of_layer, con
 	o->lay_grouped = mesh_of_layer != nullptr;
 	o->lay_field = Ha != nullptr;
 	o->num_saved = 0;
-	return BM_OK;
+	return upload_layers(o);
 }
 
 extern "C" int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p)
@@ -745,7 +752,7 @@ extern "C" int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p)
 	o->p = *p;
 	o->S_valid = false;
 	o->num_saved = 0;      // DemagCUDA::UpdateConfiguration drops the saved fields
-	return BM_OK;
+	return upload_layers(o);   // a stack's per-layer table holds the mesh values it defaults to
 }
 
 // Periodic boundary conditions (the `pbc` command: Demag::Set_PBC and Mesh::Set_Magnetic_PBC,
@@ -768,8 +775,7 @@ extern "C" int bm_ode_set_pbc(bm_ode_t o, int pbc_x, int pbc_y, int pbc_z)
 	}
 	o->S_valid = false;
 	o->num_saved = 0;
-	set_self_coeffs(o);
-	return BM_OK;
+	return set_self_coeffs(o);
 }
 
 extern "C" int bm_ode_set_eval_speedup(bm_ode_t o, int mode, double dTspeedup)
@@ -788,7 +794,7 @@ extern "C" int bm_ode_set_eval_speedup(bm_ode_t o, int mode, double dTspeedup)
 	o->num_saved = 0;
 	for (int q = 0; q < 6; q++) o->tdm[q] = 0;
 	// DemagTFunc::SelfDemag_PBC without images, cell size in metres (Boris/DemagCUDA.cpp:37)
-	set_self_coeffs(o);
+	if (int rc = set_self_coeffs(o)) return rc;
 	if (dTspeedup > 0) { o->link_dTspeedup = false; o->dTspeedup = dTspeedup; o->time_speedup = o->time; }
 	else { o->link_dTspeedup = true; o->dTspeedup = o->dT; }
 	o->S_valid = false;

@@ -65,6 +65,18 @@ __device__ __forceinline__ float rel_err(P6 m, P6 p) { return fmaxf(rel_err(m.a,
 
 constexpr int kMaxStackLayers = 16;
 
+// Per-layer values of a layer stack, kept in device memory (one table per stack, written at
+// configuration) and indexed by the cell's layer, so the stage arguments stay small.
+struct LayerTab {
+	float A[kMaxStackLayers], alpha[kMaxStackLayers], K1[kMaxStackLayers];
+	float selfc[kMaxStackLayers][3];   // evaluation speedup: each layer's own self coefficient
+	// stacks of 3D meshes: layers with equal grp are consecutive z planes of one mesh, coupled
+	// by exchange across planes with 1 / hz^2 = ihz2; Ha: each layer's constant applied field
+	int grp[kMaxStackLayers];
+	float ihz2[kMaxStackLayers];
+	float Ha[kMaxStackLayers][3];
+};
+
 // Per-mesh adaptive step control of a batched RKDP54 integration, kept on the device so a
 // captured step graph runs it (ODECommon_Base's time, dT, lte, iteration, primed state and
 // SetAdaptiveTimeStep controls, Boris/DiffEq_CommonBase.h, Boris/DiffEq_CommonBase_Iterate.cpp:15-59).
@@ -98,14 +110,10 @@ struct StageArgs {
 	// layer stack: the nz planes are separate 2D layers (one ferromagnetic mesh each, no
 	// exchange between them) with their own A, alpha and K1 (when layered != 0)
 	int layered;
-	float lay_A[kMaxStackLayers], lay_alpha[kMaxStackLayers], lay_K1[kMaxStackLayers];
-	// stacks of 3D meshes: layers with equal lay_grp are consecutive z planes of one mesh,
-	// coupled by exchange across planes with 1 / hz^2 = lay_ihz2 (when lay_grouped != 0);
-	// with lay_field != 0 each layer has its own constant applied field lay_Ha
+	const LayerTab* lay;    // per-layer values (device memory)
+	// stacks of 3D meshes: exchange across the planes of a mesh (lay_grouped != 0) and
+	// per-layer applied fields (lay_field != 0), from lay
 	int lay_grouped, lay_field;
-	int lay_grp[kMaxStackLayers];
-	float lay_ihz2[kMaxStackLayers];
-	float lay_Ha[kMaxStackLayers][3];
 	// time-dependent applied field read on the device (batched graphs): with tnow set, the
 	// stage kernel evaluates Ha0 + amp dir sinc(w (t - t0)) (wave 1) or Ha0 + slope t (wave 2)
 	// at t = *tnow in FP64 and rounds it to FP32, as the host does for a single mesh
@@ -120,7 +128,6 @@ struct StageArgs {
 	float* Hsave;
 	float ca[6], selfc[3];
 	int nsp;
-	float lay_selfc[kMaxStackLayers][3];   // layer stacks: each layer's own self coefficient
 	// batched adaptive integration: the mesh's step control; stage 0 runs while the mesh is
 	// active, stages 1..5 unless the attempt was rejected
 	const BatchCtl* ctl;
@@ -148,22 +155,7 @@ struct CellMat { float A, alpha, K1; };
 __device__ __forceinline__ CellMat cell_mat(const StageArgs& a, int k)
 {
 	if (!a.layered) return CellMat{ a.A, a.alpha, a.K1 };
-	CellMat c{ a.lay_A[0], a.lay_alpha[0], a.lay_K1[0] };
-#pragma unroll
-	for (int l = 1; l < kMaxStackLayers; l++)
-		if (k == l) c = CellMat{ a.lay_A[l], a.lay_alpha[l], a.lay_K1[l] };
-	return c;
-}
-
-// Value of layer k of a per-layer table, picked with compile-time indices.
-template <typename T>
-__device__ __forceinline__ T lay_pick(const T* v, int k)
-{
-	T r = v[0];
-#pragma unroll
-	for (int l = 1; l < kMaxStackLayers; l++)
-		if (k == l) r = v[l];
-	return r;
+	return CellMat{ a.lay->A[k], a.lay->alpha[k], a.lay->K1[k] };
 }
 
 enum { M_RKDP = 0, M_RK4 = 1, M_TEULER = 2, M_AHEUN = 3 };
@@ -312,23 +304,18 @@ __device__ __forceinline__ void stage_cell(const StageArgs& a, F3 Ha, size_t idx
 		}
 		else if (a.lay_grouped && a.nz > 1) {
 			// planes of one 3D mesh in a stack: neighbours only within the mesh (free ends)
-			const int g = lay_pick(a.lay_grp, k);
-			const bool lo = k > 0 && lay_pick(a.lay_grp, k - 1) == g, hi = k < a.nz - 1 && lay_pick(a.lay_grp, k + 1) == g;
+			const int* grp = a.lay->grp;
+			const int g = grp[k];
+			const bool lo = k > 0 && grp[k - 1] == g, hi = k < a.nz - 1 && grp[k + 1] == g;
 			if (lo && hi) lz = ld3(a.Min, idx + sz) + ld3(a.Min, idx - sz) - 2.f * m;
 			else if (hi) lz = ld3(a.Min, idx + sz) - m;
 			else if (lo) lz = ld3(a.Min, idx - sz) - m;
-			ihz2 = lay_pick(a.lay_ihz2, k);
+			ihz2 = a.lay->ihz2[k];
 		}
 		lap = a.ihx2 * lx + a.ihy2 * ly + ihz2 * lz;
 	}
 	const CellMat cm = cell_mat(a, k);
-	if (a.lay_field) {
-		F3 hl{ a.lay_Ha[0][0], a.lay_Ha[0][1], a.lay_Ha[0][2] };
-#pragma unroll
-		for (int l = 1; l < kMaxStackLayers; l++)
-			if (k == l) hl = F3{ a.lay_Ha[l][0], a.lay_Ha[l][1], a.lay_Ha[l][2] };
-		Ha = hl;
-	}
+	if (a.lay_field) Ha = F3{ a.lay->Ha[k][0], a.lay->Ha[k][1], a.lay->Ha[k][2] };
 	const F3 H = eff_field(a, cm, Ha, m, Hd, lap, Ms);
 	if (a.Heff_out) st3(a.Heff_out, idx, H);
 	(void)rk_update<METHOD, STEP>(a, idx, m, llg_rhs(a, cm.alpha, m, H, Ms), Ms, lte_loc);
