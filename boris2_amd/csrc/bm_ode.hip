@@ -437,10 +437,12 @@ int set_self_coeffs(bm_ode_s* o)
 	return upload_layers(o);
 }
 
-// Demag evaluations go through the fused row kernel (single mesh open along x, no speedup).
+// Demag evaluations go through the fused row kernel (single open-boundary mesh, no speedup;
+// periodic meshes take the unfused path, whose stage kernel wraps exchange around).
 bool row_path(const bm_ode_s* o)
 {
-	return (o->p.modules & BM_MOD_DEMAG) && o->use_row && !o->ml && !o->speedup && !o->pbc[0] && o->demag && !o->demag->full[0];
+	return (o->p.modules & BM_MOD_DEMAG) && o->use_row && !o->ml && !o->speedup && !o->pbc[0] && !o->pbc[1] && !o->pbc[2]
+		&& o->demag && !o->demag->full[0];
 }
 
 // One field evaluation + stage `step` (the effective field is that of Mcur at stagetime).
@@ -769,7 +771,6 @@ extern "C" int bm_ode_set_pbc(bm_ode_t o, int pbc_x, int pbc_y, int pbc_z)
 		&& (o->demag->pbc[0] != pbc_x || o->demag->pbc[1] != pbc_y || o->demag->pbc[2] != pbc_z))
 		return fail(BM_ERR_STATE, "bm_ode_set_pbc: the demag handle must have the same periodic images (bm_demag_create_pbc)");
 	o->pbc[0] = pbc_x; o->pbc[1] = pbc_y; o->pbc[2] = pbc_z;
-	if (pbc_x) o->use_row = false;
 	if ((o->p.modules & BM_MOD_DEMAG) && !row_path(o) && !o->Hd) {
 		if (hipMalloc(&o->Hd, o->n * 3 * sizeof(float)) != hipSuccess) { o->Hd = nullptr; return fail(BM_ERR_NOMEM, "bm_ode_set_pbc: demag field buffer"); }
 	}

@@ -331,7 +331,7 @@ __device__ __forceinline__ P6 stage_pair(const StageArgs& a, size_t idx, int i0,
 	F3 lap0{ 0.f, 0.f, 0.f }, lap1{ 0.f, 0.f, 0.f };
 	if (a.modules & BM_MOD_EXCHANGE) {
 		// x: the two cells are each other's neighbours; clamped loads keep the ends branch-free
-		// (the row kernel serves meshes open along x only)
+		// (the row kernel serves open-boundary meshes only)
 		const bool hasL = i0 > 0, hasR = i0 + 2 < a.nx;
 		const F3 L = ld3(a.Min, hasL ? idx - 1 : idx), R = ld3(a.Min, hasR ? idx + 2 : idx + 1);
 		F3 lx0 = hasL ? (m.b + L) - 2.f * m.a : m.b - m.a;
@@ -340,15 +340,11 @@ __device__ __forceinline__ P6 stage_pair(const StageArgs& a, size_t idx, int i0,
 		const size_t sy = a.nx, sz = (size_t)a.nx * a.ny;
 		if (a.ny > 1) {
 			if (j > 0 && j < a.ny - 1) ly = ld6(a.Min, idx + sy) + ld6(a.Min, idx - sy) - 2.f * m;
-			else if (a.per & 2) ly = (j == 0) ? ld6(a.Min, idx + sy) + ld6(a.Min, idx + (a.ny - 1) * sy) - 2.f * m
-			                                  : ld6(a.Min, idx - (a.ny - 1) * sy) + ld6(a.Min, idx - sy) - 2.f * m;
 			else if (j == 0) ly = ld6(a.Min, idx + sy) - m;
 			else ly = ld6(a.Min, idx - sy) - m;
 		}
 		if (a.nz > 1) {
 			if (k > 0 && k < a.nz - 1) lz = ld6(a.Min, idx + sz) + ld6(a.Min, idx - sz) - 2.f * m;
-			else if (a.per & 4) lz = (k == 0) ? ld6(a.Min, idx + sz) + ld6(a.Min, idx + (a.nz - 1) * sz) - 2.f * m
-			                                  : ld6(a.Min, idx - (a.nz - 1) * sz) + ld6(a.Min, idx - sz) - 2.f * m;
 			else if (k == 0) lz = ld6(a.Min, idx + sz) - m;
 			else lz = ld6(a.Min, idx - sz) - m;
 		}

@@ -104,8 +104,8 @@ def _ode_pair(method, n, h, p, M0, pbc, speedup=0):
 @pytest.mark.parametrize("n,pbc", [((64, 32, 1), (0, 4, 0)), ((32, 16, 4), (0, 2, 3)), ((32, 16, 1), (3, 3, 0)),
                                    ((32, 24, 4), (2, 0, 0))])
 def test_pbc_rkdp54_matches_oracle(gpu, monkeypatch, path, n, pbc):
-    """y/z-periodic meshes run the fused row kernel (its y and z neighbours wrap); x-periodic
-    ones always take the unfused path."""
+    """Periodic meshes take the unfused evaluation whatever the row-kernel switch says (the
+    stage kernel wraps exchange around); both settings must match the oracle."""
     monkeypatch.setenv("BM_ROW_KERNEL", "1" if path == "row" else "0")
     h, p = (4e-9, 4e-9, 4e-9), bench_like_params()
     M0 = tilted_x(n[0] * n[1] * n[2], p.Ms, 31, tilt=0.3)
