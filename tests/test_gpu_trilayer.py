@@ -4,7 +4,9 @@ trilayer_switch_mconv.txt, copied verbatim to tests/golden/): three permalloy me
 128 x 64 x {4, 2, 4} cells of 5 nm, 1 nm gaps, Ms 8e5 A/m, A 13 pJ/m, alpha 0.02; outer meshes
 start along -x in 25.04 mT at 5 degrees, the middle one along +x in 6.26 mT; RK4 at 0.5 ps
 (the file's evalMethod 2, dT 5e-13) for 3 ns.  All three meshes switch; every saved <M>
-component must agree with the file to 1e-4 of Ms (measured: 1.3e-5)."""
+component must agree with the file to 1e-4 of Ms (measured: 1.3e-5).  The run writes its own
+`savedata` table (boris2_amd.datatable) whose header equals the reference file's line for line
+and whose rows carry the same times."""
 from pathlib import Path
 
 import numpy as np
@@ -32,8 +34,15 @@ def reference_rows():
     return np.array(rows)
 
 
-def test_trilayer_switching_matches_the_reference_output(gpu):
+def test_trilayer_switching_matches_the_reference_output(gpu, tmp_path):
+    from boris2_amd import datatable as DT
+    from test_datatable import meshes
     ref = reference_rows()
+    ref_lines = DATA.read_text().split("\n")
+    ms, sm = meshes()
+    by = {m.name: m for m in ms}
+    cols = [DT.time_column()] + [DT.avm_column(by[n]) for n in ("permalloy0", "permalloy", "permalloy2")]
+    table = DT.DataTable(tmp_path / "trilayer.txt", ms, sm, cols, stamp=ref_lines[0])
     assert ref.shape == (300, 10)
     nx, ny = 128, 64
     hz, z0, grp = api.stack_of_meshes([(4, 5e-9, -21e-9), (2, 5e-9, 0.0), (4, 5e-9, 11e-9)])
@@ -55,7 +64,15 @@ def test_trilayer_switching_matches_the_reference_output(gpu):
         M = s.M.cpu().numpy().astype(np.float64)
         got = np.concatenate([M[np.repeat(grp == g, nl)].mean(0) for g in (0, 1, 2)])
         worst = max(worst, float(np.abs(got - r[1:]).max() / 8e5))
+        table.add_row([s.state()["time"], got[0:3], got[3:6], got[6:9]])
     s.close()
     assert worst <= 1e-4, worst
+    out = (tmp_path / "trilayer.txt").read_text().split("\n")
+    assert out[:11] == ref_lines[:11]
+    rows_out = [l for l in out[11:] if l.strip()]
+    rows_ref = [l for l in ref_lines[11:] if l.strip()]
+    assert len(rows_out) == len(rows_ref) == 300
+    for a, b in zip(rows_out, rows_ref):
+        assert a.split("\t")[0] == b.split("\t")[0]          # the same save times, as printed
     # the run is the switching event, not a quiet state: outer meshes end near +x, middle near -x
     assert ref[-1, 1] / 8e5 > 0.8 and ref[-1, 4] / 8e5 < -0.9
