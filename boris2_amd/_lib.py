@@ -44,7 +44,8 @@ class FmParams(C.Structure):
 
 class LayerDesc(C.Structure):
     _fields_ = [("nlayers", C.c_int), ("hz", C.c_void_p), ("z0", C.c_void_p), ("Ms", C.c_void_p),
-                ("A", C.c_void_p), ("alpha", C.c_void_p), ("K1", C.c_void_p)]
+                ("A", C.c_void_p), ("alpha", C.c_void_p), ("K1", C.c_void_p), ("mesh_of_layer", C.c_void_p),
+                ("Ha", C.c_void_p)]
 
 
 class OdeState(C.Structure):

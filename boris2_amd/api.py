@@ -397,8 +397,10 @@ class BatchSolver:
             for b, lay in enumerate(layers):
                 if lay is None:
                     continue
-                arrs = {k: np.ascontiguousarray(lay[k], dtype=np.float64) for k in ("hz", "z0", "Ms", "A", "alpha", "K1")
+                arrs = {k: np.ascontiguousarray(lay[k], dtype=np.float64).reshape(-1) for k in ("hz", "z0", "Ms", "A", "alpha", "K1", "Ha")
                         if lay.get(k) is not None}
+                if lay.get("mesh_of_layer") is not None:
+                    arrs["mesh_of_layer"] = np.ascontiguousarray(lay["mesh_of_layer"], dtype=np.int32)
                 self._keep.append(arrs)
                 ld[b].nlayers = len(arrs["hz"])
                 for k, a in arrs.items():

@@ -66,6 +66,7 @@ struct bm_batch_s {
 	std::vector<std::vector<float>> lay;  // layer stacks: A, alpha, K1 per layer (3 x kMaxStackLayers)
 	std::vector<int> lay_set;      // bits 0..2: A, alpha, K1 given per layer
 	LayerTab* d_laytab = nullptr;  // per-mesh layer tables of the stacks (device)
+	std::vector<int> lay_grouped, lay_field;   // stacks of 3D meshes: grouping and per-layer fields set
 	float* Msmap = nullptr;        // per-cell Ms of the stacks with per-layer Ms (ntot floats)
 	double* d_time = nullptr;      // step start time on the device (time-dependent fields)
 	double* d_dT = nullptr;
@@ -162,6 +163,8 @@ StageArgs stage_args(bm_batch_s* o, int b, int parity)
 		a.layered = 1;
 		a.ihz2 = 0.f;
 		a.lay = o->d_laytab + b;
+		a.lay_grouped = o->lay_grouped[b];
+		a.lay_field = o->lay_field[b];
 		if (o->Msmap && (o->lay_set[b] & 8)) a.Ms_map = o->Msmap + o->off[b];
 	}
 	return a;
@@ -514,6 +517,8 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 	o->ml.assign(nmesh, nullptr);
 	o->lay.assign(nmesh, std::vector<float>(3 * kMaxStackLayers, 0.f));
 	o->lay_set.assign(nmesh, 0);
+	o->lay_grouped.assign(nmesh, 0);
+	o->lay_field.assign(nmesh, 0);
 	int rc = BM_OK;
 	bool any_lay_ms = false;
 	for (int b = 0; b < nmesh && rc == BM_OK; b++) {
@@ -541,8 +546,27 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 				tab[b].alpha[l] = (o->lay_set[b] & 2) ? o->lay[b][kMaxStackLayers + l] : (float)o->p[b].alpha;
 				tab[b].K1[l] = (o->lay_set[b] & 4) ? o->lay[b][2 * kMaxStackLayers + l] : (float)o->p[b].K1;
 			}
-		if (hipMalloc(&o->d_laytab, tab.size() * sizeof(LayerTab)) != hipSuccess
-			|| hipMemcpy(o->d_laytab, tab.data(), tab.size() * sizeof(LayerTab), hipMemcpyHostToDevice) != hipSuccess)
+		// stacks of 3D meshes (as bm_ode_set_layer_meshes): planes of a mesh touch and share a thickness
+		for (int b = 0; b < nmesh && rc == BM_OK; b++) {
+			const bm_layer_desc& ld = layers[b];
+			if (ld.nlayers <= 0) continue;
+			if (ld.Ha && o->p[b].wave != 0) { rc = fail(BM_ERR_ARG, "bm_batch_create_layers: per-layer fields are constant"); break; }
+			for (int l = 0; l < ld.nlayers; l++) {
+				tab[b].grp[l] = ld.mesh_of_layer ? ld.mesh_of_layer[l] : l;
+				tab[b].ihz2[l] = (float)(1.0 / (ld.hz[l] * ld.hz[l]));
+				if (ld.Ha) for (int c = 0; c < 3; c++) tab[b].Ha[l][c] = (float)ld.Ha[3 * l + c];
+				if (l > 0 && ld.mesh_of_layer) {
+					const int g0 = ld.mesh_of_layer[l - 1], g1 = ld.mesh_of_layer[l];
+					const double gap = ld.z0[l] - ld.z0[l - 1];
+					if (g1 < g0 || (g1 == g0 && (std::fabs(ld.hz[l] - ld.hz[l - 1]) > 1e-6 * ld.hz[l] || std::fabs(gap - ld.hz[l - 1]) > 1e-6 * ld.hz[l])))
+						rc = fail(BM_ERR_ARG, "bm_batch_create_layers: the planes of a mesh must be consecutive, touching and of one thickness");
+				}
+			}
+			o->lay_grouped[b] = ld.mesh_of_layer ? 1 : 0;
+			o->lay_field[b] = ld.Ha ? 1 : 0;
+		}
+		if (rc == BM_OK && (hipMalloc(&o->d_laytab, tab.size() * sizeof(LayerTab)) != hipSuccess
+			|| hipMemcpy(o->d_laytab, tab.data(), tab.size() * sizeof(LayerTab), hipMemcpyHostToDevice) != hipSuccess))
 			rc = fail(BM_ERR_NOMEM, "bm_batch_create_layers: layer tables");
 	}
 	if (rc == BM_OK && (hipMalloc(&o->d_time, sizeof(double)) != hipSuccess || hipMalloc(&o->d_dT, sizeof(double)) != hipSuccess

@@ -255,8 +255,9 @@ int bm_ode_average_m(bm_ode_t o, double* out3, void* stream);
  * factor, workgroups per CU) on the real tables and keep the fastest;
  * bm_batch_autotune_report writes the choices and times as text lines.
  * bm_batch_create_layers: layers[b].nlayers > 0 makes mesh b a layer stack as
- * bm_ode_set_layers does (nz == nlayers <= 8, demag on; hz, z0 its layers' thicknesses and
- * bottom heights, Ms/A/alpha/K1 per layer or NULL); layers NULL = bm_batch_create. */
+ * bm_ode_set_layers does (nz == nlayers <= 16, demag on; hz, z0 its layers' thicknesses and
+ * bottom heights, Ms/A/alpha/K1 per layer or NULL); mesh_of_layer and Ha (or NULL) make it a
+ * stack of 3D meshes as bm_ode_set_layer_meshes does; layers NULL = bm_batch_create. */
 typedef struct bm_batch_s* bm_batch_t;
 enum { BM_BATCH_NO_GRAPH = 1, BM_BATCH_AUTOTUNE = 2 };
 int bm_batch_create(int method, int nmesh, const int* n3, const double* h3, const bm_fm_params* params,
@@ -265,6 +266,8 @@ typedef struct {
 	int nlayers;                                    /* 0: a plain mesh */
 	const double* hz; const double* z0;             /* nlayers values each (m) */
 	const double* Ms; const double* A; const double* alpha; const double* K1;   /* per layer or NULL */
+	const int* mesh_of_layer;                       /* stacks of 3D meshes: mesh index per layer, or NULL */
+	const double* Ha;                               /* constant applied field per layer (3 each, A/m), or NULL */
 } bm_layer_desc;
 int bm_batch_create_layers(int method, int nmesh, const int* n3, const double* h3, const bm_fm_params* params,
 	const bm_layer_desc* layers, float* M, int flags, bm_batch_t* out);

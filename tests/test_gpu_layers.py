@@ -158,3 +158,30 @@ def test_sixteen_layer_stack_matches_oracle(gpu):
         x.set_dt(1e-13); x.run_to_iteration(20)
     e = float(np.linalg.norm(s.M.cpu().numpy().astype(np.float64) - o.get_M()) / np.linalg.norm(o.get_M()))
     assert e <= 1e-5, e
+
+
+def test_batched_stacks_of_3d_meshes_equal_single_runs(gpu):
+    """Two trilayer-like stacks (their own seeds) and a plain film in one batch: each stack
+    follows its single-handle run (bm_ode_set_layer_meshes) step for step."""
+    nx, ny = 32, 16
+    p = FMParams(Ms=8e5, A=1.3e-11, alpha=0.02, modules=BM_MOD_DEMAG | BM_MOD_EXCHANGE | BM_MOD_ZEEMAN)
+    stacks = [trilayer_like(nx, ny, seed) for seed in (5, 6)]
+    film = ((48, 32, 1), (4e-9, 4e-9, 4e-9), FMParams(Ms=8e5, A=1.3e-11, alpha=0.05, Ha=(1e4, 0.0, 0.0),
+                                                     modules=BM_MOD_DEMAG | BM_MOD_EXCHANGE | BM_MOD_ZEEMAN))
+    meshes = [((nx, ny, len(hz)), (4e-9, 4e-9, 4e-9), p) for hz, *_ in stacks] + [film]
+    layers = [dict(hz=hz, z0=z0, mesh_of_layer=grp, Ha=Ha) for hz, z0, grp, Ha, _ in stacks] + [None]
+    M0 = [m.astype(np.float32) for *_, m in stacks] + [api.random_magnetisation(48 * 32, 8e5, 2).astype(np.float32)]
+    bs = api.BatchSolver(BM_EVAL_RK4, meshes, M0, layers=layers)
+    bs.set_dt(1e-13)
+    bs.run(30)
+    torch.cuda.synchronize()
+    for b, (hz, z0, grp, Ha, m0) in enumerate(stacks):
+        s = api.LayerStackSolver(BM_EVAL_RK4, nx, ny, 4e-9, 4e-9, hz, z0, p, M0=m0.astype(np.float32))
+        s.set_layer_meshes(grp, Ha)
+        s.set_dt(1e-13)
+        s.run_to_iteration(30)
+        ref = s.M.cpu().numpy().astype(np.float64)
+        e = float(np.linalg.norm(bs.mesh(b).cpu().numpy() - ref) / np.linalg.norm(ref))
+        assert e <= 2e-6, (b, e)
+        s.close()
+    bs.close()
