@@ -440,6 +440,38 @@ def cpu_baseline(cpu_steps: int):
                       f"{dt:.1f} s timed, FP64 kernel setup {setup:.1f} s not included"}
 
 
+def trilayer_run(steps: int):
+    """The reference's own multilayered example of 3D meshes (MLConv Paper/3D/trilayer_switch.bsm:
+    permalloy meshes 128x64x{4,2,4} of 5 nm with 1 nm gaps, per-mesh fields, RK4 at 0.5 ps) as a
+    stack of 10 planes: wall time of `steps` RK4 steps after 20 warm-up steps."""
+    import time
+    import numpy as np
+    import torch
+    from boris2_amd import api
+    nx, ny = 128, 64
+    hz, z0, grp = api.stack_of_meshes([(4, 5e-9, -21e-9), (2, 5e-9, 0.0), (4, 5e-9, 11e-9)])
+    b_out, b_mid = np.array([0.025037111148343075, 0.0021904566281595616, 0.0]), np.array([0.006259271503900462, 0.0005476160419954826, 0.0])
+    Ha = np.array([(b_out if g != 1 else b_mid) / api.MU0 for g in grp])
+    p = api.FMParams(Ms=8e5, A=13e-12, alpha=0.02, modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ZEEMAN)
+    nl = nx * ny
+    M0 = np.zeros((len(grp) * nl, 3), np.float32)
+    M0[:, 0] = np.repeat(np.where(grp == 1, 8e5, -8e5), nl)
+    s = api.LayerStackSolver(api.BM_EVAL_RK4, nx, ny, 5e-9, 5e-9, hz, z0, p, M0=M0)
+    s.set_layer_meshes(grp, Ha)
+    s.set_dt(5e-13)
+    s.run_to_iteration(20)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.run_to_iteration(20 + steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    s.close()
+    ncell = len(grp) * nl
+    return {"workload": "reference trilayer (3 meshes, 10 planes of 128x64, RK4 0.5 ps)", "cells": ncell, "steps": steps,
+            "ms_per_step": round(1e3 * wall / steps, 4), "cell_updates_per_s": ncell * steps / wall,
+            "wall_3ns_s": round(6000 * wall / steps, 3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -451,6 +483,7 @@ def main() -> int:
     ap.add_argument("--adjoint-steps", type=int, default=4000, help="RK4 steps of the adjoint rollout (0: skip)")
     ap.add_argument("--ml-steps", type=int, default=50, help="steps of the batched layer-stack comparison (0: skip)")
     ap.add_argument("--ml-adjoint-steps", type=int, default=1000, help="RK4 steps of the layer-stack adjoint (0: skip)")
+    ap.add_argument("--trilayer-steps", type=int, default=1000, help="RK4 steps of the reference trilayer stack (0: skip)")
     ap.add_argument("--speedup-steps", type=int, default=500, help="RKDP54 steps with the demag evaluation speedup (0: skip)")
     ap.add_argument("--speedup-mode", type=int, default=3, help="evaluation speedup mode (saved demag fields, 1..6)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -485,6 +518,8 @@ def main() -> int:
         batched["rkdp54"] = batch_rkdp_run(a.batch_steps)
     adjoint = adjoint_run(a.adjoint_steps, 100) if a.adjoint_steps > 0 else None
     multilayer = multilayer_run(a.ml_steps, a.ml_adjoint_steps) if a.ml_steps > 0 else None
+    if multilayer is not None and a.trilayer_steps > 0:
+        multilayer["trilayer"] = trilayer_run(a.trilayer_steps)
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_steps)
     value = r["ncell"] * r["accepted"] / r["wall"]
     line = {
