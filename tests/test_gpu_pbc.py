@@ -152,3 +152,18 @@ def test_pbc_images_must_match_the_demag_handle(gpu):
     s = api.FMSolver(api.BM_EVAL_RK4, n, h, p, tilted_x(512, p.Ms, 1), pbc=(2, 0, 0))
     assert s.L.bm_ode_set_pbc(s.handle, 3, 0, 0) == 5          # BM_ERR_STATE
     assert s.L.bm_ode_set_pbc(s.handle, 2, 0, 0) == 0
+
+
+def test_pbc_averaged_inputs_duplicated_outputs(gpu):
+    """Two-sublattice meshes on a periodic handle: both outputs hold the periodic field of the
+    averaged input."""
+    n, h, pbc = (32, 16, 4), (4e-9, 4e-9, 4e-9), (3, 2, 0)
+    ncell = n[0] * n[1] * n[2]
+    M1 = api.random_magnetisation(ncell, 8e5, 41).astype(np.float32)
+    M2 = api.random_magnetisation(ncell, 8e5, 42).astype(np.float32)
+    d = api.Demag(n, h, pbc)
+    H2 = torch.empty((ncell, 3), dtype=torch.float32, device="cuda")
+    H1 = d.convolute_averaged(torch.from_numpy(M1).cuda(), torch.from_numpy(M2).cuda(), H2=H2)
+    ref = O.OracleDemag(n, h, pbc=pbc)(0.5 * (M1.astype(np.float64) + M2.astype(np.float64)))
+    assert rel_l2(H1.cpu().numpy(), ref) <= 1e-6
+    assert torch.equal(H1, H2)
