@@ -185,3 +185,22 @@ def test_batched_stacks_of_3d_meshes_equal_single_runs(gpu):
         assert e <= 2e-6, (b, e)
         s.close()
     bs.close()
+
+
+def test_stack_configuration_errors(gpu):
+    """Configuration errors of stacks of 3D meshes come back as error codes, nothing runs."""
+    import ctypes as C
+    nx, ny = 16, 16
+    p = FMParams(Ms=8e5, A=1.3e-11, alpha=0.02, modules=BM_MOD_DEMAG | BM_MOD_EXCHANGE | BM_MOD_ZEEMAN)
+    plain = api.FMSolver(BM_EVAL_RK4, (nx, ny, 2), (4e-9, 4e-9, 4e-9), p)
+    assert plain.L.bm_ode_set_layer_meshes(plain.handle, None, None) == 5           # BM_ERR_STATE: no stack
+    hz, z0 = [4e-9, 4e-9, 4e-9], [0.0, 5e-9, 9e-9]                               # planes 0-1 do not touch
+    st = api.LayerStackSolver(BM_EVAL_RK4, nx, ny, 4e-9, 4e-9, hz, z0, p)
+    grp = np.array([0, 0, 1], dtype=np.int32)
+    assert st.L.bm_ode_set_layer_meshes(st.handle, grp.ctypes.data_as(C.c_void_p), None) == 1   # BM_ERR_ARG
+    grp = np.array([1, 0, 0], dtype=np.int32)
+    assert st.L.bm_ode_set_layer_meshes(st.handle, grp.ctypes.data_as(C.c_void_p), None) == 1   # not consecutive
+    assert st.L.bm_ode_set_pbc(st.handle, 1, 0, 0) == 5                          # stacks are open-boundary
+    grp = np.array([0, 1, 1], dtype=np.int32)
+    assert st.L.bm_ode_set_layer_meshes(st.handle, grp.ctypes.data_as(C.c_void_p), None) == 0
+    st.close(); plain.close()
