@@ -268,16 +268,37 @@ def batch_rkdp_run(steps: int, warm: int = 5):
             "batched_ms_per_step": 1e3 * tb / steps, "plain_ms_per_step": 1e3 * tp / steps}
 
 
-def adjoint_run(steps: int, ckpt: int):
-    """128x128x4 film of 4 nm cells, Ms map = sum of 8 Gaussian blobs in [5e5, 1.1e6] A/m,
-    A 1.3e-11, alpha 0.05, 1 ns sinc pulse of 20 mT (seeded direction); RK4 dt 250 fs,
-    checkpoints every `ckpt` steps; loss 1 - <m_final . m_target>.  Forward and backward
-    wall times (the backward includes the segment recomputation)."""
-    import torch
+def smooth_unit_field(n, rng, kmax: int = 3) -> np.ndarray:
+    """A seeded smooth random unit vector field on an n = (nx, ny, nz) mesh, (ncell, 3):
+    each component a sum of sine/cosine modes of wavenumber 0..kmax per in-plane axis (whole
+    mesh lengths; uniform through the thickness) with normally distributed amplitudes,
+    normalised per cell."""
+    nx, ny, nz = n
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y)
+    comps = []
+    for _ in range(3):
+        f = np.zeros((ny, nx))
+        for kx in range(kmax + 1):
+            for ky in range(kmax + 1):
+                a, b, c, d = rng.normal(size=4) / (1.0 + kx + ky)
+                f += (a * np.cos(2 * np.pi * kx * X) + b * np.sin(2 * np.pi * kx * X)) * \
+                     (c * np.cos(2 * np.pi * ky * Y) + d * np.sin(2 * np.pi * ky * Y))
+        comps.append(f)
+    m = np.stack(comps, axis=-1)
+    m /= np.linalg.norm(m, axis=-1, keepdims=True)
+    return np.tile(m.reshape(-1, 3), (nz, 1))
+
+
+def adjoint_problem(seed: int = 4):
+    """The differentiation workload: a 128x128x4 film of 4 nm cells; Ms map = sum of 8 seeded
+    Gaussian blobs mapped to [5e5, 1.1e6] A/m; A 1.3e-11 J/m, alpha 0.05; a 1 ns sinc pulse of
+    20 mT amplitude in a seeded direction; m0 and m_target seeded smooth random unit fields.
+    Returns (n, h, params, Ms (ncell,), m0 (ncell, 3), m_target (ncell, 3))."""
     from boris2_amd import api
-    rng = np.random.default_rng(4)
+    rng = np.random.default_rng(seed)
     n = (128, 128, 4)
-    ncell = n[0] * n[1] * n[2]
     x, y = np.meshgrid(np.arange(n[0]), np.arange(n[1]))
     blob = np.zeros((n[1], n[0]))
     for _ in range(8):
@@ -288,9 +309,20 @@ def adjoint_run(steps: int, ckpt: int):
     d = rng.normal(size=3); d /= np.linalg.norm(d)
     p = api.FMParams(Ms=8e5, A=1.3e-11, alpha=0.05, modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ZEEMAN,
                      wave=1, amp=20e-3 / api.MU0, w=2 * np.pi / 1e-9, t0=0.5e-9, dir=tuple(d))
-    m0 = rng.normal(size=(ncell, 3)); m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
-    mt = rng.normal(size=(ncell, 3)); mt /= np.linalg.norm(mt, axis=1, keepdims=True)
-    r = api.RK4Adjoint(n, (4e-9,) * 3, p, torch.tensor(Ms), 250e-15, steps, ckpt)
+    m0 = smooth_unit_field(n, rng)
+    mt = smooth_unit_field(n, rng)
+    return n, (4e-9, 4e-9, 4e-9), p, Ms, m0, mt
+
+
+def adjoint_run(steps: int, ckpt: int):
+    """The differentiation workload (adjoint_problem): RK4 dt 250 fs, checkpoints every `ckpt`
+    steps; loss 1 - <m_final . m_target>.  Forward and backward wall times (the backward
+    includes the segment recomputation)."""
+    import torch
+    from boris2_amd import api
+    n, h, p, Ms, m0, mt = adjoint_problem()
+    ncell = n[0] * n[1] * n[2]
+    r = api.RK4Adjoint(n, h, p, torch.tensor(Ms), 250e-15, steps, ckpt)
     m0t, mtt = torch.tensor(m0, dtype=torch.float32), torch.tensor(mt, dtype=torch.float32)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

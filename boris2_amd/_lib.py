@@ -21,8 +21,10 @@ EXPORTS = [
     "bm_ode_create", "bm_ode_destroy", "bm_ode_set_ms_map", "bm_ode_set_params", "bm_ode_set_layers", "bm_ode_set_dt",
     "bm_ode_set_astep", "bm_ode_set_pbc", "bm_ode_set_layer_meshes", "bm_ode_set_eval_speedup", "bm_ode_get_speedup_state", "bm_ode_advance_time", "bm_ode_run_to_iteration", "bm_ode_iterate_stage",
     "bm_ode_restore", "bm_ode_get_state", "bm_ode_keep_heff", "bm_ode_copy_heff", "bm_ode_average_m",
+    "bm_ode_set_extra_field",
     "bm_prof_begin", "bm_prof_end", "bm_prof_query",
     "bm_mlayer_create", "bm_mlayer_destroy", "bm_mlayer_convolute", "bm_mlayer_copy_kernels",
+    "bm_mlayer_forward", "bm_mlayer_multiply", "bm_mlayer_inverse",
     "bm_batch_create", "bm_batch_create_layers", "bm_batch_destroy", "bm_batch_set_dt", "bm_batch_run", "bm_batch_get_state",
     "bm_batch_mesh_offset", "bm_batch_autotune_report", "bm_batch_set_astep", "bm_batch_get_mesh_state",
     "bm_adj_create", "bm_adj_destroy", "bm_adj_forward", "bm_adj_backward", "bm_adj_zeeman_grads", "bm_adj_set_layers",
@@ -31,6 +33,10 @@ EXPORTS = [
 
 PROF_KINDS = ["fwd_x_r2c", "y_fft", "z_fft_kernel_ifft", "y_ifft", "inv_x_c2r", "y2d_fft_kernel_ifft",
               "llg_stage", "rkdp_advance", "row_c2r_stage_r2c", "fwd_x_advance", "ml_kernel_product"]
+
+
+# bm_field_fn: void (*)(void* user, const float* M, float* Hext, void* stream)
+FIELD_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)
 
 
 class FmParams(C.Structure):
@@ -84,6 +90,9 @@ def load() -> C.CDLL:
         "bm_mlayer_destroy": (i, [vp]),
         "bm_mlayer_convolute": (i, [vp, vp, vp, i, vp]),
         "bm_mlayer_copy_kernels": (i, [vp, vp, ll]),
+        "bm_mlayer_forward": (i, [vp, i, vp, vp]),
+        "bm_mlayer_multiply": (i, [vp, vp]),
+        "bm_mlayer_inverse": (i, [vp, i, vp, vp, i, vp, vp]),
         "bm_batch_create": (i, [i, i, vp, vp, vp, vp, i, C.POINTER(vp)]),
         "bm_batch_create_layers": (i, [i, i, vp, vp, vp, vp, vp, i, C.POINTER(vp)]),
         "bm_batch_destroy": (i, [vp]),
@@ -121,6 +130,7 @@ def load() -> C.CDLL:
         "bm_ode_keep_heff": (i, [vp, i]),
         "bm_ode_copy_heff": (i, [vp, vp, vp]),
         "bm_ode_average_m": (i, [vp, C.POINTER(d), vp]),
+        "bm_ode_set_extra_field": (i, [vp, FIELD_FN, vp, vp]),
         "bm_ovf2_info": (i, [C.c_char_p, vp, vp, vp, C.POINTER(i), C.POINTER(i)]),
         "bm_ovf2_read": (i, [C.c_char_p, vp, ll]),
         "bm_ovf2_write": (i, [C.c_char_p, vp, i, vp, vp, vp, i, d]),

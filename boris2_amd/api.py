@@ -39,6 +39,19 @@ def _ptr(t: torch.Tensor) -> C.c_void_p:
     return C.c_void_p(t.data_ptr())
 
 
+class _DeviceArray:
+    """A float32 device array owned by the library, exposed to torch through the CUDA array
+    interface (torch.as_tensor makes a view, no copy)."""
+
+    def __init__(self, ptr: int, count: int):
+        self.__cuda_array_interface__ = {"shape": (count,), "typestr": "<f4", "data": (int(ptr), False),
+                                         "version": 3, "strides": None}
+
+
+def _device_view(ptr: int, count: int) -> torch.Tensor:
+    return torch.as_tensor(_DeviceArray(ptr, count), device="cuda")
+
+
 def require_device() -> None:
     """Raise unless the HIP library is loadable and a gfx950 device is present."""
     L = _lib.load()
@@ -152,6 +165,18 @@ class MultiLayerDemag:
         check(self.L.bm_mlayer_convolute(self.handle, _ptr(M), _ptr(Heff), int(clear_out), _stream()))
         return Heff
 
+    def convolute_split(self, M_layers, H_layers, clear_out: bool = True, energies: torch.Tensor | None = None) -> None:
+        """The split form over per-layer arrays (each (nx*ny, 3) float32 CUDA): forward of
+        every layer, one kernel product, inverse of every layer into H_layers (set or add);
+        energies (float32 CUDA, nlayers) receives each layer's demag energy density."""
+        st = _stream()
+        for l, M in enumerate(M_layers):
+            check(self.L.bm_mlayer_forward(self.handle, l, _ptr(M), st))
+        check(self.L.bm_mlayer_multiply(self.handle, st))
+        for l, (M, H) in enumerate(zip(M_layers, H_layers)):
+            ep = None if energies is None else C.c_void_p(energies.data_ptr() + 4 * l)
+            check(self.L.bm_mlayer_inverse(self.handle, l, _ptr(M), _ptr(H), int(clear_out), ep, st))
+
     def kernels(self) -> np.ndarray:
         """(nlayers, nlayers, 6, nx+1, ny+1) float32: destination, source, component, kx, ky."""
         L, nc, nyh = self.nlayers, self.nx + 1, self.ny + 1
@@ -244,6 +269,31 @@ class FMSolver:
 
     def set_dt(self, dT: float) -> None:
         check(self.L.bm_ode_set_dt(self.handle, float(dT)))
+
+    def set_extra_field(self, fn) -> None:
+        """Fields of other modules: before every stage fn(M) is called with the (ncell, 3)
+        CUDA tensor view of the magnetisation that stage evaluates and must return the other
+        modules' summed field as an (ncell, 3) float32 tensor (A/m) on the current stream;
+        the stage adds it to the effective field (the reference's per-stage module
+        UpdateField loop).  fn None detaches it."""
+        if fn is None:
+            check(self.L.bm_ode_set_extra_field(self.handle, _lib.FIELD_FN(), None, None))
+            self._ext = None
+            return
+        ncell = self.M.shape[0]
+        Hext = torch.zeros((ncell, 3), dtype=torch.float32, device="cuda")
+        views = {}
+
+        def tramp(user, m_ptr, h_ptr, stream):
+            # the stage input is one of two arrays: the caller's M or the solver's second
+            # buffer; wrap each once as a tensor view over its device memory
+            if m_ptr not in views:
+                views[m_ptr] = self.M if m_ptr == self.M.data_ptr() else _device_view(m_ptr, ncell * 3).view(ncell, 3)
+            Hext.copy_(fn(views[m_ptr]))
+
+        cb = _lib.FIELD_FN(tramp)
+        self._ext = (cb, Hext, views)           # keep the callback and buffers alive
+        check(self.L.bm_ode_set_extra_field(self.handle, cb, None, _ptr(Hext)))
 
     def set_astep(self, err_fail=-1.0, dT_increase=-1.0, dT_min=-1.0, dT_max=-1.0) -> None:
         check(self.L.bm_ode_set_astep(self.handle, err_fail, dT_increase, dT_min, dT_max))

@@ -298,8 +298,8 @@ int build_tables(bm_batch_s* o)
 	for (size_t g = 0; g < o->gy.size(); g++)
 		for (int b : o->gy[g].meshes) {
 			PassBatch q{};
-			if (o->ml[b]) { const MlView v = mlayer_view(o->ml[b]); q.S = v.S; q.Sx = v.Sx; q.Sp = v.Sp; q.NC = v.NC; q.ntl = v.ntl; q.nlines = 3 * v.L * v.NC; }
-			else { bm_demag_s* d = o->dm[b]; q.S = d->S; q.Sx = d->Sx; q.Sp = d->Sp; q.NC = d->NC; q.ntl = d->ntl; q.nlines = 3 * d->nz * d->NC; }
+			if (o->ml[b]) { const MlView v = mlayer_view(o->ml[b]); q.S = v.S; q.Sx = v.Sx; q.Sp = v.Sp; q.NC = v.NC; q.ntl = v.ntl; q.nlines = 3 * v.L * v.NC; q.ny = v.ny; }
+			else { bm_demag_s* d = o->dm[b]; q.S = d->S; q.Sx = d->Sx; q.Sp = d->Sp; q.NC = d->NC; q.ntl = d->ntl; q.nlines = 3 * d->nz * d->NC; q.ny = d->ny; }
 			o->ty[g].h.push_back(q);
 		}
 	for (size_t g = 0; g < o->gml.size(); g++)
@@ -316,7 +316,7 @@ int build_tables(bm_batch_s* o)
 	for (size_t g = 0; g < o->gy2.size(); g++)
 		for (int b : o->gy2[g].meshes) {
 			bm_demag_s* d = o->dm[b];
-			PassBatch q{}; q.Sx = d->Sx; q.K = d->K; q.NC = d->NC; q.NYH = d->NYH; q.ntl = d->ntl;
+			PassBatch q{}; q.Sx = d->Sx; q.K = d->K; q.NC = d->NC; q.NYH = d->NYH; q.ntl = d->ntl; q.ny = d->ny;
 			o->ty2[g].h.push_back(q);
 		}
 	for (int par = 0; par < 2; par++) {
@@ -534,7 +534,12 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 				if (src[q]) { o->lay_set[b] |= 1 << q; for (int l = 0; l < ld->nlayers; l++) o->lay[b][q * kMaxStackLayers + l] = (float)src[q][l]; }
 			if (ld->Ms) { o->lay_set[b] |= 8; any_lay_ms = true; }
 		}
-		else if (o->p[b].modules & BM_MOD_DEMAG) rc = bm_demag_create(o->nx[b], o->ny[b], o->nz[b], o->h[3 * b], o->h[3 * b + 1], o->h[3 * b + 2], 0, &o->dm[b]);
+		else if (o->p[b].modules & BM_MOD_DEMAG) {
+			rc = bm_demag_create(o->nx[b], o->ny[b], o->nz[b], o->h[3 * b], o->h[3 * b + 1], o->h[3 * b + 2], 0, &o->dm[b]);
+			// batched x passes are grouped by row length and run the exact-length variant
+			if (rc == BM_OK && o->dm[b]->Nx != 2 * o->nx[b])
+				rc = fail(BM_ERR_SIZE, "bm_batch_create: batched meshes need nx in the 2^a 3^b plans 2..2048 (other nx run on single-mesh handles)");
+		}
 	}
 	// per-layer tables of the stacks (A, alpha, K1 given per layer or the mesh's)
 	if (rc == BM_OK && layers) {

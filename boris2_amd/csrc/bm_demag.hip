@@ -49,13 +49,17 @@ constexpr int kBlock = 256;
 // RKDP54 advance on the way: sM1 = M, Mout = M + E0 dT/5, and transforms Mout.
 // FULL (periodic x, N.x = nx): a row holds 2 HX cells and every packed pair carries data;
 // otherwise a row holds HX cells and the upper half of the packed pairs is the zero padding.
+// GEN (N.x larger than 2 nx, any nx <= HX, odd included): a row holds a.nx cells, loaded one
+// float at a time (rows of odd length are not 8-byte aligned) with the cells past nx zero.
 
-template <int HX, int TY, bool ADV, bool FULL = false>
+template <int HX, int TY, bool ADV, bool FULL = false, bool GEN = false>
 __global__ void __launch_bounds__(kBlock) k_fwd_x(FwdArgs a, const FwdArgs* __restrict__ bt)
 {
+	static_assert(!(GEN && (ADV || FULL)), "the general row length has no fused advance and no periodic x");
 	if (bt) a = bt[blockIdx.y];     // batched: one mesh per grid row
 	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, RPITCH = xrow_pitch<HX>(), NG = TY / RP;
 	constexpr int EL = FULL ? E : E / 2, NXR = FULL ? 2 * HX : HX;   // pairs per thread, cells per row
+	const int nxr = GEN ? a.nx : NXR;
 	__shared__ C2<float> tile[TY * RPITCH];
 	XThread<HX> th; th.init();
 	const int nwork = a.nz * a.ntiles;
@@ -67,11 +71,20 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x(FwdArgs a, const FwdArgs* __re
 		for (int g = 0; g < NG; g++) {
 			const int y = y0 + g * RP + th.rl;
 			const bool ok = y < a.ny;
-			const size_t rowoff = ((size_t)z * a.ny + (ok ? y : 0)) * (size_t)NXR * 3;
+			const size_t rowoff = ((size_t)z * a.ny + (ok ? y : 0)) * (size_t)nxr * 3;
 			const float2* row = reinterpret_cast<const float2*>(a.M + rowoff);
 #pragma unroll
 			for (int e = 0; e < EL; e++) {
 				const int k = th.t + T * e;
+				if constexpr (GEN) {
+					const float* c0 = a.M + rowoff + 6 * (size_t)k;
+					const bool v0 = ok && 2 * k < nxr, v1 = ok && 2 * k + 1 < nxr;
+					float f[6];
+#pragma unroll
+					for (int q = 0; q < 3; q++) { f[q] = v0 ? c0[q] : 0.f; f[3 + q] = v1 ? c0[3 + q] : 0.f; }
+					m[g][e][0] = make_float2(f[0], f[1]); m[g][e][1] = make_float2(f[2], f[3]); m[g][e][2] = make_float2(f[4], f[5]);
+					continue;
+				}
 #pragma unroll
 				for (int q = 0; q < 3; q++) m[g][e][q] = ok ? row[3 * k + q] : make_float2(0.f, 0.f);
 				if (ADV && ok) {
@@ -107,7 +120,7 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x(FwdArgs a, const FwdArgs* __re
 				r2c_row<HX>(th, tile + (g * RP + th.rl) * RPITCH, v);
 			}
 			__syncthreads();
-			tile_to_S<HX, TY>(tile, a.Sx, c, z, y0, a.ny, a.nz, a.ntl);
+			tile_to_S<HX, TY>(tile, a.Sx, c, z + a.zoff, y0, a.ny, a.nzl ? a.nzl : a.nz, a.ntl);
 			__syncthreads();
 		}
 	}
@@ -116,24 +129,27 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x(FwdArgs a, const FwdArgs* __re
 // ---------------------------------------------------------------------------------------
 // P5: x-direction complex-to-real inverse FFT.  Output modes: 0 planar set, 1 AoS set,
 // 2 AoS add; EN also forms per-work-item partial sums of M.H for the energy.  FULL as in P1:
-// all 2 HX outputs of a row are cells (periodic x), otherwise the first HX.
-template <int HX, int MODE, bool EN, int TY, bool FULL = false>
+// all 2 HX outputs of a row are cells (periodic x), otherwise the first HX; GEN as in P1: the
+// first nxr (<= HX) outputs of a row are its cells, stored one float at a time.
+template <int HX, int MODE, bool EN, int TY, bool FULL = false, bool GEN = false>
 __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ Sx, float* __restrict__ out,
 	int ny, int nz, int ntl, int ntiles, float scale, const float* __restrict__ M, double* __restrict__ epart,
-	const PassBatch* __restrict__ bt)
+	int nxr, int zoff, int nzl, const PassBatch* __restrict__ bt)
 {
+	static_assert(!(GEN && FULL), "the general row length has no periodic x");
 	if (bt) { const PassBatch& b = bt[blockIdx.y]; Sx = b.Sx; out = b.out; ny = b.ny; nz = b.nz; ntl = b.ntl; ntiles = b.ntiles; scale = b.scale; }
 	__shared__ double ered[EN ? kBlock : 1];
 	constexpr int E = Plan<HX>::E, T = plan_T<HX>(), RP = kBlock / T, RPITCH = xrow_pitch<HX>();
 	__shared__ C2<float> tile[TY * RPITCH];
 	XThread<HX> th; th.init();
 	constexpr int EL = FULL ? E : E / 2;
-	const int nx = FULL ? 2 * HX : HX;
+	const int nx = GEN ? nxr : FULL ? 2 * HX : HX;
 	const size_t ncell = (size_t)nx * ny * nz;
 	// the next work item's tile is loaded into registers while this one is transformed
 	XTileRegs<HX, TY> pre;
 	const int nwork = 3 * nz * ntiles;
-	auto fetch = [&](int wf) { pre.load(Sx, wf % 3, (wf / 3) % nz, (wf / (3 * nz)) * TY, ny, nz, ntl); };
+	const int nzs = nzl ? nzl : nz;      // planes per component of the Sx layout
+	auto fetch = [&](int wf) { pre.load(Sx, wf % 3, zoff + (wf / 3) % nz, (wf / (3 * nz)) * TY, ny, nzs, ntl); };
 	if ((int)blockIdx.x < nwork) fetch(blockIdx.x);
 	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
 		const int c = w % 3, z = (w / 3) % nz, y0 = (w / (3 * nz)) * TY;
@@ -154,6 +170,15 @@ __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ Sx,
 				for (int e = 0; e < EL; e++) {
 					const int n2 = 2 * (th.t + T * e);
 					float a = v[e].x * scale, b = v[e].y * scale;
+					if constexpr (GEN) {
+						const bool v0 = n2 < nx, v1 = n2 + 1 < nx;
+						float* o0 = MODE == 0 ? out + (size_t)c * ncell + rowc + n2 : out + (rowc + n2) * 3 + c;
+						float* o1 = MODE == 0 ? o0 + 1 : o0 + 3;
+						if (v0) *o0 = MODE == 2 ? *o0 + a : a;
+						if (v1) *o1 = MODE == 2 ? *o1 + b : b;
+						if (EN) esum += (v0 ? (double)(M[(rowc + n2) * 3 + c] * a) : 0.0) + (v1 ? (double)(M[(rowc + n2 + 1) * 3 + c] * b) : 0.0);
+						continue;
+					}
 					if (MODE == 0) {
 						*reinterpret_cast<float2*>(out + (size_t)c * ncell + rowc + n2) = make_float2(a, b);
 					}
@@ -186,11 +211,13 @@ __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ Sx,
 // LP * kLT consecutive Sx values.  (Measured at 512x512x32: staging the Sx side through LDS
 // so that each wave moves whole runs, or interleaving the lanes over the lines, is slower.)
 // FULL (periodic y, N.y = ny): all NY values of a line are data rows.
+// nyd: data rows of a line (ny; N.y when periodic); N.y may exceed 2 ny, the rows between
+// ny and N.y / 2 are zeros that are never stored.
 template <int NY, int DIR, bool FULL = false>
 __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, float2* __restrict__ Sx, int nlines, int Sp, int NC, int ntl,
-	const PassBatch* __restrict__ bt)
+	int nyd, const PassBatch* __restrict__ bt)
 {
-	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; Sx = b.Sx; nlines = b.nlines; Sp = b.Sp; NC = b.NC; ntl = b.ntl; }
+	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; Sx = b.Sx; nlines = b.nlines; Sp = b.Sp; NC = b.NC; ntl = b.ntl; nyd = b.ny; }
 	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = kBlock / T, LL = lds_len<NY>(), EL = FULL ? E : E / 2;
 	__shared__ C2<float> lds[LP * LL];
 	const int t = threadIdx.x % T, l = threadIdx.x / T;
@@ -204,7 +231,7 @@ __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, float2* __
 		if constexpr (DIR < 0) {
 			const int cz = ok ? line / NC : 0, kx = ok ? line % NC : 0;
 #pragma unroll
-			for (int e = 0; e < E; e++) pre[e] = (ok && e < EL) ? Sx[sx_line(cz, kx, t + T * e, NC, ntl)] : make_float2(0.f, 0.f);
+			for (int e = 0; e < E; e++) pre[e] = (ok && e < EL && t + T * e < nyd) ? Sx[sx_line(cz, kx, t + T * e, NC, ntl)] : make_float2(0.f, 0.f);
 		}
 		else {
 			const float2* q = S + (size_t)(ok ? line : 0) * Sp;
@@ -229,7 +256,8 @@ __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, float2* __
 			else {
 				const int cz = line / NC, kx = line % NC;
 #pragma unroll
-				for (int e = 0; e < EL; e++) Sx[sx_line(cz, kx, t + T * e, NC, ntl)] = make_float2(v[e].x, v[e].y);
+				for (int e = 0; e < EL; e++)
+					if (t + T * e < nyd) Sx[sx_line(cz, kx, t + T * e, NC, ntl)] = make_float2(v[e].x, v[e].y);
 			}
 		}
 	}
@@ -339,11 +367,12 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 // ---------------------------------------------------------------------------------------
 // P2' (2D): per kx line of Sx (in place), y-FFT of 3 components, 2D kernel multiply (Kxy
 // odd in y), y-IFFT.  A block takes LP consecutive lines, one per group of T lanes.
+// nyd: data rows as in P2.
 template <int NY, int BS, bool FULL = false>
 __global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ Sx, const float* __restrict__ K, int NC, int NYH, int ntl,
-	const PassBatch* __restrict__ bt)
+	int nyd, const PassBatch* __restrict__ bt)
 {
-	if (bt) { const PassBatch& b = bt[blockIdx.y]; Sx = b.Sx; K = b.K; NC = b.NC; NYH = b.NYH; ntl = b.ntl; }
+	if (bt) { const PassBatch& b = bt[blockIdx.y]; Sx = b.Sx; K = b.K; NC = b.NC; NYH = b.NYH; ntl = b.ntl; nyd = b.ny; }
 	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = BS / T, LL = lds_len<NY>(), EL = FULL ? E : E / 2;
 	__shared__ C2<float> lds[3 * LP * LL];
 	const int t = threadIdx.x % T, l = threadIdx.x / T;
@@ -359,7 +388,7 @@ __global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ Sx, const float
 		for (int c = 0; c < 3; c++)
 #pragma unroll
 			for (int e = 0; e < E; e++) {
-				const float2 a = (ok && e < EL) ? Sx[sx_line(c, kxs, t + T * e, NC, ntl)] : make_float2(0.f, 0.f);
+				const float2 a = (ok && e < EL && t + T * e < nyd) ? Sx[sx_line(c, kxs, t + T * e, NC, ntl)] : make_float2(0.f, 0.f);
 				v[c][e] = mk(a.x, a.y);
 			}
 		C2<float>* b0 = lds + l * LL;
@@ -381,7 +410,8 @@ __global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ Sx, const float
 #pragma unroll
 			for (int c = 0; c < 3; c++)
 #pragma unroll
-				for (int e = 0; e < EL; e++) Sx[sx_line(c, kx, t + T * e, NC, ntl)] = make_float2(v[c][e].x, v[c][e].y);
+				for (int e = 0; e < EL; e++)
+					if (t + T * e < nyd) Sx[sx_line(c, kx, t + T * e, NC, ntl)] = make_float2(v[c][e].x, v[c][e].y);
 		}
 	}
 }
@@ -620,18 +650,21 @@ int xtile_factor(bool fwd)
 }
 
 // full: periodic x (rows of 2 HX cells, no zero padding); only the plain transform exists
-// for it (no fused RKDP54 advance, no taller tile)
+// for it (no fused RKDP54 advance, no taller tile).  nx: cells per row when below HX (the
+// general-length variant; 0 = HX).
 int launch_fwd_x(int HX, bool full, const float* M, float2* Sx, int ny, int nz, int ntl, int ncu, hipStream_t st,
-	const float* E0 = nullptr, float* sM1 = nullptr, float* Mout = nullptr, float dT5 = 0.f)
+	const float* E0 = nullptr, float* sM1 = nullptr, float* Mout = nullptr, float dT5 = 0.f, int nx = 0, int zoff = 0, int nzl = 0)
 {
-	const bool adv = E0 != nullptr;
+	const bool adv = E0 != nullptr, gen = nx && nx != HX;
 	if (full && adv) return fail(BM_ERR_STATE, "fused advance with a periodic x axis");
-	const int f = full ? 1 : xtile_factor(true);
-	FwdArgs a{ M, Sx, ny, nz, ntl, 0, E0, sM1, Mout, dT5 };
+	if (gen && (full || adv)) return fail(BM_ERR_STATE, "general row length: no periodic x, no fused advance");
+	const int f = (full || gen) ? 1 : xtile_factor(true);
+	FwdArgs a{ M, Sx, ny, nz, ntl, 0, E0, sM1, Mout, dT5, gen ? nx : HX, zoff, nzl };
 	switch (HX) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; int TY = TY1 * f; a.ntiles = (ny + TY - 1) / TY; dim3 g(grid_for(nz * a.ntiles, ncu, 2)); \
 		const FwdArgs* nb = nullptr; \
 		if (full) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false, true>), g, dim3(kBlock), 0, st, a, nb); \
+		else if (gen) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false, false, true>), g, dim3(kBlock), 0, st, a, nb); \
 		else if (f == 1 && !adv) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false>), g, dim3(kBlock), 0, st, a, nb); \
 		else if (f == 1) hipLaunchKernelGGL((k_fwd_x<V_, TY1, true>), g, dim3(kBlock), 0, st, a, nb); \
 		else if (!adv) hipLaunchKernelGGL((k_fwd_x<V_, 2 * TY1, false>), g, dim3(kBlock), 0, st, a, nb); \
@@ -642,32 +675,32 @@ int launch_fwd_x(int HX, bool full, const float* M, float2* Sx, int ny, int nz, 
 	return fail(BM_ERR_SIZE, "no x plan");
 }
 
-// full: periodic x, as in launch_fwd_x
-template <int HX, int TY1>
-void launch_inv_x_full(int mode, dim3 g, hipStream_t st, const float2* Sx, float* out, int ny, int nz, int ntl, int nt, float scale,
-	const float* M, double* epart)
+// full: periodic x, as in launch_fwd_x; gen: rows of nx < HX cells (any nx), as in launch_fwd_x
+template <int HX, int TY1, bool FULL, bool GEN>
+void launch_inv_x_var(int mode, dim3 g, hipStream_t st, const float2* Sx, float* out, int ny, int nz, int ntl, int nt, float scale,
+	const float* M, double* epart, int nx, int zoff = 0, int nzl = 0)
 {
 	const PassBatch* nb = nullptr;
-	if (mode == 0) hipLaunchKernelGGL((k_inv_x<HX, 0, false, TY1, true>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nb);
-	else if (mode == 1 && !epart) hipLaunchKernelGGL((k_inv_x<HX, 1, false, TY1, true>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nb);
-	else if (mode == 1) hipLaunchKernelGGL((k_inv_x<HX, 1, true, TY1, true>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nb);
-	else if (!epart) hipLaunchKernelGGL((k_inv_x<HX, 2, false, TY1, true>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nb);
-	else hipLaunchKernelGGL((k_inv_x<HX, 2, true, TY1, true>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nb);
+	if (mode == 0) hipLaunchKernelGGL((k_inv_x<HX, 0, false, TY1, FULL, GEN>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nx, zoff, nzl, nb);
+	else if (mode == 1 && !epart) hipLaunchKernelGGL((k_inv_x<HX, 1, false, TY1, FULL, GEN>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nx, zoff, nzl, nb);
+	else if (mode == 1) hipLaunchKernelGGL((k_inv_x<HX, 1, true, TY1, FULL, GEN>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nx, zoff, nzl, nb);
+	else if (!epart) hipLaunchKernelGGL((k_inv_x<HX, 2, false, TY1, FULL, GEN>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nx, zoff, nzl, nb);
+	else hipLaunchKernelGGL((k_inv_x<HX, 2, true, TY1, FULL, GEN>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nx, zoff, nzl, nb);
 }
 
 int launch_inv_x(int HX, bool full, int mode, const float2* Sx, float* out, int ny, int nz, int ntl, float scale, int ncu, hipStream_t st,
-	const float* M = nullptr, double* epart = nullptr)
+	const float* M = nullptr, double* epart = nullptr, int nx = 0, int zoff = 0, int nzl = 0)
 {
-	const int f = (mode == 0 && !full) ? xtile_factor(false) : 1;
+	const bool gen = nx && nx != HX;
+	if (gen && full) return fail(BM_ERR_STATE, "general row length: no periodic x");
+	const int f = (mode == 0 && !full && !gen) ? xtile_factor(false) : 1;
 	switch (HX) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; int TY = TY1 * f, nt = (ny + TY - 1) / TY; dim3 g(grid_for(3 * nz * nt, ncu, 2)); \
-		if (full) launch_inv_x_full<V_, TY1>(mode, g, st, Sx, out, ny, nz, ntl, nt, scale, M, epart); \
-		else if (mode == 0 && f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
-		else if (mode == 0) hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
-		else if (mode == 1 && !epart) hipLaunchKernelGGL((k_inv_x<V_, 1, false, TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
-		else if (mode == 1) hipLaunchKernelGGL((k_inv_x<V_, 1, true, TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
-		else if (!epart) hipLaunchKernelGGL((k_inv_x<V_, 2, false, TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); \
-		else hipLaunchKernelGGL((k_inv_x<V_, 2, true, TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, (const PassBatch*)nullptr); return BM_OK; }
+		const PassBatch* nb = nullptr; \
+		if (full) launch_inv_x_var<V_, TY1, true, false>(mode, g, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, HX); \
+		else if (gen) launch_inv_x_var<V_, TY1, false, true>(mode, g, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nx, zoff, nzl); \
+		else if (mode == 0 && f != 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, HX, 0, 0, nb); \
+		else launch_inv_x_var<V_, TY1, false, false>(mode, g, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, HX, zoff, nzl); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -688,19 +721,19 @@ int inv_x_nwork(int HX, int ny, int nz)
 int z_xcd() { static int v = getenv("BM_Z_XCD") ? atoi(getenv("BM_Z_XCD")) : 0; return v; }
 
 template <int NY, bool FULL = false>
-void launch_y_t(int dir, dim3 g, hipStream_t st, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, const PassBatch* bt)
+void launch_y_t(int dir, dim3 g, hipStream_t st, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int nyd, const PassBatch* bt)
 {
-	if (dir < 0) hipLaunchKernelGGL((k_y<NY, -1, FULL>), g, dim3(kBlock), 0, st, S, Sx, nlines, Sp, NC, ntl, bt);
-	else hipLaunchKernelGGL((k_y<NY, 1, FULL>), g, dim3(kBlock), 0, st, S, Sx, nlines, Sp, NC, ntl, bt);
+	if (dir < 0) hipLaunchKernelGGL((k_y<NY, -1, FULL>), g, dim3(kBlock), 0, st, S, Sx, nlines, Sp, NC, ntl, nyd, bt);
+	else hipLaunchKernelGGL((k_y<NY, 1, FULL>), g, dim3(kBlock), 0, st, S, Sx, nlines, Sp, NC, ntl, nyd, bt);
 }
 
-// full: periodic y (N.y = ny)
-int launch_y(int NY, bool full, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ncu, hipStream_t st)
+// full: periodic y (N.y = ny); ny: data rows of a line
+int launch_y(int NY, bool full, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ny, int ncu, hipStream_t st)
 {
 	switch (NY) {
 #define C_(V_) case V_: { int LP = kBlock / plan_T<V_>(); dim3 g(grid8((nlines + LP - 1) / LP, ncu, 8)); \
-		if (full) launch_y_t<V_, true>(dir, g, st, S, Sx, nlines, Sp, NC, ntl, nullptr); \
-		else launch_y_t<V_>(dir, g, st, S, Sx, nlines, Sp, NC, ntl, nullptr); return BM_OK; }
+		if (full) launch_y_t<V_, true>(dir, g, st, S, Sx, nlines, Sp, NC, ntl, ny, nullptr); \
+		else launch_y_t<V_>(dir, g, st, S, Sx, nlines, Sp, NC, ntl, ny, nullptr); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -720,13 +753,13 @@ int launch_z(int NZ, bool full, float2* S, const float* K, int nz, int NC, int N
 	return fail(BM_ERR_SIZE, "no z plan");
 }
 
-// full: periodic y (N.y = ny)
-int launch_y2d(int NY, bool full, float2* Sx, const float* K, int NC, int NYH, int ntl, int ncu, hipStream_t st)
+// full: periodic y (N.y = ny); ny: data rows of a line
+int launch_y2d(int NY, bool full, float2* Sx, const float* K, int NC, int NYH, int ntl, int ny, int ncu, hipStream_t st)
 {
 	switch (NY) {
 #define C_(V_) case V_: { constexpr int BS = Plan<V_>::E > 8 ? 128 : 256; int LP = BS / plan_T<V_>(); dim3 g(grid8((NC + LP - 1) / LP, ncu, 8)); \
-		if (full) hipLaunchKernelGGL((k_y2d<V_, BS, true>), g, dim3(BS), 0, st, Sx, K, NC, NYH, ntl, (const PassBatch*)nullptr); \
-		else hipLaunchKernelGGL((k_y2d<V_, BS>), g, dim3(BS), 0, st, Sx, K, NC, NYH, ntl, (const PassBatch*)nullptr); return BM_OK; }
+		if (full) hipLaunchKernelGGL((k_y2d<V_, BS, true>), g, dim3(BS), 0, st, Sx, K, NC, NYH, ntl, ny, (const PassBatch*)nullptr); \
+		else hipLaunchKernelGGL((k_y2d<V_, BS>), g, dim3(BS), 0, st, Sx, K, NC, NYH, ntl, ny, (const PassBatch*)nullptr); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -839,7 +872,7 @@ int compute_kernels(bm_demag_s* d, hipStream_t st)
 int bm::demag_fwd_x(bm_demag_s* d, const float* M, hipStream_t st)
 {
 	prof_mark_begin(PK_FWD_X, st);
-	int rc = launch_fwd_x(d->Nx / 2, d->full[0], M, d->Sx, d->ny, d->nz, d->ntl, d->ncu, st);
+	int rc = launch_fwd_x(d->Nx / 2, d->full[0], M, d->Sx, d->ny, d->nz, d->ntl, d->ncu, st, nullptr, nullptr, nullptr, 0.f, d->nx);
 	prof_mark_end(PK_FWD_X, st);
 	return rc;
 }
@@ -861,12 +894,12 @@ int bm::demag_mid(bm_demag_s* d, hipStream_t st)
 	int rc;
 	if (d->is2d) {
 		prof_mark_begin(PK_Y2D, st);
-		rc = launch_y2d(d->Ny, d->full[1], d->Sx, d->K, d->NC, d->NYH, d->ntl, d->ncu, st);
+		rc = launch_y2d(d->Ny, d->full[1], d->Sx, d->K, d->NC, d->NYH, d->ntl, d->full[1] ? d->Ny : d->ny, d->ncu, st);
 		prof_mark_end(PK_Y2D, st);
 		return rc;
 	}
 	prof_mark_begin(PK_Y_FWD, st);
-	rc = launch_y(d->Ny, d->full[1], -1, d->S, d->Sx, nlines, d->Sp, d->NC, d->ntl, d->ncu, st);
+	rc = launch_y(d->Ny, d->full[1], -1, d->S, d->Sx, nlines, d->Sp, d->NC, d->ntl, d->full[1] ? d->Ny : d->ny, d->ncu, st);
 	prof_mark_end(PK_Y_FWD, st);
 	if (rc) return rc;
 	prof_mark_begin(PK_Z, st);
@@ -874,7 +907,7 @@ int bm::demag_mid(bm_demag_s* d, hipStream_t st)
 	prof_mark_end(PK_Z, st);
 	if (rc) return rc;
 	prof_mark_begin(PK_Y_INV, st);
-	rc = launch_y(d->Ny, d->full[1], 1, d->S, d->Sx, nlines, d->Sp, d->NC, d->ntl, d->ncu, st);
+	rc = launch_y(d->Ny, d->full[1], 1, d->S, d->Sx, nlines, d->Sp, d->NC, d->ntl, d->full[1] ? d->Ny : d->ny, d->ncu, st);
 	prof_mark_end(PK_Y_INV, st);
 	return rc;
 }
@@ -892,36 +925,50 @@ int bm::demag_planar(bm_demag_s* d, const float* M, float* H3n, hipStream_t st)
 	if (rc) return rc;
 	const float scale = (float)(1.0 / ((double)d->Nx * d->Ny * d->Nz));
 	prof_mark_begin(PK_INV_X, st);
-	rc = launch_inv_x(d->Nx / 2, d->full[0], 0, d->Sx, H3n, d->ny, d->nz, d->ntl, scale, d->ncu, st);
+	rc = launch_inv_x(d->Nx / 2, d->full[0], 0, d->Sx, H3n, d->ny, d->nz, d->ntl, scale, d->ncu, st, nullptr, nullptr, d->nx);
 	prof_mark_end(PK_INV_X, st);
 	return rc;
 }
 
-// Sizes (Boris/ConvolutionDataCUDA.cpp:24-100): N = n along a periodic axis, otherwise
-// N.x = 2 nx, N.y = 2 ny and N.z the next power of two >= 2 nz (N.z = 1 in 2D).
+// The smallest x-pass length (2^a 3^b, 2..2048) at or above v, 0 if none.
+int x_length_at_least(int v)
+{
+	int best = 0;
+#define C_(V_) if (V_ >= v && (!best || V_ < best)) best = V_;
+	BM_FOR_X_SIZES(C_)
+#undef C_
+	return best;
+}
+
+// Sizes (Boris/ConvolutionDataCUDA.cpp:24-100): N = n along a periodic axis, otherwise the
+// reference takes N.x = 2 nx, N.y = 2 ny (any n, cuFFT plans every length) and N.z the next
+// power of two >= 2 nz (N.z = 1 in 2D).  Here an open x or y axis takes the smallest length
+// at or above 2 n that has a plan (N.x / 2 and N.y in the 2^a 3^b plans): any N >= 2 n - 1
+// gives the same zero-padded (non-cyclic) convolution, so the field equals the reference's
+// at N = 2 n up to FP32 rounding.  Meshes whose x rows are not exactly N.x / 2 cells long
+// take the general-length x passes (and the unfused integrator evaluation).
 static int demag_create(int nx, int ny, int nz, double hx, double hy, double hz, const int* pbc, bm_demag_t* out)
 {
-	if (!out || nx < 2 || ny < 1 || nz < 1 || !(hx > 0) || !(hy > 0) || !(hz > 0)) return fail(BM_ERR_ARG, "bm_demag_create: bad arguments");
+	if (!out || nx < 1 || ny < 1 || nz < 1 || !(hx > 0) || !(hy > 0) || !(hz > 0)) return fail(BM_ERR_ARG, "bm_demag_create: bad arguments");
 	*out = nullptr;
 	if (pbc && (pbc[0] < 0 || pbc[1] < 0 || pbc[2] < 0)) return fail(BM_ERR_ARG, "bm_demag_create_pbc: negative image count");
-	if (nx % 2) return fail(BM_ERR_SIZE, "bm_demag_create: nx must be even");
+	const bool fx = pbc && pbc[0] > 0, fy = pbc && pbc[1] > 0, fz = nz > 1 && pbc && pbc[2] > 0;
+	if (fx && nx % 2) return fail(BM_ERR_SIZE, "bm_demag_create_pbc: a periodic nx must be even");
+	const int HX = fx ? nx / 2 : x_length_at_least(nx);
+	const int Ny = fy ? ny : x_length_at_least(2 * ny);
+	int Nz = 1;
+	if (fz) Nz = nz;
+	else if (nz > 1) { while (Nz < 2 * nz) Nz *= 2; }
+	if (!HX || !Ny || !x_supported(HX) || !x_supported(Ny) || !line_supported(2 * HX) || (nz > 1 && !z_supported(Nz)))
+		return fail(BM_ERR_SIZE, "bm_demag_create: no FFT plan for this mesh size (open axes: nx <= 2048, ny <= 1024, nz <= 64; "
+			"periodic axes: N.x / 2 and N.y in the 2^a 3^b plans 2..2048, N.z a power of two 4..128, N = n)");
 	bm_demag_s* d = new bm_demag_s();
 	d->nx = nx; d->ny = ny; d->nz = nz; d->h[0] = hx; d->h[1] = hy; d->h[2] = hz;
 	for (int q = 0; q < 3; q++) d->pbc[q] = pbc ? pbc[q] : 0;
 	d->is2d = (nz == 1);
-	d->full[0] = d->pbc[0] > 0; d->full[1] = d->pbc[1] > 0; d->full[2] = !d->is2d && d->pbc[2] > 0;
-	d->Nx = d->full[0] ? nx : 2 * nx;
-	d->Ny = d->full[1] ? ny : 2 * ny;
-	d->Nz = 1;
-	if (d->full[2]) d->Nz = nz;
-	else if (nz > 1) { while (d->Nz < 2 * nz) d->Nz *= 2; }
+	d->full[0] = fx; d->full[1] = fy; d->full[2] = fz;
+	d->Nx = 2 * HX; d->Ny = Ny; d->Nz = Nz;
 	d->NC = d->Nx / 2 + 1; d->NYH = d->Ny / 2 + 1; d->NZH = d->Nz / 2 + 1;
-	// N.x / 2 is the half-length complex FFT of the x passes; Nx, Ny, Nz also need FP64 setup plans
-	if (!x_supported(d->Nx / 2) || !x_supported(d->Ny) || !line_supported(d->Nx) || (!d->is2d && !z_supported(d->Nz))) {
-		delete d;
-		return fail(BM_ERR_SIZE, "bm_demag_create: no FFT plan for this mesh size (supported: N.x / 2 and N.y in the "
-			"2^a 3^b plans 2..2048, N.z a power of two 4..128; N.x = 2 nx, N.y = 2 ny, N.z >= 2 nz, or N = n along a periodic axis)");
-	}
 	d->ncu = device_cu_count();
 	// line pitch of the full spectra (complex elements); BM_S_PAD adds padding for tuning experiments
 	d->Sp = d->Ny;
@@ -991,7 +1038,7 @@ extern "C" int bm_demag_convolute(bm_demag_t d, const float* M, float* Heff, int
 		const int np = inv_x_nwork(HX, d->ny, d->nz);
 		if (np > d->ecap) return fail(BM_ERR_STATE, "energy partial buffer too small");
 		prof_mark_begin(PK_INV_X, st);
-		rc = launch_inv_x(HX, d->full[0], clear_out ? 1 : 2, d->Sx, Heff, d->ny, d->nz, d->ntl, scale, d->ncu, st, M, d->ebuf);
+		rc = launch_inv_x(HX, d->full[0], clear_out ? 1 : 2, d->Sx, Heff, d->ny, d->nz, d->ntl, scale, d->ncu, st, M, d->ebuf, d->nx);
 		prof_mark_end(PK_INV_X, st);
 		if (rc) return rc;
 		size_t n = (size_t)d->nx * d->ny * d->nz;
@@ -999,7 +1046,7 @@ extern "C" int bm_demag_convolute(bm_demag_t d, const float* M, float* Heff, int
 	}
 	else {
 		prof_mark_begin(PK_INV_X, st);
-		rc = launch_inv_x(HX, d->full[0], clear_out ? 1 : 2, d->Sx, Heff, d->ny, d->nz, d->ntl, scale, d->ncu, st);
+		rc = launch_inv_x(HX, d->full[0], clear_out ? 1 : 2, d->Sx, Heff, d->ny, d->nz, d->ntl, scale, d->ncu, st, nullptr, nullptr, d->nx);
 		prof_mark_end(PK_INV_X, st);
 		if (rc) return rc;
 	}
@@ -1058,7 +1105,7 @@ int bm::pass_y(int Ny, int dir, float2* S, float2* Sx, int nlines, int Sp, int N
 {
 	const int kind = dir < 0 ? PK_Y_FWD : PK_Y_INV;
 	prof_mark_begin(kind, st);
-	int rc = launch_y(Ny, false, dir, S, Sx, nlines, Sp, NC, ntl, ncu, st);
+	int rc = launch_y(Ny, false, dir, S, Sx, nlines, Sp, NC, ntl, Ny / 2, ncu, st);
 	prof_mark_end(kind, st);
 	return rc;
 }
@@ -1081,6 +1128,28 @@ int bm::setup_line_fft(int N, const double* src, float* dst, int nlines, int na,
 	long long da, long long db, long long de, int keep, int part, double sign, hipStream_t st)
 {
 	return launch_line_fft<float>(N, src, dst, nlines, na, sa, sb, se, da, db, de, keep, part, sign, st);
+}
+
+int bm::pass_fwd_x_plane(int nx, const float* M, float2* Sx, int ny, int plane, int nzl, int ntl, int ncu, hipStream_t st)
+{
+	prof_mark_begin(PK_FWD_X, st);
+	int rc = launch_fwd_x(nx, false, M, Sx, ny, 1, ntl, ncu, st, nullptr, nullptr, nullptr, 0.f, 0, plane, nzl);
+	prof_mark_end(PK_FWD_X, st);
+	return rc;
+}
+
+int bm::x_plane_parts(int nx, int ny) { return std::max(1, inv_x_nwork(nx, ny, 1)); }
+
+int bm::pass_inv_x_plane(int nx, int mode, const float2* Sx, float* out, int ny, int plane, int nzl, int ntl, float scale, int ncu,
+	hipStream_t st, const float* M, double* epart, float* energy)
+{
+	if (mode != 1 && mode != 2) return fail(BM_ERR_ARG, "plane x pass: AoS set or add");
+	prof_mark_begin(PK_INV_X, st);
+	int rc = launch_inv_x(nx, false, mode, Sx, out, ny, 1, ntl, scale, ncu, st, energy ? M : nullptr, energy ? epart : nullptr, 0, plane, nzl);
+	prof_mark_end(PK_INV_X, st);
+	if (rc || !energy) return rc;
+	hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, st, epart, inv_x_nwork(nx, ny, 1), (size_t)nx * ny, energy);
+	return BM_OK;
 }
 
 bool bm::plan_2d_ok(int nx, int Ny) { return x_supported(nx) && x_supported(Ny) && line_supported(2 * nx); }
@@ -1123,8 +1192,8 @@ int bm::batch_inv_x_planar(int nx, LaunchShape sh, const PassBatch* h_tab, const
 	const dim3 g(grid_for(mx, ncu, sh.per_cu ? sh.per_cu : 2), nb);
 	switch (nx) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; \
-		if (f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, d_tab); \
-		else hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, d_tab); return BM_OK; }
+		if (f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, V_, 0, 0, d_tab); \
+		else hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, V_, 0, 0, d_tab); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -1137,7 +1206,7 @@ int bm::batch_y(int Ny, int dir, LaunchShape sh, const PassBatch* h_tab, const P
 	switch (Ny) {
 #define C_(V_) case V_: { const int LP = kBlock / plan_T<V_>(); int mx = 1; \
 		for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].nlines + LP - 1) / LP); \
-		launch_y_t<V_>(dir, dim3(grid8(mx, ncu, pc), nb), st, nullptr, nullptr, 0, 0, 0, 0, d_tab); return BM_OK; }
+		launch_y_t<V_>(dir, dim3(grid8(mx, ncu, pc), nb), st, nullptr, nullptr, 0, 0, 0, 0, 0, d_tab); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -1163,7 +1232,7 @@ int bm::batch_y2d(int Ny, LaunchShape sh, const PassBatch* h_tab, const PassBatc
 	switch (Ny) {
 #define C_(V_) case V_: { constexpr int BS = Plan<V_>::E > 8 ? 128 : 256; const int LP = BS / plan_T<V_>(); int mx = 1; \
 		for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].NC + LP - 1) / LP); \
-		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid8(mx, ncu, pc), nb), dim3(BS), 0, st, nullptr, nullptr, 0, 0, 0, d_tab); return BM_OK; }
+		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid8(mx, ncu, pc), nb), dim3(BS), 0, st, nullptr, nullptr, 0, 0, 0, 0, d_tab); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}

@@ -61,6 +61,14 @@ int demag_mid(bm_demag_s* d, hipStream_t st);
 int pass_fwd_x(int nx, const float* M, float2* Sx, int ny, int nplanes, int ntl, int ncu, hipStream_t st);
 int pass_y(int Ny, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ncu, hipStream_t st);
 int pass_inv_x(int nx, int mode, const float2* Sx, float* out, int ny, int nplanes, int ntl, float scale, int ncu, hipStream_t st);
+// the x passes of plane `plane` of a spectra array of nzl planes per component (one layer
+// of a multilayered convolution): M / out hold that plane only (nx * ny cells, AoS); the
+// inverse sets (mode 1) or adds (mode 2) and, with energy set, writes -mu0/(2 nx ny) sum
+// M.H of the plane into *energy using epart (x_plane_parts(nx, ny) doubles)
+int pass_fwd_x_plane(int nx, const float* M, float2* Sx, int ny, int plane, int nzl, int ntl, int ncu, hipStream_t st);
+int pass_inv_x_plane(int nx, int mode, const float2* Sx, float* out, int ny, int plane, int nzl, int ntl, float scale, int ncu,
+	hipStream_t st, const float* M, double* epart, float* energy);
+int x_plane_parts(int nx, int ny);
 // FP64 real-input line FFTs of the kernel setup (see k_line_fft)
 int setup_line_fft(int N, const double* src, double* dst, int nlines, int na, long long sa, long long sb, long long se,
 	long long da, long long db, long long de, int keep, int part, double sign, hipStream_t st);
@@ -75,8 +83,13 @@ struct FwdArgs {
 	const float* M; float2* Sx;
 	int ny, nz, ntl, ntiles;
 	const float* E0; float* sM1; float* Mout; float dT5;
+	int nx;                 // cells per row (the general-length variant reads it)
+	// one layer of a multilayered spectra array: planes z of M go to plane zoff + z of Sx,
+	// whose layout has nzl planes per component (0, 0: the planes of M, nz per component)
+	int zoff, nzl;
 };
-// Per-mesh arguments of the other passes in a batched launch (the fields each pass uses).
+// Per-mesh arguments of the other passes in a batched launch (the fields each pass uses;
+// the y passes take ny as the data rows of a line).
 struct PassBatch {
 	float2* S; float2* Sx; const float* K; float* out;
 	int ny, nz, Sp, ntl, ntiles, nlines, NC, Ny, NYH;

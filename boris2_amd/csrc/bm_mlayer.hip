@@ -248,6 +248,7 @@ struct bm_mlayer_s {
 	float* K = nullptr;      // kernels: L*L pairs x 6 planes x NC x (Ny/2+1), ky fastest
 	long long kcount = 0, bytes = 0;
 	int ncu = 256;
+	double* ebuf = nullptr;  // energy partials of the per-layer inverse (bm_mlayer_inverse)
 };
 
 namespace {
@@ -381,7 +382,8 @@ extern "C" int bm_mlayer_create(int nlayers, int nx, int ny, double hx, double h
 	const size_t sxbytes = (size_t)3 * nlayers * m->ntl * m->NC * bm::kLT * sizeof(float2);
 	m->kcount = (long long)nlayers * nlayers * 6 * m->NC * m->NYH;
 	if (hipMalloc(&m->Sx, sxbytes) != hipSuccess || hipMalloc(&m->S, sbytes) != hipSuccess
-		|| hipMalloc(&m->K, (size_t)m->kcount * sizeof(float)) != hipSuccess) {
+		|| hipMalloc(&m->K, (size_t)m->kcount * sizeof(float)) != hipSuccess
+		|| hipMalloc(&m->ebuf, (size_t)bm::x_plane_parts(nx, ny) * sizeof(double)) != hipSuccess) {
 		bm_mlayer_destroy(m);
 		return fail(BM_ERR_NOMEM, "bm_mlayer_create: device memory");
 	}
@@ -398,6 +400,7 @@ extern "C" int bm_mlayer_destroy(bm_mlayer_t m)
 	if (m->Sx) (void)hipFree(m->Sx);
 	if (m->S) (void)hipFree(m->S);
 	if (m->K) (void)hipFree(m->K);
+	if (m->ebuf) (void)hipFree(m->ebuf);
 	delete m;
 	return BM_OK;
 }
@@ -467,5 +470,54 @@ extern "C" int bm_mlayer_copy_kernels(bm_mlayer_t m, float* host_out, long long 
 {
 	if (!m || !host_out || count < m->kcount) return fail(BM_ERR_ARG, "bm_mlayer_copy_kernels: bad arguments");
 	BM_HIP(hipMemcpy(host_out, m->K, (size_t)m->kcount * sizeof(float), hipMemcpyDeviceToHost));
+	return BM_OK;
+}
+
+// Split form, one call per layer mesh: the reference's ConvolutionCUDA::ForwardFFT,
+// KernelMultiplication_MultipleInputs and InverseFFT (Boris/ConvolutionCUDA.h:254-351) as
+// SDemagCUDA::UpdateField calls them for each layer's own mesh (Boris/SDemagCUDA.cpp:270+).
+// A layer's forward transform fills its planes of the shared spectra (x pass into its Sx
+// plane, then the y pass of its three component planes); the product runs once over all
+// layers; the inverse of a layer reads back its planes.
+extern "C" int bm_mlayer_forward(bm_mlayer_t m, int layer, const float* M_layer, void* stream)
+{
+	if (!m || !M_layer || layer < 0 || layer >= m->L) return fail(BM_ERR_ARG, "bm_mlayer_forward: bad arguments");
+	hipStream_t st = (hipStream_t)stream;
+	int rc = bm::pass_fwd_x_plane(m->nx, M_layer, m->Sx, m->ny, layer, m->L, m->ntl, m->ncu, st);
+	for (int c = 0; c < 3 && rc == BM_OK; c++) {
+		const size_t cz = (size_t)c * m->L + layer;
+		rc = bm::pass_y(m->Ny, -1, m->S + cz * m->NC * m->Sp, m->Sx + cz * m->ntl * m->NC * bm::kLT, m->NC, m->Sp, m->NC, m->ntl, m->ncu, st);
+	}
+	if (rc) return rc;
+	BM_HIP(hipGetLastError());
+	return BM_OK;
+}
+
+extern "C" int bm_mlayer_multiply(bm_mlayer_t m, void* stream)
+{
+	if (!m) return fail(BM_ERR_ARG, "bm_mlayer_multiply: null handle");
+	hipStream_t st = (hipStream_t)stream;
+	bm::prof_mark_begin(bm::PK_ML_MULT, st);
+	int rc = launch_ml_mult(m->L, m->S, m->K, m->NC, m->Ny, m->Sp, m->ncu, st);
+	bm::prof_mark_end(bm::PK_ML_MULT, st);
+	if (rc) return rc;
+	BM_HIP(hipGetLastError());
+	return BM_OK;
+}
+
+extern "C" int bm_mlayer_inverse(bm_mlayer_t m, int layer, const float* M_layer, float* H_layer, int clear_out, float* energy, void* stream)
+{
+	if (!m || !H_layer || layer < 0 || layer >= m->L || (energy && !M_layer)) return fail(BM_ERR_ARG, "bm_mlayer_inverse: bad arguments");
+	hipStream_t st = (hipStream_t)stream;
+	int rc = BM_OK;
+	for (int c = 0; c < 3 && rc == BM_OK; c++) {
+		const size_t cz = (size_t)c * m->L + layer;
+		rc = bm::pass_y(m->Ny, 1, m->S + cz * m->NC * m->Sp, m->Sx + cz * m->ntl * m->NC * bm::kLT, m->NC, m->Sp, m->NC, m->ntl, m->ncu, st);
+	}
+	const float scale = (float)(1.0 / ((double)m->Nx * m->Ny));
+	if (rc == BM_OK)
+		rc = bm::pass_inv_x_plane(m->nx, clear_out ? 1 : 2, m->Sx, H_layer, m->ny, layer, m->L, m->ntl, scale, m->ncu, st, M_layer, m->ebuf, energy);
+	if (rc) return rc;
+	BM_HIP(hipGetLastError());
 	return BM_OK;
 }

@@ -170,6 +170,11 @@ struct bm_ode_s {
 	int sp_n = 0;
 	float sp_ca[6] = {};
 	float* sp_save = nullptr;
+	// fields of other modules (bm_ode_set_extra_field): called before every stage with the
+	// magnetisation that stage evaluates; its output is added to the effective field
+	bm_field_fn ext_fn = nullptr;
+	void* ext_user = nullptr;
+	float* Hext = nullptr;
 };
 
 namespace {
@@ -210,6 +215,7 @@ StageArgs make_args(bm_ode_s* o, float dT)
 	a.Min = o->Mcur; a.Mout = (o->Mcur == o->M) ? o->Mb : o->M; a.Hd = o->Hd; a.sM1 = o->sM1;
 	for (int q = 0; q < 6; q++) a.E[q] = o->E[q];
 	a.Ms_map = o->Ms_map; a.Heff_out = o->keep_heff ? o->Heff_keep : nullptr; a.lte = o->d_lte;
+	a.Hext = o->ext_fn ? o->Hext : nullptr;
 	a.nx = o->nx; a.ny = o->ny; a.nz = o->nz;
 	a.per = (o->pbc[0] ? 1 : 0) | (o->pbc[1] ? 2 : 0) | (o->pbc[2] ? 4 : 0);
 	a.ihx2 = (float)(1.0 / (o->h[0] * o->h[0])); a.ihy2 = (float)(1.0 / (o->h[1] * o->h[1])); a.ihz2 = (float)(1.0 / (o->h[2] * o->h[2]));
@@ -442,13 +448,15 @@ int set_self_coeffs(bm_ode_s* o)
 bool row_path(const bm_ode_s* o)
 {
 	return (o->p.modules & BM_MOD_DEMAG) && o->use_row && !o->ml && !o->speedup && !o->pbc[0] && !o->pbc[1] && !o->pbc[2]
-		&& o->demag && !o->demag->full[0];
+		&& o->demag && !o->demag->full[0] && o->demag->Nx == 2 * o->nx;
 }
 
 // One field evaluation + stage `step` (the effective field is that of Mcur at stagetime).
 int evaluate(bm_ode_s* o, int step, float dT, hipStream_t st)
 {
 	o->evaluations++;
+	// the other modules' UpdateField of this stage (Boris/MeshBaseModules.cpp:98-111)
+	if (o->ext_fn) o->ext_fn(o->ext_user, o->Mcur, o->Hext, (void*)st);
 	const bool demag = (o->p.modules & BM_MOD_DEMAG) != 0;
 	if (demag && o->speedup) return eval_speedup(o, step, dT, st);
 	if (demag && o->ml) { int rc = bm::mlayer_planar(o->ml, o->Mcur, o->Hd, st); if (rc) return rc; o->convolutions++; return launch_stage(o, step, dT, st); }
@@ -616,8 +624,9 @@ extern "C" int bm_ode_create(int method, int nx, int ny, int nz, double hx, doub
 	// BM_ROW_KERNEL=0 selects the unfused evaluation (planar H_demag + stage kernel), kept
 	// as a cross-check of the fused row kernel
 	if (const char* e = getenv("BM_ROW_KERNEL")) o->use_row = atoi(e) != 0;
-	// a demag handle periodic along x has no zero-padded half for the row kernel's x passes
-	if (demag && demag->full[0]) o->use_row = false;
+	// the row kernel's x passes need rows of exactly N.x / 2 cells with a zero-padded half: not
+	// for a demag handle periodic along x, nor for rows shorter than N.x / 2 (general sizes)
+	if (demag && (demag->full[0] || demag->Nx != 2 * nx)) o->use_row = false;
 	int nE = 0;
 	// method defaults: Boris/DiffEq_CommonBase_Control.cpp:10-140 with Boris/DiffEq_Defs.h
 	switch (method) {
@@ -747,6 +756,13 @@ extern "C" int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p)
 	if (!o || !p) return fail(BM_ERR_ARG, "null argument");
 	if ((p->modules & BM_MOD_DEMAG) && !o->ml && (!o->demag || o->demag->nx != o->nx || o->demag->ny != o->ny || o->demag->nz != o->nz))
 		return fail(BM_ERR_STATE, "bm_ode_set_params: demag needs a demag handle of this mesh size at creation, or a layer stack");
+	// the same checks bm_ode_set_layer_meshes and bm_ode_set_pbc make, for settings that
+	// reach this call in the other order
+	if (o->lay_field && p->wave != 0)
+		return fail(BM_ERR_STATE, "bm_ode_set_params: per-layer constant fields are set (bm_ode_set_layer_meshes); no applied-field waveform");
+	if ((p->modules & BM_MOD_DEMAG) && !o->ml && (o->pbc[0] || o->pbc[1] || o->pbc[2])
+		&& (o->demag->pbc[0] != o->pbc[0] || o->demag->pbc[1] != o->pbc[1] || o->demag->pbc[2] != o->pbc[2]))
+		return fail(BM_ERR_STATE, "bm_ode_set_params: the mesh is periodic and the demag handle has other periodic images");
 	if ((p->modules & BM_MOD_DEMAG) && (!o->use_row || o->speedup) && !o->Hd) {
 		const size_t b3 = o->n * 3 * sizeof(float);
 		if (hipMalloc(&o->Hd, b3) != hipSuccess) return fail(BM_ERR_NOMEM, "bm_ode_set_params: demag field buffer");
@@ -808,6 +824,19 @@ extern "C" int bm_ode_get_speedup_state(bm_ode_t o, int* num_saved, double* tdm6
 	if (num_saved) *num_saved = o->num_saved;
 	if (tdm6) for (int q = 0; q < 6; q++) tdm6[q] = o->tdm[q];
 	if (convolutions) *convolutions = o->convolutions;
+	return BM_OK;
+}
+
+// Fields of modules outside the fused four (Boris/MeshBaseModules.cpp:98-111: every module's
+// UpdateField runs before each stage; here the caller's function computes their sum from the
+// stage's magnetisation, and the stage kernel adds it).
+extern "C" int bm_ode_set_extra_field(bm_ode_t o, bm_field_fn fn, void* user, float* Hext)
+{
+	if (!o) return fail(BM_ERR_ARG, "null handle");
+	if (fn && !Hext) return fail(BM_ERR_ARG, "bm_ode_set_extra_field: a field function needs its output array");
+	o->ext_fn = fn;
+	o->ext_user = fn ? user : nullptr;
+	o->Hext = fn ? Hext : nullptr;
 	return BM_OK;
 }
 

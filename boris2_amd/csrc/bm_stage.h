@@ -98,6 +98,7 @@ struct StageArgs {
 	float* E[6];
 	const float* Ms_map;
 	float* Heff_out;        // optional AoS copy of the effective field
+	const float* Hext;      // optional AoS field of other modules, added to the effective field
 	unsigned* lte;          // max lte as float bits
 	int nx, ny, nz;
 	int per;                // periodic exchange along x, y, z: bits 0, 1, 2
@@ -316,7 +317,8 @@ __device__ __forceinline__ void stage_cell(const StageArgs& a, F3 Ha, size_t idx
 	}
 	const CellMat cm = cell_mat(a, k);
 	if (a.lay_field) Ha = F3{ a.lay->Ha[k][0], a.lay->Ha[k][1], a.lay->Ha[k][2] };
-	const F3 H = eff_field(a, cm, Ha, m, Hd, lap, Ms);
+	F3 H = eff_field(a, cm, Ha, m, Hd, lap, Ms);
+	if (a.Hext) H = H + ld3(a.Hext, idx);
 	if (a.Heff_out) st3(a.Heff_out, idx, H);
 	(void)rk_update<METHOD, STEP>(a, idx, m, llg_rhs(a, cm.alpha, m, H, Ms), Ms, lte_loc);
 }
@@ -354,7 +356,8 @@ __device__ __forceinline__ P6 stage_pair(const StageArgs& a, size_t idx, int i0,
 	// the row kernel serves single meshes only (never a layer stack)
 	const CellMat cm{ a.A, a.alpha, a.K1 };
 	const F3 Ha{ a.Ha[0], a.Ha[1], a.Ha[2] };
-	const F3 H0 = eff_field(a, cm, Ha, m.a, Hd.a, lap0, Ms.x), H1 = eff_field(a, cm, Ha, m.b, Hd.b, lap1, Ms.y);
+	F3 H0 = eff_field(a, cm, Ha, m.a, Hd.a, lap0, Ms.x), H1 = eff_field(a, cm, Ha, m.b, Hd.b, lap1, Ms.y);
+	if (a.Hext) { const P6 he = ld6(a.Hext, idx); H0 = H0 + he.a; H1 = H1 + he.b; }
 	if (a.Heff_out) st6(a.Heff_out, idx, P6{ H0, H1 });
 	const P6 rhs{ llg_rhs(a, cm.alpha, m.a, H0, Ms.x), llg_rhs(a, cm.alpha, m.b, H1, Ms.y) };
 	return rk_update<METHOD, STEP>(a, idx, m, rhs, Ms, lte_loc);

@@ -110,6 +110,17 @@ int bm_mlayer_destroy(bm_mlayer_t m);
  * forward FFTs of all layers, KernelMultiplication_2D over all sources per destination
  * (Boris/DemagKernelCollectionCUDA_Mult.cu:610), inverse FFTs. */
 int bm_mlayer_convolute(bm_mlayer_t m, const float* M, float* Heff, int clear_out, void* stream);
+/* Split form, one call per layer mesh, for callers that hold each layer in its own array:
+ * bm_mlayer_forward for every layer, then bm_mlayer_multiply once, then bm_mlayer_inverse
+ * for every layer.  M_layer / H_layer are one layer's nx*ny cells (AoS); the inverse sets
+ * (clear_out = 1) or adds to H_layer and, if energy (device float) is non-NULL, writes the
+ * layer's -mu0/(2 nx ny) sum M.H_demag into it (M_layer then needed).  Calls of one handle
+ * share its spectra: one evaluation at a time.  Replaces the per-mesh
+ * ConvolutionCUDA::ForwardFFT / KernelMultiplication_MultipleInputs / InverseFFT
+ * (Boris/ConvolutionCUDA.h:254-351) of SDemagCUDA::UpdateField (Boris/SDemagCUDA.cpp:270+). */
+int bm_mlayer_forward(bm_mlayer_t m, int layer, const float* M_layer, void* stream);
+int bm_mlayer_multiply(bm_mlayer_t m, void* stream);
+int bm_mlayer_inverse(bm_mlayer_t m, int layer, const float* M_layer, float* H_layer, int clear_out, float* energy, void* stream);
 /* Copies the FP32 kernels to host memory (test hook): L*L pairs (destination-major), each
  * 6 planes Kxx, Kyy, Kzz, Kxy, Kxz, Kyz of (nx+1)*(ny+1), index kx*(ny+1)+ky.  Own-layer
  * pairs hold the 2D kernel (Kxz = Kyz = 0); Kxz and Kyz of z-shifted pairs are the
@@ -170,6 +181,19 @@ int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p);
  * by a later bm_ode_set_ms_map).  With BM_MOD_DEMAG set (bm_ode_set_params) the demag
  * field is the multilayered convolution.  ml must outlive o; ml NULL detaches the stack. */
 int bm_ode_set_layers(bm_ode_t o, bm_mlayer_t ml, const double* Ms, const double* A, const double* alpha, const double* K1);
+/*
+ * Fields of modules outside the four fused ones (DMI, surface exchange, other anisotropies,
+ * ...).  Before every stage the integrator calls fn(user, M, Hext, stream) with M the
+ * magnetisation that stage evaluates (device float[3n], read only) and Hext the caller's
+ * device float[3n]; fn writes the other modules' summed field into Hext (A/m), ordered on
+ * `stream`, and the stage kernel adds it to the effective field.  fn NULL detaches it.
+ * Replaces: the remaining modules' UpdateField calls of MeshBase::UpdateModulesCUDA
+ * (Boris/MeshBaseModules.cpp:98-111, Boris/ModulesCUDA.h:114) that run before each
+ * DifferentialEquationFMCUDA stage (Boris/DiffEqFMCUDA.h:38-112).  Single-mesh integrator
+ * only (not the batch or adjoint handles).
+ */
+typedef void (*bm_field_fn)(void* user, const float* M, float* Hext, void* stream);
+int bm_ode_set_extra_field(bm_ode_t o, bm_field_fn fn, void* user, float* Hext);
 /* dT, and the adaptive controls err_fail, dT_increase, dT_min, dT_max (values <= 0 keep
  * the current ones); the defaults are the reference's per-method defaults. */
 int bm_ode_set_dt(bm_ode_t o, double dT);

@@ -37,17 +37,16 @@ def test_product_sources_do_not_reference_the_oracle():
 
 
 def test_unsupported_sizes_are_rejected_before_any_device_work():
-    """Sizes without an FFT plan or outside the supported range return BM_ERR_SIZE / BM_ERR_ARG
-    (checked before any allocation, so this runs without a device)."""
+    """Open axes take any nx <= 2048, ny <= 1024 and nz <= 64 (N padded up to a planned
+    length); larger sizes return BM_ERR_SIZE and bad arguments BM_ERR_ARG, both checked
+    before any allocation, so this runs without a device."""
     import ctypes as C
     L = _lib.load()
     h = C.c_void_p()
-    cases = [((3, 8, 1), 2),        # nx odd
-             ((10, 8, 1), 2),       # nx without a plan (2 * 5)
-             ((8, 5, 1), 2),        # N.y = 10 without a plan
-             ((8, 8, 65), 2),       # N.z = 256 > 128
-             ((4096, 8, 1), 2),     # nx above 2048
-             ((0, 8, 1), 1), ((8, 0, 1), 1)]
+    cases = [((8, 8, 65), 2),       # N.z = 256 > 128
+             ((2049, 8, 1), 2),     # nx above 2048
+             ((8, 1025, 1), 2),     # N.y above 2048
+             ((0, 8, 1), 1), ((8, 0, 1), 1), ((8, 8, 0), 1)]
     for n, code in cases:
         assert L.bm_demag_create(*n, 4e-9, 4e-9, 4e-9, 0, C.byref(h)) == code, n
         assert h.value is None
@@ -59,6 +58,7 @@ def test_pbc_sizes_are_checked_before_any_device_work():
     L = _lib.load()
     h = C.c_void_p()
     cases = [((6, 8, 1, 1, 0, 0), 2),       # periodic nx = 6: N.x / 2 = 3 has no plan
+             ((7, 8, 1, 1, 0, 0), 2),       # periodic nx odd
              ((8, 10, 1, 0, 1, 0), 2),      # periodic ny = 10 has no plan
              ((8, 8, 3, 0, 0, 1), 2),       # periodic nz = 3: N.z = 3 is not a power of two
              ((8, 8, 2, 0, 0, 1), 2),       # N.z = 2 below the smallest z plan

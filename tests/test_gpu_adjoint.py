@@ -38,9 +38,11 @@ def test_gpu_adjoint_matches_oracle(gpu, ckpt):
     L, _ = r.forward(torch.tensor(m0, dtype=torch.float32), torch.tensor(mt, dtype=torch.float32))
     g, ga = r.backward()
     g = g.cpu().numpy().astype(np.float64)
+    eg, ea = np.linalg.norm(g - g0) / np.linalg.norm(g0), abs(ga - ga0) / abs(ga0)
+    print(f"ckpt {ckpt}: loss {abs(L - L0):.2e}, dL/dMs rel L2 {eg:.2e}, dL/damp rel {ea:.2e}")
     assert abs(L - L0) <= 1e-5
-    assert np.linalg.norm(g - g0) / np.linalg.norm(g0) <= 2e-3
-    assert abs(ga - ga0) <= 2e-3 * abs(ga0)
+    assert eg <= 2e-3
+    assert ea <= 2e-3
     r.close()
 
 
@@ -92,8 +94,10 @@ def test_gpu_layer_stack_adjoint_matches_oracle(gpu):
     g, _ = r.backward()
     gz = r.zeeman_grads()
     g = g.cpu().numpy().astype(np.float64)
+    eg = np.linalg.norm(g - g0) / np.linalg.norm(g0)
+    print(f"stack: loss {abs(L - L0):.2e}, dL/dMs rel L2 {eg:.2e}, slopes {gz[1:]} vs {gz0[1:]}")
     assert abs(L - L0) <= 1e-5
-    assert np.linalg.norm(g - g0) / np.linalg.norm(g0) <= 2e-3
+    assert eg <= 2e-3
     np.testing.assert_allclose(gz[1:], gz0[1:], rtol=2e-3, atol=2e-3 * np.abs(gz0[1:]).max())
     r.close()
 

@@ -5,6 +5,7 @@ and one direct FP64 oracle comparison the host finishes in seconds.
   - demag of a uniformly magnetised prism: the cell-averaged field equals Aharoni's
     closed-form demag factors (exact for the Newell tensor, whatever the mesh size);
   - the demag operator is linear and symmetric (<M1, N M2> = <M2, N M1> for equal cells);
+  - the demag field of the BorisRand random state against the FP64 oracle (1e-6);
   - one RKDP54 step from the BorisRand random state against the FP64 oracle;
   - the fused row-kernel path and the unfused path follow the same 10-step trajectory.
 * the 64 films of uneven sizes (the batched workload): batched RK4 against each film run
@@ -62,6 +63,21 @@ def test_fullsize_demag_uniform_prism_and_symmetry(gpu):
     H3 = d.convolute(2.5 * M1 - 0.75 * M2)
     ref = 2.5 * H1.double() - 0.75 * H2.double()
     assert ((H3.double() - ref).norm() / ref.norm()).item() <= 1e-6
+
+
+def test_fullsize_demag_matches_fp64_oracle(gpu):
+    """The demag field of the 512x512x32 mesh in the BorisRand random state against the FP64
+    oracle's convolution of the same M: relative L2 <= 1e-6."""
+    n, h = N_BIG, H_BIG
+    M0 = big_m0()
+    d = api.Demag(n, h)
+    Hg = d.convolute(torch.from_numpy(M0).cuda()).cpu().numpy().astype(np.float64)
+    d.close()
+    torch.cuda.empty_cache()
+    ref = O.OracleDemag(n, h, Nz_override=64)(M0.astype(np.float64))
+    e = float(np.linalg.norm(Hg - ref) / np.linalg.norm(ref))
+    print(f"512x512x32 demag rel L2 vs FP64 oracle: {e:.2e}")
+    assert e <= 1e-6, e
 
 
 def test_fullsize_rkdp54_first_step_matches_oracle(gpu):

@@ -204,3 +204,21 @@ def test_stack_configuration_errors(gpu):
     grp = np.array([0, 1, 1], dtype=np.int32)
     assert st.L.bm_ode_set_layer_meshes(st.handle, grp.ctypes.data_as(C.c_void_p), None) == 0
     st.close(); plain.close()
+
+
+def test_set_params_waveform_with_layer_fields_refused(gpu):
+    """bm_ode_set_params refuses an applied-field waveform while per-layer constant fields are
+    set (they would replace the waveform in the stage kernel), as bm_ode_set_layer_meshes
+    refuses per-layer fields under a waveform."""
+    import ctypes as C
+    nx, ny = 16, 16
+    p = FMParams(Ms=8e5, A=1.3e-11, alpha=0.02, modules=BM_MOD_DEMAG | BM_MOD_EXCHANGE | BM_MOD_ZEEMAN)
+    hz, z0 = [4e-9, 4e-9], [0.0, 4e-9]
+    st = api.LayerStackSolver(BM_EVAL_RK4, nx, ny, 4e-9, 4e-9, hz, z0, p)
+    st.set_layer_meshes(np.array([0, 0], dtype=np.int32), np.array([[0.0, 0.0, 1e4], [0.0, 0.0, -1e4]]))
+    pw = FMParams(**{**p.__dict__, "wave": 1, "amp": 1e4, "w": 1e10, "dir": (1.0, 0.0, 0.0)})
+    pc = pw.to_c()
+    assert st.L.bm_ode_set_params(st.handle, C.byref(pc)) == 5                   # BM_ERR_STATE
+    pc = p.to_c()
+    assert st.L.bm_ode_set_params(st.handle, C.byref(pc)) == 0
+    st.close()

@@ -167,3 +167,25 @@ def test_pbc_averaged_inputs_duplicated_outputs(gpu):
     ref = O.OracleDemag(n, h, pbc=pbc)(0.5 * (M1.astype(np.float64) + M2.astype(np.float64)))
     assert rel_l2(H1.cpu().numpy(), ref) <= 1e-6
     assert torch.equal(H1, H2)
+
+
+def test_set_params_demag_on_with_other_images_refused(gpu):
+    """A periodic mesh whose demag is switched on later by bm_ode_set_params needs a demag
+    handle with the same images (bm_ode_set_pbc checks this only while demag is on)."""
+    import ctypes as C
+    from boris2_amd import _lib
+    L = _lib.load()
+    n, h = (32, 16, 1), (4e-9, 4e-9, 4e-9)
+    p_off = api.FMParams(Ms=8e5, A=1.3e-11, alpha=0.02, modules=api.BM_MOD_EXCHANGE | api.BM_MOD_ZEEMAN)
+    p_on = api.FMParams(**{**p_off.__dict__, "modules": p_off.modules | api.BM_MOD_DEMAG})
+    d_open = api.Demag(n, h)
+    M = torch.from_numpy(api.random_magnetisation(512, 8e5, 1).astype(np.float32)).cuda()
+    pc_off, pc_on = p_off.to_c(), p_on.to_c()
+    hd = C.c_void_p()
+    assert L.bm_ode_create(api.BM_EVAL_RK4, *n, *h, C.byref(pc_off), d_open.handle, api._ptr(M), C.byref(hd)) == 0
+    assert L.bm_ode_set_pbc(hd, 2, 0, 0) == 0                 # demag off: only exchange wraps
+    assert L.bm_ode_set_params(hd, C.byref(pc_on)) == 5       # BM_ERR_STATE: the handle is open
+    assert L.bm_ode_set_pbc(hd, 0, 0, 0) == 0
+    assert L.bm_ode_set_params(hd, C.byref(pc_on)) == 0       # open mesh, open handle
+    assert L.bm_ode_destroy(hd) == 0
+    d_open.close()

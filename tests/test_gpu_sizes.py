@@ -1,0 +1,110 @@
+"""Mesh sizes outside the 2^a 3^b plans: the reference plans N.x = 2 nx and N.y = 2 ny for
+any n (Boris/ConvolutionDataCUDA.cpp:55-86, cuFFT); here an open axis is padded to the next
+planned length (N >= 2 n - 1 gives the same non-cyclic convolution), rows shorter than
+N.x / 2 take the general-length x passes, and y lines mask their data rows.  Checked against
+the FP64 oracle run at the reference's own sizes N = 2 n (its FFT takes any length): nx with
+factors 5 and 7, odd nx and ny, primes, 2D and 3D, the set/add/energy outputs, and
+trajectories of the integrator (which takes the unfused evaluation for such rows)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as O
+from boris2_amd import api
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+SIZES = [(10, 8, 1), (50, 25, 1), (100, 100, 1), (125, 7, 1), (33, 17, 1), (101, 3, 1), (3, 1, 1), (1000, 20, 1),
+         (2000, 6, 1), (10, 14, 3), (35, 21, 5), (100, 50, 4), (7, 9, 13), (250, 40, 2)]
+
+
+@pytest.mark.parametrize("n", SIZES, ids=lambda n: "x".join(map(str, n)))
+def test_general_size_demag_matches_oracle(gpu, n):
+    h = (4e-9, 3e-9, 5e-9)
+    ncell = n[0] * n[1] * n[2]
+    M = api.random_magnetisation(ncell, 8e5, 17 + n[0]).astype(np.float32)
+    d = api.Demag(n, h)
+    N, _ = d.info()
+    H = d.convolute(torch.from_numpy(M).cuda()).cpu().numpy()
+    ref = O.OracleDemag(n, h)(M.astype(np.float64))
+    e = rel_l2(H, ref)
+    print(f"n {n}: N {N}, rel L2 {e:.2e}")
+    assert e <= 1e-6
+    d.close()
+
+
+@pytest.mark.parametrize("n", [(50, 30, 1), (21, 11, 3)], ids=lambda n: "x".join(map(str, n)))
+def test_general_size_set_add_planar_energy(gpu, n):
+    """AoS set and add, planar output and the energy density on padded meshes."""
+    from boris2_amd import _lib
+    import ctypes as C
+    h = (4e-9, 4e-9, 4e-9)
+    ncell = n[0] * n[1] * n[2]
+    M = api.random_magnetisation(ncell, 8e5, 5).astype(np.float32)
+    ref = O.OracleDemag(n, h)(M.astype(np.float64))
+    d = api.Demag(n, h)
+    Mt = torch.from_numpy(M).cuda()
+    H0 = torch.full((ncell, 3), 7.0, dtype=torch.float32, device="cuda")
+    L = _lib.load()
+    en = torch.zeros(1, dtype=torch.float32, device="cuda")
+    _lib.check(L.bm_demag_convolute(d.handle, api._ptr(Mt), api._ptr(H0), 0, api._ptr(en), api._stream()))
+    assert rel_l2(H0.cpu().numpy() - 7.0, ref) <= 1e-6
+    e_ref = -api.MU0 / (2 * ncell) * float(np.sum(M.astype(np.float64) * ref))
+    assert abs(en.item() - e_ref) <= 1e-5 * abs(e_ref)
+    P = torch.empty(3 * ncell, dtype=torch.float32, device="cuda")
+    _lib.check(L.bm_demag_convolute_planar(d.handle, api._ptr(Mt), api._ptr(P), api._stream()))
+    assert rel_l2(P.cpu().numpy().reshape(3, -1).T, ref) <= 1e-6
+    d.close()
+
+
+@pytest.mark.parametrize("method", [api.BM_EVAL_RKDP54, api.BM_EVAL_RK4])
+@pytest.mark.parametrize("n", [(50, 30, 1), (25, 15, 3)], ids=lambda n: "x".join(map(str, n)))
+def test_general_size_trajectory_matches_oracle(gpu, method, n):
+    h = (4e-9, 4e-9, 4e-9)
+    p = api.FMParams(Ms=8e5, A=1.3e-11, alpha=0.02, K1=2e5, ea=(0.0, 0.0, 1.0), Ha=(-1.5e4, 3e3, 1e3),
+                     modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ZEEMAN | api.BM_MOD_ANIS_UNI)
+    ncell = n[0] * n[1] * n[2]
+    rng = np.random.default_rng(3)
+    th, ph = rng.uniform(0, 0.3, ncell), rng.uniform(0, 2 * np.pi, ncell)
+    M0 = (p.Ms * np.stack([np.cos(th), np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph)], 1)).astype(np.float32)
+    s = api.FMSolver(method, n, h, p, M0)
+    o = O.OracleSim(method, n, h, p)
+    o.set_M(M0.astype(np.float64))
+    s.run_to_iteration(30)
+    o.run_to_iteration(30)
+    st, so = s.state(), o.state()
+    assert st["rejections"] == so["rejections"]
+    assert abs(st["time"] - so["time"]) <= 1e-4 * so["time"]
+    dm = float(np.abs(s.average_m() - o.average_m()).max())
+    e = rel_l2(s.M.cpu().numpy(), o.get_M())
+    print(f"n {n} method {method}: <m> {dm:.2e} per-cell {e:.2e}")
+    assert dm <= 1e-4 and e <= 1e-4
+    s.close()
+
+
+def test_batch_takes_padded_ny_and_refuses_other_nx(gpu):
+    """A batch accepts meshes whose N.y is padded (ny = 25, 13) and equals their single runs;
+    a mesh whose nx has no plan is refused with BM_ERR_SIZE (it runs on its own handle)."""
+    p = api.FMParams(Ms=8e5, A=1.3e-11, alpha=0.05, Ha=(0.0, 1e4, 0.0),
+                     modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ZEEMAN)
+    h = (4e-9, 4e-9, 4e-9)
+    meshes = [((32, 25, 1), h, p), ((48, 13, 3), h, p)]
+    M0 = [api.random_magnetisation(n[0] * n[1] * n[2], p.Ms, 9 + b).astype(np.float32) for b, (n, _, _) in enumerate(meshes)]
+    bs = api.BatchSolver(api.BM_EVAL_RK4, meshes, M0)
+    bs.set_dt(1e-13)
+    bs.run(10)
+    torch.cuda.synchronize()
+    for b, (n, hh, pp) in enumerate(meshes):
+        s = api.FMSolver(api.BM_EVAL_RK4, n, hh, pp, M0[b])
+        s.set_dt(1e-13)
+        s.run_to_iteration(10)
+        assert rel_l2(bs.mesh(b).cpu().numpy(), s.M.cpu().numpy().astype(np.float64)) <= 2e-6
+        s.close()
+    bs.close()
+    with pytest.raises(RuntimeError, match="error 2"):
+        api.BatchSolver(api.BM_EVAL_RK4, [((50, 16, 1), h, p)])
