@@ -77,7 +77,7 @@ struct AdjRK4 {
 	{
 		const int d[3] = { nx, ny, nz };
 		const size_t st[3] = { 1, (size_t)nx, (size_t)nx * ny };
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if((size_t)nx * ny * nz >= kOmpMinWork)
 		for (int k = 0; k < nz; k++) for (int j = 0; j < ny; j++) for (int i = 0; i < nx; i++) {
 			const size_t idx = (size_t)i + (size_t)j * nx + (size_t)k * nx * ny;
 			const int ijk[3] = { i, j, k };

@@ -40,6 +40,11 @@
 
 namespace oracle {
 
+// OpenMP regions run in parallel only above this many cells (or padded convolution points):
+// below it the fork and join of the thread team costs more than the work (one-cell and
+// macrospin runs make millions of such regions).
+constexpr size_t kOmpMinWork = (size_t)1 << 14;
+
 static const double kMU0 = 1.256637061435916e-6;   // BorisLib/Funcs_Math_base.h:9
 static const double kGAMMA = 2.212761569e5;        // BorisLib/Funcs_Math_base.h:16
 
@@ -133,7 +138,7 @@ inline void kernels_from_tensor_3d(Kernels& K, std::vector<double>& D, bool offd
 	{
 		Fft<double> f(Nx);
 		std::vector<double> tmp((size_t)Nc * Ny * Nz * 3);
-#pragma omp parallel
+#pragma omp parallel if((size_t)Nx * Ny * Nz >= kOmpMinWork)
 		{
 			std::vector<C> a(Nx), s(Nx);
 #pragma omp for collapse(2)
@@ -154,7 +159,7 @@ inline void kernels_from_tensor_3d(Kernels& K, std::vector<double>& D, bool offd
 	{
 		Fft<double> f(Ny);
 		std::vector<double> tmp((size_t)Nc * Nyh * Nz * 3);
-#pragma omp parallel
+#pragma omp parallel if((size_t)Nx * Ny * Nz >= kOmpMinWork)
 		{
 			std::vector<C> a(Ny), s(Ny);
 #pragma omp for collapse(2)
@@ -175,7 +180,7 @@ inline void kernels_from_tensor_3d(Kernels& K, std::vector<double>& D, bool offd
 	{
 		Fft<double> f(Nz);
 		out.assign((size_t)Nc * Nyh * Nzh * 3, 0.0);
-#pragma omp parallel
+#pragma omp parallel if((size_t)Nx * Ny * Nz >= kOmpMinWork)
 		{
 			std::vector<C> a(Nz), s(Nz);
 #pragma omp for collapse(2)
@@ -279,7 +284,7 @@ struct Demag {
 	void convolute(const R* M, R* H, bool clear_out)
 	{
 		// forward x (R2C) for the nonzero rows
-#pragma omp parallel
+#pragma omp parallel if((size_t)Nx * Ny * Nz >= kOmpMinWork)
 		{
 			std::vector<C> a(Nx), s(Nx);
 #pragma omp for collapse(2)
@@ -292,7 +297,7 @@ struct Demag {
 			}
 		}
 		// forward y for the nonzero planes
-#pragma omp parallel
+#pragma omp parallel if((size_t)Nx * Ny * Nz >= kOmpMinWork)
 		{
 			std::vector<C> a(Ny), s(Ny);
 #pragma omp for collapse(2)
@@ -306,7 +311,7 @@ struct Demag {
 		}
 		if (Nz == 1) {
 			// 2D: multiply along y lines
-#pragma omp parallel for
+#pragma omp parallel for if((size_t)Nx * Ny * Nz >= kOmpMinWork)
 			for (int i = 0; i < Nc; i++) {
 				for (int j = 0; j < Ny; j++) {
 					int jj = (j <= Ny / 2) ? j : Ny - j;
@@ -322,7 +327,7 @@ struct Demag {
 			}
 		}
 		else {
-#pragma omp parallel
+#pragma omp parallel if((size_t)Nx * Ny * Nz >= kOmpMinWork)
 			{
 				std::vector<C> a[3], s(Nz);
 				for (int c = 0; c < 3; c++) a[c].resize(Nz);
@@ -353,7 +358,7 @@ struct Demag {
 			}
 		}
 		// inverse y for the nonzero planes, keep j < ny
-#pragma omp parallel
+#pragma omp parallel if((size_t)Nx * Ny * Nz >= kOmpMinWork)
 		{
 			std::vector<C> a(Ny), s(Ny);
 #pragma omp for collapse(2)
@@ -367,7 +372,7 @@ struct Demag {
 		}
 		// inverse x (C2R), scale, write
 		const R scale = (R)1 / (R)((double)Nx * Ny * Nz);
-#pragma omp parallel
+#pragma omp parallel if((size_t)Nx * Ny * Nz >= kOmpMinWork)
 		{
 			std::vector<C> a(Nx), s(Nx);
 #pragma omp for collapse(2)
@@ -650,7 +655,7 @@ struct Sim {
 	{
 		int d[3] = { nx, ny, nz };
 		size_t st[3] = { 1, (size_t)nx, (size_t)nx * ny };
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if(n >= kOmpMinWork)
 		for (int k = 0; k < nz; k++) for (int j = 0; j < ny; j++) for (int i = 0; i < nx; i++) {
 			size_t idx = (size_t)i + (size_t)j * nx + (size_t)k * nx * ny;
 			int ijk[3] = { i, j, k };
@@ -688,7 +693,7 @@ struct Sim {
 	void add_anisotropy()
 	{
 		R e[3] = { (R)p.ea[0], (R)p.ea[1], (R)p.ea[2] };
-#pragma omp parallel for
+#pragma omp parallel for if(n >= kOmpMinWork)
 		for (long long ii = 0; ii < (long long)n; ii++) {
 			size_t i = (size_t)ii;
 			R Ms = Ms_at(i);
@@ -758,7 +763,7 @@ struct Sim {
 		case EV_RKDP54: {
 			if (evalStep == 0) {
 				double lmax = 0;
-#pragma omp parallel for reduction(max:lmax)
+#pragma omp parallel for reduction(max:lmax) if(n >= kOmpMinWork)
 				for (long long ii = 0; ii < (long long)n; ii++) {
 					size_t i = (size_t)ii; R rhs[3]; llg(i, rhs);
 					R pred[3], d[3];
@@ -784,7 +789,7 @@ struct Sim {
 			}
 			else if (evalStep <= 4) {
 				int s = evalStep;
-#pragma omp parallel for
+#pragma omp parallel for if(n >= kOmpMinWork)
 				for (long long ii = 0; ii < (long long)n; ii++) {
 					size_t i = (size_t)ii; R rhs[3]; llg(i, rhs);
 					for (int c = 0; c < 3; c++) {
@@ -801,7 +806,7 @@ struct Sim {
 				evalStep++;
 			}
 			else {
-#pragma omp parallel for
+#pragma omp parallel for if(n >= kOmpMinWork)
 				for (long long ii = 0; ii < (long long)n; ii++) {
 					size_t i = (size_t)ii; R rhs[3]; llg(i, rhs);
 					for (int c = 0; c < 3; c++) {
@@ -817,7 +822,7 @@ struct Sim {
 		} break;
 		case EV_RK4: {
 			int s = evalStep;
-#pragma omp parallel for
+#pragma omp parallel for if(n >= kOmpMinWork)
 			for (long long ii = 0; ii < (long long)n; ii++) {
 				size_t i = (size_t)ii; R rhs[3]; llg(i, rhs);
 				for (int c = 0; c < 3; c++) {
@@ -835,7 +840,7 @@ struct Sim {
 		case EV_TEULER:
 		case EV_AHEUN: {
 			if (evalStep == 0) {
-#pragma omp parallel for
+#pragma omp parallel for if(n >= kOmpMinWork)
 				for (long long ii = 0; ii < (long long)n; ii++) {
 					size_t i = (size_t)ii; R rhs[3]; llg(i, rhs);
 					for (int c = 0; c < 3; c++) { size_t q = i * 3 + c; sM1[q] = M[q]; M[q] += rhs[c] * dt; }
@@ -844,7 +849,7 @@ struct Sim {
 			}
 			else {
 				double lmax = 0;
-#pragma omp parallel for reduction(max:lmax)
+#pragma omp parallel for reduction(max:lmax) if(n >= kOmpMinWork)
 				for (long long ii = 0; ii < (long long)n; ii++) {
 					size_t i = (size_t)ii; R rhs[3]; llg(i, rhs);
 					R saved[3] = { M[i * 3], M[i * 3 + 1], M[i * 3 + 2] };
