@@ -655,7 +655,8 @@ int xtile_factor(bool fwd)
 int launch_fwd_x(int HX, bool full, const float* M, float2* Sx, int ny, int nz, int ntl, int ncu, hipStream_t st,
 	const float* E0 = nullptr, float* sM1 = nullptr, float* Mout = nullptr, float dT5 = 0.f, int nx = 0, int zoff = 0, int nzl = 0)
 {
-	const bool adv = E0 != nullptr, gen = nx && nx != HX;
+	// a periodic row holds 2 HX cells and is never of the general length
+	const bool adv = E0 != nullptr, gen = !full && nx && nx != HX;
 	if (full && adv) return fail(BM_ERR_STATE, "fused advance with a periodic x axis");
 	if (gen && (full || adv)) return fail(BM_ERR_STATE, "general row length: no periodic x, no fused advance");
 	const int f = (full || gen) ? 1 : xtile_factor(true);
@@ -691,8 +692,7 @@ void launch_inv_x_var(int mode, dim3 g, hipStream_t st, const float2* Sx, float*
 int launch_inv_x(int HX, bool full, int mode, const float2* Sx, float* out, int ny, int nz, int ntl, float scale, int ncu, hipStream_t st,
 	const float* M = nullptr, double* epart = nullptr, int nx = 0, int zoff = 0, int nzl = 0)
 {
-	const bool gen = nx && nx != HX;
-	if (gen && full) return fail(BM_ERR_STATE, "general row length: no periodic x");
+	const bool gen = !full && nx && nx != HX;
 	const int f = (mode == 0 && !full && !gen) ? xtile_factor(false) : 1;
 	switch (HX) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; int TY = TY1 * f, nt = (ny + TY - 1) / TY; dim3 g(grid_for(3 * nz * nt, ncu, 2)); \
@@ -970,6 +970,7 @@ static int demag_create(int nx, int ny, int nz, double hx, double hy, double hz,
 	d->Nx = 2 * HX; d->Ny = Ny; d->Nz = Nz;
 	d->NC = d->Nx / 2 + 1; d->NYH = d->Ny / 2 + 1; d->NZH = d->Nz / 2 + 1;
 	d->ncu = device_cu_count();
+	if (const char* e = getenv("BM_ROW_WAVE")) d->row_wave = atoi(e) != 0;
 	// line pitch of the full spectra (complex elements); BM_S_PAD adds padding for tuning experiments
 	d->Sp = d->Ny;
 	if (const char* e = getenv("BM_S_PAD")) d->Sp += std::max(0, atoi(e));

@@ -190,6 +190,22 @@ template <int N, typename T> struct Twiddles {
 	__device__ __forceinline__ void init(int t) { init_stage<1>(t); }
 };
 
+// Barrier of the LDS exchanges between FFT stages.  WAVE: every line of the exchange is held
+// by lanes of one wavefront (T <= 64 threads per line, lines never straddling two waves), so
+// ordering the wave's own LDS accesses suffices: a wave's LDS instructions execute in issue
+// order, and the wavefront-scope fences keep the compiler from moving LDS accesses across
+// the point.  Otherwise the whole workgroup waits.
+template <bool WAVE>
+__device__ __forceinline__ void xsync()
+{
+	if constexpr (WAVE) {
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+	}
+	else __syncthreads();
+}
+
 // LDS element i of a line: lines with unit stride are padded by one element every 16 so
 // the radix-R scatter of a Stockham stage (lane stride R elements) hits distinct banks;
 // strided lines (several lines interleaved element by element) are conflict-free as is.
@@ -198,7 +214,7 @@ __device__ __forceinline__ int lidx(int i, int stride) { return stride == 1 ? i 
 template <int N> constexpr int lds_len() { return N + N / 16 + 1; }
 
 // One Stockham stage on registers.  LDS element i of this line lives at lds[lidx(i, stride)].
-template <int N, int s, int DIR, typename T>
+template <int N, int s, int DIR, typename T, bool WAVE = false>
 __device__ __forceinline__ void fft_stage(C2<T>* v, C2<T>* lds, int stride, int t, const Twiddles<N, T>& tw)
 {
 	constexpr int E = Plan<N>::E, TT = plan_T<N>(), R = Plan<N>::R[s], B = E / R, Ns = plan_Ns<N>(s);
@@ -225,7 +241,7 @@ __device__ __forceinline__ void fft_stage(C2<T>* v, C2<T>* lds, int stride, int 
 		for (int r = 0; r < R; r++) v[b + r * B] = u[r];
 	}
 	if constexpr (s < Plan<N>::S - 1) {
-		__syncthreads();
+		xsync<WAVE>();
 #pragma unroll
 		for (int b = 0; b < B; b++) {
 			int j = t + b * TT;
@@ -233,24 +249,24 @@ __device__ __forceinline__ void fft_stage(C2<T>* v, C2<T>* lds, int stride, int 
 #pragma unroll
 			for (int r = 0; r < R; r++) lds[lidx(d + r * Ns, stride)] = v[b + r * B];
 		}
-		__syncthreads();
+		xsync<WAVE>();
 	}
 }
 
-template <int N, int s, int DIR, typename T>
+template <int N, int s, int DIR, typename T, bool WAVE = false>
 __device__ __forceinline__ void fft_stages(C2<T>* v, C2<T>* lds, int stride, int t, const Twiddles<N, T>& tw)
 {
-	fft_stage<N, s, DIR, T>(v, lds, stride, t, tw);
-	if constexpr (s + 1 < Plan<N>::S) fft_stages<N, s + 1, DIR, T>(v, lds, stride, t, tw);
+	fft_stage<N, s, DIR, T, WAVE>(v, lds, stride, t, tw);
+	if constexpr (s + 1 < Plan<N>::S) fft_stages<N, s + 1, DIR, T, WAVE>(v, lds, stride, t, tw);
 }
 
-// Full transform.  All threads of the block must call it together (it contains
-// __syncthreads between stages); lines handled by different thread groups of the block
-// must use disjoint LDS regions.
-template <int N, int DIR, typename T>
+// Full transform.  All threads of the block (WAVE: of the wavefront) must call it together
+// (it synchronises between stages); lines handled by different thread groups must use
+// disjoint LDS regions.
+template <int N, int DIR, typename T, bool WAVE = false>
 __device__ __forceinline__ void fft(C2<T>* v, C2<T>* lds, int stride, int t, const Twiddles<N, T>& tw)
 {
-	fft_stages<N, 0, DIR, T>(v, lds, stride, t, tw);
+	fft_stages<N, 0, DIR, T, WAVE>(v, lds, stride, t, tw);
 }
 
 // Three transforms of the same length in lock step (shared barriers).

@@ -46,6 +46,8 @@ struct bm_demag_s {
 	int ecap = 0;
 	long long bytes = 0;
 	int ncu = 256;
+	bool row_wave = false;    // integrator row kernel at nx = 512: the wave-independent variant
+	                          // (BM_ROW_WAVE=1 at creation; measured slower, kept for tests)
 };
 
 namespace bm {

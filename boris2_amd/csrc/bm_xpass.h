@@ -51,15 +51,15 @@ template <int HX> struct XThread {
 
 // Forward: v[e] = (x[2k], x[2k+1]) for e < E/2 and zero above (the zero-padded half).  On
 // return LDS row lrow holds X[0..HX].  Every thread of the block must call it (barriers).
-template <int HX>
+template <int HX, bool WAVE = false>
 __device__ __forceinline__ void r2c_row(const XThread<HX>& th, C2<float>* lrow, C2<float>* v)
 {
 	constexpr int E = XThread<HX>::E, T = XThread<HX>::T;
-	fft<HX, -1, float>(v, lrow, 1, th.t, th.tw);
-	__syncthreads();
+	fft<HX, -1, float, WAVE>(v, lrow, 1, th.t, th.tw);
+	xsync<WAVE>();
 #pragma unroll
 	for (int e = 0; e < E; e++) lrow[th.t + T * e] = v[e];
-	__syncthreads();
+	xsync<WAVE>();
 	C2<float> X[E];
 #pragma unroll
 	for (int e = 0; e < E; e++) {
@@ -69,7 +69,7 @@ __device__ __forceinline__ void r2c_row(const XThread<HX>& th, C2<float>* lrow, 
 		C2<float> wb = cmul(th.wk[e], b);                    // W^k (Z[k] - conj Z[HX-k])
 		X[e] = mk(0.5f * (a.x + wb.y), 0.5f * (a.y - wb.x)); // a/2 - i wb/2
 	}
-	__syncthreads();
+	xsync<WAVE>();
 #pragma unroll
 	for (int e = 0; e < E; e++) lrow[th.t + T * e] = X[e];
 	if (th.t == 0) lrow[HX] = mk(v[0].x - v[0].y, 0.f);
@@ -77,7 +77,7 @@ __device__ __forceinline__ void r2c_row(const XThread<HX>& th, C2<float>* lrow, 
 
 // Inverse: LDS row lrow holds X[0..HX].  On return v[e], e < E/2, holds the unscaled real
 // pair (x[2n], x[2n+1]), n = t + T e (N.x times the true values).
-template <int HX>
+template <int HX, bool WAVE = false>
 __device__ __forceinline__ void c2r_row(const XThread<HX>& th, C2<float>* lrow, C2<float>* v)
 {
 	constexpr int E = XThread<HX>::E, T = XThread<HX>::T;
@@ -89,7 +89,7 @@ __device__ __forceinline__ void c2r_row(const XThread<HX>& th, C2<float>* lrow, 
 		C2<float> wb = cmulc(b, th.wk[e]);                   // W^-k (X[k] - conj X[HX-k])
 		v[e] = mk(a.x - wb.y, a.y + wb.x);                    // a + i wb
 	}
-	fft<HX, 1, float>(v, lrow, 1, th.t, th.tw);               // first LDS write waits on a barrier
+	fft<HX, 1, float, WAVE>(v, lrow, 1, th.t, th.tw);         // first LDS write waits on a barrier
 }
 
 // x-side spectra layout.  The x passes and the y passes exchange the spectra of the ny data

@@ -116,16 +116,28 @@ def test_adaptive_dt_replayed_on_oracle(gpu, path, method):
     """The adaptive run of test_3d_run_matches_oracle, with the GPU's accepted step sizes
     replayed on the FP64 oracle as fixed steps: with the same step sizes the per-cell
     difference is the stages' FP32 rounding alone (rejected attempts restore M exactly, so
-    the accepted path does not depend on them)."""
+    the accepted path does not depend on them).  The accepted step sizes are the differences
+    of the time reached at each iteration; a rejection can undo the previous step (RKDP54
+    judges a step at the next step's first stage), so a later call overwrites that entry."""
     n, h, p = (64, 48, 6), (4e-9, 4e-9, 4e-9), bench_like_params()
     ncell = n[0] * n[1] * n[2]
     M0 = api.random_magnetisation(ncell, p.Ms, 5).astype(np.float32)
     s = api.FMSolver(method, n, h, p, M0)
     o = O.OracleSim(method, n, h, p, Nz_override=16)
     o.set_M(M0.astype(np.float64))
-    for _ in range(12):
+    t_at = {0: 0.0}
+    calls = 0
+    while s.state()["iteration"] < 12:
+        calls += 1
+        assert calls <= 100
+        it0 = s.state()["iteration"]
         s.advance_time()
-        dT = s.state()["dT"]
+        st = s.state()
+        assert st["iteration"] <= it0 + 1
+        t_at = {k: v for k, v in t_at.items() if k < st["iteration"]}
+        t_at[st["iteration"]] = st["time"]
+    for k in range(1, 13):
+        dT = t_at[k] - t_at[k - 1]
         o.set_dt(dT)
         o.set_astep(dtmin=dT, dtmax=dT)
         o.advance_time()
