@@ -18,6 +18,7 @@ EXPORTS = [
     "bm_last_error", "bm_device_ok",
     "bm_demag_create", "bm_demag_create_pbc", "bm_demag_destroy", "bm_demag_info", "bm_demag_convolute",
     "bm_demag_convolute_planar", "bm_demag_copy_kernels", "bm_demag_convolute_averaged",
+    "bm_demag_convolute_averaged_static",
     "bm_ode_create", "bm_ode_destroy", "bm_ode_set_ms_map", "bm_ode_set_params", "bm_ode_set_layers", "bm_ode_set_dt",
     "bm_ode_set_astep", "bm_ode_set_pbc", "bm_ode_set_layer_meshes", "bm_ode_set_eval_speedup", "bm_ode_get_speedup_state", "bm_ode_advance_time", "bm_ode_run_to_iteration", "bm_ode_iterate_stage",
     "bm_ode_restore", "bm_ode_get_state", "bm_ode_keep_heff", "bm_ode_copy_heff", "bm_ode_average_m",
@@ -85,6 +86,7 @@ def load() -> C.CDLL:
         "bm_demag_convolute": (i, [vp, vp, vp, i, vp, vp]),
         "bm_demag_convolute_planar": (i, [vp, vp, vp, vp]),
         "bm_demag_convolute_averaged": (i, [vp, vp, vp, vp, vp, i, vp, vp]),
+        "bm_demag_convolute_averaged_static": (i, [vp, vp, vp, vp, vp, i, vp, vp]),
         "bm_demag_copy_kernels": (i, [vp, vp, ll]),
         "bm_mlayer_create": (i, [i, i, i, d, d, vp, vp, i, C.POINTER(vp)]),
         "bm_mlayer_destroy": (i, [vp]),

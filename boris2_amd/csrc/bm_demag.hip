@@ -31,6 +31,8 @@
 #include <vector>
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
+#include <string>
 
 using namespace bm;
 
@@ -987,6 +989,9 @@ static int demag_create(int nx, int ny, int nz, double hx, double hy, double hz,
 	d->bytes = (long long)(sxbytes + sbytes) + d->kcount * 4 + (long long)d->ecap * 8;
 	hipStream_t st = nullptr;
 	int rc = compute_kernels(d, st);
+	// the allocation-free entry point's scratch address on this device, resolved at set-up
+	float* sc = nullptr;
+	if (!rc) rc = static_scratch(&sc);
 	if (rc) { bm_demag_destroy(d); return rc; }
 	*out = d;
 	return BM_OK;
@@ -1081,6 +1086,65 @@ extern "C" int bm_demag_convolute_averaged(bm_demag_t d, const float* M1, const 
 	int rc = bm_demag_convolute(d, d->avg, d->hbuf, 1, energy, stream);
 	if (rc) return rc;
 	hipLaunchKernelGGL(k_dup_out, dim3(g), dim3(kBlock), 0, st, d->hbuf, H1, H2, clear_out ? 1 : 0, n3);
+	BM_HIP(hipGetLastError());
+	return BM_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Allocation-free averaged convolution.  The averaged input and, with a second output, the
+// field buffer live in `scratch`, a file-scope device array each GPU holds until the process
+// ends, instead of the handle's lazily allocated avg / hbuf: for callers that cannot call
+// hipMalloc (graph capture, serving loops).  Meshes up to kStaticCells cells.  Every call on
+// one GPU shares that GPU's array: calls that overlap on one GPU (other streams, other
+// threads) race.  The array is fully written before it is read by each call (k_avg_in writes
+// all 3n inputs, the set-mode convolution all 3n fields), so nothing needs clearing.
+constexpr size_t kStaticCells = (size_t)1 << 20;
+static __device__ __align__(256) float scratch[6 * kStaticCells];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[kMaxDevices] = {};   // device address of `scratch` on each GPU
+}
+
+// Device address of `scratch` on the current device, resolved once per device.  Host code
+// never uses the array's name as an address (on the host it is only a stand-in symbol).
+int bm::static_scratch(float** out)
+{
+	int dev = 0;
+	BM_HIP(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kMaxDevices) return fail(BM_ERR_STATE, "static scratch: device index out of range");
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if (!g_scratch_dev[dev]) {
+		void* ptr = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !ptr) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+		g_scratch_dev[dev] = (float*)ptr;
+	}
+	*out = g_scratch_dev[dev];
+	return BM_OK;
+}
+
+extern "C" int bm_demag_convolute_averaged_static(bm_demag_t d, const float* M1, const float* M2, float* H1, float* H2,
+	int clear_out, float* energy, void* stream)
+{
+	if (!d || !M1 || !M2 || !H1) return fail(BM_ERR_ARG, "bm_demag_convolute_averaged_static: null argument");
+	const size_t n = (size_t)d->nx * d->ny * d->nz, n3 = 3 * n;
+	if (n > kStaticCells)
+		return fail(BM_ERR_SIZE, "bm_demag_convolute_averaged_static: mesh of " + std::to_string(n) + " cells exceeds the static scratch ("
+			+ std::to_string(kStaticCells) + " cells)");
+	float* sc = nullptr;
+	int rc = static_scratch(&sc);
+	if (rc) return rc;
+	float* avg = sc;
+	float* hb = sc + n3;
+	hipStream_t st = (hipStream_t)stream;
+	const int g = (int)std::min<size_t>((n3 + kBlock - 1) / kBlock, (size_t)d->ncu * 8);
+	hipLaunchKernelGGL(k_avg_in, dim3(g), dim3(kBlock), 0, st, M1, M2, avg, n3);
+	if (!H2) return bm_demag_convolute(d, avg, H1, clear_out, energy, stream);
+	rc = bm_demag_convolute(d, avg, hb, 1, energy, stream);
+	if (rc) return rc;
+	hipLaunchKernelGGL(k_dup_out, dim3(g), dim3(kBlock), 0, st, hb, H1, H2, clear_out ? 1 : 0, n3);
 	BM_HIP(hipGetLastError());
 	return BM_OK;
 }

@@ -51,6 +51,9 @@ struct bm_demag_s {
 };
 
 namespace bm {
+// device address of the allocation-free averaged convolution's static scratch on the
+// current device (resolved with hipGetSymbolAddress once per device)
+int static_scratch(float** out);
 // H_demag(M) into planar output H3n (3 planes of n floats)
 int demag_planar(bm_demag_s* d, const float* M, float* H3n, hipStream_t st);
 // the convolution's passes, for the integrator's fused evaluation

@@ -79,6 +79,16 @@ int bm_demag_convolute(bm_demag_t d, const float* M, float* Heff, int clear_out,
  * DemagCUDA::UpdateField for antiferromagnetic meshes (Boris/DemagCUDA.cpp:125-129). */
 int bm_demag_convolute_averaged(bm_demag_t d, const float* M1, const float* M2, float* H1, float* H2,
 	int clear_out, float* energy, void* stream);
+/* bm_demag_convolute_averaged without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): bm_demag_convolute_averaged allocates its
+ * averaged-input and field buffers on first use, this variant keeps them in a static device
+ * array that the library holds on each GPU until the process ends (24 MiB), so it serves
+ * meshes of at most 1048576 cells (larger meshes return BM_ERR_SIZE).  Same arguments and
+ * results (bit for bit) as bm_demag_convolute_averaged.  All calls on one GPU share that
+ * GPU's array: calls that overlap on one GPU race. */
+int bm_demag_convolute_averaged_static(bm_demag_t d, const float* M1, const float* M2, float* H1, float* H2,
+	int clear_out, float* energy, void* stream);
 
 /* H_demag(M) written as three planes Hx[n], Hy[n], Hz[n] (the layout the fused LLG
  * stage kernels read).  No reference counterpart: the split the reference makes between
