@@ -69,6 +69,8 @@ struct bm_batch_s {
 	std::vector<int> lay_grouped, lay_field;   // stacks of 3D meshes: grouping and per-layer fields set
 	float* Msmap = nullptr;        // per-cell Ms of the stacks with per-layer Ms (ntot floats)
 	double* d_time = nullptr;      // step start time on the device (time-dependent fields)
+	float* d_Ha = nullptr;         // applied field of each mesh at its step time (3 floats each)
+	bool has_wave = false;         // some mesh has a time-dependent applied field
 	double* d_dT = nullptr;
 	float* M = nullptr;            // caller-owned, concatenated
 	float* Mb = nullptr;
@@ -85,6 +87,7 @@ struct bm_batch_s {
 	std::vector<Tab<PassBatch>> inv, ty, tz, ty2;
 	std::vector<Tab<MlBatch>> tml;
 	Tab<StageArgs> stage[2];               // [input parity]
+	Tab<int> choff;                        // first flat chunk of kBS cells of each mesh (nb + 1)
 	void* dev_tables = nullptr;
 	size_t dev_bytes = 0;
 	int ncu = 256;
@@ -110,6 +113,7 @@ void free_all(bm_batch_s* o)
 	for (bm_mlayer_t m : o->ml) if (m) bm_mlayer_destroy(m);
 	if (o->Msmap) (void)hipFree(o->Msmap);
 	if (o->d_time) (void)hipFree(o->d_time);
+	if (o->d_Ha) (void)hipFree(o->d_Ha);
 	if (o->d_laytab) (void)hipFree(o->d_laytab);
 	if (o->d_dT) (void)hipFree(o->d_dT);
 	if (o->Mb) (void)hipFree(o->Mb);
@@ -156,6 +160,7 @@ StageArgs stage_args(bm_batch_s* o, int b, int parity)
 	}
 	if ((p.modules & BM_MOD_ZEEMAN) && (p.wave == 1 || p.wave == 2)) {
 		a.tnow = o->d_ctl ? &o->d_ctl[b].time : o->d_time; a.wave = p.wave;
+		a.Ha_dev = o->d_Ha + 3 * b;
 		a.amp = p.amp; a.w = p.w; a.t0 = p.t0;
 		for (int c = 0; c < 3; c++) { a.Ha0[c] = p.Ha[c]; a.dir[c] = p.dir[c]; a.slope[c] = p.slope[c]; }
 	}
@@ -171,6 +176,19 @@ StageArgs stage_args(bm_batch_s* o, int b, int parity)
 }
 
 __global__ void k_time_advance(double* t, const double* dT) { *t += *dT; }
+
+// Applied field of every mesh with a time-dependent one at its current step time, one thread
+// per mesh, once per step (attempt), ahead of the stage kernels that read it.
+__global__ void k_batch_fields(const StageArgs* __restrict__ tab, int nb)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= nb) return;
+	const StageArgs& a = tab[b];
+	if (!a.tnow || !a.Ha_dev) return;
+	const F3 H = waveform_Ha(a);
+	float* o = const_cast<float*>(a.Ha_dev);
+	o[0] = H.x; o[1] = H.y; o[2] = H.z;
+}
 
 // RKDP54 step control after stage 0, one thread per mesh: the plain integrator's stage-0
 // branch (primed check, SetAdaptiveTimeStep, restore on rejection) on the device.  The new
@@ -212,20 +230,26 @@ __global__ void k_rkdp_control(BatchCtl* ctl, unsigned* lte, StageArgs* t0, Stag
 }
 
 // After the control: an accepted mesh advances (sM1 = M, M1 = M + E0 dT/5 into the second
-// array, which stage 1 reads); a rejected one restores M = sM1.  Grid rows are meshes.
-__global__ void __launch_bounds__(kBS) k_batch_rkdp_advance(const StageArgs* __restrict__ tab, const BatchCtl* __restrict__ ctl)
+// array, which stage 1 reads); a rejected one restores M = sM1.  One block per flat chunk of
+// kBS cells (3 kBS floats), mesh b owning chunks choff[b] .. choff[b + 1] - 1.
+__global__ void __launch_bounds__(kBS) k_batch_rkdp_advance(const StageArgs* __restrict__ tab, const BatchCtl* __restrict__ ctl,
+	const int* __restrict__ choff, int nb)
 {
-	const StageArgs& a = tab[blockIdx.y];
-	const BatchCtl& c = ctl[blockIdx.y];
+	const int ch = xcd_chunk(blockIdx.x, gridDim.x);
+	if (ch >= choff[nb]) return;
+	int lo = 0, hi = nb - 1;
+	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (choff[mid] <= ch) lo = mid; else hi = mid - 1; }
+	const StageArgs& a = tab[lo];
+	const BatchCtl& c = ctl[lo];
 	if (!c.active) return;
-	const size_t n3 = (size_t)a.nx * a.ny * a.nz * 3;
+	const size_t n3 = (size_t)a.nx * a.ny * a.nz * 3, q0 = (size_t)(ch - choff[lo]) * 3 * kBS + threadIdx.x;
 	float* M = const_cast<float*>(a.Min);
-	if (c.skip) {
-		for (size_t q = blockIdx.x * (size_t)kBS + threadIdx.x; q < n3; q += (size_t)gridDim.x * kBS) M[q] = a.sM1[q];
-		return;
-	}
 	const float dT5 = (float)c.dT / 5;
-	for (size_t q = blockIdx.x * (size_t)kBS + threadIdx.x; q < n3; q += (size_t)gridDim.x * kBS) {
+#pragma unroll
+	for (int r = 0; r < 3; r++) {
+		const size_t q = q0 + (size_t)r * kBS;
+		if (q >= n3) break;
+		if (c.skip) { M[q] = a.sM1[q]; continue; }
 		const float m = M[q];
 		a.sM1[q] = m;
 		a.Mout[q] = m + a.E[0][q] * dT5;
@@ -323,6 +347,8 @@ int build_tables(bm_batch_s* o)
 		o->stage[par].h.clear();
 		for (int b = 0; b < nb; b++) o->stage[par].h.push_back(stage_args(o, b, par));
 	}
+	o->choff.h.assign(1, 0);
+	for (int b = 0; b < nb; b++) o->choff.h.push_back(o->choff.h.back() + (int)((o->ncell[b] + kBS - 1) / kBS));
 	// one device allocation, 256-byte aligned pieces
 	size_t bytes = 0;
 	auto reserve = [&](size_t n) { size_t at = bytes; bytes += (n + 255) / 256 * 256; return at; };
@@ -340,7 +366,7 @@ int build_tables(bm_batch_s* o)
 	for (auto& t : o->tz) at_z.push_back(place(t));
 	for (auto& t : o->ty2) at_y2.push_back(place(t));
 	for (auto& t : o->tml) at_ml.push_back(place(t));
-	const size_t at_s0 = place(o->stage[0]), at_s1 = place(o->stage[1]);
+	const size_t at_s0 = place(o->stage[0]), at_s1 = place(o->stage[1]), at_ch = place(o->choff);
 	if (hipMalloc(&o->dev_tables, std::max<size_t>(bytes, 256)) != hipSuccess) return fail(BM_ERR_NOMEM, "batch tables");
 	o->dev_bytes = bytes;
 	char* base = (char*)o->dev_tables;
@@ -353,6 +379,7 @@ int build_tables(bm_batch_s* o)
 	for (size_t g = 0; g < o->tml.size(); g++) o->tml[g].d = (MlBatch*)(base + at_ml[g]);
 	o->stage[0].d = (StageArgs*)(base + at_s0);
 	o->stage[1].d = (StageArgs*)(base + at_s1);
+	o->choff.d = (int*)(base + at_ch);
 	return BM_OK;
 }
 
@@ -386,7 +413,7 @@ int eval_all(bm_batch_s* o, int step, int par, hipStream_t st)
 		rc = batch_y2d(o->gy2[g].size, o->gy2[g].sh[0], o->ty2[g].h.data(), o->ty2[g].d, (int)o->ty2[g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gx.size() && !rc; g++)
 		rc = batch_inv_x_planar(o->gx[g].size, o->gx[g].sh[1], o->inv[g].h.data(), o->inv[g].d, (int)o->inv[g].h.size(), o->ncu, st);
-	if (!rc) rc = batch_stage(o->mcode, step, o->stage[par].d, o->nb, o->max_cells, o->ncu, st);
+	if (!rc) rc = batch_stage(o->mcode, step, o->stage[par].d, o->nb, o->choff.d, o->choff.h.back(), st);
 	return rc;
 }
 
@@ -457,14 +484,17 @@ int autotune(bm_batch_s* o, hipStream_t st)
 
 int one_step(bm_batch_s* o, hipStream_t st)
 {
+	const int nbk = (o->nb + 63) / 64;
+	// time-dependent applied fields at each mesh's step time, once per step or attempt (the
+	// field is held at the step start time, so stages 1.. of an attempt see the same value)
+	if (o->has_wave) hipLaunchKernelGGL(k_batch_fields, dim3(nbk), dim3(64), 0, st, (const StageArgs*)o->stage[0].d, o->nb);
 	if (o->mcode == M_RKDP) {
-		const int nbk = (o->nb + 63) / 64;
 		int rc = eval_all(o, 0, 0, st);
 		if (rc) return rc;
 		hipLaunchKernelGGL(k_rkdp_control, dim3(nbk), dim3(64), 0, st, o->d_ctl, o->d_lte, o->stage[0].d, o->stage[1].d, o->nb);
-		const size_t nch = (o->max_cells * 3 + kBS - 1) / kBS;
-		const int gx = (int)std::max<size_t>(1, std::min<size_t>(nch, (size_t)o->ncu * 4));
-		hipLaunchKernelGGL(k_batch_rkdp_advance, dim3(gx, o->nb), dim3(kBS), 0, st, (const StageArgs*)o->stage[0].d, (const BatchCtl*)o->d_ctl);
+		const int nchunks = o->choff.h.back();
+		hipLaunchKernelGGL(k_batch_rkdp_advance, dim3((unsigned)(((std::max(nchunks, 1) + 7) / 8) * 8)), dim3(kBS), 0, st,
+			(const StageArgs*)o->stage[0].d, (const BatchCtl*)o->d_ctl, (const int*)o->choff.d, o->nb);
 		for (int s = 1; s < 6; s++) if ((rc = eval_all(o, s, s & 1, st))) return rc;
 		hipLaunchKernelGGL(k_rkdp_end, dim3(nbk), dim3(64), 0, st, o->d_ctl, o->nb);
 		return BM_OK;
@@ -577,6 +607,11 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 	if (rc == BM_OK && (hipMalloc(&o->d_time, sizeof(double)) != hipSuccess || hipMalloc(&o->d_dT, sizeof(double)) != hipSuccess
 		|| hipMemset(o->d_time, 0, sizeof(double)) != hipSuccess || hipMemcpy(o->d_dT, &o->dT, sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
 		rc = fail(BM_ERR_NOMEM, "bm_batch_create: time scalars");
+	for (int b = 0; b < nmesh; b++)
+		if ((o->p[b].modules & BM_MOD_ZEEMAN) && (o->p[b].wave == 1 || o->p[b].wave == 2)) o->has_wave = true;
+	if (rc == BM_OK && o->has_wave && (hipMalloc(&o->d_Ha, 3 * (size_t)nmesh * sizeof(float)) != hipSuccess
+		|| hipMemset(o->d_Ha, 0, 3 * (size_t)nmesh * sizeof(float)) != hipSuccess))
+		rc = fail(BM_ERR_NOMEM, "bm_batch_create: applied fields");
 	if (rc == BM_OK && any_lay_ms) {
 		std::vector<float> hm(o->ntot, 0.f);
 		for (int b = 0; b < nmesh; b++) {

@@ -24,13 +24,42 @@ namespace {
 
 using namespace bm;
 
-// One stage over a mesh (all the block's cells); `a` is the kernel argument or, batched, the
-// mesh's table entry read in place (a local copy of the large argument set would go to
-// scratch).
+// One cell of a stage sweep: its demag field (planar convolution output, or the evaluation
+// speedup's extrapolation / save) and the fused field + LLG + stage update.
+template <int METHOD, int STEP>
+__device__ __forceinline__ void stage_idx(const StageArgs& a, F3 Ha, size_t idx, size_t n, float& lte_loc)
+{
+	const int i = (int)(idx % a.nx), j = (int)((idx / a.nx) % a.ny), k = (int)(idx / ((size_t)a.nx * a.ny));
+	F3 Hd{ 0.f, 0.f, 0.f };
+	if (a.nsp || a.Hsave) {
+		// self coefficient of the cell: the mesh's, or its layer's in a layer stack
+		F3 sc{ a.selfc[0], a.selfc[1], a.selfc[2] };
+		if (a.layered) sc = F3{ a.lay->selfc[k][0], a.lay->selfc[k][1], a.lay->selfc[k][2] };
+		const F3 m = ld3(a.Min, idx);
+		if (a.nsp) {
+			// Demag_EvalSpeedup_AddExtrapField_AddSelf: saved fields in slot order, then self term
+			Hd = a.ca[0] * F3{ a.Hs[0][idx], a.Hs[0][n + idx], a.Hs[0][2 * n + idx] };
+#pragma unroll
+			for (int q = 1; q < 6; q++)
+				if (q < a.nsp) Hd = Hd + a.ca[q] * F3{ a.Hs[q][idx], a.Hs[q][n + idx], a.Hs[q][2 * n + idx] };
+			Hd = Hd + F3{ sc.x * m.x, sc.y * m.y, sc.z * m.z };
+		}
+		else {
+			// Demag_EvalSpeedup_AddField_SubSelf: keep the convolution minus the self term
+			Hd = F3{ a.Hd[idx], a.Hd[n + idx], a.Hd[2 * n + idx] };
+			a.Hsave[idx] = Hd.x - sc.x * m.x;
+			a.Hsave[n + idx] = Hd.y - sc.y * m.y;
+			a.Hsave[2 * n + idx] = Hd.z - sc.z * m.z;
+		}
+	}
+	else if (a.modules & BM_MOD_DEMAG) Hd = F3{ a.Hd[idx], a.Hd[n + idx], a.Hd[2 * n + idx] };
+	stage_cell<METHOD, STEP>(a, Ha, idx, i, j, k, Hd, lte_loc);
+}
+
+// One stage over a mesh (all the block's cells).
 template <int METHOD, int STEP>
 __device__ __forceinline__ void stage_sweep(const StageArgs& a)
 {
-	if (a.ctl && (STEP == 0 ? !a.ctl->active : a.ctl->skip)) return;   // uniform over the block
 	const F3 Ha = stage_Ha(a);
 	const size_t n = (size_t)a.nx * a.ny * a.nz;
 	const long long nchunk = (long long)((n + kBS - 1) / kBS);
@@ -45,39 +74,36 @@ __device__ __forceinline__ void stage_sweep(const StageArgs& a)
 		if (chunk >= nchunk) { if (a.map == 1) break; if (r * gridDim.x >= nchunk) break; continue; }
 		const size_t idx = (size_t)chunk * kBS + threadIdx.x;
 		if (idx >= n) continue;
-		const int i = (int)(idx % a.nx), j = (int)((idx / a.nx) % a.ny), k = (int)(idx / ((size_t)a.nx * a.ny));
-		F3 Hd{ 0.f, 0.f, 0.f };
-		if (a.nsp || a.Hsave) {
-			// self coefficient of the cell: the mesh's, or its layer's in a layer stack
-			F3 sc{ a.selfc[0], a.selfc[1], a.selfc[2] };
-			if (a.layered) sc = F3{ a.lay->selfc[k][0], a.lay->selfc[k][1], a.lay->selfc[k][2] };
-			const F3 m = ld3(a.Min, idx);
-			if (a.nsp) {
-				// Demag_EvalSpeedup_AddExtrapField_AddSelf: saved fields in slot order, then self term
-				Hd = a.ca[0] * F3{ a.Hs[0][idx], a.Hs[0][n + idx], a.Hs[0][2 * n + idx] };
-#pragma unroll
-				for (int q = 1; q < 6; q++)
-					if (q < a.nsp) Hd = Hd + a.ca[q] * F3{ a.Hs[q][idx], a.Hs[q][n + idx], a.Hs[q][2 * n + idx] };
-				Hd = Hd + F3{ sc.x * m.x, sc.y * m.y, sc.z * m.z };
-			}
-			else {
-				// Demag_EvalSpeedup_AddField_SubSelf: keep the convolution minus the self term
-				Hd = F3{ a.Hd[idx], a.Hd[n + idx], a.Hd[2 * n + idx] };
-				a.Hsave[idx] = Hd.x - sc.x * m.x;
-				a.Hsave[n + idx] = Hd.y - sc.y * m.y;
-				a.Hsave[2 * n + idx] = Hd.z - sc.z * m.z;
-			}
-		}
-		else if (a.modules & BM_MOD_DEMAG) Hd = F3{ a.Hd[idx], a.Hd[n + idx], a.Hd[2 * n + idx] };
-		stage_cell<METHOD, STEP>(a, Ha, idx, i, j, k, Hd, lte_loc);
+		stage_idx<METHOD, STEP>(a, Ha, idx, n, lte_loc);
 	}
 	if (stage_has_lte<METHOD, STEP>()) block_max_to(lte_loc, a.lte);
 }
 
+// Batched: one block per chunk of kBS cells over all meshes, mesh b owning the flat chunks
+// choff[b] .. choff[b + 1] - 1 (so no block is sized by another mesh); blocks take
+// XCD-contiguous chunks (gridDim.x a multiple of 8), so neighbouring chunks of a mesh, whose
+// rows the exchange stencil shares, land on one L2.  The mesh's arguments are read in place
+// from its table entry (a local copy of the large argument set would go to scratch).
 template <int METHOD, int STEP>
-__global__ void __launch_bounds__(kBS) k_stage(StageArgs a, const StageArgs* __restrict__ bt)
+__device__ __forceinline__ void stage_chunk(const StageArgs* __restrict__ bt, const int* __restrict__ choff, int nb)
 {
-	if (bt) stage_sweep<METHOD, STEP>(bt[blockIdx.y]);     // batched: one mesh per grid row
+	const int c = xcd_chunk(blockIdx.x, gridDim.x);
+	if (c >= choff[nb]) return;                       // uniform over the block
+	int lo = 0, hi = nb - 1;                          // last mesh with choff[mesh] <= c
+	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (choff[mid] <= c) lo = mid; else hi = mid - 1; }
+	const StageArgs& a = bt[lo];
+	if (a.ctl && (STEP == 0 ? !a.ctl->active : a.ctl->skip)) return;   // uniform over the block
+	const size_t n = (size_t)a.nx * a.ny * a.nz;
+	const size_t idx = (size_t)(c - choff[lo]) * kBS + threadIdx.x;
+	float lte_loc = 0.f;
+	if (idx < n) stage_idx<METHOD, STEP>(a, stage_Ha(a), idx, n, lte_loc);
+	if (stage_has_lte<METHOD, STEP>()) block_max_to(lte_loc, a.lte);
+}
+
+template <int METHOD, int STEP>
+__global__ void __launch_bounds__(kBS) k_stage(StageArgs a, const StageArgs* __restrict__ bt, const int* __restrict__ choff, int nb)
+{
+	if (bt) stage_chunk<METHOD, STEP>(bt, choff, nb);    // batched: flat chunks of all meshes
 	else stage_sweep<METHOD, STEP>(a);
 }
 
@@ -249,20 +275,20 @@ int launch_stage(bm_ode_s* o, int step, float dT, hipStream_t st)
 	dim3 g(stage_chunks(o)), b(kBS);
 	bm::prof_mark_begin(bm::PK_STAGE, st);
 	switch (o->mcode * 8 + step) {
-	case M_RKDP * 8 + 0: hipLaunchKernelGGL((k_stage<M_RKDP, 0>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
-	case M_RKDP * 8 + 1: hipLaunchKernelGGL((k_stage<M_RKDP, 1>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
-	case M_RKDP * 8 + 2: hipLaunchKernelGGL((k_stage<M_RKDP, 2>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
-	case M_RKDP * 8 + 3: hipLaunchKernelGGL((k_stage<M_RKDP, 3>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
-	case M_RKDP * 8 + 4: hipLaunchKernelGGL((k_stage<M_RKDP, 4>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
-	case M_RKDP * 8 + 5: hipLaunchKernelGGL((k_stage<M_RKDP, 5>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
-	case M_RK4 * 8 + 0: hipLaunchKernelGGL((k_stage<M_RK4, 0>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
-	case M_RK4 * 8 + 1: hipLaunchKernelGGL((k_stage<M_RK4, 1>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
-	case M_RK4 * 8 + 2: hipLaunchKernelGGL((k_stage<M_RK4, 2>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
-	case M_RK4 * 8 + 3: hipLaunchKernelGGL((k_stage<M_RK4, 3>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
-	case M_TEULER * 8 + 0: hipLaunchKernelGGL((k_stage<M_TEULER, 0>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
-	case M_TEULER * 8 + 1: hipLaunchKernelGGL((k_stage<M_TEULER, 1>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
-	case M_AHEUN * 8 + 0: hipLaunchKernelGGL((k_stage<M_AHEUN, 0>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
-	case M_AHEUN * 8 + 1: hipLaunchKernelGGL((k_stage<M_AHEUN, 1>), g, b, 0, st, a, (const StageArgs*)nullptr); break;
+	case M_RKDP * 8 + 0: hipLaunchKernelGGL((k_stage<M_RKDP, 0>), g, b, 0, st, a, (const StageArgs*)nullptr, (const int*)nullptr, 0); break;
+	case M_RKDP * 8 + 1: hipLaunchKernelGGL((k_stage<M_RKDP, 1>), g, b, 0, st, a, (const StageArgs*)nullptr, (const int*)nullptr, 0); break;
+	case M_RKDP * 8 + 2: hipLaunchKernelGGL((k_stage<M_RKDP, 2>), g, b, 0, st, a, (const StageArgs*)nullptr, (const int*)nullptr, 0); break;
+	case M_RKDP * 8 + 3: hipLaunchKernelGGL((k_stage<M_RKDP, 3>), g, b, 0, st, a, (const StageArgs*)nullptr, (const int*)nullptr, 0); break;
+	case M_RKDP * 8 + 4: hipLaunchKernelGGL((k_stage<M_RKDP, 4>), g, b, 0, st, a, (const StageArgs*)nullptr, (const int*)nullptr, 0); break;
+	case M_RKDP * 8 + 5: hipLaunchKernelGGL((k_stage<M_RKDP, 5>), g, b, 0, st, a, (const StageArgs*)nullptr, (const int*)nullptr, 0); break;
+	case M_RK4 * 8 + 0: hipLaunchKernelGGL((k_stage<M_RK4, 0>), g, b, 0, st, a, (const StageArgs*)nullptr, (const int*)nullptr, 0); break;
+	case M_RK4 * 8 + 1: hipLaunchKernelGGL((k_stage<M_RK4, 1>), g, b, 0, st, a, (const StageArgs*)nullptr, (const int*)nullptr, 0); break;
+	case M_RK4 * 8 + 2: hipLaunchKernelGGL((k_stage<M_RK4, 2>), g, b, 0, st, a, (const StageArgs*)nullptr, (const int*)nullptr, 0); break;
+	case M_RK4 * 8 + 3: hipLaunchKernelGGL((k_stage<M_RK4, 3>), g, b, 0, st, a, (const StageArgs*)nullptr, (const int*)nullptr, 0); break;
+	case M_TEULER * 8 + 0: hipLaunchKernelGGL((k_stage<M_TEULER, 0>), g, b, 0, st, a, (const StageArgs*)nullptr, (const int*)nullptr, 0); break;
+	case M_TEULER * 8 + 1: hipLaunchKernelGGL((k_stage<M_TEULER, 1>), g, b, 0, st, a, (const StageArgs*)nullptr, (const int*)nullptr, 0); break;
+	case M_AHEUN * 8 + 0: hipLaunchKernelGGL((k_stage<M_AHEUN, 0>), g, b, 0, st, a, (const StageArgs*)nullptr, (const int*)nullptr, 0); break;
+	case M_AHEUN * 8 + 1: hipLaunchKernelGGL((k_stage<M_AHEUN, 1>), g, b, 0, st, a, (const StageArgs*)nullptr, (const int*)nullptr, 0); break;
 	default: return bm::fail(BM_ERR_ARG, "bad stage");
 	}
 	bm::prof_mark_end(bm::PK_STAGE, st);
@@ -273,26 +299,25 @@ int launch_stage(bm_ode_s* o, int step, float dT, hipStream_t st)
 
 } // namespace
 
-// Batched stage launch: nb meshes, per-mesh StageArgs in d_tab; grid rows are meshes.
-int bm::batch_stage(int mcode, int step, const StageArgs* d_tab, int nb, size_t max_cells, int ncu, hipStream_t st)
+// Batched stage launch: nb meshes, per-mesh StageArgs in d_tab, mesh b owning the flat chunks
+// choff[b] .. choff[b + 1] - 1 of kBS cells (choff[nb] = nchunks); one block per chunk.
+int bm::batch_stage(int mcode, int step, const StageArgs* d_tab, int nb, const int* d_choff, int nchunks, hipStream_t st)
 {
-	size_t nch = (max_cells + kBS - 1) / kBS;
-	const int gx = (int)std::max<size_t>(8, std::min<size_t>(((nch + 7) / 8) * 8, (size_t)ncu * 16));
-	const dim3 g(gx, nb), b(kBS);
+	const dim3 g((unsigned)(((std::max(nchunks, 1) + 7) / 8) * 8)), b(kBS);
 	StageArgs a{};
 	switch (mcode * 8 + step) {
-	case M_RKDP * 8 + 0: hipLaunchKernelGGL((k_stage<M_RKDP, 0>), g, b, 0, st, a, d_tab); break;
-	case M_RKDP * 8 + 1: hipLaunchKernelGGL((k_stage<M_RKDP, 1>), g, b, 0, st, a, d_tab); break;
-	case M_RKDP * 8 + 2: hipLaunchKernelGGL((k_stage<M_RKDP, 2>), g, b, 0, st, a, d_tab); break;
-	case M_RKDP * 8 + 3: hipLaunchKernelGGL((k_stage<M_RKDP, 3>), g, b, 0, st, a, d_tab); break;
-	case M_RKDP * 8 + 4: hipLaunchKernelGGL((k_stage<M_RKDP, 4>), g, b, 0, st, a, d_tab); break;
-	case M_RKDP * 8 + 5: hipLaunchKernelGGL((k_stage<M_RKDP, 5>), g, b, 0, st, a, d_tab); break;
-	case M_RK4 * 8 + 0: hipLaunchKernelGGL((k_stage<M_RK4, 0>), g, b, 0, st, a, d_tab); break;
-	case M_RK4 * 8 + 1: hipLaunchKernelGGL((k_stage<M_RK4, 1>), g, b, 0, st, a, d_tab); break;
-	case M_RK4 * 8 + 2: hipLaunchKernelGGL((k_stage<M_RK4, 2>), g, b, 0, st, a, d_tab); break;
-	case M_RK4 * 8 + 3: hipLaunchKernelGGL((k_stage<M_RK4, 3>), g, b, 0, st, a, d_tab); break;
-	case M_TEULER * 8 + 0: hipLaunchKernelGGL((k_stage<M_TEULER, 0>), g, b, 0, st, a, d_tab); break;
-	case M_TEULER * 8 + 1: hipLaunchKernelGGL((k_stage<M_TEULER, 1>), g, b, 0, st, a, d_tab); break;
+	case M_RKDP * 8 + 0: hipLaunchKernelGGL((k_stage<M_RKDP, 0>), g, b, 0, st, a, d_tab, d_choff, nb); break;
+	case M_RKDP * 8 + 1: hipLaunchKernelGGL((k_stage<M_RKDP, 1>), g, b, 0, st, a, d_tab, d_choff, nb); break;
+	case M_RKDP * 8 + 2: hipLaunchKernelGGL((k_stage<M_RKDP, 2>), g, b, 0, st, a, d_tab, d_choff, nb); break;
+	case M_RKDP * 8 + 3: hipLaunchKernelGGL((k_stage<M_RKDP, 3>), g, b, 0, st, a, d_tab, d_choff, nb); break;
+	case M_RKDP * 8 + 4: hipLaunchKernelGGL((k_stage<M_RKDP, 4>), g, b, 0, st, a, d_tab, d_choff, nb); break;
+	case M_RKDP * 8 + 5: hipLaunchKernelGGL((k_stage<M_RKDP, 5>), g, b, 0, st, a, d_tab, d_choff, nb); break;
+	case M_RK4 * 8 + 0: hipLaunchKernelGGL((k_stage<M_RK4, 0>), g, b, 0, st, a, d_tab, d_choff, nb); break;
+	case M_RK4 * 8 + 1: hipLaunchKernelGGL((k_stage<M_RK4, 1>), g, b, 0, st, a, d_tab, d_choff, nb); break;
+	case M_RK4 * 8 + 2: hipLaunchKernelGGL((k_stage<M_RK4, 2>), g, b, 0, st, a, d_tab, d_choff, nb); break;
+	case M_RK4 * 8 + 3: hipLaunchKernelGGL((k_stage<M_RK4, 3>), g, b, 0, st, a, d_tab, d_choff, nb); break;
+	case M_TEULER * 8 + 0: hipLaunchKernelGGL((k_stage<M_TEULER, 0>), g, b, 0, st, a, d_tab, d_choff, nb); break;
+	case M_TEULER * 8 + 1: hipLaunchKernelGGL((k_stage<M_TEULER, 1>), g, b, 0, st, a, d_tab, d_choff, nb); break;
 	default: return bm::fail(BM_ERR_ARG, "batched stage: RKDP54, RK4 or TEuler");
 	}
 	return BM_OK;

@@ -117,8 +117,11 @@ struct StageArgs {
 	int lay_grouped, lay_field;
 	// time-dependent applied field read on the device (batched graphs): with tnow set, the
 	// stage kernel evaluates Ha0 + amp dir sinc(w (t - t0)) (wave 1) or Ha0 + slope t (wave 2)
-	// at t = *tnow in FP64 and rounds it to FP32, as the host does for a single mesh
+	// at t = *tnow in FP64 and rounds it to FP32, as the host does for a single mesh; a
+	// one-thread-per-mesh kernel does that once per step into Ha_dev (3 floats), which the
+	// stage kernels then read instead of Ha
 	const double* tnow;
+	const float* Ha_dev;
 	int wave;
 	double Ha0[3], amp, w, t0, dir[3], slope[3];
 	// demag evaluation speedup (Boris/DemagCUDA.cpp:106-468, kernels Boris/DemagCUDA.cu:44-413):
@@ -134,11 +137,17 @@ struct StageArgs {
 	const BatchCtl* ctl;
 };
 
-// Applied field of a stage: Ha, plus the time-dependent term at the device-held step time
-// when tnow is set (see StageArgs::tnow).
+// Applied field of a stage: Ha, or the mesh's device-held value when Ha_dev is set.
 __device__ __forceinline__ F3 stage_Ha(const StageArgs& a)
 {
-	if (!a.tnow) return F3{ a.Ha[0], a.Ha[1], a.Ha[2] };
+	if (a.Ha_dev) return F3{ a.Ha_dev[0], a.Ha_dev[1], a.Ha_dev[2] };
+	return F3{ a.Ha[0], a.Ha[1], a.Ha[2] };
+}
+
+// The time-dependent applied field at the device-held step time *a.tnow (sinc pulse or ramp
+// over the constant part), in FP64 rounded to FP32 (see StageArgs::tnow).
+__device__ __forceinline__ F3 waveform_Ha(const StageArgs& a)
+{
 	const double t = *a.tnow;
 	double H[3] = { a.Ha0[0], a.Ha0[1], a.Ha0[2] };
 	if (a.wave == 1) {
@@ -364,7 +373,7 @@ __device__ __forceinline__ P6 stage_pair(const StageArgs& a, size_t idx, int i0,
 }
 
 // Batched stage kernel over nb meshes (fixed-step methods), bm_ode.hip.
-int batch_stage(int mcode, int step, const StageArgs* d_tab, int nb, size_t max_cells, int ncu, hipStream_t st);
+int batch_stage(int mcode, int step, const StageArgs* d_tab, int nb, const int* d_choff, int nchunks, hipStream_t st);
 
 // Stages that produce an lte (reduced over the mesh to a max)
 template <int METHOD, int STEP> constexpr bool stage_has_lte() { return (METHOD == M_RKDP && STEP == 0) || (METHOD == M_AHEUN && STEP == 1); }
