@@ -80,6 +80,58 @@ def algorithmic_bytes(kind: str, n, N) -> float:
     }[kind]
 
 
+def conv_N(n):
+    """N of the zero-padded convolution of an open-boundary mesh whose nx and 2 ny have plans:
+    N.x = 2 nx, N.y = 2 ny, N.z = 1 (2D) or the next power of two >= 2 nz."""
+    nz = n[2]
+    Nz = 1
+    if nz > 1:
+        while Nz < 2 * nz:
+            Nz *= 2
+    return (2 * n[0], 2 * n[1], Nz)
+
+
+def unfused_eval_bytes(n) -> float:
+    """Algorithmic HBM bytes of one unfused field evaluation of a mesh: the convolution passes
+    (x R2C, the y / z / y passes or the 2D y pass, x C2R to planar H_demag) plus the stage
+    kernel's read of H_demag (12 B/cell)."""
+    N = conv_N(n)
+    cells = n[0] * n[1] * n[2]
+    mid = (algorithmic_bytes("y2d_fft_kernel_ifft", n, N) if n[2] == 1 else
+           algorithmic_bytes("y_fft", n, N) + algorithmic_bytes("z_fft_kernel_ifft", n, N) + algorithmic_bytes("y_ifft", n, N))
+    return algorithmic_bytes("fwd_x_r2c", n, N) + mid + algorithmic_bytes("inv_x_c2r", n, N) + 12.0 * cells
+
+
+def batched_roofline(meshes, evals: int, stage_array_passes: int, ms_per_step: float) -> dict:
+    """Whole-step roofline of a batched run: algorithmic bytes of one accepted step of every
+    mesh (evals unfused evaluations + the stage kernels' array passes of 12 B/cell) over the
+    measured time per step, against the 8 TB/s peak.  Rejected RKDP54 attempts are not counted."""
+    b = sum(evals * unfused_eval_bytes(n) + stage_array_passes * 12.0 * n[0] * n[1] * n[2] for n, *_ in meshes)
+    ach = b / (ms_per_step * 1e-3) / 1e9
+    return {"bound": "hbm", "alg_GB_per_step": round(b / 1e9, 4), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)}
+
+
+def host_info() -> dict:
+    """The host CPU model and thread count, and whether FFTW can be loaded (dlopen of
+    libfftw3.so.3): the oracle uses its own FFT either way."""
+    import ctypes
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        ctypes.CDLL("libfftw3.so.3")
+        fftw = "found (libfftw3.so.3 loads; the oracle still uses its own FFT)"
+    except OSError as e:
+        fftw = f"not found (dlopen libfftw3.so.3: {e})"
+    return {"cpu_model": model, "host_threads": os.cpu_count(), "fftw3": fftw}
+
+
 # rocprof kernel-name prefix of each launch kind (for the PMC traffic table in profiles/)
 KIND_TO_KERNEL = {"fwd_x_r2c": "k_fwd_x<512, 8, false>", "y_fft": "k_y<1024, -1>", "z_fft_kernel_ifft": "k_z<",
                   "y_ifft": "k_y<1024, 1>", "inv_x_c2r": "k_inv_x<", "llg_stage": "k_stage<", "rkdp_advance": "k_rkdp_advance",
@@ -226,6 +278,7 @@ def batch_run(steps: int):
     return {"meshes": len(meshes), "cells": cells, "steps": steps, "method": "RK4 fixed dt 50 fs",
             "batched_cell_updates_per_s": cells * steps / tb, "plain_cell_updates_per_s": cells * steps / tp,
             "batched_over_plain": tp / tb, "batched_ms_per_step": 1e3 * tb / steps, "plain_ms_per_step": 1e3 * tp / steps,
+            "roofline": batched_roofline(meshes, 4, 22, 1e3 * tb / steps),
             "autotune_s": round(t_tune, 2), "autotuned_passes": tuned}
 
 
@@ -265,7 +318,8 @@ def batch_rkdp_run(steps: int, warm: int = 5):
     return {"meshes": len(meshes), "cells": cells, "steps": steps, "method": "RKDP54 adaptive, per-mesh dT",
             "rejected_attempts": rej, "batched_cell_updates_per_s": cells * steps / tb,
             "plain_cell_updates_per_s": cells * steps / tp, "batched_over_plain": tp / tb,
-            "batched_ms_per_step": 1e3 * tb / steps, "plain_ms_per_step": 1e3 * tp / steps}
+            "batched_ms_per_step": 1e3 * tb / steps, "plain_ms_per_step": 1e3 * tp / steps,
+            "roofline": batched_roofline(meshes, 6, 52, 1e3 * tb / steps)}
 
 
 def smooth_unit_field(n, rng, kmax: int = 3) -> np.ndarray:
@@ -466,10 +520,12 @@ def cpu_baseline(cpu_steps: int):
     t0 = time.perf_counter()
     o.run_to_iteration(cpu_steps)
     dt = time.perf_counter() - t0
+    hi = host_info()
     return {"value": ncell * cpu_steps / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
             "sample": f"{cpu_steps} RKDP54 steps ({6 * cpu_steps} field evaluations) of the same 512x512x32 workload, "
-                      f"oracle FP64 with its own mixed-radix FFT (no FFTW on this host), OpenMP {cores} threads; "
-                      f"{dt:.1f} s timed, FP64 kernel setup {setup:.1f} s not included"}
+                      f"oracle FP64 with its own mixed-radix FFT, OpenMP {cores} threads; "
+                      f"{dt:.1f} s timed, FP64 kernel setup {setup:.1f} s not included",
+            **hi}
 
 
 def trilayer_run(steps: int):

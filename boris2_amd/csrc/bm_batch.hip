@@ -554,7 +554,7 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 	for (int b = 0; b < nmesh && rc == BM_OK; b++) {
 		const bm_layer_desc* ld = layers ? &layers[b] : nullptr;
 		if (ld && ld->nlayers > 0) {
-			if (ld->nlayers != o->nz[b] || ld->nlayers > kMaxStackLayers || !ld->hz || !ld->z0)
+			if (ld->nlayers != o->nz[b] || ld->nlayers > kMaxBatchStackLayers || !ld->hz || !ld->z0)
 				rc = fail(BM_ERR_ARG, "bm_batch_create_layers: a stack needs nz == nlayers <= 16 and its hz, z0");
 			else if (!(o->p[b].modules & BM_MOD_DEMAG))
 				rc = fail(BM_ERR_ARG, "bm_batch_create_layers: a layer stack needs the demag module");

@@ -28,7 +28,10 @@ namespace {
 
 using namespace bm;
 
-constexpr int kMaxLayers = 16;
+// Layers per handle: up to kRegLayers the product keeps every layer's spectra in registers
+// and works in place; above it a general product writes to a second spectra array.
+constexpr int kRegLayers = 16;
+constexpr int kMaxLayers = 64;
 
 // Position of grid index i (-1, 0, +1, ...) along one axis: i*h + o, or a table for -1, 0, +1.
 struct Axis { double h, o, p[3]; int table; };
@@ -184,6 +187,7 @@ __global__ void k_tensor_zshift(ZArgs a, double* D)
 // tr: the transposed operator (N^T)_sd = N_ds^T (the adjoint of the convolution): output s
 // sums the pair kernels K_ds over inputs d with the imaginary xz, yz parts negated (the
 // real-space kernel N_ds(-r)^T).
+// K holds one kernel of 6 planes per pair (d, s) at d * L + s.
 template <int L>
 __global__ void __launch_bounds__(256) k_ml_mult(float2* __restrict__ S, const float* __restrict__ K, int NC, int Ny, int Sp, int tr,
 	const MlBatch* __restrict__ bt)
@@ -225,16 +229,68 @@ __global__ void __launch_bounds__(256) k_ml_mult(float2* __restrict__ S, const f
 	}
 }
 
-int launch_ml_mult(int L, float2* S, const float* K, int NC, int Ny, int Sp, int ncu, hipStream_t st, int tr = 0)
+// More than kRegLayers layers: the same sums (own layer first, then the others in index order)
+// for DG destinations at a time, the sources read back from S for every group, the results
+// written to S2 (out of place, since later groups still read the inputs).  Kernels by slot:
+// kidx[d * L + s] is the slot of pair (d, s) (pairs of equal thicknesses and z shift share
+// one slot of 6 planes).
+constexpr int kDG = 8;
+__global__ void __launch_bounds__(256) k_ml_mult_gen(const float2* __restrict__ S, float2* __restrict__ S2, const float* __restrict__ K,
+	const int* __restrict__ kidx, int L, int NC, int Ny, int Sp, int tr)
+{
+	const int NYH = Ny / 2 + 1;
+	const size_t kq = (size_t)NC * NYH;
+	const int npts = NC * Ny;
+	const float tz = tr ? -1.f : 1.f;
+	for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < npts; q += gridDim.x * blockDim.x) {
+		const int ky = q % Ny, kx = q / Ny;
+		const bool up = ky > Ny / 2;
+		const int jj = up ? Ny - ky : ky;
+		const float sy = up ? -1.f : 1.f;
+		auto F = [&](int s, int c) { return S[((size_t)(c * L + s) * NC + kx) * Sp + ky]; };
+		for (int d0 = 0; d0 < L; d0 += kDG) {
+			float2 o[kDG][3];
+#pragma unroll
+			for (int g = 0; g < kDG; g++) for (int c = 0; c < 3; c++) o[g][c] = make_float2(0.f, 0.f);
+			auto acc = [&](int g, int d, int s) {
+				const float* Kp = K + (size_t)kidx[tr ? s * L + d : d * L + s] * 6 * kq + (size_t)kx * NYH + jj;
+				const float kxx = Kp[0], kyy = Kp[kq], kzz = Kp[2 * kq];
+				const float kxy = sy * Kp[3 * kq], kxz = tz * Kp[4 * kq], kyz = tz * sy * Kp[5 * kq];
+				const float2 fx = F(s, 0), fy = F(s, 1), fz = F(s, 2);
+				o[g][0].x += kxx * fx.x + kxy * fy.x - kxz * fz.y;  o[g][0].y += kxx * fx.y + kxy * fy.y + kxz * fz.x;
+				o[g][1].x += kxy * fx.x + kyy * fy.x - kyz * fz.y;  o[g][1].y += kxy * fx.y + kyy * fy.y + kyz * fz.x;
+				o[g][2].x += -kxz * fx.y - kyz * fy.y + kzz * fz.x; o[g][2].y += kxz * fx.x + kyz * fy.x + kzz * fz.y;
+			};
+#pragma unroll
+			for (int g = 0; g < kDG; g++) if (d0 + g < L) acc(g, d0 + g, d0 + g);
+			for (int s = 0; s < L; s++) {
+#pragma unroll
+				for (int g = 0; g < kDG; g++) if (d0 + g < L && s != d0 + g) acc(g, d0 + g, s);
+			}
+#pragma unroll
+			for (int g = 0; g < kDG; g++)
+				if (d0 + g < L)
+					for (int c = 0; c < 3; c++) S2[((size_t)(c * L + d0 + g) * NC + kx) * Sp + ky] = o[g][c];
+		}
+	}
+}
+
+// The kernel product; the result is in S (L <= kRegLayers, in place) or S2.
+int launch_ml_mult(int L, float2* S, float2* S2, const float* K, const int* kidx, int NC, int Ny, int Sp, int ncu, hipStream_t st, int tr = 0)
 {
 	const int npts = NC * Ny;
 	dim3 g((unsigned)std::max(1, std::min((npts + 255) / 256, ncu * 8))), b(256);
+	if (L > kRegLayers) {
+		if (!S2) return fail(BM_ERR_STATE, "multilayer demag: no second spectra array");
+		hipLaunchKernelGGL(k_ml_mult_gen, g, b, 0, st, S, S2, K, kidx, L, NC, Ny, Sp, tr);
+		return BM_OK;
+	}
 	switch (L) {
 #define C_(V_) case V_: hipLaunchKernelGGL((k_ml_mult<V_>), g, b, 0, st, S, K, NC, Ny, Sp, tr, (const MlBatch*)nullptr); return BM_OK;
 	C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8) C_(9) C_(10) C_(11) C_(12) C_(13) C_(14) C_(15) C_(16)
 #undef C_
 	}
-	return fail(BM_ERR_SIZE, "multilayer demag: at most 16 layers");
+	return fail(BM_ERR_SIZE, "multilayer demag: layer count");
 }
 
 } // namespace
@@ -245,10 +301,19 @@ struct bm_mlayer_s {
 	std::vector<double> hz, z0;
 	float2* Sx = nullptr;    // x-side spectra, row-tiled (bm_xpass.h): 3 * L * ntl * NC * kLT complex
 	float2* S = nullptr;     // full spectra S[c][layer][kx][ky]: 3 * L * NC * Ny complex
-	float* K = nullptr;      // kernels: L*L pairs x 6 planes x NC x (Ny/2+1), ky fastest
+	float2* S2 = nullptr;    // L > kRegLayers: the product's output (same layout), else null
+	// kernels, 6 planes of NC x (Ny/2+1) each, ky fastest: one per pair (d * L + s) for
+	// L <= kRegLayers; above, one per slot (pairs with equal thicknesses and z shift share one)
+	float* K = nullptr;
+	int nslot = 0;
+	std::vector<int> hkidx;  // slot of pair (d, s) at d * L + s
+	int* kidx = nullptr;     // its device copy (L > kRegLayers)
+	bool per_pair() const { return L <= kRegLayers; }
+	size_t kbase(int d, int s) const { return (size_t)(per_pair() ? d * L + s : hkidx[(size_t)d * L + s]); }
 	long long kcount = 0, bytes = 0;
 	int ncu = 256;
 	double* ebuf = nullptr;  // energy partials of the per-layer inverse (bm_mlayer_inverse)
+	float2* Sout() const { return S2 ? S2 : S; }
 };
 
 namespace {
@@ -331,12 +396,16 @@ int compute_ml_kernels(bm_mlayer_s* m, hipStream_t st)
 	double hmax = 0;
 	for (int l = 0; l < L; l++) hmax = std::max(hmax, std::max(m->hx, std::max(m->hy, m->hz[l])));
 	BM_HIP(hipMemsetAsync(m->K, 0, (size_t)m->kcount * sizeof(float), st));
-	// own-layer kernels: the single-layer 2D kernels (xx, yy, zz, xy)
+	// own-layer kernels: the single-layer 2D kernels (xx, yy, zz, xy), one per slot
+	std::vector<char> done(m->nslot, 0);
 	for (int l = 0; l < L; l++) {
+		const int slot = m->hkidx[(size_t)l * L + l];
+		if (done[slot]) continue;
+		done[slot] = 1;
 		bm_demag_t dl = nullptr;
 		int rc = bm_demag_create(m->nx, m->ny, 1, m->hx, m->hy, m->hz[l], 0, &dl);
 		if (rc) return rc;
-		float* dst = m->K + (size_t)(l * L + l) * 6 * kq;
+		float* dst = m->K + m->kbase(l, l) * 6 * kq;
 		hipError_t e = hipMemcpyAsync(dst, dl->K, 4 * kq * sizeof(float), hipMemcpyDeviceToDevice, st);
 		if (e == hipSuccess) e = hipStreamSynchronize(st);
 		bm_demag_destroy(dl);
@@ -351,8 +420,23 @@ int compute_ml_kernels(bm_mlayer_s* m, hipStream_t st)
 	for (int q = 0; q < 12 && ok; q++) ok = hipMalloc(&grids[q], gmax * sizeof(double)) == hipSuccess;
 	if (!ok) rc = fail(BM_ERR_NOMEM, "multilayer kernel setup");
 	for (int d = 0; d < L && rc == BM_OK; d++)
-		for (int s = 0; s < L && rc == BM_OK; s++)
-			if (d != s) rc = zshift_kernels(m, d, s, hmax, D, T1, grids, m->K + (size_t)(d * L + s) * 6 * kq, st);
+		for (int s = 0; s < L && rc == BM_OK; s++) {
+			const int slot = m->hkidx[(size_t)d * L + s];
+			if (d == s || done[slot]) continue;
+			done[slot] = 1;
+			rc = zshift_kernels(m, d, s, hmax, D, T1, grids, m->K + m->kbase(d, s) * 6 * kq, st);
+		}
+	// per-pair storage: pairs that share a slot take a copy of its first pair's kernel
+	if (rc == BM_OK && m->per_pair()) {
+		std::vector<int> first(m->nslot, -1);
+		for (int pr = 0; pr < L * L; pr++) {
+			const int slot = m->hkidx[pr];
+			if (first[slot] < 0) { first[slot] = pr; continue; }
+			const hipError_t e = hipMemcpyAsync(m->K + (size_t)pr * 6 * kq, m->K + (size_t)first[slot] * 6 * kq, 6 * kq * sizeof(float),
+				hipMemcpyDeviceToDevice, st);
+			if (e != hipSuccess) { rc = hip_fail(e, "multilayer kernel copies"); break; }
+		}
+	}
 	if (rc == BM_OK) { hipError_t e = hipStreamSynchronize(st); if (e != hipSuccess) rc = hip_fail(e, "multilayer kernel setup"); }
 	(void)hipStreamSynchronize(st);
 	if (D) (void)hipFree(D);
@@ -370,7 +454,7 @@ extern "C" int bm_mlayer_create(int nlayers, int nx, int ny, double hx, double h
 	(void)flags;
 	if (!out || !hz || !z0 || nlayers < 1 || nx < 2 || ny < 1 || !(hx > 0) || !(hy > 0)) return fail(BM_ERR_ARG, "bm_mlayer_create: bad arguments");
 	*out = nullptr;
-	if (nlayers > kMaxLayers) return fail(BM_ERR_SIZE, "bm_mlayer_create: at most 16 layers");
+	if (nlayers > kMaxLayers) return fail(BM_ERR_SIZE, "bm_mlayer_create: at most 64 layers");
 	for (int l = 0; l < nlayers; l++) if (!(hz[l] > 0)) return fail(BM_ERR_ARG, "bm_mlayer_create: layer thickness must be positive");
 	if (nx % 2 || !bm::plan_2d_ok(nx, 2 * ny)) return fail(BM_ERR_SIZE, "bm_mlayer_create: no FFT plan for this layer size");
 	bm_mlayer_s* m = new bm_mlayer_s();
@@ -380,14 +464,36 @@ extern "C" int bm_mlayer_create(int nlayers, int nx, int ny, double hx, double h
 	m->ntl = bm::sx_tiles(ny);
 	const size_t sbytes = (size_t)3 * nlayers * m->NC * m->Sp * sizeof(float2);
 	const size_t sxbytes = (size_t)3 * nlayers * m->ntl * m->NC * bm::kLT * sizeof(float2);
-	m->kcount = (long long)nlayers * nlayers * 6 * m->NC * m->NYH;
+	// kernel slots: a pair's kernel depends on the two thicknesses and the z shift only (the
+	// lateral cells are common), so pairs that agree in them (to 1e-12 relative) share one
+	m->hkidx.assign((size_t)nlayers * nlayers, -1);
+	struct Key { int self; double hd, hs, sh; };
+	std::vector<Key> keys;
+	auto same = [](double a, double b) { return std::fabs(a - b) <= 1e-12 * std::max(std::fabs(a), std::fabs(b)); };
+	for (int d = 0; d < nlayers; d++)
+		for (int s = 0; s < nlayers; s++) {
+			const Key k{ d == s, hz[d], hz[s], d == s ? 0.0 : z0[d] - z0[s] };
+			int slot = -1;
+			for (size_t q = 0; q < keys.size() && slot < 0; q++)
+				if (keys[q].self == k.self && same(keys[q].hd, k.hd) && same(keys[q].hs, k.hs) && same(keys[q].sh, k.sh)) slot = (int)q;
+			if (slot < 0) { slot = (int)keys.size(); keys.push_back(k); }
+			m->hkidx[(size_t)d * nlayers + s] = slot;
+		}
+	m->nslot = (int)keys.size();
+	m->kcount = (long long)(m->per_pair() ? nlayers * nlayers : m->nslot) * 6 * m->NC * m->NYH;
 	if (hipMalloc(&m->Sx, sxbytes) != hipSuccess || hipMalloc(&m->S, sbytes) != hipSuccess
+		|| (nlayers > kRegLayers && hipMalloc(&m->S2, sbytes) != hipSuccess)
 		|| hipMalloc(&m->K, (size_t)m->kcount * sizeof(float)) != hipSuccess
+		|| (nlayers > kRegLayers && hipMalloc(&m->kidx, m->hkidx.size() * sizeof(int)) != hipSuccess)
 		|| hipMalloc(&m->ebuf, (size_t)bm::x_plane_parts(nx, ny) * sizeof(double)) != hipSuccess) {
 		bm_mlayer_destroy(m);
 		return fail(BM_ERR_NOMEM, "bm_mlayer_create: device memory");
 	}
-	m->bytes = (long long)(sbytes + sxbytes) + m->kcount * 4;
+	if (m->kidx && hipMemcpy(m->kidx, m->hkidx.data(), m->hkidx.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+		bm_mlayer_destroy(m);
+		return fail(BM_ERR_HIP, "bm_mlayer_create: pair table");
+	}
+	m->bytes = (long long)(sbytes * (m->S2 ? 2 : 1) + sxbytes) + m->kcount * 4;
 	int rc = compute_ml_kernels(m, nullptr);
 	if (rc) { bm_mlayer_destroy(m); return rc; }
 	*out = m;
@@ -399,7 +505,9 @@ extern "C" int bm_mlayer_destroy(bm_mlayer_t m)
 	if (!m) return BM_OK;
 	if (m->Sx) (void)hipFree(m->Sx);
 	if (m->S) (void)hipFree(m->S);
+	if (m->S2) (void)hipFree(m->S2);
 	if (m->K) (void)hipFree(m->K);
+	if (m->kidx) (void)hipFree(m->kidx);
 	if (m->ebuf) (void)hipFree(m->ebuf);
 	delete m;
 	return BM_OK;
@@ -414,10 +522,10 @@ static int ml_convolute(bm_mlayer_s* m, const float* M, float* out, int mode, hi
 	if (rc == BM_OK) rc = bm::pass_y(m->Ny, -1, m->S, m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st);
 	if (rc == BM_OK) {
 		bm::prof_mark_begin(bm::PK_ML_MULT, st);
-		rc = launch_ml_mult(L, m->S, m->K, m->NC, m->Ny, m->Sp, m->ncu, st, tr);
+		rc = launch_ml_mult(L, m->S, m->S2, m->K, m->kidx, m->NC, m->Ny, m->Sp, m->ncu, st, tr);
 		bm::prof_mark_end(bm::PK_ML_MULT, st);
 	}
-	if (rc == BM_OK) rc = bm::pass_y(m->Ny, 1, m->S, m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st);
+	if (rc == BM_OK) rc = bm::pass_y(m->Ny, 1, m->Sout(), m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st);
 	const float scale = (float)(1.0 / ((double)m->Nx * m->Ny));
 	if (rc == BM_OK) rc = bm::pass_inv_x(m->nx, mode, m->Sx, out, m->ny, L, m->ntl, scale, m->ncu, st);
 	return rc;
@@ -448,7 +556,7 @@ int bm::batch_ml_mult(int L, LaunchShape sh, const MlBatch* h_tab, const MlBatch
 	C_(1) C_(2) C_(3) C_(4) C_(5) C_(6) C_(7) C_(8) C_(9) C_(10) C_(11) C_(12) C_(13) C_(14) C_(15) C_(16)
 #undef C_
 	}
-	return fail(BM_ERR_SIZE, "multilayer demag: at most 16 layers");
+	return fail(BM_ERR_SIZE, "batched multilayer demag: at most 16 layers per stack");
 }
 
 void bm::mlayer_dims(const bm_mlayer_s* m, int* L, int* nx, int* ny, double* hz)
@@ -468,8 +576,12 @@ extern "C" int bm_mlayer_convolute(bm_mlayer_t m, const float* M, float* Heff, i
 
 extern "C" int bm_mlayer_copy_kernels(bm_mlayer_t m, float* host_out, long long count)
 {
-	if (!m || !host_out || count < m->kcount) return fail(BM_ERR_ARG, "bm_mlayer_copy_kernels: bad arguments");
-	BM_HIP(hipMemcpy(host_out, m->K, (size_t)m->kcount * sizeof(float), hipMemcpyDeviceToHost));
+	const size_t per = (size_t)6 * m->NC * m->NYH;
+	if (!m || !host_out || count < (long long)((size_t)m->L * m->L * per)) return fail(BM_ERR_ARG, "bm_mlayer_copy_kernels: bad arguments");
+	// pair by pair (d * L + s), each from its slot
+	for (int d = 0; d < m->L; d++)
+		for (int s = 0; s < m->L; s++)
+			BM_HIP(hipMemcpy(host_out + ((size_t)d * m->L + s) * per, m->K + m->kbase(d, s) * per, per * sizeof(float), hipMemcpyDeviceToHost));
 	return BM_OK;
 }
 
@@ -498,7 +610,7 @@ extern "C" int bm_mlayer_multiply(bm_mlayer_t m, void* stream)
 	if (!m) return fail(BM_ERR_ARG, "bm_mlayer_multiply: null handle");
 	hipStream_t st = (hipStream_t)stream;
 	bm::prof_mark_begin(bm::PK_ML_MULT, st);
-	int rc = launch_ml_mult(m->L, m->S, m->K, m->NC, m->Ny, m->Sp, m->ncu, st);
+	int rc = launch_ml_mult(m->L, m->S, m->S2, m->K, m->kidx, m->NC, m->Ny, m->Sp, m->ncu, st);
 	bm::prof_mark_end(bm::PK_ML_MULT, st);
 	if (rc) return rc;
 	BM_HIP(hipGetLastError());
@@ -512,7 +624,7 @@ extern "C" int bm_mlayer_inverse(bm_mlayer_t m, int layer, const float* M_layer,
 	int rc = BM_OK;
 	for (int c = 0; c < 3 && rc == BM_OK; c++) {
 		const size_t cz = (size_t)c * m->L + layer;
-		rc = bm::pass_y(m->Ny, 1, m->S + cz * m->NC * m->Sp, m->Sx + cz * m->ntl * m->NC * bm::kLT, m->NC, m->Sp, m->NC, m->ntl, m->ncu, st);
+		rc = bm::pass_y(m->Ny, 1, m->Sout() + cz * m->NC * m->Sp, m->Sx + cz * m->ntl * m->NC * bm::kLT, m->NC, m->Sp, m->NC, m->ntl, m->ncu, st);
 	}
 	const float scale = (float)(1.0 / ((double)m->Nx * m->Ny));
 	if (rc == BM_OK)

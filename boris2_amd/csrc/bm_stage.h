@@ -63,7 +63,9 @@ __device__ __forceinline__ P6 renorm(P6 v, float2 Ms) { return P6{ renorm(v.a, M
 __device__ __forceinline__ float rel_err(F3 m, F3 p) { return norm(m - p) / norm(m); }
 __device__ __forceinline__ float rel_err(P6 m, P6 p) { return fmaxf(rel_err(m.a, p.a), rel_err(m.b, p.b)); }
 
-constexpr int kMaxStackLayers = 16;
+constexpr int kMaxStackLayers = 64;
+// a batched layer stack keeps every layer's spectra in registers in its kernel product
+constexpr int kMaxBatchStackLayers = 16;
 
 // Per-layer values of a layer stack, kept in device memory (one table per stack, written at
 // configuration) and indexed by the cell's layer, so the stage arguments stay small.

@@ -102,7 +102,7 @@ int bm_demag_convolute_planar(bm_demag_t d, const float* M, float* H3n, void* st
 int bm_demag_copy_kernels(bm_demag_t d, float* host_out, long long count);
 
 /* ------------------------------------------------------------------------------------
- * Multilayered demag of stacked 2D layers: L (<= 16) layers of the same nx x ny cells
+ * Multilayered demag of stacked 2D layers: L (<= 64) layers of the same nx x ny cells
  * (hx, hy, hz[l]), layer l's bottom face at height z0[l] (m); layer thicknesses may differ.
  * M and H are L layers of nx*ny cells stacked (layer-major, AoS per cell).
  *
@@ -183,7 +183,7 @@ int bm_ode_destroy(bm_ode_t o);
 int bm_ode_set_ms_map(bm_ode_t o, const float* Ms_map);
 int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p);
 /* Layer stack: the mesh's nz planes are the layers of the multilayered demag ml (same nx,
- * ny, nz == layer count; at most 16), as the reference's one-ferromagnetic-mesh-per-layer
+ * ny, nz == layer count; at most 64), as the reference's one-ferromagnetic-mesh-per-layer
  * stack coupled by SDemag (Boris/SDemag.cpp:173-300, SDemagCUDA::UpdateField;
  * Boris/Exchange.cpp:108, Boris/Anisotropy.cpp:82 and Boris/DiffEqFM_Equations.cpp:12 per
  * layer mesh).  Exchange acts within each layer; Ms, A, alpha, K1 may be given per layer

@@ -139,20 +139,24 @@ struct AdjRK4 {
 		}
 	}
 
-	// forward rollout from M0 = Ms * m0; returns the final M, keeps the tape
-	void forward(const double* m0, double* Mf)
+	// Checkpointing (0: keep the whole tape): with K > 0 the forward keeps M before every
+	// K-th step only and the backward recomputes one segment's tape at a time from its
+	// checkpoint, with the same arithmetic, so the gradients are the same bits.
+	int ckK = 0;
+	std::vector<std::vector<double>> ck;
+
+	// steps st0 .. st1-1 from M (advanced in place); with tape, X and Y of step st go to
+	// index st - st0 of the tape arrays
+	void run_steps(std::vector<double>& M, int st0, int st1, bool tape)
 	{
-		std::vector<double> M(3 * n), H(3 * n), lap(3 * n), k(3 * n), acc(3 * n);
-		for (size_t i = 0; i < n; i++) for (int c = 0; c < 3; c++) M[i * 3 + c] = Ms[i] * m0[i * 3 + c];
-		for (int s = 0; s < 4; s++) X[s].assign(nsteps, {});
-		Y.assign(nsteps, {});
+		std::vector<double> H(3 * n), lap(3 * n), k(3 * n), acc(3 * n);
 		static const double cst[4] = { 0.5, 0.5, 1.0, 0.0 }, wgt[4] = { 1, 2, 2, 1 };
-		for (int st = 0; st < nsteps; st++) {
+		for (int st = st0; st < st1; st++) {
 			const double t = st * dt;
 			std::vector<double> Xs = M;
 			std::fill(acc.begin(), acc.end(), 0.0);
 			for (int s = 0; s < 4; s++) {
-				X[s][st] = Xs;
+				if (tape) X[s][st - st0] = Xs;
 				field(Xs.data(), t, H, lap);
 				rhs(Xs.data(), H.data(), k.data());
 				for (size_t q = 0; q < 3 * n; q++) acc[q] += wgt[s] * k[q];
@@ -160,10 +164,29 @@ struct AdjRK4 {
 			}
 			std::vector<double> Yv(3 * n);
 			for (size_t q = 0; q < 3 * n; q++) Yv[q] = M[q] + (dt / 6) * acc[q];
-			Y[st] = Yv;
 			for (size_t i = 0; i < n; i++) {
 				const double nr = std::sqrt(dot(&Yv[i * 3], &Yv[i * 3]));
 				for (int c = 0; c < 3; c++) M[i * 3 + c] = Ms[i] * Yv[i * 3 + c] / nr;
+			}
+			if (tape) Y[st - st0] = std::move(Yv);
+		}
+	}
+
+	// forward rollout from M0 = Ms * m0; returns the final M, keeps the tape (or checkpoints)
+	void forward(const double* m0, double* Mf)
+	{
+		std::vector<double> M(3 * n);
+		for (size_t i = 0; i < n; i++) for (int c = 0; c < 3; c++) M[i * 3 + c] = Ms[i] * m0[i * 3 + c];
+		if (ckK <= 0 || ckK >= nsteps) {
+			for (int s = 0; s < 4; s++) X[s].assign(nsteps, {});
+			Y.assign(nsteps, {});
+			run_steps(M, 0, nsteps, true);
+		}
+		else {
+			ck.clear();
+			for (int st0 = 0; st0 < nsteps; st0 += ckK) {
+				ck.push_back(M);
+				run_steps(M, st0, std::min(nsteps, st0 + ckK), false);
 			}
 		}
 		for (size_t q = 0; q < 3 * n; q++) Mf[q] = M[q];
@@ -227,9 +250,21 @@ struct AdjRK4 {
 		std::vector<double> lam(3 * n), lamY(3 * n), w(3 * n), lx(3 * n), lacc(3 * n);
 		for (size_t i = 0; i < n; i++) gMs[i] = 0;
 		for (int c = 0; c < 4; c++) gz[c] = 0;
+		const bool seg = ckK > 0 && ckK < nsteps;
+		// tape index of step st: st itself, or st - st0 of the segment recomputed last
+		int base = 0;
+		auto recompute = [&](int st) {
+			const int sg = st / ckK, st0 = sg * ckK, st1 = std::min(nsteps, st0 + ckK);
+			for (int s = 0; s < 4; s++) X[s].assign(st1 - st0, {});
+			Y.assign(st1 - st0, {});
+			std::vector<double> M = ck[sg];
+			run_steps(M, st0, st1, true);
+			base = st0;
+		};
+		if (seg) recompute(nsteps - 1);
 		// final state M_N = Ms * Y/|Y| of the last step; dL/dM_N and the explicit Ms term
 		{
-			const std::vector<double>& Yv = Y[nsteps - 1];
+			const std::vector<double>& Yv = Y[nsteps - 1 - base];
 			for (size_t i = 0; i < n; i++) {
 				const double nr = std::sqrt(dot(&Yv[i * 3], &Yv[i * 3]));
 				double Mi[3];
@@ -241,7 +276,8 @@ struct AdjRK4 {
 		static const double cst[4] = { 0.5, 0.5, 1.0, 0.0 }, wgt[4] = { 1, 2, 2, 1 };
 		for (int st = nsteps - 1; st >= 0; st--) {
 			const double t = st * dt;
-			const std::vector<double>& Yv = Y[st];
+			if (seg && st < base) recompute(st);
+			const std::vector<double>& Yv = Y[st - base];
 			// renormalisation M' = Ms Y/|Y|
 			for (size_t i = 0; i < n; i++) {
 				const double nr = std::sqrt(dot(&Yv[i * 3], &Yv[i * 3]));
@@ -256,7 +292,7 @@ struct AdjRK4 {
 			for (int s = 3; s >= 0; s--) {
 				for (size_t q = 0; q < 3 * n; q++) w[q] = (dt / 6) * wgt[s] * lamY[q] + (s < 3 ? cst[s] * dt * lamXnext[q] : 0.0);
 				std::fill(lx.begin(), lx.end(), 0.0);
-				vjp(X[s][st].data(), t, w.data(), lx.data(), gMs, gz);
+				vjp(X[s][st - base].data(), t, w.data(), lx.data(), gMs, gz);
 				// X_s = M + c_{s-1} dt k_{s-1} (s > 0) or X_0 = M
 				for (size_t q = 0; q < 3 * n; q++) lacc[q] += lx[q];
 				lamXnext = lx;

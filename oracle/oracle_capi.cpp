@@ -193,6 +193,9 @@ void* or_adj_create(int nx, int ny, int nz, const double* h, const double* prm, 
 	catch (...) { return nullptr; }
 }
 void or_adj_destroy(void* h) { delete (AdjRK4*)h; }
+
+// keep M every K steps only and recompute segments in the backward pass (0: whole tape)
+void or_adj_set_checkpoint(void* h, int K) { ((AdjRK4*)h)->ckK = K; }
 // forward from M0 = Ms m0 (m0: n*3); final M into Mf; returns the loss against mt
 double or_adj_forward(void* h, const double* m0, const double* mt, double* Mf)
 {

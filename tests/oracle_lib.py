@@ -73,6 +73,7 @@ def lib() -> C.CDLL:
     L.or_mlayer_convolute_transposed.argtypes = [vp, dp, dp]
     L.or_adj_create.argtypes = [i, i, i, dp, dp, i, dp, d, i, i]; L.or_adj_create.restype = vp
     L.or_adj_destroy.argtypes = [vp]
+    L.or_adj_set_checkpoint.argtypes = [vp, i]
     L.or_adj_forward.argtypes = [vp, dp, dp, dp]; L.or_adj_forward.restype = d
     L.or_adj_backward.argtypes = [vp, dp, dp, dp, dp]
     L.or_adj_set_layers.argtypes = [vp, i, d, d, dp, dp, i, vp, vp, vp]
@@ -331,7 +332,8 @@ class OracleAdjoint:
     """FP64 discrete adjoint of the fixed-step RK4 rollout: loss 1 - <(M/Ms) . m_target> of
     the final state from M0 = Ms m0; gradients dL/dMs (per cell) and dL/d amp."""
 
-    def __init__(self, n, h, params, Ms_map, dt, nsteps, Nz_override=0):
+    def __init__(self, n, h, params, Ms_map, dt, nsteps, Nz_override=0, ckpt=0):
+        """ckpt > 0: keep M every ckpt steps and recompute segments in backward() (same bits)."""
         self.L = lib()
         self.n = tuple(n)
         self.ncell = n[0] * n[1] * n[2]
@@ -339,6 +341,8 @@ class OracleAdjoint:
         self.h = self.L.or_adj_create(*n, np.array(h, float), prm, flags, np.ascontiguousarray(Ms_map, float).reshape(-1),
                                       float(dt), int(nsteps), int(Nz_override))
         assert self.h
+        if ckpt:
+            self.L.or_adj_set_checkpoint(self.h, int(ckpt))
 
     def forward(self, m0, mt):
         Mf = np.zeros(self.ncell * 3)

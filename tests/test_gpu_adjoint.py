@@ -40,9 +40,11 @@ def test_gpu_adjoint_matches_oracle(gpu, ckpt):
     g = g.cpu().numpy().astype(np.float64)
     eg, ea = np.linalg.norm(g - g0) / np.linalg.norm(g0), abs(ga - ga0) / abs(ga0)
     print(f"ckpt {ckpt}: loss {abs(L - L0):.2e}, dL/dMs rel L2 {eg:.2e}, dL/damp rel {ea:.2e}")
-    assert abs(L - L0) <= 1e-5
-    assert eg <= 2e-3
-    assert ea <= 2e-3
+    # measured 5.5e-6 (dL/dMs) and 4.8e-6 (dL/d amp) after 40 steps: the FP32 forward's own
+    # departure from the FP64 trajectory (1.6e-6 relative) carried into the gradient
+    assert abs(L - L0) <= 1e-6
+    assert eg <= 5e-5
+    assert ea <= 5e-5
     r.close()
 
 
@@ -96,9 +98,9 @@ def test_gpu_layer_stack_adjoint_matches_oracle(gpu):
     g = g.cpu().numpy().astype(np.float64)
     eg = np.linalg.norm(g - g0) / np.linalg.norm(g0)
     print(f"stack: loss {abs(L - L0):.2e}, dL/dMs rel L2 {eg:.2e}, slopes {gz[1:]} vs {gz0[1:]}")
-    assert abs(L - L0) <= 1e-5
-    assert eg <= 2e-3
-    np.testing.assert_allclose(gz[1:], gz0[1:], rtol=2e-3, atol=2e-3 * np.abs(gz0[1:]).max())
+    assert abs(L - L0) <= 1e-6
+    assert eg <= 5e-5
+    np.testing.assert_allclose(gz[1:], gz0[1:], rtol=5e-5, atol=5e-5 * np.abs(gz0[1:]).max())
     r.close()
 
 

@@ -19,6 +19,11 @@ CASES = [
     (64, 64, 4e-9, 4e-9, [8e-9, 4e-9, 16e-9], [0.0, 12e-9, 20e-9]),
     # one layer: the plain 2D convolution
     (32, 16, 4e-9, 4e-9, [3e-9], [0.0]),
+    # 24 layers (above the 16 the register product holds: the general product, out of place),
+    # uneven thicknesses and spacings with repeats, so pairs share kernel slots
+    (16, 12, 4e-9, 4e-9, [2e-9, 4e-9, 3e-9] * 8, list(np.cumsum([0.0] + [3e-9, 5e-9, 4e-9] * 7 + [3e-9, 5e-9]))),
+    # 20 equal touching layers: a regular stack, 39 distinct kernels for 400 pairs
+    (24, 8, 4e-9, 4e-9, [4e-9] * 20, [4e-9 * k for k in range(20)]),
 ]
 
 
@@ -26,7 +31,7 @@ def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("case", CASES, ids=["irregular3", "regular2", "bench3_64", "single"])
+@pytest.mark.parametrize("case", CASES, ids=["irregular3", "regular2", "bench3_64", "single", "uneven24", "regular20"])
 def test_mlayer_field_matches_oracle(gpu, case):
     nx, ny, hx, hy, hz, z0 = case
     ncell = len(hz) * nx * ny
@@ -67,7 +72,7 @@ def test_mlayer_zshift_kernels_match_oracle(gpu):
             assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("case", CASES[:3], ids=["irregular3", "regular2", "bench3_64"])
+@pytest.mark.parametrize("case", CASES[:3] + CASES[4:5], ids=["irregular3", "regular2", "bench3_64", "uneven24"])
 def test_mlayer_split_per_layer_calls_equal_monolithic(gpu, case):
     """Forward per layer array, one product, inverse per layer array (the reference's per-mesh
     ForwardFFT / KernelMultiplication_MultipleInputs / InverseFFT): the same field as the
