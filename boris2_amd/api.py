@@ -134,6 +134,55 @@ class Demag:
             pass
 
 
+class MultiLayerMeshDemag:
+    """Multilayered demag of 2D layers that are meshes of their own: meshes is a list of
+    (nx, ny, hx, hy, hz, x0, y0, z0) per layer (cells, cell size, bottom-left corner in m).
+    M and H hold every layer's own cells concatenated, layer by layer ((sum nx*ny, 3) float32
+    CUDA); convolute_split takes one (nx*ny, 3) array per layer."""
+
+    def __init__(self, meshes):
+        require_device()
+        self.L = _lib.load()
+        self.meshes = [tuple(m) for m in meshes]
+        arr = (_lib.LayerMesh * len(self.meshes))()
+        for k, (nx, ny, hx, hy, hz, x0, y0, z0) in enumerate(self.meshes):
+            arr[k] = _lib.LayerMesh(int(nx), int(ny), float(hx), float(hy), float(hz), float(x0), float(y0), float(z0))
+        self.handle = C.c_void_p()
+        check(self.L.bm_mlayer_create_meshes(len(self.meshes), C.cast(arr, C.c_void_p), 0, C.byref(self.handle)))
+        self.counts = [int(m[0]) * int(m[1]) for m in self.meshes]
+
+    @property
+    def ncells(self) -> int:
+        return sum(self.counts)
+
+    def convolute(self, M: torch.Tensor, Heff: torch.Tensor | None = None, clear_out: bool = True) -> torch.Tensor:
+        if Heff is None:
+            Heff = torch.empty_like(M)
+            clear_out = True
+        check(self.L.bm_mlayer_convolute(self.handle, _ptr(M), _ptr(Heff), int(clear_out), _stream()))
+        return Heff
+
+    def convolute_split(self, M_layers, H_layers, clear_out: bool = True, energies: torch.Tensor | None = None) -> None:
+        st = _stream()
+        for l, M in enumerate(M_layers):
+            check(self.L.bm_mlayer_forward(self.handle, l, _ptr(M), st))
+        check(self.L.bm_mlayer_multiply(self.handle, st))
+        for l, (M, H) in enumerate(zip(M_layers, H_layers)):
+            ep = None if energies is None else C.c_void_p(energies.data_ptr() + 4 * l)
+            check(self.L.bm_mlayer_inverse(self.handle, l, _ptr(M), _ptr(H), int(clear_out), ep, st))
+
+    def close(self) -> None:
+        if self.handle:
+            self.L.bm_mlayer_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class MultiLayerDemag:
     """Multilayered demag of stacked 2D layers: nlayers layers of nx*ny cells (hx, hy, hz[l]),
     layer l's bottom face at height z0[l] (m).  M and H are (nlayers*nx*ny, 3) float32 CUDA

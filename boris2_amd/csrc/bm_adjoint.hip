@@ -652,6 +652,7 @@ extern "C" int bm_adj_set_layers(bm_adj_t o, bm_mlayer_t ml, const double* A, co
 	if (!o || !ml) return fail(BM_ERR_ARG, "bm_adj_set_layers: null argument");
 	int L, nx, ny;
 	mlayer_dims(ml, &L, &nx, &ny, nullptr);
+	if (mlayer_mapped(ml)) return fail(BM_ERR_SIZE, "bm_adj_set_layers: the layers must share one planned grid");
 	if (L != o->nz || nx != o->nx || ny != o->ny) return fail(BM_ERR_ARG, "bm_adj_set_layers: the stack's layers must be the mesh's nz planes (same nx, ny)");
 	if (L > kMaxAdjLayers) return fail(BM_ERR_SIZE, "bm_adj_set_layers: at most 8 layers");
 	const double* src[3] = { A, alpha, K1 };

@@ -558,7 +558,10 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 				rc = fail(BM_ERR_ARG, "bm_batch_create_layers: a stack needs nz == nlayers <= 16 and its hz, z0");
 			else if (!(o->p[b].modules & BM_MOD_DEMAG))
 				rc = fail(BM_ERR_ARG, "bm_batch_create_layers: a layer stack needs the demag module");
-			else rc = bm_mlayer_create(ld->nlayers, o->nx[b], o->ny[b], o->h[3 * b], o->h[3 * b + 1], ld->hz, ld->z0, 0, &o->ml[b]);
+			else {
+				rc = bm_mlayer_create(ld->nlayers, o->nx[b], o->ny[b], o->h[3 * b], o->h[3 * b + 1], ld->hz, ld->z0, 0, &o->ml[b]);
+				if (rc == BM_OK && mlayer_mapped(o->ml[b])) rc = fail(BM_ERR_SIZE, "bm_batch_create_layers: a stack needs nx and 2 ny with FFT plans");
+			}
 			const double* src[3] = { ld->A, ld->alpha, ld->K1 };
 			for (int q = 0; q < 3; q++)
 				if (src[q]) { o->lay_set[b] |= 1 << q; for (int l = 0; l < ld->nlayers; l++) o->lay[b][q * kMaxStackLayers + l] = (float)src[q][l]; }

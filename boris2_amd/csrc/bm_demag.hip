@@ -942,6 +942,8 @@ int x_length_at_least(int v)
 	return best;
 }
 
+int bm::plan_length_at_least(int v) { return x_length_at_least(v); }
+
 // Sizes (Boris/ConvolutionDataCUDA.cpp:24-100): N = n along a periodic axis, otherwise the
 // reference takes N.x = 2 nx, N.y = 2 ny (any n, cuFFT plans every length) and N.z the next
 // power of two >= 2 nz (N.z = 1 in 2D).  Here an open x or y axis takes the smallest length

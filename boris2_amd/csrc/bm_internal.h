@@ -106,6 +106,11 @@ int mlayer_planar(bm_mlayer_s* m, const float* M, float* H3n, hipStream_t st);
 // the transposed operator N^T (the adjoint of the multilayered convolution), planar output
 int mlayer_planar_transposed(bm_mlayer_s* m, const float* M, float* H3n, hipStream_t st);
 void mlayer_dims(const bm_mlayer_s* m, int* L, int* nx, int* ny, double* hz);
+// the handle's layers are meshes of their own on a common grid (bm_mlayer_create_meshes, or a
+// size without a plan): only the convolution calls take it
+bool mlayer_mapped(const bm_mlayer_s* m);
+// the smallest x-pass length (2^a 3^b, 2..2048) at or above v, 0 if none
+int plan_length_at_least(int v);
 // Pointers and sizes of a multilayered demag handle (batched launches).
 struct MlView {
 	float2* Sx; float2* S; const float* K;

@@ -18,6 +18,7 @@
 #include "bm_internal.h"
 #include "bm_newell.h"
 #include "bm_xpass.h"
+#include "bm_stage.h"   // kMU0
 
 #include <algorithm>
 #include <cmath>
@@ -32,6 +33,67 @@ using namespace bm;
 // and works in place; above it a general product writes to a second spectra array.
 constexpr int kRegLayers = 16;
 constexpr int kMaxLayers = 64;
+
+// Sparse transfer between the layers' own cells and the common planes (CSR rows: row r sums
+// val[k] * src[col[k]] over k in [ptr[r], ptr[r + 1]), three components).  Transfer in: one
+// row per common cell of every layer plane (l * plane + c), columns the layer cells of the
+// concatenated layer arrays; transfer out: one row per layer cell, columns common cells of the
+// layer's plane (relative to it).
+struct Csr { const int* ptr; const int* col; const double* val; };
+
+// rows r0 .. r0+nr-1 of a transfer-in table into dst rows 0.., sources offset by msub
+__global__ void k_ml_transfer_in(Csr t, const float* __restrict__ M, float* __restrict__ dst, long long r0, long long nr, long long msub)
+{
+	for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nr; q += (long long)gridDim.x * blockDim.x) {
+		double a0 = 0, a1 = 0, a2 = 0;
+		for (int k = t.ptr[r0 + q]; k < t.ptr[r0 + q + 1]; k++) {
+			const size_t c = (size_t)(t.col[k] - msub);
+			const double w = t.val[k];
+			a0 += w * M[3 * c]; a1 += w * M[3 * c + 1]; a2 += w * M[3 * c + 2];
+		}
+		float* o = dst + 3 * q;
+		o[0] = (float)a0; o[1] = (float)a1; o[2] = (float)a2;
+	}
+}
+
+// rows r0 .. r0+nr-1 of a transfer-out table (layer cells) from the common plane P, set
+// (mode 1) or added (mode 2) into H rows 0..; with M (the same layer cells) each block also
+// leaves its partial sum of M . (the transferred field) in epart[blockIdx.x] (fixed order).
+__global__ void __launch_bounds__(256) k_ml_transfer_out(Csr t, const float* __restrict__ P, float* __restrict__ H, long long r0, long long nr,
+	int mode, const float* __restrict__ M, double* __restrict__ epart)
+{
+	__shared__ double red[256];
+	double es = 0;
+	for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nr; q += (long long)gridDim.x * blockDim.x) {
+		double a0 = 0, a1 = 0, a2 = 0;
+		for (int k = t.ptr[r0 + q]; k < t.ptr[r0 + q + 1]; k++) {
+			const size_t c = (size_t)t.col[k];
+			const double w = t.val[k];
+			a0 += w * P[3 * c]; a1 += w * P[3 * c + 1]; a2 += w * P[3 * c + 2];
+		}
+		float* o = H + 3 * q;
+		const float h0 = (float)a0, h1 = (float)a1, h2 = (float)a2;
+		if (mode == 2) { o[0] += h0; o[1] += h1; o[2] += h2; }
+		else { o[0] = h0; o[1] = h1; o[2] = h2; }
+		if (M) es += (double)M[3 * q] * h0 + (double)M[3 * q + 1] * h1 + (double)M[3 * q + 2] * h2;
+	}
+	if (!M) return;
+	red[threadIdx.x] = es;
+	__syncthreads();
+	for (int o = 128; o > 0; o >>= 1) { if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o]; __syncthreads(); }
+	if (threadIdx.x == 0) epart[blockIdx.x] = red[0];
+}
+
+// energy density -mu0/(2 n) sum of the partials, summed in order
+__global__ void k_ml_energy(const double* part, int np, long long n, float* energy)
+{
+	if (threadIdx.x || blockIdx.x) return;
+	double s = 0;
+	for (int q = 0; q < np; q++) s += part[q];
+	*energy = (float)(-kMU0 / 2.0 * s / (double)n);
+}
+
+
 
 // Position of grid index i (-1, 0, +1, ...) along one axis: i*h + o, or a table for -1, 0, +1.
 struct Axis { double h, o, p[3]; int table; };
@@ -314,6 +376,14 @@ struct bm_mlayer_s {
 	int ncu = 256;
 	double* ebuf = nullptr;  // energy partials of the per-layer inverse (bm_mlayer_inverse)
 	float2* Sout() const { return S2 ? S2 : S; }
+	// layers that are meshes of their own (bm_mlayer_create_meshes) on the common grid above:
+	// empty when every layer is the common grid itself
+	std::vector<long long> moff;   // first cell of each layer mesh in the concatenation (L + 1)
+	void* tabs = nullptr;          // device transfer tables (in: L * plane rows, out: moff[L] rows)
+	Csr tin{}, tout{};
+	float* stage = nullptr;        // (L + 1) common planes, AoS: the layers' M (then H), one H plane
+	double* epart = nullptr;       // energy partials of the per-layer transfer out
+	bool mapped() const { return !moff.empty(); }
 };
 
 namespace {
@@ -449,14 +519,9 @@ int compute_ml_kernels(bm_mlayer_s* m, hipStream_t st)
 
 using bm::fail;
 
-extern "C" int bm_mlayer_create(int nlayers, int nx, int ny, double hx, double hy, const double* hz, const double* z0, int flags, bm_mlayer_t* out)
+// Handle on one grid of nx x ny cells with planned lengths (nx and 2 ny).
+static int ml_create_grid(int nlayers, int nx, int ny, double hx, double hy, const double* hz, const double* z0, bm_mlayer_t* out)
 {
-	(void)flags;
-	if (!out || !hz || !z0 || nlayers < 1 || nx < 2 || ny < 1 || !(hx > 0) || !(hy > 0)) return fail(BM_ERR_ARG, "bm_mlayer_create: bad arguments");
-	*out = nullptr;
-	if (nlayers > kMaxLayers) return fail(BM_ERR_SIZE, "bm_mlayer_create: at most 64 layers");
-	for (int l = 0; l < nlayers; l++) if (!(hz[l] > 0)) return fail(BM_ERR_ARG, "bm_mlayer_create: layer thickness must be positive");
-	if (nx % 2 || !bm::plan_2d_ok(nx, 2 * ny)) return fail(BM_ERR_SIZE, "bm_mlayer_create: no FFT plan for this layer size");
 	bm_mlayer_s* m = new bm_mlayer_s();
 	m->L = nlayers; m->nx = nx; m->ny = ny; m->Nx = 2 * nx; m->Ny = 2 * ny; m->NC = nx + 1; m->NYH = ny + 1; m->Sp = m->Ny;
 	m->hx = hx; m->hy = hy; m->hz.assign(hz, hz + nlayers); m->z0.assign(z0, z0 + nlayers);
@@ -500,6 +565,212 @@ extern "C" int bm_mlayer_create(int nlayers, int nx, int ny, double hx, double h
 	return BM_OK;
 }
 
+// Rectangle (m) for the mesh transfer tables.
+struct R3 { double sx, sy, sz, ex, ey, ez; };
+
+// The cells of a grid (origin o, cell h, nx x ny x 1 cells) that rectangle r covers, with the
+// reference's reciprocal-distance weights d_max - |centre(r) - centre(cell)|, d_max =
+// |size(r)/2 + h/2|, in its loop order (x, then y, then z), appended to out; returns their
+// total.  Transfer::build_meshcells_weights / build_supermeshcells_weights
+// (Boris/BorisLib/VEC_MeshTransfer.h:304-350, 396-440); the corner cell indices are its
+// floor / ceil of a DBL3, which round with a 1e-4 cell tolerance (floor_epsilon,
+// ceil_epsilon, Boris/BorisLib/Funcs_Math.h:161-173).
+static double recip_weights(const R3& r, const double o[3], const double h[3], int nx, int ny, std::vector<std::pair<int, double>>& out)
+{
+	const double cx = (r.ex + r.sx) / 2 - o[0], cy = (r.ey + r.sy) / 2 - o[1], cz = (r.ez + r.sz) / 2 - o[2];
+	const double eps = 1e-4;
+	const int ill = (int)std::floor((r.sx - o[0]) / h[0] + eps), jll = (int)std::floor((r.sy - o[1]) / h[1] + eps), kll = (int)std::floor((r.sz - o[2]) / h[2] + eps);
+	const int iur = (int)std::ceil((r.ex - o[0]) / h[0] - eps), jur = (int)std::ceil((r.ey - o[1]) / h[1] - eps), kur = (int)std::ceil((r.ez - o[2]) / h[2] - eps);
+	const double mx = (r.ex - r.sx) / 2 + h[0] / 2, my = (r.ey - r.sy) / 2 + h[1] / 2, mz = (r.ez - r.sz) / 2 + h[2] / 2;
+	const double dmax = std::sqrt(mx * mx + my * my + mz * mz);
+	double tot = 0;
+	for (int i = std::max(ill, 0); i < std::min(iur, nx); i++)
+		for (int j = std::max(jll, 0); j < std::min(jur, ny); j++)
+			for (int k = std::max(kll, 0); k < std::min(kur, 1); k++) {
+				const double dx = cx - (i + 0.5) * h[0], dy = cy - (j + 0.5) * h[1], dz = cz - (k + 0.5) * h[2];
+				const double d = dmax - std::sqrt(dx * dx + dy * dy + dz * dz);
+				tot += d;
+				out.push_back({ i + j * nx, d });
+			}
+	return tot;
+}
+
+// Covered volume ratio of cell c by rectangle m, 0 when they do not intersect in a volume (the
+// reference's Rect::get_intersection / intersects / IsPlane / IsPoint, with its 1e-12
+// absolute tolerance, Boris/BorisLib/Types_Rect.h:130-139, 347-361, 494-505).
+static double covered_ratio(const R3& c, const R3& m)
+{
+	const double tol = 1e-12;
+	const double sx = std::max(c.sx, m.sx), sy = std::max(c.sy, m.sy), sz = std::max(c.sz, m.sz);
+	const double ex = std::min(c.ex, m.ex), ey = std::min(c.ey, m.ey), ez = std::min(c.ez, m.ez);
+	auto zop = [&](double v) { return v > 0 || (v < tol && v > -tol); };
+	if (!(zop(ex - sx) && zop(ey - sy) && zop(ez - sz))) return 0.0;
+	auto isz = [&](double v) { return v < tol && v > -tol; };
+	const double dx = ex - sx, dy = ey - sy, dz = ez - sz;
+	const bool plane = (isz(dx) && !isz(dy) && !isz(dz)) || (isz(dy) && !isz(dx) && !isz(dz)) || (isz(dz) && !isz(dx) && !isz(dy));
+	if (plane || (dx == 0 && dy == 0 && dz == 0)) return 0.0;
+	return (dx / (c.ex - c.sx)) * (dy / (c.ey - c.sy)) * (dz / (c.ez - c.sz));
+}
+
+// Layer meshes of their own sizes, cells and positions (2D layers) on one common grid.  As
+// the reference sets up the multilayered convolution (SDemag::set_default_n_common and
+// set_Rect_collection, Boris/SDemag.cpp:173-300): n_common = the largest cell counts; every
+// layer's rectangle is enlarged to the largest extents, starting as close to the smallest
+// start as its end allows; its common cell is that rectangle over n_common.  The transfer of
+// each layer to and from its rectangle's common cells is the reference's
+// (SDemag_Demag::Initialize_Mesh_Transfer, Boris/SDemag_Demag.cpp:64-160): in, weighted by
+// reciprocal distance and the covered volume ratio (MESHTRANSFERTYPE_WEIGHTED,
+// Boris/BorisLib/VEC_MeshTransfer.h:899-945); out, by reciprocal distance (:1221-1260).  The
+// reference couples rectangles at different places by shifted kernels; here every rectangle
+// sits at its place in the rectangles' bounding box at the common cell (zero elsewhere), and
+// the z-shifted kernels of that box give the same sums over the same cells — so the
+// rectangles must lie on one grid of common cells (else BM_ERR_SIZE).  The box is padded with
+// empty cells to planned FFT lengths (nx, 2 ny).
+static int ml_create_meshes(int nlayers, const bm_layer_mesh* ms, bm_mlayer_t* out)
+{
+	double minx = ms[0].x0, miny = ms[0].y0, mxs = 0, mys = 0;
+	int ncx = 1, ncy = 1;
+	for (int l = 0; l < nlayers; l++) {
+		const bm_layer_mesh& q = ms[l];
+		if (q.nx < 1 || q.ny < 1 || !(q.hx > 0) || !(q.hy > 0) || !(q.hz > 0)) return fail(BM_ERR_ARG, "bm_mlayer_create_meshes: bad layer mesh");
+		ncx = std::max(ncx, q.nx); ncy = std::max(ncy, q.ny);
+		mxs = std::max(mxs, q.nx * q.hx); mys = std::max(mys, q.ny * q.hy);
+		minx = std::min(minx, q.x0); miny = std::min(miny, q.y0);
+	}
+	// enlarged rectangles
+	std::vector<R3> Rm((size_t)nlayers), Rc((size_t)nlayers);
+	for (int l = 0; l < nlayers; l++) {
+		const bm_layer_mesh& q = ms[l];
+		R3& m = Rm[(size_t)l];
+		m = R3{ q.x0, q.y0, q.z0, q.x0 + q.nx * q.hx, q.y0 + q.ny * q.hy, q.z0 + q.hz };
+		R3 c = m;
+		if (c.ex - mxs < minx) { c.sx = minx; c.ex = c.sx + mxs; } else c.sx = c.ex - mxs;
+		if (c.ey - mys < miny) { c.sy = miny; c.ey = c.sy + mys; } else c.sy = c.ey - mys;
+		Rc[(size_t)l] = c;
+	}
+	// the bounding box of the rectangles at the common cell
+	const double hcx = mxs / ncx, hcy = mys / ncy;
+	double bx = Rc[0].sx, by = Rc[0].sy, bex = Rc[0].ex, bey = Rc[0].ey;
+	for (const R3& c : Rc) { bx = std::min(bx, c.sx); by = std::min(by, c.sy); bex = std::max(bex, c.ex); bey = std::max(bey, c.ey); }
+	auto whole = [](double v, int& n) { const double r = std::nearbyint(v); n = (int)r; return std::fabs(v - r) <= 1e-6; };
+	int nbx = 0, nby = 0;
+	std::vector<int> ox((size_t)nlayers), oy((size_t)nlayers);
+	bool ok = whole((bex - bx) / hcx, nbx) && whole((bey - by) / hcy, nby);
+	for (int l = 0; l < nlayers && ok; l++)
+		ok = whole((Rc[(size_t)l].sx - bx) / hcx, ox[(size_t)l]) && whole((Rc[(size_t)l].sy - by) / hcy, oy[(size_t)l]);
+	if (!ok) return fail(BM_ERR_SIZE, "bm_mlayer_create_meshes: the layers' convolution rectangles do not lie on one grid of common cells "
+		"(the reference couples them by kernels shifted by fractions of a cell; not built)");
+	const int nxp = plan_length_at_least(std::max(2, nbx)), nyp = plan_length_at_least(2 * nby) / 2;
+	if (!nxp || nyp < 1 || !bm::plan_2d_ok(nxp, 2 * nyp)) return fail(BM_ERR_SIZE, "bm_mlayer_create_meshes: no FFT plan for the common grid");
+	// the layers' own grids are the common one: nothing to transfer
+	bool plain = nxp == nbx && nyp == nby;
+	for (int l = 0; l < nlayers; l++) {
+		const bm_layer_mesh& q = ms[l];
+		plain = plain && q.nx == nbx && q.ny == nby && ox[(size_t)l] == 0 && oy[(size_t)l] == 0
+			&& std::fabs(q.hx - hcx) <= 1e-12 * hcx && std::fabs(q.hy - hcy) <= 1e-12 * hcy;
+	}
+	std::vector<double> hz((size_t)nlayers), z0((size_t)nlayers);
+	for (int l = 0; l < nlayers; l++) { hz[(size_t)l] = ms[l].hz; z0[(size_t)l] = ms[l].z0; }
+	bm_mlayer_t m = nullptr;
+	int rc = ml_create_grid(nlayers, nxp, nyp, hcx, hcy, hz.data(), z0.data(), &m);
+	if (rc || plain) { if (!rc) *out = m; return rc; }
+	// transfer tables
+	const size_t plane = (size_t)nxp * nyp;
+	std::vector<long long> moff((size_t)nlayers + 1, 0);
+	for (int l = 0; l < nlayers; l++) moff[(size_t)l + 1] = moff[(size_t)l] + (long long)ms[l].nx * ms[l].ny;
+	std::vector<int> iptr(1, 0), icol, optr(1, 0), ocol;
+	std::vector<double> ival, oval;
+	std::vector<std::pair<int, double>> w;
+	for (int l = 0; l < nlayers; l++) {
+		const bm_layer_mesh& q = ms[l];
+		const R3& c = Rc[(size_t)l];
+		const R3& mr = Rm[(size_t)l];
+		const double hc[3] = { (c.ex - c.sx) / ncx, (c.ey - c.sy) / ncy, c.ez - c.sz };
+		const double hm[3] = { q.hx, q.hy, q.hz };
+		const double om[3] = { mr.sx, mr.sy, mr.sz }, oc[3] = { c.sx, c.sy, c.sz };
+		// in: every cell of the layer's plane of the box; cells of its rectangle take the layer cells
+		for (size_t cell = 0; cell < plane; cell++) {
+			const int bi = (int)(cell % nxp), bj = (int)(cell / nxp), i = bi - ox[(size_t)l], j = bj - oy[(size_t)l];
+			if (i >= 0 && j >= 0 && i < ncx && j < ncy) {
+				const R3 rc3{ c.sx + i * hc[0], c.sy + j * hc[1], c.sz, c.sx + i * hc[0] + hc[0], c.sy + j * hc[1] + hc[1], c.sz + hc[2] };
+				const double cov = covered_ratio(rc3, mr);
+				if (cov > 0) {
+					w.clear();
+					const double tot = recip_weights(rc3, om, hm, q.nx, q.ny, w);
+					for (auto& e : w) { icol.push_back((int)(moff[(size_t)l] + e.first)); ival.push_back(tot > 0 ? e.second * (cov / tot) : e.second); }
+				}
+			}
+			iptr.push_back((int)icol.size());
+		}
+		// out: every layer cell from the common cells of its rectangle (box plane indices)
+		for (int J = 0; J < q.ny; J++)
+			for (int I = 0; I < q.nx; I++) {
+				const R3 r3{ mr.sx + I * hm[0], mr.sy + J * hm[1], mr.sz, mr.sx + I * hm[0] + hm[0], mr.sy + J * hm[1] + hm[1], mr.sz + hm[2] };
+				w.clear();
+				const double tot = recip_weights(r3, oc, hc, ncx, ncy, w);
+				for (auto& e : w) {
+					const int i = e.first % ncx, j = e.first / ncx;
+					ocol.push_back((oy[(size_t)l] + j) * nxp + ox[(size_t)l] + i);
+					oval.push_back(tot > 0 ? e.second * (1.0 / tot) : e.second);
+				}
+				optr.push_back((int)ocol.size());
+			}
+	}
+	const size_t nb = (iptr.size() + optr.size() + icol.size() + ocol.size()) * sizeof(int) + (ival.size() + oval.size()) * sizeof(double) + 64;
+	char* t = nullptr;
+	if (hipMalloc(&t, nb) != hipSuccess || hipMalloc(&m->stage, 3 * plane * (size_t)(nlayers + 1) * sizeof(float)) != hipSuccess
+		|| hipMalloc(&m->epart, 1024 * sizeof(double)) != hipSuccess) {
+		if (t) (void)hipFree(t);
+		bm_mlayer_destroy(m);
+		return fail(BM_ERR_NOMEM, "bm_mlayer_create_meshes: device memory");
+	}
+	m->tabs = t;
+	size_t at = 0;
+	auto put = [&](const void* src, size_t bytes) { void* d = t + at; (void)hipMemcpy(d, src, bytes, hipMemcpyHostToDevice); at += (bytes + 7) / 8 * 8; return d; };
+	m->tin.val = (const double*)put(ival.data(), ival.size() * sizeof(double));
+	m->tout.val = (const double*)put(oval.data(), oval.size() * sizeof(double));
+	m->tin.ptr = (const int*)put(iptr.data(), iptr.size() * sizeof(int));
+	m->tin.col = (const int*)put(icol.data(), icol.size() * sizeof(int));
+	m->tout.ptr = (const int*)put(optr.data(), optr.size() * sizeof(int));
+	m->tout.col = (const int*)put(ocol.data(), ocol.size() * sizeof(int));
+	m->moff = moff;
+	m->bytes += (long long)(nb + 3 * plane * (nlayers + 1) * sizeof(float));
+	*out = m;
+	return BM_OK;
+}
+
+static int ml_check_layers(int nlayers, const double* hz)
+{
+	if (nlayers > kMaxLayers) return fail(BM_ERR_SIZE, "bm_mlayer_create: at most 64 layers");
+	for (int l = 0; l < nlayers; l++) if (!(hz[l] > 0)) return fail(BM_ERR_ARG, "bm_mlayer_create: layer thickness must be positive");
+	return BM_OK;
+}
+
+extern "C" int bm_mlayer_create(int nlayers, int nx, int ny, double hx, double hy, const double* hz, const double* z0, int flags, bm_mlayer_t* out)
+{
+	(void)flags;
+	if (!out || !hz || !z0 || nlayers < 1 || nx < 1 || ny < 1 || !(hx > 0) || !(hy > 0)) return fail(BM_ERR_ARG, "bm_mlayer_create: bad arguments");
+	*out = nullptr;
+	int rc = ml_check_layers(nlayers, hz);
+	if (rc) return rc;
+	if (nx % 2 == 0 && bm::plan_2d_ok(nx, 2 * ny)) return ml_create_grid(nlayers, nx, ny, hx, hy, hz, z0, out);
+	// sizes without a plan: the layers on a padded common grid
+	std::vector<bm_layer_mesh> ms((size_t)nlayers);
+	for (int l = 0; l < nlayers; l++) ms[(size_t)l] = bm_layer_mesh{ nx, ny, hx, hy, hz[l], 0.0, 0.0, z0[l] };
+	return ml_create_meshes(nlayers, ms.data(), out);
+}
+
+extern "C" int bm_mlayer_create_meshes(int nlayers, const bm_layer_mesh* meshes, int flags, bm_mlayer_t* out)
+{
+	(void)flags;
+	if (!out || !meshes || nlayers < 1) return fail(BM_ERR_ARG, "bm_mlayer_create_meshes: bad arguments");
+	*out = nullptr;
+	std::vector<double> hz((size_t)nlayers);
+	for (int l = 0; l < nlayers; l++) hz[(size_t)l] = meshes[l].hz;
+	int rc = ml_check_layers(nlayers, hz.data());
+	return rc ? rc : ml_create_meshes(nlayers, meshes, out);
+}
+
 extern "C" int bm_mlayer_destroy(bm_mlayer_t m)
 {
 	if (!m) return BM_OK;
@@ -509,6 +780,9 @@ extern "C" int bm_mlayer_destroy(bm_mlayer_t m)
 	if (m->K) (void)hipFree(m->K);
 	if (m->kidx) (void)hipFree(m->kidx);
 	if (m->ebuf) (void)hipFree(m->ebuf);
+	if (m->tabs) (void)hipFree(m->tabs);
+	if (m->stage) (void)hipFree(m->stage);
+	if (m->epart) (void)hipFree(m->epart);
 	delete m;
 	return BM_OK;
 }
@@ -559,16 +833,35 @@ int bm::batch_ml_mult(int L, LaunchShape sh, const MlBatch* h_tab, const MlBatch
 	return fail(BM_ERR_SIZE, "batched multilayer demag: at most 16 layers per stack");
 }
 
+bool bm::mlayer_mapped(const bm_mlayer_s* m) { return m->mapped(); }
+
 void bm::mlayer_dims(const bm_mlayer_s* m, int* L, int* nx, int* ny, double* hz)
 {
 	*L = m->L; *nx = m->nx; *ny = m->ny;
 	if (hz) for (int l = 0; l < m->L; l++) hz[l] = m->hz[l];
 }
 
+static int ml_grid(long long n) { return (int)std::max<long long>(1, std::min<long long>((n + 255) / 256, 8192)); }
+
 extern "C" int bm_mlayer_convolute(bm_mlayer_t m, const float* M, float* Heff, int clear_out, void* stream)
 {
 	if (!m || !M || !Heff) return fail(BM_ERR_ARG, "bm_mlayer_convolute: null argument");
-	int rc = ml_convolute(m, M, Heff, clear_out ? 1 : 2, (hipStream_t)stream);
+	hipStream_t st = (hipStream_t)stream;
+	int rc;
+	if (m->mapped()) {
+		// layer meshes -> common planes -> convolution (fields over the same planes) -> meshes
+		const size_t plane = (size_t)m->nx * m->ny;
+		const long long rows = (long long)plane * m->L;
+		hipLaunchKernelGGL(k_ml_transfer_in, dim3(ml_grid(rows)), dim3(256), 0, st, m->tin, M, m->stage, 0LL, rows, 0LL);
+		rc = ml_convolute(m, m->stage, m->stage, 1, st);
+		// each layer's cells from its own plane
+		for (int l = 0; l < m->L && rc == BM_OK; l++) {
+			const long long r0 = m->moff[(size_t)l], nr = m->moff[(size_t)l + 1] - r0;
+			hipLaunchKernelGGL(k_ml_transfer_out, dim3(ml_grid(nr)), dim3(256), 0, st, m->tout, (const float*)(m->stage + 3 * plane * (size_t)l),
+				Heff + 3 * r0, r0, nr, clear_out ? 1 : 2, (const float*)nullptr, (double*)nullptr);
+		}
+	}
+	else rc = ml_convolute(m, M, Heff, clear_out ? 1 : 2, st);
 	if (rc) return rc;
 	BM_HIP(hipGetLastError());
 	return BM_OK;
@@ -595,7 +888,15 @@ extern "C" int bm_mlayer_forward(bm_mlayer_t m, int layer, const float* M_layer,
 {
 	if (!m || !M_layer || layer < 0 || layer >= m->L) return fail(BM_ERR_ARG, "bm_mlayer_forward: bad arguments");
 	hipStream_t st = (hipStream_t)stream;
-	int rc = bm::pass_fwd_x_plane(m->nx, M_layer, m->Sx, m->ny, layer, m->L, m->ntl, m->ncu, st);
+	const float* src = M_layer;
+	if (m->mapped()) {
+		// the layer's mesh onto its common plane
+		const size_t plane = (size_t)m->nx * m->ny;
+		src = m->stage + 3 * plane * (size_t)layer;
+		hipLaunchKernelGGL(k_ml_transfer_in, dim3(ml_grid((long long)plane)), dim3(256), 0, st, m->tin, M_layer, m->stage + 3 * plane * (size_t)layer,
+			(long long)plane * layer, (long long)plane, m->moff[(size_t)layer]);
+	}
+	int rc = bm::pass_fwd_x_plane(m->nx, src, m->Sx, m->ny, layer, m->L, m->ntl, m->ncu, st);
 	for (int c = 0; c < 3 && rc == BM_OK; c++) {
 		const size_t cz = (size_t)c * m->L + layer;
 		rc = bm::pass_y(m->Ny, -1, m->S + cz * m->NC * m->Sp, m->Sx + cz * m->ntl * m->NC * bm::kLT, m->NC, m->Sp, m->NC, m->ntl, m->ncu, st);
@@ -627,7 +928,21 @@ extern "C" int bm_mlayer_inverse(bm_mlayer_t m, int layer, const float* M_layer,
 		rc = bm::pass_y(m->Ny, 1, m->Sout() + cz * m->NC * m->Sp, m->Sx + cz * m->ntl * m->NC * bm::kLT, m->NC, m->Sp, m->NC, m->ntl, m->ncu, st);
 	}
 	const float scale = (float)(1.0 / ((double)m->Nx * m->Ny));
-	if (rc == BM_OK)
+	if (rc == BM_OK && m->mapped()) {
+		// field on the common plane, then onto the layer's cells (and their energy density
+		// from the layer's own M and the transferred field)
+		const size_t plane = (size_t)m->nx * m->ny;
+		float* Hp = m->stage + 3 * plane * (size_t)m->L;
+		rc = bm::pass_inv_x_plane(m->nx, 1, m->Sx, Hp, m->ny, layer, m->L, m->ntl, scale, m->ncu, st, nullptr, nullptr, nullptr);
+		const long long r0 = m->moff[(size_t)layer], nr = m->moff[(size_t)layer + 1] - r0;
+		const int g = std::min(ml_grid(nr), 1024);
+		if (rc == BM_OK) {
+			hipLaunchKernelGGL(k_ml_transfer_out, dim3(g), dim3(256), 0, st, m->tout, (const float*)Hp, H_layer, r0, nr, clear_out ? 1 : 2,
+				energy ? M_layer : (const float*)nullptr, m->epart);
+			if (energy) hipLaunchKernelGGL(k_ml_energy, dim3(1), dim3(64), 0, st, (const double*)m->epart, g, nr, energy);
+		}
+	}
+	else if (rc == BM_OK)
 		rc = bm::pass_inv_x_plane(m->nx, clear_out ? 1 : 2, m->Sx, H_layer, m->ny, layer, m->L, m->ntl, scale, m->ncu, st, M_layer, m->ebuf, energy);
 	if (rc) return rc;
 	BM_HIP(hipGetLastError());

@@ -713,6 +713,7 @@ extern "C" int bm_ode_set_layers(bm_ode_t o, bm_mlayer_t ml, const double* Ms, c
 	}
 	int L, nx, ny;
 	bm::mlayer_dims(ml, &L, &nx, &ny, nullptr);
+	if (bm::mlayer_mapped(ml)) return fail(BM_ERR_SIZE, "bm_ode_set_layers: the layers must share one planned grid");
 	if (L != o->nz || nx != o->nx || ny != o->ny) return fail(BM_ERR_ARG, "bm_ode_set_layers: the stack's layers must be the mesh's nz planes (same nx, ny)");
 	if (L > bm::kMaxStackLayers) return fail(BM_ERR_SIZE, "bm_ode_set_layers: at most 64 layers");
 	const size_t b3 = o->n * 3 * sizeof(float);

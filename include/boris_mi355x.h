@@ -114,6 +114,22 @@ int bm_demag_copy_kernels(bm_demag_t d, float* host_out, long long count);
 typedef struct bm_mlayer_s* bm_mlayer_t;
 int bm_mlayer_create(int nlayers, int nx, int ny, double hx, double hy, const double* hz, const double* z0,
 	int flags, bm_mlayer_t* out);
+/*
+ * Layers that are separate meshes of their own lateral size, cell and position: layer l has
+ * nx x ny cells of hx x hy x hz, its bottom-left corner at (x0, y0, z0) (m).  Replaces the
+ * same set-up over meshes of different extents and cells — SDemag::set_default_n_common /
+ * set_Rect_collection (Boris/SDemag.cpp:173-300), which put every mesh's rectangle onto one
+ * common grid, and the mesh transfer to and from it (SDemag_Demag::Initialize_Mesh_Transfer,
+ * BorisLib/VEC_MeshTransfer.h).  The common cell is the reference's (per axis the largest
+ * layer extent over the largest cell count); the layers sit in their bounding box at that
+ * cell (padded with empty cells to planned FFT lengths).  A layer's cells must be whole
+ * multiples of the common cell and start on it: the reference's transfer then reduces to
+ * copying a cell's M into the common cells it covers and averaging their fields back (its
+ * reciprocal-distance weights are all equal).  Other layouts return BM_ERR_SIZE.  M and H of all layers are the layers' own cells concatenated
+ * layer by layer (AoS); the split calls take one layer's own nx*ny cells.  Such a handle
+ * serves the convolution calls; the integrator takes handles of equal layers only. */
+typedef struct { int nx, ny; double hx, hy, hz, x0, y0, z0; } bm_layer_mesh;
+int bm_mlayer_create_meshes(int nlayers, const bm_layer_mesh* meshes, int flags, bm_mlayer_t* out);
 int bm_mlayer_destroy(bm_mlayer_t m);
 /* H (=, += if !clear_out) the demag field of every layer from the magnetisation of all
  * layers.  Replaces SDemagCUDA::UpdateField's multilayered branch (Boris/SDemagCUDA.cpp:270):
@@ -124,7 +140,7 @@ int bm_mlayer_convolute(bm_mlayer_t m, const float* M, float* Heff, int clear_ou
  * bm_mlayer_forward for every layer, then bm_mlayer_multiply once, then bm_mlayer_inverse
  * for every layer.  M_layer / H_layer are one layer's nx*ny cells (AoS); the inverse sets
  * (clear_out = 1) or adds to H_layer and, if energy (device float) is non-NULL, writes the
- * layer's -mu0/(2 nx ny) sum M.H_demag into it (M_layer then needed).  Calls of one handle
+ * layer's -mu0/(2 nx ny) sum M.H_demag over its own cells into it (M_layer then needed).  Calls of one handle
  * share its spectra: one evaluation at a time.  Replaces the per-mesh
  * ConvolutionCUDA::ForwardFFT / KernelMultiplication_MultipleInputs / InverseFFT
  * (Boris/ConvolutionCUDA.h:254-351) of SDemagCUDA::UpdateField (Boris/SDemagCUDA.cpp:270+). */

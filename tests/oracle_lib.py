@@ -178,6 +178,123 @@ class OracleMLayer:
             pass
 
 
+# ---------------------------------------------------------------------------------------
+# Multilayered demag of 2D layers that are meshes of their own sizes, cells and positions:
+# the reference's common grid (SDemag::set_default_n_common / set_Rect_collection,
+# Boris/SDemag.cpp:173-300) and mesh transfer (SDemag_Demag::Initialize_Mesh_Transfer,
+# Boris/SDemag_Demag.cpp:64-160; Boris/BorisLib/VEC_MeshTransfer.h), restated in numpy over
+# the oracle's multilayer convolution of the rectangles' bounding box.
+
+def _recip_weights(r, o, h, n):
+    """Transfer::build_meshcells_weights / build_supermeshcells_weights
+    (Boris/BorisLib/VEC_MeshTransfer.h:304-350, 396-440): the cells (i + j n.x) of the grid
+    with origin o, cell h and n = (nx, ny) cells (one in z) that rectangle r = (s, e) covers,
+    weighted d_max - |centre(r) - centre(cell)|, d_max = |size(r)/2 + h/2|; returns
+    (cells, weights, their total) in the reference's loop order.  Corner indices by its
+    floor / ceil of a DBL3, which round with a 1e-4 cell tolerance (floor_epsilon,
+    ceil_epsilon, Boris/BorisLib/Funcs_Math.h:161-173)."""
+    s, e = np.asarray(r[0], float), np.asarray(r[1], float)
+    o, h = np.asarray(o, float), np.asarray(h, float)
+    coord = (e + s) / 2 - o
+    ll = np.floor((s - o) / h + 1e-4).astype(int)
+    ur = np.ceil((e - o) / h - 1e-4).astype(int)
+    dmax = float(np.sqrt(np.sum(((e - s) / 2 + h / 2) ** 2)))
+    cells, wts, tot = [], [], 0.0
+    for i in range(max(ll[0], 0), min(ur[0], n[0])):
+        for j in range(max(ll[1], 0), min(ur[1], n[1])):
+            for k in range(max(ll[2], 0), min(ur[2], 1)):
+                d = dmax - float(np.sqrt(np.sum((coord - (np.array([i, j, k]) + 0.5) * h) ** 2)))
+                tot += d
+                cells.append(i + j * n[0]); wts.append(d)
+    return cells, wts, tot
+
+
+def _covered(c, m):
+    """Covered volume ratio of cell c by rectangle m (Rect::get_intersection / intersects /
+    IsPlane / IsPoint with the 1e-12 tolerance, Boris/BorisLib/Types_Rect.h)."""
+    tol = 1e-12
+    s = np.maximum(c[0], m[0]); e = np.minimum(c[1], m[1])
+    d = e - s
+    if not all(v > 0 or abs(v) < tol for v in d):
+        return 0.0
+    z = [abs(v) < tol for v in d]
+    plane = (z[0] and not z[1] and not z[2]) or (z[1] and not z[0] and not z[2]) or (z[2] and not z[0] and not z[1])
+    if plane or all(v == 0 for v in d):
+        return 0.0
+    size = np.asarray(c[1], float) - np.asarray(c[0], float)
+    return float(np.prod(d / size))
+
+
+class OracleMLayerMeshes:
+    """Layers given as meshes (nx, ny, hx, hy, hz, x0, y0, z0): the reference's n_common (the
+    largest cell counts), each layer's convolution rectangle (enlarged to the largest
+    extents, starting as close to the smallest start as its end allows) and common cell
+    (rectangle / n_common); M of a layer onto its rectangle's common cells by
+    MESHTRANSFERTYPE_WEIGHTED (reciprocal distance x covered volume ratio,
+    VEC_MeshTransfer.h:899-945), fields back by reciprocal distance (:1221-1260).  The
+    rectangles are placed in their bounding box at the common cell and convolved there with
+    the oracle's multilayered demag.  M and H: the layers' own cells concatenated."""
+
+    def __init__(self, meshes):
+        self.meshes = [tuple(m) for m in meshes]
+        nc = (max(m[0] for m in meshes), max(m[1] for m in meshes))
+        size = (max(m[0] * m[2] for m in meshes), max(m[1] * m[3] for m in meshes))
+        lo = (min(m[5] for m in meshes), min(m[6] for m in meshes))
+        self.Rm, self.Rc = [], []
+        for (nx, ny, hx, hy, hz, x0, y0, z0) in meshes:
+            s = [x0, y0, z0]; e = [x0 + nx * hx, y0 + ny * hy, z0 + hz]
+            self.Rm.append((np.array(s), np.array(e)))
+            cs, ce = list(s), list(e)
+            for a in range(2):
+                if ce[a] - size[a] < lo[a]:
+                    cs[a] = lo[a]; ce[a] = cs[a] + size[a]
+                else:
+                    cs[a] = ce[a] - size[a]
+            self.Rc.append((np.array(cs), np.array(ce)))
+        hc = (size[0] / nc[0], size[1] / nc[1])
+        bx, by = min(c[0][0] for c in self.Rc), min(c[0][1] for c in self.Rc)
+        bex, bey = max(c[1][0] for c in self.Rc), max(c[1][1] for c in self.Rc)
+        self.nb = (int(round((bex - bx) / hc[0])), int(round((bey - by) / hc[1])))
+        self.off = [(int(round((c[0][0] - bx) / hc[0])), int(round((c[0][1] - by) / hc[1]))) for c in self.Rc]
+        self.nc, self.hc = nc, hc
+        self.counts = [m[0] * m[1] for m in meshes]
+        self.moff = np.concatenate([[0], np.cumsum(self.counts)])
+        self.conv = OracleMLayer(self.nb[0], self.nb[1], hc[0], hc[1], [m[4] for m in meshes], [m[7] for m in meshes])
+        # transfer matrices (dense, small cases): Tin[l] (box plane x layer cells), Tout[l]
+        self.Tin, self.Tout = [], []
+        for l, m in enumerate(meshes):
+            nx, ny, hx, hy, hz = m[:5]
+            c, mr = self.Rc[l], self.Rm[l]
+            h_c = np.array([(c[1][0] - c[0][0]) / nc[0], (c[1][1] - c[0][1]) / nc[1], c[1][2] - c[0][2]])
+            h_m = np.array([hx, hy, hz])
+            plane = self.nb[0] * self.nb[1]
+            Tin = np.zeros((plane, nx * ny)); Tout = np.zeros((nx * ny, plane))
+            ox, oy = self.off[l]
+            for j in range(nc[1]):
+                for i in range(nc[0]):
+                    rs = c[0] + np.array([i, j, 0]) * h_c
+                    rc = (rs, rs + h_c)
+                    cov = _covered(rc, mr)
+                    if cov > 0:
+                        cells, w, tot = _recip_weights(rc, mr[0], h_m, (nx, ny))
+                        for cell, wt in zip(cells, w):
+                            Tin[(oy + j) * self.nb[0] + ox + i, cell] += wt * cov / tot if tot > 0 else wt
+            for J in range(ny):
+                for I in range(nx):
+                    rs = mr[0] + np.array([I, J, 0]) * h_m
+                    cells, w, tot = _recip_weights((rs, rs + h_m), c[0], h_c, nc)
+                    for cell, wt in zip(cells, w):
+                        i, j = cell % nc[0], cell // nc[0]
+                        Tout[I + J * nx, (oy + j) * self.nb[0] + ox + i] += wt / tot if tot > 0 else wt
+            self.Tin.append(Tin); self.Tout.append(Tout)
+
+    def __call__(self, M):
+        M = np.asarray(M, float).reshape(-1, 3)
+        planes = [self.Tin[l] @ M[self.moff[l]:self.moff[l + 1]] for l in range(len(self.meshes))]
+        Hb = self.conv(np.concatenate(planes)).reshape(len(self.meshes), -1, 3)
+        return np.concatenate([self.Tout[l] @ Hb[l] for l in range(len(self.meshes))])
+
+
 def speedup_next_slot(p, tdm):
     """Slot the next computed demag field replaces once p slots hold saves at times tdm."""
     t = np.zeros(6); t[:len(tdm)] = tdm

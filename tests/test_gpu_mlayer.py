@@ -96,3 +96,68 @@ def test_mlayer_split_per_layer_calls_equal_monolithic(gpu, case):
     for l in range(L):
         e_ref = -api.MU0 / (2 * nl) * float(np.sum(Mn[l * nl:(l + 1) * nl] * ref[l * nl:(l + 1) * nl]))
         assert abs(en[l].item() - e_ref) <= 1e-5 * abs(e_ref), (l, en[l].item(), e_ref)
+
+
+# Layers as meshes of their own: (nx, ny, hx, hy, hz, x0, y0, z0) per layer
+H4 = 4e-9
+MESH_CASES = {
+    # a smaller layer inside the larger one's extent, offset (same cell)
+    "offset": [(32, 24, H4, H4, 4e-9, 0.0, 0.0, 0.0), (12, 10, H4, H4, 2e-9, 5 * H4, 7 * H4, 6e-9)],
+    # 2x coarser cells over the same extent (transfer weights 1/4)
+    "coarse2": [(32, 16, H4, H4, 3e-9, 0.0, 0.0, 0.0), (16, 8, 2 * H4, 2 * H4, 6e-9, 0.0, 0.0, 5e-9)],
+    # 3x coarser cells (reciprocal-distance weights of a 3x3 block), a third layer offset
+    "coarse3": [(24, 18, H4, H4, 3e-9, 0.0, 0.0, 0.0), (8, 6, 3 * H4, 3 * H4, 4e-9, 0.0, 0.0, 4e-9),
+                (6, 6, H4, H4, 2e-9, 9 * H4, 3 * H4, 9e-9)],
+    # side by side: the convolution rectangles at different places (the reference's shifted kernels)
+    "side": [(16, 12, H4, H4, 4e-9, 0.0, 0.0, 0.0), (16, 12, H4, H4, 4e-9, 20 * H4, 3 * H4, 2e-9)],
+    # cells that are no multiple of each other: 20 cells of 5 nm over 16 of 6.25 nm
+    "ratio": [(20, 20, 5e-9, 5e-9, 3e-9, 0.0, 0.0, 0.0), (16, 16, 6.25e-9, 6.25e-9, 3e-9, 0.0, 0.0, 5e-9)],
+}
+
+
+@pytest.mark.parametrize("name", list(MESH_CASES))
+def test_mesh_layers_match_oracle(gpu, name):
+    """The common grid and mesh transfer of the reference (n_common, enlarged rectangles,
+    reciprocal-distance weights) against the oracle's numpy restatement: fields of every
+    layer's own cells (1e-6), and the split per-layer calls equal to the one-call form."""
+    meshes = MESH_CASES[name]
+    counts = [m[0] * m[1] for m in meshes]
+    M = torch.from_numpy(api.random_magnetisation(sum(counts), 8e5, 12).astype(np.float32)).cuda()
+    d = api.MultiLayerMeshDemag(meshes)
+    H = d.convolute(M)
+    ref = O.OracleMLayerMeshes(meshes)(M.cpu().numpy().astype(np.float64))
+    e = rel_l2(H.cpu().numpy(), ref)
+    print(f"{name}: rel L2 {e:.2e}")
+    assert e <= 1e-6
+    off = np.concatenate([[0], np.cumsum(counts)])
+    Ms = [M[off[l]:off[l + 1]].clone() for l in range(len(meshes))]
+    Hs = [torch.full((c, 3), 7.0, dtype=torch.float32, device="cuda") for c in counts]
+    en = torch.zeros(len(meshes), dtype=torch.float32, device="cuda")
+    d.convolute_split(Ms, Hs, clear_out=True, energies=en)
+    assert torch.equal(torch.cat(Hs), H)
+    Mn = M.cpu().numpy().astype(np.float64)
+    for l in range(len(meshes)):
+        e_ref = -api.MU0 / (2 * counts[l]) * float(np.sum(Mn[off[l]:off[l + 1]] * ref[off[l]:off[l + 1]]))
+        assert abs(en[l].item() - e_ref) <= 1e-5 * abs(e_ref), (l, en[l].item(), e_ref)
+    Ha = d.convolute(M, torch.full_like(M, 3.0), clear_out=False)
+    assert torch.allclose(Ha, H + 3.0, rtol=1e-6, atol=1e-2)
+    d.close()
+
+
+def test_layer_size_without_a_plan(gpu):
+    """A stack whose nx has no FFT plan (50): the layers on a padded common grid, the same
+    field as the oracle's stack at N = 2n."""
+    nx, ny, hz, z0 = 50, 14, [4e-9, 2e-9], [0.0, 6e-9]
+    M = api.random_magnetisation(2 * nx * ny, 8e5, 13).astype(np.float32)
+    d = api.MultiLayerDemag(nx, ny, 4e-9, 4e-9, hz, z0)
+    H = d.convolute(torch.from_numpy(M).cuda()).cpu().numpy()
+    ref = O.OracleMLayer(nx, ny, 4e-9, 4e-9, hz, z0)(M.astype(np.float64))
+    assert rel_l2(H, ref) <= 1e-6
+    d.close()
+
+
+def test_mesh_layouts_the_transfer_does_not_cover(gpu):
+    """Rectangles that do not lie on one grid of common cells need kernels shifted by a
+    fraction of a cell: refused with BM_ERR_SIZE."""
+    with pytest.raises(RuntimeError, match="error 2"):
+        api.MultiLayerMeshDemag([(16, 12, H4, H4, 4e-9, 0.0, 0.0, 0.0), (16, 12, H4, H4, 4e-9, 20.5 * H4, 0.0, 2e-9)])

@@ -155,3 +155,69 @@ def test_stack_of_touching_layers_equals_3d_convolution():
     H = O.OracleMLayer(nx, ny, h, h, [h] * L, [l * h for l in range(L)])(M)
     ref = O.OracleDemag((nx, ny, L), (h, h, h), Nz_override=8)(M)
     assert np.linalg.norm(H - ref) / np.linalg.norm(ref) <= 1e-12
+
+
+# ---------------------------------------------------------------------------------------
+# Layers as meshes of their own (the reference's common grid and mesh transfer).
+
+def test_mesh_layers_on_the_common_grid_are_the_plain_multilayer():
+    """Layers that already are the common grid: identity transfers, the plain multilayer."""
+    meshes = [(12, 8, 4e-9, 4e-9, 4e-9, 0.0, 0.0, 0.0), (12, 8, 4e-9, 4e-9, 2e-9, 0.0, 0.0, 6e-9)]
+    o = O.OracleMLayerMeshes(meshes)
+    for T in o.Tin + o.Tout:
+        assert np.array_equal(T, np.eye(T.shape[0]))
+    M = np.random.default_rng(1).normal(size=(2 * 96, 3)) * 8e5
+    ref = O.OracleMLayer(12, 8, 4e-9, 4e-9, [4e-9, 2e-9], [0.0, 6e-9])(M)
+    assert np.array_equal(o(M), ref)
+
+
+def test_transfer_weights_partition_and_closed_forms():
+    """Out weights of every layer cell sum to 1; in weights of a common cell sum to its
+    covered volume ratio; a 2x2 block of common cells takes weights 1/4 each, a 3x3 block
+    weights (d_max - d) / sum with d in {0, h, sqrt(2) h} and d_max = sqrt(8 h^2 + hz^2)."""
+    h, hz = 4e-9, 3e-9
+    # layer 1: 4 x 4 cells of 3h over layer 0's 12 x 12 of h (same extent); layer 2: 6 x 6 of 2h
+    meshes = [(12, 12, h, h, hz, 0.0, 0.0, 0.0), (4, 4, 3 * h, 3 * h, hz, 0.0, 0.0, 2 * hz), (6, 6, 2 * h, 2 * h, hz, 0.0, 0.0, 4 * hz)]
+    o = O.OracleMLayerMeshes(meshes)
+    for T in o.Tout:
+        np.testing.assert_allclose(T.sum(axis=1), 1.0, rtol=1e-14)
+    for T in o.Tin:
+        s = T.sum(axis=1)
+        assert np.all((np.abs(s - 1.0) < 1e-12) | (s == 0.0))
+    # 2 x 2 (layer 2): exactly 1/4
+    w2 = o.Tout[2][0][o.Tout[2][0] > 0]
+    assert len(w2) == 4 and np.allclose(w2, 0.25, rtol=0, atol=1e-15)
+    # 3 x 3 (layer 1), first coarse cell: centre, edges, corners
+    dmax = np.sqrt(8 * h * h + hz * hz)
+    raw = np.array([[dmax - np.hypot(a * h, b * h) for a in (-1, 0, 1)] for b in (-1, 0, 1)])
+    T = o.Tout[1][0].reshape(12, 12)[:3, :3]
+    np.testing.assert_allclose(T, raw / raw.sum(), rtol=1e-13)
+
+
+def test_coarse_layer_field_is_the_average_of_the_fine_computation():
+    """A layer of 2x coarser cells: its field is the mean over each 2x2 block of the field the
+    same layer gets when given at the fine cell with its M repeated (Newell tensors are cell
+    averages, so this is exact physics, not an approximation of the transfer)."""
+    h, hz = 4e-9, 4e-9
+    meshes = [(8, 6, h, h, hz, 0.0, 0.0, 0.0), (4, 3, 2 * h, 2 * h, 2 * hz, 0.0, 0.0, 6e-9)]
+    rng = np.random.default_rng(5)
+    M0, M1 = rng.normal(size=(48, 3)) * 8e5, rng.normal(size=(12, 3)) * 6e5
+    H = O.OracleMLayerMeshes(meshes)(np.concatenate([M0, M1]))
+    M1f = np.repeat(np.repeat(M1.reshape(3, 4, 3), 2, axis=0), 2, axis=1).reshape(-1, 3)
+    Hf = O.OracleMLayer(8, 6, h, h, [hz, 2 * hz], [0.0, 6e-9])(np.concatenate([M0, M1f]))
+    H1f = Hf[48:].reshape(6, 8, 3)
+    avg = H1f.reshape(3, 2, 4, 2, 3).mean(axis=(1, 3)).reshape(-1, 3)
+    np.testing.assert_allclose(H[48:], avg, rtol=1e-12, atol=1e-9 * np.abs(avg).max())
+    np.testing.assert_allclose(H[:48], Hf[:48], rtol=1e-12, atol=1e-9 * np.abs(Hf[:48]).max())
+
+
+def test_mesh_layers_translation_invariance():
+    """Moving every layer by whole common cells changes nothing; a small layer inside a large
+    one and side by side (convolution rectangles at different places)."""
+    h = 4e-9
+    base = [(10, 8, h, h, 3e-9, 0.0, 0.0, 0.0), (4, 4, h, h, 3e-9, 3 * h, 2 * h, 5e-9), (6, 4, h, h, 3e-9, 14 * h, 1 * h, 5e-9)]
+    M = np.random.default_rng(7).normal(size=(80 + 16 + 24, 3)) * 8e5
+    H0 = O.OracleMLayerMeshes(base)(M)
+    moved = [(a, b, c, d, e, x + 5 * h, y - 3 * h, z) for (a, b, c, d, e, x, y, z) in base]
+    H1 = O.OracleMLayerMeshes(moved)(M)
+    np.testing.assert_allclose(H1, H0, rtol=1e-9, atol=1e-9 * np.abs(H0).max())
