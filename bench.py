@@ -249,7 +249,7 @@ def batch_run(steps: int):
     meshes = batched_meshes()
     dt = 50e-15
     cells = sum(n[0] * n[1] * n[2] for n, _, _, _ in meshes)
-    bs = api.BatchSolver(api.BM_EVAL_RK4, [(n, h, p) for n, h, p, _ in meshes], [M0 for *_, M0 in meshes], autotune=True)
+    bs = api.BatchSolver(api.BM_EVAL_RK4, [(n, h, p) for n, h, p, _ in meshes], [M0 for *_, M0 in meshes], autotune="radix")
     bs.set_dt(dt)
     t0 = time.perf_counter()
     bs.run(1)                 # the autotuner runs before the first step
@@ -292,7 +292,7 @@ def batch_rkdp_run(steps: int, warm: int = 5):
     from boris2_amd import api
     meshes = batched_meshes()
     cells = sum(n[0] * n[1] * n[2] for n, _, _, _ in meshes)
-    bs = api.BatchSolver(api.BM_EVAL_RKDP54, [(n, h, p) for n, h, p, _ in meshes], [M0 for *_, M0 in meshes], autotune=True)
+    bs = api.BatchSolver(api.BM_EVAL_RKDP54, [(n, h, p) for n, h, p, _ in meshes], [M0 for *_, M0 in meshes], autotune="radix")
     bs.run(warm)
     torch.cuda.synchronize()
     rej0 = sum(bs.mesh_state(b)["rejections"] for b in range(len(meshes)))
@@ -425,7 +425,7 @@ def multilayer_run(steps: int, adj_steps: int, ckpt: int = 100):
     dt = 100e-15
     cells = sum(n[0] * n[1] * n[2] for n, *_ in items)
     bs = api.BatchSolver(api.BM_EVAL_RK4, [(n, h, p) for n, h, p, _, _ in items], [M0 for *_, M0 in items],
-                         layers=[lay for *_, lay, _ in items], autotune=True)
+                         layers=[lay for *_, lay, _ in items], autotune="radix")
     bs.set_dt(dt)
     bs.run(1)
     torch.cuda.synchronize()

@@ -466,9 +466,12 @@ class BatchSolver:
     one launch per pass size per field evaluation, a step replayed from a captured hipGraph.  meshes: list of (n, h, FMParams); M0: list of per-mesh (ncell, 3)
     arrays or None.  self.M holds every mesh's magnetisation concatenated; mesh(b) is a view.
     layers: None, or per mesh None or dict(hz=..., z0=..., Ms=None, A=None, alpha=None,
-    K1=None) making that mesh's n[2] planes a layer stack (as LayerStackSolver)."""
+    K1=None) making that mesh's n[2] planes a layer stack (as LayerStackSolver).
+    autotune: False, True (launch shapes timed before the first step; results unchanged bit
+    for bit) or "radix" (the FFT passes' radix schedules timed too; results within FP32
+    rounding of the default schedules)."""
 
-    def __init__(self, method: int, meshes, M0=None, use_graph: bool = True, layers=None, autotune: bool = False):
+    def __init__(self, method: int, meshes, M0=None, use_graph: bool = True, layers=None, autotune=False):
         require_device()
         self.L = _lib.load()
         self.meshes = [(tuple(int(v) for v in n), tuple(float(v) for v in h), p) for n, h, p in meshes]
@@ -506,7 +509,7 @@ class BatchSolver:
                     setattr(ld[b], k, a.ctypes.data_as(C.c_void_p))
         check(self.L.bm_batch_create_layers(method, nb, n3.ctypes.data_as(C.c_void_p), h3.ctypes.data_as(C.c_void_p),
                                             C.cast(self._pc, C.c_void_p), C.cast(ld, C.c_void_p) if ld is not None else None,
-                                            _ptr(self.M), (0 if use_graph else 1) | (2 if autotune else 0),
+                                            _ptr(self.M), (0 if use_graph else 1) | (6 if autotune == "radix" else 2 if autotune else 0),
                                             C.byref(self.handle)))
 
     def mesh(self, b: int) -> torch.Tensor:
@@ -517,6 +520,19 @@ class BatchSolver:
         buf = C.create_string_buffer(16384)
         check(self.L.bm_batch_autotune_report(self.handle, buf, len(buf)))
         return [l for l in buf.value.decode().splitlines() if l]
+
+    def set_schedule(self, pass_: str, size: int, f: int = 0, per_cu: int = 0, radix: int = 0) -> None:
+        """Pin one pass-size group's launch shape and radix schedule (pass_: x_fwd, x_inv, y_fwd,
+        y_inv, z, y2d, ml_mult), replacing any pending autotuning."""
+        check(self.L.bm_batch_set_schedule(self.handle, pass_.encode(), int(size), int(f), int(per_cu), int(radix)))
+
+    def apply_report(self, lines) -> None:
+        """Re-apply the choices of an earlier autotune_report() (e.g. saved from another run of the
+        same batch), so a process need not tune again."""
+        for l in lines:
+            w = l.split()
+            kv = dict(x.split("=") for x in w[2:5])
+            self.set_schedule(w[0], int(w[1]), int(kv["f"]), int(kv["per_cu"]), int(kv["radix"]))
 
     def set_dt(self, dT: float) -> None:
         check(self.L.bm_batch_set_dt(self.handle, float(dT)))

@@ -97,7 +97,7 @@ struct bm_batch_s {
 	hipEvent_t ev_in = nullptr, ev_out = nullptr;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
-	bool autotune = false, tuned = false;
+	bool autotune = false, tune_radix = false, tuned = false;
 };
 
 namespace {
@@ -453,6 +453,13 @@ int autotune(bm_batch_s* o, hipStream_t st)
 	for (int f : { 1, 2 }) for (int pc : { 1, 2, 4 }) cx.push_back(LaunchShape{ f, pc });
 	for (int pc : { 4, 8, 16 }) cy.push_back(LaunchShape{ 0, pc });
 	for (int pc : { 1, 2, 4 }) cz.push_back(LaunchShape{ 0, pc });
+	// the launch shapes crossed with every radix schedule of the pass length n
+	auto radices = [&](const std::vector<LaunchShape>& base, int n) {
+		std::vector<LaunchShape> r;
+		const int nv = o->tune_radix ? radix_variants(n) : 1;
+		for (int v = 0; v < nv; v++) for (LaunchShape c : base) { c.radix = v; r.push_back(c); }
+		return r;
+	};
 	for (size_t g = 0; g < o->gx.size() && !rc; g++) {
 		Group& G = o->gx[g];
 		pick(G, 0, cx, true, [&] { return batch_fwd_x(G.size, G.sh[0], o->fwd[0][g].h.data(), o->fwd[0][g].d, (int)o->fwd[0][g].h.size(), o->ncu, st); });
@@ -461,15 +468,15 @@ int autotune(bm_batch_s* o, hipStream_t st)
 	for (size_t g = 0; g < o->gy.size() && !rc; g++) {
 		Group& G = o->gy[g];
 		for (int slot = 0; slot < 2; slot++)
-			pick(G, slot, cy, false, [&] { return batch_y(G.size, slot ? 1 : -1, G.sh[slot], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st); });
+			pick(G, slot, radices(cy, G.size), false, [&] { return batch_y(G.size, slot ? 1 : -1, G.sh[slot], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st); });
 	}
 	for (size_t g = 0; g < o->gz.size() && !rc; g++) {
 		Group& G = o->gz[g];
-		pick(G, 0, cz, false, [&] { return batch_z(G.size, G.sh[0], o->tz[g].h.data(), o->tz[g].d, (int)o->tz[g].h.size(), o->ncu, st); });
+		pick(G, 0, radices(cz, G.size), false, [&] { return batch_z(G.size, G.sh[0], o->tz[g].h.data(), o->tz[g].d, (int)o->tz[g].h.size(), o->ncu, st); });
 	}
 	for (size_t g = 0; g < o->gy2.size() && !rc; g++) {
 		Group& G = o->gy2[g];
-		pick(G, 0, cy, false, [&] { return batch_y2d(G.size, G.sh[0], o->ty2[g].h.data(), o->ty2[g].d, (int)o->ty2[g].h.size(), o->ncu, st); });
+		pick(G, 0, radices(cy, G.size), false, [&] { return batch_y2d(G.size, G.sh[0], o->ty2[g].h.data(), o->ty2[g].d, (int)o->ty2[g].h.size(), o->ncu, st); });
 	}
 	for (size_t g = 0; g < o->gml.size() && !rc; g++) {
 		Group& G = o->gml[g];
@@ -523,7 +530,8 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 	bm_batch_s* o = new bm_batch_s();
 	o->nb = nmesh;
 	o->use_graph = (flags & BM_BATCH_NO_GRAPH) ? 0 : 1;
-	o->autotune = (flags & BM_BATCH_AUTOTUNE) != 0;
+	o->tune_radix = (flags & BM_BATCH_AUTOTUNE_RADIX) != 0;
+	o->autotune = (flags & BM_BATCH_AUTOTUNE) != 0 || o->tune_radix;
 	switch (method) {
 	case BM_EVAL_RK4: o->mcode = M_RK4; o->nstages = 4; o->dT = 0.5e-12; break;
 	case BM_EVAL_TEULER: o->mcode = M_TEULER; o->nstages = 2; o->dT = 0.5e-13; break;
@@ -816,14 +824,34 @@ extern "C" int bm_batch_mesh_offset(bm_batch_t o, int mesh, long long* cell_offs
 	return BM_OK;
 }
 
-// One line per tuned pass: "<pass> <size> f=<factor> per_cu=<n> <microseconds> us".
+// The pass-size group and slot of a pass name ("x_fwd", "x_inv", "y_fwd", "y_inv", "z",
+// "y2d", "ml_mult") and size, or nullptr.
+static Group* find_group(bm_batch_s* o, const std::string& pass, int size, int& slot)
+{
+	std::vector<Group>* gs = nullptr;
+	slot = 0;
+	if (pass == "x_fwd" || pass == "x_inv") { gs = &o->gx; slot = pass == "x_inv"; }
+	else if (pass == "y_fwd" || pass == "y_inv") { gs = &o->gy; slot = pass == "y_inv"; }
+	else if (pass == "z") gs = &o->gz;
+	else if (pass == "y2d") gs = &o->gy2;
+	else if (pass == "ml_mult") gs = &o->gml;
+	if (!gs) return nullptr;
+	for (Group& G : *gs) if (G.size == size) return &G;
+	return nullptr;
+}
+
+// One line per tuned pass:
+// "<pass> <size> f=<factor> per_cu=<n> radix=<v> <schedule> <microseconds> us".
 extern "C" int bm_batch_autotune_report(bm_batch_t o, char* out, int cap)
 {
 	if (!o || !out || cap < 1) return fail(BM_ERR_ARG, "bm_batch_autotune_report: bad arguments");
 	std::string r;
-	char line[160];
+	char line[192];
 	auto add = [&](const char* pass, const Group& G, int slot) {
-		snprintf(line, sizeof(line), "%s %d f=%d per_cu=%d %.1f us\n", pass, G.size, G.sh[slot].f, G.sh[slot].per_cu, G.us[slot]);
+		const LaunchShape& c = G.sh[slot];
+		const bool fft = pass[0] == 'y' || pass[0] == 'z';
+		snprintf(line, sizeof(line), "%s %d f=%d per_cu=%d radix=%d %s %.1f us\n", pass, G.size, c.f, c.per_cu, c.radix,
+			fft ? radix_schedule(G.size, c.radix) : "-", G.us[slot]);
 		r += line;
 	};
 	if (o->tuned) {
@@ -834,5 +862,26 @@ extern "C" int bm_batch_autotune_report(bm_batch_t o, char* out, int cap)
 		for (const Group& G : o->gml) add("ml_mult", G, 0);
 	}
 	snprintf(out, (size_t)cap, "%s", r.c_str());
+	return BM_OK;
+}
+
+extern "C" int bm_batch_set_schedule(bm_batch_t o, const char* pass, int size, int f, int per_cu, int radix)
+{
+	if (!o || !pass) return fail(BM_ERR_ARG, "bm_batch_set_schedule: bad arguments");
+	int slot = 0;
+	Group* G = find_group(o, pass, size, slot);
+	if (!G) return fail(BM_ERR_ARG, "bm_batch_set_schedule: no such pass and size in this batch");
+	const bool x = pass[0] == 'x', fft = pass[0] == 'y' || pass[0] == 'z';
+	if ((x && (f < 0 || f > 2)) || (!x && f != 0) || per_cu < 0 || per_cu > 64 || radix < 0
+		|| (fft ? radix >= radix_variants(size) : radix != 0))
+		return fail(BM_ERR_ARG, "bm_batch_set_schedule: shape or radix schedule out of range for this pass");
+	G->sh[slot] = LaunchShape{ f, per_cu, radix };
+	G->us[slot] = 0;
+	if (o->exec) { (void)hipGraphExecDestroy(o->exec); o->exec = nullptr; }
+	if (o->graph) { (void)hipGraphDestroy(o->graph); o->graph = nullptr; }
+	int rc;
+	if (x && (rc = build_tables(o))) return rc;
+	o->autotune = false;
+	o->tuned = true;
 	return BM_OK;
 }

@@ -33,6 +33,8 @@
 #include <cstdlib>
 #include <mutex>
 #include <string>
+#include <type_traits>
+#include <string>
 
 using namespace bm;
 
@@ -214,16 +216,17 @@ __global__ void __launch_bounds__(kBlock) k_inv_x(const float2* __restrict__ Sx,
 // so that each wave moves whole runs, or interleaving the lanes over the lines, is slower.)
 // FULL (periodic y, N.y = ny): all NY values of a line are data rows.
 // nyd: data rows of a line (ny; N.y when periodic); N.y may exceed 2 ny, the rows between
-// ny and N.y / 2 are zeros that are never stored.
-template <int NY, int DIR, bool FULL = false>
+// ny and N.y / 2 are zeros that are never stored.  V: radix schedule (plan variant) of NY.
+template <int NY, int DIR, bool FULL = false, int V = 0>
 __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, float2* __restrict__ Sx, int nlines, int Sp, int NC, int ntl,
 	int nyd, const PassBatch* __restrict__ bt)
 {
 	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; Sx = b.Sx; nlines = b.nlines; Sp = b.Sp; NC = b.NC; ntl = b.ntl; nyd = b.ny; }
-	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = kBlock / T, LL = lds_len<NY>(), EL = FULL ? E : E / 2;
+	constexpr int PK = plan_key(NY, V);
+	constexpr int E = Plan<PK>::E, T = plan_T<PK>(), LP = kBlock / T, LL = lds_len<NY>(), EL = FULL ? E : E / 2;
 	__shared__ C2<float> lds[LP * LL];
 	const int t = threadIdx.x % T, l = threadIdx.x / T;
-	Twiddles<NY, float> tw; tw.init(t);
+	Twiddles<PK, float> tw; tw.init(t);
 	const int nwork = (nlines + LP - 1) / LP;
 	// the next line group is loaded into registers while this one is transformed
 	float2 pre[E];
@@ -248,7 +251,7 @@ __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, float2* __
 #pragma unroll
 		for (int e = 0; e < E; e++) v[e] = mk(pre[e].x, pre[e].y);
 		if (w + (int)gridDim.x < nwork) fetch(w + gridDim.x);
-		fft<NY, DIR, float>(v, lds + l * LL, 1, t, tw);
+		fft<PK, DIR, float>(v, lds + l * LL, 1, t, tw);
 		if (line < nlines) {
 			if constexpr (DIR < 0) {
 				float2* p = S + (size_t)line * Sp;
@@ -275,13 +278,14 @@ __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, float2* __
 // kyr in [CH*b, CH*b + CH]: that slab (all kz, 6 components, contiguous because the 3D
 // kernel is stored kz fastest) is read from HBM once into LDS and serves the kz and N.z-kz
 // mirrors of every column.
-// FULL (periodic z, N.z = nz): all NZ planes hold data.
-template <int NZ, bool FULL = false>
+// FULL (periodic z, N.z = nz): all NZ planes hold data.  V: radix schedule of NZ.
+template <int NZ, bool FULL = false, int V = 0>
 __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const float* __restrict__ K,
 	int nz, int NC, int Ny, int NYH, int Sp, int xmap, const PassBatch* __restrict__ bt)
 {
 	if (bt) { const PassBatch& b = bt[blockIdx.y]; S = b.S; K = b.K; nz = b.nz; NC = b.NC; Ny = b.Ny; NYH = b.NYH; Sp = b.Sp; }
-	constexpr int E = Plan<NZ>::E, TZ = plan_T<NZ>(), CY = kBlock / TZ, CH = CY / 2, NZH = NZ / 2 + 1, KR = CH + 1;
+	constexpr int PK = plan_key(NZ, V);
+	constexpr int E = Plan<PK>::E, TZ = plan_T<PK>(), CY = kBlock / TZ, CH = CY / 2, NZH = NZ / 2 + 1, KR = CH + 1;
 	constexpr int EL = FULL ? E : E / 2;
 	constexpr int KS = 6 * KR * NZH, NKL = (KS + kBlock - 1) / kBlock;
 	__shared__ C2<float> lds[3 * NZ * CY];
@@ -289,7 +293,7 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 	const int col = threadIdx.x % CY, t = threadIdx.x / CY;
 	const bool hi = col >= CH;
 	const int c = hi ? col - CH : col;
-	Twiddles<NZ, float> tw; tw.init(t);
+	Twiddles<PK, float> tw; tw.init(t);
 	const int half = Ny / 2, nwb = (half + CH - 1) / CH, nwork = NC * nwb;
 	const size_t plane = (size_t)NC * Sp, comp = (size_t)nz * plane;
 	const size_t kq = (size_t)NC * NYH * NZH;
@@ -336,8 +340,8 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 #pragma unroll
 		for (int q = 0; q < NKL; q++) { const int i = threadIdx.x + kBlock * q; if (i < KS) ks[i] = pk[q]; }
 		if (w + G < nwork) fetch(w + G);
-		fft3<NZ, -1, float>(v[0], v[1], v[2], lds + col, lds + NZ * CY + col, lds + 2 * NZ * CY + col, CY, t, tw);
-		if constexpr (Plan<NZ>::S == 1) __syncthreads();  // single-stage plans have no barrier inside
+		fft3<PK, -1, float>(v[0], v[1], v[2], lds + col, lds + NZ * CY + col, lds + 2 * NZ * CY + col, CY, t, tw);
+		if constexpr (Plan<PK>::S == 1) __syncthreads();  // single-stage plans have no barrier inside
 		const float sy = (ky <= half) ? 1.f : -1.f;
 		const float* kr = ks + (ok ? lr : 0) * NZH;
 #pragma unroll
@@ -352,7 +356,7 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 			v[1][e] = mk(kxy * mx.x + kyy * my.x + kyz * mz.x, kxy * mx.y + kyy * my.y + kyz * mz.y);
 			v[2][e] = mk(kxz * mx.x + kyz * my.x + kzz * mz.x, kxz * mx.y + kyz * my.y + kzz * mz.y);
 		}
-		fft3<NZ, 1, float>(v[0], v[1], v[2], lds + col, lds + NZ * CY + col, lds + 2 * NZ * CY + col, CY, t, tw);
+		fft3<PK, 1, float>(v[0], v[1], v[2], lds + col, lds + NZ * CY + col, lds + 2 * NZ * CY + col, CY, t, tw);
 		if (ok) {
 			float2* p = S + (size_t)kx * Sp + ky;
 #pragma unroll
@@ -369,16 +373,17 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 // ---------------------------------------------------------------------------------------
 // P2' (2D): per kx line of Sx (in place), y-FFT of 3 components, 2D kernel multiply (Kxy
 // odd in y), y-IFFT.  A block takes LP consecutive lines, one per group of T lanes.
-// nyd: data rows as in P2.
-template <int NY, int BS, bool FULL = false>
+// nyd: data rows as in P2.  V: radix schedule of NY.
+template <int NY, int BS, bool FULL = false, int V = 0>
 __global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ Sx, const float* __restrict__ K, int NC, int NYH, int ntl,
 	int nyd, const PassBatch* __restrict__ bt)
 {
 	if (bt) { const PassBatch& b = bt[blockIdx.y]; Sx = b.Sx; K = b.K; NC = b.NC; NYH = b.NYH; ntl = b.ntl; nyd = b.ny; }
-	constexpr int E = Plan<NY>::E, T = plan_T<NY>(), LP = BS / T, LL = lds_len<NY>(), EL = FULL ? E : E / 2;
+	constexpr int PK = plan_key(NY, V);
+	constexpr int E = Plan<PK>::E, T = plan_T<PK>(), LP = BS / T, LL = lds_len<NY>(), EL = FULL ? E : E / 2;
 	__shared__ C2<float> lds[3 * LP * LL];
 	const int t = threadIdx.x % T, l = threadIdx.x / T;
-	Twiddles<NY, float> tw; tw.init(t);
+	Twiddles<PK, float> tw; tw.init(t);
 	const size_t kq = (size_t)NC * NYH;
 	const int nwork = (NC + LP - 1) / LP;
 	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -394,7 +399,7 @@ __global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ Sx, const float
 				v[c][e] = mk(a.x, a.y);
 			}
 		C2<float>* b0 = lds + l * LL;
-		fft3<NY, -1, float>(v[0], v[1], v[2], b0, b0 + LP * LL, b0 + 2 * LP * LL, 1, t, tw);
+		fft3<PK, -1, float>(v[0], v[1], v[2], b0, b0 + LP * LL, b0 + 2 * LP * LL, 1, t, tw);
 #pragma unroll
 		for (int e = 0; e < E; e++) {
 			const int ky = t + T * e;
@@ -407,7 +412,7 @@ __global__ void __launch_bounds__(BS) k_y2d(float2* __restrict__ Sx, const float
 			v[1][e] = mk(kxy * mx.x + kyy * my.x, kxy * mx.y + kyy * my.y);
 			v[2][e] = mk(kzz * mz.x, kzz * mz.y);
 		}
-		fft3<NY, 1, float>(v[0], v[1], v[2], b0, b0 + LP * LL, b0 + 2 * LP * LL, 1, t, tw);
+		fft3<PK, 1, float>(v[0], v[1], v[2], b0, b0 + LP * LL, b0 + 2 * LP * LL, 1, t, tw);
 		if (ok) {
 #pragma unroll
 			for (int c = 0; c < 3; c++)
@@ -1267,13 +1272,75 @@ int bm::batch_inv_x_planar(int nx, LaunchShape sh, const PassBatch* h_tab, const
 	return fail(BM_ERR_SIZE, "no x plan");
 }
 
-int bm::batch_y(int Ny, int dir, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+// Batched passes with the radix schedule sh.radix (plan variant of the pass length).
+namespace {
+template <int NY, int V>
+int batch_y_v(int dir, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
 	const int pc = sh.per_cu ? sh.per_cu : 8;
+	constexpr int LP = kBlock / plan_T<plan_key(NY, V)>();
+	int mx = 1;
+	for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].nlines + LP - 1) / LP);
+	const dim3 g(grid8(mx, ncu, pc), nb);
+	if (dir < 0) hipLaunchKernelGGL((k_y<NY, -1, false, V>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, d_tab);
+	else hipLaunchKernelGGL((k_y<NY, 1, false, V>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, d_tab);
+	return BM_OK;
+}
+
+template <int NZ, int V>
+int batch_z_v(LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+{
+	const int pc = sh.per_cu ? sh.per_cu : 2;
+	constexpr int CH = kBlock / plan_T<plan_key(NZ, V)>() / 2;
+	int mx = 1;
+	for (int b = 0; b < nb; b++) mx = std::max(mx, h_tab[b].NC * ((h_tab[b].Ny / 2 + CH - 1) / CH));
+	hipLaunchKernelGGL((k_z<NZ, false, V>), dim3(grid8(mx, ncu, pc), nb), dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, z_xcd(), d_tab);
+	return BM_OK;
+}
+
+template <int NY, int V>
+int batch_y2d_v(LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+{
+	const int pc = sh.per_cu ? sh.per_cu : 8;
+	constexpr int PK = plan_key(NY, V), BS = Plan<PK>::E > 8 ? 128 : 256, LP = BS / plan_T<PK>();
+	int mx = 1;
+	for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].NC + LP - 1) / LP);
+	hipLaunchKernelGGL((k_y2d<NY, BS, false, V>), dim3(grid8(mx, ncu, pc), nb), dim3(BS), 0, st, nullptr, nullptr, 0, 0, 0, 0, d_tab);
+	return BM_OK;
+}
+
+// schedule v of length N, or BM_ERR_ARG when N has no such schedule
+template <int N, typename F> int with_variant(int v, F&& f)
+{
+	if (v == 0) return f(std::integral_constant<int, 0>());
+	if constexpr (plan_variants(N) > 1) if (v == 1) return f(std::integral_constant<int, 1>());
+	if constexpr (plan_variants(N) > 2) if (v == 2) return f(std::integral_constant<int, 2>());
+	return fail(BM_ERR_ARG, "no such radix schedule for this length");
+}
+}
+
+int bm::radix_variants(int n) { return plan_variants(n); }
+
+const char* bm::radix_schedule(int n, int v)
+{
+	switch (plan_key(n, v)) {
+#define S_(n_, v_) case plan_key(n_, v_): { static const std::string s = [] { std::string r; \
+		for (int q = 0; q < Plan<plan_key(n_, v_)>::S; q++) r += (q ? "x" : "") + std::to_string(Plan<plan_key(n_, v_)>::R[q]); return r; }(); return s.c_str(); }
+#define S3_(n_) S_(n_, 0) S_(n_, 1) S_(n_, 2)
+#define S2_(n_) S_(n_, 0) S_(n_, 1)
+	S2_(4) S3_(8) S3_(16) S3_(32) S3_(64) S2_(96) S3_(128) S2_(192) S3_(256) S2_(384) S3_(512) S2_(768) S3_(1024) S2_(2048)
+	S_(2, 0) S_(6, 0) S_(12, 0) S_(24, 0) S_(48, 0) S_(1536, 0) S_(4096, 0)
+#undef S_
+#undef S2_
+#undef S3_
+	}
+	return "";
+}
+
+int bm::batch_y(int Ny, int dir, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+{
 	switch (Ny) {
-#define C_(V_) case V_: { const int LP = kBlock / plan_T<V_>(); int mx = 1; \
-		for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].nlines + LP - 1) / LP); \
-		launch_y_t<V_>(dir, dim3(grid8(mx, ncu, pc), nb), st, nullptr, nullptr, 0, 0, 0, 0, 0, d_tab); return BM_OK; }
+#define C_(V_) case V_: return with_variant<V_>(sh.radix, [&](auto v) { return batch_y_v<V_, decltype(v)::value>(dir, sh, h_tab, d_tab, nb, ncu, st); });
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
@@ -1282,11 +1349,8 @@ int bm::batch_y(int Ny, int dir, LaunchShape sh, const PassBatch* h_tab, const P
 
 int bm::batch_z(int Nz, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
-	const int pc = sh.per_cu ? sh.per_cu : 2;
 	switch (Nz) {
-#define C_(V_) case V_: { const int CH = kBlock / plan_T<V_>() / 2; int mx = 1; \
-		for (int b = 0; b < nb; b++) mx = std::max(mx, h_tab[b].NC * ((h_tab[b].Ny / 2 + CH - 1) / CH)); \
-		hipLaunchKernelGGL(k_z<V_>, dim3(grid8(mx, ncu, pc), nb), dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, z_xcd(), d_tab); return BM_OK; }
+#define C_(V_) case V_: return with_variant<V_>(sh.radix, [&](auto v) { return batch_z_v<V_, decltype(v)::value>(sh, h_tab, d_tab, nb, ncu, st); });
 	BM_FOR_Z_SIZES(C_)
 #undef C_
 	}
@@ -1295,11 +1359,8 @@ int bm::batch_z(int Nz, LaunchShape sh, const PassBatch* h_tab, const PassBatch*
 
 int bm::batch_y2d(int Ny, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
-	const int pc = sh.per_cu ? sh.per_cu : 8;
 	switch (Ny) {
-#define C_(V_) case V_: { constexpr int BS = Plan<V_>::E > 8 ? 128 : 256; const int LP = BS / plan_T<V_>(); int mx = 1; \
-		for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].NC + LP - 1) / LP); \
-		hipLaunchKernelGGL((k_y2d<V_, BS>), dim3(grid8(mx, ncu, pc), nb), dim3(BS), 0, st, nullptr, nullptr, 0, 0, 0, 0, d_tab); return BM_OK; }
+#define C_(V_) case V_: return with_variant<V_>(sh.radix, [&](auto v) { return batch_y2d_v<V_, decltype(v)::value>(sh, h_tab, d_tab, nb, ncu, st); });
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}

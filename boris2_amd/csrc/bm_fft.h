@@ -46,9 +46,15 @@ __device__ __forceinline__ int xcd_chunk(int b, int nb)
 	return (b % 8) * per + b / 8;
 }
 
-template <int N> struct Plan;
-#define BM_PLAN(n, e, ...)                                                      \
-	template <> struct Plan<n> {                                                \
+// A plan is looked up by its key: the length N for the default schedule, N + 65536 v for
+// alternative schedule v >= 1 of the same length (other E and radices; the batched
+// autotuner times them against the default).  Template parameters named N below are keys.
+template <int K> struct Plan;
+constexpr int plan_key(int n, int v) { return n + 65536 * v; }
+#define BM_PLAN(n, e, ...) BM_PLANV(n, 0, e, __VA_ARGS__)
+#define BM_PLANV(n, v, e, ...)                                                  \
+	template <> struct Plan<plan_key(n, v)> {                                   \
+		static constexpr int L = n;                                             \
 		static constexpr int E = e;                                             \
 		static constexpr int R[] = { __VA_ARGS__ };                             \
 		static constexpr int S = sizeof(R) / sizeof(int);                       \
@@ -74,9 +80,43 @@ BM_PLAN(1024, 16, 16, 16, 4)
 BM_PLAN(1536, 12, 4, 4, 4, 4, 3, 2)
 BM_PLAN(2048, 16, 16, 16, 8)
 BM_PLAN(4096, 16, 16, 16, 16)
+// alternative schedules: v = 1 fewer threads per line (larger radices, fewer LDS exchanges),
+// v = 2 more threads per line (smaller radices)
+BM_PLANV(4, 1, 2, 2, 2)
+BM_PLANV(8, 1, 4, 4, 2)
+BM_PLANV(8, 2, 2, 2, 2, 2)
+BM_PLANV(16, 1, 16, 16)
+BM_PLANV(16, 2, 4, 4, 4)
+BM_PLANV(32, 1, 16, 16, 2)
+BM_PLANV(32, 2, 4, 4, 4, 2)
+BM_PLANV(64, 1, 16, 16, 4)
+BM_PLANV(64, 2, 4, 4, 4, 4)
+BM_PLANV(96, 1, 24, 8, 4, 3)
+BM_PLANV(128, 1, 16, 16, 8)
+BM_PLANV(128, 2, 4, 4, 4, 4, 2)
+BM_PLANV(192, 1, 24, 8, 8, 3)
+BM_PLANV(256, 1, 16, 16, 16)
+BM_PLANV(256, 2, 4, 4, 4, 4, 4)
+BM_PLANV(384, 1, 24, 8, 8, 3, 2)
+BM_PLANV(512, 1, 16, 16, 8, 4)
+BM_PLANV(512, 2, 4, 4, 4, 4, 4, 2)
+BM_PLANV(768, 1, 24, 8, 8, 4, 3)
+BM_PLANV(1024, 1, 8, 8, 8, 8, 2)
+BM_PLANV(1024, 2, 4, 4, 4, 4, 4, 4)
+BM_PLANV(2048, 1, 8, 8, 8, 8, 4)
 #undef BM_PLAN
+#undef BM_PLANV
+// number of schedules (the default and its alternatives) of length n
+constexpr int plan_variants(int n)
+{
+	switch (n) {
+	case 4: case 96: case 192: case 384: case 768: case 2048: return 2;
+	case 8: case 16: case 32: case 64: case 128: case 256: case 512: case 1024: return 3;
+	default: return 1;
+	}
+}
 
-template <int N> constexpr int plan_T() { return N / Plan<N>::E; }
+template <int N> constexpr int plan_T() { return Plan<N>::L / Plan<N>::E; }
 template <int N> constexpr int plan_Ns(int s) { int p = 1; for (int q = 0; q < s; q++) p *= Plan<N>::R[q]; return p; }
 template <int N> constexpr int plan_ntw()
 {
@@ -222,7 +262,7 @@ __device__ __forceinline__ void fft_stage(C2<T>* v, C2<T>* lds, int stride, int 
 #pragma unroll
 		for (int b = 0; b < B; b++)
 #pragma unroll
-			for (int r = 0; r < R; r++) v[b + r * B] = lds[lidx(t + b * TT + r * (N / R), stride)];
+			for (int r = 0; r < R; r++) v[b + r * B] = lds[lidx(t + b * TT + r * (Plan<N>::L / R), stride)];
 #pragma unroll
 		for (int b = 0; b < B; b++)
 #pragma unroll
@@ -283,7 +323,7 @@ __device__ __forceinline__ void fft3_stages(C2<T>* v0, C2<T>* v1, C2<T>* v2, C2<
 #pragma unroll
 			for (int b = 0; b < B; b++)
 #pragma unroll
-				for (int r = 0; r < R; r++) vv[c][b + r * B] = ll[c][lidx(t + b * TT + r * (N / R), stride)];
+				for (int r = 0; r < R; r++) vv[c][b + r * B] = ll[c][lidx(t + b * TT + r * (Plan<N>::L / R), stride)];
 #pragma unroll
 			for (int b = 0; b < B; b++)
 #pragma unroll

@@ -125,7 +125,12 @@ struct MlBatch { float2* S; const float* K; int NC, Ny, Sp; };
 // Launch shape of a batched pass (0 = the pass's default): f the x-pass row-tile factor
 // (1 or 2 times the base tile), per_cu the grid's workgroups per CU.  Picked per pass-size
 // group by the batched integrator's autotuner.
-struct LaunchShape { int f = 0, per_cu = 0; };
+// launch shape of a batched pass: row-tile factor (x passes), workgroups per CU, radix
+// schedule (plan variant of the pass length; 0 = default)
+struct LaunchShape { int f = 0, per_cu = 0, radix = 0; };
+// number of radix schedules of length n, and schedule v as text ("16x16x4")
+int radix_variants(int n);
+const char* radix_schedule(int n, int v);
 int batch_ml_mult(int L, LaunchShape sh, const MlBatch* h_tab, const MlBatch* d_tab, int nb, int ncu, hipStream_t st);
 // rows per work item of the x passes for nx (0 if there is no plan)
 int x_tile_rows(int nx, bool fwd, int f = 0);

@@ -303,13 +303,20 @@ int bm_ode_average_m(bm_ode_t o, double* out3, void* stream);
  * hipGraph and replayed (BM_BATCH_NO_GRAPH: launch directly).  BM_BATCH_AUTOTUNE: before
  * the first step, time candidate launch shapes of every pass-size group (x-pass row-tile
  * factor, workgroups per CU) on the real tables and keep the fastest;
- * bm_batch_autotune_report writes the choices and times as text lines.
+ * BM_BATCH_AUTOTUNE_RADIX (implies BM_BATCH_AUTOTUNE): the y, z and 2D y passes' radix
+ * schedules (alternative plans of each FFT length) are timed as well; results then differ
+ * from the default schedules by FP32 rounding (launch shapes alone change no bit).
+ * bm_batch_autotune_report writes the choices and times as text lines
+ * ("<pass> <size> f=<f> per_cu=<n> radix=<v> <radices> <us> us"); bm_batch_set_schedule
+ * pins one pass-size group's choice (e.g. read back from an earlier report), replacing any
+ * pending tuning: f (x passes 0..2, else 0; 0 = default), per_cu (0 = default, <= 64),
+ * radix (0 = default; y/z/y2d passes only); BM_ERR_ARG for a pass or size the batch lacks.
  * bm_batch_create_layers: layers[b].nlayers > 0 makes mesh b a layer stack as
  * bm_ode_set_layers does (nz == nlayers <= 16, demag on; hz, z0 its layers' thicknesses and
  * bottom heights, Ms/A/alpha/K1 per layer or NULL); mesh_of_layer and Ha (or NULL) make it a
  * stack of 3D meshes as bm_ode_set_layer_meshes does; layers NULL = bm_batch_create. */
 typedef struct bm_batch_s* bm_batch_t;
-enum { BM_BATCH_NO_GRAPH = 1, BM_BATCH_AUTOTUNE = 2 };
+enum { BM_BATCH_NO_GRAPH = 1, BM_BATCH_AUTOTUNE = 2, BM_BATCH_AUTOTUNE_RADIX = 4 };
 int bm_batch_create(int method, int nmesh, const int* n3, const double* h3, const bm_fm_params* params,
 	float* M, int flags, bm_batch_t* out);
 typedef struct {
@@ -336,6 +343,7 @@ int bm_batch_set_astep(bm_batch_t b, double err_fail, double dT_increase, double
 int bm_batch_get_mesh_state(bm_batch_t b, int mesh, double* time, double* dT, long long* iteration, long long* rejections);
 int bm_batch_mesh_offset(bm_batch_t b, int mesh, long long* cell_offset, long long* ncells);
 int bm_batch_autotune_report(bm_batch_t b, char* out, int cap);
+int bm_batch_set_schedule(bm_batch_t b, const char* pass, int size, int f, int per_cu, int radix);
 
 /* ------------------------------------------------------------------------------------
  * Differentiation of the fixed-step RK4 rollout (no reference counterpart: the reference
