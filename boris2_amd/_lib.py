@@ -28,7 +28,7 @@ EXPORTS = [
     "bm_mlayer_forward", "bm_mlayer_multiply", "bm_mlayer_inverse",
     "bm_batch_create", "bm_batch_create_layers", "bm_batch_destroy", "bm_batch_set_dt", "bm_batch_run", "bm_batch_get_state",
     "bm_batch_mesh_offset", "bm_batch_autotune_report", "bm_batch_set_schedule", "bm_batch_set_astep", "bm_batch_get_mesh_state",
-    "bm_adj_create", "bm_adj_destroy", "bm_adj_forward", "bm_adj_backward", "bm_adj_zeeman_grads", "bm_adj_set_layers",
+    "bm_adj_create", "bm_adj_destroy", "bm_adj_forward", "bm_adj_backward", "bm_adj_zeeman_grads", "bm_adj_set_layers", "bm_adj_set_pbc",
     "bm_ovf2_info", "bm_ovf2_read", "bm_ovf2_write", "bm_adj_batch_forward", "bm_adj_batch_backward",
 ]
 
@@ -119,6 +119,7 @@ def load() -> C.CDLL:
         "bm_adj_backward": (i, [vp, vp, C.POINTER(d), vp]),
         "bm_adj_zeeman_grads": (i, [vp, C.POINTER(d)]),
         "bm_adj_set_layers": (i, [vp, vp, vp, vp, vp]),
+        "bm_adj_set_pbc": (i, [vp, i, i, i]),
         "bm_adj_batch_forward": (i, [i, vp, vp, vp, vp, vp, vp]),
         "bm_adj_batch_backward": (i, [i, vp, vp, vp, vp]),
         "bm_ode_create": (i, [i, i, i, i, d, d, d, C.POINTER(FmParams), vp, vp, C.POINTER(vp)]),

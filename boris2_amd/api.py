@@ -574,10 +574,11 @@ class RK4Adjoint:
     map.  loss = 1 - <(M_final/Ms) . m_target>; backward() gives dL/dMs (per cell) and
     dL/d amp by the discrete adjoint, zeeman_grads() (dL/d amp, dL/d slope xyz).
     layers = dict(hz=..., z0=..., A=None, alpha=None, K1=None) makes the n[2] planes a stack
-    of 2D layers (hx, hy from h) coupled by the multilayered demag (params must enable demag)."""
+    of 2D layers (hx, hy from h) coupled by the multilayered demag (params must enable demag).
+    pbc: periodic images along x, y, z (exchange wraps around, demag sums the images)."""
 
     def __init__(self, n, h, params: "FMParams", Ms_map: torch.Tensor, dt: float, nsteps: int, ckpt_every: int = 100,
-                 layers: dict | None = None):
+                 layers: dict | None = None, pbc: tuple[int, int, int] = (0, 0, 0)):
         require_device()
         self.L = _lib.load()
         self.n = tuple(int(v) for v in n)
@@ -592,7 +593,7 @@ class RK4Adjoint:
             self.mlayer = MultiLayerDemag(self.n[0], self.n[1], self.h[0], self.h[1], layers["hz"], layers["z0"])
             pc = FMParams(**{**params.__dict__, "modules": params.modules & ~BM_MOD_DEMAG})
         elif params.modules & BM_MOD_DEMAG:
-            self.demag = Demag(self.n, self.h)
+            self.demag = Demag(self.n, self.h, pbc)
         self._pc = pc.to_c()
         self.handle = C.c_void_p()
         check(self.L.bm_adj_create(*self.n, *self.h, C.byref(self._pc), self.demag.handle if self.demag else None,
@@ -602,6 +603,8 @@ class RK4Adjoint:
                          for k in ("A", "alpha", "K1")]
             ptrs = [None if v is None else v.ctypes.data_as(C.c_void_p) for v in self._lay]
             check(self.L.bm_adj_set_layers(self.handle, self.mlayer.handle, *ptrs))
+        if any(pbc):
+            check(self.L.bm_adj_set_pbc(self.handle, *(int(v) for v in pbc)))
 
     def forward(self, m0: torch.Tensor, mt: torch.Tensor) -> tuple[float, torch.Tensor]:
         self._m0 = m0.detach().to(device="cuda", dtype=torch.float32).contiguous()

@@ -13,8 +13,10 @@
 // during the forward recomputation) and one demag convolution of the field adjoint u — the
 // single-mesh demag operator is symmetric (N_ij = N_ji, N(r) = N(-r)), so its adjoint is the
 // same convolution; the multilayered one runs its exact transpose (pair kernels swapped,
-// imaginary xz/yz parts negated); the free-boundary exchange Laplacian is symmetric, with
-// the per-cell prefactor 2A/(mu0 Ms^2) moved to the input side; anisotropy is local.
+// imaginary xz/yz parts negated); the exchange Laplacian is symmetric (free boundaries, and
+// periodic axes, whose wrapped stencil is circulant), with the per-cell prefactor
+// 2A/(mu0 Ms^2) moved to the input side; anisotropy is local.  A periodic demag kernel sums
+// images at +-k L, which keeps N(r) = N(-r), so periodic meshes reuse the same convolution.
 //
 // Memory: the state is checkpointed every K steps; the backward pass recomputes one segment
 // of K steps at a time with a tape of the four stage inputs, their demag fields and the
@@ -48,6 +50,7 @@ struct AdjArgs {
 	int modules;
 	const float* Ms;
 	int layered;            // layer stack: per-layer A, alpha, K1, no exchange across layers
+	int per;                // periodic exchange along x, y, z: bits 0, 1, 2
 	float lay_A[kMaxAdjLayers], lay_alpha[kMaxAdjLayers], lay_K1[kMaxAdjLayers];
 };
 
@@ -66,18 +69,25 @@ __device__ __forceinline__ F3 lap_at(const float* X, const AdjArgs& a, size_t id
 	const F3 m = ld3(X, idx);
 	F3 lx{ 0, 0, 0 }, ly{ 0, 0, 0 }, lz{ 0, 0, 0 };
 	const size_t sy = a.nx, sz = (size_t)a.nx * a.ny;
+	// periodic axes at the mesh ends: the wrapped neighbour (BorisLib/VEC_VC_Del.h:24-45)
 	if (a.nx > 1) {
 		if (i > 0 && i < a.nx - 1) lx = ld3(X, idx + 1) + ld3(X, idx - 1) - 2.f * m;
+		else if (a.per & 1) lx = (i == 0) ? ld3(X, idx + 1) + ld3(X, idx + (a.nx - 1)) - 2.f * m
+		                                  : ld3(X, idx - (a.nx - 1)) + ld3(X, idx - 1) - 2.f * m;
 		else if (i == 0) lx = ld3(X, idx + 1) - m;
 		else lx = ld3(X, idx - 1) - m;
 	}
 	if (a.ny > 1) {
 		if (j > 0 && j < a.ny - 1) ly = ld3(X, idx + sy) + ld3(X, idx - sy) - 2.f * m;
+		else if (a.per & 2) ly = (j == 0) ? ld3(X, idx + sy) + ld3(X, idx + (a.ny - 1) * sy) - 2.f * m
+		                                  : ld3(X, idx - (a.ny - 1) * sy) + ld3(X, idx - sy) - 2.f * m;
 		else if (j == 0) ly = ld3(X, idx + sy) - m;
 		else ly = ld3(X, idx - sy) - m;
 	}
 	if (a.nz > 1 && !a.layered) {
 		if (k > 0 && k < a.nz - 1) lz = ld3(X, idx + sz) + ld3(X, idx - sz) - 2.f * m;
+		else if (a.per & 4) lz = (k == 0) ? ld3(X, idx + sz) + ld3(X, idx + (a.nz - 1) * sz) - 2.f * m
+		                                  : ld3(X, idx - (a.nz - 1) * sz) + ld3(X, idx - sz) - 2.f * m;
 		else if (k == 0) lz = ld3(X, idx + sz) - m;
 		else lz = ld3(X, idx - sz) - m;
 	}
@@ -269,6 +279,7 @@ struct bm_adj_s {
 	bm_fm_params p;
 	bm_demag_t demag;
 	bm_mlayer_t ml = nullptr;     // layer stack (bm_adj_set_layers)
+	int pbc[3] = { 0, 0, 0 };     // periodic images per axis (bm_adj_set_pbc)
 	bool lay_set[3] = { false, false, false };
 	float lay[3][kMaxAdjLayers] = {};
 	double gz[4] = { 0, 0, 0, 0 };  // dL/d amp, dL/d slope of the last backward pass
@@ -327,6 +338,7 @@ AdjArgs adj_args(bm_adj_s* o, int step)
 	a.tz = (o->p.wave == 2 && (o->p.modules & BM_MOD_ZEEMAN)) ? (float)t : 0.f;
 	a.modules = o->p.modules;
 	a.Ms = o->Ms;
+	a.per = (o->pbc[0] ? 1 : 0) | (o->pbc[1] ? 2 : 0) | (o->pbc[2] ? 4 : 0);
 	return a;
 }
 
@@ -653,6 +665,7 @@ extern "C" int bm_adj_set_layers(bm_adj_t o, bm_mlayer_t ml, const double* A, co
 	int L, nx, ny;
 	mlayer_dims(ml, &L, &nx, &ny, nullptr);
 	if (mlayer_mapped(ml)) return fail(BM_ERR_SIZE, "bm_adj_set_layers: the layers must share one planned grid");
+	if (o->pbc[0] || o->pbc[1] || o->pbc[2]) return fail(BM_ERR_STATE, "bm_adj_set_layers: the mesh is periodic (layer stacks have no periodic variant)");
 	if (L != o->nz || nx != o->nx || ny != o->ny) return fail(BM_ERR_ARG, "bm_adj_set_layers: the stack's layers must be the mesh's nz planes (same nx, ny)");
 	if (L > kMaxAdjLayers) return fail(BM_ERR_SIZE, "bm_adj_set_layers: at most 8 layers");
 	const double* src[3] = { A, alpha, K1 };
@@ -662,6 +675,21 @@ extern "C" int bm_adj_set_layers(bm_adj_t o, bm_mlayer_t ml, const double* A, co
 	}
 	o->ml = ml;
 	o->p.modules |= BM_MOD_DEMAG;
+	o->forward_done = false;
+	return BM_OK;
+}
+
+// Periodic boundary conditions for the rollout (the contract of bm_ode_set_pbc): exchange
+// wraps around along every axis with images, and with demag on the handle given at creation
+// must carry the same images.  The adjoint pass needs nothing else: the wrapped Laplacian is
+// symmetric and the periodic demag operator is its own adjoint.
+extern "C" int bm_adj_set_pbc(bm_adj_t o, int pbc_x, int pbc_y, int pbc_z)
+{
+	if (!o || pbc_x < 0 || pbc_y < 0 || pbc_z < 0) return fail(BM_ERR_ARG, "bm_adj_set_pbc: bad arguments");
+	if (o->ml) return fail(BM_ERR_STATE, "bm_adj_set_pbc: layer stacks have no periodic variant");
+	if ((o->p.modules & BM_MOD_DEMAG) && (o->demag->pbc[0] != pbc_x || o->demag->pbc[1] != pbc_y || o->demag->pbc[2] != pbc_z))
+		return fail(BM_ERR_STATE, "bm_adj_set_pbc: the demag handle must have the same periodic images (bm_demag_create_pbc)");
+	o->pbc[0] = pbc_x; o->pbc[1] = pbc_y; o->pbc[2] = pbc_z;
 	o->forward_done = false;
 	return BM_OK;
 }

@@ -356,7 +356,10 @@ int bm_batch_set_schedule(bm_batch_t b, const char* pass, int size, int f, int p
  * state is checkpointed every ckpt_every steps and each segment recomputed once in the
  * backward pass.  bm_adj_set_layers (before bm_adj_forward) makes the nz planes a layer
  * stack as bm_ode_set_layers does (multilayered demag with its exact transpose in the
- * backward pass, per-layer A, alpha, K1; per-layer Ms through Ms_map). */
+ * backward pass, per-layer A, alpha, K1; per-layer Ms through Ms_map).  bm_adj_set_pbc
+ * (before bm_adj_forward) makes the mesh periodic as bm_ode_set_pbc does: exchange wraps
+ * around along each axis with images and, with demag on, the handle must have been made by
+ * bm_demag_create_pbc with the same images (BM_ERR_STATE otherwise); not for layer stacks. */
 typedef struct bm_adj_s* bm_adj_t;
 int bm_adj_create(int nx, int ny, int nz, double hx, double hy, double hz, const bm_fm_params* params,
 	bm_demag_t demag, const float* Ms_map, double dt, int nsteps, int ckpt_every, bm_adj_t* out);
@@ -366,6 +369,7 @@ int bm_adj_forward(bm_adj_t a, const float* m0, const float* mt, float* M_final,
 int bm_adj_backward(bm_adj_t a, float* dL_dMs, double* dL_damp, void* stream);
 int bm_adj_zeeman_grads(bm_adj_t a, double* out4);
 int bm_adj_set_layers(bm_adj_t a, bm_mlayer_t ml, const double* A, const double* alpha, const double* K1);
+int bm_adj_set_pbc(bm_adj_t a, int pbc_x, int pbc_y, int pbc_z);
 /* Batched adjoint (no reference counterpart): `count` adjoint handles driven in lockstep, each
  * on its own internal stream so that the instances' kernels overlap on the device (the
  * differentiable multilayer workload runs 16 stacks this way).  Work is ordered after earlier

@@ -37,6 +37,7 @@ struct AdjRK4 {
 	std::shared_ptr<MLayer<double>> ml;
 	bool layered = false;
 	std::vector<double> layA, layAlpha, layK1;
+	int pbc[3] = { 0, 0, 0 };   // periodic images per axis (set_pbc)
 	double dt = 0; int nsteps = 0;
 	// tape: per step the four stage inputs X1..X4 and the pre-renormalisation sum Y
 	std::vector<std::vector<double>> X[4], Y;
@@ -50,6 +51,14 @@ struct AdjRK4 {
 		if (p.demag) dem = new Demag<double>(nx, ny, nz, h[0], h[1], h[2], Nz_override);
 	}
 	~AdjRK4() { delete dem; }
+
+	// periodic images per axis (the `pbc` command, Boris/Demag.cpp:137-148,
+	// Boris/MeshControl.cpp:49-65): the demag tensor sums the images and exchange wraps around
+	void set_pbc(int px, int py, int pz)
+	{
+		pbc[0] = px; pbc[1] = py; pbc[2] = pz;
+		if (dem) { delete dem; dem = new Demag<double>(nx, ny, nz, h[0], h[1], h[2], 0, pbc); }
+	}
 
 	void set_layers(int L, double hx, double hy, const double* hz, const double* z0, bool with_demag,
 		const double* A, const double* alpha, const double* K1)
@@ -72,7 +81,8 @@ struct AdjRK4 {
 		return (std::fabs(x) < 1e-300) ? 1.0 : std::sin(x) / x;
 	}
 
-	// second differences / h^2 with free boundaries (the exchange Laplacian, unscaled)
+	// second differences / h^2 with free boundaries, or the wrapped neighbour at the ends of a
+	// periodic axis (BorisLib/VEC_VC_Del.h:24-45) (the exchange Laplacian, unscaled)
 	void laplacian(const double* V, double* out) const
 	{
 		const int d[3] = { nx, ny, nz };
@@ -88,6 +98,11 @@ struct AdjRK4 {
 					const double m0 = V[idx * 3 + c];
 					double df = 0;
 					if (lo && hi) df = V[(idx + st[a]) * 3 + c] + V[(idx - st[a]) * 3 + c] - 2 * m0;
+					else if (pbc[a] && d[a] > 1) {
+						const size_t up = hi ? idx + st[a] : idx - (size_t)(d[a] - 1) * st[a];
+						const size_t dn = lo ? idx - st[a] : idx + (size_t)(d[a] - 1) * st[a];
+						df = V[up * 3 + c] + V[dn * 3 + c] - 2 * m0;
+					}
 					else if (hi) df = V[(idx + st[a]) * 3 + c] - m0;
 					else if (lo) df = V[(idx - st[a]) * 3 + c] - m0;
 					l[c] += df / (h[a] * h[a]);

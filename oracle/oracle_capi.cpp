@@ -196,6 +196,8 @@ void or_adj_destroy(void* h) { delete (AdjRK4*)h; }
 
 // keep M every K steps only and recompute segments in the backward pass (0: whole tape)
 void or_adj_set_checkpoint(void* h, int K) { ((AdjRK4*)h)->ckK = K; }
+// periodic images per axis (demag tensor with images, wrapped exchange)
+void or_adj_set_pbc(void* h, int px, int py, int pz) { ((AdjRK4*)h)->set_pbc(px, py, pz); }
 // forward from M0 = Ms m0 (m0: n*3); final M into Mf; returns the loss against mt
 double or_adj_forward(void* h, const double* m0, const double* mt, double* Mf)
 {

@@ -74,6 +74,7 @@ def lib() -> C.CDLL:
     L.or_adj_create.argtypes = [i, i, i, dp, dp, i, dp, d, i, i]; L.or_adj_create.restype = vp
     L.or_adj_destroy.argtypes = [vp]
     L.or_adj_set_checkpoint.argtypes = [vp, i]
+    L.or_adj_set_pbc.argtypes = [vp, i, i, i]
     L.or_adj_forward.argtypes = [vp, dp, dp, dp]; L.or_adj_forward.restype = d
     L.or_adj_backward.argtypes = [vp, dp, dp, dp, dp]
     L.or_adj_set_layers.argtypes = [vp, i, d, d, dp, dp, i, vp, vp, vp]
@@ -449,8 +450,9 @@ class OracleAdjoint:
     """FP64 discrete adjoint of the fixed-step RK4 rollout: loss 1 - <(M/Ms) . m_target> of
     the final state from M0 = Ms m0; gradients dL/dMs (per cell) and dL/d amp."""
 
-    def __init__(self, n, h, params, Ms_map, dt, nsteps, Nz_override=0, ckpt=0):
-        """ckpt > 0: keep M every ckpt steps and recompute segments in backward() (same bits)."""
+    def __init__(self, n, h, params, Ms_map, dt, nsteps, Nz_override=0, ckpt=0, pbc=(0, 0, 0)):
+        """ckpt > 0: keep M every ckpt steps and recompute segments in backward() (same bits);
+        pbc: periodic images per axis."""
         self.L = lib()
         self.n = tuple(n)
         self.ncell = n[0] * n[1] * n[2]
@@ -460,6 +462,8 @@ class OracleAdjoint:
         assert self.h
         if ckpt:
             self.L.or_adj_set_checkpoint(self.h, int(ckpt))
+        if any(pbc):
+            self.L.or_adj_set_pbc(self.h, *(int(v) for v in pbc))
 
     def forward(self, m0, mt):
         Mf = np.zeros(self.ncell * 3)

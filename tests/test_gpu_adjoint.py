@@ -152,3 +152,32 @@ def test_batched_adjoint_equals_one_by_one(gpu):
         assert np.array_equal(rs[b].zeeman_grads(), gz)
     for r in rs:
         r.close()
+
+
+@pytest.mark.parametrize("n,pbc", [((16, 8, 1), (3, 2, 0)), ((8, 8, 4), (2, 0, 2))], ids=["2d_xy", "3d_xz"])
+def test_gpu_periodic_adjoint_matches_oracle(gpu, n, pbc):
+    """Periodic meshes (bm_adj_set_pbc with a demag handle of the same images) against the
+    oracle's periodic adjoint, itself pinned by central differences."""
+    n, p, Ms, m0, mt = problem(n=n, seed=7)
+    dt, steps = 250e-15, 30
+    a = O.OracleAdjoint(n, (4e-9,) * 3, p, Ms, dt, steps, pbc=pbc)
+    L0, _ = a.forward(m0, mt)
+    g0, ga0 = a.backward(m0, mt)
+    r = api.RK4Adjoint(n, (4e-9,) * 3, p, torch.tensor(Ms, dtype=torch.float32), dt, steps, 8, pbc=pbc)
+    L, _ = r.forward(torch.tensor(m0, dtype=torch.float32), torch.tensor(mt, dtype=torch.float32))
+    g, ga = r.backward()
+    g = g.cpu().numpy().astype(np.float64)
+    eg, ea = np.linalg.norm(g - g0) / np.linalg.norm(g0), abs(ga - ga0) / abs(ga0)
+    print(f"pbc {pbc}: loss {abs(L - L0):.2e}, dL/dMs rel L2 {eg:.2e}, dL/damp rel {ea:.2e}")
+    assert abs(L - L0) <= 1e-6
+    assert eg <= 5e-5 and ea <= 5e-5
+    r.close()
+
+
+def test_gpu_adjoint_pbc_needs_matching_demag(gpu):
+    n, p, Ms, m0, mt = problem(n=(8, 8, 1), seed=3)
+    from boris2_amd import _lib
+    r = api.RK4Adjoint(n, (4e-9,) * 3, p, torch.tensor(Ms, dtype=torch.float32), 250e-15, 4, 2)
+    with pytest.raises(RuntimeError, match="error 5"):
+        _lib.check(r.L.bm_adj_set_pbc(r.handle, 1, 0, 0))   # its demag handle has no images
+    r.close()

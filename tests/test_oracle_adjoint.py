@@ -125,3 +125,50 @@ def test_layered_adjoint_with_anisotropy_and_ramp_matches_central_fd():
         sp[c] += hsl; sm[c] -= hsl
         fd = (L(api.FMParams(**{**p.__dict__, "slope": tuple(sp)}), Ms) - L(api.FMParams(**{**p.__dict__, "slope": tuple(sm)}), Ms)) / (2 * hsl)
         assert abs(gz[1 + c] - fd) <= 1e-6 * abs(fd), (c, gz[1 + c], fd)
+
+
+PBC_CASES = [((8, 8, 1), (3, 2, 0)), ((8, 6, 4), (2, 0, 2))]
+
+
+@pytest.mark.parametrize("n,pbc", PBC_CASES, ids=["2d_xy", "3d_xz"])
+def test_periodic_adjoint_matches_central_fd(n, pbc):
+    """Periodic meshes: the wrapped exchange stencil and the image-summed demag tensor are
+    both their own transposes, so the same adjoint sweep applies; central differences of the
+    periodic forward pin it, at cells on the wrapped ends and inside."""
+    _, p, Ms, m0, mt, dt, steps = setup(n=n, steps=20, seed=5)
+    h = (4e-9,) * 3
+    loss_p = lambda Ms_, p_: O.OracleAdjoint(n, h, p_, Ms_, dt, steps, pbc=pbc).forward(m0, mt)[0]
+    a = O.OracleAdjoint(n, h, p, Ms, dt, steps, pbc=pbc)
+    L0, _ = a.forward(m0, mt)
+    g, ga = a.backward(m0, mt)
+    # the images change the rollout (else this would test the open mesh again)
+    assert abs(L0 - loss_at(n, p, Ms, m0, mt, dt, steps)) > 1e-7
+    nx, ny = n[0], n[1]
+    cells = [0, nx - 1, (ny - 1) * nx, nx * ny - 1, len(Ms) - 1, nx + 3]
+    for i in cells:
+        hs = 1e-5 * Ms[i]
+        Mp, Mm = Ms.copy(), Ms.copy()
+        Mp[i] += hs; Mm[i] -= hs
+        fd = (loss_p(Mp, p) - loss_p(Mm, p)) / (2 * hs)
+        assert abs(g[i] - fd) <= 1e-6 * abs(fd) + 1e-18, (i, g[i], fd)
+    ha = 1e-5 * p.amp
+    fd = (loss_p(Ms, api.FMParams(**{**p.__dict__, "amp": p.amp + ha}))
+          - loss_p(Ms, api.FMParams(**{**p.__dict__, "amp": p.amp - ha}))) / (2 * ha)
+    assert abs(ga - fd) <= 1e-6 * abs(fd), (ga, fd)
+
+
+@pytest.mark.parametrize("n,pbc", PBC_CASES, ids=["2d_xy", "3d_xz"])
+def test_periodic_adjoint_forward_equals_periodic_sim(n, pbc):
+    """The adjoint's periodic forward rollout is the oracle integrator's periodic RK4 run
+    (set_pbc: image tensor and wrapped exchange) on the same Ms map."""
+    _, p, Ms, m0, mt, dt, steps = setup(n=n, steps=12, seed=6)
+    a = O.OracleAdjoint(n, (4e-9,) * 3, p, Ms, dt, steps, pbc=pbc)
+    _, Mf = a.forward(m0, mt)
+    s = O.OracleSim(api.BM_EVAL_RK4, n, (4e-9,) * 3, p)
+    s.set_pbc(pbc)
+    s.set_ms_map(Ms)
+    s.set_M(m0 * Ms[:, None])
+    s.set_dt(dt)
+    s.run_to_iteration(steps)
+    M = s.get_M()
+    assert np.max(np.abs(Mf - M)) <= 1e-9 * np.max(Ms)
