@@ -27,7 +27,7 @@ EXPORTS = [
     "bm_mlayer_create", "bm_mlayer_create_meshes", "bm_mlayer_destroy", "bm_mlayer_convolute", "bm_mlayer_copy_kernels",
     "bm_mlayer_forward", "bm_mlayer_multiply", "bm_mlayer_inverse",
     "bm_batch_create", "bm_batch_create_layers", "bm_batch_destroy", "bm_batch_set_dt", "bm_batch_run", "bm_batch_get_state",
-    "bm_batch_mesh_offset", "bm_batch_autotune_report", "bm_batch_set_schedule", "bm_batch_set_astep", "bm_batch_get_mesh_state",
+    "bm_batch_mesh_offset", "bm_batch_autotune_report", "bm_batch_set_schedule", "bm_batch_set_pbc", "bm_batch_set_astep", "bm_batch_get_mesh_state",
     "bm_adj_create", "bm_adj_destroy", "bm_adj_forward", "bm_adj_backward", "bm_adj_zeeman_grads", "bm_adj_set_layers", "bm_adj_set_pbc",
     "bm_ovf2_info", "bm_ovf2_read", "bm_ovf2_write", "bm_adj_batch_forward", "bm_adj_batch_backward",
 ]
@@ -111,6 +111,7 @@ def load() -> C.CDLL:
         "bm_batch_mesh_offset": (i, [vp, i, C.POINTER(ll), C.POINTER(ll)]),
         "bm_batch_autotune_report": (i, [vp, C.c_char_p, i]),
         "bm_batch_set_schedule": (i, [vp, C.c_char_p, i, i, i, i]),
+        "bm_batch_set_pbc": (i, [vp, i, i, i, i]),
         "bm_batch_set_astep": (i, [vp, d, d, d, d]),
         "bm_batch_get_mesh_state": (i, [vp, i, C.POINTER(d), C.POINTER(d), C.POINTER(ll), C.POINTER(ll)]),
         "bm_adj_create": (i, [i, i, i, d, d, d, C.POINTER(FmParams), vp, vp, d, i, i, C.POINTER(vp)]),

@@ -469,9 +469,10 @@ class BatchSolver:
     K1=None) making that mesh's n[2] planes a layer stack (as LayerStackSolver).
     autotune: False, True (launch shapes timed before the first step; results unchanged bit
     for bit) or "radix" (the FFT passes' radix schedules timed too; results within FP32
-    rounding of the default schedules)."""
+    rounding of the default schedules).  pbc: None, or per mesh None or (px, py, pz) periodic
+    images (as FMSolver's pbc)."""
 
-    def __init__(self, method: int, meshes, M0=None, use_graph: bool = True, layers=None, autotune=False):
+    def __init__(self, method: int, meshes, M0=None, use_graph: bool = True, layers=None, autotune=False, pbc=None):
         require_device()
         self.L = _lib.load()
         self.meshes = [(tuple(int(v) for v in n), tuple(float(v) for v in h), p) for n, h, p in meshes]
@@ -511,6 +512,9 @@ class BatchSolver:
                                             C.cast(self._pc, C.c_void_p), C.cast(ld, C.c_void_p) if ld is not None else None,
                                             _ptr(self.M), (0 if use_graph else 1) | (6 if autotune == "radix" else 2 if autotune else 0),
                                             C.byref(self.handle)))
+        for b, pb in enumerate(pbc or []):
+            if pb is not None and any(pb):
+                check(self.L.bm_batch_set_pbc(self.handle, b, *(int(v) for v in pb)))
 
     def mesh(self, b: int) -> torch.Tensor:
         return self.M[int(self.offsets[b]):int(self.offsets[b + 1])]

@@ -25,10 +25,16 @@
 // multilayered demag, per-layer Ms, A, alpha, K1, as bm_ode_set_layers): its x and y passes
 // join the launches of its sizes, its kernel product is one launch per layer count, and its
 // stage arguments carry the per-layer values.
+//
+// A mesh may be periodic (bm_batch_set_pbc, the reference's `pbc` command): its demag handle
+// sums the images, and along a periodic axis N = n, so its passes run the full-length
+// variants; those form groups of their own (a group is a pass size and its periodicity),
+// and its stage arguments wrap exchange around.
 #include "bm_internal.h"
 #include "bm_stage.h"
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <vector>
 #include <string>
@@ -47,6 +53,7 @@ template <typename T> struct Tab {
 // [1] inverse; y groups: [0] forward, [1] inverse; others [0]), with the autotuned time.
 struct Group {
 	int size;
+	bool full;                 // the pass runs along a periodic axis (N = n, every value data)
 	std::vector<int> meshes;
 	LaunchShape sh[2];
 	double us[2] = { 0, 0 };
@@ -61,6 +68,8 @@ struct bm_batch_s {
 	size_t ntot = 0, max_cells = 0;
 	std::vector<bm_fm_params> p;
 	std::vector<double> h;
+	std::vector<std::array<int, 3>> pbc;   // periodic images per mesh and axis
+	bool started = false;                  // the first bm_batch_run has begun
 	std::vector<bm_demag_t> dm;
 	std::vector<bm_mlayer_t> ml;   // layer stacks: their multilayered demag (else null)
 	std::vector<std::vector<float>> lay;  // layer stacks: A, alpha, K1 per layer (3 x kMaxStackLayers)
@@ -126,10 +135,10 @@ void free_all(bm_batch_s* o)
 	if (o->dev_tables) (void)hipFree(o->dev_tables);
 }
 
-void add_to(std::vector<Group>& gs, int size, int b)
+void add_to(std::vector<Group>& gs, int size, bool full, int b)
 {
-	for (Group& g : gs) if (g.size == size) { g.meshes.push_back(b); return; }
-	gs.push_back(Group{ size, { b } });
+	for (Group& g : gs) if (g.size == size && g.full == full) { g.meshes.push_back(b); return; }
+	gs.push_back(Group{ size, full, { b } });
 }
 
 StageArgs stage_args(bm_batch_s* o, int b, int parity)
@@ -152,6 +161,7 @@ StageArgs stage_args(bm_batch_s* o, int b, int parity)
 	for (int c = 0; c < 3; c++) { a.ea[c] = (float)p.ea[c]; a.Ha[c] = (float)p.Ha[c]; }
 	a.dT = (float)o->dT;
 	a.modules = p.modules;
+	a.per = (o->pbc[b][0] ? 1 : 0) | (o->pbc[b][1] ? 2 : 0) | (o->pbc[b][2] ? 4 : 0);
 	a.map = 2;
 	if (o->d_ctl) {
 		a.ctl = o->d_ctl + b;
@@ -275,15 +285,18 @@ void build_groups(bm_batch_s* o)
 	for (int b = 0; b < nb; b++) {
 		if (o->ml[b]) {
 			const MlView v = mlayer_view(o->ml[b]);
-			add_to(o->gx, o->nx[b], b);
-			add_to(o->gy, v.Ny, b);
-			add_to(o->gml, v.L, b);
+			add_to(o->gx, o->nx[b], false, b);
+			add_to(o->gy, v.Ny, false, b);
+			add_to(o->gml, v.L, false, b);
 			continue;
 		}
-		if (!o->dm[b]) continue;
-		add_to(o->gx, o->nx[b], b);
-		if (o->nz[b] > 1) { add_to(o->gy, o->dm[b]->Ny, b); add_to(o->gz, o->dm[b]->Nz, b); }
-		else add_to(o->gy2, o->dm[b]->Ny, b);
+		const bm_demag_s* d = o->dm[b];
+		if (!d) continue;
+		// a periodic x pass transforms rows of N.x = nx cells as nx/2-point complex FFTs
+		const bool fx = d->pbc[0] > 0, fy = d->pbc[1] > 0, fz = d->pbc[2] > 0;
+		add_to(o->gx, fx ? d->Nx / 2 : o->nx[b], fx, b);
+		if (o->nz[b] > 1) { add_to(o->gy, d->Ny, fy, b); add_to(o->gz, d->Nz, fz, b); }
+		else add_to(o->gy2, d->Ny, fy, b);
 	}
 }
 
@@ -301,7 +314,9 @@ int build_tables(bm_batch_s* o)
 	o->tml.assign(o->gml.size(), Tab<MlBatch>());
 	for (size_t g = 0; g < o->gx.size(); g++) {
 		const int nx = o->gx[g].size;
-		const int tf = x_tile_rows(nx, true, o->gx[g].sh[0].f), ti = x_tile_rows(nx, false, o->gx[g].sh[1].f);
+		// the full-length (periodic) x passes run one row-tile height only
+		const bool full = o->gx[g].full;
+		const int tf = x_tile_rows(nx, true, full ? 1 : o->gx[g].sh[0].f), ti = x_tile_rows(nx, false, full ? 1 : o->gx[g].sh[1].f);
 		for (int b : o->gx[g].meshes) {
 			// a layer stack runs the x passes over its L layer planes
 			float2* Sx; int ny, nz, ntl; double scale;
@@ -400,19 +415,19 @@ int eval_all(bm_batch_s* o, int step, int par, hipStream_t st)
 {
 	int rc = BM_OK;
 	for (size_t g = 0; g < o->gx.size() && !rc; g++)
-		rc = batch_fwd_x(o->gx[g].size, o->gx[g].sh[0], o->fwd[par][g].h.data(), o->fwd[par][g].d, (int)o->fwd[par][g].h.size(), o->ncu, st);
+		rc = batch_fwd_x(o->gx[g].size, o->gx[g].full, o->gx[g].sh[0], o->fwd[par][g].h.data(), o->fwd[par][g].d, (int)o->fwd[par][g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gy.size() && !rc; g++)
-		rc = batch_y(o->gy[g].size, -1, o->gy[g].sh[0], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st);
+		rc = batch_y(o->gy[g].size, o->gy[g].full, -1, o->gy[g].sh[0], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gz.size() && !rc; g++)
-		rc = batch_z(o->gz[g].size, o->gz[g].sh[0], o->tz[g].h.data(), o->tz[g].d, (int)o->tz[g].h.size(), o->ncu, st);
+		rc = batch_z(o->gz[g].size, o->gz[g].full, o->gz[g].sh[0], o->tz[g].h.data(), o->tz[g].d, (int)o->tz[g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gml.size() && !rc; g++)
 		rc = batch_ml_mult(o->gml[g].size, o->gml[g].sh[0], o->tml[g].h.data(), o->tml[g].d, (int)o->tml[g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gy.size() && !rc; g++)
-		rc = batch_y(o->gy[g].size, 1, o->gy[g].sh[1], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st);
+		rc = batch_y(o->gy[g].size, o->gy[g].full, 1, o->gy[g].sh[1], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gy2.size() && !rc; g++)
-		rc = batch_y2d(o->gy2[g].size, o->gy2[g].sh[0], o->ty2[g].h.data(), o->ty2[g].d, (int)o->ty2[g].h.size(), o->ncu, st);
+		rc = batch_y2d(o->gy2[g].size, o->gy2[g].full, o->gy2[g].sh[0], o->ty2[g].h.data(), o->ty2[g].d, (int)o->ty2[g].h.size(), o->ncu, st);
 	for (size_t g = 0; g < o->gx.size() && !rc; g++)
-		rc = batch_inv_x_planar(o->gx[g].size, o->gx[g].sh[1], o->inv[g].h.data(), o->inv[g].d, (int)o->inv[g].h.size(), o->ncu, st);
+		rc = batch_inv_x_planar(o->gx[g].size, o->gx[g].full, o->gx[g].sh[1], o->inv[g].h.data(), o->inv[g].d, (int)o->inv[g].h.size(), o->ncu, st);
 	if (!rc) rc = batch_stage(o->mcode, step, o->stage[par].d, o->nb, o->choff.d, o->choff.h.back(), st);
 	return rc;
 }
@@ -454,29 +469,33 @@ int autotune(bm_batch_s* o, hipStream_t st)
 	for (int pc : { 4, 8, 16 }) cy.push_back(LaunchShape{ 0, pc });
 	for (int pc : { 1, 2, 4 }) cz.push_back(LaunchShape{ 0, pc });
 	// the launch shapes crossed with every radix schedule of the pass length n
-	auto radices = [&](const std::vector<LaunchShape>& base, int n) {
+	std::vector<LaunchShape> cx1;
+	for (int pc : { 1, 2, 4 }) cx1.push_back(LaunchShape{ 1, pc });
+	// the launch shapes crossed with every radix schedule of the pass length n (periodic
+	// groups: the default schedule only)
+	auto radices = [&](const std::vector<LaunchShape>& base, int n, bool full) {
 		std::vector<LaunchShape> r;
-		const int nv = o->tune_radix ? radix_variants(n) : 1;
+		const int nv = o->tune_radix && !full ? radix_variants(n) : 1;
 		for (int v = 0; v < nv; v++) for (LaunchShape c : base) { c.radix = v; r.push_back(c); }
 		return r;
 	};
 	for (size_t g = 0; g < o->gx.size() && !rc; g++) {
 		Group& G = o->gx[g];
-		pick(G, 0, cx, true, [&] { return batch_fwd_x(G.size, G.sh[0], o->fwd[0][g].h.data(), o->fwd[0][g].d, (int)o->fwd[0][g].h.size(), o->ncu, st); });
-		if (!rc) pick(G, 1, cx, true, [&] { return batch_inv_x_planar(G.size, G.sh[1], o->inv[g].h.data(), o->inv[g].d, (int)o->inv[g].h.size(), o->ncu, st); });
+		pick(G, 0, G.full ? cx1 : cx, true, [&] { return batch_fwd_x(G.size, G.full, G.sh[0], o->fwd[0][g].h.data(), o->fwd[0][g].d, (int)o->fwd[0][g].h.size(), o->ncu, st); });
+		if (!rc) pick(G, 1, G.full ? cx1 : cx, true, [&] { return batch_inv_x_planar(G.size, G.full, G.sh[1], o->inv[g].h.data(), o->inv[g].d, (int)o->inv[g].h.size(), o->ncu, st); });
 	}
 	for (size_t g = 0; g < o->gy.size() && !rc; g++) {
 		Group& G = o->gy[g];
 		for (int slot = 0; slot < 2; slot++)
-			pick(G, slot, radices(cy, G.size), false, [&] { return batch_y(G.size, slot ? 1 : -1, G.sh[slot], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st); });
+			pick(G, slot, radices(cy, G.size, G.full), false, [&] { return batch_y(G.size, G.full, slot ? 1 : -1, G.sh[slot], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st); });
 	}
 	for (size_t g = 0; g < o->gz.size() && !rc; g++) {
 		Group& G = o->gz[g];
-		pick(G, 0, radices(cz, G.size), false, [&] { return batch_z(G.size, G.sh[0], o->tz[g].h.data(), o->tz[g].d, (int)o->tz[g].h.size(), o->ncu, st); });
+		pick(G, 0, radices(cz, G.size, G.full), false, [&] { return batch_z(G.size, G.full, G.sh[0], o->tz[g].h.data(), o->tz[g].d, (int)o->tz[g].h.size(), o->ncu, st); });
 	}
 	for (size_t g = 0; g < o->gy2.size() && !rc; g++) {
 		Group& G = o->gy2[g];
-		pick(G, 0, radices(cy, G.size), false, [&] { return batch_y2d(G.size, G.sh[0], o->ty2[g].h.data(), o->ty2[g].d, (int)o->ty2[g].h.size(), o->ncu, st); });
+		pick(G, 0, radices(cy, G.size, G.full), false, [&] { return batch_y2d(G.size, G.full, G.sh[0], o->ty2[g].h.data(), o->ty2[g].d, (int)o->ty2[g].h.size(), o->ncu, st); });
 	}
 	for (size_t g = 0; g < o->gml.size() && !rc; g++) {
 		Group& G = o->gml[g];
@@ -553,6 +572,7 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 	}
 	o->dm.assign(nmesh, nullptr);
 	o->ml.assign(nmesh, nullptr);
+	o->pbc.assign(nmesh, std::array<int, 3>{ 0, 0, 0 });
 	o->lay.assign(nmesh, std::vector<float>(3 * kMaxStackLayers, 0.f));
 	o->lay_set.assign(nmesh, 0);
 	o->lay_grouped.assign(nmesh, 0);
@@ -797,6 +817,7 @@ extern "C" int bm_batch_run(bm_batch_t o, long long nsteps, void* stream)
 	if (!o || nsteps < 0) return fail(BM_ERR_ARG, "bm_batch_run: bad arguments");
 	hipStream_t st = (hipStream_t)stream;
 	int rc;
+	o->started = true;
 	if (o->autotune && !o->tuned && nsteps > 0) {
 		if (o->exec) { (void)hipGraphExecDestroy(o->exec); o->exec = nullptr; }
 		if (o->graph) { (void)hipGraphDestroy(o->graph); o->graph = nullptr; }
@@ -825,18 +846,21 @@ extern "C" int bm_batch_mesh_offset(bm_batch_t o, int mesh, long long* cell_offs
 }
 
 // The pass-size group and slot of a pass name ("x_fwd", "x_inv", "y_fwd", "y_inv", "z",
-// "y2d", "ml_mult") and size, or nullptr.
-static Group* find_group(bm_batch_s* o, const std::string& pass, int size, int& slot)
+// "y2d", "ml_mult"; "_pbc" appended for a group along a periodic axis) and size, or nullptr.
+static Group* find_group(bm_batch_s* o, std::string pass, int size, int& slot)
 {
 	std::vector<Group>* gs = nullptr;
 	slot = 0;
+	const std::string sfx = "_pbc";
+	const bool full = pass.size() > sfx.size() && pass.compare(pass.size() - sfx.size(), sfx.size(), sfx) == 0;
+	if (full) pass.resize(pass.size() - sfx.size());
 	if (pass == "x_fwd" || pass == "x_inv") { gs = &o->gx; slot = pass == "x_inv"; }
 	else if (pass == "y_fwd" || pass == "y_inv") { gs = &o->gy; slot = pass == "y_inv"; }
 	else if (pass == "z") gs = &o->gz;
 	else if (pass == "y2d") gs = &o->gy2;
 	else if (pass == "ml_mult") gs = &o->gml;
 	if (!gs) return nullptr;
-	for (Group& G : *gs) if (G.size == size) return &G;
+	for (Group& G : *gs) if (G.size == size && G.full == full) return &G;
 	return nullptr;
 }
 
@@ -850,7 +874,7 @@ extern "C" int bm_batch_autotune_report(bm_batch_t o, char* out, int cap)
 	auto add = [&](const char* pass, const Group& G, int slot) {
 		const LaunchShape& c = G.sh[slot];
 		const bool fft = pass[0] == 'y' || pass[0] == 'z';
-		snprintf(line, sizeof(line), "%s %d f=%d per_cu=%d radix=%d %s %.1f us\n", pass, G.size, c.f, c.per_cu, c.radix,
+		snprintf(line, sizeof(line), "%s%s %d f=%d per_cu=%d radix=%d %s %.1f us\n", pass, G.full ? "_pbc" : "", G.size, c.f, c.per_cu, c.radix,
 			fft ? radix_schedule(G.size, c.radix) : "-", G.us[slot]);
 		r += line;
 	};
@@ -872,8 +896,8 @@ extern "C" int bm_batch_set_schedule(bm_batch_t o, const char* pass, int size, i
 	Group* G = find_group(o, pass, size, slot);
 	if (!G) return fail(BM_ERR_ARG, "bm_batch_set_schedule: no such pass and size in this batch");
 	const bool x = pass[0] == 'x', fft = pass[0] == 'y' || pass[0] == 'z';
-	if ((x && (f < 0 || f > 2)) || (!x && f != 0) || per_cu < 0 || per_cu > 64 || radix < 0
-		|| (fft ? radix >= radix_variants(size) : radix != 0))
+	if ((x && (f < 0 || f > (G->full ? 1 : 2))) || (!x && f != 0) || per_cu < 0 || per_cu > 64 || radix < 0
+		|| (fft && !G->full ? radix >= radix_variants(size) : radix != 0))
 		return fail(BM_ERR_ARG, "bm_batch_set_schedule: shape or radix schedule out of range for this pass");
 	G->sh[slot] = LaunchShape{ f, per_cu, radix };
 	G->us[slot] = 0;
@@ -884,4 +908,34 @@ extern "C" int bm_batch_set_schedule(bm_batch_t o, const char* pass, int size, i
 	o->autotune = false;
 	o->tuned = true;
 	return BM_OK;
+}
+
+// Periodic boundary conditions for one mesh of the batch (the contract of bm_ode_set_pbc):
+// its demag handle is rebuilt with the images (bm_demag_create_pbc: along a periodic axis
+// N = n) and exchange wraps around.  Before the first bm_batch_run; the pass groups are
+// rebuilt, so schedules pinned earlier return to their defaults.
+extern "C" int bm_batch_set_pbc(bm_batch_t o, int mesh, int pbc_x, int pbc_y, int pbc_z)
+{
+	if (!o || mesh < 0 || mesh >= o->nb || pbc_x < 0 || pbc_y < 0 || pbc_z < 0) return fail(BM_ERR_ARG, "bm_batch_set_pbc: bad arguments");
+	if (o->started) return fail(BM_ERR_STATE, "bm_batch_set_pbc: set periodic images before the first bm_batch_run");
+	if (o->ml[mesh]) return fail(BM_ERR_STATE, "bm_batch_set_pbc: layer stacks have no periodic variant");
+	const int b = mesh;
+	if (o->p[b].modules & BM_MOD_DEMAG) {
+		bm_demag_t d = nullptr;
+		const double* h = &o->h[3 * b];
+		int rc = (pbc_x || pbc_y || pbc_z) ? bm_demag_create_pbc(o->nx[b], o->ny[b], o->nz[b], h[0], h[1], h[2], pbc_x, pbc_y, pbc_z, 0, &d)
+		                                   : bm_demag_create(o->nx[b], o->ny[b], o->nz[b], h[0], h[1], h[2], 0, &d);
+		if (rc) return rc;
+		if (!pbc_x && d->Nx != 2 * o->nx[b]) {
+			bm_demag_destroy(d);
+			return fail(BM_ERR_SIZE, "bm_batch_set_pbc: batched meshes need nx in the 2^a 3^b plans 2..2048");
+		}
+		bm_demag_destroy(o->dm[b]);
+		o->dm[b] = d;
+	}
+	o->pbc[b] = { pbc_x, pbc_y, pbc_z };
+	for (std::vector<Group>* gs : { &o->gx, &o->gy, &o->gz, &o->gy2, &o->gml }) gs->clear();
+	build_groups(o);
+	if (o->tuned && !o->autotune) o->tuned = false;
+	return build_tables(o);
 }

@@ -1239,16 +1239,18 @@ int bm::x_tile_rows(int nx, bool fwd, int f)
 	return 0;
 }
 
-int bm::batch_fwd_x(int nx, LaunchShape sh, const FwdArgs* h_tab, const FwdArgs* d_tab, int nb, int ncu, hipStream_t st)
+// full: periodic x (rows of 2 HX cells, one row-tile height)
+int bm::batch_fwd_x(int nx, bool full, LaunchShape sh, const FwdArgs* h_tab, const FwdArgs* d_tab, int nb, int ncu, hipStream_t st)
 {
 	int mx = 1;
 	for (int b = 0; b < nb; b++) mx = std::max(mx, h_tab[b].nz * h_tab[b].ntiles);
-	const int f = sh.f ? sh.f : xtile_factor(true);
+	const int f = full ? 1 : sh.f ? sh.f : xtile_factor(true);
 	const dim3 g(grid_for(mx, ncu, sh.per_cu ? sh.per_cu : 2), nb);
 	FwdArgs a{};
 	switch (nx) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; \
-		if (f == 1) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false>), g, dim3(kBlock), 0, st, a, d_tab); \
+		if (full) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false, true>), g, dim3(kBlock), 0, st, a, d_tab); \
+		else if (f == 1) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false>), g, dim3(kBlock), 0, st, a, d_tab); \
 		else hipLaunchKernelGGL((k_fwd_x<V_, 2 * TY1, false>), g, dim3(kBlock), 0, st, a, d_tab); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
@@ -1256,15 +1258,16 @@ int bm::batch_fwd_x(int nx, LaunchShape sh, const FwdArgs* h_tab, const FwdArgs*
 	return fail(BM_ERR_SIZE, "no x plan");
 }
 
-int bm::batch_inv_x_planar(int nx, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+int bm::batch_inv_x_planar(int nx, bool full, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
 	int mx = 1;
 	for (int b = 0; b < nb; b++) mx = std::max(mx, 3 * h_tab[b].nz * h_tab[b].ntiles);
-	const int f = sh.f ? sh.f : xtile_factor(false);
+	const int f = full ? 1 : sh.f ? sh.f : xtile_factor(false);
 	const dim3 g(grid_for(mx, ncu, sh.per_cu ? sh.per_cu : 2), nb);
 	switch (nx) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; \
-		if (f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, V_, 0, 0, d_tab); \
+		if (full) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1, true>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, V_, 0, 0, d_tab); \
+		else if (f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, V_, 0, 0, d_tab); \
 		else hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, V_, 0, 0, d_tab); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
@@ -1274,7 +1277,7 @@ int bm::batch_inv_x_planar(int nx, LaunchShape sh, const PassBatch* h_tab, const
 
 // Batched passes with the radix schedule sh.radix (plan variant of the pass length).
 namespace {
-template <int NY, int V>
+template <int NY, int V, bool FULL = false>
 int batch_y_v(int dir, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
 	const int pc = sh.per_cu ? sh.per_cu : 8;
@@ -1282,30 +1285,30 @@ int batch_y_v(int dir, LaunchShape sh, const PassBatch* h_tab, const PassBatch* 
 	int mx = 1;
 	for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].nlines + LP - 1) / LP);
 	const dim3 g(grid8(mx, ncu, pc), nb);
-	if (dir < 0) hipLaunchKernelGGL((k_y<NY, -1, false, V>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, d_tab);
-	else hipLaunchKernelGGL((k_y<NY, 1, false, V>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, d_tab);
+	if (dir < 0) hipLaunchKernelGGL((k_y<NY, -1, FULL, V>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, d_tab);
+	else hipLaunchKernelGGL((k_y<NY, 1, FULL, V>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, d_tab);
 	return BM_OK;
 }
 
-template <int NZ, int V>
+template <int NZ, int V, bool FULL = false>
 int batch_z_v(LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
 	const int pc = sh.per_cu ? sh.per_cu : 2;
 	constexpr int CH = kBlock / plan_T<plan_key(NZ, V)>() / 2;
 	int mx = 1;
 	for (int b = 0; b < nb; b++) mx = std::max(mx, h_tab[b].NC * ((h_tab[b].Ny / 2 + CH - 1) / CH));
-	hipLaunchKernelGGL((k_z<NZ, false, V>), dim3(grid8(mx, ncu, pc), nb), dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, z_xcd(), d_tab);
+	hipLaunchKernelGGL((k_z<NZ, FULL, V>), dim3(grid8(mx, ncu, pc), nb), dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0, z_xcd(), d_tab);
 	return BM_OK;
 }
 
-template <int NY, int V>
+template <int NY, int V, bool FULL = false>
 int batch_y2d_v(LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
 	const int pc = sh.per_cu ? sh.per_cu : 8;
 	constexpr int PK = plan_key(NY, V), BS = Plan<PK>::E > 8 ? 128 : 256, LP = BS / plan_T<PK>();
 	int mx = 1;
 	for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].NC + LP - 1) / LP);
-	hipLaunchKernelGGL((k_y2d<NY, BS, false, V>), dim3(grid8(mx, ncu, pc), nb), dim3(BS), 0, st, nullptr, nullptr, 0, 0, 0, 0, d_tab);
+	hipLaunchKernelGGL((k_y2d<NY, BS, FULL, V>), dim3(grid8(mx, ncu, pc), nb), dim3(BS), 0, st, nullptr, nullptr, 0, 0, 0, 0, d_tab);
 	return BM_OK;
 }
 
@@ -1337,30 +1340,37 @@ const char* bm::radix_schedule(int n, int v)
 	return "";
 }
 
-int bm::batch_y(int Ny, int dir, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+// full: periodic axis (N = n), default radix schedule only
+int bm::batch_y(int Ny, bool full, int dir, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
+	if (full && sh.radix) return fail(BM_ERR_ARG, "periodic passes take the default radix schedule");
 	switch (Ny) {
-#define C_(V_) case V_: return with_variant<V_>(sh.radix, [&](auto v) { return batch_y_v<V_, decltype(v)::value>(dir, sh, h_tab, d_tab, nb, ncu, st); });
+#define C_(V_) case V_: if (full) return batch_y_v<V_, 0, true>(dir, sh, h_tab, d_tab, nb, ncu, st); \
+		return with_variant<V_>(sh.radix, [&](auto v) { return batch_y_v<V_, decltype(v)::value>(dir, sh, h_tab, d_tab, nb, ncu, st); });
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no y plan");
 }
 
-int bm::batch_z(int Nz, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+int bm::batch_z(int Nz, bool full, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
+	if (full && sh.radix) return fail(BM_ERR_ARG, "periodic passes take the default radix schedule");
 	switch (Nz) {
-#define C_(V_) case V_: return with_variant<V_>(sh.radix, [&](auto v) { return batch_z_v<V_, decltype(v)::value>(sh, h_tab, d_tab, nb, ncu, st); });
+#define C_(V_) case V_: if (full) return batch_z_v<V_, 0, true>(sh, h_tab, d_tab, nb, ncu, st); \
+		return with_variant<V_>(sh.radix, [&](auto v) { return batch_z_v<V_, decltype(v)::value>(sh, h_tab, d_tab, nb, ncu, st); });
 	BM_FOR_Z_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no z plan");
 }
 
-int bm::batch_y2d(int Ny, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+int bm::batch_y2d(int Ny, bool full, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
+	if (full && sh.radix) return fail(BM_ERR_ARG, "periodic passes take the default radix schedule");
 	switch (Ny) {
-#define C_(V_) case V_: return with_variant<V_>(sh.radix, [&](auto v) { return batch_y2d_v<V_, decltype(v)::value>(sh, h_tab, d_tab, nb, ncu, st); });
+#define C_(V_) case V_: if (full) return batch_y2d_v<V_, 0, true>(sh, h_tab, d_tab, nb, ncu, st); \
+		return with_variant<V_>(sh.radix, [&](auto v) { return batch_y2d_v<V_, decltype(v)::value>(sh, h_tab, d_tab, nb, ncu, st); });
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}

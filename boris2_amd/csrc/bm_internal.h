@@ -136,9 +136,9 @@ int batch_ml_mult(int L, LaunchShape sh, const MlBatch* h_tab, const MlBatch* d_
 int x_tile_rows(int nx, bool fwd, int f = 0);
 // Batched passes over nb meshes sharing the pass size (tables: host copy for the grid size,
 // device copy for the kernels).
-int batch_fwd_x(int nx, LaunchShape sh, const FwdArgs* h_tab, const FwdArgs* d_tab, int nb, int ncu, hipStream_t st);
-int batch_inv_x_planar(int nx, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
-int batch_y(int Ny, int dir, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
-int batch_z(int Nz, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
-int batch_y2d(int Ny, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
+int batch_fwd_x(int nx, bool full, LaunchShape sh, const FwdArgs* h_tab, const FwdArgs* d_tab, int nb, int ncu, hipStream_t st);
+int batch_inv_x_planar(int nx, bool full, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
+int batch_y(int Ny, bool full, int dir, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
+int batch_z(int Nz, bool full, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
+int batch_y2d(int Ny, bool full, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
 }

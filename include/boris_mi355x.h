@@ -311,6 +311,12 @@ int bm_ode_average_m(bm_ode_t o, double* out3, void* stream);
  * pins one pass-size group's choice (e.g. read back from an earlier report), replacing any
  * pending tuning: f (x passes 0..2, else 0; 0 = default), per_cu (0 = default, <= 64),
  * radix (0 = default; y/z/y2d passes only); BM_ERR_ARG for a pass or size the batch lacks.
+ * bm_batch_set_pbc (before the first bm_batch_run): mesh `mesh` becomes periodic as
+ * bm_ode_set_pbc makes one mesh (its demag handle rebuilt as bm_demag_create_pbc with these
+ * images, exchange wrapping around; not for layer stacks); its passes along periodic axes
+ * form groups of their own, reported and pinned with "_pbc" appended to the pass name
+ * (default radix schedule, x-pass factor 1); pass groups are rebuilt, so set the images before
+ * pinning schedules.
  * bm_batch_create_layers: layers[b].nlayers > 0 makes mesh b a layer stack as
  * bm_ode_set_layers does (nz == nlayers <= 16, demag on; hz, z0 its layers' thicknesses and
  * bottom heights, Ms/A/alpha/K1 per layer or NULL); mesh_of_layer and Ha (or NULL) make it a
@@ -344,6 +350,7 @@ int bm_batch_get_mesh_state(bm_batch_t b, int mesh, double* time, double* dT, lo
 int bm_batch_mesh_offset(bm_batch_t b, int mesh, long long* cell_offset, long long* ncells);
 int bm_batch_autotune_report(bm_batch_t b, char* out, int cap);
 int bm_batch_set_schedule(bm_batch_t b, const char* pass, int size, int f, int per_cu, int radix);
+int bm_batch_set_pbc(bm_batch_t b, int mesh, int pbc_x, int pbc_y, int pbc_z);
 
 /* ------------------------------------------------------------------------------------
  * Differentiation of the fixed-step RK4 rollout (no reference counterpart: the reference
