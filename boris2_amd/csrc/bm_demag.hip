@@ -370,6 +370,10 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 	}
 }
 
+// workgroup size of P2' for plan key PK: 128 threads when the plan holds more than 8 values
+// per thread (register pressure of three components), unless a line needs more threads
+template <int PK> constexpr int y2d_block() { return (Plan<PK>::E > 8 && plan_T<PK>() <= 128) ? 128 : 256; }
+
 // ---------------------------------------------------------------------------------------
 // P2' (2D): per kx line of Sx (in place), y-FFT of 3 components, 2D kernel multiply (Kxy
 // odd in y), y-IFFT.  A block takes LP consecutive lines, one per group of T lanes.
@@ -623,13 +627,14 @@ __global__ void __launch_bounds__(kBlock) k_line_fft(const double* __restrict__ 
 
 // ---------------------------------------------------------------------------------------
 // Size dispatch.
-#define BM_FOR_Z_SIZES(M) M(4) M(8) M(16) M(32) M(64) M(128)
+#define BM_FOR_Z_SIZES(M) M(4) M(8) M(16) M(32) M(64) M(128) M(256)
 
 bool x_supported(int n) { switch (n) {
 #define C_(V_) case V_:
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	return true; default: return false; } }
+bool y_supported(int n) { return x_supported(n) || n == 4096; }
 bool line_supported(int n) { return x_supported(n) || n == 1536 || n == 4096; }
 bool z_supported(int n) { switch (n) {
 #define C_(V_) case V_:
@@ -741,7 +746,7 @@ int launch_y(int NY, bool full, int dir, float2* S, float2* Sx, int nlines, int 
 #define C_(V_) case V_: { int LP = kBlock / plan_T<V_>(); dim3 g(grid8((nlines + LP - 1) / LP, ncu, 8)); \
 		if (full) launch_y_t<V_, true>(dir, g, st, S, Sx, nlines, Sp, NC, ntl, ny, nullptr); \
 		else launch_y_t<V_>(dir, g, st, S, Sx, nlines, Sp, NC, ntl, ny, nullptr); return BM_OK; }
-	BM_FOR_X_SIZES(C_)
+	BM_FOR_Y_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no y plan");
@@ -764,10 +769,10 @@ int launch_z(int NZ, bool full, float2* S, const float* K, int nz, int NC, int N
 int launch_y2d(int NY, bool full, float2* Sx, const float* K, int NC, int NYH, int ntl, int ny, int ncu, hipStream_t st)
 {
 	switch (NY) {
-#define C_(V_) case V_: { constexpr int BS = Plan<V_>::E > 8 ? 128 : 256; int LP = BS / plan_T<V_>(); dim3 g(grid8((NC + LP - 1) / LP, ncu, 8)); \
+#define C_(V_) case V_: { constexpr int BS = y2d_block<V_>(); int LP = BS / plan_T<V_>(); dim3 g(grid8((NC + LP - 1) / LP, ncu, 8)); \
 		if (full) hipLaunchKernelGGL((k_y2d<V_, BS, true>), g, dim3(BS), 0, st, Sx, K, NC, NYH, ntl, ny, (const PassBatch*)nullptr); \
 		else hipLaunchKernelGGL((k_y2d<V_, BS>), g, dim3(BS), 0, st, Sx, K, NC, NYH, ntl, ny, (const PassBatch*)nullptr); return BM_OK; }
-	BM_FOR_X_SIZES(C_)
+	BM_FOR_Y_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no y plan");
@@ -949,6 +954,9 @@ int x_length_at_least(int v)
 
 int bm::plan_length_at_least(int v) { return x_length_at_least(v); }
 
+// The smallest y-pass length at or above v, 0 if none.
+int y_length_at_least(int v) { return v > 2048 ? (v <= 4096 ? 4096 : 0) : x_length_at_least(v); }
+
 // Sizes (Boris/ConvolutionDataCUDA.cpp:24-100): N = n along a periodic axis, otherwise the
 // reference takes N.x = 2 nx, N.y = 2 ny (any n, cuFFT plans every length) and N.z the next
 // power of two >= 2 nz (N.z = 1 in 2D).  Here an open x or y axis takes the smallest length
@@ -964,13 +972,13 @@ static int demag_create(int nx, int ny, int nz, double hx, double hy, double hz,
 	const bool fx = pbc && pbc[0] > 0, fy = pbc && pbc[1] > 0, fz = nz > 1 && pbc && pbc[2] > 0;
 	if (fx && nx % 2) return fail(BM_ERR_SIZE, "bm_demag_create_pbc: a periodic nx must be even");
 	const int HX = fx ? nx / 2 : x_length_at_least(nx);
-	const int Ny = fy ? ny : x_length_at_least(2 * ny);
+	const int Ny = fy ? ny : y_length_at_least(2 * ny);
 	int Nz = 1;
 	if (fz) Nz = nz;
 	else if (nz > 1) { while (Nz < 2 * nz) Nz *= 2; }
-	if (!HX || !Ny || !x_supported(HX) || !x_supported(Ny) || !line_supported(2 * HX) || (nz > 1 && !z_supported(Nz)))
-		return fail(BM_ERR_SIZE, "bm_demag_create: no FFT plan for this mesh size (open axes: nx <= 2048, ny <= 1024, nz <= 64; "
-			"periodic axes: N.x / 2 and N.y in the 2^a 3^b plans 2..2048, N.z a power of two 4..128, N = n)");
+	if (!HX || !Ny || !x_supported(HX) || !y_supported(Ny) || !line_supported(2 * HX) || (nz > 1 && !z_supported(Nz)))
+		return fail(BM_ERR_SIZE, "bm_demag_create: no FFT plan for this mesh size (open axes: nx <= 2048, ny <= 2048, nz <= 128; "
+			"periodic axes: N.x / 2 in the 2^a 3^b plans 2..2048, N.y in those or 4096, N.z a power of two 4..256, N = n)");
 	bm_demag_s* d = new bm_demag_s();
 	d->nx = nx; d->ny = ny; d->nz = nz; d->h[0] = hx; d->h[1] = hy; d->h[2] = hz;
 	for (int q = 0; q < 3; q++) d->pbc[q] = pbc ? pbc[q] : 0;
@@ -1305,7 +1313,7 @@ template <int NY, int V, bool FULL = false>
 int batch_y2d_v(LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
 {
 	const int pc = sh.per_cu ? sh.per_cu : 8;
-	constexpr int PK = plan_key(NY, V), BS = Plan<PK>::E > 8 ? 128 : 256, LP = BS / plan_T<PK>();
+	constexpr int PK = plan_key(NY, V), BS = y2d_block<PK>(), LP = BS / plan_T<PK>();
 	int mx = 1;
 	for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].NC + LP - 1) / LP);
 	hipLaunchKernelGGL((k_y2d<NY, BS, FULL, V>), dim3(grid8(mx, ncu, pc), nb), dim3(BS), 0, st, nullptr, nullptr, 0, 0, 0, 0, d_tab);
@@ -1347,7 +1355,7 @@ int bm::batch_y(int Ny, bool full, int dir, LaunchShape sh, const PassBatch* h_t
 	switch (Ny) {
 #define C_(V_) case V_: if (full) return batch_y_v<V_, 0, true>(dir, sh, h_tab, d_tab, nb, ncu, st); \
 		return with_variant<V_>(sh.radix, [&](auto v) { return batch_y_v<V_, decltype(v)::value>(dir, sh, h_tab, d_tab, nb, ncu, st); });
-	BM_FOR_X_SIZES(C_)
+	BM_FOR_Y_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no y plan");
@@ -1371,7 +1379,7 @@ int bm::batch_y2d(int Ny, bool full, LaunchShape sh, const PassBatch* h_tab, con
 	switch (Ny) {
 #define C_(V_) case V_: if (full) return batch_y2d_v<V_, 0, true>(sh, h_tab, d_tab, nb, ncu, st); \
 		return with_variant<V_>(sh.radix, [&](auto v) { return batch_y2d_v<V_, decltype(v)::value>(sh, h_tab, d_tab, nb, ncu, st); });
-	BM_FOR_X_SIZES(C_)
+	BM_FOR_Y_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no y plan");

@@ -37,15 +37,15 @@ def test_product_sources_do_not_reference_the_oracle():
 
 
 def test_unsupported_sizes_are_rejected_before_any_device_work():
-    """Open axes take any nx <= 2048, ny <= 1024 and nz <= 64 (N padded up to a planned
+    """Open axes take any nx <= 2048, ny <= 2048 and nz <= 128 (N padded up to a planned
     length); larger sizes return BM_ERR_SIZE and bad arguments BM_ERR_ARG, both checked
     before any allocation, so this runs without a device."""
     import ctypes as C
     L = _lib.load()
     h = C.c_void_p()
-    cases = [((8, 8, 65), 2),       # N.z = 256 > 128
+    cases = [((8, 8, 129), 2),      # N.z = 512 > 256
              ((2049, 8, 1), 2),     # nx above 2048
-             ((8, 1025, 1), 2),     # N.y above 2048
+             ((8, 2049, 1), 2),     # N.y above 4096
              ((0, 8, 1), 1), ((8, 0, 1), 1), ((8, 8, 0), 1)]
     for n, code in cases:
         assert L.bm_demag_create(*n, 4e-9, 4e-9, 4e-9, 0, C.byref(h)) == code, n
@@ -62,6 +62,8 @@ def test_pbc_sizes_are_checked_before_any_device_work():
              ((8, 10, 1, 0, 1, 0), 2),      # periodic ny = 10 has no plan
              ((8, 8, 3, 0, 0, 1), 2),       # periodic nz = 3: N.z = 3 is not a power of two
              ((8, 8, 2, 0, 0, 1), 2),       # N.z = 2 below the smallest z plan
+             ((8, 8, 512, 0, 0, 1), 2),     # periodic nz = 512 above the largest z plan
+             ((8, 8192, 1, 0, 1, 0), 2),    # periodic ny = 8192 above the largest y plan
              ((8, 8, 1, -1, 0, 0), 1)]      # negative image count
     for (nx, ny, nz, px, py, pz), code in cases:
         assert L.bm_demag_create_pbc(nx, ny, nz, 4e-9, 4e-9, 4e-9, px, py, pz, 0, C.byref(h)) == code

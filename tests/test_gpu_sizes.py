@@ -20,7 +20,9 @@ def rel_l2(a, b):
 
 
 SIZES = [(10, 8, 1), (50, 25, 1), (100, 100, 1), (125, 7, 1), (33, 17, 1), (101, 3, 1), (3, 1, 1), (1000, 20, 1),
-         (2000, 6, 1), (10, 14, 3), (35, 21, 5), (100, 50, 4), (7, 9, 13), (250, 40, 2)]
+         (2000, 6, 1), (10, 14, 3), (35, 21, 5), (100, 50, 4), (7, 9, 13), (250, 40, 2),
+         # N.y = 4096 (ny above 1024) and N.z = 256 (nz above 64)
+         (8, 1100, 1), (6, 1500, 2), (4, 2048, 1), (4, 4, 100), (3, 5, 128)]
 
 
 @pytest.mark.parametrize("n", SIZES, ids=lambda n: "x".join(map(str, n)))
@@ -108,3 +110,50 @@ def test_batch_takes_padded_ny_and_refuses_other_nx(gpu):
     bs.close()
     with pytest.raises(RuntimeError, match="error 2"):
         api.BatchSolver(api.BM_EVAL_RK4, [((50, 16, 1), h, p)])
+
+
+def test_largest_y_and_z_lengths_periodic(gpu):
+    """Periodic axes at the largest y and z plans: ny = 4096 (N.y = 4096) and nz = 256."""
+    for n, pbc in (((4, 4096, 1), (0, 1, 0)), ((4, 4, 256), (1, 0, 1))):
+        h = (4e-9, 3e-9, 5e-9)
+        M = api.random_magnetisation(n[0] * n[1] * n[2], 8e5, 5).astype(np.float32)
+        d = api.Demag(n, h, pbc)
+        H = d.convolute(torch.from_numpy(M).cuda()).cpu().numpy()
+        ref = O.OracleDemag(n, h, pbc=pbc)(M.astype(np.float64))
+        e = rel_l2(H, ref)
+        print(f"n {n} pbc {pbc}: rel L2 {e:.2e}")
+        assert e <= 1e-6
+        d.close()
+
+
+def test_batch_with_largest_y_and_z_lengths(gpu):
+    """A batch holding meshes of N.y = 4096 and N.z = 256 equals their single runs."""
+    p = api.FMParams(Ms=8e5, A=1.3e-11, alpha=0.05, Ha=(0.0, 1e4, 0.0),
+                     modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ZEEMAN)
+    h = (4e-9, 4e-9, 4e-9)
+    meshes = [((8, 1100, 1), h, p), ((4, 1500, 2), h, p), ((4, 4, 100), h, p)]
+    M0 = [api.random_magnetisation(n[0] * n[1] * n[2], p.Ms, 30 + b).astype(np.float32) for b, (n, _, _) in enumerate(meshes)]
+    bs = api.BatchSolver(api.BM_EVAL_RK4, meshes, M0)
+    bs.set_dt(1e-13)
+    bs.run(5)
+    torch.cuda.synchronize()
+    for b, (n, hh, pp) in enumerate(meshes):
+        s = api.FMSolver(api.BM_EVAL_RK4, n, hh, pp, M0[b])
+        s.set_dt(1e-13)
+        s.run_to_iteration(5)
+        assert rel_l2(bs.mesh(b).cpu().numpy(), s.M.cpu().numpy().astype(np.float64)) <= 2e-6
+        s.close()
+    bs.close()
+
+
+def test_largest_open_film_2048x2048(gpu):
+    """The largest open 2D film (nx = ny = 2048: N.x = N.y = 4096) against the oracle."""
+    n, h = (2048, 2048, 1), (4e-9, 4e-9, 3e-9)
+    M = api.random_magnetisation(n[0] * n[1], 8e5, 12).astype(np.float32)
+    d = api.Demag(n, h)
+    H = d.convolute(torch.from_numpy(M).cuda()).cpu().numpy()
+    ref = O.OracleDemag(n, h)(M.astype(np.float64))
+    e = rel_l2(H, ref)
+    print(f"2048x2048: rel L2 {e:.2e}")
+    assert e <= 1e-6
+    d.close()
