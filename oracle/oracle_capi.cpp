@@ -59,6 +59,19 @@ int or_tensor2d_zshift(int nx, int ny, int Nx, int Ny, const double* s3, const d
 }
 
 // Real kernels of a z-shifted 2D tensor: Kd, Ko (Ny/2+1)*(Nx/2+1)*3 each.
+// the same with px, py periodic images along x and y (N = n along a periodic axis)
+int or_tensor2d_zshift_pbc(int Nx, int Ny, const double* s3, const double* d3, double shz, int px, int py, double* Ddiag, double* Dodiag)
+{
+	try {
+		std::vector<double> Dd, Do;
+		tensor2d_zshift_pbc(Nx, Ny, s3, d3, shz, px, py, Dd, Do);
+		std::memcpy(Ddiag, Dd.data(), Dd.size() * sizeof(double));
+		std::memcpy(Dodiag, Do.data(), Do.size() * sizeof(double));
+		return 0;
+	}
+	catch (...) { return 1; }
+}
+
 int or_kernels2d_zshift(int Nx, int Ny, const double* Ddiag, const double* Dodiag, double* Kd, double* Ko)
 {
 	try {
@@ -76,6 +89,12 @@ int or_kernels2d_zshift(int Nx, int Ny, const double* Ddiag, const double* Dodia
 void* or_mlayer_create(int L, int nx, int ny, double hx, double hy, const double* hz, const double* z0)
 {
 	try { return new MLayer<double>(L, nx, ny, hx, hy, hz, z0); }
+	catch (...) { return nullptr; }
+}
+// with px, py periodic images along x and y
+void* or_mlayer_create_pbc(int L, int nx, int ny, double hx, double hy, const double* hz, const double* z0, int px, int py)
+{
+	try { return new MLayer<double>(L, nx, ny, hx, hy, hz, z0, px, py); }
 	catch (...) { return nullptr; }
 }
 void or_mlayer_destroy(void* h) { delete (MLayer<double>*)h; }
@@ -266,7 +285,8 @@ void or_sim_get_Heff(void* h, double* H)
 // Layer stack (Sim::ext_demag): the sim's nz planes become L 2D layers (hx, hy, hz[l],
 // bottom faces z0[l]) coupled by the multilayered demag; per-layer Ms, A, alpha, K1
 // (NULL keeps the sim's value).  Create the sim with demag off; with_demag adds the
-// multilayered demag.
+// multilayered demag, periodic along x and y when the sim's images were set first
+// (or_sim_set_pbc before this call).
 void or_sim_set_layers(void* h, int L, double hx, double hy, const double* hz, const double* z0, int with_demag,
 	const double* Ms, const double* A, const double* alpha, const double* K1)
 {
@@ -276,7 +296,7 @@ void or_sim_set_layers(void* h, int L, double hx, double hy, const double* hz, c
 		s->layHz.assign(hz, hz + L);
 		if (with_demag) {
 			typedef std::remove_reference<decltype(s->M[0])>::type RR;
-			std::shared_ptr<MLayer<RR>> ml(new MLayer<RR>(L, s->nx, s->ny, hx, hy, hz, z0));
+			std::shared_ptr<MLayer<RR>> ml(new MLayer<RR>(L, s->nx, s->ny, hx, hy, hz, z0, s->pbc[0], s->pbc[1]));
 			s->ext_demag = [ml](const RR* M, RR* H) { ml->convolute(M, H); };
 		}
 		if (A) s->layA.assign(A, A + L);

@@ -245,6 +245,111 @@ inline void tensor2d_zshift(int nx, int ny, int Nx, int Ny, const double s[3], c
 	}
 }
 
+// The same tensor with periodic images along x (px) and y (py), no images along z: the
+// z-shift branch of Boris/DemagTFunc_Shifted_PBC.cpp:317-460 / 464-600 (regular) and
+// Boris/DemagTFunc_Irregular_PBC.cpp:5-120 (irregular).  Along a periodic axis N = n and
+// every offset i0 in [0, N) sums the images i = i_img N + i0, i_img = -p..p (i_img outer, j_img
+// inner), each at |i| by the near-field stencil (grids out to the far-field distance) or the
+// far-field series, the off-diagonal terms signed by sign(i) sign(j) (xy), sign(i) (xz),
+// sign(j) (yz); along an open axis only [0, N/2) is computed and mirrored as without images.
+inline void tensor2d_zshift_pbc(int Nx, int Ny, const double s[3], const double d[3], double shz, int px, int py,
+	std::vector<double>& Dd, std::vector<double>& Do)
+{
+	const int A = kAsymptoticDistance;
+	const int ex = px ? Nx : Nx / 2, ey = py ? Ny : Ny / 2;   // computed offsets per axis
+	const int dx = px ? A : std::min(ex, A), dy = py ? A : std::min(ey, A);
+	const size_t NN = (size_t)Nx * Ny;
+	Dd.assign(NN * 3, 0.0); Do.assign(NN * 3, 0.0);
+	const bool regular = same_cell(s, d);
+	const double hx = regular ? s[0] : d[0], hy = regular ? s[1] : d[1], hz = regular ? s[2] : d[2];
+	auto lin = [](double h, double o) { return [h, o](int i) { return i * h + o; }; };
+	Grid3 fxx, fyy, fzz, gxy, gxz, gyz, fxx_d, fyy_d, fzz_d, gxy_d, gxz_d, gyz_d;
+	if (regular) {
+		fxx = fill_grid_at(nw_f, dx, dy, 1, lin(hx, 0), lin(hy, 0), lin(hz, shz));
+		fyy = fill_grid_at(nw_f, dy, dx, 1, lin(hy, 0), lin(hx, 0), lin(hz, shz));
+		fzz = fill_grid_at(nw_f, 1, dy, dx, lin(hz, shz), lin(hy, 0), lin(hx, 0));
+		gxy = fill_grid_at(nw_g, dx, dy, 1, lin(hx, 0), lin(hy, 0), lin(hz, shz));
+		gxz = fill_grid_at(nw_g, dx, 1, dy, lin(hx, 0), lin(hz, shz), lin(hy, 0));
+		gyz = fill_grid_at(nw_g, dy, 1, dx, lin(hy, 0), lin(hz, shz), lin(hx, 0));
+	}
+	else {
+		const double del = s[2] - d[2];
+		auto p_sd = [&](int k) { return (k < 0 ? -s[2] : k > 0 ? d[2] : 0.0) + shz; };
+		auto p_ds = [&](int k) { return (k < 0 ? -d[2] : k > 0 ? s[2] : 0.0) + shz; };
+		auto c_m = [&](int) { return -del + shz; };
+		auto c_p = [&](int) { return del + shz; };
+		fxx = fill_grid_at(nw_f, dx, dy, 1, lin(hx, 0), lin(hy, 0), p_sd);
+		fyy = fill_grid_at(nw_f, dy, dx, 1, lin(hy, 0), lin(hx, 0), p_sd);
+		fzz = fill_grid_at(nw_f, 1, dy, dx, p_ds, lin(hy, 0), lin(hx, 0));
+		gxy = fill_grid_at(nw_g, dx, dy, 1, lin(hx, 0), lin(hy, 0), p_ds);
+		gxz = fill_grid_at(nw_g, dx, 1, dy, lin(hx, 0), p_ds, lin(hy, 0));
+		gyz = fill_grid_at(nw_g, dy, 1, dx, lin(hy, 0), p_ds, lin(hx, 0));
+		fxx_d = fill_grid_at(nw_f, dx, dy, -1, lin(hx, 0), lin(hy, 0), c_m);
+		fyy_d = fill_grid_at(nw_f, dy, dx, -1, lin(hy, 0), lin(hx, 0), c_m);
+		fzz_d = fill_grid_at(nw_f, -1, dy, dx, c_p, lin(hy, 0), lin(hx, 0));
+		gxy_d = fill_grid_at(nw_g, dx, dy, -1, lin(hx, 0), lin(hy, 0), c_p);
+		gxz_d = fill_grid_at(nw_g, dx, -1, dy, lin(hx, 0), c_p, lin(hy, 0));
+		gyz_d = fill_grid_at(nw_g, dy, -1, dx, lin(hy, 0), c_p, lin(hx, 0));
+	}
+	AsymptDiag adx(hx, hy, hz), ady(hy, hx, hz), adz(hz, hy, hx);
+	AsymptOffDiag aoxy(hx, hy, hz), aoxz(hx, hz, hy), aoyz(hy, hz, hx);
+	const double tau = hx * hy * hz;
+	const double far_scale = regular ? 1.0 : s[2];
+	const double ks = std::fabs(shz / hz);
+
+#pragma omp parallel for schedule(dynamic, 1)
+	for (int j0 = 0; j0 < ey; j0++) {
+		for (int i0 = 0; i0 < ex; i0++) {
+			double vd[3] = { 0, 0, 0 }, vo[3] = { 0, 0, 0 };
+			for (int ii = -px; ii <= px; ii++)
+				for (int jj = -py; jj <= py; jj++) {
+					const int is = ii * Nx + i0, js = jj * Ny + j0;
+					const double si = (is > 0) - (is < 0), sj = (js > 0) - (js < 0);
+					const int i = std::abs(is), j = std::abs(js);
+					double td[3], to[3];
+					if (A > 0 && (i >= A || j >= A || floor_eps(ks) >= A || floor_eps(i * i + j * j + ks * ks) >= A * A)) {
+						td[0] = (-adx.eval(i * hx, j * hy, shz)) * far_scale;
+						td[1] = (-ady.eval(j * hy, i * hx, shz)) * far_scale;
+						td[2] = (-adz.eval(shz, j * hy, i * hx)) * far_scale;
+						to[0] = (-aoxy.eval(i * hx, j * hy, shz)) * far_scale;
+						to[1] = (-aoxz.eval(i * hx, shz, j * hy)) * far_scale;
+						to[2] = (-aoyz.eval(j * hy, shz, i * hx)) * far_scale;
+					}
+					else if (regular) {
+						td[0] = L27(fxx, i, j, 0, hx, hy, hz) * -1;
+						td[1] = L27(fyy, j, i, 0, hy, hx, hz) * -1;
+						td[2] = L27(fzz, 0, j, i, hz, hy, hx) * -1;
+						to[0] = L27(gxy, i, j, 0, hx, hy, hz) * -1;
+						to[1] = L27(gxz, i, 0, j, hx, hz, hy) * -1;
+						to[2] = L27(gyz, j, 0, i, hy, hz, hx) * -1;
+					}
+					else {
+						td[0] = L36_plane3(fxx, fxx_d, i, j, tau) * -1;
+						td[1] = L36_plane3(fyy, fyy_d, j, i, tau) * -1;
+						td[2] = L36_zz(fzz, fzz_d, j, i, tau) * -1;
+						to[0] = L36_plane3(gxy, gxy_d, i, j, tau) * -1;
+						to[1] = L36_plane2(gxz, gxz_d, i, j, tau) * -1;
+						to[2] = L36_plane2(gyz, gyz_d, j, i, tau) * -1;
+					}
+					for (int c = 0; c < 3; c++) vd[c] += td[c];
+					vo[0] += to[0] * (si * sj); vo[1] += to[1] * si; vo[2] += to[2] * sj;
+				}
+			const size_t idx = (size_t)i0 + (size_t)j0 * Nx;
+			auto put = [&](size_t q, double ox, double oy, double oz) {
+				for (int c = 0; c < 3; c++) Dd[q * 3 + c] = vd[c];
+				Do[q * 3 + 0] = vo[0] * ox; Do[q * 3 + 1] = vo[1] * oy; Do[q * 3 + 2] = vo[2] * oz;
+			};
+			put(idx, 1, 1, 1);
+			const size_t mx = (size_t)((Nx - i0) % Nx), my = (size_t)((Ny - j0) % Ny);
+			if (!px) {
+				if (i0) put(mx + (size_t)j0 * Nx, -1, -1, 1);
+				if (!py && i0 && j0) put(mx + my * Nx, 1, -1, -1);
+			}
+			if (!py && j0) put((size_t)i0 + my * Nx, -1, 1, -1);
+		}
+	}
+}
+
 // Real kernels of a z-shifted 2D tensor (Boris/DemagKernelCollection_Calc.cpp:288-420):
 // x then y real-input FFTs; diagonal parts Re, Re; off-diagonal (xy, xz, yz) take
 // (Im, Im, Re) along x and (-Im, Re, Im) along y.  The stored xz and yz values are the
@@ -301,10 +406,15 @@ struct MLayer {
 	std::vector<int> self;
 	Fft<R> fx, fy;
 
-	MLayer(int L_, int nx_, int ny_, double hx, double hy, const double* hz, const double* z0)
+	// px, py: periodic images along x and y (SDemag::Set_PBC, Boris/SDemag.cpp:403-408): N = n
+	// along a periodic axis, the self kernels are the periodic 2D kernels and the pair
+	// kernels the periodic z-shifted ones (Boris/DemagKernelCollection_Calc.cpp with pbc_images)
+	MLayer(int L_, int nx_, int ny_, double hx, double hy, const double* hz, const double* z0, int px = 0, int py = 0)
 		: L(L_), nx(nx_), ny(ny_)
 	{
-		Nx = 2 * nx; Ny = 2 * ny; Nc = Nx / 2 + 1; Nyh = Ny / 2 + 1;
+		Nx = px ? nx : 2 * nx; Ny = py ? ny : 2 * ny; Nc = Nx / 2 + 1; Nyh = Ny / 2 + 1;
+		const int pbc3[3] = { px, py, 0 };
+		const bool periodic = px || py;
 		fx = Fft<R>(Nx); fy = Fft<R>(Ny);
 		double hmax = 0;
 		for (int l = 0; l < L; l++) hmax = std::max(hmax, std::max(hx, std::max(hy, hz[l])));
@@ -313,7 +423,7 @@ struct MLayer {
 			for (int s = 0; s < L; s++) {
 				const size_t p = (size_t)d * L + s;
 				if (d == s) {
-					Kernels K = make_kernels(nx, ny, 1, Nx, Ny, 1, hx, hy, hz[d]);
+					Kernels K = make_kernels(nx, ny, 1, Nx, Ny, 1, hx, hy, hz[d], periodic ? pbc3 : nullptr);
 					Kd[p].assign(K.Kdiag.begin(), K.Kdiag.end());
 					Ko[p].assign(K.Kodiag.begin(), K.Kodiag.end());
 					self[p] = 1;
@@ -321,7 +431,8 @@ struct MLayer {
 				else {
 					const double sc[3] = { hx / hmax, hy / hmax, hz[s] / hmax }, dc[3] = { hx / hmax, hy / hmax, hz[d] / hmax };
 					std::vector<double> Dd, Do, kd, ko;
-					tensor2d_zshift(nx, ny, Nx, Ny, sc, dc, (z0[d] - z0[s]) / hmax, Dd, Do);
+					if (periodic) tensor2d_zshift_pbc(Nx, Ny, sc, dc, (z0[d] - z0[s]) / hmax, px, py, Dd, Do);
+					else tensor2d_zshift(nx, ny, Nx, Ny, sc, dc, (z0[d] - z0[s]) / hmax, Dd, Do);
 					kernels2d_zshift(Nx, Ny, Dd, Do, kd, ko);
 					Kd[p].assign(kd.begin(), kd.end());
 					Ko[p].assign(ko.begin(), ko.end());

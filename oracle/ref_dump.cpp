@@ -21,6 +21,9 @@
 //   ref_dump selfpbc hx hy hz nx ny nz px py pz out.bin -> SelfDemag_PBC (3 doubles)
 //   ref_dump pbc2d Nx Ny hx hy hz px py pz out.bin
 //        -> Ddiag (Nx*Ny DBL3) then Dxy (Nx*Ny doubles) of the periodic 2D tensor
+//   ref_dump zshift2dpbc Nx Ny sx sy sz dx dy dz shz px py out.bin
+//        -> Ddiag then Dodiag (Nx*Ny DBL3 each) of the 2D z-shifted tensor with px, py
+//           images in x and y (N = n along a periodic axis, 2n otherwise)
 //
 // Reference entry points used: DemagTFunc::CalcDiagTens3D/CalcOffDiagTens3D/
 // CalcDiagTens2D/CalcOffDiagTens2D (Boris/DemagTFunc.cpp:21,95,172,237),
@@ -30,7 +33,9 @@
 // (Boris/DemagTFunc_Irregular.cpp:5,121),
 // DemagTFunc::CalcDiagTens3D_PBC/CalcOffDiagTens3D_PBC/CalcDiagTens2D_PBC/
 // CalcOffDiagTens2D_PBC (Boris/DemagTFunc_PBC.cpp:186,301,5,96),
-// DemagTFunc::SelfDemag_PBC (Boris/DemagTFunc_Self.cpp:58).
+// DemagTFunc::SelfDemag_PBC (Boris/DemagTFunc_Self.cpp:58),
+// DemagTFunc::CalcDiagTens2D_Shifted_Irregular_PBC/CalcOffDiagTens2D_Shifted_Irregular_PBC
+// (Boris/DemagTFunc_Irregular_PBC.cpp:5, falling back to Boris/DemagTFunc_Shifted_PBC.cpp:317/464).
 
 #include "stdafx.h"
 #include "DemagTFunc.h"
@@ -110,6 +115,22 @@ int main(int argc, char** argv)
 		if (!Dd.resize(SZ3(N.x, N.y, 1)) || !Do.resize(SZ3(N.x, N.y, 1))) return 3;
 		if (!dtf.CalcDiagTens2D_Shifted_Irregular(Dd, n, N, s, d, shift)) return 4;
 		if (!dtf.CalcOffDiagTens2D_Shifted_Irregular(Do, n, N, s, d, shift)) return 4;
+		std::vector<double> out;
+		for (size_t i = 0; i < Dd.linear_size(); i++) { out.push_back(Dd[i].x); out.push_back(Dd[i].y); out.push_back(Dd[i].z); }
+		for (size_t i = 0; i < Do.linear_size(); i++) { out.push_back(Do[i].x); out.push_back(Do[i].y); out.push_back(Do[i].z); }
+		write_doubles(argv[13], out);
+		return 0;
+	}
+	if (mode == "zshift2dpbc" && argc == 14) {
+		INT3 N(atoi(argv[2]), atoi(argv[3]), 1);
+		DBL3 s(atof(argv[4]), atof(argv[5]), atof(argv[6]));
+		DBL3 d(atof(argv[7]), atof(argv[8]), atof(argv[9]));
+		DBL3 shift(0.0, 0.0, atof(argv[10]));
+		const int px = atoi(argv[11]), py = atoi(argv[12]);
+		VEC<DBL3> Dd, Do;
+		if (!Dd.resize(SZ3(N.x, N.y, 1)) || !Do.resize(SZ3(N.x, N.y, 1))) return 3;
+		if (!dtf.CalcDiagTens2D_Shifted_Irregular_PBC(Dd, N, s, d, shift, true, ASYMPTOTIC_DISTANCE, px, py, 0)) return 4;
+		if (!dtf.CalcOffDiagTens2D_Shifted_Irregular_PBC(Do, N, s, d, shift, true, ASYMPTOTIC_DISTANCE, px, py, 0)) return 4;
 		std::vector<double> out;
 		for (size_t i = 0; i < Dd.linear_size(); i++) { out.push_back(Dd[i].x); out.push_back(Dd[i].y); out.push_back(Dd[i].z); }
 		for (size_t i = 0; i < Do.linear_size(); i++) { out.push_back(Do[i].x); out.push_back(Do[i].y); out.push_back(Do[i].z); }

@@ -19,6 +19,11 @@ reference checkout and oracle/_ref exist; the GPU test runs only read the result
   DemagTFunc_Irregular.cpp: regular (equal cells) and irregular (thicknesses differ), with
   near-field and far-field (>= 40 cells) entries; all entries for small meshes, a seeded
   sample for large ones.
+* zshift_pbc_golden.npz — FP64 2D z-shifted tensors with periodic images along x and y
+  (multilayered demag with PBC) written by oracle/_ref/ref_dump from the reference's
+  Boris/DemagTFunc_Shifted_PBC.cpp and DemagTFunc_Irregular_PBC.cpp: regular and irregular,
+  images along x, y or both, near-field only and with far-field images.
+  (`python tests/golden/make_fixtures.py zshift_pbc` writes this file alone.)
 * pbc_golden.npz — FP64 periodic demag tensors (N = n along periodic axes, images summed)
   written by oracle/_ref/ref_dump from the reference's Boris/DemagTFunc_PBC.cpp: 2D and 3D,
   images along x, y and z, near-field only and with far-field images.
@@ -110,6 +115,7 @@ def main() -> None:
     out["single_val"] = np.array([dump("single", *d, *hh) for d, hh in singles])
     np.savez_compressed(HERE / "tensor_golden.npz", **out)
     zshift_fixtures()
+    zshift_pbc_fixtures()
     pbc_fixtures()
     print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
 
@@ -188,5 +194,39 @@ def zshift_fixtures() -> None:
     np.savez_compressed(HERE / "zshift_golden.npz", **out)
 
 
+# (name, N, source cell, destination cell, shift, x images, y images), normalised
+ZSHIFT_PBC_CASES = [
+    ("zp_reg_x", (8, 12), (1.0, 1.0, 0.5), (1.0, 1.0, 0.5), 0.5, 2, 0),
+    ("zp_irr_y", (16, 8), (1.0, 0.8, 0.5), (1.0, 0.8, 0.25), 0.9, 0, 3),
+    ("zp_irr_xy", (10, 10), (0.5, 0.5, 1.0), (0.5, 0.5, 0.5), -1.25, 3, 3),
+    ("zp_reg_xy_far", (64, 64), (0.5, 0.5, 1.0), (0.5, 0.5, 1.0), 1.0, 2, 2),
+    ("zp_irr_x_far", (96, 12), (1.0, 1.0, 0.25), (1.0, 1.0, 0.5), 2.5, 1, 0),
+]
+
+
+def zshift_pbc_fixtures() -> None:
+    rng = np.random.default_rng(20261205)
+    out = {}
+    for name, NN, s, d, sh, px, py in ZSHIFT_PBC_CASES:
+        v = dump("zshift2dpbc", *NN, *map(float, s), *map(float, d), float(sh), px, py)
+        m = NN[0] * NN[1]
+        Dd = v[:3 * m].reshape(NN[1], NN[0], 3)
+        Do = v[3 * m:].reshape(NN[1], NN[0], 3)
+        if m <= 400:
+            idx = np.array([(j, i) for j in range(NN[1]) for i in range(NN[0])])
+        else:
+            idx = np.stack([rng.integers(0, NN[1], 1500), rng.integers(0, NN[0], 1500)], axis=1)
+        out[name + "_N"], out[name + "_img"] = np.array(NN), np.array([px, py])
+        out[name + "_s"], out[name + "_d"], out[name + "_shift"] = np.array(s), np.array(d), np.array(sh)
+        out[name + "_idx"] = idx
+        out[name + "_diag"] = Dd[idx[:, 0], idx[:, 1]]
+        out[name + "_off"] = Do[idx[:, 0], idx[:, 1]]
+    out["names"] = np.array([c[0] for c in ZSHIFT_PBC_CASES])
+    np.savez_compressed(HERE / "zshift_pbc_golden.npz", **out)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["zshift_pbc"]:
+        zshift_pbc_fixtures()
+    else:
+        main()

@@ -67,7 +67,9 @@ def lib() -> C.CDLL:
     L.or_speedup_lagrange.argtypes = [i, dp, d, dp]
     L.or_tensor2d_zshift.argtypes = [i] * 4 + [dp, dp, d, dp, dp]
     L.or_kernels2d_zshift.argtypes = [i, i, dp, dp, dp, dp]
+    L.or_tensor2d_zshift_pbc.argtypes = [i, i, dp, dp, d, i, i, dp, dp]
     L.or_mlayer_create.argtypes = [i, i, i, d, d, dp, dp]; L.or_mlayer_create.restype = vp
+    L.or_mlayer_create_pbc.argtypes = [i, i, i, d, d, dp, dp, i, i]; L.or_mlayer_create_pbc.restype = vp
     L.or_mlayer_destroy.argtypes = [vp]
     L.or_mlayer_convolute.argtypes = [vp, dp, dp]
     L.or_mlayer_convolute_transposed.argtypes = [vp, dp, dp]
@@ -139,6 +141,15 @@ def tensor2d_zshift(n, N, s, d, shift):
     return a.reshape(N[1], N[0], 3), b.reshape(N[1], N[0], 3)
 
 
+def tensor2d_zshift_pbc(N, s, d, shift, px, py):
+    """The 2D z-shifted tensor with px, py periodic images (N = n along a periodic axis):
+    (Dd, Do), each (Ny, Nx, 3)."""
+    L = lib()
+    a, b = np.zeros(N[0] * N[1] * 3), np.zeros(N[0] * N[1] * 3)
+    assert L.or_tensor2d_zshift_pbc(N[0], N[1], np.array(s, float), np.array(d, float), float(shift), int(px), int(py), a, b) == 0
+    return a.reshape(N[1], N[0], 3), b.reshape(N[1], N[0], 3)
+
+
 def kernels2d_zshift(N, Dd, Do):
     """Real kernels of a z-shifted 2D tensor: (Kd, Ko), each (Ny/2+1, Nx/2+1, 3)."""
     L = lib()
@@ -151,12 +162,16 @@ def kernels2d_zshift(N, Dd, Do):
 
 class OracleMLayer:
     """The oracle's multilayered 2D demag: L layers of nx x ny cells (hx, hy, hz[l]) with
-    bottom faces at z0[l]; M and H are (L*nx*ny, 3), layer-major."""
+    bottom faces at z0[l]; M and H are (L*nx*ny, 3), layer-major.  pbc: (px, py) periodic
+    images along x and y."""
 
-    def __init__(self, nx, ny, hx, hy, hz, z0):
+    def __init__(self, nx, ny, hx, hy, hz, z0, pbc=(0, 0)):
         self.L = lib()
         self.nl = len(hz)
-        self.h = self.L.or_mlayer_create(self.nl, nx, ny, hx, hy, np.array(hz, float), np.array(z0, float))
+        if any(pbc):
+            self.h = self.L.or_mlayer_create_pbc(self.nl, nx, ny, hx, hy, np.array(hz, float), np.array(z0, float), int(pbc[0]), int(pbc[1]))
+        else:
+            self.h = self.L.or_mlayer_create(self.nl, nx, ny, hx, hy, np.array(hz, float), np.array(z0, float))
         assert self.h
 
     def __call__(self, M):
