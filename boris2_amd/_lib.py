@@ -24,7 +24,7 @@ EXPORTS = [
     "bm_ode_restore", "bm_ode_get_state", "bm_ode_keep_heff", "bm_ode_copy_heff", "bm_ode_average_m",
     "bm_ode_set_extra_field",
     "bm_prof_begin", "bm_prof_end", "bm_prof_query",
-    "bm_mlayer_create", "bm_mlayer_create_meshes", "bm_mlayer_destroy", "bm_mlayer_convolute", "bm_mlayer_copy_kernels",
+    "bm_mlayer_create", "bm_mlayer_create_pbc", "bm_mlayer_create_meshes", "bm_mlayer_destroy", "bm_mlayer_convolute", "bm_mlayer_copy_kernels",
     "bm_mlayer_forward", "bm_mlayer_multiply", "bm_mlayer_inverse",
     "bm_batch_create", "bm_batch_create_layers", "bm_batch_destroy", "bm_batch_set_dt", "bm_batch_run", "bm_batch_get_state",
     "bm_batch_mesh_offset", "bm_batch_autotune_report", "bm_batch_set_schedule", "bm_batch_set_pbc", "bm_batch_set_astep", "bm_batch_get_mesh_state",
@@ -95,6 +95,7 @@ def load() -> C.CDLL:
         "bm_demag_convolute_averaged_static": (i, [vp, vp, vp, vp, vp, i, vp, vp]),
         "bm_demag_copy_kernels": (i, [vp, vp, ll]),
         "bm_mlayer_create": (i, [i, i, i, d, d, vp, vp, i, C.POINTER(vp)]),
+        "bm_mlayer_create_pbc": (i, [i, i, i, d, d, vp, vp, i, i, i, C.POINTER(vp)]),
         "bm_mlayer_create_meshes": (i, [i, vp, i, C.POINTER(vp)]),
         "bm_mlayer_destroy": (i, [vp]),
         "bm_mlayer_convolute": (i, [vp, vp, vp, i, vp]),

@@ -186,9 +186,9 @@ class MultiLayerMeshDemag:
 class MultiLayerDemag:
     """Multilayered demag of stacked 2D layers: nlayers layers of nx*ny cells (hx, hy, hz[l]),
     layer l's bottom face at height z0[l] (m).  M and H are (nlayers*nx*ny, 3) float32 CUDA
-    tensors, layer-major."""
+    tensors, layer-major.  pbc: (px, py) periodic images along x and y."""
 
-    def __init__(self, nx: int, ny: int, hx: float, hy: float, hz, z0):
+    def __init__(self, nx: int, ny: int, hx: float, hy: float, hz, z0, pbc=(0, 0)):
         require_device()
         self.L = _lib.load()
         self.nx, self.ny = int(nx), int(ny)
@@ -198,9 +198,10 @@ class MultiLayerDemag:
             raise ValueError("hz and z0 must be 1-D of the same length")
         self.nlayers = len(self.hz)
         self.handle = C.c_void_p()
-        check(self.L.bm_mlayer_create(self.nlayers, self.nx, self.ny, float(hx), float(hy),
-                                      self.hz.ctypes.data_as(C.c_void_p), self.z0.ctypes.data_as(C.c_void_p), 0,
-                                      C.byref(self.handle)))
+        self.pbc = tuple(int(v) for v in pbc)
+        check(self.L.bm_mlayer_create_pbc(self.nlayers, self.nx, self.ny, float(hx), float(hy),
+                                          self.hz.ctypes.data_as(C.c_void_p), self.z0.ctypes.data_as(C.c_void_p),
+                                          *self.pbc, 0, C.byref(self.handle)))
 
     @property
     def ncells(self) -> int:
@@ -410,18 +411,20 @@ class LayerStackSolver(FMSolver):
     its bottom face at z0[l], the layers coupled by the multilayered demag and each with its
     own Ms, A, alpha, K1 (per-layer sequences; None keeps params' value).  The reference
     holds such a stack as one ferromagnetic mesh per layer under SDemag (Boris/SDemag.cpp).
-    M is (L*nx*ny, 3), layer-major."""
+    M is (L*nx*ny, 3), layer-major.  pbc: (px, py) periodic images along x and y (demag and
+    in-plane exchange)."""
 
     def __init__(self, method: int, nx: int, ny: int, hx: float, hy: float, hz, z0, params: FMParams,
-                 Ms=None, A=None, alpha=None, K1=None, M0=None):
+                 Ms=None, A=None, alpha=None, K1=None, M0=None, pbc=(0, 0)):
         hz = np.ascontiguousarray(hz, dtype=np.float64)
         nl = len(hz)
-        self.mlayer = MultiLayerDemag(nx, ny, hx, hy, hz, z0) if params.modules & BM_MOD_DEMAG else None
+        pbc = tuple(int(v) for v in pbc)
+        self.mlayer = MultiLayerDemag(nx, ny, hx, hy, hz, z0, pbc) if params.modules & BM_MOD_DEMAG else None
         base = FMParams(**{**params.__dict__, "modules": params.modules & ~BM_MOD_DEMAG})
         if M0 is None and Ms is not None:
             M0 = np.zeros((nl * nx * ny, 3), dtype=np.float32)
             M0[:, 0] = -np.repeat(np.asarray(Ms, dtype=np.float32), nx * ny)
-        super().__init__(method, (nx, ny, nl), (hx, hy, float(hz.max())), base, M0)
+        super().__init__(method, (nx, ny, nl), (hx, hy, float(hz.max())), base, M0, pbc=(pbc[0], pbc[1], 0))
         self._lay = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (Ms, A, alpha, K1)]
         for v in self._lay:
             if v is not None and v.shape != (nl,):
@@ -430,7 +433,7 @@ class LayerStackSolver(FMSolver):
         if self.mlayer is not None or any(v is not None for v in self._lay):
             # without demag the stack still needs its per-layer values: attach a handle anyway
             if self.mlayer is None:
-                self.mlayer = MultiLayerDemag(nx, ny, hx, hy, hz, z0)
+                self.mlayer = MultiLayerDemag(nx, ny, hx, hy, hz, z0, pbc)
             check(self.L.bm_ode_set_layers(self.handle, self.mlayer.handle, *ptrs))
         self.set_params(params)
 
@@ -594,7 +597,7 @@ class RK4Adjoint:
         if layers is not None:
             if not params.modules & BM_MOD_DEMAG:
                 raise ValueError("a layer stack needs the demag module")
-            self.mlayer = MultiLayerDemag(self.n[0], self.n[1], self.h[0], self.h[1], layers["hz"], layers["z0"])
+            self.mlayer = MultiLayerDemag(self.n[0], self.n[1], self.h[0], self.h[1], layers["hz"], layers["z0"], (pbc[0], pbc[1]))
             pc = FMParams(**{**params.__dict__, "modules": params.modules & ~BM_MOD_DEMAG})
         elif params.modules & BM_MOD_DEMAG:
             self.demag = Demag(self.n, self.h, pbc)

@@ -665,7 +665,10 @@ extern "C" int bm_adj_set_layers(bm_adj_t o, bm_mlayer_t ml, const double* A, co
 	int L, nx, ny;
 	mlayer_dims(ml, &L, &nx, &ny, nullptr);
 	if (mlayer_mapped(ml)) return fail(BM_ERR_SIZE, "bm_adj_set_layers: the layers must share one planned grid");
-	if (o->pbc[0] || o->pbc[1] || o->pbc[2]) return fail(BM_ERR_STATE, "bm_adj_set_layers: the mesh is periodic (layer stacks have no periodic variant)");
+	int mp[2];
+	bm::mlayer_pbc(ml, mp);
+	if ((o->pbc[0] || o->pbc[1] || o->pbc[2]) && (o->pbc[2] || mp[0] != o->pbc[0] || mp[1] != o->pbc[1]))
+		return fail(BM_ERR_STATE, "bm_adj_set_layers: the mesh's periodic images differ from the multilayer handle's");
 	if (L != o->nz || nx != o->nx || ny != o->ny) return fail(BM_ERR_ARG, "bm_adj_set_layers: the stack's layers must be the mesh's nz planes (same nx, ny)");
 	if (L > kMaxAdjLayers) return fail(BM_ERR_SIZE, "bm_adj_set_layers: at most 8 layers");
 	const double* src[3] = { A, alpha, K1 };
@@ -686,8 +689,15 @@ extern "C" int bm_adj_set_layers(bm_adj_t o, bm_mlayer_t ml, const double* A, co
 extern "C" int bm_adj_set_pbc(bm_adj_t o, int pbc_x, int pbc_y, int pbc_z)
 {
 	if (!o || pbc_x < 0 || pbc_y < 0 || pbc_z < 0) return fail(BM_ERR_ARG, "bm_adj_set_pbc: bad arguments");
-	if (o->ml) return fail(BM_ERR_STATE, "bm_adj_set_pbc: layer stacks have no periodic variant");
-	if ((o->p.modules & BM_MOD_DEMAG) && (o->demag->pbc[0] != pbc_x || o->demag->pbc[1] != pbc_y || o->demag->pbc[2] != pbc_z))
+	if (o->ml) {
+		// a layer stack: the multilayer handle carries the x, y images (bm_mlayer_create_pbc);
+		// its transposed operator stays exact (kernels even or odd along each periodic axis)
+		int mp[2];
+		bm::mlayer_pbc(o->ml, mp);
+		if (pbc_z || mp[0] != pbc_x || mp[1] != pbc_y)
+			return fail(BM_ERR_STATE, "bm_adj_set_pbc: a layer stack takes the x, y images of its multilayer handle (bm_mlayer_create_pbc), none along z");
+	}
+	else if ((o->p.modules & BM_MOD_DEMAG) && (o->demag->pbc[0] != pbc_x || o->demag->pbc[1] != pbc_y || o->demag->pbc[2] != pbc_z))
 		return fail(BM_ERR_STATE, "bm_adj_set_pbc: the demag handle must have the same periodic images (bm_demag_create_pbc)");
 	o->pbc[0] = pbc_x; o->pbc[1] = pbc_y; o->pbc[2] = pbc_z;
 	o->forward_done = false;

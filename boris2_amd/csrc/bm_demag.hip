@@ -709,7 +709,7 @@ int launch_inv_x(int HX, bool full, int mode, const float2* Sx, float* out, int 
 	switch (HX) {
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; int TY = TY1 * f, nt = (ny + TY - 1) / TY; dim3 g(grid_for(3 * nz * nt, ncu, 2)); \
 		const PassBatch* nb = nullptr; \
-		if (full) launch_inv_x_var<V_, TY1, true, false>(mode, g, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, HX); \
+		if (full) launch_inv_x_var<V_, TY1, true, false>(mode, g, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, HX, zoff, nzl); \
 		else if (gen) launch_inv_x_var<V_, TY1, false, true>(mode, g, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, nx, zoff, nzl); \
 		else if (mode == 0 && f != 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, HX, 0, 0, nb); \
 		else launch_inv_x_var<V_, TY1, false, false>(mode, g, st, Sx, out, ny, nz, ntl, nt, scale, M, epart, HX, zoff, nzl); return BM_OK; }
@@ -1173,27 +1173,28 @@ extern "C" int bm_demag_copy_kernels(bm_demag_t d, float* host_out, long long co
 
 // ---------------------------------------------------------------------------------------
 // Pass launchers shared with the multilayer convolution (bm_mlayer.hip).
-int bm::pass_fwd_x(int nx, const float* M, float2* Sx, int ny, int nplanes, int ntl, int ncu, hipStream_t st)
+int bm::pass_fwd_x(int nx, const float* M, float2* Sx, int ny, int nplanes, int ntl, int ncu, hipStream_t st, bool full)
 {
 	prof_mark_begin(PK_FWD_X, st);
-	int rc = launch_fwd_x(nx, false, M, Sx, ny, nplanes, ntl, ncu, st);
+	int rc = launch_fwd_x(nx, full, M, Sx, ny, nplanes, ntl, ncu, st);
 	prof_mark_end(PK_FWD_X, st);
 	return rc;
 }
 
-int bm::pass_y(int Ny, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ncu, hipStream_t st)
+int bm::pass_y(int Ny, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ncu, hipStream_t st, bool full)
 {
 	const int kind = dir < 0 ? PK_Y_FWD : PK_Y_INV;
 	prof_mark_begin(kind, st);
-	int rc = launch_y(Ny, false, dir, S, Sx, nlines, Sp, NC, ntl, Ny / 2, ncu, st);
+	int rc = launch_y(Ny, full, dir, S, Sx, nlines, Sp, NC, ntl, full ? Ny : Ny / 2, ncu, st);
 	prof_mark_end(kind, st);
 	return rc;
 }
 
-int bm::pass_inv_x(int nx, int mode, const float2* Sx, float* out, int ny, int nplanes, int ntl, float scale, int ncu, hipStream_t st)
+int bm::pass_inv_x(int nx, int mode, const float2* Sx, float* out, int ny, int nplanes, int ntl, float scale, int ncu, hipStream_t st,
+	bool full)
 {
 	prof_mark_begin(PK_INV_X, st);
-	int rc = launch_inv_x(nx, false, mode, Sx, out, ny, nplanes, ntl, scale, ncu, st);
+	int rc = launch_inv_x(nx, full, mode, Sx, out, ny, nplanes, ntl, scale, ncu, st);
 	prof_mark_end(PK_INV_X, st);
 	return rc;
 }
@@ -1210,10 +1211,10 @@ int bm::setup_line_fft(int N, const double* src, float* dst, int nlines, int na,
 	return launch_line_fft<float>(N, src, dst, nlines, na, sa, sb, se, da, db, de, keep, part, sign, st);
 }
 
-int bm::pass_fwd_x_plane(int nx, const float* M, float2* Sx, int ny, int plane, int nzl, int ntl, int ncu, hipStream_t st)
+int bm::pass_fwd_x_plane(int nx, const float* M, float2* Sx, int ny, int plane, int nzl, int ntl, int ncu, hipStream_t st, bool full)
 {
 	prof_mark_begin(PK_FWD_X, st);
-	int rc = launch_fwd_x(nx, false, M, Sx, ny, 1, ntl, ncu, st, nullptr, nullptr, nullptr, 0.f, 0, plane, nzl);
+	int rc = launch_fwd_x(nx, full, M, Sx, ny, 1, ntl, ncu, st, nullptr, nullptr, nullptr, 0.f, 0, plane, nzl);
 	prof_mark_end(PK_FWD_X, st);
 	return rc;
 }
@@ -1221,18 +1222,25 @@ int bm::pass_fwd_x_plane(int nx, const float* M, float2* Sx, int ny, int plane, 
 int bm::x_plane_parts(int nx, int ny) { return std::max(1, inv_x_nwork(nx, ny, 1)); }
 
 int bm::pass_inv_x_plane(int nx, int mode, const float2* Sx, float* out, int ny, int plane, int nzl, int ntl, float scale, int ncu,
-	hipStream_t st, const float* M, double* epart, float* energy)
+	hipStream_t st, const float* M, double* epart, float* energy, bool full)
 {
 	if (mode != 1 && mode != 2) return fail(BM_ERR_ARG, "plane x pass: AoS set or add");
 	prof_mark_begin(PK_INV_X, st);
-	int rc = launch_inv_x(nx, false, mode, Sx, out, ny, 1, ntl, scale, ncu, st, energy ? M : nullptr, energy ? epart : nullptr, 0, plane, nzl);
+	int rc = launch_inv_x(nx, full, mode, Sx, out, ny, 1, ntl, scale, ncu, st, energy ? M : nullptr, energy ? epart : nullptr, 0, plane, nzl);
 	prof_mark_end(PK_INV_X, st);
 	if (rc || !energy) return rc;
-	hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, st, epart, inv_x_nwork(nx, ny, 1), (size_t)nx * ny, energy);
+	hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, st, epart, inv_x_nwork(nx, ny, 1), (size_t)(full ? 2 * nx : nx) * ny, energy);
 	return BM_OK;
 }
 
 bool bm::plan_2d_ok(int nx, int Ny) { return x_supported(nx) && x_supported(Ny) && line_supported(2 * nx); }
+
+bool bm::plan_2d_pbc_ok(int nx, int ny, bool fx, bool fy)
+{
+	if (fx && nx % 2) return false;
+	const int HX = fx ? nx / 2 : nx, Ny = fy ? ny : 2 * ny;
+	return x_supported(HX) && line_supported(2 * HX) && y_supported(Ny);
+}
 
 // ---------------------------------------------------------------------------------------
 // Batched pass launchers: nb meshes of one pass size; h_tab (host) sizes the grid, d_tab

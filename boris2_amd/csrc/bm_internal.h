@@ -62,17 +62,19 @@ int demag_fwd_x_advance(bm_demag_s* d, const float* M, const float* E0, float* s
 int demag_mid(bm_demag_s* d, hipStream_t st);
 // single passes over "planes" of ny rows (a z plane or a layer) for other pipelines; the y
 // passes go between the row-tiled Sx (ny rows, ntl tiles per plane) and the full spectra S
-// (Ny rows, line pitch Sp), nlines = planes * NC lines
-int pass_fwd_x(int nx, const float* M, float2* Sx, int ny, int nplanes, int ntl, int ncu, hipStream_t st);
-int pass_y(int Ny, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ncu, hipStream_t st);
-int pass_inv_x(int nx, int mode, const float2* Sx, float* out, int ny, int nplanes, int ntl, float scale, int ncu, hipStream_t st);
+// (Ny rows, line pitch Sp), nlines = planes * NC lines.  nx: the x-pass size N.x / 2; full: the
+// axis is periodic (x: rows of N.x = 2 nx cells; y: N.y data rows, no zero padding)
+int pass_fwd_x(int nx, const float* M, float2* Sx, int ny, int nplanes, int ntl, int ncu, hipStream_t st, bool full = false);
+int pass_y(int Ny, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ncu, hipStream_t st, bool full = false);
+int pass_inv_x(int nx, int mode, const float2* Sx, float* out, int ny, int nplanes, int ntl, float scale, int ncu, hipStream_t st,
+	bool full = false);
 // the x passes of plane `plane` of a spectra array of nzl planes per component (one layer
 // of a multilayered convolution): M / out hold that plane only (nx * ny cells, AoS); the
 // inverse sets (mode 1) or adds (mode 2) and, with energy set, writes -mu0/(2 nx ny) sum
 // M.H of the plane into *energy using epart (x_plane_parts(nx, ny) doubles)
-int pass_fwd_x_plane(int nx, const float* M, float2* Sx, int ny, int plane, int nzl, int ntl, int ncu, hipStream_t st);
+int pass_fwd_x_plane(int nx, const float* M, float2* Sx, int ny, int plane, int nzl, int ntl, int ncu, hipStream_t st, bool full = false);
 int pass_inv_x_plane(int nx, int mode, const float2* Sx, float* out, int ny, int plane, int nzl, int ntl, float scale, int ncu,
-	hipStream_t st, const float* M, double* epart, float* energy);
+	hipStream_t st, const float* M, double* epart, float* energy, bool full = false);
 int x_plane_parts(int nx, int ny);
 // FP64 real-input line FFTs of the kernel setup (see k_line_fft)
 int setup_line_fft(int N, const double* src, double* dst, int nlines, int na, long long sa, long long sb, long long se,
@@ -111,6 +113,10 @@ void mlayer_dims(const bm_mlayer_s* m, int* L, int* nx, int* ny, double* hz);
 bool mlayer_mapped(const bm_mlayer_s* m);
 // the smallest x-pass length (2^a 3^b, 2..2048) at or above v, 0 if none
 int plan_length_at_least(int v);
+// a 2D grid of nx x ny cells has plans with x (fx) and y (fy) periodic or open
+bool plan_2d_pbc_ok(int nx, int ny, bool fx, bool fy);
+// periodic images along x and y of a multilayer handle (0, 0: open)
+void mlayer_pbc(const bm_mlayer_s* m, int out[2]);
 // Pointers and sizes of a multilayered demag handle (batched launches).
 struct MlView {
 	float2* Sx; float2* S; const float* K;

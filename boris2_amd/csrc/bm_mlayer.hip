@@ -181,6 +181,7 @@ __device__ double L36_plane2(const G3& G, const G3& Dl, int i, int k, double tau
 
 struct ZArgs {
 	int nx, ny, Nx, Ny, regular;
+	int px, py;               // periodic images along x and y (N = n along a periodic axis)
 	double hx, hy, hz, shz, ks, tau, far;
 	G3 g[6], del[6];          // f_xx, f_yy, f_zz, g_xy, g_xz, g_yz (+ their extra planes)
 	AsymDiag ad[3];
@@ -193,6 +194,74 @@ struct ZArgs {
 // source thickness for irregular pairs, as the reference does; xy plane mirror images
 // with the reference's write order (Boris/DemagTFunc_Shifted.cpp:235-449,
 // Boris/DemagTFunc_Irregular.cpp:5-235).
+// With periodic images (px or py > 0): every offset i0 (j0) in [0, N) along a periodic axis
+// sums the images i = i_img N + i0, i_img = -p..p (x images outer), each at |i| by the
+// stencils (grids out to the far-field distance) or the far-field series, the off-diagonal
+// terms signed by sign(i) sign(j) (xy), sign(i) (xz), sign(j) (yz) with sign(0) = 0; an open
+// axis is mirrored as without images (Boris/DemagTFunc_Shifted_PBC.cpp:317-600,
+// Boris/DemagTFunc_Irregular_PBC.cpp:5-235, z-shift branch).
+__device__ __forceinline__ int zsign0(int v) { return (v > 0) - (v < 0); }
+
+__device__ __forceinline__ void zshift_term(const ZArgs& a, int i, int j, double vd[3], double vo[3])
+{
+	const int A = kAsymDist;
+	const double hx = a.hx, hy = a.hy, hz = a.hz, sz = a.shz;
+	if (i >= A || j >= A || (int)floor(a.ks + 1e-4) >= A || (int)floor(i * i + j * j + a.ks * a.ks + 1e-4) >= A * A) {
+		vd[0] = (-asym_diag(a.ad[0], i * hx, j * hy, sz)) * a.far;
+		vd[1] = (-asym_diag(a.ad[1], j * hy, i * hx, sz)) * a.far;
+		vd[2] = (-asym_diag(a.ad[2], sz, j * hy, i * hx)) * a.far;
+		vo[0] = (-asym_off(a.ao[0], i * hx, j * hy, sz)) * a.far;
+		vo[1] = (-asym_off(a.ao[1], i * hx, sz, j * hy)) * a.far;
+		vo[2] = (-asym_off(a.ao[2], j * hy, sz, i * hx)) * a.far;
+	}
+	else if (a.regular) {
+		vd[0] = -stencil27(a.g[0].v, a.g[0].gx, a.g[0].gy, i, j, 0, hx, hy, hz);
+		vd[1] = -stencil27(a.g[1].v, a.g[1].gx, a.g[1].gy, j, i, 0, hy, hx, hz);
+		vd[2] = -stencil27(a.g[2].v, a.g[2].gx, a.g[2].gy, 0, j, i, hz, hy, hx);
+		vo[0] = -stencil27(a.g[3].v, a.g[3].gx, a.g[3].gy, i, j, 0, hx, hy, hz);
+		vo[1] = -stencil27(a.g[4].v, a.g[4].gx, a.g[4].gy, i, 0, j, hx, hz, hy);
+		vo[2] = -stencil27(a.g[5].v, a.g[5].gx, a.g[5].gy, j, 0, i, hy, hz, hx);
+	}
+	else {
+		vd[0] = -L36_plane3(a.g[0], a.del[0], i, j, a.tau);
+		vd[1] = -L36_plane3(a.g[1], a.del[1], j, i, a.tau);
+		vd[2] = -L36_zz(a.g[2], a.del[2], j, i, a.tau);
+		vo[0] = -L36_plane3(a.g[3], a.del[3], i, j, a.tau);
+		vo[1] = -L36_plane2(a.g[4], a.del[4], i, j, a.tau);
+		vo[2] = -L36_plane2(a.g[5], a.del[5], j, i, a.tau);
+	}
+}
+
+__global__ void k_tensor_zshift_pbc(ZArgs a, double* D)
+{
+	const size_t NN = (size_t)a.Nx * a.Ny;
+	// offsets: [0, n) per axis, i.e. [0, N) along a periodic axis and [0, N/2) along an open one
+	for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < (size_t)a.nx * a.ny; q += (size_t)gridDim.x * blockDim.x) {
+		const int i0 = (int)(q % a.nx), j0 = (int)(q / a.nx);
+		double vd[3] = { 0.0, 0.0, 0.0 }, vo[3] = { 0.0, 0.0, 0.0 };
+		for (int ii = -a.px; ii <= a.px; ii++)
+			for (int jj = -a.py; jj <= a.py; jj++) {
+				const int is = ii * a.Nx + i0, js = jj * a.Ny + j0;
+				const double si = zsign0(is), sj = zsign0(js);
+				double td[3], to[3];
+				zshift_term(a, abs(is), abs(js), td, to);
+				vd[0] += td[0]; vd[1] += td[1]; vd[2] += td[2];
+				vo[0] += to[0] * (si * sj); vo[1] += to[1] * si; vo[2] += to[2] * sj;
+			}
+		auto put = [&](size_t idx, double ox, double oy, double oz) {
+			D[idx] = vd[0]; D[NN + idx] = vd[1]; D[2 * NN + idx] = vd[2];
+			D[3 * NN + idx] = vo[0] * ox; D[4 * NN + idx] = vo[1] * oy; D[5 * NN + idx] = vo[2] * oz;
+		};
+		const size_t mx = (size_t)((a.Nx - i0) % a.Nx), my = (size_t)((a.Ny - j0) % a.Ny);
+		put((size_t)i0 + (size_t)j0 * a.Nx, 1, 1, 1);
+		if (!a.px) {
+			if (i0) put(mx + (size_t)j0 * a.Nx, -1, -1, 1);
+			if (!a.py && i0 && j0) put(mx + my * a.Nx, 1, -1, -1);
+		}
+		if (!a.py && j0) put((size_t)i0 + my * a.Nx, -1, 1, -1);
+	}
+}
+
 __global__ void k_tensor_zshift(ZArgs a, double* D)
 {
 	const size_t NN = (size_t)a.Nx * a.Ny;
@@ -359,6 +428,9 @@ int launch_ml_mult(int L, float2* S, float2* S2, const float* K, const int* kidx
 
 struct bm_mlayer_s {
 	int L, nx, ny, Nx, Ny, NC, NYH, Sp, ntl;
+	int xs = 0;                  // x-pass size N.x / 2 (nx, or nx / 2 along a periodic x)
+	int pbc[2] = { 0, 0 };       // periodic images along x and y (bm_mlayer_create_pbc)
+	bool periodic() const { return pbc[0] || pbc[1]; }
 	double hx, hy;
 	std::vector<double> hz, z0;
 	float2* Sx = nullptr;    // x-side spectra, row-tiled (bm_xpass.h): 3 * L * ntl * NC * kLT complex
@@ -398,7 +470,8 @@ int zshift_kernels(bm_mlayer_s* m, int d, int s, double hmax, double* D, double*
 	for (int c = 0; c < 3; c++) if (nzero(sc[c] * 1e-9 - dc[c] * 1e-9)) regular = false;
 	// the regular path runs on the source cell (Boris/DemagTFunc_Irregular.cpp:11)
 	const double hx = regular ? sc[0] : dc[0], hy = regular ? sc[1] : dc[1], hz = regular ? sc[2] : dc[2];
-	const int dx = std::min(nx, kAsymDist), dy = std::min(ny, kAsymDist);
+	// near-field grids: out to the far-field distance along a periodic axis (images reach it)
+	const int dx = m->pbc[0] ? kAsymDist : std::min(nx, kAsymDist), dy = m->pbc[1] ? kAsymDist : std::min(ny, kAsymDist);
 	struct Spec { int which, gx, gy, gz; Axis X, Y, Z; };
 	Spec sp[12];
 	if (regular) {
@@ -432,6 +505,7 @@ int zshift_kernels(bm_mlayer_s* m, int d, int s, double hmax, double* D, double*
 	ZArgs a;
 	std::memset(&a, 0, sizeof(a));
 	a.nx = nx; a.ny = ny; a.Nx = Nx; a.Ny = Ny; a.regular = regular ? 1 : 0;
+	a.px = m->pbc[0]; a.py = m->pbc[1];
 	a.hx = hx; a.hy = hy; a.hz = hz; a.shz = shz; a.ks = std::fabs(shz / hz); a.tau = hx * hy * hz;
 	a.far = regular ? 1.0 : sc[2];
 	const int ng = regular ? 6 : 12;
@@ -446,7 +520,9 @@ int zshift_kernels(bm_mlayer_s* m, int d, int s, double hmax, double* D, double*
 	asym_off_init(a.ao[0], hx, hy, hz); asym_off_init(a.ao[1], hx, hz, hy); asym_off_init(a.ao[2], hy, hz, hx);
 	const size_t NN = (size_t)Nx * Ny;
 	BM_HIP(hipMemsetAsync(D, 0, NN * 6 * sizeof(double), st));
-	hipLaunchKernelGGL(k_tensor_zshift, dim3((unsigned)std::min<size_t>(((size_t)nx * ny + 127) / 128, 65535)), dim3(128), 0, st, a, D);
+	const dim3 tg((unsigned)std::min<size_t>(((size_t)nx * ny + 127) / 128, 65535));
+	if (a.px || a.py) hipLaunchKernelGGL(k_tensor_zshift_pbc, tg, dim3(128), 0, st, a, D);
+	else hipLaunchKernelGGL(k_tensor_zshift, tg, dim3(128), 0, st, a, D);
 	// x then y real-input FFTs with the part each component keeps
 	// (Boris/DemagKernelCollection_Calc.cpp:288-420): diagonal Re, Re; xy Im, -Im; xz Im, Re; yz Re, Im
 	static const int px[6] = { 0, 0, 0, 1, 1, 0 }, py[6] = { 0, 0, 0, 1, 0, 1 };
@@ -473,7 +549,8 @@ int compute_ml_kernels(bm_mlayer_s* m, hipStream_t st)
 		if (done[slot]) continue;
 		done[slot] = 1;
 		bm_demag_t dl = nullptr;
-		int rc = bm_demag_create(m->nx, m->ny, 1, m->hx, m->hy, m->hz[l], 0, &dl);
+		int rc = m->periodic() ? bm_demag_create_pbc(m->nx, m->ny, 1, m->hx, m->hy, m->hz[l], m->pbc[0], m->pbc[1], 0, 0, &dl)
+		                       : bm_demag_create(m->nx, m->ny, 1, m->hx, m->hy, m->hz[l], 0, &dl);
 		if (rc) return rc;
 		float* dst = m->K + m->kbase(l, l) * 6 * kq;
 		hipError_t e = hipMemcpyAsync(dst, dl->K, 4 * kq * sizeof(float), hipMemcpyDeviceToDevice, st);
@@ -482,7 +559,7 @@ int compute_ml_kernels(bm_mlayer_s* m, hipStream_t st)
 		if (e != hipSuccess) return hip_fail(e, "multilayer self kernels");
 	}
 	if (L == 1) return BM_OK;
-	const int dx = std::min(m->nx, kAsymDist), dy = std::min(m->ny, kAsymDist);
+	const int dx = m->pbc[0] ? kAsymDist : std::min(m->nx, kAsymDist), dy = m->pbc[1] ? kAsymDist : std::min(m->ny, kAsymDist);
 	const size_t gmax = (size_t)(std::max(dx, dy) + 2) * (std::max(dx, dy) + 2) * 3;
 	double* D = nullptr; double* T1 = nullptr; double* grids[12] = { nullptr };
 	int rc = BM_OK;
@@ -519,11 +596,16 @@ int compute_ml_kernels(bm_mlayer_s* m, hipStream_t st)
 
 using bm::fail;
 
-// Handle on one grid of nx x ny cells with planned lengths (nx and 2 ny).
-static int ml_create_grid(int nlayers, int nx, int ny, double hx, double hy, const double* hz, const double* z0, bm_mlayer_t* out)
+// Handle on one grid of nx x ny cells with planned lengths (nx and 2 ny; along a periodic
+// axis N = n: nx / 2 and ny).
+static int ml_create_grid(int nlayers, int nx, int ny, double hx, double hy, const double* hz, const double* z0, bm_mlayer_t* out,
+	int px = 0, int py = 0)
 {
 	bm_mlayer_s* m = new bm_mlayer_s();
-	m->L = nlayers; m->nx = nx; m->ny = ny; m->Nx = 2 * nx; m->Ny = 2 * ny; m->NC = nx + 1; m->NYH = ny + 1; m->Sp = m->Ny;
+	m->L = nlayers; m->nx = nx; m->ny = ny;
+	m->pbc[0] = px; m->pbc[1] = py;
+	m->Nx = px ? nx : 2 * nx; m->Ny = py ? ny : 2 * ny; m->NC = m->Nx / 2 + 1; m->NYH = m->Ny / 2 + 1; m->Sp = m->Ny;
+	m->xs = m->Nx / 2;
 	m->hx = hx; m->hy = hy; m->hz.assign(hz, hz + nlayers); m->z0.assign(z0, z0 + nlayers);
 	m->ncu = bm::device_cu_count();
 	m->ntl = bm::sx_tiles(ny);
@@ -550,7 +632,7 @@ static int ml_create_grid(int nlayers, int nx, int ny, double hx, double hy, con
 		|| (nlayers > kRegLayers && hipMalloc(&m->S2, sbytes) != hipSuccess)
 		|| hipMalloc(&m->K, (size_t)m->kcount * sizeof(float)) != hipSuccess
 		|| (nlayers > kRegLayers && hipMalloc(&m->kidx, m->hkidx.size() * sizeof(int)) != hipSuccess)
-		|| hipMalloc(&m->ebuf, (size_t)bm::x_plane_parts(nx, ny) * sizeof(double)) != hipSuccess) {
+		|| hipMalloc(&m->ebuf, (size_t)bm::x_plane_parts(m->xs, ny) * sizeof(double)) != hipSuccess) {
 		bm_mlayer_destroy(m);
 		return fail(BM_ERR_NOMEM, "bm_mlayer_create: device memory");
 	}
@@ -760,6 +842,27 @@ extern "C" int bm_mlayer_create(int nlayers, int nx, int ny, double hx, double h
 	return ml_create_meshes(nlayers, ms.data(), out);
 }
 
+// Periodic images along x and y (SDemag::Set_PBC, Boris/SDemag.cpp:403-408): N = n along a
+// periodic axis (nx even with plans for nx / 2 and nx, ny with a plan), the self kernels are
+// the periodic 2D kernels and the pair kernels the periodic z-shifted ones; layers of one grid
+// only (no images along z).
+extern "C" int bm_mlayer_create_pbc(int nlayers, int nx, int ny, double hx, double hy, const double* hz, const double* z0,
+	int pbc_x, int pbc_y, int flags, bm_mlayer_t* out)
+{
+	if (pbc_x < 0 || pbc_y < 0) return fail(BM_ERR_ARG, "bm_mlayer_create_pbc: negative image count");
+	if (!pbc_x && !pbc_y) return bm_mlayer_create(nlayers, nx, ny, hx, hy, hz, z0, flags, out);
+	if (!out || !hz || !z0 || nlayers < 1 || nx < 1 || ny < 1 || !(hx > 0) || !(hy > 0)) return fail(BM_ERR_ARG, "bm_mlayer_create_pbc: bad arguments");
+	*out = nullptr;
+	int rc = ml_check_layers(nlayers, hz);
+	if (rc) return rc;
+	if (!bm::plan_2d_pbc_ok(nx, ny, pbc_x > 0, pbc_y > 0))
+		return fail(BM_ERR_SIZE, "bm_mlayer_create_pbc: no FFT plan (a periodic nx: even, with plans for nx / 2 and nx; a periodic ny: a plan "
+			"for ny; open axes: plans for nx and 2 ny)");
+	return ml_create_grid(nlayers, nx, ny, hx, hy, hz, z0, out, pbc_x, pbc_y);
+}
+
+void bm::mlayer_pbc(const bm_mlayer_s* m, int out[2]) { out[0] = m->pbc[0]; out[1] = m->pbc[1]; }
+
 extern "C" int bm_mlayer_create_meshes(int nlayers, const bm_layer_mesh* meshes, int flags, bm_mlayer_t* out)
 {
 	(void)flags;
@@ -792,16 +895,17 @@ extern "C" int bm_mlayer_destroy(bm_mlayer_t m)
 static int ml_convolute(bm_mlayer_s* m, const float* M, float* out, int mode, hipStream_t st, int tr = 0)
 {
 	const int L = m->L, nlines = 3 * L * m->NC;
-	int rc = bm::pass_fwd_x(m->nx, M, m->Sx, m->ny, L, m->ntl, m->ncu, st);
-	if (rc == BM_OK) rc = bm::pass_y(m->Ny, -1, m->S, m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st);
+	const bool fx = m->pbc[0] > 0, fy = m->pbc[1] > 0;
+	int rc = bm::pass_fwd_x(m->xs, M, m->Sx, m->ny, L, m->ntl, m->ncu, st, fx);
+	if (rc == BM_OK) rc = bm::pass_y(m->Ny, -1, m->S, m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st, fy);
 	if (rc == BM_OK) {
 		bm::prof_mark_begin(bm::PK_ML_MULT, st);
 		rc = launch_ml_mult(L, m->S, m->S2, m->K, m->kidx, m->NC, m->Ny, m->Sp, m->ncu, st, tr);
 		bm::prof_mark_end(bm::PK_ML_MULT, st);
 	}
-	if (rc == BM_OK) rc = bm::pass_y(m->Ny, 1, m->Sout(), m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st);
+	if (rc == BM_OK) rc = bm::pass_y(m->Ny, 1, m->Sout(), m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st, fy);
 	const float scale = (float)(1.0 / ((double)m->Nx * m->Ny));
-	if (rc == BM_OK) rc = bm::pass_inv_x(m->nx, mode, m->Sx, out, m->ny, L, m->ntl, scale, m->ncu, st);
+	if (rc == BM_OK) rc = bm::pass_inv_x(m->xs, mode, m->Sx, out, m->ny, L, m->ntl, scale, m->ncu, st, fx);
 	return rc;
 }
 
@@ -896,10 +1000,11 @@ extern "C" int bm_mlayer_forward(bm_mlayer_t m, int layer, const float* M_layer,
 		hipLaunchKernelGGL(k_ml_transfer_in, dim3(ml_grid((long long)plane)), dim3(256), 0, st, m->tin, M_layer, m->stage + 3 * plane * (size_t)layer,
 			(long long)plane * layer, (long long)plane, m->moff[(size_t)layer]);
 	}
-	int rc = bm::pass_fwd_x_plane(m->nx, src, m->Sx, m->ny, layer, m->L, m->ntl, m->ncu, st);
+	int rc = bm::pass_fwd_x_plane(m->xs, src, m->Sx, m->ny, layer, m->L, m->ntl, m->ncu, st, m->pbc[0] > 0);
 	for (int c = 0; c < 3 && rc == BM_OK; c++) {
 		const size_t cz = (size_t)c * m->L + layer;
-		rc = bm::pass_y(m->Ny, -1, m->S + cz * m->NC * m->Sp, m->Sx + cz * m->ntl * m->NC * bm::kLT, m->NC, m->Sp, m->NC, m->ntl, m->ncu, st);
+		rc = bm::pass_y(m->Ny, -1, m->S + cz * m->NC * m->Sp, m->Sx + cz * m->ntl * m->NC * bm::kLT, m->NC, m->Sp, m->NC, m->ntl, m->ncu, st,
+			m->pbc[1] > 0);
 	}
 	if (rc) return rc;
 	BM_HIP(hipGetLastError());
@@ -925,7 +1030,8 @@ extern "C" int bm_mlayer_inverse(bm_mlayer_t m, int layer, const float* M_layer,
 	int rc = BM_OK;
 	for (int c = 0; c < 3 && rc == BM_OK; c++) {
 		const size_t cz = (size_t)c * m->L + layer;
-		rc = bm::pass_y(m->Ny, 1, m->Sout() + cz * m->NC * m->Sp, m->Sx + cz * m->ntl * m->NC * bm::kLT, m->NC, m->Sp, m->NC, m->ntl, m->ncu, st);
+		rc = bm::pass_y(m->Ny, 1, m->Sout() + cz * m->NC * m->Sp, m->Sx + cz * m->ntl * m->NC * bm::kLT, m->NC, m->Sp, m->NC, m->ntl, m->ncu, st,
+			m->pbc[1] > 0);
 	}
 	const float scale = (float)(1.0 / ((double)m->Nx * m->Ny));
 	if (rc == BM_OK && m->mapped()) {
@@ -933,7 +1039,7 @@ extern "C" int bm_mlayer_inverse(bm_mlayer_t m, int layer, const float* M_layer,
 		// from the layer's own M and the transferred field)
 		const size_t plane = (size_t)m->nx * m->ny;
 		float* Hp = m->stage + 3 * plane * (size_t)m->L;
-		rc = bm::pass_inv_x_plane(m->nx, 1, m->Sx, Hp, m->ny, layer, m->L, m->ntl, scale, m->ncu, st, nullptr, nullptr, nullptr);
+		rc = bm::pass_inv_x_plane(m->xs, 1, m->Sx, Hp, m->ny, layer, m->L, m->ntl, scale, m->ncu, st, nullptr, nullptr, nullptr, m->pbc[0] > 0);
 		const long long r0 = m->moff[(size_t)layer], nr = m->moff[(size_t)layer + 1] - r0;
 		const int g = std::min(ml_grid(nr), 1024);
 		if (rc == BM_OK) {
@@ -943,7 +1049,8 @@ extern "C" int bm_mlayer_inverse(bm_mlayer_t m, int layer, const float* M_layer,
 		}
 	}
 	else if (rc == BM_OK)
-		rc = bm::pass_inv_x_plane(m->nx, clear_out ? 1 : 2, m->Sx, H_layer, m->ny, layer, m->L, m->ntl, scale, m->ncu, st, M_layer, m->ebuf, energy);
+		rc = bm::pass_inv_x_plane(m->xs, clear_out ? 1 : 2, m->Sx, H_layer, m->ny, layer, m->L, m->ntl, scale, m->ncu, st, M_layer, m->ebuf, energy,
+			m->pbc[0] > 0);
 	if (rc) return rc;
 	BM_HIP(hipGetLastError());
 	return BM_OK;

@@ -715,6 +715,11 @@ extern "C" int bm_ode_set_layers(bm_ode_t o, bm_mlayer_t ml, const double* Ms, c
 	bm::mlayer_dims(ml, &L, &nx, &ny, nullptr);
 	if (bm::mlayer_mapped(ml)) return fail(BM_ERR_SIZE, "bm_ode_set_layers: the layers must share one planned grid");
 	if (L != o->nz || nx != o->nx || ny != o->ny) return fail(BM_ERR_ARG, "bm_ode_set_layers: the stack's layers must be the mesh's nz planes (same nx, ny)");
+	int mp[2];
+	bm::mlayer_pbc(ml, mp);
+	if ((o->pbc[0] || o->pbc[1] || o->pbc[2]) && (o->pbc[2] || mp[0] != o->pbc[0] || mp[1] != o->pbc[1]))
+		return fail(BM_ERR_STATE, "bm_ode_set_layers: the mesh's periodic images differ from the multilayer handle's");
+	if ((mp[0] || mp[1]) && o->speedup) return fail(BM_ERR_STATE, "bm_ode_set_layers: no evaluation speedup on a periodic stack");
 	if (L > bm::kMaxStackLayers) return fail(BM_ERR_SIZE, "bm_ode_set_layers: at most 64 layers");
 	const size_t b3 = o->n * 3 * sizeof(float);
 	if (!o->Hd && hipMalloc(&o->Hd, b3) != hipSuccess) return fail(BM_ERR_NOMEM, "bm_ode_set_layers: demag field buffer");
@@ -808,8 +813,15 @@ extern "C" int bm_ode_set_params(bm_ode_t o, const bm_fm_params* p)
 extern "C" int bm_ode_set_pbc(bm_ode_t o, int pbc_x, int pbc_y, int pbc_z)
 {
 	if (!o || pbc_x < 0 || pbc_y < 0 || pbc_z < 0) return fail(BM_ERR_ARG, "bm_ode_set_pbc: bad arguments");
-	if (o->ml) return fail(BM_ERR_STATE, "bm_ode_set_pbc: layer stacks have no periodic variant");
-	if ((o->p.modules & BM_MOD_DEMAG) && o->demag
+	if (o->ml) {
+		// a layer stack: the multilayer handle carries the x, y images (bm_mlayer_create_pbc)
+		int mp[2];
+		bm::mlayer_pbc(o->ml, mp);
+		if (pbc_z || mp[0] != pbc_x || mp[1] != pbc_y)
+			return fail(BM_ERR_STATE, "bm_ode_set_pbc: a layer stack takes the x, y images of its multilayer handle (bm_mlayer_create_pbc), none along z");
+		if (o->speedup && (pbc_x || pbc_y)) return fail(BM_ERR_STATE, "bm_ode_set_pbc: no evaluation speedup on a periodic stack");
+	}
+	else if ((o->p.modules & BM_MOD_DEMAG) && o->demag
 		&& (o->demag->pbc[0] != pbc_x || o->demag->pbc[1] != pbc_y || o->demag->pbc[2] != pbc_z))
 		return fail(BM_ERR_STATE, "bm_ode_set_pbc: the demag handle must have the same periodic images (bm_demag_create_pbc)");
 	o->pbc[0] = pbc_x; o->pbc[1] = pbc_y; o->pbc[2] = pbc_z;
@@ -824,6 +836,7 @@ extern "C" int bm_ode_set_pbc(bm_ode_t o, int pbc_x, int pbc_y, int pbc_z)
 extern "C" int bm_ode_set_eval_speedup(bm_ode_t o, int mode, double dTspeedup)
 {
 	if (!o || mode < 0 || mode > 6) return fail(BM_ERR_ARG, "bm_ode_set_eval_speedup: mode must be 0..6");
+	if (mode && o->ml && (o->pbc[0] || o->pbc[1])) return fail(BM_ERR_STATE, "bm_ode_set_eval_speedup: no evaluation speedup on a periodic stack");
 	const size_t b3 = o->n * 3 * sizeof(float);
 	for (int q = 0; q < 6; q++) {
 		if (q >= mode && o->Hs[q]) { (void)hipFree(o->Hs[q]); o->Hs[q] = nullptr; }

@@ -57,8 +57,8 @@ int bm_demag_create(int nx, int ny, int nz, double hx, double hy, double hz, int
  * multiple of 4), N.y in those or 4096, a periodic nz (3D) a power of two 4..256; anything else
  * returns BM_ERR_SIZE.  All three counts zero is bm_demag_create.  The convolution calls take
  * the handle unchanged; the integrator (bm_ode_set_pbc), the adjoint (bm_adj_set_pbc) and
- * batches (bm_batch_set_pbc) take periodic meshes, the multilayer handle has no periodic
- * variant.
+ * batches (bm_batch_set_pbc) take periodic meshes; periodic layer stacks are made by
+ * bm_mlayer_create_pbc.
  */
 int bm_demag_create_pbc(int nx, int ny, int nz, double hx, double hy, double hz, int pbc_x, int pbc_y, int pbc_z,
 	int flags, bm_demag_t* out);
@@ -117,6 +117,15 @@ int bm_demag_copy_kernels(bm_demag_t d, float* host_out, long long count);
 typedef struct bm_mlayer_s* bm_mlayer_t;
 int bm_mlayer_create(int nlayers, int nx, int ny, double hx, double hy, const double* hz, const double* z0,
 	int flags, bm_mlayer_t* out);
+/* The same stack with pbc_x, pbc_y periodic images along x and y (SDemag::Set_PBC,
+ * Boris/SDemag.cpp:403-408; kernels DemagKernelCollectionCUDA_Calc.cpp:182/373 with pbc_images,
+ * tensors Boris/DemagTFunc_Shifted_PBC.cpp:317-600 and DemagTFunc_Irregular_PBC.cpp:5-235):
+ * N = n along a periodic axis (a periodic nx even with plans for nx / 2 and nx, a periodic ny
+ * with a plan), no images along z.  Both counts zero is bm_mlayer_create.  The integrator
+ * (bm_ode_set_layers, then bm_ode_set_pbc with the same x, y images) and the adjoint
+ * (bm_adj_set_layers, bm_adj_set_pbc) take such a handle; batches do not. */
+int bm_mlayer_create_pbc(int nlayers, int nx, int ny, double hx, double hy, const double* hz, const double* z0,
+	int pbc_x, int pbc_y, int flags, bm_mlayer_t* out);
 /*
  * Layers that are separate meshes of their own lateral size, cell and position: layer l has
  * nx x ny cells of hx x hy x hz, its bottom-left corner at (x0, y0, z0) (m).  Replaces the

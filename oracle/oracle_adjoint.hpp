@@ -64,7 +64,8 @@ struct AdjRK4 {
 		const double* A, const double* alpha, const double* K1)
 	{
 		layered = true;
-		if (with_demag) ml.reset(new MLayer<double>(L, nx, ny, hx, hy, hz, z0));
+		// x, y images of set_pbc (called first) make the stack periodic in-plane
+		if (with_demag) ml.reset(new MLayer<double>(L, nx, ny, hx, hy, hz, z0, pbc[0], pbc[1]));
 		if (A) layA.assign(A, A + L);
 		if (alpha) layAlpha.assign(alpha, alpha + L);
 		if (K1) layK1.assign(K1, K1 + L);
