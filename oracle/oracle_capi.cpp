@@ -59,6 +59,33 @@ int or_tensor2d_zshift(int nx, int ny, int Nx, int Ny, const double* s3, const d
 }
 
 // Real kernels of a z-shifted 2D tensor: Kd, Ko (Ny/2+1)*(Nx/2+1)*3 each.
+// 2D tensor shifted in-plane (sh3[0] or sh3[1] nonzero) and along z: Ddiag and Dodiag Nx*Ny*3 each
+int or_tensor2d_shifted(int nx, int ny, int Nx, int Ny, const double* s3, const double* d3, const double* sh3, double* Ddiag, double* Dodiag)
+{
+	try {
+		std::vector<double> Dd, Do;
+		tensor2d_shifted(nx, ny, Nx, Ny, s3, d3, sh3, Dd, Do);
+		std::memcpy(Ddiag, Dd.data(), Dd.size() * sizeof(double));
+		std::memcpy(Dodiag, Do.data(), Do.size() * sizeof(double));
+		return 0;
+	}
+	catch (...) { return 1; }
+}
+
+// complex kernels of such a tensor: Nc * Ny * 6 complex (re, im interleaved), kx fastest
+int or_kernels2d_complex(int Nx, int Ny, const double* Ddiag, const double* Dodiag, double* K)
+{
+	try {
+		const size_t NN = (size_t)Nx * Ny * 3;
+		std::vector<double> Dd(Ddiag, Ddiag + NN), Do(Dodiag, Dodiag + NN);
+		std::vector<std::complex<double>> k;
+		kernels2d_complex(Nx, Ny, Dd, Do, k);
+		std::memcpy(K, k.data(), k.size() * sizeof(std::complex<double>));
+		return 0;
+	}
+	catch (...) { return 1; }
+}
+
 // the same with px, py periodic images along x and y (N = n along a periodic axis)
 int or_tensor2d_zshift_pbc(int Nx, int Ny, const double* s3, const double* d3, double shz, int px, int py, double* Ddiag, double* Dodiag)
 {
@@ -95,6 +122,12 @@ void* or_mlayer_create(int L, int nx, int ny, double hx, double hy, const double
 void* or_mlayer_create_pbc(int L, int nx, int ny, double hx, double hy, const double* hz, const double* z0, int px, int py)
 {
 	try { return new MLayer<double>(L, nx, ny, hx, hy, hz, z0, px, py); }
+	catch (...) { return nullptr; }
+}
+// layers whose grids are offset in-plane by rx[l], ry[l] (m)
+void* or_mlayer_create_offset(int L, int nx, int ny, double hx, double hy, const double* hz, const double* z0, const double* rx, const double* ry)
+{
+	try { return new MLayer<double>(L, nx, ny, hx, hy, hz, z0, 0, 0, rx, ry); }
 	catch (...) { return nullptr; }
 }
 void or_mlayer_destroy(void* h) { delete (MLayer<double>*)h; }

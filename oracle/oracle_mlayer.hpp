@@ -245,6 +245,133 @@ inline void tensor2d_zshift(int nx, int ny, int Nx, int Ny, const double s[3], c
 	}
 }
 
+// 2D tensor between source cells s and destination cells d (normalised, equal in x and y)
+// for a destination layer shifted by (shx, shy, shz) with shx or shy nonzero: the general
+// branch of Boris/DemagTFunc_Shifted.cpp:235-343 (regular, grids of
+// Boris/DemagTFunc_fg.cpp:303-350) and Boris/DemagTFunc_Irregular.cpp:5-235 (irregular, grids
+// :411-506), no symmetry used: every offset i in (-nx, nx), j in (-ny, ny) at position
+// i hx + shx, j hy + shy, stored at (i mod Nx, j mod Ny), -N.  The near field uses the same
+// 27- and 36-term stencils as the z-shifted tensor on grids over the offsets the near field
+// reads (the reference fills them over all of [-n, n]), the far field the series
+// at the shifted position (scaled by s.z when irregular).
+inline void tensor2d_shifted(int nx, int ny, int Nx, int Ny, const double s[3], const double d[3], const double sh[3],
+	std::vector<double>& Dd, std::vector<double>& Do)
+{
+	const int A = kAsymptoticDistance;
+	const size_t NN = (size_t)Nx * Ny;
+	Dd.assign(NN * 3, 0.0); Do.assign(NN * 3, 0.0);
+	const bool regular = same_cell(s, d);
+	const double hx = regular ? s[0] : d[0], hy = regular ? s[1] : d[1], hz = regular ? s[2] : d[2];
+	const double shx = sh[0], shy = sh[1], shz = sh[2];
+	// grid index q of offset i: q = i - lo - 1 over [lo, hi], the offsets the near field can
+	// read (|i + sh/h| below the far-field distance, +-1 for the stencil) within [-n, n]
+	auto lo_of = [&](int n, double f) { return std::max(-n, (int)std::floor(-A - f) - 2); };
+	auto hi_of = [&](int n, double f) { return std::min(n, (int)std::ceil(A - f) + 2); };
+	const int lx = lo_of(nx, shx / hx), hxi = hi_of(nx, shx / hx), ly = lo_of(ny, shy / hy), hyi = hi_of(ny, shy / hy);
+	const int dx = hxi - lx - 1, dy = hyi - ly - 1;
+	auto ax = [&](double h, double o, int lo) { return [h, o, lo](int q) { return (q + lo + 1) * h + o; }; };
+	auto lin = [](double h, double o) { return [h, o](int k) { return k * h + o; }; };
+	const auto X = ax(hx, shx, lx), Y = ax(hy, shy, ly);
+	Grid3 fxx, fyy, fzz, gxy, gxz, gyz, fxx_d, fyy_d, fzz_d, gxy_d, gxz_d, gyz_d;
+	if (regular) {
+		fxx = fill_grid_at(nw_f, dx, dy, 1, X, Y, lin(hz, shz));
+		fyy = fill_grid_at(nw_f, dy, dx, 1, Y, X, lin(hz, shz));
+		fzz = fill_grid_at(nw_f, 1, dy, dx, lin(hz, shz), Y, X);
+		gxy = fill_grid_at(nw_g, dx, dy, 1, X, Y, lin(hz, shz));
+		gxz = fill_grid_at(nw_g, dx, 1, dy, X, lin(hz, shz), Y);
+		gyz = fill_grid_at(nw_g, dy, 1, dx, Y, lin(hz, shz), X);
+	}
+	else {
+		const double del = s[2] - d[2];
+		auto p_sd = [&](int k) { return (k < 0 ? -s[2] : k > 0 ? d[2] : 0.0) + shz; };
+		auto p_ds = [&](int k) { return (k < 0 ? -d[2] : k > 0 ? s[2] : 0.0) + shz; };
+		auto c_m = [&](int) { return -del + shz; };
+		auto c_p = [&](int) { return del + shz; };
+		fxx = fill_grid_at(nw_f, dx, dy, 1, X, Y, p_sd);
+		fyy = fill_grid_at(nw_f, dy, dx, 1, Y, X, p_sd);
+		fzz = fill_grid_at(nw_f, 1, dy, dx, p_ds, Y, X);
+		gxy = fill_grid_at(nw_g, dx, dy, 1, X, Y, p_ds);
+		gxz = fill_grid_at(nw_g, dx, 1, dy, X, p_ds, Y);
+		gyz = fill_grid_at(nw_g, dy, 1, dx, Y, p_ds, X);
+		fxx_d = fill_grid_at(nw_f, dx, dy, -1, X, Y, c_m);
+		fyy_d = fill_grid_at(nw_f, dy, dx, -1, Y, X, c_m);
+		fzz_d = fill_grid_at(nw_f, -1, dy, dx, c_p, Y, X);
+		gxy_d = fill_grid_at(nw_g, dx, dy, -1, X, Y, c_p);
+		gxz_d = fill_grid_at(nw_g, dx, -1, dy, X, c_p, Y);
+		gyz_d = fill_grid_at(nw_g, dy, -1, dx, Y, c_p, X);
+	}
+	AsymptDiag adx(hx, hy, hz), ady(hy, hx, hz), adz(hz, hy, hx);
+	AsymptOffDiag aoxy(hx, hy, hz), aoxz(hx, hz, hy), aoyz(hy, hz, hx);
+	const double tau = hx * hy * hz;
+	const double far_scale = regular ? 1.0 : s[2];
+	const double ks = std::fabs(shz / hz);
+
+#pragma omp parallel for schedule(dynamic, 1)
+	for (int j = -ny + 1; j < ny; j++) {
+		for (int i = -nx + 1; i < nx; i++) {
+			const double is = std::fabs(i + shx / hx), js = std::fabs(j + shy / hy);
+			const double x = i * hx + shx, y = j * hy + shy;
+			double vd[3], vo[3];
+			if (A > 0 && (floor_eps(is) >= A || floor_eps(js) >= A || floor_eps(ks) >= A || floor_eps(is * is + js * js + ks * ks) >= A * A)) {
+				vd[0] = (-adx.eval(x, y, shz)) * far_scale;
+				vd[1] = (-ady.eval(y, x, shz)) * far_scale;
+				vd[2] = (-adz.eval(shz, y, x)) * far_scale;
+				vo[0] = (-aoxy.eval(x, y, shz)) * far_scale;
+				vo[1] = (-aoxz.eval(x, shz, y)) * far_scale;
+				vo[2] = (-aoyz.eval(y, shz, x)) * far_scale;
+			}
+			else {
+				const int qi = i - lx - 1, qj = j - ly - 1;
+				if (regular) {
+					vd[0] = L27(fxx, qi, qj, 0, hx, hy, hz) * -1;
+					vd[1] = L27(fyy, qj, qi, 0, hy, hx, hz) * -1;
+					vd[2] = L27(fzz, 0, qj, qi, hz, hy, hx) * -1;
+					vo[0] = L27(gxy, qi, qj, 0, hx, hy, hz) * -1;
+					vo[1] = L27(gxz, qi, 0, qj, hx, hz, hy) * -1;
+					vo[2] = L27(gyz, qj, 0, qi, hy, hz, hx) * -1;
+				}
+				else {
+					vd[0] = L36_plane3(fxx, fxx_d, qi, qj, tau) * -1;
+					vd[1] = L36_plane3(fyy, fyy_d, qj, qi, tau) * -1;
+					vd[2] = L36_zz(fzz, fzz_d, qj, qi, tau) * -1;
+					vo[0] = L36_plane3(gxy, gxy_d, qi, qj, tau) * -1;
+					vo[1] = L36_plane2(gxz, gxz_d, qi, qj, tau) * -1;
+					vo[2] = L36_plane2(gyz, gyz_d, qj, qi, tau) * -1;
+				}
+			}
+			const size_t idx = wrap(i, Nx) + wrap(j, Ny) * Nx;
+			for (int c = 0; c < 3; c++) { Dd[idx * 3 + c] = vd[c]; Do[idx * 3 + c] = vo[c]; }
+		}
+	}
+}
+
+// Complex kernels of a general shifted 2D tensor (Boris/DemagKernelCollectionCUDA_Calc.cpp:
+// 571-745, the Complex_Full kernels): x real-input FFT keeping kx in [0, Nx/2], then y complex
+// FFT, all six components complex; output Nc x Ny x 6 complex (kx fastest), components xx,
+// yy, zz, xy, xz, yz.
+inline void kernels2d_complex(int Nx, int Ny, const std::vector<double>& Dd, const std::vector<double>& Do,
+	std::vector<std::complex<double>>& K)
+{
+	typedef std::complex<double> C;
+	const int Nc = Nx / 2 + 1;
+	K.assign((size_t)Nc * Ny * 6, C(0, 0));
+	Fft<double> fx(Nx), fy(Ny);
+	std::vector<C> a(Nx), sc(Nx), b(Ny), sb(Ny);
+	for (int c = 0; c < 6; c++) {
+		const std::vector<double>& D = c < 3 ? Dd : Do;
+		for (int j = 0; j < Ny; j++) {
+			for (int i = 0; i < Nx; i++) a[i] = C(D[((size_t)i + (size_t)j * Nx) * 3 + c % 3], 0.0);
+			fx.run(a.data(), sc.data(), -1);
+			for (int i = 0; i < Nc; i++) K[((size_t)i + (size_t)j * Nc) * 6 + c] = a[i];
+		}
+		for (int i = 0; i < Nc; i++) {
+			for (int j = 0; j < Ny; j++) b[j] = K[((size_t)i + (size_t)j * Nc) * 6 + c];
+			fy.run(b.data(), sb.data(), -1);
+			for (int j = 0; j < Ny; j++) K[((size_t)i + (size_t)j * Nc) * 6 + c] = b[j];
+		}
+	}
+}
+
 // The same tensor with periodic images along x (px) and y (py), no images along z: the
 // z-shift branch of Boris/DemagTFunc_Shifted_PBC.cpp:317-460 / 464-600 (regular) and
 // Boris/DemagTFunc_Irregular_PBC.cpp:5-120 (irregular).  Along a periodic axis N = n and
@@ -401,15 +528,21 @@ template <typename R>
 struct MLayer {
 	typedef std::complex<R> C;
 	int L, nx, ny, Nx, Ny, Nc, Nyh;
-	// per (d, s) pair: self (d == s) -> Kd + Kxy (2D self kernel); else Kd + Ko (z-shifted)
+	// per (d, s) pair: self (d == s) -> Kd + Kxy (2D self kernel); else Kd + Ko (z-shifted),
+	// or Kc (complex, Nc x Ny x 6) for a pair whose layers are offset in-plane
 	std::vector<std::vector<R>> Kd, Ko;
+	std::vector<std::vector<C>> Kc;
 	std::vector<int> self;
 	Fft<R> fx, fy;
 
 	// px, py: periodic images along x and y (SDemag::Set_PBC, Boris/SDemag.cpp:403-408): N = n
 	// along a periodic axis, the self kernels are the periodic 2D kernels and the pair
 	// kernels the periodic z-shifted ones (Boris/DemagKernelCollection_Calc.cpp with pbc_images)
-	MLayer(int L_, int nx_, int ny_, double hx, double hy, const double* hz, const double* z0, int px = 0, int py = 0)
+	// rx, ry (optional, m): in-plane offsets of the layers' grids, so a pair (d, s) with
+	// rx[d] != rx[s] or ry[d] != ry[s] couples through the tensor shifted by the difference
+	// (the reference's Complex_Full kernels, Boris/DemagKernelCollectionCUDA_Calc.cpp:571-745)
+	MLayer(int L_, int nx_, int ny_, double hx, double hy, const double* hz, const double* z0, int px = 0, int py = 0,
+		const double* rx = nullptr, const double* ry = nullptr)
 		: L(L_), nx(nx_), ny(ny_)
 	{
 		Nx = px ? nx : 2 * nx; Ny = py ? ny : 2 * ny; Nc = Nx / 2 + 1; Nyh = Ny / 2 + 1;
@@ -418,7 +551,7 @@ struct MLayer {
 		fx = Fft<R>(Nx); fy = Fft<R>(Ny);
 		double hmax = 0;
 		for (int l = 0; l < L; l++) hmax = std::max(hmax, std::max(hx, std::max(hy, hz[l])));
-		Kd.resize((size_t)L * L); Ko.resize((size_t)L * L); self.assign((size_t)L * L, 0);
+		Kd.resize((size_t)L * L); Ko.resize((size_t)L * L); Kc.resize((size_t)L * L); self.assign((size_t)L * L, 0);
 		for (int d = 0; d < L; d++)
 			for (int s = 0; s < L; s++) {
 				const size_t p = (size_t)d * L + s;
@@ -431,6 +564,16 @@ struct MLayer {
 				else {
 					const double sc[3] = { hx / hmax, hy / hmax, hz[s] / hmax }, dc[3] = { hx / hmax, hy / hmax, hz[d] / hmax };
 					std::vector<double> Dd, Do, kd, ko;
+					const double sx = rx ? (rx[d] - rx[s]) / hmax : 0.0, sy = ry ? (ry[d] - ry[s]) / hmax : 0.0;
+					if (std::fabs(sx) > 1e-12 || std::fabs(sy) > 1e-12) {
+						if (periodic) throw std::runtime_error("in-plane offset layers with periodic images");
+						const double sh[3] = { sx, sy, (z0[d] - z0[s]) / hmax };
+						tensor2d_shifted(nx, ny, Nx, Ny, sc, dc, sh, Dd, Do);
+						std::vector<std::complex<double>> kc;
+						kernels2d_complex(Nx, Ny, Dd, Do, kc);
+						Kc[p].assign(kc.begin(), kc.end());
+						continue;
+					}
 					if (periodic) tensor2d_zshift_pbc(Nx, Ny, sc, dc, (z0[d] - z0[s]) / hmax, px, py, Dd, Do);
 					else tensor2d_zshift(nx, ny, Nx, Ny, sc, dc, (z0[d] - z0[s]) / hmax, Dd, Do);
 					kernels2d_zshift(Nx, Ny, Dd, Do, kd, ko);
@@ -478,6 +621,15 @@ struct MLayer {
 					for (int i = 0; i < Nc; i++) {
 						const size_t k = (size_t)i + (size_t)jj * Nc, o = (size_t)i + (size_t)j * Nc;
 						const C fxv = F[fi(s, 0, i, j)], fyv = F[fi(s, 1, i, j)], fzv = F[fi(s, 2, i, j)];
+						if (!Kc[p].empty()) {
+							// complex kernel (transpose: its conjugate)
+							const size_t kk = ((size_t)i + (size_t)j * Nc) * 6;
+							auto K = [&](int c) { return transpose ? std::conj(Kc[p][kk + c]) : Kc[p][kk + c]; };
+							O[o] += K(0) * fxv + K(3) * fyv + K(4) * fzv;
+							O[fl + o] += K(3) * fxv + K(1) * fyv + K(5) * fzv;
+							O[2 * fl + o] += K(4) * fxv + K(5) * fyv + K(2) * fzv;
+							continue;
+						}
 						const R kxx = Kd[p][k * 3], kyy = Kd[p][k * 3 + 1], kzz = Kd[p][k * 3 + 2];
 						if (self[p]) {
 							const R kxy = sy * Ko[p][k];

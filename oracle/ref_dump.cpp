@@ -21,6 +21,9 @@
 //   ref_dump selfpbc hx hy hz nx ny nz px py pz out.bin -> SelfDemag_PBC (3 doubles)
 //   ref_dump pbc2d Nx Ny hx hy hz px py pz out.bin
 //        -> Ddiag (Nx*Ny DBL3) then Dxy (Nx*Ny doubles) of the periodic 2D tensor
+//   ref_dump shift2d nx ny Nx Ny sx sy sz dx dy dz shx shy shz out.bin
+//        -> Ddiag then Dodiag (Nx*Ny DBL3 each) of the 2D tensor shifted in-plane and along z
+//           (the general branch of the same entry point)
 //   ref_dump zshift2dpbc Nx Ny sx sy sz dx dy dz shz px py out.bin
 //        -> Ddiag then Dodiag (Nx*Ny DBL3 each) of the 2D z-shifted tensor with px, py
 //           images in x and y (N = n along a periodic axis, 2n otherwise)
@@ -119,6 +122,22 @@ int main(int argc, char** argv)
 		for (size_t i = 0; i < Dd.linear_size(); i++) { out.push_back(Dd[i].x); out.push_back(Dd[i].y); out.push_back(Dd[i].z); }
 		for (size_t i = 0; i < Do.linear_size(); i++) { out.push_back(Do[i].x); out.push_back(Do[i].y); out.push_back(Do[i].z); }
 		write_doubles(argv[13], out);
+		return 0;
+	}
+	if (mode == "shift2d" && argc == 16) {
+		INT3 n(atoi(argv[2]), atoi(argv[3]), 1);
+		INT3 N(atoi(argv[4]), atoi(argv[5]), 1);
+		DBL3 s(atof(argv[6]), atof(argv[7]), atof(argv[8]));
+		DBL3 d(atof(argv[9]), atof(argv[10]), atof(argv[11]));
+		DBL3 shift(atof(argv[12]), atof(argv[13]), atof(argv[14]));
+		VEC<DBL3> Dd, Do;
+		if (!Dd.resize(SZ3(N.x, N.y, 1)) || !Do.resize(SZ3(N.x, N.y, 1))) return 3;
+		if (!dtf.CalcDiagTens2D_Shifted_Irregular(Dd, n, N, s, d, shift)) return 4;
+		if (!dtf.CalcOffDiagTens2D_Shifted_Irregular(Do, n, N, s, d, shift)) return 4;
+		std::vector<double> out;
+		for (size_t i = 0; i < Dd.linear_size(); i++) { out.push_back(Dd[i].x); out.push_back(Dd[i].y); out.push_back(Dd[i].z); }
+		for (size_t i = 0; i < Do.linear_size(); i++) { out.push_back(Do[i].x); out.push_back(Do[i].y); out.push_back(Do[i].z); }
+		write_doubles(argv[15], out);
 		return 0;
 	}
 	if (mode == "zshift2dpbc" && argc == 14) {

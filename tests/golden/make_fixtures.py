@@ -19,6 +19,11 @@ reference checkout and oracle/_ref exist; the GPU test runs only read the result
   DemagTFunc_Irregular.cpp: regular (equal cells) and irregular (thicknesses differ), with
   near-field and far-field (>= 40 cells) entries; all entries for small meshes, a seeded
   sample for large ones.
+* shift_golden.npz — FP64 2D tensors shifted in-plane as well as along z (the general branch
+  of the reference's Boris/DemagTFunc_Shifted.cpp and DemagTFunc_Irregular.cpp, used by its
+  multilayered demag for meshes whose rectangles are offset in-plane), written by
+  oracle/_ref/ref_dump: regular and irregular, fractional offsets, near and far field.
+  (`python tests/golden/make_fixtures.py shift` writes this file alone.)
 * zshift_pbc_golden.npz — FP64 2D z-shifted tensors with periodic images along x and y
   (multilayered demag with PBC) written by oracle/_ref/ref_dump from the reference's
   Boris/DemagTFunc_Shifted_PBC.cpp and DemagTFunc_Irregular_PBC.cpp: regular and irregular,
@@ -116,6 +121,7 @@ def main() -> None:
     np.savez_compressed(HERE / "tensor_golden.npz", **out)
     zshift_fixtures()
     zshift_pbc_fixtures()
+    shift_fixtures()
     pbc_fixtures()
     print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
 
@@ -225,8 +231,42 @@ def zshift_pbc_fixtures() -> None:
     np.savez_compressed(HERE / "zshift_pbc_golden.npz", **out)
 
 
+# (name, n, N, source cell, destination cell, shift), normalised
+SHIFT_CASES = [
+    ("s_reg_x", (8, 6), (16, 12), (1.0, 1.0, 0.5), (1.0, 1.0, 0.5), (0.3, 0.0, 0.75)),
+    ("s_irr_xy", (8, 6), (16, 12), (1.0, 0.8, 1.0), (1.0, 0.8, 0.5), (0.4, -0.25, 0.75)),
+    ("s_reg_far", (50, 45), (100, 90), (0.25, 0.25, 0.5), (0.25, 0.25, 0.5), (0.1, 0.05, -0.5)),
+    ("s_irr_far", (50, 45), (100, 90), (0.25, 0.25, 1.0), (0.25, 0.25, 0.5), (-0.125, 0.2, 0.75)),
+    ("s_reg_plane", (12, 10), (24, 20), (0.5, 0.5, 0.5), (0.5, 0.5, 0.5), (0.2, 0.3, 0.0)),
+    ("s_irr_big", (20, 16), (40, 32), (0.5, 0.5, 0.25), (0.5, 0.5, 1.0), (3.6, -1.3, -1.5)),
+]
+
+
+def shift_fixtures() -> None:
+    rng = np.random.default_rng(20261206)
+    out = {}
+    for name, nn, NN, s, d, sh in SHIFT_CASES:
+        v = dump("shift2d", *nn, *NN, *map(float, s), *map(float, d), *map(float, sh))
+        m = NN[0] * NN[1]
+        Dd = v[:3 * m].reshape(NN[1], NN[0], 3)
+        Do = v[3 * m:].reshape(NN[1], NN[0], 3)
+        if m <= 800:
+            idx = np.array([(j, i) for j in range(NN[1]) for i in range(NN[0])])
+        else:
+            idx = np.stack([rng.integers(0, NN[1], 1500), rng.integers(0, NN[0], 1500)], axis=1)
+        out[name + "_n"], out[name + "_N"] = np.array(nn), np.array(NN)
+        out[name + "_s"], out[name + "_d"], out[name + "_shift"] = np.array(s), np.array(d), np.array(sh)
+        out[name + "_idx"] = idx
+        out[name + "_diag"] = Dd[idx[:, 0], idx[:, 1]]
+        out[name + "_off"] = Do[idx[:, 0], idx[:, 1]]
+    out["names"] = np.array([c[0] for c in SHIFT_CASES])
+    np.savez_compressed(HERE / "shift_golden.npz", **out)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["zshift_pbc"]:
+    if sys.argv[1:] == ["shift"]:
+        shift_fixtures()
+    elif sys.argv[1:] == ["zshift_pbc"]:
         zshift_pbc_fixtures()
     else:
         main()

@@ -68,8 +68,11 @@ def lib() -> C.CDLL:
     L.or_tensor2d_zshift.argtypes = [i] * 4 + [dp, dp, d, dp, dp]
     L.or_kernels2d_zshift.argtypes = [i, i, dp, dp, dp, dp]
     L.or_tensor2d_zshift_pbc.argtypes = [i, i, dp, dp, d, i, i, dp, dp]
+    L.or_tensor2d_shifted.argtypes = [i] * 4 + [dp, dp, dp, dp, dp]
+    L.or_kernels2d_complex.argtypes = [i, i, dp, dp, dp]
     L.or_mlayer_create.argtypes = [i, i, i, d, d, dp, dp]; L.or_mlayer_create.restype = vp
     L.or_mlayer_create_pbc.argtypes = [i, i, i, d, d, dp, dp, i, i]; L.or_mlayer_create_pbc.restype = vp
+    L.or_mlayer_create_offset.argtypes = [i, i, i, d, d, dp, dp, dp, dp]; L.or_mlayer_create_offset.restype = vp
     L.or_mlayer_destroy.argtypes = [vp]
     L.or_mlayer_convolute.argtypes = [vp, dp, dp]
     L.or_mlayer_convolute_transposed.argtypes = [vp, dp, dp]
@@ -141,6 +144,24 @@ def tensor2d_zshift(n, N, s, d, shift):
     return a.reshape(N[1], N[0], 3), b.reshape(N[1], N[0], 3)
 
 
+def tensor2d_shifted(n, N, s, d, shift):
+    """2D tensor shifted in-plane and along z (shift = (x, y, z), normalised): (Dd, Do), each
+    (Ny, Nx, 3)."""
+    L = lib()
+    a, b = np.zeros(N[0] * N[1] * 3), np.zeros(N[0] * N[1] * 3)
+    assert L.or_tensor2d_shifted(n[0], n[1], N[0], N[1], np.array(s, float), np.array(d, float), np.array(shift, float), a, b) == 0
+    return a.reshape(N[1], N[0], 3), b.reshape(N[1], N[0], 3)
+
+
+def kernels2d_complex(N, Dd, Do):
+    """Complex kernels of a shifted tensor: (Ny, Nx/2+1, 6) complex (xx, yy, zz, xy, xz, yz)."""
+    L = lib()
+    nc = N[0] // 2 + 1
+    k = np.zeros(nc * N[1] * 6 * 2)
+    assert L.or_kernels2d_complex(N[0], N[1], np.ascontiguousarray(Dd, float).reshape(-1), np.ascontiguousarray(Do, float).reshape(-1), k) == 0
+    return k.view(np.complex128).reshape(N[1], nc, 6)
+
+
 def tensor2d_zshift_pbc(N, s, d, shift, px, py):
     """The 2D z-shifted tensor with px, py periodic images (N = n along a periodic axis):
     (Dd, Do), each (Ny, Nx, 3)."""
@@ -165,10 +186,16 @@ class OracleMLayer:
     bottom faces at z0[l]; M and H are (L*nx*ny, 3), layer-major.  pbc: (px, py) periodic
     images along x and y."""
 
-    def __init__(self, nx, ny, hx, hy, hz, z0, pbc=(0, 0)):
+    def __init__(self, nx, ny, hx, hy, hz, z0, pbc=(0, 0), offsets=None):
+        """offsets: per layer (rx, ry) in-plane offsets of its grid (m); pairs offset against each
+        other couple through the in-plane shifted tensor's complex kernels."""
         self.L = lib()
         self.nl = len(hz)
-        if any(pbc):
+        if offsets is not None:
+            o = np.asarray(offsets, float).reshape(self.nl, 2)
+            self.h = self.L.or_mlayer_create_offset(self.nl, nx, ny, hx, hy, np.array(hz, float), np.array(z0, float),
+                                                    np.ascontiguousarray(o[:, 0]), np.ascontiguousarray(o[:, 1]))
+        elif any(pbc):
             self.h = self.L.or_mlayer_create_pbc(self.nl, nx, ny, hx, hy, np.array(hz, float), np.array(z0, float), int(pbc[0]), int(pbc[1]))
         else:
             self.h = self.L.or_mlayer_create(self.nl, nx, ny, hx, hy, np.array(hz, float), np.array(z0, float))
@@ -269,13 +296,22 @@ class OracleMLayerMeshes:
             self.Rc.append((np.array(cs), np.array(ce)))
         hc = (size[0] / nc[0], size[1] / nc[1])
         bx, by = min(c[0][0] for c in self.Rc), min(c[0][1] for c in self.Rc)
-        bex, bey = max(c[1][0] for c in self.Rc), max(c[1][1] for c in self.Rc)
-        self.nb = (int(round((bex - bx) / hc[0])), int(round((bey - by) / hc[1])))
-        self.off = [(int(round((c[0][0] - bx) / hc[0])), int(round((c[0][1] - by) / hc[1]))) for c in self.Rc]
+
+        def split(v):
+            # whole common cells (within 1e-6 of one) and the remaining fraction of a cell
+            r = np.floor(v + 1e-6)
+            return int(r), (0.0 if abs(v - r) <= 1e-6 else v - r)
+        parts = [(split((c[0][0] - bx) / hc[0]), split((c[0][1] - by) / hc[1])) for c in self.Rc]
+        self.off = [(px[0], py[0]) for px, py in parts]
+        res = [(px[1] * hc[0], py[1] * hc[1]) for px, py in parts]
+        self.nb = (max(o[0] for o in self.off) + nc[0], max(o[1] for o in self.off) + nc[1])
         self.nc, self.hc = nc, hc
         self.counts = [m[0] * m[1] for m in meshes]
         self.moff = np.concatenate([[0], np.cumsum(self.counts)])
-        self.conv = OracleMLayer(self.nb[0], self.nb[1], hc[0], hc[1], [m[4] for m in meshes], [m[7] for m in meshes])
+        # rectangles offset by fractions of a common cell couple through shifted kernels
+        fractional = any(r != (0.0, 0.0) for r in res)
+        self.conv = OracleMLayer(self.nb[0], self.nb[1], hc[0], hc[1], [m[4] for m in meshes], [m[7] for m in meshes],
+                                 offsets=res if fractional else None)
         # transfer matrices (dense, small cases): Tin[l] (box plane x layer cells), Tout[l]
         self.Tin, self.Tout = [], []
         for l, m in enumerate(meshes):

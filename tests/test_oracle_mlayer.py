@@ -132,7 +132,7 @@ def _even_odd_parts(Dd, Do, pbc):
     return Dd, Do
 
 
-def direct_sum_mlayer(nx, ny, hx, hy, hz, z0, M, pbc=(0, 0)):
+def direct_sum_mlayer(nx, ny, hx, hy, hz, z0, M, pbc=(0, 0), offsets=None):
     """Real-space sum over the pinned tensors: cyclic over N (N = n along a periodic axis,
     where the tensors carry the images; 2n otherwise, where the padding keeps it linear)."""
     L = len(hz)
@@ -154,7 +154,10 @@ def direct_sum_mlayer(nx, ny, hx, hy, hz, z0, M, pbc=(0, 0)):
             else:
                 sc = (hx / hmax, hy / hmax, hz[s] / hmax)
                 dc = (hx / hmax, hy / hmax, hz[d] / hmax)
-                if periodic:
+                off = (0.0, 0.0) if offsets is None else ((offsets[d][0] - offsets[s][0]) / hmax, (offsets[d][1] - offsets[s][1]) / hmax)
+                if off != (0.0, 0.0):
+                    Dd, Do = O.tensor2d_shifted((nx, ny), N, sc, dc, (off[0], off[1], (z0[d] - z0[s]) / hmax))
+                elif periodic:
                     Dd, Do = O.tensor2d_zshift_pbc(N, sc, dc, (z0[d] - z0[s]) / hmax, *pbc)
                 else:
                     Dd, Do = O.tensor2d_zshift((nx, ny), N, sc, dc, (z0[d] - z0[s]) / hmax)
@@ -331,3 +334,119 @@ def test_periodic_stack_of_touching_layers_equals_3d_periodic_convolution(pbc):
     ref = O.OracleDemag((nx, ny, L), (h, h, h), pbc=(pbc[0], pbc[1], 0))(M)
     # measured 1.4e-12 and 3.0e-12: the two stencils' roundoff summed over the images
     assert np.linalg.norm(H - ref) / np.linalg.norm(ref) <= 1e-11
+
+
+GOLD_SHIFT = Path(__file__).resolve().parent / "golden" / "shift_golden.npz"
+
+
+def test_inplane_shifted_tensor_bit_identical_to_reference_golden():
+    """Tensors shifted in-plane (by fractions of a cell) and along z against the reference's
+    general branch of DemagTFunc_Shifted.cpp / DemagTFunc_Irregular.cpp (oracle/_ref)."""
+    g = np.load(GOLD_SHIFT)
+    for name in g["names"]:
+        n, N = tuple(g[name + "_n"]), tuple(g[name + "_N"])
+        Dd, Do = O.tensor2d_shifted(n, N, g[name + "_s"], g[name + "_d"], g[name + "_shift"])
+        idx = g[name + "_idx"]
+        assert np.array_equal(Dd[idx[:, 0], idx[:, 1]], g[name + "_diag"]), name
+        assert np.array_equal(Do[idx[:, 0], idx[:, 1]], g[name + "_off"]), name
+
+
+@pytest.mark.parametrize("cells", [((0.5, 0.5, 1.0), (0.5, 0.5, 1.0)), ((0.5, 0.5, 1.0), (0.5, 0.5, 0.5))], ids=["reg", "irr"])
+def test_whole_cell_shift_is_a_translated_zshift_tensor(cells):
+    """An in-plane shift of whole cells is the z-shifted tensor read at translated offsets (the
+    embedding of offset layers in one box), to the rounding of the shifted positions."""
+    s, d = cells
+    n, N = (12, 10), (24, 20)
+    Dz, Oz = O.tensor2d_zshift(n, N, s, d, 0.75)
+    di, dj = 3, -2
+    Ds, Os = O.tensor2d_shifted((9, 8), N, s, d, (di * d[0], dj * d[1], 0.75))
+    J, I = np.meshgrid(np.arange(-7, 8), np.arange(-8, 9), indexing="ij")
+    for X, Z in ((Ds, Dz), (Os, Oz)):
+        A = X[J % N[1], I % N[0]]
+        B = Z[(J + dj) % N[1], (I + di) % N[0]]
+        # the 27/36-term stencils cancel: 1e-11 of the tensor's scale where positions round apart
+        assert np.abs(A - B).max() <= 1e-9 * np.abs(Z).max()
+
+
+def test_box_residual_shift_equals_full_offset_tensor():
+    """Layers offset by Delta + r cells: the box embedding evaluates the tensor with the residual
+    shift r at box offsets J, the reference with the full shift at mesh offsets J - Delta; the
+    same physical positions, so equal to the rounding of the positions (near and far field)."""
+    s = d = (0.5, 0.5, 0.5)
+    N = (100, 90)
+    r, D = (0.15, -0.2), (7, 4)
+    Tb, Ob = O.tensor2d_shifted((50, 45), N, s, d, (r[0] * 0.5, r[1] * 0.5, 1.0))
+    Tf, Of = O.tensor2d_shifted((43, 41), N, s, d, ((D[0] + r[0]) * 0.5, (D[1] + r[1]) * 0.5, 1.0))
+    J, I = np.meshgrid(np.arange(-40, 41), np.arange(-42, 43), indexing="ij")
+    for X, Z in ((Tb, Tf), (Ob, Of)):
+        A = X[(J + D[1]) % N[1], (I + D[0]) % N[0]]
+        B = Z[J % N[1], I % N[0]]
+        # measured 1.2e-9 (diagonal) and 3.0e-8 (off-diagonal) of the scale: the near-field
+        # stencils' cancellation 25-40 cells out, evaluated at positions that round apart
+        assert np.abs(A - B).max() <= 1e-7 * np.abs(Z).max()
+
+
+def test_complex_kernels_are_the_2d_fft_of_the_tensor():
+    N = (16, 12)
+    Dd, Do = O.tensor2d_shifted((8, 6), N, (1.0, 0.8, 1.0), (1.0, 0.8, 0.5), (0.4, -0.25, 0.75))
+    K = O.kernels2d_complex(N, Dd, Do)
+    D = np.concatenate([Dd, Do], axis=-1)
+    F = np.fft.fft2(D, axes=(0, 1))[:, :N[0] // 2 + 1]
+    assert np.abs(K - F).max() <= 1e-13 * np.abs(F).max()
+
+
+
+def test_offset_layers_convolution_equals_direct_sum():
+    """Layers whose grids are offset in-plane by fractions of a cell: pairs couple through the
+    in-plane shifted tensor's complex kernels (no symmetry); the FFT convolution equals the
+    real-space sum over the pinned shifted tensors, and its transpose is the adjoint."""
+    nx, ny, hx, hy = 8, 6, 4e-9, 4e-9
+    hz, z0 = [8e-9, 4e-9, 16e-9], [0.0, 9e-9, 15e-9]
+    offsets = [(0.0, 0.0), (1.3e-9, -0.7e-9), (0.0, 2.1e-9)]
+    rng = np.random.default_rng(11)
+    M = rng.normal(size=(3 * nx * ny, 3)) * 8e5
+    ref = direct_sum_mlayer(nx, ny, hx, hy, hz, z0, M, offsets=offsets)
+    ml = O.OracleMLayer(nx, ny, hx, hy, hz, z0, offsets=offsets)
+    H = ml(M)
+    assert np.linalg.norm(H - ref) / np.linalg.norm(ref) <= 1e-13
+    H0 = O.OracleMLayer(nx, ny, hx, hy, hz, z0)(M)
+    assert np.linalg.norm(H - H0) / np.linalg.norm(H0) > 1e-4
+    v = rng.normal(size=M.shape)
+    assert abs(np.vdot(v, H) - np.vdot(ml.transposed(v), M)) <= 1e-12 * abs(np.vdot(v, H))
+
+
+def test_fractional_mesh_offsets_equal_reference_formulation():
+    """Meshes of equal cells offset in-plane by fractions of a cell (the transfer is then the
+    identity): the box embedding with residual shifts equals the reference's formulation, every
+    pair convolved on its own n_common grid through the tensor shifted by the full rectangle
+    offset (Boris/DemagKernelCollectionCUDA_Calc.cpp:70-104, Complex_Full kernels)."""
+    h = 4e-9
+    meshes = [(8, 6, h, h, 8e-9, 0.0, 0.0, 0.0), (8, 6, h, h, 4e-9, 1.3e-9, -2.2e-9, 10e-9),
+              (8, 6, h, h, 4e-9, 5.1e-9, 0.6e-9, 16e-9)]
+    rng = np.random.default_rng(12)
+    M = rng.normal(size=(3 * 48, 3)) * 8e5
+    H = O.OracleMLayerMeshes(meshes)(M)
+    hmax = 8e-9
+    nx, ny = 8, 6
+    N = (2 * nx, 2 * ny)
+    Mr = M.reshape(3, ny, nx, 3)
+    ref = np.zeros_like(Mr)
+    for d, md in enumerate(meshes):
+        for s, ms in enumerate(meshes):
+            sc, dc = (h / hmax, h / hmax, ms[4] / hmax), (h / hmax, h / hmax, md[4] / hmax)
+            if d == s:
+                Dd, Dxy = O.tensor2d((nx, ny), N, (h, h, md[4]))
+                Do = np.zeros(Dd.shape); Do[..., 0] = Dxy
+            else:
+                sh = ((md[5] - ms[5]) / hmax, (md[6] - ms[6]) / hmax, (md[7] - ms[7]) / hmax)
+                Dd, Do = O.tensor2d_shifted((nx, ny), N, sc, dc, sh)
+            mx, my, mz = Mr[s, ..., 0], Mr[s, ..., 1], Mr[s, ..., 2]
+            for j in range(ny):
+                for i in range(nx):
+                    dd = Dd[np.ix_((j - np.arange(ny)) % N[1], (i - np.arange(nx)) % N[0])]
+                    oo = Do[np.ix_((j - np.arange(ny)) % N[1], (i - np.arange(nx)) % N[0])]
+                    ref[d, j, i, 0] += (dd[..., 0] * mx + oo[..., 0] * my + oo[..., 1] * mz).sum()
+                    ref[d, j, i, 1] += (oo[..., 0] * mx + dd[..., 1] * my + oo[..., 2] * mz).sum()
+                    ref[d, j, i, 2] += (oo[..., 1] * mx + oo[..., 2] * my + dd[..., 2] * mz).sum()
+    ref = ref.reshape(-1, 3)
+    assert np.linalg.norm(H - ref) / np.linalg.norm(ref) <= 1e-10
