@@ -96,10 +96,16 @@ __global__ void k_ml_energy(const double* part, int np, long long n, float* ener
 
 
 // Position of grid index i (-1, 0, +1, ...) along one axis: i*h + o, or a table for -1, 0, +1.
-struct Axis { double h, o, p[3]; int table; };
-__host__ __device__ inline double axis_pos(const Axis& a, int i) { return a.table ? a.p[i + 1] : i * a.h + a.o; }
-Axis lin(double h, double o) { Axis a; a.h = h; a.o = o; a.p[0] = a.p[1] = a.p[2] = 0; a.table = 0; return a; }
-Axis tab(double m, double z, double p) { Axis a; a.h = a.o = 0; a.p[0] = m; a.p[1] = z; a.p[2] = p; a.table = 1; return a; }
+// grid axis: linear i h + o, a table of three plane positions (i = -1, 0, 1), or centred
+// (cen): grid index i holds offset i + lo + 1 at (i + lo + 1) h + o
+struct Axis { double h, o, p[3]; int table, lo; };
+__host__ __device__ inline double axis_pos(const Axis& a, int i)
+{
+	return a.table == 1 ? a.p[i + 1] : a.table == 2 ? (i + a.lo + 1) * a.h + a.o : i * a.h + a.o;
+}
+Axis lin(double h, double o) { Axis a; a.h = h; a.o = o; a.p[0] = a.p[1] = a.p[2] = 0; a.table = 0; a.lo = 0; return a; }
+Axis tab(double m, double z, double p) { Axis a; a.h = a.o = 0; a.p[0] = m; a.p[1] = z; a.p[2] = p; a.table = 1; a.lo = 0; return a; }
+Axis cen(double h, double o, int lo) { Axis a = lin(h, o); a.table = 2; a.lo = lo; return a; }
 
 // Basis-function grid over i in [-1, dx], j in [-1, dy], k in [-1, dz] (stored with +1, x
 // fastest); dz = -1 gives the single plane k = -1.
@@ -182,6 +188,8 @@ __device__ double L36_plane2(const G3& G, const G3& Dl, int i, int k, double tau
 struct ZArgs {
 	int nx, ny, Nx, Ny, regular;
 	int px, py;               // periodic images along x and y (N = n along a periodic axis)
+	double sx, sy;            // in-plane shift (k_tensor_shift), normalised
+	int lox, loy;             // k_tensor_shift: first offset held by the centred grids, minus one
 	double hx, hy, hz, shz, ks, tau, far;
 	G3 g[6], del[6];          // f_xx, f_yy, f_zz, g_xy, g_xz, g_yz (+ their extra planes)
 	AsymDiag ad[3];
@@ -259,6 +267,64 @@ __global__ void k_tensor_zshift_pbc(ZArgs a, double* D)
 			if (!a.py && i0 && j0) put(mx + my * a.Nx, 1, -1, -1);
 		}
 		if (!a.py && j0) put((size_t)i0 + my * a.Nx, -1, 1, -1);
+	}
+}
+
+// Tensor shifted in-plane (sx or sy nonzero) and along z: every offset i in (-nx, nx), j in
+// (-ny, ny) at position (i hx + sx, j hy + sy, shz), no symmetry, stored at (i mod Nx, j mod Ny)
+// (the general branch of Boris/DemagTFunc_Shifted.cpp:235-343 and
+// Boris/DemagTFunc_Irregular.cpp:5-235): near field by the same stencils on grids centred on
+// the offsets the near field reads, far field by the series at the shifted position.
+__global__ void k_tensor_shift(ZArgs a, double* D)
+{
+	const size_t NN = (size_t)a.Nx * a.Ny;
+	const int A = kAsymDist, W = 2 * a.nx - 1, Hh = 2 * a.ny - 1;
+	const double hx = a.hx, hy = a.hy, hz = a.hz, sz = a.shz;
+	for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < (size_t)W * Hh; q += (size_t)gridDim.x * blockDim.x) {
+		const int i = (int)(q % W) - (a.nx - 1), j = (int)(q / W) - (a.ny - 1);
+		const double is = fabs(i + a.sx / hx), js = fabs(j + a.sy / hy), ks = a.ks;
+		const double x = i * hx + a.sx, y = j * hy + a.sy;
+		double vd[3], vo[3];
+		if ((int)floor(is + 1e-4) >= A || (int)floor(js + 1e-4) >= A || (int)floor(ks + 1e-4) >= A || (int)floor(is * is + js * js + ks * ks + 1e-4) >= A * A) {
+			vd[0] = (-asym_diag(a.ad[0], x, y, sz)) * a.far;
+			vd[1] = (-asym_diag(a.ad[1], y, x, sz)) * a.far;
+			vd[2] = (-asym_diag(a.ad[2], sz, y, x)) * a.far;
+			vo[0] = (-asym_off(a.ao[0], x, y, sz)) * a.far;
+			vo[1] = (-asym_off(a.ao[1], x, sz, y)) * a.far;
+			vo[2] = (-asym_off(a.ao[2], y, sz, x)) * a.far;
+		}
+		else {
+			const int qi = i - a.lox - 1, qj = j - a.loy - 1;
+			if (a.regular) {
+				vd[0] = -stencil27(a.g[0].v, a.g[0].gx, a.g[0].gy, qi, qj, 0, hx, hy, hz);
+				vd[1] = -stencil27(a.g[1].v, a.g[1].gx, a.g[1].gy, qj, qi, 0, hy, hx, hz);
+				vd[2] = -stencil27(a.g[2].v, a.g[2].gx, a.g[2].gy, 0, qj, qi, hz, hy, hx);
+				vo[0] = -stencil27(a.g[3].v, a.g[3].gx, a.g[3].gy, qi, qj, 0, hx, hy, hz);
+				vo[1] = -stencil27(a.g[4].v, a.g[4].gx, a.g[4].gy, qi, 0, qj, hx, hz, hy);
+				vo[2] = -stencil27(a.g[5].v, a.g[5].gx, a.g[5].gy, qj, 0, qi, hy, hz, hx);
+			}
+			else {
+				vd[0] = -L36_plane3(a.g[0], a.del[0], qi, qj, a.tau);
+				vd[1] = -L36_plane3(a.g[1], a.del[1], qj, qi, a.tau);
+				vd[2] = -L36_zz(a.g[2], a.del[2], qj, qi, a.tau);
+				vo[0] = -L36_plane3(a.g[3], a.del[3], qi, qj, a.tau);
+				vo[1] = -L36_plane2(a.g[4], a.del[4], qi, qj, a.tau);
+				vo[2] = -L36_plane2(a.g[5], a.del[5], qj, qi, a.tau);
+			}
+		}
+		const size_t idx = (size_t)((i + a.Nx) % a.Nx) + (size_t)((j + a.Ny) % a.Ny) * a.Nx;
+		D[idx] = vd[0]; D[NN + idx] = vd[1]; D[2 * NN + idx] = vd[2];
+		D[3 * NN + idx] = vo[0]; D[4 * NN + idx] = vo[1]; D[5 * NN + idx] = vo[2];
+	}
+}
+
+// complex kernel planes from the y FFTs of the x-transformed tensor's real (A) and imaginary
+// (B) parts: Re = Re A - Im B, Im = Im A + Re B
+__global__ void k_cplx_combine(const double* Are, const double* Aim, const double* Bre, const double* Bim, float* re, float* im, size_t n)
+{
+	for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+		re[q] = (float)(Are[q] - Bim[q]);
+		im[q] = (float)(Aim[q] + Bre[q]);
 	}
 }
 
@@ -407,10 +473,67 @@ __global__ void __launch_bounds__(256) k_ml_mult_gen(const float2* __restrict__ 
 }
 
 // The kernel product; the result is in S (L <= kRegLayers, in place) or S2.
-int launch_ml_mult(int L, float2* S, float2* S2, const float* K, const int* kidx, int NC, int Ny, int Sp, int ncu, hipStream_t st, int tr = 0)
+// Layers offset in-plane against each other (their pairs have complex kernels): the same sums
+// (own layer first, then the others in index order), a pair with kcoff[p] >= 0 through its
+// complex kernel (12 planes Re, Im of xx, yy, zz, xy, xz, yz, NC x Ny each, ky fastest, at
+// Kc + kcoff[p]; Boris/DemagKernelCollectionCUDA_Mult.cu's Complex_Full product), the others
+// through their real kernels (K, per pair or by slot kidx) as in k_ml_mult; out of place into S2.
+__global__ void __launch_bounds__(256) k_ml_mult_any(const float2* __restrict__ S, float2* __restrict__ S2, const float* __restrict__ K,
+	const int* __restrict__ kidx, const float* __restrict__ Kc, const long long* __restrict__ kcoff, int L, int NC, int Ny, int Sp)
+{
+	const int NYH = Ny / 2 + 1;
+	const size_t kq = (size_t)NC * NYH, kc = (size_t)NC * Ny;
+	const int npts = NC * Ny;
+	auto cm = [](float2 k, float2 f) { return make_float2(k.x * f.x - k.y * f.y, k.x * f.y + k.y * f.x); };
+	for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < npts; q += gridDim.x * blockDim.x) {
+		const int ky = q % Ny, kx = q / Ny;
+		const bool up = ky > Ny / 2;
+		const int jj = up ? Ny - ky : ky;
+		const float sy = up ? -1.f : 1.f;
+		for (int d = 0; d < L; d++) {
+			float2 o0 = make_float2(0.f, 0.f), o1 = o0, o2 = o0;
+			for (int r = 0; r < L; r++) {
+				const int s = r == 0 ? d : (r <= d ? r - 1 : r);
+				const int p = d * L + s;
+				const float2 fx = S[((size_t)(0 * L + s) * NC + kx) * Sp + ky], fy = S[((size_t)(1 * L + s) * NC + kx) * Sp + ky];
+				const float2 fz = S[((size_t)(2 * L + s) * NC + kx) * Sp + ky];
+				if (kcoff[p] >= 0) {
+					const float* Kp = Kc + kcoff[p] + (size_t)kx * Ny + ky;
+					auto k = [&](int c) { return make_float2(Kp[(2 * c) * kc], Kp[(2 * c + 1) * kc]); };
+					const float2 kxx = k(0), kyy = k(1), kzz = k(2), kxy = k(3), kxz = k(4), kyz = k(5);
+					const float2 a0 = cm(kxx, fx), a1 = cm(kxy, fy), a2 = cm(kxz, fz);
+					const float2 b0 = cm(kxy, fx), b1 = cm(kyy, fy), b2 = cm(kyz, fz);
+					const float2 c0 = cm(kxz, fx), c1 = cm(kyz, fy), c2 = cm(kzz, fz);
+					o0.x += a0.x + a1.x + a2.x; o0.y += a0.y + a1.y + a2.y;
+					o1.x += b0.x + b1.x + b2.x; o1.y += b0.y + b1.y + b2.y;
+					o2.x += c0.x + c1.x + c2.x; o2.y += c0.y + c1.y + c2.y;
+					continue;
+				}
+				const float* Kp = K + (size_t)(kidx ? kidx[p] : p) * 6 * kq + (size_t)kx * NYH + jj;
+				const float kxx = Kp[0], kyy = Kp[kq], kzz = Kp[2 * kq];
+				const float kxy = sy * Kp[3 * kq], kxz = Kp[4 * kq], kyz = sy * Kp[5 * kq];
+				o0.x += kxx * fx.x + kxy * fy.x - kxz * fz.y;  o0.y += kxx * fx.y + kxy * fy.y + kxz * fz.x;
+				o1.x += kxy * fx.x + kyy * fy.x - kyz * fz.y;  o1.y += kxy * fx.y + kyy * fy.y + kyz * fz.x;
+				o2.x += -kxz * fx.y - kyz * fy.y + kzz * fz.x; o2.y += kxz * fx.x + kyz * fy.x + kzz * fz.y;
+			}
+			S2[((size_t)(0 * L + d) * NC + kx) * Sp + ky] = o0;
+			S2[((size_t)(1 * L + d) * NC + kx) * Sp + ky] = o1;
+			S2[((size_t)(2 * L + d) * NC + kx) * Sp + ky] = o2;
+		}
+	}
+}
+
+int launch_ml_mult(int L, float2* S, float2* S2, const float* K, const int* kidx, int NC, int Ny, int Sp, int ncu, hipStream_t st, int tr = 0,
+	const float* Kc = nullptr, const long long* kcoff = nullptr)
 {
 	const int npts = NC * Ny;
 	dim3 g((unsigned)std::max(1, std::min((npts + 255) / 256, ncu * 8))), b(256);
+	if (Kc) {
+		if (!S2 || !kcoff) return fail(BM_ERR_STATE, "multilayer demag: no second spectra array");
+		if (tr) return fail(BM_ERR_STATE, "multilayer demag: no transposed product for layers offset in-plane");
+		hipLaunchKernelGGL(k_ml_mult_any, g, b, 0, st, S, S2, K, kidx, Kc, kcoff, L, NC, Ny, Sp);
+		return BM_OK;
+	}
 	if (L > kRegLayers) {
 		if (!S2) return fail(BM_ERR_STATE, "multilayer demag: no second spectra array");
 		hipLaunchKernelGGL(k_ml_mult_gen, g, b, 0, st, S, S2, K, kidx, L, NC, Ny, Sp, tr);
@@ -431,6 +554,13 @@ struct bm_mlayer_s {
 	int xs = 0;                  // x-pass size N.x / 2 (nx, or nx / 2 along a periodic x)
 	int pbc[2] = { 0, 0 };       // periodic images along x and y (bm_mlayer_create_pbc)
 	bool periodic() const { return pbc[0] || pbc[1]; }
+	// layers whose grids are offset in-plane by fractions of a cell (rx, ry per layer, m; empty
+	// otherwise): their pairs couple through complex kernels, 12 planes of NC x Ny floats each
+	// at Kc + hkcoff[d * L + s] (-1: the real kernel in K)
+	std::vector<double> rx, ry;
+	std::vector<long long> hkcoff;
+	float* Kc = nullptr;
+	long long* kcoff = nullptr;
 	double hx, hy;
 	std::vector<double> hz, z0;
 	float2* Sx = nullptr;    // x-side spectra, row-tiled (bm_xpass.h): 3 * L * ntl * NC * kLT complex
@@ -535,6 +665,88 @@ int zshift_kernels(bm_mlayer_s* m, int d, int s, double hmax, double* D, double*
 	return BM_OK;
 }
 
+// Complex kernels of pair (d, s) whose layers are offset in-plane (rx, ry differ): the tensor
+// shifted by the offset difference (k_tensor_shift), x real-input FFT keeping both parts, then
+// the complex y FFT as the real-input FFTs of those parts, into 12 planes of NC x Ny floats at
+// Kp (Boris/DemagKernelCollectionCUDA_Calc.cpp:571-745).  T holds 6 NC x Ny doubles.
+int shift_kernels(bm_mlayer_s* m, int d, int s, double hmax, double* D, double* T, double* grids[12], float* Kp, hipStream_t st)
+{
+	const int nx = m->nx, ny = m->ny, Nx = m->Nx, Ny = m->Ny, NC = m->NC;
+	const double sc[3] = { m->hx / hmax, m->hy / hmax, m->hz[s] / hmax }, dc[3] = { m->hx / hmax, m->hy / hmax, m->hz[d] / hmax };
+	const double shx = (m->rx[d] - m->rx[s]) / hmax, shy = (m->ry[d] - m->ry[s]) / hmax, shz = (m->z0[d] - m->z0[s]) / hmax;
+	bool regular = true;
+	for (int c = 0; c < 3; c++) if (nzero(sc[c] * 1e-9 - dc[c] * 1e-9)) regular = false;
+	const double hx = regular ? sc[0] : dc[0], hy = regular ? sc[1] : dc[1], hz = regular ? sc[2] : dc[2];
+	// grids over the offsets the near field reads: |i + sh / h| below the far-field distance, +-1
+	auto lo_of = [](int n, double f) { return std::max(-n, (int)std::floor(-kAsymDist - f) - 2); };
+	auto hi_of = [](int n, double f) { return std::min(n, (int)std::ceil(kAsymDist - f) + 2); };
+	const int lox = lo_of(nx, shx / hx), loy = lo_of(ny, shy / hy);
+	const int dx = hi_of(nx, shx / hx) - lox - 1, dy = hi_of(ny, shy / hy) - loy - 1;
+	const Axis X = cen(hx, shx, lox), Y = cen(hy, shy, loy);
+	struct Spec { int which, gx, gy, gz; Axis X, Y, Z; };
+	Spec sp[12];
+	if (regular) {
+		sp[0] = { 0, dx, dy, 1, X, Y, lin(hz, shz) };
+		sp[1] = { 0, dy, dx, 1, Y, X, lin(hz, shz) };
+		sp[2] = { 0, 1, dy, dx, lin(hz, shz), Y, X };
+		sp[3] = { 1, dx, dy, 1, X, Y, lin(hz, shz) };
+		sp[4] = { 1, dx, 1, dy, X, lin(hz, shz), Y };
+		sp[5] = { 1, dy, 1, dx, Y, lin(hz, shz), X };
+	}
+	else {
+		const double del = sc[2] - dc[2];
+		const Axis p_sd = tab(-sc[2] + shz, shz, dc[2] + shz), p_ds = tab(-dc[2] + shz, shz, sc[2] + shz);
+		const Axis c_m = tab(-del + shz, 0, 0), c_p = tab(del + shz, 0, 0);
+		sp[0] = { 0, dx, dy, 1, X, Y, p_sd };
+		sp[1] = { 0, dy, dx, 1, Y, X, p_sd };
+		sp[2] = { 0, 1, dy, dx, p_ds, Y, X };
+		sp[3] = { 1, dx, dy, 1, X, Y, p_ds };
+		sp[4] = { 1, dx, 1, dy, X, p_ds, Y };
+		sp[5] = { 1, dy, 1, dx, Y, p_ds, X };
+		sp[6] = { 0, dx, dy, -1, X, Y, c_m };
+		sp[7] = { 0, dy, dx, -1, Y, X, c_m };
+		sp[8] = { 0, -1, dy, dx, c_p, Y, X };
+		sp[9] = { 1, dx, dy, -1, X, Y, c_p };
+		sp[10] = { 1, dx, -1, dy, X, c_p, Y };
+		sp[11] = { 1, dy, -1, dx, Y, c_p, X };
+	}
+	ZArgs a;
+	std::memset(&a, 0, sizeof(a));
+	a.nx = nx; a.ny = ny; a.Nx = Nx; a.Ny = Ny; a.regular = regular ? 1 : 0;
+	a.hx = hx; a.hy = hy; a.hz = hz; a.shz = shz; a.ks = std::fabs(shz / hz); a.tau = hx * hy * hz;
+	a.far = regular ? 1.0 : sc[2];
+	a.sx = shx; a.sy = shy; a.lox = lox; a.loy = loy;
+	const int ng = regular ? 6 : 12;
+	for (int q = 0; q < ng; q++) {
+		const size_t cnt = (size_t)(sp[q].gx + 2) * (sp[q].gy + 2) * (sp[q].gz + 2);
+		hipLaunchKernelGGL(k_fill_grid_at, dim3((unsigned)std::min<size_t>((cnt + 255) / 256, 4096)), dim3(256), 0, st,
+			grids[q], sp[q].which, sp[q].gx, sp[q].gy, sp[q].gz, sp[q].X, sp[q].Y, sp[q].Z);
+		G3& G = q < 6 ? a.g[q] : a.del[q - 6];
+		G.v = grids[q]; G.gx = sp[q].gx + 2; G.gy = sp[q].gy + 2;
+	}
+	asym_diag_init(a.ad[0], hx, hy, hz); asym_diag_init(a.ad[1], hy, hx, hz); asym_diag_init(a.ad[2], hz, hy, hx);
+	asym_off_init(a.ao[0], hx, hy, hz); asym_off_init(a.ao[1], hx, hz, hy); asym_off_init(a.ao[2], hy, hz, hx);
+	const size_t NN = (size_t)Nx * Ny, nk = (size_t)NC * Ny;
+	BM_HIP(hipMemsetAsync(D, 0, NN * 6 * sizeof(double), st));
+	const size_t work = (size_t)(2 * nx - 1) * (2 * ny - 1);
+	hipLaunchKernelGGL(k_tensor_shift, dim3((unsigned)std::min<size_t>((work + 127) / 128, 65535)), dim3(128), 0, st, a, D);
+	double* Tre = T; double* Tim = T + nk; double* Are = T + 2 * nk; double* Aim = T + 3 * nk; double* Bre = T + 4 * nk; double* Bim = T + 5 * nk;
+	for (int q = 0; q < 6; q++) {
+		// x: both parts of the real-input FFT (kx in [0, Nx/2], layout kx + j NC)
+		int rc = setup_line_fft(Nx, D + q * NN, Tre, Ny, Ny, Nx, 0, 1, NC, 0, 1, NC, 0, 1.0, st);
+		if (rc == BM_OK) rc = setup_line_fft(Nx, D + q * NN, Tim, Ny, Ny, Nx, 0, 1, NC, 0, 1, NC, 1, 1.0, st);
+		// y: every ky (layout kx Ny + ky) of the two real parts
+		if (rc == BM_OK) rc = setup_line_fft(Ny, Tre, Are, NC, NC, 1, 0, NC, Ny, 0, 1, Ny, 0, 1.0, st);
+		if (rc == BM_OK) rc = setup_line_fft(Ny, Tre, Aim, NC, NC, 1, 0, NC, Ny, 0, 1, Ny, 1, 1.0, st);
+		if (rc == BM_OK) rc = setup_line_fft(Ny, Tim, Bre, NC, NC, 1, 0, NC, Ny, 0, 1, Ny, 0, 1.0, st);
+		if (rc == BM_OK) rc = setup_line_fft(Ny, Tim, Bim, NC, NC, 1, 0, NC, Ny, 0, 1, Ny, 1, 1.0, st);
+		if (rc) return rc;
+		hipLaunchKernelGGL(k_cplx_combine, dim3((unsigned)std::min<size_t>((nk + 255) / 256, 4096)), dim3(256), 0, st,
+			Are, Aim, Bre, Bim, Kp + (2 * q) * nk, Kp + (2 * q + 1) * nk, nk);
+	}
+	return BM_OK;
+}
+
 int compute_ml_kernels(bm_mlayer_s* m, hipStream_t st)
 {
 	const int L = m->L, Nx = m->Nx, Ny = m->Ny, NC = m->NC, NYH = m->NYH;
@@ -560,10 +772,13 @@ int compute_ml_kernels(bm_mlayer_s* m, hipStream_t st)
 	}
 	if (L == 1) return BM_OK;
 	const int dx = m->pbc[0] ? kAsymDist : std::min(m->nx, kAsymDist), dy = m->pbc[1] ? kAsymDist : std::min(m->ny, kAsymDist);
-	const size_t gmax = (size_t)(std::max(dx, dy) + 2) * (std::max(dx, dy) + 2) * 3;
+	size_t gmax = (size_t)(std::max(dx, dy) + 2) * (std::max(dx, dy) + 2) * 3;
+	// complex pairs: centred grids over at most 2 * kAsymDist + 5 offsets per axis, six NC x Ny temporaries
+	if (m->Kc) gmax = std::max(gmax, (size_t)(2 * kAsymDist + 7) * (2 * kAsymDist + 7) * 3);
+	const size_t tlen = (size_t)NC * Ny * (m->Kc ? 6 : 1);
 	double* D = nullptr; double* T1 = nullptr; double* grids[12] = { nullptr };
 	int rc = BM_OK;
-	bool ok = hipMalloc(&D, (size_t)Nx * Ny * 6 * sizeof(double)) == hipSuccess && hipMalloc(&T1, (size_t)NC * Ny * sizeof(double)) == hipSuccess;
+	bool ok = hipMalloc(&D, (size_t)Nx * Ny * 6 * sizeof(double)) == hipSuccess && hipMalloc(&T1, tlen * sizeof(double)) == hipSuccess;
 	for (int q = 0; q < 12 && ok; q++) ok = hipMalloc(&grids[q], gmax * sizeof(double)) == hipSuccess;
 	if (!ok) rc = fail(BM_ERR_NOMEM, "multilayer kernel setup");
 	for (int d = 0; d < L && rc == BM_OK; d++)
@@ -571,7 +786,9 @@ int compute_ml_kernels(bm_mlayer_s* m, hipStream_t st)
 			const int slot = m->hkidx[(size_t)d * L + s];
 			if (d == s || done[slot]) continue;
 			done[slot] = 1;
-			rc = zshift_kernels(m, d, s, hmax, D, T1, grids, m->K + m->kbase(d, s) * 6 * kq, st);
+			const long long co = m->hkcoff.empty() ? -1 : m->hkcoff[(size_t)d * L + s];
+			if (co >= 0) rc = shift_kernels(m, d, s, hmax, D, T1, grids, m->Kc + co, st);
+			else rc = zshift_kernels(m, d, s, hmax, D, T1, grids, m->K + m->kbase(d, s) * 6 * kq, st);
 		}
 	// per-pair storage: pairs that share a slot take a copy of its first pair's kernel
 	if (rc == BM_OK && m->per_pair()) {
@@ -599,7 +816,7 @@ using bm::fail;
 // Handle on one grid of nx x ny cells with planned lengths (nx and 2 ny; along a periodic
 // axis N = n: nx / 2 and ny).
 static int ml_create_grid(int nlayers, int nx, int ny, double hx, double hy, const double* hz, const double* z0, bm_mlayer_t* out,
-	int px = 0, int py = 0)
+	int px = 0, int py = 0, const double* rx = nullptr, const double* ry = nullptr)
 {
 	bm_mlayer_s* m = new bm_mlayer_s();
 	m->L = nlayers; m->nx = nx; m->ny = ny;
@@ -611,8 +828,20 @@ static int ml_create_grid(int nlayers, int nx, int ny, double hx, double hy, con
 	m->ntl = bm::sx_tiles(ny);
 	const size_t sbytes = (size_t)3 * nlayers * m->NC * m->Sp * sizeof(float2);
 	const size_t sxbytes = (size_t)3 * nlayers * m->ntl * m->NC * bm::kLT * sizeof(float2);
+	// layers offset in-plane against each other: those pairs take complex kernels
+	long long ncx = 0;
+	if (rx && ry) {
+		m->rx.assign(rx, rx + nlayers); m->ry.assign(ry, ry + nlayers);
+		m->hkcoff.assign((size_t)nlayers * nlayers, -1);
+		for (int d = 0; d < nlayers; d++)
+			for (int s = 0; s < nlayers; s++)
+				if (d != s && (std::fabs(rx[d] - rx[s]) > 1e-12 * hx || std::fabs(ry[d] - ry[s]) > 1e-12 * hy))
+					m->hkcoff[(size_t)d * nlayers + s] = (ncx++) * 12LL * m->NC * m->Ny;
+		if (!ncx) { m->rx.clear(); m->ry.clear(); m->hkcoff.clear(); }
+	}
 	// kernel slots: a pair's kernel depends on the two thicknesses and the z shift only (the
-	// lateral cells are common), so pairs that agree in them (to 1e-12 relative) share one
+	// lateral cells are common), so pairs that agree in them (to 1e-12 relative) share one;
+	// a pair with a complex kernel has a slot of its own
 	m->hkidx.assign((size_t)nlayers * nlayers, -1);
 	struct Key { int self; double hd, hs, sh; };
 	std::vector<Key> keys;
@@ -621,7 +850,8 @@ static int ml_create_grid(int nlayers, int nx, int ny, double hx, double hy, con
 		for (int s = 0; s < nlayers; s++) {
 			const Key k{ d == s, hz[d], hz[s], d == s ? 0.0 : z0[d] - z0[s] };
 			int slot = -1;
-			for (size_t q = 0; q < keys.size() && slot < 0; q++)
+			const bool cplx = !m->hkcoff.empty() && m->hkcoff[(size_t)d * nlayers + s] >= 0;
+			for (size_t q = 0; q < keys.size() && slot < 0 && !cplx; q++)
 				if (keys[q].self == k.self && same(keys[q].hd, k.hd) && same(keys[q].hs, k.hs) && same(keys[q].sh, k.sh)) slot = (int)q;
 			if (slot < 0) { slot = (int)keys.size(); keys.push_back(k); }
 			m->hkidx[(size_t)d * nlayers + s] = slot;
@@ -629,18 +859,21 @@ static int ml_create_grid(int nlayers, int nx, int ny, double hx, double hy, con
 	m->nslot = (int)keys.size();
 	m->kcount = (long long)(m->per_pair() ? nlayers * nlayers : m->nslot) * 6 * m->NC * m->NYH;
 	if (hipMalloc(&m->Sx, sxbytes) != hipSuccess || hipMalloc(&m->S, sbytes) != hipSuccess
-		|| (nlayers > kRegLayers && hipMalloc(&m->S2, sbytes) != hipSuccess)
+		|| ((nlayers > kRegLayers || ncx) && hipMalloc(&m->S2, sbytes) != hipSuccess)
+		|| (ncx && hipMalloc(&m->Kc, (size_t)ncx * 12 * m->NC * m->Ny * sizeof(float)) != hipSuccess)
+		|| (ncx && hipMalloc(&m->kcoff, m->hkcoff.size() * sizeof(long long)) != hipSuccess)
 		|| hipMalloc(&m->K, (size_t)m->kcount * sizeof(float)) != hipSuccess
 		|| (nlayers > kRegLayers && hipMalloc(&m->kidx, m->hkidx.size() * sizeof(int)) != hipSuccess)
 		|| hipMalloc(&m->ebuf, (size_t)bm::x_plane_parts(m->xs, ny) * sizeof(double)) != hipSuccess) {
 		bm_mlayer_destroy(m);
 		return fail(BM_ERR_NOMEM, "bm_mlayer_create: device memory");
 	}
-	if (m->kidx && hipMemcpy(m->kidx, m->hkidx.data(), m->hkidx.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+	if ((m->kidx && hipMemcpy(m->kidx, m->hkidx.data(), m->hkidx.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+		|| (m->kcoff && hipMemcpy(m->kcoff, m->hkcoff.data(), m->hkcoff.size() * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess)) {
 		bm_mlayer_destroy(m);
 		return fail(BM_ERR_HIP, "bm_mlayer_create: pair table");
 	}
-	m->bytes = (long long)(sbytes * (m->S2 ? 2 : 1) + sxbytes) + m->kcount * 4;
+	m->bytes = (long long)(sbytes * (m->S2 ? 2 : 1) + sxbytes) + m->kcount * 4 + ncx * 48LL * m->NC * m->Ny;
 	int rc = compute_ml_kernels(m, nullptr);
 	if (rc) { bm_mlayer_destroy(m); return rc; }
 	*out = m;
@@ -734,14 +967,22 @@ static int ml_create_meshes(int nlayers, const bm_layer_mesh* ms, bm_mlayer_t* o
 	const double hcx = mxs / ncx, hcy = mys / ncy;
 	double bx = Rc[0].sx, by = Rc[0].sy, bex = Rc[0].ex, bey = Rc[0].ey;
 	for (const R3& c : Rc) { bx = std::min(bx, c.sx); by = std::min(by, c.sy); bex = std::max(bex, c.ex); bey = std::max(bey, c.ey); }
-	auto whole = [](double v, int& n) { const double r = std::nearbyint(v); n = (int)r; return std::fabs(v - r) <= 1e-6; };
+	(void)bex; (void)bey;
+	// each rectangle starts a whole number of common cells into the box plus a fraction of one:
+	// the fractions couple the layers through kernels shifted in-plane (complex kernels)
+	auto split = [](double v, int& n, double& r) { const double f = std::floor(v + 1e-6); n = (int)f; r = std::fabs(v - f) <= 1e-6 ? 0.0 : v - f; };
 	int nbx = 0, nby = 0;
 	std::vector<int> ox((size_t)nlayers), oy((size_t)nlayers);
-	bool ok = whole((bex - bx) / hcx, nbx) && whole((bey - by) / hcy, nby);
-	for (int l = 0; l < nlayers && ok; l++)
-		ok = whole((Rc[(size_t)l].sx - bx) / hcx, ox[(size_t)l]) && whole((Rc[(size_t)l].sy - by) / hcy, oy[(size_t)l]);
-	if (!ok) return fail(BM_ERR_SIZE, "bm_mlayer_create_meshes: the layers' convolution rectangles do not lie on one grid of common cells "
-		"(the reference couples them by kernels shifted by fractions of a cell; not built)");
+	std::vector<double> rxs((size_t)nlayers), rys((size_t)nlayers);
+	bool frac = false;
+	for (int l = 0; l < nlayers; l++) {
+		double fx, fy;
+		split((Rc[(size_t)l].sx - bx) / hcx, ox[(size_t)l], fx);
+		split((Rc[(size_t)l].sy - by) / hcy, oy[(size_t)l], fy);
+		rxs[(size_t)l] = fx * hcx; rys[(size_t)l] = fy * hcy;
+		frac = frac || fx != 0.0 || fy != 0.0;
+		nbx = std::max(nbx, ox[(size_t)l] + ncx); nby = std::max(nby, oy[(size_t)l] + ncy);
+	}
 	const int nxp = plan_length_at_least(std::max(2, nbx)), nyp = plan_length_at_least(2 * nby) / 2;
 	if (!nxp || nyp < 1 || !bm::plan_2d_ok(nxp, 2 * nyp)) return fail(BM_ERR_SIZE, "bm_mlayer_create_meshes: no FFT plan for the common grid");
 	// the layers' own grids are the common one: nothing to transfer
@@ -754,7 +995,7 @@ static int ml_create_meshes(int nlayers, const bm_layer_mesh* ms, bm_mlayer_t* o
 	std::vector<double> hz((size_t)nlayers), z0((size_t)nlayers);
 	for (int l = 0; l < nlayers; l++) { hz[(size_t)l] = ms[l].hz; z0[(size_t)l] = ms[l].z0; }
 	bm_mlayer_t m = nullptr;
-	int rc = ml_create_grid(nlayers, nxp, nyp, hcx, hcy, hz.data(), z0.data(), &m);
+	int rc = ml_create_grid(nlayers, nxp, nyp, hcx, hcy, hz.data(), z0.data(), &m, 0, 0, frac ? rxs.data() : nullptr, frac ? rys.data() : nullptr);
 	if (rc || plain) { if (!rc) *out = m; return rc; }
 	// transfer tables
 	const size_t plane = (size_t)nxp * nyp;
@@ -881,6 +1122,8 @@ extern "C" int bm_mlayer_destroy(bm_mlayer_t m)
 	if (m->S) (void)hipFree(m->S);
 	if (m->S2) (void)hipFree(m->S2);
 	if (m->K) (void)hipFree(m->K);
+	if (m->Kc) (void)hipFree(m->Kc);
+	if (m->kcoff) (void)hipFree(m->kcoff);
 	if (m->kidx) (void)hipFree(m->kidx);
 	if (m->ebuf) (void)hipFree(m->ebuf);
 	if (m->tabs) (void)hipFree(m->tabs);
@@ -900,7 +1143,7 @@ static int ml_convolute(bm_mlayer_s* m, const float* M, float* out, int mode, hi
 	if (rc == BM_OK) rc = bm::pass_y(m->Ny, -1, m->S, m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st, fy);
 	if (rc == BM_OK) {
 		bm::prof_mark_begin(bm::PK_ML_MULT, st);
-		rc = launch_ml_mult(L, m->S, m->S2, m->K, m->kidx, m->NC, m->Ny, m->Sp, m->ncu, st, tr);
+		rc = launch_ml_mult(L, m->S, m->S2, m->K, m->kidx, m->NC, m->Ny, m->Sp, m->ncu, st, tr, m->Kc, m->kcoff);
 		bm::prof_mark_end(bm::PK_ML_MULT, st);
 	}
 	if (rc == BM_OK) rc = bm::pass_y(m->Ny, 1, m->Sout(), m->Sx, nlines, m->Sp, m->NC, m->ntl, m->ncu, st, fy);
@@ -1016,7 +1259,7 @@ extern "C" int bm_mlayer_multiply(bm_mlayer_t m, void* stream)
 	if (!m) return fail(BM_ERR_ARG, "bm_mlayer_multiply: null handle");
 	hipStream_t st = (hipStream_t)stream;
 	bm::prof_mark_begin(bm::PK_ML_MULT, st);
-	int rc = launch_ml_mult(m->L, m->S, m->S2, m->K, m->kidx, m->NC, m->Ny, m->Sp, m->ncu, st);
+	int rc = launch_ml_mult(m->L, m->S, m->S2, m->K, m->kidx, m->NC, m->Ny, m->Sp, m->ncu, st, 0, m->Kc, m->kcoff);
 	bm::prof_mark_end(bm::PK_ML_MULT, st);
 	if (rc) return rc;
 	BM_HIP(hipGetLastError());

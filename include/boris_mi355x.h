@@ -133,11 +133,13 @@ int bm_mlayer_create_pbc(int nlayers, int nx, int ny, double hx, double hy, cons
  * set_Rect_collection (Boris/SDemag.cpp:173-300), which put every mesh's rectangle onto one
  * common grid, and the mesh transfer to and from it (SDemag_Demag::Initialize_Mesh_Transfer,
  * BorisLib/VEC_MeshTransfer.h).  The common cell is the reference's (per axis the largest
- * layer extent over the largest cell count); the layers sit in their bounding box at that
- * cell (padded with empty cells to planned FFT lengths).  A layer's cells must be whole
- * multiples of the common cell and start on it: the reference's transfer then reduces to
- * copying a cell's M into the common cells it covers and averaging their fields back (its
- * reciprocal-distance weights are all equal).  Other layouts return BM_ERR_SIZE.  M and H of all layers are the layers' own cells concatenated
+ * layer extent over the largest cell count), the transfer the reference's weights (in:
+ * reciprocal distance times covered volume, out: reciprocal distance); the rectangles sit in
+ * their bounding box at that cell (padded with empty cells to planned FFT lengths), and a
+ * rectangle starting a fraction of a common cell off that grid couples to the others through
+ * the tensor shifted in-plane by the fraction (complex kernels: the reference's Complex_Full
+ * kernels, Boris/DemagKernelCollectionCUDA_Calc.cpp:571-745, tensors of the general branch of
+ * Boris/DemagTFunc_Shifted.cpp:235-343 / DemagTFunc_Irregular.cpp:5-235).  M and H of all layers are the layers' own cells concatenated
  * layer by layer (AoS); the split calls take one layer's own nx*ny cells.  Such a handle
  * serves the convolution calls; the integrator takes handles of equal layers only. */
 typedef struct { int nx, ny; double hx, hy, hz, x0, y0, z0; } bm_layer_mesh;

@@ -112,6 +112,13 @@ MESH_CASES = {
     "side": [(16, 12, H4, H4, 4e-9, 0.0, 0.0, 0.0), (16, 12, H4, H4, 4e-9, 20 * H4, 3 * H4, 2e-9)],
     # cells that are no multiple of each other: 20 cells of 5 nm over 16 of 6.25 nm
     "ratio": [(20, 20, 5e-9, 5e-9, 3e-9, 0.0, 0.0, 0.0), (16, 16, 6.25e-9, 6.25e-9, 3e-9, 0.0, 0.0, 5e-9)],
+    # rectangles offset by fractions of a common cell (the reference's Complex_Full kernels)
+    "frac_side": [(16, 12, H4, H4, 4e-9, 0.0, 0.0, 0.0), (16, 12, H4, H4, 4e-9, 20.5 * H4, 3.25 * H4, 2e-9)],
+    "frac_three": [(16, 12, H4, H4, 8e-9, 0.0, 0.0, 0.0), (16, 12, H4, H4, 4e-9, 1.3e-9, -2.2e-9, 10e-9),
+                   (16, 12, H4, H4, 4e-9, 5.1e-9, 0.6e-9, 16e-9)],
+    # a 2x coarser layer offset by a fraction of the common cell, and a far one (far-field pairs)
+    "frac_coarse": [(32, 16, H4, H4, 3e-9, 0.0, 0.0, 0.0), (16, 8, 2 * H4, 2 * H4, 6e-9, 0.7 * H4, 0.0, 5e-9)],
+    "frac_far": [(24, 20, H4, H4, 4e-9, 0.0, 0.0, 0.0), (6, 4, H4, H4, 4e-9, 40.4 * H4, 44.6 * H4, 30e-9)],
 }
 
 
@@ -156,8 +163,12 @@ def test_layer_size_without_a_plan(gpu):
     d.close()
 
 
-def test_mesh_layouts_the_transfer_does_not_cover(gpu):
-    """Rectangles that do not lie on one grid of common cells need kernels shifted by a
-    fraction of a cell: refused with BM_ERR_SIZE."""
+def test_fractional_offset_layers_refused_by_the_integrator(gpu):
+    """Mesh layers (common grid, transfers, complex kernels) serve the convolution calls only:
+    the integrator and the adjoint refuse such a handle with BM_ERR_SIZE."""
+    d = api.MultiLayerMeshDemag([(16, 12, H4, H4, 4e-9, 0.0, 0.0, 0.0), (16, 12, H4, H4, 4e-9, 20.5 * H4, 0.0, 2e-9)])
+    p = api.FMParams(Ms=8e5, A=1.3e-11, alpha=0.02, modules=api.BM_MOD_EXCHANGE)
+    s = api.FMSolver(api.BM_EVAL_RK4, (36, 12, 2), (H4, H4, 4e-9), p)
     with pytest.raises(RuntimeError, match="error 2"):
-        api.MultiLayerMeshDemag([(16, 12, H4, H4, 4e-9, 0.0, 0.0, 0.0), (16, 12, H4, H4, 4e-9, 20.5 * H4, 0.0, 2e-9)])
+        api._lib.check(s.L.bm_ode_set_layers(s.handle, d.handle, None, None, None, None))
+    s.close(); d.close()
