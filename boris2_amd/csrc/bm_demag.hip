@@ -634,8 +634,8 @@ bool x_supported(int n) { switch (n) {
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	return true; default: return false; } }
-bool y_supported(int n) { return x_supported(n) || n == 4096; }
-bool line_supported(int n) { return x_supported(n) || n == 1536 || n == 4096; }
+bool y_supported(int n) { return x_supported(n); }
+bool line_supported(int n) { return x_supported(n) || n == 1536 || n == 8192; }
 bool z_supported(int n) { switch (n) {
 #define C_(V_) case V_:
 	BM_FOR_Z_SIZES(C_)
@@ -778,6 +778,48 @@ int launch_y2d(int NY, bool full, float2* Sx, const float* K, int NC, int NYH, i
 	return fail(BM_ERR_SIZE, "no y plan");
 }
 
+// Real-input FFT of length 2M (the x transform of the tensor for N.x = 8192, beyond the one-
+// block plans): the M-point complex FFT of z[k] = x[2k] + i x[2k+1], then the split
+// X[k] = (Z[k] + conj Z[M-k]) / 2 - i W^k (Z[k] - conj Z[M-k]) / 2 and X[k+M] with -W^k,
+// W = exp(-i pi / M), in FP64; arguments and outputs as k_line_fft's.
+template <int M, typename TO>
+__global__ void __launch_bounds__(kBlock) k_line_rfft2(const double* __restrict__ src, TO* __restrict__ dst, int nlines, int na,
+	long long sa, long long sb, long long se, long long da, long long db, long long de, int keep, int part, double sign)
+{
+	constexpr int E = Plan<M>::E, T = plan_T<M>(), LP = kBlock / T, LL = lds_len<M>();
+	static_assert(LP == 1, "one line per block");
+	__shared__ C2<double> lds[LL];
+	const int t = threadIdx.x;
+	Twiddles<M, double> tw; tw.init(t);
+	for (int line = blockIdx.x; line < nlines; line += gridDim.x) {
+		const int a = line % na, b = line / na;
+		const double* p = src + a * sa + b * sb;
+		C2<double> v[E];
+#pragma unroll
+		for (int e = 0; e < E; e++) { const long long k = t + T * e; v[e] = mk(p[2 * k * se], p[(2 * k + 1) * se]); }
+		fft<M, -1, double>(v, lds, 1, t, tw);
+		__syncthreads();                              // the transform's last LDS reads are done
+#pragma unroll
+		for (int e = 0; e < E; e++) lds[lidx(t + T * e, 1)] = v[e];
+		__syncthreads();
+		TO* q = dst + a * da + b * db;
+#pragma unroll
+		for (int e = 0; e < E; e++) {
+			const int k = t + T * e;
+			const C2<double> zk = v[e], zm = conj(lds[lidx((M - k) % M, 1)]);
+			const C2<double> fe = 0.5 * (zk + zm), fd = zk - zm;
+			const C2<double> fo = mk(0.5 * fd.y, -0.5 * fd.x);        // -i (Z[k] - conj Z[M-k]) / 2
+			double sn, cs;
+			sincospi(-(double)k / M, &sn, &cs);
+			const C2<double> wf = cmul(mk(cs, sn), fo);
+			const C2<double> lo = fe + wf, hi = fe - wf;
+			if (k < keep) q[k * de] = (TO)(sign * (part ? lo.y : lo.x));
+			if (k + M < keep) q[(k + M) * de] = (TO)(sign * (part ? hi.y : hi.x));
+		}
+		__syncthreads();                              // before the next line's transform reuses the LDS
+	}
+}
+
 template <typename TO>
 int launch_line_fft(int N, const double* src, TO* dst, int nlines, int na, long long sa, long long sb, long long se,
 	long long da, long long db, long long de, int keep, int part, double sign, hipStream_t st)
@@ -787,8 +829,10 @@ int launch_line_fft(int N, const double* src, TO* dst, int nlines, int na, long 
 		hipLaunchKernelGGL((k_line_fft<V_, TO>), dim3(std::max(1, std::min((nlines + LP - 1) / LP, 8192))), dim3(kBlock), 0, st, \
 			src, dst, nlines, na, sa, sb, se, da, db, de, keep, part, sign); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
-	C_(1536) C_(4096)
+	C_(1536)
 #undef C_
+	case 8192: hipLaunchKernelGGL((k_line_rfft2<4096, TO>), dim3(std::max(1, std::min(nlines, 8192))), dim3(kBlock), 0, st,
+		src, dst, nlines, na, sa, sb, se, da, db, de, keep, part, sign); return BM_OK;
 	}
 	return fail(BM_ERR_SIZE, "no setup FFT plan");
 }
@@ -942,7 +986,7 @@ int bm::demag_planar(bm_demag_s* d, const float* M, float* H3n, hipStream_t st)
 	return rc;
 }
 
-// The smallest x-pass length (2^a 3^b, 2..2048) at or above v, 0 if none.
+// The smallest x-pass length (2^a 3^b, 2..4096) at or above v, 0 if none.
 int x_length_at_least(int v)
 {
 	int best = 0;
@@ -955,7 +999,7 @@ int x_length_at_least(int v)
 int bm::plan_length_at_least(int v) { return x_length_at_least(v); }
 
 // The smallest y-pass length at or above v, 0 if none.
-int y_length_at_least(int v) { return v > 2048 ? (v <= 4096 ? 4096 : 0) : x_length_at_least(v); }
+int y_length_at_least(int v) { return x_length_at_least(v); }
 
 // Sizes (Boris/ConvolutionDataCUDA.cpp:24-100): N = n along a periodic axis, otherwise the
 // reference takes N.x = 2 nx, N.y = 2 ny (any n, cuFFT plans every length) and N.z the next
@@ -977,8 +1021,8 @@ static int demag_create(int nx, int ny, int nz, double hx, double hy, double hz,
 	if (fz) Nz = nz;
 	else if (nz > 1) { while (Nz < 2 * nz) Nz *= 2; }
 	if (!HX || !Ny || !x_supported(HX) || !y_supported(Ny) || !line_supported(2 * HX) || (nz > 1 && !z_supported(Nz)))
-		return fail(BM_ERR_SIZE, "bm_demag_create: no FFT plan for this mesh size (open axes: nx <= 2048, ny <= 2048, nz <= 128; "
-			"periodic axes: N.x / 2 in the 2^a 3^b plans 2..2048, N.y in those or 4096, N.z a power of two 4..256, N = n)");
+		return fail(BM_ERR_SIZE, "bm_demag_create: no FFT plan for this mesh size (open axes: nx <= 4096, ny <= 2048, nz <= 128; "
+			"periodic axes: N.x / 2 and N.y in the 2^a 3^b plans 2..4096, N.z a power of two 4..256, N = n)");
 	bm_demag_s* d = new bm_demag_s();
 	d->nx = nx; d->ny = ny; d->nz = nz; d->h[0] = hx; d->h[1] = hy; d->h[2] = hz;
 	for (int q = 0; q < 3; q++) d->pbc[q] = pbc ? pbc[q] : 0;

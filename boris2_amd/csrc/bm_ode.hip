@@ -473,7 +473,7 @@ int set_self_coeffs(bm_ode_s* o)
 bool row_path(const bm_ode_s* o)
 {
 	return (o->p.modules & BM_MOD_DEMAG) && o->use_row && !o->ml && !o->speedup && !o->pbc[0] && !o->pbc[1] && !o->pbc[2]
-		&& o->demag && !o->demag->full[0] && o->demag->Nx == 2 * o->nx;
+		&& o->demag && !o->demag->full[0] && o->demag->Nx == 2 * o->nx && o->nx <= 2048;
 }
 
 // One field evaluation + stage `step` (the effective field is that of Mcur at stagetime).
@@ -486,7 +486,10 @@ int evaluate(bm_ode_s* o, int step, float dT, hipStream_t st)
 	if (demag && o->speedup) return eval_speedup(o, step, dT, st);
 	if (demag && o->ml) { int rc = bm::mlayer_planar(o->ml, o->Mcur, o->Hd, st); if (rc) return rc; o->convolutions++; return launch_stage(o, step, dT, st); }
 	if (row_path(o)) { o->convolutions++; return eval_row(o, step, dT, st); }
-	if (demag) { int rc = bm::demag_planar(o->demag, o->Mcur, o->Hd, st); if (rc) return rc; o->convolutions++; }
+	if (demag) {
+		if (!o->Hd) return fail(BM_ERR_STATE, "unfused demag evaluation without its field buffer");
+		int rc = bm::demag_planar(o->demag, o->Mcur, o->Hd, st); if (rc) return rc; o->convolutions++;
+	}
 	return launch_stage(o, step, dT, st);
 }
 
@@ -650,8 +653,10 @@ extern "C" int bm_ode_create(int method, int nx, int ny, int nz, double hx, doub
 	// as a cross-check of the fused row kernel
 	if (const char* e = getenv("BM_ROW_KERNEL")) o->use_row = atoi(e) != 0;
 	// the row kernel's x passes need rows of exactly N.x / 2 cells with a zero-padded half: not
-	// for a demag handle periodic along x, nor for rows shorter than N.x / 2 (general sizes)
-	if (demag && (demag->full[0] || demag->Nx != 2 * nx)) o->use_row = false;
+	// for a demag handle periodic along x, nor for rows shorter than N.x / 2 (general sizes),
+	// nor for rows beyond the row kernels' 2048 cells (those take the unfused evaluation, whose
+	// planar demag field buffer is allocated below)
+	if (demag && (demag->full[0] || demag->Nx != 2 * nx || nx > 2048)) o->use_row = false;
 	int nE = 0;
 	// method defaults: Boris/DiffEq_CommonBase_Control.cpp:10-140 with Boris/DiffEq_Defs.h
 	switch (method) {

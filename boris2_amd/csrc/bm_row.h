@@ -238,7 +238,7 @@ int launch_row_t(const bm_demag_s* d, const StageArgs& a, float scale, hipStream
 #define C_(V_) case V_: { constexpr int TY = XTile<V_>::v; const int nt = (d->ny + TY - 1) / TY; \
 		hipLaunchKernelGGL((k_row<V_, TY, METHOD, STEP, R2C>), dim3(row_grid(d->nz * nt, d->ncu)), dim3(kXBlock), 0, st, a, d->Sx, d->ntl, nt, scale); \
 		return BM_OK; }
-	BM_FOR_X_SIZES(C_)
+	BM_FOR_ROW_SIZES(C_)
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no x plan for the row kernel");

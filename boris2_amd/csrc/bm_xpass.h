@@ -19,10 +19,12 @@ namespace bm {
 
 constexpr int kXBlock = 256;
 
-// x-pass sizes HX = nx (the half-length complex FFT of a row zero padded to 2 nx)
-#define BM_FOR_X_SIZES(M) M(2) M(4) M(6) M(8) M(12) M(16) M(24) M(32) M(48) M(64) M(96) M(128) M(192) M(256) M(384) M(512) M(768) M(1024) M(2048)
-// y-pass lengths N.y: the x-pass sizes and 4096 (ny up to 2048)
-#define BM_FOR_Y_SIZES(M) BM_FOR_X_SIZES(M) M(4096)
+// x-pass sizes HX = nx (the half-length complex FFT of a row zero padded to 2 nx); the fused
+// row kernels of the integrator take them up to 2048, the stand-alone x passes up to 4096
+#define BM_FOR_ROW_SIZES(M) M(2) M(4) M(6) M(8) M(12) M(16) M(24) M(32) M(48) M(64) M(96) M(128) M(192) M(256) M(384) M(512) M(768) M(1024) M(2048)
+#define BM_FOR_X_SIZES(M) BM_FOR_ROW_SIZES(M) M(4096)
+// y-pass lengths N.y (ny up to 2048)
+#define BM_FOR_Y_SIZES(M) BM_FOR_X_SIZES(M)
 
 // LDS tile row: HX+1 spectrum values, and room for the padded FFT exchange
 template <int HX> constexpr int xrow_pitch() { return lds_len<HX>() > HX + 1 ? lds_len<HX>() : HX + 1; }

@@ -37,7 +37,7 @@ int bm_device_ok(void);
  * DemagKernelCUDA::Calculate_Demag_Kernels (Boris/DemagCUDA.cpp:31,
  * Boris/ConvolutionDataCUDA.cpp:24, Boris/DemagKernelCUDA.cpp:58/248/358/603):
  * computes the FP64 Newell tensor and its FFT on the GPU once, stores the FP32 kernels
- * and allocates the spectral scratch.  Sizes: any nx <= 2048, ny <= 2048, nz <= 128; N.x and
+ * and allocates the spectral scratch.  Sizes: any nx <= 4096, ny <= 2048, nz <= 128; N.x and
  * N.y the smallest planned lengths >= 2nx, 2ny (the 2^a 3^b plans, N.y up to 4096: the same
  * zero-padded convolution as the reference's N = 2n), N.z = 1 (nz = 1) or the next power of
  * two >= 2nz, as the reference's GPU path does; larger meshes return BM_ERR_SIZE.
@@ -53,8 +53,8 @@ int bm_demag_create(int nx, int ny, int nz, double hx, double hy, double hz, int
  * Boris/DemagKernelCUDA.cpp:91-108/556-582, Boris/DemagTFunc_PBC.cpp:5-422), reached from
  * the `pbc` console command (Demag::Set_PBC).  Along a periodic axis N = n (no zero padding)
  * and the tensor sums the images -p..p of the source; in 2D, z images sit at whole multiples
- * of the cell thickness.  Sizes: N.x / 2 in the 2^a 3^b plans 2..2048 (so a periodic nx is a
- * multiple of 4), N.y in those or 4096, a periodic nz (3D) a power of two 4..256; anything else
+ * of the cell thickness.  Sizes: N.x / 2 and N.y in the 2^a 3^b plans 2..4096 (so a periodic
+ * nx is a multiple of 4), a periodic nz (3D) a power of two 4..256; anything else
  * returns BM_ERR_SIZE.  All three counts zero is bm_demag_create.  The convolution calls take
  * the handle unchanged; the integrator (bm_ode_set_pbc), the adjoint (bm_adj_set_pbc) and
  * batches (bm_batch_set_pbc) take periodic meshes; periodic layer stacks are made by

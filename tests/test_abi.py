@@ -37,14 +37,14 @@ def test_product_sources_do_not_reference_the_oracle():
 
 
 def test_unsupported_sizes_are_rejected_before_any_device_work():
-    """Open axes take any nx <= 2048, ny <= 2048 and nz <= 128 (N padded up to a planned
+    """Open axes take any nx <= 4096, ny <= 2048 and nz <= 128 (N padded up to a planned
     length); larger sizes return BM_ERR_SIZE and bad arguments BM_ERR_ARG, both checked
     before any allocation, so this runs without a device."""
     import ctypes as C
     L = _lib.load()
     h = C.c_void_p()
     cases = [((8, 8, 129), 2),      # N.z = 512 > 256
-             ((2049, 8, 1), 2),     # nx above 2048
+             ((4097, 8, 1), 2),     # nx above 4096
              ((8, 2049, 1), 2),     # N.y above 4096
              ((0, 8, 1), 1), ((8, 0, 1), 1), ((8, 8, 0), 1)]
     for n, code in cases:

@@ -22,7 +22,9 @@ def rel_l2(a, b):
 SIZES = [(10, 8, 1), (50, 25, 1), (100, 100, 1), (125, 7, 1), (33, 17, 1), (101, 3, 1), (3, 1, 1), (1000, 20, 1),
          (2000, 6, 1), (10, 14, 3), (35, 21, 5), (100, 50, 4), (7, 9, 13), (250, 40, 2),
          # N.y = 4096 (ny above 1024) and N.z = 256 (nz above 64)
-         (8, 1100, 1), (6, 1500, 2), (4, 2048, 1), (4, 4, 100), (3, 5, 128)]
+         (8, 1100, 1), (6, 1500, 2), (4, 2048, 1), (4, 4, 100), (3, 5, 128),
+         # N.x = 8192 (nx above 2048: 4096-point x passes, the tensor's 8192-point x transform)
+         (4096, 4, 1), (3000, 5, 2), (2500, 3, 1)]
 
 
 @pytest.mark.parametrize("n", SIZES, ids=lambda n: "x".join(map(str, n)))
@@ -157,3 +159,26 @@ def test_largest_open_film_2048x2048(gpu):
     print(f"2048x2048: rel L2 {e:.2e}")
     assert e <= 1e-6
     d.close()
+
+
+@pytest.mark.parametrize("method", [api.BM_EVAL_RKDP54, api.BM_EVAL_RK4])
+def test_nx_4096_trajectory_matches_oracle(gpu, method):
+    """Rows of 4096 cells (N.x = 8192): the integrator takes the unfused evaluation."""
+    n, h = (4096, 4, 1), (4e-9, 4e-9, 4e-9)
+    p = api.FMParams(Ms=8e5, A=1.3e-11, alpha=0.02, K1=2e5, ea=(0.0, 0.0, 1.0), Ha=(-1.5e4, 3e3, 1e3),
+                     modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE | api.BM_MOD_ZEEMAN | api.BM_MOD_ANIS_UNI)
+    ncell = n[0] * n[1] * n[2]
+    rng = np.random.default_rng(4)
+    th, ph = rng.uniform(0, 0.3, ncell), rng.uniform(0, 2 * np.pi, ncell)
+    M0 = (p.Ms * np.stack([np.cos(th), np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph)], 1)).astype(np.float32)
+    s = api.FMSolver(method, n, h, p, M0)
+    o = O.OracleSim(method, n, h, p)
+    o.set_M(M0.astype(np.float64))
+    s.run_to_iteration(20)
+    o.run_to_iteration(20)
+    st, so = s.state(), o.state()
+    assert st["rejections"] == so["rejections"]
+    assert abs(st["time"] - so["time"]) <= 1e-4 * so["time"]
+    assert float(np.abs(s.average_m() - o.average_m()).max()) <= 1e-4
+    assert rel_l2(s.M.cpu().numpy(), o.get_M()) <= 1e-4
+    s.close()
