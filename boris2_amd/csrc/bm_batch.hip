@@ -104,6 +104,11 @@ struct bm_batch_s {
 	long long iteration = 0;
 	hipStream_t gs = nullptr;
 	hipEvent_t ev_in = nullptr, ev_out = nullptr;
+	// side streams on which the pass-size groups of one pass run as parallel branches
+	static constexpr int kMaxSide = 7;
+	int nside = 0;
+	hipStream_t side[kMaxSide] = {};
+	hipEvent_t ev_fork = nullptr, ev_join[kMaxSide] = {};
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	bool autotune = false, tune_radix = false, tuned = false;
@@ -118,6 +123,11 @@ void free_all(bm_batch_s* o)
 	if (o->ev_in) (void)hipEventDestroy(o->ev_in);
 	if (o->ev_out) (void)hipEventDestroy(o->ev_out);
 	if (o->gs) (void)hipStreamDestroy(o->gs);
+	if (o->ev_fork) (void)hipEventDestroy(o->ev_fork);
+	for (int k = 0; k < o->nside; k++) {
+		if (o->ev_join[k]) (void)hipEventDestroy(o->ev_join[k]);
+		if (o->side[k]) (void)hipStreamDestroy(o->side[k]);
+	}
 	for (bm_demag_t d : o->dm) if (d) bm_demag_destroy(d);
 	for (bm_mlayer_t m : o->ml) if (m) bm_mlayer_destroy(m);
 	if (o->Msmap) (void)hipFree(o->Msmap);
@@ -410,31 +420,52 @@ int upload_dt(bm_batch_s* o)
 	return BM_OK;
 }
 
+// The n pass-size groups of one pass: launch(g, stream) for each.  With side streams the
+// groups run as parallel branches (in a captured step: graph branches), joined on st before
+// the next pass; each group's launch reads and writes only its own meshes' buffers.
+template <typename F> int fan(bm_batch_s* o, size_t n, hipStream_t st, F&& launch)
+{
+	const int S = (int)std::min<size_t>(n, (size_t)o->nside + 1);
+	int rc = BM_OK;
+	if (S <= 1) {
+		for (size_t g = 0; g < n && !rc; g++) rc = launch(g, st);
+		return rc;
+	}
+	BM_HIP(hipEventRecord(o->ev_fork, st));
+	for (int k = 1; k < S; k++) BM_HIP(hipStreamWaitEvent(o->side[k - 1], o->ev_fork, 0));
+	for (size_t g = 0; g < n && !rc; g++) { const int k = (int)(g % S); rc = launch(g, k ? o->side[k - 1] : st); }
+	for (int k = 1; k < S; k++) {
+		BM_HIP(hipEventRecord(o->ev_join[k - 1], o->side[k - 1]));
+		BM_HIP(hipStreamWaitEvent(st, o->ev_join[k - 1], 0));
+	}
+	return rc;
+}
+
 // One field evaluation + stage `step` for every mesh; input magnetisation M (parity 0) or Mb.
 int eval_all(bm_batch_s* o, int step, int par, hipStream_t st)
 {
-	int rc = BM_OK;
-	for (size_t g = 0; g < o->gx.size() && !rc; g++)
-		rc = batch_fwd_x(o->gx[g].size, o->gx[g].full, o->gx[g].sh[0], o->fwd[par][g].h.data(), o->fwd[par][g].d, (int)o->fwd[par][g].h.size(), o->ncu, st);
-	for (size_t g = 0; g < o->gy.size() && !rc; g++)
-		rc = batch_y(o->gy[g].size, o->gy[g].full, -1, o->gy[g].sh[0], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st);
-	for (size_t g = 0; g < o->gz.size() && !rc; g++)
-		rc = batch_z(o->gz[g].size, o->gz[g].full, o->gz[g].sh[0], o->tz[g].h.data(), o->tz[g].d, (int)o->tz[g].h.size(), o->ncu, st);
-	for (size_t g = 0; g < o->gml.size() && !rc; g++)
-		rc = batch_ml_mult(o->gml[g].size, o->gml[g].sh[0], o->tml[g].h.data(), o->tml[g].d, (int)o->tml[g].h.size(), o->ncu, st);
-	for (size_t g = 0; g < o->gy.size() && !rc; g++)
-		rc = batch_y(o->gy[g].size, o->gy[g].full, 1, o->gy[g].sh[1], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, st);
-	for (size_t g = 0; g < o->gy2.size() && !rc; g++)
-		rc = batch_y2d(o->gy2[g].size, o->gy2[g].full, o->gy2[g].sh[0], o->ty2[g].h.data(), o->ty2[g].d, (int)o->ty2[g].h.size(), o->ncu, st);
-	for (size_t g = 0; g < o->gx.size() && !rc; g++)
-		rc = batch_inv_x_planar(o->gx[g].size, o->gx[g].full, o->gx[g].sh[1], o->inv[g].h.data(), o->inv[g].d, (int)o->inv[g].h.size(), o->ncu, st);
+	int rc = fan(o, o->gx.size(), st, [&](size_t g, hipStream_t s) {
+		return batch_fwd_x(o->gx[g].size, o->gx[g].full, o->gx[g].sh[0], o->fwd[par][g].h.data(), o->fwd[par][g].d, (int)o->fwd[par][g].h.size(), o->ncu, s); });
+	if (!rc) rc = fan(o, o->gy.size(), st, [&](size_t g, hipStream_t s) {
+		return batch_y(o->gy[g].size, o->gy[g].full, -1, o->gy[g].sh[0], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, s); });
+	if (!rc) rc = fan(o, o->gz.size(), st, [&](size_t g, hipStream_t s) {
+		return batch_z(o->gz[g].size, o->gz[g].full, o->gz[g].sh[0], o->tz[g].h.data(), o->tz[g].d, (int)o->tz[g].h.size(), o->ncu, s); });
+	if (!rc) rc = fan(o, o->gml.size(), st, [&](size_t g, hipStream_t s) {
+		return batch_ml_mult(o->gml[g].size, o->gml[g].sh[0], o->tml[g].h.data(), o->tml[g].d, (int)o->tml[g].h.size(), o->ncu, s); });
+	if (!rc) rc = fan(o, o->gy.size(), st, [&](size_t g, hipStream_t s) {
+		return batch_y(o->gy[g].size, o->gy[g].full, 1, o->gy[g].sh[1], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, s); });
+	if (!rc) rc = fan(o, o->gy2.size(), st, [&](size_t g, hipStream_t s) {
+		return batch_y2d(o->gy2[g].size, o->gy2[g].full, o->gy2[g].sh[0], o->ty2[g].h.data(), o->ty2[g].d, (int)o->ty2[g].h.size(), o->ncu, s); });
+	if (!rc) rc = fan(o, o->gx.size(), st, [&](size_t g, hipStream_t s) {
+		return batch_inv_x_planar(o->gx[g].size, o->gx[g].full, o->gx[g].sh[1], o->inv[g].h.data(), o->inv[g].d, (int)o->inv[g].h.size(), o->ncu, s); });
 	if (!rc) rc = batch_stage(o->mcode, step, o->stage[par].d, o->nb, o->choff.d, o->choff.h.back(), st);
 	return rc;
 }
 
 // Autotuner: for every pass-size group and pass, time each candidate launch shape (five
 // launches after a warm-up, HIP events) on the group's real tables and keep the fastest.
-// Candidates: x passes tile factor {1, 2} x workgroups per CU {1, 2, 4}; y passes {4, 8, 16};
+// Candidates: x passes tile factor {1, 2} x workgroups per CU {1, 2, 4}, and the half-height
+// tile at {2, 4, 8} per CU where the row length has one; y passes {4, 8, 16};
 // z pass {1, 2, 4}; 2D y pass {4, 8, 16}; multilayer product {4, 8, 16}.  Only the FFT passes
 // run (their inputs are recomputed from M by every evaluation), so the state is untouched.
 int autotune(bm_batch_s* o, hipStream_t st)
@@ -464,8 +495,11 @@ int autotune(bm_batch_s* o, hipStream_t st)
 		}
 		G.sh[slot] = best; G.us[slot] = bt;
 	};
-	std::vector<LaunchShape> cx, cy, cz;
+	std::vector<LaunchShape> cx, cxh, cy, cz;
 	for (int f : { 1, 2 }) for (int pc : { 1, 2, 4 }) cx.push_back(LaunchShape{ f, pc });
+	// with the half-height tile (twice the work items for small meshes) where the length has one
+	cxh = cx;
+	for (int pc : { 2, 4, 8 }) cxh.push_back(LaunchShape{ -2, pc });
 	for (int pc : { 4, 8, 16 }) cy.push_back(LaunchShape{ 0, pc });
 	for (int pc : { 1, 2, 4 }) cz.push_back(LaunchShape{ 0, pc });
 	// the launch shapes crossed with every radix schedule of the pass length n
@@ -481,8 +515,9 @@ int autotune(bm_batch_s* o, hipStream_t st)
 	};
 	for (size_t g = 0; g < o->gx.size() && !rc; g++) {
 		Group& G = o->gx[g];
-		pick(G, 0, G.full ? cx1 : cx, true, [&] { return batch_fwd_x(G.size, G.full, G.sh[0], o->fwd[0][g].h.data(), o->fwd[0][g].d, (int)o->fwd[0][g].h.size(), o->ncu, st); });
-		if (!rc) pick(G, 1, G.full ? cx1 : cx, true, [&] { return batch_inv_x_planar(G.size, G.full, G.sh[1], o->inv[g].h.data(), o->inv[g].d, (int)o->inv[g].h.size(), o->ncu, st); });
+		const std::vector<LaunchShape>& cands = G.full ? cx1 : x_half_tile_ok(G.size) ? cxh : cx;
+		pick(G, 0, cands, true, [&] { return batch_fwd_x(G.size, G.full, G.sh[0], o->fwd[0][g].h.data(), o->fwd[0][g].d, (int)o->fwd[0][g].h.size(), o->ncu, st); });
+		if (!rc) pick(G, 1, cands, true, [&] { return batch_inv_x_planar(G.size, G.full, G.sh[1], o->inv[g].h.data(), o->inv[g].d, (int)o->inv[g].h.size(), o->ncu, st); });
 	}
 	for (size_t g = 0; g < o->gy.size() && !rc; g++) {
 		Group& G = o->gy[g];
@@ -681,6 +716,18 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 	if (rc == BM_OK && o->use_graph) {
 		if (hipStreamCreate(&o->gs) != hipSuccess || hipEventCreateWithFlags(&o->ev_in, hipEventDisableTiming) != hipSuccess
 			|| hipEventCreateWithFlags(&o->ev_out, hipEventDisableTiming) != hipSuccess) rc = fail(BM_ERR_HIP, "bm_batch_create: stream");
+	}
+	if (rc == BM_OK) {
+		// concurrent branches per pass: 4 (64 uneven films: RK4 step 3.77 -> 3.12 ms against
+		// one branch, 8 no better); BM_BATCH_STREAMS overrides (1 = the groups one after another)
+		const char* e = getenv("BM_BATCH_STREAMS");
+		const int nbr = std::max(1, std::min(e ? atoi(e) : 4, bm_batch_s::kMaxSide + 1));
+		if (nbr > 1 && hipEventCreateWithFlags(&o->ev_fork, hipEventDisableTiming) != hipSuccess) rc = fail(BM_ERR_HIP, "bm_batch_create: events");
+		for (int k = 0; k + 1 < nbr && rc == BM_OK; k++) {
+			if (hipStreamCreate(&o->side[k]) != hipSuccess || hipEventCreateWithFlags(&o->ev_join[k], hipEventDisableTiming) != hipSuccess)
+				rc = fail(BM_ERR_HIP, "bm_batch_create: side streams");
+			o->nside = k + 1;
+		}
 	}
 	if (rc) { free_all(o); delete o; return rc; }
 	*out = o;
@@ -896,7 +943,8 @@ extern "C" int bm_batch_set_schedule(bm_batch_t o, const char* pass, int size, i
 	Group* G = find_group(o, pass, size, slot);
 	if (!G) return fail(BM_ERR_ARG, "bm_batch_set_schedule: no such pass and size in this batch");
 	const bool x = pass[0] == 'x', fft = pass[0] == 'y' || pass[0] == 'z';
-	if ((x && (f < 0 || f > (G->full ? 1 : 2))) || (!x && f != 0) || per_cu < 0 || per_cu > 64 || radix < 0
+	const bool f_ok = f == 0 || f == 1 || (!G->full && (f == 2 || (f == -2 && x_half_tile_ok(size))));
+	if ((x && !f_ok) || (!x && f != 0) || per_cu < 0 || per_cu > 64 || radix < 0
 		|| (fft && !G->full ? radix >= radix_variants(size) : radix != 0))
 		return fail(BM_ERR_ARG, "bm_batch_set_schedule: shape or radix schedule out of range for this pass");
 	G->sh[slot] = LaunchShape{ f, per_cu, radix };

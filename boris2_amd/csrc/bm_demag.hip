@@ -731,6 +731,18 @@ int inv_x_nwork(int HX, int ny, int nz)
 
 // z-pass work order switch for tuning runs: BM_Z_XCD=1 selects the XCD-contiguous order.
 int z_xcd() { static int v = getenv("BM_Z_XCD") ? atoi(getenv("BM_Z_XCD")) : 0; return v; }
+// Launch switches of the single-mesh y and z passes for tuning runs (0 or unset: defaults):
+// BM_Y_RADIX / BM_Z_RADIX a radix schedule of the pass length (lengths without it take the
+// default), BM_Y_PERCU / BM_Z_PERCU the workgroups per CU.
+int env_int(const char* n) { const char* e = getenv(n); return e ? atoi(e) : 0; }
+struct PassTune { int y_radix, y_percu, z_radix, z_percu; };
+const PassTune& pass_tune()
+{
+	static const PassTune t{ env_int("BM_Y_RADIX"), env_int("BM_Y_PERCU"), env_int("BM_Z_RADIX"), env_int("BM_Z_PERCU") };
+	return t;
+}
+int launch_y_tuned(int NY, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ny, int ncu, hipStream_t st);
+int launch_z_tuned(int NZ, float2* S, const float* K, int nz, int NC, int Ny, int NYH, int Sp, int ncu, hipStream_t st);
 
 template <int NY, bool FULL = false>
 void launch_y_t(int dir, dim3 g, hipStream_t st, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int nyd, const PassBatch* bt)
@@ -742,6 +754,7 @@ void launch_y_t(int dir, dim3 g, hipStream_t st, float2* S, float2* Sx, int nlin
 // full: periodic y (N.y = ny); ny: data rows of a line
 int launch_y(int NY, bool full, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ny, int ncu, hipStream_t st)
 {
+	if (!full && (pass_tune().y_radix || pass_tune().y_percu)) return launch_y_tuned(NY, dir, S, Sx, nlines, Sp, NC, ntl, ny, ncu, st);
 	switch (NY) {
 #define C_(V_) case V_: { int LP = kBlock / plan_T<V_>(); dim3 g(grid8((nlines + LP - 1) / LP, ncu, 8)); \
 		if (full) launch_y_t<V_, true>(dir, g, st, S, Sx, nlines, Sp, NC, ntl, ny, nullptr); \
@@ -755,6 +768,7 @@ int launch_y(int NY, bool full, int dir, float2* S, float2* Sx, int nlines, int 
 // full: periodic z (N.z = nz)
 int launch_z(int NZ, bool full, float2* S, const float* K, int nz, int NC, int Ny, int NYH, int Sp, int ncu, hipStream_t st)
 {
+	if (!full && (pass_tune().z_radix || pass_tune().z_percu)) return launch_z_tuned(NZ, S, K, nz, NC, Ny, NYH, Sp, ncu, st);
 	switch (NZ) {
 #define C_(V_) case V_: { int CH = kBlock / plan_T<V_>() / 2; int nw = NC * ((Ny / 2 + CH - 1) / CH); dim3 g(grid8(nw, ncu, 2)); \
 		if (full) hipLaunchKernelGGL((k_z<V_, true>), g, dim3(kBlock), 0, st, S, K, nz, NC, Ny, NYH, Sp, z_xcd(), (const PassBatch*)nullptr); \
@@ -1289,15 +1303,18 @@ bool bm::plan_2d_pbc_ok(int nx, int ny, bool fx, bool fy)
 // ---------------------------------------------------------------------------------------
 // Batched pass launchers: nb meshes of one pass size; h_tab (host) sizes the grid, d_tab
 // (device, same contents) is read by the kernels.
+// f: 1 the base tile, 2 twice it, -2 half of it (where that still holds a whole row group)
 int bm::x_tile_rows(int nx, bool fwd, int f)
 {
 	switch (nx) {
-#define C_(V_) case V_: return XTile<V_>::v * (f ? f : xtile_factor(fwd));
+#define C_(V_) case V_: return f == -2 ? (x_half_tile<V_>() ? XTile<V_>::v / 2 : 0) : XTile<V_>::v * (f ? f : xtile_factor(fwd));
 	BM_FOR_X_SIZES(C_)
 #undef C_
 	}
 	return 0;
 }
+
+bool bm::x_half_tile_ok(int nx) { return x_tile_rows(nx, true, -2) > 0; }
 
 // full: periodic x (rows of 2 HX cells, one row-tile height)
 int bm::batch_fwd_x(int nx, bool full, LaunchShape sh, const FwdArgs* h_tab, const FwdArgs* d_tab, int nb, int ncu, hipStream_t st)
@@ -1311,6 +1328,8 @@ int bm::batch_fwd_x(int nx, bool full, LaunchShape sh, const FwdArgs* h_tab, con
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; \
 		if (full) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false, true>), g, dim3(kBlock), 0, st, a, d_tab); \
 		else if (f == 1) hipLaunchKernelGGL((k_fwd_x<V_, TY1, false>), g, dim3(kBlock), 0, st, a, d_tab); \
+		else if (f == -2) { if constexpr (x_half_tile<V_>()) hipLaunchKernelGGL((k_fwd_x<V_, TY1 / 2, false>), g, dim3(kBlock), 0, st, a, d_tab); \
+			else return fail(BM_ERR_ARG, "no half tile for this row length"); } \
 		else hipLaunchKernelGGL((k_fwd_x<V_, 2 * TY1, false>), g, dim3(kBlock), 0, st, a, d_tab); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
@@ -1328,6 +1347,8 @@ int bm::batch_inv_x_planar(int nx, bool full, LaunchShape sh, const PassBatch* h
 #define C_(V_) case V_: { constexpr int TY1 = XTile<V_>::v; \
 		if (full) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1, true>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, V_, 0, 0, d_tab); \
 		else if (f == 1) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, V_, 0, 0, d_tab); \
+		else if (f == -2) { if constexpr (x_half_tile<V_>()) hipLaunchKernelGGL((k_inv_x<V_, 0, false, TY1 / 2>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, V_, 0, 0, d_tab); \
+			else return fail(BM_ERR_ARG, "no half tile for this row length"); } \
 		else hipLaunchKernelGGL((k_inv_x<V_, 0, false, 2 * TY1>), g, dim3(kBlock), 0, st, nullptr, nullptr, 0, 0, 0, 0, 0.f, nullptr, nullptr, V_, 0, 0, d_tab); return BM_OK; }
 	BM_FOR_X_SIZES(C_)
 #undef C_
@@ -1379,6 +1400,38 @@ template <int N, typename F> int with_variant(int v, F&& f)
 	if constexpr (plan_variants(N) > 1) if (v == 1) return f(std::integral_constant<int, 1>());
 	if constexpr (plan_variants(N) > 2) if (v == 2) return f(std::integral_constant<int, 2>());
 	return fail(BM_ERR_ARG, "no such radix schedule for this length");
+}
+
+int launch_y_tuned(int NY, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ny, int ncu, hipStream_t st)
+{
+	const int pc = pass_tune().y_percu ? pass_tune().y_percu : 8;
+	switch (NY) {
+#define C_(V_) case V_: return with_variant<V_>(pass_tune().y_radix < plan_variants(V_) ? pass_tune().y_radix : 0, [&](auto v) { \
+		constexpr int V = decltype(v)::value, LP = kBlock / plan_T<plan_key(V_, V)>(); \
+		const dim3 g(grid8((nlines + LP - 1) / LP, ncu, pc)); \
+		if (dir < 0) hipLaunchKernelGGL((k_y<V_, -1, false, V>), g, dim3(kBlock), 0, st, S, Sx, nlines, Sp, NC, ntl, ny, (const PassBatch*)nullptr); \
+		else hipLaunchKernelGGL((k_y<V_, 1, false, V>), g, dim3(kBlock), 0, st, S, Sx, nlines, Sp, NC, ntl, ny, (const PassBatch*)nullptr); \
+		return (int)BM_OK; });
+	BM_FOR_Y_SIZES(C_)
+#undef C_
+	}
+	return fail(BM_ERR_SIZE, "no y plan");
+}
+
+int launch_z_tuned(int NZ, float2* S, const float* K, int nz, int NC, int Ny, int NYH, int Sp, int ncu, hipStream_t st)
+{
+	const int pc = pass_tune().z_percu ? pass_tune().z_percu : 2;
+	switch (NZ) {
+#define C_(V_) case V_: return with_variant<V_>(pass_tune().z_radix < plan_variants(V_) ? pass_tune().z_radix : 0, [&](auto v) { \
+		constexpr int V = decltype(v)::value, CH = kBlock / plan_T<plan_key(V_, V)>() / 2; \
+		const int nw = NC * ((Ny / 2 + CH - 1) / CH); \
+		hipLaunchKernelGGL((k_z<V_, false, V>), dim3(grid8(nw, ncu, pc)), dim3(kBlock), 0, st, S, K, nz, NC, Ny, NYH, Sp, z_xcd(), \
+			(const PassBatch*)nullptr); \
+		return (int)BM_OK; });
+	BM_FOR_Z_SIZES(C_)
+#undef C_
+	}
+	return fail(BM_ERR_SIZE, "no z plan");
 }
 }
 

@@ -128,18 +128,20 @@ void mlayer_cell(const bm_mlayer_s* m, int l, double h[3]);
 double mlayer_z0(const bm_mlayer_s* m, int l);
 // Per-stack arguments of the batched multilayer kernel product.
 struct MlBatch { float2* S; const float* K; int NC, Ny, Sp; };
-// Launch shape of a batched pass (0 = the pass's default): f the x-pass row-tile factor
-// (1 or 2 times the base tile), per_cu the grid's workgroups per CU.  Picked per pass-size
-// group by the batched integrator's autotuner.
-// launch shape of a batched pass: row-tile factor (x passes), workgroups per CU, radix
-// schedule (plan variant of the pass length; 0 = default)
+// Launch shape of a batched pass (0 = the pass's default): f the x-pass row-tile factor (1 or
+// 2 times the base tile, -2 half of it), per_cu the grid's workgroups per CU, radix the radix
+// schedule (plan variant of the pass length).  Picked per pass-size group by the batched
+// integrator's autotuner.
 struct LaunchShape { int f = 0, per_cu = 0, radix = 0; };
 // number of radix schedules of length n, and schedule v as text ("16x16x4")
 int radix_variants(int n);
 const char* radix_schedule(int n, int v);
 int batch_ml_mult(int L, LaunchShape sh, const MlBatch* h_tab, const MlBatch* d_tab, int nb, int ncu, hipStream_t st);
-// rows per work item of the x passes for nx (0 if there is no plan)
+// rows per work item of the x passes for nx (0 if there is no plan); f: 0 the default, 1 the
+// base tile, 2 twice it, -2 half of it
 int x_tile_rows(int nx, bool fwd, int f = 0);
+// the x passes of length nx have a half-height tile (f = -2)
+bool x_half_tile_ok(int nx);
 // Batched passes over nb meshes sharing the pass size (tables: host copy for the grid size,
 // device copy for the kernels).
 int batch_fwd_x(int nx, bool full, LaunchShape sh, const FwdArgs* h_tab, const FwdArgs* d_tab, int nb, int ncu, hipStream_t st);

@@ -36,6 +36,8 @@ template <int HX> struct XTile {
 	static constexpr int cap = pw(8400 / xrow_pitch<HX>()) < 32 ? pw(8400 / xrow_pitch<HX>()) : 32;
 	static constexpr int v = RP > cap ? RP : cap;
 };
+// a tile of half the height still holds whole row groups (the batched passes can take it)
+template <int HX> constexpr bool x_half_tile() { return XTile<HX>::v / 2 >= XTile<HX>::RP; }
 
 // Per-thread constants of an x pass.
 template <int HX> struct XThread {

@@ -323,7 +323,8 @@ int bm_ode_average_m(bm_ode_t o, double* out3, void* stream);
  * bm_batch_autotune_report writes the choices and times as text lines
  * ("<pass> <size> f=<f> per_cu=<n> radix=<v> <radices> <us> us"); bm_batch_set_schedule
  * pins one pass-size group's choice (e.g. read back from an earlier report), replacing any
- * pending tuning: f (x passes 0..2, else 0; 0 = default), per_cu (0 = default, <= 64),
+ * pending tuning: f (x passes: 0 = default, 1 the base row tile, 2 twice it, -2 half of it
+ * where that holds a whole row group: nx 128 to 768; other passes 0), per_cu (0 = default, <= 64),
  * radix (0 = default; y/z/y2d passes only); BM_ERR_ARG for a pass or size the batch lacks.
  * bm_batch_set_pbc (before the first bm_batch_run): mesh `mesh` becomes periodic as
  * bm_ode_set_pbc makes one mesh (its demag handle rebuilt as bm_demag_create_pbc with these

@@ -122,3 +122,41 @@ def test_set_schedule_refuses_bad_requests(gpu):
         bs.set_schedule("w", 16)
     bs.set_schedule("z", 4, per_cu=2, radix=1)
     bs.close()
+
+
+# row lengths with a half-height x tile (nx 128..768), ragged ny against the 8-row spectra
+# tiles and the 4-row half tiles of nx 512 and 768
+HALF = [(128, 24, 2), (192, 40, 3), (256, 17, 2), (384, 12, 1), (512, 9, 2), (768, 8, 1)]
+
+
+def test_half_height_x_tiles_match_default_bit_for_bit(gpu):
+    rng = np.random.default_rng(5)
+    meshes = [(n, H, api.FMParams(Ms=float(rng.uniform(5e5, 1.1e6)), A=1.3e-11, alpha=0.1,
+                                  modules=api.BM_MOD_DEMAG | api.BM_MOD_EXCHANGE)) for n in HALF]
+    M0 = [api.random_magnetisation(n[0] * n[1] * n[2], p.Ms, 70 + b).astype(np.float32) for b, (n, _, p) in enumerate(meshes)]
+    ref_bs = api.BatchSolver(api.BM_EVAL_RK4, meshes, M0)
+    for n in HALF:
+        for pass_ in ("x_fwd", "x_inv"):
+            ref_bs.set_schedule(pass_, n[0], f=1, per_cu=2)
+    ref = run(ref_bs, 3)
+    bs = api.BatchSolver(api.BM_EVAL_RK4, meshes, M0)
+    for n in HALF:
+        for pass_ in ("x_fwd", "x_inv"):
+            bs.set_schedule(pass_, n[0], f=-2, per_cu=4)
+    rep = bs.autotune_report()
+    assert sum(" f=-2 " in l for l in rep) == 2 * len(HALF), rep
+    out = run(bs, 3)
+    # the tile height only groups rows differently: the same arithmetic on every row
+    assert np.array_equal(out, ref)
+    assert np.abs(out - M0_flat(M0)).max() > 0
+    with pytest.raises(RuntimeError, match="error 1"):
+        bs.set_schedule("x_fwd", 128, f=-3)
+    ref_bs.close(); bs.close()
+    small = api.BatchSolver(api.BM_EVAL_RK4, meshes_and_M0()[0][:3], meshes_and_M0()[1][:3])
+    with pytest.raises(RuntimeError, match="error 1"):
+        small.set_schedule("x_fwd", 16, f=-2)          # nx 16: the tile is one row group
+    small.close()
+
+
+def M0_flat(M0):
+    return np.concatenate([m.reshape(-1, 3) for m in M0]).astype(np.float64)
