@@ -724,7 +724,10 @@ extern "C" int bm_batch_create_layers(int method, int nmesh, const int* n3, cons
 		const int nbr = std::max(1, std::min(e ? atoi(e) : 4, bm_batch_s::kMaxSide + 1));
 		if (nbr > 1 && hipEventCreateWithFlags(&o->ev_fork, hipEventDisableTiming) != hipSuccess) rc = fail(BM_ERR_HIP, "bm_batch_create: events");
 		for (int k = 0; k + 1 < nbr && rc == BM_OK; k++) {
-			if (hipStreamCreate(&o->side[k]) != hipSuccess || hipEventCreateWithFlags(&o->ev_join[k], hipEventDisableTiming) != hipSuccess)
+			// non-blocking: no implicit ordering against the legacy default stream (the fork and
+			// join events order the branches)
+			if (hipStreamCreateWithFlags(&o->side[k], hipStreamNonBlocking) != hipSuccess
+				|| hipEventCreateWithFlags(&o->ev_join[k], hipEventDisableTiming) != hipSuccess)
 				rc = fail(BM_ERR_HIP, "bm_batch_create: side streams");
 			o->nside = k + 1;
 		}
