@@ -256,13 +256,13 @@ __global__ void __launch_bounds__(kBlock) k_y(float2* __restrict__ S, float2* __
 			if constexpr (DIR < 0) {
 				float2* p = S + (size_t)line * Sp;
 #pragma unroll
-				for (int e = 0; e < E; e++) p[t + T * e] = make_float2(v[e].x, v[e].y);
+				for (int e = 0; e < E; e++) st_nt(p + t + T * e, v[e].x, v[e].y);
 			}
 			else {
 				const int cz = line / NC, kx = line % NC;
 #pragma unroll
 				for (int e = 0; e < EL; e++)
-					if (t + T * e < nyd) Sx[sx_line(cz, kx, t + T * e, NC, ntl)] = make_float2(v[e].x, v[e].y);
+					if (t + T * e < nyd) st_nt(Sx + sx_line(cz, kx, t + T * e, NC, ntl), v[e].x, v[e].y);
 			}
 		}
 	}
@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(kBlock) k_z(float2* __restrict__ S, const floa
 #pragma unroll
 				for (int e = 0; e < EL; e++) {
 					const int z = t + TZ * e;
-					if (z < nz) p[cc * comp + z * plane] = make_float2(v[cc][e].x, v[cc][e].y);
+					if (z < nz) st_nt(p + cc * comp + z * plane, v[cc][e].x, v[cc][e].y);
 				}
 		}
 	}
