@@ -738,7 +738,9 @@ int env_int(const char* n) { const char* e = getenv(n); return e ? atoi(e) : 0; 
 struct PassTune { int y_radix, y_percu, z_radix, z_percu; };
 const PassTune& pass_tune()
 {
-	static const PassTune t{ env_int("BM_Y_RADIX"), env_int("BM_Y_PERCU"), env_int("BM_Z_RADIX"), env_int("BM_Z_PERCU") };
+	// negative values mean the defaults
+	static const PassTune t{ std::max(0, env_int("BM_Y_RADIX")), std::max(0, env_int("BM_Y_PERCU")),
+		std::max(0, env_int("BM_Z_RADIX")), std::max(0, env_int("BM_Z_PERCU")) };
 	return t;
 }
 int launch_y_tuned(int NY, int dir, float2* S, float2* Sx, int nlines, int Sp, int NC, int ntl, int ny, int ncu, hipStream_t st);
