@@ -92,9 +92,11 @@ struct bm_batch_s {
 	float* Hd = nullptr;           // planar per mesh: 3 * ncell[b] at 3 * off[b]
 	// pass groups and their tables
 	std::vector<Group> gx, gy, gz, gy2, gml;
+	std::vector<Group> gyz[2];             // thin meshes' fused y, z, y pass by N.y: [0] N.z 4, [1] N.z 8
 	std::vector<Tab<FwdArgs>> fwd[2];      // [input parity][nx group]
 	std::vector<Tab<PassBatch>> inv, ty, tz, ty2;
 	std::vector<Tab<MlBatch>> tml;
+	std::vector<Tab<PassBatch>> tyz[2];
 	Tab<StageArgs> stage[2];               // [input parity]
 	Tab<int> choff;                        // first flat chunk of kBS cells of each mesh (nb + 1)
 	void* dev_tables = nullptr;
@@ -305,7 +307,8 @@ void build_groups(bm_batch_s* o)
 		// a periodic x pass transforms rows of N.x = nx cells as nx/2-point complex FFTs
 		const bool fx = d->pbc[0] > 0, fy = d->pbc[1] > 0, fz = d->pbc[2] > 0;
 		add_to(o->gx, fx ? d->Nx / 2 : o->nx[b], fx, b);
-		if (o->nz[b] > 1) { add_to(o->gy, d->Ny, fy, b); add_to(o->gz, d->Nz, fz, b); }
+		if (o->nz[b] > 1 && demag_yzy(d)) add_to(o->gyz[d->Nz == 8 ? 1 : 0], d->Ny, false, b);
+		else if (o->nz[b] > 1) { add_to(o->gy, d->Ny, fy, b); add_to(o->gz, d->Nz, fz, b); }
 		else add_to(o->gy2, d->Ny, fy, b);
 	}
 }
@@ -322,6 +325,15 @@ int build_tables(bm_batch_s* o)
 	o->tz.assign(o->gz.size(), Tab<PassBatch>());
 	o->ty2.assign(o->gy2.size(), Tab<PassBatch>());
 	o->tml.assign(o->gml.size(), Tab<MlBatch>());
+	for (int q = 0; q < 2; q++) {
+		o->tyz[q].assign(o->gyz[q].size(), Tab<PassBatch>());
+		for (size_t g = 0; g < o->gyz[q].size(); g++)
+			for (int b : o->gyz[q][g].meshes) {
+				bm_demag_s* d = o->dm[b];
+				PassBatch pb{}; pb.Sx = d->Sx; pb.K = d->K; pb.nz = d->nz; pb.NC = d->NC; pb.NYH = d->NYH; pb.ntl = d->ntl; pb.ny = d->ny;
+				o->tyz[q][g].h.push_back(pb);
+			}
+	}
 	for (size_t g = 0; g < o->gx.size(); g++) {
 		const int nx = o->gx[g].size;
 		// the full-length (periodic) x passes run one row-tile height only
@@ -391,6 +403,8 @@ int build_tables(bm_batch_s* o)
 	for (auto& t : o->tz) at_z.push_back(place(t));
 	for (auto& t : o->ty2) at_y2.push_back(place(t));
 	for (auto& t : o->tml) at_ml.push_back(place(t));
+	std::vector<size_t> at_yz[2];
+	for (int q = 0; q < 2; q++) for (auto& t : o->tyz[q]) at_yz[q].push_back(place(t));
 	const size_t at_s0 = place(o->stage[0]), at_s1 = place(o->stage[1]), at_ch = place(o->choff);
 	if (hipMalloc(&o->dev_tables, std::max<size_t>(bytes, 256)) != hipSuccess) return fail(BM_ERR_NOMEM, "batch tables");
 	o->dev_bytes = bytes;
@@ -402,6 +416,7 @@ int build_tables(bm_batch_s* o)
 	for (size_t g = 0; g < o->tz.size(); g++) o->tz[g].d = (PassBatch*)(base + at_z[g]);
 	for (size_t g = 0; g < o->ty2.size(); g++) o->ty2[g].d = (PassBatch*)(base + at_y2[g]);
 	for (size_t g = 0; g < o->tml.size(); g++) o->tml[g].d = (MlBatch*)(base + at_ml[g]);
+	for (int q = 0; q < 2; q++) for (size_t g = 0; g < o->tyz[q].size(); g++) o->tyz[q][g].d = (PassBatch*)(base + at_yz[q][g]);
 	o->stage[0].d = (StageArgs*)(base + at_s0);
 	o->stage[1].d = (StageArgs*)(base + at_s1);
 	o->choff.d = (int*)(base + at_ch);
@@ -456,6 +471,10 @@ int eval_all(bm_batch_s* o, int step, int par, hipStream_t st)
 		return batch_y(o->gy[g].size, o->gy[g].full, 1, o->gy[g].sh[1], o->ty[g].h.data(), o->ty[g].d, (int)o->ty[g].h.size(), o->ncu, s); });
 	if (!rc) rc = fan(o, o->gy2.size(), st, [&](size_t g, hipStream_t s) {
 		return batch_y2d(o->gy2[g].size, o->gy2[g].full, o->gy2[g].sh[0], o->ty2[g].h.data(), o->ty2[g].d, (int)o->ty2[g].h.size(), o->ncu, s); });
+	for (int q = 0; q < 2 && !rc; q++)
+		rc = fan(o, o->gyz[q].size(), st, [&](size_t g, hipStream_t s) {
+			return batch_yzy(o->gyz[q][g].size, q ? 8 : 4, o->gyz[q][g].sh[0], o->tyz[q][g].h.data(), o->tyz[q][g].d, (int)o->tyz[q][g].h.size(),
+				o->ncu, s); });
 	if (!rc) rc = fan(o, o->gx.size(), st, [&](size_t g, hipStream_t s) {
 		return batch_inv_x_planar(o->gx[g].size, o->gx[g].full, o->gx[g].sh[1], o->inv[g].h.data(), o->inv[g].d, (int)o->inv[g].h.size(), o->ncu, s); });
 	if (!rc) rc = batch_stage(o->mcode, step, o->stage[par].d, o->nb, o->choff.d, o->choff.h.back(), st);
@@ -536,6 +555,11 @@ int autotune(bm_batch_s* o, hipStream_t st)
 		Group& G = o->gml[g];
 		pick(G, 0, cy, false, [&] { return batch_ml_mult(G.size, G.sh[0], o->tml[g].h.data(), o->tml[g].d, (int)o->tml[g].h.size(), o->ncu, st); });
 	}
+	for (int q = 0; q < 2 && !rc; q++)
+		for (size_t g = 0; g < o->gyz[q].size() && !rc; g++) {
+			Group& G = o->gyz[q][g];
+			pick(G, 0, cy, false, [&] { return batch_yzy(G.size, q ? 8 : 4, G.sh[0], o->tyz[q][g].h.data(), o->tyz[q][g].d, (int)o->tyz[q][g].h.size(), o->ncu, st); });
+		}
 	if (!rc) rc = build_tables(o);
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);
@@ -909,6 +933,8 @@ static Group* find_group(bm_batch_s* o, std::string pass, int size, int& slot)
 	else if (pass == "z") gs = &o->gz;
 	else if (pass == "y2d") gs = &o->gy2;
 	else if (pass == "ml_mult") gs = &o->gml;
+	else if (pass == "yzy4") gs = &o->gyz[0];
+	else if (pass == "yzy8") gs = &o->gyz[1];
 	if (!gs) return nullptr;
 	for (Group& G : *gs) if (G.size == size && G.full == full) return &G;
 	return nullptr;
@@ -923,7 +949,7 @@ extern "C" int bm_batch_autotune_report(bm_batch_t o, char* out, int cap)
 	char line[192];
 	auto add = [&](const char* pass, const Group& G, int slot) {
 		const LaunchShape& c = G.sh[slot];
-		const bool fft = pass[0] == 'y' || pass[0] == 'z';
+		const bool fft = (pass[0] == 'y' || pass[0] == 'z') && std::strncmp(pass, "yzy", 3) != 0;
 		snprintf(line, sizeof(line), "%s%s %d f=%d per_cu=%d radix=%d %s %.1f us\n", pass, G.full ? "_pbc" : "", G.size, c.f, c.per_cu, c.radix,
 			fft ? radix_schedule(G.size, c.radix) : "-", G.us[slot]);
 		r += line;
@@ -934,6 +960,8 @@ extern "C" int bm_batch_autotune_report(bm_batch_t o, char* out, int cap)
 		for (const Group& G : o->gz) add("z", G, 0);
 		for (const Group& G : o->gy2) add("y2d", G, 0);
 		for (const Group& G : o->gml) add("ml_mult", G, 0);
+		for (const Group& G : o->gyz[0]) add("yzy4", G, 0);
+		for (const Group& G : o->gyz[1]) add("yzy8", G, 0);
 	}
 	snprintf(out, (size_t)cap, "%s", r.c_str());
 	return BM_OK;
@@ -945,7 +973,8 @@ extern "C" int bm_batch_set_schedule(bm_batch_t o, const char* pass, int size, i
 	int slot = 0;
 	Group* G = find_group(o, pass, size, slot);
 	if (!G) return fail(BM_ERR_ARG, "bm_batch_set_schedule: no such pass and size in this batch");
-	const bool x = pass[0] == 'x', fft = pass[0] == 'y' || pass[0] == 'z';
+	// the fused thin-mesh pass has one schedule per length (radix 0 only)
+	const bool x = pass[0] == 'x', fft = (pass[0] == 'y' || pass[0] == 'z') && std::strncmp(pass, "yzy", 3) != 0;
 	const bool f_ok = f == 0 || f == 1 || (!G->full && (f == 2 || (f == -2 && x_half_tile_ok(size))));
 	if ((x && !f_ok) || (!x && f != 0) || per_cu < 0 || per_cu > 64 || radix < 0
 		|| (fft && !G->full ? radix >= radix_variants(size) : radix != 0))
@@ -985,7 +1014,7 @@ extern "C" int bm_batch_set_pbc(bm_batch_t o, int mesh, int pbc_x, int pbc_y, in
 		o->dm[b] = d;
 	}
 	o->pbc[b] = { pbc_x, pbc_y, pbc_z };
-	for (std::vector<Group>* gs : { &o->gx, &o->gy, &o->gz, &o->gy2, &o->gml }) gs->clear();
+	for (std::vector<Group>* gs : { &o->gx, &o->gy, &o->gz, &o->gy2, &o->gml, &o->gyz[0], &o->gyz[1] }) gs->clear();
 	build_groups(o);
 	if (o->tuned && !o->autotune) o->tuned = false;
 	return build_tables(o);

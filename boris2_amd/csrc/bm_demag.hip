@@ -794,6 +794,170 @@ int launch_y2d(int NY, bool full, float2* Sx, const float* K, int NC, int NYH, i
 	return fail(BM_ERR_SIZE, "no y plan");
 }
 
+// ---------------------------------------------------------------------------------------
+// P2 + P3 + P4 fused for thin 3D meshes (N.z = NZ = 4 or 8, so nz <= NZ / 2): per kx, the
+// 3 nz lines of Sx are y-transformed (one plane at a time, registers and the exchange buffer)
+// into an LDS line buffer; every ky of those lines then has its nz values z-transformed
+// (zero padded to NZ), multiplied by the 6-term symmetric kernel (signs as in P3) and
+// inverse z-transformed (first nz planes kept) in place; the lines are inverse y-transformed
+// back into Sx.  The full spectra S are never written: per evaluation the pass moves Sx twice
+// and the kernel once, where P2 + P3 + P4 move Sx twice, S four times and the kernel once.
+// A block takes LP consecutive kx, one per group of T lanes.  Open y and z only.
+template <int NY, int NZ, int V, int BS>
+__global__ void __launch_bounds__(BS) k_yzy(float2* __restrict__ Sx, const float* __restrict__ K, int nz, int NC, int NYH, int ntl,
+	int nyd, const PassBatch* __restrict__ bt)
+{
+	if (bt) { const PassBatch& b = bt[blockIdx.y]; Sx = b.Sx; K = b.K; nz = b.nz; NC = b.NC; NYH = b.NYH; ntl = b.ntl; nyd = b.ny; }
+	constexpr int PK = plan_key(NY, V);
+	constexpr int E = Plan<PK>::E, T = plan_T<PK>(), LP = BS / T, LL = lds_len<NY>(), EL = E / 2, ZD = NZ / 2, NZH = NZ / 2 + 1;
+	static_assert(LP * T == BS, "whole lines per block");
+	__shared__ C2<float> lds[3 * LP * LL];            // exchange buffers of the y transforms
+	__shared__ C2<float> lb[ZD * 3 * LP * NY];         // lines (z, c, l) of the y spectra
+	const int t = threadIdx.x % T, l = threadIdx.x / T;
+	Twiddles<PK, float> tw; tw.init(t);
+	const size_t kq = (size_t)NC * NYH * NZH;
+	const int nwork = (NC + LP - 1) / LP;
+	C2<float>* b0 = lds + l * LL;
+	auto line = [&](int z, int c, int ll) { return lb + ((z * 3 + c) * LP + ll) * NY; };
+	for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+		const int kx = w * LP + l;
+		const bool ok = kx < NC;
+		const int kxs = ok ? kx : 0;
+		for (int z = 0; z < nz; z++) {                 // uniform over the block
+			C2<float> v[3][E];
+#pragma unroll
+			for (int c = 0; c < 3; c++)
+#pragma unroll
+				for (int e = 0; e < E; e++) {
+					const float2 a = (ok && e < EL && t + T * e < nyd) ? Sx[sx_line(c * nz + z, kxs, t + T * e, NC, ntl)] : make_float2(0.f, 0.f);
+					v[c][e] = mk(a.x, a.y);
+				}
+			fft3<PK, -1, float>(v[0], v[1], v[2], b0, b0 + LP * LL, b0 + 2 * LP * LL, 1, t, tw);
+#pragma unroll
+			for (int c = 0; c < 3; c++)
+#pragma unroll
+				for (int e = 0; e < E; e++) line(z, c, l)[t + T * e] = v[c][e];
+		}
+		__syncthreads();
+		for (int i = threadIdx.x; i < LP * NY; i += BS) {
+			const int il = i / NY, ky = i % NY;
+			const int kxi = w * LP + il < NC ? w * LP + il : 0;
+			const int kyr = ky <= NY / 2 ? ky : NY - ky;
+			const float sy = ky <= NY / 2 ? 1.f : -1.f;
+			const float* kb = K + ((size_t)kxi * NYH + kyr) * NZH;
+			C2<float> u[3][NZ];
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+#pragma unroll
+				for (int z = 0; z < NZ; z++) u[c][z] = z < nz ? line(z < ZD ? z : 0, c, il)[ky] : mk(0.f, 0.f);
+				Dft<NZ, -1, float>::run(u[c]);
+			}
+#pragma unroll
+			for (int kz = 0; kz < NZ; kz++) {
+				const int kzr = kz <= NZ / 2 ? kz : NZ - kz;
+				const float sz = kz <= NZ / 2 ? 1.f : -1.f;
+				const float kxx = kb[kzr], kyy = kb[kq + kzr], kzz = kb[2 * kq + kzr];
+				const float kxy = sy * kb[3 * kq + kzr], kxz = sz * kb[4 * kq + kzr], kyz = sy * sz * kb[5 * kq + kzr];
+				const C2<float> mx = u[0][kz], my = u[1][kz], mz = u[2][kz];
+				u[0][kz] = mk(kxx * mx.x + kxy * my.x + kxz * mz.x, kxx * mx.y + kxy * my.y + kxz * mz.y);
+				u[1][kz] = mk(kxy * mx.x + kyy * my.x + kyz * mz.x, kxy * mx.y + kyy * my.y + kyz * mz.y);
+				u[2][kz] = mk(kxz * mx.x + kyz * my.x + kzz * mz.x, kxz * mx.y + kyz * my.y + kzz * mz.y);
+			}
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+				Dft<NZ, 1, float>::run(u[c]);
+#pragma unroll
+				for (int z = 0; z < ZD; z++) if (z < nz) line(z, c, il)[ky] = u[c][z];
+			}
+		}
+		__syncthreads();
+		for (int z = 0; z < nz; z++) {                 // uniform over the block
+			C2<float> v[3][E];
+#pragma unroll
+			for (int c = 0; c < 3; c++)
+#pragma unroll
+				for (int e = 0; e < E; e++) v[c][e] = line(z, c, l)[t + T * e];
+			fft3<PK, 1, float>(v[0], v[1], v[2], b0, b0 + LP * LL, b0 + 2 * LP * LL, 1, t, tw);
+			if (ok) {
+#pragma unroll
+				for (int c = 0; c < 3; c++)
+#pragma unroll
+					for (int e = 0; e < EL; e++)
+						if (t + T * e < nyd) Sx[sx_line(c * nz + z, kx, t + T * e, NC, ntl)] = make_float2(v[c][e].x, v[c][e].y);
+			}
+		}
+		__syncthreads();                               // the line buffer is rewritten by the next item
+	}
+}
+
+// Schedule of NY for the fused thin-mesh pass with N.z = NZ: the radix-4 schedule (4 values
+// per thread: the most lanes per line, so the most loads in flight per kx held in LDS) where
+// the length has one, else one with at most 8 values per thread (NZ = 4 only); -1 if none.
+template <int NY, int NZ> constexpr int yzy_variant()
+{
+	if constexpr (plan_variants(NY) > 2 && Plan<plan_key(NY, plan_variants(NY) > 2 ? 2 : 0)>::E <= 4) return 2;
+	else if constexpr (NZ == 4 && Plan<plan_key(NY, 0)>::E <= 8) return 0;
+	else if constexpr (NZ == 4 && plan_variants(NY) > 1 && Plan<plan_key(NY, plan_variants(NY) > 1 ? 1 : 0)>::E <= 8) return 1;
+	else return -1;
+}
+
+template <int NY, int NZ> int launch_yzy_t(float2* Sx, const float* K, int nz, int NC, int NYH, int ntl, int ny, int ncu, hipStream_t st)
+{
+	constexpr int V = yzy_variant<NY, NZ>();
+	if constexpr (V < 0) return fail(BM_ERR_SIZE, "no fused thin-mesh schedule for this length");
+	else {
+		constexpr int T = plan_T<plan_key(NY, V)>(), BS = T < 128 ? 128 : T, LP = BS / T;
+		hipLaunchKernelGGL((k_yzy<NY, NZ, V, BS>), dim3(grid8((NC + LP - 1) / LP, ncu, 8)), dim3(BS), 0, st, Sx, K, nz, NC, NYH, ntl, ny,
+			(const PassBatch*)nullptr);
+		return BM_OK;
+	}
+}
+
+// batched: the meshes of one (N.y, N.z) group, one per grid row
+template <int NY, int NZ> int batch_yzy_t(LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+{
+	constexpr int V = yzy_variant<NY, NZ>();
+	if constexpr (V < 0) return fail(BM_ERR_SIZE, "no fused thin-mesh schedule for this length");
+	else {
+		constexpr int T = plan_T<plan_key(NY, V)>(), BS = T < 128 ? 128 : T, LP = BS / T;
+		int mx = 1;
+		for (int b = 0; b < nb; b++) mx = std::max(mx, (h_tab[b].NC + LP - 1) / LP);
+		hipLaunchKernelGGL((k_yzy<NY, NZ, V, BS>), dim3(grid8(mx, ncu, sh.per_cu ? sh.per_cu : 8), nb), dim3(BS), 0, st,
+			nullptr, nullptr, 0, 0, 0, 0, 0, d_tab);
+		return BM_OK;
+	}
+}
+
+#define BM_FOR_YZY_SIZES(M) M(16) M(32) M(64) M(128) M(256) M(512) M(1024) M(2048)
+
+// the fused thin-mesh pass serves this mesh (open y and z, N.z 4 or 8, a power-of-two N.y
+// with a schedule that fits).  By default only from 512 x 512 cells per plane: measured (RK4
+// steps) 512x512x2 18 % and 1024x1024x2 13 % faster than the separate passes, 1024x512x4 3 %,
+// 256x256x4 equal, and batches of films up to 256 x 256 1-3 % slower (its LDS line buffer
+// keeps few kx in flight, which small meshes cannot make up for).
+bool yzy_ok(const bm_demag_s* d)
+{
+	if (!d->yzy || d->is2d || d->full[1] || d->full[2] || (d->Nz != 4 && d->Nz != 8)) return false;
+	if (d->yzy == 1 && (long long)d->nx * d->ny < 512LL * 512) return false;
+	switch (d->Ny) {
+#define C_(V_) case V_: return d->Nz == 4 ? yzy_variant<V_, 4>() >= 0 : yzy_variant<V_, 8>() >= 0;
+	BM_FOR_YZY_SIZES(C_)
+#undef C_
+	}
+	return false;
+}
+
+int launch_yzy(const bm_demag_s* d, hipStream_t st)
+{
+	switch (d->Ny) {
+#define C_(V_) case V_: return d->Nz == 4 ? launch_yzy_t<V_, 4>(d->Sx, d->K, d->nz, d->NC, d->NYH, d->ntl, d->ny, d->ncu, st) \
+	: launch_yzy_t<V_, 8>(d->Sx, d->K, d->nz, d->NC, d->NYH, d->ntl, d->ny, d->ncu, st);
+	BM_FOR_YZY_SIZES(C_)
+#undef C_
+	}
+	return fail(BM_ERR_SIZE, "no fused thin-mesh pass for this N.y");
+}
+
 // Real-input FFT of length 2M (the x transform of the tensor for N.x = 8192, beyond the one-
 // block plans): the M-point complex FFT of z[k] = x[2k] + i x[2k+1], then the split
 // X[k] = (Z[k] + conj Z[M-k]) / 2 - i W^k (Z[k] - conj Z[M-k]) / 2 and X[k+M] with -W^k,
@@ -970,6 +1134,13 @@ int bm::demag_mid(bm_demag_s* d, hipStream_t st)
 		prof_mark_end(PK_Y2D, st);
 		return rc;
 	}
+	if (yzy_ok(d)) {
+		// thin mesh: the y, z, y passes as one (timed as the z pass)
+		prof_mark_begin(PK_Z, st);
+		rc = launch_yzy(d, st);
+		prof_mark_end(PK_Z, st);
+		return rc;
+	}
 	prof_mark_begin(PK_Y_FWD, st);
 	rc = launch_y(d->Ny, d->full[1], -1, d->S, d->Sx, nlines, d->Sp, d->NC, d->ntl, d->full[1] ? d->Ny : d->ny, d->ncu, st);
 	prof_mark_end(PK_Y_FWD, st);
@@ -1048,6 +1219,7 @@ static int demag_create(int nx, int ny, int nz, double hx, double hy, double hz,
 	d->NC = d->Nx / 2 + 1; d->NYH = d->Ny / 2 + 1; d->NZH = d->Nz / 2 + 1;
 	d->ncu = device_cu_count();
 	if (const char* e = getenv("BM_ROW_WAVE")) d->row_wave = atoi(e) != 0;
+	if (const char* e = getenv("BM_YZY")) d->yzy = std::max(0, std::min(2, atoi(e)));
 	// line pitch of the full spectra (complex elements); BM_S_PAD adds padding for tuning experiments
 	d->Sp = d->Ny;
 	if (const char* e = getenv("BM_S_PAD")) d->Sp += std::max(0, atoi(e));
@@ -1490,4 +1662,17 @@ int bm::batch_y2d(int Ny, bool full, LaunchShape sh, const PassBatch* h_tab, con
 #undef C_
 	}
 	return fail(BM_ERR_SIZE, "no y plan");
+}
+
+bool bm::demag_yzy(const bm_demag_s* d) { return yzy_ok(d); }
+
+int bm::batch_yzy(int Ny, int Nz, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st)
+{
+	if (Nz != 4 && Nz != 8) return fail(BM_ERR_SIZE, "fused thin-mesh pass: N.z 4 or 8");
+	switch (Ny) {
+#define C_(V_) case V_: return Nz == 4 ? batch_yzy_t<V_, 4>(sh, h_tab, d_tab, nb, ncu, st) : batch_yzy_t<V_, 8>(sh, h_tab, d_tab, nb, ncu, st);
+	BM_FOR_YZY_SIZES(C_)
+#undef C_
+	}
+	return fail(BM_ERR_SIZE, "no fused thin-mesh pass for this N.y");
 }

@@ -61,6 +61,8 @@ struct bm_demag_s {
 	int ncu = 256;
 	bool row_wave = false;    // integrator row kernel at nx = 512: the wave-independent variant
 	                          // (BM_ROW_WAVE=1 at creation; measured slower, kept for tests)
+	int yzy = 1;              // fused y, z, y pass for thin meshes (N.z 4 or 8): 1 from 512 x 512
+	                          // cells per plane, 2 at any size, 0 never (BM_YZY at creation)
 };
 
 namespace bm {
@@ -162,4 +164,8 @@ int batch_inv_x_planar(int nx, bool full, LaunchShape sh, const PassBatch* h_tab
 int batch_y(int Ny, bool full, int dir, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
 int batch_z(int Nz, bool full, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
 int batch_y2d(int Ny, bool full, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
+// thin meshes (N.z 4 or 8, open y and z, a power-of-two N.y with a fitting schedule): the y, z,
+// y passes as one kernel that never writes the full spectra; demag_yzy: the handle takes it
+bool demag_yzy(const bm_demag_s* d);
+int batch_yzy(int Ny, int Nz, LaunchShape sh, const PassBatch* h_tab, const PassBatch* d_tab, int nb, int ncu, hipStream_t st);
 }

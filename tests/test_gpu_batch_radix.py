@@ -11,6 +11,13 @@ from boris2_amd import api
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True)
+def separate_yzy_passes(monkeypatch):
+    """These tests exercise the alternative radix schedules of the separate y and z passes:
+    the batches' demag handles are created with the fused thin-mesh pass off."""
+    monkeypatch.setenv("BM_YZY", "0")
+
 H = (4e-9, 4e-9, 4e-9)
 # y lengths 2 ny and z lengths 2 nz covering every length with an alternative plan:
 # 3D y 16..2048 and 384 / 96 / 768, z 4..128; 2D (y2d) 16 ... 2048
