@@ -803,6 +803,7 @@ int launch_y2d(int NY, bool full, float2* Sx, const float* K, int NC, int NYH, i
 // back into Sx.  The full spectra S are never written: per evaluation the pass moves Sx twice
 // and the kernel once, where P2 + P3 + P4 move Sx twice, S four times and the kernel once.
 // A block takes LP consecutive kx, one per group of T lanes.  Open y and z only.
+constexpr bool kYzyReg = true;
 template <int NY, int NZ, int V, int BS>
 __global__ void __launch_bounds__(BS) k_yzy(float2* __restrict__ Sx, const float* __restrict__ K, int nz, int NC, int NYH, int ntl,
 	int nyd, const PassBatch* __restrict__ bt)
@@ -811,8 +812,11 @@ __global__ void __launch_bounds__(BS) k_yzy(float2* __restrict__ Sx, const float
 	constexpr int PK = plan_key(NY, V);
 	constexpr int E = Plan<PK>::E, T = plan_T<PK>(), LP = BS / T, LL = lds_len<NY>(), EL = E / 2, ZD = NZ / 2, NZH = NZ / 2 + 1;
 	static_assert(LP * T == BS, "whole lines per block");
+	// REG: the nz planes of the three components stay in registers (at most 24 complex values
+	// per thread) and the z transforms run there, without the LDS line buffer
+	constexpr bool REG = kYzyReg && ZD * 3 * E <= 24;
 	__shared__ C2<float> lds[3 * LP * LL];            // exchange buffers of the y transforms
-	__shared__ C2<float> lb[ZD * 3 * LP * NY];         // lines (z, c, l) of the y spectra
+	__shared__ C2<float> lb[REG ? 1 : ZD * 3 * LP * NY];   // lines (z, c, l) of the y spectra
 	const int t = threadIdx.x % T, l = threadIdx.x / T;
 	Twiddles<PK, float> tw; tw.init(t);
 	const size_t kq = (size_t)NC * NYH * NZH;
@@ -823,6 +827,65 @@ __global__ void __launch_bounds__(BS) k_yzy(float2* __restrict__ Sx, const float
 		const int kx = w * LP + l;
 		const bool ok = kx < NC;
 		const int kxs = ok ? kx : 0;
+		if constexpr (REG) {
+			C2<float> v[ZD][3][E];
+#pragma unroll
+			for (int z = 0; z < ZD; z++) {
+#pragma unroll
+				for (int c = 0; c < 3; c++)
+#pragma unroll
+					for (int e = 0; e < E; e++) {
+						const float2 a = (ok && z < nz && e < EL && t + T * e < nyd) ? Sx[sx_line(c * nz + z, kxs, t + T * e, NC, ntl)]
+							: make_float2(0.f, 0.f);
+						v[z][c][e] = mk(a.x, a.y);
+					}
+				if (z < nz) fft3<PK, -1, float>(v[z][0], v[z][1], v[z][2], b0, b0 + LP * LL, b0 + 2 * LP * LL, 1, t, tw);
+			}
+#pragma unroll
+			for (int e = 0; e < E; e++) {
+				const int ky = t + T * e;
+				const int kyr = ky <= NY / 2 ? ky : NY - ky;
+				const float sy = ky <= NY / 2 ? 1.f : -1.f;
+				const float* kb = K + ((size_t)kxs * NYH + kyr) * NZH;
+				C2<float> u[3][NZ];
+#pragma unroll
+				for (int c = 0; c < 3; c++) {
+#pragma unroll
+					for (int z = 0; z < NZ; z++) u[c][z] = z < ZD ? v[z][c][e] : mk(0.f, 0.f);
+					Dft<NZ, -1, float>::run(u[c]);
+				}
+#pragma unroll
+				for (int kz = 0; kz < NZ; kz++) {
+					const int kzr = kz <= NZ / 2 ? kz : NZ - kz;
+					const float sz = kz <= NZ / 2 ? 1.f : -1.f;
+					const float kxx = kb[kzr], kyy = kb[kq + kzr], kzz = kb[2 * kq + kzr];
+					const float kxy = sy * kb[3 * kq + kzr], kxz = sz * kb[4 * kq + kzr], kyz = sy * sz * kb[5 * kq + kzr];
+					const C2<float> mx = u[0][kz], my = u[1][kz], mz = u[2][kz];
+					u[0][kz] = mk(kxx * mx.x + kxy * my.x + kxz * mz.x, kxx * mx.y + kxy * my.y + kxz * mz.y);
+					u[1][kz] = mk(kxy * mx.x + kyy * my.x + kyz * mz.x, kxy * mx.y + kyy * my.y + kyz * mz.y);
+					u[2][kz] = mk(kxz * mx.x + kyz * my.x + kzz * mz.x, kxz * mx.y + kyz * my.y + kzz * mz.y);
+				}
+#pragma unroll
+				for (int c = 0; c < 3; c++) {
+					Dft<NZ, 1, float>::run(u[c]);
+#pragma unroll
+					for (int z = 0; z < ZD; z++) v[z][c][e] = u[c][z];
+				}
+			}
+#pragma unroll
+			for (int z = 0; z < ZD; z++) {
+				if (z >= nz) continue;                   // uniform over the block
+				fft3<PK, 1, float>(v[z][0], v[z][1], v[z][2], b0, b0 + LP * LL, b0 + 2 * LP * LL, 1, t, tw);
+				if (ok) {
+#pragma unroll
+					for (int c = 0; c < 3; c++)
+#pragma unroll
+						for (int e = 0; e < EL; e++)
+							if (t + T * e < nyd) Sx[sx_line(c * nz + z, kx, t + T * e, NC, ntl)] = make_float2(v[z][c][e].x, v[z][c][e].y);
+				}
+			}
+			continue;
+		}
 		for (int z = 0; z < nz; z++) {                 // uniform over the block
 			C2<float> v[3][E];
 #pragma unroll
@@ -931,14 +994,15 @@ template <int NY, int NZ> int batch_yzy_t(LaunchShape sh, const PassBatch* h_tab
 #define BM_FOR_YZY_SIZES(M) M(16) M(32) M(64) M(128) M(256) M(512) M(1024) M(2048)
 
 // the fused thin-mesh pass serves this mesh (open y and z, N.z 4 or 8, a power-of-two N.y
-// with a schedule that fits).  By default only from 512 x 512 cells per plane: measured (RK4
-// steps) 512x512x2 18 % and 1024x1024x2 13 % faster than the separate passes, 1024x512x4 3 %,
-// 256x256x4 equal, and batches of films up to 256 x 256 1-3 % slower (its LDS line buffer
-// keeps few kx in flight, which small meshes cannot make up for).
+// with a schedule that fits).  By default only for N.z = 4 (two planes, held in registers with
+// the radix-4 schedules) from 128 x 128 cells per plane: measured (RK4 steps, against the
+// separate passes) 128x128x2 0.221 -> 0.207 ms, 256x256x2 0.256 -> 0.215, 512x512x2 0.364 ->
+// 0.284, 1024x1024x2 0.994 -> 0.885 (N.y 2048: the LDS line buffer); for N.z = 8 (the LDS line
+// buffer) 256x256x4 and 1024x512x4 within +-2 % either way, so not by default.
 bool yzy_ok(const bm_demag_s* d)
 {
 	if (!d->yzy || d->is2d || d->full[1] || d->full[2] || (d->Nz != 4 && d->Nz != 8)) return false;
-	if (d->yzy == 1 && (long long)d->nx * d->ny < 512LL * 512) return false;
+	if (d->yzy == 1 && (d->Nz != 4 || (long long)d->nx * d->ny < 128LL * 128)) return false;
 	switch (d->Ny) {
 #define C_(V_) case V_: return d->Nz == 4 ? yzy_variant<V_, 4>() >= 0 : yzy_variant<V_, 8>() >= 0;
 	BM_FOR_YZY_SIZES(C_)

@@ -61,8 +61,8 @@ struct bm_demag_s {
 	int ncu = 256;
 	bool row_wave = false;    // integrator row kernel at nx = 512: the wave-independent variant
 	                          // (BM_ROW_WAVE=1 at creation; measured slower, kept for tests)
-	int yzy = 1;              // fused y, z, y pass for thin meshes (N.z 4 or 8): 1 from 512 x 512
-	                          // cells per plane, 2 at any size, 0 never (BM_YZY at creation)
+	int yzy = 1;              // fused y, z, y pass for thin meshes (N.z 4 or 8): 1 for N.z = 4 from
+	                          // 128 x 128 cells per plane, 2 always, 0 never (BM_YZY at creation)
 };
 
 namespace bm {
